@@ -1,0 +1,259 @@
+"""MI355X (gfx950) free-energy + gradient path of classicalDFT.
+
+Python binding of the C ABI in include/cdft.h (classicaldft_amd/libcdft.so).
+The HIP library is the only compute path: if it is not built, every entry
+point raises instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcdft.so")
+
+FMT_MODELS = {"none": -1, "Rosenfeld": 0, "RSLT": 1, "WhiteBearI": 2, "WhiteBearII": 3, "esFMT": 4}
+POTENTIALS = {"LJ": 0, "tWF": 1, "WRDF": 2}
+SPLITS = {"WCA": 0, "BH": 1}
+SCHEMES = {"zero": 0, "linearE": 1, "linearF": 2, "quadraticF": 3, "gaussE": 4, "gaussF": 5}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+
+EXPORTED = [
+    "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_grid_shape", "cdft_a_vdw",
+    "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
+    "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
+]
+
+
+class CdftError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"cdft error {code}: {msg}")
+        self.code = code
+
+
+class InvalidArgument(CdftError, ValueError):
+    pass
+
+
+class _Grid(C.Structure):
+    _fields_ = [("dx", C.c_double), ("box", C.c_double * 3)]
+
+
+class _Interaction(C.Structure):
+    _fields_ = [("species_i", C.c_int), ("species_j", C.c_int), ("potential", C.c_int), ("sigma", C.c_double),
+                ("epsilon", C.c_double), ("r_cutoff", C.c_double), ("alpha", C.c_double), ("split", C.c_int),
+                ("weight_scheme", C.c_int)]
+
+
+class _Options(C.Structure):
+    _fields_ = [("fmt_model", C.c_int), ("esfmt_A", C.c_double), ("esfmt_B", C.c_double), ("precision", C.c_int),
+                ("device", C.c_int)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("free_energy", C.c_double), ("grand_potential", C.c_double), ("ideal", C.c_double),
+                ("external", C.c_double), ("hard_sphere", C.c_double), ("mean_field", C.c_double),
+                ("mass", C.c_double)]
+
+
+class _Fire(C.Structure):
+    _fields_ = [("dt", C.c_double), ("dt_max", C.c_double), ("dt_min", C.c_double), ("alpha_start", C.c_double),
+                ("f_inc", C.c_double), ("f_dec", C.c_double), ("f_alpha", C.c_double), ("n_delay", C.c_int),
+                ("max_uphill", C.c_int), ("force_tolerance", C.c_double), ("max_steps", C.c_int)]
+
+
+class _Param(C.Structure):
+    _fields_ = [("kind", C.c_int), ("rho_min", C.c_double), ("rho_max", C.c_double)]
+
+
+class _MinResult(C.Structure):
+    _fields_ = [("free_energy", C.c_double), ("grand_potential", C.c_double), ("rms_force", C.c_double),
+                ("iterations", C.c_int), ("converged", C.c_int), ("dt", C.c_double), ("alpha", C.c_double)]
+
+
+class _StepConfig(C.Structure):
+    _fields_ = [("dt", C.c_double), ("diffusion_coefficient", C.c_double), ("min_density", C.c_double),
+                ("dt_max", C.c_double), ("fp_tolerance", C.c_double), ("fp_max_iterations", C.c_int)]
+
+
+class _SimResult(C.Structure):
+    _fields_ = [("mass_initial", C.c_double), ("mass_final", C.c_double), ("time", C.c_double),
+                ("energy", C.c_double), ("dt", C.c_double), ("evaluations", C.c_long)]
+
+
+_dp = C.POINTER(C.c_double)
+_lib = None
+
+
+def lib():
+    """Load libcdft.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m classicaldft_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        L.cdft_last_error.restype = C.c_char_p
+        L.cdft_ctx_create.argtypes = [C.POINTER(_Grid), _dp, C.c_int, C.POINTER(_Interaction), C.c_int, C.c_double,
+                                      C.POINTER(_Options), C.POINTER(C.c_void_p)]
+        L.cdft_ctx_destroy.argtypes = [C.c_void_p]
+        L.cdft_ctx_destroy.restype = None
+        L.cdft_grid_shape.argtypes = [C.c_void_p, C.POINTER(C.c_long)]
+        L.cdft_a_vdw.argtypes = [C.c_void_p, C.c_int, _dp]
+        L.cdft_bulk_chemical_potential.argtypes = [C.c_void_p, C.c_double, _dp]
+        L.cdft_total.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.POINTER(_Result), _dp]
+        L.cdft_total_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.POINTER(_Result),
+                                        C.c_void_p]
+        L.cdft_total_device_repeat.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
+                                               C.POINTER(_Result), C.c_void_p]
+        L.cdft_weighted_densities.argtypes = [C.c_void_p, _dp, _dp]
+        L.cdft_fft_r2c.argtypes = [C.c_void_p, _dp, _dp]
+        L.cdft_fft_c2r.argtypes = [C.c_void_p, _dp, _dp]
+        L.cdft_minimize_grand_potential.argtypes = [C.c_void_p, C.POINTER(_Fire), C.POINTER(_Param), C.c_int, _dp,
+                                                    _dp, C.c_double, _dp, C.POINTER(_MinResult), _dp]
+        L.cdft_ddft_run.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp,
+                                    C.POINTER(_SimResult), _dp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().cdft_last_error().decode()
+        if rc == -1:
+            raise InvalidArgument(rc, msg)
+        raise CdftError(rc, msg)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+
+
+class Context:
+    """One physics::Model (grid, species, interactions, kT) plus its weights on one GPU.
+
+    Mirrors functionals::make_functional (functional.hpp:195-206); `total`
+    mirrors functionals::total (functionals.hpp:64-95).
+    """
+
+    def __init__(self, shape, dx, diameter=1.0, interactions=(), kT=1.0, model="WhiteBearII", precision="fp64",
+                 A=1.0, B=0.0, device=0):
+        g = _Grid(float(dx), (C.c_double * 3)(*[float(n) * float(dx) for n in shape]))
+        its = (_Interaction * max(1, len(interactions)))()
+        for k, it in enumerate(interactions):
+            its[k] = _Interaction(it.get("si", 0), it.get("sj", 0), POTENTIALS[it.get("potential", "LJ")],
+                                  it.get("sigma", 1.0), it.get("epsilon", 1.0), it.get("r_cutoff", -1.0),
+                                  it.get("alpha", 50.0), SPLITS[it.get("split", "WCA")],
+                                  SCHEMES[it.get("scheme", "quadraticF")])
+        opt = _Options(FMT_MODELS[model], float(A), float(B), PRECISIONS[precision], int(device))
+        d = np.array([float(diameter)])
+        h = C.c_void_p()
+        _check(lib().cdft_ctx_create(C.byref(g), _p(d), 1, its, len(interactions), float(kT), C.byref(opt),
+                                     C.byref(h)))
+        self._h = h
+        sh = (C.c_long * 3)()
+        _check(lib().cdft_grid_shape(self._h, sh))
+        self.shape = tuple(sh)
+        self.N = int(np.prod(self.shape))
+        self.M = self.shape[0] * self.shape[1] * (self.shape[2] // 2 + 1)
+        self.dx = float(dx)
+        self.precision = precision
+        self.n_inter = len(interactions)
+        self.model = model
+        self.nf = (11 if model != "none" else 0) + (1 if interactions else 0)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().cdft_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def a_vdw(self, k=0):
+        v = C.c_double()
+        _check(lib().cdft_a_vdw(self._h, k, C.byref(v)))
+        return v.value
+
+    def bulk_mu(self, rho):
+        v = C.c_double()
+        _check(lib().cdft_bulk_chemical_potential(self._h, float(rho), C.byref(v)))
+        return v.value
+
+    def total(self, rho, mu, vext=None, forces=True):
+        """Return (result dict, forces or None)."""
+        rho = _f64(rho)
+        v = None if vext is None else _f64(vext)
+        f = np.zeros(self.N) if forces else None
+        r = _Result()
+        _check(lib().cdft_total(self._h, _p(rho), _p(v), float(mu), C.byref(r), _p(f)))
+        res = {k: getattr(r, k) for k, _ in _Result._fields_}
+        return res, (f.reshape(self.shape) if forces else None)
+
+    def total_device(self, d_rho, mu, d_vext=None, d_forces=None, count=1):
+        """Evaluate on device pointers (ints, e.g. torch tensor.data_ptr())."""
+        r = _Result()
+        _check(lib().cdft_total_device_repeat(self._h, C.c_void_p(d_rho), C.c_void_p(d_vext) if d_vext else None,
+                                              float(mu), int(count), C.byref(r),
+                                              C.c_void_p(d_forces) if d_forces else None))
+        return {k: getattr(r, k) for k, _ in _Result._fields_}
+
+    def weighted_densities(self, rho):
+        """Weighted densities: 11 FMT channels (when an FMT model is set) then the mean-field convolution."""
+        out = np.zeros(max(self.nf, 1) * self.N)
+        _check(lib().cdft_weighted_densities(self._h, _p(_f64(rho)), _p(out)))
+        return out[: self.nf * self.N].reshape((self.nf,) + self.shape)
+
+    def r2c(self, x):
+        out = np.zeros(2 * self.M)
+        _check(lib().cdft_fft_r2c(self._h, _p(_f64(x)), _p(out)))
+        return (out[0::2] + 1j * out[1::2]).reshape(self.shape[0], self.shape[1], -1)
+
+    def c2r(self, X):
+        X = np.ascontiguousarray(X, dtype=np.complex128).reshape(-1)
+        buf = np.empty(2 * X.size)
+        buf[0::2] = X.real
+        buf[1::2] = X.imag
+        out = np.zeros(self.N)
+        _check(lib().cdft_fft_c2r(self._h, _p(buf), _p(out)))
+        return out.reshape(self.shape)
+
+    def minimize_grand_potential(self, rho0, mu, fire=None, param="unbounded", rho_min=1e-18, rho_max=1.0,
+                                 homogeneous_boundary=True, vext=None, max_steps=None, record_energies=True):
+        f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
+                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
+        if fire:
+            f.update(fire)
+        if max_steps is not None:
+            f["max_steps"] = max_steps
+        fs = _Fire(*[f[k] for k, _ in _Fire._fields_])
+        prm = _Param(0 if param == "unbounded" else 1, float(rho_min), float(rho_max))
+        out = np.zeros(self.N)
+        res = _MinResult()
+        en = np.full(int(f["max_steps"]) + 1, np.nan) if record_energies else None
+        _check(lib().cdft_minimize_grand_potential(self._h, C.byref(fs), C.byref(prm), int(bool(homogeneous_boundary)),
+                                                   _p(_f64(rho0)), None if vext is None else _p(_f64(vext)), float(mu),
+                                                   _p(out), C.byref(res), _p(en)))
+        r = {k: getattr(res, k) for k, _ in _MinResult._fields_}
+        r["converged"] = bool(r["converged"])
+        return out.reshape(self.shape), r, (en[: r["iterations"] + 1] if en is not None else None)
+
+    def ddft_run(self, rho0, mu, n_steps, dt=1e-4, dt_max=1.0, fp_tolerance=1e-4, fp_max_iterations=100,
+                 min_density=1e-18, vext=None, record_energies=True):
+        cfg = _StepConfig(float(dt), 1.0, float(min_density), float(dt_max), float(fp_tolerance),
+                          int(fp_max_iterations))
+        rho = _f64(rho0).copy()
+        res = _SimResult()
+        en = np.zeros(n_steps + 1) if record_energies else None
+        _check(lib().cdft_ddft_run(self._h, C.byref(cfg), int(n_steps), None if vext is None else _p(_f64(vext)),
+                                   float(mu), _p(rho), C.byref(res), _p(en)))
+        r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
+        return rho.reshape(self.shape), r, en

@@ -1,0 +1,343 @@
+// Context construction and the evaluation pipeline.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+
+namespace cdft {
+
+namespace {
+
+FftPlan make_plan(int L) {
+  FftPlan p{};
+  p.L = L;
+  p.np = 0;
+  int m = L;
+  while (m % 8 == 0 && p.np < 12) { p.rad[p.np++] = 8; m /= 8; }
+  while (m % 4 == 0 && p.np < 12) { p.rad[p.np++] = 4; m /= 4; }
+  while (m % 2 == 0 && p.np < 12) { p.rad[p.np++] = 2; m /= 2; }
+  while (m % 3 == 0 && p.np < 12) { p.rad[p.np++] = 3; m /= 3; }
+  while (m % 5 == 0 && p.np < 12) { p.rad[p.np++] = 5; m /= 5; }
+  if (m != 1)
+    throw Unsupported("grid length " + std::to_string(L) +
+                      " has a prime factor other than 2, 3, 5 (the supported FFT radices)");
+  return p;
+}
+
+template <typename T>
+T* dalloc(size_t count) {
+  void* p = nullptr;
+  if (count == 0) return nullptr;
+  CDFT_HIP(hipMalloc(&p, count * sizeof(T)));
+  return static_cast<T*>(p);
+}
+
+void* dalloc_bytes(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) return nullptr;
+  CDFT_HIP(hipMalloc(&p, bytes));
+  return p;
+}
+
+}  // namespace
+
+Context::~Context() {
+  void* bufs[] = {d_rho, d_vext, d_force, d_local, d_S0, d_SF, d_SO, d_mf, d_partials, d_res,
+                  tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2]};
+  hipSetDevice(device);
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+Context* create_context(const cdft_grid* grid, const double* diameters, int n_species,
+                        const cdft_interaction* interactions, int n_interactions, double kT,
+                        const cdft_options* options) {
+  if (!grid || !options) throw std::invalid_argument("cdft_ctx_create: grid and options are required");
+  if (grid->dx <= 0.0)
+    throw std::invalid_argument("make_grid: dx must be positive, got " + std::to_string(grid->dx));
+  long shape[3];
+  for (int d = 0; d < 3; ++d) {
+    if (grid->box[d] <= 0.0)
+      throw std::invalid_argument("make_grid: box dimension must be positive, got " + std::to_string(grid->box[d]));
+    const double n = grid->box[d] / grid->dx;
+    const long ni = std::lround(n);
+    if (std::fabs(n - static_cast<double>(ni)) > 1e-10)
+      throw std::invalid_argument("make_grid: box dimension " + std::to_string(grid->box[d]) +
+                                  " is not commensurate with dx=" + std::to_string(grid->dx));
+    shape[d] = ni;
+  }
+  if (n_species != 1)
+    throw Unsupported("the HIP path evaluates one species; got " + std::to_string(n_species));
+  if (!diameters || diameters[0] <= 0.0) throw std::invalid_argument("hard-sphere diameter must be positive");
+  if (n_interactions < 0 || n_interactions > 1)
+    throw Unsupported("the HIP path evaluates at most one mean-field interaction");
+  if (options->precision != CDFT_PRECISION_FP64 && options->precision != CDFT_PRECISION_MIXED)
+    throw std::invalid_argument("unknown precision mode");
+  const int model = options->fmt_model;
+  if (model < -1 || model > 4) throw std::invalid_argument("unknown FMT model id " + std::to_string(model));
+
+  auto* c = new Context();
+  try {
+    c->device = options->device;
+    CDFT_HIP(hipSetDevice(c->device));
+    CDFT_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    c->precision = options->precision;
+    c->dx = grid->dx;
+    c->kT = kT;
+    for (int d = 0; d < 3; ++d) c->shape[d] = shape[d];
+    c->diam.assign(diameters, diameters + n_species);
+
+    Geom& g = c->g;
+    g.nx = static_cast<int>(shape[0]);
+    g.ny = static_cast<int>(shape[1]);
+    g.nz = static_cast<int>(shape[2]);
+    g.nzh = g.nz / 2 + 1;
+    g.N = shape[0] * shape[1] * shape[2];
+    g.M = shape[0] * shape[1] * static_cast<long>(g.nzh);
+    g.nxy = shape[0] * shape[1];
+    g.dx = grid->dx;
+    g.dv = grid->dx * grid->dx * grid->dx;
+    const double PI = 3.14159265358979323846;
+    g.dkx = 2.0 * PI / (g.nx * grid->dx);
+    g.dky = 2.0 * PI / (g.ny * grid->dx);
+    g.dkz = 2.0 * PI / (g.nz * grid->dx);
+    g.inv_n = 1.0 / static_cast<double>(g.N);
+    g.R = 0.5 * diameters[0];
+
+    c->fp.model = model;
+    c->fp.R = g.R;
+    c->fp.A = (model == FMT_ESFMT) ? options->esfmt_A : 1.0;
+    c->fp.B = (model == FMT_ESFMT) ? options->esfmt_B : (model >= FMT_WHITEBEAR_I ? -1.0 : 0.0);
+
+    for (int k = 0; k < n_interactions; ++k) {
+      const cdft_interaction& in = interactions[k];
+      if (in.species_i != 0 || in.species_j != 0) throw std::invalid_argument("interaction species out of range");
+      if (in.potential < 0 || in.potential > 2) throw std::invalid_argument("unknown potential id");
+      InteractionSpec s{};
+      s.si = in.species_i;
+      s.sj = in.species_j;
+      s.split = in.split;
+      s.scheme = in.weight_scheme;
+      s.pot.kind = in.potential;
+      s.pot.sigma = in.sigma;
+      s.pot.eps = in.epsilon;
+      s.pot.rc = in.r_cutoff;
+      s.pot.alpha = in.alpha;
+      s.pot.finish();
+      c->inters.push_back(s);
+    }
+
+    c->cs.hs = model != FMT_NONE ? 1 : 0;
+    c->cs.mf = n_interactions > 0 ? 1 : 0;
+    c->cs.nf = 11 * c->cs.hs + c->cs.mf;
+    c->cs.nb = 11 * c->cs.hs;
+
+    for (int d = 0; d < 3; ++d) c->plan[d] = make_plan(static_cast<int>(shape[d]));
+    for (int d = 0; d < 3; ++d) {
+      const int L = static_cast<int>(shape[d]);
+      std::vector<double> t64(2 * static_cast<size_t>(L));
+      std::vector<float> t32(2 * static_cast<size_t>(L));
+      for (int m = 0; m < L; ++m) {
+        const double a = -2.0 * PI * static_cast<double>(m) / static_cast<double>(L);
+        t64[2 * m] = std::cos(a);
+        t64[2 * m + 1] = std::sin(a);
+        t32[2 * m] = static_cast<float>(t64[2 * m]);
+        t32[2 * m + 1] = static_cast<float>(t64[2 * m + 1]);
+      }
+      c->tw64[d] = dalloc_bytes(t64.size() * sizeof(double));
+      c->tw32[d] = dalloc_bytes(t32.size() * sizeof(float));
+      CDFT_HIP(hipMemcpy(c->tw64[d], t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+      CDFT_HIP(hipMemcpy(c->tw32[d], t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+
+    const size_t cs_bytes = c->csize();
+    const int nf_alloc = std::max(c->cs.nf, 1);
+    c->d_rho = dalloc<double>(g.N);
+    c->d_vext = dalloc<double>(g.N);
+    c->d_force = dalloc<double>(g.N);
+    c->d_local = dalloc<double>(g.N);
+    c->d_S0 = dalloc_bytes(cs_bytes * g.M);
+    c->d_SF = dalloc_bytes(cs_bytes * g.M * nf_alloc);
+    c->d_SO = dalloc_bytes(cs_bytes * g.M);
+    c->d_mf = dalloc_bytes(cs_bytes / 2 * g.M);
+    c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g) : zpoint_blocks<double>(g);
+    c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
+    c->d_res = dalloc<double>(8);
+
+    // Mean-field weight: real-space table on the host (FP64), R2C on the device
+    // (FP64 pipeline), real part kept in the working precision.  The table is
+    // even in s, so its transform is real.
+    if (c->cs.mf) {
+      const InteractionSpec& s = c->inters[0];
+      std::vector<double> table;
+      const double a = mean_field_table(s.pot, s.split, s.scheme, kT, grid->dx, shape[0], shape[1], shape[2], table);
+      c->a_vdw.push_back(a);
+      double* d_tab = dalloc<double>(g.N);
+      void* d_spec = dalloc_bytes(16 * g.M);
+      try {
+        CDFT_HIP(hipMemcpy(d_tab, table.data(), sizeof(double) * g.N, hipMemcpyHostToDevice));
+        auto* S = static_cast<cx<double>*>(d_spec);
+        const AliasMap id{0, 0.0, 0.0};
+        launch_zr2c<double>(d_tab, S, g, c->plan[2], c->tw<double>(2), id, c->stream);
+        launch_lines<double>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<double>(1), -1, c->stream);
+        launch_xlines<double>(S, 1, g, c->plan[0], c->tw<double>(0), -1, c->stream);
+        CDFT_HIP(hipGetLastError());
+        std::vector<cx<double>> spec(static_cast<size_t>(g.M));
+        CDFT_HIP(hipMemcpyAsync(spec.data(), d_spec, 16 * g.M, hipMemcpyDeviceToHost, c->stream));
+        CDFT_HIP(hipStreamSynchronize(c->stream));
+        if (c->precision == CDFT_PRECISION_MIXED) {
+          std::vector<float> re(static_cast<size_t>(g.M));
+          for (long i = 0; i < g.M; ++i) re[i] = static_cast<float>(spec[i].re);
+          CDFT_HIP(hipMemcpy(c->d_mf, re.data(), sizeof(float) * g.M, hipMemcpyHostToDevice));
+        } else {
+          std::vector<double> re(static_cast<size_t>(g.M));
+          for (long i = 0; i < g.M; ++i) re[i] = spec[i].re;
+          CDFT_HIP(hipMemcpy(c->d_mf, re.data(), sizeof(double) * g.M, hipMemcpyHostToDevice));
+        }
+      } catch (...) {
+        (void)hipFree(d_tab);
+        (void)hipFree(d_spec);
+        throw;
+      }
+      CDFT_HIP(hipFree(d_tab));
+      CDFT_HIP(hipFree(d_spec));
+    }
+  } catch (...) {
+    delete c;
+    throw;
+  }
+  return c;
+}
+
+template <typename R>
+void Context::eval_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
+  auto* S0 = static_cast<cx<R>*>(d_S0);
+  auto* SF = static_cast<cx<R>*>(d_SF);
+  auto* SO = static_cast<cx<R>*>(d_SO);
+  const long ylines = static_cast<long>(g.nzh) * g.nx;
+  const bool backward = force != nullptr && cs.nb > 0;
+  if (cs.nf > 0) {
+    launch_zr2c<R>(rho, S0, g, plan[2], tw<R>(2), amap, stream);
+    launch_lines<R>(S0, ylines, plan[1], tw<R>(1), -1, stream);
+    launch_xfwd<R>(S0, SF, static_cast<const R*>(d_mf), g, plan[0], tw<R>(0), cs, stream);
+    launch_lines<R>(SF, ylines * cs.nf, plan[1], tw<R>(1), +1, stream);
+  }
+  ChannelSet cz = cs;
+  if (!backward) cz.nb = 0;
+  double* lf = backward ? d_local : (force ? force : d_local);
+  launch_zpoint<R>(SF, rho, vext, lf, d_partials, g, plan[2], tw<R>(2), cz, fp, mu, amap, stream);
+  if (backward) {
+    launch_lines<R>(SF, ylines * cs.nb, plan[1], tw<R>(1), -1, stream);
+    launch_xbwd<R>(SF, SO, g, plan[0], tw<R>(0), cs.nb, stream);
+    launch_lines<R>(SO, ylines, plan[1], tw<R>(1), +1, stream);
+    launch_zc2r<R>(SO, 1, force, d_local, g.dv, g, plan[2], tw<R>(2), stream);
+  }
+  launch_finalize(d_partials, nblk_point, d_res, stream);
+}
+
+void Context::eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
+  if (precision == CDFT_PRECISION_MIXED) eval_t<float>(rho, vext, mu, force, amap);
+  else eval_t<double>(rho, vext, mu, force, amap);
+  CDFT_HIP(hipGetLastError());
+}
+
+void Context::result_from(const double* res, double mu, cdft_result* out) const {
+  out->ideal = res[RS_IDEAL];
+  out->external = res[RS_EXT];
+  out->hard_sphere = res[RS_HS];
+  out->mean_field = res[RS_MF];
+  out->mass = res[RS_MASS];
+  // functionals.hpp:79-92: F = F_id + F_ext + F_hs + F_mf, Omega = F - mu N
+  out->free_energy = out->ideal + out->external + out->hard_sphere + out->mean_field;
+  out->grand_potential = out->free_energy - mu * out->mass;
+}
+
+void Context::fetch_result(double mu, cdft_result* out) {
+  double res[8];
+  CDFT_HIP(hipMemcpyAsync(res, d_res, sizeof(res), hipMemcpyDeviceToHost, stream));
+  CDFT_HIP(hipStreamSynchronize(stream));
+  result_from(res, mu, out);
+}
+
+void Context::r2c(const double* d_in) {
+  const AliasMap id{0, 0.0, 0.0};
+  const long ylines = static_cast<long>(g.nzh) * g.nx;
+  if (precision == CDFT_PRECISION_MIXED) {
+    auto* S = static_cast<cx<float>*>(d_S0);
+    launch_zr2c<float>(d_in, S, g, plan[2], tw<float>(2), id, stream);
+    launch_lines<float>(S, ylines, plan[1], tw<float>(1), -1, stream);
+    launch_xlines<float>(S, 1, g, plan[0], tw<float>(0), -1, stream);
+  } else {
+    auto* S = static_cast<cx<double>*>(d_S0);
+    launch_zr2c<double>(d_in, S, g, plan[2], tw<double>(2), id, stream);
+    launch_lines<double>(S, ylines, plan[1], tw<double>(1), -1, stream);
+    launch_xlines<double>(S, 1, g, plan[0], tw<double>(0), -1, stream);
+  }
+  CDFT_HIP(hipGetLastError());
+}
+
+void Context::c2r(double* d_out) {
+  const long ylines = static_cast<long>(g.nzh) * g.nx;
+  if (precision == CDFT_PRECISION_MIXED) {
+    auto* S = static_cast<cx<float>*>(d_S0);
+    launch_xlines<float>(S, 1, g, plan[0], tw<float>(0), +1, stream);
+    launch_lines<float>(S, ylines, plan[1], tw<float>(1), +1, stream);
+    launch_zc2r<float>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), stream);
+  } else {
+    auto* S = static_cast<cx<double>*>(d_S0);
+    launch_xlines<double>(S, 1, g, plan[0], tw<double>(0), +1, stream);
+    launch_lines<double>(S, ylines, plan[1], tw<double>(1), +1, stream);
+    launch_zc2r<double>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), stream);
+  }
+  CDFT_HIP(hipGetLastError());
+}
+
+void Context::weighted_densities(const double* d_rho_in, double* d_out) {
+  const AliasMap id{0, 0.0, 0.0};
+  const long ylines = static_cast<long>(g.nzh) * g.nx;
+  if (cs.nf == 0) return;
+  if (precision == CDFT_PRECISION_MIXED) {
+    auto* S0 = static_cast<cx<float>*>(d_S0);
+    auto* SF = static_cast<cx<float>*>(d_SF);
+    launch_zr2c<float>(d_rho_in, S0, g, plan[2], tw<float>(2), id, stream);
+    launch_lines<float>(S0, ylines, plan[1], tw<float>(1), -1, stream);
+    launch_xfwd<float>(S0, SF, static_cast<const float*>(d_mf), g, plan[0], tw<float>(0), cs, stream);
+    launch_lines<float>(SF, ylines * cs.nf, plan[1], tw<float>(1), +1, stream);
+    launch_zc2r<float>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), stream);
+  } else {
+    auto* S0 = static_cast<cx<double>*>(d_S0);
+    auto* SF = static_cast<cx<double>*>(d_SF);
+    launch_zr2c<double>(d_rho_in, S0, g, plan[2], tw<double>(2), id, stream);
+    launch_lines<double>(S0, ylines, plan[1], tw<double>(1), -1, stream);
+    launch_xfwd<double>(S0, SF, static_cast<const double*>(d_mf), g, plan[0], tw<double>(0), cs, stream);
+    launch_lines<double>(SF, ylines * cs.nf, plan[1], tw<double>(1), +1, stream);
+    launch_zc2r<double>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), stream);
+  }
+  CDFT_HIP(hipGetLastError());
+}
+
+// BulkThermodynamics::chemical_potential (bulk/thermodynamics.hpp:222-237) for
+// one species: ln rho + mu_ex^FMT (uniform measures, measures.hpp:49-65) + a_vdw rho.
+double Context::bulk_mu(double rho) const {
+  const double PI = 3.14159265358979323846;
+  const double d = diam[0];
+  double mu = std::log(rho);
+  if (fp.model != FMT_NONE) {
+    const double n2 = PI * rho * d * d;
+    double w[11] = {PI / 6.0 * rho * d * d * d, n2, 0, 0, 0, n2 / 3.0, 0, 0, n2 / 3.0, 0, n2 / 3.0};
+    double gg[11];
+    fmt_phi_grad(fp, w, gg);
+    mu += gg[0] * PI / 6.0 * d * d * d + gg[1] * PI * d * d;
+    if (fp.model >= FMT_WHITEBEAR_I) mu += (gg[5] + gg[8] + gg[10]) * PI * d * d / 3.0;
+  }
+  for (size_t k = 0; k < a_vdw.size(); ++k) mu += a_vdw[k] * rho;
+  return mu;
+}
+
+}  // namespace cdft

@@ -1,0 +1,107 @@
+// Library context: geometry, FFT plans, device buffers, and the evaluation
+// pipeline shared by the C ABI entry points.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "cdft.h"
+#include "host_weights.hpp"
+#include "kernels.hpp"
+
+namespace cdft {
+
+struct HipError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct Unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define CDFT_HIP(call)                                                                                \
+  do {                                                                                                \
+    hipError_t e_ = (call);                                                                           \
+    if (e_ != hipSuccess)                                                                             \
+      throw ::cdft::HipError(std::string(#call) + " failed: " + hipGetErrorString(e_) + " at " +      \
+                             __FILE__ + ":" + std::to_string(__LINE__));                              \
+  } while (0)
+
+struct InteractionSpec {
+  int si, sj, split, scheme;
+  PairPotential pot;
+};
+
+// Results of one evaluation, in the order the fused kernel accumulates them.
+enum ResultSlot { RS_IDEAL = 0, RS_EXT = 1, RS_HS = 2, RS_MF = 3, RS_MASS = 4 };
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = CDFT_PRECISION_FP64;
+  long shape[3] = {0, 0, 0};
+  double dx = 0, kT = 1;
+  Geom g{};
+  FmtParams fp{};
+  ChannelSet cs{};
+  std::vector<double> diam;
+  std::vector<InteractionSpec> inters;
+  std::vector<double> a_vdw;
+  FftPlan plan[3]{};
+  void* tw32[3] = {nullptr, nullptr, nullptr};
+  void* tw64[3] = {nullptr, nullptr, nullptr};
+
+  // device buffers
+  double* d_rho = nullptr;    // staging of host inputs / alias-mapped density
+  double* d_vext = nullptr;
+  double* d_force = nullptr;
+  double* d_local = nullptr;  // local (ideal + external + mean-field) force
+  void* d_S0 = nullptr;       // M complex
+  void* d_SF = nullptr;       // nf * M complex (derivative channels reuse it)
+  void* d_SO = nullptr;       // M complex
+  void* d_mf = nullptr;       // M real: mean-field k-space weight
+  double* d_partials = nullptr;
+  double* d_res = nullptr;
+  int nblk_point = 0;
+
+  ~Context();
+  size_t csize() const { return precision == CDFT_PRECISION_MIXED ? 8 : 16; }
+
+  template <typename R>
+  const cx<R>* tw(int axis) const {
+    return static_cast<const cx<R>*>(sizeof(R) == 4 ? tw32[axis] : tw64[axis]);
+  }
+
+  // One F + dF/drho evaluation on device arrays (rho may be an alias variable
+  // mapped through `amap`).  Results land in d_res (device).  When `force`
+  // is null only the energy is computed.
+  void eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
+  template <typename R>
+  void eval_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
+
+  // Read back d_res and assemble a cdft_result.
+  void fetch_result(double mu, cdft_result* out);
+  void result_from(const double* res, double mu, cdft_result* out) const;
+
+  // Plain 3D transforms between a device real field and S0 (plane-major).
+  void r2c(const double* d_in);
+  void c2r(double* d_out);
+  void weighted_densities(const double* d_rho_in, double* d_out /* nf * N */);
+
+  double bulk_mu(double rho) const;
+};
+
+Context* create_context(const cdft_grid* grid, const double* diameters, int n_species,
+                        const cdft_interaction* interactions, int n_interactions, double kT,
+                        const cdft_options* options);
+
+void set_error(const std::string& msg);
+
+}  // namespace cdft
+
+struct cdft_ctx {
+  cdft::Context* impl;
+};

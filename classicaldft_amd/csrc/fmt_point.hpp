@@ -1,0 +1,174 @@
+// Pointwise Fundamental Measure Theory free-energy density and its partial
+// derivatives, evaluated in FP64 on the device.
+//
+// Follows include/dft/functionals/fmt/models.hpp of the reference:
+//   Phi  = -n0 f1(eta) + (n1 n2 - v0.v1) f2(eta) + Phi3 f3(eta)   (models.hpp:49-54)
+//   dPhi by the chain rule (models.hpp:56-81), with the closed-form
+//   derivatives of f1, f2, f3 in place of the reference's forward-mode
+//   autodiff, and the same Taylor branches below eta = 1e-6
+//   (models.hpp:200-202, 250-253, 282-292).
+// Only the 11 independent weighted densities of one species are inputs:
+// eta, n2, n_v2[3], T[6]; n1, n0 and n_v1 follow by Rosenfeld scaling
+// (hard_sphere.hpp:73-97).  The outputs are already combined into the 11
+// back-convolution channels of hard_sphere.hpp:115-134:
+//   g0 = dPhi/deta
+//   g1 = dPhi/dn2 + dPhi/dn1 / (4 pi R) + dPhi/dn0 / (4 pi R^2)
+//   g2..4 = dPhi/dv1_a + dPhi/dv0_a / (4 pi R)
+//   g5..10 = sym_ij dPhi/dT_ij  (xx, xy, xz, yy, yz, zz; sym = 2 off-diagonal)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace cdft {
+
+enum FmtModelId : int {
+  FMT_NONE = -1,
+  FMT_ROSENFELD = 0,
+  FMT_RSLT = 1,
+  FMT_WHITEBEAR_I = 2,
+  FMT_WHITEBEAR_II = 3,
+  FMT_ESFMT = 4,
+};
+
+struct FmtParams {
+  int model;
+  double R;      // hard-sphere radius
+  double A, B;   // esFMT coefficients (WhiteBear I/II use A=1, B=-1)
+};
+
+__host__ __device__ inline void f_funcs(int model, double e, double* f, double* df) {
+  const double om = 1.0 - e;
+  const double L = log(om);
+  // f1 (models.hpp:157-160): identical for every model
+  f[0] = L;
+  df[0] = -1.0 / om;
+  // f2
+  if (model == FMT_WHITEBEAR_II) {
+    if (e < 1e-6) {
+      f[1] = 1.0 + e * (1.0 + e * (10.0 / 9.0 + 7.0 / 6.0 * e));
+      df[1] = 1.0 + e * (20.0 / 9.0 + 3.5 * e);
+    } else {
+      f[1] = 1.0 / 3.0 + (4.0 / 3.0) / om + 2.0 / (3.0 * e) * L;
+      df[1] = (4.0 / 3.0) / (om * om) - 2.0 / (3.0 * e * e) * L - 2.0 / (3.0 * e) / om;
+    }
+  } else {
+    f[1] = 1.0 / om;
+    df[1] = 1.0 / (om * om);
+  }
+  // f3
+  if (model == FMT_RSLT || model == FMT_WHITEBEAR_I) {
+    if (e < 1e-6) {
+      if (model == FMT_RSLT) {
+        f[2] = 1.0 + (2.0 / 3.0) * e + 0.5 * e * e;
+        df[2] = 2.0 / 3.0 + e;
+      } else {
+        f[2] = 1.5 + e * (8.0 / 3.0 + e * (15.0 / 4.0 + 24.0 / 5.0 * e));
+        df[2] = 8.0 / 3.0 + e * (7.5 + 14.4 * e);
+      }
+    } else {
+      f[2] = 1.0 / (e * om * om) + L / (e * e);
+      df[2] = -1.0 / (e * e * om * om) + 2.0 / (e * om * om * om) - 1.0 / (om * e * e) - 2.0 * L / (e * e * e);
+    }
+  } else if (model == FMT_WHITEBEAR_II) {
+    if (e < 1e-6) {
+      f[2] = 1.5 + 7.0 / 3.0 * e + 3.25 * e * e + 4.2 * e * e * e;
+      df[2] = 7.0 / 3.0 + 6.5 * e + 12.6 * e * e;
+    } else {
+      const double g = 1.0 - 3.0 * e + e * e, gp = -3.0 + 2.0 * e;
+      const double h = e * om * om, hp = om * (1.0 - 3.0 * e);
+      f[2] = -g / h - L / (e * e);
+      df[2] = -(gp * h - g * hp) / (h * h) + 1.0 / (om * e * e) + 2.0 * L / (e * e * e);
+    }
+  } else {
+    f[2] = 1.0 / (om * om);
+    df[2] = 2.0 / (om * om * om);
+  }
+}
+
+// w[0..10] = eta, n2, v[3], T[xx,xy,xz,yy,yz,zz]; returns Phi, fills g[0..10].
+__host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double* w, double* g) {
+  const double PI = 3.14159265358979323846;
+  const double eta = w[0];
+  if (!(eta >= 1e-30)) {  // hard_sphere.hpp:199-200 (skip, derivatives stay zero)
+#pragma unroll
+    for (int i = 0; i < 11; ++i) g[i] = 0.0;
+    return 0.0;
+  }
+  const double R = P.R;
+  const double fpr = 4.0 * PI * R;
+  const double n2 = w[1];
+  const double n1 = n2 / fpr;
+  const double n0 = n2 / (fpr * R);
+  const double v[3] = {w[2], w[3], w[4]};
+  const double a = w[5], b = w[6], c = w[7], d = w[8], e = w[9], f = w[10];
+  const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const double v0v1 = vv / fpr;
+
+  double F[3], dF[3];
+  f_funcs(P.model, eta, F, dF);
+
+  double p3 = 0.0, dp3_n2 = 0.0, dp3_v[3] = {0.0, 0.0, 0.0}, dT[6] = {0, 0, 0, 0, 0, 0};
+  bool tensor = false;
+  if (P.model == FMT_ROSENFELD) {
+    const double k24 = 1.0 / (24.0 * PI);
+    p3 = k24 * (n2 * n2 * n2 - 3.0 * n2 * vv);
+    dp3_n2 = (1.0 / (8.0 * PI)) * (n2 * n2 - vv);
+    for (int k = 0; k < 3; ++k) dp3_v[k] = -(1.0 / (4.0 * PI)) * n2 * v[k];
+  } else if (P.model == FMT_RSLT) {
+    if (n2 >= 1e-30) {
+      const double xi = fmin(vv / (n2 * n2), 1.0);
+      const double q = 1.0 - xi;
+      const double k36 = 1.0 / (36.0 * PI);
+      p3 = k36 * n2 * n2 * n2 * q * q * q;
+      dp3_n2 = k36 * 3.0 * n2 * n2 * q * q * (1.0 + xi);
+      for (int k = 0; k < 3; ++k) dp3_v[k] = k36 * (-6.0) * n2 * q * q * v[k];
+    }
+  } else if (P.model != FMT_NONE) {
+    // tensor models: esFMT(A,B); WhiteBear I/II are esFMT(1,-1) (models.hpp:18-45,326-357)
+    tensor = true;
+    const double A = P.A, B = P.B;
+    const double sxx = a * a + b * b + c * c, sxy = a * b + b * d + c * e, sxz = a * c + b * e + c * f;
+    const double syy = b * b + d * d + e * e, syz = b * c + d * e + e * f, szz = c * c + e * e + f * f;
+    const double trT2 = sxx + syy + szz;
+    const double trT3 = a * sxx + d * syy + f * szz + 2.0 * (b * sxy + c * sxz + e * syz);
+    const double Tv[3] = {a * v[0] + b * v[1] + c * v[2], b * v[0] + d * v[1] + e * v[2],
+                          c * v[0] + e * v[1] + f * v[2]};
+    const double qf = v[0] * Tv[0] + v[1] * Tv[1] + v[2] * Tv[2];
+    const double k24 = 1.0 / (24.0 * PI);
+    const double n23 = n2 * n2 * n2;
+    p3 = k24 * (A * (n23 - 3.0 * n2 * vv + 3.0 * qf - trT3) + B * (n23 - 3.0 * n2 * trT2 + 2.0 * trT3));
+    dp3_n2 = k24 * (A * (3.0 * n2 * n2 - 3.0 * vv) + B * (3.0 * n2 * n2 - 3.0 * trT2));
+    for (int k = 0; k < 3; ++k) dp3_v[k] = k24 * A * (-6.0 * n2 * v[k] + 6.0 * Tv[k]);
+    const double k8 = 1.0 / (8.0 * PI), k4 = 1.0 / (4.0 * PI);
+    const double Tm[6] = {a, b, c, d, e, f};
+    const double T2m[6] = {sxx, sxy, sxz, syy, syz, szz};
+    const int ii[6] = {0, 0, 0, 1, 1, 2};
+    const int jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      dT[q] = k8 * A * (v[ii[q]] * v[jj[q]] - T2m[q]) + k4 * B * (-n2 * Tm[q] + T2m[q]);
+  }
+
+  const double s = n1 * n2 - v0v1;
+  const double phi = -n0 * F[0] + s * F[1] + p3 * F[2];
+  const double d_eta = -n0 * dF[0] + s * dF[1] + p3 * dF[2];
+  const double d_n0 = -F[0];
+  const double d_n1 = n2 * F[1];
+  const double d_n2 = n1 * F[1] + dp3_n2 * F[2];
+  g[0] = d_eta;
+  g[1] = d_n2 + d_n1 / fpr + d_n0 / (fpr * R);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const double dv0 = -v[k] * F[1];                   // dPhi/dv0 = -v1 f2
+    const double dv1 = -(v[k] / fpr) * F[1] + dp3_v[k] * F[2];  // dPhi/dv1 = -v0 f2 + dPhi3/dv1 f3
+    g[2 + k] = dv1 + dv0 / fpr;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double sym = (q == 0 || q == 3 || q == 5) ? 1.0 : 2.0;
+    g[5 + q] = tensor ? sym * dT[q] * F[2] : 0.0;
+  }
+  return phi;
+}
+
+}  // namespace cdft

@@ -1,0 +1,70 @@
+// Kernel-side shared types and host launchers of the evaluation pipeline.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+#include "fmt_point.hpp"
+
+namespace cdft {
+
+// Grid geometry as seen by the kernels (grid.hpp:26-63 of the reference).
+struct Geom {
+  int nx, ny, nz, nzh;
+  long N, M, nxy;
+  double dx, dv;
+  double dkx, dky, dkz;
+  double inv_n;
+  double R;  // hard-sphere radius of the species
+};
+
+// Forward channels: 11 FMT channels when `hs`, then the mean-field channel
+// when `mf`; nf = 11*hs + mf.  Backward channels nb = 11*hs.
+struct ChannelSet {
+  int hs, mf, nf, nb;
+};
+
+// Density parametrisation applied on load (minimization.hpp:45-59):
+// kind 0: rho = x; 1: rho = rho_min + x^2; 2: rho_min + range x^2/(1+x^2).
+struct AliasMap {
+  int kind;
+  double rho_min, range;
+  __host__ __device__ double apply(double x) const {
+    if (kind == 0) return x;
+    const double x2 = x * x;
+    return kind == 1 ? rho_min + x2 : rho_min + range * x2 / (1.0 + x2);
+  }
+  // d rho / d x (minimization.hpp:82-98)
+  __host__ __device__ double deriv(double x) const {
+    if (kind == 0) return 1.0;
+    if (kind == 1) return 2.0 * x;
+    const double x2 = x * x;
+    return 2.0 * range * x / ((1.0 + x2) * (1.0 + x2));
+  }
+};
+
+template <typename R>
+void launch_zr2c(const double* in, cx<R>* out, const Geom& g, const FftPlan& pz, const cx<R>* twz, AliasMap amap,
+                 hipStream_t st);
+template <typename R>
+void launch_zc2r(const cx<R>* S, int nch, double* out, const double* add, double scale, const Geom& g,
+                 const FftPlan& pz, const cx<R>* twz, hipStream_t st);
+template <typename R>
+void launch_lines(cx<R>* S, long nlines, const FftPlan& p, const cx<R>* tw, int sg, hipStream_t st);
+template <typename R>
+void launch_xlines(cx<R>* S, int nch, const Geom& g, const FftPlan& px, const cx<R>* twx, int sg, hipStream_t st);
+template <typename R>
+void launch_xfwd(const cx<R>* S0, cx<R>* SF, const R* mf, const Geom& g, const FftPlan& px, const cx<R>* twx,
+                 ChannelSet cs, hipStream_t st);
+template <typename R>
+void launch_xbwd(const cx<R>* SB, cx<R>* SO, const Geom& g, const FftPlan& px, const cx<R>* twx, int nb,
+                 hipStream_t st);
+template <typename R>
+int zpoint_blocks(const Geom& g);
+template <typename R>
+void launch_zpoint(cx<R>* SF, const double* rho, const double* vext, double* local_force, double* partials,
+                   const Geom& g, const FftPlan& pz, const cx<R>* twz, ChannelSet cs, FmtParams fp, double mu,
+                   AliasMap amap, hipStream_t st);
+void launch_finalize(const double* partials, int nblk, double* res, hipStream_t st);
+
+}  // namespace cdft

@@ -1,0 +1,80 @@
+"""Seeded synthetic inputs for the benchmark and parity workloads.
+
+A project-owned SplitMix64 generator keeps every input reproducible across
+machines and languages (no implementation-defined std:: distributions).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+_M1 = np.uint64(0xBF58476D1CE4E5B9)
+_M2 = np.uint64(0x94D049BB133111EB)
+
+
+def splitmix64(seed: int, n: int) -> np.ndarray:
+    """n consecutive SplitMix64 outputs (uint64) from `seed`."""
+    with np.errstate(over="ignore"):
+        state = np.uint64(seed) + _GOLDEN * (np.arange(1, n + 1, dtype=np.uint64))
+        z = state
+        z = (z ^ (z >> np.uint64(30))) * _M1
+        z = (z ^ (z >> np.uint64(27))) * _M2
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform(seed: int, n: int) -> np.ndarray:
+    """53-bit uniforms in [0, 1)."""
+    return (splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+def normal(seed: int, n: int) -> np.ndarray:
+    """Box-Muller standard normals."""
+    m = (n + 1) // 2
+    u = uniform(seed, 2 * m)
+    u1 = np.maximum(u[0::2], 1e-300)
+    u2 = u[1::2]
+    r = np.sqrt(-2.0 * np.log(u1))
+    z = np.concatenate([r * np.cos(2 * math.pi * u2), r * np.sin(2 * math.pi * u2)])
+    return z[:n]
+
+
+def droplet(shape, dx, radius=4.0, width=0.5, rho_l=0.70, rho_v=0.02, noise=1e-3, seed=2024):
+    """Spherical tanh droplet at the box centre plus uniform noise in [-noise, noise]."""
+    nx, ny, nz = shape
+    ax = [(np.arange(n) - n / 2) * dx for n in shape]
+    X, Y, Z = np.meshgrid(*ax, indexing="ij")
+    r = np.sqrt(X * X + Y * Y + Z * Z)
+    rho = rho_v + 0.5 * (rho_l - rho_v) * (1.0 - np.tanh((r - radius) / width))
+    rho += noise * (2.0 * uniform(seed, nx * ny * nz).reshape(shape) - 1.0)
+    return rho
+
+
+def slab(shape, dx, fraction=0.4, width=0.5, rho_l=0.70, rho_v=0.02, noise=1e-3, seed=2025, axis=0):
+    """Planar liquid slab over `fraction` of the box along `axis`, tanh interfaces."""
+    n = shape[axis]
+    L = n * dx
+    x = (np.arange(n) + 0.5) * dx - L / 2
+    half = fraction * L / 2
+    prof = rho_v + 0.5 * (rho_l - rho_v) * (np.tanh((x + half) / width) - np.tanh((x - half) / width))
+    sh = [1, 1, 1]
+    sh[axis] = n
+    rho = np.broadcast_to(prof.reshape(sh), shape).copy()
+    rho += noise * (2.0 * uniform(seed, int(np.prod(shape))).reshape(shape) - 1.0)
+    return rho
+
+
+def smooth_noise_fluid(shape, dx, rho0=0.5, amp=0.01, sigma=0.3, seed=7):
+    """rho0 (1 + amp xi), xi = N(0,1) noise smoothed by a periodic Gaussian of width sigma, unit RMS."""
+    xi = normal(seed, int(np.prod(shape))).reshape(shape)
+    k = [2 * math.pi * np.fft.fftfreq(n, d=dx) for n in shape]
+    KX, KY, KZ = np.meshgrid(*k, indexing="ij")
+    g = np.exp(-0.5 * sigma * sigma * (KX ** 2 + KY ** 2 + KZ ** 2))
+    xs = np.real(np.fft.ifftn(np.fft.fftn(xi) * g))
+    xs /= np.sqrt(np.mean(xs * xs))
+    return rho0 * (1.0 + amp * xs)
+
+
+LJ_WCA = dict(potential="LJ", sigma=1.0, epsilon=1.0, r_cutoff=2.5, split="WCA", scheme="quadraticF")

@@ -1,0 +1,187 @@
+/*
+ * cdft.h - C ABI of the MI355X (gfx950) free-energy + gradient library.
+ *
+ * This is the drop-in boundary for the hot path of migduroli/classicalDFT:
+ * the functions below replace, one for one, the reference C++ entry points
+ * cited beside them.  Plain pointers and sizes only: host arrays are borrowed
+ * for the duration of a call; `_device` variants take device pointers that
+ * the caller owns (allocated with hipMalloc on the context's device).
+ *
+ * Conventions (all mirror the reference):
+ *   - real fields are double, length N = Nx*Ny*Nz, idx = iz + Nz*(iy + Ny*ix)
+ *     (include/dft/grid.hpp:24-37);
+ *   - forces are delta(beta Omega)/delta rho * dV (functionals.hpp:55-63);
+ *   - mu and V_ext are in kT units;
+ *   - every function returns CDFT_OK (0) or a negative status; the message of
+ *     the last failure on the calling thread is cdft_last_error().
+ *     CDFT_EINVAL is the reference's std::invalid_argument, CDFT_ERUNTIME its
+ *     std::runtime_error (for example FIRE's "exceeded max uphill steps",
+ *     include/dft/algorithms/fire.hpp:106-108).
+ *   - a context is bound to one device and one HIP stream and is not
+ *     thread-safe; calls on one context must be serialised by the caller.
+ */
+#ifndef CDFT_H
+#define CDFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__) || defined(__clang__)
+#define CDFT_API __attribute__((visibility("default")))
+#else
+#define CDFT_API
+#endif
+
+#define CDFT_OK 0
+#define CDFT_EINVAL (-1)
+#define CDFT_ERUNTIME (-2)
+#define CDFT_EHIP (-3)
+#define CDFT_EUNSUPPORTED (-4)
+
+typedef struct cdft_ctx cdft_ctx;
+
+/* dft::make_grid(dx, box) (include/dft/grid.hpp:87-108): each box length must
+ * be an integer multiple of dx within 1e-10. */
+typedef struct {
+  double dx;
+  double box[3];
+} cdft_grid;
+
+/* physics::Interaction (include/dft/physics/interactions.hpp:23-30) with its
+ * potential (include/dft/physics/potentials.hpp:53-368). */
+typedef struct {
+  int species_i, species_j;
+  int potential;     /* 0 Lennard-Jones, 1 ten Wolde-Frenkel, 2 Wang-Ramirez-Dobnikar-Frenkel */
+  double sigma, epsilon, r_cutoff, alpha;
+  int split;         /* 0 Weeks-Chandler-Andersen, 1 Barker-Henderson */
+  int weight_scheme; /* 0 zero, 1 linear-E, 2 linear-F, 3 quadratic-F (default), 4 Gauss-E, 5 Gauss-F */
+} cdft_interaction;
+
+#define CDFT_FMT_NONE (-1)
+#define CDFT_FMT_ROSENFELD 0
+#define CDFT_FMT_RSLT 1
+#define CDFT_FMT_WHITEBEAR_I 2
+#define CDFT_FMT_WHITEBEAR_II 3
+#define CDFT_FMT_ESFMT 4
+
+#define CDFT_PRECISION_FP64 0
+#define CDFT_PRECISION_MIXED 1
+
+typedef struct {
+  int fmt_model;          /* fmt::FMTModel (include/dft/functionals/fmt/models.hpp:363-419) */
+  double esfmt_A, esfmt_B; /* esFMT(A, B), models.hpp:314-358 */
+  int precision;          /* CDFT_PRECISION_FP64 or CDFT_PRECISION_MIXED */
+  int device;             /* HIP device ordinal */
+} cdft_options;
+
+/* Replaces functionals::make_weights + make_functional
+ * (include/dft/functionals/functionals.hpp:27-33, functional.hpp:195-206):
+ * builds the FMT and mean-field weights for a physics::Model. */
+CDFT_API int cdft_ctx_create(const cdft_grid* grid, const double* diameters, int n_species,
+                    const cdft_interaction* interactions, int n_interactions, double kT,
+                    const cdft_options* options, cdft_ctx** out);
+CDFT_API void cdft_ctx_destroy(cdft_ctx* ctx);
+CDFT_API const char* cdft_last_error(void);
+
+/* Grid::shape (grid.hpp:26-33). */
+CDFT_API int cdft_grid_shape(const cdft_ctx* ctx, long shape[3]);
+/* InteractionWeight::a_vdw of interaction k (mean_field.hpp:23-28). */
+CDFT_API int cdft_a_vdw(const cdft_ctx* ctx, int k, double* a_vdw);
+/* BulkThermodynamics::chemical_potential for a one-component fluid with the
+ * grid-synchronised a_vdw (bulk/thermodynamics.hpp:222-237, functional.hpp:196-205). */
+CDFT_API int cdft_bulk_chemical_potential(const cdft_ctx* ctx, double rho, double* mu);
+
+/* functionals::Result plus the per-term split of free_energy
+ * (functionals.hpp:64-95, types.hpp:21-32). */
+typedef struct {
+  double free_energy;
+  double grand_potential;
+  double ideal;
+  double external;
+  double hard_sphere;
+  double mean_field;
+  double mass; /* sum rho dV */
+} cdft_result;
+
+/* functionals::total(model, state, weights) (functionals.hpp:64): one
+ * F + delta F / delta rho evaluation.  rho, vext (nullable: zero field) and
+ * forces (nullable) are host arrays of N doubles; mu is beta*mu. */
+CDFT_API int cdft_total(cdft_ctx* ctx, const double* rho, const double* vext, double mu, cdft_result* result,
+               double* forces);
+/* Same with device-resident arrays; returns when the result is complete. */
+CDFT_API int cdft_total_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu, cdft_result* result,
+                      double* d_forces);
+/* Enqueues `count` evaluations back to back on the context stream without
+ * host synchronisation (throughput measurement); the result of the last is
+ * returned. */
+CDFT_API int cdft_total_device_repeat(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu, int count,
+                             cdft_result* result, double* d_forces);
+
+/* hard_sphere.hpp:50-67 (convolve_weights) and mean_field.hpp:259: the 12
+ * weighted-density fields eta, n2, n_v2[x,y,z], T[xx,xy,xz,yy,yz,zz] and the
+ * mean-field convolution [w * rho] (host arrays, 12*N doubles). */
+CDFT_API int cdft_weighted_densities(cdft_ctx* ctx, const double* rho, double* out);
+
+/* math::FourierTransform::forward / backward (src/math/fourier.cpp:97-103):
+ * unnormalised 3D r2c / c2r in the reference half-complex layout
+ * idx = kz + (Nz/2+1)*(iy + Ny*ix), complex as interleaved (re, im). */
+CDFT_API int cdft_fft_r2c(cdft_ctx* ctx, const double* in, double* out_half);
+CDFT_API int cdft_fft_c2r(cdft_ctx* ctx, const double* in_half, double* out);
+
+/* fire::Fire (include/dft/algorithms/fire.hpp:33-44). */
+typedef struct {
+  double dt, dt_max, dt_min, alpha_start, f_inc, f_dec, f_alpha;
+  int n_delay, max_uphill;
+  double force_tolerance;
+  int max_steps;
+} cdft_fire;
+
+/* minimization::Parametrization (minimization.hpp:20-39): kind 0 Unbounded
+ * (rho = rho_min + x^2), kind 1 Bounded (rho_min + (rho_max-rho_min) x^2/(1+x^2)). */
+typedef struct {
+  int kind;
+  double rho_min, rho_max;
+} cdft_parametrization;
+
+/* minimization::Result (minimization.hpp:114-121) plus the final FIRE dt, alpha. */
+typedef struct {
+  double free_energy, grand_potential, rms_force;
+  int iterations, converged;
+  double dt, alpha;
+} cdft_min_result;
+
+/* Minimizer::grand_potential (minimization.hpp:280-353): FIRE2 in the alias
+ * variable at fixed mu.  vext (nullable) adds an external field, which the
+ * reference's from_profile path always zeroes (init.hpp:264-279).
+ * energies (nullable, max_steps + 1 doubles) receives state.energy after
+ * initialisation and after every step.  Returns CDFT_ERUNTIME when FIRE
+ * exceeds max_uphill. */
+CDFT_API int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire, const cdft_parametrization* param,
+                                  int use_homogeneous_boundary, const double* rho0, const double* vext, double mu,
+                                  double* rho_out, cdft_min_result* result, double* energies);
+
+/* dynamics::StepConfig (include/dft/algorithms/dynamics.hpp:26-33). */
+typedef struct {
+  double dt, diffusion_coefficient, min_density, dt_max, fp_tolerance;
+  int fp_max_iterations;
+} cdft_step_config;
+
+/* Simulation::run summary (dynamics.hpp:54-61). */
+typedef struct {
+  double mass_initial, mass_final, time, energy, dt;
+  long evaluations;
+} cdft_sim_result;
+
+/* Simulation::run with integrating_factor_step (dynamics.hpp:333-401,582-693)
+ * for one species, force callback = grand_potential_callback(mu)
+ * (functional.hpp:161-172), plus an optional external field.  rho is updated
+ * in place; energies (nullable, n_steps + 1) receives the energy per step. */
+CDFT_API int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
+                  double* rho, cdft_sim_result* result, double* energies);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CDFT_H */

@@ -1,0 +1,132 @@
+"""GPU parity of the F + dF/drho evaluation against the FP64 CPU oracle.
+
+All calls go through the C ABI (libcdft.so).  Tolerances:
+  FP64 path   - transforms and every output to 1e-11 relative (max-norm);
+  mixed path  - Omega to 1e-6 relative (the stated gate), forces and
+                weighted densities to 2e-5 of their max-norm.
+"""
+import numpy as np
+import pytest
+
+from classicaldft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+LJ = W.LJ_WCA
+
+
+def _ctx(shape, dx, model="WhiteBearII", inter=(LJ,), kT=0.8, precision="fp64", **kw):
+    import classicaldft_amd as cd
+
+    return cd.Context(shape, dx, interactions=list(inter), kT=kT, model=model, precision=precision, **kw)
+
+
+def _orc(shape, dx, model="WhiteBearII", inter=(LJ,), kT=0.8, **kw):
+    from oracle import oracle as O
+
+    return O.Oracle(shape, dx, interactions=list(inter), kT=kT, model=model, **kw)
+
+
+def _relmax(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-6)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 10, 4), (12, 8, 10), (16, 24, 20), (32, 32, 32), (5, 9, 15)])
+def test_fft_r2c_c2r(shape, precision, tol):
+    rng = np.random.default_rng(11)
+    ctx = _ctx(shape, 0.1, model="none", inter=(), precision=precision)
+    orc = _orc(shape, 0.1, model="none", inter=())
+    x = rng.standard_normal(shape)
+    X = ctx.r2c(x)
+    Xr = orc.r2c(x)
+    print(f"r2c {shape} {precision}: {_relmax(X, Xr):.3e}", flush=True)
+    assert _relmax(X, Xr) < tol
+    Z = Xr + 0.3 * (rng.standard_normal(Xr.shape) + 1j * rng.standard_normal(Xr.shape)) * np.max(np.abs(Xr))
+    y = ctx.c2r(Z)
+    yr = orc.c2r(Z)
+    print(f"c2r {shape} {precision}: {_relmax(y, yr):.3e}", flush=True)
+    assert _relmax(y, yr) < tol
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
+def test_weighted_densities(precision, tol):
+    shape, dx = (16, 20, 24), 0.1
+    rho = W.droplet(shape, dx, radius=0.7, width=0.2)
+    ctx = _ctx(shape, dx, precision=precision)
+    orc = _orc(shape, dx)
+    wd = ctx.weighted_densities(rho)
+    wr = orc.weighted_densities(rho)
+    for c in range(12):
+        e = _relmax(wd[c], wr[c])
+        print(f"channel {c} {precision}: {e:.3e}", flush=True)
+        assert e < tol, c
+
+
+@pytest.mark.parametrize("model", ["WhiteBearII", "WhiteBearI", "Rosenfeld", "RSLT", "esFMT"])
+def test_total_fp64_all_models(model):
+    shape, dx = (16, 16, 20), 0.1
+    rho = W.droplet(shape, dx, radius=0.6, width=0.2, seed=3)
+    V = 0.5 * np.cos(2 * np.pi * np.arange(shape[2]) / shape[2])[None, None, :] * np.ones(shape)
+    kw = dict(A=1.0, B=-0.5) if model == "esFMT" else {}
+    ctx = _ctx(shape, dx, model=model, **kw)
+    orc = _orc(shape, dx, model=model, **kw)
+    mu = -2.1
+    res, f = ctx.total(rho, mu, vext=V)
+    parts, fr = orc.total(rho, mu, vext=V)
+    for key, i in [("ideal", 0), ("external", 1), ("hard_sphere", 2), ("mean_field", 3), ("free_energy", 4),
+                   ("grand_potential", 5)]:
+        assert res[key] == pytest.approx(parts[i], rel=1e-11, abs=1e-12), key
+    e = _relmax(f, fr)
+    print(f"{model} force rel err {e:.3e}", flush=True)
+    assert e < 1e-11
+
+
+def test_total_mixed_droplet_gate():
+    """The metric workload's physics on a 64^3 box: Omega within 1e-6 of the FP64 oracle."""
+    shape, dx = (64, 64, 64), 0.1
+    rho = W.droplet(shape, dx, radius=2.0, width=0.5, seed=5)
+    orc = _orc(shape, dx)
+    mu = orc.bulk_mu(0.02)
+    parts, fr = orc.total(rho, mu)
+    for precision, otol, ftol in [("fp64", 1e-12, 1e-11), ("mixed", 1e-6, 2e-5)]:
+        ctx = _ctx(shape, dx, precision=precision)
+        res, f = ctx.total(rho, mu)
+        eo = abs(res["grand_potential"] - parts[5]) / abs(parts[5])
+        ef = _relmax(f, fr)
+        print(f"{precision}: Omega rel err {eo:.3e}, force rel err {ef:.3e}", flush=True)
+        assert eo < otol
+        assert ef < ftol
+
+
+def test_uniform_fluid_known_answers():
+    # hard_sphere.cpp:52-163 and mean_field.cpp:92-161 of the reference tests
+    shape, dx = (16, 16, 16), 0.1
+    ctx = _ctx(shape, dx, kT=1.0)
+    rho0 = 0.5
+    mu = ctx.bulk_mu(rho0)
+    res, f = ctx.total(np.full(shape, rho0), mu)
+    assert np.max(np.abs(f)) < 1e-12
+    orc = _orc(shape, dx, kT=1.0)
+    assert ctx.a_vdw() == pytest.approx(orc.a_vdw(), rel=1e-13)
+    assert mu == pytest.approx(orc.bulk_mu(rho0), rel=1e-13)
+    V = np.prod(shape) * dx ** 3
+    assert res["mean_field"] == pytest.approx(0.5 * ctx.a_vdw() * rho0 ** 2 * V, rel=1e-12)
+
+
+def test_zero_density_and_energy_only():
+    shape, dx = (8, 8, 8), 0.2
+    ctx = _ctx(shape, dx)
+    res, f = ctx.total(np.zeros(shape), 0.0)
+    assert res["hard_sphere"] == 0.0
+    rho = W.droplet(shape, dx, radius=0.5, width=0.2)
+    r1, _ = ctx.total(rho, -1.0)
+    r2, _ = ctx.total(rho, -1.0, forces=False)
+    assert r1["grand_potential"] == pytest.approx(r2["grand_potential"], rel=1e-14)
+
+
+def test_unsupported_length_raises():
+    import classicaldft_amd as cd
+
+    with pytest.raises(cd.CdftError):
+        cd.Context((7, 8, 8), 0.1, model="WhiteBearII")
