@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""F + dF/drho evaluations per second on one MI355X (classicalDFT hot path).
+
+Workload (the metric's configuration): Lennard-Jones fluid with WCA split,
+cutoff 2.5 sigma, kT = 0.8 eps, White-Bear II FMT with d = sigma, periodic
+128^3 grid, dx = 0.1 sigma; a spherical droplet of radius 4 sigma (tanh width
+0.5 sigma) between rho_l = 0.70 and rho_v = 0.02 plus seeded 1e-3 uniform
+noise; mu = bulk mu(rho_v).  One step = one full evaluation of Omega, F and
+delta(beta Omega)/delta rho * dV (functionals::total of the reference) on
+device-resident inputs.
+
+    python bench.py [--steps K] [--warmup W] [--precision mixed|fp64] [--no-cpu-baseline]
+
+Prints one JSON line.  Only N=1 is supported (single-GPU build).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+METRIC = "F+δF/δρ evals/s at 128³: mixed vs FP64 on GPU, Ω rel err ≤1e-6"
+PEAK_HBM_GBPS = 8000.0
+
+
+def eval_bytes(shape, s_f, s_s):
+    """Algorithmic bytes of one evaluation, stage-boundary model: N (46 s_f + 6 s_s) + 5 s_f M."""
+    nx, ny, nz = shape
+    N = nx * ny * nz
+    M = nx * ny * (nz // 2 + 1)
+    return N * (46 * s_f + 6 * s_s) + 5 * s_f * M
+
+
+def cpu_baseline(rho, mu, shape, dx, budget_s=15.0):
+    """Time the FP64 CPU oracle (a port of the reference algorithm) on the host cores."""
+    from oracle import oracle as O
+    from classicaldft_amd import workloads as W
+
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    threads = O.max_threads()
+    parts, forces = orc.total(rho, mu)  # warm-up, also the FP64 reference answer
+    times = []
+    t_start = time.perf_counter()
+    while len(times) < 8 and (time.perf_counter() - t_start) < budget_s:
+        t0 = time.perf_counter()
+        orc.total(rho, mu)
+        times.append(time.perf_counter() - t0)
+    times.sort()
+    med = times[len(times) // 2]
+    # reference-faithful variant: the per-point Phi/dPhi loop on one thread (hard_sphere.hpp:185-221)
+    orc.set_serial_pointwise(True)
+    t0 = time.perf_counter()
+    orc.total(rho, mu)
+    t_faithful = time.perf_counter() - t0
+    return {
+        "value": 1.0 / med,
+        "unit": "evals/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{len(times)} timed FP64 oracle evaluations of the same 128^3 workload (median), after 1 warm-up; "
+                  f"FFTs and pointwise loop on {threads} OpenMP threads",
+        "reference_faithful_evals_per_s": 1.0 / t_faithful,
+        "nproc": os.cpu_count(),
+        "omp_num_threads": os.environ.get("OMP_NUM_THREADS"),
+    }, parts, forces
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the same-GPU FP64 comparison run")
+    ap.add_argument("--profile-reps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import classicaldft_amd as cd
+    from classicaldft_amd import workloads as W
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(3)
+
+    shape, dx, kT = (128, 128, 128), 0.1, 0.8
+    rho = W.droplet(shape, dx, radius=4.0, width=0.5, rho_l=0.70, rho_v=0.02, noise=1e-3, seed=2024)
+    dev = torch.device("cuda:0")
+    d_rho = torch.from_numpy(rho.copy()).to(dev)
+    d_f = torch.empty_like(d_rho)
+
+    def run(precision, steps, warmup):
+        ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=kT, model="WhiteBearII", precision=precision)
+        mu = ctx.bulk_mu(0.02)
+        ctx.time_total_device(d_rho.data_ptr(), mu, warmup, d_forces=d_f.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ms = ctx.time_total_device(d_rho.data_ptr(), mu, steps, d_forces=d_f.data_ptr())
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res = ctx.total_device(d_rho.data_ptr(), mu, d_forces=d_f.data_ptr())
+        forces = d_f.cpu().numpy().reshape(shape)
+        return ctx, mu, ms, wall, res, forces
+
+    ctx, mu, ms, wall, res, forces = run(args.precision, args.steps, args.warmup)
+    ms_per_step = ms / args.steps
+    value = 1000.0 / ms_per_step
+
+    stages = ctx.profile_stages(d_rho.data_ptr(), mu, reps=args.profile_reps, d_forces=d_f.data_ptr())
+    timed = [s for s in stages if s[0] not in ("k_finalize",)]
+    dom = max(timed, key=lambda s: s[1])
+    achieved = dom[2] / (dom[1] * 1e-3) / 1e9
+
+    s_f = 4 if args.precision == "mixed" else 8
+    B = eval_bytes(shape, s_f, 8)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 fields+FFT / fp64 pointwise, reductions, state" if args.precision == "mixed" else "fp64",
+        "data": "synthetic",
+        "config": {
+            "workload": "LJ droplet F+dF/drho eval: 128^3, dx=0.1, WB-II FMT d=1, LJ WCA rc=2.5, kT=0.8, "
+                        "R=4 tanh(0.5) rho_l=0.70 rho_v=0.02, 1e-3 noise, mu=mu_bulk(rho_v)",
+            "grid": list(shape),
+            "precision": args.precision,
+            "field_bytes": s_f,
+            "state_bytes": 8,
+        },
+        "wall_ms_per_step": wall * 1000.0 / args.steps,
+        "eval_algorithmic_bytes": B,
+        "eval_achieved_GBps": B * value / 1e9,
+        "eval_roofline_frac": B * value / 1e9 / PEAK_HBM_GBPS,
+        "roofline": {
+            "kernel": dom[0],
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS,
+            "traffic": None,
+            "kernel_ms": dom[1],
+            "kernel_algorithmic_bytes": dom[2],
+        },
+        "stages": [{"kernel": n, "ms": round(t, 5), "bytes": b} for n, t, b in stages],
+        "omega": res["grand_potential"],
+    }
+
+    if not args.no_fp64 and args.precision == "mixed":
+        ctx64, mu64, ms64, _, res64, f64 = run("fp64", max(10, args.steps // 4), max(2, args.warmup // 4))
+        out["fp64_value"] = 1000.0 * max(10, args.steps // 4) / ms64
+        out["speedup_vs_fp64"] = value / out["fp64_value"]
+        out["omega_fp64_gpu"] = res64["grand_potential"]
+        out["omega_rel_err_vs_fp64_gpu"] = abs(res["grand_potential"] - res64["grand_potential"]) / abs(
+            res64["grand_potential"])
+        out["force_rel_err_vs_fp64_gpu"] = float(np.max(np.abs(forces - f64)) / np.max(np.abs(f64)))
+        del ctx64
+
+    if not args.no_cpu_baseline:
+        cb, parts, fr = cpu_baseline(rho, mu, shape, dx)
+        out["cpu_baseline"] = cb
+        out["omega_oracle"] = parts[5]
+        out["omega_rel_err_vs_oracle"] = abs(res["grand_potential"] - parts[5]) / abs(parts[5])
+        out["force_rel_err_vs_oracle"] = float(np.max(np.abs(forces - fr)) / np.max(np.abs(fr)))
+        out["gpu_over_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

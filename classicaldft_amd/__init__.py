@@ -24,6 +24,7 @@ EXPORTED = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_grid_shape", "cdft_a_vdw",
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
+    "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name",
 ]
 
 
@@ -114,6 +115,12 @@ def lib():
                                                     _dp, C.c_double, _dp, C.POINTER(_MinResult), _dp]
         L.cdft_ddft_run.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp,
                                     C.POINTER(_SimResult), _dp]
+        L.cdft_time_total_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
+                                             _dp]
+        L.cdft_profile_stages.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int, _dp,
+                                          _dp, C.POINTER(C.c_int)]
+        L.cdft_stage_name.restype = C.c_char_p
+        L.cdft_stage_name.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -205,6 +212,24 @@ class Context:
                                               float(mu), int(count), C.byref(r),
                                               C.c_void_p(d_forces) if d_forces else None))
         return {k: getattr(r, k) for k, _ in _Result._fields_}
+
+    def time_total_device(self, d_rho, mu, count, d_vext=None, d_forces=None):
+        """GPU milliseconds for `count` back-to-back evaluations (HIP events on the context stream)."""
+        ms = C.c_double()
+        _check(lib().cdft_time_total_device(self._h, C.c_void_p(d_rho), C.c_void_p(d_vext) if d_vext else None,
+                                            float(mu), int(count), C.c_void_p(d_forces) if d_forces else None,
+                                            C.byref(ms)))
+        return ms.value
+
+    def profile_stages(self, d_rho, mu, reps=10, d_vext=None, d_forces=None):
+        """Per-kernel average ms and algorithmic bytes of one evaluation: list of (name, ms, bytes)."""
+        ms = (C.c_double * 16)()
+        by = (C.c_double * 16)()
+        n = C.c_int()
+        _check(lib().cdft_profile_stages(self._h, C.c_void_p(d_rho), C.c_void_p(d_vext) if d_vext else None,
+                                         float(mu), C.c_void_p(d_forces) if d_forces else None, int(reps), ms, by,
+                                         C.byref(n)))
+        return [(lib().cdft_stage_name(i).decode(), ms[i], by[i]) for i in range(n.value)]
 
     def weighted_densities(self, rho):
         """Weighted densities: 11 FMT channels (when an FMT model is set) then the mean-field convolution."""

@@ -154,6 +154,64 @@ int cdft_total_device_repeat(cdft_ctx* ctx, const double* d_rho, const double* d
   });
 }
 
+int cdft_time_total_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu, int count,
+                           double* d_forces, double* ms) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!d_rho || !ms || count < 1) throw std::invalid_argument("cdft_time_total_device: bad argument");
+    Context* c = ctx->impl;
+    CDFT_HIP(hipSetDevice(c->device));
+    hipEvent_t e0, e1;
+    CDFT_HIP(hipEventCreate(&e0));
+    CDFT_HIP(hipEventCreate(&e1));
+    CDFT_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < count; ++i) c->eval(d_rho, d_vext, mu, d_forces, AliasMap{0, 0.0, 0.0});
+    CDFT_HIP(hipEventRecord(e1, c->stream));
+    CDFT_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    CDFT_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+int cdft_profile_stages(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu, double* d_forces,
+                        int reps, double* stage_ms, double* stage_bytes, int* n_stages) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!d_rho || !stage_ms || reps < 1) throw std::invalid_argument("cdft_profile_stages: bad argument");
+    Context* c = ctx->impl;
+    CDFT_HIP(hipSetDevice(c->device));
+    hipEvent_t ev[kNumStages + 1];
+    for (auto& e : ev) CDFT_HIP(hipEventCreate(&e));
+    for (int i = 0; i < kNumStages; ++i) stage_ms[i] = 0.0;
+    c->stage_events = ev;
+    try {
+      for (int r = 0; r < reps; ++r) {
+        // events of skipped stages repeat the previous timestamp
+        for (auto& e : ev) CDFT_HIP(hipEventRecord(e, c->stream));
+        c->eval(d_rho, d_vext, mu, d_forces, AliasMap{0, 0.0, 0.0});
+        CDFT_HIP(hipEventSynchronize(ev[kNumStages]));
+        for (int i = 0; i < kNumStages; ++i) {
+          float t = 0.0f;
+          CDFT_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+          stage_ms[i] += t / reps;
+        }
+      }
+    } catch (...) {
+      c->stage_events = nullptr;
+      throw;
+    }
+    c->stage_events = nullptr;
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (stage_bytes) c->stage_bytes(stage_bytes, d_vext != nullptr);
+    if (n_stages) *n_stages = kNumStages;
+  });
+}
+
+const char* cdft_stage_name(int i) { return (i >= 0 && i < kNumStages) ? kStageNames[i] : ""; }
+
 int cdft_weighted_densities(cdft_ctx* ctx, const double* rho, double* out) {
   return guarded([&] {
     check_ctx(ctx);

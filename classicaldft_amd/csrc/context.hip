@@ -47,7 +47,7 @@ void* dalloc_bytes(size_t bytes) {
 }  // namespace
 
 Context::~Context() {
-  void* bufs[] = {d_rho, d_vext, d_force, d_local, d_S0, d_SF, d_SO, d_mf, d_partials, d_res,
+  void* bufs[] = {d_rho, d_vext, d_force, d_local, d_S0, d_SF, d_SO, d_mf, d_partials, d_res, d_sum_part, d_shift,
                   tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2]};
   hipSetDevice(device);
   for (void* b : bufs)
@@ -169,6 +169,9 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g) : zpoint_blocks<double>(g);
     c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
     c->d_res = dalloc<double>(8);
+    c->nblk_sum = sum_blocks(g);
+    c->d_sum_part = dalloc<double>(8 * static_cast<size_t>(c->nblk_sum));
+    c->d_shift = dalloc<double>(8);
 
     // Mean-field weight: real-space table on the host (FP64), R2C on the device
     // (FP64 pipeline), real part kept in the working precision.  The table is
@@ -184,7 +187,7 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
         CDFT_HIP(hipMemcpy(d_tab, table.data(), sizeof(double) * g.N, hipMemcpyHostToDevice));
         auto* S = static_cast<cx<double>*>(d_spec);
         const AliasMap id{0, 0.0, 0.0};
-        launch_zr2c<double>(d_tab, S, g, c->plan[2], c->tw<double>(2), id, c->stream);
+        launch_zr2c<double>(d_tab, S, g, c->plan[2], c->tw<double>(2), id, nullptr, c->stream);
         launch_lines<double>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<double>(1), -1, c->stream);
         launch_xlines<double>(S, 1, g, c->plan[0], c->tw<double>(0), -1, c->stream);
         CDFT_HIP(hipGetLastError());
@@ -208,12 +211,28 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       CDFT_HIP(hipFree(d_tab));
       CDFT_HIP(hipFree(d_spec));
     }
+    // k = 0 weights of the forward channels (weights.hpp:174-222, mean_field.hpp:189-191)
+    {
+      const double R = g.R;
+      int ch = 0;
+      if (c->cs.hs) {
+        c->dc.v[0] = 4.0 * PI / 3.0 * R * R * R;
+        c->dc.v[1] = 4.0 * PI * R * R;
+        c->dc.v[5] = c->dc.v[8] = c->dc.v[10] = 4.0 * PI / 3.0 * R * R;
+        ch = 11;
+      }
+      if (c->cs.mf) c->dc.v[ch] = c->a_vdw[0];
+    }
   } catch (...) {
     delete c;
     throw;
   }
   return c;
 }
+
+const char* const kStageNames[kNumStages] = {"k_sum",    "k_zr2c",            "k_lines(y fwd rho)", "k_xfwd",
+                                              "k_lines(y inv nf)", "k_zpoint", "k_lines(y fwd nb)", "k_xbwd",
+                                              "k_lines(y inv 1)",  "k_zc2r",   "k_finalize"};
 
 template <typename R>
 void Context::eval_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
@@ -222,23 +241,66 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
   auto* SO = static_cast<cx<R>*>(d_SO);
   const long ylines = static_cast<long>(g.nzh) * g.nx;
   const bool backward = force != nullptr && cs.nb > 0;
+  auto mark = [&](int i) {
+    if (stage_events) CDFT_HIP(hipEventRecord(stage_events[i], stream));
+  };
+  mark(0);
   if (cs.nf > 0) {
-    launch_zr2c<R>(rho, S0, g, plan[2], tw<R>(2), amap, stream);
+    launch_sum(rho, amap, d_sum_part, g, stream);
+    launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
+    mark(1);
+    launch_zr2c<R>(rho, S0, g, plan[2], tw<R>(2), amap, d_shift, stream);
+    mark(2);
     launch_lines<R>(S0, ylines, plan[1], tw<R>(1), -1, stream);
+    mark(3);
     launch_xfwd<R>(S0, SF, static_cast<const R*>(d_mf), g, plan[0], tw<R>(0), cs, stream);
+    mark(4);
     launch_lines<R>(SF, ylines * cs.nf, plan[1], tw<R>(1), +1, stream);
   }
   ChannelSet cz = cs;
   if (!backward) cz.nb = 0;
   double* lf = backward ? d_local : (force ? force : d_local);
-  launch_zpoint<R>(SF, rho, vext, lf, d_partials, g, plan[2], tw<R>(2), cz, fp, mu, amap, stream);
+  mark(5);
+  launch_zpoint<R>(SF, rho, vext, lf, d_partials, g, plan[2], tw<R>(2), cz, fp, mu, amap,
+                   cs.nf > 0 ? d_shift : nullptr, dc, stream);
   if (backward) {
+    mark(6);
     launch_lines<R>(SF, ylines * cs.nb, plan[1], tw<R>(1), -1, stream);
+    mark(7);
     launch_xbwd<R>(SF, SO, g, plan[0], tw<R>(0), cs.nb, stream);
+    mark(8);
     launch_lines<R>(SO, ylines, plan[1], tw<R>(1), +1, stream);
-    launch_zc2r<R>(SO, 1, force, d_local, g.dv, g, plan[2], tw<R>(2), stream);
+    mark(9);
+    launch_zc2r<R>(SO, 1, force, d_local, g.dv, g, plan[2], tw<R>(2), nullptr, DcTable{}, stream);
   }
+  mark(10);
   launch_finalize(d_partials, nblk_point, d_res, stream);
+  mark(11);
+}
+
+// Algorithmic HBM bytes of each stage for one evaluation (stage-boundary
+// model: each kernel reads its inputs and writes its outputs once).
+void Context::stage_bytes(double* b, bool with_vext) const {
+  const double sf = static_cast<double>(csize());  // complex element
+  const double rf = sf / 2.0;                       // real element of the working precision
+  const double N = static_cast<double>(g.N), M = static_cast<double>(g.M);
+  for (int i = 0; i < kNumStages; ++i) b[i] = 0.0;
+  const double nf = cs.nf, nb = cs.nb;
+  if (cs.nf > 0) {
+    b[0] = 8.0 * N;                                     // FP64 sum of rho
+    b[1] = 8.0 * N + sf * M;                            // rho (FP64) -> rho_hat
+    b[2] = 2.0 * sf * M;                                // in place
+    b[3] = sf * M + nf * sf * M + (cs.mf ? rf * M : 0);  // rho_hat + table -> nf channels
+    b[4] = 2.0 * nf * sf * M;
+  }
+  b[5] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;  // channels, rho, V -> derivs, local
+  if (cs.nb > 0) {
+    b[6] = 2.0 * nb * sf * M;
+    b[7] = nb * sf * M + sf * M;
+    b[8] = 2.0 * sf * M;
+    b[9] = sf * M + 8.0 * N + 8.0 * N;  // spectrum + local -> force
+  }
+  b[10] = 0.0;
 }
 
 void Context::eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
@@ -270,12 +332,12 @@ void Context::r2c(const double* d_in) {
   const long ylines = static_cast<long>(g.nzh) * g.nx;
   if (precision == CDFT_PRECISION_MIXED) {
     auto* S = static_cast<cx<float>*>(d_S0);
-    launch_zr2c<float>(d_in, S, g, plan[2], tw<float>(2), id, stream);
+    launch_zr2c<float>(d_in, S, g, plan[2], tw<float>(2), id, nullptr, stream);
     launch_lines<float>(S, ylines, plan[1], tw<float>(1), -1, stream);
     launch_xlines<float>(S, 1, g, plan[0], tw<float>(0), -1, stream);
   } else {
     auto* S = static_cast<cx<double>*>(d_S0);
-    launch_zr2c<double>(d_in, S, g, plan[2], tw<double>(2), id, stream);
+    launch_zr2c<double>(d_in, S, g, plan[2], tw<double>(2), id, nullptr, stream);
     launch_lines<double>(S, ylines, plan[1], tw<double>(1), -1, stream);
     launch_xlines<double>(S, 1, g, plan[0], tw<double>(0), -1, stream);
   }
@@ -288,12 +350,12 @@ void Context::c2r(double* d_out) {
     auto* S = static_cast<cx<float>*>(d_S0);
     launch_xlines<float>(S, 1, g, plan[0], tw<float>(0), +1, stream);
     launch_lines<float>(S, ylines, plan[1], tw<float>(1), +1, stream);
-    launch_zc2r<float>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), stream);
+    launch_zc2r<float>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), nullptr, DcTable{}, stream);
   } else {
     auto* S = static_cast<cx<double>*>(d_S0);
     launch_xlines<double>(S, 1, g, plan[0], tw<double>(0), +1, stream);
     launch_lines<double>(S, ylines, plan[1], tw<double>(1), +1, stream);
-    launch_zc2r<double>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), stream);
+    launch_zc2r<double>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), nullptr, DcTable{}, stream);
   }
   CDFT_HIP(hipGetLastError());
 }
@@ -302,22 +364,24 @@ void Context::weighted_densities(const double* d_rho_in, double* d_out) {
   const AliasMap id{0, 0.0, 0.0};
   const long ylines = static_cast<long>(g.nzh) * g.nx;
   if (cs.nf == 0) return;
+  launch_sum(d_rho_in, id, d_sum_part, g, stream);
+  launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
   if (precision == CDFT_PRECISION_MIXED) {
     auto* S0 = static_cast<cx<float>*>(d_S0);
     auto* SF = static_cast<cx<float>*>(d_SF);
-    launch_zr2c<float>(d_rho_in, S0, g, plan[2], tw<float>(2), id, stream);
+    launch_zr2c<float>(d_rho_in, S0, g, plan[2], tw<float>(2), id, d_shift, stream);
     launch_lines<float>(S0, ylines, plan[1], tw<float>(1), -1, stream);
     launch_xfwd<float>(S0, SF, static_cast<const float*>(d_mf), g, plan[0], tw<float>(0), cs, stream);
     launch_lines<float>(SF, ylines * cs.nf, plan[1], tw<float>(1), +1, stream);
-    launch_zc2r<float>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), stream);
+    launch_zc2r<float>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), d_shift, dc, stream);
   } else {
     auto* S0 = static_cast<cx<double>*>(d_S0);
     auto* SF = static_cast<cx<double>*>(d_SF);
-    launch_zr2c<double>(d_rho_in, S0, g, plan[2], tw<double>(2), id, stream);
+    launch_zr2c<double>(d_rho_in, S0, g, plan[2], tw<double>(2), id, d_shift, stream);
     launch_lines<double>(S0, ylines, plan[1], tw<double>(1), -1, stream);
     launch_xfwd<double>(S0, SF, static_cast<const double*>(d_mf), g, plan[0], tw<double>(0), cs, stream);
     launch_lines<double>(SF, ylines * cs.nf, plan[1], tw<double>(1), +1, stream);
-    launch_zc2r<double>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), stream);
+    launch_zc2r<double>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), d_shift, dc, stream);
   }
   CDFT_HIP(hipGetLastError());
 }

@@ -35,6 +35,9 @@ struct InteractionSpec {
   PairPotential pot;
 };
 
+constexpr int kNumStages = 11;
+extern const char* const kStageNames[kNumStages];
+
 // Results of one evaluation, in the order the fused kernel accumulates them.
 enum ResultSlot { RS_IDEAL = 0, RS_EXT = 1, RS_HS = 2, RS_MF = 3, RS_MASS = 4 };
 
@@ -65,7 +68,12 @@ struct Context {
   void* d_mf = nullptr;       // M real: mean-field k-space weight
   double* d_partials = nullptr;
   double* d_res = nullptr;
+  double* d_sum_part = nullptr;  // block partials of the FP64 density sum
+  double* d_shift = nullptr;     // [0] = sum of rho (mean-density split)
   int nblk_point = 0;
+  int nblk_sum = 0;
+  DcTable dc{};                  // analytic k = 0 weight of every forward channel
+  hipEvent_t* stage_events = nullptr;  // when set, eval records kNumStages+1 events
 
   ~Context();
   size_t csize() const { return precision == CDFT_PRECISION_MIXED ? 8 : 16; }
@@ -92,6 +100,7 @@ struct Context {
   void weighted_densities(const double* d_rho_in, double* d_out /* nf * N */);
 
   double bulk_mu(double rho) const;
+  void stage_bytes(double* b, bool with_vext) const;
 };
 
 Context* create_context(const cdft_grid* grid, const double* diameters, int n_species,

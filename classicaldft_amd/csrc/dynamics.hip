@@ -1,26 +1,665 @@
-// FIRE / Minimizer and DDFT entry points (filled in below).
+// Device-resident FIRE2 minimisation in the alias variable
+// (Minimizer::grand_potential, include/dft/algorithms/minimization.hpp:280-353,
+// fire::Fire, include/dft/algorithms/fire.hpp:60-178) and the DDFT
+// integrating-factor scheme (integrating_factor_step + Simulation::run,
+// include/dft/algorithms/dynamics.hpp:79-401,582-693).
+//
+// State vectors (x, v, f, rho, d1, forces) stay in HBM in FP64; the host
+// keeps only the scalar control logic of the reference (FIRE's branch on the
+// power, dt/alpha adaptation, the uphill counter, the EMA convergence
+// monitor; DDFT's restarts and dt doubling) and reads back a handful of
+// FP64 reductions per step.
+//
+// Evaluations of the functional on an input identical to one already
+// evaluated are not repeated (FIRE's second initial compute, DDFT's first
+// compute of a step and its first fixed-point iterate): the evaluation is
+// deterministic, so reusing the stored result changes no output.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
 
 #include "api_guard.hpp"
 #include "context.hpp"
+
+namespace cdft {
+
+namespace {
+
+__device__ __forceinline__ void acc2(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+  s = t;
+}
+
+// Block-reduce up to 4 sums into partials[blockIdx.x*8 + i] (slots 4..7 zero).
+__device__ __forceinline__ void block_sums(const double* v, int nv, double* partials) {
+  __shared__ double sh[4][256];
+  for (int i = 0; i < 4; ++i) sh[i][threadIdx.x] = i < nv ? v[i] : 0.0;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int i = 0; i < 4; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x < 4 ? sh[threadIdx.x][0] : 0.0;
+}
+
+__device__ __forceinline__ bool on_face(long i, const Geom& g) {
+  const long iz = i % g.nz;
+  const long t = i / g.nz;
+  const long iy = t % g.ny;
+  const long ix = t / g.ny;
+  return ix == 0 || ix == g.nx - 1 || iy == 0 || iy == g.ny - 1 || iz == 0 || iz == g.nz - 1;
+}
+
+// from_density (minimization.hpp:63-77)
+__global__ void k_from_density(const double* __restrict__ rho, double* __restrict__ x, long n, AliasMap a) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double d = rho[i] - a.rho_min;
+  if (a.kind == 1) {
+    x[i] = sqrt(fmax(d, 0.0));
+  } else {
+    const double dd = fmin(fmax(d, 0.0), a.range * (1.0 - 1e-14));
+    x[i] = sqrt(dd / (a.range - dd));
+  }
+}
+
+__global__ void k_to_density(const double* __restrict__ x, double* __restrict__ rho, long n, AliasMap a) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) rho[i] = a.apply(x[i]);
+}
+
+// Sum of g over the boundary faces (grid.hpp:67-82), compact enumeration:
+// x faces, then y faces with interior x, then z faces with interior x, y.
+__global__ void __launch_bounds__(256) k_face_sum(const double* __restrict__ g, Geom G, long nb, double* partials) {
+  double s = 0.0, c = 0.0;
+  const long nx = G.nx, ny = G.ny, nz = G.nz;
+  const long fx = (nx > 1 ? 2 : 1) * ny * nz;
+  const long ixn = nx > 2 ? nx - 2 : 0;
+  const long fy = (ny > 1 ? 2 : 1) * ixn * nz;
+  const long iyn = ny > 2 ? ny - 2 : 0;
+  for (long j = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; j < nb;
+       j += static_cast<long>(gridDim.x) * blockDim.x) {
+    long ix, iy, iz;
+    if (j < fx) {
+      const long f = j / (ny * nz), r = j % (ny * nz);
+      ix = f == 0 ? 0 : nx - 1;
+      iy = r / nz;
+      iz = r % nz;
+    } else if (j < fx + fy) {
+      const long k = j - fx;
+      const long f = k / (ixn * nz), r = k % (ixn * nz);
+      iy = f == 0 ? 0 : ny - 1;
+      ix = 1 + r / nz;
+      iz = r % nz;
+    } else {
+      const long k = j - fx - fy;
+      const long f = k / (ixn * iyn), r = k % (ixn * iyn);
+      iz = f == 0 ? 0 : nz - 1;
+      ix = 1 + r / iyn;
+      iy = 1 + r % iyn;
+    }
+    acc2(s, c, g[iz + nz * (iy + ny * ix)]);
+  }
+  const double v[1] = {s + c};
+  block_sums(v, 1, partials);
+}
+
+// FIRE force in the alias variable (minimization.hpp:288-302): homogeneous
+// boundary average (grid.hpp:113-131), chain rule, negation.  Accumulates
+// sum f^2 and the power sum v.f for the next step.
+__global__ void __launch_bounds__(256) k_fire_force(const double* __restrict__ g, const double* __restrict__ x,
+                                                    const double* __restrict__ v, double* __restrict__ f, Geom G,
+                                                    int homog, const double* __restrict__ face, long nb, AliasMap a,
+                                                    double* partials) {
+  double s2 = 0, c2 = 0, sp = 0, cp = 0;
+  const double avg = (homog && nb > 0) ? face[0] / static_cast<double>(nb) : 0.0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < G.N;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    double gi = g[i];
+    if (homog && on_face(i, G)) gi = avg;
+    const double fi = -(gi * a.deriv(x[i]));
+    f[i] = fi;
+    acc2(s2, c2, fi * fi);
+    acc2(sp, cp, v[i] * fi);
+  }
+  const double vals[2] = {s2 + c2, sp + cp};
+  block_sums(vals, 2, partials);
+}
+
+// fire.hpp:110-126: optional backtrack (x -= dt_old v / 2, v = 0), then v += dt f; accumulates |v|^2
+__global__ void __launch_bounds__(256) k_fire_velocity(double* __restrict__ x, double* __restrict__ v,
+                                                       const double* __restrict__ f, long n, double dt_old,
+                                                       double dt, int backtrack, double* partials) {
+  double s = 0, c = 0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    double vi = v[i];
+    if (backtrack) {
+      x[i] -= 0.5 * dt_old * vi;
+      vi = 0.0;
+    }
+    vi += dt * f[i];
+    v[i] = vi;
+    acc2(s, c, vi * vi);
+  }
+  const double vals[1] = {s + c};
+  block_sums(vals, 1, partials);
+}
+
+// fire.hpp:128-148: v = (1 - alpha) v + alpha |v|/|f| f (when |f| > 0), x += dt v
+__global__ void __launch_bounds__(256) k_fire_mix(double* __restrict__ x, double* __restrict__ v,
+                                                  const double* __restrict__ f, long n, double dt, double alpha,
+                                                  const double* __restrict__ vsum, const double* __restrict__ fsum) {
+  const double vn = sqrt(vsum[0]), fn = sqrt(fsum[0]);
+  const bool mix = fn > 0.0;
+  const double ratio = mix ? vn / fn : 0.0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    double vi = v[i];
+    if (mix) vi = (1.0 - alpha) * vi + alpha * ratio * f[i];
+    v[i] = vi;
+    x[i] += dt * vi;
+  }
+}
+
+// ---------------------------------------------------------------- DDFT kernels
+// R = div(rho grad(F/dV)) - Laplacian(rho), periodic 7-point stencils (dynamics.hpp:135-229)
+__global__ void __launch_bounds__(256) k_ddft_rhs(const double* __restrict__ rho, const double* __restrict__ force,
+                                                  double* __restrict__ rhs, Geom G, double* partials) {
+  double s = 0, c = 0;
+  const long nx = G.nx, ny = G.ny, nz = G.nz, nyz = ny * nz;
+  const double D = 0.5 / (G.dx * G.dx), D2 = 1.0 / (G.dx * G.dx), idv = 1.0 / G.dv;
+  for (long p = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; p < G.N;
+       p += static_cast<long>(gridDim.x) * blockDim.x) {
+    const long iz = p % nz, iy = (p / nz) % ny, ix = p / nyz;
+    const long px = ((ix + 1) % nx) * nyz + iy * nz + iz, mx = ((ix - 1 + nx) % nx) * nyz + iy * nz + iz;
+    const long py = ix * nyz + ((iy + 1) % ny) * nz + iz, my = ix * nyz + ((iy - 1 + ny) % ny) * nz + iz;
+    const long pz = ix * nyz + iy * nz + (iz + 1) % nz, mz = ix * nyz + iy * nz + (iz - 1 + nz) % nz;
+    const double d0 = rho[p], x0 = force[p] * idv;
+    const double rpx = rho[px], rmx = rho[mx], rpy = rho[py], rmy = rho[my], rpz = rho[pz], rmz = rho[mz];
+    const double xpx = force[px] * idv, xmx = force[mx] * idv, xpy = force[py] * idv, xmy = force[my] * idv;
+    const double xpz = force[pz] * idv, xmz = force[mz] * idv;
+    double r = D * ((rpx + d0) * (xpx - x0) - (d0 + rmx) * (x0 - xmx)) +
+               D * ((rpy + d0) * (xpy - x0) - (d0 + rmy) * (x0 - xmy)) +
+               D * ((rpz + d0) * (xpz - x0) - (d0 + rmz) * (x0 - xmz));
+    r -= D2 * (rpx + rmx - 2.0 * d0) + D2 * (rpy + rmy - 2.0 * d0) + D2 * (rpz + rmz - 2.0 * d0);
+    rhs[p] = r;
+    acc2(s, c, r);
+  }
+  const double vals[1] = {s + c};
+  block_sums(vals, 1, partials);
+}
+
+// Integrating-factor propagator fused with the x stage of the transforms
+// (dynamics.hpp:247-289): x-FFT of R1 (z, y done), combine
+// w = U0 rho0_k + U1 R0_k + U2 (R1_k - R0_k) on the mean-free spectra (the
+// k = 0 mode is propagated separately in FP64), inverse x-FFT.
+template <typename R>
+__global__ void k_xprop(const cx<R>* __restrict__ SR1, const cx<R>* __restrict__ Srho0, const cx<R>* __restrict__ SR0,
+                        cx<R>* __restrict__ Sout, Geom g, FftPlan px, const cx<R>* __restrict__ twx, int ty,
+                        const double* __restrict__ lam, const double* __restrict__ fac, double dt, int first_iter) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
+  const int nx = g.nx, ny = g.ny, ls = nx + 1;
+  cx<R>* buf1 = buf0 + ty * ls;
+  const int y0 = blockIdx.x * ty;
+  const int kz = blockIdx.y;
+  const long off = static_cast<long>(kz) * nx * ny;
+  const cx<R>* res = nullptr;
+  if (!first_iter) {
+    for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
+      const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+      buf0[yl * ls + x] = (y < ny) ? SR1[off + x * ny + y] : cx<R>{R(0), R(0)};
+    }
+    __syncthreads();
+    res = fft_lines(buf0, buf1, ty, ls, px, -1, twx);
+    __syncthreads();
+  }
+  // lam/fac layout: [0, nx) x axis, [nx, nx+ny) y axis, [nx+ny, nx+ny+nzh) z axis
+  cx<R> wv[16];
+  int cnt = 0;
+  for (int e = threadIdx.x; e < ty * nx; e += blockDim.x, ++cnt) {
+    const int x = e / ty, yl = e - x * ty, y = min(y0 + yl, ny - 1);
+    const double L = lam[x] + lam[nx + y] + lam[nx + ny + kz];
+    const double ex = fac[x] * fac[nx + y] * fac[nx + ny + kz];
+    const bool zero = (x == 0 && y == 0 && kz == 0);
+    const double U0 = ex;
+    const double U1 = zero ? dt : (ex - 1.0) / L;
+    const double U2 = zero ? dt / 2.0 : (ex - 1.0 - dt * L) / (L * L * dt);
+    const long idx = off + x * ny + y;
+    const cx<R> r0 = SR0[idx], p0 = Srho0[idx];
+    const cx<R> r1 = first_iter ? r0 : res[yl * ls + x];
+    double wr = U0 * static_cast<double>(p0.re) + U1 * static_cast<double>(r0.re) +
+                U2 * (static_cast<double>(r1.re) - static_cast<double>(r0.re));
+    double wi = U0 * static_cast<double>(p0.im) + U1 * static_cast<double>(r0.im) +
+                U2 * (static_cast<double>(r1.im) - static_cast<double>(r0.im));
+    if (zero) wr = wi = 0.0;  // mean mode handled in FP64 by k_ddft_update
+    wv[cnt] = {static_cast<R>(wr), static_cast<R>(wi)};
+  }
+  __syncthreads();
+  cnt = 0;
+  for (int e = threadIdx.x; e < ty * nx; e += blockDim.x, ++cnt) {
+    const int x = e / ty, yl = e - x * ty;
+    buf0[yl * ls + x] = wv[cnt];
+  }
+  __syncthreads();
+  const cx<R>* r = fft_lines(buf0, buf1, ty, ls, px, +1, twx);
+  __syncthreads();
+  for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
+    const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+    if (y < ny) Sout[off + x * ny + y] = r[yl * ls + x];
+  }
+}
+
+// d1_new = mean_new + raw - mean(raw); max |d1_new - d1_old|; min d1_new (apply_propagator, dynamics.hpp:289-303).
+// sums: [0] sum rho0, [8] sum R0, [16] sum R1, [24] sum raw
+__global__ void __launch_bounds__(256) k_ddft_update(const double* __restrict__ raw, double* __restrict__ d1, long n,
+                                                     const double* __restrict__ sums, double dt, int first_iter,
+                                                     double* partials) {
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double s0 = sums[0], r0 = sums[8], r1 = first_iter ? sums[8] : sums[16];
+  const double mean_new = (s0 + dt * r0 + 0.5 * dt * (r1 - r0)) * inv_n;
+  const double shift = mean_new - sums[24] * inv_n;
+  double mx = 0.0, mn = 1e300;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const double nv = raw[i] + shift;
+    mx = fmax(mx, fabs(d1[i] - nv));
+    mn = fmin(mn, nv);
+    d1[i] = nv;
+  }
+  __shared__ double smx[256], smn[256];
+  smx[threadIdx.x] = mx;
+  smn[threadIdx.x] = mn;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + w]);
+      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[static_cast<long>(blockIdx.x) * 8 + 0] = smx[0];
+    partials[static_cast<long>(blockIdx.x) * 8 + 1] = smn[0];
+  }
+}
+
+__global__ void k_finalize_maxmin(const double* __restrict__ partials, int nblk, double* __restrict__ res) {
+  __shared__ double smx[256], smn[256];
+  double mx = 0.0, mn = 1e300;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    mx = fmax(mx, partials[static_cast<long>(b) * 8 + 0]);
+    mn = fmin(mn, partials[static_cast<long>(b) * 8 + 1]);
+  }
+  smx[threadIdx.x] = mx;
+  smn[threadIdx.x] = mn;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + w]);
+      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[0] = smx[0];
+    res[1] = smn[0];
+  }
+}
+
+constexpr int kVecBlocks = 1024;
+
+unsigned vec_blocks(long n) {
+  long b = (n + 255) / 256;
+  return static_cast<unsigned>(std::min<long>(std::max<long>(b, 1), kVecBlocks));
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) {
+    if (bytes) CDFT_HIP(hipMalloc(&p, bytes));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+long face_count(const Geom& g) {
+  const long nx = g.nx, ny = g.ny, nz = g.nz;
+  const long inner = std::max(nx - 2, 0L) * std::max(ny - 2, 0L) * std::max(nz - 2, 0L);
+  return nx * ny * nz - inner;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ Minimizer
+struct FireRun {
+  Context* c;
+  AliasMap amap;
+  int homog;
+  const double* d_vext;
+  double mu;
+  DevBuf x, v, f, part, res;  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum
+  long nb;
+
+  FireRun(Context* c_, AliasMap a, int h, const double* vx, double m)
+      : c(c_), amap(a), homog(h), d_vext(vx), mu(m), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
+        part(8 * 8 * kVecBlocks), res(8 * 24) {
+    nb = face_count(c->g);
+  }
+
+  // compute(x): energy Omega and f in d_f; device sums f^2 and v.f in res[0], res[1]
+  double compute(double* sum_f2, double* power) {
+    const Geom& g = c->g;
+    c->eval(x.as<double>(), d_vext, mu, c->d_force, amap);
+    if (homog) {
+      hipLaunchKernelGGL(k_face_sum, dim3(vec_blocks(nb)), dim3(256), 0, c->stream, c->d_force, g, nb,
+                         part.as<double>());
+      launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(nb)), res.as<double>() + 16, c->stream);
+    }
+    hipLaunchKernelGGL(k_fire_force, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, c->d_force, x.as<double>(),
+                       v.as<double>(), f.as<double>(), g, homog, res.as<double>() + 16, nb, amap, part.as<double>());
+    launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(g.N)), res.as<double>(), c->stream);
+    CDFT_HIP(hipGetLastError());
+    double er[8], fr[8];
+    CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipMemcpyAsync(fr, res.p, sizeof(fr), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    cdft_result r{};
+    c->result_from(er, mu, &r);
+    last_result = r;
+    *sum_f2 = fr[0];
+    *power = fr[1];
+    return r.grand_potential;
+  }
+  cdft_result last_result{};
+};
+
+// ------------------------------------------------------------------ DDFT
+struct DdftRun {
+  Context* c;
+  const double* d_vext;
+  double mu;
+  DevBuf rho0, d1, f0, f1, rhs, raw;
+  DevBuf Srho0, SR0, SR1, Sw;
+  DevBuf lam, fac, part, sums, mm;
+  std::vector<double> lam_h;
+  double st_dt = 0.0;
+  long evals = 0;
+
+  DdftRun(Context* c_, const double* vx, double m)
+      : c(c_), d_vext(vx), mu(m), rho0(8 * c_->g.N), d1(8 * c_->g.N), f0(8 * c_->g.N), f1(8 * c_->g.N),
+        rhs(8 * c_->g.N), raw(8 * c_->g.N), Srho0(c_->csize() * c_->g.M), SR0(c_->csize() * c_->g.M),
+        SR1(c_->csize() * c_->g.M), Sw(c_->csize() * c_->g.M),
+        lam(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)), fac(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)),
+        part(8 * 8 * kVecBlocks), sums(8 * 40), mm(8 * 8) {
+    const Geom& g = c->g;
+    // make_if_state (dynamics.hpp:79-117): Lambda_d = 2 (cos(2 pi i / N_d) - 1) / dx^2
+    const double PI = 3.14159265358979323846, Dx = 1.0 / (g.dx * g.dx);
+    lam_h.resize(g.nx + g.ny + g.nzh);
+    for (int i = 0; i < g.nx; ++i) lam_h[i] = 2.0 * Dx * (std::cos(2.0 * PI * i / g.nx) - 1.0);
+    for (int i = 0; i < g.ny; ++i) lam_h[g.nx + i] = 2.0 * Dx * (std::cos(2.0 * PI * i / g.ny) - 1.0);
+    for (int i = 0; i < g.nzh; ++i) lam_h[g.nx + g.ny + i] = 2.0 * Dx * (std::cos(2.0 * PI * i / g.nz) - 1.0);
+    CDFT_HIP(hipMemcpy(lam.p, lam_h.data(), 8 * lam_h.size(), hipMemcpyHostToDevice));
+  }
+
+  // update_timestep (dynamics.hpp:121-129)
+  void update_timestep(double dt) {
+    if (std::fabs(st_dt - dt) < 1e-30) return;
+    st_dt = dt;
+    std::vector<double> f(lam_h.size());
+    for (size_t i = 0; i < f.size(); ++i) f[i] = std::exp(lam_h[i] * dt);
+    CDFT_HIP(hipMemcpyAsync(fac.p, f.data(), 8 * f.size(), hipMemcpyHostToDevice, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+  }
+
+  double compute(const double* rho, double* forces) {
+    c->eval(rho, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
+    ++evals;
+    double er[8];
+    CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    cdft_result r{};
+    c->result_from(er, mu, &r);
+    return r.grand_potential;
+  }
+
+  // Full forward transform of a real field into S (mean removed; its FP64 sum to sums[slot]).
+  template <typename R>
+  void forward(const double* field, cx<R>* S, int slot, bool x_stage) {
+    const Geom& g = c->g;
+    const AliasMap id{0, 0.0, 0.0};
+    double* s = sums.as<double>() + 8 * slot;
+    launch_sum(field, id, part.as<double>(), g, c->stream);
+    launch_finalize(part.as<double>(), sum_blocks(g), s, c->stream);
+    launch_zr2c<R>(field, S, g, c->plan[2], c->tw<R>(2), id, s, c->stream);
+    launch_lines<R>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), -1, c->stream);
+    if (x_stage) launch_xlines<R>(S, 1, g, c->plan[0], c->tw<R>(0), -1, c->stream);
+  }
+
+  // excess_rhs_fourier (dynamics.hpp:212-229) into S (x stage optional)
+  template <typename R>
+  void rhs_forward(const double* rho, const double* forces, cx<R>* S, int slot, bool x_stage) {
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_ddft_rhs, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, rho, forces, rhs.as<double>(), g,
+                       part.as<double>());
+    forward<R>(rhs.as<double>(), S, slot, x_stage);
+  }
+
+  // one propagator application; returns {max deviation, min d1}
+  template <typename R>
+  void propagate(bool first_iter, double* dev, double* mn) {
+    const Geom& g = c->g;
+    const int ty = sizeof(R) == 4 ? 16 : 8;
+    const size_t sh = 2 * sizeof(cx<R>) * ty * (g.nx + 1);
+    if (ty * g.nx > 16 * 256) throw Unsupported("x extent too large for the propagator kernel");
+    hipLaunchKernelGGL(k_xprop<R>, dim3((g.ny + ty - 1) / ty, g.nzh), dim3(256), sh, c->stream, SR1.as<cx<R>>(),
+                       Srho0.as<cx<R>>(), SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty,
+                       lam.as<double>(), fac.as<double>(), st_dt, first_iter ? 1 : 0);
+    launch_lines<R>(Sw.as<cx<R>>(), static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), +1, c->stream);
+    launch_zc2r<R>(Sw.as<cx<R>>(), 1, raw.as<double>(), nullptr, g.inv_n, g, c->plan[2], c->tw<R>(2), nullptr,
+                   DcTable{}, c->stream);
+    const AliasMap id{0, 0.0, 0.0};
+    launch_sum(raw.as<double>(), id, part.as<double>(), g, c->stream);
+    launch_finalize(part.as<double>(), sum_blocks(g), sums.as<double>() + 24, c->stream);
+    hipLaunchKernelGGL(k_ddft_update, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, raw.as<double>(),
+                       d1.as<double>(), g.N, sums.as<double>(), st_dt, first_iter ? 1 : 0, part.as<double>());
+    hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
+                       static_cast<int>(vec_blocks(g.N)), mm.as<double>());
+    CDFT_HIP(hipGetLastError());
+    double r[2];
+    CDFT_HIP(hipMemcpyAsync(r, mm.p, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    *dev = r[0];
+    *mn = r[1];
+  }
+
+  // integrating_factor_step (dynamics.hpp:333-401); rho0 holds the densities and f0 their forces.
+  template <typename R>
+  double step(double* cfg_dt, double tol, int max_it) {
+    const long N = c->g.N;
+    rhs_forward<R>(rho0.as<double>(), f0.as<double>(), SR0.as<cx<R>>(), 1, true);
+    forward<R>(rho0.as<double>(), Srho0.as<cx<R>>(), 0, true);
+    bool restart;
+    do {
+      restart = false;
+      update_timestep(*cfg_dt);
+      CDFT_HIP(hipMemcpyAsync(d1.p, rho0.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      double deviation = 1e30;
+      for (int it = 0; it < max_it; ++it) {
+        const bool first = it == 0;  // d1 == rho0: compute({d1}) equals the step's first compute
+        if (!first) {
+          compute(d1.as<double>(), f1.as<double>());
+          rhs_forward<R>(d1.as<double>(), f1.as<double>(), SR1.as<cx<R>>(), 2, false);
+        }
+        const double old = deviation;
+        double mn = 0.0;
+        propagate<R>(first, &deviation, &mn);
+        if (mn < 0.0 || (it > 0 && deviation > old)) {
+          restart = true;
+          break;
+        }
+        if (deviation < tol) break;
+      }
+      if (!restart && deviation > tol) restart = true;
+      if (restart) *cfg_dt /= 10.0;
+    } while (restart);
+    const double e = compute(d1.as<double>(), f1.as<double>());
+    CDFT_HIP(hipMemcpyAsync(rho0.p, d1.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    CDFT_HIP(hipMemcpyAsync(f0.p, f1.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    return e;
+  }
+};
+
+}  // namespace cdft
 
 using namespace cdft;
 
 extern "C" {
 
-int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire*, const cdft_parametrization*, int, const double*,
-                                  const double*, double, double*, cdft_min_result*, double*) {
+int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire, const cdft_parametrization* param,
+                                  int use_homogeneous_boundary, const double* rho0, const double* vext, double mu,
+                                  double* rho_out, cdft_min_result* result, double* energies) {
   return guarded([&] {
     check_ctx(ctx);
-    throw Unsupported("not built yet");
+    if (!fire || !param || !rho0 || !rho_out || !result) throw std::invalid_argument("null argument");
+    if (param->kind != 0 && param->kind != 1) throw std::invalid_argument("unknown parametrization");
+    Context* c = ctx->impl;
+    CDFT_HIP(hipSetDevice(c->device));
+    const Geom& g = c->g;
+    const AliasMap amap{param->kind == 0 ? 1 : 2, param->rho_min, param->rho_max - param->rho_min};
+    if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    FireRun run(c, amap, use_homogeneous_boundary ? 1 : 0, vext ? c->d_vext : nullptr, mu);
+    CDFT_HIP(hipMemcpyAsync(c->d_rho, rho0, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_from_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, c->d_rho,
+                       run.x.as<double>(), g.N, amap);
+    CDFT_HIP(hipMemsetAsync(run.v.p, 0, 8 * g.N, c->stream));
+
+    // Fire::initialize (fire.hpp:60-84)
+    double sf2 = 0.0, power = 0.0;
+    double energy = run.compute(&sf2, &power);
+    double dt = fire->dt, alpha = fire->alpha_start;
+    int n_pos = 0, n_neg = 0, iteration = 0;
+    double rms = std::sqrt(sf2 / static_cast<double>(g.N));
+    bool converged = rms < fire->force_tolerance;
+    if (energies) energies[0] = energy;
+    const double volume = static_cast<double>(g.N) * g.dv;
+    double vv = 1.0;
+    converged = false;  // minimization.hpp:321: the loop starts unconverged
+    for (int i = 0; i < fire->max_steps && !converged; ++i) {
+      const double old_energy = energy;
+      // Fire::step (fire.hpp:86-165); `power` is v.f of the current forces
+      bool backtrack = false;
+      const double dt_old = dt;
+      if (power > 0.0 || iteration == 0) {
+        ++n_pos;
+        n_neg = 0;
+        if (n_pos > fire->n_delay) {
+          dt = std::min(dt * fire->f_inc, fire->dt_max);
+          alpha *= fire->f_alpha;
+        }
+      } else {
+        n_pos = 0;
+        ++n_neg;
+        if (n_neg > fire->max_uphill) throw std::runtime_error("fire::step: exceeded max uphill steps");
+        backtrack = true;
+        if (iteration > fire->n_delay) {
+          dt = std::max(dt * fire->f_dec, fire->dt_min);
+          alpha = fire->alpha_start;
+        }
+      }
+      const unsigned nb = vec_blocks(g.N);
+      hipLaunchKernelGGL(k_fire_velocity, dim3(nb), dim3(256), 0, c->stream, run.x.as<double>(), run.v.as<double>(),
+                         run.f.as<double>(), g.N, dt_old, dt, backtrack ? 1 : 0, run.part.as<double>());
+      launch_finalize(run.part.as<double>(), static_cast<int>(nb), run.res.as<double>() + 8, c->stream);
+      hipLaunchKernelGGL(k_fire_mix, dim3(nb), dim3(256), 0, c->stream, run.x.as<double>(), run.v.as<double>(),
+                         run.f.as<double>(), g.N, dt, alpha, run.res.as<double>() + 8, run.res.as<double>());
+      energy = run.compute(&sf2, &power);
+      rms = std::sqrt(sf2 / static_cast<double>(g.N));
+      ++iteration;
+      // convergence_monitor (minimization.hpp:105-110,329-332)
+      vv = vv * 0.9 + 0.1 * (std::fabs(energy - old_energy) / dt);
+      const double monitor = std::fabs(vv / volume);
+      converged = monitor < fire->force_tolerance;
+      if (energies) energies[i + 1] = energy;
+    }
+    hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, run.x.as<double>(), c->d_rho,
+                       g.N, amap);
+    CDFT_HIP(hipMemcpyAsync(rho_out, c->d_rho, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    // the final total() (minimization.hpp:339-343) is the last compute's evaluation
+    result->free_energy = run.last_result.free_energy;
+    result->grand_potential = run.last_result.grand_potential;
+    result->rms_force = rms;
+    result->iterations = iteration;
+    result->converged = converged ? 1 : 0;
+    result->dt = dt;
+    result->alpha = alpha;
   });
 }
 
-int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config*, int, const double*, double, double*, cdft_sim_result*,
-                  double*) {
+int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
+                  double* rho, cdft_sim_result* result, double* energies) {
   return guarded([&] {
     check_ctx(ctx);
-    throw Unsupported("not built yet");
+    if (!config || !rho || !result || n_steps < 0) throw std::invalid_argument("bad argument");
+    Context* c = ctx->impl;
+    CDFT_HIP(hipSetDevice(c->device));
+    const Geom& g = c->g;
+    if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    DdftRun run(c, vext ? c->d_vext : nullptr, mu);
+    CDFT_HIP(hipMemcpyAsync(run.rho0.p, rho, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    // mass (dynamics.hpp:591-596)
+    const AliasMap id{0, 0.0, 0.0};
+    auto mass_of = [&](const double* d) {
+      launch_sum(d, id, run.part.as<double>(), g, c->stream);
+      launch_finalize(run.part.as<double>(), sum_blocks(g), run.sums.as<double>() + 32, c->stream);
+      double s;
+      CDFT_HIP(hipMemcpyAsync(&s, run.sums.as<double>() + 32, 8, hipMemcpyDeviceToHost, c->stream));
+      CDFT_HIP(hipStreamSynchronize(c->stream));
+      return s * g.dv;
+    };
+    const double m0 = mass_of(run.rho0.as<double>());
+    double e = run.compute(run.rho0.as<double>(), run.f0.as<double>());
+    if (energies) energies[0] = e;
+    double dt = config->dt;
+    int successes = 0;
+    double time = 0.0;
+    for (int s = 1; s <= n_steps; ++s) {
+      const double before = dt;
+      e = c->precision == CDFT_PRECISION_MIXED ? run.step<float>(&dt, config->fp_tolerance, config->fp_max_iterations)
+                                               : run.step<double>(&dt, config->fp_tolerance, config->fp_max_iterations);
+      time += dt;
+      // adaptive timestep (dynamics.hpp:640-649)
+      if (dt < before) successes = 0;
+      else ++successes;
+      if (successes >= 5 && dt < config->dt_max) {
+        dt = std::min(2.0 * dt, config->dt_max);
+        successes = 0;
+      }
+      if (energies) energies[s] = e;
+    }
+    CDFT_HIP(hipMemcpyAsync(rho, run.rho0.p, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    result->mass_initial = m0;
+    result->mass_final = mass_of(run.rho0.as<double>());
+    result->time = time;
+    result->energy = e;
+    result->dt = dt;
+    result->evaluations = run.evals;
   });
 }
 

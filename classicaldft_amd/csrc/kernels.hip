@@ -112,18 +112,19 @@ __device__ __forceinline__ cx<R> channel_weight(int ch, bool adj, R kx, R ky, R 
 
 template <typename R>
 __global__ void k_zr2c(const double* __restrict__ in, cx<R>* __restrict__ out, Geom g, FftPlan pz,
-                       const cx<R>* __restrict__ twz, int nlb, AliasMap amap) {
+                       const cx<R>* __restrict__ twz, int nlb, AliasMap amap, const double* __restrict__ shift_sum) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
   const int nz = g.nz, nzh = g.nzh;
   cx<R>* buf1 = buf0 + nlb * nzh;
   const long l0 = static_cast<long>(blockIdx.x) * nlb;
   const int ls = nz;
+  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
     const int ll = e / nz, z = e - ll * nz;
     const long l = l0 + ll;
     R val = R(0);
-    if (l < g.nxy) val = static_cast<R>(amap.apply(in[l * nz + z]));
+    if (l < g.nxy) val = static_cast<R>(amap.apply(in[l * nz + z]) - shift);
     R* dst = reinterpret_cast<R*>(&buf0[(ll >> 1) * ls + z]);
     dst[ll & 1] = val;
   }
@@ -171,7 +172,8 @@ __device__ __forceinline__ void build_c2r_lines(const cx<R>* __restrict__ stage,
 // Generic z-C2R of `nch` channels: out[c*N + p] = scale * c2r(S_c)[p] + (add ? add[p] : 0)
 template <typename R>
 __global__ void k_zc2r(const cx<R>* __restrict__ S, double* __restrict__ out, const double* __restrict__ add,
-                       double scale, Geom g, FftPlan pz, const cx<R>* __restrict__ twz, int nlb) {
+                       double scale, Geom g, FftPlan pz, const cx<R>* __restrict__ twz, int nlb,
+                       const double* __restrict__ shift_sum, DcTable dc) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
   const int nz = g.nz, nzh = g.nzh;
@@ -190,15 +192,33 @@ __global__ void k_zc2r(const cx<R>* __restrict__ S, double* __restrict__ out, co
   cx<R>* res = fft_lines(buf0, buf1, nlb >> 1, nz, pz, +1, twz);
   __syncthreads();
   double* oc = out + static_cast<long>(ch) * g.N;
+  const double cst = shift_sum ? dc.v[ch] * shift_sum[0] * g.inv_n : 0.0;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
     const int ll = e / nz, z = e - ll * nz;
     const long l = l0 + ll;
     if (l >= g.nxy) continue;
     const cx<R> v = res[(ll >> 1) * nz + z];
     const long p = l * nz + z;
-    const double val = static_cast<double>((ll & 1) ? v.im : v.re) * scale;
+    const double val = static_cast<double>((ll & 1) ? v.im : v.re) * scale + cst;
     oc[p] = add ? val + add[p] : val;
   }
+}
+
+// FP64 block sums of amap(in): partials[b*8 + 0]; slots 1..7 zeroed.
+__global__ void __launch_bounds__(256) k_sum(const double* __restrict__ in, AliasMap amap, long n,
+                                             double* __restrict__ partials) {
+  __shared__ double sh[256];
+  double s = 0.0, c = 0.0;
+  const long stride = static_cast<long>(gridDim.x) * blockDim.x;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
+    two_sum(s, c, amap.apply(in[i]));
+  sh[threadIdx.x] = s + c;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x == 0 ? sh[0] : 0.0;
 }
 
 // ---------------------------------------------------------------- y lines (contiguous)
@@ -389,7 +409,8 @@ __global__ void __launch_bounds__(512) k_zpoint(cx<R>* SF, const double* __restr
                                                 const double* __restrict__ vext, double* __restrict__ local_force,
                                                 double* __restrict__ partials, Geom g, FftPlan pz,
                                                 const cx<R>* __restrict__ twz, int nlb, ChannelSet cs, FmtParams fp,
-                                                double mu, AliasMap amap) {
+                                                double mu, AliasMap amap, const double* __restrict__ shift_sum,
+                                                DcTable dc) {
   extern __shared__ unsigned char smem_raw[];
   cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
   const int nz = g.nz, nzh = g.nzh;
@@ -429,6 +450,7 @@ __global__ void __launch_bounds__(512) k_zpoint(cx<R>* SF, const double* __restr
   R (&gv)[12][PPT] = wv;
   const double dv = g.dv;
   const int mfc = cs.hs ? 11 : 0;
+  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
 #pragma unroll
   for (int m = 0; m < PPT; ++m) {
     const int q = threadIdx.x + m * blockDim.x;
@@ -446,14 +468,14 @@ __global__ void __launch_bounds__(512) k_zpoint(cx<R>* SF, const double* __restr
         two_sum(s_ext, c_ext, rho * V);
       }
       if (cs.mf) {                                // mean_field.hpp:259-266 (self pair)
-        const double conv = static_cast<double>(wv[mfc][m]);
+        const double conv = static_cast<double>(wv[mfc][m]) + dc.v[mfc] * shift;
         lf += conv * dv;
         two_sum(s_mf, c_mf, 0.5 * rho * conv);
       }
       if (cs.hs) {
         double w[11], gg[11];
 #pragma unroll
-        for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(wv[i][m]);
+        for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(wv[i][m]) + dc.v[i] * shift;
         const double phi = fmt_phi_grad(fp, w, gg);
         two_sum(s_hs, c_hs, phi);
 #pragma unroll
@@ -546,20 +568,32 @@ static int zlines_per_block(int nz) {
 
 template <typename R>
 void launch_zr2c(const double* in, cx<R>* out, const Geom& g, const FftPlan& pz, const cx<R>* twz, AliasMap amap,
-                 hipStream_t st) {
+                 const double* shift_sum, hipStream_t st) {
   const int nlb = zlines_per_block<R>(g.nz);
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const size_t sh = 2 * sizeof(cx<R>) * nlb * g.nzh;
-  hipLaunchKernelGGL(k_zr2c<R>, dim3(nblk), dim3(256), sh, st, in, out, g, pz, twz, nlb, amap);
+  hipLaunchKernelGGL(k_zr2c<R>, dim3(nblk), dim3(256), sh, st, in, out, g, pz, twz, nlb, amap, shift_sum);
 }
 
 template <typename R>
 void launch_zc2r(const cx<R>* S, int nch, double* out, const double* add, double scale, const Geom& g,
-                 const FftPlan& pz, const cx<R>* twz, hipStream_t st) {
+                 const FftPlan& pz, const cx<R>* twz, const double* shift_sum, DcTable dc, hipStream_t st) {
   const int nlb = zlines_per_block<R>(g.nz);
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const size_t sh = 2 * sizeof(cx<R>) * nlb * g.nzh;
-  hipLaunchKernelGGL(k_zc2r<R>, dim3(nblk, nch), dim3(256), sh, st, S, out, add, scale, g, pz, twz, nlb);
+  hipLaunchKernelGGL(k_zc2r<R>, dim3(nblk, nch), dim3(256), sh, st, S, out, add, scale, g, pz, twz, nlb,
+                     shift_sum, dc);
+}
+
+int sum_blocks(const Geom& g) {
+  long b = (g.N + 4095) / 4096;
+  if (b > 1024) b = 1024;
+  if (b < 1) b = 1;
+  return static_cast<int>(b);
+}
+
+void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_sum, dim3(sum_blocks(g)), dim3(256), 0, st, in, amap, g.N, partials);
 }
 
 template <typename R>
@@ -632,7 +666,7 @@ template int zpoint_blocks<double>(const Geom&);
 template <typename R>
 void launch_zpoint(cx<R>* SF, const double* rho, const double* vext, double* local_force, double* partials,
                    const Geom& g, const FftPlan& pz, const cx<R>* twz, ChannelSet cs, FmtParams fp, double mu,
-                   AliasMap amap, hipStream_t st) {
+                   AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
   const int nlb = zpoint_nlb<R>(g.nz);
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const size_t sh = 2 * sizeof(cx<R>) * nlb * g.nzh;
@@ -640,10 +674,10 @@ void launch_zpoint(cx<R>* SF, const double* rho, const double* vext, double* loc
   if (ppt > 4) throw std::runtime_error("z extent too large for the fused pointwise kernel");
   if (ppt <= 2)
     hipLaunchKernelGGL((k_zpoint<R, 2>), dim3(nblk), dim3(512), sh, st, SF, rho, vext, local_force, partials, g, pz,
-                       twz, nlb, cs, fp, mu, amap);
+                       twz, nlb, cs, fp, mu, amap, shift_sum, dc);
   else
     hipLaunchKernelGGL((k_zpoint<R, 4>), dim3(nblk), dim3(512), sh, st, SF, rho, vext, local_force, partials, g, pz,
-                       twz, nlb, cs, fp, mu, amap);
+                       twz, nlb, cs, fp, mu, amap, shift_sum, dc);
 }
 
 void launch_finalize(const double* partials, int nblk, double* res, hipStream_t st) {
@@ -652,16 +686,17 @@ void launch_finalize(const double* partials, int nblk, double* res, hipStream_t 
 
 #define CDFT_INST(R)                                                                                              \
   template void launch_zr2c<R>(const double*, cx<R>*, const Geom&, const FftPlan&, const cx<R>*, AliasMap,       \
-                               hipStream_t);                                                                      \
+                               const double*, hipStream_t);                                                       \
   template void launch_zc2r<R>(const cx<R>*, int, double*, const double*, double, const Geom&, const FftPlan&,   \
-                               const cx<R>*, hipStream_t);                                                        \
+                               const cx<R>*, const double*, DcTable, hipStream_t);                                \
   template void launch_lines<R>(cx<R>*, long, const FftPlan&, const cx<R>*, int, hipStream_t);                   \
   template void launch_xlines<R>(cx<R>*, int, const Geom&, const FftPlan&, const cx<R>*, int, hipStream_t);      \
   template void launch_xfwd<R>(const cx<R>*, cx<R>*, const R*, const Geom&, const FftPlan&, const cx<R>*,         \
                                ChannelSet, hipStream_t);                                                          \
   template void launch_xbwd<R>(const cx<R>*, cx<R>*, const Geom&, const FftPlan&, const cx<R>*, int, hipStream_t); \
   template void launch_zpoint<R>(cx<R>*, const double*, const double*, double*, double*, const Geom&,            \
-                                 const FftPlan&, const cx<R>*, ChannelSet, FmtParams, double, AliasMap, hipStream_t);
+                                 const FftPlan&, const cx<R>*, ChannelSet, FmtParams, double, AliasMap,            \
+                                 const double*, DcTable, hipStream_t);
 
 CDFT_INST(float)
 CDFT_INST(double)

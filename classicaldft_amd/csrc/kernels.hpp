@@ -43,12 +43,22 @@ struct AliasMap {
   }
 };
 
+// Mean-density split: the transforms carry rho - c with c = shift_sum[0] / N
+// (an FP64 sum of rho); each forward channel then gets back its exact k = 0
+// contribution dc.v[ch] * c in FP64.  A null shift_sum means c = 0.
+struct DcTable {
+  double v[12];
+};
+
 template <typename R>
 void launch_zr2c(const double* in, cx<R>* out, const Geom& g, const FftPlan& pz, const cx<R>* twz, AliasMap amap,
-                 hipStream_t st);
+                 const double* shift_sum, hipStream_t st);
 template <typename R>
 void launch_zc2r(const cx<R>* S, int nch, double* out, const double* add, double scale, const Geom& g,
-                 const FftPlan& pz, const cx<R>* twz, hipStream_t st);
+                 const FftPlan& pz, const cx<R>* twz, const double* shift_sum, DcTable dc, hipStream_t st);
+// FP64 block sums of amap(in) into partials[b*8]; finalize with launch_finalize.
+int sum_blocks(const Geom& g);
+void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g, hipStream_t st);
 template <typename R>
 void launch_lines(cx<R>* S, long nlines, const FftPlan& p, const cx<R>* tw, int sg, hipStream_t st);
 template <typename R>
@@ -64,7 +74,7 @@ int zpoint_blocks(const Geom& g);
 template <typename R>
 void launch_zpoint(cx<R>* SF, const double* rho, const double* vext, double* local_force, double* partials,
                    const Geom& g, const FftPlan& pz, const cx<R>* twz, ChannelSet cs, FmtParams fp, double mu,
-                   AliasMap amap, hipStream_t st);
+                   AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st);
 void launch_finalize(const double* partials, int nblk, double* res, hipStream_t st);
 
 }  // namespace cdft

@@ -118,6 +118,17 @@ CDFT_API int cdft_total_device(cdft_ctx* ctx, const double* d_rho, const double*
 CDFT_API int cdft_total_device_repeat(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu, int count,
                              cdft_result* result, double* d_forces);
 
+/* Measurement hooks (no counterpart in the reference, which has no timers):
+ * GPU time of `count` back-to-back evaluations measured with HIP events on the
+ * context stream, and the per-stage (per-kernel) average time and
+ * algorithmic HBM bytes of one evaluation; at most CDFT_MAX_STAGES stages. */
+#define CDFT_MAX_STAGES 11
+CDFT_API int cdft_time_total_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu, int count,
+                                    double* d_forces, double* ms);
+CDFT_API int cdft_profile_stages(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu,
+                                 double* d_forces, int reps, double* stage_ms, double* stage_bytes, int* n_stages);
+CDFT_API const char* cdft_stage_name(int i);
+
 /* hard_sphere.hpp:50-67 (convolve_weights) and mean_field.hpp:259: the 12
  * weighted-density fields eta, n2, n_v2[x,y,z], T[xx,xy,xz,yy,yz,zz] and the
  * mean-field convolution [w * rho] (host arrays, 12*N doubles). */

@@ -1,0 +1,116 @@
+"""GPU parity of the FIRE/Minimizer and DDFT integrating-factor paths against
+the FP64 CPU oracle, through the C ABI.
+
+FP64 path: per-step energies and final profiles to 1e-9 relative (the FIRE
+branch decisions and the DDFT restarts are identical, so the trajectories
+coincide).  Mixed path: mass conservation to 1e-12 and profiles to 1e-5.
+"""
+import numpy as np
+import pytest
+
+from classicaldft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(shape, dx, precision="fp64", model="WhiteBearII", inter=(W.LJ_WCA,)):
+    import classicaldft_amd as cd
+    from oracle import oracle as O
+
+    ctx = cd.Context(shape, dx, interactions=list(inter), kT=0.8, model=model, precision=precision)
+    orc = O.Oracle(shape, dx, interactions=list(inter), kT=0.8, model=model)
+    return ctx, orc
+
+
+@pytest.mark.parametrize("homog,param", [(True, "unbounded"), (False, "unbounded"), (True, "bounded")])
+def test_minimizer_trajectory_fp64(homog, param):
+    shape, dx = (16, 16, 16), 0.25
+    ctx, orc = _pair(shape, dx)
+    rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    mu = orc.bulk_mu(0.05)
+    fire = dict(force_tolerance=1e-9)
+    kw = dict(param=param, rho_max=1.2, homogeneous_boundary=homog, max_steps=60)
+    rg, resg, eg = ctx.minimize_grand_potential(rho0, mu, fire=fire, **kw)
+    ro, reso, eo = orc.minimize_grand_potential(rho0, mu, fire=fire, **kw)
+    assert resg["iterations"] == reso["iterations"]
+    assert resg["dt"] == pytest.approx(reso["dt"], rel=1e-14)
+    assert resg["alpha"] == pytest.approx(reso["alpha"], rel=1e-14)
+    np.testing.assert_allclose(eg, eo, rtol=1e-9, atol=1e-11)
+    e = np.max(np.abs(rg - ro)) / np.max(np.abs(ro))
+    print(f"minimizer homog={homog} {param}: profile rel err {e:.2e}", flush=True)
+    assert e < 1e-9
+    assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-9)
+    assert resg["rms_force"] == pytest.approx(reso["rms_force"], rel=1e-8)
+
+
+def test_minimizer_uphill_raises_like_reference():
+    import classicaldft_amd as cd
+
+    shape, dx = (16, 16, 16), 0.25
+    ctx, orc = _pair(shape, dx)
+    rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    mu = orc.bulk_mu(0.05)
+    fire = dict(force_tolerance=1e-9, dt=0.2, dt_max=0.4, max_uphill=0)
+    with pytest.raises(RuntimeError):
+        orc.minimize_grand_potential(rho0, mu, fire=fire, max_steps=60)
+    with pytest.raises(cd.CdftError) as ei:
+        ctx.minimize_grand_potential(rho0, mu, fire=fire, max_steps=60)
+    assert ei.value.code == -2 and "max uphill" in str(ei.value)
+
+
+def test_minimizer_mixed_tracks_fp64():
+    shape, dx = (16, 16, 16), 0.25
+    ctx, orc = _pair(shape, dx, precision="mixed")
+    rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    mu = orc.bulk_mu(0.05)
+    rg, resg, eg = ctx.minimize_grand_potential(rho0, mu, fire=dict(force_tolerance=1e-9), max_steps=30)
+    ro, reso, eo = orc.minimize_grand_potential(rho0, mu, fire=dict(force_tolerance=1e-9), max_steps=30)
+    e = np.max(np.abs(rg - ro)) / np.max(np.abs(ro))
+    print(f"mixed minimizer: profile rel err {e:.2e}, Omega rel err "
+          f"{abs(resg['grand_potential'] - reso['grand_potential']) / abs(reso['grand_potential']):.2e}", flush=True)
+    assert e < 1e-5
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-10), ("mixed", 2e-5)])
+def test_ddft_steps_match_oracle(precision, tol):
+    shape, dx = (16, 8, 8), 0.25
+    ctx, orc = _pair(shape, dx, precision=precision)
+    slab = W.slab(shape, dx, fraction=0.4, width=0.5, rho_l=0.6, rho_v=0.05, noise=1e-3)
+    mu = orc.bulk_mu(0.05)
+    rg, resg, eg = ctx.ddft_run(slab, mu, 5, dt=1e-3, dt_max=1e-3)
+    ro, reso, eo = orc.ddft_run(slab, mu, 5, dt=1e-3, dt_max=1e-3)
+    e = np.max(np.abs(rg - ro)) / np.max(np.abs(ro))
+    print(f"ddft {precision}: profile rel err {e:.2e}; mass drift "
+          f"{abs(resg['mass_final'] - resg['mass_initial']) / resg['mass_initial']:.2e}", flush=True)
+    assert e < tol
+    np.testing.assert_allclose(eg, eo, rtol=tol * 10, atol=1e-9)
+    assert resg["time"] == pytest.approx(reso["time"], rel=1e-12)
+    assert abs(resg["mass_final"] - resg["mass_initial"]) / resg["mass_initial"] < 1e-12
+
+
+def test_ddft_restart_and_dt_growth_fp64():
+    # a large first dt forces restarts (dt /= 10) and dt doubling afterwards
+    shape, dx = (16, 8, 8), 0.25
+    ctx, orc = _pair(shape, dx)
+    slab = W.slab(shape, dx, fraction=0.4, width=0.5, rho_l=0.6, rho_v=0.05, noise=1e-3)
+    mu = orc.bulk_mu(0.05)
+    rg, resg, _ = ctx.ddft_run(slab, mu, 12, dt=0.5, dt_max=0.02, fp_tolerance=1e-6)
+    ro, reso, _ = orc.ddft_run(slab, mu, 12, dt=0.5, dt_max=0.02, fp_tolerance=1e-6)
+    assert resg["dt"] == pytest.approx(reso["dt"], rel=1e-14)
+    assert resg["time"] == pytest.approx(reso["time"], rel=1e-12)
+    assert np.max(np.abs(rg - ro)) / np.max(np.abs(ro)) < 1e-9
+
+
+def test_ddft_ideal_gas_mode_decay():
+    # docs/dynamics/check/main.cpp:99-160: a small cosine mode of an ideal gas decays as exp(Lambda_1 t)
+    import math
+
+    shape, dx = (16, 4, 4), 0.5
+    ctx, _ = _pair(shape, dx, model="none", inter=())
+    x = np.arange(16)
+    rho = (0.2 * (1 + 1e-4 * np.cos(2 * math.pi * x / 16)))[:, None, None] * np.ones(shape)
+    out, res, _ = ctx.ddft_run(rho, 0.0, 20, dt=1e-2, dt_max=1e-2, fp_tolerance=1e-12)
+    lam = 2 / dx ** 2 * (math.cos(2 * math.pi / 16) - 1)
+    amp = np.sum(out[:, 0, 0] * np.cos(2 * math.pi * x / 16)) / 8
+    assert amp / (0.2e-4) == pytest.approx(math.exp(lam * res["time"]), rel=1e-3)
+    assert abs(res["mass_final"] - res["mass_initial"]) / res["mass_initial"] < 1e-12
