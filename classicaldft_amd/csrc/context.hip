@@ -46,7 +46,17 @@ void* dalloc_bytes(size_t bytes) {
 
 }  // namespace
 
+void Context::clear_graphs() {
+  for (auto& e : graphs) {
+    (void)hipGraphExecDestroy(e.exec);
+    (void)hipGraphDestroy(e.graph);
+  }
+  graphs.clear();
+}
+
 Context::~Context() {
+  hipSetDevice(device);
+  clear_graphs();
   void* bufs[] = {d_rho, d_vext, d_force, d_local, d_S0, d_SF, d_SO, d_mf, d_partials, d_res, d_sum_part, d_shift,
                   tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2]};
   hipSetDevice(device);
@@ -142,13 +152,16 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     for (int d = 0; d < 3; ++d) {
       const int L = static_cast<int>(shape[d]);
       std::vector<double> t64(2 * static_cast<size_t>(L));
-      std::vector<float> t32(2 * static_cast<size_t>(L));
+      std::vector<float> t32(4 * static_cast<size_t>(L));  // [0, L) rounded values, [L, 2L) residuals
       for (int m = 0; m < L; ++m) {
         const double a = -2.0 * PI * static_cast<double>(m) / static_cast<double>(L);
         t64[2 * m] = std::cos(a);
         t64[2 * m + 1] = std::sin(a);
-        t32[2 * m] = static_cast<float>(t64[2 * m]);
-        t32[2 * m + 1] = static_cast<float>(t64[2 * m + 1]);
+        for (int q = 0; q < 2; ++q) {
+          const float hi = static_cast<float>(t64[2 * m + q]);
+          t32[2 * m + q] = hi;
+          t32[2 * L + 2 * m + q] = static_cast<float>(t64[2 * m + q] - static_cast<double>(hi));
+        }
       }
       c->tw64[d] = dalloc_bytes(t64.size() * sizeof(double));
       c->tw32[d] = dalloc_bytes(t32.size() * sizeof(float));
@@ -165,7 +178,7 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->d_S0 = dalloc_bytes(cs_bytes * g.M);
     c->d_SF = dalloc_bytes(cs_bytes * g.M * nf_alloc);
     c->d_SO = dalloc_bytes(cs_bytes * g.M);
-    c->d_mf = dalloc_bytes(cs_bytes / 2 * g.M);
+    c->d_mf = dalloc_bytes(8 * g.M);  // FP64 values, or FP32 (value, residual) planes
     c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g) : zpoint_blocks<double>(g);
     c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
     c->d_res = dalloc<double>(8);
@@ -187,17 +200,21 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
         CDFT_HIP(hipMemcpy(d_tab, table.data(), sizeof(double) * g.N, hipMemcpyHostToDevice));
         auto* S = static_cast<cx<double>*>(d_spec);
         const AliasMap id{0, 0.0, 0.0};
-        launch_zr2c<double>(d_tab, S, g, c->plan[2], c->tw<double>(2), id, nullptr, c->stream);
-        launch_lines<double>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<double>(1), -1, c->stream);
-        launch_xlines<double>(S, 1, g, c->plan[0], c->tw<double>(0), -1, c->stream);
+        launch_zr2c<double, double>(d_tab, S, g, c->plan[2], c->tw<double>(2), id, nullptr, c->stream);
+        launch_lines<double, double>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<double>(1), -1,
+                                     c->stream);
+        launch_xlines<double, double>(S, 1, g, c->plan[0], c->tw<double>(0), -1, c->stream);
         CDFT_HIP(hipGetLastError());
         std::vector<cx<double>> spec(static_cast<size_t>(g.M));
         CDFT_HIP(hipMemcpyAsync(spec.data(), d_spec, 16 * g.M, hipMemcpyDeviceToHost, c->stream));
         CDFT_HIP(hipStreamSynchronize(c->stream));
         if (c->precision == CDFT_PRECISION_MIXED) {
-          std::vector<float> re(static_cast<size_t>(g.M));
-          for (long i = 0; i < g.M; ++i) re[i] = static_cast<float>(spec[i].re);
-          CDFT_HIP(hipMemcpy(c->d_mf, re.data(), sizeof(float) * g.M, hipMemcpyHostToDevice));
+          std::vector<float> re(2 * static_cast<size_t>(g.M));
+          for (long i = 0; i < g.M; ++i) {
+            re[i] = static_cast<float>(spec[i].re);
+            re[g.M + i] = static_cast<float>(spec[i].re - static_cast<double>(re[i]));
+          }
+          CDFT_HIP(hipMemcpy(c->d_mf, re.data(), sizeof(float) * 2 * g.M, hipMemcpyHostToDevice));
         } else {
           std::vector<double> re(static_cast<size_t>(g.M));
           for (long i = 0; i < g.M; ++i) re[i] = spec[i].re;
@@ -234,11 +251,11 @@ const char* const kStageNames[kNumStages] = {"k_sum",    "k_zr2c",            "k
                                               "k_lines(y inv nf)", "k_zpoint", "k_lines(y fwd nb)", "k_xbwd",
                                               "k_lines(y inv 1)",  "k_zc2r",   "k_finalize"};
 
-template <typename R>
+template <typename S, typename CE, typename CL>
 void Context::eval_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
-  auto* S0 = static_cast<cx<R>*>(d_S0);
-  auto* SF = static_cast<cx<R>*>(d_SF);
-  auto* SO = static_cast<cx<R>*>(d_SO);
+  auto* S0 = static_cast<cx<S>*>(d_S0);
+  auto* SF = static_cast<cx<S>*>(d_SF);
+  auto* SO = static_cast<cx<S>*>(d_SO);
   const long ylines = static_cast<long>(g.nzh) * g.nx;
   const bool backward = force != nullptr && cs.nb > 0;
   auto mark = [&](int i) {
@@ -249,29 +266,29 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     launch_sum(rho, amap, d_sum_part, g, stream);
     launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
     mark(1);
-    launch_zr2c<R>(rho, S0, g, plan[2], tw<R>(2), amap, d_shift, stream);
+    launch_zr2c<S, CL>(rho, S0, g, plan[2], tw<CL>(2), amap, d_shift, stream);
     mark(2);
-    launch_lines<R>(S0, ylines, plan[1], tw<R>(1), -1, stream);
+    launch_lines<S, CL>(S0, ylines, plan[1], tw<CL>(1), -1, stream);
     mark(3);
-    launch_xfwd<R>(S0, SF, static_cast<const R*>(d_mf), g, plan[0], tw<R>(0), cs, stream);
+    launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(d_mf), g, plan[0], tw<CE>(0), cs, stream);
     mark(4);
-    launch_lines<R>(SF, ylines * cs.nf, plan[1], tw<R>(1), +1, stream);
+    launch_lines<S, CE>(SF, ylines * cs.nf, plan[1], tw<CE>(1), +1, stream);
   }
   ChannelSet cz = cs;
   if (!backward) cz.nb = 0;
   double* lf = backward ? d_local : (force ? force : d_local);
   mark(5);
-  launch_zpoint<R>(SF, rho, vext, lf, d_partials, g, plan[2], tw<R>(2), cz, fp, mu, amap,
-                   cs.nf > 0 ? d_shift : nullptr, dc, stream);
+  launch_zpoint<S, CE, CL>(SF, rho, vext, lf, d_partials, g, plan[2], tw<CE>(2), tw<CL>(2), cz, fp, mu, amap,
+                           cs.nf > 0 ? d_shift : nullptr, dc, stream);
   if (backward) {
     mark(6);
-    launch_lines<R>(SF, ylines * cs.nb, plan[1], tw<R>(1), -1, stream);
+    launch_lines<S, CL>(SF, ylines * cs.nb, plan[1], tw<CL>(1), -1, stream);
     mark(7);
-    launch_xbwd<R>(SF, SO, g, plan[0], tw<R>(0), cs.nb, stream);
+    launch_xbwd<S, CL>(SF, SO, g, plan[0], tw<CL>(0), cs.nb, stream);
     mark(8);
-    launch_lines<R>(SO, ylines, plan[1], tw<R>(1), +1, stream);
+    launch_lines<S, CL>(SO, ylines, plan[1], tw<CL>(1), +1, stream);
     mark(9);
-    launch_zc2r<R>(SO, 1, force, d_local, g.dv, g, plan[2], tw<R>(2), nullptr, DcTable{}, stream);
+    launch_zc2r<S, CL>(SO, 1, force, d_local, g.dv, g, plan[2], tw<CL>(2), nullptr, DcTable{}, stream);
   }
   mark(10);
   launch_finalize(d_partials, nblk_point, d_res, stream);
@@ -290,7 +307,7 @@ void Context::stage_bytes(double* b, bool with_vext) const {
     b[0] = 8.0 * N;                                     // FP64 sum of rho
     b[1] = 8.0 * N + sf * M;                            // rho (FP64) -> rho_hat
     b[2] = 2.0 * sf * M;                                // in place
-    b[3] = sf * M + nf * sf * M + (cs.mf ? rf * M : 0);  // rho_hat + table -> nf channels
+    b[3] = sf * M + nf * sf * M + (cs.mf ? sf * M : 0);  // rho_hat + table (value, residual) -> nf channels
     b[4] = 2.0 * nf * sf * M;
   }
   b[5] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;  // channels, rho, V -> derivs, local
@@ -304,9 +321,39 @@ void Context::stage_bytes(double* b, bool with_vext) const {
 }
 
 void Context::eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
-  if (precision == CDFT_PRECISION_MIXED) eval_t<float>(rho, vext, mu, force, amap);
-  else eval_t<double>(rho, vext, mu, force, amap);
-  CDFT_HIP(hipGetLastError());
+  auto direct = [&] {
+    if (precision == CDFT_PRECISION_MIXED) eval_t<float, double, float>(rho, vext, mu, force, amap);
+    else eval_t<double, double, double>(rho, vext, mu, force, amap);
+  };
+  if (!use_graphs || stage_events) {
+    direct();
+    CDFT_HIP(hipGetLastError());
+    return;
+  }
+  // The ~12 launches of one evaluation are replayed from a captured graph:
+  // one submission instead of a dozen kernel launches.
+  const GraphKey key{rho, vext, force, mu, amap};
+  for (auto& e : graphs) {
+    if (e.key == key) {
+      CDFT_HIP(hipGraphLaunch(e.exec, stream));
+      return;
+    }
+  }
+  if (graphs.size() >= 16) clear_graphs();
+  GraphEntry ent{key, nullptr, nullptr};
+  CDFT_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+  try {
+    direct();
+  } catch (...) {
+    hipGraph_t g0 = nullptr;
+    (void)hipStreamEndCapture(stream, &g0);
+    if (g0) (void)hipGraphDestroy(g0);
+    throw;
+  }
+  CDFT_HIP(hipStreamEndCapture(stream, &ent.graph));
+  CDFT_HIP(hipGraphInstantiate(&ent.exec, ent.graph, nullptr, nullptr, 0));
+  graphs.push_back(ent);
+  CDFT_HIP(hipGraphLaunch(ent.exec, stream));
 }
 
 void Context::result_from(const double* res, double mu, cdft_result* out) const {
@@ -327,62 +374,59 @@ void Context::fetch_result(double mu, cdft_result* out) {
   result_from(res, mu, out);
 }
 
-void Context::r2c(const double* d_in) {
+namespace {
+template <typename S, typename C>
+void r2c_t(Context* c, const double* d_in) {
   const AliasMap id{0, 0.0, 0.0};
+  auto* Sp = static_cast<cx<S>*>(c->d_S0);
+  const Geom& g = c->g;
+  launch_zr2c<S, C>(d_in, Sp, g, c->plan[2], c->tw<C>(2), id, nullptr, c->stream);
+  launch_lines<S, C>(Sp, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<C>(1), -1, c->stream);
+  launch_xlines<S, C>(Sp, 1, g, c->plan[0], c->tw<C>(0), -1, c->stream);
+}
+
+template <typename S, typename C>
+void c2r_t(Context* c, double* d_out) {
+  auto* Sp = static_cast<cx<S>*>(c->d_S0);
+  const Geom& g = c->g;
+  launch_xlines<S, C>(Sp, 1, g, c->plan[0], c->tw<C>(0), +1, c->stream);
+  launch_lines<S, C>(Sp, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<C>(1), +1, c->stream);
+  launch_zc2r<S, C>(Sp, 1, d_out, nullptr, 1.0, g, c->plan[2], c->tw<C>(2), nullptr, DcTable{}, c->stream);
+}
+
+template <typename S, typename CE, typename CL>
+void wd_t(Context* c, const double* d_rho_in, double* d_out) {
+  const AliasMap id{0, 0.0, 0.0};
+  const Geom& g = c->g;
   const long ylines = static_cast<long>(g.nzh) * g.nx;
-  if (precision == CDFT_PRECISION_MIXED) {
-    auto* S = static_cast<cx<float>*>(d_S0);
-    launch_zr2c<float>(d_in, S, g, plan[2], tw<float>(2), id, nullptr, stream);
-    launch_lines<float>(S, ylines, plan[1], tw<float>(1), -1, stream);
-    launch_xlines<float>(S, 1, g, plan[0], tw<float>(0), -1, stream);
-  } else {
-    auto* S = static_cast<cx<double>*>(d_S0);
-    launch_zr2c<double>(d_in, S, g, plan[2], tw<double>(2), id, nullptr, stream);
-    launch_lines<double>(S, ylines, plan[1], tw<double>(1), -1, stream);
-    launch_xlines<double>(S, 1, g, plan[0], tw<double>(0), -1, stream);
-  }
+  auto* S0 = static_cast<cx<S>*>(c->d_S0);
+  auto* SF = static_cast<cx<S>*>(c->d_SF);
+  launch_sum(d_rho_in, id, c->d_sum_part, g, c->stream);
+  launch_finalize(c->d_sum_part, c->nblk_sum, c->d_shift, c->stream);
+  launch_zr2c<S, CL>(d_rho_in, S0, g, c->plan[2], c->tw<CL>(2), id, c->d_shift, c->stream);
+  launch_lines<S, CL>(S0, ylines, c->plan[1], c->tw<CL>(1), -1, c->stream);
+  launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(c->d_mf), g, c->plan[0], c->tw<CE>(0), c->cs, c->stream);
+  launch_lines<S, CE>(SF, ylines * c->cs.nf, c->plan[1], c->tw<CE>(1), +1, c->stream);
+  launch_zc2r<S, CE>(SF, c->cs.nf, d_out, nullptr, 1.0, g, c->plan[2], c->tw<CE>(2), c->d_shift, c->dc, c->stream);
+}
+}  // namespace
+
+void Context::r2c(const double* d_in) {
+  if (precision == CDFT_PRECISION_MIXED) r2c_t<float, float>(this, d_in);
+  else r2c_t<double, double>(this, d_in);
   CDFT_HIP(hipGetLastError());
 }
 
 void Context::c2r(double* d_out) {
-  const long ylines = static_cast<long>(g.nzh) * g.nx;
-  if (precision == CDFT_PRECISION_MIXED) {
-    auto* S = static_cast<cx<float>*>(d_S0);
-    launch_xlines<float>(S, 1, g, plan[0], tw<float>(0), +1, stream);
-    launch_lines<float>(S, ylines, plan[1], tw<float>(1), +1, stream);
-    launch_zc2r<float>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), nullptr, DcTable{}, stream);
-  } else {
-    auto* S = static_cast<cx<double>*>(d_S0);
-    launch_xlines<double>(S, 1, g, plan[0], tw<double>(0), +1, stream);
-    launch_lines<double>(S, ylines, plan[1], tw<double>(1), +1, stream);
-    launch_zc2r<double>(S, 1, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), nullptr, DcTable{}, stream);
-  }
+  if (precision == CDFT_PRECISION_MIXED) c2r_t<float, float>(this, d_out);
+  else c2r_t<double, double>(this, d_out);
   CDFT_HIP(hipGetLastError());
 }
 
 void Context::weighted_densities(const double* d_rho_in, double* d_out) {
-  const AliasMap id{0, 0.0, 0.0};
-  const long ylines = static_cast<long>(g.nzh) * g.nx;
   if (cs.nf == 0) return;
-  launch_sum(d_rho_in, id, d_sum_part, g, stream);
-  launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
-  if (precision == CDFT_PRECISION_MIXED) {
-    auto* S0 = static_cast<cx<float>*>(d_S0);
-    auto* SF = static_cast<cx<float>*>(d_SF);
-    launch_zr2c<float>(d_rho_in, S0, g, plan[2], tw<float>(2), id, d_shift, stream);
-    launch_lines<float>(S0, ylines, plan[1], tw<float>(1), -1, stream);
-    launch_xfwd<float>(S0, SF, static_cast<const float*>(d_mf), g, plan[0], tw<float>(0), cs, stream);
-    launch_lines<float>(SF, ylines * cs.nf, plan[1], tw<float>(1), +1, stream);
-    launch_zc2r<float>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<float>(2), d_shift, dc, stream);
-  } else {
-    auto* S0 = static_cast<cx<double>*>(d_S0);
-    auto* SF = static_cast<cx<double>*>(d_SF);
-    launch_zr2c<double>(d_rho_in, S0, g, plan[2], tw<double>(2), id, d_shift, stream);
-    launch_lines<double>(S0, ylines, plan[1], tw<double>(1), -1, stream);
-    launch_xfwd<double>(S0, SF, static_cast<const double*>(d_mf), g, plan[0], tw<double>(0), cs, stream);
-    launch_lines<double>(SF, ylines * cs.nf, plan[1], tw<double>(1), +1, stream);
-    launch_zc2r<double>(SF, cs.nf, d_out, nullptr, 1.0, g, plan[2], tw<double>(2), d_shift, dc, stream);
-  }
+  if (precision == CDFT_PRECISION_MIXED) wd_t<float, double, float>(this, d_rho_in, d_out);
+  else wd_t<double, double, double>(this, d_rho_in, d_out);
   CDFT_HIP(hipGetLastError());
 }
 

@@ -75,19 +75,43 @@ struct Context {
   DcTable dc{};                  // analytic k = 0 weight of every forward channel
   hipEvent_t* stage_events = nullptr;  // when set, eval records kNumStages+1 events
 
+  // Captured evaluation graphs, keyed by the arguments baked into them.
+  struct GraphKey {
+    const double* rho;
+    const double* vext;
+    double* force;
+    double mu;
+    AliasMap amap;
+    bool operator==(const GraphKey& o) const {
+      return rho == o.rho && vext == o.vext && force == o.force && mu == o.mu && amap.kind == o.amap.kind &&
+             amap.rho_min == o.amap.rho_min && amap.range == o.amap.range;
+    }
+  };
+  struct GraphEntry {
+    GraphKey key;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  bool use_graphs = true;
+  void clear_graphs();
+
   ~Context();
   size_t csize() const { return precision == CDFT_PRECISION_MIXED ? 8 : 16; }
 
-  template <typename R>
-  const cx<R>* tw(int axis) const {
-    return static_cast<const cx<R>*>(sizeof(R) == 4 ? tw32[axis] : tw64[axis]);
+  // Twiddle table of one axis for arithmetic type C (FP32: value + residual pairs).
+  template <typename C>
+  const cx<C>* tw(int axis) const {
+    return static_cast<const cx<C>*>(sizeof(C) == 4 ? tw32[axis] : tw64[axis]);
   }
 
   // One F + dF/drho evaluation on device arrays (rho may be an alias variable
   // mapped through `amap`).  Results land in d_res (device).  When `force`
   // is null only the energy is computed.
   void eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
-  template <typename R>
+  // S: spectral storage type; CE: arithmetic of the energy path's inverse
+  // transforms; CL: arithmetic of the other transforms.
+  template <typename S, typename CE, typename CL>
   void eval_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
 
   // Read back d_res and assemble a cdft_result.

@@ -199,60 +199,70 @@ __global__ void __launch_bounds__(256) k_ddft_rhs(const double* __restrict__ rho
 // (dynamics.hpp:247-289): x-FFT of R1 (z, y done), combine
 // w = U0 rho0_k + U1 R0_k + U2 (R1_k - R0_k) on the mean-free spectra (the
 // k = 0 mode is propagated separately in FP64), inverse x-FFT.
-template <typename R>
-__global__ void k_xprop(const cx<R>* __restrict__ SR1, const cx<R>* __restrict__ Srho0, const cx<R>* __restrict__ SR0,
-                        cx<R>* __restrict__ Sout, Geom g, FftPlan px, const cx<R>* __restrict__ twx, int ty,
-                        const double* __restrict__ lam, const double* __restrict__ fac, double dt, int first_iter) {
+template <typename S, typename C, int EPT>
+__global__ void __launch_bounds__(256) k_xprop(const cx<S>* __restrict__ SR1, const cx<S>* __restrict__ Srho0,
+                                               const cx<S>* __restrict__ SR0, cx<S>* __restrict__ Sout, Geom g,
+                                               FftPlan px, const cx<C>* __restrict__ twx, int ty,
+                                               const double* __restrict__ lam, const double* __restrict__ fac,
+                                               double dt, int first_iter) {
   extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
-  cx<R>* buf1 = buf0 + ty * ls;
   const int y0 = blockIdx.x * ty;
   const int kz = blockIdx.y;
   const long off = static_cast<long>(kz) * nx * ny;
-  const cx<R>* res = nullptr;
   if (!first_iter) {
     for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
       const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-      buf0[yl * ls + x] = (y < ny) ? SR1[off + x * ny + y] : cx<R>{R(0), R(0)};
+      const cx<S> v = (y < ny) ? SR1[off + x * ny + y] : cx<S>{S(0), S(0)};
+      buf[yl * ls + x] = {static_cast<C>(v.re), static_cast<C>(v.im)};
     }
     __syncthreads();
-    res = fft_lines(buf0, buf1, ty, ls, px, -1, twx);
-    __syncthreads();
+    fft_inplace<C, EPT>(buf, ty, ls, px, -1, twx);
   }
   // lam/fac layout: [0, nx) x axis, [nx, nx+ny) y axis, [nx+ny, nx+ny+nzh) z axis
-  cx<R> wv[16];
-  int cnt = 0;
-  for (int e = threadIdx.x; e < ty * nx; e += blockDim.x, ++cnt) {
-    const int x = e / ty, yl = e - x * ty, y = min(y0 + yl, ny - 1);
-    const double L = lam[x] + lam[nx + y] + lam[nx + ny + kz];
-    const double ex = fac[x] * fac[nx + y] * fac[nx + ny + kz];
-    const bool zero = (x == 0 && y == 0 && kz == 0);
-    const double U0 = ex;
-    const double U1 = zero ? dt : (ex - 1.0) / L;
-    const double U2 = zero ? dt / 2.0 : (ex - 1.0 - dt * L) / (L * L * dt);
-    const long idx = off + x * ny + y;
-    const cx<R> r0 = SR0[idx], p0 = Srho0[idx];
-    const cx<R> r1 = first_iter ? r0 : res[yl * ls + x];
-    double wr = U0 * static_cast<double>(p0.re) + U1 * static_cast<double>(r0.re) +
-                U2 * (static_cast<double>(r1.re) - static_cast<double>(r0.re));
-    double wi = U0 * static_cast<double>(p0.im) + U1 * static_cast<double>(r0.im) +
-                U2 * (static_cast<double>(r1.im) - static_cast<double>(r0.im));
-    if (zero) wr = wi = 0.0;  // mean mode handled in FP64 by k_ddft_update
-    wv[cnt] = {static_cast<R>(wr), static_cast<R>(wi)};
+  cx<C> wv[EPT];
+#pragma unroll
+  for (int m = 0; m < EPT; ++m) {
+    const int e = threadIdx.x + m * blockDim.x;
+    wv[m] = {C(0), C(0)};
+    if (e < ty * nx) {
+      const int x = e / ty, yl = e - x * ty, y = min(y0 + yl, ny - 1);
+      const double L = lam[x] + lam[nx + y] + lam[nx + ny + kz];
+      const double ex = fac[x] * fac[nx + y] * fac[nx + ny + kz];
+      const bool zero = (x == 0 && y == 0 && kz == 0);
+      const double U0 = ex;
+      const double U1 = zero ? dt : (ex - 1.0) / L;
+      const double U2 = zero ? dt / 2.0 : (ex - 1.0 - dt * L) / (L * L * dt);
+      const long idx = off + x * ny + y;
+      const cx<S> r0 = SR0[idx], p0 = Srho0[idx];
+      double r1re = static_cast<double>(r0.re), r1im = static_cast<double>(r0.im);
+      if (!first_iter) {
+        r1re = static_cast<double>(buf[yl * ls + x].re);
+        r1im = static_cast<double>(buf[yl * ls + x].im);
+      }
+      double wr = U0 * static_cast<double>(p0.re) + U1 * static_cast<double>(r0.re) +
+                  U2 * (r1re - static_cast<double>(r0.re));
+      double wi = U0 * static_cast<double>(p0.im) + U1 * static_cast<double>(r0.im) +
+                  U2 * (r1im - static_cast<double>(r0.im));
+      if (zero) wr = wi = 0.0;  // mean mode handled in FP64 by k_ddft_update
+      wv[m] = {static_cast<C>(wr), static_cast<C>(wi)};
+    }
   }
   __syncthreads();
-  cnt = 0;
-  for (int e = threadIdx.x; e < ty * nx; e += blockDim.x, ++cnt) {
-    const int x = e / ty, yl = e - x * ty;
-    buf0[yl * ls + x] = wv[cnt];
+#pragma unroll
+  for (int m = 0; m < EPT; ++m) {
+    const int e = threadIdx.x + m * blockDim.x;
+    if (e < ty * nx) {
+      const int x = e / ty, yl = e - x * ty;
+      buf[yl * ls + x] = wv[m];
+    }
   }
   __syncthreads();
-  const cx<R>* r = fft_lines(buf0, buf1, ty, ls, px, +1, twx);
-  __syncthreads();
+  fft_inplace<C, EPT>(buf, ty, ls, px, +1, twx);
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-    if (y < ny) Sout[off + x * ny + y] = r[yl * ls + x];
+    if (y < ny) Sout[off + x * ny + y] = {static_cast<S>(buf[yl * ls + x].re), static_cast<S>(buf[yl * ls + x].im)};
   }
 }
 
@@ -445,9 +455,9 @@ struct DdftRun {
     double* s = sums.as<double>() + 8 * slot;
     launch_sum(field, id, part.as<double>(), g, c->stream);
     launch_finalize(part.as<double>(), sum_blocks(g), s, c->stream);
-    launch_zr2c<R>(field, S, g, c->plan[2], c->tw<R>(2), id, s, c->stream);
-    launch_lines<R>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), -1, c->stream);
-    if (x_stage) launch_xlines<R>(S, 1, g, c->plan[0], c->tw<R>(0), -1, c->stream);
+    launch_zr2c<R, R>(field, S, g, c->plan[2], c->tw<R>(2), id, s, c->stream);
+    launch_lines<R, R>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), -1, c->stream);
+    if (x_stage) launch_xlines<R, R>(S, 1, g, c->plan[0], c->tw<R>(0), -1, c->stream);
   }
 
   // excess_rhs_fourier (dynamics.hpp:212-229) into S (x stage optional)
@@ -463,15 +473,22 @@ struct DdftRun {
   template <typename R>
   void propagate(bool first_iter, double* dev, double* mn) {
     const Geom& g = c->g;
-    const int ty = sizeof(R) == 4 ? 16 : 8;
-    const size_t sh = 2 * sizeof(cx<R>) * ty * (g.nx + 1);
-    if (ty * g.nx > 16 * 256) throw Unsupported("x extent too large for the propagator kernel");
-    hipLaunchKernelGGL(k_xprop<R>, dim3((g.ny + ty - 1) / ty, g.nzh), dim3(256), sh, c->stream, SR1.as<cx<R>>(),
-                       Srho0.as<cx<R>>(), SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty,
-                       lam.as<double>(), fac.as<double>(), st_dt, first_iter ? 1 : 0);
-    launch_lines<R>(Sw.as<cx<R>>(), static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), +1, c->stream);
-    launch_zc2r<R>(Sw.as<cx<R>>(), 1, raw.as<double>(), nullptr, g.inv_n, g, c->plan[2], c->tw<R>(2), nullptr,
-                   DcTable{}, c->stream);
+    const int ty = 16;
+    const size_t sh = sizeof(cx<R>) * ty * (g.nx + 1);
+    const dim3 grid((g.ny + ty - 1) / ty, g.nzh);
+    if (ty * g.nx <= 8 * 256)
+      hipLaunchKernelGGL((k_xprop<R, R, 8>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
+                         SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty, lam.as<double>(),
+                         fac.as<double>(), st_dt, first_iter ? 1 : 0);
+    else if (ty * g.nx <= 16 * 256)
+      hipLaunchKernelGGL((k_xprop<R, R, 16>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
+                         SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty, lam.as<double>(),
+                         fac.as<double>(), st_dt, first_iter ? 1 : 0);
+    else
+      throw Unsupported("x extent too large for the propagator kernel");
+    launch_lines<R, R>(Sw.as<cx<R>>(), static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), +1, c->stream);
+    launch_zc2r<R, R>(Sw.as<cx<R>>(), 1, raw.as<double>(), nullptr, g.inv_n, g, c->plan[2], c->tw<R>(2), nullptr,
+                      DcTable{}, c->stream);
     const AliasMap id{0, 0.0, 0.0};
     launch_sum(raw.as<double>(), id, part.as<double>(), g, c->stream);
     launch_finalize(part.as<double>(), sum_blocks(g), sums.as<double>() + 24, c->stream);

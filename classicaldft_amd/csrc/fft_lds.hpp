@@ -1,11 +1,17 @@
 // LDS-resident Stockham FFT building blocks for gfx950 (CDNA4).
 //
-// A work-group holds `nl` lines of length L in LDS (line stride `ls`, complex
-// elements) and transforms all of them cooperatively: each radix pass reads
-// one buffer and writes the other (ping-pong), so no thread ever waits on
-// another inside a pass; one barrier separates passes.  Radices 2, 3, 4, 5
-// and 8 cover every grid length of the form 2^a 3^b 5^c.  Twiddles come from
-// a per-length table exp(-2 pi i m / L), m < L, computed in FP64 on the host.
+// A work-group holds `nl` lines of length L in one LDS buffer (line stride
+// `ls`, complex elements) and transforms all of them cooperatively, in place:
+// in each radix pass every thread gathers its butterflies into registers,
+// the block synchronises, and every thread scatters its outputs.  Batching
+// many lines (and several channels) into one call keeps every thread busy in
+// every pass.  Radices 2, 3, 4, 5 and 8 cover every grid length of the form
+// 2^a 3^b 5^c.  Twiddles come from a per-length table exp(-2 pi i m / L),
+// m < L, computed in FP64 on the host.  The arithmetic type C is independent
+// of the type the caller stores in HBM.
+//
+// In FP32 every twiddle and every radix constant is applied as an FP32 pair
+// (value + residual), which removes the coherent part of the FP32 error.
 //
 // sign = -1 is the forward transform (e^{-ikr}), +1 the backward one; both
 // are unnormalised, as FFTW's r2c / c2r are in the reference
@@ -47,6 +53,21 @@ __device__ __forceinline__ cx<R> cmul_si(cx<R> a, int sg) {
   return sg < 0 ? cx<R>{a.im, -a.re} : cx<R>{-a.im, a.re};
 }
 
+// x * c for a real constant c.  In FP32 the constant is applied as the sum of
+// its FP32 value and its FP32 residual (c = hi + lo), so the product carries
+// the constant to ~2^-48: FP32 rounding of a constant is the same for every
+// line of a batch and would otherwise add up coherently.
+template <typename R>
+__device__ __forceinline__ R kmul(R x, double c) {
+  if constexpr (sizeof(R) == 4) {
+    const float hi = static_cast<float>(c);
+    const float lo = static_cast<float>(c - static_cast<double>(hi));
+    return fmaf(x, lo, x * hi);
+  } else {
+    return x * c;
+  }
+}
+
 // Per-axis transform plan: length and radix sequence (product == L).
 struct FftPlan {
   int L;
@@ -78,11 +99,11 @@ __device__ __forceinline__ void dft8(cx<R>* v, int sg) {
   cx<R> o[4] = {v[1], v[3], v[5], v[7]};
   dft4(e, sg);
   dft4(o, sg);
-  const R h = R(0.70710678118654752440084436210484903928);
+  constexpr double h = 0.70710678118654752440084436210484903928;
   // w8^1 = (1 + sg i)/sqrt2, w8^2 = sg i, w8^3 = (-1 + sg i)/sqrt2
-  cx<R> o1 = {h * (o[1].re - R(sg) * o[1].im), h * (o[1].im + R(sg) * o[1].re)};
+  cx<R> o1 = {kmul<R>(o[1].re - R(sg) * o[1].im, h), kmul<R>(o[1].im + R(sg) * o[1].re, h)};
   cx<R> o2 = cmul_si(o[2], sg);
-  cx<R> o3 = {h * (-o[3].re - R(sg) * o[3].im), h * (-o[3].im + R(sg) * o[3].re)};
+  cx<R> o3 = {kmul<R>(-o[3].re - R(sg) * o[3].im, h), kmul<R>(-o[3].im + R(sg) * o[3].re, h)};
   v[0] = cadd(e[0], o[0]);
   v[4] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);
@@ -95,9 +116,10 @@ __device__ __forceinline__ void dft8(cx<R>* v, int sg) {
 
 template <typename R>
 __device__ __forceinline__ void dft3(cx<R>* v, int sg) {
-  const R s3 = R(0.86602540378443864676372317075293618347);
+  constexpr double s3 = 0.86602540378443864676372317075293618347;
   cx<R> t = cadd(v[1], v[2]);
-  cx<R> d = cscale(cmul_si(csub(v[1], v[2]), sg), s3);
+  cx<R> dd = cmul_si(csub(v[1], v[2]), sg);
+  cx<R> d = {kmul<R>(dd.re, s3), kmul<R>(dd.im, s3)};
   cx<R> m = {v[0].re - R(0.5) * t.re, v[0].im - R(0.5) * t.im};
   v[0] = cadd(v[0], t);
   v[1] = cadd(m, d);
@@ -106,16 +128,16 @@ __device__ __forceinline__ void dft3(cx<R>* v, int sg) {
 
 template <typename R>
 __device__ __forceinline__ void dft5(cx<R>* v, int sg) {
-  const R c1 = R(0.30901699437494742410229341718281905886);   // cos(2pi/5)
-  const R c2 = R(-0.80901699437494742410229341718281905886);  // cos(4pi/5)
-  const R s1 = R(0.95105651629515357211643933337938214340);   // sin(2pi/5)
-  const R s2 = R(0.58778525229247312916870595463907276860);   // sin(4pi/5)
+  constexpr double c1 = 0.30901699437494742410229341718281905886;   // cos(2pi/5)
+  constexpr double c2 = -0.80901699437494742410229341718281905886;  // cos(4pi/5)
+  constexpr double s1 = 0.95105651629515357211643933337938214340;   // sin(2pi/5)
+  constexpr double s2 = 0.58778525229247312916870595463907276860;   // sin(4pi/5)
   cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
   cx<R> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  cx<R> a1 = {v[0].re + c1 * t1.re + c2 * t2.re, v[0].im + c1 * t1.im + c2 * t2.im};
-  cx<R> a2 = {v[0].re + c2 * t1.re + c1 * t2.re, v[0].im + c2 * t1.im + c1 * t2.im};
-  cx<R> b1 = cmul_si(cx<R>{s1 * t3.re + s2 * t4.re, s1 * t3.im + s2 * t4.im}, sg);
-  cx<R> b2 = cmul_si(cx<R>{s2 * t3.re - s1 * t4.re, s2 * t3.im - s1 * t4.im}, sg);
+  cx<R> a1 = {v[0].re + kmul<R>(t1.re, c1) + kmul<R>(t2.re, c2), v[0].im + kmul<R>(t1.im, c1) + kmul<R>(t2.im, c2)};
+  cx<R> a2 = {v[0].re + kmul<R>(t1.re, c2) + kmul<R>(t2.re, c1), v[0].im + kmul<R>(t1.im, c2) + kmul<R>(t2.im, c1)};
+  cx<R> b1 = cmul_si(cx<R>{kmul<R>(t3.re, s1) + kmul<R>(t4.re, s2), kmul<R>(t3.im, s1) + kmul<R>(t4.im, s2)}, sg);
+  cx<R> b2 = cmul_si(cx<R>{kmul<R>(t3.re, s2) - kmul<R>(t4.re, s1), kmul<R>(t3.im, s2) - kmul<R>(t4.im, s1)}, sg);
   v[0] = cadd(v[0], cadd(t1, t2));
   v[1] = cadd(a1, b1);
   v[4] = csub(a1, b1);
@@ -132,62 +154,81 @@ __device__ __forceinline__ void dft_r(cx<R>* v, int sg) {
   else dft8(v, sg);
 }
 
-// One Stockham pass over all lines: src -> dst.
-template <typename R, int RAD>
-__device__ __forceinline__ void stockham_pass(const cx<R>* __restrict__ src, cx<R>* __restrict__ dst, int nl,
-                                              int ls, int L, int ns, int sg, const cx<R>* __restrict__ tw) {
+// One in-place Stockham pass over `nl` lines held in one LDS buffer: every
+// thread first gathers, twiddles and transforms all of its butterflies in
+// registers, the block synchronises, then every thread writes its outputs.
+// MAXB bounds the butterflies per thread (nl * L / RAD <= MAXB * blockDim.x).
+// The FP32 twiddle table holds the rounded values in [0, L) and their
+// residuals in [L, 2L); FP64 tables hold [0, L).
+template <typename C, int RAD, int MAXB>
+__device__ __forceinline__ void pass_ip(cx<C>* buf, int nl, int ls, int L, int ns, int sg,
+                                        const cx<C>* __restrict__ tw) {
   const int nb = L / RAD;
   const int tot = nl * nb;
   const int tws = L / (ns * RAD);
-  for (int b = threadIdx.x; b < tot; b += blockDim.x) {
-    const int line = b / nb;
-    const int j = b - line * nb;
-    const int k = j % ns;
-    const cx<R>* s = src + line * ls;
-    cx<R> v[RAD];
+  cx<C> v[MAXB][RAD];
 #pragma unroll
-    for (int i = 0; i < RAD; ++i) v[i] = s[j + i * nb];
-    if (ns > 1) {
+  for (int m = 0; m < MAXB; ++m) {
+    const int b = threadIdx.x + m * blockDim.x;
+    if (b < tot) {
+      const int line = b / nb;
+      const int j = b - line * nb;
+      const int k = j % ns;
+      const cx<C>* s = buf + line * ls;
 #pragma unroll
-      for (int i = 1; i < RAD; ++i) {
-        cx<R> w = tw[i * k * tws];
-        if (sg > 0) w.im = -w.im;
-        v[i] = cmul(v[i], w);
+      for (int i = 0; i < RAD; ++i) v[m][i] = s[j + i * nb];
+      if (ns > 1) {
+#pragma unroll
+        for (int i = 1; i < RAD; ++i) {
+          const int ti = i * k * tws;
+          cx<C> w = tw[ti];
+          if (sg > 0) w.im = -w.im;
+          if constexpr (sizeof(C) == 4) {
+            cx<C> wl = tw[L + ti];
+            if (sg > 0) wl.im = -wl.im;
+            v[m][i] = cadd(cmul(v[m][i], w), cmul(v[m][i], wl));
+          } else {
+            v[m][i] = cmul(v[m][i], w);
+          }
+        }
       }
+      dft_r<C, RAD>(v[m], sg);
     }
-    dft_r<R, RAD>(v, sg);
-    const int base = (j / ns) * ns * RAD + k;
-    cx<R>* d = dst + line * ls;
-#pragma unroll
-    for (int i = 0; i < RAD; ++i) d[base + i * ns] = v[i];
   }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < MAXB; ++m) {
+    const int b = threadIdx.x + m * blockDim.x;
+    if (b < tot) {
+      const int line = b / nb;
+      const int j = b - line * nb;
+      const int k = j % ns;
+      const int base = (j / ns) * ns * RAD + k;
+      cx<C>* d = buf + line * ls;
+#pragma unroll
+      for (int i = 0; i < RAD; ++i) d[base + i * ns] = v[m][i];
+    }
+  }
+  __syncthreads();
 }
 
-// Transform `nl` lines held in buf0; returns the buffer holding the result
-// (buf0 or buf1).  Caller must __syncthreads() before (data in buf0 visible)
-// and after (before reading the result).
-template <typename R>
-__device__ cx<R>* fft_lines(cx<R>* buf0, cx<R>* buf1, int nl, int ls, const FftPlan& p, int sg,
-                            const cx<R>* __restrict__ tw) {
-  cx<R>* a = buf0;
-  cx<R>* b = buf1;
+// Transform `nl` lines held in `buf` in place.  MAXE bounds the elements per
+// thread (nl * L <= MAXE * blockDim.x).  The caller synchronises before the
+// call (inputs visible); the call ends with a barrier (outputs visible).
+template <typename C, int MAXE>
+__device__ void fft_inplace(cx<C>* buf, int nl, int ls, const FftPlan& p, int sg, const cx<C>* __restrict__ tw) {
   int ns = 1;
   for (int q = 0; q < p.np; ++q) {
     const int r = p.rad[q];
     switch (r) {
-      case 8: stockham_pass<R, 8>(a, b, nl, ls, p.L, ns, sg, tw); break;
-      case 4: stockham_pass<R, 4>(a, b, nl, ls, p.L, ns, sg, tw); break;
-      case 2: stockham_pass<R, 2>(a, b, nl, ls, p.L, ns, sg, tw); break;
-      case 3: stockham_pass<R, 3>(a, b, nl, ls, p.L, ns, sg, tw); break;
-      default: stockham_pass<R, 5>(a, b, nl, ls, p.L, ns, sg, tw); break;
+      case 8: pass_ip<C, 8, (MAXE + 7) / 8>(buf, nl, ls, p.L, ns, sg, tw); break;
+      case 4: pass_ip<C, 4, (MAXE + 3) / 4>(buf, nl, ls, p.L, ns, sg, tw); break;
+      case 2: pass_ip<C, 2, (MAXE + 1) / 2>(buf, nl, ls, p.L, ns, sg, tw); break;
+      case 3: pass_ip<C, 3, (MAXE + 2) / 3>(buf, nl, ls, p.L, ns, sg, tw); break;
+      default: pass_ip<C, 5, (MAXE + 4) / 5>(buf, nl, ls, p.L, ns, sg, tw); break;
     }
     ns *= r;
-    cx<R>* t = a;
-    a = b;
-    b = t;
-    if (q + 1 < p.np) __syncthreads();
   }
-  return a;
 }
 
 }  // namespace cdft

@@ -3,26 +3,35 @@
 // Data layout in HBM
 //   real fields (rho, V, forces, x, v):  double, idx = iz + Nz*(iy + Ny*ix)
 //                                        (grid.hpp:35-37 of the reference)
-//   spectral work arrays:                cx<R>, "plane-major": idx = iy + Ny*(ix + Nx*kz),
+//   spectral work arrays:                cx<S>, "plane-major": idx = iy + Ny*(ix + Nx*kz),
 //                                        kz in [0, Nz/2], one array of M per channel
 // The plane-major order makes the y lines contiguous, the x lines
 // Ny-strided inside one contiguous kz plane, and lets a z-line kernel that
 // owns consecutive (ix,iy) lines write every kz plane with unit stride.
 //
+// Every transform kernel is templated on the storage type S of the spectral
+// arrays in HBM and on the arithmetic type C used in LDS and registers.  The
+// mixed-precision evaluation stores FP32 everywhere (half the traffic) and
+// computes the energy path's inverse transforms in FP64: a smooth density
+// concentrates its spectrum in a few large low-k values, whose FP32
+// arithmetic errors would otherwise add up coherently in Omega.
+//
 // Evaluation schedule (one species, FMT + optional mean field):
-//   k_zr2c      z-R2C of rho                                   (real -> S0)
+//   k_sum + k_finalize   FP64 sum of rho (mean-density split)
+//   k_zr2c      z-R2C of rho - mean                             (real -> S0)
 //   k_lines     y-FFT of S0
-//   k_xfwd      x-FFT, multiply by all weights, 12 inverse x-FFTs (S0 -> SF[12])
-//   k_lines     inverse y-FFT of the 12 channels
-//   k_zpoint    per z-line: 12 C2R, pointwise Phi/dPhi + local terms + partial
-//               sums, 11 R2C of the combined derivatives      (SF -> SB[11], in place)
+//   k_xfwd      x-FFT, multiply by all weights, inverse x-FFTs  (S0 -> SF[nf])
+//   k_lines     inverse y-FFT of the nf channels
+//   k_zpoint    per z-line: nf C2R, pointwise Phi/dPhi + local terms + partial
+//               sums, 11 R2C of the combined derivatives        (SF -> SF[11], in place)
 //   k_lines     y-FFT of the 11 channels
-//   k_xbwd      x-FFT, conj-weighted channel sum, inverse x-FFT (SB -> SO)
+//   k_xbwd      x-FFT, conj-weighted channel sum, inverse x-FFT (SF -> SO)
 //   k_lines     inverse y-FFT of SO
 //   k_zc2r      z-C2R, * dV, + local force                      (SO -> force)
 //   k_finalize  deterministic compensated sum of the block partials
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -32,6 +41,8 @@
 
 namespace cdft {
 
+constexpr int NT = 256;  // threads per block of every transform kernel
+
 // ---------------------------------------------------------------- helpers
 __device__ __forceinline__ void two_sum(double& s, double& c, double x) {
   // Neumaier compensated accumulation of x into (s, c)
@@ -40,10 +51,12 @@ __device__ __forceinline__ void two_sum(double& s, double& c, double x) {
   s = t;
 }
 
-template <typename R>
-__device__ __forceinline__ R kvec(int i, int n, R dk) {
-  return dk * R(i <= n / 2 ? i : i - n);
+template <typename D, typename T>
+__device__ __forceinline__ cx<D> cvt(cx<T> a) {
+  return {static_cast<D>(a.re), static_cast<D>(a.im)};
 }
+
+__device__ __forceinline__ double kvec(int i, int n, double dk) { return dk * static_cast<double>(i <= n / 2 ? i : i - n); }
 
 // Radial FMT functions w3(k) = 4 pi (sin kR - kR cos kR)/k^3, w2(k) = 4 pi R sin(kR)/k
 // (weights.hpp:174-191), each times 1/N.  The vector channel prefactor of
@@ -63,103 +76,67 @@ __device__ __forceinline__ void radial(double k, double R, double inv_n, double*
   *w2 = 4.0 * PI * R * s / k * inv_n;
 }
 
-__device__ __forceinline__ void radial(float k, float R, float inv_n, float* w3, float* w2) {
-  const float PI = 3.14159265358979323846f;
-  const float x = k * R;
-  if (x < 1.0f) {
-    // series of (sin x - x cos x)/x^3 and sin x / x: no cancellation at small kR in FP32
-    const float x2 = x * x;
-    const float p3 = 1.0f / 3.0f -
-                     x2 * (1.0f / 30.0f - x2 * (1.0f / 840.0f - x2 * (1.0f / 45360.0f - x2 * (1.0f / 3991680.0f))));
-    const float p2 = 1.0f - x2 * (1.0f / 6.0f - x2 * (1.0f / 120.0f - x2 * (1.0f / 5040.0f - x2 * (1.0f / 362880.0f))));
-    *w3 = 4.0f * PI * R * R * R * p3 * inv_n;
-    *w2 = 4.0f * PI * R * R * p2 * inv_n;
-    return;
-  }
-  float s, c;
-  sincosf(x, &s, &c);
-  *w3 = 4.0f * PI * (s - x * c) / (k * k * k) * inv_n;
-  *w2 = 4.0f * PI * R * s / k * inv_n;
-}
-
-// Weight of channel ch at k: ch 0 w3, 1 w2, 2..4 i k_a w3 (conj when `adj`),
-// 5..10 tensor xx,xy,xz,yy,yz,zz, MF (=11) the mean-field table value.
-template <typename R>
-__device__ __forceinline__ cx<R> channel_weight(int ch, bool adj, R kx, R ky, R kz, R w3, R w2, R mf, R invR) {
-  if (ch == 0) return {w3, R(0)};
-  if (ch == 1) return {w2, R(0)};
+// Weight of channel ch at k (FP64): ch 0 w3, 1 w2, 2..4 i k_a w3 (conj when
+// `adj`), 5..10 tensor xx,xy,xz,yy,yz,zz, 11 the mean-field table value.
+__device__ __forceinline__ cx<double> channel_weight(int ch, bool adj, double kx, double ky, double kz, double w3,
+                                                     double w2, double mf, double invR) {
+  if (ch == 0) return {w3, 0.0};
+  if (ch == 1) return {w2, 0.0};
   if (ch <= 4) {
-    const R ka = ch == 2 ? kx : (ch == 3 ? ky : kz);
-    return {R(0), adj ? -ka * w3 : ka * w3};
+    const double ka = ch == 2 ? kx : (ch == 3 ? ky : kz);
+    return {0.0, adj ? -ka * w3 : ka * w3};
   }
   if (ch <= 10) {
-    const R k2 = kx * kx + ky * ky + kz * kz;
-    const R ik2 = k2 > R(0) ? R(1) / k2 : R(0);
-    const R A = w2 - R(3) * w3 * invR;
-    const R B = w3 * invR;
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double ik2 = k2 > 0.0 ? 1.0 / k2 : 0.0;
+    const double A = w2 - 3.0 * w3 * invR;
+    const double B = w3 * invR;
     const int q = ch - 5;
-    const R ki = q < 3 ? kx : (q < 5 ? ky : kz);
-    const R kj = (q == 0) ? kx : (q == 1 || q == 3) ? ky : kz;
+    const double ki = q < 3 ? kx : (q < 5 ? ky : kz);
+    const double kj = (q == 0) ? kx : (q == 1 || q == 3) ? ky : kz;
     const bool diag = (q == 0 || q == 3 || q == 5);
-    return {A * ki * kj * ik2 + (diag ? B : R(0)), R(0)};
+    return {A * ki * kj * ik2 + (diag ? B : 0.0), 0.0};
   }
-  return {mf, R(0)};
+  return {mf, 0.0};
 }
 
-// ---------------------------------------------------------------- z-line kernels
-// Lines l = ix*Ny + iy of the real field are contiguous; a block owns `nlb`
-// consecutive lines (nlb even) and packs line pairs into one complex FFT.
-
-template <typename R>
-__global__ void k_zr2c(const double* __restrict__ in, cx<R>* __restrict__ out, Geom g, FftPlan pz,
-                       const cx<R>* __restrict__ twz, int nlb, AliasMap amap, const double* __restrict__ shift_sum) {
-  extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
-  const int nz = g.nz, nzh = g.nzh;
-  cx<R>* buf1 = buf0 + nlb * nzh;
-  const long l0 = static_cast<long>(blockIdx.x) * nlb;
-  const int ls = nz;
-  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
-  for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
-    const int ll = e / nz, z = e - ll * nz;
-    const long l = l0 + ll;
-    R val = R(0);
-    if (l < g.nxy) val = static_cast<R>(amap.apply(in[l * nz + z]) - shift);
-    R* dst = reinterpret_cast<R*>(&buf0[(ll >> 1) * ls + z]);
-    dst[ll & 1] = val;
-  }
-  __syncthreads();
-  cx<R>* res = fft_lines(buf0, buf1, nlb >> 1, ls, pz, -1, twz);
-  __syncthreads();
-  for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
-    const int kz = e / nlb, ll = e - kz * nlb;
-    const long l = l0 + ll;
-    if (l >= g.nxy) continue;
-    const cx<R>* line = res + (ll >> 1) * ls;
-    const cx<R> zk = line[kz];
-    const cx<R> zm = cconj(line[kz == 0 ? 0 : nz - kz]);
-    cx<R> v;
-    if ((ll & 1) == 0) v = {R(0.5) * (zk.re + zm.re), R(0.5) * (zk.im + zm.im)};
-    else v = {R(0.5) * (zk.im - zm.im), R(-0.5) * (zk.re - zm.re)};
-    out[static_cast<long>(kz) * g.nxy + l] = v;
+// v * w for an FP64 weight: in FP32 the weight is applied as the pair
+// (FP32 value, FP32 residual), like the FFT constants.
+template <typename C>
+__device__ __forceinline__ cx<C> wmul(cx<C> v, cx<double> w) {
+  if constexpr (sizeof(C) == 4) {
+    const cx<float> hi = {static_cast<float>(w.re), static_cast<float>(w.im)};
+    const cx<float> lo = {static_cast<float>(w.re - static_cast<double>(hi.re)),
+                          static_cast<float>(w.im - static_cast<double>(hi.im))};
+    return cadd(cmul(v, hi), cmul(v, lo));
+  } else {
+    return cmul(v, w);
   }
 }
 
-// Build the packed complex z-spectrum of line pairs from a staged half
-// spectrum (stage[ll*nzh + kz]) with Hermitian projection of kz=0 and Nyquist.
-template <typename R>
-__device__ __forceinline__ void build_c2r_lines(const cx<R>* __restrict__ stage, cx<R>* __restrict__ dst, int nlb,
+// R2C unpack of a packed line pair: line spectrum Z of (a + i b); returns A[k] or B[k].
+template <typename C>
+__device__ __forceinline__ cx<C> unpack_pair(const cx<C>* line, int kz, int nz, bool odd) {
+  const cx<C> zk = line[kz];
+  const cx<C> zm = cconj(line[kz == 0 ? 0 : nz - kz]);
+  if (!odd) return {C(0.5) * (zk.re + zm.re), C(0.5) * (zk.im + zm.im)};
+  return {C(0.5) * (zk.im - zm.im), C(-0.5) * (zk.re - zm.re)};
+}
+
+// C2R pack of line pairs from staged half spectra (stage[ll*nzh + kz]) with
+// Hermitian projection of the kz = 0 and Nyquist bins; writes Z[k] for all k.
+template <typename C, typename T>
+__device__ __forceinline__ void build_c2r_lines(const cx<T>* __restrict__ stage, cx<C>* __restrict__ dst, int npairs,
                                                 int nz, int nzh) {
-  const int np = nlb >> 1;
-  for (int e = threadIdx.x; e < np * nz; e += blockDim.x) {
+  for (int e = threadIdx.x; e < npairs * nz; e += blockDim.x) {
     const int p = e / nz, k = e - p * nz;
     const bool mirror = k >= nzh;
     const int kk = mirror ? nz - k : k;
-    cx<R> A = stage[(2 * p) * nzh + kk];
-    cx<R> B = stage[(2 * p + 1) * nzh + kk];
+    cx<C> A = cvt<C>(stage[(2 * p) * nzh + kk]);
+    cx<C> B = cvt<C>(stage[(2 * p + 1) * nzh + kk]);
     if (kk == 0 || (2 * kk == nz)) {
-      A.im = R(0);
-      B.im = R(0);
+      A.im = C(0);
+      B.im = C(0);
     }
     if (mirror) {
       A.im = -A.im;
@@ -169,35 +146,67 @@ __device__ __forceinline__ void build_c2r_lines(const cx<R>* __restrict__ stage,
   }
 }
 
-// Generic z-C2R of `nch` channels: out[c*N + p] = scale * c2r(S_c)[p] + (add ? add[p] : 0)
-template <typename R>
-__global__ void k_zc2r(const cx<R>* __restrict__ S, double* __restrict__ out, const double* __restrict__ add,
-                       double scale, Geom g, FftPlan pz, const cx<R>* __restrict__ twz, int nlb,
-                       const double* __restrict__ shift_sum, DcTable dc) {
+// ---------------------------------------------------------------- z-line kernels
+// Lines l = ix*Ny + iy of the real field are contiguous; a block owns `nlb`
+// consecutive lines (nlb even) and packs line pairs into one complex FFT.
+
+template <typename S, typename C>
+__global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S>* __restrict__ out, Geom g,
+                                             FftPlan pz, const cx<C>* __restrict__ twz, int nlb, AliasMap amap,
+                                             const double* __restrict__ shift_sum) {
   extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nz = g.nz, nzh = g.nzh;
-  cx<R>* buf1 = buf0 + nlb * nzh;
+  const long l0 = static_cast<long>(blockIdx.x) * nlb;
+  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+  for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
+    const int ll = e / nz, z = e - ll * nz;
+    const long l = l0 + ll;
+    C val = C(0);
+    if (l < g.nxy) val = static_cast<C>(amap.apply(in[l * nz + z]) - shift);
+    C* dst = reinterpret_cast<C*>(&buf[(ll >> 1) * nz + z]);
+    dst[ll & 1] = val;
+  }
+  __syncthreads();
+  fft_inplace<C, 8>(buf, nlb >> 1, nz, pz, -1, twz);
+  for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
+    const int kz = e / nlb, ll = e - kz * nlb;
+    const long l = l0 + ll;
+    if (l >= g.nxy) continue;
+    out[static_cast<long>(kz) * g.nxy + l] = cvt<S>(unpack_pair(buf + (ll >> 1) * nz, kz, nz, ll & 1));
+  }
+}
+
+// Generic z-C2R of `nch` channels: out[c*N + p] = scale * c2r(S_c)[p] + cst_c + (add ? add[p] : 0)
+// with cst_c = dc.v[c] * shift_sum[0] / N (the mean-density split) when shift_sum is given.
+template <typename S, typename C>
+__global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, double* __restrict__ out,
+                                             const double* __restrict__ add, double scale, Geom g, FftPlan pz,
+                                             const cx<C>* __restrict__ twz, int nlb,
+                                             const double* __restrict__ shift_sum, DcTable dc) {
+  extern __shared__ unsigned char smem_raw[];
+  const int nz = g.nz, nzh = g.nzh;
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
+  cx<S>* stage = reinterpret_cast<cx<S>*>(buf + (nlb >> 1) * nz);
   const int ch = blockIdx.y;
-  const cx<R>* Sc = S + static_cast<long>(ch) * g.M;
+  const cx<S>* Sc = Sp + static_cast<long>(ch) * g.M;
   const long l0 = static_cast<long>(blockIdx.x) * nlb;
   for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
     const int kz = e / nlb, ll = e - kz * nlb;
     const long l = l0 + ll;
-    buf1[ll * nzh + kz] = (l < g.nxy) ? Sc[static_cast<long>(kz) * g.nxy + l] : cx<R>{R(0), R(0)};
+    stage[ll * nzh + kz] = (l < g.nxy) ? Sc[static_cast<long>(kz) * g.nxy + l] : cx<S>{S(0), S(0)};
   }
   __syncthreads();
-  build_c2r_lines(buf1, buf0, nlb, nz, nzh);
+  build_c2r_lines(stage, buf, nlb >> 1, nz, nzh);
   __syncthreads();
-  cx<R>* res = fft_lines(buf0, buf1, nlb >> 1, nz, pz, +1, twz);
-  __syncthreads();
+  fft_inplace<C, 8>(buf, nlb >> 1, nz, pz, +1, twz);
   double* oc = out + static_cast<long>(ch) * g.N;
   const double cst = shift_sum ? dc.v[ch] * shift_sum[0] * g.inv_n : 0.0;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
     const int ll = e / nz, z = e - ll * nz;
     const long l = l0 + ll;
     if (l >= g.nxy) continue;
-    const cx<R> v = res[(ll >> 1) * nz + z];
+    const cx<C> v = buf[(ll >> 1) * nz + z];
     const long p = l * nz + z;
     const double val = static_cast<double>((ll & 1) ? v.im : v.re) * scale + cst;
     oc[p] = add ? val + add[p] : val;
@@ -205,9 +214,9 @@ __global__ void k_zc2r(const cx<R>* __restrict__ S, double* __restrict__ out, co
 }
 
 // FP64 block sums of amap(in): partials[b*8 + 0]; slots 1..7 zeroed.
-__global__ void __launch_bounds__(256) k_sum(const double* __restrict__ in, AliasMap amap, long n,
-                                             double* __restrict__ partials) {
-  __shared__ double sh[256];
+__global__ void __launch_bounds__(NT) k_sum(const double* __restrict__ in, AliasMap amap, long n,
+                                            double* __restrict__ partials) {
+  __shared__ double sh[NT];
   double s = 0.0, c = 0.0;
   const long stride = static_cast<long>(gridDim.x) * blockDim.x;
   for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -222,161 +231,168 @@ __global__ void __launch_bounds__(256) k_sum(const double* __restrict__ in, Alia
 }
 
 // ---------------------------------------------------------------- y lines (contiguous)
-template <typename R>
-__global__ void k_lines(cx<R>* __restrict__ S, long nlines, FftPlan p, const cx<R>* __restrict__ tw, int nlb, int sg) {
+template <typename S, typename C>
+__global__ void __launch_bounds__(NT) k_lines(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
+                                              const cx<C>* __restrict__ tw, int nlb, int sg) {
   extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int L = p.L;
-  cx<R>* buf1 = buf0 + nlb * L;
   const long line0 = static_cast<long>(blockIdx.x) * nlb;
   const int nl = static_cast<int>(min(static_cast<long>(nlb), nlines - line0));
-  cx<R>* base = S + line0 * L;
-  for (int e = threadIdx.x; e < nl * L; e += blockDim.x) buf0[e] = base[e];
+  cx<S>* base = Sp + line0 * L;
+  for (int e = threadIdx.x; e < nl * L; e += blockDim.x) buf[e] = cvt<C>(base[e]);
   __syncthreads();
-  cx<R>* res = fft_lines(buf0, buf1, nl, L, p, sg, tw);
-  __syncthreads();
-  for (int e = threadIdx.x; e < nl * L; e += blockDim.x) base[e] = res[e];
+  fft_inplace<C, 8>(buf, nl, L, p, sg, tw);
+  for (int e = threadIdx.x; e < nl * L; e += blockDim.x) base[e] = cvt<S>(buf[e]);
 }
 
 // ---------------------------------------------------------------- x lines (strided in a kz plane)
 // Block (y-tile, kz, channel): TY consecutive y columns, all x.  LDS line stride nx+1.
-template <typename R>
-__global__ void k_xlines(cx<R>* __restrict__ S, Geom g, FftPlan px, const cx<R>* __restrict__ twx, int ty, int sg) {
+template <typename S, typename C, int MAXE>
+__global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, FftPlan px,
+                                               const cx<C>* __restrict__ twx, int ty, int sg) {
   extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
-  cx<R>* buf1 = buf0 + ty * ls;
   const int y0 = blockIdx.x * ty;
   const int kz = blockIdx.y;
-  cx<R>* plane = S + static_cast<long>(blockIdx.z) * g.M + static_cast<long>(kz) * nx * ny;
+  cx<S>* plane = Sp + static_cast<long>(blockIdx.z) * g.M + static_cast<long>(kz) * nx * ny;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-    buf0[yl * ls + x] = (y < ny) ? plane[x * ny + y] : cx<R>{R(0), R(0)};
+    buf[yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
   }
   __syncthreads();
-  cx<R>* res = fft_lines(buf0, buf1, ty, ls, px, sg, twx);
-  __syncthreads();
+  fft_inplace<C, MAXE>(buf, ty, ls, px, sg, twx);
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-    if (y < ny) plane[x * ny + y] = res[yl * ls + x];
+    if (y < ny) plane[x * ny + y] = cvt<S>(buf[yl * ls + x]);
   }
 }
 
 // Fused forward k-space stage: x-FFT of rho_hat, multiply by every channel
-// weight (analytic FMT weights, mean-field table), inverse x-FFT per channel.
-template <typename R, int EPT>
-__global__ void k_xfwd(const cx<R>* __restrict__ S0, cx<R>* __restrict__ SF, const R* __restrict__ mf, Geom g,
-                       FftPlan px, const cx<R>* __restrict__ twx, int ty, ChannelSet cs) {
+// weight (analytic FMT weights, mean-field table), inverse x-FFTs, G channels
+// per batched transform.  EPT >= ty*nx/NT elements owned per thread.
+template <typename S, typename C, int EPT, int G>
+__global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
+                                             const S* __restrict__ mf, Geom g, FftPlan px,
+                                             const cx<C>* __restrict__ twx, int ty, ChannelSet cs) {
   extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
-  cx<R>* buf1 = buf0 + ty * ls;
+  const int slab = ty * ls;
   const int y0 = blockIdx.x * ty;
   const int kz = blockIdx.y;
   const long plane_off = static_cast<long>(kz) * nx * ny;
-  const cx<R>* plane = S0 + plane_off;
+  const cx<S>* plane = S0 + plane_off;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-    buf0[yl * ls + x] = (y < ny) ? plane[x * ny + y] : cx<R>{R(0), R(0)};
+    buf[yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
   }
   __syncthreads();
-  const cx<R>* res = fft_lines(buf0, buf1, ty, ls, px, -1, twx);
-  __syncthreads();
-  cx<R> rh[EPT];
-  R w3[EPT], w2[EPT], wm[EPT];
-  const R kzv = R(g.dkz) * R(kz);
-  const R Rr = R(g.R);
+  fft_inplace<C, EPT>(buf, ty, ls, px, -1, twx);
+  cx<C> rh[EPT];
+  double w3[EPT], w2[EPT], wm[EPT];
+  const double kzv = g.dkz * kz;
 #pragma unroll
   for (int m = 0; m < EPT; ++m) {
     const int e = threadIdx.x + m * blockDim.x;
-    rh[m] = {R(0), R(0)};
-    w3[m] = w2[m] = wm[m] = R(0);
+    rh[m] = {C(0), C(0)};
+    w3[m] = w2[m] = wm[m] = 0.0;
     if (e < ty * nx) {
       const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-      rh[m] = res[yl * ls + x];
+      rh[m] = buf[yl * ls + x];
       if (y < ny) {
-        const R kx = kvec<R>(x, nx, R(g.dkx)), ky = kvec<R>(y, ny, R(g.dky));
-        const R k = sqrt(kx * kx + ky * ky + kzv * kzv);
-        if (cs.hs) radial(k, Rr, R(g.inv_n), &w3[m], &w2[m]);
-        if (cs.mf) wm[m] = mf[plane_off + x * ny + y];
+        const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
+        if (cs.hs) radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3[m], &w2[m]);
+        // mean-field table: FP64 values, or FP32 (value, residual) planes
+        if (cs.mf)
+          wm[m] = sizeof(S) == 8 ? static_cast<double>(mf[plane_off + x * ny + y])
+                                 : static_cast<double>(mf[plane_off + x * ny + y]) +
+                                       static_cast<double>(mf[g.M + plane_off + x * ny + y]);
       }
     }
   }
   __syncthreads();
-  const R invR = R(1) / Rr;
-  for (int c = 0; c < cs.nf; ++c) {
-    const int ch = cs.hs ? c : 11;
+  const double invR = 1.0 / g.R;
+  for (int c0 = 0; c0 < cs.nf; c0 += G) {
+    const int ng = min(G, cs.nf - c0);
+    for (int gi = 0; gi < ng; ++gi) {
+      const int ch = cs.hs ? c0 + gi : 11;
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-      const int e = threadIdx.x + m * blockDim.x;
-      if (e < ty * nx) {
-        const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-        const R kx = kvec<R>(x, nx, R(g.dkx)), ky = kvec<R>(min(y, ny - 1), ny, R(g.dky));
-        const cx<R> w = channel_weight<R>(ch, false, kx, ky, kzv, w3[m], w2[m], wm[m], invR);
-        buf0[yl * ls + x] = cmul(w, rh[m]);
+      for (int m = 0; m < EPT; ++m) {
+        const int e = threadIdx.x + m * blockDim.x;
+        if (e < ty * nx) {
+          const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+          const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
+          const cx<double> w = channel_weight(ch, false, kx, ky, kzv, w3[m], w2[m], wm[m], invR);
+          buf[gi * slab + yl * ls + x] = wmul(rh[m], w);
+        }
       }
     }
     __syncthreads();
-    const cx<R>* r = fft_lines(buf0, buf1, ty, ls, px, +1, twx);
-    __syncthreads();
-    cx<R>* outp = SF + static_cast<long>(c) * g.M + plane_off;
-    for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-      const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-      if (y < ny) outp[x * ny + y] = r[yl * ls + x];
+    fft_inplace<C, EPT * G>(buf, ng * ty, ls, px, +1, twx);
+    for (int gi = 0; gi < ng; ++gi) {
+      cx<S>* outp = SF + static_cast<long>(c0 + gi) * g.M + plane_off;
+      for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
+        const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+        if (y < ny) outp[x * ny + y] = cvt<S>(buf[gi * slab + yl * ls + x]);
+      }
     }
     __syncthreads();
   }
 }
 
-// Fused backward k-space stage: x-FFT of every derivative channel,
-// sum of weight * channel (conj weight on the odd vector channels,
-// hard_sphere.hpp:118-134), inverse x-FFT of the sum.
-template <typename R, int EPT>
-__global__ void k_xbwd(const cx<R>* __restrict__ SB, cx<R>* __restrict__ SO, Geom g, FftPlan px,
-                       const cx<R>* __restrict__ twx, int ty, int nb) {
+// Fused backward k-space stage: x-FFT of every derivative channel (G per
+// batched transform), sum of weight * channel with conj weights on the odd
+// vector channels (hard_sphere.hpp:118-134), inverse x-FFT of the sum.
+template <typename S, typename C, int EPT, int G>
+__global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO, Geom g,
+                                             FftPlan px, const cx<C>* __restrict__ twx, int ty, int nb) {
   extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
-  cx<R>* buf1 = buf0 + ty * ls;
+  const int slab = ty * ls;
   const int y0 = blockIdx.x * ty;
   const int kz = blockIdx.y;
   const long plane_off = static_cast<long>(kz) * nx * ny;
-  const R kzv = R(g.dkz) * R(kz);
-  const R Rr = R(g.R);
-  const R invR = R(1) / Rr;
-  cx<R> acc[EPT];
-  R w3[EPT], w2[EPT];
+  const double kzv = g.dkz * kz;
+  const double invR = 1.0 / g.R;
+  cx<C> acc[EPT];
+  double w3[EPT], w2[EPT];
 #pragma unroll
   for (int m = 0; m < EPT; ++m) {
-    acc[m] = {R(0), R(0)};
-    w3[m] = w2[m] = R(0);
+    acc[m] = {C(0), C(0)};
+    w3[m] = w2[m] = 0.0;
     const int e = threadIdx.x + m * blockDim.x;
     if (e < ty * nx) {
       const int x = e / ty, yl = e - x * ty, y = y0 + yl;
       if (y < ny) {
-        const R kx = kvec<R>(x, nx, R(g.dkx)), ky = kvec<R>(y, ny, R(g.dky));
-        radial(sqrt(kx * kx + ky * ky + kzv * kzv), Rr, R(g.inv_n), &w3[m], &w2[m]);
+        const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
+        radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3[m], &w2[m]);
       }
     }
   }
-  for (int c = 0; c < nb; ++c) {
-    const cx<R>* plane = SB + static_cast<long>(c) * g.M + plane_off;
-    for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-      const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-      buf0[yl * ls + x] = (y < ny) ? plane[x * ny + y] : cx<R>{R(0), R(0)};
+  for (int c0 = 0; c0 < nb; c0 += G) {
+    const int ng = min(G, nb - c0);
+    for (int gi = 0; gi < ng; ++gi) {
+      const cx<S>* plane = SB + static_cast<long>(c0 + gi) * g.M + plane_off;
+      for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
+        const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+        buf[gi * slab + yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
+      }
     }
     __syncthreads();
-    const cx<R>* r = fft_lines(buf0, buf1, ty, ls, px, -1, twx);
-    __syncthreads();
+    fft_inplace<C, EPT * G>(buf, ng * ty, ls, px, -1, twx);
+    for (int gi = 0; gi < ng; ++gi) {
 #pragma unroll
-    for (int m = 0; m < EPT; ++m) {
-      const int e = threadIdx.x + m * blockDim.x;
-      if (e < ty * nx) {
-        const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-        const R kx = kvec<R>(x, nx, R(g.dkx)), ky = kvec<R>(min(y, ny - 1), ny, R(g.dky));
-        const cx<R> w = channel_weight<R>(c, true, kx, ky, kzv, w3[m], w2[m], R(0), invR);
-        const cx<R> t = cmul(w, r[yl * ls + x]);
-        acc[m] = cadd(acc[m], t);
+      for (int m = 0; m < EPT; ++m) {
+        const int e = threadIdx.x + m * blockDim.x;
+        if (e < ty * nx) {
+          const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+          const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
+          const cx<double> w = channel_weight(c0 + gi, true, kx, ky, kzv, w3[m], w2[m], 0.0, invR);
+          acc[m] = cadd(acc[m], wmul(buf[gi * slab + yl * ls + x], w));
+        }
       }
     }
     __syncthreads();
@@ -386,158 +402,223 @@ __global__ void k_xbwd(const cx<R>* __restrict__ SB, cx<R>* __restrict__ SO, Geo
     const int e = threadIdx.x + m * blockDim.x;
     if (e < ty * nx) {
       const int x = e / ty, yl = e - x * ty;
-      buf0[yl * ls + x] = acc[m];
+      buf[yl * ls + x] = acc[m];
     }
   }
   __syncthreads();
-  const cx<R>* r = fft_lines(buf0, buf1, ty, ls, px, +1, twx);
-  __syncthreads();
-  cx<R>* outp = SO + plane_off;
+  fft_inplace<C, EPT>(buf, ty, ls, px, +1, twx);
+  cx<S>* outp = SO + plane_off;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = e / ty, yl = e - x * ty, y = y0 + yl;
-    if (y < ny) outp[x * ny + y] = r[yl * ls + x];
+    if (y < ny) outp[x * ny + y] = cvt<S>(buf[yl * ls + x]);
   }
 }
 
 // ---------------------------------------------------------------- fused z-line pointwise kernel
-// Per block of nlb consecutive z-lines: C2R of every forward channel, the
-// pointwise FMT Phi/dPhi (FP64), the ideal-gas, external-field and
-// mean-field local terms, compensated partial sums, and R2C of the 11
-// combined derivative channels (written in place over SF's first channels).
-template <typename R, int PPT>
-__global__ void __launch_bounds__(512) k_zpoint(cx<R>* SF, const double* __restrict__ rho_in,
-                                                const double* __restrict__ vext, double* __restrict__ local_force,
-                                                double* __restrict__ partials, Geom g, FftPlan pz,
-                                                const cx<R>* __restrict__ twz, int nlb, ChannelSet cs, FmtParams fp,
-                                                double mu, AliasMap amap, const double* __restrict__ shift_sum,
-                                                DcTable dc) {
+// Per block of nlb consecutive z-lines (PPT * NT points): C2R of every forward
+// channel (arithmetic CF, ZP_G channels per batched transform), the pointwise
+// FMT Phi/dPhi (FP64), the ideal-gas, external-field and mean-field local
+// terms, compensated partial sums, and R2C of the 11 combined derivative
+// channels (arithmetic CB) written in place over SF's first channels.  NF
+// forward channels (12: FMT + mean field, 11: FMT, 1: mean field, 0) and NB
+// backward channels (11 or 0) are compile-time so the per-point values stay
+// in registers.
+
+// Wave-then-block sum of NV doubles; thread 0 of the block gets the totals in out[].
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double* out) {
+  __shared__ double wsum[16][NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    double x = v[i];
+    for (int off = warpSize / 2; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    v[i] = x;
+  }
+  const int wave = threadIdx.x / warpSize, lane = threadIdx.x % warpSize;
+  if (lane == 0)
+#pragma unroll
+    for (int i = 0; i < NV; ++i) wsum[wave][i] = v[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int nw = (blockDim.x + warpSize - 1) / warpSize;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += wsum[w][i];
+      out[i] = t;
+    }
+  }
+}
+
+constexpr int ZP_G = 4;  // channels per batched z transform in k_zpoint
+
+template <typename S, typename CF, typename CB, int PPT, int NF, int NB>
+__global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
+                                               const double* __restrict__ vext, double* __restrict__ local_force,
+                                               double* __restrict__ partials, Geom g, FftPlan pz,
+                                               const cx<CF>* __restrict__ twf, const cx<CB>* __restrict__ twb,
+                                               int nlb, FmtParams fp, double mu, AliasMap amap,
+                                               const double* __restrict__ shift_sum, DcTable dc) {
+  constexpr bool HS = NF >= 11;
+  constexpr bool MF = (NF == 12 || NF == 1);
+  constexpr int MFC = HS ? 11 : 0;
+  constexpr int NFA = NF > 0 ? NF : 1;
   extern __shared__ unsigned char smem_raw[];
-  cx<R>* buf0 = reinterpret_cast<cx<R>*>(smem_raw);
   const int nz = g.nz, nzh = g.nzh;
-  cx<R>* buf1 = buf0 + nlb * nzh;
+  const int np = nlb >> 1;                 // packed line pairs per channel
+  const int zs = np * nz;                  // complex elements per channel in the transform buffer
+  constexpr size_t CMAX = sizeof(cx<CF>) > sizeof(cx<CB>) ? sizeof(cx<CF>) : sizeof(cx<CB>);
+  cx<CF>* fbuf = reinterpret_cast<cx<CF>*>(smem_raw);
+  cx<CB>* bbuf = reinterpret_cast<cx<CB>*>(smem_raw);
+  cx<S>* stage = reinterpret_cast<cx<S>*>(smem_raw + CMAX * ZP_G * zs);
   const long l0 = static_cast<long>(blockIdx.x) * nlb;
   const int npts = nlb * nz;
-  R wv[12][PPT];
+  const int hs = nlb * nzh;                // staged half-spectrum elements per channel
+  S wv[NFA][PPT];
 
-  for (int c = 0; c < cs.nf; ++c) {
-    const cx<R>* Sc = SF + static_cast<long>(c) * g.M;
-    for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
-      const int kz = e / nlb, ll = e - kz * nlb;
-      const long l = l0 + ll;
-      buf1[ll * nzh + kz] = (l < g.nxy) ? Sc[static_cast<long>(kz) * g.nxy + l] : cx<R>{R(0), R(0)};
+#pragma unroll
+  for (int c0 = 0; c0 < NF; c0 += ZP_G) {
+    const int ng = min(ZP_G, NF - c0);
+    for (int gi = 0; gi < ng; ++gi) {
+      const cx<S>* Sc = SF + static_cast<long>(c0 + gi) * g.M;
+      for (int e = threadIdx.x; e < hs; e += blockDim.x) {
+        const int kz = e / nlb, ll = e - kz * nlb;
+        const long l = l0 + ll;
+        stage[gi * hs + ll * nzh + kz] = (l < g.nxy) ? Sc[static_cast<long>(kz) * g.nxy + l] : cx<S>{S(0), S(0)};
+      }
     }
     __syncthreads();
-    build_c2r_lines(buf1, buf0, nlb, nz, nzh);
+    for (int gi = 0; gi < ng; ++gi) build_c2r_lines(stage + gi * hs, fbuf + gi * zs, np, nz, nzh);
     __syncthreads();
-    const cx<R>* res = fft_lines(buf0, buf1, nlb >> 1, nz, pz, +1, twz);
-    __syncthreads();
+    fft_inplace<CF, ZP_G * PPT / 2>(fbuf, ng * np, nz, pz, +1, twf);
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const int q = threadIdx.x + m * blockDim.x;
-      R val = R(0);
-      if (q < npts) {
-        const int ll = q / nz, z = q - ll * nz;
-        const cx<R> v = res[(ll >> 1) * nz + z];
-        val = (ll & 1) ? v.im : v.re;
+    for (int gi = 0; gi < ZP_G; ++gi) {
+      const int c = c0 + gi;
+      if (c < NF) {
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+          const int q = threadIdx.x + m * blockDim.x;
+          S val = S(0);
+          if (q < npts) {
+            const int ll = q / nz, z = q - ll * nz;
+            const cx<CF> v = fbuf[gi * zs + (ll >> 1) * nz + z];
+            val = static_cast<S>((ll & 1) ? v.im : v.re);
+          }
+          wv[c][m] = val;
+        }
       }
-      wv[c][m] = val;
     }
     __syncthreads();
   }
 
-  // pointwise; the combined derivatives overwrite wv[0..10] once consumed
-  double s_id = 0, c_id = 0, s_ext = 0, c_ext = 0, s_hs = 0, c_hs = 0, s_mf = 0, c_mf = 0, s_n = 0, c_n = 0;
-  R (&gv)[12][PPT] = wv;
+  // pointwise; one point at a time (rolled loop) so that only one point's FP64
+  // temporaries are live; channel values move in and out with unrolled selects.
+  double acc[5] = {0, 0, 0, 0, 0}, cmp[5] = {0, 0, 0, 0, 0};  // ideal, external, hs, mf, mass
   const double dv = g.dv;
-  const int mfc = cs.hs ? 11 : 0;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int m = 0; m < PPT; ++m) {
     const int q = threadIdx.x + m * blockDim.x;
     const long l = l0 + q / nz;
+    S cur[NFA];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      S t = wv[i][0];
+#pragma unroll
+      for (int j = 1; j < PPT; ++j) t = (m == j) ? wv[i][j] : t;
+      cur[i] = t;
+    }
+    S out[11];
+#pragma unroll
+    for (int i = 0; i < 11; ++i) out[i] = S(0);
     if (q < npts && l < g.nxy) {
       const long p = l0 * nz + q;
       const double rho = amap.apply(rho_in[p]);
       const double lr = log(fmax(rho, 1e-300));
       double lf = (lr - mu) * dv;                // ideal_gas.hpp:312-318
-      two_sum(s_id, c_id, rho * (lr - 1.0));
-      two_sum(s_n, c_n, rho);
+      two_sum(acc[0], cmp[0], rho * (lr - 1.0));
+      two_sum(acc[4], cmp[4], rho);
       if (vext) {                                 // external_field.hpp:16-31
         const double V = vext[p];
         lf += V * dv;
-        two_sum(s_ext, c_ext, rho * V);
+        two_sum(acc[1], cmp[1], rho * V);
       }
-      if (cs.mf) {                                // mean_field.hpp:259-266 (self pair)
-        const double conv = static_cast<double>(wv[mfc][m]) + dc.v[mfc] * shift;
+      if constexpr (MF) {                         // mean_field.hpp:259-266 (self pair)
+        const double conv = static_cast<double>(cur[MFC]) + dc.v[MFC] * shift;
         lf += conv * dv;
-        two_sum(s_mf, c_mf, 0.5 * rho * conv);
+        two_sum(acc[3], cmp[3], 0.5 * rho * conv);
       }
-      if (cs.hs) {
+      if constexpr (HS) {
         double w[11], gg[11];
 #pragma unroll
-        for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(wv[i][m]) + dc.v[i] * shift;
+        for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(cur[i]) + dc.v[i] * shift;
         const double phi = fmt_phi_grad(fp, w, gg);
-        two_sum(s_hs, c_hs, phi);
+        two_sum(acc[2], cmp[2], phi);
 #pragma unroll
-        for (int i = 0; i < 11; ++i) gv[i][m] = static_cast<R>(gg[i]);
+        for (int i = 0; i < 11; ++i) out[i] = static_cast<S>(gg[i]);
       }
       local_force[p] = lf;
-    } else {
+    }
+    if constexpr (HS) {
 #pragma unroll
-      for (int i = 0; i < 11; ++i) gv[i][m] = R(0);
+      for (int i = 0; i < 11; ++i)
+#pragma unroll
+        for (int j = 0; j < PPT; ++j) wv[i][j] = (m == j) ? out[i] : wv[i][j];
     }
   }
-
-  // block reduction of the five compensated sums
-  __shared__ double red[5][512];
   {
-    const double v5[5] = {s_id + c_id, s_ext + c_ext, s_hs + c_hs, s_mf + c_mf, s_n + c_n};
-    for (int i = 0; i < 5; ++i) red[i][threadIdx.x] = v5[i];
+    double tot[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) tot[i] = acc[i] + cmp[i];
+    __shared__ double out5[5];
+    block_reduce<5>(tot, out5);
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-      if (threadIdx.x < s)
-        for (int i = 0; i < 5; ++i) red[i][threadIdx.x] += red[i][threadIdx.x + s];
-      __syncthreads();
-    }
     if (threadIdx.x < 8)
-      partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x < 5 ? red[threadIdx.x][0] * dv : 0.0;
+      partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x < 5 ? out5[threadIdx.x] * dv : 0.0;
   }
 
-  // forward z R2C of the combined derivatives
-  for (int c = 0; c < cs.nb; ++c) {
+  // forward z R2C of the combined derivatives, ZP_G channels per batched transform
 #pragma unroll
-    for (int m = 0; m < PPT; ++m) {
-      const int q = threadIdx.x + m * blockDim.x;
-      if (q < npts) {
-        const int ll = q / nz, z = q - ll * nz;
-        R* dst = reinterpret_cast<R*>(&buf0[(ll >> 1) * nz + z]);
-        dst[ll & 1] = gv[c][m];
+  for (int c0 = 0; c0 < NB; c0 += ZP_G) {
+    const int ng = min(ZP_G, NB - c0);
+#pragma unroll
+    for (int gi = 0; gi < ZP_G; ++gi) {
+      const int c = c0 + gi;
+      if (c < NB) {
+#pragma unroll
+        for (int m = 0; m < PPT; ++m) {
+          const int q = threadIdx.x + m * blockDim.x;
+          if (q < npts) {
+            const int ll = q / nz, z = q - ll * nz;
+            CB* dst = reinterpret_cast<CB*>(&bbuf[gi * zs + (ll >> 1) * nz + z]);
+            dst[ll & 1] = static_cast<CB>(wv[c][m]);
+          }
+        }
       }
     }
     __syncthreads();
-    const cx<R>* res = fft_lines(buf0, buf1, nlb >> 1, nz, pz, -1, twz);
-    __syncthreads();
-    cx<R>* out = SF + static_cast<long>(c) * g.M;
-    for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
-      const int kz = e / nlb, ll = e - kz * nlb;
-      const long l = l0 + ll;
-      if (l >= g.nxy) continue;
-      const cx<R>* line = res + (ll >> 1) * nz;
-      const cx<R> zk = line[kz];
-      const cx<R> zm = cconj(line[kz == 0 ? 0 : nz - kz]);
-      cx<R> v;
-      if ((ll & 1) == 0) v = {R(0.5) * (zk.re + zm.re), R(0.5) * (zk.im + zm.im)};
-      else v = {R(0.5) * (zk.im - zm.im), R(-0.5) * (zk.re - zm.re)};
-      out[static_cast<long>(kz) * g.nxy + l] = v;
+    fft_inplace<CB, ZP_G * PPT / 2>(bbuf, ng * np, nz, pz, -1, twb);
+    for (int gi = 0; gi < ng; ++gi) {
+      cx<S>* outc = SF + static_cast<long>(c0 + gi) * g.M;
+      for (int e = threadIdx.x; e < hs; e += blockDim.x) {
+        const int kz = e / nlb, ll = e - kz * nlb;
+        const long l = l0 + ll;
+        if (l >= g.nxy) continue;
+        outc[static_cast<long>(kz) * g.nxy + l] =
+            cvt<S>(unpack_pair(bbuf + gi * zs + (ll >> 1) * nz, kz, nz, ll & 1));
+      }
     }
     __syncthreads();
   }
 }
 
-// Deterministic compensated sum of per-block partials (nblk x 8 doubles) into res[0..7].
-__global__ void k_finalize(const double* __restrict__ partials, int nblk, double* __restrict__ res) {
-  __shared__ double sh[8][256];
-  __shared__ double shc[8][256];
+// Deterministic compensated sum of per-block partials (nblk x 8 doubles) into res[0..7]:
+// strided Neumaier sums per thread, then a fixed-shape tree of (sum, compensation) pairs.
+__global__ void __launch_bounds__(NT) k_finalize(const double* __restrict__ partials, int nblk,
+                                                 double* __restrict__ res) {
+  __shared__ double sh[8][NT];
+  __shared__ double shc[8][NT];
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     double s = 0, c = 0;
     for (int b = threadIdx.x; b < nblk; b += blockDim.x) two_sum(s, c, partials[static_cast<long>(b) * 8 + i]);
@@ -545,43 +626,51 @@ __global__ void k_finalize(const double* __restrict__ partials, int nblk, double
     shc[i][threadIdx.x] = c;
   }
   __syncthreads();
-  if (threadIdx.x < 8) {
-    const int i = threadIdx.x;
-    double s = 0, c = 0;
-    for (int t = 0; t < static_cast<int>(blockDim.x); ++t) {
-      two_sum(s, c, sh[i][t]);
-      c += shc[i][t];
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        double s = sh[i][threadIdx.x], c = shc[i][threadIdx.x] + shc[i][threadIdx.x + w];
+        two_sum(s, c, sh[i][threadIdx.x + w]);
+        sh[i][threadIdx.x] = s;
+        shc[i][threadIdx.x] = c;
+      }
     }
-    res[i] = s + c;
+    __syncthreads();
   }
+  if (threadIdx.x < 8) res[threadIdx.x] = sh[threadIdx.x][0] + shc[threadIdx.x][0];
 }
 
 // ---------------------------------------------------------------- launchers
-template <typename R>
+// z-lines per block of the plain z kernels: up to 2048 complex elements (two
+// real lines per element) per transform buffer.
+template <typename C>
 static int zlines_per_block(int nz) {
-  int nlb = 2048 / nz;
+  int nlb = (sizeof(C) == 4 ? 4096 : 2048) / nz;
   if (nlb < 2) nlb = 2;
   if (nlb & 1) --nlb;
-  if (nlb > 32) nlb = 32;
+  if (nlb > 64) nlb = 64;
   return nlb;
 }
 
-template <typename R>
-void launch_zr2c(const double* in, cx<R>* out, const Geom& g, const FftPlan& pz, const cx<R>* twz, AliasMap amap,
+template <typename S, typename C>
+void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz, const cx<C>* twz, AliasMap amap,
                  const double* shift_sum, hipStream_t st) {
-  const int nlb = zlines_per_block<R>(g.nz);
+  const int nlb = zlines_per_block<C>(g.nz);
+  if ((nlb >> 1) * g.nz > 8 * NT) throw std::runtime_error("z extent too large for the z transform kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
-  const size_t sh = 2 * sizeof(cx<R>) * nlb * g.nzh;
-  hipLaunchKernelGGL(k_zr2c<R>, dim3(nblk), dim3(256), sh, st, in, out, g, pz, twz, nlb, amap, shift_sum);
+  const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz;
+  hipLaunchKernelGGL((k_zr2c<S, C>), dim3(nblk), dim3(NT), sh, st, in, out, g, pz, twz, nlb, amap, shift_sum);
 }
 
-template <typename R>
-void launch_zc2r(const cx<R>* S, int nch, double* out, const double* add, double scale, const Geom& g,
-                 const FftPlan& pz, const cx<R>* twz, const double* shift_sum, DcTable dc, hipStream_t st) {
-  const int nlb = zlines_per_block<R>(g.nz);
+template <typename S, typename C>
+void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, double scale, const Geom& g,
+                 const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st) {
+  const int nlb = zlines_per_block<C>(g.nz);
+  if ((nlb >> 1) * g.nz > 8 * NT) throw std::runtime_error("z extent too large for the z transform kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
-  const size_t sh = 2 * sizeof(cx<R>) * nlb * g.nzh;
-  hipLaunchKernelGGL(k_zc2r<R>, dim3(nblk, nch), dim3(256), sh, st, S, out, add, scale, g, pz, twz, nlb,
+  const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz + sizeof(cx<S>) * nlb * g.nzh;
+  hipLaunchKernelGGL((k_zc2r<S, C>), dim3(nblk, nch), dim3(NT), sh, st, Sp, out, add, scale, g, pz, twz, nlb,
                      shift_sum, dc);
 }
 
@@ -593,112 +682,145 @@ int sum_blocks(const Geom& g) {
 }
 
 void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g, hipStream_t st) {
-  hipLaunchKernelGGL(k_sum, dim3(sum_blocks(g)), dim3(256), 0, st, in, amap, g.N, partials);
+  hipLaunchKernelGGL(k_sum, dim3(sum_blocks(g)), dim3(NT), 0, st, in, amap, g.N, partials);
 }
 
-template <typename R>
-void launch_lines(cx<R>* S, long nlines, const FftPlan& p, const cx<R>* tw, int sg, hipStream_t st) {
-  int nlb = 2048 / p.L;
-  if (nlb < 1) nlb = 1;
+template <typename S, typename C>
+void launch_lines(cx<S>* Sp, long nlines, const FftPlan& p, const cx<C>* tw, int sg, hipStream_t st) {
+  const int nlb = 2048 / p.L;
+  if (nlb < 1) throw std::runtime_error("line length too large for the line transform kernel");
   const unsigned nblk = static_cast<unsigned>((nlines + nlb - 1) / nlb);
-  const size_t sh = 2 * sizeof(cx<R>) * nlb * p.L;
-  hipLaunchKernelGGL(k_lines<R>, dim3(nblk), dim3(256), sh, st, S, nlines, p, tw, nlb, sg);
+  const size_t sh = sizeof(cx<C>) * nlb * p.L;
+  hipLaunchKernelGGL((k_lines<S, C>), dim3(nblk), dim3(NT), sh, st, Sp, nlines, p, tw, nlb, sg);
 }
 
-template <typename R>
-static int x_tile(const Geom& g) {
-  return sizeof(R) == 4 ? 16 : 8;
+static constexpr int kXTile = 16;  // y columns per x-line block (128-byte rows in FP32)
+
+template <typename S, typename C>
+void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st) {
+  const int ty = kXTile;
+  const size_t sh = sizeof(cx<C>) * ty * (g.nx + 1);
+  dim3 grid((g.ny + ty - 1) / ty, g.nzh, nch);
+  if (ty * g.nx <= 8 * NT)
+    hipLaunchKernelGGL((k_xlines<S, C, 8>), grid, dim3(NT), sh, st, Sp, g, px, twx, ty, sg);
+  else if (ty * g.nx <= 16 * NT)
+    hipLaunchKernelGGL((k_xlines<S, C, 16>), grid, dim3(NT), sh, st, Sp, g, px, twx, ty, sg);
+  else
+    throw std::runtime_error("x extent too large for the x transform kernel");
 }
 
-template <typename R>
-void launch_xlines(cx<R>* S, int nch, const Geom& g, const FftPlan& px, const cx<R>* twx, int sg, hipStream_t st) {
-  const int ty = x_tile<R>(g);
-  const size_t sh = 2 * sizeof(cx<R>) * ty * (g.nx + 1);
-  hipLaunchKernelGGL(k_xlines<R>, dim3((g.ny + ty - 1) / ty, g.nzh, nch), dim3(256), sh, st, S, g, px, twx, ty, sg);
-}
-
-template <typename R>
-void launch_xfwd(const cx<R>* S0, cx<R>* SF, const R* mf, const Geom& g, const FftPlan& px, const cx<R>* twx,
+template <typename S, typename C>
+void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const Geom& g, const FftPlan& px, const cx<C>* twx,
                  ChannelSet cs, hipStream_t st) {
-  const int ty = x_tile<R>(g);
-  const size_t sh = 2 * sizeof(cx<R>) * ty * (g.nx + 1);
-  const int ept = (ty * g.nx + 255) / 256;
-  if (ept > 16) throw std::runtime_error("x extent too large for the fused k-space kernel");
+  const int ty = kXTile;
+  constexpr int G = sizeof(C) == 4 ? 2 : 1;  // channels per batched transform (LDS <= 64 KiB)
+  const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
-  if (ept <= 4) hipLaunchKernelGGL((k_xfwd<R, 4>), grid, dim3(256), sh, st, S0, SF, mf, g, px, twx, ty, cs);
-  else if (ept <= 8) hipLaunchKernelGGL((k_xfwd<R, 8>), grid, dim3(256), sh, st, S0, SF, mf, g, px, twx, ty, cs);
-  else hipLaunchKernelGGL((k_xfwd<R, 16>), grid, dim3(256), sh, st, S0, SF, mf, g, px, twx, ty, cs);
+  if (ty * g.nx <= 8 * NT)
+    hipLaunchKernelGGL((k_xfwd<S, C, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, g, px, twx, ty, cs);
+  else if (ty * g.nx <= 16 * NT)
+    hipLaunchKernelGGL((k_xfwd<S, C, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, g, px, twx, ty, cs);
+  else
+    throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
 
-template <typename R>
-void launch_xbwd(const cx<R>* SB, cx<R>* SO, const Geom& g, const FftPlan& px, const cx<R>* twx, int nb,
+template <typename S, typename C>
+void launch_xbwd(const cx<S>* SB, cx<S>* SO, const Geom& g, const FftPlan& px, const cx<C>* twx, int nb,
                  hipStream_t st) {
-  const int ty = x_tile<R>(g);
-  const size_t sh = 2 * sizeof(cx<R>) * ty * (g.nx + 1);
-  const int ept = (ty * g.nx + 255) / 256;
-  if (ept > 16) throw std::runtime_error("x extent too large for the fused k-space kernel");
+  const int ty = kXTile;
+  constexpr int G = sizeof(C) == 4 ? 2 : 1;
+  const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
-  if (ept <= 4) hipLaunchKernelGGL((k_xbwd<R, 4>), grid, dim3(256), sh, st, SB, SO, g, px, twx, ty, nb);
-  else if (ept <= 8) hipLaunchKernelGGL((k_xbwd<R, 8>), grid, dim3(256), sh, st, SB, SO, g, px, twx, ty, nb);
-  else hipLaunchKernelGGL((k_xbwd<R, 16>), grid, dim3(256), sh, st, SB, SO, g, px, twx, ty, nb);
+  if (ty * g.nx <= 8 * NT)
+    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, g, px, twx, ty, nb);
+  else if (ty * g.nx <= 16 * NT)
+    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, g, px, twx, ty, nb);
+  else
+    throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
 
-// z-lines per block of the fused pointwise kernel: 2048 points (FP32) or
-// 1024 points (FP64) per 512-thread block, so each thread keeps 4 or 2
-// points x 12 channels in registers.
-template <typename R>
+// z-lines per block of the fused pointwise kernel: PPT*NT points (4 per thread
+// in FP32 storage, 2 in FP64).
+template <typename S>
 static int zpoint_nlb(int nz) {
-  int nlb = (sizeof(R) == 4 ? 2048 : 1024) / nz;
+  int nlb = (sizeof(S) == 4 ? 4 : 2) * NT / nz;
   if (nlb < 2) nlb = 2;
   if (nlb & 1) --nlb;
-  if (nlb > 32) nlb = 32;
+  if (nlb > 64) nlb = 64;
   return nlb;
 }
 
-template <typename R>
+template <typename S>
 int zpoint_blocks(const Geom& g) {
-  const int nlb = zpoint_nlb<R>(g.nz);
+  const int nlb = zpoint_nlb<S>(g.nz);
   return static_cast<int>((g.nxy + nlb - 1) / nlb);
 }
 template int zpoint_blocks<float>(const Geom&);
 template int zpoint_blocks<double>(const Geom&);
 
-template <typename R>
-void launch_zpoint(cx<R>* SF, const double* rho, const double* vext, double* local_force, double* partials,
-                   const Geom& g, const FftPlan& pz, const cx<R>* twz, ChannelSet cs, FmtParams fp, double mu,
-                   AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
-  const int nlb = zpoint_nlb<R>(g.nz);
+template <typename S, typename CF, typename CB, int PPT, int NF, int NB>
+static void zpoint_go(unsigned nblk, size_t sh, hipStream_t st, cx<S>* SF, const double* rho, const double* vext,
+                      double* lf, double* partials, const Geom& g, const FftPlan& pz, const cx<CF>* twf,
+                      const cx<CB>* twb, int nlb, FmtParams fp, double mu, AliasMap amap, const double* shift_sum,
+                      DcTable dc) {
+  hipLaunchKernelGGL((k_zpoint<S, CF, CB, PPT, NF, NB>), dim3(nblk), dim3(NT), sh, st, SF, rho, vext, lf, partials,
+                     g, pz, twf, twb, nlb, fp, mu, amap, shift_sum, dc);
+}
+
+template <typename S, typename CF, typename CB>
+void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* local_force, double* partials,
+                   const Geom& g, const FftPlan& pz, const cx<CF>* twf, const cx<CB>* twb, ChannelSet cs,
+                   FmtParams fp, double mu, AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
+  constexpr int PPT = sizeof(S) == 4 ? 4 : 2;
+  const int nlb = zpoint_nlb<S>(g.nz);
+  if (nlb * g.nz > PPT * NT) throw std::runtime_error("z extent too large for the fused pointwise kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
-  const size_t sh = 2 * sizeof(cx<R>) * nlb * g.nzh;
-  const int ppt = (nlb * g.nz + 511) / 512;
-  if (ppt > 4) throw std::runtime_error("z extent too large for the fused pointwise kernel");
-  if (ppt <= 2)
-    hipLaunchKernelGGL((k_zpoint<R, 2>), dim3(nblk), dim3(512), sh, st, SF, rho, vext, local_force, partials, g, pz,
-                       twz, nlb, cs, fp, mu, amap, shift_sum, dc);
-  else
-    hipLaunchKernelGGL((k_zpoint<R, 4>), dim3(nblk), dim3(512), sh, st, SF, rho, vext, local_force, partials, g, pz,
-                       twz, nlb, cs, fp, mu, amap, shift_sum, dc);
+  const int zs = (nlb >> 1) * g.nz;
+  const size_t sh = std::max(sizeof(cx<CF>), sizeof(cx<CB>)) * ZP_G * zs + sizeof(cx<S>) * ZP_G * nlb * g.nzh;
+#define CDFT_ZP(NF, NB)                                                                                         \
+  zpoint_go<S, CF, CB, PPT, NF, NB>(nblk, sh, st, SF, rho, vext, local_force, partials, g, pz, twf, twb, nlb, fp, \
+                                    mu, amap, shift_sum, dc)
+  if (cs.nf == 12 && cs.nb == 11) CDFT_ZP(12, 11);
+  else if (cs.nf == 12 && cs.nb == 0) CDFT_ZP(12, 0);
+  else if (cs.nf == 11 && cs.nb == 11) CDFT_ZP(11, 11);
+  else if (cs.nf == 11 && cs.nb == 0) CDFT_ZP(11, 0);
+  else if (cs.nf == 1 && cs.nb == 0) CDFT_ZP(1, 0);
+  else if (cs.nf == 0 && cs.nb == 0) CDFT_ZP(0, 0);
+  else throw std::runtime_error("unsupported channel configuration");
+#undef CDFT_ZP
 }
 
 void launch_finalize(const double* partials, int nblk, double* res, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, partials, nblk, res);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NT), 0, st, partials, nblk, res);
 }
 
-#define CDFT_INST(R)                                                                                              \
-  template void launch_zr2c<R>(const double*, cx<R>*, const Geom&, const FftPlan&, const cx<R>*, AliasMap,       \
-                               const double*, hipStream_t);                                                       \
-  template void launch_zc2r<R>(const cx<R>*, int, double*, const double*, double, const Geom&, const FftPlan&,   \
-                               const cx<R>*, const double*, DcTable, hipStream_t);                                \
-  template void launch_lines<R>(cx<R>*, long, const FftPlan&, const cx<R>*, int, hipStream_t);                   \
-  template void launch_xlines<R>(cx<R>*, int, const Geom&, const FftPlan&, const cx<R>*, int, hipStream_t);      \
-  template void launch_xfwd<R>(const cx<R>*, cx<R>*, const R*, const Geom&, const FftPlan&, const cx<R>*,         \
-                               ChannelSet, hipStream_t);                                                          \
-  template void launch_xbwd<R>(const cx<R>*, cx<R>*, const Geom&, const FftPlan&, const cx<R>*, int, hipStream_t); \
-  template void launch_zpoint<R>(cx<R>*, const double*, const double*, double*, double*, const Geom&,            \
-                                 const FftPlan&, const cx<R>*, ChannelSet, FmtParams, double, AliasMap,            \
-                                 const double*, DcTable, hipStream_t);
+#define CDFT_INST_SC(S, C)                                                                                         \
+  template void launch_zr2c<S, C>(const double*, cx<S>*, const Geom&, const FftPlan&, const cx<C>*, AliasMap,      \
+                                  const double*, hipStream_t);                                                     \
+  template void launch_zc2r<S, C>(const cx<S>*, int, double*, const double*, double, const Geom&, const FftPlan&, \
+                                  const cx<C>*, const double*, DcTable, hipStream_t);                              \
+  template void launch_lines<S, C>(cx<S>*, long, const FftPlan&, const cx<C>*, int, hipStream_t);                 \
+  template void launch_xlines<S, C>(cx<S>*, int, const Geom&, const FftPlan&, const cx<C>*, int, hipStream_t);    \
+  template void launch_xfwd<S, C>(const cx<S>*, cx<S>*, const S*, const Geom&, const FftPlan&, const cx<C>*,       \
+                                  ChannelSet, hipStream_t);                                                        \
+  template void launch_xbwd<S, C>(const cx<S>*, cx<S>*, const Geom&, const FftPlan&, const cx<C>*, int,           \
+                                  hipStream_t);
 
-CDFT_INST(float)
-CDFT_INST(double)
+CDFT_INST_SC(float, float)
+CDFT_INST_SC(float, double)
+CDFT_INST_SC(double, double)
+
+template void launch_zpoint<float, double, float>(cx<float>*, const double*, const double*, double*, double*,
+                                                  const Geom&, const FftPlan&, const cx<double>*, const cx<float>*,
+                                                  ChannelSet, FmtParams, double, AliasMap, const double*, DcTable,
+                                                  hipStream_t);
+template void launch_zpoint<float, float, float>(cx<float>*, const double*, const double*, double*, double*,
+                                                 const Geom&, const FftPlan&, const cx<float>*, const cx<float>*,
+                                                 ChannelSet, FmtParams, double, AliasMap, const double*, DcTable,
+                                                 hipStream_t);
+template void launch_zpoint<double, double, double>(cx<double>*, const double*, const double*, double*, double*,
+                                                    const Geom&, const FftPlan&, const cx<double>*,
+                                                    const cx<double>*, ChannelSet, FmtParams, double, AliasMap,
+                                                    const double*, DcTable, hipStream_t);
 
 }  // namespace cdft

@@ -50,31 +50,31 @@ struct DcTable {
   double v[12];
 };
 
-template <typename R>
-void launch_zr2c(const double* in, cx<R>* out, const Geom& g, const FftPlan& pz, const cx<R>* twz, AliasMap amap,
+template <typename S, typename C>
+void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz, const cx<C>* twz, AliasMap amap,
                  const double* shift_sum, hipStream_t st);
-template <typename R>
-void launch_zc2r(const cx<R>* S, int nch, double* out, const double* add, double scale, const Geom& g,
-                 const FftPlan& pz, const cx<R>* twz, const double* shift_sum, DcTable dc, hipStream_t st);
+template <typename S, typename C>
+void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, double scale, const Geom& g,
+                 const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st);
 // FP64 block sums of amap(in) into partials[b*8]; finalize with launch_finalize.
 int sum_blocks(const Geom& g);
 void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g, hipStream_t st);
-template <typename R>
-void launch_lines(cx<R>* S, long nlines, const FftPlan& p, const cx<R>* tw, int sg, hipStream_t st);
-template <typename R>
-void launch_xlines(cx<R>* S, int nch, const Geom& g, const FftPlan& px, const cx<R>* twx, int sg, hipStream_t st);
-template <typename R>
-void launch_xfwd(const cx<R>* S0, cx<R>* SF, const R* mf, const Geom& g, const FftPlan& px, const cx<R>* twx,
+template <typename S, typename C>
+void launch_lines(cx<S>* Sp, long nlines, const FftPlan& p, const cx<C>* tw, int sg, hipStream_t st);
+template <typename S, typename C>
+void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st);
+template <typename S, typename C>
+void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const Geom& g, const FftPlan& px, const cx<C>* twx,
                  ChannelSet cs, hipStream_t st);
-template <typename R>
-void launch_xbwd(const cx<R>* SB, cx<R>* SO, const Geom& g, const FftPlan& px, const cx<R>* twx, int nb,
+template <typename S, typename C>
+void launch_xbwd(const cx<S>* SB, cx<S>* SO, const Geom& g, const FftPlan& px, const cx<C>* twx, int nb,
                  hipStream_t st);
-template <typename R>
+template <typename S>
 int zpoint_blocks(const Geom& g);
-template <typename R>
-void launch_zpoint(cx<R>* SF, const double* rho, const double* vext, double* local_force, double* partials,
-                   const Geom& g, const FftPlan& pz, const cx<R>* twz, ChannelSet cs, FmtParams fp, double mu,
-                   AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st);
+template <typename S, typename CF, typename CB>
+void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* local_force, double* partials,
+                   const Geom& g, const FftPlan& pz, const cx<CF>* twf, const cx<CB>* twb, ChannelSet cs,
+                   FmtParams fp, double mu, AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st);
 void launch_finalize(const double* partials, int nblk, double* res, hipStream_t st);
 
 }  // namespace cdft

@@ -908,6 +908,36 @@ void orc_total(void* h, const double* rho, const double* vext, const double* mu,
 
 double orc_bulk_mu(void* h, double rho) { return static_cast<Ctx*>(h)->bulk_mu(rho); }
 
+// F_hs = sum Phi dV from given weighted-density fields of species 0 (11*N, the
+// channel order of orc_weighted_densities); the per-point loop of hard_sphere.hpp:174-224.
+double orc_fhs_from_fields(void* h, const double* wd) {
+  auto* c = static_cast<Ctx*>(h);
+  const long N = c->N;
+  const double R = 0.5 * c->diam[0], fpr = 4.0 * PI * R;
+  double F = 0.0;
+#pragma omp parallel for reduction(+ : F)
+  for (long idx = 0; idx < N; ++idx) {
+    Meas m{};
+    m.eta = wd[idx];
+    m.n2 = wd[N + idx];
+    m.n1 = m.n2 / fpr;
+    m.n0 = m.n2 / (fpr * R);
+    for (int a = 0; a < 3; ++a) {
+      m.v1[a] = wd[(2 + a) * N + idx];
+      m.v0[a] = m.v1[a] / fpr;
+    }
+    int c6 = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = i; j < 3; ++j, ++c6) {
+        m.T[i * 3 + j] = wd[(5 + c6) * N + idx];
+        m.T[j * 3 + i] = m.T[i * 3 + j];
+      }
+    if (m.eta < 1e-30) continue;
+    F += phi_dphi(c->model, c->A, c->B, m, nullptr) * c->dv;
+  }
+  return F;
+}
+
 double orc_phi_dphi(int model, double A, double B, const double* m, double* dm) {
   Meas mm;
   std::memcpy(&mm, m, sizeof(Meas));

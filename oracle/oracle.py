@@ -54,6 +54,8 @@ def lib():
         for name in ("orc_fmt_weights", "orc_mf_weight_k", "orc_mf_weight_real"):
             getattr(L, name).argtypes = [C.c_void_p, C.c_int, _dp]
         L.orc_total.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.orc_fhs_from_fields.restype = C.c_double
+        L.orc_fhs_from_fields.argtypes = [C.c_void_p, _dp]
         L.orc_bulk_mu.restype = C.c_double
         L.orc_bulk_mu.argtypes = [C.c_void_p, C.c_double]
         L.orc_phi_dphi.restype = C.c_double
@@ -180,6 +182,12 @@ class Oracle:
         forces = np.zeros(S * self.N)
         lib().orc_total(self._h, _p(rho), _p(v), _p(mu), _p(parts), _p(forces))
         return parts, forces.reshape((S,) + self.shape) if S > 1 else forces.reshape(self.shape)
+
+    def fhs_from_fields(self, wd):
+        """F_hs from the 11 FMT weighted-density fields (channel order of weighted_densities)."""
+        wd = np.ascontiguousarray(wd, dtype=np.float64).reshape(-1)
+        assert wd.size >= 11 * self.N
+        return lib().orc_fhs_from_fields(self._h, _p(wd))
 
     def bulk_mu(self, rho):
         return lib().orc_bulk_mu(self._h, float(rho))
