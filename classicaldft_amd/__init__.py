@@ -19,12 +19,13 @@ POTENTIALS = {"LJ": 0, "tWF": 1, "WRDF": 2}
 SPLITS = {"WCA": 0, "BH": 1}
 SCHEMES = {"zero": 0, "linearE": 1, "linearF": 2, "quadraticF": 3, "gaussE": 4, "gaussF": 5}
 PRECISIONS = {"fp64": 0, "mixed": 1}
+SCHEDULES = {"auto": 0, "general": 1}
 
 EXPORTED = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_grid_shape", "cdft_a_vdw",
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
-    "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name",
+    "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
 ]
 
 
@@ -50,7 +51,7 @@ class _Interaction(C.Structure):
 
 class _Options(C.Structure):
     _fields_ = [("fmt_model", C.c_int), ("esfmt_A", C.c_double), ("esfmt_B", C.c_double), ("precision", C.c_int),
-                ("device", C.c_int)]
+                ("device", C.c_int), ("schedule", C.c_int)]
 
 
 class _Result(C.Structure):
@@ -120,7 +121,8 @@ def lib():
         L.cdft_profile_stages.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int, _dp,
                                           _dp, C.POINTER(C.c_int)]
         L.cdft_stage_name.restype = C.c_char_p
-        L.cdft_stage_name.argtypes = [C.c_int]
+        L.cdft_stage_name.argtypes = [C.c_void_p, C.c_int]
+        L.cdft_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -149,7 +151,7 @@ class Context:
     """
 
     def __init__(self, shape, dx, diameter=1.0, interactions=(), kT=1.0, model="WhiteBearII", precision="fp64",
-                 A=1.0, B=0.0, device=0):
+                 A=1.0, B=0.0, device=0, schedule="auto"):
         g = _Grid(float(dx), (C.c_double * 3)(*[float(n) * float(dx) for n in shape]))
         its = (_Interaction * max(1, len(interactions)))()
         for k, it in enumerate(interactions):
@@ -157,7 +159,8 @@ class Context:
                                   it.get("sigma", 1.0), it.get("epsilon", 1.0), it.get("r_cutoff", -1.0),
                                   it.get("alpha", 50.0), SPLITS[it.get("split", "WCA")],
                                   SCHEMES[it.get("scheme", "quadraticF")])
-        opt = _Options(FMT_MODELS[model], float(A), float(B), PRECISIONS[precision], int(device))
+        opt = _Options(FMT_MODELS[model], float(A), float(B), PRECISIONS[precision], int(device),
+                       SCHEDULES[schedule])
         d = np.array([float(diameter)])
         h = C.c_void_p()
         _check(lib().cdft_ctx_create(C.byref(g), _p(d), 1, its, len(interactions), float(kT), C.byref(opt),
@@ -166,6 +169,9 @@ class Context:
         sh = (C.c_long * 3)()
         _check(lib().cdft_grid_shape(self._h, sh))
         self.shape = tuple(sh)
+        sc = C.c_int()
+        _check(lib().cdft_schedule(self._h, C.byref(sc)))
+        self.schedule = "fused" if sc.value == 2 else "general"
         self.N = int(np.prod(self.shape))
         self.M = self.shape[0] * self.shape[1] * (self.shape[2] // 2 + 1)
         self.dx = float(dx)
@@ -229,7 +235,7 @@ class Context:
         _check(lib().cdft_profile_stages(self._h, C.c_void_p(d_rho), C.c_void_p(d_vext) if d_vext else None,
                                          float(mu), C.c_void_p(d_forces) if d_forces else None, int(reps), ms, by,
                                          C.byref(n)))
-        return [(lib().cdft_stage_name(i).decode(), ms[i], by[i]) for i in range(n.value)]
+        return [(lib().cdft_stage_name(self._h, i).decode(), ms[i], by[i]) for i in range(n.value)]
 
     def weighted_densities(self, rho):
         """Weighted densities: 11 FMT channels (when an FMT model is set) then the mean-field convolution."""

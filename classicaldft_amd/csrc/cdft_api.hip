@@ -104,6 +104,14 @@ int cdft_grid_shape(const cdft_ctx* ctx, long shape[3]) {
   });
 }
 
+int cdft_schedule(const cdft_ctx* ctx, int* schedule) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!schedule) throw std::invalid_argument("cdft_schedule: null argument");
+    *schedule = ctx->impl->fast ? CDFT_SCHEDULE_FUSED : CDFT_SCHEDULE_GENERAL;
+  });
+}
+
 int cdft_a_vdw(const cdft_ctx* ctx, int k, double* a_vdw) {
   return guarded([&] {
     check_ctx(ctx);
@@ -183,17 +191,18 @@ int cdft_profile_stages(cdft_ctx* ctx, const double* d_rho, const double* d_vext
     if (!d_rho || !stage_ms || reps < 1) throw std::invalid_argument("cdft_profile_stages: bad argument");
     Context* c = ctx->impl;
     CDFT_HIP(hipSetDevice(c->device));
+    const int ns = c->nstages();
     hipEvent_t ev[kNumStages + 1];
-    for (auto& e : ev) CDFT_HIP(hipEventCreate(&e));
-    for (int i = 0; i < kNumStages; ++i) stage_ms[i] = 0.0;
+    for (int i = 0; i <= ns; ++i) CDFT_HIP(hipEventCreate(&ev[i]));
+    for (int i = 0; i < ns; ++i) stage_ms[i] = 0.0;
     c->stage_events = ev;
     try {
       for (int r = 0; r < reps; ++r) {
         // events of skipped stages repeat the previous timestamp
-        for (auto& e : ev) CDFT_HIP(hipEventRecord(e, c->stream));
+        for (int i = 0; i <= ns; ++i) CDFT_HIP(hipEventRecord(ev[i], c->stream));
         c->eval(d_rho, d_vext, mu, d_forces, AliasMap{0, 0.0, 0.0});
-        CDFT_HIP(hipEventSynchronize(ev[kNumStages]));
-        for (int i = 0; i < kNumStages; ++i) {
+        CDFT_HIP(hipEventSynchronize(ev[ns]));
+        for (int i = 0; i < ns; ++i) {
           float t = 0.0f;
           CDFT_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
           stage_ms[i] += t / reps;
@@ -204,13 +213,16 @@ int cdft_profile_stages(cdft_ctx* ctx, const double* d_rho, const double* d_vext
       throw;
     }
     c->stage_events = nullptr;
-    for (auto& e : ev) (void)hipEventDestroy(e);
+    for (int i = 0; i <= ns; ++i) (void)hipEventDestroy(ev[i]);
     if (stage_bytes) c->stage_bytes(stage_bytes, d_vext != nullptr);
-    if (n_stages) *n_stages = kNumStages;
+    if (n_stages) *n_stages = ns;
   });
 }
 
-const char* cdft_stage_name(int i) { return (i >= 0 && i < kNumStages) ? kStageNames[i] : ""; }
+const char* cdft_stage_name(const cdft_ctx* ctx, int i) {
+  if (!ctx || !ctx->impl) return "";
+  return ctx->impl->stage_name(i);
+}
 
 int cdft_weighted_densities(cdft_ctx* ctx, const double* rho, double* out) {
   return guarded([&] {

@@ -57,8 +57,9 @@ void Context::clear_graphs() {
 Context::~Context() {
   hipSetDevice(device);
   clear_graphs();
-  void* bufs[] = {d_rho, d_vext, d_force, d_local, d_S0, d_SF, d_SO, d_mf, d_partials, d_res, d_sum_part, d_shift,
-                  tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2]};
+  void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
+                  d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
+                  ftw64,   ftw32};
   hipSetDevice(device);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -89,6 +90,8 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     throw Unsupported("the HIP path evaluates at most one mean-field interaction");
   if (options->precision != CDFT_PRECISION_FP64 && options->precision != CDFT_PRECISION_MIXED)
     throw std::invalid_argument("unknown precision mode");
+  if (options->schedule != CDFT_SCHEDULE_AUTO && options->schedule != CDFT_SCHEDULE_GENERAL)
+    throw std::invalid_argument("unknown schedule");
   const int model = options->fmt_model;
   if (model < -1 || model > 4) throw std::invalid_argument("unknown FMT model id " + std::to_string(model));
 
@@ -179,7 +182,14 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->d_SF = dalloc_bytes(cs_bytes * g.M * nf_alloc);
     c->d_SO = dalloc_bytes(cs_bytes * g.M);
     c->d_mf = dalloc_bytes(8 * g.M);  // FP64 values, or FP32 (value, residual) planes
+    c->fast = options->schedule == CDFT_SCHEDULE_AUTO && fast_supported(g.nx, g.ny, g.nz);
     c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g) : zpoint_blocks<double>(g);
+    if (c->fast) {
+      c->fg = make_fast_geom(g);
+      c->nblk_point = fast_point_blocks(c->fg);
+      fast_twiddles(&c->ftw64, &c->ftw32);
+      c->d_P = dalloc_bytes(cs_bytes * g.M * 2);
+    }
     c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
     c->d_res = dalloc<double>(8);
     c->nblk_sum = sum_blocks(g);
@@ -208,6 +218,13 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
         std::vector<cx<double>> spec(static_cast<size_t>(g.M));
         CDFT_HIP(hipMemcpyAsync(spec.data(), d_spec, 16 * g.M, hipMemcpyDeviceToHost, c->stream));
         CDFT_HIP(hipStreamSynchronize(c->stream));
+        if (c->fast) {
+          std::vector<double> pm(static_cast<size_t>(g.M)), fb(static_cast<size_t>(g.M));
+          for (long i = 0; i < g.M; ++i) pm[i] = spec[i].re;
+          fast_mf_layout(pm.data(), fb.data(), g);
+          c->d_mfB = dalloc<double>(g.M);
+          CDFT_HIP(hipMemcpy(c->d_mfB, fb.data(), sizeof(double) * g.M, hipMemcpyHostToDevice));
+        }
         if (c->precision == CDFT_PRECISION_MIXED) {
           std::vector<float> re(2 * static_cast<size_t>(g.M));
           for (long i = 0; i < g.M; ++i) {
@@ -250,6 +267,44 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
 const char* const kStageNames[kNumStages] = {"k_sum",    "k_zr2c",            "k_lines(y fwd rho)", "k_xfwd",
                                               "k_lines(y inv nf)", "k_zpoint", "k_lines(y fwd nb)", "k_xbwd",
                                               "k_lines(y inv 1)",  "k_zc2r",   "k_finalize"};
+
+const char* const kFastStageNames[kNumFastStages] = {"k_sum",        "kf_zfwd", "kf_plane_fwd", "kf_zpoint",
+                                                     "kf_plane_bwd", "kf_zinv", "k_finalize"};
+
+template <typename S, typename CV>
+void Context::eval_fast_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
+  auto* A0 = static_cast<cx<S>*>(d_S0);
+  auto* SF = static_cast<cx<S>*>(d_SF);
+  auto* P = static_cast<cx<S>*>(d_P);
+  const bool backward = force != nullptr && cs.nb > 0;
+  auto mark = [&](int i) {
+    if (stage_events) CDFT_HIP(hipEventRecord(stage_events[i], stream));
+  };
+  mark(0);
+  if (cs.nf > 0) {
+    launch_sum(rho, amap, d_sum_part, g, stream);
+    launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
+    mark(1);
+    launch_fast_zfwd<S>(rho, A0, fg, ftw64, amap, d_shift, stream);
+    mark(2);
+    launch_fast_plane_fwd<S, CV>(A0, SF, d_mfB, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cs, stream);
+  }
+  ChannelSet cz = cs;
+  if (!backward) cz.nb = 0;
+  double* lf = backward ? d_local : (force ? force : d_local);
+  mark(3);
+  launch_fast_zpoint<S, CV>(SF, rho, vext, lf, d_partials, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cz, fp, mu,
+                            amap, cs.nf > 0 ? d_shift : nullptr, dc, stream);
+  if (backward) {
+    mark(4);
+    launch_fast_plane_bwd<S, CV>(SF, P, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
+    mark(5);
+    launch_fast_zinv<S, CV>(P, force, d_local, g.dv, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
+  }
+  mark(6);
+  launch_finalize(d_partials, nblk_point, d_res, stream);
+  mark(7);
+}
 
 template <typename S, typename CE, typename CL>
 void Context::eval_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
@@ -303,6 +358,19 @@ void Context::stage_bytes(double* b, bool with_vext) const {
   const double N = static_cast<double>(g.N), M = static_cast<double>(g.M);
   for (int i = 0; i < kNumStages; ++i) b[i] = 0.0;
   const double nf = cs.nf, nb = cs.nb;
+  if (fast) {
+    if (cs.nf > 0) {
+      b[0] = 8.0 * N;                                  // FP64 sum of rho
+      b[1] = 8.0 * N + sf * M;                         // rho -> rho_hat parts
+      b[2] = sf * M + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // rho_hat (+ FP64 table) -> nf channels
+    }
+    b[3] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;
+    if (cs.nb > 0) {
+      b[4] = nb * sf * M + 2.0 * sf * M;               // nb channels -> two partial sums
+      b[5] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
+    }
+    return;
+  }
   if (cs.nf > 0) {
     b[0] = 8.0 * N;                                     // FP64 sum of rho
     b[1] = 8.0 * N + sf * M;                            // rho (FP64) -> rho_hat
@@ -322,8 +390,13 @@ void Context::stage_bytes(double* b, bool with_vext) const {
 
 void Context::eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
   auto direct = [&] {
-    if (precision == CDFT_PRECISION_MIXED) eval_t<float, double, float>(rho, vext, mu, force, amap);
-    else eval_t<double, double, double>(rho, vext, mu, force, amap);
+    if (fast) {
+      if (precision == CDFT_PRECISION_MIXED) eval_fast_t<float, float>(rho, vext, mu, force, amap);
+      else eval_fast_t<double, double>(rho, vext, mu, force, amap);
+    } else {
+      if (precision == CDFT_PRECISION_MIXED) eval_t<float, double, float>(rho, vext, mu, force, amap);
+      else eval_t<double, double, double>(rho, vext, mu, force, amap);
+    }
   };
   if (!use_graphs || stage_events) {
     direct();

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "cdft.h"
+#include "eval_fast.hpp"
 #include "host_weights.hpp"
 #include "kernels.hpp"
 
@@ -35,8 +36,10 @@ struct InteractionSpec {
   PairPotential pot;
 };
 
-constexpr int kNumStages = 11;
+constexpr int kNumStages = 11;  // stages of the general schedule (the most of any path)
 extern const char* const kStageNames[kNumStages];
+constexpr int kNumFastStages = 7;
+extern const char* const kFastStageNames[kNumFastStages];
 
 // Results of one evaluation, in the order the fused kernel accumulates them.
 enum ResultSlot { RS_IDEAL = 0, RS_EXT = 1, RS_HS = 2, RS_MF = 3, RS_MASS = 4 };
@@ -73,7 +76,19 @@ struct Context {
   int nblk_point = 0;
   int nblk_sum = 0;
   DcTable dc{};                  // analytic k = 0 weight of every forward channel
-  hipEvent_t* stage_events = nullptr;  // when set, eval records kNumStages+1 events
+  hipEvent_t* stage_events = nullptr;  // when set, eval records nstages()+1 events
+
+  // Fused five-kernel schedule (power-of-two grids it has kernels for).
+  bool fast = false;
+  FastGeom fg{};
+  void* ftw64 = nullptr;      // shared twiddle tables of the fused kernels
+  void* ftw32 = nullptr;
+  void* d_P = nullptr;        // 2 * M complex: the two partial sums of the backward plane kernel
+  double* d_mfB = nullptr;    // M: mean-field weight in the plane kernel's order
+  int nstages() const { return fast ? kNumFastStages : kNumStages; }
+  const char* stage_name(int i) const {
+    return (i < 0 || i >= nstages()) ? "" : (fast ? kFastStageNames[i] : kStageNames[i]);
+  }
 
   // Captured evaluation graphs, keyed by the arguments baked into them.
   struct GraphKey {
@@ -113,6 +128,10 @@ struct Context {
   // transforms; CL: arithmetic of the other transforms.
   template <typename S, typename CE, typename CL>
   void eval_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
+  // The fused schedule: S storage, CV arithmetic of the non-scalar channels
+  // and of the backward path.
+  template <typename S, typename CV>
+  void eval_fast_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
 
   // Read back d_res and assemble a cdft_result.
   void fetch_result(double mu, cdft_result* out);

@@ -1,0 +1,988 @@
+// Fused five-kernel evaluation schedule (gfx950) for power-of-two grids.
+//
+//   kf_zfwd       z-R2C of rho - mean (FP64), y-parity split    rho -> A0 (chunked split layout)
+//   kf_plane_fwd  per (kz, r, h): y- and x-FFT of rho_hat (FP64), then for
+//                 every forward channel: weight, x-inverse of the output
+//                 half ix = 2i + h, y-inverse, written in the split layout
+//   kf_zpoint     per line pair: y-parity combine + z-C2R of every channel,
+//                 pointwise Phi/dPhi + local terms + partial sums, z-R2C of
+//                 the 11 derivative channels + y-parity split (in place)
+//   kf_plane_bwd  per (kz, r, h): y-FFT and half x-FFT (rows ix = 2i + h) of
+//                 every derivative channel, conj-weighted accumulation over
+//                 the full kx range, x- and y-inverse of the sum -> partial h
+//   kf_zinv       per line pair: sum of the two partials, y-parity combine,
+//                 z-C2R, * dV + local force                    -> force
+//
+// Line transforms run in slots of one wavefront (slot_fft.hpp); a block
+// synchronises only where data crosses wavefronts (the plane transposes and
+// the chunk staging of the z-line kernels).
+//
+// Precision: with FP32 storage (mixed) the forward transform of rho and the
+// inverse transforms of the scalar channels (eta, n2, mean field) are computed
+// in FP64 -- a smooth density concentrates its spectrum in a few large low-k
+// values whose FP32 rounding would add up coherently in Omega -- while the
+// vector and tensor channels and the whole backward path use FP32.
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "device_common.hpp"
+#include "eval_fast.hpp"
+#include "fmt_point.hpp"
+#include "slot_fft.hpp"
+
+namespace cdft {
+
+namespace {
+
+constexpr int kBT = 1024;            // threads of a plane kernel block
+constexpr int kPlaneTB = 73728;      // bytes of a plane kernel's transpose buffer
+constexpr int kZP = 256;             // threads of a kf_zpoint block (4 line pairs)
+
+// Blocks are dispatched round-robin over the 8 XCDs; this maps consecutive
+// logical blocks onto one XCD so that they share its L2.
+__device__ __forceinline__ int xcd_remap(int b, int n) {
+  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+}
+
+__host__ __device__ constexpr int align16(int b) { return (b + 15) & ~15; }
+
+// Split-layout offset of element (kz, r, ix, m).
+__device__ __forceinline__ long split_idx(int kz, int r, int ix, int m, int mb, int nzh) {
+  return ((static_cast<long>(ix) * mb + (m >> 4)) * nzh + kz) * 32 + r * 16 + (m & 15);
+}
+
+// Forward channels in the order the z-line kernel transforms them: the FP64
+// (scalar) channels eta, n2 and the mean field first.
+__host__ __device__ constexpr int zorder(int nf, int pos) {
+  return nf == 1 ? 11 : (nf == 12 ? (pos < 2 ? pos : (pos == 2 ? 11 : pos - 1)) : pos);
+}
+__host__ __device__ constexpr int zpos(int nf, int ch) {
+  return nf == 1 ? 0 : (nf == 12 ? (ch < 2 ? ch : (ch == 11 ? 2 : ch + 1)) : ch);
+}
+__host__ __device__ constexpr bool zscalar(int ch) { return ch == 0 || ch == 1 || ch == 11; }
+// Storage slot of channel ch (in the channel arrays and the k = 0 table): the
+// mean field is slot 11 after the FMT channels, or slot 0 without them.
+__host__ __device__ constexpr int chan_store(int nf, int ch) { return nf == 1 ? 0 : ch; }
+__host__ __device__ constexpr bool zround64(int nf, int ns, int jr) {
+  for (int g = 0; g < ns; ++g) {
+    const int pos = jr * ns + g;
+    if (pos < nf && zscalar(zorder(nf, pos))) return true;
+  }
+  return false;
+}
+
+template <typename C>
+__device__ __forceinline__ cx<C> cneg(cx<C> a) {
+  return {-a.re, -a.im};
+}
+
+// ============================================================ kf_zfwd
+template <int NZ, typename S>
+__host__ __device__ constexpr int zfwd_smem() {
+  return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<S>))) +
+         16 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
+}
+
+template <typename S, int NZ>
+__global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho, cx<S>* __restrict__ out, FastGeom g,
+                                                  const cx<double>* __restrict__ tw, AliasMap amap,
+                                                  const double* __restrict__ shift_sum) {
+  constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<S>* cb = reinterpret_cast<cx<S>*>(smem);  // [r][m16][kz]
+  cx<double>* xbuf = reinterpret_cast<cx<double>*>(smem + align16(NZH * 32 * static_cast<int>(sizeof(cx<S>))));
+  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
+  const long chunk = blockIdx.x;
+  const int ix = static_cast<int>(chunk / g.mb);
+  const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
+  const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
+  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+  cx<double> v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int z = lane + T * q;
+    v[q] = {amap.apply(rho[la * NZ + z]) - shift, amap.apply(rho[lb * NZ + z]) - shift};
+  }
+  cx<double>* xb = xbuf + slot * XS;
+  slot_fft<double, NZ, T, -1>(v, xb, lane, tw);
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < 8; ++e) xb[xpad(lane + T * e)] = v[e];
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e <= 4; ++e) {
+    const int k = lane + T * e;
+    if (k <= NZ / 2) {
+      // spectra of the packed lines (Z = a + i b), then the y-parity split
+      const cx<double> zk = v[e], zm = xb[xpad((NZ - k) & (NZ - 1))];
+      const cx<double> A = {0.5 * (zk.re + zm.re), 0.5 * (zk.im - zm.im)};
+      const cx<double> B = {0.5 * (zk.im + zm.im), -0.5 * (zk.re - zm.re)};
+      cb[slot * NZH + k] = cvt<S>(cadd(A, B));
+      cb[(16 + slot) * NZH + k] = cvt<S>(csub(A, B));
+    }
+  }
+  __syncthreads();
+  cx<S>* o = out + chunk * static_cast<long>(NZH * 32);
+  for (int i = threadIdx.x; i < NZH * 32; i += blockDim.x) o[i] = cb[(i & 31) * NZH + (i >> 5)];
+}
+
+// ============================================================ kf_zinv
+template <int NZ, typename C>
+__host__ __device__ constexpr int zinv_smem() {
+  return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<C>))) +
+         16 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<C>));
+}
+
+// Builds the packed C2R input Z[lane + T q] of line pair (m, m + Q) from the
+// two y-parity parts b0, b1 held at part(r, kz): lines m = b0 + b1,
+// m + Q = b0 - b1; kz = 0 and Nz/2 are projected onto their real parts
+// (the Hermitian extension of the half spectrum).
+template <typename C, int NZ, typename F>
+__device__ __forceinline__ void build_c2r(cx<C> (&v)[8], int lane, F&& part) {
+  constexpr int T = NZ / 8;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int k = lane + T * q;
+    const bool mirror = k > NZ / 2;
+    const int kk = mirror ? NZ - k : k;
+    const cx<C> b0 = part(0, kk), b1 = part(1, kk);
+    cx<C> A = cadd(b0, b1), B = csub(b0, b1);
+    if (kk == 0 || kk == NZ / 2) {
+      A.im = C(0);
+      B.im = C(0);
+    }
+    if (mirror) {
+      A.im = -A.im;
+      B.im = -B.im;
+    }
+    v[q] = {A.re - B.im, A.im + B.re};
+  }
+}
+
+template <typename S, typename C, int NZ>
+__global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, double* __restrict__ out,
+                                                  const double* __restrict__ add, double scale, FastGeom g,
+                                                  const cx<C>* __restrict__ tw) {
+  constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<C>* cb = reinterpret_cast<cx<C>*>(smem);  // [r][m16][kz]
+  cx<C>* xbuf = reinterpret_cast<cx<C>*>(smem + align16(NZH * 32 * static_cast<int>(sizeof(cx<C>))));
+  const int slot = threadIdx.x / T, lane = threadIdx.x % T;
+  const long chunk = blockIdx.x;
+  const int ix = static_cast<int>(chunk / g.mb);
+  const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
+  const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
+  const cx<S>* p0 = P + chunk * static_cast<long>(NZH * 32);
+  const cx<S>* p1 = p0 + g.M;
+  for (int i = threadIdx.x; i < NZH * 32; i += blockDim.x)
+    cb[(i & 31) * NZH + (i >> 5)] = cadd(cvt<C>(p0[i]), cvt<C>(p1[i]));
+  __syncthreads();
+  cx<C> v[8];
+  build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[(r * 16 + slot) * NZH + kz]; });
+  slot_fft<C, NZ, T, +1>(v, xbuf + slot * XS, lane, tw);
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int z = lane + T * e;
+    const long pa = la * NZ + z, pb = lb * NZ + z;
+    out[pa] = static_cast<double>(v[e].re) * scale + (add ? add[pa] : 0.0);
+    out[pb] = static_cast<double>(v[e].im) * scale + (add ? add[pb] : 0.0);
+  }
+}
+
+// ============================================================ kf_zpoint
+template <int NZ, typename S>
+__host__ __device__ constexpr int zpoint_stage_bytes() {
+  return align16((64 / (NZ / 8)) * 8 * (NZ / 2 + 1) * static_cast<int>(sizeof(cx<S>)));
+}
+template <int NZ, typename S>
+__host__ __device__ constexpr int zpoint_smem() {
+  return zpoint_stage_bytes<NZ, S>() + 4 * (64 / (NZ / 8)) * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
+}
+
+// One forward round: slot `slot` of wave `wave` transforms channel position
+// jr*NS + slot of line pair `wave` (C2R), then the wave redistributes the
+// results so that lane lid holds points z = lid + 64 j of both lines.
+template <typename C, typename S, int NZ, int NF, int JR, int PPL>
+__device__ __forceinline__ void zpoint_fwd_round(S (&wv)[NF > 0 ? NF : 1][PPL], const cx<S>* stg, unsigned char* xwave,
+                                                 int wave, int lid, const cx<C>* __restrict__ tw) {
+  constexpr int T = NZ / 8, NS = 64 / T, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
+  const int slot = lid / T, lane = lid % T;
+  cx<C>* xw = reinterpret_cast<cx<C>*>(xwave);
+  cx<C> v[8];
+  build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cvt<C>(stg[((slot * 2 + r) * 4 + wave) * NZH + kz]); });
+  slot_fft<C, NZ, T, +1>(v, xw + slot * XS, lane, tw);
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < 8; ++e) xw[slot * (8 * T) + lane + T * e] = v[e];
+  wave_sync();
+#pragma unroll
+  for (int gs = 0; gs < NS; ++gs) {
+    const int pos = JR * NS + gs;
+    if (pos < NF) {
+#pragma unroll
+      for (int j = 0; j < PPL / 2; ++j) {
+        const cx<C> t = xw[gs * (8 * T) + lid + 64 * j];
+        wv[pos][2 * j] = static_cast<S>(t.re);
+        wv[pos][2 * j + 1] = static_cast<S>(t.im);
+      }
+    }
+  }
+}
+
+// One backward round: derivative channels JR*NS + slot of line pair `wave`
+// go through the z-R2C and the y-parity split into the staging buffer.
+template <typename C, typename S, int NZ, int NF, int NB, int JR, int PPL>
+__device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NF > 0 ? NF : 1][PPL], cx<S>* stg, unsigned char* xwave,
+                                                 int wave, int lid, const cx<C>* __restrict__ tw) {
+  constexpr int T = NZ / 8, NS = 64 / T, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
+  const int slot = lid / T, lane = lid % T;
+  cx<C>* xw = reinterpret_cast<cx<C>*>(xwave);
+  wave_sync();
+#pragma unroll
+  for (int gs = 0; gs < NS; ++gs) {
+    const int c = JR * NS + gs;
+#pragma unroll
+    for (int j = 0; j < PPL / 2; ++j) {
+      cx<C> t = {C(0), C(0)};
+      if (c < NB) t = {static_cast<C>(wv[zpos(NF, c)][2 * j]), static_cast<C>(wv[zpos(NF, c)][2 * j + 1])};
+      xw[gs * (8 * T) + lid + 64 * j] = t;
+    }
+  }
+  wave_sync();
+  cx<C> v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = xw[slot * (8 * T) + lane + T * q];
+  wave_sync();
+  slot_fft<C, NZ, T, -1>(v, xw + slot * XS, lane, tw);
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e < 8; ++e) xw[slot * XS + xpad(lane + T * e)] = v[e];
+  wave_sync();
+#pragma unroll
+  for (int e = 0; e <= 4; ++e) {
+    const int k = lane + T * e;
+    if (k <= NZ / 2) {
+      const cx<C> zk = v[e], zm = xw[slot * XS + xpad((NZ - k) & (NZ - 1))];
+      const cx<C> A = {C(0.5) * (zk.re + zm.re), C(0.5) * (zk.im - zm.im)};
+      const cx<C> B = {C(0.5) * (zk.im + zm.im), C(-0.5) * (zk.re - zm.re)};
+      stg[((slot * 2 + 0) * 4 + wave) * NZH + k] = cvt<S>(cadd(A, B));
+      stg[((slot * 2 + 1) * 4 + wave) * NZH + k] = cvt<S>(csub(A, B));
+    }
+  }
+}
+
+template <typename S, typename CV, int NZ, int NF, int NB>
+__global__ void __launch_bounds__(kZP) kf_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
+                                                 const double* __restrict__ vext, double* __restrict__ lf,
+                                                 double* __restrict__ partials, FastGeom g,
+                                                 const cx<double>* __restrict__ tw64, const cx<CV>* __restrict__ twv,
+                                                 FmtParams fp, double mu, AliasMap amap,
+                                                 const double* __restrict__ shift_sum, DcTable dc) {
+  constexpr int T = NZ / 8, NS = 64 / T, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
+  constexpr int PPL = NZ / 32;  // points per lane: z = lid + 64 j of both lines
+  constexpr int NFA = NF > 0 ? NF : 1;
+  constexpr int NRF = (NF + NS - 1) / NS, NRB = (NB + NS - 1) / NS;
+  constexpr bool HS = NF >= 11;
+  constexpr bool MF = (NF == 12 || NF == 1);
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<S>* stg = reinterpret_cast<cx<S>*>(smem);  // [gs][r][p][kz]
+  const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
+  unsigned char* xwave = smem + zpoint_stage_bytes<NZ, S>() + wave * NS * XS * static_cast<int>(sizeof(cx<double>));
+  const int logical = xcd_remap(blockIdx.x, gridDim.x);
+  const long chunk = logical >> 2;
+  const int qb = logical & 3;
+  const int ix = static_cast<int>(chunk / g.mb);
+  const int m = static_cast<int>(chunk % g.mb) * 16 + qb * 4 + wave;
+  const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
+  const long cbase = chunk * static_cast<long>(NZH * 32) + qb * 4;
+  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+  S wv[NFA][PPL];
+
+  // ---- forward rounds: stage NS channels of the block's 4 line pairs, C2R
+  auto stage_in = [&](int jr) {
+    for (int i = threadIdx.x; i < NS * NZH * 8; i += kZP) {
+      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+      const int kz = t % NZH, gs = t / NZH;
+      const int pos = jr * NS + gs;
+      cx<S> val = {S(0), S(0)};
+      if (pos < NF) val = SF[static_cast<long>(chan_store(NF, zorder(NF, pos))) * g.M + cbase + (kz * 2 + r) * 16 + p];
+      stg[((gs * 2 + r) * 4 + p) * NZH + kz] = val;
+    }
+  };
+#define CDFT_ZF(JR)                                                                                   \
+  if constexpr ((JR) < NRF) {                                                                         \
+    stage_in(JR);                                                                                     \
+    __syncthreads();                                                                                  \
+    if constexpr (zround64(NF, NS, JR))                                                               \
+      zpoint_fwd_round<double, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, tw64);                  \
+    else                                                                                              \
+      zpoint_fwd_round<CV, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, twv);                       \
+    __syncthreads();                                                                                  \
+  }
+  CDFT_ZF(0)
+  CDFT_ZF(1)
+  CDFT_ZF(2)
+  CDFT_ZF(3)
+  CDFT_ZF(4)
+  CDFT_ZF(5)
+#undef CDFT_ZF
+  static_assert(NRF <= 6 && NRB <= 6, "too many z rounds");
+
+  // ---- pointwise: one point at a time (rolled), channel values in and out by selects
+  double acc[5] = {0, 0, 0, 0, 0}, cmp[5] = {0, 0, 0, 0, 0};  // ideal, external, hs, mf, mass
+  const double dv = g.dv;
+#pragma unroll 1
+  for (int pt = 0; pt < PPL; ++pt) {
+    const int z = lid + 64 * (pt >> 1);
+    const long p = ((pt & 1) ? lb : la) * NZ + z;
+    S cur[NFA];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) {
+      S t = wv[i][0];
+#pragma unroll
+      for (int j = 1; j < PPL; ++j) t = (pt == j) ? wv[i][j] : t;
+      cur[i] = t;
+    }
+    S outv[11];
+#pragma unroll
+    for (int i = 0; i < 11; ++i) outv[i] = S(0);
+    const double rho = amap.apply(rho_in[p]);
+    const double lr = log(fmax(rho, 1e-300));
+    double lfv = (lr - mu) * dv;  // ideal_gas.hpp:312-318
+    two_sum(acc[0], cmp[0], rho * (lr - 1.0));
+    two_sum(acc[4], cmp[4], rho);
+    if (vext) {  // external_field.hpp:16-31
+      const double V = vext[p];
+      lfv += V * dv;
+      two_sum(acc[1], cmp[1], rho * V);
+    }
+    if constexpr (MF) {  // mean_field.hpp:259-266 (self pair)
+      const double conv = static_cast<double>(cur[zpos(NF, 11)]) + dc.v[chan_store(NF, 11)] * shift;
+      lfv += conv * dv;
+      two_sum(acc[3], cmp[3], 0.5 * rho * conv);
+    }
+    if constexpr (HS) {
+      double w[11], gg[11];
+#pragma unroll
+      for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(cur[zpos(NF, i)]) + dc.v[i] * shift;
+      const double phi = fmt_phi_grad(fp, w, gg);
+      two_sum(acc[2], cmp[2], phi);
+#pragma unroll
+      for (int i = 0; i < 11; ++i) outv[i] = static_cast<S>(gg[i]);
+    }
+    lf[p] = lfv;
+    if constexpr (NB > 0) {
+#pragma unroll
+      for (int i = 0; i < 11; ++i)
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) wv[zpos(NF, i)][j] = (pt == j) ? outv[i] : wv[zpos(NF, i)][j];
+    }
+  }
+  {
+    double tot[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) tot[i] = acc[i] + cmp[i];
+    __shared__ double out5[5];
+    block_reduce<5>(tot, out5);
+    __syncthreads();
+    if (threadIdx.x < 8)
+      partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x < 5 ? out5[threadIdx.x] * dv : 0.0;
+  }
+
+  // ---- backward rounds: R2C of NS derivative channels, split, coalesced store
+#define CDFT_ZB(JR)                                                                                           \
+  if constexpr ((JR) < NRB) {                                                                                 \
+    zpoint_bwd_round<CV, S, NZ, NF, NB, JR, PPL>(wv, stg, xwave, wave, lid, twv);                             \
+    __syncthreads();                                                                                          \
+    for (int i = threadIdx.x; i < NS * NZH * 8; i += kZP) {                                                   \
+      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;                                                      \
+      const int kz = t % NZH, gs = t / NZH;                                                                   \
+      const int c = (JR) * NS + gs;                                                                           \
+      if (c < NB) SF[static_cast<long>(c) * g.M + cbase + (kz * 2 + r) * 16 + p] = stg[((gs * 2 + r) * 4 + p) * NZH + kz]; \
+    }                                                                                                         \
+    __syncthreads();                                                                                          \
+  }
+  CDFT_ZB(0)
+  CDFT_ZB(1)
+  CDFT_ZB(2)
+  CDFT_ZB(3)
+  CDFT_ZB(4)
+  CDFT_ZB(5)
+#undef CDFT_ZB
+}
+
+// ============================================================ plane kernels
+// A plane kernel block (1024 threads) owns the half plane (kz, r) and the
+// output / input row parity h.  Slot shapes per phase (all 1024 lanes busy at
+// 128 x 128 grids):
+//   y-FFT of full rows (NX rows, Q points):  Q/8 lanes x 8 elements
+//   y-FFT of half rows (NX/2 rows):          Q/4 lanes x 4 elements
+//   x-FFT (NX points, one per column):       NX/8 lanes x 8 elements
+//   half x-FFT (NX/2 points):                NX/8 lanes x 4 elements
+// FP64 slot exchanges go through the buffers one real part at a time.
+template <int NX, int Q>
+struct PlaneCfg {
+  static constexpr int BT = kBT;
+  static constexpr int NY = 2 * Q, MB = Q / 16, NXH = NX / 2;
+  static constexpr int TYF = Q / 8, SBYF = BT / TYF, RPF = (NX + SBYF - 1) / SBYF;
+  static constexpr int TYH = Q / 4, SBYH = BT / TYH, RPH = (NXH + SBYH - 1) / SBYH;
+  static constexpr int TX = NX / 8, SBX = BT / TX, CPS = (Q + SBX - 1) / SBX;
+  static constexpr int XBYTES = 8;  // bytes per slot-buffer element
+  static constexpr int xbuf_elems() {
+    int a = SBYF * slot_buf_len<Q>();
+    int b = SBYH * slot_buf_len<Q>();
+    int c = SBX * slot_buf_len<NX>();
+    int m = a > b ? a : b;
+    return m > c ? m : c;
+  }
+  static_assert(TYH <= 64 && TX <= 64 && TYF >= 1, "unsupported plane shape");
+};
+
+template <int NX, int Q>
+__host__ __device__ constexpr int plane_smem() {
+  return kPlaneTB + PlaneCfg<NX, Q>::xbuf_elems() * PlaneCfg<NX, Q>::XBYTES;
+}
+
+// Per-element FMT weight factors in arithmetic C: w3, w2 and
+// aik2 = (w2 - 3 w3/R) / k^2 (0 at k = 0), from which every channel weight
+// follows without a division (weights.hpp:174-222): ch 0 w3, 1 w2,
+// 2..4 i k_a w3, 5..10 aik2 k_i k_j + delta_ij w3/R.
+template <typename C>
+struct WFac {
+  C w3, w2, aik2;
+};
+template <typename C>
+__device__ __forceinline__ WFac<C> wfactors(double kx, double ky, double kz, const FastGeom& g) {
+  const double k2 = kx * kx + ky * ky + kz * kz;
+  double w3, w2;
+  radial(sqrt(k2), g.R, g.inv_n, &w3, &w2);
+  const double a = k2 > 0.0 ? (w2 - 3.0 * w3 / g.R) / k2 : 0.0;
+  return {static_cast<C>(w3), static_cast<C>(w2), static_cast<C>(a)};
+}
+// Weight of channel ch; `adj` conjugates it (the odd vector channels change
+// sign), as the back-convolution does (hard_sphere.hpp:118-134).
+template <typename C>
+__device__ __forceinline__ cx<C> chan_weight(int ch, bool adj, C kx, C ky, C kz, const WFac<C>& f, C invR) {
+  if (ch == 0) return {f.w3, C(0)};
+  if (ch == 1) return {f.w2, C(0)};
+  if (ch <= 4) {
+    const C ka = (ch == 2 ? kx : (ch == 3 ? ky : kz)) * f.w3;
+    return {C(0), adj ? -ka : ka};
+  }
+  const int q = ch - 5;
+  const C ki = q < 3 ? kx : (q < 5 ? ky : kz);
+  const C kj = (q == 0) ? kx : (q == 1 || q == 3) ? ky : kz;
+  const bool diag = (q == 0 || q == 3 || q == 5);
+  return {f.aik2 * ki * kj + (diag ? f.w3 * invR : C(0)), C(0)};
+}
+
+// Forward-path work of one channel in a kf_plane_fwd block: Wof(cc, e) is the
+// weighted spectrum at column xs + SBX cc, kx = xl + TX e.  x-inverse of the
+// output half ix = 2i + h (DIF: (W[kx] +- W[kx + NX/2]) twiddled), then the
+// y-inverse of every row of that half, stored in the split layout.
+template <typename CC, typename S, int NX, int Q, typename WF>
+__device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned char* xraw, cx<S>* __restrict__ out,
+                                                  const FastGeom& g, int kz, int r, int h,
+                                                  const cx<CC>* __restrict__ tw) {
+  using P = PlaneCfg<NX, Q>;
+  constexpr int QP = Q + 1;
+  constexpr int XS = slot_buf_len<P::NXH>(), YS = slot_buf_len<Q>();
+  const int tid = threadIdx.x;
+  const int xs = tid / P::TX, xl = tid % P::TX, ys = tid / P::TYH, yl = tid % P::TYH;
+#pragma unroll
+  for (int cc = 0; cc < P::CPS; ++cc) {
+    const int col = xs + P::SBX * cc;
+    if (col < Q) {
+      cx<CC> zt[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const cx<CC> Wa = cvt<CC>(Wof(cc, e)), Wb = cvt<CC>(Wof(cc, e + 4));
+        cx<CC> zz = h ? csub(Wa, Wb) : cadd(Wa, Wb);
+        if (h) zz = twiddle<CC, NX, +1>(zz, xl + P::TX * e, tw);
+        zt[e] = zz;
+      }
+      slot_fft<CC, P::NXH, P::TX, +1, sizeof(CC) == 8>(zt, xraw + xs * XS * P::XBYTES, xl, tw);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) tb[(xl + P::TX * e) * QP + col] = cvt<S>(zt[e]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rr = 0; rr < P::RPH; ++rr) {
+    const int row = ys + P::SBYH * rr;
+    if (row < P::NXH) {
+      cx<CC> v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = cvt<CC>(tb[row * QP + yl + P::TYH * q]);
+      slot_fft<CC, Q, P::TYH, +1, sizeof(CC) == 8>(v, xraw + ys * YS * P::XBYTES, yl, tw);
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = yl + P::TYH * e;
+        cx<CC> b = v[e];
+        if (r) b = twiddle<CC, P::NY, +1>(b, m, tw);
+        out[split_idx(kz, r, 2 * row + h, m, P::MB, g.nzh)] = cvt<S>(b);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+template <typename S, typename CV, int NX, int Q, int NF>
+__global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<S>* __restrict__ A0, cx<S>* __restrict__ SF,
+                                                    const double* __restrict__ mfB, FastGeom g,
+                                                    const cx<double>* __restrict__ tw64,
+                                                    const cx<CV>* __restrict__ twv) {
+  using P = PlaneCfg<NX, Q>;
+  constexpr int CPS = P::CPS, NXP = NX + 1;
+  constexpr int NBT = (Q * NXP * static_cast<int>(sizeof(cx<S>)) > kPlaneTB) ? 2 : 1;
+  static_assert(Q * NXP * static_cast<int>(sizeof(cx<S>)) <= NBT * kPlaneTB, "transpose buffer too small");
+  static_assert((NX / 2) * (Q + 1) * static_cast<int>(sizeof(cx<S>)) <= kPlaneTB, "transpose buffer too small");
+  constexpr bool HS = NF >= 11;
+  constexpr bool MF = (NF == 12 || NF == 1);
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<S>* tb = reinterpret_cast<cx<S>*>(smem);
+  unsigned char* xraw = smem + kPlaneTB;
+  const int tid = threadIdx.x;
+  const int ys = tid / P::TYF, yl = tid % P::TYF, xs = tid / P::TX, xl = tid % P::TX;
+  const int unit = xcd_remap(blockIdx.x, gridDim.x);
+  const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+
+  // y- and x-FFT of the rho part (FP64); X[cc][e] = rho_hat(kx = xl + TX e, ky = r + 2 col)
+  cx<double> X[CPS][8];
+  {
+    cx<double> yv[P::RPF][8];
+#pragma unroll
+    for (int rr = 0; rr < P::RPF; ++rr) {
+      const int row = ys + P::SBYF * rr;
+      if (row < NX) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int mm = yl + P::TYF * q;
+          cx<double> a = cvt<double>(A0[split_idx(kz, r, row, mm, P::MB, g.nzh)]);
+          if (r) a = twiddle<double, P::NY, -1>(a, mm, tw64);
+          yv[rr][q] = a;
+        }
+        slot_fft<double, Q, P::TYF, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * P::XBYTES, yl, tw64);
+      }
+    }
+#pragma unroll
+    for (int bt = 0; bt < NBT; ++bt) {
+      const int s0 = bt * (Q / NBT), s1 = s0 + Q / NBT;
+      __syncthreads();
+#pragma unroll
+      for (int rr = 0; rr < P::RPF; ++rr) {
+        const int row = ys + P::SBYF * rr;
+        if (row < NX) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int s = yl + P::TYF * e;
+            if (s >= s0 && s < s1) tb[(s - s0) * NXP + row] = cvt<S>(yv[rr][e]);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int cc = 0; cc < CPS; ++cc) {
+        const int col = xs + P::SBX * cc;
+        if (col >= s0 && col < s1) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) X[cc][q] = cvt<double>(tb[(col - s0) * NXP + xl + P::TX * q]);
+          slot_fft<double, NX, P::TX, -1, true>(X[cc], xraw + xs * slot_buf_len<NX>() * P::XBYTES, xl, tw64);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  const double kzv = kz * g.dkz, invR = 1.0 / g.R;
+  auto kxof = [&](int e) { return kvec(xl + P::TX * e, NX, g.dkx); };
+  auto kyof = [&](int cc) {
+    const int col = xs + P::SBX * cc;
+    return kvec(r + 2 * (col < Q ? col : 0), P::NY, g.dky);
+  };
+  // FP64 channels first: eta and n2 (weights.hpp:174-191), the mean field (FP64 table)
+  if constexpr (HS) {
+#pragma unroll 1
+    for (int ch = 0; ch <= 1; ++ch) {
+      plane_fwd_channel<double, S, NX, Q>(
+          [&](int cc, int e) {
+            double kx = kxof(e);
+            // keeps the weights inside the channel loop: recomputing them is
+            // cheaper than holding 8 FP64 pairs per lane across it
+            asm volatile("" : "+v"(kx));
+            const double ky = kyof(cc);
+            double w3, w2;
+            radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3, &w2);
+            return cscale(X[cc][e], ch == 0 ? w3 : w2);
+          },
+          tb, xraw, SF + static_cast<long>(ch) * g.M, g, kz, r, h, tw64);
+    }
+  }
+  if constexpr (MF) {  // mean_field.hpp:145-196
+    const double* mrow0 = mfB + (static_cast<long>(kz) * 2 + r) * Q * NX;
+    plane_fwd_channel<double, S, NX, Q>(
+        [&](int cc, int e) {
+          const int col = xs + P::SBX * cc;
+          return cscale(X[cc][e], col < Q ? mrow0[static_cast<long>(col) * NX + xl + P::TX * e] : 0.0);
+        },
+        tb, xraw, SF + chan_store(NF, 11) * g.M, g, kz, r, h, tw64);
+  }
+  if constexpr (HS) {
+    // vector (i k_a w3) and tensor (aik2 k_i k_j + delta_ij w3/R) channels in CV,
+    // from the spectrum and per-element weight factors narrowed to CV (weights.hpp:195-222)
+    cx<CV> XC[CPS][8];
+    WFac<CV> wf[CPS][8];
+    CV kxc[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kxof(e));
+#pragma unroll
+    for (int cc = 0; cc < CPS; ++cc)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        XC[cc][e] = cvt<CV>(X[cc][e]);
+        wf[cc][e] = wfactors<CV>(kxof(e), kyof(cc), kzv, g);
+      }
+    const CV kzc = static_cast<CV>(kzv), irc = static_cast<CV>(invR);
+#pragma unroll 1
+    for (int ch = 2; ch <= 10; ++ch) {
+      plane_fwd_channel<CV, S, NX, Q>(
+          [&](int cc, int e) -> cx<CV> {
+            return cmul(chan_weight<CV>(ch, false, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc),
+                        XC[cc][e]);
+          },
+          tb, xraw, SF + static_cast<long>(ch) * g.M, g, kz, r, h, twv);
+    }
+  }
+}
+
+template <typename S, typename CV, int NX, int Q>
+__global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ P, FastGeom g,
+                                                    const cx<CV>* __restrict__ tw) {
+  using PC = PlaneCfg<NX, Q>;
+  constexpr int CPS = PC::CPS, NXH = PC::NXH;
+  constexpr int NXHP = NXH + 1, QP = Q + 1;
+  constexpr int NBR = (NX * QP * static_cast<int>(sizeof(cx<S>)) > kPlaneTB) ? 2 : 1;
+  constexpr int RPI = (NX / NBR + PC::SBYF - 1) / PC::SBYF;
+  static_assert(Q * NXHP * static_cast<int>(sizeof(cx<S>)) <= kPlaneTB, "transpose buffer too small");
+  static_assert(NX * QP * static_cast<int>(sizeof(cx<S>)) <= NBR * kPlaneTB, "transpose buffer too small");
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<S>* tb = reinterpret_cast<cx<S>*>(smem);
+  unsigned char* xraw = smem + kPlaneTB;
+  constexpr int SPL = sizeof(CV) == 8;  // FP64 exchanges one real part at a time
+  const int tid = threadIdx.x;
+  const int xs = tid / PC::TX, xl = tid % PC::TX;
+  const int ysh = tid / PC::TYH, ylh = tid % PC::TYH, ysf = tid / PC::TYF, ylf = tid % PC::TYF;
+  const int unit = xcd_remap(blockIdx.x, gridDim.x);
+  const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+  const double kzv = kz * g.dkz;
+  const CV invR = static_cast<CV>(1.0 / g.R);
+
+  // weight factors of the lane's accumulator elements (kx = xl + TX e, column xs + SBX cc),
+  // computed in FP64 and kept in the backward path's precision
+  WFac<CV> wf[CPS][8];
+  CV kxc[8];
+  cx<CV> acc[CPS][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kvec(xl + PC::TX * e, NX, g.dkx));
+#pragma unroll
+  for (int cc = 0; cc < CPS; ++cc) {
+    const int col = xs + PC::SBX * cc;
+    const double kyv = kvec(r + 2 * (col < Q ? col : 0), PC::NY, g.dky);
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      wf[cc][e] = wfactors<CV>(kvec(xl + PC::TX * e, NX, g.dkx), kyv, kzv, g);
+      acc[cc][e] = {CV(0), CV(0)};
+    }
+  }
+  const CV kzc = static_cast<CV>(kzv);
+#pragma unroll 1
+  for (int ch = 0; ch < 11; ++ch) {
+    const cx<S>* in = SB + static_cast<long>(ch) * g.M;
+    // y-FFT of the rows ix = 2i + h
+#pragma unroll
+    for (int rr = 0; rr < PC::RPH; ++rr) {
+      const int i = ysh + PC::SBYH * rr;
+      if (i < NXH) {
+        cx<CV> v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int mm = ylh + PC::TYH * q;
+          cx<CV> a = cvt<CV>(in[split_idx(kz, r, 2 * i + h, mm, PC::MB, g.nzh)]);
+          if (r) a = twiddle<CV, PC::NY, -1>(a, mm, tw);
+          v[q] = a;
+        }
+        slot_fft<CV, Q, PC::TYH, -1, SPL>(v, xraw + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, tw);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tb[(ylh + PC::TYH * e) * NXHP + i] = cvt<S>(v[e]);
+      }
+    }
+    __syncthreads();
+    // half x-FFT (DIT over the rows of parity h), conj-weighted accumulation over all kx
+#pragma unroll
+    for (int cc = 0; cc < CPS; ++cc) {
+      const int col = xs + PC::SBX * cc;
+      if (col < Q) {
+        const CV kyv = static_cast<CV>(kvec(r + 2 * col, PC::NY, g.dky));
+        cx<CV> f[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) f[q] = cvt<CV>(tb[col * NXHP + xl + PC::TX * q]);
+        slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xraw + xs * slot_buf_len<NX>() * PC::XBYTES, xl, tw);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int kxa = xl + PC::TX * e, kxb = kxa + NXH;
+          cx<CV> oa = f[e];
+          if (h) oa = twiddle<CV, NX, -1>(oa, kxa, tw);
+          const cx<CV> ob = h ? cneg(oa) : oa;
+          (void)kxb;
+          const cx<CV> wa = chan_weight<CV>(ch, true, kxc[e], kyv, kzc, wf[cc][e], invR);
+          const cx<CV> wb = chan_weight<CV>(ch, true, kxc[e + 4], kyv, kzc, wf[cc][e + 4], invR);
+          acc[cc][e] = cadd(acc[cc][e], cmul(wa, oa));
+          acc[cc][e + 4] = cadd(acc[cc][e + 4], cmul(wb, ob));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // x-inverse of the accumulated spectrum (all kx), then y-inverse of every row
+#pragma unroll
+  for (int cc = 0; cc < CPS; ++cc) {
+    const int col = xs + PC::SBX * cc;
+    if (col < Q) slot_fft<CV, NX, PC::TX, +1, SPL>(acc[cc], xraw + xs * slot_buf_len<NX>() * PC::XBYTES, xl, tw);
+  }
+  cx<S>* out = P + static_cast<long>(h) * g.M;
+#pragma unroll
+  for (int bt = 0; bt < NBR; ++bt) {
+    const int i0 = bt * (NX / NBR), i1 = i0 + NX / NBR;
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < CPS; ++cc) {
+      const int col = xs + PC::SBX * cc;
+      if (col < Q) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int ixx = xl + PC::TX * e;
+          if (ixx >= i0 && ixx < i1) tb[(ixx - i0) * QP + col] = cvt<S>(acc[cc][e]);
+        }
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < RPI; ++rr) {
+      const int row = ysf + PC::SBYF * rr;
+      if (row < NX / NBR) {
+        cx<CV> v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = cvt<CV>(tb[row * QP + ylf + PC::TYF * q]);
+        slot_fft<CV, Q, PC::TYF, +1, SPL>(v, xraw + ysf * slot_buf_len<Q>() * PC::XBYTES, ylf, tw);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int mm = ylf + PC::TYF * e;
+          cx<CV> b = v[e];
+          if (r) b = twiddle<CV, PC::NY, +1>(b, mm, tw);
+          out[split_idx(kz, r, i0 + row, mm, PC::MB, g.nzh)] = cvt<S>(b);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace
+
+// ============================================================ host side
+bool fast_supported(int nx, int ny, int nz) {
+  auto ok = [](int n) { return n == 64 || n == 128; };
+  return ok(nx) && ok(ny) && ok(nz);
+}
+
+FastGeom make_fast_geom(const Geom& g) {
+  FastGeom f{};
+  f.nx = g.nx;
+  f.ny = g.ny;
+  f.nz = g.nz;
+  f.nzh = g.nzh;
+  f.q = g.ny / 2;
+  f.mb = f.q / 16;
+  f.N = g.N;
+  f.M = g.M;
+  f.dkx = g.dkx;
+  f.dky = g.dky;
+  f.dkz = g.dkz;
+  f.dv = g.dv;
+  f.inv_n = g.inv_n;
+  f.R = g.R;
+  return f;
+}
+
+void fast_twiddles(void** tw64, void** tw32) {
+  const double PI = 3.14159265358979323846;
+  std::vector<double> t64(2 * kTwLen);
+  std::vector<float> t32(4 * kTwLen);
+  for (int m = 0; m < kTwLen; ++m) {
+    const double a = -2.0 * PI * m / kTwLen;
+    t64[2 * m] = std::cos(a);
+    t64[2 * m + 1] = std::sin(a);
+    for (int q = 0; q < 2; ++q) {
+      const float hi = static_cast<float>(t64[2 * m + q]);
+      t32[2 * m + q] = hi;
+      t32[2 * kTwLen + 2 * m + q] = static_cast<float>(t64[2 * m + q] - static_cast<double>(hi));
+    }
+  }
+  if (hipMalloc(tw64, t64.size() * sizeof(double)) != hipSuccess ||
+      hipMalloc(tw32, t32.size() * sizeof(float)) != hipSuccess)
+    throw std::runtime_error("hipMalloc of the twiddle tables failed");
+  if (hipMemcpy(*tw64, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(*tw32, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    throw std::runtime_error("hipMemcpy of the twiddle tables failed");
+}
+
+void fast_mf_layout(const double* pm, double* out, const Geom& g) {
+  const int q = g.ny / 2;
+  for (int kz = 0; kz < g.nzh; ++kz)
+    for (int r = 0; r < 2; ++r)
+      for (int s = 0; s < q; ++s)
+        for (int kx = 0; kx < g.nx; ++kx) {
+          const int ky = r + 2 * s;
+          out[((static_cast<long>(kz) * 2 + r) * q + s) * g.nx + kx] =
+              pm[static_cast<long>(kz) * g.nx * g.ny + static_cast<long>(kx) * g.ny + ky];
+        }
+}
+
+int fast_point_blocks(const FastGeom& g) { return g.nx * g.mb * 4; }
+
+namespace {
+template <typename T>
+const T* as(const void* p) {
+  return static_cast<const T*>(p);
+}
+// Dynamic LDS above 64 KiB has to be enabled per kernel (once per process).
+template <typename K>
+void set_smem(K kernel, int bytes) {
+  static bool done = false;
+  if (done) return;
+  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+      hipSuccess)
+    throw std::runtime_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+  done = true;
+}
+}  // namespace
+
+template <typename S>
+void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const void* tw64, AliasMap amap,
+                      const double* shift_sum, hipStream_t st) {
+  const unsigned nblk = static_cast<unsigned>(g.nx * g.mb);
+#define CDFT_L(NZ)                                                                                     \
+  {                                                                                                    \
+    auto k = kf_zfwd<S, NZ>;                                                                           \
+    const int sm = zfwd_smem<NZ, S>();                                                                 \
+    set_smem(k, sm);                                                                                   \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(2 * NZ), sm, st, rho, out, g,                               \
+                       as<cx<double>>(tw64), amap, shift_sum);                                         \
+  }
+  if (g.nz == 64) CDFT_L(64) else if (g.nz == 128) CDFT_L(128) else throw std::runtime_error("fast path: nz");
+#undef CDFT_L
+}
+
+template <typename S, typename CV>
+void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
+                      const void* twv, hipStream_t st) {
+  const unsigned nblk = static_cast<unsigned>(g.nx * g.mb);
+#define CDFT_L(NZ)                                                                                      \
+  {                                                                                                     \
+    auto k = kf_zinv<S, CV, NZ>;                                                                        \
+    const int sm = zinv_smem<NZ, CV>();                                                                 \
+    set_smem(k, sm);                                                                                    \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(2 * NZ), sm, st, P, out, add, scale, g,                      \
+                       as<cx<CV>>(twv));                                                                \
+  }
+  if (g.nz == 64) CDFT_L(64) else if (g.nz == 128) CDFT_L(128) else throw std::runtime_error("fast path: nz");
+#undef CDFT_L
+}
+
+template <typename S, typename CV>
+void launch_fast_plane_fwd(const cx<S>* A0, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
+                           const void* twv, ChannelSet cs, hipStream_t st) {
+  const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
+#define CDFT_L(NX, Q, NF)                                                                             \
+  {                                                                                                   \
+    auto k = kf_plane_fwd<S, CV, NX, Q, NF>;                                                          \
+    const int sm = plane_smem<NX, Q>();                                                               \
+    set_smem(k, sm);                                                                                  \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, A0, SF, mfB, g,                              \
+                       as<cx<double>>(tw64), as<cx<CV>>(twv));                                        \
+  }
+#define CDFT_NF(NX, Q)                                                                                \
+  if (cs.nf == 12) CDFT_L(NX, Q, 12) else if (cs.nf == 11) CDFT_L(NX, Q, 11) else if (cs.nf == 1) CDFT_L(NX, Q, 1) \
+  else throw std::runtime_error("fast path: channel set");
+  if (g.nx == 64 && g.q == 32) CDFT_NF(64, 32)
+  else if (g.nx == 64 && g.q == 64) CDFT_NF(64, 64)
+  else if (g.nx == 128 && g.q == 32) CDFT_NF(128, 32)
+  else if (g.nx == 128 && g.q == 64) CDFT_NF(128, 64)
+  else throw std::runtime_error("fast path: plane shape");
+#undef CDFT_NF
+#undef CDFT_L
+}
+
+template <typename S, typename CV>
+void launch_fast_plane_bwd(const cx<S>* SB, cx<S>* P, const FastGeom& g, const void* /*tw64*/, const void* twv,
+                           hipStream_t st) {
+  const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
+#define CDFT_L(NX, Q)                                                                                 \
+  {                                                                                                   \
+    auto k = kf_plane_bwd<S, CV, NX, Q>;                                                              \
+    const int sm = plane_smem<NX, Q>();                                                               \
+    set_smem(k, sm);                                                                                  \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, SB, P, g, as<cx<CV>>(twv));                  \
+  }
+  if (g.nx == 64 && g.q == 32) CDFT_L(64, 32)
+  else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
+  else if (g.nx == 128 && g.q == 32) CDFT_L(128, 32)
+  else if (g.nx == 128 && g.q == 64) CDFT_L(128, 64)
+  else throw std::runtime_error("fast path: plane shape");
+#undef CDFT_L
+}
+
+template <typename S, typename CV>
+void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double* lf, double* partials,
+                        const FastGeom& g, const void* tw64, const void* twv, ChannelSet cs, FmtParams fp, double mu,
+                        AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
+  const unsigned nblk = static_cast<unsigned>(fast_point_blocks(g));
+#define CDFT_L(NZ, NF, NB)                                                                             \
+  {                                                                                                    \
+    auto k = kf_zpoint<S, CV, NZ, NF, NB>;                                                             \
+    const int sm = zpoint_smem<NZ, S>();                                                               \
+    set_smem(k, sm);                                                                                   \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kZP), sm, st, SF, rho, vext, lf, partials, g,               \
+                       as<cx<double>>(tw64), as<cx<CV>>(twv), fp, mu, amap, shift_sum, dc);            \
+  }
+#define CDFT_CH(NZ)                                                                                    \
+  if (cs.nf == 12 && cs.nb == 11) CDFT_L(NZ, 12, 11)                                                   \
+  else if (cs.nf == 12 && cs.nb == 0) CDFT_L(NZ, 12, 0)                                                \
+  else if (cs.nf == 11 && cs.nb == 11) CDFT_L(NZ, 11, 11)                                              \
+  else if (cs.nf == 11 && cs.nb == 0) CDFT_L(NZ, 11, 0)                                                \
+  else if (cs.nf == 1 && cs.nb == 0) CDFT_L(NZ, 1, 0)                                                  \
+  else if (cs.nf == 0 && cs.nb == 0) CDFT_L(NZ, 0, 0)                                                  \
+  else throw std::runtime_error("fast path: channel set");
+  if (g.nz == 64) CDFT_CH(64) else if (g.nz == 128) CDFT_CH(128) else throw std::runtime_error("fast path: nz");
+#undef CDFT_CH
+#undef CDFT_L
+}
+
+#define CDFT_INST(S, CV)                                                                                         \
+  template void launch_fast_zfwd<S>(const double*, cx<S>*, const FastGeom&, const void*, AliasMap, const double*, \
+                                    hipStream_t);                                                                \
+  template void launch_fast_plane_fwd<S, CV>(const cx<S>*, cx<S>*, const double*, const FastGeom&, const void*,   \
+                                             const void*, ChannelSet, hipStream_t);                              \
+  template void launch_fast_zpoint<S, CV>(cx<S>*, const double*, const double*, double*, double*,                 \
+                                          const FastGeom&, const void*, const void*, ChannelSet, FmtParams,       \
+                                          double, AliasMap, const double*, DcTable, hipStream_t);                 \
+  template void launch_fast_plane_bwd<S, CV>(const cx<S>*, cx<S>*, const FastGeom&, const void*, const void*,     \
+                                             hipStream_t);                                                       \
+  template void launch_fast_zinv<S, CV>(const cx<S>*, double*, const double*, double, const FastGeom&,           \
+                                        const void*, hipStream_t);
+CDFT_INST(float, float)
+CDFT_INST(double, double)
+#undef CDFT_INST
+
+}  // namespace cdft

@@ -1,0 +1,57 @@
+// Fused five-kernel evaluation schedule for power-of-two grids.
+//
+// Spectral data of every field lives in a "chunked split" layout.  The y axis
+// is split by parity (radix-2 decimation done inside the z-line kernels):
+// for a z-transformed field x~(ix, iy, kz) the two parts are
+//   a_r(kz, ix, m) = x~(ix, m, kz) + (-1)^r x~(ix, m + Q, kz),  Q = Ny/2,
+// and element (kz, r, ix, m) is stored at
+//   ((ix * (Q/16) + m/16) * (Nz/2+1) + kz) * 32 + r * 16 + m % 16,
+// so the 16 consecutive m of one (kz, r) share a 128-byte segment and the
+// data of 16 line pairs (m .. m+15 and m+Q .. m+Q+15 at one ix) form one
+// contiguous chunk.  A plane kernel owns one (kz, r) half-plane, the z-line
+// kernels own chunks; both read and write whole segments.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace cdft {
+
+struct FastGeom {
+  int nx, ny, nz, nzh, q, mb;  // q = ny/2, mb = q/16
+  long N, M;
+  double dkx, dky, dkz, dv, inv_n, R;
+};
+
+// True when the fused schedule has kernels for this grid.
+bool fast_supported(int nx, int ny, int nz);
+FastGeom make_fast_geom(const Geom& g);
+
+// FP64 and FP32 (value, residual) twiddle tables of length kTwLen.
+void fast_twiddles(void** tw64, void** tw32);
+
+// Mean-field weight (real FP64 values) from plane-major order
+// [kz][ix][iy] into the plane kernel's order [kz][r][s][kx] (ky = r + 2 s).
+void fast_mf_layout(const double* plane_major, double* out, const Geom& g);
+
+int fast_point_blocks(const FastGeom& g);
+
+template <typename S>
+void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const void* tw64, AliasMap amap,
+                      const double* shift_sum, hipStream_t st);
+template <typename S, typename CV>
+void launch_fast_plane_fwd(const cx<S>* A0, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
+                           const void* twv, ChannelSet cs, hipStream_t st);
+template <typename S, typename CV>
+void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double* lf, double* partials,
+                        const FastGeom& g, const void* tw64, const void* twv, ChannelSet cs, FmtParams fp, double mu,
+                        AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st);
+template <typename S, typename CV>
+void launch_fast_plane_bwd(const cx<S>* SB, cx<S>* P, const FastGeom& g, const void* tw64, const void* twv,
+                           hipStream_t st);
+template <typename S, typename CV>
+void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
+                      const void* twv, hipStream_t st);
+
+}  // namespace cdft

@@ -35,6 +35,7 @@
 #include <cmath>
 #include <stdexcept>
 
+#include "device_common.hpp"
 #include "fft_lds.hpp"
 #include "fmt_point.hpp"
 #include "kernels.hpp"
@@ -42,63 +43,6 @@
 namespace cdft {
 
 constexpr int NT = 256;  // threads per block of every transform kernel
-
-// ---------------------------------------------------------------- helpers
-__device__ __forceinline__ void two_sum(double& s, double& c, double x) {
-  // Neumaier compensated accumulation of x into (s, c)
-  const double t = s + x;
-  c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
-  s = t;
-}
-
-template <typename D, typename T>
-__device__ __forceinline__ cx<D> cvt(cx<T> a) {
-  return {static_cast<D>(a.re), static_cast<D>(a.im)};
-}
-
-__device__ __forceinline__ double kvec(int i, int n, double dk) { return dk * static_cast<double>(i <= n / 2 ? i : i - n); }
-
-// Radial FMT functions w3(k) = 4 pi (sin kR - kR cos kR)/k^3, w2(k) = 4 pi R sin(kR)/k
-// (weights.hpp:174-191), each times 1/N.  The vector channel prefactor of
-// weights.hpp:195-201 equals w3 and the tensor coefficients of
-// weights.hpp:206-222 are A = w2 - 3 w3/R, B = w3/R (identities of the closed forms).
-__device__ __forceinline__ void radial(double k, double R, double inv_n, double* w3, double* w2) {
-  const double PI = 3.14159265358979323846;
-  if (k < 1e-12) {
-    *w3 = (4.0 * PI / 3.0) * R * R * R * inv_n;
-    *w2 = 4.0 * PI * R * R * inv_n;
-    return;
-  }
-  const double x = k * R;
-  double s, c;
-  sincos(x, &s, &c);
-  *w3 = 4.0 * PI * (s - x * c) / (k * k * k) * inv_n;
-  *w2 = 4.0 * PI * R * s / k * inv_n;
-}
-
-// Weight of channel ch at k (FP64): ch 0 w3, 1 w2, 2..4 i k_a w3 (conj when
-// `adj`), 5..10 tensor xx,xy,xz,yy,yz,zz, 11 the mean-field table value.
-__device__ __forceinline__ cx<double> channel_weight(int ch, bool adj, double kx, double ky, double kz, double w3,
-                                                     double w2, double mf, double invR) {
-  if (ch == 0) return {w3, 0.0};
-  if (ch == 1) return {w2, 0.0};
-  if (ch <= 4) {
-    const double ka = ch == 2 ? kx : (ch == 3 ? ky : kz);
-    return {0.0, adj ? -ka * w3 : ka * w3};
-  }
-  if (ch <= 10) {
-    const double k2 = kx * kx + ky * ky + kz * kz;
-    const double ik2 = k2 > 0.0 ? 1.0 / k2 : 0.0;
-    const double A = w2 - 3.0 * w3 * invR;
-    const double B = w3 * invR;
-    const int q = ch - 5;
-    const double ki = q < 3 ? kx : (q < 5 ? ky : kz);
-    const double kj = (q == 0) ? kx : (q == 1 || q == 3) ? ky : kz;
-    const bool diag = (q == 0 || q == 3 || q == 5);
-    return {A * ki * kj * ik2 + (diag ? B : 0.0), 0.0};
-  }
-  return {mf, 0.0};
-}
 
 // v * w for an FP64 weight: in FP32 the weight is applied as the pair
 // (FP32 value, FP32 residual), like the FFT constants.
@@ -423,32 +367,6 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
 // forward channels (12: FMT + mean field, 11: FMT, 1: mean field, 0) and NB
 // backward channels (11 or 0) are compile-time so the per-point values stay
 // in registers.
-
-// Wave-then-block sum of NV doubles; thread 0 of the block gets the totals in out[].
-template <int NV>
-__device__ __forceinline__ void block_reduce(double (&v)[NV], double* out) {
-  __shared__ double wsum[16][NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    double x = v[i];
-    for (int off = warpSize / 2; off > 0; off >>= 1) x += __shfl_xor(x, off);
-    v[i] = x;
-  }
-  const int wave = threadIdx.x / warpSize, lane = threadIdx.x % warpSize;
-  if (lane == 0)
-#pragma unroll
-    for (int i = 0; i < NV; ++i) wsum[wave][i] = v[i];
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int nw = (blockDim.x + warpSize - 1) / warpSize;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double t = 0.0;
-      for (int w = 0; w < nw; ++w) t += wsum[w][i];
-      out[i] = t;
-    }
-  }
-}
 
 constexpr int ZP_G = 4;  // channels per batched z transform in k_zpoint
 

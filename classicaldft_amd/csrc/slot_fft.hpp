@@ -1,0 +1,180 @@
+// Register-resident line FFTs for the fused evaluation kernels (gfx950).
+//
+// A "slot" is a group of T consecutive lanes of one wavefront (T divides 64)
+// that transforms one line of compile-time length L.  Every lane holds
+// E = L / T elements in registers:
+//   on entry  v[q] = x[lane + T q]   (q < E)
+//   on exit   v[e] = X[lane + T e]   (e < E)
+// The transform is a Stockham autosort FFT whose first pass has radix E (its
+// inputs are exactly the lane's registers) and whose last pass writes its
+// outputs straight back to registers in the order above; the passes between
+// exchange data through the slot's private, padded LDS buffer.  Lanes of a
+// slot live in one wavefront, so the exchanges need only wavefront-level
+// ordering, never a work-group barrier: waves of a block run their
+// transforms independently and overlap one another's memory traffic.
+//
+// sign SG = -1 is the forward transform exp(-i k x), +1 the backward one;
+// both are unnormalised (src/math/fourier.cpp:41-57 of the reference).
+// Twiddles of every power-of-two length L <= kTwLen come from one table
+// tw[m] = exp(-2 pi i m / kTwLen) (FP64), or its FP32 pairs (values in
+// [0, kTwLen), residuals in [kTwLen, 2 kTwLen)), read with stride kTwLen / L.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "fft_lds.hpp"
+
+namespace cdft {
+
+constexpr int kTwLen = 512;
+
+// Orders LDS traffic of the lanes of one wavefront (no s_barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Padded position inside a slot buffer: one spare element every 16 spreads
+// the strided writes of the early passes over the LDS banks.
+__host__ __device__ constexpr int xpad(int x) { return x + (x >> 4); }
+template <int L>
+__host__ __device__ constexpr int slot_buf_len() {
+  return xpad(L - 1) + 1;
+}
+
+// Radix sequence of an (L, T) slot: radix E first, then the largest of 8, 4,
+// 2 that divides both E and the remaining length.
+template <int L, int T>
+struct SlotPlan {
+  static_assert(L % T == 0, "slot width must divide the line length");
+  static constexpr int E = L / T;
+  static_assert(E == 2 || E == 4 || E == 8, "2, 4 or 8 elements per lane");
+  static_assert((L & (L - 1)) == 0, "power-of-two line length");
+  static constexpr int rad(int p) {
+    int rem = L;
+    for (int i = 0;; ++i) {
+      int r = E;
+      if (i > 0) {
+        r = 8;
+        while (r > 1 && ((E % r) != 0 || (rem % r) != 0)) r >>= 1;
+      }
+      if (i == p) return r;
+      rem /= r;
+    }
+  }
+  static constexpr int npass() {
+    int rem = L, n = 0;
+    while (rem > 1) {
+      rem /= rad(n);
+      ++n;
+    }
+    return n;
+  }
+};
+
+// v * exp(SG 2 pi i m / L) from the shared table.
+template <typename C, int L, int SG>
+__device__ __forceinline__ cx<C> twiddle(cx<C> v, int m, const cx<C>* __restrict__ tw) {
+  static_assert(kTwLen % L == 0, "line length exceeds the twiddle table");
+  cx<C> w = tw[m * (kTwLen / L)];
+  if (SG > 0) w.im = -w.im;
+  if constexpr (sizeof(C) == 4) {
+    cx<C> wl = tw[kTwLen + m * (kTwLen / L)];
+    if (SG > 0) wl.im = -wl.im;
+    return cadd(cmul(v, w), cmul(v, wl));
+  } else {
+    return cmul(v, w);
+  }
+}
+
+// Exchange element of a slot buffer: complex values, or with SPLIT one real
+// part at a time (an FP64 exchange then needs 8 bytes per element, not 16).
+template <typename C, bool SPLIT>
+struct SlotXchg {
+  static constexpr int bytes = SPLIT ? static_cast<int>(sizeof(C)) : static_cast<int>(sizeof(cx<C>));
+};
+template <typename C, int L, bool SPLIT>
+__host__ __device__ constexpr int slot_buf_bytes() {
+  return slot_buf_len<L>() * SlotXchg<C, SPLIT>::bytes;
+}
+
+template <typename C, int L, int T, int SG, bool SPLIT, int P, int NS>
+struct SlotPass {
+  using Plan = SlotPlan<L, T>;
+  static constexpr int E = Plan::E;
+  static constexpr int R = Plan::rad(P);
+  static constexpr int NB = E / R;  // butterflies per lane
+  static constexpr int NP = Plan::npass();
+  static_assert(E % R == 0, "radix must divide the elements per lane");
+
+  // u holds the inputs of this pass: butterfly b = lane + T i reads x[b + (L/R) q]
+  __device__ __forceinline__ static void run(cx<C> (&v)[E], cx<C> (&u)[NB][R], void* xb, int lane,
+                                             const cx<C>* __restrict__ tw) {
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      if constexpr (NS > 1) {
+        const int k = (lane + T * i) & (NS - 1);
+#pragma unroll
+        for (int q = 1; q < R; ++q) u[i][q] = twiddle<C, L, SG>(u[i][q], q * k * (L / (NS * R)), tw);
+      }
+      dft_r<C, R>(u[i], SG);
+    }
+    if constexpr (P == NP - 1) {
+      // outputs of butterfly b at b + NS q = lane + T (i + NB q)
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+#pragma unroll
+        for (int q = 0; q < R; ++q) v[i + NB * q] = u[i][q];
+    } else {
+      using Next = SlotPass<C, L, T, SG, SPLIT, P + 1, NS * R>;
+      cx<C> w[Next::NB][Next::R];
+      // outputs of butterfly b at (b/NS) NS R + b%NS + NS q; inputs of the next pass at b' + (L/R') q'
+      auto put = [&](auto get, auto* buf) {
+        wave_sync();  // every lane is done with the buffer's previous contents
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+          const int b = lane + T * i;
+          const int base = (b / NS) * NS * R + (b & (NS - 1));
+#pragma unroll
+          for (int q = 0; q < R; ++q) buf[xpad(base + NS * q)] = get(u[i][q]);
+        }
+        wave_sync();
+      };
+      if constexpr (SPLIT) {
+        C* xr = static_cast<C*>(xb);
+        put([](cx<C> a) { return a.re; }, xr);
+#pragma unroll
+        for (int i = 0; i < Next::NB; ++i)
+#pragma unroll
+          for (int q = 0; q < Next::R; ++q) w[i][q].re = xr[xpad(lane + T * i + (L / Next::R) * q)];
+        put([](cx<C> a) { return a.im; }, xr);
+#pragma unroll
+        for (int i = 0; i < Next::NB; ++i)
+#pragma unroll
+          for (int q = 0; q < Next::R; ++q) w[i][q].im = xr[xpad(lane + T * i + (L / Next::R) * q)];
+      } else {
+        cx<C>* xc = static_cast<cx<C>*>(xb);
+        put([](cx<C> a) { return a; }, xc);
+#pragma unroll
+        for (int i = 0; i < Next::NB; ++i)
+#pragma unroll
+          for (int q = 0; q < Next::R; ++q) w[i][q] = xc[xpad(lane + T * i + (L / Next::R) * q)];
+      }
+      Next::run(v, w, xb, lane, tw);
+    }
+  }
+};
+
+// One line transform by one slot; `xb` is the slot's buffer of
+// slot_buf_bytes<C, L, SPLIT>() bytes.  All lanes of the slot call it together.
+template <typename C, int L, int T, int SG, bool SPLIT = false>
+__device__ __forceinline__ void slot_fft(cx<C> (&v)[L / T], void* xb, int lane, const cx<C>* __restrict__ tw) {
+  constexpr int E = L / T;
+  cx<C> u[1][E];  // first pass: radix E, butterfly b = lane reads x[lane + T q] = v[q]
+#pragma unroll
+  for (int q = 0; q < E; ++q) u[0][q] = v[q];
+  SlotPass<C, L, T, SG, SPLIT, 0, 1>::run(v, u, xb, lane, tw);
+}
+
+}  // namespace cdft

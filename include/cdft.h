@@ -68,11 +68,20 @@ typedef struct {
 #define CDFT_PRECISION_FP64 0
 #define CDFT_PRECISION_MIXED 1
 
+/* Kernel schedule of an evaluation.  AUTO picks the fused five-kernel
+ * schedule when the grid has kernels for it (every axis 64 or 128) and the
+ * general per-axis schedule otherwise; GENERAL forces the latter.  Both give
+ * the same results to rounding. */
+#define CDFT_SCHEDULE_AUTO 0
+#define CDFT_SCHEDULE_GENERAL 1
+#define CDFT_SCHEDULE_FUSED 2
+
 typedef struct {
   int fmt_model;          /* fmt::FMTModel (include/dft/functionals/fmt/models.hpp:363-419) */
   double esfmt_A, esfmt_B; /* esFMT(A, B), models.hpp:314-358 */
   int precision;          /* CDFT_PRECISION_FP64 or CDFT_PRECISION_MIXED */
   int device;             /* HIP device ordinal */
+  int schedule;           /* CDFT_SCHEDULE_AUTO or CDFT_SCHEDULE_GENERAL */
 } cdft_options;
 
 /* Replaces functionals::make_weights + make_functional
@@ -86,6 +95,10 @@ CDFT_API const char* cdft_last_error(void);
 
 /* Grid::shape (grid.hpp:26-33). */
 CDFT_API int cdft_grid_shape(const cdft_ctx* ctx, long shape[3]);
+/* The schedule the context evaluates with: CDFT_SCHEDULE_FUSED or
+ * CDFT_SCHEDULE_GENERAL (no counterpart in the reference, which has one
+ * CPU schedule). */
+CDFT_API int cdft_schedule(const cdft_ctx* ctx, int* schedule);
 /* InteractionWeight::a_vdw of interaction k (mean_field.hpp:23-28). */
 CDFT_API int cdft_a_vdw(const cdft_ctx* ctx, int k, double* a_vdw);
 /* BulkThermodynamics::chemical_potential for a one-component fluid with the
@@ -127,7 +140,7 @@ CDFT_API int cdft_time_total_device(cdft_ctx* ctx, const double* d_rho, const do
                                     double* d_forces, double* ms);
 CDFT_API int cdft_profile_stages(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu,
                                  double* d_forces, int reps, double* stage_ms, double* stage_bytes, int* n_stages);
-CDFT_API const char* cdft_stage_name(int i);
+CDFT_API const char* cdft_stage_name(const cdft_ctx* ctx, int i);
 
 /* hard_sphere.hpp:50-67 (convolve_weights) and mean_field.hpp:259: the 12
  * weighted-density fields eta, n2, n_v2[x,y,z], T[xx,xy,xz,yy,yz,zz] and the
