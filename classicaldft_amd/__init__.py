@@ -12,7 +12,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcdft.so")
+# The in-tree build; CDFT_LIBRARY names another build of the same sources.
+LIB_PATH = os.environ.get("CDFT_LIBRARY") or os.path.join(HERE, "libcdft.so")
 
 FMT_MODELS = {"none": -1, "Rosenfeld": 0, "RSLT": 1, "WhiteBearI": 2, "WhiteBearII": 3, "esFMT": 4}
 POTENTIALS = {"LJ": 0, "tWF": 1, "WRDF": 2}

@@ -59,7 +59,7 @@ Context::~Context() {
   clear_graphs();
   void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
                   d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
-                  ftw64,   ftw32};
+                  ftw64,   ftw32,   d_XR};
   hipSetDevice(device);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -189,6 +189,7 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       c->nblk_point = fast_point_blocks(c->fg);
       fast_twiddles(&c->ftw64, &c->ftw32);
       c->d_P = dalloc_bytes(cs_bytes * g.M * 2);
+      c->d_XR = dalloc_bytes(16 * g.M);
     }
     c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
     c->d_res = dalloc<double>(8);
@@ -268,8 +269,8 @@ const char* const kStageNames[kNumStages] = {"k_sum",    "k_zr2c",            "k
                                               "k_lines(y inv nf)", "k_zpoint", "k_lines(y fwd nb)", "k_xbwd",
                                               "k_lines(y inv 1)",  "k_zc2r",   "k_finalize"};
 
-const char* const kFastStageNames[kNumFastStages] = {"k_sum",        "kf_zfwd", "kf_plane_fwd", "kf_zpoint",
-                                                     "kf_plane_bwd", "kf_zinv", "k_finalize"};
+const char* const kFastStageNames[kNumFastStages] = {"k_sum",     "kf_zfwd",      "kf_rho_xy", "kf_plane_fwd",
+                                                     "kf_zpoint", "kf_plane_bwd", "kf_zinv",   "k_finalize"};
 
 template <typename S, typename CV>
 void Context::eval_fast_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
@@ -287,23 +288,25 @@ void Context::eval_fast_t(const double* rho, const double* vext, double mu, doub
     mark(1);
     launch_fast_zfwd<S>(rho, A0, fg, ftw64, amap, d_shift, stream);
     mark(2);
-    launch_fast_plane_fwd<S, CV>(A0, SF, d_mfB, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cs, stream);
+    launch_fast_rho_xy<S>(A0, d_XR, fg, ftw64, stream);
+    mark(3);
+    launch_fast_plane_fwd<S, CV>(d_XR, SF, d_mfB, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cs, stream);
   }
   ChannelSet cz = cs;
   if (!backward) cz.nb = 0;
   double* lf = backward ? d_local : (force ? force : d_local);
-  mark(3);
+  mark(4);
   launch_fast_zpoint<S, CV>(SF, rho, vext, lf, d_partials, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cz, fp, mu,
                             amap, cs.nf > 0 ? d_shift : nullptr, dc, stream);
   if (backward) {
-    mark(4);
-    launch_fast_plane_bwd<S, CV>(SF, P, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
     mark(5);
+    launch_fast_plane_bwd<S, CV>(SF, P, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
+    mark(6);
     launch_fast_zinv<S, CV>(P, force, d_local, g.dv, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
   }
-  mark(6);
-  launch_finalize(d_partials, nblk_point, d_res, stream);
   mark(7);
+  launch_finalize(d_partials, nblk_point, d_res, stream);
+  mark(8);
 }
 
 template <typename S, typename CE, typename CL>
@@ -362,12 +365,13 @@ void Context::stage_bytes(double* b, bool with_vext) const {
     if (cs.nf > 0) {
       b[0] = 8.0 * N;                                  // FP64 sum of rho
       b[1] = 8.0 * N + sf * M;                         // rho -> rho_hat parts
-      b[2] = sf * M + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // rho_hat (+ FP64 table) -> nf channels
+      b[2] = sf * M + 16.0 * M;                        // parts -> FP64 spectrum
+      b[3] = 16.0 * M + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // spectrum (+ FP64 table) -> nf channels
     }
-    b[3] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;
+    b[4] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;
     if (cs.nb > 0) {
-      b[4] = nb * sf * M + 2.0 * sf * M;               // nb channels -> two partial sums
-      b[5] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
+      b[5] = nb * sf * M + 2.0 * sf * M;               // nb channels -> two partial sums
+      b[6] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
     }
     return;
   }

@@ -38,7 +38,7 @@ struct InteractionSpec {
 
 constexpr int kNumStages = 11;  // stages of the general schedule (the most of any path)
 extern const char* const kStageNames[kNumStages];
-constexpr int kNumFastStages = 7;
+constexpr int kNumFastStages = 8;
 extern const char* const kFastStageNames[kNumFastStages];
 
 // Results of one evaluation, in the order the fused kernel accumulates them.
@@ -84,6 +84,7 @@ struct Context {
   void* ftw64 = nullptr;      // shared twiddle tables of the fused kernels
   void* ftw32 = nullptr;
   void* d_P = nullptr;        // 2 * M complex: the two partial sums of the backward plane kernel
+  void* d_XR = nullptr;       // M complex FP64: rho_hat for the forward plane kernel
   double* d_mfB = nullptr;    // M: mean-field weight in the plane kernel's order
   int nstages() const { return fast ? kNumFastStages : kNumStages; }
   const char* stage_name(int i) const {

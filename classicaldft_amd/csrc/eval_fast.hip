@@ -79,11 +79,30 @@ __device__ __forceinline__ cx<C> cneg(cx<C> a) {
   return {-a.re, -a.im};
 }
 
+// Bytes of a twiddle table in LDS (FP64: kTwLen values; FP32: values and residuals).
+template <typename C>
+__host__ __device__ constexpr int tw_bytes() {
+  return (sizeof(C) == 4 ? 2 : 1) * kTwLen * static_cast<int>(sizeof(cx<C>));
+}
+// Copies a twiddle table into LDS (the caller synchronises before first use):
+// twiddle reads then cost an LDS access instead of an L1/L2 round trip.
+template <typename C>
+__device__ __forceinline__ cx<C>* stage_twiddles(unsigned char* dst, const cx<C>* __restrict__ src) {
+  cx<C>* d = reinterpret_cast<cx<C>*>(dst);
+  constexpr int n = (sizeof(C) == 4 ? 2 : 1) * kTwLen;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = src[i];
+  return d;
+}
+
 // ============================================================ kf_zfwd
 template <int NZ, typename S>
-__host__ __device__ constexpr int zfwd_smem() {
+__host__ __device__ constexpr int zfwd_tw_off() {
   return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<S>))) +
          16 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
+}
+template <int NZ, typename S>
+__host__ __device__ constexpr int zfwd_smem() {
+  return zfwd_tw_off<NZ, S>() + tw_bytes<double>();
 }
 
 template <typename S, int NZ>
@@ -100,14 +119,16 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
   const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
   const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+  const cx<double>* twl = stage_twiddles<double>(smem + zfwd_tw_off<NZ, S>(), tw);
   cx<double> v[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int z = lane + T * q;
     v[q] = {amap.apply(rho[la * NZ + z]) - shift, amap.apply(rho[lb * NZ + z]) - shift};
   }
+  __syncthreads();
   cx<double>* xb = xbuf + slot * XS;
-  slot_fft<double, NZ, T, -1>(v, xb, lane, tw);
+  slot_fft<double, NZ, T, -1>(v, xb, lane, twl);
   wave_sync();
 #pragma unroll
   for (int e = 0; e < 8; ++e) xb[xpad(lane + T * e)] = v[e];
@@ -131,9 +152,13 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
 
 // ============================================================ kf_zinv
 template <int NZ, typename C>
-__host__ __device__ constexpr int zinv_smem() {
+__host__ __device__ constexpr int zinv_tw_off() {
   return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<C>))) +
          16 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<C>));
+}
+template <int NZ, typename C>
+__host__ __device__ constexpr int zinv_smem() {
+  return zinv_tw_off<NZ, C>() + tw_bytes<C>();
 }
 
 // Builds the packed C2R input Z[lane + T q] of line pair (m, m + Q) from the
@@ -177,12 +202,13 @@ __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, d
   const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
   const cx<S>* p0 = P + chunk * static_cast<long>(NZH * 32);
   const cx<S>* p1 = p0 + g.M;
+  const cx<C>* twl = stage_twiddles<C>(smem + zinv_tw_off<NZ, C>(), tw);
   for (int i = threadIdx.x; i < NZH * 32; i += blockDim.x)
     cb[(i & 31) * NZH + (i >> 5)] = cadd(cvt<C>(p0[i]), cvt<C>(p1[i]));
   __syncthreads();
   cx<C> v[8];
   build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[(r * 16 + slot) * NZH + kz]; });
-  slot_fft<C, NZ, T, +1>(v, xbuf + slot * XS, lane, tw);
+  slot_fft<C, NZ, T, +1>(v, xbuf + slot * XS, lane, twl);
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int z = lane + T * e;
@@ -198,8 +224,12 @@ __host__ __device__ constexpr int zpoint_stage_bytes() {
   return align16((64 / (NZ / 8)) * 8 * (NZ / 2 + 1) * static_cast<int>(sizeof(cx<S>)));
 }
 template <int NZ, typename S>
-__host__ __device__ constexpr int zpoint_smem() {
+__host__ __device__ constexpr int zpoint_tw_off() {
   return zpoint_stage_bytes<NZ, S>() + 4 * (64 / (NZ / 8)) * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
+}
+template <int NZ, typename S>
+__host__ __device__ constexpr int zpoint_smem() {
+  return zpoint_tw_off<NZ, S>() + 2 * tw_bytes<double>();
 }
 
 // One forward round: slot `slot` of wave `wave` transforms channel position
@@ -275,7 +305,7 @@ __device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NF > 0 ? NF : 1][
 }
 
 template <typename S, typename CV, int NZ, int NF, int NB>
-__global__ void __launch_bounds__(kZP) kf_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
+__global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
                                                  const double* __restrict__ vext, double* __restrict__ lf,
                                                  double* __restrict__ partials, FastGeom g,
                                                  const cx<double>* __restrict__ tw64, const cx<CV>* __restrict__ twv,
@@ -300,6 +330,10 @@ __global__ void __launch_bounds__(kZP) kf_zpoint(cx<S>* SF, const double* __rest
   const long cbase = chunk * static_cast<long>(NZH * 32) + qb * 4;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   S wv[NFA][PPL];
+  // twiddle tables in LDS; the first barrier below publishes them
+  const cx<double>* tw64l = stage_twiddles<double>(smem + zpoint_tw_off<NZ, S>(), tw64);
+  const cx<CV>* twvl = stage_twiddles<CV>(smem + zpoint_tw_off<NZ, S>() + tw_bytes<double>(), twv);
+  if constexpr (NRF == 0) __syncthreads();
 
   // ---- forward rounds: stage NS channels of the block's 4 line pairs, C2R
   auto stage_in = [&](int jr) {
@@ -317,9 +351,9 @@ __global__ void __launch_bounds__(kZP) kf_zpoint(cx<S>* SF, const double* __rest
     stage_in(JR);                                                                                     \
     __syncthreads();                                                                                  \
     if constexpr (zround64(NF, NS, JR))                                                               \
-      zpoint_fwd_round<double, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, tw64);                  \
+      zpoint_fwd_round<double, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, tw64l);                 \
     else                                                                                              \
-      zpoint_fwd_round<CV, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, twv);                       \
+      zpoint_fwd_round<CV, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, twvl);                      \
     __syncthreads();                                                                                  \
   }
   CDFT_ZF(0)
@@ -395,7 +429,7 @@ __global__ void __launch_bounds__(kZP) kf_zpoint(cx<S>* SF, const double* __rest
   // ---- backward rounds: R2C of NS derivative channels, split, coalesced store
 #define CDFT_ZB(JR)                                                                                           \
   if constexpr ((JR) < NRB) {                                                                                 \
-    zpoint_bwd_round<CV, S, NZ, NF, NB, JR, PPL>(wv, stg, xwave, wave, lid, twv);                             \
+    zpoint_bwd_round<CV, S, NZ, NF, NB, JR, PPL>(wv, stg, xwave, wave, lid, twvl);                            \
     __syncthreads();                                                                                          \
     for (int i = threadIdx.x; i < NS * NZH * 8; i += kZP) {                                                   \
       const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;                                                      \
@@ -422,33 +456,22 @@ __global__ void __launch_bounds__(kZP) kf_zpoint(cx<S>* SF, const double* __rest
 //   y-FFT of half rows (NX/2 rows):          Q/4 lanes x 4 elements
 //   x-FFT (NX points, one per column):       NX/8 lanes x 8 elements
 //   half x-FFT (NX/2 points):                NX/8 lanes x 4 elements
-// FP64 slot exchanges go through the buffers one real part at a time.
+// FP64 slot exchanges go through the buffers one real part at a time, so a
+// slot-buffer element is 8 bytes in every phase.
 template <int NX, int Q>
 struct PlaneCfg {
   static constexpr int BT = kBT;
   static constexpr int NY = 2 * Q, MB = Q / 16, NXH = NX / 2;
-  static constexpr int TYF = Q / 8, SBYF = BT / TYF, RPF = (NX + SBYF - 1) / SBYF;
+  static constexpr int TYF = Q / 8, SBYF = BT / TYF;
   static constexpr int TYH = Q / 4, SBYH = BT / TYH, RPH = (NXH + SBYH - 1) / SBYH;
   static constexpr int TX = NX / 8, SBX = BT / TX, CPS = (Q + SBX - 1) / SBX;
-  static constexpr int XBYTES = 8;  // bytes per slot-buffer element
-  static constexpr int xbuf_elems() {
-    int a = SBYF * slot_buf_len<Q>();
-    int b = SBYH * slot_buf_len<Q>();
-    int c = SBX * slot_buf_len<NX>();
-    int m = a > b ? a : b;
-    return m > c ? m : c;
-  }
+  static constexpr int XBYTES = 8;
   static_assert(TYH <= 64 && TX <= 64 && TYF >= 1, "unsupported plane shape");
 };
 
-template <int NX, int Q>
-__host__ __device__ constexpr int plane_smem() {
-  return kPlaneTB + PlaneCfg<NX, Q>::xbuf_elems() * PlaneCfg<NX, Q>::XBYTES;
-}
-
-// Per-element FMT weight factors in arithmetic C: w3, w2 and
-// aik2 = (w2 - 3 w3/R) / k^2 (0 at k = 0), from which every channel weight
-// follows without a division (weights.hpp:174-222): ch 0 w3, 1 w2,
+// Weight factors of one element in arithmetic C: w3, w2 and
+// aik2 = (w2 - 3 w3/R) / k^2 (0 at k = 0), from which every FMT channel
+// weight follows without a division (weights.hpp:174-222): ch 0 w3, 1 w2,
 // 2..4 i k_a w3, 5..10 aik2 k_i k_j + delta_ij w3/R.
 template <typename C>
 struct WFac {
@@ -479,13 +502,113 @@ __device__ __forceinline__ cx<C> chan_weight(int ch, bool adj, C kx, C ky, C kz,
   return {f.aik2 * ki * kj + (diag ? f.w3 * invR : C(0)), C(0)};
 }
 
-// Forward-path work of one channel in a kf_plane_fwd block: Wof(cc, e) is the
-// weighted spectrum at column xs + SBX cc, kx = xl + TX e.  x-inverse of the
-// output half ix = 2i + h (DIF: (W[kx] +- W[kx + NX/2]) twiddled), then the
-// y-inverse of every row of that half, stored in the split layout.
-template <typename CC, typename S, int NX, int Q, typename WF>
-__device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned char* xraw, cx<S>* __restrict__ out,
-                                                  const FastGeom& g, int kz, int r, int h,
+// ---------------------------------------------------------------- kf_rho_xy
+// FP64 y- and x-FFT of the rho parts: block (kz, r, c) writes the full
+// spectrum rho_hat(kx, ky = r + 2 s, kz) of the columns s in half c as
+//   XR[((kz * 2 + r) * Q + s) * NX + kx]   (complex FP64).
+template <int NX, int Q, typename S>
+struct RhoCfg {
+  using P = PlaneCfg<NX, Q>;
+  static constexpr int NC = Q / 2;                 // columns per block
+  static constexpr int TXR = NX / 4, SXR = kBT / TXR, CPR = (NC + SXR - 1) / SXR;
+  static constexpr int NXP = NX + 1;
+  static constexpr int tb_bytes = align16(NC * NXP * static_cast<int>(sizeof(cx<double>)));
+  static constexpr int xb_elems() {
+    const int a = P::SBYF * slot_buf_len<Q>(), b = SXR * slot_buf_len<NX>();
+    return a > b ? a : b;
+  }
+  static constexpr int tw_off = tb_bytes + xb_elems() * P::XBYTES;
+  static constexpr int smem = tw_off + tw_bytes<double>();
+};
+
+template <typename S, int NX, int Q>
+__global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, cx<double>* __restrict__ XR,
+                                                 FastGeom g, const cx<double>* __restrict__ tw) {
+  using P = PlaneCfg<NX, Q>;
+  using RC = RhoCfg<NX, Q, S>;
+  constexpr int RPF = (NX + P::SBYF - 1) / P::SBYF;
+  extern __shared__ __align__(16) unsigned char smem[];
+  cx<double>* tb = reinterpret_cast<cx<double>*>(smem);
+  unsigned char* xraw = smem + RC::tb_bytes;
+  const cx<double>* twl = stage_twiddles<double>(smem + RC::tw_off, tw);
+  const int tid = threadIdx.x;
+  const int ys = tid / P::TYF, yl = tid % P::TYF, xs = tid / RC::TXR, xl = tid % RC::TXR;
+  const int unit = blockIdx.x;
+  const int c = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+  const int s0 = c * RC::NC;
+  cx<double> yv[RPF][8];
+#pragma unroll
+  for (int rr = 0; rr < RPF; ++rr) {
+    const int row = ys + P::SBYF * rr;
+    if (row < NX) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) yv[rr][q] = cvt<double>(A0[split_idx(kz, r, row, yl + P::TYF * q, P::MB, g.nzh)]);
+    }
+  }
+  __syncthreads();  // twiddles
+#pragma unroll
+  for (int rr = 0; rr < RPF; ++rr) {
+    const int row = ys + P::SBYF * rr;
+    if (row < NX) {
+      if (r) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) yv[rr][q] = twiddle<double, P::NY, -1>(yv[rr][q], yl + P::TYF * q, twl);
+      }
+      slot_fft<double, Q, P::TYF, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * P::XBYTES, yl, twl);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int sc = yl + P::TYF * e - s0;
+        if (sc >= 0 && sc < RC::NC) tb[sc * RC::NXP + row] = yv[rr][e];
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int cc = 0; cc < RC::CPR; ++cc) {
+    const int sc = xs + RC::SXR * cc;
+    if (sc < RC::NC) {
+      cx<double> v[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[q] = tb[sc * RC::NXP + xl + RC::TXR * q];
+      slot_fft<double, NX, RC::TXR, -1, true>(v, xraw + xs * slot_buf_len<NX>() * P::XBYTES, xl, twl);
+      cx<double>* o = XR + ((static_cast<long>(kz) * 2 + r) * Q + s0 + sc) * NX;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) o[xl + RC::TXR * e] = v[e];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- kf_plane_fwd
+template <int NX, int Q, typename S, typename CV>
+struct FwdCfg {
+  using P = PlaneCfg<NX, Q>;
+  static constexpr int QP = Q + 1;
+  static constexpr int tb_bytes = align16(P::NXH * QP * static_cast<int>(sizeof(cx<S>)));
+  static constexpr int xx_bytes = align16(P::SBX * slot_buf_len<P::NXH>() * P::XBYTES);  // x-slot buffers
+  static constexpr int xy_bytes = align16(P::SBYH * slot_buf_len<Q>() * P::XBYTES);      // y-slot buffers
+  static constexpr int tw_all = tw_bytes<double>() + tw_bytes<CV>();
+  // Double buffering (two transpose buffers, separate x- and y-slot buffers)
+  // lets consecutive channels overlap with one barrier each; it needs the room.
+  static constexpr bool DB = 2 * tb_bytes + xx_bytes + xy_bytes + tw_all <= 160 * 1024;
+  static constexpr int tb2_off = DB ? tb_bytes : 0;
+  static constexpr int xx_off = DB ? 2 * tb_bytes : tb_bytes;
+  static constexpr int xy_off = DB ? xx_off + xx_bytes : xx_off;
+  static constexpr int tw_off = xy_off + (DB ? xy_bytes : (xx_bytes > xy_bytes ? xx_bytes : xy_bytes));
+  static constexpr int smem = tw_off + tw_all;
+};
+
+// One channel in a kf_plane_fwd block: Wof(cc, e) is the weighted spectrum at
+// column xs + SBX cc, kx = xl + TX e.  x-inverse of the output half
+// ix = 2i + h (DIF: (W[kx] +- W[kx + NX/2]) twiddled), then the y-inverse of
+// every row of that half, stored in the split layout.  With DB, channels
+// alternate between two transpose buffers and the x- and y-slot buffers are
+// distinct, so the one barrier per channel suffices: a wave enters channel
+// c+2 (which reuses channel c's transpose buffer) only after every wave has
+// passed channel c+1's barrier, i.e. finished channel c.  Without DB a second
+// barrier closes every channel.
+template <typename CC, typename S, int NX, int Q, bool DB, typename WF>
+__device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned char* xbx, unsigned char* xby,
+                                                  cx<S>* __restrict__ out, const FastGeom& g, int kz, int r, int h,
                                                   const cx<CC>* __restrict__ tw) {
   using P = PlaneCfg<NX, Q>;
   constexpr int QP = Q + 1;
@@ -504,7 +627,7 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
         if (h) zz = twiddle<CC, NX, +1>(zz, xl + P::TX * e, tw);
         zt[e] = zz;
       }
-      slot_fft<CC, P::NXH, P::TX, +1, sizeof(CC) == 8>(zt, xraw + xs * XS * P::XBYTES, xl, tw);
+      slot_fft<CC, P::NXH, P::TX, +1, sizeof(CC) == 8>(zt, xbx + xs * XS * P::XBYTES, xl, tw);
 #pragma unroll
       for (int e = 0; e < 4; ++e) tb[(xl + P::TX * e) * QP + col] = cvt<S>(zt[e]);
     }
@@ -517,7 +640,7 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
       cx<CC> v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = cvt<CC>(tb[row * QP + yl + P::TYH * q]);
-      slot_fft<CC, Q, P::TYH, +1, sizeof(CC) == 8>(v, xraw + ys * YS * P::XBYTES, yl, tw);
+      slot_fft<CC, Q, P::TYH, +1, sizeof(CC) == 8>(v, xby + ys * YS * P::XBYTES, yl, tw);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int m = yl + P::TYH * e;
@@ -527,86 +650,52 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
       }
     }
   }
-  __syncthreads();
+  if constexpr (!DB) __syncthreads();
 }
 
 template <typename S, typename CV, int NX, int Q, int NF>
-__global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<S>* __restrict__ A0, cx<S>* __restrict__ SF,
+__global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict__ XR, cx<S>* __restrict__ SF,
                                                     const double* __restrict__ mfB, FastGeom g,
                                                     const cx<double>* __restrict__ tw64,
                                                     const cx<CV>* __restrict__ twv) {
   using P = PlaneCfg<NX, Q>;
-  constexpr int CPS = P::CPS, NXP = NX + 1;
-  constexpr int NBT = (Q * NXP * static_cast<int>(sizeof(cx<S>)) > kPlaneTB) ? 2 : 1;
-  static_assert(Q * NXP * static_cast<int>(sizeof(cx<S>)) <= NBT * kPlaneTB, "transpose buffer too small");
-  static_assert((NX / 2) * (Q + 1) * static_cast<int>(sizeof(cx<S>)) <= kPlaneTB, "transpose buffer too small");
+  using FC = FwdCfg<NX, Q, S, CV>;
+  constexpr int CPS = P::CPS;
   constexpr bool HS = NF >= 11;
   constexpr bool MF = (NF == 12 || NF == 1);
   extern __shared__ __align__(16) unsigned char smem[];
-  cx<S>* tb = reinterpret_cast<cx<S>*>(smem);
-  unsigned char* xraw = smem + kPlaneTB;
+  auto tbuf = [&](int n) { return reinterpret_cast<cx<S>*>(smem + (n & 1) * FC::tb2_off); };
+  unsigned char* xbx = smem + FC::xx_off;
+  unsigned char* xby = smem + FC::xy_off;
+  const cx<double>* tw64l = stage_twiddles<double>(smem + FC::tw_off, tw64);
+  const cx<CV>* twvl = stage_twiddles<CV>(smem + FC::tw_off + tw_bytes<double>(), twv);
   const int tid = threadIdx.x;
-  const int ys = tid / P::TYF, yl = tid % P::TYF, xs = tid / P::TX, xl = tid % P::TX;
+  const int xs = tid / P::TX, xl = tid % P::TX;
   const int unit = xcd_remap(blockIdx.x, gridDim.x);
   const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
 
-  // y- and x-FFT of the rho part (FP64); X[cc][e] = rho_hat(kx = xl + TX e, ky = r + 2 col)
+  // FP64 spectrum of the lane's elements: X[cc][e] = rho_hat(kx = xl + TX e, ky = r + 2 col)
   cx<double> X[CPS][8];
-  {
-    cx<double> yv[P::RPF][8];
 #pragma unroll
-    for (int rr = 0; rr < P::RPF; ++rr) {
-      const int row = ys + P::SBYF * rr;
-      if (row < NX) {
+  for (int cc = 0; cc < CPS; ++cc) {
+    const int col = xs + P::SBX * cc;
+    const cx<double>* xr = XR + ((static_cast<long>(kz) * 2 + r) * Q + (col < Q ? col : 0)) * NX;
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const int mm = yl + P::TYF * q;
-          cx<double> a = cvt<double>(A0[split_idx(kz, r, row, mm, P::MB, g.nzh)]);
-          if (r) a = twiddle<double, P::NY, -1>(a, mm, tw64);
-          yv[rr][q] = a;
-        }
-        slot_fft<double, Q, P::TYF, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * P::XBYTES, yl, tw64);
-      }
-    }
-#pragma unroll
-    for (int bt = 0; bt < NBT; ++bt) {
-      const int s0 = bt * (Q / NBT), s1 = s0 + Q / NBT;
-      __syncthreads();
-#pragma unroll
-      for (int rr = 0; rr < P::RPF; ++rr) {
-        const int row = ys + P::SBYF * rr;
-        if (row < NX) {
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int s = yl + P::TYF * e;
-            if (s >= s0 && s < s1) tb[(s - s0) * NXP + row] = cvt<S>(yv[rr][e]);
-          }
-        }
-      }
-      __syncthreads();
-#pragma unroll
-      for (int cc = 0; cc < CPS; ++cc) {
-        const int col = xs + P::SBX * cc;
-        if (col >= s0 && col < s1) {
-#pragma unroll
-          for (int q = 0; q < 8; ++q) X[cc][q] = cvt<double>(tb[(col - s0) * NXP + xl + P::TX * q]);
-          slot_fft<double, NX, P::TX, -1, true>(X[cc], xraw + xs * slot_buf_len<NX>() * P::XBYTES, xl, tw64);
-        }
-      }
-    }
+    for (int e = 0; e < 8; ++e) X[cc][e] = xr[xl + P::TX * e];
   }
-  __syncthreads();
+  __syncthreads();  // twiddles
   const double kzv = kz * g.dkz, invR = 1.0 / g.R;
   auto kxof = [&](int e) { return kvec(xl + P::TX * e, NX, g.dkx); };
   auto kyof = [&](int cc) {
     const int col = xs + P::SBX * cc;
     return kvec(r + 2 * (col < Q ? col : 0), P::NY, g.dky);
   };
+  int nch = 0;
   // FP64 channels first: eta and n2 (weights.hpp:174-191), the mean field (FP64 table)
   if constexpr (HS) {
 #pragma unroll 1
-    for (int ch = 0; ch <= 1; ++ch) {
-      plane_fwd_channel<double, S, NX, Q>(
+    for (int ch = 0; ch <= 1; ++ch, ++nch) {
+      plane_fwd_channel<double, S, NX, Q, FC::DB>(
           [&](int cc, int e) {
             double kx = kxof(e);
             // keeps the weights inside the channel loop: recomputing them is
@@ -617,17 +706,18 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<S>* __restrict__ A0
             radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3, &w2);
             return cscale(X[cc][e], ch == 0 ? w3 : w2);
           },
-          tb, xraw, SF + static_cast<long>(ch) * g.M, g, kz, r, h, tw64);
+          tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, g, kz, r, h, tw64l);
     }
   }
   if constexpr (MF) {  // mean_field.hpp:145-196
     const double* mrow0 = mfB + (static_cast<long>(kz) * 2 + r) * Q * NX;
-    plane_fwd_channel<double, S, NX, Q>(
+    plane_fwd_channel<double, S, NX, Q, FC::DB>(
         [&](int cc, int e) {
           const int col = xs + P::SBX * cc;
           return cscale(X[cc][e], col < Q ? mrow0[static_cast<long>(col) * NX + xl + P::TX * e] : 0.0);
         },
-        tb, xraw, SF + chan_store(NF, 11) * g.M, g, kz, r, h, tw64);
+        tbuf(nch), xbx, xby, SF + chan_store(NF, 11) * g.M, g, kz, r, h, tw64l);
+    ++nch;
   }
   if constexpr (HS) {
     // vector (i k_a w3) and tensor (aik2 k_i k_j + delta_ij w3/R) channels in CV,
@@ -646,31 +736,66 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<S>* __restrict__ A0
       }
     const CV kzc = static_cast<CV>(kzv), irc = static_cast<CV>(invR);
 #pragma unroll 1
-    for (int ch = 2; ch <= 10; ++ch) {
-      plane_fwd_channel<CV, S, NX, Q>(
+    for (int ch = 2; ch <= 10; ++ch, ++nch) {
+      plane_fwd_channel<CV, S, NX, Q, FC::DB>(
           [&](int cc, int e) -> cx<CV> {
             return cmul(chan_weight<CV>(ch, false, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc),
                         XC[cc][e]);
           },
-          tb, xraw, SF + static_cast<long>(ch) * g.M, g, kz, r, h, twv);
+          tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, g, kz, r, h, twvl);
     }
   }
 }
+
+// ---------------------------------------------------------------- kf_plane_bwd
+template <int NX, int Q, typename S, typename CV>
+struct BwdCfg {
+  using P = PlaneCfg<NX, Q>;
+  static constexpr int NXHP = P::NXH + 1, QP = Q + 1;
+  static constexpr int tb1 = align16(Q * NXHP * static_cast<int>(sizeof(cx<S>)));  // one channel transpose
+  static constexpr int xy_bytes = align16(P::SBYH * slot_buf_len<Q>() * P::XBYTES);
+  static constexpr int xx_bytes = align16(P::SBX * slot_buf_len<P::NXH>() * P::XBYTES);
+  // final phase: full x-FFT slots and full-row y-FFT slots (after barriers, any region)
+  static constexpr int xf_bytes() {
+    const int a = align16(P::SBX * slot_buf_len<NX>() * P::XBYTES), b = align16(P::SBYF * slot_buf_len<Q>() * P::XBYTES);
+    return a > b ? a : b;
+  }
+  static constexpr int tw_b = tw_bytes<CV>();
+  static constexpr bool DB = 2 * tb1 + xy_bytes + xx_bytes + tw_b <= 160 * 1024;
+  static constexpr int ntb = DB ? 2 : 1;
+  // the final transpose takes half the rows at a time when all of them do not fit
+  static constexpr int tb3_half = align16((NX / 2) * QP * static_cast<int>(sizeof(cx<S>)));
+  static constexpr int xx_off = ntb * tb1 > tb3_half ? ntb * tb1 : tb3_half;
+  static constexpr int xy_off = DB ? xx_off + xx_bytes : xx_off;
+  static constexpr int slots_end() {
+    const int e1 = DB ? xy_off + xy_bytes : xx_off + (xx_bytes > xy_bytes ? xx_bytes : xy_bytes);
+    const int e2 = xx_off + xf_bytes();
+    return e1 > e2 ? e1 : e2;
+  }
+  static constexpr int tw_off = slots_end();
+  static constexpr int smem = tw_off + tw_b;
+  static constexpr int tb3_bytes = xx_off;  // the final transpose uses the channel transpose region
+  static constexpr int rows3() {            // rows of the final transpose per batch
+    return NX * QP * static_cast<int>(sizeof(cx<S>)) <= tb3_bytes ? NX : NX / 2;
+  }
+};
 
 template <typename S, typename CV, int NX, int Q>
 __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ P, FastGeom g,
                                                     const cx<CV>* __restrict__ tw) {
   using PC = PlaneCfg<NX, Q>;
+  using BC = BwdCfg<NX, Q, S, CV>;
   constexpr int CPS = PC::CPS, NXH = PC::NXH;
-  constexpr int NXHP = NXH + 1, QP = Q + 1;
-  constexpr int NBR = (NX * QP * static_cast<int>(sizeof(cx<S>)) > kPlaneTB) ? 2 : 1;
-  constexpr int RPI = (NX / NBR + PC::SBYF - 1) / PC::SBYF;
-  static_assert(Q * NXHP * static_cast<int>(sizeof(cx<S>)) <= kPlaneTB, "transpose buffer too small");
-  static_assert(NX * QP * static_cast<int>(sizeof(cx<S>)) <= NBR * kPlaneTB, "transpose buffer too small");
+  constexpr int NXHP = BC::NXHP, QP = BC::QP;
+  constexpr int ROWS3 = BC::rows3(), NBR = NX / ROWS3;
+  constexpr int RPI = (ROWS3 + PC::SBYF - 1) / PC::SBYF;
+  static_assert(ROWS3 * QP * static_cast<int>(sizeof(cx<S>)) <= BC::tb3_bytes, "transpose buffer too small");
   extern __shared__ __align__(16) unsigned char smem[];
-  cx<S>* tb = reinterpret_cast<cx<S>*>(smem);
-  unsigned char* xraw = smem + kPlaneTB;
-  constexpr int SPL = sizeof(CV) == 8;  // FP64 exchanges one real part at a time
+  auto tbuf = [&](int n) { return reinterpret_cast<cx<S>*>(smem + (n & 1) * (BC::DB ? BC::tb1 : 0)); };
+  unsigned char* xbx = smem + BC::xx_off;
+  unsigned char* xby = smem + BC::xy_off;
+  const cx<CV>* twl = stage_twiddles<CV>(smem + BC::tw_off, tw);
+  constexpr bool SPL = sizeof(CV) == 8;  // FP64 exchanges one real part at a time
   const int tid = threadIdx.x;
   const int xs = tid / PC::TX, xl = tid % PC::TX;
   const int ysh = tid / PC::TYH, ylh = tid % PC::TYH, ysf = tid / PC::TYF, ylf = tid % PC::TYF;
@@ -679,6 +804,19 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   const double kzv = kz * g.dkz;
   const CV invR = static_cast<CV>(1.0 / g.R);
 
+  // inputs of the first channel are loaded before anything else
+  cx<S> nxt[PC::RPH][4];
+  auto load_in = [&](int ch) {
+    const cx<S>* in = SB + static_cast<long>(ch) * g.M;
+#pragma unroll
+    for (int rr = 0; rr < PC::RPH; ++rr) {
+      const int i = ysh + PC::SBYH * rr;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        nxt[rr][q] = in[split_idx(kz, r, 2 * (i < NXH ? i : 0) + h, ylh + PC::TYH * q, PC::MB, g.nzh)];
+    }
+  };
+  load_in(0);
   // weight factors of the lane's accumulator elements (kx = xl + TX e, column xs + SBX cc),
   // computed in FP64 and kept in the backward path's precision
   WFac<CV> wf[CPS][8];
@@ -697,25 +835,28 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
     }
   }
   const CV kzc = static_cast<CV>(kzv);
+  __syncthreads();  // twiddles
 #pragma unroll 1
   for (int ch = 0; ch < 11; ++ch) {
-    const cx<S>* in = SB + static_cast<long>(ch) * g.M;
+    cx<S>* tb = tbuf(ch);
     // y-FFT of the rows ix = 2i + h
+    cx<CV> v[PC::RPH][4];
+#pragma unroll
+    for (int rr = 0; rr < PC::RPH; ++rr)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[rr][q] = cvt<CV>(nxt[rr][q]);
+    if (ch + 1 < 11) load_in(ch + 1);  // in flight while this channel is transformed
 #pragma unroll
     for (int rr = 0; rr < PC::RPH; ++rr) {
       const int i = ysh + PC::SBYH * rr;
       if (i < NXH) {
-        cx<CV> v[4];
+        if (r) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const int mm = ylh + PC::TYH * q;
-          cx<CV> a = cvt<CV>(in[split_idx(kz, r, 2 * i + h, mm, PC::MB, g.nzh)]);
-          if (r) a = twiddle<CV, PC::NY, -1>(a, mm, tw);
-          v[q] = a;
+          for (int q = 0; q < 4; ++q) v[rr][q] = twiddle<CV, PC::NY, -1>(v[rr][q], ylh + PC::TYH * q, twl);
         }
-        slot_fft<CV, Q, PC::TYH, -1, SPL>(v, xraw + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, tw);
+        slot_fft<CV, Q, PC::TYH, -1, SPL>(v[rr], xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, twl);
 #pragma unroll
-        for (int e = 0; e < 4; ++e) tb[(ylh + PC::TYH * e) * NXHP + i] = cvt<S>(v[e]);
+        for (int e = 0; e < 4; ++e) tb[(ylh + PC::TYH * e) * NXHP + i] = cvt<S>(v[rr][e]);
       }
     }
     __syncthreads();
@@ -728,14 +869,12 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         cx<CV> f[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) f[q] = cvt<CV>(tb[col * NXHP + xl + PC::TX * q]);
-        slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xraw + xs * slot_buf_len<NX>() * PC::XBYTES, xl, tw);
+        slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, twl);
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const int kxa = xl + PC::TX * e, kxb = kxa + NXH;
           cx<CV> oa = f[e];
-          if (h) oa = twiddle<CV, NX, -1>(oa, kxa, tw);
+          if (h) oa = twiddle<CV, NX, -1>(oa, xl + PC::TX * e, twl);
           const cx<CV> ob = h ? cneg(oa) : oa;
-          (void)kxb;
           const cx<CV> wa = chan_weight<CV>(ch, true, kxc[e], kyv, kzc, wf[cc][e], invR);
           const cx<CV> wb = chan_weight<CV>(ch, true, kxc[e + 4], kyv, kzc, wf[cc][e + 4], invR);
           acc[cc][e] = cadd(acc[cc][e], cmul(wa, oa));
@@ -743,18 +882,23 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         }
       }
     }
-    __syncthreads();
+    // With DB the next channel writes the other transpose buffer and other
+    // slot buffers; a wave reaches channel ch+2 (this channel's buffers) only
+    // after every wave has passed channel ch+1's barrier.
+    if constexpr (!BC::DB) __syncthreads();
   }
+  __syncthreads();
+  cx<S>* tb3 = reinterpret_cast<cx<S>*>(smem);
   // x-inverse of the accumulated spectrum (all kx), then y-inverse of every row
 #pragma unroll
   for (int cc = 0; cc < CPS; ++cc) {
     const int col = xs + PC::SBX * cc;
-    if (col < Q) slot_fft<CV, NX, PC::TX, +1, SPL>(acc[cc], xraw + xs * slot_buf_len<NX>() * PC::XBYTES, xl, tw);
+    if (col < Q) slot_fft<CV, NX, PC::TX, +1, SPL>(acc[cc], xbx + xs * slot_buf_len<NX>() * PC::XBYTES, xl, twl);
   }
   cx<S>* out = P + static_cast<long>(h) * g.M;
 #pragma unroll
   for (int bt = 0; bt < NBR; ++bt) {
-    const int i0 = bt * (NX / NBR), i1 = i0 + NX / NBR;
+    const int i0 = bt * ROWS3, i1 = i0 + ROWS3;
     __syncthreads();
 #pragma unroll
     for (int cc = 0; cc < CPS; ++cc) {
@@ -763,7 +907,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int ixx = xl + PC::TX * e;
-          if (ixx >= i0 && ixx < i1) tb[(ixx - i0) * QP + col] = cvt<S>(acc[cc][e]);
+          if (ixx >= i0 && ixx < i1) tb3[(ixx - i0) * QP + col] = cvt<S>(acc[cc][e]);
         }
       }
     }
@@ -771,16 +915,16 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
 #pragma unroll
     for (int rr = 0; rr < RPI; ++rr) {
       const int row = ysf + PC::SBYF * rr;
-      if (row < NX / NBR) {
+      if (row < ROWS3) {
         cx<CV> v[8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = cvt<CV>(tb[row * QP + ylf + PC::TYF * q]);
-        slot_fft<CV, Q, PC::TYF, +1, SPL>(v, xraw + ysf * slot_buf_len<Q>() * PC::XBYTES, ylf, tw);
+        for (int q = 0; q < 8; ++q) v[q] = cvt<CV>(tb3[row * QP + ylf + PC::TYF * q]);
+        slot_fft<CV, Q, PC::TYF, +1, SPL>(v, xbx + ysf * slot_buf_len<Q>() * PC::XBYTES, ylf, twl);
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
           const int mm = ylf + PC::TYF * e;
           cx<CV> b = v[e];
-          if (r) b = twiddle<CV, PC::NY, +1>(b, mm, tw);
+          if (r) b = twiddle<CV, PC::NY, +1>(b, mm, twl);
           out[split_idx(kz, r, i0 + row, mm, PC::MB, g.nzh)] = cvt<S>(b);
         }
       }
@@ -819,16 +963,18 @@ void fast_twiddles(void** tw64, void** tw32) {
   const double PI = 3.14159265358979323846;
   std::vector<double> t64(2 * kTwLen);
   std::vector<float> t32(4 * kTwLen);
-  for (int m = 0; m < kTwLen; ++m) {
-    const double a = -2.0 * PI * m / kTwLen;
-    t64[2 * m] = std::cos(a);
-    t64[2 * m + 1] = std::sin(a);
-    for (int q = 0; q < 2; ++q) {
-      const float hi = static_cast<float>(t64[2 * m + q]);
-      t32[2 * m + q] = hi;
-      t32[2 * kTwLen + 2 * m + q] = static_cast<float>(t64[2 * m + q] - static_cast<double>(hi));
+  for (int L = 4; L <= kTwMax; L *= 2)
+    for (int m = 0; m < L; ++m) {
+      const int i = L - 4 + m;
+      const double a = -2.0 * PI * m / L;
+      t64[2 * i] = std::cos(a);
+      t64[2 * i + 1] = std::sin(a);
+      for (int q = 0; q < 2; ++q) {
+        const float hi = static_cast<float>(t64[2 * i + q]);
+        t32[2 * i + q] = hi;
+        t32[2 * kTwLen + 2 * i + q] = static_cast<float>(t64[2 * i + q] - static_cast<double>(hi));
+      }
     }
-  }
   if (hipMalloc(tw64, t64.size() * sizeof(double)) != hipSuccess ||
       hipMalloc(tw32, t32.size() * sizeof(float)) != hipSuccess)
     throw std::runtime_error("hipMalloc of the twiddle tables failed");
@@ -900,16 +1046,35 @@ void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double sca
 #undef CDFT_L
 }
 
+template <typename S>
+void launch_fast_rho_xy(const cx<S>* A0, void* XR, const FastGeom& g, const void* tw64, hipStream_t st) {
+  const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
+#define CDFT_L(NX, Q)                                                                                 \
+  {                                                                                                   \
+    auto k = kf_rho_xy<S, NX, Q>;                                                                     \
+    const int sm = RhoCfg<NX, Q, S>::smem;                                                            \
+    set_smem(k, sm);                                                                                  \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, A0, static_cast<cx<double>*>(XR), g,         \
+                       as<cx<double>>(tw64));                                                         \
+  }
+  if (g.nx == 64 && g.q == 32) CDFT_L(64, 32)
+  else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
+  else if (g.nx == 128 && g.q == 32) CDFT_L(128, 32)
+  else if (g.nx == 128 && g.q == 64) CDFT_L(128, 64)
+  else throw std::runtime_error("fast path: plane shape");
+#undef CDFT_L
+}
+
 template <typename S, typename CV>
-void launch_fast_plane_fwd(const cx<S>* A0, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
+void launch_fast_plane_fwd(const void* XR, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
                            const void* twv, ChannelSet cs, hipStream_t st) {
   const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
 #define CDFT_L(NX, Q, NF)                                                                             \
   {                                                                                                   \
     auto k = kf_plane_fwd<S, CV, NX, Q, NF>;                                                          \
-    const int sm = plane_smem<NX, Q>();                                                               \
+    const int sm = FwdCfg<NX, Q, S, CV>::smem;                                                        \
     set_smem(k, sm);                                                                                  \
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, A0, SF, mfB, g,                              \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, as<cx<double>>(XR), SF, mfB, g,              \
                        as<cx<double>>(tw64), as<cx<CV>>(twv));                                        \
   }
 #define CDFT_NF(NX, Q)                                                                                \
@@ -931,7 +1096,7 @@ void launch_fast_plane_bwd(const cx<S>* SB, cx<S>* P, const FastGeom& g, const v
 #define CDFT_L(NX, Q)                                                                                 \
   {                                                                                                   \
     auto k = kf_plane_bwd<S, CV, NX, Q>;                                                              \
-    const int sm = plane_smem<NX, Q>();                                                               \
+    const int sm = BwdCfg<NX, Q, S, CV>::smem;                                                        \
     set_smem(k, sm);                                                                                  \
     hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, SB, P, g, as<cx<CV>>(twv));                  \
   }
@@ -972,7 +1137,8 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
 #define CDFT_INST(S, CV)                                                                                         \
   template void launch_fast_zfwd<S>(const double*, cx<S>*, const FastGeom&, const void*, AliasMap, const double*, \
                                     hipStream_t);                                                                \
-  template void launch_fast_plane_fwd<S, CV>(const cx<S>*, cx<S>*, const double*, const FastGeom&, const void*,   \
+  template void launch_fast_rho_xy<S>(const cx<S>*, void*, const FastGeom&, const void*, hipStream_t);          \
+  template void launch_fast_plane_fwd<S, CV>(const void*, cx<S>*, const double*, const FastGeom&, const void*,    \
                                              const void*, ChannelSet, hipStream_t);                              \
   template void launch_fast_zpoint<S, CV>(cx<S>*, const double*, const double*, double*, double*,                 \
                                           const FastGeom&, const void*, const void*, ChannelSet, FmtParams,       \

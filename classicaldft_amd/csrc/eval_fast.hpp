@@ -40,8 +40,12 @@ int fast_point_blocks(const FastGeom& g);
 template <typename S>
 void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const void* tw64, AliasMap amap,
                       const double* shift_sum, hipStream_t st);
+// FP64 spectrum rho_hat of the y-parity parts (M complex doubles, order
+// [kz][r][s][kx], ky = r + 2 s), read by the forward plane kernel.
+template <typename S>
+void launch_fast_rho_xy(const cx<S>* A0, void* XR, const FastGeom& g, const void* tw64, hipStream_t st);
 template <typename S, typename CV>
-void launch_fast_plane_fwd(const cx<S>* A0, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
+void launch_fast_plane_fwd(const void* XR, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
                            const void* twv, ChannelSet cs, hipStream_t st);
 template <typename S, typename CV>
 void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double* lf, double* partials,

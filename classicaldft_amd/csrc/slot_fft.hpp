@@ -15,9 +15,10 @@
 //
 // sign SG = -1 is the forward transform exp(-i k x), +1 the backward one;
 // both are unnormalised (src/math/fourier.cpp:41-57 of the reference).
-// Twiddles of every power-of-two length L <= kTwLen come from one table
-// tw[m] = exp(-2 pi i m / kTwLen) (FP64), or its FP32 pairs (values in
-// [0, kTwLen), residuals in [kTwLen, 2 kTwLen)), read with stride kTwLen / L.
+// Twiddles come from one table holding, for every power-of-two length
+// 4 <= L <= kTwMax, the L values exp(-2 pi i m / L) at offset L - 4 (so the
+// butterflies of a pass read consecutive entries); FP32 tables hold the
+// rounded values in [0, kTwLen) and their residuals in [kTwLen, 2 kTwLen).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,13 +27,16 @@
 
 namespace cdft {
 
-constexpr int kTwLen = 512;
+constexpr int kTwMax = 256;            // longest line with a twiddle table
+constexpr int kTwLen = 2 * kTwMax - 4;  // entries of the concatenated tables (L = 4 .. kTwMax)
 
-// Orders LDS traffic of the lanes of one wavefront (no s_barrier).
+// Orders the LDS traffic of the lanes of one wavefront (no s_barrier).  The
+// fences cover LDS only, so global loads (twiddles, the next inputs) can
+// still be issued early across it.
 __device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
   __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
 }
 
 // Padded position inside a slot buffer: one spare element every 16 spreads
@@ -73,14 +77,14 @@ struct SlotPlan {
   }
 };
 
-// v * exp(SG 2 pi i m / L) from the shared table.
+// v * exp(SG 2 pi i m / L), 0 <= m < L, from the concatenated tables.
 template <typename C, int L, int SG>
 __device__ __forceinline__ cx<C> twiddle(cx<C> v, int m, const cx<C>* __restrict__ tw) {
-  static_assert(kTwLen % L == 0, "line length exceeds the twiddle table");
-  cx<C> w = tw[m * (kTwLen / L)];
+  static_assert(L >= 4 && L <= kTwMax && (L & (L - 1)) == 0, "no twiddle table for this length");
+  cx<C> w = tw[L - 4 + m];
   if (SG > 0) w.im = -w.im;
   if constexpr (sizeof(C) == 4) {
-    cx<C> wl = tw[kTwLen + m * (kTwLen / L)];
+    cx<C> wl = tw[kTwLen + L - 4 + m];
     if (SG > 0) wl.im = -wl.im;
     return cadd(cmul(v, w), cmul(v, wl));
   } else {
