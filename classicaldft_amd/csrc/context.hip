@@ -186,6 +186,9 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g) : zpoint_blocks<double>(g);
     if (c->fast) {
       c->fg = make_fast_geom(g);
+      hipDeviceProp_t prop{};
+      CDFT_HIP(hipGetDeviceProperties(&prop, c->device));
+      c->fg.ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
       c->nblk_point = fast_point_blocks(c->fg);
       fast_twiddles(&c->ftw64, &c->ftw32);
       c->d_P = dalloc_bytes(cs_bytes * g.M * 2);

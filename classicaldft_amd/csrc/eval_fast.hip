@@ -24,6 +24,7 @@
 // vector and tensor channels and the whole backward path use FP32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -653,16 +654,26 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
   if constexpr (!DB) __syncthreads();
 }
 
+// Work of the forward plane kernel: (unit, channel) items, units
+// u = (kz * 2 + r) * 2 + h, channels in the order FP64 (eta, n2, mean field)
+// then CV.  The 4 (Nz/2 + 1) units do not divide evenly over the CUs, so
+// blocks take contiguous ranges of items (at most two units each) instead of
+// one unit each: the last round of blocks then holds one item more at most.
+__host__ __device__ constexpr int fwd_nf64(int nf) { return nf == 12 ? 3 : (nf == 11 ? 2 : nf); }
+__host__ __device__ constexpr int fwd_channel(int nf, int pos) {
+  // nf 12: eta, n2, mf, 2..10; nf 11: eta, n2, 2..10; nf 1: mf
+  return nf == 1 ? 11 : (pos < 2 ? pos : (nf == 12 ? (pos == 2 ? 11 : pos - 1) : pos));
+}
+
 template <typename S, typename CV, int NX, int Q, int NF>
 __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict__ XR, cx<S>* __restrict__ SF,
                                                     const double* __restrict__ mfB, FastGeom g,
                                                     const cx<double>* __restrict__ tw64,
-                                                    const cx<CV>* __restrict__ twv) {
+                                                    const cx<CV>* __restrict__ twv, int nunits) {
   using P = PlaneCfg<NX, Q>;
   using FC = FwdCfg<NX, Q, S, CV>;
   constexpr int CPS = P::CPS;
-  constexpr bool HS = NF >= 11;
-  constexpr bool MF = (NF == 12 || NF == 1);
+  constexpr int N64 = fwd_nf64(NF);
   extern __shared__ __align__(16) unsigned char smem[];
   auto tbuf = [&](int n) { return reinterpret_cast<cx<S>*>(smem + (n & 1) * FC::tb2_off); };
   unsigned char* xbx = smem + FC::xx_off;
@@ -671,78 +682,79 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
   const cx<CV>* twvl = stage_twiddles<CV>(smem + FC::tw_off + tw_bytes<double>(), twv);
   const int tid = threadIdx.x;
   const int xs = tid / P::TX, xl = tid % P::TX;
-  const int unit = xcd_remap(blockIdx.x, gridDim.x);
-  const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
-
-  // FP64 spectrum of the lane's elements: X[cc][e] = rho_hat(kx = xl + TX e, ky = r + 2 col)
-  cx<double> X[CPS][8];
-#pragma unroll
-  for (int cc = 0; cc < CPS; ++cc) {
-    const int col = xs + P::SBX * cc;
-    const cx<double>* xr = XR + ((static_cast<long>(kz) * 2 + r) * Q + (col < Q ? col : 0)) * NX;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) X[cc][e] = xr[xl + P::TX * e];
-  }
+  const long items = static_cast<long>(nunits) * NF;
+  const long i0 = items * blockIdx.x / gridDim.x, i1 = items * (blockIdx.x + 1) / gridDim.x;
+  const double invR = 1.0 / g.R;
   __syncthreads();  // twiddles
-  const double kzv = kz * g.dkz, invR = 1.0 / g.R;
-  auto kxof = [&](int e) { return kvec(xl + P::TX * e, NX, g.dkx); };
-  auto kyof = [&](int cc) {
-    const int col = xs + P::SBX * cc;
-    return kvec(r + 2 * (col < Q ? col : 0), P::NY, g.dky);
-  };
   int nch = 0;
-  // FP64 channels first: eta and n2 (weights.hpp:174-191), the mean field (FP64 table)
-  if constexpr (HS) {
-#pragma unroll 1
-    for (int ch = 0; ch <= 1; ++ch, ++nch) {
+  for (long it = i0; it < i1;) {
+    const int unit = static_cast<int>(it / NF);
+    const int p0 = static_cast<int>(it - static_cast<long>(unit) * NF);
+    const int p1 = static_cast<int>(min(static_cast<long>(NF), p0 + (i1 - it)));
+    it += p1 - p0;
+    const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+    // FP64 spectrum of the lane's elements, read where it is used (L1/L2):
+    // Xof(cc, e) = rho_hat(kx = xl + TX e, ky = r + 2 col)
+    const cx<double>* xu = XR + (static_cast<long>(kz) * 2 + r) * Q * NX + xl;
+    auto Xof = [&](int cc, int e) {
+      const int col = xs + P::SBX * cc;
+      return xu[static_cast<long>(col < Q ? col : 0) * NX + P::TX * e];
+    };
+    const double kzv = kz * g.dkz;
+    auto kxof = [&](int e) { return kvec(xl + P::TX * e, NX, g.dkx); };
+    auto kyof = [&](int cc) {
+      const int col = xs + P::SBX * cc;
+      return kvec(r + 2 * (col < Q ? col : 0), P::NY, g.dky);
+    };
+    // FP64 channels: eta and n2 (weights.hpp:174-191), the mean field (FP64 table,
+    // mean_field.hpp:145-196)
+    for (int pos = p0; pos < min(p1, N64); ++pos, ++nch) {
+      const int ch = fwd_channel(NF, pos);
+      const double* mrow0 = mfB + (static_cast<long>(kz) * 2 + r) * Q * NX;
+      // an opaque copy of the lane's x position keeps the wave vectors and
+      // weights of the lane's elements inside this loop: recomputing them is
+      // cheaper than holding them (8 FP64 values per lane) across it
+      int xlo = xl;
+      asm volatile("" : "+v"(xlo));
       plane_fwd_channel<double, S, NX, Q, FC::DB>(
           [&](int cc, int e) {
-            double kx = kxof(e);
-            // keeps the weights inside the channel loop: recomputing them is
-            // cheaper than holding 8 FP64 pairs per lane across it
-            asm volatile("" : "+v"(kx));
-            const double ky = kyof(cc);
+            const int col = xs + P::SBX * cc;
+            if (ch == 11) return cscale(Xof(cc, e), col < Q ? mrow0[static_cast<long>(col) * NX + xlo + P::TX * e] : 0.0);
+            const double kx = kvec(xlo + P::TX * e, NX, g.dkx), ky = kyof(cc);
             double w3, w2;
             radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3, &w2);
-            return cscale(X[cc][e], ch == 0 ? w3 : w2);
+            return cscale(Xof(cc, e), ch == 0 ? w3 : w2);
           },
-          tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, g, kz, r, h, tw64l);
+          tbuf(nch), xbx, xby, SF + chan_store(NF, ch) * g.M, g, kz, r, h, tw64l);
     }
-  }
-  if constexpr (MF) {  // mean_field.hpp:145-196
-    const double* mrow0 = mfB + (static_cast<long>(kz) * 2 + r) * Q * NX;
-    plane_fwd_channel<double, S, NX, Q, FC::DB>(
-        [&](int cc, int e) {
-          const int col = xs + P::SBX * cc;
-          return cscale(X[cc][e], col < Q ? mrow0[static_cast<long>(col) * NX + xl + P::TX * e] : 0.0);
-        },
-        tbuf(nch), xbx, xby, SF + chan_store(NF, 11) * g.M, g, kz, r, h, tw64l);
-    ++nch;
-  }
-  if constexpr (HS) {
-    // vector (i k_a w3) and tensor (aik2 k_i k_j + delta_ij w3/R) channels in CV,
-    // from the spectrum and per-element weight factors narrowed to CV (weights.hpp:195-222)
-    cx<CV> XC[CPS][8];
-    WFac<CV> wf[CPS][8];
-    CV kxc[8];
+    if constexpr (NF >= 11) {
+      if (p1 > N64) {
+        // vector (i k_a w3) and tensor (aik2 k_i k_j + delta_ij w3/R) channels in CV,
+        // from the spectrum and per-element weight factors narrowed to CV (weights.hpp:195-222)
+        cx<CV> XC[CPS][8];
+        WFac<CV> wf[CPS][8];
+        CV kxc[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kxof(e));
+        for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kxof(e));
 #pragma unroll
-    for (int cc = 0; cc < CPS; ++cc)
+        for (int cc = 0; cc < CPS; ++cc)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        XC[cc][e] = cvt<CV>(X[cc][e]);
-        wf[cc][e] = wfactors<CV>(kxof(e), kyof(cc), kzv, g);
-      }
-    const CV kzc = static_cast<CV>(kzv), irc = static_cast<CV>(invR);
+          for (int e = 0; e < 8; ++e) {
+            XC[cc][e] = cvt<CV>(Xof(cc, e));
+            wf[cc][e] = wfactors<CV>(kxof(e), kyof(cc), kzv, g);
+          }
+        const CV kzc = static_cast<CV>(kzv), irc = static_cast<CV>(invR);
 #pragma unroll 1
-    for (int ch = 2; ch <= 10; ++ch, ++nch) {
-      plane_fwd_channel<CV, S, NX, Q, FC::DB>(
-          [&](int cc, int e) -> cx<CV> {
-            return cmul(chan_weight<CV>(ch, false, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc),
-                        XC[cc][e]);
-          },
-          tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, g, kz, r, h, twvl);
+        for (int pos = max(p0, N64); pos < p1; ++pos, ++nch) {
+          const int ch = fwd_channel(NF, pos);
+          plane_fwd_channel<CV, S, NX, Q, FC::DB>(
+              [&](int cc, int e) -> cx<CV> {
+                return cmul(chan_weight<CV>(ch, false, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc),
+                            XC[cc][e]);
+              },
+              tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, g, kz, r, h, twvl);
+        }
+      }
     }
   }
 }
@@ -1068,14 +1080,15 @@ void launch_fast_rho_xy(const cx<S>* A0, void* XR, const FastGeom& g, const void
 template <typename S, typename CV>
 void launch_fast_plane_fwd(const void* XR, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
                            const void* twv, ChannelSet cs, hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
+  const int nunits = g.nzh * 4;
+  const unsigned nblk = static_cast<unsigned>(std::min<long>(g.ncu, static_cast<long>(nunits) * cs.nf));
 #define CDFT_L(NX, Q, NF)                                                                             \
   {                                                                                                   \
     auto k = kf_plane_fwd<S, CV, NX, Q, NF>;                                                          \
     const int sm = FwdCfg<NX, Q, S, CV>::smem;                                                        \
     set_smem(k, sm);                                                                                  \
     hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, as<cx<double>>(XR), SF, mfB, g,              \
-                       as<cx<double>>(tw64), as<cx<CV>>(twv));                                        \
+                       as<cx<double>>(tw64), as<cx<CV>>(twv), nunits);                                \
   }
 #define CDFT_NF(NX, Q)                                                                                \
   if (cs.nf == 12) CDFT_L(NX, Q, 12) else if (cs.nf == 11) CDFT_L(NX, Q, 11) else if (cs.nf == 1) CDFT_L(NX, Q, 1) \

@@ -20,6 +20,7 @@ namespace cdft {
 
 struct FastGeom {
   int nx, ny, nz, nzh, q, mb;  // q = ny/2, mb = q/16
+  int ncu;                     // compute units of the device (blocks of a balanced launch)
   long N, M;
   double dkx, dky, dkz, dv, inv_n, R;
 };

@@ -175,6 +175,11 @@ struct SlotPass {
 template <typename C, int L, int T, int SG, bool SPLIT = false>
 __device__ __forceinline__ void slot_fft(cx<C> (&v)[L / T], void* xb, int lane, const cx<C>* __restrict__ tw) {
   constexpr int E = L / T;
+  // The exchange addresses depend only on the lane; left visible, the
+  // compiler hoists those of every pass of every transform in a kernel to its
+  // entry and keeps them live (or spilled) throughout.  An opaque copy of the
+  // lane index keeps them local to the transform.
+  asm volatile("" : "+v"(lane));
   cx<C> u[1][E];  // first pass: radix E, butterfly b = lane reads x[lane + T q] = v[q]
 #pragma unroll
   for (int q = 0; q < E; ++q) u[0][q] = v[q];
