@@ -59,7 +59,7 @@ Context::~Context() {
   clear_graphs();
   void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
                   d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
-                  ftw64,   ftw32,   d_XR};
+                  ftw64,   ftw32,   d_XR,     d_WT};
   hipSetDevice(device);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -191,8 +191,12 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       c->fg.ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
       c->nblk_point = fast_point_blocks(c->fg);
       fast_twiddles(&c->ftw64, &c->ftw32);
-      c->d_P = dalloc_bytes(cs_bytes * g.M * 2);
+      c->d_P = dalloc_bytes(cs_bytes * g.M * 4);  // partial sets [set][h] of the backward plane kernel
       c->d_XR = dalloc_bytes(16 * g.M);
+      c->d_WT = dalloc_bytes(16 * g.M);
+      launch_fast_wtable(c->d_WT, c->fg, c->stream);
+      CDFT_HIP(hipGetLastError());
+      CDFT_HIP(hipStreamSynchronize(c->stream));
     }
     c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
     c->d_res = dalloc<double>(8);
@@ -293,7 +297,7 @@ void Context::eval_fast_t(const double* rho, const double* vext, double mu, doub
     mark(2);
     launch_fast_rho_xy<S>(A0, d_XR, fg, ftw64, stream);
     mark(3);
-    launch_fast_plane_fwd<S, CV>(d_XR, SF, d_mfB, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cs, stream);
+    launch_fast_plane_fwd<S, CV>(d_XR, d_WT, SF, d_mfB, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cs, stream);
   }
   ChannelSet cz = cs;
   if (!backward) cz.nb = 0;
@@ -303,7 +307,7 @@ void Context::eval_fast_t(const double* rho, const double* vext, double mu, doub
                             amap, cs.nf > 0 ? d_shift : nullptr, dc, stream);
   if (backward) {
     mark(5);
-    launch_fast_plane_bwd<S, CV>(SF, P, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
+    launch_fast_plane_bwd<S, CV>(SF, d_WT, P, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
     mark(6);
     launch_fast_zinv<S, CV>(P, force, d_local, g.dv, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
   }
@@ -369,11 +373,11 @@ void Context::stage_bytes(double* b, bool with_vext) const {
       b[0] = 8.0 * N;                                  // FP64 sum of rho
       b[1] = 8.0 * N + sf * M;                         // rho -> rho_hat parts
       b[2] = sf * M + 16.0 * M;                        // parts -> FP64 spectrum
-      b[3] = 16.0 * M + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // spectrum (+ FP64 table) -> nf channels
+      b[3] = 16.0 * M + (cs.hs ? 16.0 * M : 0.0) + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // spectrum, weights -> channels
     }
     b[4] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;
     if (cs.nb > 0) {
-      b[5] = nb * sf * M + 2.0 * sf * M;               // nb channels -> two partial sums
+      b[5] = nb * sf * M + 16.0 * M + 2.0 * sf * M;    // nb channels, weights -> partial sums (h = 0, 1)
       b[6] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
     }
     return;

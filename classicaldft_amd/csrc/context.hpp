@@ -83,8 +83,9 @@ struct Context {
   FastGeom fg{};
   void* ftw64 = nullptr;      // shared twiddle tables of the fused kernels
   void* ftw32 = nullptr;
-  void* d_P = nullptr;        // 2 * M complex: the two partial sums of the backward plane kernel
+  void* d_P = nullptr;        // 4 * M complex: partial sums [set][h] of the backward plane kernel
   void* d_XR = nullptr;       // M complex FP64: rho_hat for the forward plane kernel
+  void* d_WT = nullptr;       // M complex FP64: radial FMT weights (w3, w2) in the plane kernels' order
   double* d_mfB = nullptr;    // M: mean-field weight in the plane kernel's order
   int nstages() const { return fast ? kNumFastStages : kNumStages; }
   const char* stage_name(int i) const {

@@ -188,10 +188,12 @@ __device__ __forceinline__ void build_c2r(cx<C> (&v)[8], int lane, F&& part) {
   }
 }
 
+__host__ __device__ inline bool bwd_unit_split(int unit, int nunits, int nblk);
+
 template <typename S, typename C, int NZ>
 __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, double* __restrict__ out,
                                                   const double* __restrict__ add, double scale, FastGeom g,
-                                                  const cx<C>* __restrict__ tw) {
+                                                  const cx<C>* __restrict__ tw, int bwd_blocks) {
   constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   extern __shared__ __align__(16) unsigned char smem[];
   cx<C>* cb = reinterpret_cast<cx<C>*>(smem);  // [r][m16][kz]
@@ -201,11 +203,17 @@ __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, d
   const int ix = static_cast<int>(chunk / g.mb);
   const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
   const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
+  // partial sets of the backward plane kernel: [set][h] at (2 set + h) M
   const cx<S>* p0 = P + chunk * static_cast<long>(NZH * 32);
-  const cx<S>* p1 = p0 + g.M;
   const cx<C>* twl = stage_twiddles<C>(smem + zinv_tw_off<NZ, C>(), tw);
-  for (int i = threadIdx.x; i < NZH * 32; i += blockDim.x)
-    cb[(i & 31) * NZH + (i >> 5)] = cadd(cvt<C>(p0[i]), cvt<C>(p1[i]));
+  const int nunits = 4 * NZH;
+  for (int i = threadIdx.x; i < NZH * 32; i += blockDim.x) {
+    cx<C> a = cadd(cvt<C>(p0[i]), cvt<C>(p0[g.M + i]));
+    const int u = ((i >> 5) * 2 + ((i >> 4) & 1)) * 2;  // unit of (kz, r), h = 0
+    if (bwd_unit_split(u, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[2 * g.M + i]));
+    if (bwd_unit_split(u + 1, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[3 * g.M + i]));
+    cb[(i & 31) * NZH + (i >> 5)] = a;
+  }
   __syncthreads();
   cx<C> v[8];
   build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[(r * 16 + slot) * NZH + kz]; });
@@ -478,13 +486,31 @@ template <typename C>
 struct WFac {
   C w3, w2, aik2;
 };
+// From the FP64 radial weights w = (w3, w2) of the weight table.
 template <typename C>
-__device__ __forceinline__ WFac<C> wfactors(double kx, double ky, double kz, const FastGeom& g) {
-  const double k2 = kx * kx + ky * ky + kz * kz;
-  double w3, w2;
-  radial(sqrt(k2), g.R, g.inv_n, &w3, &w2);
-  const double a = k2 > 0.0 ? (w2 - 3.0 * w3 / g.R) / k2 : 0.0;
-  return {static_cast<C>(w3), static_cast<C>(w2), static_cast<C>(a)};
+__device__ __forceinline__ WFac<C> wfactors(cx<double> w, double kx, double ky, double kz, double invR) {
+  const C k2 = static_cast<C>(kx * kx + ky * ky + kz * kz);
+  const C num = static_cast<C>(w.im - 3.0 * w.re * invR);
+  return {static_cast<C>(w.re), static_cast<C>(w.im), k2 > C(0) ? num / k2 : C(0)};
+}
+
+// FP64 radial FMT weights (w3, w2) (weights.hpp:174-191, times 1/N) of every
+// element of the half spectrum, in the plane kernels' order [kz][r][s][kx]
+// (ky = r + 2 s).  They depend only on the grid and the radius: computed once.
+__global__ void __launch_bounds__(256) kf_wtable(cx<double>* __restrict__ WT, FastGeom g) {
+  const long n = g.M;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const int kx = static_cast<int>(i % g.nx);
+    const long t = i / g.nx;
+    const int s = static_cast<int>(t % g.q);
+    const long u = t / g.q;
+    const int r = static_cast<int>(u & 1), kz = static_cast<int>(u >> 1);
+    const double kxv = kvec(kx, g.nx, g.dkx), kyv = kvec(r + 2 * s, g.ny, g.dky), kzv = kz * g.dkz;
+    double w3, w2;
+    radial(sqrt(kxv * kxv + kyv * kyv + kzv * kzv), g.R, g.inv_n, &w3, &w2);
+    WT[i] = {w3, w2};
+  }
 }
 // Weight of channel ch; `adj` conjugates it (the odd vector channels change
 // sign), as the back-convolution does (hard_sphere.hpp:118-134).
@@ -666,7 +692,8 @@ __host__ __device__ constexpr int fwd_channel(int nf, int pos) {
 }
 
 template <typename S, typename CV, int NX, int Q, int NF>
-__global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict__ XR, cx<S>* __restrict__ SF,
+__global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict__ XR,
+                                                    const cx<double>* __restrict__ WT, cx<S>* __restrict__ SF,
                                                     const double* __restrict__ mfB, FastGeom g,
                                                     const cx<double>* __restrict__ tw64,
                                                     const cx<CV>* __restrict__ twv, int nunits) {
@@ -695,10 +722,14 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
     const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
     // FP64 spectrum of the lane's elements, read where it is used (L1/L2):
     // Xof(cc, e) = rho_hat(kx = xl + TX e, ky = r + 2 col)
-    const cx<double>* xu = XR + (static_cast<long>(kz) * 2 + r) * Q * NX + xl;
+    const long uoff = (static_cast<long>(kz) * 2 + r) * Q * NX + xl;
     auto Xof = [&](int cc, int e) {
       const int col = xs + P::SBX * cc;
-      return xu[static_cast<long>(col < Q ? col : 0) * NX + P::TX * e];
+      return XR[uoff + static_cast<long>(col < Q ? col : 0) * NX + P::TX * e];
+    };
+    auto Wof = [&](int cc, int e) {  // (w3, w2) of the element
+      const int col = xs + P::SBX * cc;
+      return WT[uoff + static_cast<long>(col < Q ? col : 0) * NX + P::TX * e];
     };
     const double kzv = kz * g.dkz;
     auto kxof = [&](int e) { return kvec(xl + P::TX * e, NX, g.dkx); };
@@ -711,19 +742,12 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
     for (int pos = p0; pos < min(p1, N64); ++pos, ++nch) {
       const int ch = fwd_channel(NF, pos);
       const double* mrow0 = mfB + (static_cast<long>(kz) * 2 + r) * Q * NX;
-      // an opaque copy of the lane's x position keeps the wave vectors and
-      // weights of the lane's elements inside this loop: recomputing them is
-      // cheaper than holding them (8 FP64 values per lane) across it
-      int xlo = xl;
-      asm volatile("" : "+v"(xlo));
       plane_fwd_channel<double, S, NX, Q, FC::DB>(
           [&](int cc, int e) {
             const int col = xs + P::SBX * cc;
-            if (ch == 11) return cscale(Xof(cc, e), col < Q ? mrow0[static_cast<long>(col) * NX + xlo + P::TX * e] : 0.0);
-            const double kx = kvec(xlo + P::TX * e, NX, g.dkx), ky = kyof(cc);
-            double w3, w2;
-            radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3, &w2);
-            return cscale(Xof(cc, e), ch == 0 ? w3 : w2);
+            if (ch == 11) return cscale(Xof(cc, e), col < Q ? mrow0[static_cast<long>(col) * NX + xl + P::TX * e] : 0.0);
+            const cx<double> w = Wof(cc, e);
+            return cscale(Xof(cc, e), ch == 0 ? w.re : w.im);
           },
           tbuf(nch), xbx, xby, SF + chan_store(NF, ch) * g.M, g, kz, r, h, tw64l);
     }
@@ -741,7 +765,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
 #pragma unroll
           for (int e = 0; e < 8; ++e) {
             XC[cc][e] = cvt<CV>(Xof(cc, e));
-            wf[cc][e] = wfactors<CV>(kxof(e), kyof(cc), kzv, g);
+            wf[cc][e] = wfactors<CV>(Wof(cc, e), kxof(e), kyof(cc), kzv, invR);
           }
         const CV kzc = static_cast<CV>(kzv), irc = static_cast<CV>(invR);
 #pragma unroll 1
@@ -792,9 +816,26 @@ struct BwdCfg {
   }
 };
 
+// Work of the backward plane kernel: (unit, channel) items as in the forward
+// kernel, 11 channels per unit.  A block accumulates the channels of its range
+// per unit and writes the inverse transform of each accumulated part: the part
+// that starts at channel 0 into partial set 0, a continuation (the unit's
+// remaining channels, in the next block) into set 1.  Every range holds at
+// least 11 items (blocks <= units), so a unit has at most two parts;
+// bwd_unit_split() tells the consumer which units have a set-1 part.
+__host__ __device__ inline int bwd_block_of_item(long item, long items, int nblk) {
+  return static_cast<int>(((item + 1) * nblk - 1) / items);
+}
+__host__ __device__ inline bool bwd_unit_split(int unit, int nunits, int nblk) {
+  const long items = static_cast<long>(nunits) * 11;
+  return bwd_block_of_item(static_cast<long>(unit) * 11, items, nblk) !=
+         bwd_block_of_item(static_cast<long>(unit) * 11 + 10, items, nblk);
+}
+
 template <typename S, typename CV, int NX, int Q>
-__global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ P, FastGeom g,
-                                                    const cx<CV>* __restrict__ tw) {
+__global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB, const cx<double>* __restrict__ WT,
+                                                    cx<S>* __restrict__ P, FastGeom g, const cx<CV>* __restrict__ tw,
+                                                    int nunits) {
   using PC = PlaneCfg<NX, Q>;
   using BC = BwdCfg<NX, Q, S, CV>;
   constexpr int CPS = PC::CPS, NXH = PC::NXH;
@@ -811,136 +852,147 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   const int tid = threadIdx.x;
   const int xs = tid / PC::TX, xl = tid % PC::TX;
   const int ysh = tid / PC::TYH, ylh = tid % PC::TYH, ysf = tid / PC::TYF, ylf = tid % PC::TYF;
-  const int unit = xcd_remap(blockIdx.x, gridDim.x);
-  const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
-  const double kzv = kz * g.dkz;
   const CV invR = static_cast<CV>(1.0 / g.R);
-
-  // inputs of the first channel are loaded before anything else
-  cx<S> nxt[PC::RPH][4];
-  auto load_in = [&](int ch) {
-    const cx<S>* in = SB + static_cast<long>(ch) * g.M;
-#pragma unroll
-    for (int rr = 0; rr < PC::RPH; ++rr) {
-      const int i = ysh + PC::SBYH * rr;
-#pragma unroll
-      for (int q = 0; q < 4; ++q)
-        nxt[rr][q] = in[split_idx(kz, r, 2 * (i < NXH ? i : 0) + h, ylh + PC::TYH * q, PC::MB, g.nzh)];
-    }
-  };
-  load_in(0);
-  // weight factors of the lane's accumulator elements (kx = xl + TX e, column xs + SBX cc),
-  // computed in FP64 and kept in the backward path's precision
-  WFac<CV> wf[CPS][8];
-  CV kxc[8];
-  cx<CV> acc[CPS][8];
-#pragma unroll
-  for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kvec(xl + PC::TX * e, NX, g.dkx));
-#pragma unroll
-  for (int cc = 0; cc < CPS; ++cc) {
-    const int col = xs + PC::SBX * cc;
-    const double kyv = kvec(r + 2 * (col < Q ? col : 0), PC::NY, g.dky);
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      wf[cc][e] = wfactors<CV>(kvec(xl + PC::TX * e, NX, g.dkx), kyv, kzv, g);
-      acc[cc][e] = {CV(0), CV(0)};
-    }
-  }
-  const CV kzc = static_cast<CV>(kzv);
+  const long items = static_cast<long>(nunits) * 11;
+  const long i0 = items * blockIdx.x / gridDim.x, i1 = items * (blockIdx.x + 1) / gridDim.x;
   __syncthreads();  // twiddles
+  for (long it = i0; it < i1;) {
+    const int unit = static_cast<int>(it / 11);
+    const int c0 = static_cast<int>(it - static_cast<long>(unit) * 11);
+    const int c1 = static_cast<int>(min(11L, c0 + (i1 - it)));
+    it += c1 - c0;
+    const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+    const double kzv = kz * g.dkz;
+
+    cx<S> nxt[PC::RPH][4];
+    auto load_in = [&](int ch) {
+      const cx<S>* in = SB + static_cast<long>(ch) * g.M;
+#pragma unroll
+      for (int rr = 0; rr < PC::RPH; ++rr) {
+        const int i = ysh + PC::SBYH * rr;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          nxt[rr][q] = in[split_idx(kz, r, 2 * (i < NXH ? i : 0) + h, ylh + PC::TYH * q, PC::MB, g.nzh)];
+      }
+    };
+    load_in(c0);
+    // weight factors of the lane's accumulator elements (kx = xl + TX e, column xs + SBX cc),
+    // computed in FP64 and kept in the backward path's precision
+    WFac<CV> wf[CPS][8];
+    CV kxc[8];
+    cx<CV> acc[CPS][8];
+    int xlo = xl;  // opaque: keeps the per-element set-up inside the unit loop
+    asm volatile("" : "+v"(xlo));
+    const cx<double>* wu = WT + (static_cast<long>(kz) * 2 + r) * Q * NX + xlo;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kvec(xlo + PC::TX * e, NX, g.dkx));
+#pragma unroll
+    for (int cc = 0; cc < CPS; ++cc) {
+      const int col = xs + PC::SBX * cc;
+      const double kyv = kvec(r + 2 * (col < Q ? col : 0), PC::NY, g.dky);
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        wf[cc][e] = wfactors<CV>(wu[static_cast<long>(col < Q ? col : 0) * NX + PC::TX * e],
+                                 kvec(xlo + PC::TX * e, NX, g.dkx), kyv, kzv, 1.0 / g.R);
+        acc[cc][e] = {CV(0), CV(0)};
+      }
+    }
+    const CV kzc = static_cast<CV>(kzv);
 #pragma unroll 1
-  for (int ch = 0; ch < 11; ++ch) {
-    cx<S>* tb = tbuf(ch);
-    // y-FFT of the rows ix = 2i + h
-    cx<CV> v[PC::RPH][4];
+    for (int ch = c0; ch < c1; ++ch) {
+      cx<S>* tb = tbuf(ch);
+      // y-FFT of the rows ix = 2i + h
+      cx<CV> v[PC::RPH][4];
 #pragma unroll
-    for (int rr = 0; rr < PC::RPH; ++rr)
+      for (int rr = 0; rr < PC::RPH; ++rr)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[rr][q] = cvt<CV>(nxt[rr][q]);
-    if (ch + 1 < 11) load_in(ch + 1);  // in flight while this channel is transformed
+        for (int q = 0; q < 4; ++q) v[rr][q] = cvt<CV>(nxt[rr][q]);
+      if (ch + 1 < c1) load_in(ch + 1);  // in flight while this channel is transformed
 #pragma unroll
-    for (int rr = 0; rr < PC::RPH; ++rr) {
-      const int i = ysh + PC::SBYH * rr;
-      if (i < NXH) {
-        if (r) {
+      for (int rr = 0; rr < PC::RPH; ++rr) {
+        const int i = ysh + PC::SBYH * rr;
+        if (i < NXH) {
+          if (r) {
 #pragma unroll
-          for (int q = 0; q < 4; ++q) v[rr][q] = twiddle<CV, PC::NY, -1>(v[rr][q], ylh + PC::TYH * q, twl);
+            for (int q = 0; q < 4; ++q) v[rr][q] = twiddle<CV, PC::NY, -1>(v[rr][q], ylh + PC::TYH * q, twl);
+          }
+          slot_fft<CV, Q, PC::TYH, -1, SPL>(v[rr], xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, twl);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) tb[(ylh + PC::TYH * e) * NXHP + i] = cvt<S>(v[rr][e]);
         }
-        slot_fft<CV, Q, PC::TYH, -1, SPL>(v[rr], xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, twl);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) tb[(ylh + PC::TYH * e) * NXHP + i] = cvt<S>(v[rr][e]);
       }
+      __syncthreads();
+      // half x-FFT (DIT over the rows of parity h), conj-weighted accumulation over all kx
+#pragma unroll
+      for (int cc = 0; cc < CPS; ++cc) {
+        const int col = xs + PC::SBX * cc;
+        if (col < Q) {
+          const CV kyv = static_cast<CV>(kvec(r + 2 * col, PC::NY, g.dky));
+          cx<CV> f[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) f[q] = cvt<CV>(tb[col * NXHP + xl + PC::TX * q]);
+          slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, twl);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            cx<CV> oa = f[e];
+            if (h) oa = twiddle<CV, NX, -1>(oa, xl + PC::TX * e, twl);
+            const cx<CV> ob = h ? cneg(oa) : oa;
+            const cx<CV> wa = chan_weight<CV>(ch, true, kxc[e], kyv, kzc, wf[cc][e], invR);
+            const cx<CV> wb = chan_weight<CV>(ch, true, kxc[e + 4], kyv, kzc, wf[cc][e + 4], invR);
+            acc[cc][e] = cadd(acc[cc][e], cmul(wa, oa));
+            acc[cc][e + 4] = cadd(acc[cc][e + 4], cmul(wb, ob));
+          }
+        }
+      }
+      // With DB the next channel writes the other transpose buffer and other
+      // slot buffers; a wave reaches channel ch+2 (this channel's buffers) only
+      // after every wave has passed channel ch+1's barrier.
+      if constexpr (!BC::DB) __syncthreads();
     }
     __syncthreads();
-    // half x-FFT (DIT over the rows of parity h), conj-weighted accumulation over all kx
+    // x-inverse of the accumulated spectrum (all kx), then y-inverse of every row
 #pragma unroll
     for (int cc = 0; cc < CPS; ++cc) {
       const int col = xs + PC::SBX * cc;
-      if (col < Q) {
-        const CV kyv = static_cast<CV>(kvec(r + 2 * col, PC::NY, g.dky));
-        cx<CV> f[4];
+      if (col < Q) slot_fft<CV, NX, PC::TX, +1, SPL>(acc[cc], xbx + xs * slot_buf_len<NX>() * PC::XBYTES, xl, twl);
+    }
+    cx<S>* tb3 = reinterpret_cast<cx<S>*>(smem);
+    cx<S>* out = P + static_cast<long>((c0 > 0 ? 2 : 0) + h) * g.M;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) f[q] = cvt<CV>(tb[col * NXHP + xl + PC::TX * q]);
-        slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, twl);
+    for (int bt = 0; bt < NBR; ++bt) {
+      const int r0 = bt * ROWS3, r1 = r0 + ROWS3;
+      __syncthreads();
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          cx<CV> oa = f[e];
-          if (h) oa = twiddle<CV, NX, -1>(oa, xl + PC::TX * e, twl);
-          const cx<CV> ob = h ? cneg(oa) : oa;
-          const cx<CV> wa = chan_weight<CV>(ch, true, kxc[e], kyv, kzc, wf[cc][e], invR);
-          const cx<CV> wb = chan_weight<CV>(ch, true, kxc[e + 4], kyv, kzc, wf[cc][e + 4], invR);
-          acc[cc][e] = cadd(acc[cc][e], cmul(wa, oa));
-          acc[cc][e + 4] = cadd(acc[cc][e + 4], cmul(wb, ob));
+      for (int cc = 0; cc < CPS; ++cc) {
+        const int col = xs + PC::SBX * cc;
+        if (col < Q) {
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int ixx = xl + PC::TX * e;
+            if (ixx >= r0 && ixx < r1) tb3[(ixx - r0) * QP + col] = cvt<S>(acc[cc][e]);
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int rr = 0; rr < RPI; ++rr) {
+        const int row = ysf + PC::SBYF * rr;
+        if (row < ROWS3) {
+          cx<CV> v[8];
+#pragma unroll
+          for (int q = 0; q < 8; ++q) v[q] = cvt<CV>(tb3[row * QP + ylf + PC::TYF * q]);
+          slot_fft<CV, Q, PC::TYF, +1, SPL>(v, xbx + ysf * slot_buf_len<Q>() * PC::XBYTES, ylf, twl);
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const int mm = ylf + PC::TYF * e;
+            cx<CV> b = v[e];
+            if (r) b = twiddle<CV, PC::NY, +1>(b, mm, twl);
+            out[split_idx(kz, r, r0 + row, mm, PC::MB, g.nzh)] = cvt<S>(b);
+          }
         }
       }
     }
-    // With DB the next channel writes the other transpose buffer and other
-    // slot buffers; a wave reaches channel ch+2 (this channel's buffers) only
-    // after every wave has passed channel ch+1's barrier.
-    if constexpr (!BC::DB) __syncthreads();
-  }
-  __syncthreads();
-  cx<S>* tb3 = reinterpret_cast<cx<S>*>(smem);
-  // x-inverse of the accumulated spectrum (all kx), then y-inverse of every row
-#pragma unroll
-  for (int cc = 0; cc < CPS; ++cc) {
-    const int col = xs + PC::SBX * cc;
-    if (col < Q) slot_fft<CV, NX, PC::TX, +1, SPL>(acc[cc], xbx + xs * slot_buf_len<NX>() * PC::XBYTES, xl, twl);
-  }
-  cx<S>* out = P + static_cast<long>(h) * g.M;
-#pragma unroll
-  for (int bt = 0; bt < NBR; ++bt) {
-    const int i0 = bt * ROWS3, i1 = i0 + ROWS3;
-    __syncthreads();
-#pragma unroll
-    for (int cc = 0; cc < CPS; ++cc) {
-      const int col = xs + PC::SBX * cc;
-      if (col < Q) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int ixx = xl + PC::TX * e;
-          if (ixx >= i0 && ixx < i1) tb3[(ixx - i0) * QP + col] = cvt<S>(acc[cc][e]);
-        }
-      }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < RPI; ++rr) {
-      const int row = ysf + PC::SBYF * rr;
-      if (row < ROWS3) {
-        cx<CV> v[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) v[q] = cvt<CV>(tb3[row * QP + ylf + PC::TYF * q]);
-        slot_fft<CV, Q, PC::TYF, +1, SPL>(v, xbx + ysf * slot_buf_len<Q>() * PC::XBYTES, ylf, twl);
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int mm = ylf + PC::TYF * e;
-          cx<CV> b = v[e];
-          if (r) b = twiddle<CV, PC::NY, +1>(b, mm, twl);
-          out[split_idx(kz, r, i0 + row, mm, PC::MB, g.nzh)] = cvt<S>(b);
-        }
-      }
-    }
+    __syncthreads();  // the next part reuses the transpose and slot buffers
   }
 }
 
@@ -1052,7 +1104,7 @@ void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double sca
     const int sm = zinv_smem<NZ, CV>();                                                                 \
     set_smem(k, sm);                                                                                    \
     hipLaunchKernelGGL(k, dim3(nblk), dim3(2 * NZ), sm, st, P, out, add, scale, g,                      \
-                       as<cx<CV>>(twv));                                                                \
+                       as<cx<CV>>(twv), fast_bwd_blocks(g));                                            \
   }
   if (g.nz == 64) CDFT_L(64) else if (g.nz == 128) CDFT_L(128) else throw std::runtime_error("fast path: nz");
 #undef CDFT_L
@@ -1077,9 +1129,13 @@ void launch_fast_rho_xy(const cx<S>* A0, void* XR, const FastGeom& g, const void
 #undef CDFT_L
 }
 
+void launch_fast_wtable(void* WT, const FastGeom& g, hipStream_t st) {
+  hipLaunchKernelGGL(kf_wtable, dim3(1024), dim3(256), 0, st, static_cast<cx<double>*>(WT), g);
+}
+
 template <typename S, typename CV>
-void launch_fast_plane_fwd(const void* XR, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
-                           const void* twv, ChannelSet cs, hipStream_t st) {
+void launch_fast_plane_fwd(const void* XR, const void* WT, cx<S>* SF, const double* mfB, const FastGeom& g,
+                           const void* tw64, const void* twv, ChannelSet cs, hipStream_t st) {
   const int nunits = g.nzh * 4;
   const unsigned nblk = static_cast<unsigned>(std::min<long>(g.ncu, static_cast<long>(nunits) * cs.nf));
 #define CDFT_L(NX, Q, NF)                                                                             \
@@ -1087,8 +1143,8 @@ void launch_fast_plane_fwd(const void* XR, cx<S>* SF, const double* mfB, const F
     auto k = kf_plane_fwd<S, CV, NX, Q, NF>;                                                          \
     const int sm = FwdCfg<NX, Q, S, CV>::smem;                                                        \
     set_smem(k, sm);                                                                                  \
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, as<cx<double>>(XR), SF, mfB, g,              \
-                       as<cx<double>>(tw64), as<cx<CV>>(twv), nunits);                                \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, as<cx<double>>(XR), as<cx<double>>(WT), SF, mfB, \
+                       g, as<cx<double>>(tw64), as<cx<CV>>(twv), nunits);                             \
   }
 #define CDFT_NF(NX, Q)                                                                                \
   if (cs.nf == 12) CDFT_L(NX, Q, 12) else if (cs.nf == 11) CDFT_L(NX, Q, 11) else if (cs.nf == 1) CDFT_L(NX, Q, 1) \
@@ -1102,16 +1158,20 @@ void launch_fast_plane_fwd(const void* XR, cx<S>* SF, const double* mfB, const F
 #undef CDFT_L
 }
 
+int fast_bwd_blocks(const FastGeom& g) { return std::min(g.ncu, g.nzh * 4); }
+
 template <typename S, typename CV>
-void launch_fast_plane_bwd(const cx<S>* SB, cx<S>* P, const FastGeom& g, const void* /*tw64*/, const void* twv,
+void launch_fast_plane_bwd(const cx<S>* SB, const void* WT, cx<S>* P, const FastGeom& g, const void* twv,
                            hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
+  const int nunits = g.nzh * 4;
+  const unsigned nblk = static_cast<unsigned>(fast_bwd_blocks(g));
 #define CDFT_L(NX, Q)                                                                                 \
   {                                                                                                   \
     auto k = kf_plane_bwd<S, CV, NX, Q>;                                                              \
     const int sm = BwdCfg<NX, Q, S, CV>::smem;                                                        \
     set_smem(k, sm);                                                                                  \
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, SB, P, g, as<cx<CV>>(twv));                  \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, SB, as<cx<double>>(WT), P, g, as<cx<CV>>(twv), \
+                       nunits);                                                                       \
   }
   if (g.nx == 64 && g.q == 32) CDFT_L(64, 32)
   else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
@@ -1151,12 +1211,12 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
   template void launch_fast_zfwd<S>(const double*, cx<S>*, const FastGeom&, const void*, AliasMap, const double*, \
                                     hipStream_t);                                                                \
   template void launch_fast_rho_xy<S>(const cx<S>*, void*, const FastGeom&, const void*, hipStream_t);          \
-  template void launch_fast_plane_fwd<S, CV>(const void*, cx<S>*, const double*, const FastGeom&, const void*,    \
-                                             const void*, ChannelSet, hipStream_t);                              \
+  template void launch_fast_plane_fwd<S, CV>(const void*, const void*, cx<S>*, const double*, const FastGeom&,     \
+                                             const void*, const void*, ChannelSet, hipStream_t);                 \
   template void launch_fast_zpoint<S, CV>(cx<S>*, const double*, const double*, double*, double*,                 \
                                           const FastGeom&, const void*, const void*, ChannelSet, FmtParams,       \
                                           double, AliasMap, const double*, DcTable, hipStream_t);                 \
-  template void launch_fast_plane_bwd<S, CV>(const cx<S>*, cx<S>*, const FastGeom&, const void*, const void*,     \
+  template void launch_fast_plane_bwd<S, CV>(const cx<S>*, const void*, cx<S>*, const FastGeom&, const void*,     \
                                              hipStream_t);                                                       \
   template void launch_fast_zinv<S, CV>(const cx<S>*, double*, const double*, double, const FastGeom&,           \
                                         const void*, hipStream_t);

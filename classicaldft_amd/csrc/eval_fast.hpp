@@ -37,6 +37,7 @@ void fast_twiddles(void** tw64, void** tw32);
 void fast_mf_layout(const double* plane_major, double* out, const Geom& g);
 
 int fast_point_blocks(const FastGeom& g);
+int fast_bwd_blocks(const FastGeom& g);
 
 template <typename S>
 void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const void* tw64, AliasMap amap,
@@ -45,15 +46,18 @@ void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const vo
 // [kz][r][s][kx], ky = r + 2 s), read by the forward plane kernel.
 template <typename S>
 void launch_fast_rho_xy(const cx<S>* A0, void* XR, const FastGeom& g, const void* tw64, hipStream_t st);
+// FP64 radial FMT weights (w3, w2) of every element in the plane kernels'
+// order (M complex doubles), computed once per context.
+void launch_fast_wtable(void* WT, const FastGeom& g, hipStream_t st);
 template <typename S, typename CV>
-void launch_fast_plane_fwd(const void* XR, cx<S>* SF, const double* mfB, const FastGeom& g, const void* tw64,
-                           const void* twv, ChannelSet cs, hipStream_t st);
+void launch_fast_plane_fwd(const void* XR, const void* WT, cx<S>* SF, const double* mfB, const FastGeom& g,
+                           const void* tw64, const void* twv, ChannelSet cs, hipStream_t st);
 template <typename S, typename CV>
 void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double* lf, double* partials,
                         const FastGeom& g, const void* tw64, const void* twv, ChannelSet cs, FmtParams fp, double mu,
                         AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st);
 template <typename S, typename CV>
-void launch_fast_plane_bwd(const cx<S>* SB, cx<S>* P, const FastGeom& g, const void* tw64, const void* twv,
+void launch_fast_plane_bwd(const cx<S>* SB, const void* WT, cx<S>* P, const FastGeom& g, const void* twv,
                            hipStream_t st);
 template <typename S, typename CV>
 void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
