@@ -58,7 +58,7 @@ Context::~Context() {
   hipSetDevice(device);
   clear_graphs();
   void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
-                  d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
+                  d_sum_part, d_shift, d_fin, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
                   ftw64,   ftw32,   d_XR,     d_WT};
   hipSetDevice(device);
   for (void* b : bufs)
@@ -200,6 +200,9 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     }
     c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
     c->d_res = dalloc<double>(8);
+    c->d_fin = dalloc_bytes(kFinScratchBytes);
+    CDFT_HIP(hipMemset(c->d_fin, 0, kFinScratchBytes));
+    c->fin = fin_scratch(c->d_fin);
     c->nblk_sum = sum_blocks(g);
     c->d_sum_part = dalloc<double>(8 * static_cast<size_t>(c->nblk_sum));
     c->d_shift = dalloc<double>(8);
@@ -291,7 +294,7 @@ void Context::eval_fast_t(const double* rho, const double* vext, double mu, doub
   mark(0);
   if (cs.nf > 0) {
     launch_sum(rho, amap, d_sum_part, g, stream);
-    launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
+    launch_finalize(d_sum_part, nblk_sum, fin, d_shift, stream);
     mark(1);
     launch_fast_zfwd<S>(rho, A0, fg, ftw64, amap, d_shift, stream);
     mark(2);
@@ -312,7 +315,7 @@ void Context::eval_fast_t(const double* rho, const double* vext, double mu, doub
     launch_fast_zinv<S, CV>(P, force, d_local, g.dv, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
   }
   mark(7);
-  launch_finalize(d_partials, nblk_point, d_res, stream);
+  launch_finalize(d_partials, nblk_point, fin, d_res, stream);
   mark(8);
 }
 
@@ -329,7 +332,7 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
   mark(0);
   if (cs.nf > 0) {
     launch_sum(rho, amap, d_sum_part, g, stream);
-    launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
+    launch_finalize(d_sum_part, nblk_sum, fin, d_shift, stream);
     mark(1);
     launch_zr2c<S, CL>(rho, S0, g, plan[2], tw<CL>(2), amap, d_shift, stream);
     mark(2);
@@ -356,7 +359,7 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     launch_zc2r<S, CL>(SO, 1, force, d_local, g.dv, g, plan[2], tw<CL>(2), nullptr, DcTable{}, stream);
   }
   mark(10);
-  launch_finalize(d_partials, nblk_point, d_res, stream);
+  launch_finalize(d_partials, nblk_point, fin, d_res, stream);
   mark(11);
 }
 
@@ -486,7 +489,7 @@ void wd_t(Context* c, const double* d_rho_in, double* d_out) {
   auto* S0 = static_cast<cx<S>*>(c->d_S0);
   auto* SF = static_cast<cx<S>*>(c->d_SF);
   launch_sum(d_rho_in, id, c->d_sum_part, g, c->stream);
-  launch_finalize(c->d_sum_part, c->nblk_sum, c->d_shift, c->stream);
+  launch_finalize(c->d_sum_part, c->nblk_sum, c->fin, c->d_shift, c->stream);
   launch_zr2c<S, CL>(d_rho_in, S0, g, c->plan[2], c->tw<CL>(2), id, c->d_shift, c->stream);
   launch_lines<S, CL>(S0, ylines, c->plan[1], c->tw<CL>(1), -1, c->stream);
   launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(c->d_mf), g, c->plan[0], c->tw<CE>(0), c->cs, c->stream);

@@ -73,7 +73,9 @@ struct Context {
   double* d_res = nullptr;
   double* d_sum_part = nullptr;  // block partials of the FP64 density sum
   double* d_shift = nullptr;     // [0] = sum of rho (mean-density split)
-  int nblk_point = 0;
+  void* d_fin = nullptr;      // scratch of the partial-sum finalisation (kFinScratchBytes)
+  FinScratch fin{};
+  int nblk_point = 0;  // rows of 8 partial sums written by the point kernel
   int nblk_sum = 0;
   DcTable dc{};                  // analytic k = 0 weight of every forward channel
   hipEvent_t* stage_events = nullptr;  // when set, eval records nstages()+1 events

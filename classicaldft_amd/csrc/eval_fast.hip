@@ -85,13 +85,35 @@ template <typename C>
 __host__ __device__ constexpr int tw_bytes() {
   return (sizeof(C) == 4 ? 2 : 1) * kTwLen * static_cast<int>(sizeof(cx<C>));
 }
-// Copies a twiddle table into LDS (the caller synchronises before first use):
-// twiddle reads then cost an LDS access instead of an L1/L2 round trip.
-template <typename C>
+// Block-wide copy of N elements by NT threads: element i = tid + NT k is
+// load(i), then store(i, value).  Every load of a thread is issued before its
+// first store, so the loads' memory latencies overlap; a rolled loop would
+// wait for each load in turn.
+template <int NT, int N, typename T, typename LD, typename ST>
+__device__ __forceinline__ void block_copy(LD&& load, ST&& store) {
+  constexpr int K = (N + NT - 1) / NT;
+  T r[K];
+  const int tid = static_cast<int>(threadIdx.x);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = tid + NT * k;
+    if (N % NT == 0 || i < N) r[k] = load(i);
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = tid + NT * k;
+    if (N % NT == 0 || i < N) store(i, r[k]);
+  }
+}
+
+// Copies a twiddle table into LDS with NT threads (the caller synchronises
+// before first use): twiddle reads then cost an LDS access instead of an
+// L1/L2 round trip.
+template <typename C, int NT>
 __device__ __forceinline__ cx<C>* stage_twiddles(unsigned char* dst, const cx<C>* __restrict__ src) {
   cx<C>* d = reinterpret_cast<cx<C>*>(dst);
   constexpr int n = (sizeof(C) == 4 ? 2 : 1) * kTwLen;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) d[i] = src[i];
+  block_copy<NT, n, cx<C>>([&](int i) { return src[i]; }, [&](int i, cx<C> v) { d[i] = v; });
   return d;
 }
 
@@ -120,7 +142,7 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
   const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
   const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
-  const cx<double>* twl = stage_twiddles<double>(smem + zfwd_tw_off<NZ, S>(), tw);
+  const cx<double>* twl = stage_twiddles<double, 2 * NZ>(smem + zfwd_tw_off<NZ, S>(), tw);
   cx<double> v[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -148,7 +170,8 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
   }
   __syncthreads();
   cx<S>* o = out + chunk * static_cast<long>(NZH * 32);
-  for (int i = threadIdx.x; i < NZH * 32; i += blockDim.x) o[i] = cb[(i & 31) * NZH + (i >> 5)];
+  block_copy<2 * NZ, NZH * 32, cx<S>>([&](int i) { return cb[(i & 31) * NZH + (i >> 5)]; },
+                                      [&](int i, cx<S> v) { o[i] = v; });
 }
 
 // ============================================================ kf_zinv
@@ -205,15 +228,17 @@ __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, d
   const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
   // partial sets of the backward plane kernel: [set][h] at (2 set + h) M
   const cx<S>* p0 = P + chunk * static_cast<long>(NZH * 32);
-  const cx<C>* twl = stage_twiddles<C>(smem + zinv_tw_off<NZ, C>(), tw);
+  const cx<C>* twl = stage_twiddles<C, 2 * NZ>(smem + zinv_tw_off<NZ, C>(), tw);
   const int nunits = 4 * NZH;
-  for (int i = threadIdx.x; i < NZH * 32; i += blockDim.x) {
-    cx<C> a = cadd(cvt<C>(p0[i]), cvt<C>(p0[g.M + i]));
-    const int u = ((i >> 5) * 2 + ((i >> 4) & 1)) * 2;  // unit of (kz, r), h = 0
-    if (bwd_unit_split(u, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[2 * g.M + i]));
-    if (bwd_unit_split(u + 1, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[3 * g.M + i]));
-    cb[(i & 31) * NZH + (i >> 5)] = a;
-  }
+  block_copy<2 * NZ, NZH * 32, cx<C>>(
+      [&](int i) {
+        cx<C> a = cadd(cvt<C>(p0[i]), cvt<C>(p0[g.M + i]));
+        const int u = ((i >> 5) * 2 + ((i >> 4) & 1)) * 2;  // unit of (kz, r), h = 0
+        if (bwd_unit_split(u, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[2 * g.M + i]));
+        if (bwd_unit_split(u + 1, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[3 * g.M + i]));
+        return a;
+      },
+      [&](int i, cx<C> a) { cb[(i & 31) * NZH + (i >> 5)] = a; });
   __syncthreads();
   cx<C> v[8];
   build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[(r * 16 + slot) * NZH + kz]; });
@@ -340,20 +365,27 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   S wv[NFA][PPL];
   // twiddle tables in LDS; the first barrier below publishes them
-  const cx<double>* tw64l = stage_twiddles<double>(smem + zpoint_tw_off<NZ, S>(), tw64);
-  const cx<CV>* twvl = stage_twiddles<CV>(smem + zpoint_tw_off<NZ, S>() + tw_bytes<double>(), twv);
+  const cx<double>* tw64l = stage_twiddles<double, kZP>(smem + zpoint_tw_off<NZ, S>(), tw64);
+  const cx<CV>* twvl = stage_twiddles<CV, kZP>(smem + zpoint_tw_off<NZ, S>() + tw_bytes<double>(), twv);
   if constexpr (NRF == 0) __syncthreads();
 
   // ---- forward rounds: stage NS channels of the block's 4 line pairs, C2R
   auto stage_in = [&](int jr) {
-    for (int i = threadIdx.x; i < NS * NZH * 8; i += kZP) {
-      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
-      const int kz = t % NZH, gs = t / NZH;
-      const int pos = jr * NS + gs;
-      cx<S> val = {S(0), S(0)};
-      if (pos < NF) val = SF[static_cast<long>(chan_store(NF, zorder(NF, pos))) * g.M + cbase + (kz * 2 + r) * 16 + p];
-      stg[((gs * 2 + r) * 4 + p) * NZH + kz] = val;
-    }
+    block_copy<kZP, NS * NZH * 8, cx<S>>(
+        [&](int i) {
+          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+          const int kz = t % NZH, gs = t / NZH;
+          const int pos = jr * NS + gs;
+          cx<S> val = {S(0), S(0)};
+          if (pos < NF)
+            val = SF[static_cast<long>(chan_store(NF, zorder(NF, pos))) * g.M + cbase + (kz * 2 + r) * 16 + p];
+          return val;
+        },
+        [&](int i, cx<S> val) {
+          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+          const int kz = t % NZH, gs = t / NZH;
+          stg[((gs * 2 + r) * 4 + p) * NZH + kz] = val;
+        });
   };
 #define CDFT_ZF(JR)                                                                                   \
   if constexpr ((JR) < NRF) {                                                                         \
@@ -424,28 +456,60 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
         for (int j = 0; j < PPL; ++j) wv[zpos(NF, i)][j] = (pt == j) ? outv[i] : wv[zpos(NF, i)][j];
     }
   }
+  // Partial sums: each wave reduces its lanes and parks the totals in LDS;
+  // the block's row is written after the next block barrier (the first
+  // backward round's), so no barrier is spent on it.
+  __shared__ double wsum[kZP / 64][8];
   {
     double tot[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) tot[i] = acc[i] + cmp[i];
-    __shared__ double out5[5];
-    block_reduce<5>(tot, out5);
+    for (int i = 0; i < 5; ++i) {
+      double x = acc[i] + cmp[i];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+      tot[i] = x;
+    }
+    if (lid < 8) {
+      double t = 0.0;
+#pragma unroll
+      for (int i = 0; i < 5; ++i) t = lid == i ? tot[i] * dv : t;
+      wsum[wave][lid] = t;
+    }
+  }
+  auto write_row = [&]() {
+    if (threadIdx.x < 8) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < kZP / 64; ++w) t += wsum[w][threadIdx.x];
+      partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = t;
+    }
+  };
+  if constexpr (NRB == 0) {
     __syncthreads();
-    if (threadIdx.x < 8)
-      partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x < 5 ? out5[threadIdx.x] * dv : 0.0;
+    write_row();
   }
 
   // ---- backward rounds: R2C of NS derivative channels, split, coalesced store
+  auto stage_out = [&](int jr) {
+    block_copy<kZP, NS * NZH * 8, cx<S>>(
+        [&](int i) {
+          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+          const int kz = t % NZH, gs = t / NZH;
+          return stg[((gs * 2 + r) * 4 + p) * NZH + kz];
+        },
+        [&](int i, cx<S> val) {
+          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+          const int kz = t % NZH, gs = t / NZH;
+          const int c = jr * NS + gs;
+          if (c < NB) SF[static_cast<long>(c) * g.M + cbase + (kz * 2 + r) * 16 + p] = val;
+        });
+  };
 #define CDFT_ZB(JR)                                                                                           \
   if constexpr ((JR) < NRB) {                                                                                 \
     zpoint_bwd_round<CV, S, NZ, NF, NB, JR, PPL>(wv, stg, xwave, wave, lid, twvl);                            \
     __syncthreads();                                                                                          \
-    for (int i = threadIdx.x; i < NS * NZH * 8; i += kZP) {                                                   \
-      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;                                                      \
-      const int kz = t % NZH, gs = t / NZH;                                                                   \
-      const int c = (JR) * NS + gs;                                                                           \
-      if (c < NB) SF[static_cast<long>(c) * g.M + cbase + (kz * 2 + r) * 16 + p] = stg[((gs * 2 + r) * 4 + p) * NZH + kz]; \
-    }                                                                                                         \
+    if constexpr ((JR) == 0) write_row();                                                                     \
+    stage_out(JR);                                                                                            \
     __syncthreads();                                                                                          \
   }
   CDFT_ZB(0)
@@ -557,7 +621,7 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
   extern __shared__ __align__(16) unsigned char smem[];
   cx<double>* tb = reinterpret_cast<cx<double>*>(smem);
   unsigned char* xraw = smem + RC::tb_bytes;
-  const cx<double>* twl = stage_twiddles<double>(smem + RC::tw_off, tw);
+  const cx<double>* twl = stage_twiddles<double, kBT>(smem + RC::tw_off, tw);
   const int tid = threadIdx.x;
   const int ys = tid / P::TYF, yl = tid % P::TYF, xs = tid / RC::TXR, xl = tid % RC::TXR;
   const int unit = blockIdx.x;
@@ -705,8 +769,8 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
   auto tbuf = [&](int n) { return reinterpret_cast<cx<S>*>(smem + (n & 1) * FC::tb2_off); };
   unsigned char* xbx = smem + FC::xx_off;
   unsigned char* xby = smem + FC::xy_off;
-  const cx<double>* tw64l = stage_twiddles<double>(smem + FC::tw_off, tw64);
-  const cx<CV>* twvl = stage_twiddles<CV>(smem + FC::tw_off + tw_bytes<double>(), twv);
+  const cx<double>* tw64l = stage_twiddles<double, kBT>(smem + FC::tw_off, tw64);
+  const cx<CV>* twvl = stage_twiddles<CV, kBT>(smem + FC::tw_off + tw_bytes<double>(), twv);
   const int tid = threadIdx.x;
   const int xs = tid / P::TX, xl = tid % P::TX;
   const long items = static_cast<long>(nunits) * NF;
@@ -847,7 +911,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   auto tbuf = [&](int n) { return reinterpret_cast<cx<S>*>(smem + (n & 1) * (BC::DB ? BC::tb1 : 0)); };
   unsigned char* xbx = smem + BC::xx_off;
   unsigned char* xby = smem + BC::xy_off;
-  const cx<CV>* twl = stage_twiddles<CV>(smem + BC::tw_off, tw);
+  const cx<CV>* twl = stage_twiddles<CV, kBT>(smem + BC::tw_off, tw);
   constexpr bool SPL = sizeof(CV) == 8;  // FP64 exchanges one real part at a time
   const int tid = threadIdx.x;
   const int xs = tid / PC::TX, xl = tid % PC::TX;

@@ -36,24 +36,31 @@ struct FmtParams {
   double A, B;   // esFMT coefficients (WhiteBear I/II use A=1, B=-1)
 };
 
+// f1, f2, f3 and their derivatives at packing fraction e.  The quotients
+// are formed from the two reciprocals 1/(1-e) and 1/e, each taken from one
+// FP64 division: a division costs about ten FP64 instructions on the device,
+// a product one.
 __host__ __device__ inline void f_funcs(int model, double e, double* f, double* df) {
   const double om = 1.0 - e;
   const double L = log(om);
+  const double r = 1.0 / (e * om);
+  const double io = e * r, ie = om * r;  // 1/(1-e), 1/e
+  const double io2 = io * io, ie2 = ie * ie;
   // f1 (models.hpp:157-160): identical for every model
   f[0] = L;
-  df[0] = -1.0 / om;
+  df[0] = -io;
   // f2
   if (model == FMT_WHITEBEAR_II) {
     if (e < 1e-6) {
       f[1] = 1.0 + e * (1.0 + e * (10.0 / 9.0 + 7.0 / 6.0 * e));
       df[1] = 1.0 + e * (20.0 / 9.0 + 3.5 * e);
     } else {
-      f[1] = 1.0 / 3.0 + (4.0 / 3.0) / om + 2.0 / (3.0 * e) * L;
-      df[1] = (4.0 / 3.0) / (om * om) - 2.0 / (3.0 * e * e) * L - 2.0 / (3.0 * e) / om;
+      f[1] = 1.0 / 3.0 + (4.0 / 3.0) * io + (2.0 / 3.0) * ie * L;
+      df[1] = (4.0 / 3.0) * io2 - (2.0 / 3.0) * ie2 * L - (2.0 / 3.0) * ie * io;
     }
   } else {
-    f[1] = 1.0 / om;
-    df[1] = 1.0 / (om * om);
+    f[1] = io;
+    df[1] = io2;
   }
   // f3
   if (model == FMT_RSLT || model == FMT_WHITEBEAR_I) {
@@ -66,22 +73,24 @@ __host__ __device__ inline void f_funcs(int model, double e, double* f, double* 
         df[2] = 8.0 / 3.0 + e * (7.5 + 14.4 * e);
       }
     } else {
-      f[2] = 1.0 / (e * om * om) + L / (e * e);
-      df[2] = -1.0 / (e * e * om * om) + 2.0 / (e * om * om * om) - 1.0 / (om * e * e) - 2.0 * L / (e * e * e);
+      f[2] = ie * io2 + L * ie2;
+      df[2] = -ie2 * io2 + 2.0 * ie * io2 * io - io * ie2 - 2.0 * L * ie2 * ie;
     }
   } else if (model == FMT_WHITEBEAR_II) {
     if (e < 1e-6) {
       f[2] = 1.5 + 7.0 / 3.0 * e + 3.25 * e * e + 4.2 * e * e * e;
       df[2] = 7.0 / 3.0 + 6.5 * e + 12.6 * e * e;
     } else {
+      // -g/h - L/e^2 with g = 1 - 3e + e^2, h = e (1-e)^2
       const double g = 1.0 - 3.0 * e + e * e, gp = -3.0 + 2.0 * e;
       const double h = e * om * om, hp = om * (1.0 - 3.0 * e);
-      f[2] = -g / h - L / (e * e);
-      df[2] = -(gp * h - g * hp) / (h * h) + 1.0 / (om * e * e) + 2.0 * L / (e * e * e);
+      const double ih = ie * io2;
+      f[2] = -g * ih - L * ie2;
+      df[2] = -(gp * h - g * hp) * ih * ih + io * ie2 + 2.0 * L * ie2 * ie;
     }
   } else {
-    f[2] = 1.0 / (om * om);
-    df[2] = 2.0 / (om * om * om);
+    f[2] = io2;
+    df[2] = 2.0 * io2 * io;
   }
 }
 
@@ -95,14 +104,15 @@ __host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double*
     return 0.0;
   }
   const double R = P.R;
-  const double fpr = 4.0 * PI * R;
+  // 1/(4 pi R) and 1/(4 pi R^2): the same for every point (hoisted out of loops)
+  const double ifr = 1.0 / (4.0 * PI * R), ifr2 = ifr / R;
   const double n2 = w[1];
-  const double n1 = n2 / fpr;
-  const double n0 = n2 / (fpr * R);
+  const double n1 = n2 * ifr;
+  const double n0 = n2 * ifr2;
   const double v[3] = {w[2], w[3], w[4]};
   const double a = w[5], b = w[6], c = w[7], d = w[8], e = w[9], f = w[10];
   const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-  const double v0v1 = vv / fpr;
+  const double v0v1 = vv * ifr;
 
   double F[3], dF[3];
   f_funcs(P.model, eta, F, dF);
@@ -156,12 +166,12 @@ __host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double*
   const double d_n1 = n2 * F[1];
   const double d_n2 = n1 * F[1] + dp3_n2 * F[2];
   g[0] = d_eta;
-  g[1] = d_n2 + d_n1 / fpr + d_n0 / (fpr * R);
+  g[1] = d_n2 + d_n1 * ifr + d_n0 * ifr2;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double dv0 = -v[k] * F[1];                   // dPhi/dv0 = -v1 f2
-    const double dv1 = -(v[k] / fpr) * F[1] + dp3_v[k] * F[2];  // dPhi/dv1 = -v0 f2 + dPhi3/dv1 f3
-    g[2 + k] = dv1 + dv0 / fpr;
+    const double dv0 = -v[k] * F[1];                            // dPhi/dv0 = -v1 f2
+    const double dv1 = -(v[k] * ifr) * F[1] + dp3_v[k] * F[2];  // dPhi/dv1 = -v0 f2 + dPhi3/dv1 f3
+    g[2 + k] = dv1 + dv0 * ifr;
   }
 #pragma unroll
   for (int q = 0; q < 6; ++q) {

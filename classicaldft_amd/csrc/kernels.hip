@@ -158,20 +158,36 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
 }
 
 // FP64 block sums of amap(in): partials[b*8 + 0]; slots 1..7 zeroed.
+// Per-block sums of rho.  Each thread issues kSumUnroll loads (stride: the
+// whole grid) before adding them, so their latencies overlap.
+constexpr int kSumUnroll = 8;
 __global__ void __launch_bounds__(NT) k_sum(const double* __restrict__ in, AliasMap amap, long n,
                                             double* __restrict__ partials) {
-  __shared__ double sh[NT];
+  __shared__ double sh[NT / 64];
   double s = 0.0, c = 0.0;
   const long stride = static_cast<long>(gridDim.x) * blockDim.x;
-  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride)
-    two_sum(s, c, amap.apply(in[i]));
-  sh[threadIdx.x] = s + c;
-  __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-    __syncthreads();
+  for (long base = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; base < n; base += kSumUnroll * stride) {
+    double v[kSumUnroll];
+#pragma unroll
+    for (int u = 0; u < kSumUnroll; ++u) {
+      const long i = base + u * stride;
+      v[u] = i < n ? amap.apply(in[i]) : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kSumUnroll; ++u) two_sum(s, c, v[u]);
   }
-  if (threadIdx.x < 8) partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x == 0 ? sh[0] : 0.0;
+  double x = s + c;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = x;
+  __syncthreads();
+  if (threadIdx.x < 8) {
+    double t = 0.0;
+    if (threadIdx.x == 0)
+#pragma unroll
+      for (int w = 0; w < NT / 64; ++w) t += sh[w];
+    partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = t;
+  }
 }
 
 // ---------------------------------------------------------------- y lines (contiguous)
@@ -532,31 +548,73 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
 
 // Deterministic compensated sum of per-block partials (nblk x 8 doubles) into res[0..7]:
 // strided Neumaier sums per thread, then a fixed-shape tree of (sum, compensation) pairs.
-__global__ void __launch_bounds__(NT) k_finalize(const double* __restrict__ partials, int nblk,
-                                                 double* __restrict__ res) {
-  __shared__ double sh[8][NT];
-  __shared__ double shc[8][NT];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    double s = 0, c = 0;
-    for (int b = threadIdx.x; b < nblk; b += blockDim.x) two_sum(s, c, partials[static_cast<long>(b) * 8 + i]);
-    sh[i][threadIdx.x] = s;
-    shc[i][threadIdx.x] = c;
-  }
+// Merges the (s, c) pairs of the NT / 8 threads that own one column (thread
+// t: column t % 8, row t / 8) in a fixed tree order; thread t < 8 returns
+// the column total in (s, c).
+__device__ __forceinline__ void merge_columns(double& s, double& c) {
+  __shared__ double sh[NT];
+  __shared__ double shc[NT];
+  const int t = threadIdx.x, row = t >> 3;
+  sh[t] = s;
+  shc[t] = c;
   __syncthreads();
-  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        double s = sh[i][threadIdx.x], c = shc[i][threadIdx.x] + shc[i][threadIdx.x + w];
-        two_sum(s, c, sh[i][threadIdx.x + w]);
-        sh[i][threadIdx.x] = s;
-        shc[i][threadIdx.x] = c;
-      }
+  for (int w = NT / 16; w > 0; w >>= 1) {
+    if (row < w) {
+      double s2 = sh[t], c2 = shc[t] + shc[t + 8 * w];
+      two_sum(s2, c2, sh[t + 8 * w]);
+      sh[t] = s2;
+      shc[t] = c2;
     }
     __syncthreads();
   }
-  if (threadIdx.x < 8) res[threadIdx.x] = sh[threadIdx.x][0] + shc[threadIdx.x][0];
+  s = sh[t & 7];
+  c = shc[t & 7];
+}
+
+// Deterministic compensated column sums of `nrows` rows of 8 partial sums.
+// Block b reduces a contiguous range of rows (thread t: column t % 8, rows
+// t / 8 + (NT / 8) j, eight loads issued before they are added) into one
+// (sum, compensation) row of the scratch; the last block to arrive reduces
+// those rows in block order, so the result does not depend on which block
+// finishes last.
+__global__ void __launch_bounds__(NT) k_finalize(const double* __restrict__ partials, int nrows, FinScratch fs,
+                                                 double* __restrict__ res) {
+  constexpr int RS = NT / 8;  // rows per sweep
+  __shared__ bool last;
+  const int t = threadIdx.x, comp = t & 7, row = t >> 3;
+  const int per = (nrows + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x);
+  const int r0 = static_cast<int>(blockIdx.x) * per, r1 = min(nrows, r0 + per);
+  double s = 0.0, c = 0.0;
+  for (int b0 = r0 + row; b0 < r1; b0 += 8 * RS) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int b = b0 + u * RS;
+      v[u] = b < r1 ? partials[static_cast<long>(b) * 8 + comp] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) two_sum(s, c, v[u]);
+  }
+  merge_columns(s, c);
+  if (t < 8) {
+    fs.rows[blockIdx.x * 16 + t] = s;
+    fs.rows[blockIdx.x * 16 + 8 + t] = c;
+  }
+  __threadfence();
+  __syncthreads();
+  if (t == 0) last = atomicAdd(fs.counter, 1u) == gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  s = 0.0;
+  c = 0.0;
+  for (int b = row; b < static_cast<int>(gridDim.x); b += RS) {
+    two_sum(s, c, fs.rows[b * 16 + comp]);
+    c += fs.rows[b * 16 + 8 + comp];
+  }
+  merge_columns(s, c);
+  if (t < 8) res[t] = s + c;
+  if (t == 0) *fs.counter = 0u;  // ready for the next call
 }
 
 // ---------------------------------------------------------------- launchers
@@ -593,7 +651,7 @@ void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, doubl
 }
 
 int sum_blocks(const Geom& g) {
-  long b = (g.N + 4095) / 4096;
+  long b = (g.N + NT * kSumUnroll - 1) / (NT * kSumUnroll);
   if (b > 1024) b = 1024;
   if (b < 1) b = 1;
   return static_cast<int>(b);
@@ -708,8 +766,16 @@ void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* loc
 #undef CDFT_ZP
 }
 
-void launch_finalize(const double* partials, int nblk, double* res, hipStream_t st) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(NT), 0, st, partials, nblk, res);
+FinScratch fin_scratch(void* base) {
+  auto* b = static_cast<unsigned char*>(base);
+  return {reinterpret_cast<double*>(b), reinterpret_cast<unsigned*>(b + kFinBlocks * 16 * sizeof(double))};
+}
+
+void launch_finalize(const double* partials, int nrows, FinScratch fs, double* res, hipStream_t st) {
+  constexpr int RS = NT / 8;
+  int nb = (nrows + RS - 1) / RS;
+  nb = nb < 1 ? 1 : (nb > kFinBlocks ? kFinBlocks : nb);
+  hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(NT), 0, st, partials, nrows, fs, res);
 }
 
 #define CDFT_INST_SC(S, C)                                                                                         \
