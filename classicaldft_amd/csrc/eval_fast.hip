@@ -80,10 +80,10 @@ __device__ __forceinline__ cx<C> cneg(cx<C> a) {
   return {-a.re, -a.im};
 }
 
-// Bytes of a twiddle table in LDS (FP64: kTwLen values; FP32: values and residuals).
+// Bytes of a twiddle table in LDS (kTwLen complex values).
 template <typename C>
 __host__ __device__ constexpr int tw_bytes() {
-  return (sizeof(C) == 4 ? 2 : 1) * kTwLen * static_cast<int>(sizeof(cx<C>));
+  return kTwLen * static_cast<int>(sizeof(cx<C>));
 }
 // Block-wide copy of N elements by NT threads: element i = tid + NT k is
 // load(i), then store(i, value).  Every load of a thread is issued before its
@@ -112,7 +112,7 @@ __device__ __forceinline__ void block_copy(LD&& load, ST&& store) {
 template <typename C, int NT>
 __device__ __forceinline__ cx<C>* stage_twiddles(unsigned char* dst, const cx<C>* __restrict__ src) {
   cx<C>* d = reinterpret_cast<cx<C>*>(dst);
-  constexpr int n = (sizeof(C) == 4 ? 2 : 1) * kTwLen;
+  constexpr int n = kTwLen;
   block_copy<NT, n, cx<C>>([&](int i) { return src[i]; }, [&](int i, cx<C> v) { d[i] = v; });
   return d;
 }
@@ -1090,18 +1090,15 @@ FastGeom make_fast_geom(const Geom& g) {
 void fast_twiddles(void** tw64, void** tw32) {
   const double PI = 3.14159265358979323846;
   std::vector<double> t64(2 * kTwLen);
-  std::vector<float> t32(4 * kTwLen);
+  std::vector<float> t32(2 * kTwLen);
   for (int L = 4; L <= kTwMax; L *= 2)
     for (int m = 0; m < L; ++m) {
       const int i = L - 4 + m;
       const double a = -2.0 * PI * m / L;
       t64[2 * i] = std::cos(a);
       t64[2 * i + 1] = std::sin(a);
-      for (int q = 0; q < 2; ++q) {
-        const float hi = static_cast<float>(t64[2 * i + q]);
-        t32[2 * i + q] = hi;
-        t32[2 * kTwLen + 2 * i + q] = static_cast<float>(t64[2 * i + q] - static_cast<double>(hi));
-      }
+      t32[2 * i] = static_cast<float>(t64[2 * i]);
+      t32[2 * i + 1] = static_cast<float>(t64[2 * i + 1]);
     }
   if (hipMalloc(tw64, t64.size() * sizeof(double)) != hipSuccess ||
       hipMalloc(tw32, t32.size() * sizeof(float)) != hipSuccess)

@@ -29,7 +29,7 @@ struct FastGeom {
 bool fast_supported(int nx, int ny, int nz);
 FastGeom make_fast_geom(const Geom& g);
 
-// FP64 and FP32 (value, residual) twiddle tables of length kTwLen.
+// FP64 and FP32 twiddle tables of length kTwLen.
 void fast_twiddles(void** tw64, void** tw32);
 
 // Mean-field weight (real FP64 values) from plane-major order
