@@ -774,7 +774,10 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
   const int tid = threadIdx.x;
   const int xs = tid / P::TX, xl = tid % P::TX;
   const long items = static_cast<long>(nunits) * NF;
-  const long i0 = items * blockIdx.x / gridDim.x, i1 = items * (blockIdx.x + 1) / gridDim.x;
+  // consecutive item ranges (the two halves h of one (kz, r) slice read the
+  // same spectrum and weights) go to blocks of one XCD, which share its L2
+  const long lb = xcd_remap(blockIdx.x, gridDim.x);
+  const long i0 = items * lb / gridDim.x, i1 = items * (lb + 1) / gridDim.x;
   const double invR = 1.0 / g.R;
   __syncthreads();  // twiddles
   int nch = 0;
@@ -918,7 +921,8 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   const int ysh = tid / PC::TYH, ylh = tid % PC::TYH, ysf = tid / PC::TYF, ylf = tid % PC::TYF;
   const CV invR = static_cast<CV>(1.0 / g.R);
   const long items = static_cast<long>(nunits) * 11;
-  const long i0 = items * blockIdx.x / gridDim.x, i1 = items * (blockIdx.x + 1) / gridDim.x;
+  const long lb = xcd_remap(blockIdx.x, gridDim.x);  // neighbouring ranges share an XCD's L2
+  const long i0 = items * lb / gridDim.x, i1 = items * (lb + 1) / gridDim.x;
   __syncthreads();  // twiddles
   for (long it = i0; it < i1;) {
     const int unit = static_cast<int>(it / 11);
