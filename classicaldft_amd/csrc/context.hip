@@ -58,8 +58,8 @@ Context::~Context() {
   hipSetDevice(device);
   clear_graphs();
   void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
-                  d_sum_part, d_shift, d_fin, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
-                  ftw64,   ftw32,   d_XR,     d_WT};
+                  d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
+                  ftw64,   ftw32,   d_XR,     d_WT,     d_D0};
   hipSetDevice(device);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -193,6 +193,7 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       fast_twiddles(&c->ftw64, &c->ftw32);
       c->d_P = dalloc_bytes(cs_bytes * g.M * 4);  // partial sets [set][h] of the backward plane kernel
       c->d_XR = dalloc_bytes(16 * g.M);
+      c->d_D0 = dalloc<double>(static_cast<size_t>(g.nx) * g.ny);
       c->d_WT = dalloc_bytes(16 * g.M);
       launch_fast_wtable(c->d_WT, c->fg, c->stream);
       CDFT_HIP(hipGetLastError());
@@ -200,9 +201,6 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     }
     c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
     c->d_res = dalloc<double>(8);
-    c->d_fin = dalloc_bytes(kFinScratchBytes);
-    CDFT_HIP(hipMemset(c->d_fin, 0, kFinScratchBytes));
-    c->fin = fin_scratch(c->d_fin);
     c->nblk_sum = sum_blocks(g);
     c->d_sum_part = dalloc<double>(8 * static_cast<size_t>(c->nblk_sum));
     c->d_shift = dalloc<double>(8);
@@ -279,7 +277,7 @@ const char* const kStageNames[kNumStages] = {"k_sum",    "k_zr2c",            "k
                                               "k_lines(y inv nf)", "k_zpoint", "k_lines(y fwd nb)", "k_xbwd",
                                               "k_lines(y inv 1)",  "k_zc2r",   "k_finalize"};
 
-const char* const kFastStageNames[kNumFastStages] = {"k_sum",     "kf_zfwd",      "kf_rho_xy", "kf_plane_fwd",
+const char* const kFastStageNames[kNumFastStages] = {"kf_zfwd",      "kf_rho_xy", "kf_plane_fwd",
                                                      "kf_zpoint", "kf_plane_bwd", "kf_zinv",   "k_finalize"};
 
 template <typename S, typename CV>
@@ -293,30 +291,27 @@ void Context::eval_fast_t(const double* rho, const double* vext, double mu, doub
   };
   mark(0);
   if (cs.nf > 0) {
-    launch_sum(rho, amap, d_sum_part, g, stream);
-    launch_finalize(d_sum_part, nblk_sum, fin, d_shift, stream);
+    launch_fast_zfwd<S>(rho, A0, d_D0, fg, ftw64, amap, stream);
     mark(1);
-    launch_fast_zfwd<S>(rho, A0, fg, ftw64, amap, d_shift, stream);
+    launch_fast_rho_xy<S>(A0, d_D0, d_XR, d_shift, fg, ftw64, stream);
     mark(2);
-    launch_fast_rho_xy<S>(A0, d_XR, fg, ftw64, stream);
-    mark(3);
     launch_fast_plane_fwd<S, CV>(d_XR, d_WT, SF, d_mfB, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cs, stream);
   }
   ChannelSet cz = cs;
   if (!backward) cz.nb = 0;
   double* lf = backward ? d_local : (force ? force : d_local);
-  mark(4);
+  mark(3);
   launch_fast_zpoint<S, CV>(SF, rho, vext, lf, d_partials, fg, ftw64, sizeof(CV) == 4 ? ftw32 : ftw64, cz, fp, mu,
                             amap, cs.nf > 0 ? d_shift : nullptr, dc, stream);
   if (backward) {
-    mark(5);
+    mark(4);
     launch_fast_plane_bwd<S, CV>(SF, d_WT, P, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
-    mark(6);
+    mark(5);
     launch_fast_zinv<S, CV>(P, force, d_local, g.dv, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
   }
+  mark(6);
+  launch_finalize(d_partials, nblk_point, d_res, stream);
   mark(7);
-  launch_finalize(d_partials, nblk_point, fin, d_res, stream);
-  mark(8);
 }
 
 template <typename S, typename CE, typename CL>
@@ -332,7 +327,7 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
   mark(0);
   if (cs.nf > 0) {
     launch_sum(rho, amap, d_sum_part, g, stream);
-    launch_finalize(d_sum_part, nblk_sum, fin, d_shift, stream);
+    launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
     mark(1);
     launch_zr2c<S, CL>(rho, S0, g, plan[2], tw<CL>(2), amap, d_shift, stream);
     mark(2);
@@ -359,7 +354,7 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     launch_zc2r<S, CL>(SO, 1, force, d_local, g.dv, g, plan[2], tw<CL>(2), nullptr, DcTable{}, stream);
   }
   mark(10);
-  launch_finalize(d_partials, nblk_point, fin, d_res, stream);
+  launch_finalize(d_partials, nblk_point, d_res, stream);
   mark(11);
 }
 
@@ -373,15 +368,14 @@ void Context::stage_bytes(double* b, bool with_vext) const {
   const double nf = cs.nf, nb = cs.nb;
   if (fast) {
     if (cs.nf > 0) {
-      b[0] = 8.0 * N;                                  // FP64 sum of rho
-      b[1] = 8.0 * N + sf * M;                         // rho -> rho_hat parts
-      b[2] = sf * M + 16.0 * M;                        // parts -> FP64 spectrum
-      b[3] = 16.0 * M + (cs.hs ? 16.0 * M : 0.0) + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // spectrum, weights -> channels
+      b[0] = 8.0 * N + sf * M;                         // rho -> rho_hat parts
+      b[1] = sf * M + 16.0 * M;                        // parts -> FP64 spectrum
+      b[2] = 16.0 * M + (cs.hs ? 16.0 * M : 0.0) + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // spectrum, weights -> channels
     }
-    b[4] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;
+    b[3] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;
     if (cs.nb > 0) {
-      b[5] = nb * sf * M + 16.0 * M + 2.0 * sf * M;    // nb channels, weights -> partial sums (h = 0, 1)
-      b[6] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
+      b[4] = nb * sf * M + 16.0 * M + 2.0 * sf * M;    // nb channels, weights -> partial sums (h = 0, 1)
+      b[5] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
     }
     return;
   }
@@ -489,7 +483,7 @@ void wd_t(Context* c, const double* d_rho_in, double* d_out) {
   auto* S0 = static_cast<cx<S>*>(c->d_S0);
   auto* SF = static_cast<cx<S>*>(c->d_SF);
   launch_sum(d_rho_in, id, c->d_sum_part, g, c->stream);
-  launch_finalize(c->d_sum_part, c->nblk_sum, c->fin, c->d_shift, c->stream);
+  launch_finalize(c->d_sum_part, c->nblk_sum, c->d_shift, c->stream);
   launch_zr2c<S, CL>(d_rho_in, S0, g, c->plan[2], c->tw<CL>(2), id, c->d_shift, c->stream);
   launch_lines<S, CL>(S0, ylines, c->plan[1], c->tw<CL>(1), -1, c->stream);
   launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(c->d_mf), g, c->plan[0], c->tw<CE>(0), c->cs, c->stream);

@@ -38,7 +38,7 @@ struct InteractionSpec {
 
 constexpr int kNumStages = 11;  // stages of the general schedule (the most of any path)
 extern const char* const kStageNames[kNumStages];
-constexpr int kNumFastStages = 8;
+constexpr int kNumFastStages = 7;
 extern const char* const kFastStageNames[kNumFastStages];
 
 // Results of one evaluation, in the order the fused kernel accumulates them.
@@ -73,8 +73,6 @@ struct Context {
   double* d_res = nullptr;
   double* d_sum_part = nullptr;  // block partials of the FP64 density sum
   double* d_shift = nullptr;     // [0] = sum of rho (mean-density split)
-  void* d_fin = nullptr;      // scratch of the partial-sum finalisation (kFinScratchBytes)
-  FinScratch fin{};
   int nblk_point = 0;  // rows of 8 partial sums written by the point kernel
   int nblk_sum = 0;
   DcTable dc{};                  // analytic k = 0 weight of every forward channel
@@ -87,6 +85,7 @@ struct Context {
   void* ftw32 = nullptr;
   void* d_P = nullptr;        // 4 * M complex: partial sums [set][h] of the backward plane kernel
   void* d_XR = nullptr;       // M complex FP64: rho_hat for the forward plane kernel
+  double* d_D0 = nullptr;     // nx * ny FP64: kz = 0 plane (line sums) of the z-transformed rho
   void* d_WT = nullptr;       // M complex FP64: radial FMT weights (w3, w2) in the plane kernels' order
   double* d_mfB = nullptr;    // M: mean-field weight in the plane kernel's order
   int nstages() const { return fast ? kNumFastStages : kNumStages; }

@@ -378,11 +378,11 @@ struct FireRun {
     if (homog) {
       hipLaunchKernelGGL(k_face_sum, dim3(vec_blocks(nb)), dim3(256), 0, c->stream, c->d_force, g, nb,
                          part.as<double>());
-      launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(nb)), c->fin, res.as<double>() + 16, c->stream);
+      launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(nb)), res.as<double>() + 16, c->stream);
     }
     hipLaunchKernelGGL(k_fire_force, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, c->d_force, x.as<double>(),
                        v.as<double>(), f.as<double>(), g, homog, res.as<double>() + 16, nb, amap, part.as<double>());
-    launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(g.N)), c->fin, res.as<double>(), c->stream);
+    launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(g.N)), res.as<double>(), c->stream);
     CDFT_HIP(hipGetLastError());
     double er[8], fr[8];
     CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
@@ -454,7 +454,7 @@ struct DdftRun {
     const AliasMap id{0, 0.0, 0.0};
     double* s = sums.as<double>() + 8 * slot;
     launch_sum(field, id, part.as<double>(), g, c->stream);
-    launch_finalize(part.as<double>(), sum_blocks(g), c->fin, s, c->stream);
+    launch_finalize(part.as<double>(), sum_blocks(g), s, c->stream);
     launch_zr2c<R, R>(field, S, g, c->plan[2], c->tw<R>(2), id, s, c->stream);
     launch_lines<R, R>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), -1, c->stream);
     if (x_stage) launch_xlines<R, R>(S, 1, g, c->plan[0], c->tw<R>(0), -1, c->stream);
@@ -491,7 +491,7 @@ struct DdftRun {
                       DcTable{}, c->stream);
     const AliasMap id{0, 0.0, 0.0};
     launch_sum(raw.as<double>(), id, part.as<double>(), g, c->stream);
-    launch_finalize(part.as<double>(), sum_blocks(g), c->fin, sums.as<double>() + 24, c->stream);
+    launch_finalize(part.as<double>(), sum_blocks(g), sums.as<double>() + 24, c->stream);
     hipLaunchKernelGGL(k_ddft_update, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, raw.as<double>(),
                        d1.as<double>(), g.N, sums.as<double>(), st_dt, first_iter ? 1 : 0, part.as<double>());
     hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
@@ -601,7 +601,7 @@ int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire, const cd
       const unsigned nb = vec_blocks(g.N);
       hipLaunchKernelGGL(k_fire_velocity, dim3(nb), dim3(256), 0, c->stream, run.x.as<double>(), run.v.as<double>(),
                          run.f.as<double>(), g.N, dt_old, dt, backtrack ? 1 : 0, run.part.as<double>());
-      launch_finalize(run.part.as<double>(), static_cast<int>(nb), c->fin, run.res.as<double>() + 8, c->stream);
+      launch_finalize(run.part.as<double>(), static_cast<int>(nb), run.res.as<double>() + 8, c->stream);
       hipLaunchKernelGGL(k_fire_mix, dim3(nb), dim3(256), 0, c->stream, run.x.as<double>(), run.v.as<double>(),
                          run.f.as<double>(), g.N, dt, alpha, run.res.as<double>() + 8, run.res.as<double>());
       energy = run.compute(&sf2, &power);
@@ -643,7 +643,7 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
     const AliasMap id{0, 0.0, 0.0};
     auto mass_of = [&](const double* d) {
       launch_sum(d, id, run.part.as<double>(), g, c->stream);
-      launch_finalize(run.part.as<double>(), sum_blocks(g), c->fin, run.sums.as<double>() + 32, c->stream);
+      launch_finalize(run.part.as<double>(), sum_blocks(g), run.sums.as<double>() + 32, c->stream);
       double s;
       CDFT_HIP(hipMemcpyAsync(&s, run.sums.as<double>() + 32, 8, hipMemcpyDeviceToHost, c->stream));
       CDFT_HIP(hipStreamSynchronize(c->stream));

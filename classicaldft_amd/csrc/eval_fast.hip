@@ -42,10 +42,12 @@ constexpr int kBT = 1024;            // threads of a plane kernel block
 constexpr int kPlaneTB = 73728;      // bytes of a plane kernel's transpose buffer
 constexpr int kZP = 256;             // threads of a kf_zpoint block (4 line pairs)
 
-// Blocks are dispatched round-robin over the 8 XCDs; this maps consecutive
-// logical blocks onto one XCD so that they share its L2.
-__device__ __forceinline__ int xcd_remap(int b, int n) {
-  return (n & 7) ? b : (b & 7) * (n >> 3) + (b >> 3);
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8);
+// this maps consecutive logical blocks onto one XCD so that they share its
+// L2.  XCD x runs n/8 blocks, plus one for x < n % 8; its blocks take the
+// logical numbers after those of XCDs 0 .. x-1 (a bijection of [0, n)).
+__host__ __device__ constexpr int xcd_remap(int b, int n) {
+  return (b & 7) * (n >> 3) + ((b & 7) < (n & 7) ? (b & 7) : (n & 7)) + (b >> 3);
 }
 
 __host__ __device__ constexpr int align16(int b) { return (b + 15) & ~15; }
@@ -129,9 +131,12 @@ __host__ __device__ constexpr int zfwd_smem() {
 }
 
 template <typename S, int NZ>
+// The kz = 0 coefficients (the line sums, which carry the mean density) are
+// also written in FP64 to D0[chunk * 32 + r * 16 + m % 16]: kf_rho_xy takes
+// the kz = 0 plane from there, so its FP32 rounding never meets the mean.
 __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho, cx<S>* __restrict__ out, FastGeom g,
                                                   const cx<double>* __restrict__ tw, AliasMap amap,
-                                                  const double* __restrict__ shift_sum) {
+                                                  double* __restrict__ D0) {
   constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   extern __shared__ __align__(16) unsigned char smem[];
   cx<S>* cb = reinterpret_cast<cx<S>*>(smem);  // [r][m16][kz]
@@ -141,13 +146,12 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
   const int ix = static_cast<int>(chunk / g.mb);
   const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
   const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
-  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   const cx<double>* twl = stage_twiddles<double, 2 * NZ>(smem + zfwd_tw_off<NZ, S>(), tw);
   cx<double> v[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
     const int z = lane + T * q;
-    v[q] = {amap.apply(rho[la * NZ + z]) - shift, amap.apply(rho[lb * NZ + z]) - shift};
+    v[q] = {amap.apply(rho[la * NZ + z]), amap.apply(rho[lb * NZ + z])};
   }
   __syncthreads();
   cx<double>* xb = xbuf + slot * XS;
@@ -166,6 +170,10 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
       const cx<double> B = {0.5 * (zk.im + zm.im), -0.5 * (zk.re - zm.re)};
       cb[slot * NZH + k] = cvt<S>(cadd(A, B));
       cb[(16 + slot) * NZH + k] = cvt<S>(csub(A, B));
+      if (k == 0) {
+        D0[chunk * 32 + slot] = A.re + B.re;
+        D0[chunk * 32 + 16 + slot] = A.re - B.re;
+      }
     }
   }
   __syncthreads();
@@ -612,8 +620,13 @@ struct RhoCfg {
   static constexpr int smem = tw_off + tw_bytes<double>();
 };
 
+// The kz = 0 plane comes from the FP64 line sums D0 of kf_zfwd.  The
+// (0, 0, 0) coefficient, the sum of rho, goes to sum_out[0] and is replaced by
+// 0: the channels are transformed without their mean, which the point kernel
+// adds back in FP64 (the k = 0 weights times the mean density).
 template <typename S, int NX, int Q>
-__global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, cx<double>* __restrict__ XR,
+__global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, const double* __restrict__ D0,
+                                                 cx<double>* __restrict__ XR, double* __restrict__ sum_out,
                                                  FastGeom g, const cx<double>* __restrict__ tw) {
   using P = PlaneCfg<NX, Q>;
   using RC = RhoCfg<NX, Q, S>;
@@ -624,7 +637,7 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
   const cx<double>* twl = stage_twiddles<double, kBT>(smem + RC::tw_off, tw);
   const int tid = threadIdx.x;
   const int ys = tid / P::TYF, yl = tid % P::TYF, xs = tid / RC::TXR, xl = tid % RC::TXR;
-  const int unit = blockIdx.x;
+  const int unit = xcd_remap(blockIdx.x, gridDim.x);  // the two column halves c of a slice share an L2
   const int c = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
   const int s0 = c * RC::NC;
   cx<double> yv[RPF][8];
@@ -633,7 +646,11 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
     const int row = ys + P::SBYF * rr;
     if (row < NX) {
 #pragma unroll
-      for (int q = 0; q < 8; ++q) yv[rr][q] = cvt<double>(A0[split_idx(kz, r, row, yl + P::TYF * q, P::MB, g.nzh)]);
+      for (int q = 0; q < 8; ++q) {
+        const int m = yl + P::TYF * q;
+        yv[rr][q] = kz == 0 ? cx<double>{D0[(static_cast<long>(row) * P::MB + (m >> 4)) * 32 + r * 16 + (m & 15)], 0.0}
+                            : cvt<double>(A0[split_idx(kz, r, row, m, P::MB, g.nzh)]);
+      }
     }
   }
   __syncthreads();  // twiddles
@@ -662,6 +679,10 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = tb[sc * RC::NXP + xl + RC::TXR * q];
       slot_fft<double, NX, RC::TXR, -1, true>(v, xraw + xs * slot_buf_len<NX>() * P::XBYTES, xl, twl);
+      if (kz == 0 && r == 0 && s0 + sc == 0 && xl == 0) {
+        sum_out[0] = v[0].re;
+        v[0] = {0.0, 0.0};
+      }
       cx<double>* o = XR + ((static_cast<long>(kz) * 2 + r) * Q + s0 + sc) * NX;
 #pragma unroll
       for (int e = 0; e < 4; ++e) o[xl + RC::TXR * e] = v[e];
@@ -1144,8 +1165,8 @@ void set_smem(K kernel, int bytes) {
 }  // namespace
 
 template <typename S>
-void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const void* tw64, AliasMap amap,
-                      const double* shift_sum, hipStream_t st) {
+void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom& g, const void* tw64, AliasMap amap,
+                      hipStream_t st) {
   const unsigned nblk = static_cast<unsigned>(g.nx * g.mb);
 #define CDFT_L(NZ)                                                                                     \
   {                                                                                                    \
@@ -1153,7 +1174,7 @@ void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const vo
     const int sm = zfwd_smem<NZ, S>();                                                                 \
     set_smem(k, sm);                                                                                   \
     hipLaunchKernelGGL(k, dim3(nblk), dim3(2 * NZ), sm, st, rho, out, g,                               \
-                       as<cx<double>>(tw64), amap, shift_sum);                                         \
+                       as<cx<double>>(tw64), amap, D0);                                                \
   }
   if (g.nz == 64) CDFT_L(64) else if (g.nz == 128) CDFT_L(128) else throw std::runtime_error("fast path: nz");
 #undef CDFT_L
@@ -1176,15 +1197,16 @@ void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double sca
 }
 
 template <typename S>
-void launch_fast_rho_xy(const cx<S>* A0, void* XR, const FastGeom& g, const void* tw64, hipStream_t st) {
+void launch_fast_rho_xy(const cx<S>* A0, const double* D0, void* XR, double* sum_out, const FastGeom& g,
+                        const void* tw64, hipStream_t st) {
   const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
 #define CDFT_L(NX, Q)                                                                                 \
   {                                                                                                   \
     auto k = kf_rho_xy<S, NX, Q>;                                                                     \
     const int sm = RhoCfg<NX, Q, S>::smem;                                                            \
     set_smem(k, sm);                                                                                  \
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, A0, static_cast<cx<double>*>(XR), g,         \
-                       as<cx<double>>(tw64));                                                         \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, A0, D0, static_cast<cx<double>*>(XR), sum_out, \
+                       g, as<cx<double>>(tw64));                                                      \
   }
   if (g.nx == 64 && g.q == 32) CDFT_L(64, 32)
   else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
@@ -1273,9 +1295,10 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
 }
 
 #define CDFT_INST(S, CV)                                                                                         \
-  template void launch_fast_zfwd<S>(const double*, cx<S>*, const FastGeom&, const void*, AliasMap, const double*, \
+  template void launch_fast_zfwd<S>(const double*, cx<S>*, double*, const FastGeom&, const void*, AliasMap,      \
                                     hipStream_t);                                                                \
-  template void launch_fast_rho_xy<S>(const cx<S>*, void*, const FastGeom&, const void*, hipStream_t);          \
+  template void launch_fast_rho_xy<S>(const cx<S>*, const double*, void*, double*, const FastGeom&, const void*, \
+                                      hipStream_t);                                                              \
   template void launch_fast_plane_fwd<S, CV>(const void*, const void*, cx<S>*, const double*, const FastGeom&,     \
                                              const void*, const void*, ChannelSet, hipStream_t);                 \
   template void launch_fast_zpoint<S, CV>(cx<S>*, const double*, const double*, double*, double*,                 \

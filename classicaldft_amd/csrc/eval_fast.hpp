@@ -39,13 +39,17 @@ void fast_mf_layout(const double* plane_major, double* out, const Geom& g);
 int fast_point_blocks(const FastGeom& g);
 int fast_bwd_blocks(const FastGeom& g);
 
+// z-R2C of rho into the split layout; D0 (nx * ny doubles) receives the FP64
+// kz = 0 plane (the line sums).
 template <typename S>
-void launch_fast_zfwd(const double* rho, cx<S>* out, const FastGeom& g, const void* tw64, AliasMap amap,
-                      const double* shift_sum, hipStream_t st);
+void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom& g, const void* tw64, AliasMap amap,
+                      hipStream_t st);
 // FP64 spectrum rho_hat of the y-parity parts (M complex doubles, order
-// [kz][r][s][kx], ky = r + 2 s), read by the forward plane kernel.
+// [kz][r][s][kx], ky = r + 2 s), read by the forward plane kernel, without
+// its (0, 0, 0) coefficient; sum_out[0] receives that coefficient (sum of rho).
 template <typename S>
-void launch_fast_rho_xy(const cx<S>* A0, void* XR, const FastGeom& g, const void* tw64, hipStream_t st);
+void launch_fast_rho_xy(const cx<S>* A0, const double* D0, void* XR, double* sum_out, const FastGeom& g,
+                        const void* tw64, hipStream_t st);
 // FP64 radial FMT weights (w3, w2) of every element in the plane kernels'
 // order (M complex doubles), computed once per context.
 void launch_fast_wtable(void* WT, const FastGeom& g, hipStream_t st);

@@ -548,17 +548,33 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
 
 // Deterministic compensated sum of per-block partials (nblk x 8 doubles) into res[0..7]:
 // strided Neumaier sums per thread, then a fixed-shape tree of (sum, compensation) pairs.
-// Merges the (s, c) pairs of the NT / 8 threads that own one column (thread
-// t: column t % 8, row t / 8) in a fixed tree order; thread t < 8 returns
-// the column total in (s, c).
-__device__ __forceinline__ void merge_columns(double& s, double& c) {
-  __shared__ double sh[NT];
-  __shared__ double shc[NT];
-  const int t = threadIdx.x, row = t >> 3;
+// Deterministic compensated column sums of `nrows` rows of 8 partial sums by
+// one block of kFinThreads threads.  Thread t owns column t % 8 of rows
+// t / 8 + (kFinThreads / 8) j and issues its loads 16 at a time before adding
+// them; the per-thread (sum, compensation) pairs of a column then merge in a
+// fixed tree order.
+constexpr int kFinThreads = 1024;
+__global__ void __launch_bounds__(kFinThreads) k_finalize(const double* __restrict__ partials, int nrows,
+                                                          double* __restrict__ res) {
+  constexpr int RS = kFinThreads / 8, U = 16;  // rows per sweep, loads in flight per thread
+  __shared__ double sh[kFinThreads];
+  __shared__ double shc[kFinThreads];
+  const int t = threadIdx.x, comp = t & 7, row = t >> 3;
+  double s = 0.0, c = 0.0;
+  for (int b0 = row; b0 < nrows; b0 += U * RS) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * RS;
+      v[u] = b < nrows ? partials[static_cast<long>(b) * 8 + comp] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) two_sum(s, c, v[u]);
+  }
   sh[t] = s;
   shc[t] = c;
   __syncthreads();
-  for (int w = NT / 16; w > 0; w >>= 1) {
+  for (int w = RS / 2; w > 0; w >>= 1) {
     if (row < w) {
       double s2 = sh[t], c2 = shc[t] + shc[t + 8 * w];
       two_sum(s2, c2, sh[t + 8 * w]);
@@ -567,54 +583,7 @@ __device__ __forceinline__ void merge_columns(double& s, double& c) {
     }
     __syncthreads();
   }
-  s = sh[t & 7];
-  c = shc[t & 7];
-}
-
-// Deterministic compensated column sums of `nrows` rows of 8 partial sums.
-// Block b reduces a contiguous range of rows (thread t: column t % 8, rows
-// t / 8 + (NT / 8) j, eight loads issued before they are added) into one
-// (sum, compensation) row of the scratch; the last block to arrive reduces
-// those rows in block order, so the result does not depend on which block
-// finishes last.
-__global__ void __launch_bounds__(NT) k_finalize(const double* __restrict__ partials, int nrows, FinScratch fs,
-                                                 double* __restrict__ res) {
-  constexpr int RS = NT / 8;  // rows per sweep
-  __shared__ bool last;
-  const int t = threadIdx.x, comp = t & 7, row = t >> 3;
-  const int per = (nrows + static_cast<int>(gridDim.x) - 1) / static_cast<int>(gridDim.x);
-  const int r0 = static_cast<int>(blockIdx.x) * per, r1 = min(nrows, r0 + per);
-  double s = 0.0, c = 0.0;
-  for (int b0 = r0 + row; b0 < r1; b0 += 8 * RS) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      const int b = b0 + u * RS;
-      v[u] = b < r1 ? partials[static_cast<long>(b) * 8 + comp] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < 8; ++u) two_sum(s, c, v[u]);
-  }
-  merge_columns(s, c);
-  if (t < 8) {
-    fs.rows[blockIdx.x * 16 + t] = s;
-    fs.rows[blockIdx.x * 16 + 8 + t] = c;
-  }
-  __threadfence();
-  __syncthreads();
-  if (t == 0) last = atomicAdd(fs.counter, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  s = 0.0;
-  c = 0.0;
-  for (int b = row; b < static_cast<int>(gridDim.x); b += RS) {
-    two_sum(s, c, fs.rows[b * 16 + comp]);
-    c += fs.rows[b * 16 + 8 + comp];
-  }
-  merge_columns(s, c);
-  if (t < 8) res[t] = s + c;
-  if (t == 0) *fs.counter = 0u;  // ready for the next call
+  if (t < 8) res[t] = sh[t] + shc[t];
 }
 
 // ---------------------------------------------------------------- launchers
@@ -766,16 +735,8 @@ void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* loc
 #undef CDFT_ZP
 }
 
-FinScratch fin_scratch(void* base) {
-  auto* b = static_cast<unsigned char*>(base);
-  return {reinterpret_cast<double*>(b), reinterpret_cast<unsigned*>(b + kFinBlocks * 16 * sizeof(double))};
-}
-
-void launch_finalize(const double* partials, int nrows, FinScratch fs, double* res, hipStream_t st) {
-  constexpr int RS = NT / 8;
-  int nb = (nrows + RS - 1) / RS;
-  nb = nb < 1 ? 1 : (nb > kFinBlocks ? kFinBlocks : nb);
-  hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(NT), 0, st, partials, nrows, fs, res);
+void launch_finalize(const double* partials, int nrows, double* res, hipStream_t st) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, st, partials, nrows, res);
 }
 
 #define CDFT_INST_SC(S, C)                                                                                         \

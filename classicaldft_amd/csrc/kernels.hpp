@@ -75,18 +75,7 @@ template <typename S, typename CF, typename CB>
 void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* local_force, double* partials,
                    const Geom& g, const FftPlan& pz, const cx<CF>* twf, const cx<CB>* twb, ChannelSet cs,
                    FmtParams fp, double mu, AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st);
-// Scratch of launch_finalize: per-block (sum, compensation) rows and the
-// blocks' arrival counter, which is zero between calls (the last block of
-// each call resets it).  One scratch per stream: calls on one stream run in
-// order, calls that overlap must not share it.
-struct FinScratch {
-  double* rows;
-  unsigned* counter;
-};
-constexpr int kFinBlocks = 64;
-constexpr size_t kFinScratchBytes = kFinBlocks * 16 * sizeof(double) + 256;
-FinScratch fin_scratch(void* base);  // carve a kFinScratchBytes buffer (counter zeroed by the caller)
 // res[0..7] = deterministic compensated column sums of nrows rows of 8 doubles.
-void launch_finalize(const double* partials, int nrows, FinScratch fs, double* res, hipStream_t st);
+void launch_finalize(const double* partials, int nrows, double* res, hipStream_t st);
 
 }  // namespace cdft
