@@ -367,14 +367,16 @@ void Context::stage_bytes(double* b, bool with_vext) const {
   for (int i = 0; i < kNumStages; ++i) b[i] = 0.0;
   const double nf = cs.nf, nb = cs.nb;
   if (fast) {
+    // plane channels: 10 forward (n_v,z and T_zz are formed in the z-line kernel), 9 backward
+    const double nfp = cs.nf == 12 ? 10.0 : (cs.nf == 11 ? 9.0 : nf), nbp = cs.nb > 0 ? 9.0 : 0.0;
     if (cs.nf > 0) {
-      b[0] = 8.0 * N + sf * M;                         // rho -> rho_hat parts
+      b[0] = 8.0 * N + sf * M + 8.0 * g.nx * g.ny;     // rho -> rho_hat parts (+ FP64 kz = 0 plane)
       b[1] = sf * M + 16.0 * M;                        // parts -> FP64 spectrum
-      b[2] = 16.0 * M + (cs.hs ? 16.0 * M : 0.0) + nf * sf * M + (cs.mf ? 8.0 * M : 0.0);  // spectrum, weights -> channels
+      b[2] = 16.0 * M + (cs.hs ? 16.0 * M : 0.0) + nfp * sf * M + (cs.mf ? 8.0 * M : 0.0);  // spectrum, weights -> channels
     }
-    b[3] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;
+    b[3] = nfp * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nbp * sf * M + 8.0 * N;
     if (cs.nb > 0) {
-      b[4] = nb * sf * M + 16.0 * M + 2.0 * sf * M;    // nb channels, weights -> partial sums (h = 0, 1)
+      b[4] = nbp * sf * M + 16.0 * M + 2.0 * sf * M;   // channels, weights -> partial sums (h = 0, 1)
       b[5] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
     }
     return;

@@ -1,14 +1,15 @@
 // Fused five-kernel evaluation schedule (gfx950) for power-of-two grids.
 //
 //   kf_zfwd       z-R2C of rho - mean (FP64), y-parity split    rho -> A0 (chunked split layout)
-//   kf_plane_fwd  per (kz, r, h): y- and x-FFT of rho_hat (FP64), then for
-//                 every forward channel: weight, x-inverse of the output
-//                 half ix = 2i + h, y-inverse, written in the split layout
-//   kf_zpoint     per line pair: y-parity combine + z-C2R of every channel,
+//   kf_plane_fwd  per (kz, r, h): for each of the 10 plane channels: weight
+//                 times rho_hat, x-inverse of the output half ix = 2i + h,
+//                 y-inverse, written in the split layout
+//   kf_zpoint     per line pair: y-parity combine + z-C2R of every channel
+//                 (n_v,z from eta's spectrum, T_zz by the trace identity),
 //                 pointwise Phi/dPhi + local terms + partial sums, z-R2C of
-//                 the 11 derivative channels + y-parity split (in place)
+//                 the derivative channels + y-parity split (in place, 9 out)
 //   kf_plane_bwd  per (kz, r, h): y-FFT and half x-FFT (rows ix = 2i + h) of
-//                 every derivative channel, conj-weighted accumulation over
+//                 the 9 derivative channels, conj-weighted accumulation over
 //                 the full kx range, x- and y-inverse of the sum -> partial h
 //   kf_zinv       per line pair: sum of the two partials, y-parity combine,
 //                 z-C2R, * dV + local force                    -> force
@@ -57,25 +58,61 @@ __device__ __forceinline__ long split_idx(int kz, int r, int ix, int m, int mb, 
   return ((static_cast<long>(ix) * mb + (m >> 4)) * nzh + kz) * 32 + r * 16 + (m & 15);
 }
 
-// Forward channels in the order the z-line kernel transforms them: the FP64
-// (scalar) channels eta, n2 and the mean field first.
-__host__ __device__ constexpr int zorder(int nf, int pos) {
-  return nf == 1 ? 11 : (nf == 12 ? (pos < 2 ? pos : (pos == 2 ? 11 : pos - 1)) : pos);
+// Channel bookkeeping of the fused schedule.  Two identities of the FMT
+// weights (weights.hpp:195-222) spare four plane transforms per evaluation:
+//   n_v,z = i kz w3 rho_hat = i kz eta_hat, which the z-line kernel forms
+//     from eta's z-spectrum (kz is fixed along a z-line's spectrum), and
+//   T_zz = n2 - T_xx - T_yy, since sum_i (aik2 k_i^2 + w3/R) = w2;
+// and, for the back-convolution (hard_sphere.hpp:115-134), their adjoints:
+//   dPhi/dT_zz folds into the n2, T_xx and T_yy derivatives before the
+//     transforms, and the z-vector derivative's z-spectrum, times -i kz,
+//     into eta's.
+// So the forward plane kernel produces 10 channels (eta, n2, mf, v_x, v_y,
+// T_xx, T_xy, T_xz, T_yy, T_yz) and the backward one consumes 9.
+//
+// Forward slot positions of the z-line kernel (round pos / NS, slot pos % NS):
+// the channel staged there, kVzFromEta for the slot that builds n_v,z from
+// eta's staged spectrum (round 0, slot 0), -1 for an idle slot.  The FP64
+// (scalar) channels eta, n2 and the mean field come first.
+//   nf 12: eta n2 mf vz* vx vy Txx Txy Txz Tyy Tyz      nf 11: eta n2 vz* vx vy Txx Txy Txz Tyy Tyz
+constexpr int kVzFromEta = -2;
+__host__ __device__ constexpr int zf_chan(int nf, int pos) {
+  return nf == 1    ? (pos == 0 ? 11 : -1)
+         : nf == 12 ? (pos < 2 ? pos : pos == 2 ? 11 : pos == 3 ? kVzFromEta : pos < 6 ? pos - 2 : pos < 11 ? pos - 1 : -1)
+         : nf == 11 ? (pos < 2 ? pos : pos == 2 ? kVzFromEta : pos < 5 ? pos - 1 : pos < 10 ? pos : -1)
+                    : -1;
 }
+// positions holding a channel value
+__host__ __device__ constexpr int zf_count(int nf) { return nf == 12 ? 11 : (nf == 11 ? 10 : nf); }
+// position of channel ch (-1: T_zz, formed pointwise)
 __host__ __device__ constexpr int zpos(int nf, int ch) {
-  return nf == 1 ? 0 : (nf == 12 ? (ch < 2 ? ch : (ch == 11 ? 2 : ch + 1)) : ch);
+  return nf == 1    ? (ch == 11 ? 0 : -1)
+         : nf == 12 ? (ch < 2 ? ch : ch == 11 ? 2 : ch == 4 ? 3 : ch < 4 ? ch + 2 : ch < 10 ? ch + 1 : -1)
+                    : (ch < 2 ? ch : ch == 4 ? 2 : ch < 4 ? ch + 1 : ch < 10 ? ch : -1);
 }
+// Backward positions: the derivative channel R2C-transformed at position b;
+// b = 1 (the z vector) is folded into b = 0 (eta) in the same round.
+constexpr int kZB = 10;
+__host__ __device__ constexpr int zb_chan(int b) { return b == 0 ? 0 : b == 1 ? 4 : b < 5 ? b - 1 : b < 10 ? b : -1; }
 __host__ __device__ constexpr bool zscalar(int ch) { return ch == 0 || ch == 1 || ch == 11; }
 // Storage slot of channel ch (in the channel arrays and the k = 0 table): the
 // mean field is slot 11 after the FMT channels, or slot 0 without them.
 __host__ __device__ constexpr int chan_store(int nf, int ch) { return nf == 1 ? 0 : ch; }
+// FP64 rounds: those with a scalar channel (or n_v,z, formed next to eta)
 __host__ __device__ constexpr bool zround64(int nf, int ns, int jr) {
   for (int g = 0; g < ns; ++g) {
-    const int pos = jr * ns + g;
-    if (pos < nf && zscalar(zorder(nf, pos))) return true;
+    const int c = zf_chan(nf, jr * ns + g);
+    if (c == kVzFromEta || (c >= 0 && zscalar(c))) return true;
   }
   return false;
 }
+// Plane channels: forward (eta, n2, mf first), backward (v_z and T_zz folded away).
+__host__ __device__ constexpr int fwd_nplane(int nf) { return nf == 12 ? 10 : (nf == 11 ? 9 : nf); }
+__host__ __device__ constexpr int fwd_channel(int nf, int pos) {
+  return nf == 1 ? 11 : (nf == 12 ? (pos < 2 ? pos : pos == 2 ? 11 : pos < 5 ? pos - 1 : pos) : (pos < 4 ? pos : pos + 1));
+}
+constexpr int kBwdPlaneCh = 9;
+__host__ __device__ constexpr int bwd_channel(int i) { return i < 4 ? i : i + 1; }
 
 template <typename C>
 __device__ __forceinline__ cx<C> cneg(cx<C> a) {
@@ -276,15 +313,23 @@ __host__ __device__ constexpr int zpoint_smem() {
 
 // One forward round: slot `slot` of wave `wave` transforms channel position
 // jr*NS + slot of line pair `wave` (C2R), then the wave redistributes the
-// results so that lane lid holds points z = lid + 64 j of both lines.
-template <typename C, typename S, int NZ, int NF, int JR, int PPL>
-__device__ __forceinline__ void zpoint_fwd_round(S (&wv)[NF > 0 ? NF : 1][PPL], const cx<S>* stg, unsigned char* xwave,
-                                                 int wave, int lid, const cx<C>* __restrict__ tw) {
+// results so that lane lid holds points z = lid + 64 j of both lines.  The
+// n_v,z slot of round 0 transforms i kz times eta's staged spectrum.
+template <typename C, typename S, int NZ, int NF, int JR, int PPL, int NWV>
+__device__ __forceinline__ void zpoint_fwd_round(S (&wv)[NWV][PPL], const cx<S>* stg, unsigned char* xwave, int wave,
+                                                 int lid, const cx<C>* __restrict__ tw, double dkz) {
   constexpr int T = NZ / 8, NS = 64 / T, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   const int slot = lid / T, lane = lid % T;
   cx<C>* xw = reinterpret_cast<cx<C>*>(xwave);
+  const bool vz = NF >= 11 && JR == 0 && slot == zpos(NF, 4);
+  const int src = vz ? zpos(NF, 0) : slot;
+  const C dk = static_cast<C>(dkz);
   cx<C> v[8];
-  build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cvt<C>(stg[((slot * 2 + r) * 4 + wave) * NZH + kz]); });
+  build_c2r<C, NZ>(v, lane, [&](int r, int kz) {
+    const cx<C> b = cvt<C>(stg[((src * 2 + r) * 4 + wave) * NZH + kz]);
+    const C k = static_cast<C>(kz) * dk;
+    return vz ? cx<C>{-b.im * k, b.re * k} : b;
+  });
   slot_fft<C, NZ, T, +1>(v, xw + slot * XS, lane, tw);
   wave_sync();
 #pragma unroll
@@ -293,7 +338,7 @@ __device__ __forceinline__ void zpoint_fwd_round(S (&wv)[NF > 0 ? NF : 1][PPL], 
 #pragma unroll
   for (int gs = 0; gs < NS; ++gs) {
     const int pos = JR * NS + gs;
-    if (pos < NF) {
+    if (pos < zf_count(NF)) {
 #pragma unroll
       for (int j = 0; j < PPL / 2; ++j) {
         const cx<C> t = xw[gs * (8 * T) + lid + 64 * j];
@@ -304,11 +349,12 @@ __device__ __forceinline__ void zpoint_fwd_round(S (&wv)[NF > 0 ? NF : 1][PPL], 
   }
 }
 
-// One backward round: derivative channels JR*NS + slot of line pair `wave`
-// go through the z-R2C and the y-parity split into the staging buffer.
-template <typename C, typename S, int NZ, int NF, int NB, int JR, int PPL>
-__device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NF > 0 ? NF : 1][PPL], cx<S>* stg, unsigned char* xwave,
-                                                 int wave, int lid, const cx<C>* __restrict__ tw) {
+// One backward round: the derivative channels at positions JR*NS + slot
+// (wv[b] holds channel zb_chan(b)) of line pair `wave` go through the z-R2C
+// and the y-parity split into the staging buffer.
+template <typename C, typename S, int NZ, int JR, int PPL, int NWV>
+__device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NWV][PPL], cx<S>* stg, unsigned char* xwave, int wave,
+                                                 int lid, const cx<C>* __restrict__ tw) {
   constexpr int T = NZ / 8, NS = 64 / T, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   const int slot = lid / T, lane = lid % T;
   cx<C>* xw = reinterpret_cast<cx<C>*>(xwave);
@@ -319,7 +365,7 @@ __device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NF > 0 ? NF : 1][
 #pragma unroll
     for (int j = 0; j < PPL / 2; ++j) {
       cx<C> t = {C(0), C(0)};
-      if (c < NB) t = {static_cast<C>(wv[zpos(NF, c)][2 * j]), static_cast<C>(wv[zpos(NF, c)][2 * j + 1])};
+      if (c < kZB) t = {static_cast<C>(wv[c][2 * j]), static_cast<C>(wv[c][2 * j + 1])};
       xw[gs * (8 * T) + lid + 64 * j] = t;
     }
   }
@@ -355,8 +401,9 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
                                                  const double* __restrict__ shift_sum, DcTable dc) {
   constexpr int T = NZ / 8, NS = 64 / T, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   constexpr int PPL = NZ / 32;  // points per lane: z = lid + 64 j of both lines
-  constexpr int NFA = NF > 0 ? NF : 1;
-  constexpr int NRF = (NF + NS - 1) / NS, NRB = (NB + NS - 1) / NS;
+  constexpr int NWV = zf_count(NF) > 0 ? zf_count(NF) : 1;
+  constexpr int NRF = (zf_count(NF) + NS - 1) / NS, NRB = NB > 0 ? (kZB + NS - 1) / NS : 0;
+  static_assert(NB == 0 || (NF >= 11 && NWV >= kZB), "derivative channels need the FMT positions");
   constexpr bool HS = NF >= 11;
   constexpr bool MF = (NF == 12 || NF == 1);
   extern __shared__ __align__(16) unsigned char smem[];
@@ -371,7 +418,7 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
   const long cbase = chunk * static_cast<long>(NZH * 32) + qb * 4;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
-  S wv[NFA][PPL];
+  S wv[NWV][PPL];
   // twiddle tables in LDS; the first barrier below publishes them
   const cx<double>* tw64l = stage_twiddles<double, kZP>(smem + zpoint_tw_off<NZ, S>(), tw64);
   const cx<CV>* twvl = stage_twiddles<CV, kZP>(smem + zpoint_tw_off<NZ, S>() + tw_bytes<double>(), twv);
@@ -383,10 +430,9 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
         [&](int i) {
           const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
           const int kz = t % NZH, gs = t / NZH;
-          const int pos = jr * NS + gs;
+          const int ch = zf_chan(NF, jr * NS + gs);
           cx<S> val = {S(0), S(0)};
-          if (pos < NF)
-            val = SF[static_cast<long>(chan_store(NF, zorder(NF, pos))) * g.M + cbase + (kz * 2 + r) * 16 + p];
+          if (ch >= 0) val = SF[static_cast<long>(chan_store(NF, ch)) * g.M + cbase + (kz * 2 + r) * 16 + p];
           return val;
         },
         [&](int i, cx<S> val) {
@@ -400,9 +446,9 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
     stage_in(JR);                                                                                     \
     __syncthreads();                                                                                  \
     if constexpr (zround64(NF, NS, JR))                                                               \
-      zpoint_fwd_round<double, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, tw64l);                 \
+      zpoint_fwd_round<double, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, tw64l, g.dkz);          \
     else                                                                                              \
-      zpoint_fwd_round<CV, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, twvl);                      \
+      zpoint_fwd_round<CV, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, twvl, g.dkz);               \
     __syncthreads();                                                                                  \
   }
   CDFT_ZF(0)
@@ -421,17 +467,17 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   for (int pt = 0; pt < PPL; ++pt) {
     const int z = lid + 64 * (pt >> 1);
     const long p = ((pt & 1) ? lb : la) * NZ + z;
-    S cur[NFA];
+    S cur[NWV];
 #pragma unroll
-    for (int i = 0; i < NF; ++i) {
+    for (int i = 0; i < NWV; ++i) {
       S t = wv[i][0];
 #pragma unroll
       for (int j = 1; j < PPL; ++j) t = (pt == j) ? wv[i][j] : t;
       cur[i] = t;
     }
-    S outv[11];
+    S outv[kZB];
 #pragma unroll
-    for (int i = 0; i < 11; ++i) outv[i] = S(0);
+    for (int i = 0; i < kZB; ++i) outv[i] = S(0);
     const double rho = amap.apply(rho_in[p]);
     const double lr = log(fmax(rho, 1e-300));
     double lfv = (lr - mu) * dv;  // ideal_gas.hpp:312-318
@@ -450,18 +496,23 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
     if constexpr (HS) {
       double w[11], gg[11];
 #pragma unroll
-      for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(cur[zpos(NF, i)]) + dc.v[i] * shift;
+      for (int i = 0; i < 10; ++i) w[i] = static_cast<double>(cur[zpos(NF, i)]) + dc.v[i] * shift;
+      w[10] = w[1] - w[5] - w[8];  // T_zz = n2 - T_xx - T_yy
       const double phi = fmt_phi_grad(fp, w, gg);
       two_sum(acc[2], cmp[2], phi);
+      // adjoint of the T_zz identity
+      gg[1] += gg[10];
+      gg[5] -= gg[10];
+      gg[8] -= gg[10];
 #pragma unroll
-      for (int i = 0; i < 11; ++i) outv[i] = static_cast<S>(gg[i]);
+      for (int b = 0; b < kZB; ++b) outv[b] = static_cast<S>(gg[zb_chan(b)]);
     }
     lf[p] = lfv;
     if constexpr (NB > 0) {
 #pragma unroll
-      for (int i = 0; i < 11; ++i)
+      for (int b = 0; b < kZB; ++b)
 #pragma unroll
-        for (int j = 0; j < PPL; ++j) wv[zpos(NF, i)][j] = (pt == j) ? outv[i] : wv[zpos(NF, i)][j];
+        for (int j = 0; j < PPL; ++j) wv[b][j] = (pt == j) ? outv[b] : wv[b][j];
     }
   }
   // Partial sums: each wave reduces its lanes and parks the totals in LDS;
@@ -498,23 +549,31 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   }
 
   // ---- backward rounds: R2C of NS derivative channels, split, coalesced store
+  // eta's derivative spectrum takes -i kz times the z vector's (positions 0 and 1)
+  const S dks = static_cast<S>(g.dkz);
   auto stage_out = [&](int jr) {
     block_copy<kZP, NS * NZH * 8, cx<S>>(
         [&](int i) {
           const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
           const int kz = t % NZH, gs = t / NZH;
-          return stg[((gs * 2 + r) * 4 + p) * NZH + kz];
+          cx<S> v = stg[((gs * 2 + r) * 4 + p) * NZH + kz];
+          if (jr == 0 && gs == 0) {
+            const cx<S> vz = stg[((1 * 2 + r) * 4 + p) * NZH + kz];
+            const S k = static_cast<S>(kz) * dks;
+            v = {v.re + k * vz.im, v.im - k * vz.re};
+          }
+          return v;
         },
         [&](int i, cx<S> val) {
           const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
           const int kz = t % NZH, gs = t / NZH;
-          const int c = jr * NS + gs;
-          if (c < NB) SF[static_cast<long>(c) * g.M + cbase + (kz * 2 + r) * 16 + p] = val;
+          const int b = jr * NS + gs;
+          if (b < kZB && b != 1) SF[static_cast<long>(zb_chan(b)) * g.M + cbase + (kz * 2 + r) * 16 + p] = val;
         });
   };
 #define CDFT_ZB(JR)                                                                                           \
   if constexpr ((JR) < NRB) {                                                                                 \
-    zpoint_bwd_round<CV, S, NZ, NF, NB, JR, PPL>(wv, stg, xwave, wave, lid, twvl);                            \
+    zpoint_bwd_round<CV, S, NZ, JR, PPL>(wv, stg, xwave, wave, lid, twvl);                                    \
     __syncthreads();                                                                                          \
     if constexpr ((JR) == 0) write_row();                                                                     \
     stage_out(JR);                                                                                            \
@@ -771,10 +830,6 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
 // blocks take contiguous ranges of items (at most two units each) instead of
 // one unit each: the last round of blocks then holds one item more at most.
 __host__ __device__ constexpr int fwd_nf64(int nf) { return nf == 12 ? 3 : (nf == 11 ? 2 : nf); }
-__host__ __device__ constexpr int fwd_channel(int nf, int pos) {
-  // nf 12: eta, n2, mf, 2..10; nf 11: eta, n2, 2..10; nf 1: mf
-  return nf == 1 ? 11 : (pos < 2 ? pos : (nf == 12 ? (pos == 2 ? 11 : pos - 1) : pos));
-}
 
 template <typename S, typename CV, int NX, int Q, int NF>
 __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict__ XR,
@@ -794,7 +849,8 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
   const cx<CV>* twvl = stage_twiddles<CV, kBT>(smem + FC::tw_off + tw_bytes<double>(), twv);
   const int tid = threadIdx.x;
   const int xs = tid / P::TX, xl = tid % P::TX;
-  const long items = static_cast<long>(nunits) * NF;
+  constexpr int NFP = fwd_nplane(NF);
+  const long items = static_cast<long>(nunits) * NFP;
   // consecutive item ranges (the two halves h of one (kz, r) slice read the
   // same spectrum and weights) go to blocks of one XCD, which share its L2
   const long lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -803,9 +859,9 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
   __syncthreads();  // twiddles
   int nch = 0;
   for (long it = i0; it < i1;) {
-    const int unit = static_cast<int>(it / NF);
-    const int p0 = static_cast<int>(it - static_cast<long>(unit) * NF);
-    const int p1 = static_cast<int>(min(static_cast<long>(NF), p0 + (i1 - it)));
+    const int unit = static_cast<int>(it / NFP);
+    const int p0 = static_cast<int>(it - static_cast<long>(unit) * NFP);
+    const int p1 = static_cast<int>(min(static_cast<long>(NFP), p0 + (i1 - it)));
     it += p1 - p0;
     const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
     // FP64 spectrum of the lane's elements, read where it is used (L1/L2):
@@ -905,19 +961,20 @@ struct BwdCfg {
 };
 
 // Work of the backward plane kernel: (unit, channel) items as in the forward
-// kernel, 11 channels per unit.  A block accumulates the channels of its range
-// per unit and writes the inverse transform of each accumulated part: the part
-// that starts at channel 0 into partial set 0, a continuation (the unit's
-// remaining channels, in the next block) into set 1.  Every range holds at
-// least 11 items (blocks <= units), so a unit has at most two parts;
+// kernel, kBwdPlaneCh channels per unit.  A block accumulates the channels of
+// its range per unit and writes the inverse transform of each accumulated
+// part: the part that starts at channel 0 into partial set 0, a continuation
+// (the unit's remaining channels, in the next block) into set 1.  Every range
+// holds at least kBwdPlaneCh items (blocks <= units), so a unit has at most
+// two parts;
 // bwd_unit_split() tells the consumer which units have a set-1 part.
 __host__ __device__ inline int bwd_block_of_item(long item, long items, int nblk) {
   return static_cast<int>(((item + 1) * nblk - 1) / items);
 }
 __host__ __device__ inline bool bwd_unit_split(int unit, int nunits, int nblk) {
-  const long items = static_cast<long>(nunits) * 11;
-  return bwd_block_of_item(static_cast<long>(unit) * 11, items, nblk) !=
-         bwd_block_of_item(static_cast<long>(unit) * 11 + 10, items, nblk);
+  const long items = static_cast<long>(nunits) * kBwdPlaneCh;
+  return bwd_block_of_item(static_cast<long>(unit) * kBwdPlaneCh, items, nblk) !=
+         bwd_block_of_item(static_cast<long>(unit) * kBwdPlaneCh + kBwdPlaneCh - 1, items, nblk);
 }
 
 template <typename S, typename CV, int NX, int Q>
@@ -941,21 +998,21 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   const int xs = tid / PC::TX, xl = tid % PC::TX;
   const int ysh = tid / PC::TYH, ylh = tid % PC::TYH, ysf = tid / PC::TYF, ylf = tid % PC::TYF;
   const CV invR = static_cast<CV>(1.0 / g.R);
-  const long items = static_cast<long>(nunits) * 11;
+  const long items = static_cast<long>(nunits) * kBwdPlaneCh;
   const long lb = xcd_remap(blockIdx.x, gridDim.x);  // neighbouring ranges share an XCD's L2
   const long i0 = items * lb / gridDim.x, i1 = items * (lb + 1) / gridDim.x;
   __syncthreads();  // twiddles
   for (long it = i0; it < i1;) {
-    const int unit = static_cast<int>(it / 11);
-    const int c0 = static_cast<int>(it - static_cast<long>(unit) * 11);
-    const int c1 = static_cast<int>(min(11L, c0 + (i1 - it)));
+    const int unit = static_cast<int>(it / kBwdPlaneCh);
+    const int c0 = static_cast<int>(it - static_cast<long>(unit) * kBwdPlaneCh);
+    const int c1 = static_cast<int>(min(static_cast<long>(kBwdPlaneCh), c0 + (i1 - it)));
     it += c1 - c0;
     const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
     const double kzv = kz * g.dkz;
 
     cx<S> nxt[PC::RPH][4];
-    auto load_in = [&](int ch) {
-      const cx<S>* in = SB + static_cast<long>(ch) * g.M;
+    auto load_in = [&](int ci) {
+      const cx<S>* in = SB + static_cast<long>(bwd_channel(ci)) * g.M;
 #pragma unroll
       for (int rr = 0; rr < PC::RPH; ++rr) {
         const int i = ysh + PC::SBYH * rr;
@@ -1026,8 +1083,8 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
             cx<CV> oa = f[e];
             if (h) oa = twiddle<CV, NX, -1>(oa, xl + PC::TX * e, twl);
             const cx<CV> ob = h ? cneg(oa) : oa;
-            const cx<CV> wa = chan_weight<CV>(ch, true, kxc[e], kyv, kzc, wf[cc][e], invR);
-            const cx<CV> wb = chan_weight<CV>(ch, true, kxc[e + 4], kyv, kzc, wf[cc][e + 4], invR);
+            const cx<CV> wa = chan_weight<CV>(bwd_channel(ch), true, kxc[e], kyv, kzc, wf[cc][e], invR);
+            const cx<CV> wb = chan_weight<CV>(bwd_channel(ch), true, kxc[e + 4], kyv, kzc, wf[cc][e + 4], invR);
             acc[cc][e] = cadd(acc[cc][e], cmul(wa, oa));
             acc[cc][e + 4] = cadd(acc[cc][e + 4], cmul(wb, ob));
           }
