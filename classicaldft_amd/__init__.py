@@ -116,7 +116,7 @@ def lib():
         L.cdft_minimize_grand_potential.argtypes = [C.c_void_p, C.POINTER(_Fire), C.POINTER(_Param), C.c_int, _dp,
                                                     _dp, C.c_double, _dp, C.POINTER(_MinResult), _dp]
         L.cdft_ddft_run.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp,
-                                    C.POINTER(_SimResult), _dp]
+                                    C.POINTER(_SimResult), _dp, _dp]
         L.cdft_time_total_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
                                              _dp]
         L.cdft_profile_stages.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int, _dp,
@@ -286,6 +286,6 @@ class Context:
         res = _SimResult()
         en = np.zeros(n_steps + 1) if record_energies else None
         _check(lib().cdft_ddft_run(self._h, C.byref(cfg), int(n_steps), None if vext is None else _p(_f64(vext)),
-                                   float(mu), _p(rho), C.byref(res), _p(en)))
+                                   float(mu), _p(rho), C.byref(res), _p(en), None))
         r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
         return rho.reshape(self.shape), r, en

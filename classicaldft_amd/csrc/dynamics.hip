@@ -629,7 +629,7 @@ int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire, const cd
 }
 
 int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
-                  double* rho, cdft_sim_result* result, double* energies) {
+                  double* rho, cdft_sim_result* result, double* energies, double* times) {
   return guarded([&] {
     check_ctx(ctx);
     if (!config || !rho || !result || n_steps < 0) throw std::invalid_argument("bad argument");
@@ -652,6 +652,7 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
     const double m0 = mass_of(run.rho0.as<double>());
     double e = run.compute(run.rho0.as<double>(), run.f0.as<double>());
     if (energies) energies[0] = e;
+    if (times) times[0] = 0.0;
     double dt = config->dt;
     int successes = 0;
     double time = 0.0;
@@ -668,6 +669,7 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
         successes = 0;
       }
       if (energies) energies[s] = e;
+      if (times) times[s] = time;
     }
     CDFT_HIP(hipMemcpyAsync(rho, run.rho0.p, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
     CDFT_HIP(hipStreamSynchronize(c->stream));

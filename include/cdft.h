@@ -200,9 +200,12 @@ typedef struct {
 /* Simulation::run with integrating_factor_step (dynamics.hpp:333-401,582-693)
  * for one species, force callback = grand_potential_callback(mu)
  * (functional.hpp:161-172), plus an optional external field.  rho is updated
- * in place; energies (nullable, n_steps + 1) receives the energy per step. */
+ * in place; energies and times (each nullable, n_steps + 1 values) receive
+ * the energy and the elapsed time after the initial state and after every
+ * step (SimulationResult::energies / times, recorded there every
+ * log_interval steps). */
 CDFT_API int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
-                  double* rho, cdft_sim_result* result, double* energies);
+                  double* rho, cdft_sim_result* result, double* energies, double* times);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,533 @@
+// C++ facade of the free-energy + gradient path over the C ABI of cdft.h.
+//
+// It restates the reference's C++ interfaces for this path, with
+// std::vector<double> (dft::vec) in place of arma::vec, so that code written
+// against the reference's
+//   dft::make_grid, dft::physics::Model, dft::functionals::{make_weights,
+//   total, make_functional, Functional}, dft::algorithms::fire::Fire,
+//   dft::algorithms::minimization::Minimizer and
+//   dft::algorithms::dynamics::Simulation
+// compiles against it with the container type changed.  Every evaluation
+// runs in libcdft.so on the GPU; errors surface as the exception types the
+// reference throws (std::invalid_argument for bad arguments, std::runtime_error
+// otherwise).  Header-only, C++17; link with -lcdft.
+//
+// Limits of the GPU path (std::invalid_argument when exceeded): one species,
+// at most one mean-field interaction, grid lengths with prime factors 2, 3, 5.
+#pragma once
+
+#include <array>
+#include <cmath>
+#include <functional>
+#include <memory>
+#include <optional>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <variant>
+#include <vector>
+
+#include "cdft.h"
+
+namespace dft {
+
+using vec = std::vector<double>;
+
+namespace detail {
+// Maps a cdft status to the reference's exception types.
+inline void check(int rc) {
+  if (rc == CDFT_OK) return;
+  const std::string msg = cdft_last_error();
+  if (rc == CDFT_EINVAL || rc == CDFT_EUNSUPPORTED) throw std::invalid_argument(msg);
+  throw std::runtime_error(msg);
+}
+}  // namespace detail
+
+// ------------------------------------------------------------------ grid.hpp:26-103
+struct Grid {
+  double dx;
+  std::array<double, 3> box_size;
+  std::array<long, 3> shape;
+
+  double cell_volume() const noexcept { return dx * dx * dx; }
+  long total_points() const noexcept { return shape[0] * shape[1] * shape[2]; }
+  long flat_index(long ix, long iy, long iz) const noexcept { return iz + shape[2] * (iy + shape[1] * ix); }
+};
+
+// Same validation and messages as the reference's make_grid (grid.hpp:87-103).
+inline Grid make_grid(double dx, std::array<double, 3> box) {
+  if (dx <= 0.0) throw std::invalid_argument("make_grid: dx must be positive, got " + std::to_string(dx));
+  std::array<long, 3> shape{};
+  for (int d = 0; d < 3; ++d) {
+    if (box[d] <= 0.0)
+      throw std::invalid_argument("make_grid: box dimension must be positive, got " + std::to_string(box[d]));
+    const double n = box[d] / dx;
+    const long ni = static_cast<long>(std::round(n));
+    if (std::abs(n - static_cast<double>(ni)) > 1e-10)
+      throw std::invalid_argument("make_grid: box dimension " + std::to_string(box[d]) +
+                                  " is not commensurate with dx=" + std::to_string(dx));
+    shape[d] = ni;
+  }
+  return Grid{dx, box, shape};
+}
+
+// ------------------------------------------------------------------ types.hpp:17-48
+struct Density {
+  vec values;
+  vec external_field;
+};
+struct Species {
+  std::string name;
+  double hard_sphere_diameter;
+};
+struct SpeciesState {
+  Density density;
+  vec force;
+  double chemical_potential{0.0};
+  std::optional<double> fixed_mass;
+};
+struct State {
+  std::vector<SpeciesState> species;
+  double temperature;
+};
+
+// ------------------------------------------------------------------ physics
+namespace physics {
+namespace potentials {
+// potentials.hpp:18-21
+enum class SplitScheme { WeeksChandlerAndersen, BarkerHenderson };
+// potentials.hpp:57-63, 122-128, 188-194 (parameters only: the attractive
+// tails and lattice weights are built inside the library)
+struct LennardJones {
+  double sigma{1.0}, epsilon{1.0}, r_cutoff{-1.0};
+};
+struct TenWoldeFrenkel {
+  double sigma{1.0}, epsilon{1.0}, r_cutoff{-1.0}, alpha{50.0};
+};
+struct WangRamirezDobnikarFrenkel {
+  double sigma{1.0}, epsilon{1.0}, r_cutoff{3.0};
+};
+using Potential = std::variant<LennardJones, TenWoldeFrenkel, WangRamirezDobnikarFrenkel>;
+
+inline LennardJones make_lennard_jones(double sigma, double epsilon, double r_cutoff = -1.0) {
+  return {sigma, epsilon, r_cutoff};
+}
+inline TenWoldeFrenkel make_ten_wolde_frenkel(double sigma, double epsilon, double r_cutoff = -1.0,
+                                              double alpha = 50.0) {
+  return {sigma, epsilon, r_cutoff, alpha};
+}
+inline WangRamirezDobnikarFrenkel make_wang_ramirez_dobnikar_frenkel(double sigma, double epsilon,
+                                                                     double r_cutoff = 3.0) {
+  return {sigma, epsilon, r_cutoff};
+}
+}  // namespace potentials
+
+// interactions.hpp:11-31
+enum class WeightScheme {
+  InterpolationZero,
+  InterpolationLinearE,
+  InterpolationLinearF,
+  InterpolationQuadraticF,
+  GaussE,
+  GaussF,
+};
+struct Interaction {
+  int species_i{0};
+  int species_j{0};
+  potentials::Potential potential;
+  potentials::SplitScheme split{potentials::SplitScheme::WeeksChandlerAndersen};
+  WeightScheme weight_scheme{WeightScheme::InterpolationQuadraticF};
+  int gauss_order{5};
+};
+
+// model.hpp:15-20
+struct Model {
+  Grid grid;
+  std::vector<Species> species;
+  std::vector<Interaction> interactions;
+  double temperature{1.0};
+};
+}  // namespace physics
+
+// ------------------------------------------------------------------ functionals
+namespace functionals {
+namespace fmt {
+// models.hpp:153-419: the model tags and FMTModel::from_name
+struct Rosenfeld {};
+struct RSLT {};
+struct WhiteBearI {};
+struct WhiteBearII {};
+struct EsFMT {
+  double A{1.0};
+  double B{0.0};
+};
+
+class FMTModel {
+ public:
+  using VariantType = std::variant<Rosenfeld, RSLT, WhiteBearI, WhiteBearII, EsFMT>;
+  FMTModel(Rosenfeld m) : data_(m) {}
+  FMTModel(RSLT m) : data_(m) {}
+  FMTModel(WhiteBearI m) : data_(m) {}
+  FMTModel(WhiteBearII m) : data_(m) {}
+  FMTModel(EsFMT m) : data_(m) {}
+
+  static FMTModel from_name(const std::string& name) {
+    if (name == "Rosenfeld") return Rosenfeld{};
+    if (name == "RSLT") return RSLT{};
+    if (name == "WhiteBearI") return WhiteBearI{};
+    if (name == "WhiteBearII") return WhiteBearII{};
+    if (name == "esFMT") return EsFMT{};
+    throw std::invalid_argument("Unknown FMT model: " + name);
+  }
+  const VariantType& variant() const { return data_; }
+
+  // cdft_options.fmt_model and the esFMT coefficients
+  int id() const { return static_cast<int>(data_.index()); }
+  double esfmt_a() const { return std::holds_alternative<EsFMT>(data_) ? std::get<EsFMT>(data_).A : 1.0; }
+  double esfmt_b() const { return std::holds_alternative<EsFMT>(data_) ? std::get<EsFMT>(data_).B : -1.0; }
+
+ private:
+  VariantType data_;
+};
+
+// The cdft_options.fmt_model values follow the variant order.
+static_assert(CDFT_FMT_ROSENFELD == 0 && CDFT_FMT_RSLT == 1 && CDFT_FMT_WHITEBEAR_I == 2 &&
+                  CDFT_FMT_WHITEBEAR_II == 3 && CDFT_FMT_ESFMT == 4,
+              "FMT model numbering");
+}  // namespace fmt
+
+// types.hpp:11-23
+struct Contribution {
+  double free_energy{0.0};
+  std::vector<vec> forces;
+};
+struct Result {
+  double free_energy{0.0};
+  double grand_potential{0.0};
+  std::vector<vec> forces;
+};
+
+// Precision of the device evaluation (no counterpart in the reference, which
+// is FP64 throughout): FP64, or FP32 fields and transforms with FP64
+// pointwise terms, reductions and state.
+enum class Precision { FP64 = CDFT_PRECISION_FP64, Mixed = CDFT_PRECISION_MIXED };
+
+// functionals.hpp:19-33: the weights of a model.  Here they live on the GPU,
+// in a cdft context, and Weights shares ownership of it.
+struct Weights {
+  fmt::FMTModel fmt_model = fmt::WhiteBearII{};
+  std::shared_ptr<cdft_ctx> ctx;
+  long n_points{0};
+};
+
+namespace detail {
+inline cdft_interaction to_c(const physics::Interaction& in) {
+  cdft_interaction c{};
+  c.species_i = in.species_i;
+  c.species_j = in.species_j;
+  c.split = in.split == physics::potentials::SplitScheme::WeeksChandlerAndersen ? 0 : 1;
+  c.weight_scheme = static_cast<int>(in.weight_scheme);
+  c.alpha = 50.0;
+  if (auto* lj = std::get_if<physics::potentials::LennardJones>(&in.potential)) {
+    c.potential = 0;
+    c.sigma = lj->sigma;
+    c.epsilon = lj->epsilon;
+    c.r_cutoff = lj->r_cutoff;
+  } else if (auto* tw = std::get_if<physics::potentials::TenWoldeFrenkel>(&in.potential)) {
+    c.potential = 1;
+    c.sigma = tw->sigma;
+    c.epsilon = tw->epsilon;
+    c.r_cutoff = tw->r_cutoff;
+    c.alpha = tw->alpha;
+  } else {
+    const auto& w = std::get<physics::potentials::WangRamirezDobnikarFrenkel>(in.potential);
+    c.potential = 2;
+    c.sigma = w.sigma;
+    c.epsilon = w.epsilon;
+    c.r_cutoff = w.r_cutoff;
+  }
+  return c;
+}
+
+inline bool all_zero(const vec& v) {
+  for (double x : v)
+    if (x != 0.0) return false;
+  return true;
+}
+}  // namespace detail
+
+// functionals.hpp:27-33 (+ the grid-synchronised a_vdw of functional.hpp:195-206).
+inline Weights make_weights(const fmt::FMTModel& fmt_model, const physics::Model& model,
+                            Precision precision = Precision::FP64, int device = 0) {
+  if (model.species.size() != 1) throw std::invalid_argument("the GPU path evaluates one species");
+  cdft_grid g{model.grid.dx, {model.grid.box_size[0], model.grid.box_size[1], model.grid.box_size[2]}};
+  std::vector<double> diam;
+  for (const auto& s : model.species) diam.push_back(s.hard_sphere_diameter);
+  std::vector<cdft_interaction> inter;
+  for (const auto& i : model.interactions) inter.push_back(detail::to_c(i));
+  cdft_options o{};
+  o.fmt_model = fmt_model.id();
+  o.esfmt_A = fmt_model.esfmt_a();
+  o.esfmt_B = fmt_model.esfmt_b();
+  o.precision = static_cast<int>(precision);
+  o.device = device;
+  o.schedule = CDFT_SCHEDULE_AUTO;
+  cdft_ctx* raw = nullptr;
+  ::dft::detail::check(cdft_ctx_create(&g, diam.data(), static_cast<int>(diam.size()), inter.data(),
+                                       static_cast<int>(inter.size()), model.temperature, &o, &raw));
+  return Weights{fmt_model, std::shared_ptr<cdft_ctx>(raw, cdft_ctx_destroy), model.grid.total_points()};
+}
+
+// functionals.hpp:64-95: Omega, F and delta(beta Omega)/delta rho * dV of a
+// one-species state (its external field included when non-zero).
+inline Result total(const physics::Model& model, const State& state, const Weights& weights) {
+  if (state.species.size() != 1) throw std::invalid_argument("the GPU path evaluates one species");
+  const SpeciesState& s = state.species[0];
+  const long n = model.grid.total_points();
+  if (static_cast<long>(s.density.values.size()) != n)
+    throw std::invalid_argument("density has " + std::to_string(s.density.values.size()) + " values, grid has " +
+                                std::to_string(n));
+  const bool has_v = !s.density.external_field.empty() && !detail::all_zero(s.density.external_field);
+  if (has_v && static_cast<long>(s.density.external_field.size()) != n)
+    throw std::invalid_argument("external field size does not match the grid");
+  Result r;
+  r.forces.assign(1, vec(static_cast<size_t>(n)));
+  cdft_result cr{};
+  ::dft::detail::check(cdft_total(weights.ctx.get(), s.density.values.data(),
+                                  has_v ? s.density.external_field.data() : nullptr, s.chemical_potential, &cr,
+                                  r.forces[0].data()));
+  r.free_energy = cr.free_energy;
+  r.grand_potential = cr.grand_potential;
+  return r;
+}
+
+namespace init {
+// init.hpp:28-58
+inline State from_profile(const physics::Model& model, const vec& rho) {
+  const long n = model.grid.total_points();
+  State st{{}, model.temperature};
+  st.species.push_back(
+      SpeciesState{Density{rho, vec(static_cast<size_t>(n), 0.0)}, vec(static_cast<size_t>(n), 0.0), 0.0, std::nullopt});
+  return st;
+}
+inline State from_profiles(const physics::Model& model, const std::vector<vec>& profiles) {
+  const long n = model.grid.total_points();
+  State st{{}, model.temperature};
+  for (const auto& rho : profiles)
+    st.species.push_back(
+      SpeciesState{Density{rho, vec(static_cast<size_t>(n), 0.0)}, vec(static_cast<size_t>(n), 0.0), 0.0, std::nullopt});
+  return st;
+}
+}  // namespace init
+}  // namespace functionals
+
+namespace algorithms {
+namespace dynamics {
+// dynamics.hpp:21
+using ForceCallback = std::function<std::pair<double, std::vector<vec>>(const std::vector<vec>&)>;
+}  // namespace dynamics
+}  // namespace algorithms
+
+namespace functionals {
+struct Functional;
+
+// The callable that Functional::grand_potential_callback returns; Simulation
+// recognises it and runs the whole DDFT loop on the device.
+struct GrandPotentialCallback {
+  const Functional* functional;
+  double chemical_potential;
+  std::pair<double, std::vector<vec>> operator()(const std::vector<vec>& densities) const;
+};
+
+// functional.hpp:22-107
+struct Functional {
+  physics::Model model;
+  Weights weights;
+
+  Result evaluate(const State& state) const { return total(model, state, weights); }
+
+  Result evaluate(const vec& rho, double chemical_potential) const {
+    auto state = init::from_profile(model, rho);
+    state.species[0].chemical_potential = chemical_potential;
+    return total(model, state, weights);
+  }
+
+  double grand_potential(const vec& rho, double chemical_potential) const {
+    return evaluate(rho, chemical_potential).grand_potential;
+  }
+
+  // BulkThermodynamics::chemical_potential of the one-component fluid with the
+  // grid-synchronised a_vdw (bulk/thermodynamics.hpp:141-151).
+  double bulk_chemical_potential(double rho) const {
+    double mu = 0.0;
+    ::dft::detail::check(cdft_bulk_chemical_potential(weights.ctx.get(), rho, &mu));
+    return mu;
+  }
+
+  algorithms::dynamics::ForceCallback grand_potential_callback(double chemical_potential) const {
+    return GrandPotentialCallback{this, chemical_potential};
+  }
+};
+
+inline std::pair<double, std::vector<vec>> GrandPotentialCallback::operator()(const std::vector<vec>& densities) const {
+  auto state = init::from_profiles(functional->model, densities);
+  for (auto& sp : state.species) sp.chemical_potential = chemical_potential;
+  auto result = total(functional->model, state, functional->weights);
+  return {result.grand_potential, std::move(result.forces)};
+}
+
+// functional.hpp:195-206
+inline Functional make_functional(const fmt::FMTModel& fmt_model, physics::Model model,
+                                  Precision precision = Precision::FP64, int device = 0) {
+  auto w = make_weights(fmt_model, model, precision, device);
+  return Functional{std::move(model), std::move(w)};
+}
+}  // namespace functionals
+
+namespace algorithms {
+namespace fire {
+// fire.hpp:33-44 (defaults as there)
+struct Fire {
+  double dt{1e-3};
+  double dt_max{1e-2};
+  double dt_min{1e-8};
+  double alpha_start{0.1};
+  double f_inc{1.1};
+  double f_dec{0.5};
+  double f_alpha{0.99};
+  int n_delay{5};
+  int max_uphill{20};
+  double force_tolerance{0.1};
+  int max_steps{10000};
+};
+}  // namespace fire
+
+namespace minimization {
+// minimization.hpp:27-39
+struct Unbounded {
+  double rho_min{1e-18};
+};
+struct Bounded {
+  double rho_min{1e-18};
+  double rho_max{1.0};
+};
+using Parametrization = std::variant<Unbounded, Bounded>;
+
+// minimization.hpp:114-121
+struct Result {
+  std::vector<vec> densities;
+  double free_energy{0.0};
+  double grand_potential{0.0};
+  double rms_force{0.0};
+  int iterations{0};
+  bool converged{false};
+};
+
+// minimization.hpp:123-134: grand_potential runs FIRE2 in the alias variable
+// on the device (the uphill limit throws std::runtime_error, as fire.hpp:106-108).
+struct Minimizer {
+  fire::Fire fire{};
+  Parametrization param = Unbounded{};
+  bool use_homogeneous_boundary{true};
+  int log_interval{100};
+
+  Result grand_potential(const physics::Model& model, const functionals::Weights& weights, const vec& initial_density,
+                         double chemical_potential) const {
+    if (static_cast<long>(initial_density.size()) != model.grid.total_points())
+      throw std::invalid_argument("initial density size does not match the grid");
+    cdft_fire f{fire.dt,      fire.dt_max,  fire.dt_min,     fire.alpha_start,    fire.f_inc,   fire.f_dec,
+                fire.f_alpha, fire.n_delay, fire.max_uphill, fire.force_tolerance, fire.max_steps};
+    cdft_parametrization p{};
+    if (auto* u = std::get_if<Unbounded>(&param)) {
+      p.kind = 0;
+      p.rho_min = u->rho_min;
+      p.rho_max = 1.0;
+    } else {
+      const auto& b = std::get<Bounded>(param);
+      p.kind = 1;
+      p.rho_min = b.rho_min;
+      p.rho_max = b.rho_max;
+    }
+    Result r;
+    r.densities.assign(1, vec(initial_density.size()));
+    cdft_min_result cr{};
+    ::dft::detail::check(cdft_minimize_grand_potential(weights.ctx.get(), &f, &p, use_homogeneous_boundary ? 1 : 0,
+                                                       initial_density.data(), nullptr, chemical_potential,
+                                                       r.densities[0].data(), &cr, nullptr));
+    r.free_energy = cr.free_energy;
+    r.grand_potential = cr.grand_potential;
+    r.rms_force = cr.rms_force;
+    r.iterations = cr.iterations;
+    r.converged = cr.converged != 0;
+    return r;
+  }
+};
+}  // namespace minimization
+
+namespace dynamics {
+// dynamics.hpp:26-33
+struct StepConfig {
+  double dt{1e-4};
+  double diffusion_coefficient{1.0};
+  double min_density{1e-18};
+  double dt_max{1.0};
+  double fp_tolerance{1e-4};
+  int fp_max_iterations{100};
+};
+// dynamics.hpp:54-61 (snapshots are not recorded by the device loop)
+struct SimulationResult {
+  std::vector<vec> densities;
+  std::vector<double> times;
+  std::vector<double> energies;
+  double mass_initial{0.0};
+  double mass_final{0.0};
+};
+using BoundaryCondition = std::function<void(std::vector<vec>&)>;
+using StopCondition = std::function<bool(int step, double time, double energy)>;
+
+// dynamics.hpp:564-590: Lutsko's integrating-factor scheme with fixed-point
+// iteration and adaptive dt, run on the device.  The force callback must be a
+// Functional::grand_potential_callback (the device loop evaluates that
+// functional itself); boundary and stop callbacks are not supported there.
+struct Simulation {
+  StepConfig step;
+  int n_steps{1000};
+  int snapshot_interval{100};
+  int log_interval{50};
+  double energy_offset{0.0};
+  BoundaryCondition boundary{};
+  StopCondition stop_condition{};
+
+  SimulationResult run(std::vector<vec> densities, const Grid& grid, const ForceCallback& force_fn) const {
+    const auto* cb = force_fn.target<functionals::GrandPotentialCallback>();
+    if (cb == nullptr)
+      throw std::invalid_argument("Simulation::run on the GPU needs a Functional::grand_potential_callback");
+    if (boundary || stop_condition)
+      throw std::invalid_argument("Simulation::run on the GPU supports neither boundary nor stop callbacks");
+    if (densities.size() != 1 || static_cast<long>(densities[0].size()) != grid.total_points())
+      throw std::invalid_argument("Simulation::run on the GPU takes one density of the grid's size");
+    cdft_step_config c{step.dt, step.diffusion_coefficient, step.min_density, step.dt_max, step.fp_tolerance,
+                       step.fp_max_iterations};
+    std::vector<double> e(static_cast<size_t>(n_steps) + 1), t(static_cast<size_t>(n_steps) + 1);
+    cdft_sim_result cr{};
+    ::dft::detail::check(cdft_ddft_run(cb->functional->weights.ctx.get(), &c, n_steps, nullptr, cb->chemical_potential,
+                                       densities[0].data(), &cr, e.data(), t.data()));
+    // recorded like the reference: the initial state, then every log_interval
+    // steps and the last step (dynamics.hpp:603-604,650-653)
+    SimulationResult r;
+    for (int s = 0; s <= n_steps; ++s) {
+      if (s == 0 || (log_interval > 0 && (s % log_interval == 0 || s == n_steps))) {
+        r.times.push_back(t[static_cast<size_t>(s)]);
+        r.energies.push_back(e[static_cast<size_t>(s)]);
+      }
+    }
+    r.densities = std::move(densities);
+    r.mass_initial = cr.mass_initial;
+    r.mass_final = cr.mass_final;
+    return r;
+  }
+};
+}  // namespace dynamics
+}  // namespace algorithms
+
+}  // namespace dft

@@ -1,0 +1,156 @@
+// Exercises the C++ facade (include/cdft/dft.hpp) the way the reference's own
+// callers use its interfaces (make_functional -> evaluate / callback,
+// Minimizer::grand_potential, Simulation::run).
+//
+//   facade_check cpu                       argument validation and error mapping (no GPU calls)
+//   facade_check gpu RHO_IN MU_RHO OUT_DIR 16^3 LJ droplet: writes forces.bin, fire_rho.bin,
+//                                          ddft_rho.bin to OUT_DIR and one JSON line to stdout
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+
+#include "cdft/dft.hpp"
+
+namespace {
+
+int failures = 0;
+
+template <typename E, typename F>
+void expect_throw(const char* what, F&& f, const char* needle) {
+  try {
+    f();
+  } catch (const E& e) {
+    if (needle && std::string(e.what()).find(needle) == std::string::npos) {
+      std::printf("FAIL %s: message '%s' lacks '%s'\n", what, e.what(), needle);
+      ++failures;
+    }
+    return;
+  } catch (const std::exception& e) {
+    std::printf("FAIL %s: wrong exception type: %s\n", what, e.what());
+    ++failures;
+    return;
+  }
+  std::printf("FAIL %s: no exception\n", what);
+  ++failures;
+}
+
+dft::physics::Model lj_model(double dx, double box, int n_species = 1) {
+  using namespace dft;
+  physics::Model m;
+  m.grid = make_grid(dx, {box, box, box});
+  for (int i = 0; i < n_species; ++i) m.species.push_back(Species{"argon", 1.0});
+  physics::Interaction in;
+  in.potential = physics::potentials::make_lennard_jones(1.0, 1.0, 2.5);
+  m.interactions.push_back(in);
+  m.temperature = 0.8;
+  return m;
+}
+
+int run_cpu() {
+  using namespace dft;
+  const Grid g = make_grid(0.1, {3.2, 1.6, 0.8});
+  if (g.shape[0] != 32 || g.shape[1] != 16 || g.shape[2] != 8 || g.total_points() != 4096 ||
+      g.flat_index(1, 2, 3) != 3 + 8 * (2 + 16 * 1)) {
+    std::printf("FAIL make_grid shape/index\n");
+    ++failures;
+  }
+  expect_throw<std::invalid_argument>("dx", [] { (void)make_grid(0.0, {1.0, 1.0, 1.0}); }, "dx must be positive");
+  expect_throw<std::invalid_argument>("box", [] { (void)make_grid(0.1, {1.0, -1.0, 1.0}); },
+                                      "box dimension must be positive");
+  expect_throw<std::invalid_argument>("commensurate", [] { (void)make_grid(0.3, {1.0, 1.0, 1.0}); },
+                                      "not commensurate");
+  expect_throw<std::invalid_argument>("fmt name", [] { (void)functionals::fmt::FMTModel::from_name("WhiteBearIII"); },
+                                      "Unknown FMT model: WhiteBearIII");
+  if (functionals::fmt::FMTModel::from_name("WhiteBearII").id() != CDFT_FMT_WHITEBEAR_II) {
+    std::printf("FAIL fmt id\n");
+    ++failures;
+  }
+  // rejected before any device work: two species
+  expect_throw<std::invalid_argument>("species", [] {
+    (void)functionals::make_functional(functionals::fmt::WhiteBearII{}, lj_model(0.1, 1.6, 2));
+  }, "one species");
+  std::printf(failures ? "facade cpu checks: %d failures\n" : "facade cpu checks: ok\n", failures);
+  return failures ? 1 : 0;
+}
+
+dft::vec read_doubles(const std::string& path, long n) {
+  dft::vec v(static_cast<size_t>(n));
+  std::ifstream f(path, std::ios::binary);
+  f.read(reinterpret_cast<char*>(v.data()), static_cast<std::streamsize>(8 * n));
+  if (!f) throw std::runtime_error("cannot read " + path);
+  return v;
+}
+
+void write_doubles(const std::string& path, const dft::vec& v) {
+  std::ofstream f(path, std::ios::binary);
+  f.write(reinterpret_cast<const char*>(v.data()), static_cast<std::streamsize>(8 * v.size()));
+}
+
+int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) {
+  using namespace dft;
+  const auto model = lj_model(0.25, 4.0);  // 16^3
+  auto func = functionals::make_functional(functionals::fmt::WhiteBearII{}, model);
+  const long n = model.grid.total_points();
+  const vec rho = read_doubles(rho_path, n);
+  const double mu = func.bulk_chemical_potential(mu_rho);
+
+  const auto res = func.evaluate(rho, mu);
+  write_doubles(out + "/forces.bin", res.forces[0]);
+  // the callback the dynamics use gives the same numbers
+  const auto cb = func.grand_potential_callback(mu);
+  const auto [omega_cb, forces_cb] = cb({rho});
+  double dmax = 0.0;
+  for (long i = 0; i < n; ++i) dmax = std::max(dmax, std::abs(forces_cb[0][static_cast<size_t>(i)] - res.forces[0][static_cast<size_t>(i)]));
+
+  algorithms::minimization::Minimizer mz;
+  mz.fire.force_tolerance = 1e-9;
+  mz.fire.max_steps = 40;
+  const auto mres = mz.grand_potential(model, func.weights, rho, mu);
+  write_doubles(out + "/fire_rho.bin", mres.densities[0]);
+
+  algorithms::dynamics::Simulation sim;
+  sim.step.dt = 1e-3;
+  sim.step.dt_max = 1e-3;
+  sim.n_steps = 6;
+  sim.log_interval = 2;
+  const auto sres = sim.run({rho}, model.grid, cb);
+  write_doubles(out + "/ddft_rho.bin", sres.densities[0]);
+  // a callback that is not the functional's cannot run on the device
+  bool rejected = false;
+  try {
+    algorithms::dynamics::ForceCallback other = [](const std::vector<vec>& d) {
+      return std::make_pair(0.0, d);
+    };
+    (void)sim.run({rho}, model.grid, other);
+  } catch (const std::invalid_argument&) {
+    rejected = true;
+  }
+
+  std::printf("{\"mu\": %.17g, \"free_energy\": %.17g, \"grand_potential\": %.17g, \"callback_omega\": %.17g, "
+              "\"callback_force_maxdiff\": %.3g, \"fire_iterations\": %d, \"fire_grand_potential\": %.17g, "
+              "\"ddft_times\": [",
+              mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential);
+  for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
+  std::printf("], \"ddft_energies\": [");
+  for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);
+  std::printf("], \"ddft_mass\": [%.17g, %.17g], \"foreign_callback_rejected\": %s}\n", sres.mass_initial,
+              sres.mass_final, rejected ? "true" : "false");
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  try {
+    if (argc >= 2 && std::strcmp(argv[1], "cpu") == 0) return run_cpu();
+    if (argc >= 5 && std::strcmp(argv[1], "gpu") == 0) return run_gpu(argv[2], std::stod(argv[3]), argv[4]);
+  } catch (const std::exception& e) {
+    std::printf("error: %s\n", e.what());
+    return 2;
+  }
+  std::printf("usage: facade_check cpu | gpu RHO_IN MU_RHO OUT_DIR\n");
+  return 2;
+}

@@ -1,0 +1,60 @@
+"""GPU parity of the C++ facade (include/cdft/dft.hpp): a C++ program written
+against the reference's interfaces (make_functional, Functional::evaluate and
+grand_potential_callback, Minimizer::grand_potential, Simulation::run)
+against the FP64 CPU oracle on the same 16^3 droplet.  Tolerances as for the
+C ABI: fields and energies 1e-11 relative, FIRE/DDFT trajectories 1e-9.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from classicaldft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+BIN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "facade_check")
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def test_cpp_facade_matches_oracle(tmp_path):
+    from oracle import oracle as O
+
+    assert os.path.exists(BIN), "build the facade check first (make -C tests/cpp)"
+    shape, dx = (16, 16, 16), 0.25
+    rho = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    rho.reshape(-1).astype(np.float64).tofile(tmp_path / "rho.bin")
+    print("facade_check gpu 16^3 droplet", flush=True)
+    r = subprocess.run([BIN, "gpu", str(tmp_path / "rho.bin"), "0.05", str(tmp_path)], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    mu = orc.bulk_mu(0.05)
+    assert out["mu"] == pytest.approx(mu, rel=1e-12)
+    parts, fr = orc.total(rho, mu)
+    assert out["free_energy"] == pytest.approx(parts[4], rel=1e-11)
+    assert out["grand_potential"] == pytest.approx(parts[5], rel=1e-11)
+    f = np.fromfile(tmp_path / "forces.bin").reshape(shape)
+    assert _rel(f, fr) < 1e-11
+    assert out["callback_omega"] == out["grand_potential"] and out["callback_force_maxdiff"] == 0.0
+
+    # Minimizer defaults of the reference: Unbounded, homogeneous boundary on
+    ro, reso, _ = orc.minimize_grand_potential(rho, mu, fire=dict(force_tolerance=1e-9), max_steps=40)
+    assert out["fire_iterations"] == reso["iterations"]
+    assert out["fire_grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-9)
+    assert _rel(np.fromfile(tmp_path / "fire_rho.bin").reshape(shape), ro) < 1e-9
+
+    rd, resd, ed = orc.ddft_run(rho, mu, 6, dt=1e-3, dt_max=1e-3)
+    assert _rel(np.fromfile(tmp_path / "ddft_rho.bin").reshape(shape), rd) < 1e-9
+    # recorded at step 0 and every log_interval = 2 steps, as Simulation::run does
+    np.testing.assert_allclose(out["ddft_energies"], ed[[0, 2, 4, 6]], rtol=1e-9)
+    assert out["ddft_times"][0] == 0.0 and out["ddft_times"][-1] == pytest.approx(resd["time"], rel=1e-12)
+    assert out["ddft_mass"][0] == pytest.approx(resd["mass_initial"], rel=1e-12)
+    assert out["foreign_callback_rejected"]
