@@ -392,7 +392,10 @@ __device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NWV][PPL], cx<S>*
   }
 }
 
-template <typename S, typename CV, int NZ, int NF, int NB>
+// FM: the FMT model fixed at compile time (its branches fold away), or
+// kFmtRuntime to read it from the parameters.
+constexpr int kFmtRuntime = -100;
+template <typename S, typename CV, int NZ, int NF, int NB, int FM>
 __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
                                                  const double* __restrict__ vext, double* __restrict__ lf,
                                                  double* __restrict__ partials, FastGeom g,
@@ -461,7 +464,11 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   static_assert(NRF <= 6 && NRB <= 6, "too many z rounds");
 
   // ---- pointwise: one point at a time (rolled), channel values in and out by selects
-  double acc[5] = {0, 0, 0, 0, 0}, cmp[5] = {0, 0, 0, 0, 0};  // ideal, external, hs, mf, mass
+  // per-lane sums of its PPL points (plain FP64: a handful of terms); the
+  // sums over lanes, waves and blocks follow, the last one compensated
+  double acc[5] = {0, 0, 0, 0, 0};  // ideal, external, hs, mf, mass
+  FmtParams fpk = fp;
+  if constexpr (FM != kFmtRuntime) fpk.model = FM;
   const double dv = g.dv;
 #pragma unroll 1
   for (int pt = 0; pt < PPL; ++pt) {
@@ -481,25 +488,25 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
     const double rho = amap.apply(rho_in[p]);
     const double lr = log(fmax(rho, 1e-300));
     double lfv = (lr - mu) * dv;  // ideal_gas.hpp:312-318
-    two_sum(acc[0], cmp[0], rho * (lr - 1.0));
-    two_sum(acc[4], cmp[4], rho);
+    acc[0] += rho * (lr - 1.0);
+    acc[4] += rho;
     if (vext) {  // external_field.hpp:16-31
       const double V = vext[p];
       lfv += V * dv;
-      two_sum(acc[1], cmp[1], rho * V);
+      acc[1] += rho * V;
     }
     if constexpr (MF) {  // mean_field.hpp:259-266 (self pair)
       const double conv = static_cast<double>(cur[zpos(NF, 11)]) + dc.v[chan_store(NF, 11)] * shift;
       lfv += conv * dv;
-      two_sum(acc[3], cmp[3], 0.5 * rho * conv);
+      acc[3] += 0.5 * rho * conv;
     }
     if constexpr (HS) {
       double w[11], gg[11];
 #pragma unroll
       for (int i = 0; i < 10; ++i) w[i] = static_cast<double>(cur[zpos(NF, i)]) + dc.v[i] * shift;
       w[10] = w[1] - w[5] - w[8];  // T_zz = n2 - T_xx - T_yy
-      const double phi = fmt_phi_grad(fp, w, gg);
-      two_sum(acc[2], cmp[2], phi);
+      const double phi = fmt_phi_grad(fpk, w, gg);
+      acc[2] += phi;
       // adjoint of the T_zz identity
       gg[1] += gg[10];
       gg[5] -= gg[10];
@@ -523,7 +530,7 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
     double tot[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
-      double x = acc[i] + cmp[i];
+      double x = acc[i];
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
       tot[i] = x;
@@ -1330,9 +1337,12 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
                         const FastGeom& g, const void* tw64, const void* twv, ChannelSet cs, FmtParams fp, double mu,
                         AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
   const unsigned nblk = static_cast<unsigned>(fast_point_blocks(g));
+  // White-Bear II (the reference's default model) has its own instantiation
+  const bool wb2 = fp.model == FMT_WHITEBEAR_II;
 #define CDFT_L(NZ, NF, NB)                                                                             \
   {                                                                                                    \
-    auto k = kf_zpoint<S, CV, NZ, NF, NB>;                                                             \
+    auto k = (NF >= 11 && wb2) ? kf_zpoint<S, CV, NZ, NF, NB, (NF >= 11 ? FMT_WHITEBEAR_II : kFmtRuntime)> \
+                               : kf_zpoint<S, CV, NZ, NF, NB, kFmtRuntime>;                            \
     const int sm = zpoint_smem<NZ, S>();                                                               \
     set_smem(k, sm);                                                                                   \
     hipLaunchKernelGGL(k, dim3(nblk), dim3(kZP), sm, st, SF, rho, vext, lf, partials, g,               \
