@@ -650,21 +650,62 @@ __global__ void __launch_bounds__(256) kf_wtable(cx<double>* __restrict__ WT, Fa
     WT[i] = {w3, w2};
   }
 }
-// Weight of channel ch; `adj` conjugates it (the odd vector channels change
-// sign), as the back-convolution does (hard_sphere.hpp:118-134).
+// Weight of channel ch (weights.hpp:195-222; `adj` conjugates it, so the odd
+// vector channels change sign, as the back-convolution does,
+// hard_sphere.hpp:118-134) as a linear form in the element's weight factors:
+// ch 0 w3, 1 w2, 2..4 i k_a w3, 5..10 aik2 k_i k_j + delta_ij w3/R, i.e.
+//   W = (a3 w3 + a2 w2 + at aik2 k_i k_j) + i av k_i w3,
+//   k_i = ix kx + iy ky + iz kz,  k_j = jx kx + jy ky + jz kz,
+// whose coefficients depend on the channel only: set up once per channel,
+// it is applied to every element without branching on the channel.
+// Forward weight of channel ch for one element, by cases (the forward plane
+// kernel keeps its per-channel state in fewer registers this way).
 template <typename C>
-__device__ __forceinline__ cx<C> chan_weight(int ch, bool adj, C kx, C ky, C kz, const WFac<C>& f, C invR) {
+__device__ __forceinline__ cx<C> chan_weight(int ch, C kx, C ky, C kz, const WFac<C>& f, C invR) {
   if (ch == 0) return {f.w3, C(0)};
   if (ch == 1) return {f.w2, C(0)};
-  if (ch <= 4) {
-    const C ka = (ch == 2 ? kx : (ch == 3 ? ky : kz)) * f.w3;
-    return {C(0), adj ? -ka : ka};
-  }
+  if (ch <= 4) return {C(0), (ch == 2 ? kx : (ch == 3 ? ky : kz)) * f.w3};
   const int q = ch - 5;
   const C ki = q < 3 ? kx : (q < 5 ? ky : kz);
   const C kj = (q == 0) ? kx : (q == 1 || q == 3) ? ky : kz;
   const bool diag = (q == 0 || q == 3 || q == 5);
   return {f.aik2 * ki * kj + (diag ? f.w3 * invR : C(0)), C(0)};
+}
+template <typename C>
+struct ChanForm {
+  C a3, a2, at, av, ix, iy, iz, jx, jy, jz;
+};
+template <typename C>
+__device__ __forceinline__ ChanForm<C> chan_form(int ch, bool adj, C invR) {
+  ChanForm<C> f{C(0), C(0), C(0), C(0), C(0), C(0), C(0), C(0), C(0), C(0)};
+  int i = 0, j = 0;
+  if (ch == 0) {
+    f.a3 = C(1);
+  } else if (ch == 1) {
+    f.a2 = C(1);
+  } else if (ch <= 4) {
+    f.av = adj ? C(-1) : C(1);
+    i = ch - 2;
+  } else {
+    const int q = ch - 5;
+    i = q < 3 ? 0 : (q < 5 ? 1 : 2);
+    j = (q == 0) ? 0 : ((q == 1 || q == 3) ? 1 : 2);
+    f.at = C(1);
+    if (q == 0 || q == 3 || q == 5) f.a3 = invR;
+  }
+  f.ix = i == 0 ? C(1) : C(0);
+  f.iy = i == 1 ? C(1) : C(0);
+  f.iz = i == 2 ? C(1) : C(0);
+  f.jx = j == 0 ? C(1) : C(0);
+  f.jy = j == 1 ? C(1) : C(0);
+  f.jz = j == 2 ? C(1) : C(0);
+  return f;
+}
+template <typename C>
+__device__ __forceinline__ cx<C> apply_form(const ChanForm<C>& f, C kx, C ky, C kz, const WFac<C>& w) {
+  const C ki = f.ix * kx + (f.iy * ky + f.iz * kz);
+  const C kj = f.jx * kx + (f.jy * ky + f.jz * kz);
+  return {f.a3 * w.w3 + f.a2 * w.w2 + f.at * w.aik2 * ki * kj, f.av * ki * w.w3};
 }
 
 // ---------------------------------------------------------------- kf_rho_xy
@@ -924,8 +965,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
           const int ch = fwd_channel(NF, pos);
           plane_fwd_channel<CV, S, NX, Q, FC::DB>(
               [&](int cc, int e) -> cx<CV> {
-                return cmul(chan_weight<CV>(ch, false, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc),
-                            XC[cc][e]);
+                return cmul(chan_weight<CV>(ch, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc), XC[cc][e]);
               },
               tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, g, kz, r, h, twvl);
         }
@@ -1075,6 +1115,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         }
       }
       __syncthreads();
+      const ChanForm<CV> form = chan_form<CV>(bwd_channel(ch), true, invR);
       // half x-FFT (DIT over the rows of parity h), conj-weighted accumulation over all kx
 #pragma unroll
       for (int cc = 0; cc < CPS; ++cc) {
@@ -1090,8 +1131,8 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
             cx<CV> oa = f[e];
             if (h) oa = twiddle<CV, NX, -1>(oa, xl + PC::TX * e, twl);
             const cx<CV> ob = h ? cneg(oa) : oa;
-            const cx<CV> wa = chan_weight<CV>(bwd_channel(ch), true, kxc[e], kyv, kzc, wf[cc][e], invR);
-            const cx<CV> wb = chan_weight<CV>(bwd_channel(ch), true, kxc[e + 4], kyv, kzc, wf[cc][e + 4], invR);
+            const cx<CV> wa = apply_form<CV>(form, kxc[e], kyv, kzc, wf[cc][e]);
+            const cx<CV> wb = apply_form<CV>(form, kxc[e + 4], kyv, kzc, wf[cc][e + 4]);
             acc[cc][e] = cadd(acc[cc][e], cmul(wa, oa));
             acc[cc][e + 4] = cadd(acc[cc][e + 4], cmul(wb, ob));
           }
