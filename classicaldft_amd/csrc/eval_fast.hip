@@ -463,14 +463,19 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
 #undef CDFT_ZF
   static_assert(NRF <= 6 && NRB <= 6, "too many z rounds");
 
-  // ---- pointwise: one point at a time (rolled), channel values in and out by selects
+  // ---- pointwise: one point at a time, channel values in and out of the
+  // lane's register arrays
   // per-lane sums of its PPL points (plain FP64: a handful of terms); the
   // sums over lanes, waves and blocks follow, the last one compensated
   double acc[5] = {0, 0, 0, 0, 0};  // ideal, external, hs, mf, mass
   FmtParams fpk = fp;
   if constexpr (FM != kFmtRuntime) fpk.model = FM;
+  if constexpr (FM == FMT_WHITEBEAR_II) {  // esFMT(1, -1) (models.hpp:326-357)
+    fpk.A = 1.0;
+    fpk.B = -1.0;
+  }
   const double dv = g.dv;
-#pragma unroll 1
+#pragma unroll
   for (int pt = 0; pt < PPL; ++pt) {
     const int z = lid + 64 * (pt >> 1);
     const long p = ((pt & 1) ? lb : la) * NZ + z;
