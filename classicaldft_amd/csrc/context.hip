@@ -277,8 +277,10 @@ const char* const kStageNames[kNumStages] = {"k_sum",    "k_zr2c",            "k
                                               "k_lines(y inv nf)", "k_zpoint", "k_lines(y fwd nb)", "k_xbwd",
                                               "k_lines(y inv 1)",  "k_zc2r",   "k_finalize"};
 
-const char* const kFastStageNames[kNumFastStages] = {"kf_zfwd",      "kf_rho_xy", "kf_plane_fwd",
-                                                     "kf_zpoint", "kf_plane_bwd", "kf_zinv",   "k_finalize"};
+// the energy sums are finalised inside kf_zinv (by k_finalize in the
+// stage of kf_zinv when no forces are asked for)
+const char* const kFastStageNames[kNumFastStages] = {"kf_zfwd",   "kf_rho_xy",    "kf_plane_fwd",
+                                                     "kf_zpoint", "kf_plane_bwd", "kf_zinv"};
 
 template <typename S, typename CV>
 void Context::eval_fast_t(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
@@ -307,11 +309,16 @@ void Context::eval_fast_t(const double* rho, const double* vext, double mu, doub
     mark(4);
     launch_fast_plane_bwd<S, CV>(SF, d_WT, P, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
     mark(5);
-    launch_fast_zinv<S, CV>(P, force, d_local, g.dv, fg, sizeof(CV) == 4 ? ftw32 : ftw64, stream);
+    // the energy sums are reduced by one extra block of the last kernel
+    launch_fast_zinv<S, CV>(P, force, d_local, g.dv, fg, sizeof(CV) == 4 ? ftw32 : ftw64, d_partials, nblk_point,
+                            d_res, stream);
+    mark(6);
+  } else {
+    mark(4);
+    mark(5);
+    launch_finalize(d_partials, nblk_point, d_res, stream);
+    mark(6);
   }
-  mark(6);
-  launch_finalize(d_partials, nblk_point, d_res, stream);
-  mark(7);
 }
 
 template <typename S, typename CE, typename CL>

@@ -38,7 +38,7 @@ struct InteractionSpec {
 
 constexpr int kNumStages = 11;  // stages of the general schedule (the most of any path)
 extern const char* const kStageNames[kNumStages];
-constexpr int kNumFastStages = 7;
+constexpr int kNumFastStages = 6;
 extern const char* const kFastStageNames[kNumFastStages];
 
 // Results of one evaluation, in the order the fused kernel accumulates them.

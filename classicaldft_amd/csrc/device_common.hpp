@@ -16,6 +16,42 @@ __device__ __forceinline__ void two_sum(double& s, double& c, double x) {
   s = t;
 }
 
+// Deterministic compensated column sums res[0..7] of `nrows` rows of 8
+// partial sums, by one block of NT threads (sh, shc: 2 NT doubles of LDS).
+// Thread t owns column t % 8 of rows t / 8 + (NT / 8) j and issues its loads
+// 16 at a time before adding them; the per-thread (sum, compensation) pairs
+// of a column then merge in a fixed tree order.
+template <int NT>
+__device__ __forceinline__ void reduce_rows(const double* __restrict__ partials, int nrows, double* __restrict__ res,
+                                            double* sh, double* shc) {
+  constexpr int RS = NT / 8, U = 16;  // rows per sweep, loads in flight per thread
+  const int t = threadIdx.x, comp = t & 7, row = t >> 3;
+  double s = 0.0, c = 0.0;
+  for (int b0 = row; b0 < nrows; b0 += U * RS) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int b = b0 + u * RS;
+      v[u] = b < nrows ? partials[static_cast<long>(b) * 8 + comp] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) two_sum(s, c, v[u]);
+  }
+  sh[t] = s;
+  shc[t] = c;
+  __syncthreads();
+  for (int w = RS / 2; w > 0; w >>= 1) {
+    if (row < w) {
+      double s2 = sh[t], c2 = shc[t] + shc[t + 8 * w];
+      two_sum(s2, c2, sh[t + 8 * w]);
+      sh[t] = s2;
+      shc[t] = c2;
+    }
+    __syncthreads();
+  }
+  if (t < 8) res[t] = sh[t] + shc[t];
+}
+
 template <typename D, typename T>
 __device__ __forceinline__ cx<D> cvt(cx<T> a) {
   return {static_cast<D>(a.re), static_cast<D>(a.im)};

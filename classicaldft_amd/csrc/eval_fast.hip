@@ -258,12 +258,22 @@ __device__ __forceinline__ void build_c2r(cx<C> (&v)[8], int lane, F&& part) {
 
 __host__ __device__ inline bool bwd_unit_split(int unit, int nunits, int nblk);
 
+// With fin_rows > 0 the grid has one block more, which does the final
+// reduction of the point kernel's partial sums (reduce_rows) while the other
+// blocks transform: the evaluation then needs no separate finalisation launch.
 template <typename S, typename C, int NZ>
 __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, double* __restrict__ out,
                                                   const double* __restrict__ add, double scale, FastGeom g,
-                                                  const cx<C>* __restrict__ tw, int bwd_blocks) {
+                                                  const cx<C>* __restrict__ tw, int bwd_blocks,
+                                                  const double* __restrict__ partials, int fin_rows,
+                                                  double* __restrict__ res) {
   constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   extern __shared__ __align__(16) unsigned char smem[];
+  if (fin_rows > 0 && blockIdx.x == gridDim.x - 1) {
+    double* sh = reinterpret_cast<double*>(smem);
+    reduce_rows<2 * NZ>(partials, fin_rows, res, sh, sh + 2 * NZ);
+    return;
+  }
   cx<C>* cb = reinterpret_cast<cx<C>*>(smem);  // [r][m16][kz]
   cx<C>* xbuf = reinterpret_cast<cx<C>*>(smem + align16(NZH * 32 * static_cast<int>(sizeof(cx<C>))));
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
@@ -1292,15 +1302,15 @@ void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom&
 
 template <typename S, typename CV>
 void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
-                      const void* twv, hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(g.nx * g.mb);
+                      const void* twv, const double* partials, int fin_rows, double* res, hipStream_t st) {
+  const unsigned nblk = static_cast<unsigned>(g.nx * g.mb) + (fin_rows > 0 ? 1u : 0u);
 #define CDFT_L(NZ)                                                                                      \
   {                                                                                                     \
     auto k = kf_zinv<S, CV, NZ>;                                                                        \
     const int sm = zinv_smem<NZ, CV>();                                                                 \
     set_smem(k, sm);                                                                                    \
     hipLaunchKernelGGL(k, dim3(nblk), dim3(2 * NZ), sm, st, P, out, add, scale, g,                      \
-                       as<cx<CV>>(twv), fast_bwd_blocks(g));                                            \
+                       as<cx<CV>>(twv), fast_bwd_blocks(g), partials, fin_rows, res);                   \
   }
   if (g.nz == 64) CDFT_L(64) else if (g.nz == 128) CDFT_L(128) else throw std::runtime_error("fast path: nz");
 #undef CDFT_L
@@ -1420,7 +1430,7 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
   template void launch_fast_plane_bwd<S, CV>(const cx<S>*, const void*, cx<S>*, const FastGeom&, const void*,     \
                                              hipStream_t);                                                       \
   template void launch_fast_zinv<S, CV>(const cx<S>*, double*, const double*, double, const FastGeom&,           \
-                                        const void*, hipStream_t);
+                                        const void*, const double*, int, double*, hipStream_t);
 CDFT_INST(float, float)
 CDFT_INST(double, double)
 #undef CDFT_INST

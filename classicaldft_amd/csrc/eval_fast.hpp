@@ -63,8 +63,10 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
 template <typename S, typename CV>
 void launch_fast_plane_bwd(const cx<S>* SB, const void* WT, cx<S>* P, const FastGeom& g, const void* twv,
                            hipStream_t st);
+// With fin_rows > 0 one extra block also reduces fin_rows rows of partial
+// sums into res[0..7] (as launch_finalize does).
 template <typename S, typename CV>
 void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
-                      const void* twv, hipStream_t st);
+                      const void* twv, const double* partials, int fin_rows, double* res, hipStream_t st);
 
 }  // namespace cdft

@@ -549,41 +549,13 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
 // Deterministic compensated sum of per-block partials (nblk x 8 doubles) into res[0..7]:
 // strided Neumaier sums per thread, then a fixed-shape tree of (sum, compensation) pairs.
 // Deterministic compensated column sums of `nrows` rows of 8 partial sums by
-// one block of kFinThreads threads.  Thread t owns column t % 8 of rows
-// t / 8 + (kFinThreads / 8) j and issues its loads 16 at a time before adding
-// them; the per-thread (sum, compensation) pairs of a column then merge in a
-// fixed tree order.
+// one block (reduce_rows).
 constexpr int kFinThreads = 1024;
 __global__ void __launch_bounds__(kFinThreads) k_finalize(const double* __restrict__ partials, int nrows,
                                                           double* __restrict__ res) {
-  constexpr int RS = kFinThreads / 8, U = 16;  // rows per sweep, loads in flight per thread
   __shared__ double sh[kFinThreads];
   __shared__ double shc[kFinThreads];
-  const int t = threadIdx.x, comp = t & 7, row = t >> 3;
-  double s = 0.0, c = 0.0;
-  for (int b0 = row; b0 < nrows; b0 += U * RS) {
-    double v[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const int b = b0 + u * RS;
-      v[u] = b < nrows ? partials[static_cast<long>(b) * 8 + comp] : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) two_sum(s, c, v[u]);
-  }
-  sh[t] = s;
-  shc[t] = c;
-  __syncthreads();
-  for (int w = RS / 2; w > 0; w >>= 1) {
-    if (row < w) {
-      double s2 = sh[t], c2 = shc[t] + shc[t + 8 * w];
-      two_sum(s2, c2, sh[t + 8 * w]);
-      sh[t] = s2;
-      shc[t] = c2;
-    }
-    __syncthreads();
-  }
-  if (t < 8) res[t] = sh[t] + shc[t];
+  reduce_rows<kFinThreads>(partials, nrows, res, sh, shc);
 }
 
 // ---------------------------------------------------------------- launchers
