@@ -724,13 +724,17 @@ __device__ __forceinline__ cx<C> apply_form(const ChanForm<C>& f, C kx, C ky, C 
 }
 
 // ---------------------------------------------------------------- kf_rho_xy
-// FP64 y- and x-FFT of the rho parts: block (kz, r, c) writes the full
-// spectrum rho_hat(kx, ky = r + 2 s, kz) of the columns s in half c as
+// FP64 y- and x-FFT of the rho parts: block (kz, r) transforms the y lines
+// of its half plane once, then the columns in two halves c through one
+// transpose buffer, and writes the full spectrum rho_hat(kx, ky = r + 2 s, kz)
 //   XR[((kz * 2 + r) * Q + s) * NX + kx]   (complex FP64).
+// (One block per half plane runs the 2 (Nz/2 + 1) blocks in a single round
+// on the CUs; splitting a half plane over two blocks would repeat its y
+// transforms and need a second round.)
 template <int NX, int Q, typename S>
 struct RhoCfg {
   using P = PlaneCfg<NX, Q>;
-  static constexpr int NC = Q / 2;                 // columns per block
+  static constexpr int NC = Q / 2;                 // columns per half
   static constexpr int TXR = NX / 4, SXR = kBT / TXR, CPR = (NC + SXR - 1) / SXR;
   static constexpr int NXP = NX + 1;
   static constexpr int tb_bytes = align16(NC * NXP * static_cast<int>(sizeof(cx<double>)));
@@ -759,9 +763,8 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
   const cx<double>* twl = stage_twiddles<double, kBT>(smem + RC::tw_off, tw);
   const int tid = threadIdx.x;
   const int ys = tid / P::TYF, yl = tid % P::TYF, xs = tid / RC::TXR, xl = tid % RC::TXR;
-  const int unit = xcd_remap(blockIdx.x, gridDim.x);  // the two column halves c of a slice share an L2
-  const int c = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
-  const int s0 = c * RC::NC;
+  const int unit = blockIdx.x;
+  const int r = unit & 1, kz = unit >> 1;
   cx<double> yv[RPF][8];
 #pragma unroll
   for (int rr = 0; rr < RPF; ++rr) {
@@ -785,29 +788,42 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
         for (int q = 0; q < 8; ++q) yv[rr][q] = twiddle<double, P::NY, -1>(yv[rr][q], yl + P::TYF * q, twl);
       }
       slot_fft<double, Q, P::TYF, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * P::XBYTES, yl, twl);
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int sc = yl + P::TYF * e - s0;
-        if (sc >= 0 && sc < RC::NC) tb[sc * RC::NXP + row] = yv[rr][e];
-      }
     }
   }
-  __syncthreads();
 #pragma unroll
-  for (int cc = 0; cc < RC::CPR; ++cc) {
-    const int sc = xs + RC::SXR * cc;
-    if (sc < RC::NC) {
-      cx<double> v[4];
+  for (int c = 0; c < 2; ++c) {
+    const int s0 = c * RC::NC;
+    // transpose the lane's outputs of columns [s0, s0 + NC); tb is not the
+    // y-slot memory, but its previous half is read until the barrier
+    if (c > 0) __syncthreads();
 #pragma unroll
-      for (int q = 0; q < 4; ++q) v[q] = tb[sc * RC::NXP + xl + RC::TXR * q];
-      slot_fft<double, NX, RC::TXR, -1, true>(v, xraw + xs * slot_buf_len<NX>() * P::XBYTES, xl, twl);
-      if (kz == 0 && r == 0 && s0 + sc == 0 && xl == 0) {
-        sum_out[0] = v[0].re;
-        v[0] = {0.0, 0.0};
+    for (int rr = 0; rr < RPF; ++rr) {
+      const int row = ys + P::SBYF * rr;
+      if (row < NX) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int sc = yl + P::TYF * e - s0;
+          if (sc >= 0 && sc < RC::NC) tb[sc * RC::NXP + row] = yv[rr][e];
+        }
       }
-      cx<double>* o = XR + ((static_cast<long>(kz) * 2 + r) * Q + s0 + sc) * NX;
+    }
+    __syncthreads();  // the transposed half is complete; every y (or x) slot exchange is over
 #pragma unroll
-      for (int e = 0; e < 4; ++e) o[xl + RC::TXR * e] = v[e];
+    for (int cc = 0; cc < RC::CPR; ++cc) {
+      const int sc = xs + RC::SXR * cc;
+      if (sc < RC::NC) {
+        cx<double> v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = tb[sc * RC::NXP + xl + RC::TXR * q];
+        slot_fft<double, NX, RC::TXR, -1, true>(v, xraw + xs * slot_buf_len<NX>() * P::XBYTES, xl, twl);
+        if (kz == 0 && r == 0 && s0 + sc == 0 && xl == 0) {
+          sum_out[0] = v[0].re;
+          v[0] = {0.0, 0.0};
+        }
+        cx<double>* o = XR + ((static_cast<long>(kz) * 2 + r) * Q + s0 + sc) * NX;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) o[xl + RC::TXR * e] = v[e];
+      }
     }
   }
 }
@@ -1319,7 +1335,7 @@ void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double sca
 template <typename S>
 void launch_fast_rho_xy(const cx<S>* A0, const double* D0, void* XR, double* sum_out, const FastGeom& g,
                         const void* tw64, hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
+  const unsigned nblk = static_cast<unsigned>(g.nzh * 2);
 #define CDFT_L(NX, Q)                                                                                 \
   {                                                                                                   \
     auto k = kf_rho_xy<S, NX, Q>;                                                                     \
