@@ -437,32 +437,49 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   const cx<CV>* twvl = stage_twiddles<CV, kZP>(smem + zpoint_tw_off<NZ, S>() + tw_bytes<double>(), twv);
   if constexpr (NRF == 0) __syncthreads();
 
-  // ---- forward rounds: stage NS channels of the block's 4 line pairs, C2R
-  auto stage_in = [&](int jr) {
-    block_copy<kZP, NS * NZH * 8, cx<S>>(
-        [&](int i) {
-          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
-          const int kz = t % NZH, gs = t / NZH;
-          const int ch = zf_chan(NF, jr * NS + gs);
-          cx<S> val = {S(0), S(0)};
-          if (ch >= 0) val = SF[static_cast<long>(chan_store(NF, ch)) * g.M + cbase + (kz * 2 + r) * 16 + p];
-          return val;
-        },
-        [&](int i, cx<S> val) {
-          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
-          const int kz = t % NZH, gs = t / NZH;
-          stg[((gs * 2 + r) * 4 + p) * NZH + kz] = val;
-        });
+  // ---- forward rounds: stage NS channels of the block's 4 line pairs, C2R.
+  // The loads of round JR + 1 are issued into registers before round JR's
+  // transforms, so their memory latency overlaps that work.
+  constexpr int NSI = NS * NZH * 8, KPF = (NSI + kZP - 1) / kZP;
+  cx<S> pf[KPF];
+  // (every lane loads, from channel 0 where its slot stages nothing: a load
+  // under a condition would make the transforms wait for it)
+  auto load_round = [&](int jr) {
+#pragma unroll
+    for (int k = 0; k < KPF; ++k) {
+      const int i = min(static_cast<int>(threadIdx.x) + kZP * k, NSI - 1);
+      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+      const int kz = t % NZH, gs = t / NZH;
+      const int ch = zf_chan(NF, jr * NS + gs);
+      pf[k] = SF[static_cast<long>(ch >= 0 ? chan_store(NF, ch) : 0) * g.M + cbase + (kz * 2 + r) * 16 + p];
+    }
   };
+  auto store_round = [&](int jr) {
+#pragma unroll
+    for (int k = 0; k < KPF; ++k) {
+      const int i = static_cast<int>(threadIdx.x) + kZP * k;
+      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+      const int kz = t % NZH, gs = t / NZH;
+      if (i < NSI) stg[((gs * 2 + r) * 4 + p) * NZH + kz] = zf_chan(NF, jr * NS + gs) >= 0 ? pf[k] : cx<S>{S(0), S(0)};
+    }
+  };
+  if constexpr (NRF > 0) {
+    load_round(0);
+    store_round(0);
+    __syncthreads();
+  }
 #define CDFT_ZF(JR)                                                                                   \
   if constexpr ((JR) < NRF) {                                                                         \
-    stage_in(JR);                                                                                     \
-    __syncthreads();                                                                                  \
+    if constexpr ((JR) + 1 < NRF) load_round((JR) + 1);                                               \
     if constexpr (zround64(NF, NS, JR))                                                               \
       zpoint_fwd_round<double, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, tw64l, g.dkz);          \
     else                                                                                              \
       zpoint_fwd_round<CV, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, twvl, g.dkz);               \
     __syncthreads();                                                                                  \
+    if constexpr ((JR) + 1 < NRF) {                                                                   \
+      store_round((JR) + 1);                                                                          \
+      __syncthreads();                                                                                \
+    }                                                                                                 \
   }
   CDFT_ZF(0)
   CDFT_ZF(1)
