@@ -661,7 +661,12 @@ template <typename C>
 __device__ __forceinline__ WFac<C> wfactors(cx<double> w, double kx, double ky, double kz, double invR) {
   const C k2 = static_cast<C>(kx * kx + ky * ky + kz * kz);
   const C num = static_cast<C>(w.im - 3.0 * w.re * invR);
-  return {static_cast<C>(w.re), static_cast<C>(w.im), k2 > C(0) ? num / k2 : C(0)};
+  C a;
+  if constexpr (sizeof(C) == 4)
+    a = num * __builtin_amdgcn_rcpf(k2);  // 1-ulp reciprocal: FP32 weights carry 6e-8 anyway
+  else
+    a = num / k2;
+  return {static_cast<C>(w.re), static_cast<C>(w.im), k2 > C(0) ? a : C(0)};
 }
 
 // FP64 radial FMT weights (w3, w2) (weights.hpp:174-191, times 1/N) of every
