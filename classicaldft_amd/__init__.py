@@ -27,6 +27,7 @@ EXPORTED = [
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
+    "cdft_minimize_grand_potential_batch",
 ]
 
 
@@ -115,6 +116,9 @@ def lib():
         L.cdft_fft_c2r.argtypes = [C.c_void_p, _dp, _dp]
         L.cdft_minimize_grand_potential.argtypes = [C.c_void_p, C.POINTER(_Fire), C.POINTER(_Param), C.c_int, _dp,
                                                     _dp, C.c_double, _dp, C.POINTER(_MinResult), _dp]
+        L.cdft_minimize_grand_potential_batch.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(_Fire),
+                                                          C.POINTER(_Param), C.c_int, C.POINTER(_dp), _dp,
+                                                          C.POINTER(_dp), C.POINTER(_MinResult), C.POINTER(C.c_int)]
         L.cdft_ddft_run.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp,
                                     C.POINTER(_SimResult), _dp, _dp]
         L.cdft_time_total_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
@@ -289,3 +293,40 @@ class Context:
                                    float(mu), _p(rho), C.byref(res), _p(en), None))
         r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
         return rho.reshape(self.shape), r, en
+
+
+def minimize_grand_potential_batch(contexts, rho0s, mus, fire=None, param="unbounded", rho_min=1e-18, rho_max=1.0,
+                                   homogeneous_boundary=True, max_steps=None):
+    """Independent Minimizer::grand_potential runs of several contexts advanced in lockstep on one GPU
+    (cdft_minimize_grand_potential_batch).  Returns a list of (rho, result, status) per run; a run whose
+    FIRE exceeded max_uphill has status CDFT_ERUNTIME (-2) instead of raising."""
+    n = len(contexts)
+    if not (len(rho0s) == len(mus) == n):
+        raise ValueError("contexts, rho0s and mus differ in length")
+    f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
+             n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
+    if fire:
+        f.update(fire)
+    if max_steps is not None:
+        f["max_steps"] = max_steps
+    fs = _Fire(*[f[k] for k, _ in _Fire._fields_])
+    prm = _Param(0 if param == "unbounded" else 1, float(rho_min), float(rho_max))
+    ins = [_f64(r) for r in rho0s]
+    outs = [np.zeros(ctx.N) for ctx in contexts]
+    for ctx, r in zip(contexts, ins):
+        if r.size != ctx.N:
+            raise ValueError("a density does not match its context's grid")
+    hs = (C.c_void_p * max(n, 1))(*[ctx._h for ctx in contexts])
+    pin = (_dp * max(n, 1))(*[_p(r) for r in ins])
+    pout = (_dp * max(n, 1))(*[_p(o) for o in outs])
+    mu = np.array([float(m) for m in mus] or [0.0])
+    res = (_MinResult * max(n, 1))()
+    st = (C.c_int * max(n, 1))()
+    _check(lib().cdft_minimize_grand_potential_batch(n, hs, C.byref(fs), C.byref(prm), int(bool(homogeneous_boundary)),
+                                                     pin, _p(mu), pout, res, st))
+    out = []
+    for i, ctx in enumerate(contexts):
+        r = {k: getattr(res[i], k) for k, _ in _MinResult._fields_}
+        r["converged"] = bool(r["converged"])
+        out.append((outs[i].reshape(ctx.shape), r, int(st[i])))
+    return out

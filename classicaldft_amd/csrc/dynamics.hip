@@ -18,6 +18,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -356,23 +357,43 @@ long face_count(const Geom& g) {
 }  // namespace
 
 // ------------------------------------------------------------------ Minimizer
+// Device FIRE2 in the alias variable (Minimizer::grand_potential,
+// minimization.hpp:280-353; fire::Fire, fire.hpp:60-165) as a state machine:
+// launch_*() queue a step's device work on the context's stream without
+// waiting, after_*() read its reductions once the stream has drained.  One
+// host thread can so advance several independent minimisations (contexts
+// with their own streams) in lockstep, the device working on all at once.
 struct FireRun {
   Context* c;
   AliasMap amap;
   int homog;
   const double* d_vext;
   double mu;
+  cdft_fire fire;
   DevBuf x, v, f, part, res;  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum
   long nb;
+  double* host = nullptr;     // pinned: [0..7] energy sums, [8..15] force sums
+  double* energies = nullptr;  // optional record: after initialisation and every step
+  // FIRE state (fire.hpp:46-58) and the convergence monitor (minimization.hpp:105-110)
+  double energy = 0.0, old_energy = 0.0, sf2 = 0.0, power = 0.0, dt = 0.0, alpha = 0.0, rms = 0.0, vv = 1.0;
+  int n_pos = 0, n_neg = 0, iteration = 0;
+  bool converged = false;
+  cdft_result last_result{};
 
-  FireRun(Context* c_, AliasMap a, int h, const double* vx, double m)
-      : c(c_), amap(a), homog(h), d_vext(vx), mu(m), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
+  FireRun(Context* c_, AliasMap a, int h, const double* vx, double m, const cdft_fire& fr)
+      : c(c_), amap(a), homog(h), d_vext(vx), mu(m), fire(fr), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
         part(8 * 8 * kVecBlocks), res(8 * 24) {
     nb = face_count(c->g);
+    CDFT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), 16 * sizeof(double), hipHostMallocDefault));
   }
+  ~FireRun() {
+    if (host) (void)hipHostFree(host);
+  }
+  FireRun(const FireRun&) = delete;
+  FireRun& operator=(const FireRun&) = delete;
 
   // compute(x): energy Omega and f in d_f; device sums f^2 and v.f in res[0], res[1]
-  double compute(double* sum_f2, double* power) {
+  void launch_compute() {
     const Geom& g = c->g;
     c->eval(x.as<double>(), d_vext, mu, c->d_force, amap);
     if (homog) {
@@ -384,18 +405,97 @@ struct FireRun {
                        v.as<double>(), f.as<double>(), g, homog, res.as<double>() + 16, nb, amap, part.as<double>());
     launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(g.N)), res.as<double>(), c->stream);
     CDFT_HIP(hipGetLastError());
-    double er[8], fr[8];
-    CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
-    CDFT_HIP(hipMemcpyAsync(fr, res.p, sizeof(fr), hipMemcpyDeviceToHost, c->stream));
-    CDFT_HIP(hipStreamSynchronize(c->stream));
-    cdft_result r{};
-    c->result_from(er, mu, &r);
-    last_result = r;
-    *sum_f2 = fr[0];
-    *power = fr[1];
-    return r.grand_potential;
+    CDFT_HIP(hipMemcpyAsync(host, c->d_res, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipMemcpyAsync(host + 8, res.p, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
-  cdft_result last_result{};
+  void read_compute() {
+    cdft_result r{};
+    c->result_from(host, mu, &r);
+    last_result = r;
+    energy = r.grand_potential;
+    sf2 = host[8];
+    power = host[9];
+    rms = std::sqrt(sf2 / static_cast<double>(c->g.N));
+  }
+  void sync() { CDFT_HIP(hipStreamSynchronize(c->stream)); }
+
+  // Fire::initialize (fire.hpp:60-84) and the extra compute of minimization.hpp:312-313
+  void launch_start(const double* rho0) {
+    const Geom& g = c->g;
+    CDFT_HIP(hipSetDevice(c->device));
+    CDFT_HIP(hipMemcpyAsync(c->d_rho, rho0, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_from_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, c->d_rho,
+                       x.as<double>(), g.N, amap);
+    CDFT_HIP(hipMemsetAsync(v.p, 0, 8 * g.N, c->stream));
+    launch_compute();
+  }
+  void after_start() {
+    read_compute();
+    dt = fire.dt;
+    alpha = fire.alpha_start;
+    converged = false;  // minimization.hpp:321: the loop starts unconverged
+    if (energies) energies[0] = energy;
+  }
+  bool active() const { return !converged && iteration < fire.max_steps; }
+
+  // Fire::step (fire.hpp:86-165); `power` is v.f of the current forces
+  void launch_step() {
+    const Geom& g = c->g;
+    old_energy = energy;
+    bool backtrack = false;
+    const double dt_old = dt;
+    if (power > 0.0 || iteration == 0) {
+      ++n_pos;
+      n_neg = 0;
+      if (n_pos > fire.n_delay) {
+        dt = std::min(dt * fire.f_inc, fire.dt_max);
+        alpha *= fire.f_alpha;
+      }
+    } else {
+      n_pos = 0;
+      ++n_neg;
+      if (n_neg > fire.max_uphill) throw std::runtime_error("fire::step: exceeded max uphill steps");
+      backtrack = true;
+      if (iteration > fire.n_delay) {
+        dt = std::max(dt * fire.f_dec, fire.dt_min);
+        alpha = fire.alpha_start;
+      }
+    }
+    CDFT_HIP(hipSetDevice(c->device));
+    const unsigned nbk = vec_blocks(g.N);
+    hipLaunchKernelGGL(k_fire_velocity, dim3(nbk), dim3(256), 0, c->stream, x.as<double>(), v.as<double>(),
+                       f.as<double>(), g.N, dt_old, dt, backtrack ? 1 : 0, part.as<double>());
+    launch_finalize(part.as<double>(), static_cast<int>(nbk), res.as<double>() + 8, c->stream);
+    hipLaunchKernelGGL(k_fire_mix, dim3(nbk), dim3(256), 0, c->stream, x.as<double>(), v.as<double>(),
+                       f.as<double>(), g.N, dt, alpha, res.as<double>() + 8, res.as<double>());
+    launch_compute();
+  }
+  void after_step() {
+    read_compute();
+    ++iteration;
+    // convergence_monitor (minimization.hpp:105-110,329-332)
+    vv = vv * 0.9 + 0.1 * (std::fabs(energy - old_energy) / dt);
+    const double volume = static_cast<double>(c->g.N) * c->g.dv;
+    converged = std::fabs(vv / volume) < fire.force_tolerance;
+    if (energies) energies[iteration] = energy;
+  }
+
+  // the final total() (minimization.hpp:339-343) is the last compute's evaluation
+  void finish(double* rho_out, cdft_min_result* result) {
+    const Geom& g = c->g;
+    CDFT_HIP(hipSetDevice(c->device));
+    hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, x.as<double>(), c->d_rho,
+                       g.N, amap);
+    CDFT_HIP(hipMemcpyAsync(rho_out, c->d_rho, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
+    sync();
+    result->free_energy = last_result.free_energy;
+    result->grand_potential = last_result.grand_potential;
+    result->rms_force = rms;
+    result->iterations = iteration;
+    result->converged = converged ? 1 : 0;
+    result->dt = dt;
+    result->alpha = alpha;
+  }
 };
 
 // ------------------------------------------------------------------ DDFT
@@ -559,72 +659,73 @@ int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire, const cd
     const Geom& g = c->g;
     const AliasMap amap{param->kind == 0 ? 1 : 2, param->rho_min, param->rho_max - param->rho_min};
     if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * g.N, hipMemcpyHostToDevice, c->stream));
-    FireRun run(c, amap, use_homogeneous_boundary ? 1 : 0, vext ? c->d_vext : nullptr, mu);
-    CDFT_HIP(hipMemcpyAsync(c->d_rho, rho0, 8 * g.N, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_from_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, c->d_rho,
-                       run.x.as<double>(), g.N, amap);
-    CDFT_HIP(hipMemsetAsync(run.v.p, 0, 8 * g.N, c->stream));
+    FireRun run(c, amap, use_homogeneous_boundary ? 1 : 0, vext ? c->d_vext : nullptr, mu, *fire);
+    run.energies = energies;
+    run.launch_start(rho0);
+    run.sync();
+    run.after_start();
+    while (run.active()) {
+      run.launch_step();
+      run.sync();
+      run.after_step();
+    }
+    run.finish(rho_out, result);
+  });
+}
 
-    // Fire::initialize (fire.hpp:60-84)
-    double sf2 = 0.0, power = 0.0;
-    double energy = run.compute(&sf2, &power);
-    double dt = fire->dt, alpha = fire->alpha_start;
-    int n_pos = 0, n_neg = 0, iteration = 0;
-    double rms = std::sqrt(sf2 / static_cast<double>(g.N));
-    bool converged = rms < fire->force_tolerance;
-    if (energies) energies[0] = energy;
-    const double volume = static_cast<double>(g.N) * g.dv;
-    double vv = 1.0;
-    converged = false;  // minimization.hpp:321: the loop starts unconverged
-    for (int i = 0; i < fire->max_steps && !converged; ++i) {
-      const double old_energy = energy;
-      // Fire::step (fire.hpp:86-165); `power` is v.f of the current forces
-      bool backtrack = false;
-      const double dt_old = dt;
-      if (power > 0.0 || iteration == 0) {
-        ++n_pos;
-        n_neg = 0;
-        if (n_pos > fire->n_delay) {
-          dt = std::min(dt * fire->f_inc, fire->dt_max);
-          alpha *= fire->f_alpha;
-        }
-      } else {
-        n_pos = 0;
-        ++n_neg;
-        if (n_neg > fire->max_uphill) throw std::runtime_error("fire::step: exceeded max uphill steps");
-        backtrack = true;
-        if (iteration > fire->n_delay) {
-          dt = std::max(dt * fire->f_dec, fire->dt_min);
-          alpha = fire->alpha_start;
+int cdft_minimize_grand_potential_batch(int n, cdft_ctx* const* ctxs, const cdft_fire* fire,
+                                        const cdft_parametrization* param, int use_homogeneous_boundary,
+                                        const double* const* rho0, const double* mu, double* const* rho_out,
+                                        cdft_min_result* results, int* status) {
+  return guarded([&] {
+    if (n < 0 || (n > 0 && (!ctxs || !fire || !param || !rho0 || !mu || !rho_out || !results || !status)))
+      throw std::invalid_argument("null argument");
+    if (param->kind != 0 && param->kind != 1) throw std::invalid_argument("unknown parametrization");
+    for (int i = 0; i < n; ++i) {
+      check_ctx(ctxs[i]);
+      if (!rho0[i] || !rho_out[i]) throw std::invalid_argument("null density");
+      for (int j = 0; j < i; ++j)
+        if (ctxs[j] == ctxs[i]) throw std::invalid_argument("the runs of a batch need distinct contexts");
+      if (ctxs[i]->impl->device != ctxs[0]->impl->device)
+        throw std::invalid_argument("the contexts of a batch must share a device");
+    }
+    const AliasMap amap{param->kind == 0 ? 1 : 2, param->rho_min, param->rho_max - param->rho_min};
+    std::vector<std::unique_ptr<FireRun>> runs;
+    for (int i = 0; i < n; ++i) {
+      Context* c = ctxs[i]->impl;
+      CDFT_HIP(hipSetDevice(c->device));
+      runs.emplace_back(new FireRun(c, amap, use_homogeneous_boundary ? 1 : 0, nullptr, mu[i], *fire));
+      status[i] = CDFT_OK;
+    }
+    for (auto& r : runs) r->launch_start(rho0[&r - &runs[0]]);
+    for (auto& r : runs) {
+      r->sync();
+      r->after_start();
+    }
+    // lockstep: queue every active run's step, then wait for and read each
+    std::vector<char> launched(static_cast<size_t>(n), 0);
+    for (;;) {
+      bool any = false;
+      for (int i = 0; i < n; ++i) {
+        launched[static_cast<size_t>(i)] = 0;
+        if (status[i] != CDFT_OK || !runs[static_cast<size_t>(i)]->active()) continue;
+        try {
+          runs[static_cast<size_t>(i)]->launch_step();
+          launched[static_cast<size_t>(i)] = 1;
+          any = true;
+        } catch (const std::runtime_error& e) {  // FIRE's uphill limit ends that run only
+          if (dynamic_cast<const HipError*>(&e)) throw;
+          status[i] = CDFT_ERUNTIME;
         }
       }
-      const unsigned nb = vec_blocks(g.N);
-      hipLaunchKernelGGL(k_fire_velocity, dim3(nb), dim3(256), 0, c->stream, run.x.as<double>(), run.v.as<double>(),
-                         run.f.as<double>(), g.N, dt_old, dt, backtrack ? 1 : 0, run.part.as<double>());
-      launch_finalize(run.part.as<double>(), static_cast<int>(nb), run.res.as<double>() + 8, c->stream);
-      hipLaunchKernelGGL(k_fire_mix, dim3(nb), dim3(256), 0, c->stream, run.x.as<double>(), run.v.as<double>(),
-                         run.f.as<double>(), g.N, dt, alpha, run.res.as<double>() + 8, run.res.as<double>());
-      energy = run.compute(&sf2, &power);
-      rms = std::sqrt(sf2 / static_cast<double>(g.N));
-      ++iteration;
-      // convergence_monitor (minimization.hpp:105-110,329-332)
-      vv = vv * 0.9 + 0.1 * (std::fabs(energy - old_energy) / dt);
-      const double monitor = std::fabs(vv / volume);
-      converged = monitor < fire->force_tolerance;
-      if (energies) energies[i + 1] = energy;
+      if (!any) break;
+      for (int i = 0; i < n; ++i) {
+        if (!launched[static_cast<size_t>(i)]) continue;
+        runs[static_cast<size_t>(i)]->sync();
+        runs[static_cast<size_t>(i)]->after_step();
+      }
     }
-    hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, run.x.as<double>(), c->d_rho,
-                       g.N, amap);
-    CDFT_HIP(hipMemcpyAsync(rho_out, c->d_rho, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
-    CDFT_HIP(hipStreamSynchronize(c->stream));
-    // the final total() (minimization.hpp:339-343) is the last compute's evaluation
-    result->free_energy = run.last_result.free_energy;
-    result->grand_potential = run.last_result.grand_potential;
-    result->rms_force = rms;
-    result->iterations = iteration;
-    result->converged = converged ? 1 : 0;
-    result->dt = dt;
-    result->alpha = alpha;
+    for (int i = 0; i < n; ++i) runs[static_cast<size_t>(i)]->finish(rho_out[i], &results[i]);
   });
 }
 

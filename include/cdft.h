@@ -185,6 +185,22 @@ CDFT_API int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire,
                                   int use_homogeneous_boundary, const double* rho0, const double* vext, double mu,
                                   double* rho_out, cdft_min_result* result, double* energies);
 
+/* Several independent Minimizer::grand_potential runs advanced in lockstep
+ * (no counterpart in the reference, which solves one state point at a time;
+ * for many small state points that leave most of the device idle alone).
+ * Run i minimises on context ctxs[i] from rho0[i] at mu[i] with the shared
+ * FIRE settings; the contexts must be distinct and on one device, their
+ * grids may differ.  Every step of every active run is queued on its
+ * context's stream before any is waited for, so the device works on all of
+ * them at once; each run follows exactly the trajectory
+ * cdft_minimize_grand_potential gives it.  status[i] receives the run's
+ * status (CDFT_ERUNTIME when FIRE exceeded max_uphill, which ends that run
+ * only); the call itself fails on invalid arguments and device errors. */
+CDFT_API int cdft_minimize_grand_potential_batch(int n, cdft_ctx* const* ctxs, const cdft_fire* fire,
+                                        const cdft_parametrization* param, int use_homogeneous_boundary,
+                                        const double* const* rho0, const double* mu, double* const* rho_out,
+                                        cdft_min_result* results, int* status);
+
 /* dynamics::StepConfig (include/dft/algorithms/dynamics.hpp:26-33). */
 typedef struct {
   double dt, diffusion_coefficient, min_density, dt_max, fp_tolerance;

@@ -114,3 +114,42 @@ def test_ddft_ideal_gas_mode_decay():
     amp = np.sum(out[:, 0, 0] * np.cos(2 * math.pi * x / 16)) / 8
     assert amp / (0.2e-4) == pytest.approx(math.exp(lam * res["time"]), rel=1e-3)
     assert abs(res["mass_final"] - res["mass_initial"]) / res["mass_initial"] < 1e-12
+
+
+def test_minimizer_batch_matches_single_runs():
+    # runs of different grids (fused 64^3 and general-schedule shapes) advanced in lockstep give exactly
+    # the trajectories of separate calls
+    import classicaldft_amd as cd
+
+    specs = [((16, 16, 16), 0.25, 0.05, 3), ((12, 16, 20), 0.25, 0.08, 5), ((64, 64, 64), 0.1, 0.02, 7)]
+    ctxs, rho0s, mus = [], [], []
+    for shape, dx, rv, seed in specs:
+        ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+        ctxs.append(ctx)
+        rho0s.append(W.droplet(shape, dx, radius=0.3 * min(shape) * dx, width=0.4, rho_l=0.6, rho_v=rv, noise=1e-3,
+                               seed=seed))
+        mus.append(ctx.bulk_mu(rv))
+    fire = dict(force_tolerance=1e-9)
+    print("batch of", [s[0] for s in specs], flush=True)
+    out = cd.minimize_grand_potential_batch(ctxs, rho0s, mus, fire=fire, max_steps=30)
+    for (rb, resb, st), ctx, r0, mu in zip(out, ctxs, rho0s, mus):
+        assert st == 0
+        rs, ress, _ = ctx.minimize_grand_potential(r0, mu, fire=fire, max_steps=30)
+        assert resb["iterations"] == ress["iterations"] == 30
+        for k in ("grand_potential", "free_energy", "rms_force", "dt", "alpha"):
+            assert resb[k] == ress[k], k
+        assert np.array_equal(rb, rs)
+
+
+def test_minimizer_batch_reports_uphill_per_run():
+    import classicaldft_amd as cd
+
+    shape, dx = (16, 16, 16), 0.25
+    ctxs = [cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII") for _ in range(2)]
+    rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    mu = ctxs[0].bulk_mu(0.05)
+    fire = dict(force_tolerance=1e-9, dt=0.2, dt_max=0.4, max_uphill=0)
+    out = cd.minimize_grand_potential_batch(ctxs, [rho0, rho0], [mu, mu], fire=fire, max_steps=60)
+    assert [st for _, _, st in out] == [-2, -2]
+    with pytest.raises(ValueError):
+        cd.minimize_grand_potential_batch([ctxs[0], ctxs[0]], [rho0, rho0], [mu, mu], fire=fire, max_steps=5)
