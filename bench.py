@@ -83,6 +83,51 @@ def measured_traffic(kernel, precision):
     return None if e is None else e["hbm_bytes_per_launch"]
 
 
+def batched_fire(cd, W, precision, n=16, steps=40, sizes=(48, 64, 80, 96)):
+    """Batched form (a bounded sample of the 64-state-point workload): n seeded LJ state points of uneven
+    grid size (kT ~ U(0.7, 1.2), bulk rho ~ U(0.05, 0.80), droplet or slab), `steps` FIRE steps each, run
+    one after another and then advanced in lockstep by cdft_minimize_grand_potential_batch."""
+    import torch
+
+    pts = W.batch_state_points(n, sizes=sizes)
+    ctxs, mus = [], []
+    for p in pts:
+        ctx = cd.Context(p["shape"], p["dx"], interactions=[W.LJ_WCA], kT=p["kT"], model="WhiteBearII",
+                         precision=precision)
+        ctxs.append(ctx)
+        mus.append(ctx.bulk_mu(p["rho_bulk"]))
+    fire = dict(force_tolerance=1e-30)  # every run takes all its steps (or stops at FIRE's uphill limit)
+    rho0s = [p["rho0"] for p in pts]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq = []
+    for ctx, r0, mu in zip(ctxs, rho0s, mus):
+        try:
+            _, res, _ = ctx.minimize_grand_potential(r0, mu, fire=fire, max_steps=steps, record_energies=False)
+            seq.append(res["iterations"])
+        except cd.CdftError:
+            seq.append(-1)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    out = cd.minimize_grand_potential_batch(ctxs, rho0s, mus, fire=fire, max_steps=steps)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t0
+    done = sum(r["iterations"] for _, r, st in out if st == 0)
+    return {
+        "state_points": n,
+        "grids": [p["shape"][0] for p in pts],
+        "fire_steps_each": steps,
+        "precision": precision,
+        "uphill_stops": sum(1 for _, _, st in out if st != 0),
+        "sequential_s": t_seq,
+        "batched_s": t_bat,
+        "batched_fire_steps_per_s": done / t_bat,
+        "speedup_vs_sequential": t_seq / t_bat,
+        "same_iterations": [r["iterations"] if st == 0 else -1 for _, r, st in out] == seq,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +137,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the same-GPU FP64 comparison run")
     ap.add_argument("--profile-reps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential FIRE measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -199,6 +245,9 @@ def main():
             res64["grand_potential"])
         out["force_rel_err_vs_fp64_gpu"] = float(np.max(np.abs(forces - f64)) / np.max(np.abs(f64)))
         del ctx64
+
+    if not args.no_batch:
+        out["batched"] = batched_fire(cd, W, args.precision)
 
     if not args.no_cpu_baseline:
         cb, parts, fr = cpu_baseline(rho, mu, shape, dx)

@@ -78,3 +78,28 @@ def smooth_noise_fluid(shape, dx, rho0=0.5, amp=0.01, sigma=0.3, seed=7):
 
 
 LJ_WCA = dict(potential="LJ", sigma=1.0, epsilon=1.0, r_cutoff=2.5, split="WCA", scheme="quadraticF")
+
+
+BATCH_SIZES = (48, 64, 80, 96, 128, 160)
+
+
+def batch_state_points(n, seed=5, sizes=BATCH_SIZES):
+    """n seeded state points of uneven grid size for the batched form: cubic grid of a size drawn from `sizes`
+    (dx = 0.1), kT ~ U(0.7, 1.2), bulk density ~ U(0.05, 0.80), a tanh droplet or slab between that density and a
+    liquid / vapour partner.  Returns a list of dicts with shape, dx, kT, rho_bulk and the initial profile."""
+    u = uniform(seed, 5 * n).reshape(n, 5)
+    pts = []
+    for i in range(n):
+        L = sizes[min(int(u[i, 0] * len(sizes)), len(sizes) - 1)]
+        shape, dx = (L, L, L), 0.1
+        kT = 0.7 + 0.5 * u[i, 1]
+        rb = 0.05 + 0.75 * u[i, 2]
+        other = 0.7 if rb < 0.4 else 0.05
+        if u[i, 3] < 0.5:
+            prof = droplet(shape, dx, radius=0.25 * L * dx, width=0.5, rho_l=other if rb < 0.4 else rb,
+                           rho_v=rb if rb < 0.4 else other, noise=1e-3, seed=seed + 17 * i)
+        else:
+            prof = slab(shape, dx, fraction=0.4, width=0.5, rho_l=max(rb, other), rho_v=min(rb, other), noise=1e-3,
+                        seed=seed + 17 * i)
+        pts.append(dict(shape=shape, dx=dx, kT=kT, rho_bulk=rb, rho0=prof))
+    return pts
