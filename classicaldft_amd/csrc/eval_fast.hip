@@ -294,6 +294,14 @@ __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, d
         return a;
       },
       [&](int i, cx<C> a) { cb[(i & 31) * NZH + (i >> 5)] = a; });
+  // the local force of the lane's points, loaded while the transform runs
+  double ad[2][8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const int z = lane + T * e;
+    ad[0][e] = add ? add[la * NZ + z] : 0.0;
+    ad[1][e] = add ? add[lb * NZ + z] : 0.0;
+  }
   __syncthreads();
   cx<C> v[8];
   build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[(r * 16 + slot) * NZH + kz]; });
@@ -301,9 +309,8 @@ __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, d
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int z = lane + T * e;
-    const long pa = la * NZ + z, pb = lb * NZ + z;
-    out[pa] = static_cast<double>(v[e].re) * scale + (add ? add[pa] : 0.0);
-    out[pb] = static_cast<double>(v[e].im) * scale + (add ? add[pb] : 0.0);
+    out[la * NZ + z] = static_cast<double>(v[e].re) * scale + ad[0][e];
+    out[lb * NZ + z] = static_cast<double>(v[e].im) * scale + ad[1][e];
   }
 }
 
@@ -559,19 +566,28 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   // backward round's), so no barrier is spent on it.
   __shared__ double wsum[kZP / 64][8];
   {
-    double tot[5];
+    // through the wave's own LDS exchange region (free between the rounds):
+    // lanes 0..39 add 8 lanes' values of one sum each, lanes 0..4 the 8 of those
+    double* red = reinterpret_cast<double*>(xwave);
+    wave_sync();
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-      double x = acc[i];
+    for (int i = 0; i < 5; ++i) red[i * 64 + lid] = acc[i];
+    wave_sync();
+    double t = 0.0;
+    if (lid < 40) {
+      const double* src = red + (lid >> 3) * 64 + (lid & 7) * 8;
 #pragma unroll
-      for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-      tot[i] = x;
+      for (int j = 0; j < 8; ++j) t += src[j];
     }
+    wave_sync();
+    if (lid < 40) red[lid] = t;
+    wave_sync();
     if (lid < 8) {
-      double t = 0.0;
+      double u = 0.0;
+      if (lid < 5)
 #pragma unroll
-      for (int i = 0; i < 5; ++i) t = lid == i ? tot[i] * dv : t;
-      wsum[wave][lid] = t;
+        for (int j = 0; j < 8; ++j) u += red[lid * 8 + j];
+      wsum[wave][lid] = u * dv;
     }
   }
   auto write_row = [&]() {
