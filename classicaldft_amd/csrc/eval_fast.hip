@@ -449,25 +449,34 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   // transforms, so their memory latency overlaps that work.
   constexpr int NSI = NS * NZH * 8, KPF = (NSI + kZP - 1) / kZP;
   cx<S> pf[KPF];
+  // element i = tid + kZP k of a round: (slot gs, part r, line pair p, kz);
+  // its offset in a channel's chunk and in the staging buffer are the same
+  // every round and computed once
+  int goff[KPF], soff[KPF], gsk[KPF];
+#pragma unroll
+  for (int k = 0; k < KPF; ++k) {
+    const int i = min(static_cast<int>(threadIdx.x) + kZP * k, NSI - 1);
+    const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+    const int kz = t % NZH, gs = t / NZH;
+    goff[k] = (kz * 2 + r) * 16 + p;
+    soff[k] = ((gs * 2 + r) * 4 + p) * NZH + kz;
+    gsk[k] = gs;
+  }
+  const cx<S>* sfc = SF + cbase;
   // (every lane loads, from channel 0 where its slot stages nothing: a load
   // under a condition would make the transforms wait for it)
   auto load_round = [&](int jr) {
 #pragma unroll
     for (int k = 0; k < KPF; ++k) {
-      const int i = min(static_cast<int>(threadIdx.x) + kZP * k, NSI - 1);
-      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
-      const int kz = t % NZH, gs = t / NZH;
-      const int ch = zf_chan(NF, jr * NS + gs);
-      pf[k] = SF[static_cast<long>(ch >= 0 ? chan_store(NF, ch) : 0) * g.M + cbase + (kz * 2 + r) * 16 + p];
+      const int ch = zf_chan(NF, jr * NS + gsk[k]);
+      pf[k] = sfc[static_cast<long>(ch >= 0 ? chan_store(NF, ch) : 0) * g.M + goff[k]];
     }
   };
   auto store_round = [&](int jr) {
 #pragma unroll
     for (int k = 0; k < KPF; ++k) {
-      const int i = static_cast<int>(threadIdx.x) + kZP * k;
-      const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
-      const int kz = t % NZH, gs = t / NZH;
-      if (i < NSI) stg[((gs * 2 + r) * 4 + p) * NZH + kz] = zf_chan(NF, jr * NS + gs) >= 0 ? pf[k] : cx<S>{S(0), S(0)};
+      if (NSI % kZP == 0 || static_cast<int>(threadIdx.x) + kZP * k < NSI)
+        stg[soff[k]] = zf_chan(NF, jr * NS + gsk[k]) >= 0 ? pf[k] : cx<S>{S(0), S(0)};
     }
   };
   if constexpr (NRF > 0) {
@@ -607,24 +616,23 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   // eta's derivative spectrum takes -i kz times the z vector's (positions 0 and 1)
   const S dks = static_cast<S>(g.dkz);
   auto stage_out = [&](int jr) {
-    block_copy<kZP, NS * NZH * 8, cx<S>>(
-        [&](int i) {
-          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
-          const int kz = t % NZH, gs = t / NZH;
-          cx<S> v = stg[((gs * 2 + r) * 4 + p) * NZH + kz];
-          if (jr == 0 && gs == 0) {
-            const cx<S> vz = stg[((1 * 2 + r) * 4 + p) * NZH + kz];
-            const S k = static_cast<S>(kz) * dks;
-            v = {v.re + k * vz.im, v.im - k * vz.re};
-          }
-          return v;
-        },
-        [&](int i, cx<S> val) {
-          const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
-          const int kz = t % NZH, gs = t / NZH;
-          const int b = jr * NS + gs;
-          if (b < kZB && b != 1) SF[static_cast<long>(zb_chan(b)) * g.M + cbase + (kz * 2 + r) * 16 + p] = val;
-        });
+    cx<S> vals[KPF];
+#pragma unroll
+    for (int k = 0; k < KPF; ++k) {
+      cx<S> v = stg[soff[k]];
+      if (jr == 0 && gsk[k] == 0) {
+        const cx<S> vz = stg[soff[k] + 8 * NZH];        // position 1 of the same (r, p, kz)
+        const S kk = static_cast<S>(goff[k] >> 5) * dks;  // kz = goff >> 5
+        v = {v.re + kk * vz.im, v.im - kk * vz.re};
+      }
+      vals[k] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < KPF; ++k) {
+      const int b = jr * NS + gsk[k];
+      if ((NSI % kZP == 0 || static_cast<int>(threadIdx.x) + kZP * k < NSI) && b < kZB && b != 1)
+        SF[static_cast<long>(zb_chan(b)) * g.M + cbase + goff[k]] = vals[k];
+    }
   };
 #define CDFT_ZB(JR)                                                                                           \
   if constexpr ((JR) < NRB) {                                                                                 \
