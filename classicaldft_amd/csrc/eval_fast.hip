@@ -894,7 +894,8 @@ struct FwdCfg {
 };
 
 // One channel in a kf_plane_fwd block: Wof(cc, e) is the weighted spectrum at
-// column xs + SBX cc, kx = xl + TX e.  x-inverse of the output half
+// column xs + SBX cc, kx = xl + TX e; soff[rr][e] is the split-layout offset
+// of the lane's output (row ys + SBYH rr, m = yl + TYH e) in a channel array.  x-inverse of the output half
 // ix = 2i + h (DIF: (W[kx] +- W[kx + NX/2]) twiddled), then the y-inverse of
 // every row of that half, stored in the split layout.  With DB, channels
 // alternate between two transpose buffers and the x- and y-slot buffers are
@@ -904,7 +905,8 @@ struct FwdCfg {
 // barrier closes every channel.
 template <typename CC, typename S, int NX, int Q, bool DB, typename WF>
 __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned char* xbx, unsigned char* xby,
-                                                  cx<S>* __restrict__ out, const FastGeom& g, int kz, int r, int h,
+                                                  cx<S>* __restrict__ out,
+                                                  const int (&soff)[PlaneCfg<NX, Q>::RPH][4], int r, int h,
                                                   const cx<CC>* __restrict__ tw) {
   using P = PlaneCfg<NX, Q>;
   constexpr int QP = Q + 1;
@@ -942,7 +944,7 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
         const int m = yl + P::TYH * e;
         cx<CC> b = v[e];
         if (r) b = twiddle<CC, P::NY, +1>(b, m, tw);
-        out[split_idx(kz, r, 2 * row + h, m, P::MB, g.nzh)] = cvt<S>(b);
+        out[soff[rr][e]] = cvt<S>(b);
       }
     }
   }
@@ -989,6 +991,14 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
     const int p1 = static_cast<int>(min(static_cast<long>(NFP), p0 + (i1 - it)));
     it += p1 - p0;
     const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+    int soff[P::RPH][4];  // the lane's output offsets, the same for every channel of the unit
+#pragma unroll
+    for (int rr = 0; rr < P::RPH; ++rr)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int row = min(tid / P::TYH + P::SBYH * rr, P::NXH - 1), m = tid % P::TYH + P::TYH * e;
+        soff[rr][e] = static_cast<int>(split_idx(kz, r, 2 * row + h, m, P::MB, g.nzh));
+      }
     // FP64 spectrum of the lane's elements, read where it is used (L1/L2):
     // Xof(cc, e) = rho_hat(kx = xl + TX e, ky = r + 2 col)
     const long uoff = (static_cast<long>(kz) * 2 + r) * Q * NX + xl;
@@ -1018,7 +1028,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
             const cx<double> w = Wof(cc, e);
             return cscale(Xof(cc, e), ch == 0 ? w.re : w.im);
           },
-          tbuf(nch), xbx, xby, SF + chan_store(NF, ch) * g.M, g, kz, r, h, tw64l);
+          tbuf(nch), xbx, xby, SF + chan_store(NF, ch) * g.M, soff, r, h, tw64l);
     }
     if constexpr (NF >= 11) {
       if (p1 > N64) {
@@ -1044,7 +1054,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
               [&](int cc, int e) -> cx<CV> {
                 return cmul(chan_weight<CV>(ch, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc), XC[cc][e]);
               },
-              tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, g, kz, r, h, twvl);
+              tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, soff, r, h, twvl);
         }
       }
     }
@@ -1135,15 +1145,20 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
     const double kzv = kz * g.dkz;
 
     cx<S> nxt[PC::RPH][4];
+    int loff[PC::RPH][4];  // the lane's input offsets, the same for every channel of the unit
+#pragma unroll
+    for (int rr = 0; rr < PC::RPH; ++rr) {
+      const int i = ysh + PC::SBYH * rr;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        loff[rr][q] = static_cast<int>(split_idx(kz, r, 2 * (i < NXH ? i : 0) + h, ylh + PC::TYH * q, PC::MB, g.nzh));
+    }
     auto load_in = [&](int ci) {
       const cx<S>* in = SB + static_cast<long>(bwd_channel(ci)) * g.M;
 #pragma unroll
-      for (int rr = 0; rr < PC::RPH; ++rr) {
-        const int i = ysh + PC::SBYH * rr;
+      for (int rr = 0; rr < PC::RPH; ++rr)
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-          nxt[rr][q] = in[split_idx(kz, r, 2 * (i < NXH ? i : 0) + h, ylh + PC::TYH * q, PC::MB, g.nzh)];
-      }
+        for (int q = 0; q < 4; ++q) nxt[rr][q] = in[loff[rr][q]];
     };
     load_in(c0);
     // weight factors of the lane's accumulator elements (kx = xl + TX e, column xs + SBX cc),
