@@ -27,7 +27,8 @@ EXPORTED = [
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
-    "cdft_minimize_grand_potential_batch",
+    "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
+    "cdft_ddft_adjoint_wells",
 ]
 
 
@@ -87,6 +88,16 @@ class _SimResult(C.Structure):
                 ("energy", C.c_double), ("dt", C.c_double), ("evaluations", C.c_long)]
 
 
+class _AdjointConfig(C.Structure):
+    _fields_ = [("dt", C.c_double), ("fp_iterations", C.c_int), ("n_steps", C.c_int),
+                ("checkpoint_interval", C.c_int)]
+
+
+class _AdjointResult(C.Structure):
+    _fields_ = [("loss", C.c_double), ("min_density", C.c_double), ("evaluations", C.c_long),
+                ("hessian_products", C.c_long), ("checkpoints", C.c_int), ("store_bytes", C.c_double)]
+
+
 _dp = C.POINTER(C.c_double)
 _lib = None
 
@@ -128,6 +139,11 @@ def lib():
         L.cdft_stage_name.restype = C.c_char_p
         L.cdft_stage_name.argtypes = [C.c_void_p, C.c_int]
         L.cdft_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.cdft_hessian_vector.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.cdft_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.cdft_ddft_fixed_run.argtypes = [C.c_void_p, C.POINTER(_AdjointConfig), _dp, C.c_double, _dp, _dp]
+        L.cdft_ddft_adjoint_wells.argtypes = [C.c_void_p, C.POINTER(_AdjointConfig), C.c_int, _dp, C.c_double, _dp,
+                                              _dp, C.POINTER(_AdjointResult), _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -293,6 +309,47 @@ class Context:
                                    float(mu), _p(rho), C.byref(res), _p(en), None))
         r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
         return rho.reshape(self.shape), r, en
+
+    # ------------------------------------------------------------ adjoint (FP64 contexts)
+    def hessian_vector(self, rho, u):
+        """H(rho) u, the Jacobian of the forces of total() applied to u."""
+        out = np.zeros(self.N)
+        _check(lib().cdft_hessian_vector(self._h, _p(_f64(rho)), _p(_f64(u)), _p(out)))
+        return out.reshape(self.shape)
+
+    def wells_potential(self, params):
+        """Sum of Gaussian wells; params: (n_wells, 5) rows of (A, w, cx, cy, cz)."""
+        prm = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 5)
+        out = np.zeros(self.N)
+        _check(lib().cdft_wells_potential(self._h, prm.shape[0], _p(prm.reshape(-1)), _p(out)))
+        return out.reshape(self.shape)
+
+    def ddft_fixed_run(self, rho0, mu, n_steps, dt, fp_iterations, vext=None):
+        """Fixed-dt DDFT map with exactly fp_iterations iterations per step; returns (rho, min density)."""
+        cfg = _AdjointConfig(float(dt), int(fp_iterations), int(n_steps), 0)
+        rho = _f64(rho0).copy()
+        mn = C.c_double(0.0)
+        _check(lib().cdft_ddft_fixed_run(self._h, C.byref(cfg), None if vext is None else _p(_f64(vext)), float(mu),
+                                         _p(rho), C.byref(mn)))
+        return rho.reshape(self.shape), mn.value
+
+    def ddft_adjoint_wells(self, rho0, target, params, mu, n_steps, dt, fp_iterations, checkpoint_interval=0,
+                           want_fields=False):
+        """Loss sum (rho_K - target)^2 dV of the fixed-dt DDFT map under a well potential, and its gradient
+        with respect to the (n_wells, 5) well parameters.  With want_fields also dL/drho0, dL/dV and rho_K."""
+        prm = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 5)
+        cfg = _AdjointConfig(float(dt), int(fp_iterations), int(n_steps), int(checkpoint_interval))
+        res = _AdjointResult()
+        grad = np.zeros(prm.size)
+        fields = [np.zeros(self.N) for _ in range(3)] if want_fields else [None, None, None]
+        _check(lib().cdft_ddft_adjoint_wells(self._h, C.byref(cfg), prm.shape[0], _p(prm.reshape(-1)), float(mu),
+                                             _p(_f64(rho0)), _p(_f64(target)), C.byref(res), _p(grad),
+                                             *[_p(f) for f in fields]))
+        r = {k: getattr(res, k) for k, _ in _AdjointResult._fields_}
+        out = [grad.reshape(prm.shape), r]
+        if want_fields:
+            out.append(tuple(f.reshape(self.shape) for f in fields))
+        return tuple(out)
 
 
 def minimize_grand_potential_batch(contexts, rho0s, mus, fire=None, param="unbounded", rho_min=1e-18, rho_max=1.0,

@@ -1,0 +1,367 @@
+// DDFT integrating-factor machinery shared by the forward scheme
+// (cdft_ddft_run) and its adjoint: the right-hand side stencil, the
+// propagator fused with the x stage of the transforms, the FP64 update of
+// the iterate, and DdftRun, which owns the device state of one run
+// (integrating_factor_step, include/dft/algorithms/dynamics.hpp:79-401).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "context.hpp"
+
+namespace cdft {
+
+// Kernels, helpers and DdftRun have internal linkage: each translation unit
+// that includes this header gets its own copy.
+namespace {
+
+__device__ __forceinline__ void acc2(double& s, double& c, double x) {
+  const double t = s + x;
+  c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);
+  s = t;
+}
+
+// Block-reduce up to 4 sums into partials[blockIdx.x*8 + i] (slots 4..7 zero).
+__device__ __forceinline__ void block_sums(const double* v, int nv, double* partials) {
+  __shared__ double sh[4][256];
+  for (int i = 0; i < 4; ++i) sh[i][threadIdx.x] = i < nv ? v[i] : 0.0;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int i = 0; i < 4; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x < 4 ? sh[threadIdx.x][0] : 0.0;
+}
+
+// ---------------------------------------------------------------- DDFT kernels
+// R = div(rho grad(F/dV)) - Laplacian(rho), periodic 7-point stencils (dynamics.hpp:135-229)
+__global__ void __launch_bounds__(256) k_ddft_rhs(const double* __restrict__ rho, const double* __restrict__ force,
+                                                  double* __restrict__ rhs, Geom G, double* partials) {
+  double s = 0, c = 0;
+  const long nx = G.nx, ny = G.ny, nz = G.nz, nyz = ny * nz;
+  const double D = 0.5 / (G.dx * G.dx), D2 = 1.0 / (G.dx * G.dx), idv = 1.0 / G.dv;
+  for (long p = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; p < G.N;
+       p += static_cast<long>(gridDim.x) * blockDim.x) {
+    const long iz = p % nz, iy = (p / nz) % ny, ix = p / nyz;
+    const long px = ((ix + 1) % nx) * nyz + iy * nz + iz, mx = ((ix - 1 + nx) % nx) * nyz + iy * nz + iz;
+    const long py = ix * nyz + ((iy + 1) % ny) * nz + iz, my = ix * nyz + ((iy - 1 + ny) % ny) * nz + iz;
+    const long pz = ix * nyz + iy * nz + (iz + 1) % nz, mz = ix * nyz + iy * nz + (iz - 1 + nz) % nz;
+    const double d0 = rho[p], x0 = force[p] * idv;
+    const double rpx = rho[px], rmx = rho[mx], rpy = rho[py], rmy = rho[my], rpz = rho[pz], rmz = rho[mz];
+    const double xpx = force[px] * idv, xmx = force[mx] * idv, xpy = force[py] * idv, xmy = force[my] * idv;
+    const double xpz = force[pz] * idv, xmz = force[mz] * idv;
+    double r = D * ((rpx + d0) * (xpx - x0) - (d0 + rmx) * (x0 - xmx)) +
+               D * ((rpy + d0) * (xpy - x0) - (d0 + rmy) * (x0 - xmy)) +
+               D * ((rpz + d0) * (xpz - x0) - (d0 + rmz) * (x0 - xmz));
+    r -= D2 * (rpx + rmx - 2.0 * d0) + D2 * (rpy + rmy - 2.0 * d0) + D2 * (rpz + rmz - 2.0 * d0);
+    rhs[p] = r;
+    acc2(s, c, r);
+  }
+  const double vals[1] = {s + c};
+  block_sums(vals, 1, partials);
+}
+
+// Integrating-factor propagator fused with the x stage of the transforms
+// (dynamics.hpp:247-289): x-FFT of R1 (z, y done), combine
+// w = U0 rho0_k + U1 R0_k + U2 (R1_k - R0_k) on the mean-free spectra (the
+// k = 0 mode is propagated separately in FP64), inverse x-FFT.
+template <typename S, typename C, int EPT>
+__global__ void __launch_bounds__(256) k_xprop(const cx<S>* __restrict__ SR1, const cx<S>* __restrict__ Srho0,
+                                               const cx<S>* __restrict__ SR0, cx<S>* __restrict__ Sout, Geom g,
+                                               FftPlan px, const cx<C>* __restrict__ twx, int ty,
+                                               const double* __restrict__ lam, const double* __restrict__ fac,
+                                               double dt, int first_iter) {
+  extern __shared__ unsigned char smem_raw[];
+  cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
+  const int nx = g.nx, ny = g.ny, ls = nx + 1;
+  const int y0 = blockIdx.x * ty;
+  const int kz = blockIdx.y;
+  const long off = static_cast<long>(kz) * nx * ny;
+  if (!first_iter) {
+    for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
+      const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+      const cx<S> v = (y < ny) ? SR1[off + x * ny + y] : cx<S>{S(0), S(0)};
+      buf[yl * ls + x] = {static_cast<C>(v.re), static_cast<C>(v.im)};
+    }
+    __syncthreads();
+    fft_inplace<C, EPT>(buf, ty, ls, px, -1, twx);
+  }
+  // lam/fac layout: [0, nx) x axis, [nx, nx+ny) y axis, [nx+ny, nx+ny+nzh) z axis
+  cx<C> wv[EPT];
+#pragma unroll
+  for (int m = 0; m < EPT; ++m) {
+    const int e = threadIdx.x + m * blockDim.x;
+    wv[m] = {C(0), C(0)};
+    if (e < ty * nx) {
+      const int x = e / ty, yl = e - x * ty, y = min(y0 + yl, ny - 1);
+      const double L = lam[x] + lam[nx + y] + lam[nx + ny + kz];
+      const double ex = fac[x] * fac[nx + y] * fac[nx + ny + kz];
+      const bool zero = (x == 0 && y == 0 && kz == 0);
+      const double U0 = ex;
+      const double U1 = zero ? dt : (ex - 1.0) / L;
+      const double U2 = zero ? dt / 2.0 : (ex - 1.0 - dt * L) / (L * L * dt);
+      const long idx = off + x * ny + y;
+      const cx<S> r0 = SR0[idx], p0 = Srho0[idx];
+      double r1re = static_cast<double>(r0.re), r1im = static_cast<double>(r0.im);
+      if (!first_iter) {
+        r1re = static_cast<double>(buf[yl * ls + x].re);
+        r1im = static_cast<double>(buf[yl * ls + x].im);
+      }
+      double wr = U0 * static_cast<double>(p0.re) + U1 * static_cast<double>(r0.re) +
+                  U2 * (r1re - static_cast<double>(r0.re));
+      double wi = U0 * static_cast<double>(p0.im) + U1 * static_cast<double>(r0.im) +
+                  U2 * (r1im - static_cast<double>(r0.im));
+      if (zero) wr = wi = 0.0;  // mean mode handled in FP64 by k_ddft_update
+      wv[m] = {static_cast<C>(wr), static_cast<C>(wi)};
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < EPT; ++m) {
+    const int e = threadIdx.x + m * blockDim.x;
+    if (e < ty * nx) {
+      const int x = e / ty, yl = e - x * ty;
+      buf[yl * ls + x] = wv[m];
+    }
+  }
+  __syncthreads();
+  fft_inplace<C, EPT>(buf, ty, ls, px, +1, twx);
+  for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
+    const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+    if (y < ny) Sout[off + x * ny + y] = {static_cast<S>(buf[yl * ls + x].re), static_cast<S>(buf[yl * ls + x].im)};
+  }
+}
+
+// d1_new = mean_new + raw - mean(raw); max |d1_new - d1_old|; min d1_new (apply_propagator, dynamics.hpp:289-303).
+// sums: [0] sum rho0, [8] sum R0, [16] sum R1, [24] sum raw
+__global__ void __launch_bounds__(256) k_ddft_update(const double* __restrict__ raw, double* __restrict__ d1, long n,
+                                                     const double* __restrict__ sums, double dt, int first_iter,
+                                                     double* partials) {
+  const double inv_n = 1.0 / static_cast<double>(n);
+  const double s0 = sums[0], r0 = sums[8], r1 = first_iter ? sums[8] : sums[16];
+  const double mean_new = (s0 + dt * r0 + 0.5 * dt * (r1 - r0)) * inv_n;
+  const double shift = mean_new - sums[24] * inv_n;
+  double mx = 0.0, mn = 1e300;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const double nv = raw[i] + shift;
+    mx = fmax(mx, fabs(d1[i] - nv));
+    mn = fmin(mn, nv);
+    d1[i] = nv;
+  }
+  __shared__ double smx[256], smn[256];
+  smx[threadIdx.x] = mx;
+  smn[threadIdx.x] = mn;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + w]);
+      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[static_cast<long>(blockIdx.x) * 8 + 0] = smx[0];
+    partials[static_cast<long>(blockIdx.x) * 8 + 1] = smn[0];
+  }
+}
+
+__global__ void k_finalize_maxmin(const double* __restrict__ partials, int nblk, double* __restrict__ res) {
+  __shared__ double smx[256], smn[256];
+  double mx = 0.0, mn = 1e300;
+  for (int b = threadIdx.x; b < nblk; b += blockDim.x) {
+    mx = fmax(mx, partials[static_cast<long>(b) * 8 + 0]);
+    mn = fmin(mn, partials[static_cast<long>(b) * 8 + 1]);
+  }
+  smx[threadIdx.x] = mx;
+  smn[threadIdx.x] = mn;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) {
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + w]);
+      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    res[0] = smx[0];
+    res[1] = smn[0];
+  }
+}
+
+constexpr int kVecBlocks = 1024;
+
+unsigned vec_blocks(long n) {
+  long b = (n + 255) / 256;
+  return static_cast<unsigned>(std::min<long>(std::max<long>(b, 1), kVecBlocks));
+}
+
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  explicit DevBuf(size_t bytes) {
+    if (bytes) CDFT_HIP(hipMalloc(&p, bytes));
+  }
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  template <typename T>
+  T* as() const {
+    return static_cast<T*>(p);
+  }
+};
+
+// ------------------------------------------------------------------ DDFT
+struct DdftRun {
+  Context* c;
+  const double* d_vext;
+  double mu;
+  DevBuf rho0, d1, f0, f1, rhs, raw;
+  DevBuf Srho0, SR0, SR1, Sw;
+  DevBuf lam, fac, part, sums, mm;
+  std::vector<double> lam_h;
+  double st_dt = 0.0;
+  long evals = 0;
+
+  DdftRun(Context* c_, const double* vx, double m)
+      : c(c_), d_vext(vx), mu(m), rho0(8 * c_->g.N), d1(8 * c_->g.N), f0(8 * c_->g.N), f1(8 * c_->g.N),
+        rhs(8 * c_->g.N), raw(8 * c_->g.N), Srho0(c_->csize() * c_->g.M), SR0(c_->csize() * c_->g.M),
+        SR1(c_->csize() * c_->g.M), Sw(c_->csize() * c_->g.M),
+        lam(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)), fac(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)),
+        part(8 * 8 * kVecBlocks), sums(8 * 40), mm(8 * 8) {
+    const Geom& g = c->g;
+    // make_if_state (dynamics.hpp:79-117): Lambda_d = 2 (cos(2 pi i / N_d) - 1) / dx^2
+    const double PI = 3.14159265358979323846, Dx = 1.0 / (g.dx * g.dx);
+    lam_h.resize(g.nx + g.ny + g.nzh);
+    for (int i = 0; i < g.nx; ++i) lam_h[i] = 2.0 * Dx * (std::cos(2.0 * PI * i / g.nx) - 1.0);
+    for (int i = 0; i < g.ny; ++i) lam_h[g.nx + i] = 2.0 * Dx * (std::cos(2.0 * PI * i / g.ny) - 1.0);
+    for (int i = 0; i < g.nzh; ++i) lam_h[g.nx + g.ny + i] = 2.0 * Dx * (std::cos(2.0 * PI * i / g.nz) - 1.0);
+    CDFT_HIP(hipMemcpy(lam.p, lam_h.data(), 8 * lam_h.size(), hipMemcpyHostToDevice));
+  }
+
+  // update_timestep (dynamics.hpp:121-129)
+  void update_timestep(double dt) {
+    if (std::fabs(st_dt - dt) < 1e-30) return;
+    st_dt = dt;
+    std::vector<double> f(lam_h.size());
+    for (size_t i = 0; i < f.size(); ++i) f[i] = std::exp(lam_h[i] * dt);
+    CDFT_HIP(hipMemcpyAsync(fac.p, f.data(), 8 * f.size(), hipMemcpyHostToDevice, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+  }
+
+  double compute(const double* rho, double* forces) {
+    c->eval(rho, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
+    ++evals;
+    double er[8];
+    CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    cdft_result r{};
+    c->result_from(er, mu, &r);
+    return r.grand_potential;
+  }
+
+  // Full forward transform of a real field into S (mean removed; its FP64 sum to sums[slot]).
+  template <typename R>
+  void forward(const double* field, cx<R>* S, int slot, bool x_stage) {
+    const Geom& g = c->g;
+    const AliasMap id{0, 0.0, 0.0};
+    double* s = sums.as<double>() + 8 * slot;
+    launch_sum(field, id, part.as<double>(), g, c->stream);
+    launch_finalize(part.as<double>(), sum_blocks(g), s, c->stream);
+    launch_zr2c<R, R>(field, S, g, c->plan[2], c->tw<R>(2), id, s, c->stream);
+    launch_lines<R, R>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), -1, c->stream);
+    if (x_stage) launch_xlines<R, R>(S, 1, g, c->plan[0], c->tw<R>(0), -1, c->stream);
+  }
+
+  // excess_rhs_fourier (dynamics.hpp:212-229) into S (x stage optional)
+  template <typename R>
+  void rhs_forward(const double* rho, const double* forces, cx<R>* S, int slot, bool x_stage) {
+    const Geom& g = c->g;
+    hipLaunchKernelGGL(k_ddft_rhs, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, rho, forces, rhs.as<double>(), g,
+                       part.as<double>());
+    forward<R>(rhs.as<double>(), S, slot, x_stage);
+  }
+
+  // one propagator application; returns {max deviation, min d1}
+  template <typename R>
+  void propagate(bool first_iter, double* dev, double* mn) {
+    const Geom& g = c->g;
+    const int ty = 16;
+    const size_t sh = sizeof(cx<R>) * ty * (g.nx + 1);
+    const dim3 grid((g.ny + ty - 1) / ty, g.nzh);
+    if (ty * g.nx <= 8 * 256)
+      hipLaunchKernelGGL((k_xprop<R, R, 8>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
+                         SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty, lam.as<double>(),
+                         fac.as<double>(), st_dt, first_iter ? 1 : 0);
+    else if (ty * g.nx <= 16 * 256)
+      hipLaunchKernelGGL((k_xprop<R, R, 16>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
+                         SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty, lam.as<double>(),
+                         fac.as<double>(), st_dt, first_iter ? 1 : 0);
+    else
+      throw Unsupported("x extent too large for the propagator kernel");
+    launch_lines<R, R>(Sw.as<cx<R>>(), static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), +1, c->stream);
+    launch_zc2r<R, R>(Sw.as<cx<R>>(), 1, raw.as<double>(), nullptr, g.inv_n, g, c->plan[2], c->tw<R>(2), nullptr,
+                      DcTable{}, c->stream);
+    const AliasMap id{0, 0.0, 0.0};
+    launch_sum(raw.as<double>(), id, part.as<double>(), g, c->stream);
+    launch_finalize(part.as<double>(), sum_blocks(g), sums.as<double>() + 24, c->stream);
+    hipLaunchKernelGGL(k_ddft_update, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, raw.as<double>(),
+                       d1.as<double>(), g.N, sums.as<double>(), st_dt, first_iter ? 1 : 0, part.as<double>());
+    hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
+                       static_cast<int>(vec_blocks(g.N)), mm.as<double>());
+    CDFT_HIP(hipGetLastError());
+    double r[2];
+    CDFT_HIP(hipMemcpyAsync(r, mm.p, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    *dev = r[0];
+    *mn = r[1];
+  }
+
+  // integrating_factor_step (dynamics.hpp:333-401); rho0 holds the densities and f0 their forces.
+  template <typename R>
+  double step(double* cfg_dt, double tol, int max_it) {
+    const long N = c->g.N;
+    rhs_forward<R>(rho0.as<double>(), f0.as<double>(), SR0.as<cx<R>>(), 1, true);
+    forward<R>(rho0.as<double>(), Srho0.as<cx<R>>(), 0, true);
+    bool restart;
+    do {
+      restart = false;
+      update_timestep(*cfg_dt);
+      CDFT_HIP(hipMemcpyAsync(d1.p, rho0.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      double deviation = 1e30;
+      for (int it = 0; it < max_it; ++it) {
+        const bool first = it == 0;  // d1 == rho0: compute({d1}) equals the step's first compute
+        if (!first) {
+          compute(d1.as<double>(), f1.as<double>());
+          rhs_forward<R>(d1.as<double>(), f1.as<double>(), SR1.as<cx<R>>(), 2, false);
+        }
+        const double old = deviation;
+        double mn = 0.0;
+        propagate<R>(first, &deviation, &mn);
+        if (mn < 0.0 || (it > 0 && deviation > old)) {
+          restart = true;
+          break;
+        }
+        if (deviation < tol) break;
+      }
+      if (!restart && deviation > tol) restart = true;
+      if (restart) *cfg_dt /= 10.0;
+    } while (restart);
+    const double e = compute(d1.as<double>(), f1.as<double>());
+    CDFT_HIP(hipMemcpyAsync(rho0.p, d1.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    CDFT_HIP(hipMemcpyAsync(f0.p, f1.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    return e;
+  }
+};
+
+
+}  // namespace
+
+}  // namespace cdft

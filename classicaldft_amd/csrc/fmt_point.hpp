@@ -15,11 +15,40 @@
 //   g1 = dPhi/dn2 + dPhi/dn1 / (4 pi R) + dPhi/dn0 / (4 pi R^2)
 //   g2..4 = dPhi/dv1_a + dPhi/dv0_a / (4 pi R)
 //   g5..10 = sym_ij dPhi/dT_ij  (xx, xy, xz, yy, yz, zz; sym = 2 off-diagonal)
+//
+// Both functions are templates on the scalar type: double for the evaluation,
+// Dual (value + directional derivative) for the Hessian-vector product of the
+// adjoint, where the derivative parts of g are sum_b d2Phi/dn_a dn_b * dn_b.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace cdft {
+
+// Forward-mode dual number: v + d*eps with eps^2 = 0.
+struct Dual {
+  double v, d;
+  Dual() = default;
+  __host__ __device__ constexpr Dual(double x) : v(x), d(0.0) {}
+  __host__ __device__ constexpr Dual(double x, double y) : v(x), d(y) {}
+};
+__host__ __device__ inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+__host__ __device__ inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
+__host__ __device__ inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
+__host__ __device__ inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.v * b.d + a.d * b.v}; }
+__host__ __device__ inline Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return {q, (a.d - q * b.d) / b.v};
+}
+__host__ __device__ inline Dual& operator+=(Dual& a, Dual b) { return a = a + b; }
+
+// value part (the scalar itself for double): branches test values only
+__host__ __device__ inline double vof(double x) { return x; }
+__host__ __device__ inline double vof(Dual x) { return x.v; }
+__host__ __device__ inline double dlog(double x) { return log(x); }
+__host__ __device__ inline Dual dlog(Dual x) { return {log(x.v), x.d / x.v}; }
+__host__ __device__ inline double dmin(double x, double c) { return fmin(x, c); }
+__host__ __device__ inline Dual dmin(Dual x, double c) { return x.v < c ? x : Dual(c); }
 
 enum FmtModelId : int {
   FMT_NONE = -1,
@@ -40,18 +69,19 @@ struct FmtParams {
 // are formed from the two reciprocals 1/(1-e) and 1/e, each taken from one
 // FP64 division: a division costs about ten FP64 instructions on the device,
 // a product one.
-__host__ __device__ inline void f_funcs(int model, double e, double* f, double* df) {
-  const double om = 1.0 - e;
-  const double L = log(om);
-  const double r = 1.0 / (e * om);
-  const double io = e * r, ie = om * r;  // 1/(1-e), 1/e
-  const double io2 = io * io, ie2 = ie * ie;
+template <typename T>
+__host__ __device__ inline void f_funcs(int model, T e, T* f, T* df) {
+  const T om = 1.0 - e;
+  const T L = dlog(om);
+  const T r = 1.0 / (e * om);
+  const T io = e * r, ie = om * r;  // 1/(1-e), 1/e
+  const T io2 = io * io, ie2 = ie * ie;
   // f1 (models.hpp:157-160): identical for every model
   f[0] = L;
   df[0] = -io;
   // f2
   if (model == FMT_WHITEBEAR_II) {
-    if (e < 1e-6) {
+    if (vof(e) < 1e-6) {
       f[1] = 1.0 + e * (1.0 + e * (10.0 / 9.0 + 7.0 / 6.0 * e));
       df[1] = 1.0 + e * (20.0 / 9.0 + 3.5 * e);
     } else {
@@ -64,7 +94,7 @@ __host__ __device__ inline void f_funcs(int model, double e, double* f, double* 
   }
   // f3
   if (model == FMT_RSLT || model == FMT_WHITEBEAR_I) {
-    if (e < 1e-6) {
+    if (vof(e) < 1e-6) {
       if (model == FMT_RSLT) {
         f[2] = 1.0 + (2.0 / 3.0) * e + 0.5 * e * e;
         df[2] = 2.0 / 3.0 + e;
@@ -77,14 +107,14 @@ __host__ __device__ inline void f_funcs(int model, double e, double* f, double* 
       df[2] = -ie2 * io2 + 2.0 * ie * io2 * io - io * ie2 - 2.0 * L * ie2 * ie;
     }
   } else if (model == FMT_WHITEBEAR_II) {
-    if (e < 1e-6) {
+    if (vof(e) < 1e-6) {
       f[2] = 1.5 + 7.0 / 3.0 * e + 3.25 * e * e + 4.2 * e * e * e;
       df[2] = 7.0 / 3.0 + 6.5 * e + 12.6 * e * e;
     } else {
       // -g/h - L/e^2 with g = 1 - 3e + e^2, h = e (1-e)^2
-      const double g = 1.0 - 3.0 * e + e * e, gp = -3.0 + 2.0 * e;
-      const double h = e * om * om, hp = om * (1.0 - 3.0 * e);
-      const double ih = ie * io2;
+      const T g = 1.0 - 3.0 * e + e * e, gp = -3.0 + 2.0 * e;
+      const T h = e * om * om, hp = om * (1.0 - 3.0 * e);
+      const T ih = ie * io2;
       f[2] = -g * ih - L * ie2;
       df[2] = -(gp * h - g * hp) * ih * ih + io * ie2 + 2.0 * L * ie2 * ie;
     }
@@ -95,29 +125,30 @@ __host__ __device__ inline void f_funcs(int model, double e, double* f, double* 
 }
 
 // w[0..10] = eta, n2, v[3], T[xx,xy,xz,yy,yz,zz]; returns Phi, fills g[0..10].
-__host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double* w, double* g) {
+template <typename T>
+__host__ __device__ inline T fmt_phi_grad_t(const FmtParams& P, const T* w, T* g) {
   const double PI = 3.14159265358979323846;
-  const double eta = w[0];
-  if (!(eta >= 1e-30)) {  // hard_sphere.hpp:199-200 (skip, derivatives stay zero)
+  const T eta = w[0];
+  if (!(vof(eta) >= 1e-30)) {  // hard_sphere.hpp:199-200 (skip, derivatives stay zero)
 #pragma unroll
-    for (int i = 0; i < 11; ++i) g[i] = 0.0;
-    return 0.0;
+    for (int i = 0; i < 11; ++i) g[i] = T(0.0);
+    return T(0.0);
   }
   const double R = P.R;
   // 1/(4 pi R) and 1/(4 pi R^2): the same for every point (hoisted out of loops)
   const double ifr = 1.0 / (4.0 * PI * R), ifr2 = ifr / R;
-  const double n2 = w[1];
-  const double n1 = n2 * ifr;
-  const double n0 = n2 * ifr2;
-  const double v[3] = {w[2], w[3], w[4]};
-  const double a = w[5], b = w[6], c = w[7], d = w[8], e = w[9], f = w[10];
-  const double vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
-  const double v0v1 = vv * ifr;
+  const T n2 = w[1];
+  const T n1 = n2 * ifr;
+  const T n0 = n2 * ifr2;
+  const T v[3] = {w[2], w[3], w[4]};
+  const T a = w[5], b = w[6], c = w[7], d = w[8], e = w[9], f = w[10];
+  const T vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const T v0v1 = vv * ifr;
 
-  double F[3], dF[3];
-  f_funcs(P.model, eta, F, dF);
+  T F[3], dF[3];
+  f_funcs<T>(P.model, eta, F, dF);
 
-  double p3 = 0.0, dp3_n2 = 0.0, dp3_v[3] = {0.0, 0.0, 0.0}, dT[6] = {0, 0, 0, 0, 0, 0};
+  T p3 = 0.0, dp3_n2 = 0.0, dp3_v[3] = {0.0, 0.0, 0.0}, dT[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   bool tensor = false;
   if (P.model == FMT_ROSENFELD) {
     const double k24 = 1.0 / (24.0 * PI);
@@ -125,9 +156,9 @@ __host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double*
     dp3_n2 = (1.0 / (8.0 * PI)) * (n2 * n2 - vv);
     for (int k = 0; k < 3; ++k) dp3_v[k] = -(1.0 / (4.0 * PI)) * n2 * v[k];
   } else if (P.model == FMT_RSLT) {
-    if (n2 >= 1e-30) {
-      const double xi = fmin(vv / (n2 * n2), 1.0);
-      const double q = 1.0 - xi;
+    if (vof(n2) >= 1e-30) {
+      const T xi = dmin(vv / (n2 * n2), 1.0);
+      const T q = 1.0 - xi;
       const double k36 = 1.0 / (36.0 * PI);
       p3 = k36 * n2 * n2 * n2 * q * q * q;
       dp3_n2 = k36 * 3.0 * n2 * n2 * q * q * (1.0 + xi);
@@ -137,21 +168,21 @@ __host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double*
     // tensor models: esFMT(A,B); WhiteBear I/II are esFMT(1,-1) (models.hpp:18-45,326-357)
     tensor = true;
     const double A = P.A, B = P.B;
-    const double sxx = a * a + b * b + c * c, sxy = a * b + b * d + c * e, sxz = a * c + b * e + c * f;
-    const double syy = b * b + d * d + e * e, syz = b * c + d * e + e * f, szz = c * c + e * e + f * f;
-    const double trT2 = sxx + syy + szz;
-    const double trT3 = a * sxx + d * syy + f * szz + 2.0 * (b * sxy + c * sxz + e * syz);
-    const double Tv[3] = {a * v[0] + b * v[1] + c * v[2], b * v[0] + d * v[1] + e * v[2],
-                          c * v[0] + e * v[1] + f * v[2]};
-    const double qf = v[0] * Tv[0] + v[1] * Tv[1] + v[2] * Tv[2];
+    const T sxx = a * a + b * b + c * c, sxy = a * b + b * d + c * e, sxz = a * c + b * e + c * f;
+    const T syy = b * b + d * d + e * e, syz = b * c + d * e + e * f, szz = c * c + e * e + f * f;
+    const T trT2 = sxx + syy + szz;
+    const T trT3 = a * sxx + d * syy + f * szz + 2.0 * (b * sxy + c * sxz + e * syz);
+    const T Tv[3] = {a * v[0] + b * v[1] + c * v[2], b * v[0] + d * v[1] + e * v[2],
+                     c * v[0] + e * v[1] + f * v[2]};
+    const T qf = v[0] * Tv[0] + v[1] * Tv[1] + v[2] * Tv[2];
     const double k24 = 1.0 / (24.0 * PI);
-    const double n23 = n2 * n2 * n2;
+    const T n23 = n2 * n2 * n2;
     p3 = k24 * (A * (n23 - 3.0 * n2 * vv + 3.0 * qf - trT3) + B * (n23 - 3.0 * n2 * trT2 + 2.0 * trT3));
     dp3_n2 = k24 * (A * (3.0 * n2 * n2 - 3.0 * vv) + B * (3.0 * n2 * n2 - 3.0 * trT2));
     for (int k = 0; k < 3; ++k) dp3_v[k] = k24 * A * (-6.0 * n2 * v[k] + 6.0 * Tv[k]);
     const double k8 = 1.0 / (8.0 * PI), k4 = 1.0 / (4.0 * PI);
-    const double Tm[6] = {a, b, c, d, e, f};
-    const double T2m[6] = {sxx, sxy, sxz, syy, syz, szz};
+    const T Tm[6] = {a, b, c, d, e, f};
+    const T T2m[6] = {sxx, sxy, sxz, syy, syz, szz};
     const int ii[6] = {0, 0, 0, 1, 1, 2};
     const int jj[6] = {0, 1, 2, 1, 2, 2};
 #pragma unroll
@@ -159,26 +190,30 @@ __host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double*
       dT[q] = k8 * A * (v[ii[q]] * v[jj[q]] - T2m[q]) + k4 * B * (-n2 * Tm[q] + T2m[q]);
   }
 
-  const double s = n1 * n2 - v0v1;
-  const double phi = -n0 * F[0] + s * F[1] + p3 * F[2];
-  const double d_eta = -n0 * dF[0] + s * dF[1] + p3 * dF[2];
-  const double d_n0 = -F[0];
-  const double d_n1 = n2 * F[1];
-  const double d_n2 = n1 * F[1] + dp3_n2 * F[2];
+  const T s = n1 * n2 - v0v1;
+  const T phi = -n0 * F[0] + s * F[1] + p3 * F[2];
+  const T d_eta = -n0 * dF[0] + s * dF[1] + p3 * dF[2];
+  const T d_n0 = -F[0];
+  const T d_n1 = n2 * F[1];
+  const T d_n2 = n1 * F[1] + dp3_n2 * F[2];
   g[0] = d_eta;
   g[1] = d_n2 + d_n1 * ifr + d_n0 * ifr2;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const double dv0 = -v[k] * F[1];                            // dPhi/dv0 = -v1 f2
-    const double dv1 = -(v[k] * ifr) * F[1] + dp3_v[k] * F[2];  // dPhi/dv1 = -v0 f2 + dPhi3/dv1 f3
+    const T dv0 = -v[k] * F[1];                            // dPhi/dv0 = -v1 f2
+    const T dv1 = -(v[k] * ifr) * F[1] + dp3_v[k] * F[2];  // dPhi/dv1 = -v0 f2 + dPhi3/dv1 f3
     g[2 + k] = dv1 + dv0 * ifr;
   }
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     const double sym = (q == 0 || q == 3 || q == 5) ? 1.0 : 2.0;
-    g[5 + q] = tensor ? sym * dT[q] * F[2] : 0.0;
+    g[5 + q] = tensor ? sym * dT[q] * F[2] : T(0.0);
   }
   return phi;
+}
+
+__host__ __device__ inline double fmt_phi_grad(const FmtParams& P, const double* w, double* g) {
+  return fmt_phi_grad_t<double>(P, w, g);
 }
 
 }  // namespace cdft

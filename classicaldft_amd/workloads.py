@@ -103,3 +103,27 @@ def batch_state_points(n, seed=5, sizes=BATCH_SIZES):
                         seed=seed + 17 * i)
         pts.append(dict(shape=shape, dx=dx, kT=kT, rho_bulk=rb, rho0=prof))
     return pts
+
+
+def wells(shape, dx, n_wells=16, seed=11, amp=(-2.0, 0.0), width=(0.5, 1.5)):
+    """Seeded Gaussian wells (n_wells, 5): amplitude A ~ U(amp) kT, width w ~ U(width) sigma, centre uniform in
+    the periodic box.  V(r) = sum_j A_j exp(-|r - c_j|^2 / (2 w_j^2)) with minimum-image distances."""
+    u = uniform(seed, 5 * n_wells).reshape(n_wells, 5)
+    L = np.array(shape, dtype=np.float64) * dx
+    prm = np.empty((n_wells, 5))
+    prm[:, 0] = amp[0] + (amp[1] - amp[0]) * u[:, 0]
+    prm[:, 1] = width[0] + (width[1] - width[0]) * u[:, 1]
+    prm[:, 2:5] = u[:, 2:5] * L
+    return prm
+
+
+def adjoint_problem(shape=(128, 128, 128), dx=0.1, n_wells=16, rho_bulk=0.3, seed=11, perturb=0.1):
+    """The DDFT adjoint workload: a fluid at rho_bulk with 1e-3 uniform noise, the seeded wells, and wells whose
+    amplitudes are scaled by (1 + perturb u), u ~ U(-1, 1), to generate the target profile.  Returns
+    (rho0, params, target_params)."""
+    n = int(np.prod(shape))
+    rho0 = rho_bulk * (1.0 + 1e-3 * (2.0 * uniform(seed + 1, n).reshape(shape) - 1.0))
+    prm = wells(shape, dx, n_wells, seed)
+    tgt = prm.copy()
+    tgt[:, 0] *= 1.0 + perturb * (2.0 * uniform(seed + 2, n_wells) - 1.0)
+    return rho0, prm, tgt

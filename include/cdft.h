@@ -223,6 +223,63 @@ typedef struct {
 CDFT_API int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
                   double* rho, cdft_sim_result* result, double* energies, double* times);
 
+/* ---------------------------------------------------------------- adjoint
+ * The reference has no derivative of its dynamics; the entry points below
+ * extend its DDFT integrating-factor step (dynamics.hpp:333-401) with a
+ * reverse-mode adjoint.  They require a context of CDFT_PRECISION_FP64.
+ *
+ * cdft_hessian_vector: out = H(rho) u with H = d(force)/d(rho), the Jacobian
+ * of the forces of cdft_total (delta(beta Omega)/delta rho * dV): dV u / rho
+ * (ideal), dV W_mf u (mean field) and dV sum_ab W_a^T Phi_ab W_b u (FMT, the
+ * second derivatives of Phi by forward-mode dual numbers).  H is symmetric.
+ * Host arrays of N values. */
+CDFT_API int cdft_hessian_vector(cdft_ctx* ctx, const double* rho, const double* u, double* out);
+
+/* External potential of n_wells Gaussian wells (kT units):
+ *   V(r) = sum_j A_j exp(-|r - c_j|^2 / (2 w_j^2)),
+ * r = (ix, iy, iz) dx, minimum-image distances in the periodic box.
+ * params holds 5 values per well: A, w, cx, cy, cz.  vext: N values (host). */
+CDFT_API int cdft_wells_potential(cdft_ctx* ctx, int n_wells, const double* params, double* vext);
+
+/* The map the adjoint differentiates: n_steps integrating-factor steps with
+ * a fixed dt and exactly fp_iterations fixed-point iterations per step (the
+ * reference's tolerance exit, restarts and dt adaptation are branches and
+ * have no derivative). */
+typedef struct {
+  double dt;
+  int fp_iterations;        /* >= 1 */
+  int n_steps;              /* K >= 1 */
+  int checkpoint_interval;  /* steps between stored densities; <= 0: ceil(sqrt(K)) */
+} cdft_adjoint_config;
+
+typedef struct {
+  double loss;             /* sum (rho_K - target)^2 dV */
+  double min_density;      /* smallest density among all iterates of the forward map */
+  long evaluations;        /* functional evaluations (forward pass and segment recomputation) */
+  long hessian_products;   /* Hessian-vector products of the backward pass */
+  int checkpoints;         /* densities stored by the forward pass */
+  double store_bytes;      /* device bytes of the checkpoints and of one segment's step states */
+} cdft_adjoint_result;
+
+/* Forward map only: rho (host, N values) is advanced in place.  vext may be
+ * null.  min_density (nullable) receives the smallest density of any iterate. */
+CDFT_API int cdft_ddft_fixed_run(cdft_ctx* ctx, const cdft_adjoint_config* config, const double* vext, double mu,
+                        double* rho, double* min_density);
+
+/* Loss L = sum_r (rho_K(r) - target(r))^2 dV after the forward map from rho0
+ * under the well potential, and its gradient by the adjoint, run backwards
+ * through the K steps with checkpointing (densities every
+ * checkpoint_interval steps; each segment is recomputed once and its step
+ * states kept on the device).
+ *   grad:      5 * n_wells values, dL/d(A, w, cx, cy, cz) of every well;
+ *   grad_rho0: N values or null, dL/d(rho0);
+ *   grad_vext: N values or null, dL/dV(r) for the potential as a free field;
+ *   rho_final: N values or null, rho_K. */
+CDFT_API int cdft_ddft_adjoint_wells(cdft_ctx* ctx, const cdft_adjoint_config* config, int n_wells,
+                            const double* params, double mu, const double* rho0, const double* target,
+                            cdft_adjoint_result* result, double* grad, double* grad_rho0, double* grad_vext,
+                            double* rho_final);
+
 #ifdef __cplusplus
 }
 #endif

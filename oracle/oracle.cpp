@@ -21,6 +21,7 @@
 //   FIRE2 .............................. include/dft/algorithms/fire.hpp:60-178
 //   Minimizer::grand_potential ......... include/dft/algorithms/minimization.hpp:45-110,280-353
 //   DDFT integrating-factor step ....... include/dft/algorithms/dynamics.hpp:79-401,582-693
+//   fixed-dt DDFT map + Gaussian wells . (no counterpart: the map the adjoint differentiates)
 //
 // Where the reference's output depends on FFTW behaviour it does not document
 // (c2r of a half spectrum that is not Hermitian, which happens for the odd
@@ -1234,6 +1235,25 @@ struct DdftWork {
     *dt_used = cfg[0];
     return e;
   }
+  // The map the adjoint differentiates: the step above with a fixed dt,
+  // exactly P fixed-point iterations, no tolerance exit and no restart.
+  void fixed_step(std::vector<double>& rho, double dt, int P, double* mn) {
+    const long N = c->N, M = c->M;
+    std::vector<double> f(N);
+    compute(rho, f);
+    std::vector<cd> rhs0(M), rho0k(M), rhs1(M);
+    excess_rhs_fourier(rho, f, rhs0.data());
+    c->fft->r2c(rho.data(), rho0k.data());
+    update_timestep(dt);
+    std::vector<double> d1 = rho;
+    for (int it = 0; it < P; ++it) {
+      compute(d1, f);
+      excess_rhs_fourier(d1, f, rhs1.data());
+      apply_propagator(rho0k.data(), rhs0.data(), rhs1.data(), d1);
+      *mn = std::min(*mn, *std::min_element(d1.begin(), d1.end()));
+    }
+    rho = d1;
+  }
 };
 }  // namespace
 
@@ -1275,6 +1295,43 @@ int orc_ddft_run(void* h, const double* cfg_in, int n_steps, const double* vext,
   out[4] = e;
   out[5] = static_cast<double>(w.evals);
   return 0;
+}
+
+// Fixed-dt forward map of the DDFT adjoint: n_steps steps of fixed_step.
+// Returns the smallest density of any iterate.
+double orc_ddft_fixed_run(void* h, double dt, int fp_iterations, int n_steps, const double* vext, double mu,
+                          double* rho_inout) {
+  auto* c = static_cast<Ctx*>(h);
+  DdftWork w(c, vext, mu);
+  std::vector<double> rho(rho_inout, rho_inout + c->N);
+  double mn = 1e300;
+  for (int s = 0; s < n_steps; ++s) w.fixed_step(rho, dt, fp_iterations, &mn);
+  std::memcpy(rho_inout, rho.data(), sizeof(double) * c->N);
+  return mn;
+}
+
+// Sum of Gaussian wells V(r) = sum_j A_j exp(-|r - c_j|^2 / (2 w_j^2)) at
+// r = (ix, iy, iz) dx with minimum-image distances; params: (A, w, cx, cy, cz) per well.
+void orc_wells_potential(void* h, int n_wells, const double* params, double* out) {
+  auto* c = static_cast<Ctx*>(h);
+  const double L[3] = {c->nx * c->dx, c->ny * c->dx, c->nz * c->dx};
+  for (long ix = 0; ix < c->nx; ++ix)
+    for (long iy = 0; iy < c->ny; ++iy)
+      for (long iz = 0; iz < c->nz; ++iz) {
+        const double r[3] = {ix * c->dx, iy * c->dx, iz * c->dx};
+        double v = 0.0;
+        for (int j = 0; j < n_wells; ++j) {
+          const double* q = params + 5 * j;
+          double d2 = 0.0;
+          for (int a = 0; a < 3; ++a) {
+            double e = r[a] - q[2 + a];
+            e -= L[a] * std::nearbyint(e / L[a]);
+            d2 += e * e;
+          }
+          v += q[0] * std::exp(-d2 / (2.0 * q[1] * q[1]));
+        }
+        out[iz + c->nz * (iy + c->ny * ix)] = v;
+      }
 }
 
 }  // extern "C"

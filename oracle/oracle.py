@@ -67,6 +67,9 @@ def lib():
                                       C.c_double, _dp, _dp, _dp]
         L.orc_ddft_run.restype = C.c_int
         L.orc_ddft_run.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp, _dp, _dp]
+        L.orc_ddft_fixed_run.restype = C.c_double
+        L.orc_ddft_fixed_run.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, _dp, C.c_double, _dp]
+        L.orc_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         _lib = L
     return _lib
 
@@ -227,3 +230,17 @@ class Oracle:
         res = dict(mass_initial=out[0], mass_final=out[1], dt=out[2], time=out[3], energy=out[4],
                    evals=int(out[5]))
         return rho.reshape(self.shape), res, energies
+
+    def ddft_fixed_run(self, rho0, mu, n_steps, dt, fp_iterations, vext=None):
+        """Fixed-dt DDFT map (exactly fp_iterations iterations per step); returns (rho, min density)."""
+        rho = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1).copy()
+        v = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
+        mn = lib().orc_ddft_fixed_run(self._h, float(dt), int(fp_iterations), int(n_steps), _p(v), float(mu),
+                                      _p(rho))
+        return rho.reshape(self.shape), mn
+
+    def wells_potential(self, params):
+        prm = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 5)
+        out = np.zeros(self.N)
+        lib().orc_wells_potential(self._h, prm.shape[0], _p(prm.reshape(-1)), _p(out))
+        return out.reshape(self.shape)
