@@ -128,6 +128,60 @@ def batched_fire(cd, W, precision, n=16, steps=40, sizes=(48, 64, 80, 96)):
     }
 
 
+def adjoint_ddft(cd, W, shape=(128, 128, 128), K=400, P=2, dt=1e-4, n_wells=16, fd_all=False):
+    """The DDFT adjoint workload (FP64): hard-sphere (White-Bear II) + LJ fluid at kT = 0.9 on `shape`, dx = 0.1,
+    n_wells seeded Gaussian wells, loss = sum (rho_K - target)^2 dV after K fixed-dt steps (P fixed-point
+    iterations each), target from the same run with perturbed amplitudes; the gradient by the checkpointed
+    adjoint, checked against central differences (h = 1e-5) of the forward map for the amplitude, width and
+    z-centre of well 0 (of every well with fd_all)."""
+    import numpy as np
+    import torch
+
+    dx = 0.1
+    ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=0.9, model="WhiteBearII", precision="fp64")
+    rho0, prm, tprm = W.adjoint_problem(shape, dx, n_wells=n_wells, rho_bulk=0.3)
+    mu = ctx.bulk_mu(0.3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    target, _ = ctx.ddft_fixed_run(rho0, mu, K, dt, P, vext=ctx.wells_potential(tprm))
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    grad, res = ctx.ddft_adjoint_wells(rho0, target, prm, mu, K, dt, P)
+    t_adj = time.perf_counter() - t0
+
+    def loss(p):
+        rho, _ = ctx.ddft_fixed_run(rho0, mu, K, dt, P, vext=ctx.wells_potential(p))
+        return float(np.sum((rho - target) ** 2) * dx ** 3)
+
+    h = 1e-5
+    checks = [(j, k) for j in (range(n_wells) if fd_all else [0]) for k in (0, 1, 4)]
+    errs = []
+    for j, k in checks:
+        pp, pm = prm.copy(), prm.copy()
+        pp[j, k] += h
+        pm[j, k] -= h
+        fd = (loss(pp) - loss(pm)) / (2 * h)
+        errs.append(abs(grad[j, k] - fd) / max(abs(fd), 1e-300))
+    return {
+        "grid": list(shape),
+        "steps": K,
+        "fp_iterations": P,
+        "dt": dt,
+        "wells": n_wells,
+        "precision": "fp64",
+        "loss": res["loss"],
+        "forward_s": t_fwd,
+        "gradient_s": t_adj,
+        "gradient_over_forward": t_adj / t_fwd,
+        "evaluations": res["evaluations"],
+        "hessian_products": res["hessian_products"],
+        "checkpoints": res["checkpoints"],
+        "store_gb": res["store_bytes"] / 1e9,
+        "fd_checked": len(checks),
+        "fd_max_rel_err": max(errs),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +192,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the same-GPU FP64 comparison run")
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential FIRE measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the DDFT adjoint measurement")
+    ap.add_argument("--adjoint-fd-all", action="store_true",
+                    help="check the adjoint gradient against differences for (A, w, cz) of every well")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -248,6 +305,9 @@ def main():
 
     if not args.no_batch:
         out["batched"] = batched_fire(cd, W, args.precision)
+
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_ddft(cd, W, fd_all=args.adjoint_fd_all)
 
     if not args.no_cpu_baseline:
         cb, parts, fr = cpu_baseline(rho, mu, shape, dx)
