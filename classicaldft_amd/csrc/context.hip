@@ -405,16 +405,18 @@ void Context::stage_bytes(double* b, bool with_vext) const {
   b[10] = 0.0;
 }
 
+void Context::eval_direct(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
+  if (fast) {
+    if (precision == CDFT_PRECISION_MIXED) eval_fast_t<float, float>(rho, vext, mu, force, amap);
+    else eval_fast_t<double, double>(rho, vext, mu, force, amap);
+  } else {
+    if (precision == CDFT_PRECISION_MIXED) eval_t<float, double, float>(rho, vext, mu, force, amap);
+    else eval_t<double, double, double>(rho, vext, mu, force, amap);
+  }
+}
+
 void Context::eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap) {
-  auto direct = [&] {
-    if (fast) {
-      if (precision == CDFT_PRECISION_MIXED) eval_fast_t<float, float>(rho, vext, mu, force, amap);
-      else eval_fast_t<double, double>(rho, vext, mu, force, amap);
-    } else {
-      if (precision == CDFT_PRECISION_MIXED) eval_t<float, double, float>(rho, vext, mu, force, amap);
-      else eval_t<double, double, double>(rho, vext, mu, force, amap);
-    }
-  };
+  auto direct = [&] { eval_direct(rho, vext, mu, force, amap); };
   if (!use_graphs || stage_events) {
     direct();
     CDFT_HIP(hipGetLastError());

@@ -127,6 +127,9 @@ struct Context {
   // mapped through `amap`).  Results land in d_res (device).  When `force`
   // is null only the energy is computed.
   void eval(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
+  // The same launches issued directly on `stream` (no cached graph), for
+  // callers that capture a larger graph around the evaluation.
+  void eval_direct(const double* rho, const double* vext, double mu, double* force, AliasMap amap);
   // S: spectral storage type; CE: arithmetic of the energy path's inverse
   // transforms; CL: arithmetic of the other transforms.
   template <typename S, typename CE, typename CL>

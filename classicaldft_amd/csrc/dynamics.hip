@@ -114,10 +114,100 @@ __global__ void __launch_bounds__(256) k_fire_force(const double* __restrict__ g
   block_sums(vals, 2, partials);
 }
 
+// FIRE2 state (fire.hpp:46-58) and the convergence monitor
+// (minimization.hpp:105-110), kept on the device: whole steps replay from a
+// graph with no host decision between them.
+struct FireCtl {
+  double energy, old_energy, dt, dt_old, alpha, vv;
+  int n_pos, n_neg, iteration, backtrack;
+  int skip;   // this step changes nothing: the run has ended
+  int state;  // FireState
+};
+enum FireState : int { FS_RUN = 0, FS_CONVERGED = 1, FS_MAX_STEPS = 2, FS_UPHILL = 3 };
+
+struct FireConst {
+  cdft_fire p;
+  double mu, volume;
+};
+
+// Omega from the evaluation sums, in the operation order of Context::result_from
+__device__ __forceinline__ double fire_omega(const double* res, double mu) {
+#pragma clang fp contract(off)
+  const double F = res[RS_IDEAL] + res[RS_EXT] + res[RS_HS] + res[RS_MF];
+  return F - mu * res[RS_MASS];
+}
+
+// Fire::initialize (fire.hpp:60-84) after the first compute
+__global__ void k_fire_init(FireCtl* ctl, const double* __restrict__ res, FireConst k, double* energies) {
+  FireCtl c{};
+  c.energy = fire_omega(res, k.mu);
+  c.dt = k.p.dt;
+  c.alpha = k.p.alpha_start;
+  c.vv = 1.0;
+  c.skip = 1;
+  c.state = k.p.max_steps > 0 ? FS_RUN : FS_MAX_STEPS;
+  *ctl = c;
+  if (energies) energies[0] = c.energy;
+}
+
+// Fire::step (fire.hpp:86-108): the branch on the power v.f of the current forces (fres[1])
+__global__ void k_fire_begin(FireCtl* ctl, const double* __restrict__ fres, FireConst k) {
+#pragma clang fp contract(off)
+  FireCtl c = *ctl;
+  if (c.state != FS_RUN) {
+    ctl->skip = 1;
+    return;
+  }
+  c.old_energy = c.energy;
+  c.backtrack = 0;
+  c.dt_old = c.dt;
+  if (fres[1] > 0.0 || c.iteration == 0) {
+    ++c.n_pos;
+    c.n_neg = 0;
+    if (c.n_pos > k.p.n_delay) {
+      c.dt = fmin(c.dt * k.p.f_inc, k.p.dt_max);
+      c.alpha *= k.p.f_alpha;
+    }
+  } else {
+    c.n_pos = 0;
+    ++c.n_neg;
+    if (c.n_neg > k.p.max_uphill) {  // the reference throws here
+      c.state = FS_UPHILL;
+      c.skip = 1;
+      *ctl = c;
+      return;
+    }
+    c.backtrack = 1;
+    if (c.iteration > k.p.n_delay) {
+      c.dt = fmax(c.dt * k.p.f_dec, k.p.dt_min);
+      c.alpha = k.p.alpha_start;
+    }
+  }
+  c.skip = 0;
+  *ctl = c;
+}
+
+// After the step's compute: energy, iteration count, convergence_monitor (minimization.hpp:105-110,329-332)
+__global__ void k_fire_end(FireCtl* ctl, const double* __restrict__ res, FireConst k, double* energies) {
+#pragma clang fp contract(off)
+  FireCtl c = *ctl;
+  if (c.skip) return;
+  c.energy = fire_omega(res, k.mu);
+  ++c.iteration;
+  c.vv = c.vv * 0.9 + 0.1 * (fabs(c.energy - c.old_energy) / c.dt);
+  if (fabs(c.vv / k.volume) < k.p.force_tolerance) c.state = FS_CONVERGED;
+  else if (c.iteration >= k.p.max_steps) c.state = FS_MAX_STEPS;
+  if (energies) energies[c.iteration] = c.energy;
+  *ctl = c;
+}
+
 // fire.hpp:110-126: optional backtrack (x -= dt_old v / 2, v = 0), then v += dt f; accumulates |v|^2
 __global__ void __launch_bounds__(256) k_fire_velocity(double* __restrict__ x, double* __restrict__ v,
-                                                       const double* __restrict__ f, long n, double dt_old,
-                                                       double dt, int backtrack, double* partials) {
+                                                       const double* __restrict__ f, long n,
+                                                       const FireCtl* __restrict__ ctl, double* partials) {
+  if (ctl->skip) return;
+  const double dt_old = ctl->dt_old, dt = ctl->dt;
+  const bool backtrack = ctl->backtrack != 0;
   double s = 0, c = 0;
   for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * blockDim.x) {
@@ -136,8 +226,11 @@ __global__ void __launch_bounds__(256) k_fire_velocity(double* __restrict__ x, d
 
 // fire.hpp:128-148: v = (1 - alpha) v + alpha |v|/|f| f (when |f| > 0), x += dt v
 __global__ void __launch_bounds__(256) k_fire_mix(double* __restrict__ x, double* __restrict__ v,
-                                                  const double* __restrict__ f, long n, double dt, double alpha,
-                                                  const double* __restrict__ vsum, const double* __restrict__ fsum) {
+                                                  const double* __restrict__ f, long n,
+                                                  const FireCtl* __restrict__ ctl, const double* __restrict__ vsum,
+                                                  const double* __restrict__ fsum) {
+  if (ctl->skip) return;
+  const double dt = ctl->dt, alpha = ctl->alpha;
   const double vn = sqrt(vsum[0]), fn = sqrt(fsum[0]);
   const bool mix = fn > 0.0;
   const double ratio = mix ? vn / fn : 0.0;
@@ -160,44 +253,49 @@ long face_count(const Geom& g) {
 
 // ------------------------------------------------------------------ Minimizer
 // Device FIRE2 in the alias variable (Minimizer::grand_potential,
-// minimization.hpp:280-353; fire::Fire, fire.hpp:60-165) as a state machine:
-// launch_*() queue a step's device work on the context's stream without
-// waiting, after_*() read its reductions once the stream has drained.  One
-// host thread can so advance several independent minimisations (contexts
-// with their own streams) in lockstep, the device working on all at once.
+// minimization.hpp:280-353; fire::Fire, fire.hpp:60-165).  The whole step,
+// control included, runs on the device and is captured once as a graph; the
+// host replays it in blocks of kFireBlock steps and reads the 64-byte
+// control state between blocks.  A step after the run has ended (converged,
+// step limit, uphill limit) changes nothing, so a block may overrun the end.
+// Several runs (contexts with their own streams) advance together by
+// launching every run's block before waiting for any.
+constexpr int kFireBlock = 8;
+
 struct FireRun {
   Context* c;
   AliasMap amap;
   int homog;
   const double* d_vext;
-  double mu;
-  cdft_fire fire;
-  DevBuf x, v, f, part, res;  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum
+  FireConst k;
+  DevBuf x, v, f, part, res, ctl, en;  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum
   long nb;
-  double* host = nullptr;     // pinned: [0..7] energy sums, [8..15] force sums
-  double* energies = nullptr;  // optional record: after initialisation and every step
-  // FIRE state (fire.hpp:46-58) and the convergence monitor (minimization.hpp:105-110)
-  double energy = 0.0, old_energy = 0.0, sf2 = 0.0, power = 0.0, dt = 0.0, alpha = 0.0, rms = 0.0, vv = 1.0;
-  int n_pos = 0, n_neg = 0, iteration = 0;
-  bool converged = false;
-  cdft_result last_result{};
+  FireCtl* host = nullptr;  // pinned copy of the control state
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
 
-  FireRun(Context* c_, AliasMap a, int h, const double* vx, double m, const cdft_fire& fr)
-      : c(c_), amap(a), homog(h), d_vext(vx), mu(m), fire(fr), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
-        part(8 * 8 * kVecBlocks), res(8 * 24) {
+  FireRun(Context* c_, AliasMap a, int h, const double* vx, double m, const cdft_fire& fr, bool record)
+      : c(c_), amap(a), homog(h), d_vext(vx), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
+        part(8 * 8 * kVecBlocks), res(8 * 24), ctl(sizeof(FireCtl)),
+        en(record ? 8 * (static_cast<size_t>(std::max(fr.max_steps, 0)) + 1) : 0) {
+    k.p = fr;
+    k.mu = m;
+    k.volume = static_cast<double>(c->g.N) * c->g.dv;
     nb = face_count(c->g);
-    CDFT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), 16 * sizeof(double), hipHostMallocDefault));
+    CDFT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(FireCtl), hipHostMallocDefault));
   }
   ~FireRun() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
     if (host) (void)hipHostFree(host);
   }
   FireRun(const FireRun&) = delete;
   FireRun& operator=(const FireRun&) = delete;
 
-  // compute(x): energy Omega and f in d_f; device sums f^2 and v.f in res[0], res[1]
+  // compute(x): Omega sums in d_res, forces f; device sums f^2 and v.f in res[0], res[1]
   void launch_compute() {
     const Geom& g = c->g;
-    c->eval(x.as<double>(), d_vext, mu, c->d_force, amap);
+    c->eval_direct(x.as<double>(), d_vext, k.mu, c->d_force, amap);
     if (homog) {
       hipLaunchKernelGGL(k_face_sum, dim3(vec_blocks(nb)), dim3(256), 0, c->stream, c->d_force, g, nb,
                          part.as<double>());
@@ -206,20 +304,22 @@ struct FireRun {
     hipLaunchKernelGGL(k_fire_force, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, c->d_force, x.as<double>(),
                        v.as<double>(), f.as<double>(), g, homog, res.as<double>() + 16, nb, amap, part.as<double>());
     launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(g.N)), res.as<double>(), c->stream);
-    CDFT_HIP(hipGetLastError());
-    CDFT_HIP(hipMemcpyAsync(host, c->d_res, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    CDFT_HIP(hipMemcpyAsync(host + 8, res.p, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   }
-  void read_compute() {
-    cdft_result r{};
-    c->result_from(host, mu, &r);
-    last_result = r;
-    energy = r.grand_potential;
-    sf2 = host[8];
-    power = host[9];
-    rms = std::sqrt(sf2 / static_cast<double>(c->g.N));
+
+  // one Fire::step (fire.hpp:86-165) with its compute and the monitor
+  void launch_step_kernels() {
+    const Geom& g = c->g;
+    const unsigned nbk = vec_blocks(g.N);
+    auto* cp = ctl.as<FireCtl>();
+    hipLaunchKernelGGL(k_fire_begin, dim3(1), dim3(1), 0, c->stream, cp, res.as<double>(), k);
+    hipLaunchKernelGGL(k_fire_velocity, dim3(nbk), dim3(256), 0, c->stream, x.as<double>(), v.as<double>(),
+                       f.as<double>(), g.N, cp, part.as<double>());
+    launch_finalize(part.as<double>(), static_cast<int>(nbk), res.as<double>() + 8, c->stream);
+    hipLaunchKernelGGL(k_fire_mix, dim3(nbk), dim3(256), 0, c->stream, x.as<double>(), v.as<double>(),
+                       f.as<double>(), g.N, cp, res.as<double>() + 8, res.as<double>());
+    launch_compute();
+    hipLaunchKernelGGL(k_fire_end, dim3(1), dim3(1), 0, c->stream, cp, c->d_res, k, en.as<double>());
   }
-  void sync() { CDFT_HIP(hipStreamSynchronize(c->stream)); }
 
   // Fire::initialize (fire.hpp:60-84) and the extra compute of minimization.hpp:312-313
   void launch_start(const double* rho0) {
@@ -230,73 +330,65 @@ struct FireRun {
                        x.as<double>(), g.N, amap);
     CDFT_HIP(hipMemsetAsync(v.p, 0, 8 * g.N, c->stream));
     launch_compute();
-  }
-  void after_start() {
-    read_compute();
-    dt = fire.dt;
-    alpha = fire.alpha_start;
-    converged = false;  // minimization.hpp:321: the loop starts unconverged
-    if (energies) energies[0] = energy;
-  }
-  bool active() const { return !converged && iteration < fire.max_steps; }
-
-  // Fire::step (fire.hpp:86-165); `power` is v.f of the current forces
-  void launch_step() {
-    const Geom& g = c->g;
-    old_energy = energy;
-    bool backtrack = false;
-    const double dt_old = dt;
-    if (power > 0.0 || iteration == 0) {
-      ++n_pos;
-      n_neg = 0;
-      if (n_pos > fire.n_delay) {
-        dt = std::min(dt * fire.f_inc, fire.dt_max);
-        alpha *= fire.f_alpha;
+    hipLaunchKernelGGL(k_fire_init, dim3(1), dim3(1), 0, c->stream, ctl.as<FireCtl>(), c->d_res, k,
+                       en.as<double>());
+    CDFT_HIP(hipGetLastError());
+    if (c->use_graphs) {
+      CDFT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      try {
+        launch_step_kernels();
+      } catch (...) {
+        hipGraph_t g0 = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g0);
+        if (g0) (void)hipGraphDestroy(g0);
+        throw;
       }
-    } else {
-      n_pos = 0;
-      ++n_neg;
-      if (n_neg > fire.max_uphill) throw std::runtime_error("fire::step: exceeded max uphill steps");
-      backtrack = true;
-      if (iteration > fire.n_delay) {
-        dt = std::max(dt * fire.f_dec, fire.dt_min);
-        alpha = fire.alpha_start;
-      }
+      CDFT_HIP(hipStreamEndCapture(c->stream, &graph));
+      CDFT_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     }
+    poll();
+  }
+
+  // queue up to kFireBlock steps (no more than the step limit leaves), then a read of the control state
+  void launch_block() {
     CDFT_HIP(hipSetDevice(c->device));
-    const unsigned nbk = vec_blocks(g.N);
-    hipLaunchKernelGGL(k_fire_velocity, dim3(nbk), dim3(256), 0, c->stream, x.as<double>(), v.as<double>(),
-                       f.as<double>(), g.N, dt_old, dt, backtrack ? 1 : 0, part.as<double>());
-    launch_finalize(part.as<double>(), static_cast<int>(nbk), res.as<double>() + 8, c->stream);
-    hipLaunchKernelGGL(k_fire_mix, dim3(nbk), dim3(256), 0, c->stream, x.as<double>(), v.as<double>(),
-                       f.as<double>(), g.N, dt, alpha, res.as<double>() + 8, res.as<double>());
-    launch_compute();
+    const int left = std::max(1, k.p.max_steps - host->iteration);
+    for (int i = 0; i < std::min(kFireBlock, left); ++i) {
+      if (exec) CDFT_HIP(hipGraphLaunch(exec, c->stream));
+      else launch_step_kernels();
+    }
+    CDFT_HIP(hipGetLastError());
+    poll();
   }
-  void after_step() {
-    read_compute();
-    ++iteration;
-    // convergence_monitor (minimization.hpp:105-110,329-332)
-    vv = vv * 0.9 + 0.1 * (std::fabs(energy - old_energy) / dt);
-    const double volume = static_cast<double>(c->g.N) * c->g.dv;
-    converged = std::fabs(vv / volume) < fire.force_tolerance;
-    if (energies) energies[iteration] = energy;
-  }
+  void poll() { CDFT_HIP(hipMemcpyAsync(host, ctl.p, sizeof(FireCtl), hipMemcpyDeviceToHost, c->stream)); }
+  void sync() { CDFT_HIP(hipStreamSynchronize(c->stream)); }
+  bool running() const { return host->state == FS_RUN; }
+  bool uphill() const { return host->state == FS_UPHILL; }
 
   // the final total() (minimization.hpp:339-343) is the last compute's evaluation
-  void finish(double* rho_out, cdft_min_result* result) {
+  void finish(double* rho_out, cdft_min_result* result, double* energies) {
     const Geom& g = c->g;
     CDFT_HIP(hipSetDevice(c->device));
     hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, x.as<double>(), c->d_rho,
                        g.N, amap);
     CDFT_HIP(hipMemcpyAsync(rho_out, c->d_rho, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
+    double er[8], fs[8];
+    CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipMemcpyAsync(fs, res.p, sizeof(fs), hipMemcpyDeviceToHost, c->stream));
+    const FireCtl* h = host;
+    if (energies && en.p)
+      CDFT_HIP(hipMemcpyAsync(energies, en.p, 8 * (static_cast<size_t>(h->iteration) + 1), hipMemcpyDeviceToHost,
+                              c->stream));
     sync();
-    result->free_energy = last_result.free_energy;
-    result->grand_potential = last_result.grand_potential;
-    result->rms_force = rms;
-    result->iterations = iteration;
-    result->converged = converged ? 1 : 0;
-    result->dt = dt;
-    result->alpha = alpha;
+    cdft_result r{};
+    c->result_from(er, k.mu, &r);
+    result->free_energy = r.free_energy;
+    result->grand_potential = r.grand_potential;
+    result->rms_force = std::sqrt(fs[0] / static_cast<double>(g.N));
+    result->iterations = h->iteration;
+    result->converged = h->state == FS_CONVERGED ? 1 : 0;
+    result->dt = h->dt;
+    result->alpha = h->alpha;
   }
 };
 
@@ -318,17 +410,16 @@ int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire, const cd
     const Geom& g = c->g;
     const AliasMap amap{param->kind == 0 ? 1 : 2, param->rho_min, param->rho_max - param->rho_min};
     if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * g.N, hipMemcpyHostToDevice, c->stream));
-    FireRun run(c, amap, use_homogeneous_boundary ? 1 : 0, vext ? c->d_vext : nullptr, mu, *fire);
-    run.energies = energies;
+    FireRun run(c, amap, use_homogeneous_boundary ? 1 : 0, vext ? c->d_vext : nullptr, mu, *fire,
+                energies != nullptr);
     run.launch_start(rho0);
     run.sync();
-    run.after_start();
-    while (run.active()) {
-      run.launch_step();
+    while (run.running()) {
+      run.launch_block();
       run.sync();
-      run.after_step();
     }
-    run.finish(rho_out, result);
+    if (run.uphill()) throw std::runtime_error("fire::step: exceeded max uphill steps");
+    run.finish(rho_out, result, energies);
   });
 }
 
@@ -353,38 +444,25 @@ int cdft_minimize_grand_potential_batch(int n, cdft_ctx* const* ctxs, const cdft
     for (int i = 0; i < n; ++i) {
       Context* c = ctxs[i]->impl;
       CDFT_HIP(hipSetDevice(c->device));
-      runs.emplace_back(new FireRun(c, amap, use_homogeneous_boundary ? 1 : 0, nullptr, mu[i], *fire));
-      status[i] = CDFT_OK;
+      runs.emplace_back(new FireRun(c, amap, use_homogeneous_boundary ? 1 : 0, nullptr, mu[i], *fire, false));
     }
-    for (auto& r : runs) r->launch_start(rho0[&r - &runs[0]]);
-    for (auto& r : runs) {
-      r->sync();
-      r->after_start();
-    }
-    // lockstep: queue every active run's step, then wait for and read each
-    std::vector<char> launched(static_cast<size_t>(n), 0);
+    for (int i = 0; i < n; ++i) runs[static_cast<size_t>(i)]->launch_start(rho0[i]);
+    for (auto& r : runs) r->sync();
+    // every running run's block is queued before any is waited for
+    std::vector<FireRun*> live;
     for (;;) {
-      bool any = false;
-      for (int i = 0; i < n; ++i) {
-        launched[static_cast<size_t>(i)] = 0;
-        if (status[i] != CDFT_OK || !runs[static_cast<size_t>(i)]->active()) continue;
-        try {
-          runs[static_cast<size_t>(i)]->launch_step();
-          launched[static_cast<size_t>(i)] = 1;
-          any = true;
-        } catch (const std::runtime_error& e) {  // FIRE's uphill limit ends that run only
-          if (dynamic_cast<const HipError*>(&e)) throw;
-          status[i] = CDFT_ERUNTIME;
-        }
-      }
-      if (!any) break;
-      for (int i = 0; i < n; ++i) {
-        if (!launched[static_cast<size_t>(i)]) continue;
-        runs[static_cast<size_t>(i)]->sync();
-        runs[static_cast<size_t>(i)]->after_step();
-      }
+      live.clear();
+      for (auto& r : runs)
+        if (r->running()) live.push_back(r.get());
+      if (live.empty()) break;
+      for (FireRun* r : live) r->launch_block();
+      for (FireRun* r : live) r->sync();
     }
-    for (int i = 0; i < n; ++i) runs[static_cast<size_t>(i)]->finish(rho_out[i], &results[i]);
+    for (int i = 0; i < n; ++i) {
+      FireRun& r = *runs[static_cast<size_t>(i)];
+      status[i] = r.uphill() ? CDFT_ERUNTIME : CDFT_OK;  // FIRE's uphill limit ends that run only
+      r.finish(rho_out[i], &results[i], nullptr);
+    }
   });
 }
 
