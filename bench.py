@@ -182,6 +182,38 @@ def adjoint_ddft(cd, W, shape=(128, 128, 128), K=400, P=2, dt=1e-4, n_wells=16, 
     }
 
 
+def ddft_slab(cd, W, steps=2000, shape=(256, 128, 128), dt=1e-4, precision="mixed"):
+    """The DDFT slab workload: a planar LJ liquid slab (rho_l = 0.70 over 0.4 L along x, rho_v = 0.02, 1e-3 noise,
+    kT = 0.8, WB-II + LJ WCA, dx = 0.1) relaxed by the integrating-factor scheme at fixed dt (dt_max = dt) for
+    `steps` steps.  Reports steps/s, evaluations/s and the relative mass change."""
+    import torch
+
+    dx = 0.1
+    ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII", precision=precision)
+    rho0 = W.slab(shape, dx, fraction=0.4, width=0.5, rho_l=0.70, rho_v=0.02, noise=1e-3, axis=0)
+    mu = ctx.bulk_mu(0.02)
+    ctx.ddft_run(rho0, mu, 2, dt=dt, dt_max=dt, record_energies=False)  # warm-up (plans, graphs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, r, en = ctx.ddft_run(rho0, mu, steps, dt=dt, dt_max=dt)
+    t = time.perf_counter() - t0
+    return {
+        "grid": list(shape),
+        "schedule": ctx.schedule,
+        "precision": precision,
+        "steps": steps,
+        "dt": dt,
+        "dt_final": r["dt"],
+        "seconds": t,
+        "steps_per_s": steps / t,
+        "evaluations": r["evaluations"],
+        "evals_per_s": r["evaluations"] / t,
+        "mass_rel_change": abs(r["mass_final"] - r["mass_initial"]) / r["mass_initial"],
+        "energy_first": float(en[0]),
+        "energy_last": float(en[-1]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +225,8 @@ def main():
     ap.add_argument("--profile-reps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential FIRE measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the DDFT adjoint measurement")
+    ap.add_argument("--ddft-steps", type=int, default=2000,
+                    help="steps of the 256x128x128 DDFT slab run (0 skips it)")
     ap.add_argument("--adjoint-fd-all", action="store_true",
                     help="check the adjoint gradient against differences for (A, w, cz) of every well")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -308,6 +342,9 @@ def main():
 
     if not args.no_adjoint:
         out["adjoint"] = adjoint_ddft(cd, W, fd_all=args.adjoint_fd_all)
+
+    if args.ddft_steps > 0:
+        out["ddft_slab"] = ddft_slab(cd, W, steps=args.ddft_steps)
 
     if not args.no_cpu_baseline:
         cb, parts, fr = cpu_baseline(rho, mu, shape, dx)

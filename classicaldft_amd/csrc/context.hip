@@ -182,7 +182,7 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->d_SF = dalloc_bytes(cs_bytes * g.M * nf_alloc);
     c->d_SO = dalloc_bytes(cs_bytes * g.M);
     c->d_mf = dalloc_bytes(8 * g.M);  // FP64 values, or FP32 (value, residual) planes
-    c->fast = options->schedule == CDFT_SCHEDULE_AUTO && fast_supported(g.nx, g.ny, g.nz);
+    c->fast = options->schedule == CDFT_SCHEDULE_AUTO && fast_supported(g.nx, g.ny, g.nz, c->precision);
     c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g) : zpoint_blocks<double>(g);
     if (c->fast) {
       c->fg = make_fast_geom(g);

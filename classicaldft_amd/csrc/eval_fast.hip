@@ -26,10 +26,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
+#include "cdft.h"
 #include "device_common.hpp"
 #include "eval_fast.hpp"
 #include "fmt_point.hpp"
@@ -771,8 +773,8 @@ __device__ __forceinline__ cx<C> apply_form(const ChanForm<C>& f, C kx, C ky, C 
 
 // ---------------------------------------------------------------- kf_rho_xy
 // FP64 y- and x-FFT of the rho parts: block (kz, r) transforms the y lines
-// of its half plane once, then the columns in two halves c through one
-// transpose buffer, and writes the full spectrum rho_hat(kx, ky = r + 2 s, kz)
+// of its half plane once, then the columns in NPART batches (two; four for
+// NX = 256, to fit the transpose buffer in LDS) through one transpose buffer, and writes the full spectrum rho_hat(kx, ky = r + 2 s, kz)
 //   XR[((kz * 2 + r) * Q + s) * NX + kx]   (complex FP64).
 // (One block per half plane runs the 2 (Nz/2 + 1) blocks in a single round
 // on the CUs; splitting a half plane over two blocks would repeat its y
@@ -780,7 +782,8 @@ __device__ __forceinline__ cx<C> apply_form(const ChanForm<C>& f, C kx, C ky, C 
 template <int NX, int Q, typename S>
 struct RhoCfg {
   using P = PlaneCfg<NX, Q>;
-  static constexpr int NC = Q / 2;                 // columns per half
+  static constexpr int NPART = NX >= 256 ? 4 : 2;  // column batches of the transpose (LDS size)
+  static constexpr int NC = Q / NPART;             // columns per batch
   static constexpr int TXR = NX / 4, SXR = kBT / TXR, CPR = (NC + SXR - 1) / SXR;
   static constexpr int NXP = NX + 1;
   static constexpr int tb_bytes = align16(NC * NXP * static_cast<int>(sizeof(cx<double>)));
@@ -837,10 +840,10 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
     }
   }
 #pragma unroll
-  for (int c = 0; c < 2; ++c) {
+  for (int c = 0; c < RC::NPART; ++c) {
     const int s0 = c * RC::NC;
     // transpose the lane's outputs of columns [s0, s0 + NC); tb is not the
-    // y-slot memory, but its previous half is read until the barrier
+    // y-slot memory, but its previous batch is read until the barrier
     if (c > 0) __syncthreads();
 #pragma unroll
     for (int rr = 0; rr < RPF; ++rr) {
@@ -853,7 +856,7 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
         }
       }
     }
-    __syncthreads();  // the transposed half is complete; every y (or x) slot exchange is over
+    __syncthreads();  // the transposed batch is complete; every y (or x) slot exchange is over
 #pragma unroll
     for (int cc = 0; cc < RC::CPR; ++cc) {
       const int sc = xs + RC::SXR * cc;
@@ -1285,9 +1288,12 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
 }  // namespace
 
 // ============================================================ host side
-bool fast_supported(int nx, int ny, int nz) {
+bool fast_supported(int nx, int ny, int nz, int precision) {
   auto ok = [](int n) { return n == 64 || n == 128; };
-  return ok(nx) && ok(ny) && ok(nz);
+  if (!ok(ny) || !ok(nz)) return false;
+  if (ok(nx)) return true;
+  // x extent 256: the FP64 plane kernels of a 256 x 128 plane need more LDS than a CU has
+  return nx == 256 && (precision == CDFT_PRECISION_MIXED || ny == 64);
 }
 
 FastGeom make_fast_geom(const Geom& g) {
@@ -1349,15 +1355,20 @@ template <typename T>
 const T* as(const void* p) {
   return static_cast<const T*>(p);
 }
-// Dynamic LDS above 64 KiB has to be enabled per kernel (once per process).
+// Dynamic LDS above 64 KiB has to be enabled per kernel, once per process.
+// Instantiations of one kernel template share a function type, so the record
+// is kept per kernel address.
 template <typename K>
 void set_smem(K kernel, int bytes) {
-  static bool done = false;
-  if (done) return;
-  if (hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
-      hipSuccess)
+  static std::mutex mu;
+  static std::vector<const void*> done;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lock(mu);
+  if (std::find(done.begin(), done.end(), key) != done.end()) return;
+  if (bytes > 160 * 1024) throw std::runtime_error("fused kernel needs more LDS than a CU has");
+  if (hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
     throw std::runtime_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
-  done = true;
+  done.push_back(key);
 }
 }  // namespace
 
@@ -1409,6 +1420,8 @@ void launch_fast_rho_xy(const cx<S>* A0, const double* D0, void* XR, double* sum
   else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
   else if (g.nx == 128 && g.q == 32) CDFT_L(128, 32)
   else if (g.nx == 128 && g.q == 64) CDFT_L(128, 64)
+  else if (g.nx == 256 && g.q == 32) CDFT_L(256, 32)
+  else if (g.nx == 256 && g.q == 64) CDFT_L(256, 64)
   else throw std::runtime_error("fast path: plane shape");
 #undef CDFT_L
 }
@@ -1437,6 +1450,8 @@ void launch_fast_plane_fwd(const void* XR, const void* WT, cx<S>* SF, const doub
   else if (g.nx == 64 && g.q == 64) CDFT_NF(64, 64)
   else if (g.nx == 128 && g.q == 32) CDFT_NF(128, 32)
   else if (g.nx == 128 && g.q == 64) CDFT_NF(128, 64)
+  else if (g.nx == 256 && g.q == 32) CDFT_NF(256, 32)
+  else if (g.nx == 256 && g.q == 64) CDFT_NF(256, 64)
   else throw std::runtime_error("fast path: plane shape");
 #undef CDFT_NF
 #undef CDFT_L
@@ -1461,6 +1476,8 @@ void launch_fast_plane_bwd(const cx<S>* SB, const void* WT, cx<S>* P, const Fast
   else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
   else if (g.nx == 128 && g.q == 32) CDFT_L(128, 32)
   else if (g.nx == 128 && g.q == 64) CDFT_L(128, 64)
+  else if (g.nx == 256 && g.q == 32) CDFT_L(256, 32)
+  else if (g.nx == 256 && g.q == 64) CDFT_L(256, 64)
   else throw std::runtime_error("fast path: plane shape");
 #undef CDFT_L
 }

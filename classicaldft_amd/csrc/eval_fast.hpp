@@ -26,7 +26,7 @@ struct FastGeom {
 };
 
 // True when the fused schedule has kernels for this grid.
-bool fast_supported(int nx, int ny, int nz);
+bool fast_supported(int nx, int ny, int nz, int precision);
 FastGeom make_fast_geom(const Geom& g);
 
 // FP64 and FP32 twiddle tables of length kTwLen.

@@ -1,6 +1,6 @@
-"""GPU parity of the fused five-kernel evaluation schedule (grids whose axes
-are 64 or 128) against the FP64 CPU oracle and against the general schedule,
-through the C ABI.
+"""GPU parity of the fused six-kernel evaluation schedule (y and z extents 64
+or 128, x extent 64, 128 or 256) against the FP64 CPU oracle and against the
+general schedule, through the C ABI.
 
 Tolerances: FP64 fields and forces 1e-11 relative to their maximum, energies
 1e-12; mixed precision Omega 1e-6 relative, forces 2e-5 relative to max|force|.
@@ -41,7 +41,8 @@ def _check_parts(res, parts, rtol, atol):
         assert res[k] == pytest.approx(parts[i], rel=rtol, abs=atol), k
 
 
-@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 64), (64, 128, 64), (64, 64, 128)])
+@pytest.mark.parametrize("shape", [(64, 64, 64), (128, 64, 64), (64, 128, 64), (64, 64, 128), (256, 64, 64),
+                                   (256, 64, 128)])
 def test_fused_fp64_matches_oracle(shape):
     dx = 0.1
     ctx = _ctx(shape, dx)
@@ -107,6 +108,31 @@ def test_fused_matches_general_schedule(precision):
           f"vs {rb['grand_potential']:.12e}", flush=True)
     assert e < tol
     assert ra["grand_potential"] == pytest.approx(rb["grand_potential"], rel=1e-12 if precision == "fp64" else 1e-6)
+
+
+@pytest.mark.parametrize("shape,precision", [((256, 128, 128), "mixed"), ((256, 64, 128), "fp64"),
+                                             ((256, 128, 64), "mixed")])
+def test_fused_x256_matches_general_schedule(shape, precision):
+    # x extent 256 (the DDFT slab grid is 256 x 128 x 128)
+    dx = 0.1
+    rho = W.slab(shape, dx, fraction=0.4, width=0.5, rho_l=0.7, rho_v=0.02, noise=1e-3, axis=0)
+    a = _ctx(shape, dx, precision=precision)
+    b = _ctx(shape, dx, precision=precision, schedule="general")
+    assert a.schedule == "fused" and b.schedule == "general"
+    mu = a.bulk_mu(0.02)
+    ra, fa = a.total(rho, mu)
+    rb, fb = b.total(rho, mu)
+    e = _rel(fa, fb)
+    print(f"fused vs general {shape} {precision}: force {e:.2e}, Omega {ra['grand_potential']:.12e} "
+          f"vs {rb['grand_potential']:.12e}", flush=True)
+    assert e < (1e-12 if precision == "fp64" else 2e-5)
+    assert ra["grand_potential"] == pytest.approx(rb["grand_potential"], rel=1e-12 if precision == "fp64" else 1e-6)
+
+
+def test_fp64_256x128_plane_uses_general_schedule():
+    # its FP64 plane kernels would need more LDS than a CU has
+    assert _ctx((256, 128, 64), 0.1).schedule == "general"
+    assert _ctx((256, 128, 64), 0.1, precision="mixed").schedule == "fused"
 
 
 def test_fused_mixed_gate_64():
