@@ -28,7 +28,7 @@ EXPORTED = [
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
-    "cdft_ddft_adjoint_wells",
+    "cdft_ddft_adjoint_wells", "cdft_autotune",
 ]
 
 
@@ -139,6 +139,7 @@ def lib():
         L.cdft_stage_name.restype = C.c_char_p
         L.cdft_stage_name.argtypes = [C.c_void_p, C.c_int]
         L.cdft_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.cdft_autotune.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.cdft_hessian_vector.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.cdft_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.cdft_ddft_fixed_run.argtypes = [C.c_void_p, C.POINTER(_AdjointConfig), _dp, C.c_double, _dp, _dp]
@@ -309,6 +310,13 @@ class Context:
                                    float(mu), _p(rho), C.byref(res), _p(en), None))
         r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
         return rho.reshape(self.shape), r, en
+
+    def autotune(self, reps=3):
+        """Tune the general schedule's FFT radix orders and launch shapes for this grid; returns the time of one
+        evaluation (microseconds) before and after."""
+        a, b = C.c_double(), C.c_double()
+        _check(lib().cdft_autotune(self._h, int(reps), C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # ------------------------------------------------------------ adjoint (FP64 contexts)
     def hessian_vector(self, rho, u):

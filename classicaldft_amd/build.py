@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcdft.so")
 ARCH = os.environ.get("CDFT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["eval_fast.hip", "kernels.hip", "context.hip", "cdft_api.hip", "dynamics.hip", "adjoint.hip"]
+SOURCES = ["eval_fast.hip", "kernels.hip", "context.hip", "cdft_api.hip", "dynamics.hip", "adjoint.hip", "autotune.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}",
@@ -55,7 +55,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = False) -> str:
     os.makedirs(BUILD, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(5, len(SOURCES))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
     cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]

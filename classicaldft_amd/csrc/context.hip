@@ -183,7 +183,8 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->d_SO = dalloc_bytes(cs_bytes * g.M);
     c->d_mf = dalloc_bytes(8 * g.M);  // FP64 values, or FP32 (value, residual) planes
     c->fast = options->schedule == CDFT_SCHEDULE_AUTO && fast_supported(g.nx, g.ny, g.nz, c->precision);
-    c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g) : zpoint_blocks<double>(g);
+    c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g, c->plan[2])
+                                                         : zpoint_blocks<double>(g, c->plan[2]);
     if (c->fast) {
       c->fg = make_fast_geom(g);
       hipDeviceProp_t prop{};
@@ -199,7 +200,9 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       CDFT_HIP(hipGetLastError());
       CDFT_HIP(hipStreamSynchronize(c->stream));
     }
-    c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->nblk_point));
+    // room for the most blocks a pointwise launch shape can have (2 z lines per block)
+    c->max_point_rows = c->fast ? c->nblk_point : std::max<long>(c->nblk_point, (g.nxy + 1) / 2);
+    c->d_partials = dalloc<double>(8 * static_cast<size_t>(c->max_point_rows));
     c->d_res = dalloc<double>(8);
     c->nblk_sum = sum_blocks(g);
     c->d_sum_part = dalloc<double>(8 * static_cast<size_t>(c->nblk_sum));

@@ -74,6 +74,7 @@ struct Context {
   double* d_sum_part = nullptr;  // block partials of the FP64 density sum
   double* d_shift = nullptr;     // [0] = sum of rho (mean-density split)
   int nblk_point = 0;  // rows of 8 partial sums written by the point kernel
+  long max_point_rows = 0;  // rows d_partials has room for
   int nblk_sum = 0;
   DcTable dc{};                  // analytic k = 0 weight of every forward channel
   hipEvent_t* stage_events = nullptr;  // when set, eval records nstages()+1 events

@@ -72,7 +72,14 @@ __device__ __forceinline__ R kmul(R x, double c) {
 struct FftPlan {
   int L;
   int np;
-  int rad[12];
+  int rad[12];  // radices of the passes, in order
+  // Launch shape of the general-schedule kernels along this axis (0: the
+  // built-in default; set by the autotuner):
+  //   lines  lines per block of the axis' line transforms;
+  //   tile   x axis: y columns per block of the x kernels;
+  //          z axis: z lines per block of the pointwise kernel.
+  int lines;
+  int tile;
 };
 
 // ---- radix-r DFT kernels: v[m] <- sum_i v[i] exp(sg * 2 pi i m i / r) -------

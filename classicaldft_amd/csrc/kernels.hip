@@ -562,8 +562,9 @@ __global__ void __launch_bounds__(kFinThreads) k_finalize(const double* __restri
 // z-lines per block of the plain z kernels: up to 2048 complex elements (two
 // real lines per element) per transform buffer.
 template <typename C>
-static int zlines_per_block(int nz) {
-  int nlb = (sizeof(C) == 4 ? 4096 : 2048) / nz;
+static int zlines_per_block(const FftPlan& pz) {
+  if (pz.lines > 0) return pz.lines;
+  int nlb = (sizeof(C) == 4 ? 4096 : 2048) / pz.L;
   if (nlb < 2) nlb = 2;
   if (nlb & 1) --nlb;
   if (nlb > 64) nlb = 64;
@@ -573,7 +574,7 @@ static int zlines_per_block(int nz) {
 template <typename S, typename C>
 void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz, const cx<C>* twz, AliasMap amap,
                  const double* shift_sum, hipStream_t st) {
-  const int nlb = zlines_per_block<C>(g.nz);
+  const int nlb = zlines_per_block<C>(pz);
   if ((nlb >> 1) * g.nz > 8 * NT) throw std::runtime_error("z extent too large for the z transform kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz;
@@ -583,7 +584,7 @@ void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz,
 template <typename S, typename C>
 void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, double scale, const Geom& g,
                  const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st) {
-  const int nlb = zlines_per_block<C>(g.nz);
+  const int nlb = zlines_per_block<C>(pz);
   if ((nlb >> 1) * g.nz > 8 * NT) throw std::runtime_error("z extent too large for the z transform kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz + sizeof(cx<S>) * nlb * g.nzh;
@@ -604,7 +605,7 @@ void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g
 
 template <typename S, typename C>
 void launch_lines(cx<S>* Sp, long nlines, const FftPlan& p, const cx<C>* tw, int sg, hipStream_t st) {
-  const int nlb = 2048 / p.L;
+  const int nlb = p.lines > 0 ? p.lines : 2048 / p.L;
   if (nlb < 1) throw std::runtime_error("line length too large for the line transform kernel");
   const unsigned nblk = static_cast<unsigned>((nlines + nlb - 1) / nlb);
   const size_t sh = sizeof(cx<C>) * nlb * p.L;
@@ -615,7 +616,7 @@ static constexpr int kXTile = 16;  // y columns per x-line block (128-byte rows 
 
 template <typename S, typename C>
 void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st) {
-  const int ty = kXTile;
+  const int ty = px.tile > 0 ? px.tile : kXTile;
   const size_t sh = sizeof(cx<C>) * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh, nch);
   if (ty * g.nx <= 8 * NT)
@@ -629,7 +630,7 @@ void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const c
 template <typename S, typename C>
 void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const Geom& g, const FftPlan& px, const cx<C>* twx,
                  ChannelSet cs, hipStream_t st) {
-  const int ty = kXTile;
+  const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;  // channels per batched transform (LDS <= 64 KiB)
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
@@ -644,7 +645,7 @@ void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const Geom& g, const F
 template <typename S, typename C>
 void launch_xbwd(const cx<S>* SB, cx<S>* SO, const Geom& g, const FftPlan& px, const cx<C>* twx, int nb,
                  hipStream_t st) {
-  const int ty = kXTile;
+  const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
@@ -659,8 +660,9 @@ void launch_xbwd(const cx<S>* SB, cx<S>* SO, const Geom& g, const FftPlan& px, c
 // z-lines per block of the fused pointwise kernel: PPT*NT points (4 per thread
 // in FP32 storage, 2 in FP64).
 template <typename S>
-static int zpoint_nlb(int nz) {
-  int nlb = (sizeof(S) == 4 ? 4 : 2) * NT / nz;
+static int zpoint_nlb(const FftPlan& pz) {
+  if (pz.tile > 0) return pz.tile;
+  int nlb = (sizeof(S) == 4 ? 4 : 2) * NT / pz.L;
   if (nlb < 2) nlb = 2;
   if (nlb & 1) --nlb;
   if (nlb > 64) nlb = 64;
@@ -668,12 +670,12 @@ static int zpoint_nlb(int nz) {
 }
 
 template <typename S>
-int zpoint_blocks(const Geom& g) {
-  const int nlb = zpoint_nlb<S>(g.nz);
+int zpoint_blocks(const Geom& g, const FftPlan& pz) {
+  const int nlb = zpoint_nlb<S>(pz);
   return static_cast<int>((g.nxy + nlb - 1) / nlb);
 }
-template int zpoint_blocks<float>(const Geom&);
-template int zpoint_blocks<double>(const Geom&);
+template int zpoint_blocks<float>(const Geom&, const FftPlan&);
+template int zpoint_blocks<double>(const Geom&, const FftPlan&);
 
 template <typename S, typename CF, typename CB, int PPT, int NF, int NB>
 static void zpoint_go(unsigned nblk, size_t sh, hipStream_t st, cx<S>* SF, const double* rho, const double* vext,
@@ -689,7 +691,7 @@ void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* loc
                    const Geom& g, const FftPlan& pz, const cx<CF>* twf, const cx<CB>* twb, ChannelSet cs,
                    FmtParams fp, double mu, AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
   constexpr int PPT = sizeof(S) == 4 ? 4 : 2;
-  const int nlb = zpoint_nlb<S>(g.nz);
+  const int nlb = zpoint_nlb<S>(pz);
   if (nlb * g.nz > PPT * NT) throw std::runtime_error("z extent too large for the fused pointwise kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const int zs = (nlb >> 1) * g.nz;

@@ -70,7 +70,7 @@ template <typename S, typename C>
 void launch_xbwd(const cx<S>* SB, cx<S>* SO, const Geom& g, const FftPlan& px, const cx<C>* twx, int nb,
                  hipStream_t st);
 template <typename S>
-int zpoint_blocks(const Geom& g);
+int zpoint_blocks(const Geom& g, const FftPlan& pz);
 template <typename S, typename CF, typename CB>
 void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* local_force, double* partials,
                    const Geom& g, const FftPlan& pz, const cx<CF>* twf, const cx<CB>* twb, ChannelSet cs,

@@ -68,10 +68,11 @@ typedef struct {
 #define CDFT_PRECISION_FP64 0
 #define CDFT_PRECISION_MIXED 1
 
-/* Kernel schedule of an evaluation.  AUTO picks the fused five-kernel
- * schedule when the grid has kernels for it (every axis 64 or 128) and the
- * general per-axis schedule otherwise; GENERAL forces the latter.  Both give
- * the same results to rounding. */
+/* Kernel schedule of an evaluation.  AUTO picks the fused six-kernel
+ * schedule when the grid has kernels for it (y and z extents 64 or 128, x
+ * extent 64, 128 or 256; in FP64 an x extent of 256 needs a y extent of 64)
+ * and the general per-axis schedule otherwise; GENERAL forces the latter.
+ * Both give the same results to rounding. */
 #define CDFT_SCHEDULE_AUTO 0
 #define CDFT_SCHEDULE_GENERAL 1
 #define CDFT_SCHEDULE_FUSED 2
@@ -222,6 +223,16 @@ typedef struct {
  * log_interval steps). */
 CDFT_API int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
                   double* rho, cdft_sim_result* result, double* energies, double* times);
+
+/* Autotuner (no counterpart in the reference, which leaves FFT planning to
+ * FFTW_ESTIMATE, src/math/fourier.cpp:34-57): for grids the fused kernels do
+ * not cover, times the evaluation under alternative FFT radix orders and
+ * launch shapes and keeps the fastest whose Omega agrees with the default's.
+ * Results are cached per grid, precision and channel set for the process, so
+ * later contexts of the same shape take them without timing.  us_default and
+ * us_tuned (nullable) receive the time of one evaluation before and after, in
+ * microseconds; a fused-schedule context is timed and left as it is. */
+CDFT_API int cdft_autotune(cdft_ctx* ctx, int reps, double* us_default, double* us_tuned);
 
 /* ---------------------------------------------------------------- adjoint
  * The reference has no derivative of its dynamics; the entry points below
