@@ -246,7 +246,8 @@ struct Hessian {
         launch_zr2c<double, double>(dg.as<double>() + ch * g.N, SF + ch * g.M, g, c->plan[2], c->tw<double>(2), id,
                                     nullptr, c->stream);
       launch_lines<double, double>(SF, ylines * c->cs.nb, c->plan[1], c->tw<double>(1), -1, c->stream);
-      launch_xbwd<double, double>(SF, SO, g, c->plan[0], c->tw<double>(0), c->cs.nb, c->stream);
+      launch_xbwd<double, double>(SF, SO, static_cast<const cx<double>*>(c->d_WTg), g, c->plan[0],
+                                  c->tw<double>(0), c->cs.nb, c->stream);
       launch_lines<double, double>(SO, ylines, c->plan[1], c->tw<double>(1), +1, c->stream);
       launch_zc2r<double, double>(SO, 1, out, local.as<double>(), g.dv, g, c->plan[2], c->tw<double>(2), nullptr,
                                   DcTable{}, c->stream);

@@ -59,7 +59,7 @@ Context::~Context() {
   clear_graphs();
   void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
                   d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
-                  ftw64,   ftw32,   d_XR,     d_WT,     d_D0};
+                  ftw64,   ftw32,   d_XR,     d_WT,     d_D0,     d_WTg};
   hipSetDevice(device);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
@@ -182,6 +182,11 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->d_SF = dalloc_bytes(cs_bytes * g.M * nf_alloc);
     c->d_SO = dalloc_bytes(cs_bytes * g.M);
     c->d_mf = dalloc_bytes(8 * g.M);  // FP64 values, or FP32 (value, residual) planes
+    if (c->cs.hs) {
+      c->d_WTg = dalloc_bytes(16 * g.M);
+      launch_wtable_general(static_cast<cx<double>*>(c->d_WTg), g, c->stream);
+      CDFT_HIP(hipGetLastError());
+    }
     c->fast = options->schedule == CDFT_SCHEDULE_AUTO && fast_supported(g.nx, g.ny, g.nz, c->precision);
     c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(g, c->plan[2])
                                                          : zpoint_blocks<double>(g, c->plan[2]);
@@ -343,7 +348,8 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     mark(2);
     launch_lines<S, CL>(S0, ylines, plan[1], tw<CL>(1), -1, stream);
     mark(3);
-    launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(d_mf), g, plan[0], tw<CE>(0), cs, stream);
+    launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(d_mf), static_cast<const cx<double>*>(d_WTg), g, plan[0],
+                       tw<CE>(0), cs, stream);
     mark(4);
     launch_lines<S, CE>(SF, ylines * cs.nf, plan[1], tw<CE>(1), +1, stream);
   }
@@ -357,7 +363,7 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     mark(6);
     launch_lines<S, CL>(SF, ylines * cs.nb, plan[1], tw<CL>(1), -1, stream);
     mark(7);
-    launch_xbwd<S, CL>(SF, SO, g, plan[0], tw<CL>(0), cs.nb, stream);
+    launch_xbwd<S, CL>(SF, SO, static_cast<const cx<double>*>(d_WTg), g, plan[0], tw<CL>(0), cs.nb, stream);
     mark(8);
     launch_lines<S, CL>(SO, ylines, plan[1], tw<CL>(1), +1, stream);
     mark(9);
@@ -500,7 +506,8 @@ void wd_t(Context* c, const double* d_rho_in, double* d_out) {
   launch_finalize(c->d_sum_part, c->nblk_sum, c->d_shift, c->stream);
   launch_zr2c<S, CL>(d_rho_in, S0, g, c->plan[2], c->tw<CL>(2), id, c->d_shift, c->stream);
   launch_lines<S, CL>(S0, ylines, c->plan[1], c->tw<CL>(1), -1, c->stream);
-  launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(c->d_mf), g, c->plan[0], c->tw<CE>(0), c->cs, c->stream);
+  launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(c->d_mf), static_cast<const cx<double>*>(c->d_WTg), g,
+                     c->plan[0], c->tw<CE>(0), c->cs, c->stream);
   launch_lines<S, CE>(SF, ylines * c->cs.nf, c->plan[1], c->tw<CE>(1), +1, c->stream);
   launch_zc2r<S, CE>(SF, c->cs.nf, d_out, nullptr, 1.0, g, c->plan[2], c->tw<CE>(2), c->d_shift, c->dc, c->stream);
 }

@@ -69,6 +69,7 @@ struct Context {
   void* d_SF = nullptr;       // nf * M complex (derivative channels reuse it)
   void* d_SO = nullptr;       // M complex
   void* d_mf = nullptr;       // M real: mean-field k-space weight
+  void* d_WTg = nullptr;      // M complex FP64: radial FMT weights (w3, w2) in the general schedule's order
   double* d_partials = nullptr;
   double* d_res = nullptr;
   double* d_sum_part = nullptr;  // block partials of the FP64 density sum

@@ -77,6 +77,35 @@ __device__ __forceinline__ void radial(double k, double R, double inv_n, double*
   *w2 = 4.0 * PI * R * s / k * inv_n;
 }
 
+// Radial factors of one element: w3, w2 and aik2 = (w2 - 3 w3/R) / k^2 (0 at
+// k = 0), from which every FMT channel weight follows without a division.
+struct WRad {
+  double w3, w2, aik2;
+};
+__device__ __forceinline__ WRad wrad(cx<double> w, double k2, double invR) {
+  return {w.re, w.im, k2 > 0.0 ? (w.im - 3.0 * w.re * invR) / k2 : 0.0};
+}
+// Weight of channel ch from the radial factors (the closed forms of
+// weights.hpp:174-222): ch 0 w3, 1 w2, 2..4 i k_a w3 (conj when `adj`), 5..10
+// tensor aik2 k_i k_j + delta_ij w3 / R, 11 the mean-field table value.
+__device__ __forceinline__ cx<double> channel_weight_t(int ch, bool adj, double kx, double ky, double kz, WRad w,
+                                                       double mf, double invR) {
+  if (ch == 0) return {w.w3, 0.0};
+  if (ch == 1) return {w.w2, 0.0};
+  if (ch <= 4) {
+    const double ka = ch == 2 ? kx : (ch == 3 ? ky : kz);
+    return {0.0, adj ? -ka * w.w3 : ka * w.w3};
+  }
+  if (ch <= 10) {
+    const int q = ch - 5;
+    const double ki = q < 3 ? kx : (q < 5 ? ky : kz);
+    const double kj = (q == 0) ? kx : (q == 1 || q == 3) ? ky : kz;
+    const bool diag = (q == 0 || q == 3 || q == 5);
+    return {w.aik2 * ki * kj + (diag ? w.w3 * invR : 0.0), 0.0};
+  }
+  return {mf, 0.0};
+}
+
 // Weight of channel ch at k (FP64): ch 0 w3, 1 w2, 2..4 i k_a w3 (conj when
 // `adj`), 5..10 tensor xx,xy,xz,yy,yz,zz, 11 the mean-field table value.
 __device__ __forceinline__ cx<double> channel_weight(int ch, bool adj, double kx, double ky, double kz, double w3,

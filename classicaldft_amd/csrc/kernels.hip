@@ -234,8 +234,8 @@ __global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, F
 // per batched transform.  EPT >= ty*nx/NT elements owned per thread.
 template <typename S, typename C, int EPT, int G>
 __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
-                                             const S* __restrict__ mf, Geom g, FftPlan px,
-                                             const cx<C>* __restrict__ twx, int ty, ChannelSet cs) {
+                                             const S* __restrict__ mf, const cx<double>* __restrict__ wt, Geom g,
+                                             FftPlan px, const cx<C>* __restrict__ twx, int ty, ChannelSet cs) {
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
@@ -251,19 +251,22 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
   __syncthreads();
   fft_inplace<C, EPT>(buf, ty, ls, px, -1, twx);
   cx<C> rh[EPT];
-  double w3[EPT], w2[EPT], wm[EPT];
+  WRad wr[EPT];
+  double wm[EPT];
   const double kzv = g.dkz * kz;
+  const double invR = 1.0 / g.R;
 #pragma unroll
   for (int m = 0; m < EPT; ++m) {
     const int e = threadIdx.x + m * blockDim.x;
     rh[m] = {C(0), C(0)};
-    w3[m] = w2[m] = wm[m] = 0.0;
+    wr[m] = {0.0, 0.0, 0.0};
+    wm[m] = 0.0;
     if (e < ty * nx) {
       const int x = e / ty, yl = e - x * ty, y = y0 + yl;
       rh[m] = buf[yl * ls + x];
       if (y < ny) {
         const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
-        if (cs.hs) radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3[m], &w2[m]);
+        if (cs.hs) wr[m] = wrad(wt[plane_off + x * ny + y], kx * kx + ky * ky + kzv * kzv, invR);
         // mean-field table: FP64 values, or FP32 (value, residual) planes
         if (cs.mf)
           wm[m] = sizeof(S) == 8 ? static_cast<double>(mf[plane_off + x * ny + y])
@@ -273,7 +276,6 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
     }
   }
   __syncthreads();
-  const double invR = 1.0 / g.R;
   for (int c0 = 0; c0 < cs.nf; c0 += G) {
     const int ng = min(G, cs.nf - c0);
     for (int gi = 0; gi < ng; ++gi) {
@@ -284,7 +286,7 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
         if (e < ty * nx) {
           const int x = e / ty, yl = e - x * ty, y = y0 + yl;
           const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
-          const cx<double> w = channel_weight(ch, false, kx, ky, kzv, w3[m], w2[m], wm[m], invR);
+          const cx<double> w = channel_weight_t(ch, false, kx, ky, kzv, wr[m], wm[m], invR);
           buf[gi * slab + yl * ls + x] = wmul(rh[m], w);
         }
       }
@@ -306,8 +308,9 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
 // batched transform), sum of weight * channel with conj weights on the odd
 // vector channels (hard_sphere.hpp:118-134), inverse x-FFT of the sum.
 template <typename S, typename C, int EPT, int G>
-__global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO, Geom g,
-                                             FftPlan px, const cx<C>* __restrict__ twx, int ty, int nb) {
+__global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
+                                             const cx<double>* __restrict__ wt, Geom g, FftPlan px,
+                                             const cx<C>* __restrict__ twx, int ty, int nb) {
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
@@ -318,17 +321,17 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
   const double kzv = g.dkz * kz;
   const double invR = 1.0 / g.R;
   cx<C> acc[EPT];
-  double w3[EPT], w2[EPT];
+  WRad wr[EPT];
 #pragma unroll
   for (int m = 0; m < EPT; ++m) {
     acc[m] = {C(0), C(0)};
-    w3[m] = w2[m] = 0.0;
+    wr[m] = {0.0, 0.0, 0.0};
     const int e = threadIdx.x + m * blockDim.x;
     if (e < ty * nx) {
       const int x = e / ty, yl = e - x * ty, y = y0 + yl;
       if (y < ny) {
         const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
-        radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3[m], &w2[m]);
+        wr[m] = wrad(wt[plane_off + x * ny + y], kx * kx + ky * ky + kzv * kzv, invR);
       }
     }
   }
@@ -350,7 +353,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
         if (e < ty * nx) {
           const int x = e / ty, yl = e - x * ty, y = y0 + yl;
           const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
-          const cx<double> w = channel_weight(c0 + gi, true, kx, ky, kzv, w3[m], w2[m], 0.0, invR);
+          const cx<double> w = channel_weight_t(c0 + gi, true, kx, ky, kzv, wr[m], 0.0, invR);
           acc[m] = cadd(acc[m], wmul(buf[gi * slab + yl * ls + x], w));
         }
       }
@@ -628,31 +631,31 @@ void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const c
 }
 
 template <typename S, typename C>
-void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const Geom& g, const FftPlan& px, const cx<C>* twx,
-                 ChannelSet cs, hipStream_t st) {
+void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, const Geom& g, const FftPlan& px,
+                 const cx<C>* twx, ChannelSet cs, hipStream_t st) {
   const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;  // channels per batched transform (LDS <= 64 KiB)
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   if (ty * g.nx <= 8 * NT)
-    hipLaunchKernelGGL((k_xfwd<S, C, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, g, px, twx, ty, cs);
+    hipLaunchKernelGGL((k_xfwd<S, C, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, ty, cs);
   else if (ty * g.nx <= 16 * NT)
-    hipLaunchKernelGGL((k_xfwd<S, C, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, g, px, twx, ty, cs);
+    hipLaunchKernelGGL((k_xfwd<S, C, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, ty, cs);
   else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
 
 template <typename S, typename C>
-void launch_xbwd(const cx<S>* SB, cx<S>* SO, const Geom& g, const FftPlan& px, const cx<C>* twx, int nb,
-                 hipStream_t st) {
+void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g, const FftPlan& px,
+                 const cx<C>* twx, int nb, hipStream_t st) {
   const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   if (ty * g.nx <= 8 * NT)
-    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, g, px, twx, ty, nb);
+    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, nb);
   else if (ty * g.nx <= 16 * NT)
-    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, g, px, twx, ty, nb);
+    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, nb);
   else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
@@ -709,6 +712,27 @@ void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* loc
 #undef CDFT_ZP
 }
 
+// FP64 radial weights (w3, w2) of every element of the half spectrum in the
+// general schedule's order kz * Nx * Ny + kx * Ny + ky; they depend only on the
+// grid and the radius, so the x kernels read them instead of evaluating
+// sincos per element and launch.
+__global__ void __launch_bounds__(NT) k_wtable_general(cx<double>* __restrict__ wt, Geom g) {
+  const long nxy = static_cast<long>(g.nx) * g.ny;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < g.M;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const long kz = i / nxy, r = i - kz * nxy;
+    const int x = static_cast<int>(r / g.ny), y = static_cast<int>(r - static_cast<long>(x) * g.ny);
+    const double kx = kvec(x, g.nx, g.dkx), ky = kvec(y, g.ny, g.dky), kzv = g.dkz * static_cast<double>(kz);
+    double w3, w2;
+    radial(sqrt(kx * kx + ky * ky + kzv * kzv), g.R, g.inv_n, &w3, &w2);
+    wt[i] = {w3, w2};
+  }
+}
+
+void launch_wtable_general(cx<double>* wt, const Geom& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_wtable_general, dim3(1024), dim3(NT), 0, st, wt, g);
+}
+
 void launch_finalize(const double* partials, int nrows, double* res, hipStream_t st) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kFinThreads), 0, st, partials, nrows, res);
 }
@@ -720,10 +744,10 @@ void launch_finalize(const double* partials, int nrows, double* res, hipStream_t
                                   const cx<C>*, const double*, DcTable, hipStream_t);                              \
   template void launch_lines<S, C>(cx<S>*, long, const FftPlan&, const cx<C>*, int, hipStream_t);                 \
   template void launch_xlines<S, C>(cx<S>*, int, const Geom&, const FftPlan&, const cx<C>*, int, hipStream_t);    \
-  template void launch_xfwd<S, C>(const cx<S>*, cx<S>*, const S*, const Geom&, const FftPlan&, const cx<C>*,       \
-                                  ChannelSet, hipStream_t);                                                        \
-  template void launch_xbwd<S, C>(const cx<S>*, cx<S>*, const Geom&, const FftPlan&, const cx<C>*, int,           \
-                                  hipStream_t);
+  template void launch_xfwd<S, C>(const cx<S>*, cx<S>*, const S*, const cx<double>*, const Geom&, const FftPlan&, \
+                                  const cx<C>*, ChannelSet, hipStream_t);                                          \
+  template void launch_xbwd<S, C>(const cx<S>*, cx<S>*, const cx<double>*, const Geom&, const FftPlan&,           \
+                                  const cx<C>*, int, hipStream_t);
 
 CDFT_INST_SC(float, float)
 CDFT_INST_SC(float, double)

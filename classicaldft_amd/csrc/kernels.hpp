@@ -63,12 +63,14 @@ template <typename S, typename C>
 void launch_lines(cx<S>* Sp, long nlines, const FftPlan& p, const cx<C>* tw, int sg, hipStream_t st);
 template <typename S, typename C>
 void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st);
+// wt: FP64 radial weights (w3, w2) per element, from launch_wtable_general.
 template <typename S, typename C>
-void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const Geom& g, const FftPlan& px, const cx<C>* twx,
-                 ChannelSet cs, hipStream_t st);
+void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, const Geom& g, const FftPlan& px,
+                 const cx<C>* twx, ChannelSet cs, hipStream_t st);
 template <typename S, typename C>
-void launch_xbwd(const cx<S>* SB, cx<S>* SO, const Geom& g, const FftPlan& px, const cx<C>* twx, int nb,
-                 hipStream_t st);
+void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g, const FftPlan& px,
+                 const cx<C>* twx, int nb, hipStream_t st);
+void launch_wtable_general(cx<double>* wt, const Geom& g, hipStream_t st);
 template <typename S>
 int zpoint_blocks(const Geom& g, const FftPlan& pz);
 template <typename S, typename CF, typename CB>
