@@ -28,7 +28,7 @@ EXPORTED = [
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
-    "cdft_ddft_adjoint_wells", "cdft_autotune",
+    "cdft_ddft_adjoint_wells", "cdft_autotune", "cdft_minimize_fixed_mass",
 ]
 
 
@@ -130,6 +130,8 @@ def lib():
         L.cdft_minimize_grand_potential_batch.argtypes = [C.c_int, C.POINTER(C.c_void_p), C.POINTER(_Fire),
                                                           C.POINTER(_Param), C.c_int, C.POINTER(_dp), _dp,
                                                           C.POINTER(_dp), C.POINTER(_MinResult), C.POINTER(C.c_int)]
+        L.cdft_minimize_fixed_mass.argtypes = [C.c_void_p, C.POINTER(_Fire), C.POINTER(_Param), C.c_int, _dp, _dp,
+                                               C.c_double, C.c_double, _dp, C.POINTER(_MinResult), _dp]
         L.cdft_ddft_run.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp,
                                     C.POINTER(_SimResult), _dp, _dp]
         L.cdft_time_total_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
@@ -295,6 +297,27 @@ class Context:
         _check(lib().cdft_minimize_grand_potential(self._h, C.byref(fs), C.byref(prm), int(bool(homogeneous_boundary)),
                                                    _p(_f64(rho0)), None if vext is None else _p(_f64(vext)), float(mu),
                                                    _p(out), C.byref(res), _p(en)))
+        r = {k: getattr(res, k) for k, _ in _MinResult._fields_}
+        r["converged"] = bool(r["converged"])
+        return out.reshape(self.shape), r, (en[: r["iterations"] + 1] if en is not None else None)
+
+    def minimize_fixed_mass(self, rho0, mu, target_mass, fire=None, param="unbounded", rho_min=1e-18, rho_max=1.0,
+                            homogeneous_boundary=True, vext=None, max_steps=None, record_energies=True):
+        """Minimizer::fixed_mass: FIRE at fixed total mass; returns (rho, result, F after each step)."""
+        f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
+                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
+        if fire:
+            f.update(fire)
+        if max_steps is not None:
+            f["max_steps"] = max_steps
+        fs = _Fire(*[f[k] for k, _ in _Fire._fields_])
+        prm = _Param(0 if param == "unbounded" else 1, float(rho_min), float(rho_max))
+        out = np.zeros(self.N)
+        res = _MinResult()
+        en = np.full(int(f["max_steps"]) + 1, np.nan) if record_energies else None
+        _check(lib().cdft_minimize_fixed_mass(self._h, C.byref(fs), C.byref(prm), int(bool(homogeneous_boundary)),
+                                              _p(_f64(rho0)), None if vext is None else _p(_f64(vext)), float(mu),
+                                              float(target_mass), _p(out), C.byref(res), _p(en)))
         r = {k: getattr(res, k) for k, _ in _MinResult._fields_}
         r["converged"] = bool(r["converged"])
         return out.reshape(self.shape), r, (en[: r["iterations"] + 1] if en is not None else None)

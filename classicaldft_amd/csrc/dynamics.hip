@@ -95,16 +95,21 @@ __global__ void __launch_bounds__(256) k_face_sum(const double* __restrict__ g, 
 // FIRE force in the alias variable (minimization.hpp:288-302): homogeneous
 // boundary average (grid.hpp:113-131), chain rule, negation.  Accumulates
 // sum f^2 and the power sum v.f for the next step.
+// With fixed mass (minimization.hpp:207-211) the Lagrange multiplier
+// lambda = lam[0] / target (lam[0] = sum g rho after the boundary average)
+// shifts every component by -lambda dV first; lam is null otherwise.
 __global__ void __launch_bounds__(256) k_fire_force(const double* __restrict__ g, const double* __restrict__ x,
                                                     const double* __restrict__ v, double* __restrict__ f, Geom G,
                                                     int homog, const double* __restrict__ face, long nb, AliasMap a,
-                                                    double* partials) {
+                                                    const double* __restrict__ lam, double target, double* partials) {
   double s2 = 0, c2 = 0, sp = 0, cp = 0;
   const double avg = (homog && nb > 0) ? face[0] / static_cast<double>(nb) : 0.0;
+  const double shift = lam ? (lam[0] / target) * G.dv : 0.0;
   for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < G.N;
        i += static_cast<long>(gridDim.x) * blockDim.x) {
     double gi = g[i];
     if (homog && on_face(i, G)) gi = avg;
+    gi -= shift;
     const double fi = -(gi * a.deriv(x[i]));
     f[i] = fi;
     acc2(s2, c2, fi * fi);
@@ -112,6 +117,32 @@ __global__ void __launch_bounds__(256) k_fire_force(const double* __restrict__ g
   }
   const double vals[2] = {s2 + c2, sp + cp};
   block_sums(vals, 2, partials);
+}
+
+// Fixed mass (minimization.hpp:196-201): rho *= target / (sum rho dV) when the mass is positive.
+__global__ void __launch_bounds__(256) k_scale_to_mass(double* __restrict__ rho, long n,
+                                                       const double* __restrict__ sum, double target, double dv) {
+  const double mass = sum[0] * dv;
+  if (!(mass > 0.0)) return;
+  const double s = target / mass;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x)
+    rho[i] *= s;
+}
+
+// sum g rho with the boundary components of g replaced by their average (minimization.hpp:203-210)
+__global__ void __launch_bounds__(256) k_lambda_dot(const double* __restrict__ g, const double* __restrict__ rho,
+                                                    Geom G, int homog, const double* __restrict__ face, long nb,
+                                                    double* partials) {
+  double s = 0, c = 0;
+  const double avg = (homog && nb > 0) ? face[0] / static_cast<double>(nb) : 0.0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < G.N;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const double gi = (homog && on_face(i, G)) ? avg : g[i];
+    acc2(s, c, gi * rho[i]);
+  }
+  const double vals[1] = {s + c};
+  block_sums(vals, 1, partials);
 }
 
 // FIRE2 state (fire.hpp:46-58) and the convergence monitor
@@ -268,15 +299,18 @@ struct FireRun {
   int homog;
   const double* d_vext;
   FireConst k;
-  DevBuf x, v, f, part, res, ctl, en;  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum
+  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum, [24..31] mass sum, [32..39] sum g rho
+  DevBuf x, v, f, part, res, ctl, en;
   long nb;
+  bool fixed_mass = false;  // Minimizer::fixed_mass: rescale rho to `target`, mu = 0 in the loop
+  double target = 0.0;
   FireCtl* host = nullptr;  // pinned copy of the control state
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
   FireRun(Context* c_, AliasMap a, int h, const double* vx, double m, const cdft_fire& fr, bool record)
       : c(c_), amap(a), homog(h), d_vext(vx), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
-        part(8 * 8 * kVecBlocks), res(8 * 24), ctl(sizeof(FireCtl)),
+        part(8 * 8 * kVecBlocks), res(8 * 40), ctl(sizeof(FireCtl)),
         en(record ? 8 * (static_cast<size_t>(std::max(fr.max_steps, 0)) + 1) : 0) {
     k.p = fr;
     k.mu = m;
@@ -295,15 +329,35 @@ struct FireRun {
   // compute(x): Omega sums in d_res, forces f; device sums f^2 and v.f in res[0], res[1]
   void launch_compute() {
     const Geom& g = c->g;
-    c->eval_direct(x.as<double>(), d_vext, k.mu, c->d_force, amap);
+    const unsigned nbk = vec_blocks(g.N);
+    if (fixed_mass) {
+      // rho = to_density(x) rescaled to the target mass, evaluated at mu = 0 (minimization.hpp:193-206)
+      const AliasMap id{0, 0.0, 0.0};
+      hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, x.as<double>(), c->d_rho,
+                         g.N, amap);
+      launch_sum(c->d_rho, id, c->d_sum_part, g, c->stream);
+      launch_finalize(c->d_sum_part, c->nblk_sum, res.as<double>() + 24, c->stream);
+      hipLaunchKernelGGL(k_scale_to_mass, dim3(nbk), dim3(256), 0, c->stream, c->d_rho, g.N,
+                         res.as<double>() + 24, target, g.dv);
+      c->eval_direct(c->d_rho, d_vext, 0.0, c->d_force, id);
+    } else {
+      c->eval_direct(x.as<double>(), d_vext, k.mu, c->d_force, amap);
+    }
     if (homog) {
       hipLaunchKernelGGL(k_face_sum, dim3(vec_blocks(nb)), dim3(256), 0, c->stream, c->d_force, g, nb,
                          part.as<double>());
       launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(nb)), res.as<double>() + 16, c->stream);
     }
-    hipLaunchKernelGGL(k_fire_force, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, c->d_force, x.as<double>(),
-                       v.as<double>(), f.as<double>(), g, homog, res.as<double>() + 16, nb, amap, part.as<double>());
-    launch_finalize(part.as<double>(), static_cast<int>(vec_blocks(g.N)), res.as<double>(), c->stream);
+    if (fixed_mass) {
+      hipLaunchKernelGGL(k_lambda_dot, dim3(nbk), dim3(256), 0, c->stream, c->d_force, c->d_rho, g, homog,
+                         res.as<double>() + 16, nb, part.as<double>());
+      launch_finalize(part.as<double>(), static_cast<int>(nbk), res.as<double>() + 32, c->stream);
+    }
+    hipLaunchKernelGGL(k_fire_force, dim3(nbk), dim3(256), 0, c->stream, c->d_force, x.as<double>(),
+                       v.as<double>(), f.as<double>(), g, homog, res.as<double>() + 16, nb, amap,
+                       fixed_mass ? static_cast<const double*>(res.as<double>() + 32) : nullptr, target,
+                       part.as<double>());
+    launch_finalize(part.as<double>(), static_cast<int>(nbk), res.as<double>(), c->stream);
   }
 
   // one Fire::step (fire.hpp:86-165) with its compute and the monitor
@@ -365,12 +419,19 @@ struct FireRun {
   bool running() const { return host->state == FS_RUN; }
   bool uphill() const { return host->state == FS_UPHILL; }
 
-  // the final total() (minimization.hpp:339-343) is the last compute's evaluation
-  void finish(double* rho_out, cdft_min_result* result, double* energies) {
+  // The final total() (minimization.hpp:339-343) is the last compute's evaluation.  With fixed
+  // mass the final density is the last compute's rescaled one, evaluated at the chemical potential
+  // (minimization.hpp:256-266).
+  void finish(double* rho_out, cdft_min_result* result, double* energies, double mu_final = 0.0) {
     const Geom& g = c->g;
     CDFT_HIP(hipSetDevice(c->device));
-    hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, x.as<double>(), c->d_rho,
-                       g.N, amap);
+    if (fixed_mass) {
+      const AliasMap id{0, 0.0, 0.0};
+      c->eval_direct(c->d_rho, d_vext, mu_final, nullptr, id);
+    } else {
+      hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, x.as<double>(),
+                         c->d_rho, g.N, amap);
+    }
     CDFT_HIP(hipMemcpyAsync(rho_out, c->d_rho, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
     double er[8], fs[8];
     CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
@@ -381,7 +442,7 @@ struct FireRun {
                               c->stream));
     sync();
     cdft_result r{};
-    c->result_from(er, k.mu, &r);
+    c->result_from(er, fixed_mass ? mu_final : k.mu, &r);
     result->free_energy = r.free_energy;
     result->grand_potential = r.grand_potential;
     result->rms_force = std::sqrt(fs[0] / static_cast<double>(g.N));
@@ -420,6 +481,35 @@ int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire, const cd
     }
     if (run.uphill()) throw std::runtime_error("fire::step: exceeded max uphill steps");
     run.finish(rho_out, result, energies);
+  });
+}
+
+int cdft_minimize_fixed_mass(cdft_ctx* ctx, const cdft_fire* fire, const cdft_parametrization* param,
+                             int use_homogeneous_boundary, const double* rho0, const double* vext, double mu,
+                             double target_mass, double* rho_out, cdft_min_result* result, double* energies) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!fire || !param || !rho0 || !rho_out || !result) throw std::invalid_argument("null argument");
+    if (param->kind != 0 && param->kind != 1) throw std::invalid_argument("unknown parametrization");
+    if (!(target_mass > 0.0)) throw std::invalid_argument("fixed_mass: target mass must be positive");
+    Context* c = ctx->impl;
+    CDFT_HIP(hipSetDevice(c->device));
+    const Geom& g = c->g;
+    const AliasMap amap{param->kind == 0 ? 1 : 2, param->rho_min, param->rho_max - param->rho_min};
+    if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    // the loop's energy is F (mu = 0); the chemical potential enters only the final result
+    FireRun run(c, amap, use_homogeneous_boundary ? 1 : 0, vext ? c->d_vext : nullptr, 0.0, *fire,
+                energies != nullptr);
+    run.fixed_mass = true;
+    run.target = target_mass;
+    run.launch_start(rho0);
+    run.sync();
+    while (run.running()) {
+      run.launch_block();
+      run.sync();
+    }
+    if (run.uphill()) throw std::runtime_error("fire::step: exceeded max uphill steps");
+    run.finish(rho_out, result, energies, mu);
   });
 }
 

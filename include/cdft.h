@@ -186,6 +186,18 @@ CDFT_API int cdft_minimize_grand_potential(cdft_ctx* ctx, const cdft_fire* fire,
                                   int use_homogeneous_boundary, const double* rho0, const double* vext, double mu,
                                   double* rho_out, cdft_min_result* result, double* energies);
 
+/* Minimizer::fixed_mass (minimization.hpp:182-278): FIRE2 in the alias
+ * variable with the density rescaled to target_mass before every evaluation,
+ * the evaluation at mu = 0 (its Omega is F), and the Lagrange multiplier
+ * lambda = sum(g rho) / target_mass taken off the forces, g -= lambda dV, after
+ * the homogeneous-boundary average.  rho_out receives the rescaled final
+ * density; result holds its F and Omega at chemical potential mu.  energies
+ * (nullable, max_steps + 1 values) receive F after initialisation and every
+ * step. */
+CDFT_API int cdft_minimize_fixed_mass(cdft_ctx* ctx, const cdft_fire* fire, const cdft_parametrization* param,
+                             int use_homogeneous_boundary, const double* rho0, const double* vext, double mu,
+                             double target_mass, double* rho_out, cdft_min_result* result, double* energies);
+
 /* Several independent Minimizer::grand_potential runs advanced in lockstep
  * (no counterpart in the reference, which solves one state point at a time;
  * for many small state points that leave most of the device idle alone).

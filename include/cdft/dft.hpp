@@ -431,8 +431,21 @@ struct Minimizer {
   bool use_homogeneous_boundary{true};
   int log_interval{100};
 
+  // minimization.hpp:280-353
   Result grand_potential(const physics::Model& model, const functionals::Weights& weights, const vec& initial_density,
                          double chemical_potential) const {
+    return run(model, weights, initial_density, chemical_potential, -1.0);
+  }
+
+  // minimization.hpp:182-278
+  Result fixed_mass(const physics::Model& model, const functionals::Weights& weights, const vec& initial_density,
+                    double chemical_potential, double target_mass) const {
+    return run(model, weights, initial_density, chemical_potential, target_mass);
+  }
+
+ private:
+  Result run(const physics::Model& model, const functionals::Weights& weights, const vec& initial_density,
+             double chemical_potential, double target_mass) const {
     if (static_cast<long>(initial_density.size()) != model.grid.total_points())
       throw std::invalid_argument("initial density size does not match the grid");
     cdft_fire f{fire.dt,      fire.dt_max,  fire.dt_min,     fire.alpha_start,    fire.f_inc,   fire.f_dec,
@@ -451,9 +464,15 @@ struct Minimizer {
     Result r;
     r.densities.assign(1, vec(initial_density.size()));
     cdft_min_result cr{};
-    ::dft::detail::check(cdft_minimize_grand_potential(weights.ctx.get(), &f, &p, use_homogeneous_boundary ? 1 : 0,
-                                                       initial_density.data(), nullptr, chemical_potential,
-                                                       r.densities[0].data(), &cr, nullptr));
+    const int homog = use_homogeneous_boundary ? 1 : 0;
+    if (target_mass > 0.0)
+      ::dft::detail::check(cdft_minimize_fixed_mass(weights.ctx.get(), &f, &p, homog, initial_density.data(), nullptr,
+                                                    chemical_potential, target_mass, r.densities[0].data(), &cr,
+                                                    nullptr));
+    else
+      ::dft::detail::check(cdft_minimize_grand_potential(weights.ctx.get(), &f, &p, homog, initial_density.data(),
+                                                         nullptr, chemical_potential, r.densities[0].data(), &cr,
+                                                         nullptr));
     r.free_energy = cr.free_energy;
     r.grand_potential = cr.grand_potential;
     r.rms_force = cr.rms_force;

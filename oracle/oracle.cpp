@@ -813,6 +813,163 @@ struct FireState {
 
 using namespace orc;
 
+namespace {
+int minimize(Ctx* c, const double* fire_p, int param_kind, double rho_min, double rho_max,
+                    int homogeneous_boundary, const double* rho0, const double* vext, double mu, double target,
+                    double* rho_out, double* out, double* energies) {
+  const bool fixed = target > 0.0;  // Minimizer::fixed_mass; otherwise grand_potential
+  const long N = c->N;
+  FireParams fp{fire_p[0], fire_p[1], fire_p[2], fire_p[3], fire_p[4], fire_p[5], fire_p[6],
+                static_cast<int>(fire_p[7]), static_cast<int>(fire_p[8]), fire_p[9], static_cast<int>(fire_p[10])};
+  double range = rho_max - rho_min;
+  auto to_density = [&](const std::vector<double>& x, std::vector<double>& rho) {
+    for (long i = 0; i < N; ++i) {
+      double x2 = x[i] * x[i];
+      rho[i] = param_kind == 0 ? rho_min + x2 : rho_min + range * x2 / (1.0 + x2);
+    }
+  };
+  // boundary mask (grid.hpp:67-82)
+  std::vector<char> bd(static_cast<size_t>(N), 0);
+  for (long ix = 0; ix < c->nx; ++ix)
+    for (long iy = 0; iy < c->ny; ++iy)
+      for (long iz = 0; iz < c->nz; ++iz)
+        bd[iz + c->nz * (iy + c->ny * ix)] =
+            (ix == 0 || ix == c->nx - 1 || iy == 0 || iy == c->ny - 1 || iz == 0 || iz == c->nz - 1);
+  std::vector<double> rho(static_cast<size_t>(N)), grad(static_cast<size_t>(N));
+  double parts[6];
+  const double zero = 0.0;
+  // fixed mass: rho rescaled to the target, evaluated at mu = 0 (minimization.hpp:193-206)
+  auto rescale = [&]() {
+    double m = 0.0;
+    for (long i = 0; i < N; ++i) m += rho[i];
+    m *= c->dv;
+    if (m > 0.0)
+      for (long i = 0; i < N; ++i) rho[i] *= target / m;
+  };
+  auto compute = [&](const std::vector<double>& x, std::vector<double>& f) -> double {
+    to_density(x, rho);
+    if (fixed) rescale();
+    c->total(rho.data(), vext, fixed ? &zero : &mu, parts, grad.data());
+    if (homogeneous_boundary) {  // grid.hpp:113-131
+      double sum = 0.0;
+      long cnt = 0;
+      for (long i = 0; i < N; ++i)
+        if (bd[i]) { sum += grad[i]; ++cnt; }
+      if (cnt > 0) {
+        double avg = sum / static_cast<double>(cnt);
+        for (long i = 0; i < N; ++i)
+          if (bd[i]) grad[i] = avg;
+      }
+    }
+    if (fixed) {  // Lagrange multiplier of the mass (minimization.hpp:208-210)
+      double d = 0.0;
+      for (long i = 0; i < N; ++i) d += grad[i] * rho[i];
+      const double lambda = d / target;
+      for (long i = 0; i < N; ++i) grad[i] -= lambda * c->dv;
+    }
+    for (long i = 0; i < N; ++i) {  // transform_force and negate (minimization.hpp:82-98,300-301)
+      double dr;
+      if (param_kind == 0) dr = 2.0 * x[i];
+      else {
+        double x2 = x[i] * x[i];
+        dr = 2.0 * range * x[i] / ((1.0 + x2) * (1.0 + x2));
+      }
+      f[i] = -(grad[i] * dr);
+    }
+    return parts[5];
+  };
+  // from_density (minimization.hpp:63-77)
+  std::vector<double> x(static_cast<size_t>(N)), v(static_cast<size_t>(N), 0.0), f(static_cast<size_t>(N));
+  for (long i = 0; i < N; ++i) {
+    if (param_kind == 0) x[i] = std::sqrt(std::max(rho0[i] - rho_min, 0.0));
+    else {
+      double d = std::min(std::max(rho0[i] - rho_min, 0.0), range * (1.0 - 1e-14));
+      x[i] = std::sqrt(d / (range - d));
+    }
+  }
+  // Fire::initialize (fire.hpp:60-84), then the extra compute at minimization.hpp:313
+  FireState st{};
+  st.energy = compute(x, f);
+  double sf2 = 0.0;
+  for (long i = 0; i < N; ++i) sf2 += f[i] * f[i];
+  st.dt = fp.dt;
+  st.alpha = fp.alpha_start;
+  st.rms_force = std::sqrt(sf2 / static_cast<double>(N));
+  st.converged = st.rms_force < fp.force_tolerance;
+  compute(x, f);
+  if (energies) energies[0] = st.energy;
+  double volume = static_cast<double>(N) * c->dv;
+  double vv = 1.0;
+  bool converged = false;
+  int rc = 0;
+  for (int it = 0; it < fp.max_steps && !converged; ++it) {
+    double old_e = st.energy;
+    // Fire::step (fire.hpp:86-165)
+    double P = 0.0;
+    for (long i = 0; i < N; ++i) P += v[i] * f[i];
+    if (P > 0.0 || st.iteration == 0) {
+      st.n_positive++;
+      st.n_negative = 0;
+      if (st.n_positive > fp.n_delay) {
+        st.dt = std::min(st.dt * fp.f_inc, fp.dt_max);
+        st.alpha *= fp.f_alpha;
+      }
+    } else {
+      st.n_positive = 0;
+      st.n_negative++;
+      if (st.n_negative > fp.max_uphill) { rc = -1; break; }
+      for (long i = 0; i < N; ++i) {
+        x[i] -= 0.5 * st.dt * v[i];
+        v[i] = 0.0;
+      }
+      if (st.iteration > fp.n_delay) {
+        st.dt = std::max(st.dt * fp.f_dec, fp.dt_min);
+        st.alpha = fp.alpha_start;
+      }
+    }
+    for (long i = 0; i < N; ++i) v[i] += st.dt * f[i];
+    double vn = 0.0, fn = 0.0;
+    for (long i = 0; i < N; ++i) {
+      vn += v[i] * v[i];
+      fn += f[i] * f[i];
+    }
+    vn = std::sqrt(vn);
+    fn = std::sqrt(fn);
+    if (fn > 0.0) {
+      double ratio = vn / fn;
+      for (long i = 0; i < N; ++i) v[i] = (1.0 - st.alpha) * v[i] + st.alpha * ratio * f[i];
+    }
+    for (long i = 0; i < N; ++i) x[i] += st.dt * v[i];
+    st.energy = compute(x, f);
+    double s2 = 0.0;
+    for (long i = 0; i < N; ++i) s2 += f[i] * f[i];
+    st.rms_force = std::sqrt(s2 / static_cast<double>(N));
+    st.iteration++;
+    st.converged = st.rms_force < fp.force_tolerance;
+    // convergence_monitor (minimization.hpp:105-110)
+    vv = vv * 0.9 + 0.1 * (std::fabs(st.energy - old_e) / st.dt);
+    double monitor = std::fabs(vv / volume);
+    converged = monitor < fp.force_tolerance;
+    st.converged = converged;
+    if (energies) energies[it + 1] = st.energy;
+  }
+  to_density(x, rho);
+  if (fixed) rescale();  // minimization.hpp:256-260
+  std::memcpy(rho_out, rho.data(), sizeof(double) * N);
+  double fparts[6];
+  c->total(rho.data(), vext, &mu, fparts, nullptr);
+  out[0] = fparts[4];
+  out[1] = fparts[5];
+  out[2] = st.rms_force;
+  out[3] = st.iteration;
+  out[4] = st.converged ? 1.0 : 0.0;
+  out[5] = st.dt;
+  out[6] = st.alpha;
+  return rc;
+}
+
+}  // namespace
+
 extern "C" {
 
 // potential params per interaction: kind, sigma, eps, rc, alpha, split, scheme
@@ -967,138 +1124,18 @@ double orc_f(int model, int which, double eta, double* df) {
 int orc_minimize_gp(void* h, const double* fire_p, int param_kind, double rho_min, double rho_max,
                     int homogeneous_boundary, const double* rho0, const double* vext, double mu, double* rho_out,
                     double* out, double* energies) {
-  auto* c = static_cast<Ctx*>(h);
-  const long N = c->N;
-  FireParams fp{fire_p[0], fire_p[1], fire_p[2], fire_p[3], fire_p[4], fire_p[5], fire_p[6],
-                static_cast<int>(fire_p[7]), static_cast<int>(fire_p[8]), fire_p[9], static_cast<int>(fire_p[10])};
-  double range = rho_max - rho_min;
-  auto to_density = [&](const std::vector<double>& x, std::vector<double>& rho) {
-    for (long i = 0; i < N; ++i) {
-      double x2 = x[i] * x[i];
-      rho[i] = param_kind == 0 ? rho_min + x2 : rho_min + range * x2 / (1.0 + x2);
-    }
-  };
-  // boundary mask (grid.hpp:67-82)
-  std::vector<char> bd(static_cast<size_t>(N), 0);
-  for (long ix = 0; ix < c->nx; ++ix)
-    for (long iy = 0; iy < c->ny; ++iy)
-      for (long iz = 0; iz < c->nz; ++iz)
-        bd[iz + c->nz * (iy + c->ny * ix)] =
-            (ix == 0 || ix == c->nx - 1 || iy == 0 || iy == c->ny - 1 || iz == 0 || iz == c->nz - 1);
-  std::vector<double> rho(static_cast<size_t>(N)), grad(static_cast<size_t>(N));
-  double parts[6];
-  auto compute = [&](const std::vector<double>& x, std::vector<double>& f) -> double {
-    to_density(x, rho);
-    c->total(rho.data(), vext, &mu, parts, grad.data());
-    if (homogeneous_boundary) {  // grid.hpp:113-131
-      double sum = 0.0;
-      long cnt = 0;
-      for (long i = 0; i < N; ++i)
-        if (bd[i]) { sum += grad[i]; ++cnt; }
-      if (cnt > 0) {
-        double avg = sum / static_cast<double>(cnt);
-        for (long i = 0; i < N; ++i)
-          if (bd[i]) grad[i] = avg;
-      }
-    }
-    for (long i = 0; i < N; ++i) {  // transform_force and negate (minimization.hpp:82-98,300-301)
-      double dr;
-      if (param_kind == 0) dr = 2.0 * x[i];
-      else {
-        double x2 = x[i] * x[i];
-        dr = 2.0 * range * x[i] / ((1.0 + x2) * (1.0 + x2));
-      }
-      f[i] = -(grad[i] * dr);
-    }
-    return parts[5];
-  };
-  // from_density (minimization.hpp:63-77)
-  std::vector<double> x(static_cast<size_t>(N)), v(static_cast<size_t>(N), 0.0), f(static_cast<size_t>(N));
-  for (long i = 0; i < N; ++i) {
-    if (param_kind == 0) x[i] = std::sqrt(std::max(rho0[i] - rho_min, 0.0));
-    else {
-      double d = std::min(std::max(rho0[i] - rho_min, 0.0), range * (1.0 - 1e-14));
-      x[i] = std::sqrt(d / (range - d));
-    }
-  }
-  // Fire::initialize (fire.hpp:60-84), then the extra compute at minimization.hpp:313
-  FireState st{};
-  st.energy = compute(x, f);
-  double sf2 = 0.0;
-  for (long i = 0; i < N; ++i) sf2 += f[i] * f[i];
-  st.dt = fp.dt;
-  st.alpha = fp.alpha_start;
-  st.rms_force = std::sqrt(sf2 / static_cast<double>(N));
-  st.converged = st.rms_force < fp.force_tolerance;
-  compute(x, f);
-  if (energies) energies[0] = st.energy;
-  double volume = static_cast<double>(N) * c->dv;
-  double vv = 1.0;
-  bool converged = false;
-  int rc = 0;
-  for (int it = 0; it < fp.max_steps && !converged; ++it) {
-    double old_e = st.energy;
-    // Fire::step (fire.hpp:86-165)
-    double P = 0.0;
-    for (long i = 0; i < N; ++i) P += v[i] * f[i];
-    if (P > 0.0 || st.iteration == 0) {
-      st.n_positive++;
-      st.n_negative = 0;
-      if (st.n_positive > fp.n_delay) {
-        st.dt = std::min(st.dt * fp.f_inc, fp.dt_max);
-        st.alpha *= fp.f_alpha;
-      }
-    } else {
-      st.n_positive = 0;
-      st.n_negative++;
-      if (st.n_negative > fp.max_uphill) { rc = -1; break; }
-      for (long i = 0; i < N; ++i) {
-        x[i] -= 0.5 * st.dt * v[i];
-        v[i] = 0.0;
-      }
-      if (st.iteration > fp.n_delay) {
-        st.dt = std::max(st.dt * fp.f_dec, fp.dt_min);
-        st.alpha = fp.alpha_start;
-      }
-    }
-    for (long i = 0; i < N; ++i) v[i] += st.dt * f[i];
-    double vn = 0.0, fn = 0.0;
-    for (long i = 0; i < N; ++i) {
-      vn += v[i] * v[i];
-      fn += f[i] * f[i];
-    }
-    vn = std::sqrt(vn);
-    fn = std::sqrt(fn);
-    if (fn > 0.0) {
-      double ratio = vn / fn;
-      for (long i = 0; i < N; ++i) v[i] = (1.0 - st.alpha) * v[i] + st.alpha * ratio * f[i];
-    }
-    for (long i = 0; i < N; ++i) x[i] += st.dt * v[i];
-    st.energy = compute(x, f);
-    double s2 = 0.0;
-    for (long i = 0; i < N; ++i) s2 += f[i] * f[i];
-    st.rms_force = std::sqrt(s2 / static_cast<double>(N));
-    st.iteration++;
-    st.converged = st.rms_force < fp.force_tolerance;
-    // convergence_monitor (minimization.hpp:105-110)
-    vv = vv * 0.9 + 0.1 * (std::fabs(st.energy - old_e) / st.dt);
-    double monitor = std::fabs(vv / volume);
-    converged = monitor < fp.force_tolerance;
-    st.converged = converged;
-    if (energies) energies[it + 1] = st.energy;
-  }
-  to_density(x, rho);
-  std::memcpy(rho_out, rho.data(), sizeof(double) * N);
-  double fparts[6];
-  c->total(rho.data(), vext, &mu, fparts, nullptr);
-  out[0] = fparts[4];
-  out[1] = fparts[5];
-  out[2] = st.rms_force;
-  out[3] = st.iteration;
-  out[4] = st.converged ? 1.0 : 0.0;
-  out[5] = st.dt;
-  out[6] = st.alpha;
-  return rc;
+  return minimize(static_cast<Ctx*>(h), fire_p, param_kind, rho_min, rho_max, homogeneous_boundary, rho0, vext, mu,
+                  -1.0, rho_out, out, energies);
+}
+
+// Minimizer::fixed_mass (minimization.hpp:182-278): same arguments plus the
+// target mass; out[] as for orc_minimize_gp (F and Omega of the rescaled
+// final density at mu); energies receive F (mu = 0 in the loop).
+int orc_minimize_fixed_mass(void* h, const double* fire_p, int param_kind, double rho_min, double rho_max,
+                            int homogeneous_boundary, const double* rho0, const double* vext, double mu,
+                            double target_mass, double* rho_out, double* out, double* energies) {
+  return minimize(static_cast<Ctx*>(h), fire_p, param_kind, rho_min, rho_max, homogeneous_boundary, rho0, vext, mu,
+                  target_mass, rho_out, out, energies);
 }
 
 // ---------------------------------------------------------------------------

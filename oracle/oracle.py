@@ -65,6 +65,9 @@ def lib():
         L.orc_minimize_gp.restype = C.c_int
         L.orc_minimize_gp.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                       C.c_double, _dp, _dp, _dp]
+        L.orc_minimize_fixed_mass.restype = C.c_int
+        L.orc_minimize_fixed_mass.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
+                                              C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_ddft_run.restype = C.c_int
         L.orc_ddft_run.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp, _dp, _dp]
         L.orc_ddft_fixed_run.restype = C.c_double
@@ -197,6 +200,15 @@ class Oracle:
 
     def minimize_grand_potential(self, rho0, mu, fire=None, param="unbounded", rho_min=1e-18, rho_max=1.0,
                                  homogeneous_boundary=True, vext=None, max_steps=None):
+        return self._minimize(rho0, mu, -1.0, fire, param, rho_min, rho_max, homogeneous_boundary, vext, max_steps)
+
+    def minimize_fixed_mass(self, rho0, mu, target_mass, fire=None, param="unbounded", rho_min=1e-18, rho_max=1.0,
+                            homogeneous_boundary=True, vext=None, max_steps=None):
+        """Minimizer::fixed_mass; the energies are F (mu = 0 inside the loop)."""
+        return self._minimize(rho0, mu, float(target_mass), fire, param, rho_min, rho_max, homogeneous_boundary,
+                              vext, max_steps)
+
+    def _minimize(self, rho0, mu, target, fire, param, rho_min, rho_max, homogeneous_boundary, vext, max_steps):
         f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
                  n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
         if fire:
@@ -210,9 +222,14 @@ class Oracle:
         rho_out = np.zeros(self.N)
         out = np.zeros(7)
         energies = np.full(int(f["max_steps"]) + 1, np.nan)
-        rc = lib().orc_minimize_gp(self._h, _p(fp), 0 if param == "unbounded" else 1, rho_min, rho_max,
-                                   int(bool(homogeneous_boundary)), _p(rho0), _p(v), float(mu), _p(rho_out),
-                                   _p(out), _p(energies))
+        if target > 0.0:
+            rc = lib().orc_minimize_fixed_mass(self._h, _p(fp), 0 if param == "unbounded" else 1, rho_min, rho_max,
+                                               int(bool(homogeneous_boundary)), _p(rho0), _p(v), float(mu),
+                                               float(target), _p(rho_out), _p(out), _p(energies))
+        else:
+            rc = lib().orc_minimize_gp(self._h, _p(fp), 0 if param == "unbounded" else 1, rho_min, rho_max,
+                                       int(bool(homogeneous_boundary)), _p(rho0), _p(v), float(mu), _p(rho_out),
+                                       _p(out), _p(energies))
         if rc != 0:
             raise RuntimeError("fire::step: exceeded max uphill steps")
         res = dict(free_energy=out[0], grand_potential=out[1], rms_force=out[2], iterations=int(out[3]),

@@ -153,3 +153,39 @@ def test_minimizer_batch_reports_uphill_per_run():
     assert [st for _, _, st in out] == [-2, -2]
     with pytest.raises(ValueError):
         cd.minimize_grand_potential_batch([ctxs[0], ctxs[0]], [rho0, rho0], [mu, mu], fire=fire, max_steps=5)
+
+
+@pytest.mark.parametrize("homog,param", [(True, "unbounded"), (False, "bounded")])
+def test_fixed_mass_trajectory_fp64(homog, param):
+    shape, dx = (16, 16, 16), 0.25
+    ctx, orc = _pair(shape, dx)
+    rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    target = 0.9 * float(np.sum(rho0)) * dx ** 3
+    mu = orc.bulk_mu(0.05)
+    fire = dict(force_tolerance=1e-9)
+    kw = dict(param=param, rho_max=1.2, homogeneous_boundary=homog, max_steps=60)
+    rg, resg, eg = ctx.minimize_fixed_mass(rho0, mu, target, fire=fire, **kw)
+    ro, reso, eo = orc.minimize_fixed_mass(rho0, mu, target, fire=fire, **kw)
+    assert resg["iterations"] == reso["iterations"]
+    np.testing.assert_allclose(eg, eo, rtol=1e-9, atol=1e-11)
+    e = np.max(np.abs(rg - ro)) / np.max(np.abs(ro))
+    print(f"fixed mass homog={homog} {param}: profile rel err {e:.2e}", flush=True)
+    assert e < 1e-9
+    assert float(np.sum(rg)) * dx ** 3 == pytest.approx(target, rel=1e-12)
+    assert resg["free_energy"] == pytest.approx(reso["free_energy"], rel=1e-9)
+    assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-9)
+
+
+def test_fixed_mass_mixed_tracks_fp64():
+    shape, dx = (64, 64, 64), 0.1
+    rho0 = W.droplet(shape, dx, radius=2.0, width=0.5, rho_l=0.7, rho_v=0.02, noise=1e-3, seed=9)
+    target = float(np.sum(rho0)) * dx ** 3
+    out = {}
+    for prec in ("fp64", "mixed"):
+        ctx, orc = _pair(shape, dx, precision=prec)
+        mu = ctx.bulk_mu(0.02)
+        out[prec] = ctx.minimize_fixed_mass(rho0, mu, target, fire=dict(force_tolerance=1e-12), max_steps=30)
+    (r64, s64, e64), (rmx, smx, emx) = out["fp64"], out["mixed"]
+    assert float(np.sum(rmx)) * dx ** 3 == pytest.approx(target, rel=1e-12)
+    assert smx["free_energy"] == pytest.approx(s64["free_energy"], rel=1e-6)
+    assert np.max(np.abs(rmx - r64)) / np.max(np.abs(r64)) < 1e-4

@@ -51,6 +51,12 @@ def test_cpp_facade_matches_oracle(tmp_path):
     assert out["fire_grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-9)
     assert _rel(np.fromfile(tmp_path / "fire_rho.bin").reshape(shape), ro) < 1e-9
 
+    rf, resf, _ = orc.minimize_fixed_mass(rho, mu, out["fixed_mass_target"], fire=dict(force_tolerance=1e-9),
+                                          max_steps=40)
+    assert out["fixed_mass_iterations"] == resf["iterations"]
+    assert out["fixed_mass_free_energy"] == pytest.approx(resf["free_energy"], rel=1e-9)
+    assert _rel(np.fromfile(tmp_path / "fixed_mass_rho.bin").reshape(shape), rf) < 1e-9
+
     rd, resd, ed = orc.ddft_run(rho, mu, 6, dt=1e-3, dt_max=1e-3)
     assert _rel(np.fromfile(tmp_path / "ddft_rho.bin").reshape(shape), rd) < 1e-9
     # recorded at step 0 and every log_interval = 2 steps, as Simulation::run does

@@ -560,3 +560,41 @@ def test_fixed_ddft_converges_to_reference_step():
     assert res["dt"] == dt
     got, _ = orc.ddft_fixed_run(rho0, mu, 3, dt, 40, vext=V)
     assert np.max(np.abs(got - ref)) / np.max(np.abs(ref)) < 1e-10
+
+
+# ------------------------------------------------------------------ Minimizer::fixed_mass
+def test_fixed_mass_minimizer_keeps_mass_and_lowers_F():
+    from classicaldft_amd import workloads as W
+
+    shape, dx = (12, 12, 12), 0.25
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    rho0 = W.droplet(shape, dx, radius=1.0, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    target = 0.9 * float(np.sum(rho0)) * dx ** 3
+    mu = orc.bulk_mu(0.05)
+    rho, res, en = orc.minimize_fixed_mass(rho0, mu, target, fire=dict(force_tolerance=1e-12), max_steps=80)
+    assert np.sum(rho) * dx ** 3 == pytest.approx(target, rel=1e-13)
+    # energies are F (the loop runs at mu = 0); the final result is at mu
+    parts, _ = orc.total(rho, mu)
+    assert res["free_energy"] == pytest.approx(parts[4], rel=1e-12)
+    assert res["grand_potential"] == pytest.approx(parts[4] - mu * target, rel=1e-10)
+    assert en[-1] < en[0]
+
+
+def test_fixed_mass_converges_to_a_stationary_profile():
+    # Euler-Lagrange condition of the constrained problem: at convergence
+    # dF/drho is uniform and equals the multiplier lambda = sum(g rho) / N
+    from classicaldft_amd import workloads as W
+
+    shape, dx = (12, 12, 12), 0.25
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=1.2, model="WhiteBearII")
+    rho0 = 0.05 * (1.0 + 0.05 * (2.0 * W.uniform(9, 12 ** 3).reshape(shape) - 1.0))
+    target = 0.05 * 12 ** 3 * dx ** 3
+    rho, res, _ = orc.minimize_fixed_mass(rho0, 0.0, target, fire=dict(force_tolerance=1e-14, dt=0.02, dt_max=0.2),
+                                          max_steps=3000, homogeneous_boundary=False)
+    _, g = orc.total(rho, 0.0)
+    lam = float(np.sum(g * rho)) / target
+    dev = float(np.max(np.abs(g / dx ** 3 - lam)))
+    print(f"fixed mass: {res['iterations']} steps, max |dF/drho - lambda| = {dev:.2e}, lambda = {lam:.6f}")
+    assert dev < 1e-6 * abs(lam)
+    # lambda is the bulk chemical potential of the uniform fluid at the mean density
+    assert lam == pytest.approx(orc.bulk_mu(0.05), rel=1e-6)
