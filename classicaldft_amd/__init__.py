@@ -28,7 +28,7 @@ EXPORTED = [
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
-    "cdft_ddft_adjoint_wells", "cdft_autotune", "cdft_minimize_fixed_mass",
+    "cdft_ddft_adjoint_wells", "cdft_autotune", "cdft_minimize_fixed_mass", "cdft_smallest_eigenpair",
 ]
 
 
@@ -88,6 +88,16 @@ class _SimResult(C.Structure):
                 ("energy", C.c_double), ("dt", C.c_double), ("evaluations", C.c_long)]
 
 
+class _EigenConfig(C.Structure):
+    _fields_ = [("tolerance", C.c_double), ("max_iterations", C.c_int), ("hessian_eps", C.c_double),
+                ("fixed_boundary", C.c_int)]
+
+
+class _EigenResult(C.Structure):
+    _fields_ = [("eigenvalue", C.c_double), ("iterations", C.c_int), ("converged", C.c_int),
+                ("hessian_products", C.c_int)]
+
+
 class _AdjointConfig(C.Structure):
     _fields_ = [("dt", C.c_double), ("fp_iterations", C.c_int), ("n_steps", C.c_int),
                 ("checkpoint_interval", C.c_int)]
@@ -142,6 +152,8 @@ def lib():
         L.cdft_stage_name.argtypes = [C.c_void_p, C.c_int]
         L.cdft_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.cdft_autotune.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.cdft_smallest_eigenpair.argtypes = [C.c_void_p, C.POINTER(_EigenConfig), _dp, _dp, C.c_double, _dp,
+                                              C.c_ulonglong, _dp, C.POINTER(_EigenResult)]
         L.cdft_hessian_vector.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.cdft_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.cdft_ddft_fixed_run.argtypes = [C.c_void_p, C.POINTER(_AdjointConfig), _dp, C.c_double, _dp, _dp]
@@ -333,6 +345,21 @@ class Context:
                                    float(mu), _p(rho), C.byref(res), _p(en), None))
         r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
         return rho.reshape(self.shape), r, en
+
+    def smallest_eigenpair(self, rho, mu, initial_guess=None, seed=0, tolerance=1e-4, max_iterations=300,
+                           hessian_eps=1e-6, fixed_boundary=False, vext=None):
+        """EigenvalueSolver::solve: smallest eigenpair of the grand-potential Hessian (hessian_eps = 0: exact
+        products); returns (eigenvector, result dict)."""
+        cfg = _EigenConfig(float(tolerance), int(max_iterations), float(hessian_eps), int(bool(fixed_boundary)))
+        res = _EigenResult()
+        vec = np.zeros(self.N)
+        _check(lib().cdft_smallest_eigenpair(self._h, C.byref(cfg), _p(_f64(rho)),
+                                             None if vext is None else _p(_f64(vext)), float(mu),
+                                             None if initial_guess is None else _p(_f64(initial_guess)),
+                                             int(seed), _p(vec), C.byref(res)))
+        r = {k: getattr(res, k) for k, _ in _EigenResult._fields_}
+        r["converged"] = bool(r["converged"])
+        return vec.reshape(self.shape), r
 
     def autotune(self, reps=3):
         """Tune the general schedule's FFT radix orders and launch shapes for this grid; returns the time of one

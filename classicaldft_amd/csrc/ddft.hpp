@@ -19,6 +19,15 @@ namespace cdft {
 // that includes this header gets its own copy.
 namespace {
 
+// Boundary mask of the grid (grid.hpp:67-82): a point on any face of the box.
+__device__ __forceinline__ bool on_face(long i, const Geom& g) {
+  const long iz = i % g.nz;
+  const long t = i / g.nz;
+  const long iy = t % g.ny;
+  const long ix = t / g.ny;
+  return ix == 0 || ix == g.nx - 1 || iy == 0 || iy == g.ny - 1 || iz == 0 || iz == g.nz - 1;
+}
+
 __device__ __forceinline__ void acc2(double& s, double& c, double x) {
   const double t = s + x;
   c += (fabs(s) >= fabs(x)) ? ((s - t) + x) : ((x - t) + s);

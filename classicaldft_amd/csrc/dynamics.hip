@@ -30,14 +30,6 @@ namespace cdft {
 
 namespace {
 
-__device__ __forceinline__ bool on_face(long i, const Geom& g) {
-  const long iz = i % g.nz;
-  const long t = i / g.nz;
-  const long iy = t % g.ny;
-  const long ix = t / g.ny;
-  return ix == 0 || ix == g.nx - 1 || iy == 0 || iy == g.ny - 1 || iz == 0 || iz == g.nz - 1;
-}
-
 // from_density (minimization.hpp:63-77)
 __global__ void k_from_density(const double* __restrict__ rho, double* __restrict__ x, long n, AliasMap a) {
   const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;

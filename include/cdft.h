@@ -246,6 +246,29 @@ CDFT_API int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_
  * microseconds; a fused-schedule context is timed and left as it is. */
 CDFT_API int cdft_autotune(cdft_ctx* ctx, int reps, double* us_default, double* us_tuned);
 
+/* saddle_point::EigenvalueSolver (include/dft/algorithms/saddle_point.hpp:136-155). */
+typedef struct {
+  double tolerance;     /* residual norm |Hv - lambda v| to stop at */
+  int max_iterations;
+  double hessian_eps;   /* > 0: forward-difference products; 0: exact products (FP64 contexts) */
+  int fixed_boundary;   /* zero the boundary forces, as Search::find's force function does */
+} cdft_eigen_config;
+
+typedef struct {
+  double eigenvalue;
+  int iterations, converged, hessian_products;
+} cdft_eigen_result;
+
+/* EigenvalueSolver::solve (saddle_point.hpp:159-270): the smallest eigenpair
+ * of the Hessian of the grand potential at rho (forces of cdft_total at mu,
+ * vext nullable) by LOBPCG.  initial_guess (nullable, N values) is the start;
+ * without it the start is the seeded standard-normal sequence of `seed`
+ * (SplitMix64 + Box-Muller, standing in for arma::randn).  eigenvector
+ * receives the normalised eigenvector (N values). */
+CDFT_API int cdft_smallest_eigenpair(cdft_ctx* ctx, const cdft_eigen_config* config, const double* rho,
+                            const double* vext, double mu, const double* initial_guess, unsigned long long seed,
+                            double* eigenvector, cdft_eigen_result* result);
+
 /* ---------------------------------------------------------------- adjoint
  * The reference has no derivative of its dynamics; the entry points below
  * extend its DDFT integrating-factor step (dynamics.hpp:333-401) with a

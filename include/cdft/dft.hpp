@@ -483,6 +483,47 @@ struct Minimizer {
 };
 }  // namespace minimization
 
+namespace saddle_point {
+// EigenvalueResult (saddle_point.hpp:147-152)
+struct EigenvalueResult {
+  vec eigenvector;
+  double eigenvalue{0.0};
+  int iterations{0};
+  bool converged{false};
+};
+
+// EigenvalueSolver (saddle_point.hpp:154-270).  The reference's solve() takes
+// any ForceFunction; on the device the force function is the functional's own
+// grand-potential gradient at mu (with the boundary forces zeroed when
+// fixed_boundary, as Search::find does), so solve() takes the weights and mu.
+// hessian_eps = 0 selects exact Hessian-vector products (FP64 functionals).
+struct EigenvalueSolver {
+  double tolerance{1e-4};
+  int max_iterations{300};
+  double hessian_eps{1e-6};
+  int log_interval{0};
+  bool fixed_boundary{false};
+  unsigned long long seed{0};  // start vector when no initial guess is given (stands in for arma::randn)
+
+  EigenvalueResult solve(const functionals::Weights& weights, double chemical_potential, const vec& rho,
+                         const vec& initial_guess = {}) const {
+    if (static_cast<long>(rho.size()) != weights.n_points)
+      throw std::invalid_argument("density size does not match the grid");
+    cdft_eigen_config cfg{tolerance, max_iterations, hessian_eps, fixed_boundary ? 1 : 0};
+    cdft_eigen_result cr{};
+    EigenvalueResult r;
+    r.eigenvector.assign(rho.size(), 0.0);
+    const double* guess = initial_guess.size() == rho.size() ? initial_guess.data() : nullptr;
+    ::dft::detail::check(cdft_smallest_eigenpair(weights.ctx.get(), &cfg, rho.data(), nullptr, chemical_potential,
+                                                 guess, seed, r.eigenvector.data(), &cr));
+    r.eigenvalue = cr.eigenvalue;
+    r.iterations = cr.iterations;
+    r.converged = cr.converged != 0;
+    return r;
+  }
+};
+}  // namespace saddle_point
+
 namespace dynamics {
 // dynamics.hpp:26-33
 struct StepConfig {

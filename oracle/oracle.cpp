@@ -22,6 +22,7 @@
 //   Minimizer::grand_potential ......... include/dft/algorithms/minimization.hpp:45-110,280-353
 //   DDFT integrating-factor step ....... include/dft/algorithms/dynamics.hpp:79-401,582-693
 //   fixed-dt DDFT map + Gaussian wells . (no counterpart: the map the adjoint differentiates)
+//   smallest Hessian eigenpair ......... include/dft/algorithms/saddle_point.hpp:28-38,159-270
 //
 // Where the reference's output depends on FFTW behaviour it does not document
 // (c2r of a half spectrum that is not Hermitian, which happens for the odd
@@ -1332,6 +1333,132 @@ int orc_ddft_run(void* h, const double* cfg_in, int n_steps, const double* vext,
   out[4] = e;
   out[5] = static_cast<double>(w.evals);
   return 0;
+}
+
+// EigenvalueSolver::solve (saddle_point.hpp:159-270): LOBPCG for the
+// smallest eigenpair of the grand-potential Hessian with the forward
+// difference products of saddle_point.hpp:28-38 (forces of total() at mu;
+// boundary forces zeroed when `fixed`, grid.hpp:135-142).  The projected
+// 2x2 / 3x3 problem is solved by Jacobi rotations, eigenvector sign c0 >= 0
+// (arma::eig_sym's sign is unspecified).  v0: start (normalised here).
+// out: eigenvalue, iterations, converged.
+void orc_smallest_eigenpair(void* h, double tol, int max_it, double eps, int fixed, const double* rho_in,
+                            const double* vext, double mu, const double* v0, double* v_out, double* out) {
+  auto* c = static_cast<Ctx*>(h);
+  const long N = c->N;
+  std::vector<char> bd(static_cast<size_t>(N), 0);
+  for (long ix = 0; ix < c->nx; ++ix)
+    for (long iy = 0; iy < c->ny; ++iy)
+      for (long iz = 0; iz < c->nz; ++iz)
+        bd[iz + c->nz * (iy + c->ny * ix)] =
+            (ix == 0 || ix == c->nx - 1 || iy == 0 || iy == c->ny - 1 || iz == 0 || iz == c->nz - 1);
+  using V = std::vector<double>;
+  const V rho(rho_in, rho_in + N);
+  double parts[6];
+  auto force = [&](const V& r, V& f) {
+    c->total(r.data(), vext, &mu, parts, f.data());
+    if (fixed)
+      for (long i = 0; i < N; ++i)
+        if (bd[i]) f[i] = 0.0;
+  };
+  V f0(N), f1(N), tmp(N);
+  force(rho, f0);
+  auto hvp = [&](const V& x, V& out) {
+    for (long i = 0; i < N; ++i) tmp[i] = rho[i] + eps * x[i];
+    force(tmp, f1);
+    for (long i = 0; i < N; ++i) out[i] = (f1[i] - f0[i]) / eps;
+  };
+  auto dot = [&](const V& a, const V& b) {
+    double s = 0.0;
+    for (long i = 0; i < N; ++i) s += a[i] * b[i];
+    return s;
+  };
+  auto eig = [&](int n, double A[3][3], double* cv) {  // Jacobi rotations
+    double W[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 64; ++sweep) {
+      double off = 0.0, dg = 0.0;
+      for (int i = 0; i < n; ++i) {
+        dg += A[i][i] * A[i][i];
+        for (int j = i + 1; j < n; ++j) off += A[i][j] * A[i][j];
+      }
+      if (off <= 1e-32 * dg || off == 0.0) break;
+      for (int p = 0; p < n; ++p)
+        for (int q = p + 1; q < n; ++q) {
+          if (A[p][q] == 0.0) continue;
+          double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+          double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+          double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+          for (int k = 0; k < n; ++k) {
+            double a = A[k][p], b = A[k][q];
+            A[k][p] = cs * a - sn * b;
+            A[k][q] = sn * a + cs * b;
+          }
+          for (int k = 0; k < n; ++k) {
+            double a = A[p][k], b = A[q][k];
+            A[p][k] = cs * a - sn * b;
+            A[q][k] = sn * a + cs * b;
+          }
+          for (int k = 0; k < n; ++k) {
+            double a = W[k][p], b = W[k][q];
+            W[k][p] = cs * a - sn * b;
+            W[k][q] = sn * a + cs * b;
+          }
+        }
+    }
+    int m = 0;
+    for (int i = 1; i < n; ++i)
+      if (A[i][i] < A[m][m]) m = i;
+    double sg = W[0][m] < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) cv[i] = sg * W[i][m];
+  };
+  V v(v0, v0 + N), hv(N), r(N), q1(N), hq1(N), q2(N), hq2(N), p, vnew(N);
+  double nv = std::sqrt(dot(v, v));
+  for (double& x : v) x /= nv;
+  hvp(v, hv);
+  double lambda = dot(v, hv);
+  bool conv = false;
+  int it = 0;
+  for (; it < max_it; ++it) {
+    for (long i = 0; i < N; ++i) r[i] = hv[i] - lambda * v[i];
+    if (std::sqrt(dot(r, r)) < tol) { conv = true; break; }
+    double rq = dot(r, v);
+    for (long i = 0; i < N; ++i) q1[i] = r[i] - rq * v[i];
+    double q1n = std::sqrt(dot(q1, q1));
+    if (q1n < 1e-30) break;
+    for (double& x : q1) x /= q1n;
+    hvp(q1, hq1);
+    int dim = 2;
+    if (!p.empty()) {
+      double a = dot(p, v), b = dot(p, q1);
+      for (long i = 0; i < N; ++i) q2[i] = p[i] - a * v[i] - b * q1[i];
+      double q2n = std::sqrt(dot(q2, q2));
+      if (q2n > 1e-10) {
+        for (double& x : q2) x /= q2n;
+        hvp(q2, hq2);
+        dim = 3;
+      }
+    }
+    double H[3][3] = {{dot(v, hv), dot(v, hq1), 0}, {0, dot(q1, hq1), 0}, {0, 0, 0}};
+    H[1][0] = H[0][1];
+    if (dim == 3) {
+      H[0][2] = H[2][0] = dot(v, hq2);
+      H[1][2] = H[2][1] = dot(q1, hq2);
+      H[2][2] = dot(q2, hq2);
+    }
+    double cv[3] = {0, 0, 0};
+    eig(dim, H, cv);
+    for (long i = 0; i < N; ++i) vnew[i] = cv[0] * v[i] + cv[1] * q1[i] + (dim == 3 ? cv[2] * q2[i] : 0.0);
+    p.resize(N);
+    for (long i = 0; i < N; ++i) p[i] = vnew[i] - v[i];
+    double vn = std::sqrt(dot(vnew, vnew));
+    for (long i = 0; i < N; ++i) v[i] = vnew[i] / vn;
+    hvp(v, hv);
+    lambda = dot(v, hv);
+  }
+  std::memcpy(v_out, v.data(), sizeof(double) * N);
+  out[0] = lambda;
+  out[1] = it;
+  out[2] = conv ? 1.0 : 0.0;
 }
 
 // Fixed-dt forward map of the DDFT adjoint: n_steps steps of fixed_step.

@@ -73,6 +73,8 @@ def lib():
         L.orc_ddft_fixed_run.restype = C.c_double
         L.orc_ddft_fixed_run.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, _dp, C.c_double, _dp]
         L.orc_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.orc_smallest_eigenpair.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp,
+                                             C.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -255,6 +257,19 @@ class Oracle:
         mn = lib().orc_ddft_fixed_run(self._h, float(dt), int(fp_iterations), int(n_steps), _p(v), float(mu),
                                       _p(rho))
         return rho.reshape(self.shape), mn
+
+    def smallest_eigenpair(self, rho, mu, v0, tolerance=1e-4, max_iterations=300, hessian_eps=1e-6,
+                           fixed_boundary=False, vext=None):
+        """EigenvalueSolver::solve with forward-difference products; returns (eigenvector, eigenvalue, iterations,
+        converged)."""
+        r = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
+        v = np.ascontiguousarray(v0, dtype=np.float64).reshape(-1)
+        ve = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
+        vo = np.zeros(self.N)
+        out = np.zeros(3)
+        lib().orc_smallest_eigenpair(self._h, float(tolerance), int(max_iterations), float(hessian_eps),
+                                     int(bool(fixed_boundary)), _p(r), _p(ve), float(mu), _p(v), _p(vo), _p(out))
+        return vo.reshape(self.shape), out[0], int(out[1]), bool(out[2])
 
     def wells_potential(self, params):
         prm = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 5)

@@ -115,6 +115,10 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   mass0 *= model.grid.cell_volume();
   const auto fres = mz.fixed_mass(model, func.weights, rho, mu, 0.9 * mass0);
   write_doubles(out + "/fixed_mass_rho.bin", fres.densities[0]);
+  algorithms::saddle_point::EigenvalueSolver es;
+  es.tolerance = 1e-6;
+  es.seed = 17;
+  const auto eres = es.solve(func.weights, mu, rho);
 
   algorithms::dynamics::Simulation sim;
   sim.step.dt = 1e-3;
@@ -137,9 +141,10 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   std::printf("{\"mu\": %.17g, \"free_energy\": %.17g, \"grand_potential\": %.17g, \"callback_omega\": %.17g, "
               "\"callback_force_maxdiff\": %.3g, \"fire_iterations\": %d, \"fire_grand_potential\": %.17g, "
               "\"fixed_mass_target\": %.17g, \"fixed_mass_iterations\": %d, \"fixed_mass_free_energy\": %.17g, "
+              "\"eigenvalue\": %.17g, \"eigen_iterations\": %d, "
               "\"ddft_times\": [",
               mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential,
-              0.9 * mass0, fres.iterations, fres.free_energy);
+              0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations);
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);

@@ -57,6 +57,11 @@ def test_cpp_facade_matches_oracle(tmp_path):
     assert out["fixed_mass_free_energy"] == pytest.approx(resf["free_energy"], rel=1e-9)
     assert _rel(np.fromfile(tmp_path / "fixed_mass_rho.bin").reshape(shape), rf) < 1e-9
 
+    # EigenvalueSolver defaults (forward differences, eps 1e-6) from the seeded start
+    _, lam, it, _ = orc.smallest_eigenpair(rho, mu, W.normal(17, rho.size), tolerance=1e-6)
+    assert out["eigen_iterations"] == it
+    assert out["eigenvalue"] == pytest.approx(lam, rel=1e-6)
+
     rd, resd, ed = orc.ddft_run(rho, mu, 6, dt=1e-3, dt_max=1e-3)
     assert _rel(np.fromfile(tmp_path / "ddft_rho.bin").reshape(shape), rd) < 1e-9
     # recorded at step 0 and every log_interval = 2 steps, as Simulation::run does

@@ -598,3 +598,31 @@ def test_fixed_mass_converges_to_a_stationary_profile():
     assert dev < 1e-6 * abs(lam)
     # lambda is the bulk chemical potential of the uniform fluid at the mean density
     assert lam == pytest.approx(orc.bulk_mu(0.05), rel=1e-6)
+
+
+# ------------------------------------------------------------------ EigenvalueSolver (saddle_point.hpp)
+def test_eigen_solver_matches_dense_hessian():
+    # LOBPCG with forward-difference products against the lowest eigenvalue of
+    # the dense (symmetrised) finite-difference Hessian of the same forces
+    from classicaldft_amd import workloads as W
+
+    shape, dx = (8, 8, 8), 0.3
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.9, model="WhiteBearII")
+    rho = W.droplet(shape, dx, radius=0.7, width=0.3, rho_l=0.5, rho_v=0.05, noise=1e-3, seed=2)
+    mu = orc.bulk_mu(0.05)
+    n = rho.size
+    eps = 1e-6
+    _, f0 = orc.total(rho, mu)
+    H = np.zeros((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+        H[:, j] = ((orc.total(rho + eps * e.reshape(shape), mu)[1] - f0) / eps).ravel()
+    lam_dense = np.linalg.eigvalsh(0.5 * (H + H.T))[0]
+    v0 = W.normal(5, n)
+    v, lam, it, conv = orc.smallest_eigenpair(rho, mu, v0, tolerance=1e-7, max_iterations=2000, hessian_eps=eps)
+    print(f"eigen: LOBPCG {lam:.10e} in {it} iterations, dense {lam_dense:.10e}")
+    assert conv
+    assert lam == pytest.approx(lam_dense, rel=1e-6, abs=1e-9)
+    hv = 0.5 * (H + H.T) @ v.ravel()
+    assert np.linalg.norm(hv - lam * v.ravel()) < 1e-5 * abs(lam_dense) + 1e-8
