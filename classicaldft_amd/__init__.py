@@ -29,6 +29,7 @@ EXPORTED = [
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
     "cdft_ddft_adjoint_wells", "cdft_autotune", "cdft_minimize_fixed_mass", "cdft_smallest_eigenpair",
+    "cdft_saddle_search",
 ]
 
 
@@ -98,6 +99,17 @@ class _EigenResult(C.Structure):
                 ("hessian_products", C.c_int)]
 
 
+class _SaddleConfig(C.Structure):
+    _fields_ = [("fire", _Fire), ("eigen", _EigenConfig), ("param", _Param), ("omega_tolerance", C.c_double),
+                ("tolerance_start", C.c_double), ("tolerance_factor", C.c_double), ("max_outer_iterations", C.c_int),
+                ("eigenvalue_clear_count", C.c_int)]
+
+
+class _SaddleResult(C.Structure):
+    _fields_ = [("grand_potential", C.c_double), ("eigenvalue", C.c_double), ("outer_iterations", C.c_int),
+                ("converged", C.c_int), ("evaluations", C.c_long)]
+
+
 class _AdjointConfig(C.Structure):
     _fields_ = [("dt", C.c_double), ("fp_iterations", C.c_int), ("n_steps", C.c_int),
                 ("checkpoint_interval", C.c_int)]
@@ -152,6 +164,8 @@ def lib():
         L.cdft_stage_name.argtypes = [C.c_void_p, C.c_int]
         L.cdft_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.cdft_autotune.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.cdft_saddle_search.argtypes = [C.c_void_p, C.POINTER(_SaddleConfig), _dp, _dp, C.c_double, C.c_double,
+                                         C.c_ulonglong, _dp, _dp, C.POINTER(_SaddleResult)]
         L.cdft_smallest_eigenpair.argtypes = [C.c_void_p, C.POINTER(_EigenConfig), _dp, _dp, C.c_double, _dp,
                                               C.c_ulonglong, _dp, C.POINTER(_EigenResult)]
         L.cdft_hessian_vector.argtypes = [C.c_void_p, _dp, _dp, _dp]
@@ -360,6 +374,28 @@ class Context:
         r = {k: getattr(res, k) for k, _ in _EigenResult._fields_}
         r["converged"] = bool(r["converged"])
         return vec.reshape(self.shape), r
+
+    def saddle_search(self, rho0, mu, target_mass, seed=0, fire=None, eigen=None, param="unbounded", rho_min=1e-18,
+                      rho_max=1.0, omega_tolerance=1e-5, tolerance_start=1e-3, tolerance_factor=10.0,
+                      max_outer_iterations=20, eigenvalue_clear_count=2, vext=None):
+        """saddle_point::Search::find (critical cluster at fixed mass); returns (rho, eigenvector, result dict)."""
+        f = dict(dt=1e-3, dt_max=1.0, dt_min=1e-8, alpha_start=0.01, f_inc=1.1, f_dec=0.5, f_alpha=1.0, n_delay=5,
+                 max_uphill=20, force_tolerance=1e-4, max_steps=100000)
+        f.update(fire or {})
+        e = dict(tolerance=1e-4, max_iterations=300, hessian_eps=1e-6)
+        e.update(eigen or {})
+        cfg = _SaddleConfig(_Fire(*[f[k] for k, _ in _Fire._fields_]),
+                            _EigenConfig(float(e["tolerance"]), int(e["max_iterations"]), float(e["hessian_eps"]), 1),
+                            _Param(0 if param == "unbounded" else 1, float(rho_min), float(rho_max)),
+                            float(omega_tolerance), float(tolerance_start), float(tolerance_factor),
+                            int(max_outer_iterations), int(eigenvalue_clear_count))
+        res = _SaddleResult()
+        ro, eo = np.zeros(self.N), np.zeros(self.N)
+        _check(lib().cdft_saddle_search(self._h, C.byref(cfg), _p(_f64(rho0)), None if vext is None else _p(_f64(vext)),
+                                        float(mu), float(target_mass), int(seed), _p(ro), _p(eo), C.byref(res)))
+        r = {k: getattr(res, k) for k, _ in _SaddleResult._fields_}
+        r["converged"] = bool(r["converged"])
+        return ro.reshape(self.shape), eo.reshape(self.shape), r
 
     def autotune(self, reps=3):
         """Tune the general schedule's FFT radix orders and launch shapes for this grid; returns the time of one

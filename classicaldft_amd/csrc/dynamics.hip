@@ -25,6 +25,7 @@
 #include "api_guard.hpp"
 #include "context.hpp"
 #include "ddft.hpp"
+#include "eigen.hpp"
 
 namespace cdft {
 
@@ -137,6 +138,82 @@ __global__ void __launch_bounds__(256) k_lambda_dot(const double* __restrict__ g
   block_sums(vals, 1, partials);
 }
 
+// ---- Search::find (saddle_point.hpp:367-589): interior/boundary sums, the
+// interior mass rescale, and the force of the eigenvector-following FIRE run.
+
+// sum over interior (face = 0) or boundary (face = 1) points of a (times b when given)
+__global__ void __launch_bounds__(256) k_region_sum(const double* __restrict__ a, const double* __restrict__ b,
+                                                    Geom G, int face, double* partials) {
+  double s = 0, c = 0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < G.N;
+       i += static_cast<long>(gridDim.x) * blockDim.x)
+    if (on_face(i, G) == (face != 0)) acc2(s, c, b ? a[i] * b[i] : a[i]);
+  const double vals[1] = {s + c};
+  block_sums(vals, 1, partials);
+}
+
+// saddle_point.hpp:451-461: interior densities scaled to the interior target mass
+__global__ void __launch_bounds__(256) k_scale_interior(double* __restrict__ rho, const double* __restrict__ sum,
+                                                        double mt, Geom G) {
+  const double m = sum[0] * G.dv;
+  if (!(m > 0.0 && mt > 0.0)) return;
+  const double s = mt / m;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < G.N;
+       i += static_cast<long>(gridDim.x) * blockDim.x)
+    if (!on_face(i, G)) rho[i] *= s;
+}
+
+// saddle_point.hpp:463-484: fixed boundary, interior Lagrange multiplier
+// mu_L = sum_int g rho / mt, alias transform; f = the alias force, and the
+// partial sums of dir . f
+__global__ void __launch_bounds__(256) k_saddle_alias(const double* __restrict__ g, const double* __restrict__ x,
+                                                      AliasMap a, const double* __restrict__ lsum, double mt,
+                                                      const double* __restrict__ dir, double* __restrict__ f,
+                                                      Geom G, double* partials) {
+  const double muL = mt > 0.0 ? lsum[0] / mt : lsum[0];
+  double s = 0, c = 0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < G.N;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const double gi = on_face(i, G) ? 0.0 : g[i] - muL * G.dv;
+    const double fa = gi * a.deriv(x[i]);
+    f[i] = fa;
+    acc2(s, c, dir[i] * fa);
+  }
+  const double vals[1] = {s + c};
+  block_sums(vals, 1, partials);
+}
+
+// saddle_point.hpp:486-489: f <- -(f - (dir . f) dir); sums f^2 and v.f as k_fire_force does
+__global__ void __launch_bounds__(256) k_saddle_project(double* __restrict__ f, const double* __restrict__ dir,
+                                                        const double* __restrict__ psum, const double* __restrict__ v,
+                                                        long n, double* partials) {
+  const double pr = psum[0];
+  double s2 = 0, c2 = 0, sp = 0, cp = 0;
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x) {
+    const double fi = -(f[i] - pr * dir[i]);
+    f[i] = fi;
+    acc2(s2, c2, fi * fi);
+    acc2(sp, cp, v[i] * fi);
+  }
+  const double vals[2] = {s2 + c2, sp + cp};
+  block_sums(vals, 2, partials);
+}
+
+// dir = e * d rho / d x (the alias transform of the eigenvector, saddle_point.hpp:429-431)
+__global__ void __launch_bounds__(256) k_alias_dir(const double* __restrict__ e, const double* __restrict__ x,
+                                                   AliasMap a, double* __restrict__ dir, long n) {
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x)
+    dir[i] = e[i] * a.deriv(x[i]);
+}
+
+__global__ void __launch_bounds__(256) k_scale(double* __restrict__ y, double s, long n) {
+  for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * blockDim.x)
+    y[i] *= s;
+}
+
 // FIRE2 state (fire.hpp:46-58) and the convergence monitor
 // (minimization.hpp:105-110), kept on the device: whole steps replay from a
 // graph with no host decision between them.
@@ -151,6 +228,8 @@ enum FireState : int { FS_RUN = 0, FS_CONVERGED = 1, FS_MAX_STEPS = 2, FS_UPHILL
 struct FireConst {
   cdft_fire p;
   double mu, volume;
+  int rms_stop;  // stop on FIRE's own rms-force test (fire.hpp:101-103) instead of the energy monitor
+  double ndof;   // degrees of freedom of the rms force
 };
 
 // Omega from the evaluation sums, in the operation order of Context::result_from
@@ -160,8 +239,9 @@ __device__ __forceinline__ double fire_omega(const double* res, double mu) {
   return F - mu * res[RS_MASS];
 }
 
-// Fire::initialize (fire.hpp:60-84) after the first compute
-__global__ void k_fire_init(FireCtl* ctl, const double* __restrict__ res, FireConst k, double* energies) {
+// Fire::initialize (fire.hpp:60-84) after the first compute; fres[0] = sum f^2
+__global__ void k_fire_init(FireCtl* ctl, const double* __restrict__ res, const double* __restrict__ fres,
+                            FireConst k, double* energies) {
   FireCtl c{};
   c.energy = fire_omega(res, k.mu);
   c.dt = k.p.dt;
@@ -169,6 +249,7 @@ __global__ void k_fire_init(FireCtl* ctl, const double* __restrict__ res, FireCo
   c.vv = 1.0;
   c.skip = 1;
   c.state = k.p.max_steps > 0 ? FS_RUN : FS_MAX_STEPS;
+  if (k.rms_stop && sqrt(fres[0] / k.ndof) < k.p.force_tolerance) c.state = FS_CONVERGED;
   *ctl = c;
   if (energies) energies[0] = c.energy;
 }
@@ -211,14 +292,17 @@ __global__ void k_fire_begin(FireCtl* ctl, const double* __restrict__ fres, Fire
 }
 
 // After the step's compute: energy, iteration count, convergence_monitor (minimization.hpp:105-110,329-332)
-__global__ void k_fire_end(FireCtl* ctl, const double* __restrict__ res, FireConst k, double* energies) {
+__global__ void k_fire_end(FireCtl* ctl, const double* __restrict__ res, const double* __restrict__ fres,
+                           FireConst k, double* energies) {
 #pragma clang fp contract(off)
   FireCtl c = *ctl;
   if (c.skip) return;
   c.energy = fire_omega(res, k.mu);
   ++c.iteration;
   c.vv = c.vv * 0.9 + 0.1 * (fabs(c.energy - c.old_energy) / c.dt);
-  if (fabs(c.vv / k.volume) < k.p.force_tolerance) c.state = FS_CONVERGED;
+  const bool conv = k.rms_stop ? sqrt(fres[0] / k.ndof) < k.p.force_tolerance
+                               : fabs(c.vv / k.volume) < k.p.force_tolerance;
+  if (conv) c.state = FS_CONVERGED;
   else if (c.iteration >= k.p.max_steps) c.state = FS_MAX_STEPS;
   if (energies) energies[c.iteration] = c.energy;
   *ctl = c;
@@ -291,22 +375,28 @@ struct FireRun {
   int homog;
   const double* d_vext;
   FireConst k;
-  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum, [24..31] mass sum, [32..39] sum g rho
+  // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum, [24..31] mass sum, [32..39] sum g rho,
+  // [40..47] dir . f
   DevBuf x, v, f, part, res, ctl, en;
   long nb;
   bool fixed_mass = false;  // Minimizer::fixed_mass: rescale rho to `target`, mu = 0 in the loop
   double target = 0.0;
+  // Search::find's FIRE run: interior mass `target`, fixed boundary, forces projected off `dir`
+  bool saddle = false;
+  const double* dir = nullptr;
   FireCtl* host = nullptr;  // pinned copy of the control state
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 
   FireRun(Context* c_, AliasMap a, int h, const double* vx, double m, const cdft_fire& fr, bool record)
       : c(c_), amap(a), homog(h), d_vext(vx), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
-        part(8 * 8 * kVecBlocks), res(8 * 40), ctl(sizeof(FireCtl)),
+        part(8 * 8 * kVecBlocks), res(8 * 48), ctl(sizeof(FireCtl)),
         en(record ? 8 * (static_cast<size_t>(std::max(fr.max_steps, 0)) + 1) : 0) {
     k.p = fr;
     k.mu = m;
     k.volume = static_cast<double>(c->g.N) * c->g.dv;
+    k.rms_stop = 0;
+    k.ndof = static_cast<double>(c->g.N);
     nb = face_count(c->g);
     CDFT_HIP(hipHostMalloc(reinterpret_cast<void**>(&host), sizeof(FireCtl), hipHostMallocDefault));
   }
@@ -322,6 +412,27 @@ struct FireRun {
   void launch_compute() {
     const Geom& g = c->g;
     const unsigned nbk = vec_blocks(g.N);
+    if (saddle) {  // saddle_point.hpp:444-491
+      const AliasMap id{0, 0.0, 0.0};
+      double* r = res.as<double>();
+      hipLaunchKernelGGL(k_to_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, x.as<double>(), c->d_rho,
+                         g.N, amap);
+      hipLaunchKernelGGL(k_region_sum, dim3(nbk), dim3(256), 0, c->stream, c->d_rho,
+                         static_cast<const double*>(nullptr), g, 0, part.as<double>());
+      launch_finalize(part.as<double>(), static_cast<int>(nbk), r + 24, c->stream);
+      hipLaunchKernelGGL(k_scale_interior, dim3(nbk), dim3(256), 0, c->stream, c->d_rho, r + 24, target, g);
+      c->eval_direct(c->d_rho, d_vext, k.mu, c->d_force, id);
+      hipLaunchKernelGGL(k_region_sum, dim3(nbk), dim3(256), 0, c->stream, c->d_force,
+                         static_cast<const double*>(c->d_rho), g, 0, part.as<double>());
+      launch_finalize(part.as<double>(), static_cast<int>(nbk), r + 32, c->stream);
+      hipLaunchKernelGGL(k_saddle_alias, dim3(nbk), dim3(256), 0, c->stream, c->d_force, x.as<double>(), amap,
+                         r + 32, target, dir, f.as<double>(), g, part.as<double>());
+      launch_finalize(part.as<double>(), static_cast<int>(nbk), r + 40, c->stream);
+      hipLaunchKernelGGL(k_saddle_project, dim3(nbk), dim3(256), 0, c->stream, f.as<double>(), dir, r + 40,
+                         v.as<double>(), g.N, part.as<double>());
+      launch_finalize(part.as<double>(), static_cast<int>(nbk), r, c->stream);
+      return;
+    }
     if (fixed_mass) {
       // rho = to_density(x) rescaled to the target mass, evaluated at mu = 0 (minimization.hpp:193-206)
       const AliasMap id{0, 0.0, 0.0};
@@ -364,20 +475,22 @@ struct FireRun {
     hipLaunchKernelGGL(k_fire_mix, dim3(nbk), dim3(256), 0, c->stream, x.as<double>(), v.as<double>(),
                        f.as<double>(), g.N, cp, res.as<double>() + 8, res.as<double>());
     launch_compute();
-    hipLaunchKernelGGL(k_fire_end, dim3(1), dim3(1), 0, c->stream, cp, c->d_res, k, en.as<double>());
+    hipLaunchKernelGGL(k_fire_end, dim3(1), dim3(1), 0, c->stream, cp, c->d_res, res.as<double>(), k,
+                       en.as<double>());
   }
 
   // Fire::initialize (fire.hpp:60-84) and the extra compute of minimization.hpp:312-313
-  void launch_start(const double* rho0) {
+  void launch_start(const double* rho0, bool on_device = false) {
     const Geom& g = c->g;
     CDFT_HIP(hipSetDevice(c->device));
-    CDFT_HIP(hipMemcpyAsync(c->d_rho, rho0, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    CDFT_HIP(hipMemcpyAsync(c->d_rho, rho0, 8 * g.N, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                            c->stream));
     hipLaunchKernelGGL(k_from_density, dim3((g.N + 255) / 256), dim3(256), 0, c->stream, c->d_rho,
                        x.as<double>(), g.N, amap);
     CDFT_HIP(hipMemsetAsync(v.p, 0, 8 * g.N, c->stream));
     launch_compute();
-    hipLaunchKernelGGL(k_fire_init, dim3(1), dim3(1), 0, c->stream, ctl.as<FireCtl>(), c->d_res, k,
-                       en.as<double>());
+    hipLaunchKernelGGL(k_fire_init, dim3(1), dim3(1), 0, c->stream, ctl.as<FireCtl>(), c->d_res,
+                       static_cast<const double*>(res.as<double>()), k, en.as<double>());
     CDFT_HIP(hipGetLastError());
     if (c->use_graphs) {
       CDFT_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -502,6 +615,114 @@ int cdft_minimize_fixed_mass(cdft_ctx* ctx, const cdft_fire* fire, const cdft_pa
     }
     if (run.uphill()) throw std::runtime_error("fire::step: exceeded max uphill steps");
     run.finish(rho_out, result, energies, mu);
+  });
+}
+
+int cdft_saddle_search(cdft_ctx* ctx, const cdft_saddle_config* config, const double* rho0, const double* vext,
+                       double mu, double target_mass, unsigned long long seed, double* rho_out, double* eigenvector,
+                       cdft_saddle_result* result) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!config || !rho0 || !rho_out || !result) throw std::invalid_argument("null argument");
+    const cdft_parametrization& pr = config->param;
+    if (pr.kind != 0 && pr.kind != 1) throw std::invalid_argument("unknown parametrization");
+    if (config->eigen.hessian_eps < 0.0 || config->max_outer_iterations < 0)
+      throw std::invalid_argument("bad saddle search configuration");
+    Context* c = ctx->impl;
+    if (config->eigen.hessian_eps == 0.0) require_fp64(c);
+    CDFT_HIP(hipSetDevice(c->device));
+    const Geom& g = c->g;
+    const long N = g.N;
+    const size_t nb = 8 * static_cast<size_t>(N);
+    const AliasMap amap{pr.kind == 0 ? 1 : 2, pr.rho_min, pr.rho_max - pr.rho_min};
+    if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, nb, hipMemcpyHostToDevice, c->stream));
+    const double* dvx = vext ? c->d_vext : nullptr;
+    DevBuf rho(nb), eig(nb), xcur(nb), dir(nb);
+    CDFT_HIP(hipMemcpyAsync(rho.p, rho0, nb, hipMemcpyHostToDevice, c->stream));
+    EigenRun er(c, dvx, mu, config->eigen.hessian_eps, 1);
+    std::unique_ptr<Hessian> hs;
+    if (config->eigen.hessian_eps == 0.0) {
+      hs.reset(new Hessian(c));
+      er.exact = hs.get();
+    }
+    const dim3 vb(vec_blocks(N));
+    auto host_sum = [&](const double* a, const double* b, int face) {
+      hipLaunchKernelGGL(k_region_sum, vb, dim3(256), 0, c->stream, a, b, g, face, er.part.as<double>());
+      launch_finalize(er.part.as<double>(), static_cast<int>(vec_blocks(N)), er.red.as<double>(), c->stream);
+      double h = 0.0;
+      CDFT_HIP(hipMemcpyAsync(&h, er.red.p, 8, hipMemcpyDeviceToHost, c->stream));
+      CDFT_HIP(hipStreamSynchronize(c->stream));
+      return h;
+    };
+    const long nbd = face_count(g);  // boundary points
+    double omega = 0.0, lambda = 0.0, ftol = config->tolerance_start;
+    bool converged = false, have_eig = false;
+    long evals = 0;
+    int outer = 0;
+    for (; outer < config->max_outer_iterations; ++outer) {
+      const double omega_old = omega;
+      // 1. smallest eigenpair at rho (saddle_point.hpp:411-420)
+      CDFT_HIP(hipMemcpyAsync(er.rho.p, rho.p, nb, hipMemcpyDeviceToDevice, c->stream));
+      if (outer >= config->eigenvalue_clear_count && have_eig) {
+        CDFT_HIP(hipMemcpyAsync(er.v.p, eig.p, nb, hipMemcpyDeviceToDevice, c->stream));
+      } else {
+        std::vector<double> v0 = seeded_normals(static_cast<uint64_t>(seed) + static_cast<uint64_t>(outer), N);
+        CDFT_HIP(hipMemcpyAsync(er.v.p, v0.data(), nb, hipMemcpyHostToDevice, c->stream));
+        CDFT_HIP(hipStreamSynchronize(c->stream));
+      }
+      const double vn = std::sqrt(er.dots({{er.v.as<double>(), er.v.as<double>()}})[0]);
+      er.lincomb(er.v.as<double>(), 1.0 / vn, er.v.as<double>());
+      er.base_forces();
+      int it = 0, conv = 0;
+      er.solve(config->eigen.tolerance, config->eigen.max_iterations, &lambda, &it, &conv);
+      CDFT_HIP(hipMemcpyAsync(eig.p, er.v.p, nb, hipMemcpyDeviceToDevice, c->stream));
+      have_eig = true;
+      // 2. the alias-space direction (saddle_point.hpp:428-435) and the boundary mass (:437-447)
+      hipLaunchKernelGGL(k_from_density, dim3((N + 255) / 256), dim3(256), 0, c->stream, rho.as<double>(),
+                         xcur.as<double>(), N, amap);
+      hipLaunchKernelGGL(k_alias_dir, vb, dim3(256), 0, c->stream, eig.as<double>(), xcur.as<double>(), amap,
+                         dir.as<double>(), N);
+      const double dn = std::sqrt(er.dots({{dir.as<double>(), dir.as<double>()}})[0]);
+      if (dn > 1e-30) hipLaunchKernelGGL(k_scale, vb, dim3(256), 0, c->stream, dir.as<double>(), 1.0 / dn, N);
+      const double sb = host_sum(rho.as<double>(), nullptr, 1);
+      const double avg = nbd > 0 ? sb / static_cast<double>(nbd) : 0.0;
+      const double m_boundary = avg * g.dv * static_cast<double>(nbd);
+      // 3. FIRE at fixed interior mass with the eigen-direction projection (saddle_point.hpp:449-520)
+      cdft_fire fc = config->fire;
+      fc.force_tolerance = std::max(ftol, config->fire.force_tolerance);
+      ftol /= config->tolerance_factor;
+      FireRun run(c, amap, 0, dvx, mu, fc, false);
+      run.saddle = true;
+      run.target = target_mass - m_boundary;
+      run.dir = dir.as<double>();
+      run.k.rms_stop = 1;
+      run.launch_start(rho.as<double>(), true);
+      run.sync();
+      while (run.running()) {
+        run.launch_block();
+        run.sync();
+      }
+      if (run.uphill()) throw std::runtime_error("fire::step: exceeded max uphill steps");
+      evals += 1 + run.host->iteration;
+      // 4. the final density is the last compute's; Omega at mu (saddle_point.hpp:523-545)
+      CDFT_HIP(hipMemcpyAsync(rho.p, c->d_rho, nb, hipMemcpyDeviceToDevice, c->stream));
+      c->eval_direct(rho.as<double>(), dvx, mu, nullptr, AliasMap{0, 0.0, 0.0});
+      cdft_result r{};
+      c->fetch_result(mu, &r);
+      omega = r.grand_potential;
+      if (outer > 0 && std::fabs(omega - omega_old) < config->omega_tolerance) {
+        converged = true;
+        break;
+      }
+    }
+    CDFT_HIP(hipMemcpyAsync(rho_out, rho.p, nb, hipMemcpyDeviceToHost, c->stream));
+    if (eigenvector) CDFT_HIP(hipMemcpyAsync(eigenvector, eig.p, nb, hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    result->grand_potential = omega;
+    result->eigenvalue = lambda;
+    result->outer_iterations = outer + 1;  // as the reference counts it (saddle_point.hpp:584)
+    result->converged = converged ? 1 : 0;
+    result->evaluations = evals;
   });
 }
 

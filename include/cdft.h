@@ -269,6 +269,36 @@ CDFT_API int cdft_smallest_eigenpair(cdft_ctx* ctx, const cdft_eigen_config* con
                             const double* vext, double mu, const double* initial_guess, unsigned long long seed,
                             double* eigenvector, cdft_eigen_result* result);
 
+/* saddle_point::Search (saddle_point.hpp:313-347). */
+typedef struct {
+  cdft_fire fire;             /* Search::fire (dt 1e-3, dt_max 1, alpha_start 0.01, f_alpha 1, tol 1e-4) */
+  cdft_eigen_config eigen;    /* Search::eigenvalue; fixed_boundary is forced on */
+  cdft_parametrization param; /* Search::param (Unbounded{1e-18}) */
+  double omega_tolerance, tolerance_start, tolerance_factor;
+  int max_outer_iterations, eigenvalue_clear_count;
+} cdft_saddle_config;
+
+typedef struct {
+  double grand_potential, eigenvalue;
+  int outer_iterations, converged;
+  long evaluations;   /* functional evaluations of the FIRE runs */
+} cdft_saddle_result;
+
+/* Search::find (saddle_point.hpp:367-589): the critical cluster by
+ * eigenvector following at fixed mass.  Per outer iteration: the smallest
+ * eigenpair of the Hessian (cdft_smallest_eigenpair with a fixed boundary;
+ * the previous eigenvector as start from outer iteration
+ * eigenvalue_clear_count on, else the seeded normal sequence of seed + outer),
+ * then FIRE in the alias variable with the interior mass held at target_mass
+ * minus the boundary mass, the boundary forces zeroed, the interior Lagrange
+ * multiplier removed and the force component along the (alias-transformed)
+ * eigenvector removed, stopped on its rms force at a tolerance tightened by
+ * tolerance_factor each outer iteration; stops when Omega changes by less
+ * than omega_tolerance.  rho_out and eigenvector (nullable) receive N values. */
+CDFT_API int cdft_saddle_search(cdft_ctx* ctx, const cdft_saddle_config* config, const double* rho0,
+                       const double* vext, double mu, double target_mass, unsigned long long seed, double* rho_out,
+                       double* eigenvector, cdft_saddle_result* result);
+
 /* ---------------------------------------------------------------- adjoint
  * The reference has no derivative of its dynamics; the entry points below
  * extend its DDFT integrating-factor step (dynamics.hpp:333-401) with a

@@ -522,6 +522,63 @@ struct EigenvalueSolver {
     return r;
   }
 };
+// Result (saddle_point.hpp:313-320)
+struct Result {
+  vec density;
+  vec eigenvector;
+  double grand_potential{0.0};
+  double eigenvalue{0.0};
+  int outer_iterations{0};
+  bool converged{false};
+};
+
+// Search (saddle_point.hpp:324-347): the critical cluster by eigenvector
+// following at fixed mass.  The eigen solves use the start vectors of
+// seed + outer iteration until eigenvalue_clear_count.
+struct Search {
+  fire::Fire fire{1e-3, 1.0, 1e-8, 0.01, 1.1, 0.5, 1.0, 5, 20, 1e-4, 100000};
+  EigenvalueSolver eigenvalue{1e-4, 300, 1e-6};
+  minimization::Parametrization param = minimization::Unbounded{1e-18};
+  double omega_tolerance{1e-5};
+  double tolerance_start{1e-3};
+  double tolerance_factor{10.0};
+  int max_outer_iterations{20};
+  int eigenvalue_clear_count{2};
+  unsigned long long seed{0};
+
+  Result find(const physics::Model& model, const functionals::Weights& weights, const vec& initial_density,
+              double chemical_potential, double target_mass) const {
+    if (static_cast<long>(initial_density.size()) != model.grid.total_points())
+      throw std::invalid_argument("initial density size does not match the grid");
+    cdft_saddle_config cfg{};
+    cfg.fire = {fire.dt,      fire.dt_max,  fire.dt_min,     fire.alpha_start,    fire.f_inc,   fire.f_dec,
+                fire.f_alpha, fire.n_delay, fire.max_uphill, fire.force_tolerance, fire.max_steps};
+    cfg.eigen = {eigenvalue.tolerance, eigenvalue.max_iterations, eigenvalue.hessian_eps, 1};
+    if (auto* u = std::get_if<minimization::Unbounded>(&param)) {
+      cfg.param = {0, u->rho_min, 1.0};
+    } else {
+      const auto& b = std::get<minimization::Bounded>(param);
+      cfg.param = {1, b.rho_min, b.rho_max};
+    }
+    cfg.omega_tolerance = omega_tolerance;
+    cfg.tolerance_start = tolerance_start;
+    cfg.tolerance_factor = tolerance_factor;
+    cfg.max_outer_iterations = max_outer_iterations;
+    cfg.eigenvalue_clear_count = eigenvalue_clear_count;
+    Result r;
+    r.density.assign(initial_density.size(), 0.0);
+    r.eigenvector.assign(initial_density.size(), 0.0);
+    cdft_saddle_result cr{};
+    ::dft::detail::check(cdft_saddle_search(weights.ctx.get(), &cfg, initial_density.data(), nullptr,
+                                            chemical_potential, target_mass, seed, r.density.data(),
+                                            r.eigenvector.data(), &cr));
+    r.grand_potential = cr.grand_potential;
+    r.eigenvalue = cr.eigenvalue;
+    r.outer_iterations = cr.outer_iterations;
+    r.converged = cr.converged != 0;
+    return r;
+  }
+};
 }  // namespace saddle_point
 
 namespace dynamics {

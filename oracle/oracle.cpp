@@ -971,6 +971,134 @@ int minimize(Ctx* c, const double* fire_p, int param_kind, double rho_min, doubl
 
 }  // namespace
 
+namespace {
+// EigenvalueSolver::solve (saddle_point.hpp:159-270): LOBPCG for the
+// smallest eigenpair of the grand-potential Hessian with the forward
+// difference products of saddle_point.hpp:28-38 (forces of total() at mu;
+// boundary forces zeroed when `fixed`, grid.hpp:135-142).  The projected
+// 2x2 / 3x3 problem is solved by Jacobi rotations, eigenvector sign c0 >= 0
+// (arma::eig_sym's sign is unspecified).  v0: start (normalised here).
+// out: eigenvalue, iterations, converged.
+void lobpcg(Ctx* c, double tol, int max_it, double eps, int fixed, const double* rho_in, const double* vext,
+                   double mu, const double* v0, double* v_out, double* out) {
+  const long N = c->N;
+  std::vector<char> bd(static_cast<size_t>(N), 0);
+  for (long ix = 0; ix < c->nx; ++ix)
+    for (long iy = 0; iy < c->ny; ++iy)
+      for (long iz = 0; iz < c->nz; ++iz)
+        bd[iz + c->nz * (iy + c->ny * ix)] =
+            (ix == 0 || ix == c->nx - 1 || iy == 0 || iy == c->ny - 1 || iz == 0 || iz == c->nz - 1);
+  using V = std::vector<double>;
+  const V rho(rho_in, rho_in + N);
+  double parts[6];
+  auto force = [&](const V& r, V& f) {
+    c->total(r.data(), vext, &mu, parts, f.data());
+    if (fixed)
+      for (long i = 0; i < N; ++i)
+        if (bd[i]) f[i] = 0.0;
+  };
+  V f0(N), f1(N), tmp(N);
+  force(rho, f0);
+  auto hvp = [&](const V& x, V& out) {
+    for (long i = 0; i < N; ++i) tmp[i] = rho[i] + eps * x[i];
+    force(tmp, f1);
+    for (long i = 0; i < N; ++i) out[i] = (f1[i] - f0[i]) / eps;
+  };
+  auto dot = [&](const V& a, const V& b) {
+    double s = 0.0;
+    for (long i = 0; i < N; ++i) s += a[i] * b[i];
+    return s;
+  };
+  auto eig = [&](int n, double A[3][3], double* cv) {  // Jacobi rotations
+    double W[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+    for (int sweep = 0; sweep < 64; ++sweep) {
+      double off = 0.0, dg = 0.0;
+      for (int i = 0; i < n; ++i) {
+        dg += A[i][i] * A[i][i];
+        for (int j = i + 1; j < n; ++j) off += A[i][j] * A[i][j];
+      }
+      if (off <= 1e-32 * dg || off == 0.0) break;
+      for (int p = 0; p < n; ++p)
+        for (int q = p + 1; q < n; ++q) {
+          if (A[p][q] == 0.0) continue;
+          double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
+          double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+          double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
+          for (int k = 0; k < n; ++k) {
+            double a = A[k][p], b = A[k][q];
+            A[k][p] = cs * a - sn * b;
+            A[k][q] = sn * a + cs * b;
+          }
+          for (int k = 0; k < n; ++k) {
+            double a = A[p][k], b = A[q][k];
+            A[p][k] = cs * a - sn * b;
+            A[q][k] = sn * a + cs * b;
+          }
+          for (int k = 0; k < n; ++k) {
+            double a = W[k][p], b = W[k][q];
+            W[k][p] = cs * a - sn * b;
+            W[k][q] = sn * a + cs * b;
+          }
+        }
+    }
+    int m = 0;
+    for (int i = 1; i < n; ++i)
+      if (A[i][i] < A[m][m]) m = i;
+    double sg = W[0][m] < 0.0 ? -1.0 : 1.0;
+    for (int i = 0; i < n; ++i) cv[i] = sg * W[i][m];
+  };
+  V v(v0, v0 + N), hv(N), r(N), q1(N), hq1(N), q2(N), hq2(N), p, vnew(N);
+  double nv = std::sqrt(dot(v, v));
+  for (double& x : v) x /= nv;
+  hvp(v, hv);
+  double lambda = dot(v, hv);
+  bool conv = false;
+  int it = 0;
+  for (; it < max_it; ++it) {
+    for (long i = 0; i < N; ++i) r[i] = hv[i] - lambda * v[i];
+    if (std::sqrt(dot(r, r)) < tol) { conv = true; break; }
+    double rq = dot(r, v);
+    for (long i = 0; i < N; ++i) q1[i] = r[i] - rq * v[i];
+    double q1n = std::sqrt(dot(q1, q1));
+    if (q1n < 1e-30) break;
+    for (double& x : q1) x /= q1n;
+    hvp(q1, hq1);
+    int dim = 2;
+    if (!p.empty()) {
+      double a = dot(p, v), b = dot(p, q1);
+      for (long i = 0; i < N; ++i) q2[i] = p[i] - a * v[i] - b * q1[i];
+      double q2n = std::sqrt(dot(q2, q2));
+      if (q2n > 1e-10) {
+        for (double& x : q2) x /= q2n;
+        hvp(q2, hq2);
+        dim = 3;
+      }
+    }
+    double H[3][3] = {{dot(v, hv), dot(v, hq1), 0}, {0, dot(q1, hq1), 0}, {0, 0, 0}};
+    H[1][0] = H[0][1];
+    if (dim == 3) {
+      H[0][2] = H[2][0] = dot(v, hq2);
+      H[1][2] = H[2][1] = dot(q1, hq2);
+      H[2][2] = dot(q2, hq2);
+    }
+    double cv[3] = {0, 0, 0};
+    eig(dim, H, cv);
+    for (long i = 0; i < N; ++i) vnew[i] = cv[0] * v[i] + cv[1] * q1[i] + (dim == 3 ? cv[2] * q2[i] : 0.0);
+    p.resize(N);
+    for (long i = 0; i < N; ++i) p[i] = vnew[i] - v[i];
+    double vn = std::sqrt(dot(vnew, vnew));
+    for (long i = 0; i < N; ++i) v[i] = vnew[i] / vn;
+    hvp(v, hv);
+    lambda = dot(v, hv);
+  }
+  std::memcpy(v_out, v.data(), sizeof(double) * N);
+  out[0] = lambda;
+  out[1] = it;
+  out[2] = conv ? 1.0 : 0.0;
+}
+
+}  // namespace
+
 extern "C" {
 
 // potential params per interaction: kind, sigma, eps, rc, alpha, split, scheme
@@ -1335,130 +1463,185 @@ int orc_ddft_run(void* h, const double* cfg_in, int n_steps, const double* vext,
   return 0;
 }
 
-// EigenvalueSolver::solve (saddle_point.hpp:159-270): LOBPCG for the
-// smallest eigenpair of the grand-potential Hessian with the forward
-// difference products of saddle_point.hpp:28-38 (forces of total() at mu;
-// boundary forces zeroed when `fixed`, grid.hpp:135-142).  The projected
-// 2x2 / 3x3 problem is solved by Jacobi rotations, eigenvector sign c0 >= 0
-// (arma::eig_sym's sign is unspecified).  v0: start (normalised here).
-// out: eigenvalue, iterations, converged.
 void orc_smallest_eigenpair(void* h, double tol, int max_it, double eps, int fixed, const double* rho_in,
                             const double* vext, double mu, const double* v0, double* v_out, double* out) {
+  lobpcg(static_cast<Ctx*>(h), tol, max_it, eps, fixed, rho_in, vext, mu, v0, v_out, out);
+}
+
+// SplitMix64 uniforms, Box-Muller normals (cosine branch first, then the sine branch).
+static std::vector<double> seeded_normals(uint64_t seed, long n) {
+  const long m = (n + 1) / 2;
+  std::vector<double> u(static_cast<size_t>(2 * m)), z(static_cast<size_t>(n));
+  for (long i = 0; i < 2 * m; ++i) {
+    uint64_t x = seed + 0x9E3779B97F4A7C15ull * static_cast<uint64_t>(i + 1);
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    u[i] = static_cast<double>(x >> 11) * (1.0 / 9007199254740992.0);
+  }
+  for (long k = 0; k < m; ++k) {
+    const double r = std::sqrt(-2.0 * std::log(std::max(u[2 * k], 1e-300))), a = 2.0 * PI * u[2 * k + 1];
+    if (k < n) z[k] = r * std::cos(a);
+    if (m + k < n) z[m + k] = r * std::sin(a);
+  }
+  return z;
+}
+
+// Search::find (saddle_point.hpp:367-589).  cfg: fire[11] (as orc_minimize_gp),
+// eigen tolerance, max iterations, hessian eps, param kind, rho_min, rho_max,
+// omega tolerance, tolerance start, tolerance factor, max outer iterations,
+// eigenvalue clear count.  The start of outer iteration k's eigen solve is
+// the seeded normal sequence of seed + k until the clear count.
+// out: Omega, eigenvalue, outer iterations (as the reference counts them), converged.
+int orc_saddle_search(void* h, const double* cfg, const double* rho0, const double* vext, double mu,
+                      double target_mass, unsigned long long seed, double* rho_out, double* eig_out, double* out) {
   auto* c = static_cast<Ctx*>(h);
   const long N = c->N;
+  FireParams fp{cfg[0], cfg[1], cfg[2], cfg[3], cfg[4], cfg[5], cfg[6], static_cast<int>(cfg[7]),
+                static_cast<int>(cfg[8]), cfg[9], static_cast<int>(cfg[10])};
+  const double etol = cfg[11], eps = cfg[13];
+  const int emax = static_cast<int>(cfg[12]);
+  const int kind = static_cast<int>(cfg[14]);
+  const double rho_min = cfg[15], range = cfg[16] - cfg[15];
+  const double omega_tol = cfg[17], tol_factor = cfg[19];
+  double ftol = cfg[18];
+  const int max_outer = static_cast<int>(cfg[20]), clear = static_cast<int>(cfg[21]);
   std::vector<char> bd(static_cast<size_t>(N), 0);
+  long nbd = 0;
   for (long ix = 0; ix < c->nx; ++ix)
     for (long iy = 0; iy < c->ny; ++iy)
-      for (long iz = 0; iz < c->nz; ++iz)
-        bd[iz + c->nz * (iy + c->ny * ix)] =
-            (ix == 0 || ix == c->nx - 1 || iy == 0 || iy == c->ny - 1 || iz == 0 || iz == c->nz - 1);
-  using V = std::vector<double>;
-  const V rho(rho_in, rho_in + N);
-  double parts[6];
-  auto force = [&](const V& r, V& f) {
-    c->total(r.data(), vext, &mu, parts, f.data());
-    if (fixed)
+      for (long iz = 0; iz < c->nz; ++iz) {
+        const bool b = ix == 0 || ix == c->nx - 1 || iy == 0 || iy == c->ny - 1 || iz == 0 || iz == c->nz - 1;
+        bd[iz + c->nz * (iy + c->ny * ix)] = b;
+        nbd += b;
+      }
+  auto to_rho = [&](double x) { const double x2 = x * x; return kind == 0 ? rho_min + x2 : rho_min + range * x2 / (1.0 + x2); };
+  auto drho = [&](double x) {
+    if (kind == 0) return 2.0 * x;
+    const double x2 = x * x;
+    return 2.0 * range * x / ((1.0 + x2) * (1.0 + x2));
+  };
+  auto from_rho = [&](double r) {
+    if (kind == 0) return std::sqrt(std::max(r - rho_min, 0.0));
+    const double d = std::min(std::max(r - rho_min, 0.0), range * (1.0 - 1e-14));
+    return std::sqrt(d / (range - d));
+  };
+  std::vector<double> rho(rho0, rho0 + N), eig(N), dir(N), grad(N), rl(N);
+  double omega = 0.0, lambda = 0.0, parts[6];
+  bool converged = false, have = false;
+  int outer = 0;
+  for (; outer < max_outer; ++outer) {
+    const double omega_old = omega;
+    std::vector<double> v0 = (outer >= clear && have) ? eig : seeded_normals(seed + static_cast<uint64_t>(outer), N);
+    double eo[3];
+    lobpcg(c, etol, emax, eps, 1, rho.data(), vext, mu, v0.data(), eig.data(), eo);
+    lambda = eo[0];
+    have = true;
+    std::vector<double> x(N);
+    for (long i = 0; i < N; ++i) x[i] = from_rho(rho[i]);
+    double dn = 0.0;
+    for (long i = 0; i < N; ++i) {
+      dir[i] = eig[i] * drho(x[i]);
+      dn += dir[i] * dir[i];
+    }
+    dn = std::sqrt(dn);
+    if (dn > 1e-30)
+      for (double& d : dir) d /= dn;
+    double sb = 0.0;
+    for (long i = 0; i < N; ++i)
+      if (bd[i]) sb += rho[i];
+    const double m_boundary = (nbd > 0 ? sb / static_cast<double>(nbd) : 0.0) * c->dv * static_cast<double>(nbd);
+    const double mt = target_mass - m_boundary;
+    auto compute = [&](const std::vector<double>& xx, std::vector<double>& f) {
+      double mi = 0.0;
+      for (long i = 0; i < N; ++i) {
+        rl[i] = to_rho(xx[i]);
+        if (!bd[i]) mi += rl[i];
+      }
+      mi *= c->dv;
+      if (mi > 0.0 && mt > 0.0)
+        for (long i = 0; i < N; ++i)
+          if (!bd[i]) rl[i] *= mt / mi;
+      c->total(rl.data(), vext, &mu, parts, grad.data());
+      double muL = 0.0;
       for (long i = 0; i < N; ++i)
-        if (bd[i]) f[i] = 0.0;
-  };
-  V f0(N), f1(N), tmp(N);
-  force(rho, f0);
-  auto hvp = [&](const V& x, V& out) {
-    for (long i = 0; i < N; ++i) tmp[i] = rho[i] + eps * x[i];
-    force(tmp, f1);
-    for (long i = 0; i < N; ++i) out[i] = (f1[i] - f0[i]) / eps;
-  };
-  auto dot = [&](const V& a, const V& b) {
-    double s = 0.0;
-    for (long i = 0; i < N; ++i) s += a[i] * b[i];
-    return s;
-  };
-  auto eig = [&](int n, double A[3][3], double* cv) {  // Jacobi rotations
-    double W[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
-    for (int sweep = 0; sweep < 64; ++sweep) {
-      double off = 0.0, dg = 0.0;
-      for (int i = 0; i < n; ++i) {
-        dg += A[i][i] * A[i][i];
-        for (int j = i + 1; j < n; ++j) off += A[i][j] * A[i][j];
+        if (!bd[i]) muL += grad[i] * rl[i];
+      if (mt > 0.0) muL /= mt;
+      double pj = 0.0;
+      for (long i = 0; i < N; ++i) {
+        const double gi = bd[i] ? 0.0 : grad[i] - muL * c->dv;
+        f[i] = gi * drho(xx[i]);
+        pj += dir[i] * f[i];
       }
-      if (off <= 1e-32 * dg || off == 0.0) break;
-      for (int p = 0; p < n; ++p)
-        for (int q = p + 1; q < n; ++q) {
-          if (A[p][q] == 0.0) continue;
-          double th = (A[q][q] - A[p][p]) / (2.0 * A[p][q]);
-          double t = (th >= 0.0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
-          double cs = 1.0 / std::sqrt(t * t + 1.0), sn = t * cs;
-          for (int k = 0; k < n; ++k) {
-            double a = A[k][p], b = A[k][q];
-            A[k][p] = cs * a - sn * b;
-            A[k][q] = sn * a + cs * b;
-          }
-          for (int k = 0; k < n; ++k) {
-            double a = A[p][k], b = A[q][k];
-            A[p][k] = cs * a - sn * b;
-            A[q][k] = sn * a + cs * b;
-          }
-          for (int k = 0; k < n; ++k) {
-            double a = W[k][p], b = W[k][q];
-            W[k][p] = cs * a - sn * b;
-            W[k][q] = sn * a + cs * b;
-          }
+      for (long i = 0; i < N; ++i) f[i] = -(f[i] - pj * dir[i]);
+      return parts[5];
+    };
+    FireParams fc = fp;
+    fc.force_tolerance = std::max(ftol, fp.force_tolerance);
+    ftol /= tol_factor;
+    // Fire::initialize + the extra compute, then Fire::step until the rms force test holds (fire.hpp:60-165)
+    std::vector<double> v(N, 0.0), f(N);
+    double energy = compute(x, f), dt = fc.dt, alpha = fc.alpha_start;
+    auto rms = [&]() {
+      double s2 = 0.0;
+      for (double q : f) s2 += q * q;
+      return std::sqrt(s2 / static_cast<double>(N));
+    };
+    bool fconv = rms() < fc.force_tolerance;
+    int npos = 0, nneg = 0, iter = 0;
+    for (int it = 0; it < fc.max_steps && !fconv; ++it) {
+      double P = 0.0;
+      for (long i = 0; i < N; ++i) P += v[i] * f[i];
+      if (P > 0.0 || iter == 0) {
+        ++npos;
+        nneg = 0;
+        if (npos > fc.n_delay) {
+          dt = std::min(dt * fc.f_inc, fc.dt_max);
+          alpha *= fc.f_alpha;
         }
-    }
-    int m = 0;
-    for (int i = 1; i < n; ++i)
-      if (A[i][i] < A[m][m]) m = i;
-    double sg = W[0][m] < 0.0 ? -1.0 : 1.0;
-    for (int i = 0; i < n; ++i) cv[i] = sg * W[i][m];
-  };
-  V v(v0, v0 + N), hv(N), r(N), q1(N), hq1(N), q2(N), hq2(N), p, vnew(N);
-  double nv = std::sqrt(dot(v, v));
-  for (double& x : v) x /= nv;
-  hvp(v, hv);
-  double lambda = dot(v, hv);
-  bool conv = false;
-  int it = 0;
-  for (; it < max_it; ++it) {
-    for (long i = 0; i < N; ++i) r[i] = hv[i] - lambda * v[i];
-    if (std::sqrt(dot(r, r)) < tol) { conv = true; break; }
-    double rq = dot(r, v);
-    for (long i = 0; i < N; ++i) q1[i] = r[i] - rq * v[i];
-    double q1n = std::sqrt(dot(q1, q1));
-    if (q1n < 1e-30) break;
-    for (double& x : q1) x /= q1n;
-    hvp(q1, hq1);
-    int dim = 2;
-    if (!p.empty()) {
-      double a = dot(p, v), b = dot(p, q1);
-      for (long i = 0; i < N; ++i) q2[i] = p[i] - a * v[i] - b * q1[i];
-      double q2n = std::sqrt(dot(q2, q2));
-      if (q2n > 1e-10) {
-        for (double& x : q2) x /= q2n;
-        hvp(q2, hq2);
-        dim = 3;
+      } else {
+        npos = 0;
+        if (++nneg > fc.max_uphill) return -1;
+        for (long i = 0; i < N; ++i) {
+          x[i] -= 0.5 * dt * v[i];
+          v[i] = 0.0;
+        }
+        if (iter > fc.n_delay) {
+          dt = std::max(dt * fc.f_dec, fc.dt_min);
+          alpha = fc.alpha_start;
+        }
       }
+      for (long i = 0; i < N; ++i) v[i] += dt * f[i];
+      double vn = 0.0, fn = 0.0;
+      for (long i = 0; i < N; ++i) {
+        vn += v[i] * v[i];
+        fn += f[i] * f[i];
+      }
+      if (fn > 0.0) {
+        const double ratio = std::sqrt(vn) / std::sqrt(fn);
+        for (long i = 0; i < N; ++i) v[i] = (1.0 - alpha) * v[i] + alpha * ratio * f[i];
+      }
+      for (long i = 0; i < N; ++i) x[i] += dt * v[i];
+      energy = compute(x, f);
+      ++iter;
+      fconv = rms() < fc.force_tolerance;
     }
-    double H[3][3] = {{dot(v, hv), dot(v, hq1), 0}, {0, dot(q1, hq1), 0}, {0, 0, 0}};
-    H[1][0] = H[0][1];
-    if (dim == 3) {
-      H[0][2] = H[2][0] = dot(v, hq2);
-      H[1][2] = H[2][1] = dot(q1, hq2);
-      H[2][2] = dot(q2, hq2);
+    (void)energy;
+    rho = rl;  // the last compute's density: to_density(x) with the interior rescaled
+    c->total(rho.data(), vext, &mu, parts, nullptr);
+    omega = parts[5];
+    if (outer > 0 && std::fabs(omega - omega_old) < omega_tol) {
+      converged = true;
+      break;
     }
-    double cv[3] = {0, 0, 0};
-    eig(dim, H, cv);
-    for (long i = 0; i < N; ++i) vnew[i] = cv[0] * v[i] + cv[1] * q1[i] + (dim == 3 ? cv[2] * q2[i] : 0.0);
-    p.resize(N);
-    for (long i = 0; i < N; ++i) p[i] = vnew[i] - v[i];
-    double vn = std::sqrt(dot(vnew, vnew));
-    for (long i = 0; i < N; ++i) v[i] = vnew[i] / vn;
-    hvp(v, hv);
-    lambda = dot(v, hv);
   }
-  std::memcpy(v_out, v.data(), sizeof(double) * N);
-  out[0] = lambda;
-  out[1] = it;
-  out[2] = conv ? 1.0 : 0.0;
+  std::memcpy(rho_out, rho.data(), sizeof(double) * N);
+  if (eig_out) std::memcpy(eig_out, eig.data(), sizeof(double) * N);
+  out[0] = omega;
+  out[1] = lambda;
+  out[2] = outer + 1;
+  out[3] = converged ? 1.0 : 0.0;
+  return 0;
 }
 
 // Fixed-dt forward map of the DDFT adjoint: n_steps steps of fixed_step.

@@ -75,6 +75,9 @@ def lib():
         L.orc_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.orc_smallest_eigenpair.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp,
                                              C.c_double, _dp, _dp, _dp]
+        L.orc_saddle_search.restype = C.c_int
+        L.orc_saddle_search.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.c_ulonglong, _dp, _dp,
+                                        _dp]
         _lib = L
     return _lib
 
@@ -270,6 +273,30 @@ class Oracle:
         lib().orc_smallest_eigenpair(self._h, float(tolerance), int(max_iterations), float(hessian_eps),
                                      int(bool(fixed_boundary)), _p(r), _p(ve), float(mu), _p(v), _p(vo), _p(out))
         return vo.reshape(self.shape), out[0], int(out[1]), bool(out[2])
+
+    def saddle_search(self, rho0, mu, target_mass, seed=0, fire=None, eigen=None, param="unbounded", rho_min=1e-18,
+                      rho_max=1.0, omega_tolerance=1e-5, tolerance_start=1e-3, tolerance_factor=10.0,
+                      max_outer_iterations=20, eigenvalue_clear_count=2, vext=None):
+        """saddle_point::Search::find; returns (rho, eigenvector, result dict)."""
+        f = dict(dt=1e-3, dt_max=1.0, dt_min=1e-8, alpha_start=0.01, f_inc=1.1, f_dec=0.5, f_alpha=1.0, n_delay=5,
+                 max_uphill=20, force_tolerance=1e-4, max_steps=100000)
+        f.update(fire or {})
+        e = dict(tolerance=1e-4, max_iterations=300, hessian_eps=1e-6)
+        e.update(eigen or {})
+        cfg = np.array([f["dt"], f["dt_max"], f["dt_min"], f["alpha_start"], f["f_inc"], f["f_dec"], f["f_alpha"],
+                        f["n_delay"], f["max_uphill"], f["force_tolerance"], f["max_steps"], e["tolerance"],
+                        e["max_iterations"], e["hessian_eps"], 0 if param == "unbounded" else 1, rho_min, rho_max,
+                        omega_tolerance, tolerance_start, tolerance_factor, max_outer_iterations,
+                        eigenvalue_clear_count], dtype=np.float64)
+        r0 = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
+        ve = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
+        ro, eo, out = np.zeros(self.N), np.zeros(self.N), np.zeros(4)
+        rc = lib().orc_saddle_search(self._h, _p(cfg), _p(r0), _p(ve), float(mu), float(target_mass), int(seed),
+                                     _p(ro), _p(eo), _p(out))
+        if rc != 0:
+            raise RuntimeError("fire::step: exceeded max uphill steps")
+        return ro.reshape(self.shape), eo.reshape(self.shape), dict(
+            grand_potential=out[0], eigenvalue=out[1], outer_iterations=int(out[2]), converged=bool(out[3]))
 
     def wells_potential(self, params):
         prm = np.ascontiguousarray(params, dtype=np.float64).reshape(-1, 5)

@@ -119,6 +119,12 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   es.tolerance = 1e-6;
   es.seed = 17;
   const auto eres = es.solve(func.weights, mu, rho);
+  algorithms::saddle_point::Search search;
+  search.fire.max_steps = 300;
+  search.eigenvalue.max_iterations = 100;
+  search.max_outer_iterations = 3;
+  search.seed = 1;
+  const auto sres_saddle = search.find(model, func.weights, rho, mu, mass0);
 
   algorithms::dynamics::Simulation sim;
   sim.step.dt = 1e-3;
@@ -141,10 +147,12 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   std::printf("{\"mu\": %.17g, \"free_energy\": %.17g, \"grand_potential\": %.17g, \"callback_omega\": %.17g, "
               "\"callback_force_maxdiff\": %.3g, \"fire_iterations\": %d, \"fire_grand_potential\": %.17g, "
               "\"fixed_mass_target\": %.17g, \"fixed_mass_iterations\": %d, \"fixed_mass_free_energy\": %.17g, "
-              "\"eigenvalue\": %.17g, \"eigen_iterations\": %d, "
+              "\"eigenvalue\": %.17g, \"eigen_iterations\": %d, \"saddle_omega\": %.17g, "
+              "\"saddle_eigenvalue\": %.17g, \"saddle_outer\": %d, "
               "\"ddft_times\": [",
               mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential,
-              0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations);
+              0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations,
+              sres_saddle.grand_potential, sres_saddle.eigenvalue, sres_saddle.outer_iterations);
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);

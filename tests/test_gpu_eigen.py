@@ -1,8 +1,11 @@
 """GPU checks of the smallest-eigenpair solver (EigenvalueSolver::solve,
-saddle_point.hpp:159-270) through the C ABI: forward-difference mode against
+saddle_point.hpp:159-270) and the saddle search (Search::find,
+saddle_point.hpp:367-589) through the C ABI: forward-difference mode against
 the FP64 oracle from the same seeded start (eigenvalue 1e-6, eigenvector
-overlap 1 - 1e-6), exact mode against the dense exact Hessian (1e-9), and the
-two modes against each other on a fused-schedule grid."""
+overlap 1 - 1e-6), exact mode against the dense exact Hessian (1e-9), the two
+modes against each other on a fused-schedule grid, and the saddle search
+against the oracle's (Omega 1e-6, eigenvalue 1e-4, density 1e-5: its FIRE runs
+stop at an rms force of 1e-4)."""
 import numpy as np
 import pytest
 
@@ -78,3 +81,32 @@ def test_eigen_exact_and_fd_agree_on_fused_grid():
           f"({rf['iterations']} it)", flush=True)
     assert re_["converged"] and rf["converged"]
     assert re_["eigenvalue"] == pytest.approx(rf["eigenvalue"], rel=1e-4)
+
+
+def test_saddle_search_matches_oracle():
+    # Search::find from a small droplet at fixed mass: a critical cluster with one negative eigenvalue
+    from oracle import oracle as O
+
+    shape, dx = (16, 16, 16), 0.25
+    import classicaldft_amd as cd
+
+    ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    rho0 = W.droplet(shape, dx, radius=1.1, width=0.4, rho_l=0.6, rho_v=0.05, noise=0.0, seed=3)
+    target = float(np.sum(rho0)) * dx ** 3
+    mu = orc.bulk_mu(0.05)
+    kw = dict(seed=1, fire=dict(max_steps=300), eigen=dict(max_iterations=100), max_outer_iterations=3)
+    rg, eg, resg = ctx.saddle_search(rho0, mu, target, **kw)
+    ro, eo, reso = orc.saddle_search(rho0, mu, target, **kw)
+    e = float(np.max(np.abs(rg - ro)) / np.max(np.abs(ro)))
+    print(f"saddle: gpu Omega {resg['grand_potential']:.12e} lambda {resg['eigenvalue']:.6e} "
+          f"({resg['outer_iterations']} outer, {resg['evaluations']} evals); oracle {reso['grand_potential']:.12e} "
+          f"lambda {reso['eigenvalue']:.6e}; density {e:.1e}", flush=True)
+    assert resg["outer_iterations"] == reso["outer_iterations"] and resg["converged"] == reso["converged"]
+    assert resg["eigenvalue"] < 0.0  # a saddle: one unstable direction
+    # each FIRE run stops at rms force < 1e-4, so the end states agree to that level, not to rounding
+    assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-6)
+    assert resg["eigenvalue"] == pytest.approx(reso["eigenvalue"], rel=1e-4)
+    assert e < 1e-5
+    assert float(np.sum(rg)) * dx ** 3 == pytest.approx(target, rel=1e-12)
+    assert abs(float(np.sum(eg * eo))) > 1.0 - 1e-6

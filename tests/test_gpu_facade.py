@@ -62,6 +62,14 @@ def test_cpp_facade_matches_oracle(tmp_path):
     assert out["eigen_iterations"] == it
     assert out["eigenvalue"] == pytest.approx(lam, rel=1e-6)
 
+    # Search::find from the droplet at its own mass
+    mass0 = float(np.sum(rho)) * dx ** 3
+    _, _, rs = orc.saddle_search(rho, mu, mass0, seed=1, fire=dict(max_steps=300), eigen=dict(max_iterations=100),
+                                 max_outer_iterations=3)
+    assert out["saddle_outer"] == rs["outer_iterations"]
+    assert out["saddle_omega"] == pytest.approx(rs["grand_potential"], rel=1e-6)
+    assert out["saddle_eigenvalue"] == pytest.approx(rs["eigenvalue"], rel=1e-4, abs=1e-7)
+
     rd, resd, ed = orc.ddft_run(rho, mu, 6, dt=1e-3, dt_max=1e-3)
     assert _rel(np.fromfile(tmp_path / "ddft_rho.bin").reshape(shape), rd) < 1e-9
     # recorded at step 0 and every log_interval = 2 steps, as Simulation::run does
