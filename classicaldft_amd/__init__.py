@@ -29,7 +29,7 @@ EXPORTED = [
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
     "cdft_ddft_adjoint_wells", "cdft_autotune", "cdft_minimize_fixed_mass", "cdft_smallest_eigenpair",
-    "cdft_saddle_search",
+    "cdft_saddle_search", "cdft_ddft_split_operator", "cdft_ddft_crank_nicolson", "cdft_picard_solve",
 ]
 
 
@@ -110,6 +110,16 @@ class _SaddleResult(C.Structure):
                 ("converged", C.c_int), ("evaluations", C.c_long)]
 
 
+class _Picard(C.Structure):
+    _fields_ = [("mixing", C.c_double), ("min_density", C.c_double), ("tolerance", C.c_double),
+                ("max_iterations", C.c_int)]
+
+
+class _PicardResult(C.Structure):
+    _fields_ = [("grand_potential", C.c_double), ("residual", C.c_double), ("iterations", C.c_int),
+                ("converged", C.c_int)]
+
+
 class _AdjointConfig(C.Structure):
     _fields_ = [("dt", C.c_double), ("fp_iterations", C.c_int), ("n_steps", C.c_int),
                 ("checkpoint_interval", C.c_int)]
@@ -164,6 +174,11 @@ def lib():
         L.cdft_stage_name.argtypes = [C.c_void_p, C.c_int]
         L.cdft_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.cdft_autotune.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.cdft_ddft_split_operator.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp, _dp]
+        L.cdft_ddft_crank_nicolson.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp, _dp,
+                                               C.POINTER(C.c_int)]
+        L.cdft_picard_solve.argtypes = [C.c_void_p, C.POINTER(_Picard), _dp, C.c_double, C.c_double, _dp, _dp,
+                                        C.POINTER(_PicardResult)]
         L.cdft_saddle_search.argtypes = [C.c_void_p, C.POINTER(_SaddleConfig), _dp, _dp, C.c_double, C.c_double,
                                          C.c_ulonglong, _dp, _dp, C.POINTER(_SaddleResult)]
         L.cdft_smallest_eigenpair.argtypes = [C.c_void_p, C.POINTER(_EigenConfig), _dp, _dp, C.c_double, _dp,
@@ -396,6 +411,40 @@ class Context:
         r = {k: getattr(res, k) for k, _ in _SaddleResult._fields_}
         r["converged"] = bool(r["converged"])
         return ro.reshape(self.shape), eo.reshape(self.shape), r
+
+    def ddft_split_operator(self, rho0, mu, n_steps, dt=1e-4, diffusion_coefficient=1.0, min_density=1e-18,
+                            vext=None):
+        """split_operator_step, n_steps times; returns (rho, energy)."""
+        cfg = _StepConfig(float(dt), float(diffusion_coefficient), float(min_density), float(dt), 1e-4, 100)
+        rho = _f64(rho0).copy()
+        e = C.c_double()
+        _check(lib().cdft_ddft_split_operator(self._h, C.byref(cfg), int(n_steps), None if vext is None else
+                                              _p(_f64(vext)), float(mu), _p(rho), C.byref(e)))
+        return rho.reshape(self.shape), e.value
+
+    def ddft_crank_nicolson(self, rho0, mu, n_steps, dt=1e-4, diffusion_coefficient=1.0, min_density=1e-18,
+                            fp_tolerance=1e-4, fp_max_iterations=100, vext=None):
+        """crank_nicholson_step, n_steps times; returns (rho, energy, iterations of the last step)."""
+        cfg = _StepConfig(float(dt), float(diffusion_coefficient), float(min_density), float(dt), float(fp_tolerance),
+                          int(fp_max_iterations))
+        rho = _f64(rho0).copy()
+        e, it = C.c_double(), C.c_int()
+        _check(lib().cdft_ddft_crank_nicolson(self._h, C.byref(cfg), int(n_steps), None if vext is None else
+                                              _p(_f64(vext)), float(mu), _p(rho), C.byref(e), C.byref(it)))
+        return rho.reshape(self.shape), e.value, it.value
+
+    def picard(self, rho0, mu, mixing=0.01, min_density=1e-30, tolerance=1e-6, max_iterations=5000, target_mass=0.0,
+               background=None, vext=None):
+        """Picard::solve; returns (rho, result dict)."""
+        cfg = _Picard(float(mixing), float(min_density), float(tolerance), int(max_iterations))
+        res = _PicardResult()
+        rho = _f64(rho0).copy()
+        _check(lib().cdft_picard_solve(self._h, C.byref(cfg), None if vext is None else _p(_f64(vext)), float(mu),
+                                       float(target_mass), None if background is None else _p(_f64(background)),
+                                       _p(rho), C.byref(res)))
+        r = {k: getattr(res, k) for k, _ in _PicardResult._fields_}
+        r["converged"] = bool(r["converged"])
+        return rho.reshape(self.shape), r
 
     def autotune(self, reps=3):
         """Tune the general schedule's FFT radix orders and launch shapes for this grid; returns the time of one

@@ -299,6 +299,42 @@ CDFT_API int cdft_saddle_search(cdft_ctx* ctx, const cdft_saddle_config* config,
                        const double* vext, double mu, double target_mass, unsigned long long seed, double* rho_out,
                        double* eigenvector, cdft_saddle_result* result);
 
+/* split_operator_step (include/dft/algorithms/dynamics.hpp:405-462), n_steps
+ * times: half a step of exact diffusion exp(-D k^2 dt / 2) (continuous k^2,
+ * compute_k_squared), a forward-Euler step of the excess term, another half
+ * step; densities clamped at min_density after each part.  The force
+ * callback is the grand-potential gradient at mu (vext nullable).  energy
+ * (nullable) receives the energy of the final density. */
+CDFT_API int cdft_ddft_split_operator(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext,
+                             double mu, double* rho, double* energy);
+
+/* crank_nicholson_step (dynamics.hpp:464-538), n_steps times: the fixed-point
+ * iteration rho1_k = ((1 - a) rho0_k + dt/2 (R0_k + R1_k)) / (1 + a),
+ * a = D k^2 dt / 2, until the largest change is below fp_tolerance or
+ * fp_max_iterations.  iterations (nullable) receives the last step's count. */
+CDFT_API int cdft_ddft_crank_nicolson(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext,
+                             double mu, double* rho, double* energy, int* iterations);
+
+/* picard::Picard (include/dft/algorithms/picard.hpp:25-33). */
+typedef struct {
+  double mixing, min_density, tolerance;
+  int max_iterations;
+} cdft_picard;
+
+/* picard::PicardResult (picard.hpp:17-23). */
+typedef struct {
+  double grand_potential, residual;
+  int iterations, converged;
+} cdft_picard_result;
+
+/* Picard::solve (picard.hpp:63-124) with the grand-potential forces at mu:
+ * rho *= exp(-mixing f / dV) until the rms force or the change of Omega is
+ * below the tolerance.  target_mass > 0 applies fixed_mass_constraint
+ * (minimization.hpp:157-180) after every update with background (nullable:
+ * zero).  rho (N values) is updated in place. */
+CDFT_API int cdft_picard_solve(cdft_ctx* ctx, const cdft_picard* config, const double* vext, double mu,
+                      double target_mass, const double* background, double* rho, cdft_picard_result* result);
+
 /* ---------------------------------------------------------------- adjoint
  * The reference has no derivative of its dynamics; the entry points below
  * extend its DDFT integrating-factor step (dynamics.hpp:333-401) with a

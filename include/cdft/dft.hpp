@@ -644,7 +644,150 @@ struct Simulation {
     return r;
   }
 };
+
+// dynamics.hpp:395-401
+struct StepResult {
+  std::vector<vec> densities;
+  double energy{0.0};
+  double dt_used{0.0};
+};
+
+// compute_k_squared (dynamics.hpp:310-318): |k|^2 of the half spectrum, Nyquist positive
+inline vec compute_k_squared(const Grid& grid) {
+  const long nx = grid.shape[0], ny = grid.shape[1], nzh = grid.shape[2] / 2 + 1;
+  const double PI = 3.14159265358979323846;
+  const double dk[3] = {2.0 * PI / (nx * grid.dx), 2.0 * PI / (ny * grid.dx), 2.0 * PI / (grid.shape[2] * grid.dx)};
+  vec k2(static_cast<size_t>(nx * ny * nzh));
+  size_t i = 0;
+  for (long ix = 0; ix < nx; ++ix)
+    for (long iy = 0; iy < ny; ++iy)
+      for (long iz = 0; iz < nzh; ++iz, ++i) {
+        const double kx = dk[0] * static_cast<double>(ix <= nx / 2 ? ix : ix - nx);
+        const double ky = dk[1] * static_cast<double>(iy <= ny / 2 ? iy : iy - ny);
+        const double kz = dk[2] * static_cast<double>(iz);
+        k2[i] = kx * kx + ky * ky + kz * kz;
+      }
+  return k2;
+}
+
+// diffusion_propagator (dynamics.hpp:320-322)
+inline vec diffusion_propagator(const vec& k_squared, double D, double dt) {
+  vec p(k_squared.size());
+  for (size_t i = 0; i < p.size(); ++i) p[i] = std::exp(-D * k_squared[i] * dt);
+  return p;
+}
+
+namespace detail {
+inline const functionals::GrandPotentialCallback& device_callback(const ForceCallback& compute, const Grid& grid,
+                                                                  const std::vector<vec>& densities) {
+  const auto* cb = compute.target<functionals::GrandPotentialCallback>();
+  if (cb == nullptr) throw std::invalid_argument("the device steps need a Functional::grand_potential_callback");
+  if (densities.size() != 1 || static_cast<long>(densities[0].size()) != grid.total_points())
+    throw std::invalid_argument("the device steps take one density of the grid's size");
+  return *cb;
+}
+}  // namespace detail
+
+// split_operator_step (dynamics.hpp:410-462).  The device forms
+// sqrt(exp(-D k^2 dt)) itself from config, so `propagator` must be
+// diffusion_propagator(k_squared, config.diffusion_coefficient, config.dt),
+// as every reference caller passes it.
+inline StepResult split_operator_step(const std::vector<vec>& densities, const Grid& grid, const vec& k_squared,
+                                      const vec& propagator, const ForceCallback& compute, const StepConfig& config) {
+  const auto& cb = detail::device_callback(compute, grid, densities);
+  if (k_squared.size() != propagator.size()) throw std::invalid_argument("k_squared and propagator sizes differ");
+  cdft_step_config c{config.dt, config.diffusion_coefficient, config.min_density, config.dt_max,
+                     config.fp_tolerance, config.fp_max_iterations};
+  StepResult r{densities, 0.0, config.dt};
+  ::dft::detail::check(cdft_ddft_split_operator(cb.functional->weights.ctx.get(), &c, 1, nullptr,
+                                                cb.chemical_potential, r.densities[0].data(), &r.energy));
+  return r;
+}
+
+// crank_nicholson_step (dynamics.hpp:472-538)
+inline StepResult crank_nicholson_step(const std::vector<vec>& densities, const Grid& grid, const vec& k_squared,
+                                       const ForceCallback& compute, const StepConfig& config) {
+  const auto& cb = detail::device_callback(compute, grid, densities);
+  (void)k_squared;  // formed on the device from the grid
+  cdft_step_config c{config.dt, config.diffusion_coefficient, config.min_density, config.dt_max,
+                     config.fp_tolerance, config.fp_max_iterations};
+  StepResult r{densities, 0.0, config.dt};
+  ::dft::detail::check(cdft_ddft_crank_nicolson(cb.functional->weights.ctx.get(), &c, 1, nullptr,
+                                                cb.chemical_potential, r.densities[0].data(), &r.energy, nullptr));
+  return r;
+}
 }  // namespace dynamics
+
+namespace picard {
+using ForceFunction = dynamics::ForceCallback;
+using Constraint = std::function<std::vector<vec>(const std::vector<vec>&)>;
+
+// fixed_mass_constraint (minimization.hpp:157-180) as a callable the device
+// solver recognises (one species)
+struct FixedMassConstraint {
+  double target_mass;
+  vec background;
+  double cell_volume;
+  double min_density;
+  std::vector<vec> operator()(const std::vector<vec>& densities) const {
+    auto result = densities;
+    if (result.empty()) return result;
+    double cur = 0.0;
+    for (size_t i = 0; i < result[0].size(); ++i) cur += result[0][i] - background[i];
+    cur *= cell_volume;
+    if (std::abs(cur) > 1e-30)
+      for (size_t i = 0; i < result[0].size(); ++i)
+        result[0][i] = std::max(background[i] + target_mass / cur * (result[0][i] - background[i]), min_density);
+    return result;
+  }
+};
+inline Constraint fixed_mass_constraint(std::vector<double> target_masses, std::vector<vec> backgrounds,
+                                        double cell_volume, double min_density = 1e-30) {
+  if (target_masses.size() != 1 || backgrounds.size() != 1)
+    throw std::invalid_argument("fixed_mass_constraint: one species on the device");
+  return FixedMassConstraint{target_masses[0], std::move(backgrounds[0]), cell_volume, min_density};
+}
+
+// PicardResult (picard.hpp:17-23)
+struct PicardResult {
+  std::vector<vec> densities;
+  double grand_potential{0.0};
+  double residual{0.0};
+  int iterations{0};
+  bool converged{false};
+};
+
+// Picard (picard.hpp:25-33); solve() runs on the device with the functional's
+// grand-potential callback and no constraint or a fixed_mass_constraint.
+struct Picard {
+  double mixing{0.01};
+  double min_density{1e-30};
+  double tolerance{1e-6};
+  int max_iterations{5000};
+  int log_interval{500};
+
+  PicardResult solve(std::vector<vec> densities, const ForceFunction& compute, double cell_volume,
+                     const Constraint& constraint = {}) const {
+    const auto* cb = compute.target<functionals::GrandPotentialCallback>();
+    if (cb == nullptr) throw std::invalid_argument("Picard::solve on the GPU needs a Functional::grand_potential_callback");
+    if (densities.size() != 1) throw std::invalid_argument("Picard::solve on the GPU takes one density");
+    (void)cell_volume;  // the functional's grid fixes it
+    double target = 0.0;
+    const double* bg = nullptr;
+    if (constraint) {
+      const auto* fm = constraint.target<FixedMassConstraint>();
+      if (fm == nullptr) throw std::invalid_argument("Picard::solve on the GPU supports only fixed_mass_constraint");
+      target = fm->target_mass;
+      bg = fm->background.data();
+    }
+    cdft_picard c{mixing, min_density, tolerance, max_iterations};
+    cdft_picard_result cr{};
+    ::dft::detail::check(cdft_picard_solve(cb->functional->weights.ctx.get(), &c, nullptr, cb->chemical_potential,
+                                           target, bg, densities[0].data(), &cr));
+    return PicardResult{std::move(densities), cr.grand_potential, cr.residual, cr.iterations, cr.converged != 0};
+  }
+};
+}  // namespace picard
 }  // namespace algorithms
 
 }  // namespace dft

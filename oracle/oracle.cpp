@@ -23,6 +23,9 @@
 //   DDFT integrating-factor step ....... include/dft/algorithms/dynamics.hpp:79-401,582-693
 //   fixed-dt DDFT map + Gaussian wells . (no counterpart: the map the adjoint differentiates)
 //   smallest Hessian eigenpair ......... include/dft/algorithms/saddle_point.hpp:28-38,159-270
+//   saddle search ...................... include/dft/algorithms/saddle_point.hpp:367-589
+//   split-operator / Crank-Nicolson .... include/dft/algorithms/dynamics.hpp:405-538
+//   Picard ............................. include/dft/algorithms/picard.hpp:63-124
 //
 // Where the reference's output depends on FFTW behaviour it does not document
 // (c2r of a half spectrum that is not Hermitian, which happens for the odd
@@ -36,6 +39,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <limits>
 #include <complex>
 #include <cstdint>
 #include <cstring>
@@ -1312,8 +1316,14 @@ struct DdftWork {
   }
   // divergence_flux + subtract_laplacian + r2c (dynamics.hpp:135-229)
   void excess_rhs_fourier(const std::vector<double>& rho, const std::vector<double>& forces, cd* out) {
+    std::vector<double> rhs(c->N);
+    excess_rhs_real(rho, forces, rhs);
+    c->fft->r2c(rhs.data(), out);
+  }
+  // excess_rhs_real (dynamics.hpp:233-241)
+  void excess_rhs_real(const std::vector<double>& rho, const std::vector<double>& forces, std::vector<double>& rhs) {
     const long nx = c->nx, ny = c->ny, nz = c->nz, N = c->N;
-    std::vector<double> x(N), rhs(N);
+    std::vector<double> x(N);
     for (long i = 0; i < N; ++i) x[i] = forces[i] / c->dv;
     double D = 0.5 / (c->dx * c->dx), D2 = 1.0 / (c->dx * c->dx);
     long nyz = ny * nz;
@@ -1336,7 +1346,76 @@ struct DdftWork {
         }
       }
     }
-    c->fft->r2c(rhs.data(), out);
+  }
+  // compute_k_squared (dynamics.hpp:310-318): continuous |k|^2, Nyquist positive
+  std::vector<double> k_squared() const {
+    std::vector<double> k2(static_cast<size_t>(c->M));
+    const double dkx = 2.0 * PI / (c->nx * c->dx), dky = 2.0 * PI / (c->ny * c->dx), dkz = 2.0 * PI / (c->nz * c->dx);
+    long pos = 0;
+    for (long ix = 0; ix < c->nx; ++ix)
+      for (long iy = 0; iy < c->ny; ++iy)
+        for (long iz = 0; iz < c->nzh; ++iz, ++pos) {
+          const double kx = dkx * static_cast<double>(ix <= c->nx / 2 ? ix : ix - c->nx);
+          const double ky = dky * static_cast<double>(iy <= c->ny / 2 ? iy : iy - c->ny);
+          const double kz = dkz * static_cast<double>(iz);
+          k2[pos] = kx * kx + ky * ky + kz * kz;
+        }
+    return k2;
+  }
+  // split_operator_step (dynamics.hpp:410-462)
+  double split_step(std::vector<double>& rho, double D, double dt, double lo) {
+    const long N = c->N, M = c->M;
+    const std::vector<double> k2 = k_squared();
+    std::vector<cd> fk(M);
+    std::vector<double> f(N), rhs(N), tmp(N);
+    const double inv_n = 1.0 / static_cast<double>(N);
+    auto half = [&](const std::vector<double>& in, std::vector<double>& out) {
+      c->fft->r2c(in.data(), fk.data());
+      for (long i = 0; i < M; ++i) fk[i] *= std::sqrt(std::exp(-D * k2[i] * dt));
+      c->fft->c2r(fk.data(), out.data());
+      for (long i = 0; i < N; ++i) out[i] = std::max(out[i] * inv_n, lo);
+    };
+    std::vector<double> rs(N);
+    half(rho, rs);
+    compute(rs, f);
+    excess_rhs_real(rs, f, rhs);
+    for (long i = 0; i < N; ++i) tmp[i] = std::max(rs[i] + dt * rhs[i], lo);
+    half(tmp, rho);
+    return compute(rho, f);
+  }
+  // crank_nicholson_step (dynamics.hpp:472-538)
+  double cn_step(std::vector<double>& rho, double D, double dt, double lo, double tol, int max_it, int* iters) {
+    const long N = c->N, M = c->M;
+    const std::vector<double> k2 = k_squared();
+    std::vector<cd> p0(M), r0(M), r1(M), w(M);
+    std::vector<double> f(N), rhs(N), d1 = rho, nw(N);
+    c->fft->r2c(rho.data(), p0.data());
+    compute(rho, f);
+    excess_rhs_fourier(rho, f, r0.data());
+    const double inv_n = 1.0 / static_cast<double>(N);
+    int it = 0;
+    for (; it < max_it; ++it) {
+      compute(d1, f);
+      excess_rhs_fourier(d1, f, r1.data());
+      for (long i = 0; i < M; ++i) {
+        const double a = D * k2[i] * dt * 0.5;
+        w[i] = ((1.0 - a) * p0[i] + (dt * 0.5) * (r0[i] + r1[i])) * (1.0 / (1.0 + a));
+      }
+      c->fft->c2r(w.data(), nw.data());
+      double md = 0.0;
+      for (long i = 0; i < N; ++i) {
+        const double v = std::max(nw[i] * inv_n, lo);
+        md = std::max(md, std::fabs(v - d1[i]));
+        d1[i] = v;
+      }
+      if (md < tol) {
+        ++it;
+        break;
+      }
+    }
+    *iters = it;
+    rho = d1;
+    return compute(rho, f);
   }
   // apply_propagator (dynamics.hpp:247-304)
   double apply_propagator(const cd* rho0, const cd* r0, const cd* r1, std::vector<double>& d1) {
@@ -1642,6 +1721,71 @@ int orc_saddle_search(void* h, const double* cfg, const double* rho0, const doub
   out[2] = outer + 1;
   out[3] = converged ? 1.0 : 0.0;
   return 0;
+}
+
+// split_operator_step (dynamics.hpp:405-462) n_steps times; cfg: dt, D, min_density.  Returns the final energy.
+double orc_ddft_split_operator(void* h, const double* cfg, int n_steps, const double* vext, double mu,
+                               double* rho_inout) {
+  auto* c = static_cast<Ctx*>(h);
+  DdftWork w(c, vext, mu);
+  std::vector<double> rho(rho_inout, rho_inout + c->N);
+  double e = 0.0;
+  for (int s = 0; s < n_steps; ++s) e = w.split_step(rho, cfg[1], cfg[0], cfg[2]);
+  std::memcpy(rho_inout, rho.data(), sizeof(double) * c->N);
+  return e;
+}
+
+// crank_nicholson_step (dynamics.hpp:464-538) n_steps times; cfg: dt, D, min_density, tolerance, max iterations.
+// out: final energy, iterations of the last step.
+void orc_ddft_crank_nicolson(void* h, const double* cfg, int n_steps, const double* vext, double mu,
+                             double* rho_inout, double* out) {
+  auto* c = static_cast<Ctx*>(h);
+  DdftWork w(c, vext, mu);
+  std::vector<double> rho(rho_inout, rho_inout + c->N);
+  double e = 0.0;
+  int it = 0;
+  for (int s = 0; s < n_steps; ++s) e = w.cn_step(rho, cfg[1], cfg[0], cfg[2], cfg[3], static_cast<int>(cfg[4]), &it);
+  std::memcpy(rho_inout, rho.data(), sizeof(double) * c->N);
+  out[0] = e;
+  out[1] = it;
+}
+
+// Picard::solve (picard.hpp:63-124); cfg: mixing, min_density, tolerance, max iterations.
+// target_mass > 0 applies fixed_mass_constraint (minimization.hpp:157-180) with background (nullable: 0).
+// out: Omega, residual, iterations, converged.
+void orc_picard(void* h, const double* cfg, const double* vext, double mu, double target_mass, const double* bg,
+                double* rho, double* out) {
+  auto* c = static_cast<Ctx*>(h);
+  const long N = c->N;
+  const double mixing = cfg[0], lo = cfg[1], tol = cfg[2];
+  const int max_it = static_cast<int>(cfg[3]);
+  std::vector<double> f(N);
+  double parts[6], omega = 0.0, omega_prev = std::numeric_limits<double>::max(), residual = 0.0;
+  int it = 0;
+  for (; it < max_it; ++it) {
+    c->total(rho, vext, &mu, parts, f.data());
+    omega = parts[5];
+    double s2 = 0.0;
+    for (long i = 0; i < N; ++i) s2 += f[i] * f[i];
+    residual = std::sqrt(s2 / static_cast<double>(N));
+    if (residual < tol || (it > 0 && std::fabs(omega - omega_prev) < tol)) break;
+    omega_prev = omega;
+    for (long i = 0; i < N; ++i) rho[i] = std::max(rho[i] * std::exp(-mixing * f[i] / c->dv), lo);
+    if (target_mass > 0.0) {
+      double cur = 0.0;
+      for (long i = 0; i < N; ++i) cur += rho[i] - (bg ? bg[i] : 0.0);
+      cur *= c->dv;
+      if (std::fabs(cur) > 1e-30)
+        for (long i = 0; i < N; ++i) {
+          const double b = bg ? bg[i] : 0.0;
+          rho[i] = std::max(b + (target_mass / cur) * (rho[i] - b), lo);
+        }
+    }
+  }
+  out[0] = omega;
+  out[1] = residual;
+  out[2] = it;
+  out[3] = (residual < tol || (it > 0 && std::fabs(omega - omega_prev) < tol)) ? 1.0 : 0.0;
 }
 
 // Fixed-dt forward map of the DDFT adjoint: n_steps steps of fixed_step.

@@ -75,6 +75,10 @@ def lib():
         L.orc_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.orc_smallest_eigenpair.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int, _dp, _dp,
                                              C.c_double, _dp, _dp, _dp]
+        L.orc_ddft_split_operator.restype = C.c_double
+        L.orc_ddft_split_operator.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp]
+        L.orc_ddft_crank_nicolson.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp, _dp]
+        L.orc_picard.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_saddle_search.restype = C.c_int
         L.orc_saddle_search.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.c_ulonglong, _dp, _dp,
                                         _dp]
@@ -273,6 +277,34 @@ class Oracle:
         lib().orc_smallest_eigenpair(self._h, float(tolerance), int(max_iterations), float(hessian_eps),
                                      int(bool(fixed_boundary)), _p(r), _p(ve), float(mu), _p(v), _p(vo), _p(out))
         return vo.reshape(self.shape), out[0], int(out[1]), bool(out[2])
+
+    def ddft_split_operator(self, rho0, mu, n_steps, dt=1e-4, diffusion_coefficient=1.0, min_density=1e-18,
+                            vext=None):
+        rho = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1).copy()
+        ve = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
+        cfg = np.array([dt, diffusion_coefficient, min_density])
+        e = lib().orc_ddft_split_operator(self._h, _p(cfg), int(n_steps), _p(ve), float(mu), _p(rho))
+        return rho.reshape(self.shape), e
+
+    def ddft_crank_nicolson(self, rho0, mu, n_steps, dt=1e-4, diffusion_coefficient=1.0, min_density=1e-18,
+                            fp_tolerance=1e-4, fp_max_iterations=100, vext=None):
+        rho = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1).copy()
+        ve = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
+        cfg = np.array([dt, diffusion_coefficient, min_density, fp_tolerance, fp_max_iterations], dtype=np.float64)
+        out = np.zeros(2)
+        lib().orc_ddft_crank_nicolson(self._h, _p(cfg), int(n_steps), _p(ve), float(mu), _p(rho), _p(out))
+        return rho.reshape(self.shape), out[0], int(out[1])
+
+    def picard(self, rho0, mu, mixing=0.01, min_density=1e-30, tolerance=1e-6, max_iterations=5000, target_mass=0.0,
+               background=None, vext=None):
+        rho = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1).copy()
+        ve = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
+        bg = None if background is None else np.ascontiguousarray(background, dtype=np.float64).reshape(-1)
+        cfg = np.array([mixing, min_density, tolerance, max_iterations], dtype=np.float64)
+        out = np.zeros(4)
+        lib().orc_picard(self._h, _p(cfg), _p(ve), float(mu), float(target_mass), _p(bg), _p(rho), _p(out))
+        return rho.reshape(self.shape), dict(grand_potential=out[0], residual=out[1], iterations=int(out[2]),
+                                             converged=bool(out[3]))
 
     def saddle_search(self, rho0, mu, target_mass, seed=0, fire=None, eigen=None, param="unbounded", rho_min=1e-18,
                       rho_max=1.0, omega_tolerance=1e-5, tolerance_start=1e-3, tolerance_factor=10.0,

@@ -125,6 +125,23 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   search.max_outer_iterations = 3;
   search.seed = 1;
   const auto sres_saddle = search.find(model, func.weights, rho, mu, mass0);
+  algorithms::dynamics::StepConfig sc;
+  sc.dt = 1e-3;
+  const auto k2 = algorithms::dynamics::compute_k_squared(model.grid);
+  const auto prop = algorithms::dynamics::diffusion_propagator(k2, sc.diffusion_coefficient, sc.dt);
+  const auto split = algorithms::dynamics::split_operator_step({rho}, model.grid, k2, prop, cb, sc);
+  sc.fp_tolerance = 1e-9;
+  const auto cn = algorithms::dynamics::crank_nicholson_step({rho}, model.grid, k2, cb, sc);
+  algorithms::picard::Picard pc;
+  pc.mixing = 0.02;
+  pc.tolerance = 1e-8;
+  pc.max_iterations = 200;
+  const auto pres = pc.solve({rho}, cb, model.grid.cell_volume(),
+                             algorithms::picard::fixed_mass_constraint({mass0}, {vec(rho.size(), 0.0)},
+                                                                       model.grid.cell_volume()));
+  write_doubles(out + "/split_rho.bin", split.densities[0]);
+  write_doubles(out + "/cn_rho.bin", cn.densities[0]);
+  write_doubles(out + "/picard_rho.bin", pres.densities[0]);
 
   algorithms::dynamics::Simulation sim;
   sim.step.dt = 1e-3;
@@ -148,11 +165,13 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
               "\"callback_force_maxdiff\": %.3g, \"fire_iterations\": %d, \"fire_grand_potential\": %.17g, "
               "\"fixed_mass_target\": %.17g, \"fixed_mass_iterations\": %d, \"fixed_mass_free_energy\": %.17g, "
               "\"eigenvalue\": %.17g, \"eigen_iterations\": %d, \"saddle_omega\": %.17g, "
-              "\"saddle_eigenvalue\": %.17g, \"saddle_outer\": %d, "
+              "\"saddle_eigenvalue\": %.17g, \"saddle_outer\": %d, \"split_energy\": %.17g, "
+              "\"cn_energy\": %.17g, \"picard_iterations\": %d, "
               "\"ddft_times\": [",
               mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential,
               0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations,
-              sres_saddle.grand_potential, sres_saddle.eigenvalue, sres_saddle.outer_iterations);
+              sres_saddle.grand_potential, sres_saddle.eigenvalue, sres_saddle.outer_iterations, split.energy,
+              cn.energy, pres.iterations);
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);

@@ -189,3 +189,41 @@ def test_fixed_mass_mixed_tracks_fp64():
     assert float(np.sum(rmx)) * dx ** 3 == pytest.approx(target, rel=1e-12)
     assert smx["free_energy"] == pytest.approx(s64["free_energy"], rel=1e-6)
     assert np.max(np.abs(rmx - r64)) / np.max(np.abs(r64)) < 1e-4
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_split_operator_and_crank_nicolson_match_oracle(precision):
+    shape, dx = (16, 16, 16), 0.25
+    ctx, orc = _pair(shape, dx, precision=precision)
+    rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    mu = orc.bulk_mu(0.05)
+    tol = 1e-10 if precision == "fp64" else 2e-5
+    rg, eg = ctx.ddft_split_operator(rho0, mu, 4, dt=1e-3)
+    ro, eo = orc.ddft_split_operator(rho0, mu, 4, dt=1e-3)
+    es = float(np.max(np.abs(rg - ro)) / np.max(np.abs(ro)))
+    rg2, eg2, itg = ctx.ddft_crank_nicolson(rho0, mu, 3, dt=1e-3, fp_tolerance=1e-9)
+    ro2, eo2, ito = orc.ddft_crank_nicolson(rho0, mu, 3, dt=1e-3, fp_tolerance=1e-9)
+    ec = float(np.max(np.abs(rg2 - ro2)) / np.max(np.abs(ro2)))
+    print(f"{precision}: split-operator {es:.1e}, Crank-Nicolson {ec:.1e} ({itg} / {ito} iterations)", flush=True)
+    assert es < tol and ec < tol
+    assert eg == pytest.approx(eo, rel=tol) and eg2 == pytest.approx(eo2, rel=tol)
+    if precision == "fp64":
+        assert itg == ito
+
+
+@pytest.mark.parametrize("target", [0.0, 1.0])
+def test_picard_matches_oracle(target):
+    shape, dx = (16, 16, 16), 0.25
+    ctx, orc = _pair(shape, dx)
+    rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    mu = orc.bulk_mu(0.05)
+    mass = target * float(np.sum(rho0)) * dx ** 3
+    kw = dict(mixing=0.02, tolerance=1e-8, max_iterations=200, target_mass=mass)
+    rg, resg = ctx.picard(rho0, mu, **kw)
+    ro, reso = orc.picard(rho0, mu, **kw)
+    e = float(np.max(np.abs(rg - ro)) / np.max(np.abs(ro)))
+    print(f"picard target={target}: {resg['iterations']} iterations, profile {e:.1e}", flush=True)
+    assert resg["iterations"] == reso["iterations"] and resg["converged"] == reso["converged"]
+    assert e < 1e-10
+    assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-10)
+    assert resg["residual"] == pytest.approx(reso["residual"], rel=1e-8)

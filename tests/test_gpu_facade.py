@@ -70,6 +70,18 @@ def test_cpp_facade_matches_oracle(tmp_path):
     assert out["saddle_omega"] == pytest.approx(rs["grand_potential"], rel=1e-6)
     assert out["saddle_eigenvalue"] == pytest.approx(rs["eigenvalue"], rel=1e-4, abs=1e-7)
 
+    # the other DDFT steps and Picard with a fixed-mass constraint
+    rs1, es1 = orc.ddft_split_operator(rho, mu, 1, dt=1e-3)
+    assert out["split_energy"] == pytest.approx(es1, rel=1e-10)
+    assert _rel(np.fromfile(tmp_path / "split_rho.bin").reshape(shape), rs1) < 1e-10
+    rc1, ec1, _ = orc.ddft_crank_nicolson(rho, mu, 1, dt=1e-3, fp_tolerance=1e-9)
+    assert out["cn_energy"] == pytest.approx(ec1, rel=1e-10)
+    assert _rel(np.fromfile(tmp_path / "cn_rho.bin").reshape(shape), rc1) < 1e-10
+    rp, rpres = orc.picard(rho, mu, mixing=0.02, tolerance=1e-8, max_iterations=200, target_mass=mass0,
+                           background=np.zeros(shape))
+    assert out["picard_iterations"] == rpres["iterations"]
+    assert _rel(np.fromfile(tmp_path / "picard_rho.bin").reshape(shape), rp) < 1e-10
+
     rd, resd, ed = orc.ddft_run(rho, mu, 6, dt=1e-3, dt_max=1e-3)
     assert _rel(np.fromfile(tmp_path / "ddft_rho.bin").reshape(shape), rd) < 1e-9
     # recorded at step 0 and every log_interval = 2 steps, as Simulation::run does

@@ -626,3 +626,58 @@ def test_eigen_solver_matches_dense_hessian():
     assert lam == pytest.approx(lam_dense, rel=1e-6, abs=1e-9)
     hv = 0.5 * (H + H.T) @ v.ravel()
     assert np.linalg.norm(hv - lam * v.ravel()) < 1e-5 * abs(lam_dense) + 1e-8
+
+
+# ------------------------------------------------------------------ split-operator, Crank-Nicolson, Picard
+def _ideal_mode(shape, dx, rho0=0.2, eps=1e-4):
+    x = np.arange(shape[0])
+    return (rho0 * (1 + eps * np.cos(2 * PI * x / shape[0])))[:, None, None] * np.ones(shape)
+
+
+def _mode_amplitude(rho, n):
+    x = np.arange(n)
+    return np.sum(rho[:, 0, 0] * np.cos(2 * PI * x / n)) / (n / 2)
+
+
+def test_split_operator_ideal_gas_mode_decay():
+    # ideal gas: the excess term vanishes to O(eps^2), so a mode decays by exp(-D k^2 dt) per step
+    shape, dx = (16, 4, 4), 0.5
+    orc = O.Oracle(shape, dx, model="none")
+    rho = _ideal_mode(shape, dx)
+    k2 = (2 * PI / (16 * dx)) ** 2
+    dt, n = 1e-2, 5
+    out, _ = orc.ddft_split_operator(rho, math.log(0.2), n, dt=dt, diffusion_coefficient=1.0)
+    ratio = _mode_amplitude(out, 16) / _mode_amplitude(rho, 16)
+    assert ratio == pytest.approx(math.exp(-k2 * dt * n), rel=1e-6)
+    assert float(np.sum(out)) == pytest.approx(float(np.sum(rho)), rel=1e-13)
+
+
+def test_crank_nicolson_ideal_gas_mode_decay():
+    shape, dx = (16, 4, 4), 0.5
+    orc = O.Oracle(shape, dx, model="none")
+    rho = _ideal_mode(shape, dx)
+    k2 = (2 * PI / (16 * dx)) ** 2
+    dt, n = 1e-2, 5
+    out, _, it = orc.ddft_crank_nicolson(rho, math.log(0.2), n, dt=dt, fp_tolerance=1e-14)
+    a = k2 * dt / 2
+    ratio = _mode_amplitude(out, 16) / _mode_amplitude(rho, 16)
+    assert ratio == pytest.approx(((1 - a) / (1 + a)) ** n, rel=1e-6)
+    assert it >= 2
+
+
+def test_picard_ideal_gas_reaches_exp_mu():
+    # ln rho - mu = 0: rho = exp(mu) everywhere; with a fixed mass the uniform density target / V
+    shape, dx = (8, 8, 8), 0.5
+    orc = O.Oracle(shape, dx, model="none")
+    rho0 = 0.1 * (1.0 + 0.2 * np.random.default_rng(3).random(shape))
+    # one update: rho exp(-mixing (ln rho - mu) dV / dV), exactly
+    one, _ = orc.picard(rho0, math.log(0.3), mixing=0.5, tolerance=1e-30, max_iterations=1)
+    np.testing.assert_allclose(one, rho0 * np.exp(-0.5 * (np.log(rho0) - math.log(0.3))), rtol=1e-13)
+    # converged (the change of Omega stops it once it is below the tolerance)
+    out, res = orc.picard(rho0, math.log(0.3), mixing=0.5, tolerance=1e-12, max_iterations=2000)
+    assert res["converged"]
+    np.testing.assert_allclose(out, 0.3, rtol=1e-6)
+    target = 0.25 * rho0.size * dx ** 3
+    out, res = orc.picard(rho0, math.log(0.3), mixing=0.5, tolerance=1e-12, max_iterations=2000, target_mass=target)
+    np.testing.assert_allclose(out, 0.25, rtol=1e-6)
+    assert float(np.sum(out)) * dx ** 3 == pytest.approx(target, rel=1e-13)
