@@ -8,10 +8,53 @@
 #include <vector>
 
 #include "context.hpp"
+#include "device_common.hpp"
 
 namespace cdft {
 
 namespace {
+
+// One thread per lattice cell: the mean-field weight table w(s) dV / N in the
+// real layout with minimum-image offsets s, zero beyond the cut-off reach
+// (mean_field.hpp:160-196), and per-block sums of w dV for a_vdw.
+constexpr int kMfThreads = 256;
+__global__ __launch_bounds__(kMfThreads) void k_mf_table(PairPotential pot, int split, int scheme, double kT,
+                                                         double dx, int nx, int ny, int nz, double reach2,
+                                                         double* __restrict__ table, double* __restrict__ wsum) {
+  const long n = static_cast<long>(nx) * ny * nz;
+  const long i = static_cast<long>(blockIdx.x) * kMfThreads + threadIdx.x;
+  double v[1] = {0.0};
+  if (i < n) {
+    const long iz = i % nz, iy = (i / nz) % ny, ix = i / (static_cast<long>(nz) * ny);
+    const long sx = ix <= nx / 2 ? ix : ix - nx;
+    const long sy = iy <= ny / 2 ? iy : iy - ny;
+    const long sz = iz <= nz / 2 ? iz : iz - nz;
+    const double d2 = static_cast<double>(sx * sx + sy * sy + sz * sz) * dx * dx;
+    double w = 0.0;
+    if (!(pot.rc > 0.0 && d2 > reach2)) w = lattice_cell_weight(pot, split, scheme, kT, dx, sx, sy, sz);
+    const double dv = dx * dx * dx;
+    table[i] = w * dv / static_cast<double>(n);
+    v[0] = w * dv;
+  }
+  block_reduce<1>(v, wsum + blockIdx.x);
+}
+
+// Real part of the weight spectrum in the working precision: FP64 values, or
+// FP32 (value, residual) planes.
+__global__ __launch_bounds__(256) void k_mf_store(const cx<double>* __restrict__ spec, long m, int mixed,
+                                                  void* __restrict__ mf) {
+  const long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= m) return;
+  const double re = spec[i].re;
+  if (mixed) {
+    float* f = static_cast<float*>(mf);
+    const float hi = static_cast<float>(re);
+    f[i] = hi;
+    f[m + i] = static_cast<float>(re - static_cast<double>(hi));
+  } else {
+    static_cast<double*>(mf)[i] = re;
+  }
+}
 
 FftPlan make_plan(int L) {
   FftPlan p{};
@@ -55,12 +98,11 @@ void Context::clear_graphs() {
 }
 
 Context::~Context() {
-  hipSetDevice(device);
+  (void)hipSetDevice(device);  // a destructor has no error path; frees below are best effort
   clear_graphs();
   void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
                   d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
                   ftw64,   ftw32,   d_XR,     d_WT,     d_D0,     d_WTg};
-  hipSetDevice(device);
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (stream) (void)hipStreamDestroy(stream);
@@ -213,54 +255,53 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->d_sum_part = dalloc<double>(8 * static_cast<size_t>(c->nblk_sum));
     c->d_shift = dalloc<double>(8);
 
-    // Mean-field weight: real-space table on the host (FP64), R2C on the device
-    // (FP64 pipeline), real part kept in the working precision.  The table is
-    // even in s, so its transform is real.
+    // Mean-field weight: real-space table built on the device (FP64), R2C in
+    // the FP64 pipeline, real part kept in the working precision.  The table
+    // is even in s, so its transform is real.
     if (c->cs.mf) {
       const InteractionSpec& s = c->inters[0];
-      std::vector<double> table;
-      const double a = mean_field_table(s.pot, s.split, s.scheme, kT, grid->dx, shape[0], shape[1], shape[2], table);
-      c->a_vdw.push_back(a);
       double* d_tab = dalloc<double>(g.N);
       void* d_spec = dalloc_bytes(16 * g.M);
+      const long nblk = (g.N + kMfThreads - 1) / kMfThreads;
+      double* d_wsum = nullptr;
       try {
-        CDFT_HIP(hipMemcpy(d_tab, table.data(), sizeof(double) * g.N, hipMemcpyHostToDevice));
+        d_wsum = dalloc<double>(static_cast<size_t>(nblk));
+        const double reach = s.pot.rc + 0.5 * std::sqrt(3.0) * grid->dx;
+        k_mf_table<<<static_cast<unsigned>(nblk), kMfThreads, 0, c->stream>>>(
+            s.pot, s.split, s.scheme, kT, grid->dx, g.nx, g.ny, g.nz, reach * reach, d_tab, d_wsum);
         auto* S = static_cast<cx<double>*>(d_spec);
         const AliasMap id{0, 0.0, 0.0};
         launch_zr2c<double, double>(d_tab, S, g, c->plan[2], c->tw<double>(2), id, nullptr, c->stream);
         launch_lines<double, double>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<double>(1), -1,
                                      c->stream);
         launch_xlines<double, double>(S, 1, g, c->plan[0], c->tw<double>(0), -1, c->stream);
-        CDFT_HIP(hipGetLastError());
-        std::vector<cx<double>> spec(static_cast<size_t>(g.M));
-        CDFT_HIP(hipMemcpyAsync(spec.data(), d_spec, 16 * g.M, hipMemcpyDeviceToHost, c->stream));
-        CDFT_HIP(hipStreamSynchronize(c->stream));
+        k_mf_store<<<static_cast<unsigned>((g.M + 255) / 256), 256, 0, c->stream>>>(
+            S, g.M, c->precision == CDFT_PRECISION_MIXED ? 1 : 0, c->d_mf);
         if (c->fast) {
-          std::vector<double> pm(static_cast<size_t>(g.M)), fb(static_cast<size_t>(g.M));
-          for (long i = 0; i < g.M; ++i) pm[i] = spec[i].re;
-          fast_mf_layout(pm.data(), fb.data(), g);
           c->d_mfB = dalloc<double>(g.M);
-          CDFT_HIP(hipMemcpy(c->d_mfB, fb.data(), sizeof(double) * g.M, hipMemcpyHostToDevice));
+          launch_fast_mf_layout(S, c->d_mfB, g, c->stream);
         }
-        if (c->precision == CDFT_PRECISION_MIXED) {
-          std::vector<float> re(2 * static_cast<size_t>(g.M));
-          for (long i = 0; i < g.M; ++i) {
-            re[i] = static_cast<float>(spec[i].re);
-            re[g.M + i] = static_cast<float>(spec[i].re - static_cast<double>(re[i]));
-          }
-          CDFT_HIP(hipMemcpy(c->d_mf, re.data(), sizeof(float) * 2 * g.M, hipMemcpyHostToDevice));
-        } else {
-          std::vector<double> re(static_cast<size_t>(g.M));
-          for (long i = 0; i < g.M; ++i) re[i] = spec[i].re;
-          CDFT_HIP(hipMemcpy(c->d_mf, re.data(), sizeof(double) * g.M, hipMemcpyHostToDevice));
+        CDFT_HIP(hipGetLastError());
+        // a_vdw = sum of w dV: per-block sums reduced on the host in a fixed order
+        std::vector<double> wsum(static_cast<size_t>(nblk));
+        CDFT_HIP(hipMemcpyAsync(wsum.data(), d_wsum, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
+        CDFT_HIP(hipStreamSynchronize(c->stream));
+        double a = 0.0, comp = 0.0;
+        for (double x : wsum) {  // Neumaier summation
+          const double t = a + x;
+          comp += std::fabs(a) >= std::fabs(x) ? (a - t) + x : (x - t) + a;
+          a = t;
         }
+        c->a_vdw.push_back(a + comp);
       } catch (...) {
         (void)hipFree(d_tab);
         (void)hipFree(d_spec);
+        if (d_wsum) (void)hipFree(d_wsum);
         throw;
       }
       CDFT_HIP(hipFree(d_tab));
       CDFT_HIP(hipFree(d_spec));
+      CDFT_HIP(hipFree(d_wsum));
     }
     // k = 0 weights of the forward channels (weights.hpp:174-222, mean_field.hpp:189-191)
     {
@@ -378,7 +419,6 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
 // model: each kernel reads its inputs and writes its outputs once).
 void Context::stage_bytes(double* b, bool with_vext) const {
   const double sf = static_cast<double>(csize());  // complex element
-  const double rf = sf / 2.0;                       // real element of the working precision
   const double N = static_cast<double>(g.N), M = static_cast<double>(g.M);
   for (int i = 0; i < kNumStages; ++i) b[i] = 0.0;
   const double nf = cs.nf, nb = cs.nb;

@@ -1336,16 +1336,24 @@ void fast_twiddles(void** tw64, void** tw32) {
     throw std::runtime_error("hipMemcpy of the twiddle tables failed");
 }
 
-void fast_mf_layout(const double* pm, double* out, const Geom& g) {
-  const int q = g.ny / 2;
-  for (int kz = 0; kz < g.nzh; ++kz)
-    for (int r = 0; r < 2; ++r)
-      for (int s = 0; s < q; ++s)
-        for (int kx = 0; kx < g.nx; ++kx) {
-          const int ky = r + 2 * s;
-          out[((static_cast<long>(kz) * 2 + r) * q + s) * g.nx + kx] =
-              pm[static_cast<long>(kz) * g.nx * g.ny + static_cast<long>(kx) * g.ny + ky];
-        }
+namespace {
+__global__ __launch_bounds__(256) void k_fast_mf_layout(const cx<double>* __restrict__ spec, double* __restrict__ out,
+                                                        int nx, int ny, long m) {
+  const long i = static_cast<long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= m) return;
+  const long plane = static_cast<long>(nx) * ny;
+  const long kz = i / plane, rem = i - kz * plane;
+  const int kx = static_cast<int>(rem / ny), ky = static_cast<int>(rem % ny);
+  const int q = ny / 2, r = ky & 1, s = ky >> 1;
+  out[((kz * 2 + r) * q + s) * nx + kx] = spec[i].re;
+}
+}  // namespace
+
+void launch_fast_mf_layout(const cx<double>* spec, double* out, const Geom& g, hipStream_t stream) {
+  if (g.ny % 2 != 0 || g.M != static_cast<long>(g.nzh) * g.nx * g.ny)
+    throw std::logic_error("fused mean-field layout needs an even y extent");
+  const long m = g.M;
+  k_fast_mf_layout<<<static_cast<unsigned>((m + 255) / 256), 256, 0, stream>>>(spec, out, g.nx, g.ny, m);
 }
 
 int fast_point_blocks(const FastGeom& g) { return g.nx * g.mb * 4; }

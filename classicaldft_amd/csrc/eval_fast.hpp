@@ -32,9 +32,10 @@ FastGeom make_fast_geom(const Geom& g);
 // FP64 and FP32 twiddle tables of length kTwLen.
 void fast_twiddles(void** tw64, void** tw32);
 
-// Mean-field weight (real FP64 values) from plane-major order
-// [kz][ix][iy] into the plane kernel's order [kz][r][s][kx] (ky = r + 2 s).
-void fast_mf_layout(const double* plane_major, double* out, const Geom& g);
+// Mean-field weight: real parts of the FP64 spectrum in plane-major order
+// [kz][kx][ky] scattered into the plane kernel's order [kz][r][s][kx]
+// (ky = r + 2 s), on the device.
+void launch_fast_mf_layout(const cx<double>* spec, double* out, const Geom& g, hipStream_t stream);
 
 int fast_point_blocks(const FastGeom& g);
 int fast_bwd_blocks(const FastGeom& g);

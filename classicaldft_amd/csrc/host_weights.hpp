@@ -1,14 +1,16 @@
-// Host-side (init-time) mean-field weight generation, FP64.
+// Mean-field pair potentials and lattice cell weights (init time), FP64.
 //
 // The pair potentials and their WCA / Barker-Henderson attractive tails follow
 // include/dft/physics/potentials.hpp:53-368 of the reference; the lattice cell
 // weights (zero / linear / 27-point quadratic-F rules) and the real-space
-// weight table with dV/N baked in follow mean_field.hpp:41-196.  The table is
-// built once per context on the host and transformed on the device.
+// weight table with dV/N baked in follow mean_field.hpp:41-196.  The pair
+// potential and the cell rule compile for host and device; contexts build the
+// table on the device (k_mf_table in context.hip).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cmath>
-#include <vector>
 
 namespace cdft {
 
@@ -17,7 +19,7 @@ struct PairPotential {
   double sigma = 1.0, eps = 1.0, rc = -1.0, alpha = 50.0;
   double shift = 0.0, r_min = 0.0, v_min = 0.0, r_zero = 0.0, eps_eff = 0.0;
 
-  double bare(double r) const {
+  __host__ __device__ double bare(double r) const {
     switch (kind) {
       case 0: {
         const double y2 = (sigma / r) * (sigma / r);
@@ -61,7 +63,7 @@ struct PairPotential {
   }
 
   // attractive tail; split 0 = WCA (flat v_min inside r_min), 1 = BH (zero inside r_zero)
-  double tail(double r, int split) const {
+  __host__ __device__ double tail(double r, int split) const {
     if (kind == 2) {
       if (r >= rc) return 0.0;
       if (split == 1) return r >= r_min ? bare(r) : 0.0;
@@ -75,18 +77,18 @@ struct PairPotential {
 };
 
 // Average of tail(r)/kT over one lattice cell at integer offset (sx,sy,sz).
-inline double lattice_cell_weight(const PairPotential& pot, int split, int scheme, double kT, double dx, long sx,
-                                  long sy, long sz) {
+__host__ __device__ inline double lattice_cell_weight(const PairPotential& pot, int split, int scheme, double kT,
+                                                     double dx, long sx, long sy, long sz) {
   if (scheme == 0) {
     const double r = dx * std::sqrt(static_cast<double>(sx * sx + sy * sy + sz * sz));
     return pot.tail(r, split) / kT;
   }
   if (scheme == 3) {  // quadratic-F: offsets {-1/2, 0, 1/2}, equal weights 1/3 per axis
-    static const double off[3] = {-0.5, 0.0, 0.5};
     double acc = 0.0;
-    for (double ox : off)
-      for (double oy : off)
-        for (double oz : off) {
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) {
+          const double ox = 0.5 * (a - 1), oy = 0.5 * (b - 1), oz = 0.5 * (c - 1);
           const double rx = (sx + ox) * dx, ry = (sy + oy) * dx, rz = (sz + oz) * dx;
           acc += pot.tail(std::sqrt(rx * rx + ry * ry + rz * rz), split) / kT;
         }
@@ -101,31 +103,6 @@ inline double lattice_cell_weight(const PairPotential& pot, int split, int schem
         acc += pot.tail(std::sqrt(rx * rx + ry * ry + rz * rz), split) / kT;
       }
   return acc / 8.0;
-}
-
-// Real-space weight table w(s) dV / N in the reference real layout; returns a_vdw = sum w dV.
-inline double mean_field_table(const PairPotential& pot, int split, int scheme, double kT, double dx, long nx,
-                               long ny, long nz, std::vector<double>& table) {
-  const long n = nx * ny * nz;
-  const double dv = dx * dx * dx;
-  table.assign(static_cast<size_t>(n), 0.0);
-  const double reach = pot.rc + 0.5 * std::sqrt(3.0) * dx;
-  double a_vdw = 0.0;
-  for (long ix = 0; ix < nx; ++ix) {
-    const long sx = ix <= nx / 2 ? ix : ix - nx;
-    for (long iy = 0; iy < ny; ++iy) {
-      const long sy = iy <= ny / 2 ? iy : iy - ny;
-      for (long iz = 0; iz < nz; ++iz) {
-        const long sz = iz <= nz / 2 ? iz : iz - nz;
-        const double d2 = static_cast<double>(sx * sx + sy * sy + sz * sz) * dx * dx;
-        if (pot.rc > 0.0 && d2 > reach * reach) continue;
-        const double w = lattice_cell_weight(pot, split, scheme, kT, dx, sx, sy, sz);
-        table[static_cast<size_t>(iz + nz * (iy + ny * ix))] = w * dv / static_cast<double>(n);
-        a_vdw += w * dv;
-      }
-    }
-  }
-  return a_vdw;
 }
 
 }  // namespace cdft

