@@ -25,6 +25,7 @@ SCHEDULES = {"auto": 0, "general": 1}
 EXPORTED = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_grid_shape", "cdft_a_vdw",
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
+    "cdft_species_count", "cdft_total_species", "cdft_total_species_device", "cdft_time_total_species_device",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
@@ -150,6 +151,12 @@ def lib():
         L.cdft_a_vdw.argtypes = [C.c_void_p, C.c_int, _dp]
         L.cdft_bulk_chemical_potential.argtypes = [C.c_void_p, C.c_double, _dp]
         L.cdft_total.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.POINTER(_Result), _dp]
+        L.cdft_species_count.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.cdft_total_species.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(_Result), _dp]
+        L.cdft_total_species_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.POINTER(_Result),
+                                                C.c_void_p]
+        L.cdft_time_total_species_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.c_int, C.c_void_p,
+                                                     C.POINTER(C.c_double)]
         L.cdft_total_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.POINTER(_Result),
                                         C.c_void_p]
         L.cdft_total_device_repeat.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
@@ -216,7 +223,9 @@ class Context:
     """
 
     def __init__(self, shape, dx, diameter=1.0, interactions=(), kT=1.0, model="WhiteBearII", precision="fp64",
-                 A=1.0, B=0.0, device=0, schedule="auto"):
+                 A=1.0, B=0.0, device=0, schedule="auto", diameters=None):
+        """`diameters` (a sequence) makes a context of that many species; a
+        mixture is evaluated with `total_species`."""
         g = _Grid(float(dx), (C.c_double * 3)(*[float(n) * float(dx) for n in shape]))
         its = (_Interaction * max(1, len(interactions)))()
         for k, it in enumerate(interactions):
@@ -226,9 +235,9 @@ class Context:
                                   SCHEMES[it.get("scheme", "quadraticF")])
         opt = _Options(FMT_MODELS[model], float(A), float(B), PRECISIONS[precision], int(device),
                        SCHEDULES[schedule])
-        d = np.array([float(diameter)])
+        d = np.array([float(diameter)] if diameters is None else [float(x) for x in diameters])
         h = C.c_void_p()
-        _check(lib().cdft_ctx_create(C.byref(g), _p(d), 1, its, len(interactions), float(kT), C.byref(opt),
+        _check(lib().cdft_ctx_create(C.byref(g), _p(d), len(d), its, len(interactions), float(kT), C.byref(opt),
                                      C.byref(h)))
         self._h = h
         sh = (C.c_long * 3)()
@@ -244,6 +253,9 @@ class Context:
         self.n_inter = len(interactions)
         self.model = model
         self.nf = (11 if model != "none" else 0) + (1 if interactions else 0)
+        ns = C.c_int()
+        _check(lib().cdft_species_count(self._h, C.byref(ns)))
+        self.n_species = ns.value
 
     def close(self):
         if getattr(self, "_h", None):
@@ -275,6 +287,40 @@ class Context:
         _check(lib().cdft_total(self._h, _p(rho), _p(v), float(mu), C.byref(r), _p(f)))
         res = {k: getattr(r, k) for k, _ in _Result._fields_}
         return res, (f.reshape(self.shape) if forces else None)
+
+    def total_species(self, rho, mu, vext=None, forces=True):
+        """functionals::total for S species: rho and vext are (S, *shape), mu has S
+        values.  Return (result dict, forces (S, *shape) or None)."""
+        S = self.n_species
+        rho = _f64(rho)
+        if rho.size != S * self.N:
+            raise ValueError(f"rho has {rho.size} values, expected {S} x {self.N}")
+        v = None if vext is None else _f64(vext)
+        if v is not None and v.size != S * self.N:
+            raise ValueError("vext size does not match the species and grid")
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (S,)))
+        f = np.zeros(S * self.N) if forces else None
+        r = _Result()
+        _check(lib().cdft_total_species(self._h, _p(rho), _p(v), _p(m), C.byref(r), _p(f)))
+        res = {k: getattr(r, k) for k, _ in _Result._fields_}
+        return res, (f.reshape((S,) + self.shape) if forces else None)
+
+    def total_species_device(self, d_rho, mu, d_vext=None, d_forces=None):
+        """Mixture evaluation on device pointers (S * N doubles each)."""
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (self.n_species,)))
+        r = _Result()
+        _check(lib().cdft_total_species_device(self._h, C.c_void_p(d_rho), C.c_void_p(d_vext) if d_vext else None,
+                                               _p(m), C.byref(r), C.c_void_p(d_forces) if d_forces else None))
+        return {k: getattr(r, k) for k, _ in _Result._fields_}
+
+    def time_total_species_device(self, d_rho, mu, count, d_vext=None, d_forces=None):
+        """GPU milliseconds for `count` back-to-back mixture evaluations."""
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (self.n_species,)))
+        ms = C.c_double()
+        _check(lib().cdft_time_total_species_device(self._h, C.c_void_p(d_rho),
+                                                    C.c_void_p(d_vext) if d_vext else None, _p(m), int(count),
+                                                    C.c_void_p(d_forces) if d_forces else None, C.byref(ms)))
+        return ms.value
 
     def total_device(self, d_rho, mu, d_vext=None, d_forces=None, count=1):
         """Evaluate on device pointers (ints, e.g. torch tensor.data_ptr())."""

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libcdft.so")
 ARCH = os.environ.get("CDFT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["eval_fast.hip", "kernels.hip", "context.hip", "cdft_api.hip", "dynamics.hip", "adjoint.hip", "autotune.hip", "eigen.hip", "steppers.hip"]
+SOURCES = ["eval_fast.hip", "kernels.hip", "context.hip", "cdft_api.hip", "dynamics.hip", "adjoint.hip", "autotune.hip", "eigen.hip", "steppers.hip", "mixture.hip"]
 
 CFLAGS = [
     f"--offload-arch={ARCH}",

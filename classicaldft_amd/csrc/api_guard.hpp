@@ -33,8 +33,19 @@ int guarded(F&& fn) {
   }
 }
 
-inline void check_ctx(const cdft_ctx* ctx) {
+// Any context, one species or a mixture.
+inline void check_ctx_any(const cdft_ctx* ctx) {
   if (!ctx || !ctx->impl) throw std::invalid_argument("null context");
+}
+
+// A one-species context: the algorithms (minimisation, DDFT, saddle search,
+// Picard) and the one-species evaluation entry points, as in the reference
+// (minimization.hpp:206-342 and dynamics.hpp act on species[0]).
+inline void check_ctx(const cdft_ctx* ctx) {
+  check_ctx_any(ctx);
+  if (ctx->impl->n_species != 1)
+    throw Unsupported("this entry point acts on one species; the context has " +
+                      std::to_string(ctx->impl->n_species) + " (use cdft_total_species)");
 }
 
 }  // namespace cdft

@@ -99,14 +99,14 @@ void cdft_ctx_destroy(cdft_ctx* ctx) {
 
 int cdft_grid_shape(const cdft_ctx* ctx, long shape[3]) {
   return guarded([&] {
-    check_ctx(ctx);
+    check_ctx_any(ctx);
     for (int d = 0; d < 3; ++d) shape[d] = ctx->impl->shape[d];
   });
 }
 
 int cdft_schedule(const cdft_ctx* ctx, int* schedule) {
   return guarded([&] {
-    check_ctx(ctx);
+    check_ctx_any(ctx);
     if (!schedule) throw std::invalid_argument("cdft_schedule: null argument");
     *schedule = ctx->impl->fast ? CDFT_SCHEDULE_FUSED : CDFT_SCHEDULE_GENERAL;
   });
@@ -114,7 +114,7 @@ int cdft_schedule(const cdft_ctx* ctx, int* schedule) {
 
 int cdft_a_vdw(const cdft_ctx* ctx, int k, double* a_vdw) {
   return guarded([&] {
-    check_ctx(ctx);
+    check_ctx_any(ctx);
     if (k < 0 || k >= static_cast<int>(ctx->impl->a_vdw.size())) throw std::invalid_argument("no such interaction");
     *a_vdw = ctx->impl->a_vdw[k];
   });

@@ -102,10 +102,51 @@ Context::~Context() {
   clear_graphs();
   void* bufs[] = {d_rho,   d_vext,  d_force, d_local,  d_S0,     d_SF,     d_SO,  d_mf,  d_partials, d_res,
                   d_sum_part, d_shift, tw32[0], tw32[1], tw32[2], tw64[0], tw64[1], tw64[2], d_P, d_mfB,
-                  ftw64,   ftw32,   d_XR,     d_WT,     d_D0,     d_WTg};
+                  ftw64,   ftw32,   d_XR,     d_WT,     d_D0,     d_WTg, d_mix, d_mix_mf};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (stream) (void)hipStreamDestroy(stream);
+}
+
+double mf_weight_spectrum(Context* c, const InteractionSpec& s, cx<double>* spec) {
+  const Geom& g = c->g;
+  double* d_tab = dalloc<double>(g.N);
+  const long nblk = (g.N + kMfThreads - 1) / kMfThreads;
+  double* d_wsum = nullptr;
+  double a = 0.0, comp = 0.0;
+  try {
+    d_wsum = dalloc<double>(static_cast<size_t>(nblk));
+    const double reach = s.pot.rc + 0.5 * std::sqrt(3.0) * g.dx;
+    k_mf_table<<<static_cast<unsigned>(nblk), kMfThreads, 0, c->stream>>>(
+        s.pot, s.split, s.scheme, c->kT, g.dx, g.nx, g.ny, g.nz, reach * reach, d_tab, d_wsum);
+    const AliasMap id{0, 0.0, 0.0};
+    launch_zr2c<double, double>(d_tab, spec, g, c->plan[2], c->tw<double>(2), id, nullptr, c->stream);
+    launch_lines<double, double>(spec, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<double>(1), -1,
+                                 c->stream);
+    launch_xlines<double, double>(spec, 1, g, c->plan[0], c->tw<double>(0), -1, c->stream);
+    CDFT_HIP(hipGetLastError());
+    // a_vdw = sum of w dV: per-block sums reduced on the host in a fixed order
+    std::vector<double> wsum(static_cast<size_t>(nblk));
+    CDFT_HIP(hipMemcpyAsync(wsum.data(), d_wsum, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+    for (double x : wsum) {  // Neumaier summation
+      const double t = a + x;
+      comp += std::fabs(a) >= std::fabs(x) ? (a - t) + x : (x - t) + a;
+      a = t;
+    }
+  } catch (...) {
+    (void)hipFree(d_tab);
+    if (d_wsum) (void)hipFree(d_wsum);
+    throw;
+  }
+  CDFT_HIP(hipFree(d_tab));
+  CDFT_HIP(hipFree(d_wsum));
+  return a + comp;
+}
+
+void store_real_spectrum(const cx<double>* spec, double* out, long m, hipStream_t stream) {
+  k_mf_store<<<static_cast<unsigned>((m + 255) / 256), 256, 0, stream>>>(spec, m, 0, out);
+  CDFT_HIP(hipGetLastError());
 }
 
 Context* create_context(const cdft_grid* grid, const double* diameters, int n_species,
@@ -125,13 +166,21 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
                                   " is not commensurate with dx=" + std::to_string(grid->dx));
     shape[d] = ni;
   }
-  if (n_species != 1)
-    throw Unsupported("the HIP path evaluates one species; got " + std::to_string(n_species));
-  if (!diameters || diameters[0] <= 0.0) throw std::invalid_argument("hard-sphere diameter must be positive");
-  if (n_interactions < 0 || n_interactions > 1)
-    throw Unsupported("the HIP path evaluates at most one mean-field interaction");
+  if (n_species < 1 || n_species > kMaxSpecies)
+    throw Unsupported("species count must be between 1 and " + std::to_string(kMaxSpecies) + "; got " +
+                      std::to_string(n_species));
+  if (!diameters) throw std::invalid_argument("hard-sphere diameters are required");
+  for (int s = 0; s < n_species; ++s)
+    if (!(diameters[s] > 0.0)) throw std::invalid_argument("hard-sphere diameter must be positive");
+  if (n_interactions < 0) throw std::invalid_argument("negative interaction count");
+  if (n_species == 1 && n_interactions > 1)
+    throw Unsupported("a one-species context evaluates at most one mean-field interaction");
+  if (n_interactions > kMaxMixInteractions)
+    throw Unsupported("at most " + std::to_string(kMaxMixInteractions) + " mean-field interactions");
   if (options->precision != CDFT_PRECISION_FP64 && options->precision != CDFT_PRECISION_MIXED)
     throw std::invalid_argument("unknown precision mode");
+  if (n_species > 1 && options->precision != CDFT_PRECISION_FP64)
+    throw Unsupported("mixtures are evaluated in FP64 only");
   if (options->schedule != CDFT_SCHEDULE_AUTO && options->schedule != CDFT_SCHEDULE_GENERAL)
     throw std::invalid_argument("unknown schedule");
   const int model = options->fmt_model;
@@ -147,6 +196,7 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->kT = kT;
     for (int d = 0; d < 3; ++d) c->shape[d] = shape[d];
     c->diam.assign(diameters, diameters + n_species);
+    c->n_species = n_species;
 
     Geom& g = c->g;
     g.nx = static_cast<int>(shape[0]);
@@ -172,7 +222,8 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
 
     for (int k = 0; k < n_interactions; ++k) {
       const cdft_interaction& in = interactions[k];
-      if (in.species_i != 0 || in.species_j != 0) throw std::invalid_argument("interaction species out of range");
+      if (in.species_i < 0 || in.species_i >= n_species || in.species_j < 0 || in.species_j >= n_species)
+        throw std::invalid_argument("interaction species out of range");
       if (in.potential < 0 || in.potential > 2) throw std::invalid_argument("unknown potential id");
       InteractionSpec s{};
       s.si = in.species_i;
@@ -212,6 +263,11 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       c->tw32[d] = dalloc_bytes(t32.size() * sizeof(float));
       CDFT_HIP(hipMemcpy(c->tw64[d], t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
       CDFT_HIP(hipMemcpy(c->tw32[d], t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+    }
+
+    if (n_species > 1) {  // mixtures: FP64 general schedule with their own buffers
+      init_mixture(c);
+      return c;
     }
 
     const size_t cs_bytes = c->csize();
@@ -255,26 +311,13 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     c->d_sum_part = dalloc<double>(8 * static_cast<size_t>(c->nblk_sum));
     c->d_shift = dalloc<double>(8);
 
-    // Mean-field weight: real-space table built on the device (FP64), R2C in
-    // the FP64 pipeline, real part kept in the working precision.  The table
-    // is even in s, so its transform is real.
+    // Mean-field weight: FP64 spectrum, real part kept in the working precision
+    // (and in the fused layout when that schedule runs).
     if (c->cs.mf) {
-      const InteractionSpec& s = c->inters[0];
-      double* d_tab = dalloc<double>(g.N);
       void* d_spec = dalloc_bytes(16 * g.M);
-      const long nblk = (g.N + kMfThreads - 1) / kMfThreads;
-      double* d_wsum = nullptr;
       try {
-        d_wsum = dalloc<double>(static_cast<size_t>(nblk));
-        const double reach = s.pot.rc + 0.5 * std::sqrt(3.0) * grid->dx;
-        k_mf_table<<<static_cast<unsigned>(nblk), kMfThreads, 0, c->stream>>>(
-            s.pot, s.split, s.scheme, kT, grid->dx, g.nx, g.ny, g.nz, reach * reach, d_tab, d_wsum);
         auto* S = static_cast<cx<double>*>(d_spec);
-        const AliasMap id{0, 0.0, 0.0};
-        launch_zr2c<double, double>(d_tab, S, g, c->plan[2], c->tw<double>(2), id, nullptr, c->stream);
-        launch_lines<double, double>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<double>(1), -1,
-                                     c->stream);
-        launch_xlines<double, double>(S, 1, g, c->plan[0], c->tw<double>(0), -1, c->stream);
+        c->a_vdw.push_back(mf_weight_spectrum(c, c->inters[0], S));
         k_mf_store<<<static_cast<unsigned>((g.M + 255) / 256), 256, 0, c->stream>>>(
             S, g.M, c->precision == CDFT_PRECISION_MIXED ? 1 : 0, c->d_mf);
         if (c->fast) {
@@ -282,26 +325,12 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
           launch_fast_mf_layout(S, c->d_mfB, g, c->stream);
         }
         CDFT_HIP(hipGetLastError());
-        // a_vdw = sum of w dV: per-block sums reduced on the host in a fixed order
-        std::vector<double> wsum(static_cast<size_t>(nblk));
-        CDFT_HIP(hipMemcpyAsync(wsum.data(), d_wsum, sizeof(double) * nblk, hipMemcpyDeviceToHost, c->stream));
         CDFT_HIP(hipStreamSynchronize(c->stream));
-        double a = 0.0, comp = 0.0;
-        for (double x : wsum) {  // Neumaier summation
-          const double t = a + x;
-          comp += std::fabs(a) >= std::fabs(x) ? (a - t) + x : (x - t) + a;
-          a = t;
-        }
-        c->a_vdw.push_back(a + comp);
       } catch (...) {
-        (void)hipFree(d_tab);
         (void)hipFree(d_spec);
-        if (d_wsum) (void)hipFree(d_wsum);
         throw;
       }
-      CDFT_HIP(hipFree(d_tab));
       CDFT_HIP(hipFree(d_spec));
-      CDFT_HIP(hipFree(d_wsum));
     }
     // k = 0 weights of the forward channels (weights.hpp:174-222, mean_field.hpp:189-191)
     {

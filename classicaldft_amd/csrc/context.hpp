@@ -90,6 +90,14 @@ struct Context {
   double* d_D0 = nullptr;     // nx * ny FP64: kz = 0 plane (line sums) of the z-transformed rho
   void* d_WT = nullptr;       // M complex FP64: radial FMT weights (w3, w2) in the plane kernels' order
   double* d_mfB = nullptr;    // M: mean-field weight in the plane kernel's order
+
+  // Mixtures (more than one species): FP64 general schedule.  d_mix is one
+  // block holding the species spectra, channel spectra and fields, backward
+  // spectra, local forces, input staging and partial sums (mixture.hip);
+  // d_mix_mf holds the real mean-field weight spectrum of every interaction.
+  int n_species = 1;
+  void* d_mix = nullptr;
+  double* d_mix_mf = nullptr;
   int nstages() const { return fast ? kNumFastStages : kNumStages; }
   const char* stage_name(int i) const {
     return (i < 0 || i >= nstages()) ? "" : (fast ? kFastStageNames[i] : kStageNames[i]);
@@ -157,6 +165,18 @@ struct Context {
 Context* create_context(const cdft_grid* grid, const double* diameters, int n_species,
                         const cdft_interaction* interactions, int n_interactions, double kT,
                         const cdft_options* options);
+
+// Species and interaction limits of a context (kernel argument arrays).
+constexpr int kMaxSpecies = 8;
+constexpr int kMaxMixInteractions = 36;  // every pair of 8 species, self pairs included
+
+// Mean-field weight spectrum of one interaction (M complex FP64, plane-major,
+// real-valued), built on the device; returns a_vdw = sum of w dV.
+double mf_weight_spectrum(Context* c, const InteractionSpec& s, cx<double>* spec);
+// out[i] = spec[i].re for i < m (FP64), on `stream`.
+void store_real_spectrum(const cx<double>* spec, double* out, long m, hipStream_t stream);
+// Buffers and weights of a mixture context (defined with the mixture kernels).
+void init_mixture(Context* c);
 
 void set_error(const std::string& msg);
 

@@ -124,7 +124,69 @@ __host__ __device__ inline void f_funcs(int model, T e, T* f, T* df) {
   }
 }
 
-// w[0..10] = eta, n2, v[3], T[xx,xy,xz,yy,yz,zz]; returns Phi, fills g[0..10].
+// The model-specific third term Phi3(n2, n_v2, T) and its partial
+// derivatives (models.hpp:18-45,200-357); identical for one species and for
+// mixtures, whose n2, n_v2 and T are sums over species.
+template <typename T>
+struct Phi3 {
+  T p3, dn2, dv[3], dT[6];
+  bool tensor;
+};
+template <typename T>
+__host__ __device__ __forceinline__ Phi3<T> fmt_phi3_t(const FmtParams& P, T n2, const T* v, T vv, const T* Tm) {
+  const double PI = 3.14159265358979323846;
+  Phi3<T> r;
+  r.p3 = 0.0;
+  r.dn2 = 0.0;
+  r.tensor = false;
+  for (int k = 0; k < 3; ++k) r.dv[k] = 0.0;
+  for (int q = 0; q < 6; ++q) r.dT[q] = 0.0;
+  if (P.model == FMT_ROSENFELD) {
+    const double k24 = 1.0 / (24.0 * PI);
+    r.p3 = k24 * (n2 * n2 * n2 - 3.0 * n2 * vv);
+    r.dn2 = (1.0 / (8.0 * PI)) * (n2 * n2 - vv);
+    for (int k = 0; k < 3; ++k) r.dv[k] = -(1.0 / (4.0 * PI)) * n2 * v[k];
+  } else if (P.model == FMT_RSLT) {
+    if (vof(n2) >= 1e-30) {
+      const T xi = dmin(vv / (n2 * n2), 1.0);
+      const T q = 1.0 - xi;
+      const double k36 = 1.0 / (36.0 * PI);
+      r.p3 = k36 * n2 * n2 * n2 * q * q * q;
+      r.dn2 = k36 * 3.0 * n2 * n2 * q * q * (1.0 + xi);
+      for (int k = 0; k < 3; ++k) r.dv[k] = k36 * (-6.0) * n2 * q * q * v[k];
+    }
+  } else if (P.model != FMT_NONE) {
+    // tensor models: esFMT(A,B); WhiteBear I/II are esFMT(1,-1) (models.hpp:18-45,326-357)
+    r.tensor = true;
+    const double A = P.A, B = P.B;
+    const T a = Tm[0], b = Tm[1], c = Tm[2], d = Tm[3], e = Tm[4], f = Tm[5];
+    const T sxx = a * a + b * b + c * c, sxy = a * b + b * d + c * e, sxz = a * c + b * e + c * f;
+    const T syy = b * b + d * d + e * e, syz = b * c + d * e + e * f, szz = c * c + e * e + f * f;
+    const T trT2 = sxx + syy + szz;
+    const T trT3 = a * sxx + d * syy + f * szz + 2.0 * (b * sxy + c * sxz + e * syz);
+    const T Tv[3] = {a * v[0] + b * v[1] + c * v[2], b * v[0] + d * v[1] + e * v[2],
+                     c * v[0] + e * v[1] + f * v[2]};
+    const T qf = v[0] * Tv[0] + v[1] * Tv[1] + v[2] * Tv[2];
+    const double k24 = 1.0 / (24.0 * PI);
+    const T n23 = n2 * n2 * n2;
+    r.p3 = k24 * (A * (n23 - 3.0 * n2 * vv + 3.0 * qf - trT3) + B * (n23 - 3.0 * n2 * trT2 + 2.0 * trT3));
+    r.dn2 = k24 * (A * (3.0 * n2 * n2 - 3.0 * vv) + B * (3.0 * n2 * n2 - 3.0 * trT2));
+    for (int k = 0; k < 3; ++k) r.dv[k] = k24 * A * (-6.0 * n2 * v[k] + 6.0 * Tv[k]);
+    const double k8 = 1.0 / (8.0 * PI), k4 = 1.0 / (4.0 * PI);
+    const T T2m[6] = {sxx, sxy, sxz, syy, syz, szz};
+    const int ii[6] = {0, 0, 0, 1, 1, 2};
+    const int jj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      r.dT[q] = k8 * A * (v[ii[q]] * v[jj[q]] - T2m[q]) + k4 * B * (-n2 * Tm[q] + T2m[q]);
+  }
+  return r;
+}
+
+// One species.  w[0..10] = eta, n2, v[3], T[xx,xy,xz,yy,yz,zz]; n1, n0 and
+// n_v1 follow by Rosenfeld scaling with the species radius
+// (hard_sphere.hpp:80-86).  Returns Phi and fills g[0..10] with the
+// derivatives already combined for the 11 weights (hard_sphere.hpp:118-134).
 template <typename T>
 __host__ __device__ inline T fmt_phi_grad_t(const FmtParams& P, const T* w, T* g) {
   const double PI = 3.14159265358979323846;
@@ -141,73 +203,73 @@ __host__ __device__ inline T fmt_phi_grad_t(const FmtParams& P, const T* w, T* g
   const T n1 = n2 * ifr;
   const T n0 = n2 * ifr2;
   const T v[3] = {w[2], w[3], w[4]};
-  const T a = w[5], b = w[6], c = w[7], d = w[8], e = w[9], f = w[10];
   const T vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
   const T v0v1 = vv * ifr;
 
   T F[3], dF[3];
   f_funcs<T>(P.model, eta, F, dF);
-
-  T p3 = 0.0, dp3_n2 = 0.0, dp3_v[3] = {0.0, 0.0, 0.0}, dT[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  bool tensor = false;
-  if (P.model == FMT_ROSENFELD) {
-    const double k24 = 1.0 / (24.0 * PI);
-    p3 = k24 * (n2 * n2 * n2 - 3.0 * n2 * vv);
-    dp3_n2 = (1.0 / (8.0 * PI)) * (n2 * n2 - vv);
-    for (int k = 0; k < 3; ++k) dp3_v[k] = -(1.0 / (4.0 * PI)) * n2 * v[k];
-  } else if (P.model == FMT_RSLT) {
-    if (vof(n2) >= 1e-30) {
-      const T xi = dmin(vv / (n2 * n2), 1.0);
-      const T q = 1.0 - xi;
-      const double k36 = 1.0 / (36.0 * PI);
-      p3 = k36 * n2 * n2 * n2 * q * q * q;
-      dp3_n2 = k36 * 3.0 * n2 * n2 * q * q * (1.0 + xi);
-      for (int k = 0; k < 3; ++k) dp3_v[k] = k36 * (-6.0) * n2 * q * q * v[k];
-    }
-  } else if (P.model != FMT_NONE) {
-    // tensor models: esFMT(A,B); WhiteBear I/II are esFMT(1,-1) (models.hpp:18-45,326-357)
-    tensor = true;
-    const double A = P.A, B = P.B;
-    const T sxx = a * a + b * b + c * c, sxy = a * b + b * d + c * e, sxz = a * c + b * e + c * f;
-    const T syy = b * b + d * d + e * e, syz = b * c + d * e + e * f, szz = c * c + e * e + f * f;
-    const T trT2 = sxx + syy + szz;
-    const T trT3 = a * sxx + d * syy + f * szz + 2.0 * (b * sxy + c * sxz + e * syz);
-    const T Tv[3] = {a * v[0] + b * v[1] + c * v[2], b * v[0] + d * v[1] + e * v[2],
-                     c * v[0] + e * v[1] + f * v[2]};
-    const T qf = v[0] * Tv[0] + v[1] * Tv[1] + v[2] * Tv[2];
-    const double k24 = 1.0 / (24.0 * PI);
-    const T n23 = n2 * n2 * n2;
-    p3 = k24 * (A * (n23 - 3.0 * n2 * vv + 3.0 * qf - trT3) + B * (n23 - 3.0 * n2 * trT2 + 2.0 * trT3));
-    dp3_n2 = k24 * (A * (3.0 * n2 * n2 - 3.0 * vv) + B * (3.0 * n2 * n2 - 3.0 * trT2));
-    for (int k = 0; k < 3; ++k) dp3_v[k] = k24 * A * (-6.0 * n2 * v[k] + 6.0 * Tv[k]);
-    const double k8 = 1.0 / (8.0 * PI), k4 = 1.0 / (4.0 * PI);
-    const T Tm[6] = {a, b, c, d, e, f};
-    const T T2m[6] = {sxx, sxy, sxz, syy, syz, szz};
-    const int ii[6] = {0, 0, 0, 1, 1, 2};
-    const int jj[6] = {0, 1, 2, 1, 2, 2};
-#pragma unroll
-    for (int q = 0; q < 6; ++q)
-      dT[q] = k8 * A * (v[ii[q]] * v[jj[q]] - T2m[q]) + k4 * B * (-n2 * Tm[q] + T2m[q]);
-  }
+  const Phi3<T> t = fmt_phi3_t<T>(P, n2, v, vv, w + 5);
 
   const T s = n1 * n2 - v0v1;
-  const T phi = -n0 * F[0] + s * F[1] + p3 * F[2];
-  const T d_eta = -n0 * dF[0] + s * dF[1] + p3 * dF[2];
+  const T phi = -n0 * F[0] + s * F[1] + t.p3 * F[2];
+  const T d_eta = -n0 * dF[0] + s * dF[1] + t.p3 * dF[2];
   const T d_n0 = -F[0];
   const T d_n1 = n2 * F[1];
-  const T d_n2 = n1 * F[1] + dp3_n2 * F[2];
+  const T d_n2 = n1 * F[1] + t.dn2 * F[2];
   g[0] = d_eta;
   g[1] = d_n2 + d_n1 * ifr + d_n0 * ifr2;
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
-    const T dv0 = -v[k] * F[1];                            // dPhi/dv0 = -v1 f2
-    const T dv1 = -(v[k] * ifr) * F[1] + dp3_v[k] * F[2];  // dPhi/dv1 = -v0 f2 + dPhi3/dv1 f3
+    const T dv0 = -v[k] * F[1];                           // dPhi/dv0 = -v1 f2
+    const T dv1 = -(v[k] * ifr) * F[1] + t.dv[k] * F[2];  // dPhi/dv1 = -v0 f2 + dPhi3/dv1 f3
     g[2 + k] = dv1 + dv0 * ifr;
   }
 #pragma unroll
   for (int q = 0; q < 6; ++q) {
     const double sym = (q == 0 || q == 3 || q == 5) ? 1.0 : 2.0;
-    g[5 + q] = tensor ? sym * dT[q] * F[2] : T(0.0);
+    g[5 + q] = t.tensor ? sym * t.dT[q] * F[2] : T(0.0);
+  }
+  return phi;
+}
+
+// Mixtures (hard_sphere.hpp:174-224): m[0..15] = eta, n2, n1, n0, n_v2[3],
+// n_v1[3], T[6], each already summed over species with its own radius.
+// Returns Phi and fills g[0..15] with the partial derivatives in the same
+// order (tensor off-diagonals doubled); the per-species weight combination
+// happens in k-space.
+template <typename T>
+__host__ __device__ inline T fmt_phi_grad_mix_t(const FmtParams& P, const T* m, T* g) {
+  const T eta = m[0];
+  if (!(vof(eta) >= 1e-30)) {  // hard_sphere.hpp:199-200
+#pragma unroll
+    for (int i = 0; i < 16; ++i) g[i] = T(0.0);
+    return T(0.0);
+  }
+  const T n2 = m[1], n1 = m[2], n0 = m[3];
+  const T* v = m + 4;   // n_v2 (the reference's Measures::v1)
+  const T* v0 = m + 7;  // n_v1 (Measures::v0)
+  const T vv = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+  const T v0v1 = v0[0] * v[0] + v0[1] * v[1] + v0[2] * v[2];
+
+  T F[3], dF[3];
+  f_funcs<T>(P.model, eta, F, dF);
+  const Phi3<T> t = fmt_phi3_t<T>(P, n2, v, vv, m + 10);
+
+  const T s = n1 * n2 - v0v1;
+  const T phi = -n0 * F[0] + s * F[1] + t.p3 * F[2];
+  g[0] = -n0 * dF[0] + s * dF[1] + t.p3 * dF[2];
+  g[1] = n1 * F[1] + t.dn2 * F[2];
+  g[2] = n2 * F[1];
+  g[3] = -F[0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    g[4 + k] = -v0[k] * F[1] + t.dv[k] * F[2];  // d/d n_v2
+    g[7 + k] = -v[k] * F[1];                    // d/d n_v1
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const double sym = (q == 0 || q == 3 || q == 5) ? 1.0 : 2.0;
+    g[10 + q] = t.tensor ? sym * t.dT[q] * F[2] : T(0.0);
   }
   return phi;
 }

@@ -1,0 +1,452 @@
+// Multi-species functionals::total (functionals.hpp:64-95) on the device:
+// FP64, general schedule.
+//
+// Hard spheres (hard_sphere.hpp:174-254): the 16 measures eta, n2, n1, n0,
+// n_v2[3], n_v1[3], T[6] are sums over species of each species' weighted
+// density, with n1, n0 and n_v1 scaled by that species' radius
+// (hard_sphere.hpp:80-86,186-198).  Being linear in the densities, the sums
+// are formed in k-space before one inverse transform per channel.  The
+// pointwise kernel returns Phi and its 16 partial derivatives; each species'
+// force is the k-space combination of those with its own weights
+// (w2 <- d n2 + d n1 / (4 pi R) + d n0 / (4 pi R^2), the vector weight,
+// conjugated, <- d n_v2 + d n_v1 / (4 pi R); hard_sphere.hpp:118-134,236-254).
+//
+// Mean field (mean_field.hpp:233-281): per species s, the potential
+// U_s = sum over self pairs (s, s) of w * rho_s plus half of w * rho_t over
+// cross pairs (s, t); the force gets U_s dV and F_mf = 1/2 sum_s rho_s U_s dV,
+// which equals the reference's 1/2 rho_i (w * rho_j) dV per listed pair.
+//
+// Ideal gas and external field per species (ideal_gas.hpp:18-38,
+// external_field.hpp:16-31); Omega = F - sum_s mu_s N_s.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "api_guard.hpp"
+#include "context.hpp"
+#include "device_common.hpp"
+
+namespace cdft {
+namespace {
+
+constexpr int kMixFmt = 16;  // eta, n2, n1, n0, n_v2[3], n_v1[3], T[6]
+constexpr int kMixThreads = 256;
+
+struct MixParams {
+  int S, ninter, hs;
+  double R[kMaxSpecies], invR[kMaxSpecies], ifr[kMaxSpecies], ifr2[kMaxSpecies];  // 1/(4 pi R), 1/(4 pi R^2)
+  int si[kMaxMixInteractions], sj[kMaxMixInteractions];
+};
+
+struct MixPoint {
+  int S, hs, nfm, nu;  // species, FMT on, FMT channel count, mean-field potential count
+  double mu[kMaxSpecies];
+  double dv;
+  FmtParams fp;
+};
+
+// Carving of Context::d_mix.
+struct MixLayout {
+  int nch = 0, nblk = 0;
+  size_t R = 0, F = 0, X = 0, B = 0, local = 0, rho = 0, vext = 0, force = 0, part = 0, res = 0, total = 0;
+};
+
+MixLayout mix_layout(const Context* c) {
+  MixLayout L;
+  const size_t S = static_cast<size_t>(c->n_species), N = static_cast<size_t>(c->g.N),
+               M = static_cast<size_t>(c->g.M);
+  const int nfm = c->cs.hs ? kMixFmt : 0, nu = c->inters.empty() ? 0 : c->n_species;
+  L.nch = std::max(nfm + nu, 1);
+  L.nblk = static_cast<int>(std::min<long>(1024, (c->g.N + kMixThreads - 1) / kMixThreads));
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off += (bytes + 255) / 256 * 256;
+    return at;
+  };
+  L.R = take(16 * S * M);
+  L.F = take(16 * static_cast<size_t>(L.nch) * M);
+  L.X = take(8 * static_cast<size_t>(L.nch) * N);
+  L.B = take(16 * S * M);
+  L.local = take(8 * S * N);
+  L.rho = take(8 * S * N);
+  L.vext = take(8 * S * N);
+  L.force = take(8 * S * N);
+  L.part = take(8 * 8 * static_cast<size_t>(L.nblk));
+  L.res = take(8 * 8);
+  L.total = off;
+  return L;
+}
+
+template <typename T>
+T* at(const Context* c, size_t off) {
+  return reinterpret_cast<T*>(static_cast<char*>(c->d_mix) + off);
+}
+
+__device__ __forceinline__ cx<double> cmul(cx<double> a, cx<double> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+__device__ __forceinline__ cx<double> cscale(cx<double> a, double s) { return {a.re * s, a.im * s}; }
+__device__ __forceinline__ cx<double> cadd(cx<double> a, cx<double> b) { return {a.re + b.re, a.im + b.im}; }
+
+__device__ __forceinline__ void kvec3(long i, const Geom& g, double* kx, double* ky, double* kz) {
+  const long nxy = static_cast<long>(g.nx) * g.ny;
+  const long z = i / nxy, r = i - z * nxy;
+  const int x = static_cast<int>(r / g.ny), y = static_cast<int>(r - static_cast<long>(x) * g.ny);
+  *kx = kvec(x, g.nx, g.dkx);
+  *ky = kvec(y, g.ny, g.dky);
+  *kz = g.dkz * static_cast<double>(z);
+}
+
+// Forward channels at each spectral element: the 16 summed FMT measures
+// (when hs) and the S mean-field potentials (when there are interactions).
+__global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __restrict__ rhoK,
+                                                         const double* __restrict__ mfw, cx<double>* __restrict__ F,
+                                                         Geom g, MixParams p) {
+  const long M = g.M;
+  for (long i = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; i < M;
+       i += static_cast<long>(gridDim.x) * kMixThreads) {
+    cx<double> rk[kMaxSpecies];
+    for (int s = 0; s < p.S; ++s) rk[s] = rhoK[s * M + i];
+    int ch = 0;
+    if (p.hs) {
+      double kx, ky, kz;
+      kvec3(i, g, &kx, &ky, &kz);
+      const double k2 = kx * kx + ky * ky + kz * kz;
+      const double k = sqrt(k2);
+      cx<double> acc[kMixFmt];
+#pragma unroll
+      for (int c = 0; c < kMixFmt; ++c) acc[c] = {0.0, 0.0};
+      for (int s = 0; s < p.S; ++s) {
+        double w3, w2;
+        radial(k, p.R[s], g.inv_n, &w3, &w2);
+        const WRad wr = wrad({w3, w2}, k2, p.invR[s]);
+        const cx<double> r = rk[s];
+        acc[0] = cadd(acc[0], cscale(r, w3));
+        acc[1] = cadd(acc[1], cscale(r, w2));
+        acc[2] = cadd(acc[2], cscale(r, w2 * p.ifr[s]));
+        acc[3] = cadd(acc[3], cscale(r, w2 * p.ifr2[s]));
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const cx<double> wv = cmul(channel_weight_t(2 + a, false, kx, ky, kz, wr, 0.0, p.invR[s]), r);
+          acc[4 + a] = cadd(acc[4 + a], wv);
+          acc[7 + a] = cadd(acc[7 + a], cscale(wv, p.ifr[s]));
+        }
+#pragma unroll
+        for (int q = 0; q < 6; ++q)
+          acc[10 + q] = cadd(acc[10 + q], cscale(r, channel_weight_t(5 + q, false, kx, ky, kz, wr, 0.0, p.invR[s]).re));
+      }
+#pragma unroll
+      for (int c = 0; c < kMixFmt; ++c) F[c * M + i] = acc[c];
+      ch = kMixFmt;
+    }
+    if (p.ninter > 0) {
+      cx<double> u[kMaxSpecies];
+      for (int s = 0; s < p.S; ++s) u[s] = {0.0, 0.0};
+      for (int q = 0; q < p.ninter; ++q) {
+        const double w = mfw[q * M + i];
+        const int a = p.si[q], b = p.sj[q];
+        if (a == b) {
+          u[a] = cadd(u[a], cscale(rk[a], w));
+        } else {
+          u[a] = cadd(u[a], cscale(rk[b], 0.5 * w));
+          u[b] = cadd(u[b], cscale(rk[a], 0.5 * w));
+        }
+      }
+      for (int s = 0; s < p.S; ++s) F[(ch + s) * M + i] = u[s];
+    }
+  }
+}
+
+// Pointwise: Phi and its 16 partials (written over the measure fields), the
+// local force of every species and the block sums
+// [F_id, F_ext, F_hs, F_mf, sum mu rho, sum rho] (each still without dV).
+__global__ __launch_bounds__(kMixThreads) void k_mix_point(double* __restrict__ X, const double* __restrict__ rho,
+                                                           const double* __restrict__ vext,
+                                                           double* __restrict__ local, double* __restrict__ partials,
+                                                           long N, MixPoint mp) {
+  double sums[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (long p = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; p < N;
+       p += static_cast<long>(gridDim.x) * kMixThreads) {
+    if (mp.hs) {
+      double m[kMixFmt], d[kMixFmt];
+#pragma unroll
+      for (int c = 0; c < kMixFmt; ++c) m[c] = X[c * N + p];
+      sums[2] += fmt_phi_grad_mix_t<double>(mp.fp, m, d);
+#pragma unroll
+      for (int c = 0; c < kMixFmt; ++c) X[c * N + p] = d[c];
+    }
+    for (int s = 0; s < mp.S; ++s) {
+      const double r = rho[s * N + p];
+      const double lr = log(fmax(r, 1e-300));  // ideal_gas.hpp:27
+      double f = lr - mp.mu[s];
+      sums[0] += r * (lr - 1.0);
+      sums[4] += mp.mu[s] * r;
+      sums[5] += r;
+      if (vext) {
+        const double v = vext[s * N + p];
+        f += v;
+        sums[1] += r * v;
+      }
+      if (mp.nu) {
+        const double u = X[(mp.nfm + s) * N + p];
+        f += u;
+        sums[3] += 0.5 * r * u;
+      }
+      local[s * N + p] = f * mp.dv;
+    }
+  }
+  block_reduce<8>(sums, partials + 8 * static_cast<long>(blockIdx.x));
+}
+
+// Backward: B_s = sum over the 16 derivative spectra times species s' weights.
+__global__ __launch_bounds__(kMixThreads) void k_mix_bwd(const cx<double>* __restrict__ D, cx<double>* __restrict__ B,
+                                                         Geom g, MixParams p) {
+  const long M = g.M;
+  for (long i = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; i < M;
+       i += static_cast<long>(gridDim.x) * kMixThreads) {
+    double kx, ky, kz;
+    kvec3(i, g, &kx, &ky, &kz);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double k = sqrt(k2);
+    cx<double> d[kMixFmt];
+#pragma unroll
+    for (int c = 0; c < kMixFmt; ++c) d[c] = D[c * M + i];
+    for (int s = 0; s < p.S; ++s) {
+      double w3, w2;
+      radial(k, p.R[s], g.inv_n, &w3, &w2);
+      const WRad wr = wrad({w3, w2}, k2, p.invR[s]);
+      cx<double> b = cscale(d[0], w3);
+      const cx<double> scal = cadd(cadd(d[1], cscale(d[2], p.ifr[s])), cscale(d[3], p.ifr2[s]));
+      b = cadd(b, cscale(scal, w2));
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const cx<double> dv = cadd(d[4 + a], cscale(d[7 + a], p.ifr[s]));
+        b = cadd(b, cmul(channel_weight_t(2 + a, true, kx, ky, kz, wr, 0.0, p.invR[s]), dv));
+      }
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        b = cadd(b, cscale(d[10 + q], channel_weight_t(5 + q, false, kx, ky, kz, wr, 0.0, p.invR[s]).re));
+      B[s * M + i] = b;
+    }
+  }
+}
+
+unsigned spectral_blocks(long M) { return static_cast<unsigned>(std::min<long>(2048, (M + kMixThreads - 1) / kMixThreads)); }
+
+MixParams mix_params(const Context* c) {
+  const double PI = 3.14159265358979323846;
+  MixParams p{};
+  p.S = c->n_species;
+  p.ninter = static_cast<int>(c->inters.size());
+  p.hs = c->cs.hs;
+  for (int s = 0; s < p.S; ++s) {
+    const double R = 0.5 * c->diam[s];
+    p.R[s] = R;
+    p.invR[s] = 1.0 / R;
+    p.ifr[s] = 1.0 / (4.0 * PI * R);
+    p.ifr2[s] = p.ifr[s] / R;
+  }
+  for (int q = 0; q < p.ninter; ++q) {
+    p.si[q] = c->inters[q].si;
+    p.sj[q] = c->inters[q].sj;
+  }
+  return p;
+}
+
+// Forward transform of `count` real fields (N apart) into spectra (M apart).
+void r2c_fields(Context* c, const double* in, cx<double>* out, int count) {
+  const Geom& g = c->g;
+  const AliasMap id{0, 0.0, 0.0};
+  for (int f = 0; f < count; ++f)
+    launch_zr2c<double, double>(in + static_cast<long>(f) * g.N, out + static_cast<long>(f) * g.M, g, c->plan[2],
+                                c->tw<double>(2), id, nullptr, c->stream);
+  launch_lines<double, double>(out, static_cast<long>(g.nzh) * g.nx * count, c->plan[1], c->tw<double>(1), -1,
+                               c->stream);
+  launch_xlines<double, double>(out, count, g, c->plan[0], c->tw<double>(0), -1, c->stream);
+}
+
+// Inverse x and y transforms of `count` spectra in place (the z C2R follows).
+void inverse_xy(Context* c, cx<double>* sp, int count) {
+  const Geom& g = c->g;
+  launch_xlines<double, double>(sp, count, g, c->plan[0], c->tw<double>(0), +1, c->stream);
+  launch_lines<double, double>(sp, static_cast<long>(g.nzh) * g.nx * count, c->plan[1], c->tw<double>(1), +1,
+                               c->stream);
+}
+
+// One evaluation on device arrays (S * N each; vext and force nullable);
+// the sums land in the layout's res block.
+void mixture_eval(Context* c, const double* rho, const double* vext, const double* mu, double* force) {
+  const Geom& g = c->g;
+  const MixLayout L = mix_layout(c);
+  const MixParams p = mix_params(c);
+  const int S = c->n_species;
+  const int nfm = c->cs.hs ? kMixFmt : 0, nu = c->inters.empty() ? 0 : S;
+  auto* R = at<cx<double>>(c, L.R);
+  auto* F = at<cx<double>>(c, L.F);
+  auto* X = at<double>(c, L.X);
+  auto* B = at<cx<double>>(c, L.B);
+  auto* local = at<double>(c, L.local);
+  auto* part = at<double>(c, L.part);
+  auto* res = at<double>(c, L.res);
+
+  if (nfm + nu > 0) {
+    r2c_fields(c, rho, R, S);
+    k_mix_fwd<<<spectral_blocks(g.M), kMixThreads, 0, c->stream>>>(R, c->d_mix_mf, F, g, p);
+    inverse_xy(c, F, nfm + nu);
+    launch_zc2r<double, double>(F, nfm + nu, X, nullptr, 1.0, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
+                                c->stream);
+  }
+  MixPoint mp{};
+  mp.S = S;
+  mp.hs = c->cs.hs;
+  mp.nfm = nfm;
+  mp.nu = nu;
+  for (int s = 0; s < S; ++s) mp.mu[s] = mu[s];
+  mp.dv = g.dv;
+  mp.fp = c->fp;
+  k_mix_point<<<L.nblk, kMixThreads, 0, c->stream>>>(X, rho, vext, local, part, g.N, mp);
+  if (force) {
+    if (nfm > 0) {
+      r2c_fields(c, X, F, kMixFmt);
+      k_mix_bwd<<<spectral_blocks(g.M), kMixThreads, 0, c->stream>>>(F, B, g, p);
+      inverse_xy(c, B, S);
+      for (int s = 0; s < S; ++s)
+        launch_zc2r<double, double>(B + static_cast<long>(s) * g.M, 1, force + static_cast<long>(s) * g.N,
+                                    local + static_cast<long>(s) * g.N, g.dv, g, c->plan[2], c->tw<double>(2),
+                                    nullptr, DcTable{}, c->stream);
+    } else {
+      CDFT_HIP(hipMemcpyAsync(force, local, sizeof(double) * S * g.N, hipMemcpyDeviceToDevice, c->stream));
+    }
+  }
+  launch_finalize(part, L.nblk, res, c->stream);
+  CDFT_HIP(hipGetLastError());
+}
+
+void mixture_result(Context* c, cdft_result* out) {
+  const MixLayout L = mix_layout(c);
+  double res[8];
+  CDFT_HIP(hipMemcpyAsync(res, at<double>(c, L.res), sizeof(res), hipMemcpyDeviceToHost, c->stream));
+  CDFT_HIP(hipStreamSynchronize(c->stream));
+  const double dv = c->g.dv;
+  out->ideal = res[0] * dv;
+  out->external = res[1] * dv;
+  out->hard_sphere = res[2] * dv;
+  out->mean_field = res[3] * dv;
+  out->mass = res[5] * dv;
+  // functionals.hpp:79-92: F = F_id + F_ext + F_hs + F_mf, Omega = F - sum_s mu_s N_s
+  out->free_energy = out->ideal + out->external + out->hard_sphere + out->mean_field;
+  out->grand_potential = out->free_energy - res[4] * dv;
+}
+
+void check_mu(const double* mu, int S) {
+  if (!mu) throw std::invalid_argument("chemical potentials are required");
+  for (int s = 0; s < S; ++s)
+    if (!std::isfinite(mu[s])) throw std::invalid_argument("chemical potential is not finite");
+}
+
+}  // namespace
+
+void init_mixture(Context* c) {
+  const Geom& g = c->g;
+  const MixLayout L = mix_layout(c);
+  CDFT_HIP(hipMalloc(&c->d_mix, L.total));
+  const int ni = static_cast<int>(c->inters.size());
+  if (ni == 0) return;
+  CDFT_HIP(hipMalloc(reinterpret_cast<void**>(&c->d_mix_mf), sizeof(double) * g.M * ni));
+  // the B block (S * M complex, S >= 2) is free scratch at set-up
+  auto* spec = at<cx<double>>(c, L.B);
+  for (int q = 0; q < ni; ++q) {
+    c->a_vdw.push_back(mf_weight_spectrum(c, c->inters[q], spec));
+    store_real_spectrum(spec, c->d_mix_mf + static_cast<long>(q) * g.M, g.M, c->stream);
+  }
+  CDFT_HIP(hipStreamSynchronize(c->stream));
+}
+
+}  // namespace cdft
+
+using namespace cdft;
+
+extern "C" {
+
+int cdft_species_count(const cdft_ctx* ctx, int* n_species) {
+  return guarded([&] {
+    check_ctx_any(ctx);
+    if (!n_species) throw std::invalid_argument("null argument");
+    *n_species = ctx->impl->n_species;
+  });
+}
+
+// One-species contexts take the one-species path with mu[0].
+int cdft_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, const double* mu,
+                              cdft_result* result, double* d_forces) {
+  int n = 0;
+  const int rc = cdft_species_count(ctx, &n);
+  if (rc != CDFT_OK) return rc;
+  if (!mu) return guarded([] { throw std::invalid_argument("chemical potentials are required"); });
+  if (n == 1) return cdft_total_device(ctx, d_rho, d_vext, mu[0], result, d_forces);
+  return guarded([&] {
+    Context* c = ctx->impl;
+    if (!d_rho || !result) throw std::invalid_argument("cdft_total_species_device: rho and result are required");
+    check_mu(mu, c->n_species);
+    CDFT_HIP(hipSetDevice(c->device));
+    mixture_eval(c, d_rho, d_vext, mu, d_forces);
+    mixture_result(c, result);
+  });
+}
+
+int cdft_total_species(cdft_ctx* ctx, const double* rho, const double* vext, const double* mu, cdft_result* result,
+                       double* forces) {
+  int n = 0;
+  const int rc = cdft_species_count(ctx, &n);
+  if (rc != CDFT_OK) return rc;
+  if (!mu) return guarded([] { throw std::invalid_argument("chemical potentials are required"); });
+  if (n == 1) return cdft_total(ctx, rho, vext, mu[0], result, forces);
+  return guarded([&] {
+    Context* c = ctx->impl;
+    if (!rho || !result) throw std::invalid_argument("cdft_total_species: rho and result are required");
+    check_mu(mu, c->n_species);
+    CDFT_HIP(hipSetDevice(c->device));
+    const MixLayout L = mix_layout(c);
+    const size_t bytes = sizeof(double) * c->n_species * c->g.N;
+    double* d_rho = at<double>(c, L.rho);
+    double* d_vext = vext ? at<double>(c, L.vext) : nullptr;
+    double* d_force = forces ? at<double>(c, L.force) : nullptr;
+    CDFT_HIP(hipMemcpyAsync(d_rho, rho, bytes, hipMemcpyHostToDevice, c->stream));
+    if (vext) CDFT_HIP(hipMemcpyAsync(d_vext, vext, bytes, hipMemcpyHostToDevice, c->stream));
+    mixture_eval(c, d_rho, d_vext, mu, d_force);
+    mixture_result(c, result);
+    if (forces) {
+      CDFT_HIP(hipMemcpyAsync(forces, d_force, bytes, hipMemcpyDeviceToHost, c->stream));
+      CDFT_HIP(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int cdft_time_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, const double* mu,
+                                   int count, double* d_forces, double* ms) {
+  return guarded([&] {
+    check_ctx_any(ctx);
+    Context* c = ctx->impl;
+    if (c->n_species == 1) throw std::invalid_argument("one-species context: use cdft_time_total_device");
+    if (!d_rho || !ms || count < 1) throw std::invalid_argument("cdft_time_total_species_device: bad argument");
+    check_mu(mu, c->n_species);
+    CDFT_HIP(hipSetDevice(c->device));
+    hipEvent_t e0, e1;
+    CDFT_HIP(hipEventCreate(&e0));
+    CDFT_HIP(hipEventCreate(&e1));
+    CDFT_HIP(hipEventRecord(e0, c->stream));
+    for (int i = 0; i < count; ++i) mixture_eval(c, d_rho, d_vext, mu, d_forces);
+    CDFT_HIP(hipEventRecord(e1, c->stream));
+    CDFT_HIP(hipEventSynchronize(e1));
+    float t = 0.0f;
+    CDFT_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  });
+}
+
+}  // extern "C"

@@ -87,7 +87,11 @@ typedef struct {
 
 /* Replaces functionals::make_weights + make_functional
  * (include/dft/functionals/functionals.hpp:27-33, functional.hpp:195-206):
- * builds the FMT and mean-field weights for a physics::Model. */
+ * builds the FMT and mean-field weights for a physics::Model.  n_species is
+ * 1..8.  A context of more than one species (a mixture; FP64 precision only,
+ * up to 36 interactions between any species pairs) is evaluated by
+ * cdft_total_species; the other evaluation and algorithm entry points act on
+ * one species and return CDFT_EUNSUPPORTED for it. */
 CDFT_API int cdft_ctx_create(const cdft_grid* grid, const double* diameters, int n_species,
                     const cdft_interaction* interactions, int n_interactions, double kT,
                     const cdft_options* options, cdft_ctx** out);
@@ -131,6 +135,24 @@ CDFT_API int cdft_total_device(cdft_ctx* ctx, const double* d_rho, const double*
  * returned. */
 CDFT_API int cdft_total_device_repeat(cdft_ctx* ctx, const double* d_rho, const double* d_vext, double mu, int count,
                              cdft_result* result, double* d_forces);
+
+/* physics::Model::species.size() (physics/model.hpp:15-22). */
+CDFT_API int cdft_species_count(const cdft_ctx* ctx, int* n_species);
+/* functionals::total for a state of any number of species
+ * (functionals.hpp:64-95 with hard_sphere.hpp:174-254 and
+ * mean_field.hpp:233-281): rho, vext (nullable) and forces (nullable) are host
+ * arrays of n_species * N doubles, species-major; mu holds n_species values of
+ * beta*mu.  result->mass is the total sum rho dV over species.  On a
+ * one-species context this is cdft_total with mu[0]. */
+CDFT_API int cdft_total_species(cdft_ctx* ctx, const double* rho, const double* vext, const double* mu,
+                                cdft_result* result, double* forces);
+/* Same with device-resident arrays; returns when the result is complete. */
+CDFT_API int cdft_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, const double* mu,
+                                       cdft_result* result, double* d_forces);
+/* GPU time of `count` back-to-back mixture evaluations (HIP events on the
+ * context stream; no counterpart in the reference). */
+CDFT_API int cdft_time_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext,
+                                            const double* mu, int count, double* d_forces, double* ms);
 
 /* Measurement hooks (no counterpart in the reference, which has no timers):
  * GPU time of `count` back-to-back evaluations measured with HIP events on the

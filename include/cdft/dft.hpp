@@ -16,6 +16,7 @@
 // at most one mean-field interaction, grid lengths with prime factors 2, 3, 5.
 #pragma once
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <functional>
@@ -259,7 +260,7 @@ inline bool all_zero(const vec& v) {
 // functionals.hpp:27-33 (+ the grid-synchronised a_vdw of functional.hpp:195-206).
 inline Weights make_weights(const fmt::FMTModel& fmt_model, const physics::Model& model,
                             Precision precision = Precision::FP64, int device = 0) {
-  if (model.species.size() != 1) throw std::invalid_argument("the GPU path evaluates one species");
+  if (model.species.empty()) throw std::invalid_argument("the model has no species");
   cdft_grid g{model.grid.dx, {model.grid.box_size[0], model.grid.box_size[1], model.grid.box_size[2]}};
   std::vector<double> diam;
   for (const auto& s : model.species) diam.push_back(s.hard_sphere_diameter);
@@ -278,10 +279,44 @@ inline Weights make_weights(const fmt::FMTModel& fmt_model, const physics::Model
   return Weights{fmt_model, std::shared_ptr<cdft_ctx>(raw, cdft_ctx_destroy), model.grid.total_points()};
 }
 
+namespace detail {
+// functionals.hpp:64-95 for more than one species: the densities, fields and
+// forces travel species-major through cdft_total_species.
+inline Result total_species(const physics::Model& model, const State& state, const Weights& weights) {
+  const size_t S = state.species.size();
+  if (S != model.species.size())
+    throw std::invalid_argument("the state has " + std::to_string(S) + " species, the model " +
+                                std::to_string(model.species.size()));
+  const size_t n = static_cast<size_t>(model.grid.total_points());
+  vec rho(S * n), v(S * n, 0.0), f(S * n), mu(S);
+  bool has_v = false;
+  for (size_t k = 0; k < S; ++k) {
+    const Density& d = state.species[k].density;
+    if (d.values.size() != n) throw std::invalid_argument("density size does not match the grid");
+    std::copy(d.values.begin(), d.values.end(), rho.begin() + static_cast<long>(k * n));
+    if (!d.external_field.empty() && !all_zero(d.external_field)) {
+      if (d.external_field.size() != n) throw std::invalid_argument("external field size does not match the grid");
+      std::copy(d.external_field.begin(), d.external_field.end(), v.begin() + static_cast<long>(k * n));
+      has_v = true;
+    }
+    mu[k] = state.species[k].chemical_potential;
+  }
+  cdft_result cr{};
+  ::dft::detail::check(cdft_total_species(weights.ctx.get(), rho.data(), has_v ? v.data() : nullptr, mu.data(), &cr,
+                                          f.data()));
+  Result r;
+  for (size_t k = 0; k < S; ++k)
+    r.forces.emplace_back(f.begin() + static_cast<long>(k * n), f.begin() + static_cast<long>((k + 1) * n));
+  r.free_energy = cr.free_energy;
+  r.grand_potential = cr.grand_potential;
+  return r;
+}
+}  // namespace detail
+
 // functionals.hpp:64-95: Omega, F and delta(beta Omega)/delta rho * dV of a
-// one-species state (its external field included when non-zero).
+// state (external fields included when non-zero).
 inline Result total(const physics::Model& model, const State& state, const Weights& weights) {
-  if (state.species.size() != 1) throw std::invalid_argument("the GPU path evaluates one species");
+  if (state.species.size() != 1) return detail::total_species(model, state, weights);
   const SpeciesState& s = state.species[0];
   const long n = model.grid.total_points();
   if (static_cast<long>(s.density.values.size()) != n)
@@ -301,8 +336,18 @@ inline Result total(const physics::Model& model, const State& state, const Weigh
   return r;
 }
 
+}  // namespace functionals
+
+// init.hpp:11-60 (namespace dft::init)
 namespace init {
-// init.hpp:28-58
+// a uniform density for every species of the model (init.hpp:13-25)
+inline State homogeneous(const physics::Model& model, double density) {
+  const auto n = static_cast<size_t>(model.grid.total_points());
+  State st{{}, model.temperature};
+  for (size_t k = 0; k < model.species.size(); ++k)
+    st.species.push_back(SpeciesState{Density{vec(n, density), vec(n, 0.0)}, vec(n, 0.0), 0.0, std::nullopt});
+  return st;
+}
 inline State from_profile(const physics::Model& model, const vec& rho) {
   const long n = model.grid.total_points();
   State st{{}, model.temperature};
@@ -319,7 +364,6 @@ inline State from_profiles(const physics::Model& model, const std::vector<vec>& 
   return st;
 }
 }  // namespace init
-}  // namespace functionals
 
 namespace algorithms {
 namespace dynamics {

@@ -68,10 +68,21 @@ int run_cpu() {
     std::printf("FAIL fmt id\n");
     ++failures;
   }
-  // rejected before any device work: two species
+  // rejected before any device work: a state whose species count differs from the model's
   expect_throw<std::invalid_argument>("species", [] {
-    (void)functionals::make_functional(functionals::fmt::WhiteBearII{}, lj_model(0.1, 1.6, 2));
-  }, "one species");
+    const auto model = lj_model(0.1, 1.6, 1);
+    const auto n = static_cast<size_t>(model.grid.total_points());
+    const auto state = init::from_profiles(model, {vec(n, 0.1), vec(n, 0.1)});
+    (void)functionals::total(model, state, functionals::Weights{});
+  }, "the state has 2 species, the model 1");
+  {
+    const auto st = init::homogeneous(lj_model(0.1, 1.6, 3), 0.25);
+    if (st.species.size() != 3 || st.species[2].density.values.size() != 4096 ||
+        st.species[1].density.values[17] != 0.25 || st.temperature != 0.8) {
+      std::printf("FAIL init::homogeneous\n");
+      ++failures;
+    }
+  }
   std::printf(failures ? "facade cpu checks: %d failures\n" : "facade cpu checks: ok\n", failures);
   return failures ? 1 : 0;
 }
@@ -143,6 +154,25 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   write_doubles(out + "/cn_rho.bin", cn.densities[0]);
   write_doubles(out + "/picard_rho.bin", pres.densities[0]);
 
+  // a binary mixture (diameters 1.0 and 0.8, a self and a cross pair) through the same facade calls
+  physics::Model mix = model;
+  mix.species.push_back(Species{"krypton", 0.8});
+  physics::Interaction cross;
+  cross.species_i = 0;
+  cross.species_j = 1;
+  cross.potential = physics::potentials::make_lennard_jones(0.9, 0.7, 2.5);
+  mix.interactions.push_back(cross);
+  auto mix_func = functionals::make_functional(functionals::fmt::WhiteBearII{}, mix);
+  vec rho_b(rho.size());
+  for (size_t i = 0; i < rho.size(); ++i) rho_b[i] = 0.5 * rho[i] + 0.02;
+  State mix_state = init::from_profiles(mix, {rho, rho_b});
+  mix_state.species[0].chemical_potential = mu;
+  mix_state.species[1].chemical_potential = -0.3;
+  const auto mres_mix = functionals::total(mix, mix_state, mix_func.weights);
+  vec mix_forces(mres_mix.forces[0]);
+  mix_forces.insert(mix_forces.end(), mres_mix.forces[1].begin(), mres_mix.forces[1].end());
+  write_doubles(out + "/mix_forces.bin", mix_forces);
+
   algorithms::dynamics::Simulation sim;
   sim.step.dt = 1e-3;
   sim.step.dt_max = 1e-3;
@@ -166,12 +196,12 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
               "\"fixed_mass_target\": %.17g, \"fixed_mass_iterations\": %d, \"fixed_mass_free_energy\": %.17g, "
               "\"eigenvalue\": %.17g, \"eigen_iterations\": %d, \"saddle_omega\": %.17g, "
               "\"saddle_eigenvalue\": %.17g, \"saddle_outer\": %d, \"split_energy\": %.17g, "
-              "\"cn_energy\": %.17g, \"picard_iterations\": %d, "
+              "\"cn_energy\": %.17g, \"picard_iterations\": %d, \"mixture_grand_potential\": %.17g, "
               "\"ddft_times\": [",
               mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential,
               0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations,
               sres_saddle.grand_potential, sres_saddle.eigenvalue, sres_saddle.outer_iterations, split.energy,
-              cn.energy, pres.iterations);
+              cn.energy, pres.iterations, mres_mix.grand_potential);
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);

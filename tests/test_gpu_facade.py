@@ -45,6 +45,13 @@ def test_cpp_facade_matches_oracle(tmp_path):
     assert _rel(f, fr) < 1e-11
     assert out["callback_omega"] == out["grand_potential"] and out["callback_force_maxdiff"] == 0.0
 
+    # a binary mixture through functionals::total (diameters 1.0, 0.8; self pair 0-0, cross pair 0-1)
+    mix = O.Oracle(shape, dx, diameters=(1.0, 0.8), kT=0.8, model="WhiteBearII",
+                   interactions=[W.LJ_WCA, dict(W.LJ_WCA, si=0, sj=1, sigma=0.9, epsilon=0.7)])
+    pm, fm = mix.total(np.stack([rho, 0.5 * rho + 0.02]), (mu, -0.3))
+    assert out["mixture_grand_potential"] == pytest.approx(pm[5], rel=1e-11)
+    assert _rel(np.fromfile(tmp_path / "mix_forces.bin").reshape((2,) + shape), fm) < 1e-11
+
     # Minimizer defaults of the reference: Unbounded, homogeneous boundary on
     ro, reso, _ = orc.minimize_grand_potential(rho, mu, fire=dict(force_tolerance=1e-9), max_steps=40)
     assert out["fire_iterations"] == reso["iterations"]

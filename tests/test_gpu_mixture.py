@@ -1,0 +1,128 @@
+"""GPU parity of multi-species evaluation (functionals::total for a mixture,
+functionals.hpp:64-95 with hard_sphere.hpp:174-254 and mean_field.hpp:233-281)
+against the FP64 CPU oracle, through the C ABI (cdft_total_species).
+
+Tolerances: energies 1e-12 relative (1e-14 absolute), forces 1e-11 relative
+to max|force| per species.
+"""
+import numpy as np
+import pytest
+
+from classicaldft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+LJ = W.LJ_WCA
+PARTS = ("ideal", "external", "hard_sphere", "mean_field", "free_energy", "grand_potential")
+
+
+def _pair(shape, diams, inter, model, kT=1.0, **kw):
+    import classicaldft_amd as cd
+    from oracle import oracle as O
+
+    dx = kw.pop("dx")
+    ctx = cd.Context(shape, dx, diameters=diams, interactions=list(inter), kT=kT, model=model, **kw)
+    orc = O.Oracle(shape, dx, diameters=diams, interactions=list(inter), kT=kT, model=model,
+                   A=kw.get("A", 1.0), B=kw.get("B", 0.0))
+    return ctx, orc
+
+
+def _densities(shape, dx, rho0s, seed):
+    return np.stack([W.smooth_noise_fluid(shape, dx, rho0=r, amp=0.08 * r, seed=seed + s)
+                     for s, r in enumerate(rho0s)])
+
+
+def _compare(ctx, orc, rho, mus, vext=None, tag=""):
+    res, f = ctx.total_species(rho, mus, vext=vext)
+    parts, fo = orc.total(rho, mus, vext=vext)
+    S = rho.shape[0]
+    print(f"{tag}: Omega gpu {res['grand_potential']:.15e} oracle {parts[5]:.15e}", flush=True)
+    for i, k in enumerate(PARTS):
+        assert res[k] == pytest.approx(parts[i], rel=1e-12, abs=1e-14), k
+    assert res["mass"] == pytest.approx(float(np.sum(rho)) * ctx.dx ** 3, rel=1e-13)
+    for s in range(S):
+        e = float(np.max(np.abs(f[s] - fo[s])) / np.max(np.abs(fo[s])))
+        assert e < 1e-11, (s, e)
+
+
+@pytest.mark.parametrize("model", ["Rosenfeld", "RSLT", "WhiteBearI", "WhiteBearII", "esFMT"])
+def test_binary_mixture_matches_oracle(model):
+    shape, dx = (32, 32, 32), 0.15
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=1, sigma=0.8, epsilon=0.6),
+             dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
+    ctx, orc = _pair(shape, (1.0, 0.8), inter, model, kT=1.1, dx=dx, A=1.2, B=-0.7)
+    assert ctx.n_species == 2 and ctx.schedule == "general"
+    for k in range(3):
+        assert ctx.a_vdw(k) == pytest.approx(orc.a_vdw(k), rel=1e-12)
+    rho = _densities(shape, dx, (0.3, 0.2), seed=11)
+    _compare(ctx, orc, rho, (0.4, -0.3), tag=f"binary {model}")
+
+
+def test_ternary_mixture_with_external_field_on_a_ragged_grid():
+    # 2,3,5-smooth extents exercise the radix-3/5 transforms; one cross pair only
+    shape, dx = (30, 24, 20), 0.2
+    inter = [dict(LJ, si=2, sj=0, epsilon=0.5)]
+    ctx, orc = _pair(shape, (1.0, 0.7, 1.3), inter, "WhiteBearII", kT=0.9, dx=dx)
+    rho = _densities(shape, dx, (0.2, 0.15, 0.1), seed=21)
+    z = (np.arange(shape[2]) + 0.5) * dx
+    vext = np.stack([np.broadcast_to(a * np.cos(2 * np.pi * z / (shape[2] * dx)), shape)
+                     for a in (0.5, -0.3, 0.2)])
+    _compare(ctx, orc, rho, (0.1, 0.0, -0.5), vext=vext, tag="ternary")
+
+
+def test_mixture_with_a_vacuum_region_and_ideal_only():
+    # eta < 1e-30 points are skipped (hard_sphere.hpp:199); ideal + mean field without FMT
+    shape, dx = (24, 24, 24), 0.2
+    rho = _densities(shape, dx, (0.3, 0.2), seed=31)
+    rho[:, :, :, :6] = 0.0
+    ctx, orc = _pair(shape, (1.0, 0.8), [dict(LJ, si=0, sj=1)], "WhiteBearII", dx=dx)
+    _compare(ctx, orc, rho, (0.0, 0.0), tag="vacuum")
+    ctx2, orc2 = _pair(shape, (1.0, 0.8), [dict(LJ, si=0, sj=1)], "none", dx=dx)
+    _compare(ctx2, orc2, np.maximum(rho, 1e-6), (0.0, 0.0), tag="ideal+mf")
+
+
+def test_uniform_binary_matches_reference_mixture_test():
+    # hard_sphere.cpp:208-248: uniform forces that differ between species
+    shape, dx = (16, 16, 16), 0.2
+    ctx, orc = _pair(shape, (1.0, 0.8), [], "Rosenfeld", dx=dx)
+    rho = np.stack([np.full(shape, 0.3), np.full(shape, 0.2)])
+    res, f = ctx.total_species(rho, (0.0, 0.0))
+    assert res["hard_sphere"] > 0.0
+    for s in range(2):
+        assert np.max(np.abs(f[s] - f[s].mean())) < 1e-8 * abs(f[s].mean())
+    assert abs(f[0].flat[0] - f[1].flat[0]) > 1e-10
+    _compare(ctx, orc, rho, (0.0, 0.0), tag="uniform")
+
+
+def test_one_species_context_through_species_entry_matches_total():
+    import classicaldft_amd as cd
+
+    shape, dx = (32, 32, 32), 0.15
+    ctx = cd.Context(shape, dx, interactions=[LJ], kT=0.8, model="WhiteBearII")
+    rho = W.smooth_noise_fluid(shape, dx, rho0=0.4, amp=0.03, seed=5)
+    r1, f1 = ctx.total(rho, 0.2)
+    r2, f2 = ctx.total_species(rho[None], (0.2,))
+    assert r1 == r2
+    assert np.array_equal(f1, f2[0])
+
+
+def test_mixture_contexts_refuse_one_species_entry_points_and_bad_input():
+    import classicaldft_amd as cd
+
+    shape, dx = (16, 16, 16), 0.2
+    ctx = cd.Context(shape, dx, diameters=(1.0, 0.8), model="WhiteBearII")
+    rho = np.full(shape, 0.3)
+    with pytest.raises(cd.CdftError) as e:
+        ctx.total(rho, 0.0)
+    assert e.value.code == -4  # CDFT_EUNSUPPORTED
+    with pytest.raises(cd.CdftError) as e:
+        cd.Context(shape, dx, diameters=(1.0, 0.8), model="WhiteBearII", precision="mixed")
+    assert e.value.code == -4
+    with pytest.raises(cd.InvalidArgument):
+        cd.Context(shape, dx, diameters=(1.0, 0.8), interactions=[dict(LJ, si=0, sj=2)])
+    with pytest.raises(cd.InvalidArgument):
+        cd.Context(shape, dx, diameters=(1.0, -0.8))
+    with pytest.raises(ValueError):
+        ctx.total_species(rho, (0.0, 0.0))  # one species' worth of values
+    with pytest.raises(cd.InvalidArgument):
+        ctx.total_species(np.stack([rho, rho]), (0.0, float("nan")))

@@ -681,3 +681,105 @@ def test_picard_ideal_gas_reaches_exp_mu():
     out, res = orc.picard(rho0, math.log(0.3), mixing=0.5, tolerance=1e-12, max_iterations=2000, target_mass=target)
     np.testing.assert_allclose(out, 0.25, rtol=1e-6)
     assert float(np.sum(out)) * dx ** 3 == pytest.approx(target, rel=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# Mixtures (hard_sphere.hpp:174-254, mean_field.hpp:233-281, functionals.hpp:64-95)
+# ---------------------------------------------------------------------------
+
+LJ_MIX = dict(potential="LJ", sigma=1.0, epsilon=1.0, r_cutoff=2.5, split="WCA", scheme="quadraticF")
+
+
+def _rosenfeld_mixture(rhos, diams):
+    """Bulk Rosenfeld (scaled-particle / Percus-Yevick compressibility) mixture:
+    Phi and d Phi / d rho_s from n3 = sum rho pi d^3/6, n2 = sum rho pi d^2,
+    n1 = sum rho d/2, n0 = sum rho."""
+    rhos, diams = np.asarray(rhos, float), np.asarray(diams, float)
+    dn = np.stack([np.pi * diams ** 3 / 6, np.pi * diams ** 2, diams / 2, np.ones_like(diams)])  # [alpha, s]
+    n3, n2, n1, n0 = dn @ rhos
+    e = 1.0 - n3
+    phi = -n0 * np.log(e) + n1 * n2 / e + n2 ** 3 / (24 * np.pi * e ** 2)
+    d_n3 = n0 / e + n1 * n2 / e ** 2 + n2 ** 3 / (12 * np.pi * e ** 3)
+    d_n2 = n1 / e + n2 ** 2 / (8 * np.pi * e ** 2)
+    d_n1 = n2 / e
+    d_n0 = -np.log(e)
+    mu_ex = np.array([d_n3, d_n2, d_n1, d_n0]) @ dn
+    return phi, mu_ex
+
+
+def test_oracle_mixture_uniform_matches_scaled_particle_theory():
+    # binary fluid of the reference's hard_sphere.cpp:208-248 (d = 1.0, 0.8; rho = 0.3, 0.2)
+    shape, dx = (16, 16, 16), 0.2
+    diams, rhos, mus = (1.0, 0.8), (0.3, 0.2), (0.4, -0.7)
+    orc = O.Oracle(shape, dx, diameters=diams, model="Rosenfeld")
+    rho = np.stack([np.full(shape, r) for r in rhos])
+    parts, f = orc.total(rho, mus)
+    phi, mu_ex = _rosenfeld_mixture(rhos, diams)
+    vol, dv = np.prod(shape) * dx ** 3, dx ** 3
+    assert parts[2] == pytest.approx(phi * vol, rel=1e-12)
+    assert parts[0] == pytest.approx(sum(r * (np.log(r) - 1) for r in rhos) * vol, rel=1e-13)
+    assert parts[5] == pytest.approx(parts[4] - sum(m * r for m, r in zip(mus, rhos)) * vol, rel=1e-13)
+    for s in range(2):
+        expect = (np.log(rhos[s]) - mus[s] + mu_ex[s]) * dv
+        assert np.max(np.abs(f[s] - expect)) < 1e-12 * abs(expect) + 1e-15
+    assert abs(mu_ex[0] - mu_ex[1]) > 0.1  # the species differ
+
+
+def test_oracle_mixture_equal_diameters_reduce_to_one_species():
+    # two species of one diameter: the summed measures are those of rho_a + rho_b
+    from classicaldft_amd import workloads as W
+
+    shape, dx = (20, 18, 16), 0.2
+    ra = W.smooth_noise_fluid(shape, dx, rho0=0.35, amp=0.05, seed=3)
+    rb = W.smooth_noise_fluid(shape, dx, rho0=0.25, amp=0.05, seed=4)
+    for model in ("WhiteBearII", "RSLT"):
+        mix = O.Oracle(shape, dx, diameters=(1.0, 1.0), model=model)
+        one = O.Oracle(shape, dx, diameters=(1.0,), model=model)
+        pm, fm = mix.total(np.stack([ra, rb]), (0.0, 0.0))
+        p1, f1 = one.total(ra + rb, 0.0)
+        assert pm[2] == pytest.approx(p1[2], rel=1e-12)
+        dv = dx ** 3
+        hs1 = f1 - np.log(ra + rb) * dv
+        for s, r in enumerate((ra, rb)):
+            hs = fm[s] - np.log(r) * dv
+            assert np.max(np.abs(hs - hs1)) < 1e-11 * np.max(np.abs(hs1))
+
+
+@pytest.mark.parametrize("model", ["WhiteBearII", "Rosenfeld", "esFMT"])
+def test_oracle_mixture_forces_match_finite_differences(model):
+    # d Omega / d rho_s(p) = force_s(p) with unequal radii and a cross pair: pins the
+    # per-species radius in the weight combination and the halves of the cross term
+    from classicaldft_amd import workloads as W
+
+    shape, dx = (12, 12, 12), 0.25
+    diams = (1.0, 0.6)
+    inter = [dict(LJ_MIX, si=0, sj=0), dict(LJ_MIX, si=0, sj=1, epsilon=0.7)]
+    orc = O.Oracle(shape, dx, diameters=diams, interactions=inter, kT=1.2, model=model, A=1.2, B=-0.6)
+    rho = np.stack([W.smooth_noise_fluid(shape, dx, rho0=0.3, amp=0.06, seed=7),
+                    W.smooth_noise_fluid(shape, dx, rho0=0.2, amp=0.06, seed=8)])
+    mus = (0.3, -0.2)
+    _, f = orc.total(rho, mus)
+    h = 1e-6
+    rng = np.random.default_rng(1)
+    for s in range(2):
+        for _ in range(4):
+            p = tuple(int(i) for i in rng.integers(0, 12, size=3))
+            up, dn = rho.copy(), rho.copy()
+            up[(s,) + p] += h
+            dn[(s,) + p] -= h
+            fd = (orc.total(up, mus)[0][5] - orc.total(dn, mus)[0][5]) / (2 * h)
+            assert fd == pytest.approx(f[(s,) + p], rel=2e-6, abs=1e-9)
+
+
+def test_oracle_mixture_cross_mean_field_closed_form():
+    # mean_field.cpp:183-229 of the reference: F = 1/2 a rho_a rho_b V, forces 1/2 a rho_other dV
+    shape, dx = (16, 16, 16), 0.2
+    orc = O.Oracle(shape, dx, diameters=(1.0, 0.8), interactions=[dict(LJ_MIX, si=0, sj=1)], kT=1.0,
+                   model="none")
+    ra, rb = 0.3, 0.2
+    parts, f = orc.total(np.stack([np.full(shape, ra), np.full(shape, rb)]), (0.0, 0.0))
+    a = orc.a_vdw(0)
+    vol, dv = np.prod(shape) * dx ** 3, dx ** 3
+    assert parts[3] == pytest.approx(0.5 * a * ra * rb * vol, rel=1e-12)
+    assert np.max(np.abs(f[0] - (np.log(ra) + 0.5 * a * rb) * dv)) < 1e-14
+    assert np.max(np.abs(f[1] - (np.log(rb) + 0.5 * a * ra) * dv)) < 1e-14
