@@ -103,6 +103,8 @@ __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S
   const int nz = g.nz, nzh = g.nzh;
   const long l0 = static_cast<long>(blockIdx.x) * nlb;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+  in += static_cast<long>(blockIdx.y) * g.N;  // field blockIdx.y of a multi-field launch
+  out += static_cast<long>(blockIdx.y) * g.M;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
     const int ll = e / nz, z = e - ll * nz;
     const long l = l0 + ll;
@@ -121,7 +123,7 @@ __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S
   }
 }
 
-// Generic z-C2R of `nch` channels: out[c*N + p] = scale * c2r(S_c)[p] + cst_c + (add ? add[p] : 0)
+// Generic z-C2R of `nch` channels: out[c*N + p] = scale * c2r(S_c)[p] + cst_c + (add ? add[c*N + p] : 0)
 // with cst_c = dc.v[c] * shift_sum[0] / N (the mean-density split) when shift_sum is given.
 template <typename S, typename C>
 __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, double* __restrict__ out,
@@ -145,6 +147,7 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
   __syncthreads();
   fft_inplace<C, 8>(buf, nlb >> 1, nz, pz, +1, twz);
   double* oc = out + static_cast<long>(ch) * g.N;
+  const double* ac = add ? add + static_cast<long>(ch) * g.N : nullptr;
   const double cst = shift_sum ? dc.v[ch] * shift_sum[0] * g.inv_n : 0.0;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
     const int ll = e / nz, z = e - ll * nz;
@@ -153,7 +156,7 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
     const cx<C> v = buf[(ll >> 1) * nz + z];
     const long p = l * nz + z;
     const double val = static_cast<double>((ll & 1) ? v.im : v.re) * scale + cst;
-    oc[p] = add ? val + add[p] : val;
+    oc[p] = ac ? val + ac[p] : val;
   }
 }
 
@@ -585,6 +588,19 @@ void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz,
 }
 
 template <typename S, typename C>
+void launch_zr2c_fields(const double* in, int nfields, cx<S>* out, const Geom& g, const FftPlan& pz,
+                        const cx<C>* twz, hipStream_t st) {
+  const int nlb = zlines_per_block<C>(pz);
+  if ((nlb >> 1) * g.nz > 8 * NT) throw std::runtime_error("z extent too large for the z transform kernel");
+  if (nfields < 1 || nfields > 65535) throw std::runtime_error("field count out of range");
+  const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
+  const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz;
+  const AliasMap id{0, 0.0, 0.0};
+  hipLaunchKernelGGL((k_zr2c<S, C>), dim3(nblk, nfields), dim3(NT), sh, st, in, out, g, pz, twz, nlb, id,
+                     static_cast<const double*>(nullptr));
+}
+
+template <typename S, typename C>
 void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, double scale, const Geom& g,
                  const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st) {
   const int nlb = zlines_per_block<C>(pz);
@@ -740,6 +756,8 @@ void launch_finalize(const double* partials, int nrows, double* res, hipStream_t
 #define CDFT_INST_SC(S, C)                                                                                         \
   template void launch_zr2c<S, C>(const double*, cx<S>*, const Geom&, const FftPlan&, const cx<C>*, AliasMap,      \
                                   const double*, hipStream_t);                                                     \
+  template void launch_zr2c_fields<S, C>(const double*, int, cx<S>*, const Geom&, const FftPlan&, const cx<C>*,    \
+                                         hipStream_t);                                                             \
   template void launch_zc2r<S, C>(const cx<S>*, int, double*, const double*, double, const Geom&, const FftPlan&, \
                                   const cx<C>*, const double*, DcTable, hipStream_t);                              \
   template void launch_lines<S, C>(cx<S>*, long, const FftPlan&, const cx<C>*, int, hipStream_t);                 \

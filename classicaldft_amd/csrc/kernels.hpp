@@ -53,6 +53,11 @@ struct DcTable {
 template <typename S, typename C>
 void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz, const cx<C>* twz, AliasMap amap,
                  const double* shift_sum, hipStream_t st);
+// z-R2C of `nfields` real fields (N apart) into spectra (M apart), one launch,
+// no density map or mean split.
+template <typename S, typename C>
+void launch_zr2c_fields(const double* in, int nfields, cx<S>* out, const Geom& g, const FftPlan& pz,
+                        const cx<C>* twz, hipStream_t st);
 template <typename S, typename C>
 void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, double scale, const Geom& g,
                  const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st);

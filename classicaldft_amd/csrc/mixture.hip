@@ -106,10 +106,10 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __res
                                                          const double* __restrict__ mfw, cx<double>* __restrict__ F,
                                                          Geom g, MixParams p) {
   const long M = g.M;
+  // (species values are read where used: a register array indexed by a
+  // runtime species number would live in scratch memory)
   for (long i = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; i < M;
        i += static_cast<long>(gridDim.x) * kMixThreads) {
-    cx<double> rk[kMaxSpecies];
-    for (int s = 0; s < p.S; ++s) rk[s] = rhoK[s * M + i];
     int ch = 0;
     if (p.hs) {
       double kx, ky, kz;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __res
         double w3, w2;
         radial(k, p.R[s], g.inv_n, &w3, &w2);
         const WRad wr = wrad({w3, w2}, k2, p.invR[s]);
-        const cx<double> r = rk[s];
+        const cx<double> r = rhoK[s * M + i];
         acc[0] = cadd(acc[0], cscale(r, w3));
         acc[1] = cadd(acc[1], cscale(r, w2));
         acc[2] = cadd(acc[2], cscale(r, w2 * p.ifr[s]));
@@ -143,19 +143,19 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __res
       ch = kMixFmt;
     }
     if (p.ninter > 0) {
-      cx<double> u[kMaxSpecies];
-      for (int s = 0; s < p.S; ++s) u[s] = {0.0, 0.0};
-      for (int q = 0; q < p.ninter; ++q) {
-        const double w = mfw[q * M + i];
-        const int a = p.si[q], b = p.sj[q];
-        if (a == b) {
-          u[a] = cadd(u[a], cscale(rk[a], w));
-        } else {
-          u[a] = cadd(u[a], cscale(rk[b], 0.5 * w));
-          u[b] = cadd(u[b], cscale(rk[a], 0.5 * w));
+      for (int s = 0; s < p.S; ++s) {
+        cx<double> u = {0.0, 0.0};
+        for (int q = 0; q < p.ninter; ++q) {
+          const int a = p.si[q], b = p.sj[q];
+          if (a != s && b != s) continue;
+          const double w = mfw[q * M + i];
+          if (a == b)
+            u = cadd(u, cscale(rhoK[s * M + i], w));  // self pair
+          else
+            u = cadd(u, cscale(rhoK[(a == s ? b : a) * M + i], 0.5 * w));  // half of a cross pair
         }
+        F[(ch + s) * M + i] = u;
       }
-      for (int s = 0; s < p.S; ++s) F[(ch + s) * M + i] = u[s];
     }
   }
 }
@@ -259,10 +259,7 @@ MixParams mix_params(const Context* c) {
 // Forward transform of `count` real fields (N apart) into spectra (M apart).
 void r2c_fields(Context* c, const double* in, cx<double>* out, int count) {
   const Geom& g = c->g;
-  const AliasMap id{0, 0.0, 0.0};
-  for (int f = 0; f < count; ++f)
-    launch_zr2c<double, double>(in + static_cast<long>(f) * g.N, out + static_cast<long>(f) * g.M, g, c->plan[2],
-                                c->tw<double>(2), id, nullptr, c->stream);
+  launch_zr2c_fields<double, double>(in, count, out, g, c->plan[2], c->tw<double>(2), c->stream);
   launch_lines<double, double>(out, static_cast<long>(g.nzh) * g.nx * count, c->plan[1], c->tw<double>(1), -1,
                                c->stream);
   launch_xlines<double, double>(out, count, g, c->plan[0], c->tw<double>(0), -1, c->stream);
@@ -313,10 +310,8 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
       r2c_fields(c, X, F, kMixFmt);
       k_mix_bwd<<<spectral_blocks(g.M), kMixThreads, 0, c->stream>>>(F, B, g, p);
       inverse_xy(c, B, S);
-      for (int s = 0; s < S; ++s)
-        launch_zc2r<double, double>(B + static_cast<long>(s) * g.M, 1, force + static_cast<long>(s) * g.N,
-                                    local + static_cast<long>(s) * g.N, g.dv, g, c->plan[2], c->tw<double>(2),
-                                    nullptr, DcTable{}, c->stream);
+      launch_zc2r<double, double>(B, S, force, local, g.dv, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
+                                  c->stream);
     } else {
       CDFT_HIP(hipMemcpyAsync(force, local, sizeof(double) * S * g.N, hipMemcpyDeviceToDevice, c->stream));
     }
