@@ -11,7 +11,7 @@
 
 #include "cdft.h"
 #include "eval_fast.hpp"
-#include "host_weights.hpp"
+#include "pair_potential.hpp"
 #include "kernels.hpp"
 
 namespace cdft {
