@@ -26,6 +26,7 @@ EXPORTED = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_grid_shape", "cdft_a_vdw",
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
     "cdft_species_count", "cdft_total_species", "cdft_total_species_device", "cdft_time_total_species_device",
+    "cdft_bulk_free_energy_density", "cdft_bulk_chemical_potentials", "cdft_bulk_pressure",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
@@ -152,6 +153,9 @@ def lib():
         L.cdft_bulk_chemical_potential.argtypes = [C.c_void_p, C.c_double, _dp]
         L.cdft_total.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.POINTER(_Result), _dp]
         L.cdft_species_count.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.cdft_bulk_free_energy_density.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
+        L.cdft_bulk_chemical_potentials.argtypes = [C.c_void_p, _dp, _dp]
+        L.cdft_bulk_pressure.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
         L.cdft_total_species.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(_Result), _dp]
         L.cdft_total_species_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.POINTER(_Result),
                                                 C.c_void_p]
@@ -276,6 +280,31 @@ class Context:
     def bulk_mu(self, rho):
         v = C.c_double()
         _check(lib().cdft_bulk_chemical_potential(self._h, float(rho), C.byref(v)))
+        return v.value
+
+    def _bulk_rho(self, rho):
+        r = np.ascontiguousarray(np.atleast_1d(np.asarray(rho, dtype=np.float64)))
+        if r.size != self.n_species:
+            raise ValueError(f"expected {self.n_species} bulk densities, got {r.size}")
+        return r
+
+    def bulk_free_energy_density(self, rho):
+        """BulkThermodynamics::free_energy_density (kT units per volume); rho has n_species values."""
+        v = C.c_double()
+        _check(lib().cdft_bulk_free_energy_density(self._h, _p(self._bulk_rho(rho)), C.byref(v)))
+        return v.value
+
+    def bulk_chemical_potentials(self, rho):
+        """BulkThermodynamics::chemical_potentials: one value per species."""
+        r = self._bulk_rho(rho)
+        mu = np.zeros(self.n_species)
+        _check(lib().cdft_bulk_chemical_potentials(self._h, _p(r), _p(mu)))
+        return mu
+
+    def bulk_pressure(self, rho):
+        """BulkThermodynamics::pressure = sum rho mu - f."""
+        v = C.c_double()
+        _check(lib().cdft_bulk_pressure(self._h, _p(self._bulk_rho(rho)), C.byref(v)))
         return v.value
 
     def total(self, rho, mu, vext=None, forces=True):

@@ -336,6 +336,78 @@ void mixture_result(Context* c, cdft_result* out) {
   out->grand_potential = out->free_energy - res[4] * dv;
 }
 
+// Bulk (uniform) thermodynamics, bulk/thermodynamics.hpp:17-255: the summed
+// uniform measures of fmt::make_uniform_measures (measures.hpp:49-65) in the
+// mixture channel order.
+void uniform_measures(const Context* c, const double* rho, double (&m)[kMixFmt]) {
+  const double PI = 3.14159265358979323846;
+  for (double& x : m) x = 0.0;
+  for (int s = 0; s < c->n_species; ++s) {
+    const double d = c->diam[s], r = rho[s];
+    const double n2 = PI * r * d * d;
+    m[0] += PI / 6.0 * r * d * d * d;
+    m[1] += n2;
+    m[2] += 0.5 * d * r;
+    m[3] += r;
+    m[10] += n2 / 3.0;  // T_xx, T_yy, T_zz
+    m[13] += n2 / 3.0;
+    m[15] += n2 / 3.0;
+  }
+}
+
+void check_bulk_rho(const double* rho, int S) {
+  if (!rho) throw std::invalid_argument("densities are required");
+  for (int s = 0; s < S; ++s)
+    if (!(rho[s] > 0.0) || !std::isfinite(rho[s])) throw std::invalid_argument("bulk densities must be positive");
+}
+
+// BulkThermodynamics::free_energy_density (thermodynamics.hpp:206-220).  As
+// in the reference, a cross pair counts a rho_i rho_j here while the
+// functional and the chemical potential give it half of that.
+double bulk_free_energy_density(const Context* c, const double* rho) {
+  double f = 0.0;
+  for (int s = 0; s < c->n_species; ++s) f += rho[s] * (std::log(rho[s]) - 1.0);  // ideal, :19-27
+  if (c->fp.model != FMT_NONE) {  // hard spheres, :38-52
+    double m[kMixFmt], g[kMixFmt];
+    uniform_measures(c, rho, m);
+    f += fmt_phi_grad_mix_t<double>(c->fp, m, g);
+  }
+  for (size_t q = 0; q < c->inters.size(); ++q) {
+    const int i = c->inters[q].si, j = c->inters[q].sj;
+    f += (i == j ? 0.5 : 1.0) * c->a_vdw[q] * rho[i] * rho[j];
+  }
+  return f;
+}
+
+// BulkThermodynamics::chemical_potential (thermodynamics.hpp:222-237).
+void bulk_chemical_potentials(const Context* c, const double* rho, double* mu) {
+  const double PI = 3.14159265358979323846;
+  double m[kMixFmt], g[kMixFmt];
+  const bool hs = c->fp.model != FMT_NONE;
+  if (hs) {
+    uniform_measures(c, rho, m);
+    fmt_phi_grad_mix_t<double>(c->fp, m, g);
+  }
+  for (int s = 0; s < c->n_species; ++s) {
+    double v = std::log(rho[s]);
+    if (hs) {  // excess_chemical_potential, :54-89 (the tensor partials are zero for scalar models)
+      const double d = c->diam[s];
+      v += g[0] * PI / 6.0 * d * d * d + g[1] * PI * d * d + g[2] * 0.5 * d + g[3];
+      v += (g[10] + g[13] + g[15]) * PI * d * d / 3.0;
+    }
+    for (size_t q = 0; q < c->inters.size(); ++q) {
+      const int i = c->inters[q].si, j = c->inters[q].sj;
+      if (i == s && j == s)
+        v += c->a_vdw[q] * rho[j];
+      else if (i == s)
+        v += 0.5 * c->a_vdw[q] * rho[j];
+      else if (j == s)
+        v += 0.5 * c->a_vdw[q] * rho[i];
+    }
+    mu[s] = v;
+  }
+}
+
 void check_mu(const double* mu, int S) {
   if (!mu) throw std::invalid_argument("chemical potentials are required");
   for (int s = 0; s < S; ++s)
@@ -417,6 +489,38 @@ int cdft_total_species(cdft_ctx* ctx, const double* rho, const double* vext, con
       CDFT_HIP(hipMemcpyAsync(forces, d_force, bytes, hipMemcpyDeviceToHost, c->stream));
       CDFT_HIP(hipStreamSynchronize(c->stream));
     }
+  });
+}
+
+int cdft_bulk_free_energy_density(const cdft_ctx* ctx, const double* rho, double* f) {
+  return guarded([&] {
+    check_ctx_any(ctx);
+    if (!f) throw std::invalid_argument("null argument");
+    check_bulk_rho(rho, ctx->impl->n_species);
+    *f = bulk_free_energy_density(ctx->impl, rho);
+  });
+}
+
+int cdft_bulk_chemical_potentials(const cdft_ctx* ctx, const double* rho, double* mu) {
+  return guarded([&] {
+    check_ctx_any(ctx);
+    if (!mu) throw std::invalid_argument("null argument");
+    check_bulk_rho(rho, ctx->impl->n_species);
+    bulk_chemical_potentials(ctx->impl, rho, mu);
+  });
+}
+
+int cdft_bulk_pressure(const cdft_ctx* ctx, const double* rho, double* p) {
+  return guarded([&] {
+    check_ctx_any(ctx);
+    if (!p) throw std::invalid_argument("null argument");
+    const Context* c = ctx->impl;
+    check_bulk_rho(rho, c->n_species);
+    std::vector<double> mu(static_cast<size_t>(c->n_species));
+    bulk_chemical_potentials(c, rho, mu.data());
+    double v = -bulk_free_energy_density(c, rho);  // P = sum rho mu - f (thermodynamics.hpp:239-246)
+    for (int s = 0; s < c->n_species; ++s) v += rho[s] * mu[s];
+    *p = v;
   });
 }
 

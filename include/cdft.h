@@ -149,6 +149,15 @@ CDFT_API int cdft_total_species(cdft_ctx* ctx, const double* rho, const double* 
 /* Same with device-resident arrays; returns when the result is complete. */
 CDFT_API int cdft_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, const double* mu,
                                        cdft_result* result, double* d_forces);
+/* bulk::BulkThermodynamics (bulk/thermodynamics.hpp:200-255) for the context's
+ * species, FMT model and grid-synchronised a_vdw: rho and mu hold n_species
+ * positive bulk densities and the chemical potentials (kT units).  As in the
+ * reference, the free-energy density counts a cross pair as a*rho_i*rho_j
+ * while the chemical potentials give it half (thermodynamics.hpp:206-237).
+ * The grand-potential density is -pressure. */
+CDFT_API int cdft_bulk_free_energy_density(const cdft_ctx* ctx, const double* rho, double* f);
+CDFT_API int cdft_bulk_chemical_potentials(const cdft_ctx* ctx, const double* rho, double* mu);
+CDFT_API int cdft_bulk_pressure(const cdft_ctx* ctx, const double* rho, double* p);
 /* GPU time of `count` back-to-back mixture evaluations (HIP events on the
  * context stream; no counterpart in the reference). */
 CDFT_API int cdft_time_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext,

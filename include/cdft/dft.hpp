@@ -373,6 +373,54 @@ using ForceCallback = std::function<std::pair<double, std::vector<vec>>(const st
 }  // namespace algorithms
 
 namespace functionals {
+namespace bulk {
+// bulk/thermodynamics.hpp:200-255: the homogeneous-fluid equation of state of
+// the model's species with the grid-synchronised a_vdw (one scalar evaluation
+// on the host per call).  As in the reference, the free-energy density counts
+// a cross pair as a rho_i rho_j and the chemical potentials give it half.
+struct BulkThermodynamics {
+  std::vector<Species> species;
+  fmt::FMTModel fmt_model;
+  std::shared_ptr<cdft_ctx> ctx;
+
+  double free_energy_density(const vec& rho) const {
+    check_size(rho);
+    double f = 0.0;
+    ::dft::detail::check(cdft_bulk_free_energy_density(ctx.get(), rho.data(), &f));
+    return f;
+  }
+  vec chemical_potentials(const vec& rho) const {
+    check_size(rho);
+    vec mu(rho.size());
+    ::dft::detail::check(cdft_bulk_chemical_potentials(ctx.get(), rho.data(), mu.data()));
+    return mu;
+  }
+  double chemical_potential(const vec& rho, size_t species_idx) const {
+    if (species_idx >= species.size()) throw std::invalid_argument("species index out of range");
+    return chemical_potentials(rho)[species_idx];
+  }
+  double pressure(const vec& rho) const {
+    check_size(rho);
+    double p = 0.0;
+    ::dft::detail::check(cdft_bulk_pressure(ctx.get(), rho.data(), &p));
+    return p;
+  }
+  double grand_potential_density(const vec& rho) const { return -pressure(rho); }
+
+ private:
+  void check_size(const vec& rho) const {
+    if (rho.size() != species.size())
+      throw std::invalid_argument("expected " + std::to_string(species.size()) + " bulk densities, got " +
+                                  std::to_string(rho.size()));
+  }
+};
+
+// make_bulk_thermodynamics(species, weights) (thermodynamics.hpp:261-275)
+inline BulkThermodynamics make_bulk_thermodynamics(const std::vector<Species>& species, const Weights& weights) {
+  return BulkThermodynamics{species, weights.fmt_model, weights.ctx};
+}
+}  // namespace bulk
+
 struct Functional;
 
 // The callable that Functional::grand_potential_callback returns; Simulation
@@ -411,6 +459,9 @@ struct Functional {
   algorithms::dynamics::ForceCallback grand_potential_callback(double chemical_potential) const {
     return GrandPotentialCallback{this, chemical_potential};
   }
+
+  // functional.hpp:48-50
+  bulk::BulkThermodynamics bulk() const { return bulk::make_bulk_thermodynamics(model.species, weights); }
 };
 
 inline std::pair<double, std::vector<vec>> GrandPotentialCallback::operator()(const std::vector<vec>& densities) const {

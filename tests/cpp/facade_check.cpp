@@ -172,6 +172,9 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   vec mix_forces(mres_mix.forces[0]);
   mix_forces.insert(mix_forces.end(), mres_mix.forces[1].begin(), mres_mix.forces[1].end());
   write_doubles(out + "/mix_forces.bin", mix_forces);
+  const auto eos = mix_func.bulk();
+  const vec mix_mu = eos.chemical_potentials({0.3, 0.2});
+  const double mix_p = eos.pressure({0.3, 0.2});
 
   algorithms::dynamics::Simulation sim;
   sim.step.dt = 1e-3;
@@ -197,11 +200,12 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
               "\"eigenvalue\": %.17g, \"eigen_iterations\": %d, \"saddle_omega\": %.17g, "
               "\"saddle_eigenvalue\": %.17g, \"saddle_outer\": %d, \"split_energy\": %.17g, "
               "\"cn_energy\": %.17g, \"picard_iterations\": %d, \"mixture_grand_potential\": %.17g, "
+              "\"mixture_bulk_mu\": [%.17g, %.17g], \"mixture_bulk_pressure\": %.17g, "
               "\"ddft_times\": [",
               mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential,
               0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations,
               sres_saddle.grand_potential, sres_saddle.eigenvalue, sres_saddle.outer_iterations, split.energy,
-              cn.energy, pres.iterations, mres_mix.grand_potential);
+              cn.energy, pres.iterations, mres_mix.grand_potential, mix_mu[0], mix_mu[1], mix_p);
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);

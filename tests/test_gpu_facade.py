@@ -51,6 +51,13 @@ def test_cpp_facade_matches_oracle(tmp_path):
     pm, fm = mix.total(np.stack([rho, 0.5 * rho + 0.02]), (mu, -0.3))
     assert out["mixture_grand_potential"] == pytest.approx(pm[5], rel=1e-11)
     assert _rel(np.fromfile(tmp_path / "mix_forces.bin").reshape((2,) + shape), fm) < 1e-11
+    # Functional::bulk() of the mixture: the same equation of state as the ctypes binding
+    import classicaldft_amd as cd
+
+    cm = cd.Context(shape, dx, diameters=(1.0, 0.8), kT=0.8, model="WhiteBearII",
+                    interactions=[W.LJ_WCA, dict(W.LJ_WCA, si=0, sj=1, sigma=0.9, epsilon=0.7)])
+    assert np.array_equal(out["mixture_bulk_mu"], cm.bulk_chemical_potentials([0.3, 0.2]))
+    assert out["mixture_bulk_pressure"] == cm.bulk_pressure([0.3, 0.2])
 
     # Minimizer defaults of the reference: Unbounded, homogeneous boundary on
     ro, reso, _ = orc.minimize_grand_potential(rho, mu, fire=dict(force_tolerance=1e-9), max_steps=40)

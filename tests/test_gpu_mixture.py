@@ -126,3 +126,83 @@ def test_mixture_contexts_refuse_one_species_entry_points_and_bad_input():
         ctx.total_species(rho, (0.0, 0.0))  # one species' worth of values
     with pytest.raises(cd.InvalidArgument):
         ctx.total_species(np.stack([rho, rho]), (0.0, float("nan")))
+
+
+# ---------------------------------------------------------------------------
+# Bulk thermodynamics (bulk/thermodynamics.hpp:17-255) for any species count
+# ---------------------------------------------------------------------------
+
+def test_bulk_one_species_matches_bulk_mu():
+    import classicaldft_amd as cd
+
+    for model in ("WhiteBearII", "Rosenfeld", "RSLT", "none"):
+        ctx = cd.Context((16, 16, 16), 0.2, interactions=[LJ], kT=0.9, model=model)
+        for rho in (0.05, 0.4, 0.8):
+            assert ctx.bulk_chemical_potentials(rho)[0] == pytest.approx(ctx.bulk_mu(rho), rel=1e-13, abs=1e-14)
+
+
+def test_bulk_binary_rosenfeld_matches_scaled_particle_theory():
+    import classicaldft_amd as cd
+    from closed_forms import rosenfeld_mixture
+
+    diams, rhos = (1.0, 0.8), np.array([0.3, 0.2])
+    ctx = cd.Context((16, 16, 16), 0.2, diameters=diams, model="Rosenfeld")
+    phi, mu_ex = rosenfeld_mixture(rhos, diams)
+    f_id = float(np.sum(rhos * (np.log(rhos) - 1.0)))
+    assert ctx.bulk_free_energy_density(rhos) == pytest.approx(f_id + phi, rel=1e-13)
+    assert np.allclose(ctx.bulk_chemical_potentials(rhos), np.log(rhos) + mu_ex, rtol=1e-13, atol=0)
+    mu = ctx.bulk_chemical_potentials(rhos)
+    assert ctx.bulk_pressure(rhos) == pytest.approx(float(rhos @ mu) - f_id - phi, rel=1e-13)
+
+
+@pytest.mark.parametrize("model", ["WhiteBearII", "esFMT", "RSLT"])
+def test_bulk_chemical_potentials_are_derivatives_with_self_pairs(model):
+    # with self pairs only, the reference's free energy and chemical potentials are consistent
+    import classicaldft_amd as cd
+
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=1, sigma=0.8, epsilon=0.6)]
+    ctx = cd.Context((16, 16, 16), 0.2, diameters=(1.0, 0.7), interactions=inter, kT=1.1, model=model,
+                     A=1.3, B=-0.4)
+    rhos = np.array([0.25, 0.3])
+    mu = ctx.bulk_chemical_potentials(rhos)
+    h = 1e-6
+    for s in range(2):
+        up, dn = rhos.copy(), rhos.copy()
+        up[s] += h
+        dn[s] -= h
+        fd = (ctx.bulk_free_energy_density(up) - ctx.bulk_free_energy_density(dn)) / (2 * h)
+        assert fd == pytest.approx(mu[s], rel=1e-8)
+
+
+def test_bulk_matches_the_functional_on_a_uniform_mixture():
+    # uniform state: the functional's force / dV is mu_bulk - mu; its F / V equals the bulk
+    # free-energy density except for the reference's extra half of every cross pair
+    import classicaldft_amd as cd
+    from oracle import oracle as O
+
+    shape, dx = (16, 16, 16), 0.2
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
+    ctx = cd.Context(shape, dx, diameters=(1.0, 0.8), interactions=inter, kT=1.0, model="WhiteBearII")
+    rhos = np.array([0.3, 0.2])
+    mus = (0.1, -0.4)
+    res, f = ctx.total_species(np.stack([np.full(shape, r) for r in rhos]), mus)
+    mu_b = ctx.bulk_chemical_potentials(rhos)
+    dv, vol = dx ** 3, np.prod(shape) * dx ** 3
+    for s in range(2):
+        assert np.max(np.abs(f[s] / dv - (mu_b[s] - mus[s]))) < 1e-11
+    a01 = ctx.a_vdw(1)
+    assert ctx.bulk_free_energy_density(rhos) == pytest.approx(
+        res["free_energy"] / vol + 0.5 * a01 * rhos[0] * rhos[1], rel=1e-12)
+    # and the a_vdw the bulk uses is the oracle's
+    orc = O.Oracle(shape, dx, diameters=(1.0, 0.8), interactions=inter, kT=1.0, model="WhiteBearII")
+    assert a01 == pytest.approx(orc.a_vdw(1), rel=1e-12)
+
+
+def test_bulk_rejects_bad_densities():
+    import classicaldft_amd as cd
+
+    ctx = cd.Context((16, 16, 16), 0.2, diameters=(1.0, 0.8), model="WhiteBearII")
+    with pytest.raises(ValueError):
+        ctx.bulk_chemical_potentials([0.3])
+    with pytest.raises(cd.InvalidArgument):
+        ctx.bulk_chemical_potentials([0.3, 0.0])

@@ -689,22 +689,7 @@ def test_picard_ideal_gas_reaches_exp_mu():
 
 LJ_MIX = dict(potential="LJ", sigma=1.0, epsilon=1.0, r_cutoff=2.5, split="WCA", scheme="quadraticF")
 
-
-def _rosenfeld_mixture(rhos, diams):
-    """Bulk Rosenfeld (scaled-particle / Percus-Yevick compressibility) mixture:
-    Phi and d Phi / d rho_s from n3 = sum rho pi d^3/6, n2 = sum rho pi d^2,
-    n1 = sum rho d/2, n0 = sum rho."""
-    rhos, diams = np.asarray(rhos, float), np.asarray(diams, float)
-    dn = np.stack([np.pi * diams ** 3 / 6, np.pi * diams ** 2, diams / 2, np.ones_like(diams)])  # [alpha, s]
-    n3, n2, n1, n0 = dn @ rhos
-    e = 1.0 - n3
-    phi = -n0 * np.log(e) + n1 * n2 / e + n2 ** 3 / (24 * np.pi * e ** 2)
-    d_n3 = n0 / e + n1 * n2 / e ** 2 + n2 ** 3 / (12 * np.pi * e ** 3)
-    d_n2 = n1 / e + n2 ** 2 / (8 * np.pi * e ** 2)
-    d_n1 = n2 / e
-    d_n0 = -np.log(e)
-    mu_ex = np.array([d_n3, d_n2, d_n1, d_n0]) @ dn
-    return phi, mu_ex
+from closed_forms import rosenfeld_mixture  # noqa: E402
 
 
 def test_oracle_mixture_uniform_matches_scaled_particle_theory():
@@ -714,7 +699,7 @@ def test_oracle_mixture_uniform_matches_scaled_particle_theory():
     orc = O.Oracle(shape, dx, diameters=diams, model="Rosenfeld")
     rho = np.stack([np.full(shape, r) for r in rhos])
     parts, f = orc.total(rho, mus)
-    phi, mu_ex = _rosenfeld_mixture(rhos, diams)
+    phi, mu_ex = rosenfeld_mixture(rhos, diams)
     vol, dv = np.prod(shape) * dx ** 3, dx ** 3
     assert parts[2] == pytest.approx(phi * vol, rel=1e-12)
     assert parts[0] == pytest.approx(sum(r * (np.log(r) - 1) for r in rhos) * vol, rel=1e-13)
