@@ -206,3 +206,23 @@ def test_bulk_rejects_bad_densities():
         ctx.bulk_chemical_potentials([0.3])
     with pytest.raises(cd.InvalidArgument):
         ctx.bulk_chemical_potentials([0.3, 0.0])
+
+
+def test_eight_species_with_every_pair_matches_oracle():
+    # the capacity of a context: 8 species and all 36 pairs (self pairs included)
+    import classicaldft_amd as cd
+
+    shape, dx = (16, 16, 16), 0.25
+    diams = tuple(0.6 + 0.1 * s for s in range(8))
+    inter = [dict(LJ, si=i, sj=j, sigma=0.5 * (diams[i] + diams[j]), epsilon=0.3 + 0.05 * (i + j))
+             for i in range(8) for j in range(i, 8)]
+    assert len(inter) == 36
+    ctx, orc = _pair(shape, diams, inter, "WhiteBearI", kT=1.5, dx=dx)
+    rho = _densities(shape, dx, tuple(0.02 + 0.01 * s for s in range(8)), seed=41)
+    _compare(ctx, orc, rho, tuple(0.1 * s - 0.3 for s in range(8)), tag="8 species")
+    with pytest.raises(cd.CdftError) as e:
+        cd.Context(shape, dx, diameters=(1.0,) * 9)
+    assert e.value.code == -4
+    with pytest.raises(cd.CdftError) as e:
+        cd.Context(shape, dx, diameters=diams, interactions=inter + [dict(LJ, si=0, sj=1)])
+    assert e.value.code == -4
