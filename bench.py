@@ -222,6 +222,35 @@ def ddft_slab(cd, W, steps=2000, shape=(256, 128, 128), dt=1e-4, precision="mixe
     }
 
 
+def converged_profile(cd, W, rho0, shape, dx, kT, steps=500):
+    """The headline droplet relaxed by Minimizer::grand_potential (reference defaults: Unbounded alias,
+    homogeneous boundary) for a fixed number of FIRE steps in mixed precision and in FP64 on the GPU.
+    Reports how far the mixed profile and Omega end from the FP64 ones, and both step rates."""
+    import numpy as np
+    import torch
+
+    out = {"steps": steps}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=kT, model="WhiteBearII", precision=prec)
+        mu = ctx.bulk_mu(0.02)
+        fire = dict(force_tolerance=1e-12)  # run every step
+        ctx.minimize_grand_potential(rho0, mu, fire=fire, max_steps=2, record_energies=False)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r, info, _ = ctx.minimize_grand_potential(rho0, mu, fire=fire, max_steps=steps, record_energies=False)
+        t = time.perf_counter() - t0
+        res[prec] = (r, info)
+        out[f"{prec}_fire_steps_per_s"] = info["iterations"] / t
+        out[f"{prec}_iterations"] = info["iterations"]
+    (rm, im), (rd, idd) = res["mixed"], res["fp64"]
+    out["profile_max_rel_err"] = float(np.max(np.abs(rm - rd)) / np.max(np.abs(rd)))
+    out["omega_rel_err"] = abs(im["grand_potential"] - idd["grand_potential"]) / abs(idd["grand_potential"])
+    out["omega_fp64"] = idd["grand_potential"]
+    out["rms_force_fp64"] = idd["rms_force"]
+    return out
+
+
 def mixture(cd, W, shape=(128, 128, 128), evals=50):
     """Binary LJ mixture (diameters 1.0 and 0.8; self pairs LJ(1, 1) and LJ(0.8, 0.6), cross pair
     LJ(0.9, 0.8), WCA, r_c = 2.5; kT = 1.0, WB-II, dx = 0.1) on smooth noisy densities of mean 0.3 and 0.2:
@@ -257,6 +286,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential FIRE measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the DDFT adjoint measurement")
     ap.add_argument("--no-mixture", action="store_true", help="skip the binary-mixture evaluation measurement")
+    ap.add_argument("--profile-steps", type=int, default=500,
+                    help="FIRE steps of the mixed-vs-FP64 converged-profile comparison (0 skips it)")
     ap.add_argument("--ddft-steps", type=int, default=2000,
                     help="steps of the 256x128x128 DDFT slab run (0 skips it)")
     ap.add_argument("--adjoint-fd-all", action="store_true",
@@ -368,6 +399,9 @@ def main():
             res64["grand_potential"])
         out["force_rel_err_vs_fp64_gpu"] = float(np.max(np.abs(forces - f64)) / np.max(np.abs(f64)))
         del ctx64
+
+    if args.profile_steps > 0:
+        out["converged_profile"] = converged_profile(cd, W, rho, shape, dx, kT, steps=args.profile_steps)
 
     if not args.no_batch:
         out["batched"] = batched_fire(cd, W, args.precision)
