@@ -222,6 +222,28 @@ def ddft_slab(cd, W, steps=2000, shape=(256, 128, 128), dt=1e-4, precision="mixe
     }
 
 
+def stream_copy_gbps(reps=5):
+    """Device-to-device copy of 1 GiB of doubles (reads 1 GiB, writes 1 GiB), timed with HIP events on
+    torch's current stream; median of `reps` copies, in GB/s of bytes moved."""
+    import torch
+
+    n = 1 << 27
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    rates = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        rates.append(2.0 * 8 * n / (e0.elapsed_time(e1) * 1e-3) / 1e9)
+    del src, dst
+    torch.cuda.empty_cache()
+    return sorted(rates)[len(rates) // 2]
+
+
 def converged_profile(cd, W, rho0, shape, dx, kT, steps=500):
     """The headline droplet relaxed by Minimizer::grand_potential (reference defaults: Unbounded alias,
     homogeneous boundary) for a fixed number of FIRE steps in mixed precision and in FP64 on the GPU.
@@ -399,6 +421,8 @@ def main():
             res64["grand_potential"])
         out["force_rel_err_vs_fp64_gpu"] = float(np.max(np.abs(forces - f64)) / np.max(np.abs(f64)))
         del ctx64
+
+    out["stream_copy_GBps"] = stream_copy_gbps()
 
     if args.profile_steps > 0:
         out["converged_profile"] = converged_profile(cd, W, rho, shape, dx, kT, steps=args.profile_steps)

@@ -481,7 +481,27 @@ inline Functional make_functional(const fmt::FMTModel& fmt_model, physics::Model
 
 namespace algorithms {
 namespace fire {
-// fire.hpp:33-44 (defaults as there)
+// fire.hpp:15-25: the optimiser's state between steps.
+struct FireState {
+  std::vector<vec> x;
+  std::vector<vec> v;
+  double dt{0.0};
+  double alpha{0.0};
+  int n_positive{0};
+  int n_negative{0};
+  double energy{0.0};
+  double rms_force{0.0};
+  int iteration{0};
+  bool converged{false};
+};
+
+// fire.hpp:30: (energy, forces = -gradient) at positions x.
+using ForceFunction = std::function<std::pair<double, std::vector<vec>>(const std::vector<vec>&)>;
+
+// fire.hpp:33-178: FIRE2 over any force function (defaults as there).  The
+// update runs on the host over the caller's vectors; `compute` does the heavy
+// work (the functional's callbacks evaluate on the device).
+// Minimizer::grand_potential runs its whole FIRE loop on the device instead.
 struct Fire {
   double dt{1e-3};
   double dt_max{1e-2};
@@ -494,6 +514,100 @@ struct Fire {
   int max_uphill{20};
   double force_tolerance{0.1};
   int max_steps{10000};
+
+  FireState initialize(std::vector<vec> x0, const ForceFunction& compute) const {
+    auto ef = compute(x0);
+    FireState st;
+    st.v.reserve(x0.size());
+    for (const auto& xs : x0) st.v.emplace_back(xs.size(), 0.0);
+    st.x = std::move(x0);
+    st.dt = dt;
+    st.alpha = alpha_start;
+    st.energy = ef.first;
+    st.rms_force = rms(ef.second);
+    st.converged = st.rms_force < force_tolerance;
+    return st;
+  }
+
+  // One FIRE2 step from `forces` (the forces at state.x); returns the new
+  // state and the forces at its positions.
+  std::pair<FireState, std::vector<vec>> step(FireState st, const std::vector<vec>& forces,
+                                              const ForceFunction& compute) const {
+    if (forces.size() != st.x.size()) throw std::invalid_argument("fire::step: force and position counts differ");
+    double power = 0.0;  // P = v . f
+    for (size_t s = 0; s < st.x.size(); ++s) power += dot(st.v[s], forces[s]);
+    if (power > 0.0 || st.iteration == 0) {
+      ++st.n_positive;
+      st.n_negative = 0;
+      if (st.n_positive > n_delay) {
+        st.dt = std::min(st.dt * f_inc, dt_max);
+        st.alpha *= f_alpha;
+      }
+    } else {
+      st.n_positive = 0;
+      if (++st.n_negative > max_uphill) throw std::runtime_error("fire::step: exceeded max uphill steps");
+      // back up half a step with the current dt and stop, then shrink dt (FIRE2)
+      for (size_t s = 0; s < st.x.size(); ++s)
+        for (size_t i = 0; i < st.x[s].size(); ++i) {
+          st.x[s][i] -= 0.5 * st.dt * st.v[s][i];
+          st.v[s][i] = 0.0;
+        }
+      if (st.iteration > n_delay) {
+        st.dt = std::max(st.dt * f_dec, dt_min);
+        st.alpha = alpha_start;
+      }
+    }
+    // semi-implicit Euler: velocity, mixing, then position
+    double vv = 0.0, ff = 0.0;
+    for (size_t s = 0; s < st.x.size(); ++s) {
+      for (size_t i = 0; i < st.v[s].size(); ++i) st.v[s][i] += st.dt * forces[s][i];
+      vv += dot(st.v[s], st.v[s]);
+      ff += dot(forces[s], forces[s]);
+    }
+    const double vn = std::sqrt(vv), fn = std::sqrt(ff);
+    if (fn > 0.0) {
+      const double c = st.alpha * (vn / fn);
+      for (size_t s = 0; s < st.x.size(); ++s)
+        for (size_t i = 0; i < st.v[s].size(); ++i) st.v[s][i] = (1.0 - st.alpha) * st.v[s][i] + c * forces[s][i];
+    }
+    for (size_t s = 0; s < st.x.size(); ++s)
+      for (size_t i = 0; i < st.x[s].size(); ++i) st.x[s][i] += st.dt * st.v[s][i];
+    auto ef = compute(st.x);
+    st.energy = ef.first;
+    st.rms_force = rms(ef.second);
+    ++st.iteration;
+    st.converged = st.rms_force < force_tolerance;
+    return {std::move(st), std::move(ef.second)};
+  }
+
+  // Steps until converged or max_steps (fire.hpp:166-176; the forces at x0
+  // are computed twice, as there).
+  FireState minimize(std::vector<vec> x0, const ForceFunction& compute) const {
+    FireState st = initialize(std::move(x0), compute);
+    std::vector<vec> forces = compute(st.x).second;
+    for (int i = 0; i < max_steps && !st.converged; ++i) {
+      auto next = step(std::move(st), forces, compute);
+      st = std::move(next.first);
+      forces = std::move(next.second);
+    }
+    return st;
+  }
+
+ private:
+  static double dot(const vec& a, const vec& b) {
+    if (a.size() != b.size()) throw std::invalid_argument("fire: vector sizes differ");
+    double t = 0.0;
+    for (size_t i = 0; i < a.size(); ++i) t += a[i] * b[i];
+    return t;
+  }
+  static double rms(const std::vector<vec>& f) {
+    double t = 0.0, n = 0.0;
+    for (const auto& fs : f) {
+      t += dot(fs, fs);
+      n += static_cast<double>(fs.size());
+    }
+    return std::sqrt(t / n);
+  }
 };
 }  // namespace fire
 
@@ -796,6 +910,44 @@ inline StepResult split_operator_step(const std::vector<vec>& densities, const G
   StepResult r{densities, 0.0, config.dt};
   ::dft::detail::check(cdft_ddft_split_operator(cb.functional->weights.ctx.get(), &c, 1, nullptr,
                                                 cb.chemical_potential, r.densities[0].data(), &r.energy));
+  return r;
+}
+
+namespace _internal {
+// dynamics.hpp:70-77.  The device forms the propagator factors from the grid
+// itself, so the state keeps only the grid and the last dt.
+struct IntegratingFactorState {
+  std::vector<long> shape;
+  double dx{0.0};
+  long fourier_total{0};
+  double dt{0.0};
+};
+// dynamics.hpp:79-129
+inline IntegratingFactorState make_if_state(const Grid& grid) {
+  return IntegratingFactorState{{grid.shape[0], grid.shape[1], grid.shape[2]},
+                                grid.dx,
+                                grid.shape[0] * grid.shape[1] * (grid.shape[2] / 2 + 1),
+                                0.0};
+}
+}  // namespace _internal
+
+// integrating_factor_step (dynamics.hpp:333-400) on the device: one step of
+// the integrating-factor scheme with its fixed-point iteration; a restart
+// divides config.dt by 10, as there, and the step never lengthens dt.
+inline StepResult integrating_factor_step(const std::vector<vec>& densities, const Grid& grid,
+                                          _internal::IntegratingFactorState& st, const ForceCallback& compute,
+                                          StepConfig& config) {
+  const auto& cb = detail::device_callback(compute, grid, densities);
+  cdft_step_config c{config.dt, config.diffusion_coefficient, config.min_density, config.dt_max,
+                     config.fp_tolerance, config.fp_max_iterations};
+  StepResult r{densities, 0.0, config.dt};
+  cdft_sim_result cr{};
+  ::dft::detail::check(cdft_ddft_run(cb.functional->weights.ctx.get(), &c, 1, nullptr, cb.chemical_potential,
+                                     r.densities[0].data(), &cr, nullptr, nullptr));
+  config.dt = cr.dt;
+  st.dt = cr.dt;
+  r.energy = cr.energy;
+  r.dt_used = cr.dt;
   return r;
 }
 

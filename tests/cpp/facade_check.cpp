@@ -49,8 +49,104 @@ dft::physics::Model lj_model(double dx, double box, int n_species = 1) {
   return m;
 }
 
+// fire.cpp:12-19 of the reference tests: E = sum x^2 / 2, f = -x
+std::pair<double, std::vector<dft::vec>> quadratic_force(const std::vector<dft::vec>& x) {
+  double e = 0.0;
+  std::vector<dft::vec> f(x.size());
+  for (size_t s = 0; s < x.size(); ++s) {
+    f[s].resize(x[s].size());
+    for (size_t i = 0; i < x[s].size(); ++i) {
+      e += 0.5 * x[s][i] * x[s][i];
+      f[s][i] = -x[s][i];
+    }
+  }
+  return {e, f};
+}
+
+double norm(const dft::vec& v) {
+  double t = 0.0;
+  for (double a : v) t += a * a;
+  return std::sqrt(t);
+}
+
+void check(bool ok, const char* what) {
+  if (!ok) {
+    std::printf("FAIL %s\n", what);
+    ++failures;
+  }
+}
+
+// The known-answer tests of the reference's fire.cpp:21-140 on the facade's host FIRE2.
+void fire_checks() {
+  using dft::algorithms::fire::Fire;
+  {
+    Fire cfg;
+    const auto st = cfg.initialize({{1.0, 2.0, 3.0}}, quadratic_force);
+    check(std::abs(st.energy - 7.0) < 1e-15 && st.rms_force > 0.0 && st.iteration == 0 && !st.converged &&
+              st.v.size() == 1 && norm(st.v[0]) == 0.0,
+          "fire initialize");
+  }
+  {
+    Fire cfg;
+    cfg.dt = 0.01;
+    cfg.force_tolerance = 1e-6;
+    const auto st = cfg.initialize({{1.0, 2.0, 3.0}}, quadratic_force);
+    const auto next = cfg.step(st, quadratic_force(st.x).second, quadratic_force);
+    check(next.first.energy < st.energy && next.first.iteration == 1, "fire step reduces energy");
+  }
+  for (const auto& x0 : std::vector<std::vector<dft::vec>>{{{5.0}}, {{3.0, -2.0, 1.0}}, {{2.0, 1.0}, {-1.0, 3.0}}}) {
+    Fire cfg;
+    cfg.dt = 0.01;
+    cfg.force_tolerance = 1e-6;
+    cfg.max_steps = 5000;
+    const auto r = cfg.minimize(x0, quadratic_force);
+    bool small = true;
+    for (const auto& xs : r.x) small = small && norm(xs) < 1e-3;
+    check(r.converged && small && r.energy < 1e-6, "fire minimize quadratic");
+  }
+  {
+    auto rosenbrock = [](const std::vector<dft::vec>& x) -> std::pair<double, std::vector<dft::vec>> {
+      const double a = x[0][0], b = x[0][1];
+      const double e = (1.0 - a) * (1.0 - a) + 100.0 * (b - a * a) * (b - a * a);
+      const double gx = -2.0 * (1.0 - a) - 400.0 * a * (b - a * a), gy = 200.0 * (b - a * a);
+      return {e, {{-gx, -gy}}};
+    };
+    Fire cfg;
+    cfg.dt = 1e-4;
+    cfg.dt_max = 1e-3;
+    cfg.force_tolerance = 0.1;
+    cfg.max_steps = 50000;
+    const auto r = cfg.minimize({{-1.0, 1.0}}, rosenbrock);
+    check(r.energy < 0.1 * rosenbrock({{-1.0, 1.0}}).first, "fire Rosenbrock progress");
+  }
+  {
+    Fire cfg;
+    cfg.force_tolerance = 1e-6;
+    check(cfg.initialize({{0.0, 0.0, 0.0}}, quadratic_force).converged, "fire at minimum");
+  }
+  {
+    int count = 0;
+    auto alternating = [&count](const std::vector<dft::vec>& x) -> std::pair<double, std::vector<dft::vec>> {
+      ++count;
+      const double sg = (count % 2 == 0) ? -1.0 : 1.0;
+      const double e = sg * (x[0][0] * x[0][0] + x[0][1] * x[0][1]);
+      return {e, {{sg * -2.0 * x[0][0], sg * -2.0 * x[0][1]}}};
+    };
+    Fire cfg;
+    cfg.dt = 0.5;
+    cfg.dt_max = 0.5;
+    cfg.n_delay = 0;
+    cfg.max_uphill = 3;
+    cfg.force_tolerance = 1e-10;
+    cfg.max_steps = 5000;
+    expect_throw<std::runtime_error>("fire uphill", [&] { (void)cfg.minimize({{1.0, 1.0}}, alternating); },
+                                     "exceeded max uphill steps");
+  }
+}
+
 int run_cpu() {
   using namespace dft;
+  fire_checks();
   const Grid g = make_grid(0.1, {3.2, 1.6, 0.8});
   if (g.shape[0] != 32 || g.shape[1] != 16 || g.shape[2] != 8 || g.total_points() != 4096 ||
       g.flat_index(1, 2, 3) != 3 + 8 * (2 + 16 * 1)) {
@@ -176,6 +272,20 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   const vec mix_mu = eos.chemical_potentials({0.3, 0.2});
   const double mix_p = eos.pressure({0.3, 0.2});
 
+  // three single integrating-factor steps at fixed dt
+  auto if_state = algorithms::dynamics::_internal::make_if_state(model.grid);
+  algorithms::dynamics::StepConfig if_cfg;
+  if_cfg.dt = 1e-3;
+  if_cfg.dt_max = 1e-3;
+  std::vector<vec> if_rho{rho};
+  double if_energy = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    const auto sr = algorithms::dynamics::integrating_factor_step(if_rho, model.grid, if_state, cb, if_cfg);
+    if_rho = sr.densities;
+    if_energy = sr.energy;
+  }
+  write_doubles(out + "/ifstep_rho.bin", if_rho[0]);
+
   algorithms::dynamics::Simulation sim;
   sim.step.dt = 1e-3;
   sim.step.dt_max = 1e-3;
@@ -201,11 +311,13 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
               "\"saddle_eigenvalue\": %.17g, \"saddle_outer\": %d, \"split_energy\": %.17g, "
               "\"cn_energy\": %.17g, \"picard_iterations\": %d, \"mixture_grand_potential\": %.17g, "
               "\"mixture_bulk_mu\": [%.17g, %.17g], \"mixture_bulk_pressure\": %.17g, "
+              "\"ifstep_energy\": %.17g, \"ifstep_dt\": %.17g, "
               "\"ddft_times\": [",
               mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential,
               0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations,
               sres_saddle.grand_potential, sres_saddle.eigenvalue, sres_saddle.outer_iterations, split.energy,
-              cn.energy, pres.iterations, mres_mix.grand_potential, mix_mu[0], mix_mu[1], mix_p);
+              cn.energy, pres.iterations, mres_mix.grand_potential, mix_mu[0], mix_mu[1], mix_p, if_energy,
+              if_cfg.dt);
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);

@@ -59,6 +59,12 @@ def test_cpp_facade_matches_oracle(tmp_path):
     assert np.array_equal(out["mixture_bulk_mu"], cm.bulk_chemical_potentials([0.3, 0.2]))
     assert out["mixture_bulk_pressure"] == cm.bulk_pressure([0.3, 0.2])
 
+    # integrating_factor_step three times at fixed dt = the oracle's 3-step run with dt_max = dt
+    rif, resif, enif = orc.ddft_run(rho, mu, 3, dt=1e-3, dt_max=1e-3)
+    assert _rel(np.fromfile(tmp_path / "ifstep_rho.bin").reshape(shape), rif) < 1e-10
+    assert out["ifstep_energy"] == pytest.approx(enif[-1], rel=1e-10)
+    assert out["ifstep_dt"] == pytest.approx(resif["dt"], rel=1e-15)
+
     # Minimizer defaults of the reference: Unbounded, homogeneous boundary on
     ro, reso, _ = orc.minimize_grand_potential(rho, mu, fire=dict(force_tolerance=1e-9), max_steps=40)
     assert out["fire_iterations"] == reso["iterations"]
