@@ -26,7 +26,7 @@ EXPORTED = [
     "cdft_ctx_create", "cdft_ctx_destroy", "cdft_last_error", "cdft_grid_shape", "cdft_a_vdw",
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
     "cdft_species_count", "cdft_total_species", "cdft_total_species_device", "cdft_time_total_species_device",
-    "cdft_bulk_free_energy_density", "cdft_bulk_chemical_potentials", "cdft_bulk_pressure",
+    "cdft_bulk_free_energy_density", "cdft_bulk_chemical_potentials", "cdft_bulk_pressure", "cdft_contributions",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
@@ -156,6 +156,7 @@ def lib():
         L.cdft_bulk_free_energy_density.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
         L.cdft_bulk_chemical_potentials.argtypes = [C.c_void_p, _dp, _dp]
         L.cdft_bulk_pressure.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
+        L.cdft_contributions.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(_Result), _dp, _dp, _dp, _dp]
         L.cdft_total_species.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(_Result), _dp]
         L.cdft_total_species_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.POINTER(_Result),
                                                 C.c_void_p]
@@ -281,6 +282,22 @@ class Context:
         v = C.c_double()
         _check(lib().cdft_bulk_chemical_potential(self._h, float(rho), C.byref(v)))
         return v.value
+
+    def contributions(self, rho, mu, vext=None):
+        """The terms of functionals::total separately: (result dict, {"ideal", "external", "hard_sphere",
+        "mean_field"} force arrays of shape (n_species, *shape))."""
+        S = self.n_species
+        rho = _f64(rho)
+        if rho.size != S * self.N:
+            raise ValueError(f"rho has {rho.size} values, expected {S} x {self.N}")
+        v = None if vext is None else _f64(vext)
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (S,)))
+        f = {k: np.zeros(S * self.N) for k in ("ideal", "external", "hard_sphere", "mean_field")}
+        r = _Result()
+        _check(lib().cdft_contributions(self._h, _p(rho), _p(v), _p(m), C.byref(r), _p(f["ideal"]),
+                                        _p(f["external"]), _p(f["hard_sphere"]), _p(f["mean_field"])))
+        res = {k: getattr(r, k) for k, _ in _Result._fields_}
+        return res, {k: a.reshape((S,) + self.shape) for k, a in f.items()}
 
     def _bulk_rho(self, rho):
         r = np.ascontiguousarray(np.atleast_1d(np.asarray(rho, dtype=np.float64)))

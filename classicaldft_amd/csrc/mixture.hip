@@ -234,6 +234,29 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_bwd(const cx<double>* __res
   }
 }
 
+// Per-term split of the forces of one species (functionals.hpp:64-95): the
+// ideal (ideal_gas.hpp:29-33), external (external_field.hpp:24-28) and
+// mean-field (conv dV, mean_field.hpp:259-277) forces are pointwise; the
+// hard-sphere force is the total force less those three.
+__global__ __launch_bounds__(kMixThreads) void k_split_terms(const double* __restrict__ rho,
+                                                             const double* __restrict__ vext,
+                                                             const double* __restrict__ conv,
+                                                             const double* __restrict__ total, double mu, double dv,
+                                                             long n, int hs, double* __restrict__ f_id,
+                                                             double* __restrict__ f_ext, double* __restrict__ f_mf,
+                                                             double* __restrict__ f_hs) {
+  for (long p = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; p < n;
+       p += static_cast<long>(gridDim.x) * kMixThreads) {
+    const double id = (log(fmax(rho[p], 1e-300)) - mu) * dv;
+    const double ext = vext ? vext[p] * dv : 0.0;
+    const double mf = conv ? conv[p] * dv : 0.0;
+    if (f_id) f_id[p] = id;
+    if (f_ext) f_ext[p] = ext;
+    if (f_mf) f_mf[p] = mf;
+    if (f_hs) f_hs[p] = hs ? total[p] - id - ext - mf : 0.0;
+  }
+}
+
 unsigned spectral_blocks(long M) { return static_cast<unsigned>(std::min<long>(2048, (M + kMixThreads - 1) / kMixThreads)); }
 
 MixParams mix_params(const Context* c) {
@@ -489,6 +512,71 @@ int cdft_total_species(cdft_ctx* ctx, const double* rho, const double* vext, con
       CDFT_HIP(hipMemcpyAsync(forces, d_force, bytes, hipMemcpyDeviceToHost, c->stream));
       CDFT_HIP(hipStreamSynchronize(c->stream));
     }
+  });
+}
+
+int cdft_contributions(cdft_ctx* ctx, const double* rho, const double* vext, const double* mu, cdft_result* result,
+                       double* f_ideal, double* f_external, double* f_hard_sphere, double* f_mean_field) {
+  return guarded([&] {
+    check_ctx_any(ctx);
+    Context* c = ctx->impl;
+    if (!rho || !result) throw std::invalid_argument("cdft_contributions: rho and result are required");
+    const int S = c->n_species;
+    check_mu(mu, S);
+    CDFT_HIP(hipSetDevice(c->device));
+    const long N = c->g.N;
+    const size_t field = sizeof(double) * N;
+    double* outs[4] = {f_ideal, f_external, f_hard_sphere, f_mean_field};
+    // device block: [4 S fields of output] [S*N rho] [S*N vext] [S*N total] [nf*N weighted densities]
+    const bool one = S == 1;
+    const long nwd = (one && c->cs.mf && (f_mean_field || f_hard_sphere)) ? c->cs.nf : 0;
+    double* blk = nullptr;
+    CDFT_HIP(hipMalloc(&blk, field * (4 * S + 3 * S + nwd)));
+    try {
+      double* d_out = blk;
+      double* d_rho = blk + 4L * S * N;
+      double* d_vext = vext ? d_rho + static_cast<long>(S) * N : nullptr;
+      double* d_tot = d_rho + 2L * S * N;
+      double* d_wd = d_rho + 3L * S * N;
+      CDFT_HIP(hipMemcpyAsync(d_rho, rho, field * S, hipMemcpyHostToDevice, c->stream));
+      if (vext) CDFT_HIP(hipMemcpyAsync(d_vext, vext, field * S, hipMemcpyHostToDevice, c->stream));
+      const double* conv[kMaxSpecies] = {};
+      if (one) {  // (no cached graph: this call's buffers are its own)
+        c->eval_direct(d_rho, d_vext, mu[0], d_tot, AliasMap{0, 0.0, 0.0});
+        c->fetch_result(mu[0], result);
+        if (nwd > 0) {
+          c->weighted_densities(d_rho, d_wd);
+          conv[0] = d_wd + (nwd - 1) * N;  // the mean-field convolution is the last channel
+        }
+      } else {
+        mixture_eval(c, d_rho, d_vext, mu, d_tot);
+        mixture_result(c, result);
+        if (!c->inters.empty()) {  // the per-species potentials U_s stay in the channel fields
+          const MixLayout L = mix_layout(c);
+          const int nfm = c->cs.hs ? kMixFmt : 0;
+          for (int s = 0; s < S; ++s) conv[s] = at<double>(c, L.X) + static_cast<long>(nfm + s) * N;
+        }
+      }
+      const unsigned nb = static_cast<unsigned>(std::min<long>(2048, (N + kMixThreads - 1) / kMixThreads));
+      for (int s = 0; s < S; ++s) {
+        const long o = static_cast<long>(s) * N;
+        double* dst[4];
+        for (int t = 0; t < 4; ++t) dst[t] = outs[t] ? d_out + (static_cast<long>(t) * S * N + o) : nullptr;
+        k_split_terms<<<nb, kMixThreads, 0, c->stream>>>(d_rho + o, d_vext ? d_vext + o : nullptr, conv[s],
+                                                         d_tot + o, mu[s], c->g.dv, N, c->cs.hs, dst[0], dst[1],
+                                                         dst[3], dst[2]);
+      }
+      CDFT_HIP(hipGetLastError());
+      for (int t = 0; t < 4; ++t)
+        if (outs[t])
+          CDFT_HIP(hipMemcpyAsync(outs[t], d_out + static_cast<long>(t) * S * N, field * S, hipMemcpyDeviceToHost,
+                                  c->stream));
+      CDFT_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      (void)hipFree(blk);
+      throw;
+    }
+    CDFT_HIP(hipFree(blk));
   });
 }
 

@@ -149,6 +149,15 @@ CDFT_API int cdft_total_species(cdft_ctx* ctx, const double* rho, const double* 
 /* Same with device-resident arrays; returns when the result is complete. */
 CDFT_API int cdft_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext, const double* mu,
                                        cdft_result* result, double* d_forces);
+/* The terms of functionals::total separately, for any species count:
+ * ideal_gas (ideal_gas.hpp:18-38), external_field (external_field.hpp:16-31),
+ * hard_sphere (hard_sphere.hpp:263-298) and mean_field (mean_field.hpp:233-281)
+ * forces, each nullable, n_species * N doubles species-major; result holds
+ * the energy of every term as cdft_total_species gives it.  The hard-sphere
+ * forces are the total less the three pointwise terms. */
+CDFT_API int cdft_contributions(cdft_ctx* ctx, const double* rho, const double* vext, const double* mu,
+                                cdft_result* result, double* f_ideal, double* f_external, double* f_hard_sphere,
+                                double* f_mean_field);
 /* bulk::BulkThermodynamics (bulk/thermodynamics.hpp:200-255) for the context's
  * species, FMT model and grid-synchronised a_vdw: rho and mu hold n_species
  * positive bulk densities and the chemical potentials (kT units).  As in the

@@ -213,12 +213,24 @@ struct Result {
 // pointwise terms, reductions and state.
 enum class Precision { FP64 = CDFT_PRECISION_FP64, Mixed = CDFT_PRECISION_MIXED };
 
+// FMTWeights / MeanFieldWeights (weights.hpp, mean_field.hpp:23-35): handles
+// on the same device context, for hard_sphere() and mean_field().
+struct FMTWeights {
+  std::shared_ptr<cdft_ctx> ctx;
+  int fmt_model_id{CDFT_FMT_WHITEBEAR_II};  // the model the context evaluates
+};
+struct MeanFieldWeights {
+  std::shared_ptr<cdft_ctx> ctx;
+};
+
 // functionals.hpp:19-33: the weights of a model.  Here they live on the GPU,
 // in a cdft context, and Weights shares ownership of it.
 struct Weights {
   fmt::FMTModel fmt_model = fmt::WhiteBearII{};
   std::shared_ptr<cdft_ctx> ctx;
   long n_points{0};
+  FMTWeights fmt;
+  MeanFieldWeights mean_field;
 };
 
 namespace detail {
@@ -276,7 +288,8 @@ inline Weights make_weights(const fmt::FMTModel& fmt_model, const physics::Model
   cdft_ctx* raw = nullptr;
   ::dft::detail::check(cdft_ctx_create(&g, diam.data(), static_cast<int>(diam.size()), inter.data(),
                                        static_cast<int>(inter.size()), model.temperature, &o, &raw));
-  return Weights{fmt_model, std::shared_ptr<cdft_ctx>(raw, cdft_ctx_destroy), model.grid.total_points()};
+  std::shared_ptr<cdft_ctx> ctx(raw, cdft_ctx_destroy);
+  return Weights{fmt_model, ctx, model.grid.total_points(), FMTWeights{ctx, fmt_model.id()}, MeanFieldWeights{ctx}};
 }
 
 namespace detail {
@@ -336,6 +349,59 @@ inline Result total(const physics::Model& model, const State& state, const Weigh
   return r;
 }
 
+namespace detail {
+// One term of functionals::total through cdft_contributions: term 2 hard
+// spheres, 3 mean field; the state's external fields and chemical potentials
+// are read as total() reads them.
+inline Contribution device_term(const std::shared_ptr<cdft_ctx>& ctx, const Grid& grid, const State& state,
+                                int term) {
+  if (!ctx) throw std::invalid_argument("the weights hold no device context");
+  const size_t S = state.species.size(), n = static_cast<size_t>(grid.total_points());
+  vec rho(S * n), v(S * n, 0.0), f(S * n), mu(S);
+  bool has_v = false;
+  for (size_t k = 0; k < S; ++k) {
+    const Density& d = state.species[k].density;
+    if (d.values.size() != n) throw std::invalid_argument("density size does not match the grid");
+    std::copy(d.values.begin(), d.values.end(), rho.begin() + static_cast<long>(k * n));
+    if (!d.external_field.empty() && !all_zero(d.external_field)) {
+      if (d.external_field.size() != n) throw std::invalid_argument("external field size does not match the grid");
+      std::copy(d.external_field.begin(), d.external_field.end(), v.begin() + static_cast<long>(k * n));
+      has_v = true;
+    }
+    mu[k] = state.species[k].chemical_potential;
+  }
+  int n_species = 0;
+  ::dft::detail::check(cdft_species_count(ctx.get(), &n_species));
+  if (static_cast<size_t>(n_species) != S)
+    throw std::invalid_argument("the state has " + std::to_string(S) + " species, the weights " +
+                                std::to_string(n_species));
+  cdft_result cr{};
+  ::dft::detail::check(cdft_contributions(ctx.get(), rho.data(), has_v ? v.data() : nullptr, mu.data(), &cr, nullptr,
+                                          nullptr, term == 2 ? f.data() : nullptr, term == 3 ? f.data() : nullptr));
+  Contribution c;
+  c.free_energy = term == 2 ? cr.hard_sphere : cr.mean_field;
+  for (size_t k = 0; k < S; ++k)
+    c.forces.emplace_back(f.begin() + static_cast<long>(k * n), f.begin() + static_cast<long>((k + 1) * n));
+  return c;
+}
+}  // namespace detail
+
+// hard_sphere.hpp:263-298: F_hs and delta F_hs / delta rho dV of every species
+// (the model and species are those the weights were built with).
+inline Contribution hard_sphere(const fmt::FMTModel& model, const Grid& grid, const State& state,
+                                const std::vector<Species>& species, const FMTWeights& weights) {
+  if (model.id() != weights.fmt_model_id)
+    throw std::invalid_argument("hard_sphere: the FMT model differs from the one the weights were built for");
+  if (species.size() != state.species.size()) throw std::invalid_argument("species and state sizes differ");
+  return detail::device_term(weights.ctx, grid, state, 2);
+}
+
+// mean_field.hpp:233-281: F_mf and its forces for every species.
+inline Contribution mean_field(const Grid& grid, const State& state, const std::vector<Species>& species,
+                               const MeanFieldWeights& weights) {
+  if (species.size() != state.species.size()) throw std::invalid_argument("species and state sizes differ");
+  return detail::device_term(weights.ctx, grid, state, 3);
+}
 }  // namespace functionals
 
 // init.hpp:11-60 (namespace dft::init)

@@ -268,6 +268,18 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   vec mix_forces(mres_mix.forces[0]);
   mix_forces.insert(mix_forces.end(), mres_mix.forces[1].begin(), mres_mix.forces[1].end());
   write_doubles(out + "/mix_forces.bin", mix_forces);
+  // the hard-sphere and mean-field terms of the mixture through the reference's signatures
+  const auto hs_term = functionals::hard_sphere(functionals::fmt::WhiteBearII{}, mix.grid, mix_state, mix.species,
+                                                mix_func.weights.fmt);
+  const auto mf_term = functionals::mean_field(mix.grid, mix_state, mix.species, mix_func.weights.mean_field);
+  vec terms(hs_term.forces[0]);
+  for (const auto* f : {&hs_term.forces[1], &mf_term.forces[0], &mf_term.forces[1]})
+    terms.insert(terms.end(), f->begin(), f->end());
+  write_doubles(out + "/mix_terms.bin", terms);
+  expect_throw<std::invalid_argument>("hard_sphere model", [&] {
+    (void)functionals::hard_sphere(functionals::fmt::Rosenfeld{}, mix.grid, mix_state, mix.species,
+                                   mix_func.weights.fmt);
+  }, "FMT model differs");
   const auto eos = mix_func.bulk();
   const vec mix_mu = eos.chemical_potentials({0.3, 0.2});
   const double mix_p = eos.pressure({0.3, 0.2});
@@ -311,13 +323,14 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
               "\"saddle_eigenvalue\": %.17g, \"saddle_outer\": %d, \"split_energy\": %.17g, "
               "\"cn_energy\": %.17g, \"picard_iterations\": %d, \"mixture_grand_potential\": %.17g, "
               "\"mixture_bulk_mu\": [%.17g, %.17g], \"mixture_bulk_pressure\": %.17g, "
-              "\"ifstep_energy\": %.17g, \"ifstep_dt\": %.17g, "
+              "\"ifstep_energy\": %.17g, \"ifstep_dt\": %.17g, \"mixture_hs_energy\": %.17g, "
+              "\"mixture_mf_energy\": %.17g, "
               "\"ddft_times\": [",
               mu, res.free_energy, res.grand_potential, omega_cb, dmax, mres.iterations, mres.grand_potential,
               0.9 * mass0, fres.iterations, fres.free_energy, eres.eigenvalue, eres.iterations,
               sres_saddle.grand_potential, sres_saddle.eigenvalue, sres_saddle.outer_iterations, split.energy,
               cn.energy, pres.iterations, mres_mix.grand_potential, mix_mu[0], mix_mu[1], mix_p, if_energy,
-              if_cfg.dt);
+              if_cfg.dt, hs_term.free_energy, mf_term.free_energy);
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);

@@ -51,6 +51,20 @@ def test_cpp_facade_matches_oracle(tmp_path):
     pm, fm = mix.total(np.stack([rho, 0.5 * rho + 0.02]), (mu, -0.3))
     assert out["mixture_grand_potential"] == pytest.approx(pm[5], rel=1e-11)
     assert _rel(np.fromfile(tmp_path / "mix_forces.bin").reshape((2,) + shape), fm) < 1e-11
+    # hard_sphere() and mean_field() of the mixture: the oracle's terms (an FMT-free oracle gives
+    # ideal + mean field)
+    mix_none = O.Oracle(shape, dx, diameters=(1.0, 0.8), kT=0.8, model="none",
+                        interactions=[W.LJ_WCA, dict(W.LJ_WCA, si=0, sj=1, sigma=0.9, epsilon=0.7)])
+    rho2 = np.stack([rho, 0.5 * rho + 0.02])
+    pn, fn = mix_none.total(rho2, (mu, -0.3))
+    tm = np.fromfile(tmp_path / "mix_terms.bin").reshape((4,) + shape)
+    ideal = [(np.log(rho2[s]) - m) * dx ** 3 for s, m in enumerate((mu, -0.3))]
+    for s in range(2):
+        assert _rel(tm[s], fm[s] - fn[s]) < 1e-11
+        assert _rel(tm[2 + s], fn[s] - ideal[s]) < 1e-11
+    assert out["mixture_hs_energy"] == pytest.approx(pm[2], rel=1e-12)
+    assert out["mixture_mf_energy"] == pytest.approx(pm[3], rel=1e-12)
+
     # Functional::bulk() of the mixture: the same equation of state as the ctypes binding
     import classicaldft_amd as cd
 
