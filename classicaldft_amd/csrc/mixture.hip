@@ -257,7 +257,9 @@ __global__ __launch_bounds__(kMixThreads) void k_split_terms(const double* __res
   }
 }
 
-unsigned spectral_blocks(long M) { return static_cast<unsigned>(std::min<long>(2048, (M + kMixThreads - 1) / kMixThreads)); }
+unsigned spectral_blocks(long M) {
+  return static_cast<unsigned>(std::min<long>(2048, (M + kMixThreads - 1) / kMixThreads));
+}
 
 MixParams mix_params(const Context* c) {
   const double PI = 3.14159265358979323846;

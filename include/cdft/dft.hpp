@@ -415,18 +415,16 @@ inline State homogeneous(const physics::Model& model, double density) {
   return st;
 }
 inline State from_profile(const physics::Model& model, const vec& rho) {
-  const long n = model.grid.total_points();
+  const auto n = static_cast<size_t>(model.grid.total_points());
   State st{{}, model.temperature};
-  st.species.push_back(
-      SpeciesState{Density{rho, vec(static_cast<size_t>(n), 0.0)}, vec(static_cast<size_t>(n), 0.0), 0.0, std::nullopt});
+  st.species.push_back(SpeciesState{Density{rho, vec(n, 0.0)}, vec(n, 0.0), 0.0, std::nullopt});
   return st;
 }
 inline State from_profiles(const physics::Model& model, const std::vector<vec>& profiles) {
-  const long n = model.grid.total_points();
+  const auto n = static_cast<size_t>(model.grid.total_points());
   State st{{}, model.temperature};
   for (const auto& rho : profiles)
-    st.species.push_back(
-      SpeciesState{Density{rho, vec(static_cast<size_t>(n), 0.0)}, vec(static_cast<size_t>(n), 0.0), 0.0, std::nullopt});
+    st.species.push_back(SpeciesState{Density{rho, vec(n, 0.0)}, vec(n, 0.0), 0.0, std::nullopt});
   return st;
 }
 }  // namespace init
