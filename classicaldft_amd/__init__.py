@@ -27,6 +27,7 @@ EXPORTED = [
     "cdft_bulk_chemical_potential", "cdft_total", "cdft_total_device", "cdft_total_device_repeat",
     "cdft_species_count", "cdft_total_species", "cdft_total_species_device", "cdft_time_total_species_device",
     "cdft_bulk_free_energy_density", "cdft_bulk_chemical_potentials", "cdft_bulk_pressure", "cdft_contributions",
+    "cdft_bulk_batch",
     "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
@@ -156,6 +157,7 @@ def lib():
         L.cdft_bulk_free_energy_density.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
         L.cdft_bulk_chemical_potentials.argtypes = [C.c_void_p, _dp, _dp]
         L.cdft_bulk_pressure.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_double)]
+        L.cdft_bulk_batch.argtypes = [C.c_void_p, C.c_long, _dp, _dp, _dp, _dp]
         L.cdft_contributions.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(_Result), _dp, _dp, _dp, _dp]
         L.cdft_total_species.argtypes = [C.c_void_p, _dp, _dp, _dp, C.POINTER(_Result), _dp]
         L.cdft_total_species_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, _dp, C.POINTER(_Result),
@@ -317,6 +319,15 @@ class Context:
         mu = np.zeros(self.n_species)
         _check(lib().cdft_bulk_chemical_potentials(self._h, _p(r), _p(mu)))
         return mu
+
+    def bulk_batch(self, rho):
+        """The bulk equation of state of many states on the device: rho has shape (n_states, n_species);
+        returns (f, mu, p) with shapes (n,), (n, n_species), (n,)."""
+        r = np.ascontiguousarray(np.asarray(rho, dtype=np.float64).reshape(-1, self.n_species))
+        n = r.shape[0]
+        f, mu, p = np.zeros(n), np.zeros((n, self.n_species)), np.zeros(n)
+        _check(lib().cdft_bulk_batch(self._h, n, _p(r), _p(f), _p(mu), _p(p)))
+        return f, mu, p
 
     def bulk_pressure(self, rho):
         """BulkThermodynamics::pressure = sum rho mu - f."""

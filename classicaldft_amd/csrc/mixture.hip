@@ -361,14 +361,37 @@ void mixture_result(Context* c, cdft_result* out) {
   out->grand_potential = out->free_energy - res[4] * dv;
 }
 
-// Bulk (uniform) thermodynamics, bulk/thermodynamics.hpp:17-255: the summed
-// uniform measures of fmt::make_uniform_measures (measures.hpp:49-65) in the
-// mixture channel order.
-void uniform_measures(const Context* c, const double* rho, double (&m)[kMixFmt]) {
+// Bulk (uniform) thermodynamics, bulk/thermodynamics.hpp:17-255, for host
+// calls and for the batched device kernel alike: the model in a plain struct.
+struct BulkParams {
+  int S, ninter;
+  FmtParams fp;
+  double diam[kMaxSpecies];
+  int si[kMaxMixInteractions], sj[kMaxMixInteractions];
+  double a_vdw[kMaxMixInteractions];
+};
+
+BulkParams bulk_params(const Context* c) {
+  BulkParams b{};
+  b.S = c->n_species;
+  b.ninter = static_cast<int>(c->inters.size());
+  b.fp = c->fp;
+  for (int s = 0; s < b.S; ++s) b.diam[s] = c->diam[s];
+  for (int q = 0; q < b.ninter; ++q) {
+    b.si[q] = c->inters[q].si;
+    b.sj[q] = c->inters[q].sj;
+    b.a_vdw[q] = c->a_vdw[q];
+  }
+  return b;
+}
+
+// The summed uniform measures of fmt::make_uniform_measures (measures.hpp:49-65)
+// in the mixture channel order.
+__host__ __device__ void uniform_measures(const BulkParams& b, const double* rho, double (&m)[kMixFmt]) {
   const double PI = 3.14159265358979323846;
-  for (double& x : m) x = 0.0;
-  for (int s = 0; s < c->n_species; ++s) {
-    const double d = c->diam[s], r = rho[s];
+  for (int c = 0; c < kMixFmt; ++c) m[c] = 0.0;
+  for (int s = 0; s < b.S; ++s) {
+    const double d = b.diam[s], r = rho[s];
     const double n2 = PI * r * d * d;
     m[0] += PI / 6.0 * r * d * d * d;
     m[1] += n2;
@@ -389,47 +412,66 @@ void check_bulk_rho(const double* rho, int S) {
 // BulkThermodynamics::free_energy_density (thermodynamics.hpp:206-220).  As
 // in the reference, a cross pair counts a rho_i rho_j here while the
 // functional and the chemical potential give it half of that.
-double bulk_free_energy_density(const Context* c, const double* rho) {
+__host__ __device__ double bulk_free_energy_density(const BulkParams& b, const double* rho) {
   double f = 0.0;
-  for (int s = 0; s < c->n_species; ++s) f += rho[s] * (std::log(rho[s]) - 1.0);  // ideal, :19-27
-  if (c->fp.model != FMT_NONE) {  // hard spheres, :38-52
+  for (int s = 0; s < b.S; ++s) f += rho[s] * (log(rho[s]) - 1.0);  // ideal, :19-27
+  if (b.fp.model != FMT_NONE) {  // hard spheres, :38-52
     double m[kMixFmt], g[kMixFmt];
-    uniform_measures(c, rho, m);
-    f += fmt_phi_grad_mix_t<double>(c->fp, m, g);
+    uniform_measures(b, rho, m);
+    f += fmt_phi_grad_mix_t<double>(b.fp, m, g);
   }
-  for (size_t q = 0; q < c->inters.size(); ++q) {
-    const int i = c->inters[q].si, j = c->inters[q].sj;
-    f += (i == j ? 0.5 : 1.0) * c->a_vdw[q] * rho[i] * rho[j];
-  }
+  for (int q = 0; q < b.ninter; ++q) f += (b.si[q] == b.sj[q] ? 0.5 : 1.0) * b.a_vdw[q] * rho[b.si[q]] * rho[b.sj[q]];
   return f;
 }
 
-// BulkThermodynamics::chemical_potential (thermodynamics.hpp:222-237).
-void bulk_chemical_potentials(const Context* c, const double* rho, double* mu) {
+// BulkThermodynamics::chemical_potential (thermodynamics.hpp:222-237) of every species.
+__host__ __device__ void bulk_chemical_potentials(const BulkParams& b, const double* rho, double* mu) {
   const double PI = 3.14159265358979323846;
   double m[kMixFmt], g[kMixFmt];
-  const bool hs = c->fp.model != FMT_NONE;
+  const bool hs = b.fp.model != FMT_NONE;
   if (hs) {
-    uniform_measures(c, rho, m);
-    fmt_phi_grad_mix_t<double>(c->fp, m, g);
+    uniform_measures(b, rho, m);
+    fmt_phi_grad_mix_t<double>(b.fp, m, g);
   }
-  for (int s = 0; s < c->n_species; ++s) {
-    double v = std::log(rho[s]);
+  for (int s = 0; s < b.S; ++s) {
+    double v = log(rho[s]);
     if (hs) {  // excess_chemical_potential, :54-89 (the tensor partials are zero for scalar models)
-      const double d = c->diam[s];
+      const double d = b.diam[s];
       v += g[0] * PI / 6.0 * d * d * d + g[1] * PI * d * d + g[2] * 0.5 * d + g[3];
       v += (g[10] + g[13] + g[15]) * PI * d * d / 3.0;
     }
-    for (size_t q = 0; q < c->inters.size(); ++q) {
-      const int i = c->inters[q].si, j = c->inters[q].sj;
+    for (int q = 0; q < b.ninter; ++q) {
+      const int i = b.si[q], j = b.sj[q];
       if (i == s && j == s)
-        v += c->a_vdw[q] * rho[j];
+        v += b.a_vdw[q] * rho[j];
       else if (i == s)
-        v += 0.5 * c->a_vdw[q] * rho[j];
+        v += 0.5 * b.a_vdw[q] * rho[j];
       else if (j == s)
-        v += 0.5 * c->a_vdw[q] * rho[i];
+        v += 0.5 * b.a_vdw[q] * rho[i];
     }
     mu[s] = v;
+  }
+}
+
+// Many bulk states at once, one thread each (phase-diagram and coexistence
+// scans): f, the chemical potentials and P = sum rho mu - f
+// (thermodynamics.hpp:206-255); each output nullable.
+__global__ __launch_bounds__(kMixThreads) void k_bulk_batch(BulkParams b, long n, const double* __restrict__ rho,
+                                                            double* __restrict__ f, double* __restrict__ mu,
+                                                            double* __restrict__ p) {
+  const long i = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x;
+  if (i >= n) return;
+  double r[kMaxSpecies], m[kMaxSpecies];
+  for (int s = 0; s < b.S; ++s) r[s] = rho[i * b.S + s];
+  const double fe = bulk_free_energy_density(b, r);
+  bulk_chemical_potentials(b, r, m);
+  if (f) f[i] = fe;
+  if (mu)
+    for (int s = 0; s < b.S; ++s) mu[i * b.S + s] = m[s];
+  if (p) {
+    double v = -fe;
+    for (int s = 0; s < b.S; ++s) v += r[s] * m[s];
+    p[i] = v;
   }
 }
 
@@ -587,7 +629,7 @@ int cdft_bulk_free_energy_density(const cdft_ctx* ctx, const double* rho, double
     check_ctx_any(ctx);
     if (!f) throw std::invalid_argument("null argument");
     check_bulk_rho(rho, ctx->impl->n_species);
-    *f = bulk_free_energy_density(ctx->impl, rho);
+    *f = bulk_free_energy_density(bulk_params(ctx->impl), rho);
   });
 }
 
@@ -596,7 +638,7 @@ int cdft_bulk_chemical_potentials(const cdft_ctx* ctx, const double* rho, double
     check_ctx_any(ctx);
     if (!mu) throw std::invalid_argument("null argument");
     check_bulk_rho(rho, ctx->impl->n_species);
-    bulk_chemical_potentials(ctx->impl, rho, mu);
+    bulk_chemical_potentials(bulk_params(ctx->impl), rho, mu);
   });
 }
 
@@ -607,10 +649,46 @@ int cdft_bulk_pressure(const cdft_ctx* ctx, const double* rho, double* p) {
     const Context* c = ctx->impl;
     check_bulk_rho(rho, c->n_species);
     std::vector<double> mu(static_cast<size_t>(c->n_species));
-    bulk_chemical_potentials(c, rho, mu.data());
-    double v = -bulk_free_energy_density(c, rho);  // P = sum rho mu - f (thermodynamics.hpp:239-246)
+    const BulkParams b = bulk_params(c);
+    bulk_chemical_potentials(b, rho, mu.data());
+    double v = -bulk_free_energy_density(b, rho);  // P = sum rho mu - f (thermodynamics.hpp:239-246)
     for (int s = 0; s < c->n_species; ++s) v += rho[s] * mu[s];
     *p = v;
+  });
+}
+
+int cdft_bulk_batch(const cdft_ctx* ctx, long n_states, const double* rho, double* f, double* mu, double* p) {
+  return guarded([&] {
+    check_ctx_any(ctx);
+    const Context* c = ctx->impl;
+    const int S = c->n_species;
+    if (n_states < 0 || (n_states > 0 && !rho)) throw std::invalid_argument("cdft_bulk_batch: bad argument");
+    if (n_states == 0) return;
+    for (long i = 0; i < n_states * S; ++i)
+      if (!(rho[i] > 0.0) || !std::isfinite(rho[i])) throw std::invalid_argument("bulk densities must be positive");
+    CDFT_HIP(hipSetDevice(c->device));
+    const size_t nr = sizeof(double) * n_states * S, n1 = sizeof(double) * n_states;
+    double* blk = nullptr;
+    CDFT_HIP(hipMalloc(&blk, 2 * nr + 2 * n1));
+    try {
+      double* d_rho = blk;
+      double* d_mu = blk + n_states * S;
+      double* d_f = d_mu + n_states * S;
+      double* d_p = d_f + n_states;
+      CDFT_HIP(hipMemcpyAsync(d_rho, rho, nr, hipMemcpyHostToDevice, c->stream));
+      const unsigned nb = static_cast<unsigned>((n_states + kMixThreads - 1) / kMixThreads);
+      k_bulk_batch<<<nb, kMixThreads, 0, c->stream>>>(bulk_params(c), n_states, d_rho, f ? d_f : nullptr,
+                                                      mu ? d_mu : nullptr, p ? d_p : nullptr);
+      CDFT_HIP(hipGetLastError());
+      if (f) CDFT_HIP(hipMemcpyAsync(f, d_f, n1, hipMemcpyDeviceToHost, c->stream));
+      if (mu) CDFT_HIP(hipMemcpyAsync(mu, d_mu, nr, hipMemcpyDeviceToHost, c->stream));
+      if (p) CDFT_HIP(hipMemcpyAsync(p, d_p, n1, hipMemcpyDeviceToHost, c->stream));
+      CDFT_HIP(hipStreamSynchronize(c->stream));
+    } catch (...) {
+      (void)hipFree(blk);
+      throw;
+    }
+    CDFT_HIP(hipFree(blk));
   });
 }
 

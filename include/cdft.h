@@ -167,6 +167,13 @@ CDFT_API int cdft_contributions(cdft_ctx* ctx, const double* rho, const double* 
 CDFT_API int cdft_bulk_free_energy_density(const cdft_ctx* ctx, const double* rho, double* f);
 CDFT_API int cdft_bulk_chemical_potentials(const cdft_ctx* ctx, const double* rho, double* mu);
 CDFT_API int cdft_bulk_pressure(const cdft_ctx* ctx, const double* rho, double* p);
+/* The same equation of state for n_states bulk states at once on the device,
+ * one thread per state (for phase-diagram and coexistence scans;
+ * bulk/coexistence.hpp and phase_diagram.hpp evaluate it state by state):
+ * rho holds n_states * n_species densities (state-major); f and p receive
+ * n_states values and mu n_states * n_species; each output nullable. */
+CDFT_API int cdft_bulk_batch(const cdft_ctx* ctx, long n_states, const double* rho, double* f, double* mu,
+                             double* p);
 /* GPU time of `count` back-to-back mixture evaluations (HIP events on the
  * context stream; no counterpart in the reference). */
 CDFT_API int cdft_time_total_species_device(cdft_ctx* ctx, const double* d_rho, const double* d_vext,

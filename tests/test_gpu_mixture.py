@@ -226,3 +226,23 @@ def test_eight_species_with_every_pair_matches_oracle():
     with pytest.raises(cd.CdftError) as e:
         cd.Context(shape, dx, diameters=diams, interactions=inter + [dict(LJ, si=0, sj=1)])
     assert e.value.code == -4
+
+
+@pytest.mark.parametrize("model,diams", [("WhiteBearII", (1.0,)), ("esFMT", (1.0, 0.7, 1.2)), ("none", (1.0, 0.8))])
+def test_bulk_batch_on_the_device_matches_the_host_calls(model, diams):
+    import classicaldft_amd as cd
+
+    S = len(diams)
+    inter = [dict(LJ, si=i, sj=j, epsilon=0.5 + 0.1 * (i + j)) for i in range(S) for j in range(i, S)]
+    ctx = cd.Context((16, 16, 16), 0.2, diameters=diams, interactions=inter, kT=1.1, model=model, A=1.2, B=-0.5)
+    rng = np.random.default_rng(3)
+    rho = rng.uniform(0.01, 0.7 / S, size=(257, S))
+    f, mu, p = ctx.bulk_batch(rho)
+    for i in range(0, 257, 16):
+        assert f[i] == pytest.approx(ctx.bulk_free_energy_density(rho[i]), rel=1e-13, abs=1e-15)
+        assert np.allclose(mu[i], ctx.bulk_chemical_potentials(rho[i]), rtol=1e-13, atol=1e-15)
+        assert p[i] == pytest.approx(ctx.bulk_pressure(rho[i]), rel=1e-12, abs=1e-14)
+    if S == 1:
+        assert np.allclose(mu[:, 0], [ctx.bulk_mu(r) for r in rho[:, 0]], rtol=1e-13, atol=1e-15)
+    with pytest.raises(cd.InvalidArgument):
+        ctx.bulk_batch(np.zeros((2, S)))
