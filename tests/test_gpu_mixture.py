@@ -246,3 +246,15 @@ def test_bulk_batch_on_the_device_matches_the_host_calls(model, diams):
         assert np.allclose(mu[:, 0], [ctx.bulk_mu(r) for r in rho[:, 0]], rtol=1e-13, atol=1e-15)
     with pytest.raises(cd.InvalidArgument):
         ctx.bulk_batch(np.zeros((2, S)))
+
+
+def test_binary_mixture_at_64cubed_matches_oracle():
+    # a grid the fused one-species kernels would take: the mixture still runs the general transforms
+    shape, dx = (64, 64, 64), 0.1
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=1, sigma=0.8, epsilon=0.6),
+             dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
+    ctx, orc = _pair(shape, (1.0, 0.8), inter, "WhiteBearII", kT=1.0, dx=dx)
+    assert ctx.schedule == "general"
+    rho = np.stack([W.droplet(shape, dx, radius=1.5, width=0.4, rho_l=0.5, rho_v=0.03, noise=1e-3, seed=61),
+                    W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.2, rho_v=0.05, noise=1e-3, seed=62)])
+    _compare(ctx, orc, rho, (-1.5, -1.2), tag="binary 64^3")
