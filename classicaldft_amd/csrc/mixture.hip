@@ -32,6 +32,10 @@ namespace cdft {
 namespace {
 
 constexpr int kMixFmt = 16;  // eta, n2, n1, n0, n_v2[3], n_v1[3], T[6]
+// Transformed FMT channels: all but T_zz, which follows from the trace
+// identity T_zz = n2 - T_xx - T_yy of every species' weighted densities (the
+// diagonal tensor weights sum to w2), so its derivative folds into those three.
+constexpr int kMixFmtT = 15;
 constexpr int kMixThreads = 256;
 
 struct MixParams {
@@ -57,7 +61,7 @@ MixLayout mix_layout(const Context* c) {
   MixLayout L;
   const size_t S = static_cast<size_t>(c->n_species), N = static_cast<size_t>(c->g.N),
                M = static_cast<size_t>(c->g.M);
-  const int nfm = c->cs.hs ? kMixFmt : 0, nu = c->inters.empty() ? 0 : c->n_species;
+  const int nfm = c->cs.hs ? kMixFmtT : 0, nu = c->inters.empty() ? 0 : c->n_species;
   L.nch = std::max(nfm + nu, 1);
   L.nblk = static_cast<int>(std::min<long>(1024, (c->g.N + kMixThreads - 1) / kMixThreads));
   size_t off = 0;
@@ -116,9 +120,9 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __res
       kvec3(i, g, &kx, &ky, &kz);
       const double k2 = kx * kx + ky * ky + kz * kz;
       const double k = sqrt(k2);
-      cx<double> acc[kMixFmt];
+      cx<double> acc[kMixFmtT];
 #pragma unroll
-      for (int c = 0; c < kMixFmt; ++c) acc[c] = {0.0, 0.0};
+      for (int c = 0; c < kMixFmtT; ++c) acc[c] = {0.0, 0.0};
       for (int s = 0; s < p.S; ++s) {
         double w3, w2;
         radial(k, p.R[s], g.inv_n, &w3, &w2);
@@ -135,12 +139,14 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __res
           acc[7 + a] = cadd(acc[7 + a], cscale(wv, p.ifr[s]));
         }
 #pragma unroll
-        for (int q = 0; q < 6; ++q)
-          acc[10 + q] = cadd(acc[10 + q], cscale(r, channel_weight_t(5 + q, false, kx, ky, kz, wr, 0.0, p.invR[s]).re));
+        for (int q = 0; q < 5; ++q) {  // T_xx .. T_yz (T_zz by the trace identity)
+          const double wq = channel_weight_t(5 + q, false, kx, ky, kz, wr, 0.0, p.invR[s]).re;
+          acc[10 + q] = cadd(acc[10 + q], cscale(r, wq));
+        }
       }
 #pragma unroll
-      for (int c = 0; c < kMixFmt; ++c) F[c * M + i] = acc[c];
-      ch = kMixFmt;
+      for (int c = 0; c < kMixFmtT; ++c) F[c * M + i] = acc[c];
+      ch = kMixFmtT;
     }
     if (p.ninter > 0) {
       for (int s = 0; s < p.S; ++s) {
@@ -173,10 +179,14 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_point(double* __restrict__ 
     if (mp.hs) {
       double m[kMixFmt], d[kMixFmt];
 #pragma unroll
-      for (int c = 0; c < kMixFmt; ++c) m[c] = X[c * N + p];
+      for (int c = 0; c < kMixFmtT; ++c) m[c] = X[c * N + p];
+      m[15] = m[1] - m[10] - m[13];  // T_zz
       sums[2] += fmt_phi_grad_mix_t<double>(mp.fp, m, d);
+      d[1] += d[15];  // adjoint of the trace identity
+      d[10] -= d[15];
+      d[13] -= d[15];
 #pragma unroll
-      for (int c = 0; c < kMixFmt; ++c) X[c * N + p] = d[c];
+      for (int c = 0; c < kMixFmtT; ++c) X[c * N + p] = d[c];
     }
     for (int s = 0; s < mp.S; ++s) {
       const double r = rho[s * N + p];
@@ -211,9 +221,9 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_bwd(const cx<double>* __res
     kvec3(i, g, &kx, &ky, &kz);
     const double k2 = kx * kx + ky * ky + kz * kz;
     const double k = sqrt(k2);
-    cx<double> d[kMixFmt];
+    cx<double> d[kMixFmtT];  // the T_zz derivative is folded into n2, T_xx and T_yy
 #pragma unroll
-    for (int c = 0; c < kMixFmt; ++c) d[c] = D[c * M + i];
+    for (int c = 0; c < kMixFmtT; ++c) d[c] = D[c * M + i];
     for (int s = 0; s < p.S; ++s) {
       double w3, w2;
       radial(k, p.R[s], g.inv_n, &w3, &w2);
@@ -227,7 +237,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_bwd(const cx<double>* __res
         b = cadd(b, cmul(channel_weight_t(2 + a, true, kx, ky, kz, wr, 0.0, p.invR[s]), dv));
       }
 #pragma unroll
-      for (int q = 0; q < 6; ++q)
+      for (int q = 0; q < 5; ++q)
         b = cadd(b, cscale(d[10 + q], channel_weight_t(5 + q, false, kx, ky, kz, wr, 0.0, p.invR[s]).re));
       B[s * M + i] = b;
     }
@@ -305,7 +315,7 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
   const MixLayout L = mix_layout(c);
   const MixParams p = mix_params(c);
   const int S = c->n_species;
-  const int nfm = c->cs.hs ? kMixFmt : 0, nu = c->inters.empty() ? 0 : S;
+  const int nfm = c->cs.hs ? kMixFmtT : 0, nu = c->inters.empty() ? 0 : S;
   auto* R = at<cx<double>>(c, L.R);
   auto* F = at<cx<double>>(c, L.F);
   auto* X = at<double>(c, L.X);
@@ -332,7 +342,7 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
   k_mix_point<<<L.nblk, kMixThreads, 0, c->stream>>>(X, rho, vext, local, part, g.N, mp);
   if (force) {
     if (nfm > 0) {
-      r2c_fields(c, X, F, kMixFmt);
+      r2c_fields(c, X, F, kMixFmtT);
       k_mix_bwd<<<spectral_blocks(g.M), kMixThreads, 0, c->stream>>>(F, B, g, p);
       inverse_xy(c, B, S);
       launch_zc2r<double, double>(B, S, force, local, g.dv, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
@@ -597,7 +607,7 @@ int cdft_contributions(cdft_ctx* ctx, const double* rho, const double* vext, con
         mixture_result(c, result);
         if (!c->inters.empty()) {  // the per-species potentials U_s stay in the channel fields
           const MixLayout L = mix_layout(c);
-          const int nfm = c->cs.hs ? kMixFmt : 0;
+          const int nfm = c->cs.hs ? kMixFmtT : 0;
           for (int s = 0; s < S; ++s) conv[s] = at<double>(c, L.X) + static_cast<long>(nfm + s) * N;
         }
       }
