@@ -32,9 +32,10 @@ constexpr long kLdsLimit = 65536;  // dynamic LDS without the per-kernel attribu
 
 struct TuneKey {
   long nx, ny, nz;
-  int precision, nf, nb;
+  int precision, nf, nb, species;
   bool operator==(const TuneKey& o) const {
-    return nx == o.nx && ny == o.ny && nz == o.nz && precision == o.precision && nf == o.nf && nb == o.nb;
+    return nx == o.nx && ny == o.ny && nz == o.nz && precision == o.precision && nf == o.nf && nb == o.nb &&
+           species == o.species;
   }
 };
 struct TuneEntry {
@@ -45,7 +46,7 @@ std::mutex g_tune_mu;
 std::vector<TuneEntry> g_tuned;
 
 TuneKey key_of(const Context* c) {
-  return {c->shape[0], c->shape[1], c->shape[2], c->precision, c->cs.nf, c->cs.nb};
+  return {c->shape[0], c->shape[1], c->shape[2], c->precision, c->cs.nf, c->cs.nb, c->n_species};
 }
 
 __global__ void k_tune_density(double* rho, Geom g) {
@@ -120,6 +121,7 @@ int default_zpoint(const Context* c) {
 void apply(Context* c, const FftPlan pl[3]) {
   for (int d = 0; d < 3; ++d) c->plan[d] = pl[d];
   c->clear_graphs();
+  if (c->n_species > 1) return;  // mixtures size their launches from the plans alone
   c->nblk_point = c->precision == CDFT_PRECISION_MIXED ? zpoint_blocks<float>(c->g, c->plan[2])
                                                       : zpoint_blocks<double>(c->g, c->plan[2]);
   if (c->nblk_point > c->max_point_rows) throw std::logic_error("autotune: pointwise blocks exceed the partials");
@@ -138,6 +140,7 @@ struct Timer {
   }
   // microseconds per evaluation, and Omega of the last one
   double time(int reps, double* omega) {
+    if (c->n_species > 1) return mixture_tune_time(c, reps, omega);
     const AliasMap id{0, 0.0, 0.0};
     c->eval_direct(c->d_rho, nullptr, 0.0, c->d_force, id);
     CDFT_HIP(hipEventRecord(a, c->stream));
@@ -163,12 +166,14 @@ extern "C" {
 
 int cdft_autotune(cdft_ctx* ctx, int reps, double* us_default, double* us_tuned) {
   return guarded([&] {
-    check_ctx(ctx);
+    check_ctx_any(ctx);
     if (reps < 1) throw std::invalid_argument("reps must be >= 1");
     Context* c = ctx->impl;
     CDFT_HIP(hipSetDevice(c->device));
-    hipLaunchKernelGGL(k_tune_density, dim3(vec_blocks(c->g.N)), dim3(256), 0, c->stream, c->d_rho, c->g);
-    CDFT_HIP(hipGetLastError());
+    if (c->n_species == 1) {  // (a mixture context fills its own staging densities)
+      hipLaunchKernelGGL(k_tune_density, dim3(vec_blocks(c->g.N)), dim3(256), 0, c->stream, c->d_rho, c->g);
+      CDFT_HIP(hipGetLastError());
+    }
     Timer tm(c);
     FftPlan best[3] = {c->plan[0], c->plan[1], c->plan[2]};
     double omega0 = 0.0;

@@ -177,6 +177,10 @@ double mf_weight_spectrum(Context* c, const InteractionSpec& s, cx<double>* spec
 void store_real_spectrum(const cx<double>* spec, double* out, long m, hipStream_t stream);
 // Buffers and weights of a mixture context (defined with the mixture kernels).
 void init_mixture(Context* c);
+// Autotuning of a mixture context: `reps` evaluations (with forces) of a
+// fixed smooth density in the context's staging buffers; microseconds per
+// evaluation, and Omega of the last.
+double mixture_tune_time(Context* c, int reps, double* omega);
 
 void set_error(const std::string& msg);
 

@@ -491,7 +491,53 @@ void check_mu(const double* mu, int S) {
     if (!std::isfinite(mu[s])) throw std::invalid_argument("chemical potential is not finite");
 }
 
+// A smooth positive density for every species (autotuning input).
+__global__ __launch_bounds__(kMixThreads) void k_mix_tune_density(double* rho, Geom g, int S) {
+  const long n = g.N * S;
+  for (long i = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kMixThreads) {
+    const long s = i / g.N, p = i - s * g.N;
+    const long iz = p % g.nz, iy = (p / g.nz) % g.ny, ix = p / (static_cast<long>(g.ny) * g.nz);
+    const double a = 6.283185307179586 * (static_cast<double>(ix) / g.nx + static_cast<double>(iy) / g.ny);
+    rho[i] = (0.3 + 0.2 * sin(a) * cos(6.283185307179586 * static_cast<double>(iz) / g.nz)) / (1.0 + s);
+  }
+}
+
 }  // namespace
+
+double mixture_tune_time(Context* c, int reps, double* omega) {
+  const MixLayout L = mix_layout(c);
+  double* rho = at<double>(c, L.rho);
+  double* force = at<double>(c, L.force);
+  const Geom& g = c->g;
+  k_mix_tune_density<<<spectral_blocks(g.N * c->n_species), kMixThreads, 0, c->stream>>>(rho, g, c->n_species);
+  CDFT_HIP(hipGetLastError());
+  const std::vector<double> mu(static_cast<size_t>(c->n_species), 0.0);
+  mixture_eval(c, rho, nullptr, mu.data(), force);
+  hipEvent_t a = nullptr, b = nullptr;
+  CDFT_HIP(hipEventCreate(&a));
+  if (hipEventCreate(&b) != hipSuccess) {
+    (void)hipEventDestroy(a);
+    throw HipError("hipEventCreate failed");
+  }
+  float ms = 0.0f;
+  try {
+    CDFT_HIP(hipEventRecord(a, c->stream));
+    for (int i = 0; i < reps; ++i) mixture_eval(c, rho, nullptr, mu.data(), force);
+    CDFT_HIP(hipEventRecord(b, c->stream));
+    cdft_result r{};
+    mixture_result(c, &r);
+    CDFT_HIP(hipEventElapsedTime(&ms, a, b));
+    *omega = r.grand_potential;
+  } catch (...) {
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    throw;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return 1000.0 * ms / reps;
+}
 
 void init_mixture(Context* c) {
   const Geom& g = c->g;

@@ -287,8 +287,10 @@ CDFT_API int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_
  * FFTW_ESTIMATE, src/math/fourier.cpp:34-57): for grids the fused kernels do
  * not cover, times the evaluation under alternative FFT radix orders and
  * launch shapes and keeps the fastest whose Omega agrees with the default's.
- * Results are cached per grid, precision and channel set for the process, so
- * later contexts of the same shape take them without timing.  us_default and
+ * Mixture contexts (which always run the general transforms) are tuned on
+ * their own evaluation.  Results are cached per grid, precision, channel set
+ * and species count for the process, so later contexts of the same shape take
+ * them without timing.  us_default and
  * us_tuned (nullable) receive the time of one evaluation before and after, in
  * microseconds; a fused-schedule context is timed and left as it is. */
 CDFT_API int cdft_autotune(cdft_ctx* ctx, int reps, double* us_default, double* us_tuned);

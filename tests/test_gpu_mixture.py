@@ -258,3 +258,15 @@ def test_binary_mixture_at_64cubed_matches_oracle():
     rho = np.stack([W.droplet(shape, dx, radius=1.5, width=0.4, rho_l=0.5, rho_v=0.03, noise=1e-3, seed=61),
                     W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.2, rho_v=0.05, noise=1e-3, seed=62)])
     _compare(ctx, orc, rho, (-1.5, -1.2), tag="binary 64^3")
+
+
+def test_autotuned_mixture_still_matches_oracle():
+    # cdft_autotune on a mixture context times the mixture evaluation; results change only by rounding
+    shape, dx = (30, 24, 20), 0.2
+    inter = [dict(LJ, si=0, sj=1, epsilon=0.6)]
+    ctx, orc = _pair(shape, (1.0, 0.7), inter, "WhiteBearII", kT=1.0, dx=dx)
+    us0, us1 = ctx.autotune(reps=3)
+    print(f"mixture autotune {shape}: {us0:.1f} -> {us1:.1f} us", flush=True)
+    assert us1 <= us0 * 1.05
+    rho = _densities(shape, dx, (0.25, 0.15), seed=71)
+    _compare(ctx, orc, rho, (0.0, -0.2), tag="autotuned mixture")
