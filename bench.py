@@ -273,7 +273,7 @@ def converged_profile(cd, W, rho0, shape, dx, kT, steps=500):
     return out
 
 
-def mixture(cd, W, shape=(128, 128, 128), evals=50):
+def mixture(cd, W, shape=(128, 128, 128), evals=50, autotune=True):
     """Binary LJ mixture (diameters 1.0 and 0.8; self pairs LJ(1, 1) and LJ(0.8, 0.6), cross pair
     LJ(0.9, 0.8), WCA, r_c = 2.5; kT = 1.0, WB-II, dx = 0.1) on smooth noisy densities of mean 0.3 and 0.2:
     FP64 evaluations per second with forces after autotuning the context, timed with HIP events on the
@@ -285,7 +285,7 @@ def mixture(cd, W, shape=(128, 128, 128), evals=50):
     inter = [dict(W.LJ_WCA, si=0, sj=0), dict(W.LJ_WCA, si=1, sj=1, sigma=0.8, epsilon=0.6),
              dict(W.LJ_WCA, si=0, sj=1, sigma=0.9, epsilon=0.8)]
     ctx = cd.Context(shape, dx, diameters=(1.0, 0.8), interactions=inter, kT=1.0, model="WhiteBearII")
-    us_default, us_tuned = ctx.autotune(reps=5)
+    us_default, us_tuned = ctx.autotune(reps=5) if autotune else (None, None)
     rho = np.stack([W.smooth_noise_fluid(shape, dx, rho0=0.3, amp=0.02, seed=1),
                     W.smooth_noise_fluid(shape, dx, rho0=0.2, amp=0.02, seed=2)])
     d_rho = torch.tensor(rho, device="cuda:0").contiguous()
@@ -311,6 +311,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-vs-sequential FIRE measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the DDFT adjoint measurement")
     ap.add_argument("--no-mixture", action="store_true", help="skip the binary-mixture evaluation measurement")
+    ap.add_argument("--mixture-untuned", action="store_true",
+                    help="time the mixture on the default launch shapes (no autotuning trials in a profile)")
     ap.add_argument("--profile-steps", type=int, default=500,
                     help="FIRE steps of the mixed-vs-FP64 converged-profile comparison (0 skips it)")
     ap.add_argument("--ddft-steps", type=int, default=2000,
@@ -440,7 +442,7 @@ def main():
         out["ddft_slab"] = ddft_slab(cd, W, steps=args.ddft_steps)
 
     if not args.no_mixture:
-        out["mixture"] = mixture(cd, W)
+        out["mixture"] = mixture(cd, W, autotune=not args.mixture_untuned)
 
     if not args.no_cpu_baseline:
         cb, parts, fr = cpu_baseline(rho, mu, shape, dx)
