@@ -1,6 +1,6 @@
 // FP64 CPU oracle for the classicalDFT free-energy + gradient path.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker: the pytest suite, the
+// TEST INFRASTRUCTURE ONLY.  This file is the oracle: the pytest suite, the
 // smoke() check and the CPU-baseline leg of bench.py load it (through ctypes)
 // to compare the HIP library against.  Nothing in the product links it.
 //

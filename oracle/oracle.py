@@ -1,7 +1,7 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/liboracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
-as the checker and by the CPU-baseline leg of ``bench.py``.  The product
+as the reference it compares against and by the CPU-baseline leg of ``bench.py``.  The product
 (``classicaldft_amd``) never imports this module.
 
 The C++ source (oracle/oracle.cpp) restates the reference algorithm; each

@@ -1,4 +1,4 @@
-"""Pins the FP64 CPU oracle (oracle/oracle.cpp) before it is trusted as the checker.
+"""Pins the FP64 CPU oracle (oracle/oracle.cpp) before the GPU tests compare against it.
 
 Each check is chosen so that a plausible restatement mistake (dropped term,
 wrong sign or index, transposed operand, wrong normalisation) fails it:
