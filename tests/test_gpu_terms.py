@@ -86,3 +86,26 @@ def test_ideal_only_context_has_zero_hard_sphere_term():
     rho = W.smooth_noise_fluid(shape, dx, rho0=0.3, amp=0.02, seed=3)
     res, t = ctx.contributions(rho, 0.0)
     assert res["hard_sphere"] == 0.0 and np.all(t["hard_sphere"] == 0.0)
+
+
+def test_mixed_precision_terms_on_the_fused_schedule():
+    # the mean-field term comes from the weighted-density pass, the total from the fused kernels:
+    # the hard-sphere term keeps the mixed-precision force tolerance
+    import classicaldft_amd as cd
+    from oracle import oracle as O
+
+    shape, dx = (64, 64, 64), 0.1
+    ctx = cd.Context(shape, dx, interactions=[LJ], kT=0.8, model="WhiteBearII", precision="mixed")
+    assert ctx.schedule == "fused"
+    orc = O.Oracle(shape, dx, interactions=[LJ], kT=0.8, model="WhiteBearII")
+    rho = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=8)
+    mu = ctx.bulk_mu(0.05)
+    res, t = ctx.contributions(rho, mu)
+    parts, fo = orc.total(rho, mu)
+    dv = dx ** 3
+    ideal = (np.log(rho) - mu) * dv
+    mf = orc.weighted_densities(rho)[11] * dv
+    assert _rel(t["ideal"][0], ideal) < 1e-13
+    assert _rel(t["mean_field"][0], mf) < 2e-5
+    assert _rel(t["hard_sphere"][0], fo - ideal - mf) < 2e-5
+    assert res["grand_potential"] == pytest.approx(parts[5], rel=1e-6)
