@@ -270,3 +270,34 @@ def test_autotuned_mixture_still_matches_oracle():
     assert us1 <= us0 * 1.05
     rho = _densities(shape, dx, (0.25, 0.15), seed=71)
     _compare(ctx, orc, rho, (0.0, -0.2), tag="autotuned mixture")
+
+
+def test_mixture_contexts_and_calls_return_their_device_memory():
+    import gc
+
+    import torch
+
+    import classicaldft_amd as cd
+
+    shape, dx = (64, 64, 64), 0.1
+    inter = [dict(LJ, si=0, sj=1)]
+    rho = _densities(shape, dx, (0.3, 0.2), seed=81)
+
+    def cycle():
+        ctx = cd.Context(shape, dx, diameters=(1.0, 0.8), interactions=inter, kT=1.0, model="WhiteBearII")
+        ctx.total_species(rho, (0.0, 0.0))
+        ctx.contributions(rho, (0.0, 0.0))
+        ctx.bulk_batch(np.full((1000, 2), 0.2))
+        ctx.close()
+
+    cycle()  # first use: code objects and caches
+    gc.collect()
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    for _ in range(5):
+        cycle()
+    gc.collect()
+    torch.cuda.synchronize()
+    free1 = torch.cuda.mem_get_info()[0]
+    print(f"device memory change over 5 mixture contexts: {(free0 - free1) / 2**20:.1f} MiB", flush=True)
+    assert free0 - free1 < 64 * 2**20
