@@ -273,11 +273,26 @@ def test_autotuned_mixture_still_matches_oracle():
 
 
 def test_mixture_contexts_and_calls_return_their_device_memory():
+    # free device memory from the HIP runtime itself (hipMemGetInfo), around repeated mixture
+    # contexts and calls; the runtime's last-error slot must be clear when the test starts
+    import ctypes as C
     import gc
 
-    import torch
-
     import classicaldft_amd as cd
+
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipGetErrorName.restype = C.c_char_p
+    last = hip.hipGetLastError()
+    n = C.c_int()
+    rc = hip.hipGetDeviceCount(C.byref(n))
+    print(f"hip last error at start: {last} ({hip.hipGetErrorName(last).decode()}); device count rc {rc}, "
+          f"{n.value}", flush=True)
+
+    def free_bytes():
+        assert hip.hipDeviceSynchronize() == 0
+        f, t = C.c_size_t(), C.c_size_t()
+        assert hip.hipMemGetInfo(C.byref(f), C.byref(t)) == 0
+        return f.value
 
     shape, dx = (64, 64, 64), 0.1
     inter = [dict(LJ, si=0, sj=1)]
@@ -292,12 +307,11 @@ def test_mixture_contexts_and_calls_return_their_device_memory():
 
     cycle()  # first use: code objects and caches
     gc.collect()
-    torch.cuda.synchronize()
-    free0 = torch.cuda.mem_get_info()[0]
+    free0 = free_bytes()
     for _ in range(5):
         cycle()
     gc.collect()
-    torch.cuda.synchronize()
-    free1 = torch.cuda.mem_get_info()[0]
+    free1 = free_bytes()
     print(f"device memory change over 5 mixture contexts: {(free0 - free1) / 2**20:.1f} MiB", flush=True)
     assert free0 - free1 < 64 * 2**20
+    assert last == 0, "a library call left an error in the HIP runtime's last-error slot"
