@@ -388,7 +388,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 fields+FFT / fp64 pointwise, reductions, state" if args.precision == "mixed" else "fp64",
+        "dtype": "f32+f64" if args.precision == "mixed" else "f64",
+        "dtype_detail": ("f32 fields and vector/tensor transforms; f64 scalar-channel transforms, pointwise "
+                         "terms, reductions and state") if args.precision == "mixed" else "f64 throughout",
         "data": "synthetic",
         "config": {
             "workload": "LJ droplet F+dF/drho eval: 128^3, dx=0.1, WB-II FMT d=1, LJ WCA rc=2.5, kT=0.8, "
