@@ -1,0 +1,181 @@
+"""GPU parity at the stated workload configurations, against the FP64 CPU oracle
+on the same seeded inputs, through the C ABI.
+
+- the 128^3 LJ droplet evaluation (the throughput metric's input): FP64 energies
+  1e-12 and forces 1e-11; mixed Omega 1e-6 and forces 2e-5 (relative to
+  max|force|);
+- the 64^3 hard-sphere fluid with V = 0.5 cos(2 pi z / L): one evaluation, then
+  300 FIRE steps of Minimizer::grand_potential in the alias variable (FP64:
+  per-step energies 1e-9, profile 1e-8; identical step count, dt and alpha);
+- batched Minimizer runs over uneven grids (48^3 and 80^3 on the general
+  transforms, 64^3 fused), each member against the oracle's single run;
+- DDFT integrating-factor steps on fused-schedule grids (64^3 FP64 and mixed,
+  256x64x64 mixed) with mass conserved to 1e-12.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from classicaldft_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PARTS = ("ideal", "external", "hard_sphere", "mean_field", "free_energy", "grand_potential")
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _ctx(shape, dx, **kw):
+    import classicaldft_amd as cd
+
+    return cd.Context(shape, dx, **kw)
+
+
+def _orc(shape, dx, **kw):
+    from oracle import oracle as O
+
+    kw.pop("precision", None)
+    return O.Oracle(shape, dx, **kw)
+
+
+@pytest.fixture(scope="module")
+def droplet128():
+    from oracle import oracle as O
+
+    shape, dx = (128, 128, 128), 0.1
+    rho = W.droplet(shape, dx)  # R = 4, width 0.5, rho_l 0.70, rho_v 0.02, 1e-3 noise, seed 2024
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    mu = orc.bulk_mu(0.02)
+    parts, f = orc.total(rho, mu)
+    return shape, dx, rho, mu, parts, f
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_metric_droplet_128cubed(droplet128, precision):
+    shape, dx, rho, mu, parts, fr = droplet128
+    ctx = _ctx(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII", precision=precision)
+    assert ctx.schedule == "fused"
+    assert ctx.bulk_mu(0.02) == pytest.approx(mu, rel=1e-13)
+    res, f = ctx.total(rho, mu)
+    eo = abs(res["grand_potential"] - parts[5]) / abs(parts[5])
+    ef = _rel(f, fr)
+    print(f"128^3 droplet {precision}: Omega rel err {eo:.2e}, force rel err {ef:.2e}", flush=True)
+    if precision == "fp64":
+        for i, k in enumerate(PARTS):
+            assert res[k] == pytest.approx(parts[i], rel=1e-12, abs=1e-9), k
+        assert ef < 1e-11
+    else:
+        assert eo <= 1e-6
+        assert ef <= 2e-5
+        for i, k in enumerate(PARTS):
+            assert res[k] == pytest.approx(parts[i], rel=1e-6, abs=1e-3), k
+
+
+def _config1():
+    shape, dx = (64, 64, 64), 0.1
+    rho = W.smooth_noise_fluid(shape, dx, rho0=0.5, amp=0.01, sigma=0.3, seed=7)
+    L = shape[2] * dx
+    z = np.arange(shape[2]) * dx
+    vext = np.broadcast_to(0.5 * np.cos(2 * math.pi * z / L), shape).copy()
+    return shape, dx, rho, vext
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_config1_hard_sphere_fluid_evaluation(precision):
+    shape, dx, rho, vext = _config1()
+    kw = dict(interactions=[], kT=1.0, model="WhiteBearII")
+    ctx = _ctx(shape, dx, precision=precision, **kw)
+    orc = _orc(shape, dx, **kw)
+    mu = orc.bulk_mu(0.5)
+    assert ctx.bulk_mu(0.5) == pytest.approx(mu, rel=1e-13)
+    res, f = ctx.total(rho, mu, vext=vext)
+    parts, fr = orc.total(rho, mu, vext)
+    eo = abs(res["grand_potential"] - parts[5]) / abs(parts[5])
+    ef = _rel(f, fr)
+    print(f"config 1 evaluation {precision}: Omega rel err {eo:.2e}, force rel err {ef:.2e}", flush=True)
+    if precision == "fp64":
+        for i, k in enumerate(PARTS):
+            assert res[k] == pytest.approx(parts[i], rel=1e-12, abs=1e-10), k
+        assert ef < 1e-11
+    else:
+        assert eo <= 1e-6 and ef <= 2e-5
+
+
+def test_config1_fire_300_steps_fp64():
+    shape, dx, rho, vext = _config1()
+    kw = dict(interactions=[], kT=1.0, model="WhiteBearII")
+    ctx = _ctx(shape, dx, precision="fp64", **kw)
+    orc = _orc(shape, dx, **kw)
+    mu = orc.bulk_mu(0.5)
+    # force_tolerance 0 keeps both runs going for exactly 300 steps; the
+    # homogeneous-boundary averaging is off (a periodic box)
+    fire = dict(force_tolerance=0.0)
+    mk = dict(fire=fire, max_steps=300, homogeneous_boundary=False, vext=vext)
+    rg, resg, eg = ctx.minimize_grand_potential(rho, mu, **mk)
+    ro, reso, eo = orc.minimize_grand_potential(rho, mu, **mk)
+    e = _rel(rg, ro)
+    print(f"config 1 FIRE: {resg['iterations']} steps, Omega {resg['grand_potential']:.12e} vs "
+          f"{reso['grand_potential']:.12e}, profile rel err {e:.2e}", flush=True)
+    assert resg["iterations"] == reso["iterations"] == 300
+    assert resg["dt"] == pytest.approx(reso["dt"], rel=1e-12)
+    assert resg["alpha"] == pytest.approx(reso["alpha"], rel=1e-12)
+    np.testing.assert_allclose(eg, eo, rtol=1e-9, atol=1e-9)
+    assert e < 1e-8
+    assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-10)
+    # the relaxation lowered Omega
+    assert eg[-1] < eg[0]
+
+
+def test_batch_members_match_oracle():
+    import classicaldft_amd as cd
+
+    specs = [(48, 0.05, 0.8, 3), (64, 0.02, 0.9, 5), (80, 0.6, 1.1, 7)]
+    ctxs, orcs, rho0s, mus = [], [], [], []
+    for n, rv, kT, seed in specs:
+        shape, dx = (n, n, n), 0.1
+        kw = dict(interactions=[W.LJ_WCA], kT=kT, model="WhiteBearII")
+        ctx = _ctx(shape, dx, **kw)
+        ctxs.append(ctx)
+        orcs.append(_orc(shape, dx, **kw))
+        rho0s.append(W.droplet(shape, dx, radius=0.25 * n * dx, width=0.5, rho_l=0.7 if rv < 0.4 else 0.05,
+                               rho_v=rv, noise=1e-3, seed=seed))
+        mus.append(ctx.bulk_mu(rv))
+    assert [c.schedule for c in ctxs] == ["general", "fused", "general"]
+    fire = dict(force_tolerance=1e-12)
+    steps = 20
+    out = cd.minimize_grand_potential_batch(ctxs, rho0s, mus, fire=fire, max_steps=steps)
+    for (rb, resb, st), orc, r0, mu, (n, *_) in zip(out, orcs, rho0s, mus, specs):
+        assert st == 0
+        ro, reso, _ = orc.minimize_grand_potential(r0, mu, fire=fire, max_steps=steps)
+        e = _rel(rb, ro)
+        print(f"batch member {n}^3: {resb['iterations']} steps, Omega {resb['grand_potential']:.12e} vs "
+              f"{reso['grand_potential']:.12e}, profile rel err {e:.2e}", flush=True)
+        assert resb["iterations"] == reso["iterations"] == steps
+        assert resb["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-10)
+        assert resb["dt"] == pytest.approx(reso["dt"], rel=1e-12)
+        assert e < 1e-9
+
+
+@pytest.mark.parametrize("shape,precision,tol", [((64, 64, 64), "fp64", 1e-10), ((64, 64, 64), "mixed", 2e-5),
+                                                 ((256, 64, 64), "mixed", 2e-5)])
+def test_ddft_on_fused_grid_matches_oracle(shape, precision, tol):
+    dx = 0.1
+    kw = dict(interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    ctx = _ctx(shape, dx, precision=precision, **kw)
+    assert ctx.schedule == "fused"
+    orc = _orc(shape, dx, **kw)
+    slab = W.slab(shape, dx, fraction=0.4, width=0.5, rho_l=0.7, rho_v=0.02, noise=1e-3)
+    mu = orc.bulk_mu(0.02)
+    steps = 3
+    rg, resg, eg = ctx.ddft_run(slab, mu, steps, dt=1e-4, dt_max=1e-4)
+    ro, reso, eo = orc.ddft_run(slab, mu, steps, dt=1e-4, dt_max=1e-4)
+    e = _rel(rg, ro)
+    drift = abs(resg["mass_final"] - resg["mass_initial"]) / resg["mass_initial"]
+    print(f"ddft {shape} {precision}: profile rel err {e:.2e}, mass drift {drift:.2e}", flush=True)
+    assert e < tol
+    np.testing.assert_allclose(eg, eo, rtol=tol * 10, atol=1e-9)
+    assert resg["time"] == pytest.approx(reso["time"], rel=1e-12)
+    assert drift < 1e-12
