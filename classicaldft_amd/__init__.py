@@ -275,6 +275,17 @@ class Context:
         except Exception:
             pass
 
+    def _field(self, a, name, species=1):
+        """`a` as a contiguous float64 vector of species * N values; ValueError for any other size (the
+        native entry points read and write exactly that many values)."""
+        v = _f64(a)
+        if v.size != species * self.N:
+            raise ValueError(f"{name} has {v.size} values, expected {species} x {self.N} (grid {self.shape})")
+        return v
+
+    def _opt(self, a, name, species=1):
+        return None if a is None else self._field(a, name, species)
+
     def a_vdw(self, k=0):
         v = C.c_double()
         _check(lib().cdft_a_vdw(self._h, k, C.byref(v)))
@@ -289,10 +300,8 @@ class Context:
         """The terms of functionals::total separately: (result dict, {"ideal", "external", "hard_sphere",
         "mean_field"} force arrays of shape (n_species, *shape))."""
         S = self.n_species
-        rho = _f64(rho)
-        if rho.size != S * self.N:
-            raise ValueError(f"rho has {rho.size} values, expected {S} x {self.N}")
-        v = None if vext is None else _f64(vext)
+        rho = self._field(rho, "rho", S)
+        v = self._opt(vext, "vext", S)
         m = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (S,)))
         f = {k: np.zeros(S * self.N) for k in ("ideal", "external", "hard_sphere", "mean_field")}
         r = _Result()
@@ -337,8 +346,8 @@ class Context:
 
     def total(self, rho, mu, vext=None, forces=True):
         """Return (result dict, forces or None)."""
-        rho = _f64(rho)
-        v = None if vext is None else _f64(vext)
+        rho = self._field(rho, "rho")
+        v = self._opt(vext, "vext")
         f = np.zeros(self.N) if forces else None
         r = _Result()
         _check(lib().cdft_total(self._h, _p(rho), _p(v), float(mu), C.byref(r), _p(f)))
@@ -349,12 +358,8 @@ class Context:
         """functionals::total for S species: rho and vext are (S, *shape), mu has S
         values.  Return (result dict, forces (S, *shape) or None)."""
         S = self.n_species
-        rho = _f64(rho)
-        if rho.size != S * self.N:
-            raise ValueError(f"rho has {rho.size} values, expected {S} x {self.N}")
-        v = None if vext is None else _f64(vext)
-        if v is not None and v.size != S * self.N:
-            raise ValueError("vext size does not match the species and grid")
+        rho = self._field(rho, "rho", S)
+        v = self._opt(vext, "vext", S)
         m = np.ascontiguousarray(np.broadcast_to(np.asarray(mu, dtype=np.float64), (S,)))
         f = np.zeros(S * self.N) if forces else None
         r = _Result()
@@ -408,16 +413,18 @@ class Context:
     def weighted_densities(self, rho):
         """Weighted densities: 11 FMT channels (when an FMT model is set) then the mean-field convolution."""
         out = np.zeros(max(self.nf, 1) * self.N)
-        _check(lib().cdft_weighted_densities(self._h, _p(_f64(rho)), _p(out)))
+        _check(lib().cdft_weighted_densities(self._h, _p(self._field(rho, "rho")), _p(out)))
         return out[: self.nf * self.N].reshape((self.nf,) + self.shape)
 
     def r2c(self, x):
         out = np.zeros(2 * self.M)
-        _check(lib().cdft_fft_r2c(self._h, _p(_f64(x)), _p(out)))
+        _check(lib().cdft_fft_r2c(self._h, _p(self._field(x, "x")), _p(out)))
         return (out[0::2] + 1j * out[1::2]).reshape(self.shape[0], self.shape[1], -1)
 
     def c2r(self, X):
         X = np.ascontiguousarray(X, dtype=np.complex128).reshape(-1)
+        if X.size != self.M:
+            raise ValueError(f"spectrum has {X.size} values, expected {self.M}")
         buf = np.empty(2 * X.size)
         buf[0::2] = X.real
         buf[1::2] = X.imag
@@ -439,7 +446,8 @@ class Context:
         res = _MinResult()
         en = np.full(int(f["max_steps"]) + 1, np.nan) if record_energies else None
         _check(lib().cdft_minimize_grand_potential(self._h, C.byref(fs), C.byref(prm), int(bool(homogeneous_boundary)),
-                                                   _p(_f64(rho0)), None if vext is None else _p(_f64(vext)), float(mu),
+                                                   _p(self._field(rho0, "rho0")), _p(self._opt(vext, "vext")),
+                                                   float(mu),
                                                    _p(out), C.byref(res), _p(en)))
         r = {k: getattr(res, k) for k, _ in _MinResult._fields_}
         r["converged"] = bool(r["converged"])
@@ -460,7 +468,7 @@ class Context:
         res = _MinResult()
         en = np.full(int(f["max_steps"]) + 1, np.nan) if record_energies else None
         _check(lib().cdft_minimize_fixed_mass(self._h, C.byref(fs), C.byref(prm), int(bool(homogeneous_boundary)),
-                                              _p(_f64(rho0)), None if vext is None else _p(_f64(vext)), float(mu),
+                                              _p(self._field(rho0, "rho0")), _p(self._opt(vext, "vext")), float(mu),
                                               float(target_mass), _p(out), C.byref(res), _p(en)))
         r = {k: getattr(res, k) for k, _ in _MinResult._fields_}
         r["converged"] = bool(r["converged"])
@@ -470,10 +478,10 @@ class Context:
                  min_density=1e-18, vext=None, record_energies=True):
         cfg = _StepConfig(float(dt), 1.0, float(min_density), float(dt_max), float(fp_tolerance),
                           int(fp_max_iterations))
-        rho = _f64(rho0).copy()
+        rho = self._field(rho0, "rho0").copy()
         res = _SimResult()
         en = np.zeros(n_steps + 1) if record_energies else None
-        _check(lib().cdft_ddft_run(self._h, C.byref(cfg), int(n_steps), None if vext is None else _p(_f64(vext)),
+        _check(lib().cdft_ddft_run(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
                                    float(mu), _p(rho), C.byref(res), _p(en), None))
         r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
         return rho.reshape(self.shape), r, en
@@ -485,9 +493,9 @@ class Context:
         cfg = _EigenConfig(float(tolerance), int(max_iterations), float(hessian_eps), int(bool(fixed_boundary)))
         res = _EigenResult()
         vec = np.zeros(self.N)
-        _check(lib().cdft_smallest_eigenpair(self._h, C.byref(cfg), _p(_f64(rho)),
-                                             None if vext is None else _p(_f64(vext)), float(mu),
-                                             None if initial_guess is None else _p(_f64(initial_guess)),
+        _check(lib().cdft_smallest_eigenpair(self._h, C.byref(cfg), _p(self._field(rho, "rho")),
+                                             _p(self._opt(vext, "vext")), float(mu),
+                                             _p(self._opt(initial_guess, "initial_guess")),
                                              int(seed), _p(vec), C.byref(res)))
         r = {k: getattr(res, k) for k, _ in _EigenResult._fields_}
         r["converged"] = bool(r["converged"])
@@ -509,7 +517,7 @@ class Context:
                             int(max_outer_iterations), int(eigenvalue_clear_count))
         res = _SaddleResult()
         ro, eo = np.zeros(self.N), np.zeros(self.N)
-        _check(lib().cdft_saddle_search(self._h, C.byref(cfg), _p(_f64(rho0)), None if vext is None else _p(_f64(vext)),
+        _check(lib().cdft_saddle_search(self._h, C.byref(cfg), _p(self._field(rho0, "rho0")), _p(self._opt(vext, "vext")),
                                         float(mu), float(target_mass), int(seed), _p(ro), _p(eo), C.byref(res)))
         r = {k: getattr(res, k) for k, _ in _SaddleResult._fields_}
         r["converged"] = bool(r["converged"])
@@ -519,10 +527,10 @@ class Context:
                             vext=None):
         """split_operator_step, n_steps times; returns (rho, energy)."""
         cfg = _StepConfig(float(dt), float(diffusion_coefficient), float(min_density), float(dt), 1e-4, 100)
-        rho = _f64(rho0).copy()
+        rho = self._field(rho0, "rho0").copy()
         e = C.c_double()
-        _check(lib().cdft_ddft_split_operator(self._h, C.byref(cfg), int(n_steps), None if vext is None else
-                                              _p(_f64(vext)), float(mu), _p(rho), C.byref(e)))
+        _check(lib().cdft_ddft_split_operator(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
+                                              float(mu), _p(rho), C.byref(e)))
         return rho.reshape(self.shape), e.value
 
     def ddft_crank_nicolson(self, rho0, mu, n_steps, dt=1e-4, diffusion_coefficient=1.0, min_density=1e-18,
@@ -530,10 +538,10 @@ class Context:
         """crank_nicholson_step, n_steps times; returns (rho, energy, iterations of the last step)."""
         cfg = _StepConfig(float(dt), float(diffusion_coefficient), float(min_density), float(dt), float(fp_tolerance),
                           int(fp_max_iterations))
-        rho = _f64(rho0).copy()
+        rho = self._field(rho0, "rho0").copy()
         e, it = C.c_double(), C.c_int()
-        _check(lib().cdft_ddft_crank_nicolson(self._h, C.byref(cfg), int(n_steps), None if vext is None else
-                                              _p(_f64(vext)), float(mu), _p(rho), C.byref(e), C.byref(it)))
+        _check(lib().cdft_ddft_crank_nicolson(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
+                                              float(mu), _p(rho), C.byref(e), C.byref(it)))
         return rho.reshape(self.shape), e.value, it.value
 
     def picard(self, rho0, mu, mixing=0.01, min_density=1e-30, tolerance=1e-6, max_iterations=5000, target_mass=0.0,
@@ -541,9 +549,9 @@ class Context:
         """Picard::solve; returns (rho, result dict)."""
         cfg = _Picard(float(mixing), float(min_density), float(tolerance), int(max_iterations))
         res = _PicardResult()
-        rho = _f64(rho0).copy()
-        _check(lib().cdft_picard_solve(self._h, C.byref(cfg), None if vext is None else _p(_f64(vext)), float(mu),
-                                       float(target_mass), None if background is None else _p(_f64(background)),
+        rho = self._field(rho0, "rho0").copy()
+        _check(lib().cdft_picard_solve(self._h, C.byref(cfg), _p(self._opt(vext, "vext")), float(mu),
+                                       float(target_mass), _p(self._opt(background, "background")),
                                        _p(rho), C.byref(res)))
         r = {k: getattr(res, k) for k, _ in _PicardResult._fields_}
         r["converged"] = bool(r["converged"])
@@ -560,7 +568,7 @@ class Context:
     def hessian_vector(self, rho, u):
         """H(rho) u, the Jacobian of the forces of total() applied to u."""
         out = np.zeros(self.N)
-        _check(lib().cdft_hessian_vector(self._h, _p(_f64(rho)), _p(_f64(u)), _p(out)))
+        _check(lib().cdft_hessian_vector(self._h, _p(self._field(rho, "rho")), _p(self._field(u, "u")), _p(out)))
         return out.reshape(self.shape)
 
     def wells_potential(self, params):
@@ -573,9 +581,9 @@ class Context:
     def ddft_fixed_run(self, rho0, mu, n_steps, dt, fp_iterations, vext=None):
         """Fixed-dt DDFT map with exactly fp_iterations iterations per step; returns (rho, min density)."""
         cfg = _AdjointConfig(float(dt), int(fp_iterations), int(n_steps), 0)
-        rho = _f64(rho0).copy()
+        rho = self._field(rho0, "rho0").copy()
         mn = C.c_double(0.0)
-        _check(lib().cdft_ddft_fixed_run(self._h, C.byref(cfg), None if vext is None else _p(_f64(vext)), float(mu),
+        _check(lib().cdft_ddft_fixed_run(self._h, C.byref(cfg), _p(self._opt(vext, "vext")), float(mu),
                                          _p(rho), C.byref(mn)))
         return rho.reshape(self.shape), mn.value
 
@@ -589,7 +597,8 @@ class Context:
         grad = np.zeros(prm.size)
         fields = [np.zeros(self.N) for _ in range(3)] if want_fields else [None, None, None]
         _check(lib().cdft_ddft_adjoint_wells(self._h, C.byref(cfg), prm.shape[0], _p(prm.reshape(-1)), float(mu),
-                                             _p(_f64(rho0)), _p(_f64(target)), C.byref(res), _p(grad),
+                                             _p(self._field(rho0, "rho0")), _p(self._field(target, "target")),
+                                             C.byref(res), _p(grad),
                                              *[_p(f) for f in fields]))
         r = {k: getattr(res, k) for k, _ in _AdjointResult._fields_}
         out = [grad.reshape(prm.shape), r]
