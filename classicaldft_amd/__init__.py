@@ -28,7 +28,7 @@ EXPORTED = [
     "cdft_species_count", "cdft_total_species", "cdft_total_species_device", "cdft_time_total_species_device",
     "cdft_bulk_free_energy_density", "cdft_bulk_chemical_potentials", "cdft_bulk_pressure", "cdft_contributions",
     "cdft_bulk_batch",
-    "cdft_weighted_densities", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
+    "cdft_weighted_densities", "cdft_weighted_densities_noalloc", "cdft_fft_r2c", "cdft_fft_c2r", "cdft_minimize_grand_potential", "cdft_ddft_run",
     "cdft_time_total_device", "cdft_profile_stages", "cdft_stage_name", "cdft_schedule",
     "cdft_minimize_grand_potential_batch", "cdft_hessian_vector", "cdft_wells_potential", "cdft_ddft_fixed_run",
     "cdft_ddft_adjoint_wells", "cdft_autotune", "cdft_minimize_fixed_mass", "cdft_smallest_eigenpair",
@@ -169,6 +169,7 @@ def lib():
         L.cdft_total_device_repeat.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int,
                                                C.POINTER(_Result), C.c_void_p]
         L.cdft_weighted_densities.argtypes = [C.c_void_p, _dp, _dp]
+        L.cdft_weighted_densities_noalloc.argtypes = [C.c_void_p, _dp, _dp]
         L.cdft_fft_r2c.argtypes = [C.c_void_p, _dp, _dp]
         L.cdft_fft_c2r.argtypes = [C.c_void_p, _dp, _dp]
         L.cdft_minimize_grand_potential.argtypes = [C.c_void_p, C.POINTER(_Fire), C.POINTER(_Param), C.c_int, _dp,
@@ -410,10 +411,12 @@ class Context:
                                          C.byref(n)))
         return [(lib().cdft_stage_name(self._h, i).decode(), ms[i], by[i]) for i in range(n.value)]
 
-    def weighted_densities(self, rho):
-        """Weighted densities: 11 FMT channels (when an FMT model is set) then the mean-field convolution."""
+    def weighted_densities(self, rho, noalloc=False):
+        """Weighted densities: 11 FMT channels (when an FMT model is set) then the mean-field convolution.
+        noalloc: through cdft_weighted_densities_noalloc (no device allocation; the library's workspace)."""
         out = np.zeros(max(self.nf, 1) * self.N)
-        _check(lib().cdft_weighted_densities(self._h, _p(self._field(rho, "rho")), _p(out)))
+        fn = lib().cdft_weighted_densities_noalloc if noalloc else lib().cdft_weighted_densities
+        _check(fn(self._h, _p(self._field(rho, "rho")), _p(out)))
         return out[: self.nf * self.N].reshape((self.nf,) + self.shape)
 
     def r2c(self, x):

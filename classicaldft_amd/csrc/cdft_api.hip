@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,6 +72,32 @@ void reference_to_spectrum(Context* c, const double* in) {
   }
 }
 
+// Staging of cdft_weighted_densities_noalloc: the n_fields * N doubles of the
+// weighted densities land here between the kernels and the copy to the host.
+// One copy per device, held for the life of the process.  In host code the
+// array's name is not a device address: kernels and copies take the address
+// resolved by scratch_on() for the current device.
+constexpr size_t kScratchBytes = static_cast<size_t>(CDFT_NOALLOC_SCRATCH_BYTES);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of `scratch` on `device`, which must be the current device;
+// looked up once per device (first at context creation) and kept.
+void* scratch_on(int device) {
+  if (device < 0 || device >= kMaxScratchDevices) throw std::invalid_argument("device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    CDFT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw HipError("hipGetSymbolAddress returned a null scratch address");
+    g_scratch_ptr[device] = p;
+  }
+  return g_scratch_ptr[device];
+}
+
 }  // namespace
 }  // namespace cdft
 
@@ -87,6 +114,14 @@ int cdft_ctx_create(const cdft_grid* grid, const double* diameters, int n_specie
     if (!out) throw std::invalid_argument("cdft_ctx_create: out is null");
     *out = nullptr;
     Context* c = create_context(grid, diameters, n_species, interactions, n_interactions, kT, options);
+    try {
+      // set-up of the no-allocation entry point's workspace on this device
+      CDFT_HIP(hipSetDevice(c->device));
+      (void)scratch_on(c->device);
+    } catch (...) {
+      delete c;
+      throw;
+    }
     *out = new cdft_ctx{c};
   });
 }
@@ -242,6 +277,26 @@ int cdft_weighted_densities(cdft_ctx* ctx, const double* rho, double* out) {
       throw;
     }
     CDFT_HIP(hipFree(d_out));
+  });
+}
+
+int cdft_weighted_densities_noalloc(cdft_ctx* ctx, const double* rho, double* out) {
+  return guarded([&] {
+    check_ctx(ctx);
+    if (!rho || !out) throw std::invalid_argument("cdft_weighted_densities_noalloc: null argument");
+    Context* c = ctx->impl;
+    const long n = c->g.N * c->cs.nf;
+    if (static_cast<size_t>(n) * sizeof(double) > kScratchBytes)
+      throw std::invalid_argument("cdft_weighted_densities_noalloc: " + std::to_string(c->cs.nf) + " fields of " +
+                                  std::to_string(c->g.N) + " points need " +
+                                  std::to_string(static_cast<size_t>(n) * sizeof(double)) +
+                                  " bytes, more than the " + std::to_string(kScratchBytes) +
+                                  "-byte workspace");
+    CDFT_HIP(hipSetDevice(c->device));
+    double* d_out = static_cast<double*>(scratch_on(c->device));
+    upload(c, c->d_rho, rho, c->g.N);
+    c->weighted_densities(c->d_rho, d_out);
+    if (n > 0) download(c, out, d_out, n);
   });
 }
 

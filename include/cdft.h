@@ -195,6 +195,17 @@ CDFT_API const char* cdft_stage_name(const cdft_ctx* ctx, int i);
  * mean-field convolution [w * rho] (host arrays, 12*N doubles). */
 CDFT_API int cdft_weighted_densities(cdft_ctx* ctx, const double* rho, double* out);
 
+/* cdft_weighted_densities without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory during
+ * a call (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The fields are staged in a library-wide device workspace
+ * of CDFT_NOALLOC_SCRATCH_BYTES per device, held until the process ends; a
+ * context whose n_fields * N doubles do not fit is refused with CDFT_EINVAL.
+ * Every call on one device shares that workspace, so calls that overlap on a
+ * device race: serialise them. */
+#define CDFT_NOALLOC_SCRATCH_BYTES (64L << 20)
+CDFT_API int cdft_weighted_densities_noalloc(cdft_ctx* ctx, const double* rho, double* out);
+
 /* math::FourierTransform::forward / backward (src/math/fourier.cpp:97-103):
  * unnormalised 3D r2c / c2r in the reference half-complex layout
  * idx = kz + (Nz/2+1)*(iy + Ny*ix), complex as interleaved (re, im). */
