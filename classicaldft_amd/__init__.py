@@ -89,7 +89,77 @@ class _StepConfig(C.Structure):
 
 class _SimResult(C.Structure):
     _fields_ = [("mass_initial", C.c_double), ("mass_final", C.c_double), ("time", C.c_double),
-                ("energy", C.c_double), ("dt", C.c_double), ("evaluations", C.c_long)]
+                ("energy", C.c_double), ("dt", C.c_double), ("evaluations", C.c_long), ("steps", C.c_int)]
+
+
+_FORCE_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double))
+_PROJECT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.POINTER(C.c_double))
+_STOP_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_int))
+_SNAPSHOT_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double))
+
+
+class _Hooks(C.Structure):
+    _fields_ = [("user", C.c_void_p), ("force", _FORCE_FN), ("project", _PROJECT_FN), ("stop", _STOP_FN),
+                ("snapshot", _SNAPSHOT_FN), ("snapshot_interval", C.c_int)]
+
+
+class _HostCallbacks:
+    """Python callables behind a cdft_hooks (the reference's ForceCallback, boundary and stop conditions,
+    snapshots and Picard constraints).  An exception inside a callback is kept and re-raised after the
+    native call has returned its error status."""
+
+    def __init__(self, shape, force=None, project=None, stop=None, snapshot=None, snapshot_interval=0):
+        self.shape, self.n = tuple(shape), int(np.prod(shape))
+        self.error = None
+        self.fns = {}
+
+        def guard(fn):
+            def wrapped(*a):
+                try:
+                    fn(*a)
+                    return 0
+                except BaseException as e:  # noqa: BLE001 - handed back to the caller after the native call
+                    self.error = e
+                    return 1
+            return wrapped
+
+        def arr(p):
+            return np.ctypeslib.as_array(p, shape=(self.n,))
+
+        if force is not None:
+            def _force(_u, rho, e, f):
+                en, fr = force(arr(rho).reshape(self.shape).copy())
+                fr = np.asarray(fr, dtype=np.float64).reshape(-1)
+                if fr.size != self.n:
+                    raise ValueError(f"force callback returned {fr.size} values, expected {self.n}")
+                e[0] = float(en)
+                arr(f)[:] = fr
+            self.fns["force"] = _FORCE_FN(guard(_force))
+        if project is not None:
+            def _project(_u, rho):
+                r = arr(rho)
+                out = project(r.reshape(self.shape).copy())
+                r[:] = np.asarray(out, dtype=np.float64).reshape(-1)
+            self.fns["project"] = _PROJECT_FN(guard(_project))
+        if stop is not None:
+            def _stop(_u, step, t, e, flag):
+                flag[0] = 1 if stop(int(step), float(t), float(e)) else 0
+            self.fns["stop"] = _STOP_FN(guard(_stop))
+        if snapshot is not None:
+            def _snapshot(_u, step, t, e, rho):
+                snapshot(int(step), float(t), float(e), arr(rho).reshape(self.shape).copy())
+            self.fns["snapshot"] = _SNAPSHOT_FN(guard(_snapshot))
+        self.struct = _Hooks(None, *[self.fns.get(k, C.cast(None, t)) for k, t in
+                                     (("force", _FORCE_FN), ("project", _PROJECT_FN), ("stop", _STOP_FN),
+                                      ("snapshot", _SNAPSHOT_FN))], int(snapshot_interval))
+
+    def ref(self):
+        return C.byref(self.struct) if self.fns else None
+
+    def check(self, rc):
+        if rc != 0 and self.error is not None:
+            raise self.error
+        _check(rc)
 
 
 class _EigenConfig(C.Structure):
@@ -179,8 +249,8 @@ def lib():
                                                           C.POINTER(_dp), C.POINTER(_MinResult), C.POINTER(C.c_int)]
         L.cdft_minimize_fixed_mass.argtypes = [C.c_void_p, C.POINTER(_Fire), C.POINTER(_Param), C.c_int, _dp, _dp,
                                                C.c_double, C.c_double, _dp, C.POINTER(_MinResult), _dp]
-        L.cdft_ddft_run.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp,
-                                    C.POINTER(_SimResult), _dp, _dp]
+        L.cdft_ddft_run.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, C.POINTER(_Hooks),
+                                    _dp, C.POINTER(_SimResult), _dp, _dp]
         L.cdft_time_total_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_void_p,
                                              _dp]
         L.cdft_profile_stages.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_int, _dp,
@@ -189,11 +259,12 @@ def lib():
         L.cdft_stage_name.argtypes = [C.c_void_p, C.c_int]
         L.cdft_schedule.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.cdft_autotune.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
-        L.cdft_ddft_split_operator.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp, _dp]
-        L.cdft_ddft_crank_nicolson.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double, _dp, _dp,
-                                               C.POINTER(C.c_int)]
-        L.cdft_picard_solve.argtypes = [C.c_void_p, C.POINTER(_Picard), _dp, C.c_double, C.c_double, _dp, _dp,
-                                        C.POINTER(_PicardResult)]
+        L.cdft_ddft_split_operator.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double,
+                                               C.POINTER(_Hooks), _dp, _dp]
+        L.cdft_ddft_crank_nicolson.argtypes = [C.c_void_p, C.POINTER(_StepConfig), C.c_int, _dp, C.c_double,
+                                               C.POINTER(_Hooks), _dp, _dp, C.POINTER(C.c_int)]
+        L.cdft_picard_solve.argtypes = [C.c_void_p, C.POINTER(_Picard), _dp, C.c_double, _dp, _dp, C.POINTER(_Hooks),
+                                        _dp, C.POINTER(_PicardResult)]
         L.cdft_saddle_search.argtypes = [C.c_void_p, C.POINTER(_SaddleConfig), _dp, _dp, C.c_double, C.c_double,
                                          C.c_ulonglong, _dp, _dp, C.POINTER(_SaddleResult)]
         L.cdft_smallest_eigenpair.argtypes = [C.c_void_p, C.POINTER(_EigenConfig), _dp, _dp, C.c_double, _dp,
@@ -478,16 +549,22 @@ class Context:
         return out.reshape(self.shape), r, (en[: r["iterations"] + 1] if en is not None else None)
 
     def ddft_run(self, rho0, mu, n_steps, dt=1e-4, dt_max=1.0, fp_tolerance=1e-4, fp_max_iterations=100,
-                 min_density=1e-18, vext=None, record_energies=True):
+                 min_density=1e-18, vext=None, record_energies=True, force=None, boundary=None, stop=None,
+                 snapshot=None, snapshot_interval=0):
+        """Simulation::run (integrating-factor DDFT).  force(rho) -> (energy, forces) replaces the context's
+        functional; boundary(rho) -> rho is applied to the initial state and after every step;
+        stop(step, time, energy) -> bool ends the run; snapshot(step, time, energy, rho) receives the density
+        at step 0 and every snapshot_interval steps.  Returns (rho, result dict, energies of the steps taken)."""
         cfg = _StepConfig(float(dt), 1.0, float(min_density), float(dt_max), float(fp_tolerance),
                           int(fp_max_iterations))
         rho = self._field(rho0, "rho0").copy()
         res = _SimResult()
         en = np.zeros(n_steps + 1) if record_energies else None
-        _check(lib().cdft_ddft_run(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
-                                   float(mu), _p(rho), C.byref(res), _p(en), None))
+        cb = _HostCallbacks(self.shape, force, boundary, stop, snapshot, snapshot_interval)
+        cb.check(lib().cdft_ddft_run(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
+                                     float(mu), cb.ref(), _p(rho), C.byref(res), _p(en), None))
         r = {k: getattr(res, k) for k, _ in _SimResult._fields_}
-        return rho.reshape(self.shape), r, en
+        return rho.reshape(self.shape), r, (en[: r["steps"] + 1] if en is not None else None)
 
     def smallest_eigenpair(self, rho, mu, initial_guess=None, seed=0, tolerance=1e-4, max_iterations=300,
                            hessian_eps=1e-6, fixed_boundary=False, vext=None):
@@ -527,35 +604,42 @@ class Context:
         return ro.reshape(self.shape), eo.reshape(self.shape), r
 
     def ddft_split_operator(self, rho0, mu, n_steps, dt=1e-4, diffusion_coefficient=1.0, min_density=1e-18,
-                            vext=None):
-        """split_operator_step, n_steps times; returns (rho, energy)."""
+                            vext=None, force=None):
+        """split_operator_step, n_steps times (force(rho) -> (energy, forces) replaces the functional);
+        returns (rho, energy)."""
         cfg = _StepConfig(float(dt), float(diffusion_coefficient), float(min_density), float(dt), 1e-4, 100)
         rho = self._field(rho0, "rho0").copy()
         e = C.c_double()
-        _check(lib().cdft_ddft_split_operator(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
-                                              float(mu), _p(rho), C.byref(e)))
+        cb = _HostCallbacks(self.shape, force)
+        cb.check(lib().cdft_ddft_split_operator(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
+                                                float(mu), cb.ref(), _p(rho), C.byref(e)))
         return rho.reshape(self.shape), e.value
 
     def ddft_crank_nicolson(self, rho0, mu, n_steps, dt=1e-4, diffusion_coefficient=1.0, min_density=1e-18,
-                            fp_tolerance=1e-4, fp_max_iterations=100, vext=None):
-        """crank_nicholson_step, n_steps times; returns (rho, energy, iterations of the last step)."""
+                            fp_tolerance=1e-4, fp_max_iterations=100, vext=None, force=None):
+        """crank_nicholson_step, n_steps times (force as for ddft_split_operator); returns (rho, energy,
+        iterations of the last step)."""
         cfg = _StepConfig(float(dt), float(diffusion_coefficient), float(min_density), float(dt), float(fp_tolerance),
                           int(fp_max_iterations))
         rho = self._field(rho0, "rho0").copy()
         e, it = C.c_double(), C.c_int()
-        _check(lib().cdft_ddft_crank_nicolson(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
-                                              float(mu), _p(rho), C.byref(e), C.byref(it)))
+        cb = _HostCallbacks(self.shape, force)
+        cb.check(lib().cdft_ddft_crank_nicolson(self._h, C.byref(cfg), int(n_steps), _p(self._opt(vext, "vext")),
+                                                float(mu), cb.ref(), _p(rho), C.byref(e), C.byref(it)))
         return rho.reshape(self.shape), e.value, it.value
 
-    def picard(self, rho0, mu, mixing=0.01, min_density=1e-30, tolerance=1e-6, max_iterations=5000, target_mass=0.0,
-               background=None, vext=None):
-        """Picard::solve; returns (rho, result dict)."""
+    def picard(self, rho0, mu, mixing=0.01, min_density=1e-30, tolerance=1e-6, max_iterations=5000,
+               target_mass=None, background=None, vext=None, force=None, constraint=None):
+        """Picard::solve; target_mass (None: unconstrained) applies fixed_mass_constraint with `background`;
+        force(rho) -> (energy, forces) replaces the functional and constraint(rho) -> rho is applied after
+        every update.  Returns (rho, result dict)."""
         cfg = _Picard(float(mixing), float(min_density), float(tolerance), int(max_iterations))
         res = _PicardResult()
         rho = self._field(rho0, "rho0").copy()
-        _check(lib().cdft_picard_solve(self._h, C.byref(cfg), _p(self._opt(vext, "vext")), float(mu),
-                                       float(target_mass), _p(self._opt(background, "background")),
-                                       _p(rho), C.byref(res)))
+        tgt = None if target_mass is None else C.byref(C.c_double(float(target_mass)))
+        cb = _HostCallbacks(self.shape, force, constraint)
+        cb.check(lib().cdft_picard_solve(self._h, C.byref(cfg), _p(self._opt(vext, "vext")), float(mu), tgt,
+                                         _p(self._opt(background, "background")), cb.ref(), _p(rho), C.byref(res)))
         r = {k: getattr(res, k) for k, _ in _PicardResult._fields_}
         r["converged"] = bool(r["converged"])
         return rho.reshape(self.shape), r

@@ -9,6 +9,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "context.hpp"
@@ -159,7 +161,8 @@ __global__ void __launch_bounds__(256) k_ddft_update(const double* __restrict__ 
   for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * blockDim.x) {
     const double nv = raw[i] + shift;
-    mx = fmax(mx, fabs(d1[i] - nv));
+    // fmax drops a NaN operand: a non-finite iterate is reported as an infinite deviation
+    mx = fmax(mx, isfinite(nv) ? fabs(d1[i] - nv) : INFINITY);
     mn = fmin(mn, nv);
     d1[i] = nv;
   }
@@ -227,6 +230,61 @@ struct DevBuf {
   }
 };
 
+// ---------------------------------------------------------------- host callbacks
+// Runs the callbacks of a cdft_hooks (any force callable, boundary / stop
+// conditions, snapshots and constraints, as the reference's algorithms take
+// them: dynamics.hpp:21-22,545-567, picard.hpp:16-21) on host copies of
+// device fields.  A nonzero callback status aborts the entry point.
+struct HostHooks {
+  const cdft_hooks* h = nullptr;
+  std::vector<double> a, b;
+
+  bool force() const { return h && h->force; }
+  bool project() const { return h && h->project; }
+  bool stop() const { return h && h->stop; }
+  bool snapshot() const { return h && h->snapshot; }
+
+  void fetch(Context* c, const double* d) {
+    a.resize(static_cast<size_t>(c->g.N));
+    CDFT_HIP(hipMemcpyAsync(a.data(), d, 8 * c->g.N, hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+  }
+  void store(Context* c, double* d, const std::vector<double>& v) {
+    CDFT_HIP(hipMemcpyAsync(d, v.data(), 8 * c->g.N, hipMemcpyHostToDevice, c->stream));
+    CDFT_HIP(hipStreamSynchronize(c->stream));
+  }
+  // energy and forces of the force callback at d_rho (forces to d_forces)
+  double eval(Context* c, const double* d_rho, double* d_forces) {
+    fetch(c, d_rho);
+    b.assign(static_cast<size_t>(c->g.N), 0.0);
+    double e = 0.0;
+    if (h->force(h->user, a.data(), &e, b.data()) != 0) throw std::runtime_error("the force callback failed");
+    if (d_forces) store(c, d_forces, b);
+    return e;
+  }
+  // the boundary condition / constraint on d_rho, in place
+  void apply_project(Context* c, double* d_rho) {
+    fetch(c, d_rho);
+    if (h->project(h->user, a.data()) != 0) throw std::runtime_error("the boundary / constraint callback failed");
+    store(c, d_rho, a);
+  }
+  bool should_stop(int step, double time, double e) const {
+    int flag = 0;
+    if (h->stop(h->user, step, time, e, &flag) != 0) throw std::runtime_error("the stop callback failed");
+    return flag != 0;
+  }
+  void take_snapshot(Context* c, int step, double time, double e, const double* d_rho) {
+    fetch(c, d_rho);
+    if (h->snapshot(h->user, step, time, e, a.data()) != 0) throw std::runtime_error("the snapshot callback failed");
+  }
+};
+
+// Restart limit of one integrating-factor step: the reference divides dt by
+// 10 on every restart with no floor (dynamics.hpp:374-392); here a step that
+// restarts more than this many times fails instead of running on with a
+// vanishing dt.
+constexpr int kMaxRestarts = 30;
+
 // ------------------------------------------------------------------ DDFT
 struct DdftRun {
   Context* c;
@@ -238,6 +296,7 @@ struct DdftRun {
   std::vector<double> lam_h;
   double st_dt = 0.0;
   long evals = 0;
+  HostHooks* hooks = nullptr;  // a host force callback in place of the context's functional
 
   DdftRun(Context* c_, const double* vx, double m)
       : c(c_), d_vext(vx), mu(m), rho0(8 * c_->g.N), d1(8 * c_->g.N), f0(8 * c_->g.N), f1(8 * c_->g.N),
@@ -266,8 +325,9 @@ struct DdftRun {
   }
 
   double compute(const double* rho, double* forces) {
-    c->eval(rho, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
     ++evals;
+    if (hooks && hooks->force()) return hooks->eval(c, rho, forces);
+    c->eval(rho, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
     double er[8];
     CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
     CDFT_HIP(hipStreamSynchronize(c->stream));
@@ -340,6 +400,7 @@ struct DdftRun {
     rhs_forward<R>(rho0.as<double>(), f0.as<double>(), SR0.as<cx<R>>(), 1, true);
     forward<R>(rho0.as<double>(), Srho0.as<cx<R>>(), 0, true);
     bool restart;
+    int restarts = 0;
     do {
       restart = false;
       update_timestep(*cfg_dt);
@@ -354,6 +415,9 @@ struct DdftRun {
         const double old = deviation;
         double mn = 0.0;
         propagate<R>(first, &deviation, &mn);
+        if (!std::isfinite(deviation) || !std::isfinite(mn))
+          throw std::runtime_error("integrating_factor_step: the iterate is not finite (dt = " +
+                                   std::to_string(*cfg_dt) + ")");
         if (mn < 0.0 || (it > 0 && deviation > old)) {
           restart = true;
           break;
@@ -361,7 +425,12 @@ struct DdftRun {
         if (deviation < tol) break;
       }
       if (!restart && deviation > tol) restart = true;
-      if (restart) *cfg_dt /= 10.0;
+      if (restart) {
+        if (++restarts > kMaxRestarts)
+          throw std::runtime_error("integrating_factor_step: no convergence after " + std::to_string(kMaxRestarts) +
+                                   " restarts (dt = " + std::to_string(*cfg_dt) + ")");
+        *cfg_dt /= 10.0;
+      }
     } while (restart);
     const double e = compute(d1.as<double>(), f1.as<double>());
     CDFT_HIP(hipMemcpyAsync(rho0.p, d1.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));

@@ -770,7 +770,7 @@ int cdft_minimize_grand_potential_batch(int n, cdft_ctx* const* ctxs, const cdft
 }
 
 int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
-                  double* rho, cdft_sim_result* result, double* energies, double* times) {
+                  const cdft_hooks* hooks, double* rho, cdft_sim_result* result, double* energies, double* times) {
   return guarded([&] {
     check_ctx(ctx);
     if (!config || !rho || !result || n_steps < 0) throw std::invalid_argument("bad argument");
@@ -778,8 +778,12 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
     CDFT_HIP(hipSetDevice(c->device));
     const Geom& g = c->g;
     if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    HostHooks hk{hooks};
     DdftRun run(c, vext ? c->d_vext : nullptr, mu);
+    run.hooks = &hk;
     CDFT_HIP(hipMemcpyAsync(run.rho0.p, rho, 8 * g.N, hipMemcpyHostToDevice, c->stream));
+    // the boundary condition on the initial state (dynamics.hpp:586-588)
+    if (hk.project()) hk.apply_project(c, run.rho0.as<double>());
     // mass (dynamics.hpp:591-596)
     const AliasMap id{0, 0.0, 0.0};
     auto mass_of = [&](const double* d) {
@@ -794,14 +798,25 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
     double e = run.compute(run.rho0.as<double>(), run.f0.as<double>());
     if (energies) energies[0] = e;
     if (times) times[0] = 0.0;
+    if (hk.snapshot()) hk.take_snapshot(c, 0, 0.0, e, run.rho0.as<double>());  // dynamics.hpp:603-608
     double dt = config->dt;
-    int successes = 0;
+    int successes = 0, steps = 0;
     double time = 0.0;
+    bool forces_stale = false;  // the boundary condition changed the density after its forces were taken
     for (int s = 1; s <= n_steps; ++s) {
+      if (forces_stale) {
+        run.compute(run.rho0.as<double>(), run.f0.as<double>());  // the step's first compute (dynamics.hpp:345)
+        forces_stale = false;
+      }
       const double before = dt;
       e = c->precision == CDFT_PRECISION_MIXED ? run.step<float>(&dt, config->fp_tolerance, config->fp_max_iterations)
                                                : run.step<double>(&dt, config->fp_tolerance, config->fp_max_iterations);
       time += dt;
+      steps = s;
+      if (hk.project()) {  // dynamics.hpp:629-631
+        hk.apply_project(c, run.rho0.as<double>());
+        forces_stale = true;
+      }
       // adaptive timestep (dynamics.hpp:640-649)
       if (dt < before) successes = 0;
       else ++successes;
@@ -811,6 +826,9 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
       }
       if (energies) energies[s] = e;
       if (times) times[s] = time;
+      if (hk.snapshot() && hooks->snapshot_interval > 0 && s % hooks->snapshot_interval == 0)  // :668-675
+        hk.take_snapshot(c, s, time, e, run.rho0.as<double>());
+      if (hk.stop() && hk.should_stop(s, time, e)) break;  // dynamics.hpp:677-682
     }
     CDFT_HIP(hipMemcpyAsync(rho, run.rho0.p, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
     CDFT_HIP(hipStreamSynchronize(c->stream));
@@ -820,6 +838,7 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
     result->energy = e;
     result->dt = dt;
     result->evaluations = run.evals;
+    result->steps = steps;
   });
 }
 

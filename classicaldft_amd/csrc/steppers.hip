@@ -5,7 +5,9 @@
 //   Picard::solve         (include/dft/algorithms/picard.hpp:63-124) with the
 //                         optional fixed_mass_constraint (minimization.hpp:157-180)
 // The force callback is the functional's grand-potential gradient
-// (cdft_total's forces at mu, optional external field).  Transforms are the
+// (cdft_total's forces at mu, optional external field) or a host callback
+// (cdft_hooks::force); Picard's constraint may also be a host callback
+// (cdft_hooks::project).  Transforms are the
 // FP64 general-schedule r2c / c2r; k^2 is the continuous |k|^2 of
 // compute_k_squared (dynamics.hpp:310-318, Nyquist bin positive).
 #include <hip/hip_runtime.h>
@@ -130,6 +132,7 @@ struct Stepper {
   long N;
   DevBuf rho, d1, f, rhs, raw, S0, SR0, SR1, W, part, red;
   long evals = 0;
+  HostHooks* hooks = nullptr;  // a host force callback in place of the context's functional
 
   Stepper(Context* c_, const double* vx, double m)
       : c(c_), d_vext(vx), mu(m), N(c_->g.N), rho(8 * N), d1(8 * N), f(8 * N), rhs(8 * N), raw(8 * N),
@@ -140,8 +143,9 @@ struct Stepper {
   dim3 sb() const { return dim3(vec_blocks(c->g.M)); }
 
   double compute(const double* d, double* forces) {
-    c->eval(d, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
     ++evals;
+    if (hooks && hooks->force()) return hooks->eval(c, d, forces);
+    c->eval(d, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
     cdft_result r{};
     c->fetch_result(mu, &r);
     return r.grand_potential;
@@ -215,6 +219,12 @@ void check_step_config(const cdft_step_config* cfg, int n_steps) {
   if (!(cfg->dt > 0.0)) throw std::invalid_argument("step config: dt must be positive");
 }
 
+// The split-operator and Crank-Nicolson steps take a force callback only.
+void check_force_hooks_only(const cdft_hooks* h) {
+  if (h && (h->project || h->stop || h->snapshot))
+    throw std::invalid_argument("this step takes a force callback only (no boundary, stop or snapshot callbacks)");
+}
+
 }  // namespace
 
 }  // namespace cdft
@@ -224,16 +234,19 @@ using namespace cdft;
 extern "C" {
 
 int cdft_ddft_split_operator(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext,
-                             double mu, double* rho, double* energy) {
+                             double mu, const cdft_hooks* hooks, double* rho, double* energy) {
   return guarded([&] {
     check_ctx(ctx);
     check_step_config(config, n_steps);
+    check_force_hooks_only(hooks);
     if (!rho) throw std::invalid_argument("null density");
     Context* c = ctx->impl;
     CDFT_HIP(hipSetDevice(c->device));
     const long N = c->g.N;
     if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * N, hipMemcpyHostToDevice, c->stream));
+    HostHooks hk{hooks};
     Stepper st(c, vext ? c->d_vext : nullptr, mu);
+    st.hooks = &hk;
     CDFT_HIP(hipMemcpyAsync(st.rho.p, rho, 8 * N, hipMemcpyHostToDevice, c->stream));
     double e = 0.0;
     for (int s = 0; s < n_steps; ++s)
@@ -245,16 +258,19 @@ int cdft_ddft_split_operator(cdft_ctx* ctx, const cdft_step_config* config, int 
 }
 
 int cdft_ddft_crank_nicolson(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext,
-                             double mu, double* rho, double* energy, int* iterations) {
+                             double mu, const cdft_hooks* hooks, double* rho, double* energy, int* iterations) {
   return guarded([&] {
     check_ctx(ctx);
     check_step_config(config, n_steps);
+    check_force_hooks_only(hooks);
     if (!rho) throw std::invalid_argument("null density");
     Context* c = ctx->impl;
     CDFT_HIP(hipSetDevice(c->device));
     const long N = c->g.N;
     if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * N, hipMemcpyHostToDevice, c->stream));
+    HostHooks hk{hooks};
     Stepper st(c, vext ? c->d_vext : nullptr, mu);
+    st.hooks = &hk;
     CDFT_HIP(hipMemcpyAsync(st.rho.p, rho, 8 * N, hipMemcpyHostToDevice, c->stream));
     double e = 0.0;
     int it = 0;
@@ -268,19 +284,26 @@ int cdft_ddft_crank_nicolson(cdft_ctx* ctx, const cdft_step_config* config, int 
   });
 }
 
-int cdft_picard_solve(cdft_ctx* ctx, const cdft_picard* config, const double* vext, double mu, double target_mass,
-                      const double* background, double* rho, cdft_picard_result* result) {
+int cdft_picard_solve(cdft_ctx* ctx, const cdft_picard* config, const double* vext, double mu,
+                      const double* target, const double* background, const cdft_hooks* hooks, double* rho,
+                      cdft_picard_result* result) {
   return guarded([&] {
     check_ctx(ctx);
     if (!config || !rho || !result || config->max_iterations < 0) throw std::invalid_argument("bad argument");
+    if (hooks && (hooks->stop || hooks->snapshot))
+      throw std::invalid_argument("Picard takes force and constraint callbacks only");
+    if (target && !std::isfinite(*target)) throw std::invalid_argument("Picard: the target mass is not finite");
     Context* c = ctx->impl;
     CDFT_HIP(hipSetDevice(c->device));
     const Geom& g = c->g;
     const long N = g.N;
     if (vext) CDFT_HIP(hipMemcpyAsync(c->d_vext, vext, 8 * N, hipMemcpyHostToDevice, c->stream));
+    HostHooks hk{hooks};
     Stepper st(c, vext ? c->d_vext : nullptr, mu);
+    st.hooks = &hk;
     CDFT_HIP(hipMemcpyAsync(st.rho.p, rho, 8 * N, hipMemcpyHostToDevice, c->stream));
-    const bool constrained = target_mass > 0.0;
+    const bool constrained = target != nullptr;
+    const double target_mass = constrained ? *target : 0.0;
     if (constrained && background) CDFT_HIP(hipMemcpyAsync(st.d1.p, background, 8 * N, hipMemcpyHostToDevice, c->stream));
     const double* bg = (constrained && background) ? st.d1.as<double>() : nullptr;
     double omega = 0.0, omega_prev = 0.0, residual = 0.0;
@@ -310,6 +333,7 @@ int cdft_picard_solve(cdft_ctx* ctx, const cdft_picard* config, const double* ve
                            target_mass, g.dv, config->min_density, N);
       }
       CDFT_HIP(hipGetLastError());
+      if (hk.project()) hk.apply_project(c, r);  // picard.hpp:113-115
     }
     CDFT_HIP(hipMemcpyAsync(rho, r, 8 * N, hipMemcpyDeviceToHost, c->stream));
     CDFT_HIP(hipStreamSynchronize(c->stream));

@@ -282,17 +282,48 @@ typedef struct {
 typedef struct {
   double mass_initial, mass_final, time, energy, dt;
   long evaluations;
+  int steps;  /* steps taken: n_steps, or fewer when a stop callback ended the run */
 } cdft_sim_result;
 
+/* Host callbacks of the algorithm entry points.  The reference's algorithms
+ * take any force callable (dynamics::ForceCallback, dynamics.hpp:21-22), a
+ * boundary condition and a stop condition (Simulation::boundary and
+ * stop_condition, dynamics.hpp:545-562,567) and record snapshots
+ * (dynamics.hpp:43-58); Picard takes any constraint (picard.hpp:21).  Every
+ * member is nullable.  Callbacks receive `user` and host arrays of N doubles
+ * valid for the duration of the call; a nonzero return aborts the entry point
+ * with CDFT_ERUNTIME.  The transforms, stencils and propagators stay on the
+ * device; only the fields the callbacks see travel to the host.
+ *   force:    the energy and the forces (delta(beta Omega)/delta rho * dV) at
+ *             rho; null: the context's functional at mu with vext.
+ *   project:  modifies rho in place: the boundary condition of a DDFT run
+ *             (applied to the initial state and after every step), or a
+ *             Picard constraint (applied after every update).
+ *   stop:     after every DDFT step; sets *stop nonzero to end the run there.
+ *   snapshot: receives the density after step 0 and after every
+ *             snapshot_interval steps (snapshot_interval <= 0: step 0 only). */
+typedef struct {
+  void* user;
+  int (*force)(void* user, const double* rho, double* energy, double* forces);
+  int (*project)(void* user, double* rho);
+  int (*stop)(void* user, int step, double time, double energy, int* stop);
+  int (*snapshot)(void* user, int step, double time, double energy, const double* rho);
+  int snapshot_interval;
+} cdft_hooks;
+
 /* Simulation::run with integrating_factor_step (dynamics.hpp:333-401,582-693)
- * for one species, force callback = grand_potential_callback(mu)
- * (functional.hpp:161-172), plus an optional external field.  rho is updated
- * in place; energies and times (each nullable, n_steps + 1 values) receive
- * the energy and the elapsed time after the initial state and after every
- * step (SimulationResult::energies / times, recorded there every
- * log_interval steps). */
+ * for one species.  The force callback is hooks->force, or (hooks or its
+ * force null) the context's grand_potential_callback(mu) (functional.hpp:161-172)
+ * plus an optional external field; hooks (nullable) may also carry the
+ * boundary condition (project), the stop condition and the snapshot receiver.
+ * rho is updated in place; energies and times (each nullable, n_steps + 1
+ * values) receive the energy and the elapsed time after the initial state
+ * and after every step taken (SimulationResult::energies / times, recorded
+ * there every log_interval steps).  A step that has to restart more than 30
+ * times (dt cut by 1e30), or whose iterate is not finite, fails with
+ * CDFT_ERUNTIME; the reference has no such floor and would keep dividing dt. */
 CDFT_API int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext, double mu,
-                  double* rho, cdft_sim_result* result, double* energies, double* times);
+                  const cdft_hooks* hooks, double* rho, cdft_sim_result* result, double* energies, double* times);
 
 /* Autotuner (no counterpart in the reference, which leaves FFT planning to
  * FFTW_ESTIMATE, src/math/fourier.cpp:34-57): for grids the fused kernels do
@@ -363,17 +394,19 @@ CDFT_API int cdft_saddle_search(cdft_ctx* ctx, const cdft_saddle_config* config,
  * times: half a step of exact diffusion exp(-D k^2 dt / 2) (continuous k^2,
  * compute_k_squared), a forward-Euler step of the excess term, another half
  * step; densities clamped at min_density after each part.  The force
- * callback is the grand-potential gradient at mu (vext nullable).  energy
+ * callback is hooks->force, or (hooks or its force null) the grand-potential
+ * gradient at mu (vext nullable); no other hook applies here.  energy
  * (nullable) receives the energy of the final density. */
 CDFT_API int cdft_ddft_split_operator(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext,
-                             double mu, double* rho, double* energy);
+                             double mu, const cdft_hooks* hooks, double* rho, double* energy);
 
 /* crank_nicholson_step (dynamics.hpp:464-538), n_steps times: the fixed-point
  * iteration rho1_k = ((1 - a) rho0_k + dt/2 (R0_k + R1_k)) / (1 + a),
  * a = D k^2 dt / 2, until the largest change is below fp_tolerance or
- * fp_max_iterations.  iterations (nullable) receives the last step's count. */
+ * fp_max_iterations.  Force callback as for cdft_ddft_split_operator.
+ * iterations (nullable) receives the last step's count. */
 CDFT_API int cdft_ddft_crank_nicolson(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, const double* vext,
-                             double mu, double* rho, double* energy, int* iterations);
+                             double mu, const cdft_hooks* hooks, double* rho, double* energy, int* iterations);
 
 /* picard::Picard (include/dft/algorithms/picard.hpp:25-33). */
 typedef struct {
@@ -387,13 +420,17 @@ typedef struct {
   int iterations, converged;
 } cdft_picard_result;
 
-/* Picard::solve (picard.hpp:63-124) with the grand-potential forces at mu:
- * rho *= exp(-mixing f / dV) until the rms force or the change of Omega is
- * below the tolerance.  target_mass > 0 applies fixed_mass_constraint
- * (minimization.hpp:157-180) after every update with background (nullable:
- * zero).  rho (N values) is updated in place. */
+/* Picard::solve (picard.hpp:63-124): rho *= exp(-mixing f / dV) until the
+ * rms force or the change of Omega is below the tolerance.  The forces are
+ * hooks->force, or (hooks or its force null) the grand-potential gradient at
+ * mu (vext nullable).  target_mass (nullable: no constraint) applies
+ * fixed_mass_constraint (minimization.hpp:157-180) after every update, for
+ * any target, positive, zero or negative, with background (nullable: zero);
+ * hooks->project (nullable) is then applied as a further constraint.  rho
+ * (N values) is updated in place. */
 CDFT_API int cdft_picard_solve(cdft_ctx* ctx, const cdft_picard* config, const double* vext, double mu,
-                      double target_mass, const double* background, double* rho, cdft_picard_result* result);
+                      const double* target_mass, const double* background, const cdft_hooks* hooks, double* rho,
+                      cdft_picard_result* result);
 
 /* ---------------------------------------------------------------- adjoint
  * The reference has no derivative of its dynamics; the entry points below

@@ -12,14 +12,20 @@
 // reference throws (std::invalid_argument for bad arguments, std::runtime_error
 // otherwise).  Header-only, C++17; link with -lcdft.
 //
-// Limits of the GPU path (std::invalid_argument when exceeded): one species,
-// at most one mean-field interaction, grid lengths with prime factors 2, 3, 5.
+// Limits of the GPU path (std::invalid_argument when exceeded): up to 8
+// species for functionals::total (the algorithms act on one species, as the
+// reference's do), grid lengths with prime factors 2, 3, 5.  The algorithms
+// take any force callable: the functional's own grand_potential_callback
+// evaluates on the device with no host round trip; another callable runs on
+// the host with the transforms, stencils and propagators on the device.
 #pragma once
 
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <exception>
 #include <functional>
+#include <limits>
 #include <memory>
 #include <optional>
 #include <stdexcept>
@@ -864,21 +870,142 @@ struct StepConfig {
   double fp_tolerance{1e-4};
   int fp_max_iterations{100};
 };
-// dynamics.hpp:54-61 (snapshots are not recorded by the device loop)
+// dynamics.hpp:43-49
+struct Snapshot {
+  int step{0};
+  double time{0.0};
+  double energy{0.0};
+  std::vector<vec> densities;
+};
+// dynamics.hpp:52-58
 struct SimulationResult {
   std::vector<vec> densities;
   std::vector<double> times;
   std::vector<double> energies;
+  std::vector<Snapshot> snapshots;
   double mass_initial{0.0};
   double mass_final{0.0};
 };
 using BoundaryCondition = std::function<void(std::vector<vec>&)>;
 using StopCondition = std::function<bool(int step, double time, double energy)>;
 
-// dynamics.hpp:564-590: Lutsko's integrating-factor scheme with fixed-point
-// iteration and adaptive dt, run on the device.  The force callback must be a
-// Functional::grand_potential_callback (the device loop evaluates that
-// functional itself); boundary and stop callbacks are not supported there.
+namespace detail {
+// The host side of a cdft_hooks: the reference's std::function callbacks
+// behind C function pointers.  An exception thrown by a callback is kept and
+// rethrown once the library call has returned its error status.
+struct Callbacks {
+  size_t n = 0;
+  const ForceCallback* force = nullptr;
+  const BoundaryCondition* boundary = nullptr;
+  const std::function<std::vector<vec>(const std::vector<vec>&)>* constraint = nullptr;
+  const StopCondition* stop = nullptr;
+  std::vector<Snapshot>* snapshots = nullptr;
+  std::exception_ptr error;
+
+  cdft_hooks hooks(int snapshot_interval = 0) {
+    cdft_hooks h{};
+    h.user = this;
+    if (force) h.force = &force_fn;
+    if (boundary || constraint) h.project = &project_fn;
+    if (stop) h.stop = &stop_fn;
+    if (snapshots) h.snapshot = &snapshot_fn;
+    h.snapshot_interval = snapshot_interval;
+    return h;
+  }
+  // the status of a library call; a callback's own exception comes first
+  void check(int rc) const {
+    if (rc != CDFT_OK && error) std::rethrow_exception(error);
+    ::dft::detail::check(rc);
+  }
+
+ private:
+  template <typename F>
+  static int guard(void* user, F&& fn) {
+    auto* self = static_cast<Callbacks*>(user);
+    try {
+      fn(*self);
+      return 0;
+    } catch (...) {
+      self->error = std::current_exception();
+      return 1;
+    }
+  }
+  static int force_fn(void* user, const double* rho, double* energy, double* forces) {
+    return guard(user, [&](Callbacks& c) {
+      std::vector<vec> d{vec(rho, rho + c.n)};
+      auto r = (*c.force)(d);
+      if (r.second.empty() || r.second[0].size() != c.n)
+        throw std::invalid_argument("the force callback returned forces of the wrong size");
+      *energy = r.first;
+      std::copy(r.second[0].begin(), r.second[0].end(), forces);
+    });
+  }
+  static int project_fn(void* user, double* rho) {
+    return guard(user, [&](Callbacks& c) {
+      std::vector<vec> d{vec(rho, rho + c.n)};
+      if (c.boundary) (*c.boundary)(d);
+      else d = (*c.constraint)(d);
+      if (d.size() != 1 || d[0].size() != c.n)
+        throw std::invalid_argument("the boundary condition or constraint changed the densities' shape");
+      std::copy(d[0].begin(), d[0].end(), rho);
+    });
+  }
+  static int stop_fn(void* user, int step, double time, double energy, int* flag) {
+    return guard(user, [&](Callbacks& c) { *flag = (*c.stop)(step, time, energy) ? 1 : 0; });
+  }
+  static int snapshot_fn(void* user, int step, double time, double energy, const double* rho) {
+    return guard(user, [&](Callbacks& c) {
+      c.snapshots->push_back(Snapshot{step, time, energy, {vec(rho, rho + c.n)}});
+    });
+  }
+};
+
+// A device context that only transforms (no FMT model, no interactions): the
+// grid's FFTs, stencils and propagators for steps driven by a host force
+// callback.
+inline std::shared_ptr<cdft_ctx> transform_context(const Grid& grid, int device = 0) {
+  cdft_grid g{grid.dx, {grid.box_size[0], grid.box_size[1], grid.box_size[2]}};
+  const double diameter = 1.0;
+  cdft_options o{};
+  o.fmt_model = CDFT_FMT_NONE;
+  o.esfmt_A = 1.0;
+  o.esfmt_B = 0.0;
+  o.precision = CDFT_PRECISION_FP64;
+  o.device = device;
+  o.schedule = CDFT_SCHEDULE_AUTO;
+  cdft_ctx* raw = nullptr;
+  ::dft::detail::check(cdft_ctx_create(&g, &diameter, 1, nullptr, 0, 1.0, &o, &raw));
+  return std::shared_ptr<cdft_ctx>(raw, cdft_ctx_destroy);
+}
+
+// The device context and chemical potential a force callback evaluates with:
+// the functional's own for its grand_potential_callback (no host callback),
+// a transform-only context of the grid otherwise (`cb` is then set to
+// receive the callable).
+inline std::shared_ptr<cdft_ctx> context_for(const ForceCallback& compute, const Grid& grid,
+                                             const std::vector<vec>& densities, Callbacks& cb, double* mu) {
+  if (!compute) throw std::invalid_argument("the force callback is empty");
+  if (densities.size() != 1 || static_cast<long>(densities[0].size()) != grid.total_points())
+    throw std::invalid_argument("the device steps take one density of the grid's size");
+  cb.n = densities[0].size();
+  if (const auto* g = compute.target<functionals::GrandPotentialCallback>()) {
+    if (g->functional->weights.n_points != grid.total_points())
+      throw std::invalid_argument("the functional's grid differs from the step's grid");
+    *mu = g->chemical_potential;
+    return g->functional->weights.ctx;
+  }
+  cb.force = &compute;
+  *mu = 0.0;
+  return transform_context(grid);
+}
+}  // namespace detail
+
+// dynamics.hpp:564-693: Lutsko's integrating-factor scheme with fixed-point
+// iteration and adaptive dt, with the stencils, transforms and propagator on
+// the device.  A Functional::grand_potential_callback is evaluated on the
+// device; any other force callback, the boundary condition and the stop
+// condition run on the host between device stages.  Snapshots are recorded
+// as there: the initial state, then every snapshot_interval steps.
 struct Simulation {
   StepConfig step;
   int n_steps{1000};
@@ -889,23 +1016,24 @@ struct Simulation {
   StopCondition stop_condition{};
 
   SimulationResult run(std::vector<vec> densities, const Grid& grid, const ForceCallback& force_fn) const {
-    const auto* cb = force_fn.target<functionals::GrandPotentialCallback>();
-    if (cb == nullptr)
-      throw std::invalid_argument("Simulation::run on the GPU needs a Functional::grand_potential_callback");
-    if (boundary || stop_condition)
-      throw std::invalid_argument("Simulation::run on the GPU supports neither boundary nor stop callbacks");
-    if (densities.size() != 1 || static_cast<long>(densities[0].size()) != grid.total_points())
-      throw std::invalid_argument("Simulation::run on the GPU takes one density of the grid's size");
+    if (n_steps < 0) throw std::invalid_argument("Simulation::run: negative n_steps");
+    SimulationResult r;
+    detail::Callbacks cb;
+    double mu = 0.0;
+    const auto ctx = detail::context_for(force_fn, grid, densities, cb, &mu);
+    if (boundary) cb.boundary = &boundary;
+    if (stop_condition) cb.stop = &stop_condition;
+    cb.snapshots = &r.snapshots;
+    const cdft_hooks hooks = cb.hooks(snapshot_interval);
     cdft_step_config c{step.dt, step.diffusion_coefficient, step.min_density, step.dt_max, step.fp_tolerance,
                        step.fp_max_iterations};
     std::vector<double> e(static_cast<size_t>(n_steps) + 1), t(static_cast<size_t>(n_steps) + 1);
     cdft_sim_result cr{};
-    ::dft::detail::check(cdft_ddft_run(cb->functional->weights.ctx.get(), &c, n_steps, nullptr, cb->chemical_potential,
-                                       densities[0].data(), &cr, e.data(), t.data()));
+    cb.check(cdft_ddft_run(ctx.get(), &c, n_steps, nullptr, mu, &hooks, densities[0].data(), &cr, e.data(),
+                           t.data()));
     // recorded like the reference: the initial state, then every log_interval
     // steps and the last step (dynamics.hpp:603-604,650-653)
-    SimulationResult r;
-    for (int s = 0; s <= n_steps; ++s) {
+    for (int s = 0; s <= cr.steps; ++s) {
       if (s == 0 || (log_interval > 0 && (s % log_interval == 0 || s == n_steps))) {
         r.times.push_back(t[static_cast<size_t>(s)]);
         r.energies.push_back(e[static_cast<size_t>(s)]);
@@ -950,64 +1078,81 @@ inline vec diffusion_propagator(const vec& k_squared, double D, double dt) {
   return p;
 }
 
-namespace detail {
-inline const functionals::GrandPotentialCallback& device_callback(const ForceCallback& compute, const Grid& grid,
-                                                                  const std::vector<vec>& densities) {
-  const auto* cb = compute.target<functionals::GrandPotentialCallback>();
-  if (cb == nullptr) throw std::invalid_argument("the device steps need a Functional::grand_potential_callback");
-  if (densities.size() != 1 || static_cast<long>(densities[0].size()) != grid.total_points())
-    throw std::invalid_argument("the device steps take one density of the grid's size");
-  return *cb;
-}
-}  // namespace detail
-
-// split_operator_step (dynamics.hpp:410-462).  The device forms
+// split_operator_step (dynamics.hpp:410-462) on the device.  The device forms
 // sqrt(exp(-D k^2 dt)) itself from config, so `propagator` must be
 // diffusion_propagator(k_squared, config.diffusion_coefficient, config.dt),
-// as every reference caller passes it.
+// as every reference caller passes it.  A callback other than the
+// functional's own runs on the host, with a transform context of the grid
+// made for the call.
 inline StepResult split_operator_step(const std::vector<vec>& densities, const Grid& grid, const vec& k_squared,
                                       const vec& propagator, const ForceCallback& compute, const StepConfig& config) {
-  const auto& cb = detail::device_callback(compute, grid, densities);
   if (k_squared.size() != propagator.size()) throw std::invalid_argument("k_squared and propagator sizes differ");
+  detail::Callbacks cb;
+  double mu = 0.0;
+  const auto ctx = detail::context_for(compute, grid, densities, cb, &mu);
+  const cdft_hooks hooks = cb.hooks();
   cdft_step_config c{config.dt, config.diffusion_coefficient, config.min_density, config.dt_max,
                      config.fp_tolerance, config.fp_max_iterations};
   StepResult r{densities, 0.0, config.dt};
-  ::dft::detail::check(cdft_ddft_split_operator(cb.functional->weights.ctx.get(), &c, 1, nullptr,
-                                                cb.chemical_potential, r.densities[0].data(), &r.energy));
+  cb.check(cdft_ddft_split_operator(ctx.get(), &c, 1, nullptr, mu, &hooks, r.densities[0].data(), &r.energy));
   return r;
 }
 
 namespace _internal {
 // dynamics.hpp:70-77.  The device forms the propagator factors from the grid
-// itself, so the state keeps only the grid and the last dt.
+// itself, so the state keeps the grid and the last dt, plus the transform
+// context that steps driven by a host force callback run on (made at the
+// first such step and reused by the later ones).
 struct IntegratingFactorState {
   std::vector<long> shape;
   double dx{0.0};
   long fourier_total{0};
   double dt{0.0};
+  std::shared_ptr<cdft_ctx> transforms;
 };
 // dynamics.hpp:79-129
 inline IntegratingFactorState make_if_state(const Grid& grid) {
   return IntegratingFactorState{{grid.shape[0], grid.shape[1], grid.shape[2]},
                                 grid.dx,
                                 grid.shape[0] * grid.shape[1] * (grid.shape[2] / 2 + 1),
-                                0.0};
+                                0.0,
+                                nullptr};
 }
 }  // namespace _internal
 
 // integrating_factor_step (dynamics.hpp:333-400) on the device: one step of
 // the integrating-factor scheme with its fixed-point iteration; a restart
-// divides config.dt by 10, as there, and the step never lengthens dt.
+// divides config.dt by 10, as there, and the step never lengthens dt.  The
+// functional's grand_potential_callback evaluates on the device; any other
+// callback is called on the host (once per force evaluation: the reference's
+// repeated evaluation of the unchanged initial density is taken once).
 inline StepResult integrating_factor_step(const std::vector<vec>& densities, const Grid& grid,
                                           _internal::IntegratingFactorState& st, const ForceCallback& compute,
                                           StepConfig& config) {
-  const auto& cb = detail::device_callback(compute, grid, densities);
+  if (st.shape.size() != 3 || st.shape[0] != grid.shape[0] || st.shape[1] != grid.shape[1] ||
+      st.shape[2] != grid.shape[2])
+    throw std::invalid_argument("integrating_factor_step: the state was made for another grid");
+  detail::Callbacks cb;
+  double mu = 0.0;
+  std::shared_ptr<cdft_ctx> ctx;
+  if (const auto* g = compute.target<functionals::GrandPotentialCallback>()) {
+    ctx = detail::context_for(compute, grid, densities, cb, &mu);
+    (void)g;
+  } else {
+    if (!compute) throw std::invalid_argument("the force callback is empty");
+    if (densities.size() != 1 || static_cast<long>(densities[0].size()) != grid.total_points())
+      throw std::invalid_argument("the device steps take one density of the grid's size");
+    if (!st.transforms) st.transforms = detail::transform_context(grid);
+    ctx = st.transforms;
+    cb.n = densities[0].size();
+    cb.force = &compute;
+  }
+  const cdft_hooks hooks = cb.hooks();
   cdft_step_config c{config.dt, config.diffusion_coefficient, config.min_density, config.dt_max,
                      config.fp_tolerance, config.fp_max_iterations};
   StepResult r{densities, 0.0, config.dt};
   cdft_sim_result cr{};
-  ::dft::detail::check(cdft_ddft_run(cb.functional->weights.ctx.get(), &c, 1, nullptr, cb.chemical_potential,
-                                     r.densities[0].data(), &cr, nullptr, nullptr));
+  cb.check(cdft_ddft_run(ctx.get(), &c, 1, nullptr, mu, &hooks, r.densities[0].data(), &cr, nullptr, nullptr));
   config.dt = cr.dt;
   st.dt = cr.dt;
   r.energy = cr.energy;
@@ -1015,16 +1160,20 @@ inline StepResult integrating_factor_step(const std::vector<vec>& densities, con
   return r;
 }
 
-// crank_nicholson_step (dynamics.hpp:472-538)
+// crank_nicholson_step (dynamics.hpp:472-538) on the device (callbacks as for
+// split_operator_step)
 inline StepResult crank_nicholson_step(const std::vector<vec>& densities, const Grid& grid, const vec& k_squared,
                                        const ForceCallback& compute, const StepConfig& config) {
-  const auto& cb = detail::device_callback(compute, grid, densities);
   (void)k_squared;  // formed on the device from the grid
+  detail::Callbacks cb;
+  double mu = 0.0;
+  const auto ctx = detail::context_for(compute, grid, densities, cb, &mu);
+  const cdft_hooks hooks = cb.hooks();
   cdft_step_config c{config.dt, config.diffusion_coefficient, config.min_density, config.dt_max,
                      config.fp_tolerance, config.fp_max_iterations};
   StepResult r{densities, 0.0, config.dt};
-  ::dft::detail::check(cdft_ddft_crank_nicolson(cb.functional->weights.ctx.get(), &c, 1, nullptr,
-                                                cb.chemical_potential, r.densities[0].data(), &r.energy, nullptr));
+  cb.check(cdft_ddft_crank_nicolson(ctx.get(), &c, 1, nullptr, mu, &hooks, r.densities[0].data(), &r.energy,
+                                    nullptr));
   return r;
 }
 }  // namespace dynamics
@@ -1043,6 +1192,8 @@ struct FixedMassConstraint {
   std::vector<vec> operator()(const std::vector<vec>& densities) const {
     auto result = densities;
     if (result.empty()) return result;
+    if (result[0].size() != background.size())
+      throw std::invalid_argument("fixed_mass_constraint: density and background sizes differ");
     double cur = 0.0;
     for (size_t i = 0; i < result[0].size(); ++i) cur += result[0][i] - background[i];
     cur *= cell_volume;
@@ -1068,8 +1219,11 @@ struct PicardResult {
   bool converged{false};
 };
 
-// Picard (picard.hpp:25-33); solve() runs on the device with the functional's
-// grand-potential callback and no constraint or a fixed_mass_constraint.
+// Picard (picard.hpp:25-33).  With the functional's grand_potential_callback
+// solve() runs on the device, the fixed_mass_constraint too; another
+// constraint is called on the host after every device update.  Picard has no
+// transforms of its own, so with any other force callable the pointwise
+// update runs on the host over the caller's vectors, as fire::Fire does.
 struct Picard {
   double mixing{0.01};
   double min_density{1e-30};
@@ -1079,23 +1233,68 @@ struct Picard {
 
   PicardResult solve(std::vector<vec> densities, const ForceFunction& compute, double cell_volume,
                      const Constraint& constraint = {}) const {
+    if (!compute) throw std::invalid_argument("Picard::solve: the force function is empty");
     const auto* cb = compute.target<functionals::GrandPotentialCallback>();
-    if (cb == nullptr) throw std::invalid_argument("Picard::solve on the GPU needs a Functional::grand_potential_callback");
+    if (cb == nullptr) return solve_host(std::move(densities), compute, cell_volume, constraint);
     if (densities.size() != 1) throw std::invalid_argument("Picard::solve on the GPU takes one density");
+    const long n = cb->functional->weights.n_points;
+    if (static_cast<long>(densities[0].size()) != n)
+      throw std::invalid_argument("Picard::solve: the density has " + std::to_string(densities[0].size()) +
+                                  " values, the functional's grid " + std::to_string(n));
     (void)cell_volume;  // the functional's grid fixes it
-    double target = 0.0;
+    const double* target = nullptr;
     const double* bg = nullptr;
+    dynamics::detail::Callbacks hooks_cb;
+    hooks_cb.n = densities[0].size();
     if (constraint) {
-      const auto* fm = constraint.target<FixedMassConstraint>();
-      if (fm == nullptr) throw std::invalid_argument("Picard::solve on the GPU supports only fixed_mass_constraint");
-      target = fm->target_mass;
-      bg = fm->background.data();
+      if (const auto* fm = constraint.target<FixedMassConstraint>()) {
+        if (static_cast<long>(fm->background.size()) != n)
+          throw std::invalid_argument("fixed_mass_constraint: the background has " +
+                                      std::to_string(fm->background.size()) + " values, the grid " +
+                                      std::to_string(n));
+        target = &fm->target_mass;
+        bg = fm->background.data();
+      } else {
+        hooks_cb.constraint = &constraint;
+      }
     }
+    const cdft_hooks hooks = hooks_cb.hooks();
     cdft_picard c{mixing, min_density, tolerance, max_iterations};
     cdft_picard_result cr{};
-    ::dft::detail::check(cdft_picard_solve(cb->functional->weights.ctx.get(), &c, nullptr, cb->chemical_potential,
-                                           target, bg, densities[0].data(), &cr));
+    hooks_cb.check(cdft_picard_solve(cb->functional->weights.ctx.get(), &c, nullptr, cb->chemical_potential, target,
+                                     bg, &hooks, densities[0].data(), &cr));
     return PicardResult{std::move(densities), cr.grand_potential, cr.residual, cr.iterations, cr.converged != 0};
+  }
+
+ private:
+  // picard.hpp:63-124 over any force function, on the host
+  PicardResult solve_host(std::vector<vec> densities, const ForceFunction& compute, double cell_volume,
+                          const Constraint& constraint) const {
+    double residual = 0.0, omega = 0.0, omega_prev = std::numeric_limits<double>::max();
+    int iter = 0;
+    for (iter = 0; iter < max_iterations; ++iter) {
+      auto ef = compute(densities);
+      omega = ef.first;
+      if (ef.second.size() != densities.size())
+        throw std::invalid_argument("Picard::solve: the force function returned another species count");
+      double sum_f2 = 0.0, dof = 0.0;
+      for (const auto& f : ef.second) {
+        for (double x : f) sum_f2 += x * x;
+        dof += static_cast<double>(f.size());
+      }
+      residual = std::sqrt(sum_f2 / dof);
+      if (residual < tolerance || (iter > 0 && std::abs(omega - omega_prev) < tolerance)) break;
+      omega_prev = omega;
+      for (size_t s = 0; s < densities.size(); ++s) {
+        if (ef.second[s].size() != densities[s].size())
+          throw std::invalid_argument("Picard::solve: force and density sizes differ");
+        for (size_t i = 0; i < densities[s].size(); ++i)
+          densities[s][i] = std::max(densities[s][i] * std::exp(-mixing * ef.second[s][i] / cell_volume), min_density);
+      }
+      if (constraint) densities = constraint(densities);
+    }
+    const bool conv = residual < tolerance || (iter > 0 && std::abs(omega - omega_prev) < tolerance);
+    return PicardResult{std::move(densities), omega, residual, iter, conv};
   }
 };
 }  // namespace picard

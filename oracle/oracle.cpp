@@ -1281,6 +1281,9 @@ int orc_minimize_fixed_mass(void* h, const double* fire_p, int param_kind, doubl
 // energies (optional, n_steps+1): energy after each step
 // ---------------------------------------------------------------------------
 namespace {
+// Restart limit of one integrating-factor step, as in the library.
+constexpr int kMaxRestarts = 30;
+
 struct DdftWork {
   Ctx* c;
   const double* vext;
@@ -1439,7 +1442,7 @@ struct DdftWork {
     double md = 0.0;
     for (long i = 0; i < N; ++i) {
       double nv = re[i] * inv_n;
-      double dev = std::fabs(d1[i] - nv);
+      double dev = std::isfinite(nv) ? std::fabs(d1[i] - nv) : std::numeric_limits<double>::infinity();
       if (dev > md) md = dev;
       d1[i] = nv;
     }
@@ -1455,6 +1458,7 @@ struct DdftWork {
     c->fft->r2c(rho.data(), rho0k.data());
     std::vector<double> d1 = rho;
     bool restart;
+    int restarts = 0;
     do {
       restart = false;
       update_timestep(cfg[0]);
@@ -1466,11 +1470,15 @@ struct DdftWork {
         double old = dev;
         dev = apply_propagator(rho0k.data(), rhs0.data(), rhs1.data(), d1);
         double mn = *std::min_element(d1.begin(), d1.end());
+        // the restatement adds the library's guard: a non-finite iterate ends the run
+        if (!std::isfinite(dev) || !std::isfinite(mn)) throw std::runtime_error("non-finite iterate");
         if (mn < 0.0 || (it > 0 && dev > old)) { restart = true; break; }
         if (dev < cfg[4]) break;
       }
       if (!restart && dev > cfg[4]) restart = true;
       if (restart) {
+        // the library's restart limit (the reference has none, dynamics.hpp:374-392)
+        if (++restarts > kMaxRestarts) throw std::runtime_error("no convergence within the restart limit");
         cfg[0] /= 10.0;
         d1 = rho;
       }
@@ -1502,13 +1510,22 @@ struct DdftWork {
 };
 }  // namespace
 
+// Simulation::run (dynamics.hpp:582-693).  mask (nullable) is a
+// reservoir_boundary (dynamics.hpp:545-553): the marked cells are reset to
+// `reservoir` on the initial state and after every step.
 int orc_ddft_run(void* h, const double* cfg_in, int n_steps, const double* vext, double mu, double* rho_inout,
-                 double* out, double* energies) {
+                 double* out, double* energies, const unsigned char* mask, double reservoir) {
   auto* c = static_cast<Ctx*>(h);
   DdftWork w(c, vext, mu);
   double cfg[6];
   std::memcpy(cfg, cfg_in, sizeof(cfg));
   std::vector<double> rho(rho_inout, rho_inout + c->N);
+  auto boundary = [&]() {
+    if (mask)
+      for (long i = 0; i < c->N; ++i)
+        if (mask[i]) rho[i] = reservoir;
+  };
+  boundary();
   double m0 = 0.0;
   for (double r : rho) m0 += r;
   m0 *= c->dv;
@@ -1519,8 +1536,13 @@ int orc_ddft_run(void* h, const double* cfg_in, int n_steps, const double* vext,
   double time = 0.0, e = e0;
   for (int s = 1; s <= n_steps; ++s) {
     double before = cfg[0], used = 0.0;
-    e = w.step(rho, cfg, &used);
+    try {
+      e = w.step(rho, cfg, &used);
+    } catch (const std::runtime_error&) {
+      return -2;  // no convergence within the restart limit, or a non-finite iterate
+    }
     time += used;
+    boundary();
     if (used < before) succ = 0;
     else ++succ;
     if (succ >= 5 && cfg[0] < cfg[3]) {
@@ -1753,7 +1775,9 @@ void orc_ddft_crank_nicolson(void* h, const double* cfg, int n_steps, const doub
 // Picard::solve (picard.hpp:63-124); cfg: mixing, min_density, tolerance, max iterations.
 // target_mass > 0 applies fixed_mass_constraint (minimization.hpp:157-180) with background (nullable: 0).
 // out: Omega, residual, iterations, converged.
-void orc_picard(void* h, const double* cfg, const double* vext, double mu, double target_mass, const double* bg,
+// target (nullable: no constraint) is fixed_mass_constraint's target mass of
+// the excess over bg, applied for any sign (minimization.hpp:159-180).
+void orc_picard(void* h, const double* cfg, const double* vext, double mu, const double* target, const double* bg,
                 double* rho, double* out) {
   auto* c = static_cast<Ctx*>(h);
   const long N = c->N;
@@ -1771,7 +1795,8 @@ void orc_picard(void* h, const double* cfg, const double* vext, double mu, doubl
     if (residual < tol || (it > 0 && std::fabs(omega - omega_prev) < tol)) break;
     omega_prev = omega;
     for (long i = 0; i < N; ++i) rho[i] = std::max(rho[i] * std::exp(-mixing * f[i] / c->dv), lo);
-    if (target_mass > 0.0) {
+    if (target) {
+      const double target_mass = *target;
       double cur = 0.0;
       for (long i = 0; i < N; ++i) cur += rho[i] - (bg ? bg[i] : 0.0);
       cur *= c->dv;

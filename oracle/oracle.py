@@ -69,7 +69,8 @@ def lib():
         L.orc_minimize_fixed_mass.argtypes = [C.c_void_p, _dp, C.c_int, C.c_double, C.c_double, C.c_int, _dp, _dp,
                                               C.c_double, C.c_double, _dp, _dp, _dp]
         L.orc_ddft_run.restype = C.c_int
-        L.orc_ddft_run.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp, _dp, _dp]
+        L.orc_ddft_run.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp, _dp, _dp,
+                                   C.POINTER(C.c_ubyte), C.c_double]
         L.orc_ddft_fixed_run.restype = C.c_double
         L.orc_ddft_fixed_run.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, _dp, C.c_double, _dp]
         L.orc_wells_potential.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
@@ -78,7 +79,7 @@ def lib():
         L.orc_ddft_split_operator.restype = C.c_double
         L.orc_ddft_split_operator.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp]
         L.orc_ddft_crank_nicolson.argtypes = [C.c_void_p, _dp, C.c_int, _dp, C.c_double, _dp, _dp]
-        L.orc_picard.argtypes = [C.c_void_p, _dp, _dp, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_picard.argtypes = [C.c_void_p, _dp, _dp, C.c_double, _dp, _dp, _dp, _dp]
         L.orc_saddle_search.restype = C.c_int
         L.orc_saddle_search.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.c_ulonglong, _dp, _dp,
                                         _dp]
@@ -246,13 +247,21 @@ class Oracle:
         return rho_out.reshape(self.shape), res, energies[: res["iterations"] + 1]
 
     def ddft_run(self, rho0, mu, n_steps, dt=1e-4, dt_max=1.0, fp_tolerance=1e-4, fp_max_iterations=100,
-                 min_density=1e-18, vext=None):
+                 min_density=1e-18, vext=None, reservoir_mask=None, reservoir_density=0.0):
+        """Simulation::run; reservoir_mask (bool array) applies reservoir_boundary(mask, reservoir_density).
+        Raises RuntimeError when a step exceeds the restart limit or its iterate is not finite."""
         cfg = np.array([dt, 1.0, min_density, dt_max, fp_tolerance, fp_max_iterations], dtype=np.float64)
         rho = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1).copy()
         v = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
         out = np.zeros(6)
         energies = np.zeros(n_steps + 1)
-        lib().orc_ddft_run(self._h, _p(cfg), int(n_steps), _p(v), float(mu), _p(rho), _p(out), _p(energies))
+        mask = None if reservoir_mask is None else np.ascontiguousarray(reservoir_mask, dtype=np.uint8).reshape(-1)
+        rc = lib().orc_ddft_run(self._h, _p(cfg), int(n_steps), _p(v), float(mu), _p(rho), _p(out), _p(energies),
+                                None if mask is None else mask.ctypes.data_as(C.POINTER(C.c_ubyte)),
+                                float(reservoir_density))
+        if rc != 0:
+            raise RuntimeError("integrating_factor_step: no convergence within the restart limit, or a "
+                               "non-finite iterate")
         res = dict(mass_initial=out[0], mass_final=out[1], dt=out[2], time=out[3], energy=out[4],
                    evals=int(out[5]))
         return rho.reshape(self.shape), res, energies
@@ -295,14 +304,15 @@ class Oracle:
         lib().orc_ddft_crank_nicolson(self._h, _p(cfg), int(n_steps), _p(ve), float(mu), _p(rho), _p(out))
         return rho.reshape(self.shape), out[0], int(out[1])
 
-    def picard(self, rho0, mu, mixing=0.01, min_density=1e-30, tolerance=1e-6, max_iterations=5000, target_mass=0.0,
-               background=None, vext=None):
+    def picard(self, rho0, mu, mixing=0.01, min_density=1e-30, tolerance=1e-6, max_iterations=5000,
+               target_mass=None, background=None, vext=None):
         rho = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1).copy()
         ve = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
         bg = None if background is None else np.ascontiguousarray(background, dtype=np.float64).reshape(-1)
         cfg = np.array([mixing, min_density, tolerance, max_iterations], dtype=np.float64)
         out = np.zeros(4)
-        lib().orc_picard(self._h, _p(cfg), _p(ve), float(mu), float(target_mass), _p(bg), _p(rho), _p(out))
+        tgt = None if target_mass is None else np.array([float(target_mass)])
+        lib().orc_picard(self._h, _p(cfg), _p(ve), float(mu), _p(tgt), _p(bg), _p(rho), _p(out))
         return rho.reshape(self.shape), dict(grand_potential=out[0], residual=out[1], iterations=int(out[2]),
                                              converged=bool(out[3]))
 

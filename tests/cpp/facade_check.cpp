@@ -4,7 +4,13 @@
 //
 //   facade_check cpu                       argument validation and error mapping (no GPU calls)
 //   facade_check gpu RHO_IN MU_RHO OUT_DIR 16^3 LJ droplet: writes forces.bin, fire_rho.bin,
-//                                          ddft_rho.bin to OUT_DIR and one JSON line to stdout
+//                                          ddft_rho.bin, ... to OUT_DIR and one JSON line to stdout
+//   facade_check callbacks RHO_IN OUT_DIR  the algorithms with callables other than the functional's own:
+//                                          the ideal-gas callback of the reference's dynamics tests on
+//                                          8^3, a total()-wrapping lambda, a reservoir boundary, a stop
+//                                          condition, snapshots, Picard constraints; one JSON line
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -305,13 +311,10 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   sim.log_interval = 2;
   const auto sres = sim.run({rho}, model.grid, cb);
   write_doubles(out + "/ddft_rho.bin", sres.densities[0]);
-  // a callback that is not the functional's cannot run on the device
+  // a density of the wrong size is refused before any device work
   bool rejected = false;
   try {
-    algorithms::dynamics::ForceCallback other = [](const std::vector<vec>& d) {
-      return std::make_pair(0.0, d);
-    };
-    (void)sim.run({rho}, model.grid, other);
+    (void)sim.run({vec(10, 0.1)}, model.grid, cb);
   } catch (const std::invalid_argument&) {
     rejected = true;
   }
@@ -334,8 +337,200 @@ int run_gpu(const std::string& rho_path, double mu_rho, const std::string& out) 
   for (size_t i = 0; i < sres.times.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.times[i]);
   std::printf("], \"ddft_energies\": [");
   for (size_t i = 0; i < sres.energies.size(); ++i) std::printf("%s%.17g", i ? ", " : "", sres.energies[i]);
-  std::printf("], \"ddft_mass\": [%.17g, %.17g], \"foreign_callback_rejected\": %s}\n", sres.mass_initial,
+  std::printf("], \"ddft_mass\": [%.17g, %.17g], \"wrong_size_rejected\": %s}\n", sres.mass_initial,
               sres.mass_final, rejected ? "true" : "false");
+  return 0;
+}
+
+// The reference's dynamics tests' force callback (tests/unit/algorithms/dynamics.cpp:17-27):
+// ideal gas only, F = sum rho (ln rho - 1) dV, force ln(rho) dV, densities clamped at 1e-18.
+std::pair<double, std::vector<dft::vec>> ideal_gas_forces(const std::vector<dft::vec>& densities, double dv) {
+  double energy = 0.0;
+  std::vector<dft::vec> forces(densities.size());
+  for (size_t s = 0; s < densities.size(); ++s) {
+    forces[s].resize(densities[s].size());
+    for (size_t i = 0; i < densities[s].size(); ++i) {
+      const double r = std::max(densities[s][i], 1e-18);
+      energy += r * (std::log(r) - 1.0) * dv;
+      forces[s][i] = std::log(r) * dv;
+    }
+  }
+  return {energy, forces};
+}
+
+void write_json_array(const char* key, const std::vector<double>& v, bool last = false) {
+  std::printf("\"%s\": [", key);
+  for (size_t i = 0; i < v.size(); ++i) std::printf("%s%.17g", i ? ", " : "", v[i]);
+  std::printf(last ? "]" : "], ");
+}
+
+int run_callbacks(const std::string& rho_path, const std::string& out) {
+  using namespace dft;
+  namespace dyn = algorithms::dynamics;
+  // 1. the ideal-gas callback on the reference test's grid (dx = 0.5, box 4: 8^3) with a Gaussian bump
+  const Grid grid{0.5, {4.0, 4.0, 4.0}, {8, 8, 8}};
+  const double dv = grid.cell_volume();
+  const long n8 = grid.total_points();
+  vec rho8(static_cast<size_t>(n8), 0.5);
+  for (long ix = 0; ix < 8; ++ix)
+    for (long iy = 0; iy < 8; ++iy)
+      for (long iz = 0; iz < 8; ++iz) {
+        const double x = (ix - 4.0) * 0.5, y = (iy - 4.0) * 0.5, z = (iz - 4.0) * 0.5;
+        rho8[static_cast<size_t>(grid.flat_index(ix, iy, iz))] += 0.1 * std::exp(-(x * x + y * y + z * z));
+      }
+  write_doubles(out + "/ideal_rho0.bin", rho8);
+  dyn::ForceCallback ideal = [dv](const std::vector<vec>& d) { return ideal_gas_forces(d, dv); };
+  dyn::Simulation sim;
+  sim.step.dt = 1e-2;
+  sim.step.dt_max = 1e-2;
+  sim.n_steps = 5;
+  sim.log_interval = 1;
+  sim.snapshot_interval = 2;
+  const auto ig = sim.run({rho8}, grid, ideal);
+  write_doubles(out + "/ideal_ddft_rho.bin", ig.densities[0]);
+  const auto k2 = dyn::compute_k_squared(grid);
+  dyn::StepConfig sc;
+  sc.dt = 1e-3;
+  const auto prop = dyn::diffusion_propagator(k2, sc.diffusion_coefficient, sc.dt);
+  const auto igs = dyn::split_operator_step({rho8}, grid, k2, prop, ideal, sc);
+  sc.fp_tolerance = 1e-10;
+  const auto igc = dyn::crank_nicholson_step({rho8}, grid, k2, ideal, sc);
+  write_doubles(out + "/ideal_split_rho.bin", igs.densities[0]);
+  write_doubles(out + "/ideal_cn_rho.bin", igc.densities[0]);
+  auto st8 = dyn::_internal::make_if_state(grid);
+  dyn::StepConfig ic;
+  ic.dt = 1e-2;
+  ic.dt_max = 1e-2;
+  std::vector<vec> r8{rho8};
+  for (int k = 0; k < 3; ++k) r8 = dyn::integrating_factor_step(r8, grid, st8, ideal, ic).densities;
+  write_doubles(out + "/ideal_ifstep_rho.bin", r8[0]);
+
+  // 2. the LJ droplet functional behind a total()-wrapping lambda (nucleation.cpp:279-291) and behind
+  //    its own callback: the same trajectory
+  const auto model = lj_model(0.25, 4.0);  // 16^3
+  auto func = functionals::make_functional(functionals::fmt::WhiteBearII{}, model);
+  const long n = model.grid.total_points();
+  const vec rho = read_doubles(rho_path, n);
+  const double mu = func.bulk_chemical_potential(0.05);
+  int lambda_calls = 0;
+  dyn::ForceCallback wrapped = [&](const std::vector<vec>& d) {
+    ++lambda_calls;
+    auto state = init::from_profile(model, d[0]);
+    state.species[0].chemical_potential = mu;
+    auto result = functionals::total(model, state, func.weights);
+    return std::make_pair(result.grand_potential, result.forces);
+  };
+  dyn::Simulation sim16;
+  sim16.step.dt = 1e-3;
+  sim16.step.dt_max = 1e-3;
+  sim16.n_steps = 6;
+  sim16.log_interval = 2;
+  const auto via_lambda = sim16.run({rho}, model.grid, wrapped);
+  const auto via_device = sim16.run({rho}, model.grid, func.grand_potential_callback(mu));
+  double traj_diff = 0.0;
+  for (long i = 0; i < n; ++i)
+    traj_diff = std::max(traj_diff, std::abs(via_lambda.densities[0][static_cast<size_t>(i)] -
+                                             via_device.densities[0][static_cast<size_t>(i)]));
+  double energy_diff = 0.0;
+  for (size_t i = 0; i < via_lambda.energies.size(); ++i)
+    energy_diff = std::max(energy_diff, std::abs(via_lambda.energies[i] - via_device.energies[i]));
+
+  // 3. a reservoir boundary (dynamics.hpp:545-553: marked cells reset to a reservoir density after each
+  //    step) on the x = 0 face, a stop condition after step 4 and snapshots every 2 steps
+  std::vector<unsigned char> mask(static_cast<size_t>(n), 0);
+  for (long iy = 0; iy < model.grid.shape[1]; ++iy)
+    for (long iz = 0; iz < model.grid.shape[2]; ++iz) mask[static_cast<size_t>(model.grid.flat_index(0, iy, iz))] = 1;
+  dyn::Simulation simb = sim16;
+  simb.n_steps = 10;
+  simb.snapshot_interval = 2;
+  simb.boundary = [&](std::vector<vec>& d) {
+    for (auto& r : d)
+      for (size_t i = 0; i < r.size(); ++i)
+        if (mask[i]) r[i] = 0.05;
+  };
+  std::vector<int> stop_steps;
+  simb.stop_condition = [&](int step, double, double) {
+    stop_steps.push_back(step);
+    return step >= 4;
+  };
+  const auto sb = simb.run({rho}, model.grid, func.grand_potential_callback(mu));
+  write_doubles(out + "/boundary_rho.bin", sb.densities[0]);
+  for (const auto& snap : sb.snapshots)
+    write_doubles(out + "/snapshot_" + std::to_string(snap.step) + ".bin", snap.densities[0]);
+  std::vector<double> snap_steps, snap_times, snap_energies;
+  for (const auto& snap : sb.snapshots) {
+    snap_steps.push_back(snap.step);
+    snap_times.push_back(snap.time);
+    snap_energies.push_back(snap.energy);
+  }
+
+  // 4. an exception thrown by a callback reaches the caller unchanged
+  bool rethrown = false;
+  try {
+    dyn::ForceCallback bad = [](const std::vector<vec>&) -> std::pair<double, std::vector<vec>> {
+      throw std::domain_error("callback says no");
+    };
+    (void)sim16.run({rho}, model.grid, bad);
+  } catch (const std::domain_error& e) {
+    rethrown = std::string(e.what()) == "callback says no";
+  }
+
+  // 5. Picard: the functional's callback with the fixed-mass constraint as an opaque lambda (a host
+  //    projection between device updates), and the total()-wrapping lambda (the host loop)
+  double mass0 = 0.0;
+  for (double r : rho) mass0 += r;
+  mass0 *= model.grid.cell_volume();
+  algorithms::picard::Picard pc;
+  pc.mixing = 0.02;
+  pc.tolerance = 1e-8;
+  pc.max_iterations = 200;
+  const auto fm = algorithms::picard::fixed_mass_constraint({mass0}, {vec(rho.size(), 0.0)}, model.grid.cell_volume());
+  algorithms::picard::Constraint opaque = [&fm](const std::vector<vec>& d) { return fm(d); };
+  const auto p_device = pc.solve({rho}, func.grand_potential_callback(mu), model.grid.cell_volume(), fm);
+  const auto p_opaque = pc.solve({rho}, func.grand_potential_callback(mu), model.grid.cell_volume(), opaque);
+  const auto p_host = pc.solve({rho}, wrapped, model.grid.cell_volume(), fm);
+  double pd_opaque = 0.0, pd_host = 0.0;
+  for (long i = 0; i < n; ++i) {
+    const size_t k = static_cast<size_t>(i);
+    pd_opaque = std::max(pd_opaque, std::abs(p_opaque.densities[0][k] - p_device.densities[0][k]));
+    pd_host = std::max(pd_host, std::abs(p_host.densities[0][k] - p_device.densities[0][k]));
+  }
+  // a background of the wrong size is refused
+  bool bg_rejected = false;
+  try {
+    (void)pc.solve({rho}, func.grand_potential_callback(mu), model.grid.cell_volume(),
+                   algorithms::picard::fixed_mass_constraint({mass0}, {vec(10, 0.0)}, model.grid.cell_volume()));
+  } catch (const std::invalid_argument&) {
+    bg_rejected = true;
+  }
+  bool size_rejected = false;
+  try {
+    (void)pc.solve({vec(10, 0.1)}, func.grand_potential_callback(mu), model.grid.cell_volume());
+  } catch (const std::invalid_argument&) {
+    size_rejected = true;
+  }
+
+  std::printf("{");
+  write_json_array("ideal_times", ig.times);
+  write_json_array("ideal_energies", ig.energies);
+  std::vector<double> ig_snaps;
+  for (const auto& snap : ig.snapshots) ig_snaps.push_back(snap.step);
+  write_json_array("ideal_snapshot_steps", ig_snaps);
+  std::printf("\"ideal_mass\": [%.17g, %.17g], \"ideal_split_energy\": %.17g, \"ideal_cn_energy\": %.17g, "
+              "\"lambda_traj_maxdiff\": %.3g, \"lambda_energy_maxdiff\": %.3g, \"lambda_calls\": %d, ",
+              ig.mass_initial, ig.mass_final, igs.energy, igc.energy, traj_diff, energy_diff, lambda_calls);
+  write_json_array("boundary_times", sb.times);
+  write_json_array("boundary_energies", sb.energies);
+  write_json_array("snapshot_steps", snap_steps);
+  write_json_array("snapshot_times", snap_times);
+  write_json_array("snapshot_energies", snap_energies);
+  std::vector<double> ss(stop_steps.begin(), stop_steps.end());
+  write_json_array("stop_calls", ss);
+  std::printf("\"boundary_mass\": [%.17g, %.17g], \"exception_rethrown\": %s, \"picard_iterations\": [%d, %d, %d], "
+              "\"picard_opaque_maxdiff\": %.3g, \"picard_host_maxdiff\": %.3g, \"picard_bg_rejected\": %s, "
+              "\"picard_size_rejected\": %s}\n",
+              sb.mass_initial, sb.mass_final, rethrown ? "true" : "false", p_device.iterations, p_opaque.iterations,
+              p_host.iterations, pd_opaque, pd_host, bg_rejected ? "true" : "false", size_rejected ? "true" : "false");
   return 0;
 }
 
@@ -345,10 +540,11 @@ int main(int argc, char** argv) {
   try {
     if (argc >= 2 && std::strcmp(argv[1], "cpu") == 0) return run_cpu();
     if (argc >= 5 && std::strcmp(argv[1], "gpu") == 0) return run_gpu(argv[2], std::stod(argv[3]), argv[4]);
+    if (argc >= 4 && std::strcmp(argv[1], "callbacks") == 0) return run_callbacks(argv[2], argv[3]);
   } catch (const std::exception& e) {
     std::printf("error: %s\n", e.what());
     return 2;
   }
-  std::printf("usage: facade_check cpu | gpu RHO_IN MU_RHO OUT_DIR\n");
+  std::printf("usage: facade_check cpu | gpu RHO_IN MU_RHO OUT_DIR | callbacks RHO_IN OUT_DIR\n");
   return 2;
 }

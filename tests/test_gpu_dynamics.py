@@ -211,19 +211,118 @@ def test_split_operator_and_crank_nicolson_match_oracle(precision):
         assert itg == ito
 
 
-@pytest.mark.parametrize("target", [0.0, 1.0])
+@pytest.mark.parametrize("target", [None, 1.0, -0.3])
 def test_picard_matches_oracle(target):
+    # None: unconstrained; 1.0: the initial mass; -0.4: a negative excess mass over a uniform background
+    # (a bubble in a liquid), which fixed_mass_constraint applies like any other target
     shape, dx = (16, 16, 16), 0.25
     ctx, orc = _pair(shape, dx)
     rho0 = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
     mu = orc.bulk_mu(0.05)
-    mass = target * float(np.sum(rho0)) * dx ** 3
-    kw = dict(mixing=0.02, tolerance=1e-8, max_iterations=200, target_mass=mass)
+    kw = dict(mixing=0.02, tolerance=1e-8, max_iterations=200)
+    if target is not None and target > 0:
+        kw["target_mass"] = target * float(np.sum(rho0)) * dx ** 3
+    elif target is not None:
+        bg = np.full(shape, 0.2)
+        kw["background"] = bg
+        kw["target_mass"] = target * float(np.sum(np.abs(rho0 - bg))) * dx ** 3
     rg, resg = ctx.picard(rho0, mu, **kw)
     ro, reso = orc.picard(rho0, mu, **kw)
     e = float(np.max(np.abs(rg - ro)) / np.max(np.abs(ro)))
     print(f"picard target={target}: {resg['iterations']} iterations, profile {e:.1e}", flush=True)
     assert resg["iterations"] == reso["iterations"] and resg["converged"] == reso["converged"]
-    assert e < 1e-10
-    assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=1e-10)
-    assert resg["residual"] == pytest.approx(reso["residual"], rel=1e-8)
+    # the negative-target run takes 200 iterations; each rescale divides by the summed excess mass, whose
+    # last bits differ between the device's compensated sum and the oracle's plain one
+    tol = 1e-9 if target is not None and target < 0 else 1e-10
+    assert e < tol
+    assert resg["grand_potential"] == pytest.approx(reso["grand_potential"], rel=tol)
+    assert resg["residual"] == pytest.approx(reso["residual"], rel=100 * tol)
+    if target is not None:
+        bg = kw.get("background", np.zeros(shape))
+        assert float(np.sum(rg - bg)) * dx ** 3 == pytest.approx(kw["target_mass"], rel=1e-12)
+
+
+def _ideal_setup():
+    shape, dx = (8, 8, 8), 0.5
+    ctx, orc = _pair(shape, dx, model="none", inter=())
+    ax = (np.arange(8) - 4.0) * dx
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    rho = 0.5 + 0.1 * np.exp(-(X * X + Y * Y + Z * Z))
+    dv = dx ** 3
+
+    def ideal(r):  # the reference's dynamics-test callback (tests/unit/algorithms/dynamics.cpp:17-27)
+        r = np.maximum(r, 1e-18)
+        return float(np.sum(r * (np.log(r) - 1.0)) * dv), np.log(r) * dv
+
+    return ctx, orc, rho, ideal
+
+
+def test_ddft_host_callbacks_through_the_c_abi():
+    # a Python force callback equal to the context's own (ideal-gas) functional gives its trajectory; the
+    # stop callback ends the run after step 3 and the snapshot callback sees steps 0..3
+    ctx, orc, rho, ideal = _ideal_setup()
+    r1, res1, e1 = ctx.ddft_run(rho, 0.0, 5, dt=1e-2, dt_max=1e-2, force=ideal)
+    r2, res2, e2 = ctx.ddft_run(rho, 0.0, 5, dt=1e-2, dt_max=1e-2)
+    ro, reso, eo = orc.ddft_run(rho, 0.0, 5, dt=1e-2, dt_max=1e-2)
+    assert np.max(np.abs(r1 - r2)) / np.max(r2) < 1e-14
+    assert np.max(np.abs(r1 - ro)) / np.max(ro) < 1e-12
+    np.testing.assert_allclose(e1, eo, rtol=1e-12)
+    snaps, stops = [], []
+    r3, res3, e3 = ctx.ddft_run(rho, 0.0, 10, dt=1e-2, dt_max=1e-2, force=ideal,
+                                stop=lambda s, t, e: stops.append(s) or s >= 3,
+                                snapshot=lambda s, t, e, r: snaps.append((s, r)), snapshot_interval=1)
+    assert res3["steps"] == 3 and stops == [1, 2, 3] and len(e3) == 4
+    assert [s for s, _ in snaps] == [0, 1, 2, 3]
+    r3o, _, _ = orc.ddft_run(rho, 0.0, 3, dt=1e-2, dt_max=1e-2)
+    assert np.max(np.abs(snaps[-1][1] - r3o)) / np.max(r3o) < 1e-12
+    assert np.array_equal(snaps[-1][1], r3)
+    # a boundary callback resets a face; the oracle's reservoir boundary is the same rule
+    mask = np.zeros(rho.shape, dtype=bool)
+    mask[:, 0] = True
+
+    def reservoir(r):
+        r[mask] = 0.45
+        return r
+
+    rb, resb, eb = ctx.ddft_run(rho, 0.0, 4, dt=1e-2, dt_max=1e-2, boundary=reservoir)
+    rbo, resbo, ebo = orc.ddft_run(rho, 0.0, 4, dt=1e-2, dt_max=1e-2, reservoir_mask=mask, reservoir_density=0.45)
+    assert np.max(np.abs(rb - rbo)) / np.max(rbo) < 1e-12
+    np.testing.assert_allclose(eb, ebo, rtol=1e-12)
+    assert resb["mass_final"] == pytest.approx(resbo["mass_final"], rel=1e-13)
+    # the split-operator and Crank-Nicolson steps with the callback
+    rs1, es1 = ctx.ddft_split_operator(rho, 0.0, 2, dt=1e-3, force=ideal)
+    rs2, es2 = ctx.ddft_split_operator(rho, 0.0, 2, dt=1e-3)
+    rc1, ec1, _ = ctx.ddft_crank_nicolson(rho, 0.0, 2, dt=1e-3, force=ideal, fp_tolerance=1e-10)
+    rc2, ec2, _ = ctx.ddft_crank_nicolson(rho, 0.0, 2, dt=1e-3, fp_tolerance=1e-10)
+    assert np.max(np.abs(rs1 - rs2)) < 1e-14 and np.max(np.abs(rc1 - rc2)) < 1e-14
+    assert es1 == pytest.approx(es2, rel=1e-13) and ec1 == pytest.approx(ec2, rel=1e-13)
+
+
+def test_ddft_restart_limit_and_non_finite_iterates_end_the_run():
+    # every attempt misses a tolerance no iterate can reach: after the restart limit the step fails
+    # instead of dividing dt forever (the reference has no floor, dynamics.hpp:374-392); every kernel
+    # completes and the host returns the error
+    import classicaldft_amd as cd
+
+    ctx, orc, rho, ideal = _ideal_setup()
+    kw = dict(dt=1e-2, fp_tolerance=1e-300, fp_max_iterations=1)
+    with pytest.raises(RuntimeError):
+        orc.ddft_run(rho, 0.0, 1, **kw)
+    with pytest.raises(cd.CdftError) as ei:
+        ctx.ddft_run(rho, 0.0, 1, **kw)
+    assert ei.value.code == -2 and "restarts" in str(ei.value)
+
+    def nan_force(r):
+        e, f = ideal(r)
+        f[3, 3, 3] = np.nan
+        return e, f
+
+    with pytest.raises(cd.CdftError) as ei:
+        ctx.ddft_run(rho, 0.0, 2, dt=1e-2, force=nan_force)
+    assert ei.value.code == -2 and "not finite" in str(ei.value)
+    # an exception raised inside a callback reaches the caller
+    with pytest.raises(KeyError):
+        ctx.ddft_run(rho, 0.0, 1, force=lambda r: {}["missing"])
+    # the context still works afterwards
+    r, res, _ = ctx.ddft_run(rho, 0.0, 1, dt=1e-2)
+    assert np.all(np.isfinite(r))

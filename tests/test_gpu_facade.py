@@ -122,4 +122,77 @@ def test_cpp_facade_matches_oracle(tmp_path):
     np.testing.assert_allclose(out["ddft_energies"], ed[[0, 2, 4, 6]], rtol=1e-9)
     assert out["ddft_times"][0] == 0.0 and out["ddft_times"][-1] == pytest.approx(resd["time"], rel=1e-12)
     assert out["ddft_mass"][0] == pytest.approx(resd["mass_initial"], rel=1e-12)
-    assert out["foreign_callback_rejected"]
+    assert out["wrong_size_rejected"]
+
+
+def test_cpp_facade_generic_callbacks_match_oracle(tmp_path):
+    """The reference's callers with callables other than the functional's own: the ideal-gas force
+    callback of tests/unit/algorithms/dynamics.cpp:17-27 through Simulation::run, split_operator_step,
+    crank_nicholson_step and integrating_factor_step; a total()-wrapping lambda as in
+    nucleation.cpp:279-291; a reservoir boundary, a stop condition and snapshots; Picard constraints."""
+    from oracle import oracle as O
+
+    assert os.path.exists(BIN), "build the facade check first (make -C tests/cpp)"
+    shape, dx = (16, 16, 16), 0.25
+    rho = W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.6, rho_v=0.05, noise=1e-3, seed=3)
+    rho.reshape(-1).astype(np.float64).tofile(tmp_path / "rho.bin")
+    print("facade_check callbacks", flush=True)
+    r = subprocess.run([BIN, "callbacks", str(tmp_path / "rho.bin"), str(tmp_path)], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+
+    # the ideal-gas callback equals an FMT-free, interaction-free functional at mu = 0 (its clamp at 1e-18
+    # never acts here), so the oracle of that functional is its reference
+    s8 = (8, 8, 8)
+    rho8 = np.fromfile(tmp_path / "ideal_rho0.bin").reshape(s8)
+    ig = O.Oracle(s8, 0.5, model="none")
+    ro, reso, eo = ig.ddft_run(rho8, 0.0, 5, dt=1e-2, dt_max=1e-2)
+    assert _rel(np.fromfile(tmp_path / "ideal_ddft_rho.bin").reshape(s8), ro) < 1e-12
+    np.testing.assert_allclose(out["ideal_energies"], eo, rtol=1e-12)
+    assert out["ideal_times"][-1] == pytest.approx(reso["time"], rel=1e-14)
+    assert out["ideal_snapshot_steps"] == [0, 2, 4]
+    assert out["ideal_mass"][1] == pytest.approx(out["ideal_mass"][0], rel=1e-13)
+    rs, es = ig.ddft_split_operator(rho8, 0.0, 1, dt=1e-3)
+    assert _rel(np.fromfile(tmp_path / "ideal_split_rho.bin").reshape(s8), rs) < 1e-12
+    assert out["ideal_split_energy"] == pytest.approx(es, rel=1e-12)
+    rc, ec, _ = ig.ddft_crank_nicolson(rho8, 0.0, 1, dt=1e-3, fp_tolerance=1e-10)
+    assert _rel(np.fromfile(tmp_path / "ideal_cn_rho.bin").reshape(s8), rc) < 1e-12
+    assert out["ideal_cn_energy"] == pytest.approx(ec, rel=1e-12)
+    ri, _, _ = ig.ddft_run(rho8, 0.0, 3, dt=1e-2, dt_max=1e-2)
+    assert _rel(np.fromfile(tmp_path / "ideal_ifstep_rho.bin").reshape(s8), ri) < 1e-12
+
+    # the total()-wrapping lambda runs the same trajectory as the functional's own callback, one host
+    # call per evaluation (the initial state, then per step its iterations and the final state)
+    assert out["lambda_traj_maxdiff"] < 1e-14 and out["lambda_energy_maxdiff"] < 1e-12
+    assert out["lambda_calls"] >= 1 + 6 * 2
+
+    # reservoir boundary on the x = 0 face, stopped after step 4, snapshots every 2 steps
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    mu = orc.bulk_mu(0.05)
+    mask = np.zeros(shape, dtype=bool)
+    mask[0] = True
+    rb, resb, eb = orc.ddft_run(rho, mu, 4, dt=1e-3, dt_max=1e-3, reservoir_mask=mask, reservoir_density=0.05)
+    got = np.fromfile(tmp_path / "boundary_rho.bin").reshape(shape)
+    assert _rel(got, rb) < 1e-10
+    assert np.all(got[0] == 0.05)
+    assert out["stop_calls"] == [1, 2, 3, 4]
+    np.testing.assert_allclose(out["boundary_energies"], eb[[0, 2, 4]], rtol=1e-10)
+    assert out["boundary_mass"][0] == pytest.approx(resb["mass_initial"], rel=1e-12)
+    assert out["boundary_mass"][1] == pytest.approx(resb["mass_final"], rel=1e-12)
+    assert out["snapshot_steps"] == [0, 2, 4]
+    np.testing.assert_allclose(out["snapshot_energies"], eb[[0, 2, 4]], rtol=1e-10)
+    for k in (2, 4):
+        rk, _, _ = orc.ddft_run(rho, mu, k, dt=1e-3, dt_max=1e-3, reservoir_mask=mask, reservoir_density=0.05)
+        assert _rel(np.fromfile(tmp_path / f"snapshot_{k}.bin").reshape(shape), rk) < 1e-10
+    r0 = rho.copy()
+    r0[0] = 0.05
+    assert np.array_equal(np.fromfile(tmp_path / "snapshot_0.bin").reshape(shape), r0)
+
+    assert out["exception_rethrown"]
+    # Picard: an opaque fixed-mass lambda (host projection) and the host loop over the wrapped functional
+    # reach the device solver's answer
+    it_dev, it_opaque, it_host = out["picard_iterations"]
+    assert it_opaque == it_dev and out["picard_opaque_maxdiff"] < 1e-12
+    assert it_host == it_dev and out["picard_host_maxdiff"] < 1e-10
+    assert out["picard_bg_rejected"] and out["picard_size_rejected"]

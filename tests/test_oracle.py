@@ -681,6 +681,12 @@ def test_picard_ideal_gas_reaches_exp_mu():
     out, res = orc.picard(rho0, math.log(0.3), mixing=0.5, tolerance=1e-12, max_iterations=2000, target_mass=target)
     np.testing.assert_allclose(out, 0.25, rtol=1e-6)
     assert float(np.sum(out)) * dx ** 3 == pytest.approx(target, rel=1e-13)
+    # a negative excess over a uniform background 0.5 (fixed_mass_constraint takes any sign): 0.5 - 0.1
+    V = rho0.size * dx ** 3
+    out, res = orc.picard(rho0, math.log(0.3), mixing=0.5, tolerance=1e-12, max_iterations=2000,
+                          target_mass=-0.1 * V, background=np.full(shape, 0.5))
+    np.testing.assert_allclose(out, 0.4, rtol=1e-6)
+    assert float(np.sum(out - 0.5)) * dx ** 3 == pytest.approx(-0.1 * V, rel=1e-12)
 
 
 # ---------------------------------------------------------------------------
@@ -768,3 +774,42 @@ def test_oracle_mixture_cross_mean_field_closed_form():
     assert parts[3] == pytest.approx(0.5 * a * ra * rb * vol, rel=1e-12)
     assert np.max(np.abs(f[0] - (np.log(ra) + 0.5 * a * rb) * dv)) < 1e-14
     assert np.max(np.abs(f[1] - (np.log(rb) + 0.5 * a * ra) * dv)) < 1e-14
+
+
+def test_ddft_restart_limit_and_non_finite_iterate():
+    # the restatement's guards (the library's): a step that restarts more than 30 times, or whose iterate
+    # is not finite, ends the run with an error
+    shape, dx = (8, 8, 8), 0.5
+    orc = O.Oracle(shape, dx, model="none")
+    x = np.arange(8)
+    rho = (0.3 * (1 + 0.1 * np.cos(2 * PI * x / 8)))[:, None, None] * np.ones(shape)
+    with pytest.raises(RuntimeError):
+        orc.ddft_run(rho, 0.0, 1, dt=1e-2, fp_tolerance=1e-300, fp_max_iterations=1)
+    bad = rho.copy()
+    bad[1, 2, 3] = np.nan
+    with pytest.raises(RuntimeError):
+        orc.ddft_run(bad, 0.0, 1, dt=1e-2)
+    # a reachable tolerance runs normally
+    out, res, _ = orc.ddft_run(rho, 0.0, 2, dt=1e-2, dt_max=1e-2)
+    assert res["time"] == pytest.approx(2e-2, rel=1e-14)
+
+
+def test_ddft_reservoir_boundary():
+    # reservoir_boundary (dynamics.hpp:545-553): marked cells are reset after every step; no mark leaves
+    # the run unchanged
+    shape, dx = (8, 8, 8), 0.5
+    orc = O.Oracle(shape, dx, model="none")
+    rng = np.random.default_rng(8)
+    rho = 0.3 + 0.05 * rng.random(shape)
+    mask = np.zeros(shape, dtype=bool)
+    mask[0] = True
+    out, res, _ = orc.ddft_run(rho, 0.0, 3, dt=1e-2, dt_max=1e-2, reservoir_mask=mask, reservoir_density=0.2)
+    assert np.all(out[0] == 0.2)
+    start = rho.copy()
+    start[0] = 0.2
+    assert res["mass_initial"] == pytest.approx(float(np.sum(start)) * dx ** 3, rel=1e-14)
+    plain, _, _ = orc.ddft_run(rho, 0.0, 3, dt=1e-2, dt_max=1e-2)
+    none, _, _ = orc.ddft_run(rho, 0.0, 3, dt=1e-2, dt_max=1e-2, reservoir_mask=np.zeros(shape, dtype=bool))
+    assert np.array_equal(plain, none)
+    # mass flows: the unmarked cells differ from the closed run
+    assert np.max(np.abs(out[1:] - plain[1:])) > 1e-6
