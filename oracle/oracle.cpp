@@ -814,6 +814,57 @@ struct FireState {
   bool converged;
 };
 
+// Fire::step (fire.hpp:86-165): one FIRE2 step of (x, v) from the forces f
+// at x; compute(x, f) returns the energy at the new x and fills f there.
+// Returns false when the uphill limit is exceeded (the reference throws).
+template <typename Compute>
+bool fire_step(const FireParams& fp, FireState& st, std::vector<double>& x, std::vector<double>& v,
+               std::vector<double>& f, Compute&& compute) {
+  const size_t n = x.size();
+  double P = 0.0;
+  for (size_t i = 0; i < n; ++i) P += v[i] * f[i];
+  if (P > 0.0 || st.iteration == 0) {
+    st.n_positive++;
+    st.n_negative = 0;
+    if (st.n_positive > fp.n_delay) {
+      st.dt = std::min(st.dt * fp.f_inc, fp.dt_max);
+      st.alpha *= fp.f_alpha;
+    }
+  } else {
+    st.n_positive = 0;
+    st.n_negative++;
+    if (st.n_negative > fp.max_uphill) return false;
+    for (size_t i = 0; i < n; ++i) {  // back up half a step with the current dt, stop
+      x[i] -= 0.5 * st.dt * v[i];
+      v[i] = 0.0;
+    }
+    if (st.iteration > fp.n_delay) {
+      st.dt = std::max(st.dt * fp.f_dec, fp.dt_min);
+      st.alpha = fp.alpha_start;
+    }
+  }
+  for (size_t i = 0; i < n; ++i) v[i] += st.dt * f[i];
+  double vn = 0.0, fn = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    vn += v[i] * v[i];
+    fn += f[i] * f[i];
+  }
+  vn = std::sqrt(vn);
+  fn = std::sqrt(fn);
+  if (fn > 0.0) {
+    const double ratio = vn / fn;
+    for (size_t i = 0; i < n; ++i) v[i] = (1.0 - st.alpha) * v[i] + st.alpha * ratio * f[i];
+  }
+  for (size_t i = 0; i < n; ++i) x[i] += st.dt * v[i];
+  st.energy = compute(x, f);
+  double s2 = 0.0;
+  for (size_t i = 0; i < n; ++i) s2 += f[i] * f[i];
+  st.rms_force = std::sqrt(s2 / static_cast<double>(n));
+  st.iteration++;
+  st.converged = st.rms_force < fp.force_tolerance;
+  return true;
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -909,48 +960,10 @@ int minimize(Ctx* c, const double* fire_p, int param_kind, double rho_min, doubl
   int rc = 0;
   for (int it = 0; it < fp.max_steps && !converged; ++it) {
     double old_e = st.energy;
-    // Fire::step (fire.hpp:86-165)
-    double P = 0.0;
-    for (long i = 0; i < N; ++i) P += v[i] * f[i];
-    if (P > 0.0 || st.iteration == 0) {
-      st.n_positive++;
-      st.n_negative = 0;
-      if (st.n_positive > fp.n_delay) {
-        st.dt = std::min(st.dt * fp.f_inc, fp.dt_max);
-        st.alpha *= fp.f_alpha;
-      }
-    } else {
-      st.n_positive = 0;
-      st.n_negative++;
-      if (st.n_negative > fp.max_uphill) { rc = -1; break; }
-      for (long i = 0; i < N; ++i) {
-        x[i] -= 0.5 * st.dt * v[i];
-        v[i] = 0.0;
-      }
-      if (st.iteration > fp.n_delay) {
-        st.dt = std::max(st.dt * fp.f_dec, fp.dt_min);
-        st.alpha = fp.alpha_start;
-      }
+    if (!fire_step(fp, st, x, v, f, compute)) {
+      rc = -1;
+      break;
     }
-    for (long i = 0; i < N; ++i) v[i] += st.dt * f[i];
-    double vn = 0.0, fn = 0.0;
-    for (long i = 0; i < N; ++i) {
-      vn += v[i] * v[i];
-      fn += f[i] * f[i];
-    }
-    vn = std::sqrt(vn);
-    fn = std::sqrt(fn);
-    if (fn > 0.0) {
-      double ratio = vn / fn;
-      for (long i = 0; i < N; ++i) v[i] = (1.0 - st.alpha) * v[i] + st.alpha * ratio * f[i];
-    }
-    for (long i = 0; i < N; ++i) x[i] += st.dt * v[i];
-    st.energy = compute(x, f);
-    double s2 = 0.0;
-    for (long i = 0; i < N; ++i) s2 += f[i] * f[i];
-    st.rms_force = std::sqrt(s2 / static_cast<double>(N));
-    st.iteration++;
-    st.converged = st.rms_force < fp.force_tolerance;
     // convergence_monitor (minimization.hpp:105-110)
     vv = vv * 0.9 + 0.1 * (std::fabs(st.energy - old_e) / st.dt);
     double monitor = std::fabs(vv / volume);
@@ -1811,6 +1824,47 @@ void orc_picard(void* h, const double* cfg, const double* vext, double mu, const
   out[1] = residual;
   out[2] = it;
   out[3] = (residual < tol || (it > 0 && std::fabs(omega - omega_prev) < tol)) ? 1.0 : 0.0;
+}
+
+// Fire::initialize then `steps` Fire::step calls (fire.hpp:60-165) over any
+// force function: force(user, x, f) returns the energy at x and fills the
+// forces f.  trace (steps + 1 rows of 2n + 6 values: x, v, dt, alpha,
+// n_positive, n_negative, energy, rms force) records the state after
+// initialisation and after every step.  Returns the steps taken, or
+// -(steps taken + 1) when the uphill limit was exceeded.
+int orc_fire_trace(const double* fire_p, int n, const double* x0, int steps,
+                   double (*force)(void* user, const double* x, double* f), void* user, double* trace) {
+  FireParams fp{fire_p[0], fire_p[1], fire_p[2], fire_p[3], fire_p[4], fire_p[5], fire_p[6],
+                static_cast<int>(fire_p[7]), static_cast<int>(fire_p[8]), fire_p[9], static_cast<int>(fire_p[10])};
+  std::vector<double> x(x0, x0 + n), v(static_cast<size_t>(n), 0.0), f(static_cast<size_t>(n));
+  auto compute = [&](const std::vector<double>& xs, std::vector<double>& fs) { return force(user, xs.data(), fs.data()); };
+  FireState st{};
+  st.energy = compute(x, f);
+  double s2 = 0.0;
+  for (double g : f) s2 += g * g;
+  st.dt = fp.dt;
+  st.alpha = fp.alpha_start;
+  st.rms_force = std::sqrt(s2 / static_cast<double>(n));
+  const size_t row = 2 * static_cast<size_t>(n) + 6;
+  auto record = [&](int k) {
+    double* r = trace + row * static_cast<size_t>(k);
+    for (int i = 0; i < n; ++i) {
+      r[i] = x[static_cast<size_t>(i)];
+      r[n + i] = v[static_cast<size_t>(i)];
+    }
+    r[2 * n] = st.dt;
+    r[2 * n + 1] = st.alpha;
+    r[2 * n + 2] = st.n_positive;
+    r[2 * n + 3] = st.n_negative;
+    r[2 * n + 4] = st.energy;
+    r[2 * n + 5] = st.rms_force;
+  };
+  record(0);
+  for (int k = 0; k < steps; ++k) {
+    if (!fire_step(fp, st, x, v, f, compute)) return -(k + 1);
+    record(k + 1);
+  }
+  return steps;
 }
 
 // Fixed-dt forward map of the DDFT adjoint: n_steps steps of fixed_step.

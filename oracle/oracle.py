@@ -83,8 +83,37 @@ def lib():
         L.orc_saddle_search.restype = C.c_int
         L.orc_saddle_search.argtypes = [C.c_void_p, _dp, _dp, _dp, C.c_double, C.c_double, C.c_ulonglong, _dp, _dp,
                                         _dp]
+        L.orc_fire_trace.restype = C.c_int
+        L.orc_fire_trace.argtypes = [_dp, C.c_int, _dp, C.c_int, _FIRE_FORCE, C.c_void_p, _dp]
         _lib = L
     return _lib
+
+
+_FIRE_FORCE = C.CFUNCTYPE(C.c_double, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double))
+
+
+def fire_trace(x0, force, steps, **fire):
+    """Fire::initialize and `steps` Fire::step calls over force(x) -> (energy, forces).  Returns
+    (trace, status): trace rows are (x, v, dt, alpha, n_positive, n_negative, energy, rms force) after
+    initialisation and after each step; status is the steps taken, or -(k + 1) when step k exceeded the
+    uphill limit."""
+    f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99, n_delay=5,
+             max_uphill=20, force_tolerance=0.1, max_steps=10000)
+    f.update(fire)
+    fp = np.array([f[k] for k in ("dt", "dt_max", "dt_min", "alpha_start", "f_inc", "f_dec", "f_alpha", "n_delay",
+                                  "max_uphill", "force_tolerance", "max_steps")], dtype=np.float64)
+    x0 = np.ascontiguousarray(x0, dtype=np.float64).reshape(-1)
+    n = x0.size
+
+    def cb(_u, xp, fp_out):
+        e, g = force(np.ctypeslib.as_array(xp, shape=(n,)).copy())
+        np.ctypeslib.as_array(fp_out, shape=(n,))[:] = g
+        return float(e)
+
+    fn = _FIRE_FORCE(cb)
+    trace = np.zeros((steps + 1, 2 * n + 6))
+    st = lib().orc_fire_trace(_p(fp), n, _p(x0), int(steps), fn, None, _p(trace))
+    return trace, int(st)
 
 
 def _p(a):
