@@ -23,6 +23,12 @@ import sys
 import time
 
 METRIC = "F+δF/δρ evals/s at 128³: mixed vs FP64 on GPU, Ω rel err ≤1e-6"
+W_BATCH_SIZES = (48, 64, 80, 96, 128, 160)  # the batched workload's grid sizes
+
+
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
 PEAK_HBM_GBPS = 8000.0
 
 
@@ -83,11 +89,11 @@ def measured_traffic(kernel, precision):
     return None if e is None else e["hbm_bytes_per_launch"]
 
 
-def batched_fire(cd, W, precision, n=16, steps=40, sizes=(48, 64, 80, 96)):
-    """Batched form (a bounded sample of the 64-state-point workload): n seeded LJ state points of uneven
-    grid size (kT ~ U(0.7, 1.2), bulk rho ~ U(0.05, 0.80), droplet or slab), `steps` FIRE steps each, run
-    one after another and then advanced in lockstep by cdft_minimize_grand_potential_batch.  Every context is
-    autotuned first (the general-schedule sizes are timed once per size)."""
+def batched_fire(cd, W, precision, n=64, steps=500, sizes=W_BATCH_SIZES):
+    """Batched form: n seeded LJ state points of uneven grid size (sizes drawn from 48^3 .. 160^3, kT ~
+    U(0.7, 1.2), bulk rho ~ U(0.05, 0.80), droplet or slab), `steps` FIRE steps each, run one after another
+    and then advanced together by cdft_minimize_grand_potential_batch.  Every context is autotuned first (the
+    general-schedule sizes are timed once per size)."""
     import torch
 
     pts = W.batch_state_points(n, sizes=sizes)
@@ -107,7 +113,9 @@ def batched_fire(cd, W, precision, n=16, steps=40, sizes=(48, 64, 80, 96)):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     seq = []
-    for ctx, r0, mu in zip(ctxs, rho0s, mus):
+    for i, (ctx, r0, mu) in enumerate(zip(ctxs, rho0s, mus)):
+        if i % 16 == 0:
+            progress(f"batched: sequential run {i}/{len(ctxs)}")
         try:
             _, res, _ = ctx.minimize_grand_potential(r0, mu, fire=fire, max_steps=steps, record_energies=False)
             seq.append(res["iterations"])
@@ -136,7 +144,7 @@ def batched_fire(cd, W, precision, n=16, steps=40, sizes=(48, 64, 80, 96)):
     }
 
 
-def adjoint_ddft(cd, W, shape=(128, 128, 128), K=400, P=2, dt=1e-4, n_wells=16, fd_all=False):
+def adjoint_ddft(cd, W, shape=(128, 128, 128), K=400, P=2, dt=1e-4, n_wells=16, fd_all=True):
     """The DDFT adjoint workload (FP64): hard-sphere (White-Bear II) + LJ fluid at kT = 0.9 on `shape`, dx = 0.1,
     n_wells seeded Gaussian wells, loss = sum (rho_K - target)^2 dV after K fixed-dt steps (P fixed-point
     iterations each), target from the same run with perturbed amplitudes; the gradient by the checkpointed
@@ -164,7 +172,9 @@ def adjoint_ddft(cd, W, shape=(128, 128, 128), K=400, P=2, dt=1e-4, n_wells=16, 
     h = 1e-5
     checks = [(j, k) for j in (range(n_wells) if fd_all else [0]) for k in (0, 1, 4)]
     errs = []
-    for j, k in checks:
+    for i, (j, k) in enumerate(checks):
+        if i % 12 == 0:
+            progress(f"adjoint: finite differences {i}/{len(checks)}")
         pp, pm = prm.copy(), prm.copy()
         pp[j, k] += h
         pm[j, k] -= h
@@ -244,32 +254,49 @@ def stream_copy_gbps(reps=5):
     return sorted(rates)[len(rates) // 2]
 
 
-def converged_profile(cd, W, rho0, shape, dx, kT, steps=500):
-    """The headline droplet relaxed by Minimizer::grand_potential (reference defaults: Unbounded alias,
-    homogeneous boundary) for a fixed number of FIRE steps in mixed precision and in FP64 on the GPU.
-    Reports how far the mixed profile and Omega end from the FP64 ones, and both step rates."""
+def converged_profile(cd, W, rho0, shape, dx, kT, tol=1e-8, cap=60000):
+    """The headline droplet relaxed by Minimizer::grand_potential (Unbounded alias, no boundary averaging:
+    a periodic box) until max |delta(beta Omega)/delta rho| (no dV) < tol: FP64 on the GPU, then mixed
+    precision with the same rule and a step cap of 1.5x the FP64 run (its FP32 rounding leaves a floor).
+    At mu = mu_bulk(rho_v) the droplet is a critical nucleus and evaporates, so the converged profile is the
+    uniform vapor rho = rho_v, which the bulk equation of state fixes exactly.  Reports steps, step rates,
+    the residual max force of each final profile under the FP64 functional, the FP64 profile's distance
+    from rho_v, and how far the mixed profile and Omega end from the FP64 ones."""
     import numpy as np
     import torch
 
-    out = {"steps": steps}
+    out = {"rule": f"max |delta(beta Omega)/delta rho| < {tol:g} (no dV), homogeneous_boundary off", "tol": tol}
+    ctx64 = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=kT, model="WhiteBearII", precision="fp64")
+    mu = ctx64.bulk_mu(0.02)
+    dv = dx ** 3
+
+    def max_force(r):
+        _, f = ctx64.total(r, mu)
+        return float(np.max(np.abs(f)) / dv)
+
     res = {}
-    for prec in ("mixed", "fp64"):
-        ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=kT, model="WhiteBearII", precision=prec)
-        mu = ctx.bulk_mu(0.02)
-        fire = dict(force_tolerance=1e-12)  # run every step
-        ctx.minimize_grand_potential(rho0, mu, fire=fire, max_steps=2, record_energies=False)  # warm-up
+    for prec in ("fp64", "mixed"):
+        ctx = ctx64 if prec == "fp64" else cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=kT,
+                                                       model="WhiteBearII", precision=prec)
+        steps = cap if prec == "fp64" else max(2000, int(1.5 * res["fp64"][1]["iterations"]))
+        fire = dict(force_tolerance=0.0, force_max_tolerance=tol)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        r, info, _ = ctx.minimize_grand_potential(rho0, mu, fire=fire, max_steps=steps, record_energies=False)
+        r, info, _ = ctx.minimize_grand_potential(rho0, mu, fire=fire, max_steps=steps, homogeneous_boundary=False,
+                                                  record_energies=False)
         t = time.perf_counter() - t0
         res[prec] = (r, info)
-        out[f"{prec}_fire_steps_per_s"] = info["iterations"] / t
         out[f"{prec}_iterations"] = info["iterations"]
-    (rm, im), (rd, idd) = res["mixed"], res["fp64"]
+        out[f"{prec}_reached_tol"] = bool(info["converged"])
+        out[f"{prec}_fire_steps_per_s"] = info["iterations"] / t
+        out[f"{prec}_max_force_fp64_functional"] = max_force(r)
+        print(f"converged profile {prec}: {info['iterations']} steps in {t:.1f} s", file=sys.stderr, flush=True)
+    (rd, idd), (rm, im) = res["fp64"], res["mixed"]
+    out["fp64_max_abs_dev_from_rho_v"] = float(np.max(np.abs(rd - 0.02)))
+    out["mixed_floor_max_force"] = out["mixed_max_force_fp64_functional"]
     out["profile_max_rel_err"] = float(np.max(np.abs(rm - rd)) / np.max(np.abs(rd)))
     out["omega_rel_err"] = abs(im["grand_potential"] - idd["grand_potential"]) / abs(idd["grand_potential"])
     out["omega_fp64"] = idd["grand_potential"]
-    out["rms_force_fp64"] = idd["rms_force"]
     return out
 
 
@@ -313,12 +340,15 @@ def main():
     ap.add_argument("--no-mixture", action="store_true", help="skip the binary-mixture evaluation measurement")
     ap.add_argument("--mixture-untuned", action="store_true",
                     help="time the mixture on the default launch shapes (no autotuning trials in a profile)")
-    ap.add_argument("--profile-steps", type=int, default=500,
-                    help="FIRE steps of the mixed-vs-FP64 converged-profile comparison (0 skips it)")
+    ap.add_argument("--no-converged", action="store_true",
+                    help="skip relaxing the droplet to max |dOmega/drho| < 1e-8 (mixed vs FP64)")
     ap.add_argument("--ddft-steps", type=int, default=2000,
                     help="steps of the 256x128x128 DDFT slab run (0 skips it)")
-    ap.add_argument("--adjoint-fd-all", action="store_true",
-                    help="check the adjoint gradient against differences for (A, w, cz) of every well")
+    ap.add_argument("--adjoint-fd-sample", action="store_true",
+                    help="check the adjoint gradient against differences for well 0 only (default: all 48)")
+    ap.add_argument("--reps", type=int, default=5, help="timed repetitions of K evaluations (median reported)")
+    ap.add_argument("--batch-points", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -353,20 +383,33 @@ def main():
     d_rho = torch.from_numpy(rho.copy()).to(dev)
     d_f = torch.empty_like(d_rho)
 
-    def run(precision, steps, warmup):
+    def run(precision, steps, warmup, reps):
+        """`reps` timed blocks of `steps` back-to-back evaluations after `warmup` untimed ones; each block is
+        bracketed by device synchronisation and timed with HIP events on the context stream.  Returns the
+        per-block milliseconds (the median is reported) and the wall time of all blocks."""
         ctx = cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=kT, model="WhiteBearII", precision=precision)
         mu = ctx.bulk_mu(0.02)
-        ctx.time_total_device(d_rho.data_ptr(), mu, warmup, d_forces=d_f.data_ptr())
+        ctx.time_total_device(d_rho.data_ptr(), mu, max(warmup, 1), d_forces=d_f.data_ptr())
+        blocks = []
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ms = ctx.time_total_device(d_rho.data_ptr(), mu, steps, d_forces=d_f.data_ptr())
-        torch.cuda.synchronize()
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            blocks.append(ctx.time_total_device(d_rho.data_ptr(), mu, steps, d_forces=d_f.data_ptr()))
+            torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         res = ctx.total_device(d_rho.data_ptr(), mu, d_forces=d_f.data_ptr())
         forces = d_f.cpu().numpy().reshape(shape)
-        return ctx, mu, ms, wall, res, forces
+        return ctx, mu, blocks, wall, res, forces
 
-    ctx, mu, ms, wall, res, forces = run(args.precision, args.steps, args.warmup)
+    def median(v):
+        v = sorted(v)
+        return v[len(v) // 2] if len(v) % 2 else 0.5 * (v[len(v) // 2 - 1] + v[len(v) // 2])
+
+    reps = max(1, args.reps)
+    progress(f"{args.precision} evaluation: {reps} x {args.steps} evaluations")
+    ctx, mu, blocks, wall, res, forces = run(args.precision, args.steps, args.warmup, reps)
+    ms = median(blocks)
     ms_per_step = ms / args.steps
     value = 1000.0 / ms_per_step
 
@@ -400,7 +443,10 @@ def main():
             "field_bytes": s_f,
             "state_bytes": 8,
         },
-        "wall_ms_per_step": wall * 1000.0 / args.steps,
+        "wall_ms_per_step": wall * 1000.0 / (args.steps * reps),
+        "timing": {"repetitions": reps, "evals_per_rep": args.steps,
+                   "evals_per_s_each_rep": [1000.0 * args.steps / b for b in blocks],
+                   "spread_rel": (max(blocks) - min(blocks)) / ms, "statistic": "median"},
         "eval_algorithmic_bytes": B,
         "eval_achieved_GBps": B * value / 1e9,
         "eval_roofline_frac": B * value / 1e9 / PEAK_HBM_GBPS,
@@ -420,8 +466,10 @@ def main():
     }
 
     if not args.no_fp64 and args.precision == "mixed":
-        ctx64, mu64, ms64, _, res64, f64 = run("fp64", max(10, args.steps // 4), max(2, args.warmup // 4))
-        out["fp64_value"] = 1000.0 * max(10, args.steps // 4) / ms64
+        progress(f"fp64 evaluation: {reps} x {args.steps} evaluations")
+        ctx64, mu64, blocks64, _, res64, f64 = run("fp64", args.steps, args.warmup, reps)
+        out["fp64_value"] = 1000.0 * args.steps / median(blocks64)
+        out["fp64_evals_per_s_each_rep"] = [1000.0 * args.steps / b for b in blocks64]
         out["speedup_vs_fp64"] = value / out["fp64_value"]
         out["omega_fp64_gpu"] = res64["grand_potential"]
         out["omega_rel_err_vs_fp64_gpu"] = abs(res["grand_potential"] - res64["grand_potential"]) / abs(
@@ -431,22 +479,28 @@ def main():
 
     out["stream_copy_GBps"] = stream_copy_gbps()
 
-    if args.profile_steps > 0:
-        out["converged_profile"] = converged_profile(cd, W, rho, shape, dx, kT, steps=args.profile_steps)
+    if not args.no_converged:
+        progress("converged profile")
+        out["converged_profile"] = converged_profile(cd, W, rho, shape, dx, kT)
 
     if not args.no_batch:
-        out["batched"] = batched_fire(cd, W, args.precision)
+        progress(f"batched: {args.batch_points} state points x {args.batch_steps} FIRE steps")
+        out["batched"] = batched_fire(cd, W, args.precision, n=args.batch_points, steps=args.batch_steps)
 
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_ddft(cd, W, fd_all=args.adjoint_fd_all)
+        progress("adjoint")
+        out["adjoint"] = adjoint_ddft(cd, W, fd_all=not args.adjoint_fd_sample)
 
     if args.ddft_steps > 0:
+        progress("ddft slab")
         out["ddft_slab"] = ddft_slab(cd, W, steps=args.ddft_steps)
 
     if not args.no_mixture:
+        progress("mixture")
         out["mixture"] = mixture(cd, W, autotune=not args.mixture_untuned)
 
     if not args.no_cpu_baseline:
+        progress("cpu baseline")
         cb, parts, fr = cpu_baseline(rho, mu, shape, dx)
         out["cpu_baseline"] = cb
         out["omega_oracle"] = parts[5]

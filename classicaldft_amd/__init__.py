@@ -70,7 +70,8 @@ class _Result(C.Structure):
 class _Fire(C.Structure):
     _fields_ = [("dt", C.c_double), ("dt_max", C.c_double), ("dt_min", C.c_double), ("alpha_start", C.c_double),
                 ("f_inc", C.c_double), ("f_dec", C.c_double), ("f_alpha", C.c_double), ("n_delay", C.c_int),
-                ("max_uphill", C.c_int), ("force_tolerance", C.c_double), ("max_steps", C.c_int)]
+                ("max_uphill", C.c_int), ("force_tolerance", C.c_double), ("max_steps", C.c_int),
+                ("force_max_tolerance", C.c_double)]
 
 
 class _Param(C.Structure):
@@ -297,7 +298,7 @@ def _f64(a):
 class Context:
     """One physics::Model (grid, species, interactions, kT) plus its weights on one GPU.
 
-    Mirrors functionals::make_functional (functional.hpp:195-206); `total`
+    Mirrors functionals::make_functional (functional.hpp:93-107); `total`
     mirrors functionals::total (functionals.hpp:64-95).
     """
 
@@ -509,7 +510,7 @@ class Context:
     def minimize_grand_potential(self, rho0, mu, fire=None, param="unbounded", rho_min=1e-18, rho_max=1.0,
                                  homogeneous_boundary=True, vext=None, max_steps=None, record_energies=True):
         f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
-                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
+                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000, force_max_tolerance=0.0)
         if fire:
             f.update(fire)
         if max_steps is not None:
@@ -531,7 +532,7 @@ class Context:
                             homogeneous_boundary=True, vext=None, max_steps=None, record_energies=True):
         """Minimizer::fixed_mass: FIRE at fixed total mass; returns (rho, result, F after each step)."""
         f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
-                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
+                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000, force_max_tolerance=0.0)
         if fire:
             f.update(fire)
         if max_steps is not None:
@@ -586,7 +587,7 @@ class Context:
                       max_outer_iterations=20, eigenvalue_clear_count=2, vext=None):
         """saddle_point::Search::find (critical cluster at fixed mass); returns (rho, eigenvector, result dict)."""
         f = dict(dt=1e-3, dt_max=1.0, dt_min=1e-8, alpha_start=0.01, f_inc=1.1, f_dec=0.5, f_alpha=1.0, n_delay=5,
-                 max_uphill=20, force_tolerance=1e-4, max_steps=100000)
+                 max_uphill=20, force_tolerance=1e-4, max_steps=100000, force_max_tolerance=0.0)
         f.update(fire or {})
         e = dict(tolerance=1e-4, max_iterations=300, hessian_eps=1e-6)
         e.update(eigen or {})
@@ -703,7 +704,7 @@ def minimize_grand_potential_batch(contexts, rho0s, mus, fire=None, param="unbou
     if not (len(rho0s) == len(mus) == n):
         raise ValueError("contexts, rho0s and mus differ in length")
     f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
-             n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
+             n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000, force_max_tolerance=0.0)
     if fire:
         f.update(fire)
     if max_steps is not None:

@@ -332,7 +332,7 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       }
       CDFT_HIP(hipFree(d_spec));
     }
-    // k = 0 weights of the forward channels (weights.hpp:174-222, mean_field.hpp:189-191)
+    // k = 0 weights of the forward channels (weights.hpp:91-184, mean_field.hpp:189-191)
     {
       const double R = g.R;
       int ch = 0;

@@ -60,9 +60,9 @@ __device__ __forceinline__ cx<D> cvt(cx<T> a) {
 __device__ __forceinline__ double kvec(int i, int n, double dk) { return dk * static_cast<double>(i <= n / 2 ? i : i - n); }
 
 // Radial FMT functions w3(k) = 4 pi (sin kR - kR cos kR)/k^3, w2(k) = 4 pi R sin(kR)/k
-// (weights.hpp:174-191), each times 1/N.  The vector channel prefactor of
-// weights.hpp:195-201 equals w3 and the tensor coefficients of
-// weights.hpp:206-222 are A = w2 - 3 w3/R, B = w3/R (identities of the closed forms).
+// (weights.hpp:91-108,159-162), each times 1/N.  The vector channel prefactor of
+// weights.hpp:110-119,164-168 equals w3 and the tensor coefficients of
+// weights.hpp:121-137,170-178 are A = w2 - 3 w3/R, B = w3/R (identities of the closed forms).
 __device__ __forceinline__ void radial(double k, double R, double inv_n, double* w3, double* w2) {
   const double PI = 3.14159265358979323846;
   if (k < 1e-12) {
@@ -86,7 +86,7 @@ __device__ __forceinline__ WRad wrad(cx<double> w, double k2, double invR) {
   return {w.re, w.im, k2 > 0.0 ? (w.im - 3.0 * w.re * invR) / k2 : 0.0};
 }
 // Weight of channel ch from the radial factors (the closed forms of
-// weights.hpp:174-222): ch 0 w3, 1 w2, 2..4 i k_a w3 (conj when `adj`), 5..10
+// weights.hpp:91-184): ch 0 w3, 1 w2, 2..4 i k_a w3 (conj when `adj`), 5..10
 // tensor aik2 k_i k_j + delta_ij w3 / R, 11 the mean-field table value.
 __device__ __forceinline__ cx<double> channel_weight_t(int ch, bool adj, double kx, double ky, double kz, WRad w,
                                                        double mf, double invR) {

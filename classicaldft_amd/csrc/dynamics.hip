@@ -91,11 +91,15 @@ __global__ void __launch_bounds__(256) k_face_sum(const double* __restrict__ g, 
 // With fixed mass (minimization.hpp:207-211) the Lagrange multiplier
 // lambda = lam[0] / target (lam[0] = sum g rho after the boundary average)
 // shifts every component by -lambda dV first; lam is null otherwise.
+// With mpart set, the block maxima of |g| / dV (the density-space force after
+// the boundary average and the multiplier, without dV) go to mpart[b*8] for
+// the optional max-force stop.
 __global__ void __launch_bounds__(256) k_fire_force(const double* __restrict__ g, const double* __restrict__ x,
                                                     const double* __restrict__ v, double* __restrict__ f, Geom G,
                                                     int homog, const double* __restrict__ face, long nb, AliasMap a,
-                                                    const double* __restrict__ lam, double target, double* partials) {
-  double s2 = 0, c2 = 0, sp = 0, cp = 0;
+                                                    const double* __restrict__ lam, double target, double* partials,
+                                                    double* mpart) {
+  double s2 = 0, c2 = 0, sp = 0, cp = 0, gmax = 0.0;
   const double avg = (homog && nb > 0) ? face[0] / static_cast<double>(nb) : 0.0;
   const double shift = lam ? (lam[0] / target) * G.dv : 0.0;
   for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < G.N;
@@ -103,6 +107,8 @@ __global__ void __launch_bounds__(256) k_fire_force(const double* __restrict__ g
     double gi = g[i];
     if (homog && on_face(i, G)) gi = avg;
     gi -= shift;
+    // fmax drops a NaN operand: a non-finite force counts as infinite
+    gmax = fmax(gmax, isfinite(gi) ? fabs(gi) : INFINITY);
     const double fi = -(gi * a.deriv(x[i]));
     f[i] = fi;
     acc2(s2, c2, fi * fi);
@@ -110,6 +116,19 @@ __global__ void __launch_bounds__(256) k_fire_force(const double* __restrict__ g
   }
   const double vals[2] = {s2 + c2, sp + cp};
   block_sums(vals, 2, partials);
+  if (mpart) {
+    __shared__ double smx[256];
+    smx[threadIdx.x] = gmax;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+      if (threadIdx.x < w) smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + w]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      mpart[static_cast<long>(blockIdx.x) * 8 + 0] = smx[0] / G.dv;
+      mpart[static_cast<long>(blockIdx.x) * 8 + 1] = 0.0;
+    }
+  }
 }
 
 // Fixed mass (minimization.hpp:196-201): rho *= target / (sum rho dV) when the mass is positive.
@@ -300,8 +319,9 @@ __global__ void k_fire_end(FireCtl* ctl, const double* __restrict__ res, const d
   c.energy = fire_omega(res, k.mu);
   ++c.iteration;
   c.vv = c.vv * 0.9 + 0.1 * (fabs(c.energy - c.old_energy) / c.dt);
-  const bool conv = k.rms_stop ? sqrt(fres[0] / k.ndof) < k.p.force_tolerance
-                               : fabs(c.vv / k.volume) < k.p.force_tolerance;
+  bool conv = k.rms_stop ? sqrt(fres[0] / k.ndof) < k.p.force_tolerance
+                         : fabs(c.vv / k.volume) < k.p.force_tolerance;
+  if (k.p.force_max_tolerance > 0.0 && fres[48] < k.p.force_max_tolerance) conv = true;  // the max-force stop
   if (conv) c.state = FS_CONVERGED;
   else if (c.iteration >= k.p.max_steps) c.state = FS_MAX_STEPS;
   if (energies) energies[c.iteration] = c.energy;
@@ -376,8 +396,8 @@ struct FireRun {
   const double* d_vext;
   FireConst k;
   // res: [0..7] force sums, [8..15] |v|^2, [16..23] face sum, [24..31] mass sum, [32..39] sum g rho,
-  // [40..47] dir . f
-  DevBuf x, v, f, part, res, ctl, en;
+  // [40..47] dir . f, [48..55] max |force| / dV
+  DevBuf x, v, f, part, mpart, res, ctl, en;
   long nb;
   bool fixed_mass = false;  // Minimizer::fixed_mass: rescale rho to `target`, mu = 0 in the loop
   double target = 0.0;
@@ -390,7 +410,8 @@ struct FireRun {
 
   FireRun(Context* c_, AliasMap a, int h, const double* vx, double m, const cdft_fire& fr, bool record)
       : c(c_), amap(a), homog(h), d_vext(vx), x(8 * c_->g.N), v(8 * c_->g.N), f(8 * c_->g.N),
-        part(8 * 8 * kVecBlocks), res(8 * 48), ctl(sizeof(FireCtl)),
+        part(8 * 8 * kVecBlocks), mpart(fr.force_max_tolerance > 0.0 ? 8 * 8 * kVecBlocks : 0), res(8 * 56),
+        ctl(sizeof(FireCtl)),
         en(record ? 8 * (static_cast<size_t>(std::max(fr.max_steps, 0)) + 1) : 0) {
     k.p = fr;
     k.mu = m;
@@ -459,8 +480,11 @@ struct FireRun {
     hipLaunchKernelGGL(k_fire_force, dim3(nbk), dim3(256), 0, c->stream, c->d_force, x.as<double>(),
                        v.as<double>(), f.as<double>(), g, homog, res.as<double>() + 16, nb, amap,
                        fixed_mass ? static_cast<const double*>(res.as<double>() + 32) : nullptr, target,
-                       part.as<double>());
+                       part.as<double>(), mpart.as<double>());
     launch_finalize(part.as<double>(), static_cast<int>(nbk), res.as<double>(), c->stream);
+    if (mpart.p)
+      hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, mpart.as<double>(),
+                         static_cast<int>(nbk), res.as<double>() + 48);
   }
 
   // one Fire::step (fire.hpp:86-165) with its compute and the monitor
@@ -690,6 +714,7 @@ int cdft_saddle_search(cdft_ctx* ctx, const cdft_saddle_config* config, const do
       // 3. FIRE at fixed interior mass with the eigen-direction projection (saddle_point.hpp:449-520)
       cdft_fire fc = config->fire;
       fc.force_tolerance = std::max(ftol, config->fire.force_tolerance);
+      fc.force_max_tolerance = 0.0;  // the search stops on FIRE's rms force only
       ftol /= config->tolerance_factor;
       FireRun run(c, amap, 0, dvx, mu, fc, false);
       run.saddle = true;

@@ -61,7 +61,7 @@ __device__ __forceinline__ long split_idx(int kz, int r, int ix, int m, int mb, 
 }
 
 // Channel bookkeeping of the fused schedule.  Two identities of the FMT
-// weights (weights.hpp:195-222) spare four plane transforms per evaluation:
+// weights (weights.hpp:110-137,164-178) spare four plane transforms per evaluation:
 //   n_v,z = i kz w3 rho_hat = i kz eta_hat, which the z-line kernel forms
 //     from eta's z-spectrum (kz is fixed along a z-line's spectrum), and
 //   T_zz = n2 - T_xx - T_yy, since sum_i (aik2 k_i^2 + w3/R) = w2;
@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
     for (int i = 0; i < kZB; ++i) outv[i] = S(0);
     const double rho = amap.apply(rho_in[p]);
     const double lr = log(fmax(rho, 1e-300));
-    double lfv = (lr - mu) * dv;  // ideal_gas.hpp:312-318
+    double lfv = (lr - mu) * dv;  // ideal_gas.hpp:27-33
     acc[0] += rho * (lr - 1.0);
     acc[4] += rho;
     if (vext) {  // external_field.hpp:16-31
@@ -676,7 +676,7 @@ struct PlaneCfg {
 
 // Weight factors of one element in arithmetic C: w3, w2 and
 // aik2 = (w2 - 3 w3/R) / k^2 (0 at k = 0), from which every FMT channel
-// weight follows without a division (weights.hpp:174-222): ch 0 w3, 1 w2,
+// weight follows without a division (weights.hpp:91-184): ch 0 w3, 1 w2,
 // 2..4 i k_a w3, 5..10 aik2 k_i k_j + delta_ij w3/R.
 template <typename C>
 struct WFac {
@@ -695,7 +695,7 @@ __device__ __forceinline__ WFac<C> wfactors(cx<double> w, double kx, double ky, 
   return {static_cast<C>(w.re), static_cast<C>(w.im), k2 > C(0) ? a : C(0)};
 }
 
-// FP64 radial FMT weights (w3, w2) (weights.hpp:174-191, times 1/N) of every
+// FP64 radial FMT weights (w3, w2) (weights.hpp:91-108,159-162, times 1/N) of every
 // element of the half spectrum, in the plane kernels' order [kz][r][s][kx]
 // (ky = r + 2 s).  They depend only on the grid and the radius: computed once.
 __global__ void __launch_bounds__(256) kf_wtable(cx<double>* __restrict__ WT, FastGeom g) {
@@ -713,7 +713,7 @@ __global__ void __launch_bounds__(256) kf_wtable(cx<double>* __restrict__ WT, Fa
     WT[i] = {w3, w2};
   }
 }
-// Weight of channel ch (weights.hpp:195-222; `adj` conjugates it, so the odd
+// Weight of channel ch (weights.hpp:110-137,164-178; `adj` conjugates it, so the odd
 // vector channels change sign, as the back-convolution does,
 // hard_sphere.hpp:118-134) as a linear form in the element's weight factors:
 // ch 0 w3, 1 w2, 2..4 i k_a w3, 5..10 aik2 k_i k_j + delta_ij w3/R, i.e.
@@ -1019,7 +1019,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
       const int col = xs + P::SBX * cc;
       return kvec(r + 2 * (col < Q ? col : 0), P::NY, g.dky);
     };
-    // FP64 channels: eta and n2 (weights.hpp:174-191), the mean field (FP64 table,
+    // FP64 channels: eta and n2 (weights.hpp:91-108,159-162), the mean field (FP64 table,
     // mean_field.hpp:145-196)
     for (int pos = p0; pos < min(p1, N64); ++pos, ++nch) {
       const int ch = fwd_channel(NF, pos);
@@ -1036,7 +1036,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
     if constexpr (NF >= 11) {
       if (p1 > N64) {
         // vector (i k_a w3) and tensor (aik2 k_i k_j + delta_ij w3/R) channels in CV,
-        // from the spectrum and per-element weight factors narrowed to CV (weights.hpp:195-222)
+        // from the spectrum and per-element weight factors narrowed to CV (weights.hpp:110-137,164-178)
         cx<CV> XC[CPS][8];
         WFac<CV> wf[CPS][8];
         CV kxc[8];

@@ -86,7 +86,7 @@ typedef struct {
 } cdft_options;
 
 /* Replaces functionals::make_weights + make_functional
- * (include/dft/functionals/functionals.hpp:27-33, functional.hpp:195-206):
+ * (include/dft/functionals/functionals.hpp:27-33, functional.hpp:93-107):
  * builds the FMT and mean-field weights for a physics::Model.  n_species is
  * 1..8.  A context of more than one species (a mixture; FP64 precision only,
  * up to 36 interactions between any species pairs) is evaluated by
@@ -107,7 +107,7 @@ CDFT_API int cdft_schedule(const cdft_ctx* ctx, int* schedule);
 /* InteractionWeight::a_vdw of interaction k (mean_field.hpp:23-28). */
 CDFT_API int cdft_a_vdw(const cdft_ctx* ctx, int k, double* a_vdw);
 /* BulkThermodynamics::chemical_potential for a one-component fluid with the
- * grid-synchronised a_vdw (bulk/thermodynamics.hpp:222-237, functional.hpp:196-205). */
+ * grid-synchronised a_vdw (bulk/thermodynamics.hpp:222-237, functional.hpp:93-102). */
 CDFT_API int cdft_bulk_chemical_potential(const cdft_ctx* ctx, double rho, double* mu);
 
 /* functionals::Result plus the per-term split of free_energy
@@ -212,12 +212,19 @@ CDFT_API int cdft_weighted_densities_noalloc(cdft_ctx* ctx, const double* rho, d
 CDFT_API int cdft_fft_r2c(cdft_ctx* ctx, const double* in, double* out_half);
 CDFT_API int cdft_fft_c2r(cdft_ctx* ctx, const double* in_half, double* out);
 
-/* fire::Fire (include/dft/algorithms/fire.hpp:33-44). */
+/* fire::Fire (include/dft/algorithms/fire.hpp:33-44).  force_max_tolerance
+ * has no counterpart in the reference: when positive, the Minimizer runs
+ * also stop once max |delta(beta Omega)/delta rho| (density space, without
+ * the dV factor, of the forces FIRE relaxes: after the homogeneous-boundary
+ * average and the fixed-mass multiplier) falls below it, checked after every
+ * step; zero (the default of a zero-initialised struct) leaves only the
+ * reference's energy monitor. */
 typedef struct {
   double dt, dt_max, dt_min, alpha_start, f_inc, f_dec, f_alpha;
   int n_delay, max_uphill;
   double force_tolerance;
   int max_steps;
+  double force_max_tolerance;
 } cdft_fire;
 
 /* minimization::Parametrization (minimization.hpp:20-39): kind 0 Unbounded
@@ -236,7 +243,7 @@ typedef struct {
 
 /* Minimizer::grand_potential (minimization.hpp:280-353): FIRE2 in the alias
  * variable at fixed mu.  vext (nullable) adds an external field, which the
- * reference's from_profile path always zeroes (init.hpp:264-279).
+ * reference's from_profile path always zeroes (init.hpp:28-42).
  * energies (nullable, max_steps + 1 doubles) receives state.energy after
  * initialisation and after every step.  Returns CDFT_ERUNTIME when FIRE
  * exceeds max_uphill. */
@@ -313,7 +320,7 @@ typedef struct {
 
 /* Simulation::run with integrating_factor_step (dynamics.hpp:333-401,582-693)
  * for one species.  The force callback is hooks->force, or (hooks or its
- * force null) the context's grand_potential_callback(mu) (functional.hpp:161-172)
+ * force null) the context's grand_potential_callback(mu) (functional.hpp:62-73)
  * plus an optional external field; hooks (nullable) may also carry the
  * boundary condition (project), the stop condition and the snapshot receiver.
  * rho is updated in place; energies and times (each nullable, n_steps + 1

@@ -275,7 +275,7 @@ inline bool all_zero(const vec& v) {
 }
 }  // namespace detail
 
-// functionals.hpp:27-33 (+ the grid-synchronised a_vdw of functional.hpp:195-206).
+// functionals.hpp:27-33 (+ the grid-synchronised a_vdw of functional.hpp:93-107).
 inline Weights make_weights(const fmt::FMTModel& fmt_model, const physics::Model& model,
                             Precision precision = Precision::FP64, int device = 0) {
   if (model.species.empty()) throw std::invalid_argument("the model has no species");
@@ -541,7 +541,7 @@ inline std::pair<double, std::vector<vec>> GrandPotentialCallback::operator()(co
   return {result.grand_potential, std::move(result.forces)};
 }
 
-// functional.hpp:195-206
+// functional.hpp:93-107
 inline Functional make_functional(const fmt::FMTModel& fmt_model, physics::Model model,
                                   Precision precision = Precision::FP64, int device = 0) {
   auto w = make_weights(fmt_model, model, precision, device);
@@ -728,7 +728,7 @@ struct Minimizer {
     if (static_cast<long>(initial_density.size()) != model.grid.total_points())
       throw std::invalid_argument("initial density size does not match the grid");
     cdft_fire f{fire.dt,      fire.dt_max,  fire.dt_min,     fire.alpha_start,    fire.f_inc,   fire.f_dec,
-                fire.f_alpha, fire.n_delay, fire.max_uphill, fire.force_tolerance, fire.max_steps};
+                fire.f_alpha, fire.n_delay, fire.max_uphill, fire.force_tolerance, fire.max_steps, 0.0};
     cdft_parametrization p{};
     if (auto* u = std::get_if<Unbounded>(&param)) {
       p.kind = 0;
@@ -831,7 +831,7 @@ struct Search {
       throw std::invalid_argument("initial density size does not match the grid");
     cdft_saddle_config cfg{};
     cfg.fire = {fire.dt,      fire.dt_max,  fire.dt_min,     fire.alpha_start,    fire.f_inc,   fire.f_dec,
-                fire.f_alpha, fire.n_delay, fire.max_uphill, fire.force_tolerance, fire.max_steps};
+                fire.f_alpha, fire.n_delay, fire.max_uphill, fire.force_tolerance, fire.max_steps, 0.0};
     cfg.eigen = {eigenvalue.tolerance, eigenvalue.max_iterations, eigenvalue.hessian_eps, 1};
     if (auto* u = std::get_if<minimization::Unbounded>(&param)) {
       cfg.param = {0, u->rho_min, 1.0};

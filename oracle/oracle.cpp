@@ -512,7 +512,7 @@ struct Ctx {
     fft = std::make_unique<Fft3d>(nx, ny, nz);
   }
 
-  // weights.hpp:174-222 and 229-269 (channel order w3, w2, wv2[x,y,z], wT xx,xy,xz,yy,yz,zz)
+  // weights.hpp:91-184 and 229-269 (channel order w3, w2, wv2[x,y,z], wT xx,xy,xz,yy,yz,zz)
   void make_fmt_weights() {
     fmtw.clear();
     for (double d : diam) {
@@ -804,6 +804,8 @@ struct FireParams {
   int n_delay, max_uphill;
   double force_tolerance;
   int max_steps;
+  // > 0: the library's max-force stop (no counterpart in the reference)
+  double force_max_tolerance = 0.0;
 };
 
 struct FireState {
@@ -875,9 +877,12 @@ int minimize(Ctx* c, const double* fire_p, int param_kind, double rho_min, doubl
                     double* rho_out, double* out, double* energies) {
   const bool fixed = target > 0.0;  // Minimizer::fixed_mass; otherwise grand_potential
   const long N = c->N;
+  // fire_p holds 12 values: the 11 FIRE parameters and the max-force stop
   FireParams fp{fire_p[0], fire_p[1], fire_p[2], fire_p[3], fire_p[4], fire_p[5], fire_p[6],
-                static_cast<int>(fire_p[7]), static_cast<int>(fire_p[8]), fire_p[9], static_cast<int>(fire_p[10])};
+                static_cast<int>(fire_p[7]), static_cast<int>(fire_p[8]), fire_p[9], static_cast<int>(fire_p[10]),
+                fire_p[11]};
   double range = rho_max - rho_min;
+  double gmax = 0.0;  // max |force| / dV of the last compute (density space, what FIRE relaxes)
   auto to_density = [&](const std::vector<double>& x, std::vector<double>& rho) {
     for (long i = 0; i < N; ++i) {
       double x2 = x[i] * x[i];
@@ -923,6 +928,10 @@ int minimize(Ctx* c, const double* fire_p, int param_kind, double rho_min, doubl
       const double lambda = d / target;
       for (long i = 0; i < N; ++i) grad[i] -= lambda * c->dv;
     }
+    gmax = 0.0;
+    for (long i = 0; i < N; ++i)
+      gmax = std::max(gmax, std::isfinite(grad[i]) ? std::fabs(grad[i]) : std::numeric_limits<double>::infinity());
+    gmax /= c->dv;
     for (long i = 0; i < N; ++i) {  // transform_force and negate (minimization.hpp:82-98,300-301)
       double dr;
       if (param_kind == 0) dr = 2.0 * x[i];
@@ -968,6 +977,7 @@ int minimize(Ctx* c, const double* fire_p, int param_kind, double rho_min, doubl
     vv = vv * 0.9 + 0.1 * (std::fabs(st.energy - old_e) / st.dt);
     double monitor = std::fabs(vv / volume);
     converged = monitor < fp.force_tolerance;
+    if (fp.force_max_tolerance > 0.0 && gmax < fp.force_max_tolerance) converged = true;
     st.converged = converged;
     if (energies) energies[it + 1] = st.energy;
   }

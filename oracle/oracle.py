@@ -249,13 +249,14 @@ class Oracle:
 
     def _minimize(self, rho0, mu, target, fire, param, rho_min, rho_max, homogeneous_boundary, vext, max_steps):
         f = dict(dt=1e-3, dt_max=1e-2, dt_min=1e-8, alpha_start=0.1, f_inc=1.1, f_dec=0.5, f_alpha=0.99,
-                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000)
+                 n_delay=5, max_uphill=20, force_tolerance=0.1, max_steps=10000, force_max_tolerance=0.0)
         if fire:
             f.update(fire)
         if max_steps is not None:
             f["max_steps"] = max_steps
         fp = np.array([f["dt"], f["dt_max"], f["dt_min"], f["alpha_start"], f["f_inc"], f["f_dec"], f["f_alpha"],
-                       f["n_delay"], f["max_uphill"], f["force_tolerance"], f["max_steps"]], dtype=np.float64)
+                       f["n_delay"], f["max_uphill"], f["force_tolerance"], f["max_steps"],
+                       f["force_max_tolerance"]], dtype=np.float64)
         rho0 = np.ascontiguousarray(rho0, dtype=np.float64).reshape(-1)
         v = None if vext is None else np.ascontiguousarray(vext, dtype=np.float64).reshape(-1)
         rho_out = np.zeros(self.N)

@@ -82,7 +82,7 @@ void check(bool ok, const char* what) {
   }
 }
 
-// The known-answer tests of the reference's fire.cpp:21-140 on the facade's host FIRE2.
+// The known-answer tests of the reference's fire.cpp:21-137 on the facade's host FIRE2.
 void fire_checks() {
   using dft::algorithms::fire::Fire;
   {

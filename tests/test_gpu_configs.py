@@ -10,7 +10,10 @@ on the same seeded inputs, through the C ABI.
 - batched Minimizer runs over uneven grids (48^3 and 80^3 on the general
   transforms, 64^3 fused), each member against the oracle's single run;
 - DDFT integrating-factor steps on fused-schedule grids (64^3 FP64 and mixed,
-  256x64x64 mixed) with mass conserved to 1e-12.
+  256x64x64 mixed) with mass conserved to 1e-12;
+- the droplet relaxed until max |delta(beta Omega)/delta rho| < 1e-8 (the
+  convergence rule of the droplet workload) on a 16^3 grid of the same box as
+  the 64^3 version, against the oracle and the exact uniform vapor.
 """
 import math
 
@@ -179,3 +182,28 @@ def test_ddft_on_fused_grid_matches_oracle(shape, precision, tol):
     np.testing.assert_allclose(eg, eo, rtol=tol * 10, atol=1e-9)
     assert resg["time"] == pytest.approx(reso["time"], rel=1e-12)
     assert drift < 1e-12
+
+
+def test_droplet_converges_to_max_force_tolerance():
+    # config 2's stop rule, max |delta(beta Omega)/delta rho| < 1e-8 (no dV), on a 16^3 grid of the same
+    # 6.4 sigma box as the 64^3 version: the droplet at mu(rho_v) is a critical nucleus, so grand-canonical
+    # FIRE evaporates it to the uniform vapor, where the bulk equation of state fixes rho = rho_v exactly
+    shape, dx = (16, 16, 16), 0.4
+    kw = dict(interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII")
+    rho = W.droplet(shape, dx, radius=2.0, width=0.5, rho_l=0.7, rho_v=0.02, noise=1e-3, seed=2024)
+    ctx = _ctx(shape, dx, precision="fp64", **kw)
+    orc = _orc(shape, dx, **kw)
+    mu = orc.bulk_mu(0.02)
+    fire = dict(force_tolerance=0.0, force_max_tolerance=1e-8)
+    mk = dict(fire=fire, max_steps=40000, homogeneous_boundary=False)
+    rg, resg, _ = ctx.minimize_grand_potential(rho, mu, record_energies=False, **mk)
+    ro, reso, _ = orc.minimize_grand_potential(rho, mu, **mk)
+    print(f"converged droplet: gpu {resg['iterations']} / oracle {reso['iterations']} steps, "
+          f"profile {_rel(rg, ro):.2e}, max |rho - rho_v| {np.max(np.abs(rg - 0.02)):.2e}", flush=True)
+    assert resg["converged"] and reso["converged"]
+    # the stop compares a rounding-level quantity with a threshold: the runs may cross it one step apart
+    assert abs(resg["iterations"] - reso["iterations"]) <= 1
+    assert _rel(rg, ro) < 1e-9
+    _, f = ctx.total(rg, mu)
+    assert np.max(np.abs(f)) / dx ** 3 < 1e-8
+    assert np.max(np.abs(rg - 0.02)) < 1e-8

@@ -123,7 +123,7 @@ def test_weight_closed_forms_against_quadrature():
         # shell: 4 pi R^2 j0(kR)
         assert w[(1,) + p].real == pytest.approx(4 * PI * R * R * np.sinc(k * R / PI), rel=1e-10, abs=1e-12)
         # vector: the e^{-ik.r} transform of delta(R-r) r_hat is -4 pi i R^2 j1(kR) k_hat.  The
-        # reference's channel (weights.hpp:193-201) carries the opposite overall sign; every
+        # reference's channel (weights.hpp:110-119,164-168) carries the opposite overall sign; every
         # physical output is even in it.  Check magnitude and the stated sign.
         x = k * R
         j1 = math.sin(x) / x ** 2 - math.cos(x) / x

@@ -221,3 +221,23 @@ def test_saddle_search_returns_a_saddle():
     # the returned value comes from forward differences (eps 1e-6) and a 1e-4 residual tolerance
     assert res["eigenvalue"] == pytest.approx(w[0], rel=2e-3)
     assert abs(float(U[:, 0] @ ef[inner])) / np.linalg.norm(ef[inner]) > 0.9999
+
+
+def test_max_force_stop():
+    # the library's stop on max |delta(beta Omega)/delta rho| (no dV factor): the returned density meets it,
+    # checked against the functional itself, and the energy monitor alone would have run on
+    shape, dx = (8, 8, 8), 0.5
+    orc = O.Oracle(shape, dx, interactions=[W.LJ_WCA], kT=0.9, model="WhiteBearII")
+    rng = np.random.default_rng(4)
+    rho0 = 0.3 * (1.0 + 0.05 * rng.standard_normal(shape))
+    mu = orc.bulk_mu(0.3)
+    fire = dict(dt=1e-2, dt_max=0.1, force_tolerance=0.0, force_max_tolerance=1e-7)
+    r, res, e = orc.minimize_grand_potential(rho0, mu, fire=fire, max_steps=20000, homogeneous_boundary=False)
+    assert res["converged"] and res["iterations"] < 20000
+    _, f = orc.total(r, mu)
+    assert np.max(np.abs(f)) / dx ** 3 < 1e-7
+    # one step earlier the criterion was not met
+    r1, res1, _ = orc.minimize_grand_potential(rho0, mu, fire=dict(fire, force_max_tolerance=0.0),
+                                               max_steps=res["iterations"] - 1, homogeneous_boundary=False)
+    _, f1 = orc.total(r1, mu)
+    assert np.max(np.abs(f1)) / dx ** 3 >= 1e-7
