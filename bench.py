@@ -254,14 +254,15 @@ def stream_copy_gbps(reps=5):
     return sorted(rates)[len(rates) // 2]
 
 
-def converged_profile(cd, W, rho0, shape, dx, kT, tol=1e-8, cap=60000):
+def converged_profile(cd, W, rho0, shape, dx, kT, tol=1e-8, cap=60000, dt_max=0.3):
     """The headline droplet relaxed by Minimizer::grand_potential (Unbounded alias, no boundary averaging:
-    a periodic box) until max |delta(beta Omega)/delta rho| (no dV) < tol: FP64 on the GPU, then mixed
-    precision with the same rule and a step cap of 1.5x the FP64 run (its FP32 rounding leaves a floor).
-    At mu = mu_bulk(rho_v) the droplet is a critical nucleus and evaporates, so the converged profile is the
-    uniform vapor rho = rho_v, which the bulk equation of state fixes exactly.  Reports steps, step rates,
-    the residual max force of each final profile under the FP64 functional, the FP64 profile's distance
-    from rho_v, and how far the mixed profile and Omega end from the FP64 ones."""
+    a periodic box; FIRE dt_max = 0.3) until max |delta(beta Omega)/delta rho| (no dV) < tol: FP64 on the
+    GPU, then mixed precision with the same rule and a step cap of 1.5x the FP64 run (its FP32 rounding
+    leaves a floor).  At mu = mu_bulk(rho_v) in this 12.8 sigma box the R = 4 droplet is supercritical and
+    the box fills with the liquid whose bulk chemical potential is mu: the converged profile is that uniform
+    liquid, which the bulk equation of state fixes exactly (its root is found by bisection).  Reports steps,
+    step rates, the residual max force of each final profile under the FP64 functional, the FP64 profile's
+    distance from the bulk root, and how far the mixed profile and Omega end from the FP64 ones."""
     import numpy as np
     import torch
 
@@ -279,7 +280,7 @@ def converged_profile(cd, W, rho0, shape, dx, kT, tol=1e-8, cap=60000):
         ctx = ctx64 if prec == "fp64" else cd.Context(shape, dx, interactions=[W.LJ_WCA], kT=kT,
                                                        model="WhiteBearII", precision=prec)
         steps = cap if prec == "fp64" else max(2000, int(1.5 * res["fp64"][1]["iterations"]))
-        fire = dict(force_tolerance=0.0, force_max_tolerance=tol)
+        fire = dict(force_tolerance=0.0, force_max_tolerance=tol, dt_max=dt_max)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r, info, _ = ctx.minimize_grand_potential(rho0, mu, fire=fire, max_steps=steps, homogeneous_boundary=False,
@@ -292,7 +293,17 @@ def converged_profile(cd, W, rho0, shape, dx, kT, tol=1e-8, cap=60000):
         out[f"{prec}_max_force_fp64_functional"] = max_force(r)
         print(f"converged profile {prec}: {info['iterations']} steps in {t:.1f} s", file=sys.stderr, flush=True)
     (rd, idd), (rm, im) = res["fp64"], res["mixed"]
-    out["fp64_max_abs_dev_from_rho_v"] = float(np.max(np.abs(rd - 0.02)))
+    # the uniform states at this mu: the vapour root rho_v and the liquid root in (0.4, 1.0) of
+    # mu_bulk(rho) = mu (mu_bulk rises on the liquid branch)
+    lo, hi = 0.4, 1.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        lo, hi = (mid, hi) if ctx64.bulk_mu(mid) < mu else (lo, mid)
+    rho_l = 0.5 * (lo + hi)
+    root = rho_l if float(np.mean(rd)) > 0.3 else 0.02
+    out["fp64_final_phase"] = "liquid" if root == rho_l else "vapour"
+    out["bulk_root_density"] = root
+    out["fp64_max_abs_dev_from_bulk_root"] = float(np.max(np.abs(rd - root)))
     out["mixed_floor_max_force"] = out["mixed_max_force_fp64_functional"]
     out["profile_max_rel_err"] = float(np.max(np.abs(rm - rd)) / np.max(np.abs(rd)))
     out["omega_rel_err"] = abs(im["grand_potential"] - idd["grand_potential"]) / abs(idd["grand_potential"])
