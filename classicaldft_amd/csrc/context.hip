@@ -293,9 +293,9 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       hipDeviceProp_t prop{};
       CDFT_HIP(hipGetDeviceProperties(&prop, c->device));
       c->fg.ncu = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
-      c->nblk_point = fast_point_blocks(c->fg);
+      c->nblk_point = fast_point_blocks(c->fg, c->precision != CDFT_PRECISION_MIXED);
       fast_twiddles(&c->ftw64, &c->ftw32);
-      c->d_P = dalloc_bytes(cs_bytes * g.M * 4);  // partial sets [set][h] of the backward plane kernel
+      c->d_P = dalloc_bytes(cs_bytes * g.M * 2);  // partial sets 0 and 1 of the backward plane kernel
       c->d_XR = dalloc_bytes(16 * g.M);
       c->d_D0 = dalloc<double>(static_cast<size_t>(g.nx) * g.ny);
       c->d_WT = dalloc_bytes(16 * g.M);

@@ -85,7 +85,7 @@ struct Context {
   FastGeom fg{};
   void* ftw64 = nullptr;      // shared twiddle tables of the fused kernels
   void* ftw32 = nullptr;
-  void* d_P = nullptr;        // 4 * M complex: partial sums [set][h] of the backward plane kernel
+  void* d_P = nullptr;        // 2 * M complex: partial sets 0 and 1 of the backward plane kernel
   void* d_XR = nullptr;       // M complex FP64: rho_hat for the forward plane kernel
   double* d_D0 = nullptr;     // nx * ny FP64: kz = 0 plane (line sums) of the z-transformed rho
   void* d_WT = nullptr;       // M complex FP64: radial FMT weights (w3, w2) in the plane kernels' order

@@ -1,18 +1,34 @@
-// Fused five-kernel evaluation schedule (gfx950) for power-of-two grids.
+// Fused six-kernel evaluation schedule (gfx950) for power-of-two grids.
 //
 //   kf_zfwd       z-R2C of rho - mean (FP64), y-parity split    rho -> A0 (chunked split layout)
-//   kf_plane_fwd  per (kz, r, h): for each of the 10 plane channels: weight
-//                 times rho_hat, x-inverse of the output half ix = 2i + h,
-//                 y-inverse, written in the split layout
-//   kf_zpoint     per line pair: y-parity combine + z-C2R of every channel
-//                 (n_v,z from eta's spectrum, T_zz by the trace identity),
-//                 pointwise Phi/dPhi + local terms + partial sums, z-R2C of
-//                 the derivative channels + y-parity split (in place, 9 out)
-//   kf_plane_bwd  per (kz, r, h): y-FFT and half x-FFT (rows ix = 2i + h) of
-//                 the 9 derivative channels, conj-weighted accumulation over
-//                 the full kx range, x- and y-inverse of the sum -> partial h
-//   kf_zinv       per line pair: sum of the two partials, y-parity combine,
-//                 z-C2R, * dV + local force                    -> force
+//   kf_rho_xy     per (kz, r): y- and x-FFT of rho's z-spectrum (FP64) -> rho_hat
+//   kf_plane_fwd  per (kz, r, p): for each of the 10 plane channels: weight
+//                 times rho_hat on the kx = 2k + p sub-plane, its x- and
+//                 y-inverse (NX/2 x NY/2 points), written in the split layout
+//   kf_zpoint     per line quad: x-parity and y-parity combine + z-C2R of
+//                 every channel (n_v,z from eta's spectrum, T_zz by the trace
+//                 identity), pointwise Phi/dPhi + local terms + partial sums,
+//                 z-R2C of the derivative channels + y- and x-parity split
+//                 (in place, 9 out)
+//   kf_plane_bwd  per (kz, r, p): y- and x-FFT of the 9 derivative channels'
+//                 sub-planes, conj-weighted accumulation, x- and y-inverse of
+//                 the sum -> partial
+//   kf_zinv       per line quad: sum of the partials, x-parity and y-parity
+//                 combine, z-C2R, * dV + local force           -> force
+//
+// Parity splits.  A transform of length 2L along y or x is split into two of
+// length L by its first radix-2 stage, so that the plane kernels work on
+// independent (kz, r, p) sub-planes: along y by ky parity r (rows y and
+// y + NY/2 combine to r = 0 and (difference) * exp(-2 pi i y / NY) to r = 1,
+// in the z-line kernels), along x by kx parity p.  For x the inverse
+// transforms end with
+//   x(i) = Z0(i) + exp(2 pi i i / NX) Z1(i),  x(i + NX/2) = Z0(i) - exp(...) Z1(i)
+// (Zp the NX/2-point inverse of the kx = 2k + p values) and the forward one
+// starts with
+//   d0(i) = d(i) + d(i + NX/2),  d1(i) = exp(-2 pi i i / NX) (d(i) - d(i + NX/2)),
+// both done by the z-line kernels, whose blocks own the lines at x = i and
+// i + NX/2 (one twiddle per block).  So no plane block reads another's
+// sub-plane or transforms more than its own quarter of a kz plane.
 //
 // Line transforms run in slots of one wavefront (slot_fft.hpp); a block
 // synchronises only where data crosses wavefronts (the plane transposes and
@@ -29,6 +45,7 @@
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "cdft.h"
@@ -42,8 +59,13 @@ namespace cdft {
 namespace {
 
 constexpr int kBT = 1024;            // threads of a plane kernel block
-constexpr int kPlaneTB = 73728;      // bytes of a plane kernel's transpose buffer
-constexpr int kZP = 256;             // threads of a kf_zpoint block (4 line pairs)
+// Waves of a kf_zpoint block (one line pair each): 4 with FP32 storage (two
+// blocks share a CU and overlap each other's barriers), 8 with FP64 storage,
+// whose 4-wave block alone would fill most of a CU's LDS.
+template <typename S>
+__host__ __device__ constexpr int zp_waves() {
+  return sizeof(S) == 8 ? 8 : 4;
+}
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8);
 // this maps consecutive logical blocks onto one XCD so that they share its
@@ -55,9 +77,16 @@ __host__ __device__ constexpr int xcd_remap(int b, int n) {
 
 __host__ __device__ constexpr int align16(int b) { return (b + 15) & ~15; }
 
-// Split-layout offset of element (kz, r, ix, m).
+// Split-layout offset of element (kz, r, ix, m).  Between the plane kernels
+// and the z-line kernels, ix = 2 i + p holds half row i of kx-parity part p.
 __device__ __forceinline__ long split_idx(int kz, int r, int ix, int m, int mb, int nzh) {
   return ((static_cast<long>(ix) * mb + (m >> 4)) * nzh + kz) * 32 + r * 16 + (m & 15);
+}
+
+// Plane kernels' order of a half spectrum: element (kx = 2 k + p, ky = r + 2 s, kz)
+// of sub-plane u = (kz * 2 + r) * 2 + p at u * Q * NX/2 + s * NX/2 + k.
+__host__ __device__ inline long hp_idx(int kz, int r, int p, int s, int k, int q, int nxh) {
+  return ((((static_cast<long>(kz) * 2 + r) * 2 + p) * q + s) * nxh) + k;
 }
 
 // Channel bookkeeping of the fused schedule.  Two identities of the FMT
@@ -222,10 +251,16 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
 }
 
 // ============================================================ kf_zinv
+// A block (4 NZ threads, 32 slots) owns half row i and 16 consecutive m: the
+// line pairs (x, m) / (x, m + NY/2) at x = i and x = i + NX/2.
+template <int NZ>
+__host__ __device__ constexpr int zinv_threads() {
+  return 4 * NZ;
+}
 template <int NZ, typename C>
 __host__ __device__ constexpr int zinv_tw_off() {
-  return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<C>))) +
-         16 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<C>));
+  return align16(2 * (NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<C>))) +
+         32 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<C>));
 }
 template <int NZ, typename C>
 __host__ __device__ constexpr int zinv_smem() {
@@ -264,38 +299,60 @@ __host__ __device__ inline bool bwd_unit_split(int unit, int nunits, int nblk);
 // reduction of the point kernel's partial sums (reduce_rows) while the other
 // blocks transform: the evaluation then needs no separate finalisation launch.
 template <typename S, typename C, int NZ>
-__global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, double* __restrict__ out,
-                                                  const double* __restrict__ add, double scale, FastGeom g,
-                                                  const cx<C>* __restrict__ tw, int bwd_blocks,
-                                                  const double* __restrict__ partials, int fin_rows,
-                                                  double* __restrict__ res) {
-  constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
+__global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __restrict__ P, double* __restrict__ out,
+                                                               const double* __restrict__ add, double scale,
+                                                               FastGeom g, const cx<C>* __restrict__ tw,
+                                                               int bwd_blocks, const double* __restrict__ partials,
+                                                               int fin_rows, double* __restrict__ res) {
+  constexpr int NT = zinv_threads<NZ>(), T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   extern __shared__ __align__(16) unsigned char smem[];
   if (fin_rows > 0 && blockIdx.x == gridDim.x - 1) {
     double* sh = reinterpret_cast<double*>(smem);
-    reduce_rows<2 * NZ>(partials, fin_rows, res, sh, sh + 2 * NZ);
+    reduce_rows<NT>(partials, fin_rows, res, sh, sh + NT);
     return;
   }
-  cx<C>* cb = reinterpret_cast<cx<C>*>(smem);  // [r][m16][kz]
-  cx<C>* xbuf = reinterpret_cast<cx<C>*>(smem + align16(NZH * 32 * static_cast<int>(sizeof(cx<C>))));
+  cx<C>* cb = reinterpret_cast<cx<C>*>(smem);  // [xh][r][m16][kz]
+  cx<C>* xbuf = reinterpret_cast<cx<C>*>(smem + align16(2 * NZH * 32 * static_cast<int>(sizeof(cx<C>))));
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  const long chunk = blockIdx.x;
-  const int ix = static_cast<int>(chunk / g.mb);
-  const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
-  const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
-  // partial sets of the backward plane kernel: [set][h] at (2 set + h) M
-  const cx<S>* p0 = P + chunk * static_cast<long>(NZH * 32);
-  const cx<C>* twl = stage_twiddles<C, 2 * NZ>(smem + zinv_tw_off<NZ, C>(), tw);
+  const int xh = slot >> 4, ms = slot & 15;
+  const long blk = blockIdx.x;
+  const int i = static_cast<int>(blk / g.mb), mc = static_cast<int>(blk % g.mb);
+  const int m = mc * 16 + ms;
+  const long la = static_cast<long>(i + xh * (g.nx / 2)) * g.ny + m, lb = la + g.q;
+  // partial sets of the backward plane kernel at set * M; parts p = 0, 1 of
+  // half row i are the chunks of ix = 2 i and 2 i + 1
+  const long ochunk = static_cast<long>(g.mb) * NZH * 32;
+  const cx<S>* p0 = P + (2L * i * g.mb + mc) * static_cast<long>(NZH * 32);
+  const cx<C>* twl = stage_twiddles<C, NT>(smem + zinv_tw_off<NZ, C>(), tw);
   const int nunits = 4 * NZH;
-  block_copy<2 * NZ, NZH * 32, cx<C>>(
-      [&](int i) {
-        cx<C> a = cadd(cvt<C>(p0[i]), cvt<C>(p0[g.M + i]));
-        const int u = ((i >> 5) * 2 + ((i >> 4) & 1)) * 2;  // unit of (kz, r), h = 0
-        if (bwd_unit_split(u, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[2 * g.M + i]));
-        if (bwd_unit_split(u + 1, nunits, bwd_blocks)) a = cadd(a, cvt<C>(p0[3 * g.M + i]));
-        return a;
-      },
-      [&](int i, cx<C> a) { cb[(i & 31) * NZH + (i >> 5)] = a; });
+  {
+    // exp(2 pi i i / NX), read from global memory (the LDS copy is not yet published)
+    cx<C> w = tw[g.nx - 4 + i];
+    w.im = -w.im;
+    constexpr int K = (NZH * 32 + NT - 1) / NT;
+    cx<C> e[K], o[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = static_cast<int>(threadIdx.x) + NT * k;
+      if (j < NZH * 32) {
+        const int u = ((j >> 5) * 2 + ((j >> 4) & 1)) * 2;  // sub-plane (kz, r, p = 0)
+        e[k] = cvt<C>(p0[j]);
+        o[k] = cvt<C>(p0[ochunk + j]);
+        if (bwd_unit_split(u, nunits, bwd_blocks)) e[k] = cadd(e[k], cvt<C>(p0[g.M + j]));
+        if (bwd_unit_split(u + 1, nunits, bwd_blocks)) o[k] = cadd(o[k], cvt<C>(p0[g.M + ochunk + j]));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int j = static_cast<int>(threadIdx.x) + NT * k;
+      if (j < NZH * 32) {
+        const cx<C> wo = cmul(o[k], w);
+        const int c = ((j >> 4) & 1) * 16 + (j & 15), kz = j >> 5;  // (r, m % 16)
+        cb[c * NZH + kz] = cadd(e[k], wo);
+        cb[(32 + c) * NZH + kz] = csub(e[k], wo);
+      }
+    }
+  }
   // the local force of the lane's points, loaded while the transform runs
   double ad[2][8];
 #pragma unroll
@@ -306,7 +363,7 @@ __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, d
   }
   __syncthreads();
   cx<C> v[8];
-  build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[(r * 16 + slot) * NZH + kz]; });
+  build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[((xh * 2 + r) * 16 + ms) * NZH + kz]; });
   slot_fft<C, NZ, T, +1>(v, xbuf + slot * XS, lane, twl);
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
@@ -319,11 +376,12 @@ __global__ void __launch_bounds__(2 * NZ) kf_zinv(const cx<S>* __restrict__ P, d
 // ============================================================ kf_zpoint
 template <int NZ, typename S>
 __host__ __device__ constexpr int zpoint_stage_bytes() {
-  return align16((64 / (NZ / 8)) * 8 * (NZ / 2 + 1) * static_cast<int>(sizeof(cx<S>)));
+  return align16((64 / (NZ / 8)) * 2 * zp_waves<S>() * (NZ / 2 + 1) * static_cast<int>(sizeof(cx<S>)));
 }
 template <int NZ, typename S>
 __host__ __device__ constexpr int zpoint_tw_off() {
-  return zpoint_stage_bytes<NZ, S>() + 4 * (64 / (NZ / 8)) * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
+  return zpoint_stage_bytes<NZ, S>() +
+         zp_waves<S>() * (64 / (NZ / 8)) * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
 }
 template <int NZ, typename S>
 __host__ __device__ constexpr int zpoint_smem() {
@@ -345,7 +403,7 @@ __device__ __forceinline__ void zpoint_fwd_round(S (&wv)[NWV][PPL], const cx<S>*
   const C dk = static_cast<C>(dkz);
   cx<C> v[8];
   build_c2r<C, NZ>(v, lane, [&](int r, int kz) {
-    const cx<C> b = cvt<C>(stg[((src * 2 + r) * 4 + wave) * NZH + kz]);
+    const cx<C> b = cvt<C>(stg[((src * 2 + r) * zp_waves<S>() + wave) * NZH + kz]);
     const C k = static_cast<C>(kz) * dk;
     return vz ? cx<C>{-b.im * k, b.re * k} : b;
   });
@@ -405,8 +463,8 @@ __device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NWV][PPL], cx<S>*
       const cx<C> zk = v[e], zm = xw[slot * XS + xpad((NZ - k) & (NZ - 1))];
       const cx<C> A = {C(0.5) * (zk.re + zm.re), C(0.5) * (zk.im - zm.im)};
       const cx<C> B = {C(0.5) * (zk.im + zm.im), C(-0.5) * (zk.re - zm.re)};
-      stg[((slot * 2 + 0) * 4 + wave) * NZH + k] = cvt<S>(cadd(A, B));
-      stg[((slot * 2 + 1) * 4 + wave) * NZH + k] = cvt<S>(csub(A, B));
+      stg[((slot * 2 + 0) * zp_waves<S>() + wave) * NZH + k] = cvt<S>(cadd(A, B));
+      stg[((slot * 2 + 1) * zp_waves<S>() + wave) * NZH + k] = cvt<S>(csub(A, B));
     }
   }
 }
@@ -415,13 +473,14 @@ __device__ __forceinline__ void zpoint_bwd_round(const S (&wv)[NWV][PPL], cx<S>*
 // kFmtRuntime to read it from the parameters.
 constexpr int kFmtRuntime = -100;
 template <typename S, typename CV, int NZ, int NF, int NB, int FM>
-__global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
+__global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
                                                  const double* __restrict__ vext, double* __restrict__ lf,
                                                  double* __restrict__ partials, FastGeom g,
                                                  const cx<double>* __restrict__ tw64, const cx<CV>* __restrict__ twv,
                                                  FmtParams fp, double mu, AliasMap amap,
                                                  const double* __restrict__ shift_sum, DcTable dc) {
   constexpr int T = NZ / 8, NS = 64 / T, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
+  constexpr int kZW = zp_waves<S>(), kZP = 64 * kZW, MW = kZW / 2;  // waves, threads, columns
   constexpr int PPL = NZ / 32;  // points per lane: z = lid + 64 j of both lines
   constexpr int NWV = zf_count(NF) > 0 ? zf_count(NF) : 1;
   constexpr int NRF = (zf_count(NF) + NS - 1) / NS, NRB = NB > 0 ? (kZB + NS - 1) / NS : 0;
@@ -429,16 +488,23 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   constexpr bool HS = NF >= 11;
   constexpr bool MF = (NF == 12 || NF == 1);
   extern __shared__ __align__(16) unsigned char smem[];
-  cx<S>* stg = reinterpret_cast<cx<S>*>(smem);  // [gs][r][p][kz]
+  cx<S>* stg = reinterpret_cast<cx<S>*>(smem);  // [gs][r][line pair][kz]
   const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
   unsigned char* xwave = smem + zpoint_stage_bytes<NZ, S>() + wave * NS * XS * static_cast<int>(sizeof(cx<double>));
+  // Block: half row i and the MW columns m0 .. m0 + MW - 1 (m0 = 16 mc + MW mq);
+  // wave (xh, mm) = (wave / MW, wave % MW) owns the line pair at
+  // x = i + xh NX/2, m = m0 + mm.  The blocks of one 16-column group are
+  // consecutive, so they share an XCD's L2 and each chunk row is fetched once.
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const long chunk = logical >> 2;
-  const int qb = logical & 3;
-  const int ix = static_cast<int>(chunk / g.mb);
-  const int m = static_cast<int>(chunk % g.mb) * 16 + qb * 4 + wave;
-  const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
-  const long cbase = chunk * static_cast<long>(NZH * 32) + qb * 4;
+  const int mq = logical % (16 / MW);
+  const long grp = logical / (16 / MW);
+  const int irow = static_cast<int>(grp / g.mb), mc = static_cast<int>(grp % g.mb);
+  const int xh = wave / MW, mm = wave % MW;
+  const int m = mc * 16 + MW * mq + mm;
+  const long la = static_cast<long>(irow + xh * (g.nx / 2)) * g.ny + m, lb = la + g.q;
+  // chunk of ix = 2 i (part p = 0) at column m0; part 1 is ochunk further
+  const long cbase = (2L * irow * g.mb + mc) * static_cast<long>(NZH * 32) + MW * mq;
+  const long ochunk = static_cast<long>(g.mb) * NZH * 32;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   S wv[NWV][PPL];
   // twiddle tables in LDS; the first barrier below publishes them
@@ -446,22 +512,25 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   const cx<CV>* twvl = stage_twiddles<CV, kZP>(smem + zpoint_tw_off<NZ, S>() + tw_bytes<double>(), twv);
   if constexpr (NRF == 0) __syncthreads();
 
-  // ---- forward rounds: stage NS channels of the block's 4 line pairs, C2R.
+  // ---- forward rounds: stage NS channels of the block's line pairs, C2R.
   // The loads of round JR + 1 are issued into registers before round JR's
   // transforms, so their memory latency overlaps that work.
-  constexpr int NSI = NS * NZH * 8, KPF = (NSI + kZP - 1) / kZP;
-  cx<S> pf[KPF];
-  // element i = tid + kZP k of a round: (slot gs, part r, line pair p, kz);
-  // its offset in a channel's chunk and in the staging buffer are the same
-  // every round and computed once
+  // Pair j = tid + kZP k of a round: (slot gs, part r, column mm, kz).  It
+  // loads the kx-parity parts E (ix = 2 i) and O (ix = 2 i + 1) and stages
+  // E + w O for the line pair at x = i and E - w O for x = i + NX/2,
+  // w = exp(2 pi i i / NX); its offsets in a channel's chunk and in the
+  // staging buffer are the same every round and computed once.
+  constexpr int NSP = NS * NZH * 2 * MW, KPF = (NSP + kZP - 1) / kZP;
+  constexpr int XH1 = MW * NZH;  // staging distance of the x = i + NX/2 line pair
+  cx<S> pfe[KPF], pfo[KPF];
   int goff[KPF], soff[KPF], gsk[KPF];
 #pragma unroll
   for (int k = 0; k < KPF; ++k) {
-    const int i = min(static_cast<int>(threadIdx.x) + kZP * k, NSI - 1);
-    const int p = i & 3, r = (i >> 2) & 1, t = i >> 3;
+    const int j = min(static_cast<int>(threadIdx.x) + kZP * k, NSP - 1);
+    const int c = j % MW, r = (j / MW) & 1, t = j / (2 * MW);
     const int kz = t % NZH, gs = t / NZH;
-    goff[k] = (kz * 2 + r) * 16 + p;
-    soff[k] = ((gs * 2 + r) * 4 + p) * NZH + kz;
+    goff[k] = (kz * 2 + r) * 16 + c;
+    soff[k] = ((gs * 2 + r) * kZW + c) * NZH + kz;  // line pair (xh = 0, mm = c)
     gsk[k] = gs;
   }
   const cx<S>* sfc = SF + cbase;
@@ -471,19 +540,34 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
 #pragma unroll
     for (int k = 0; k < KPF; ++k) {
       const int ch = zf_chan(NF, jr * NS + gsk[k]);
-      pf[k] = sfc[static_cast<long>(ch >= 0 ? chan_store(NF, ch) : 0) * g.M + goff[k]];
+      const cx<S>* src = sfc + static_cast<long>(ch >= 0 ? chan_store(NF, ch) : 0) * g.M + goff[k];
+      pfe[k] = src[0];
+      pfo[k] = src[ochunk];
     }
   };
-  auto store_round = [&](int jr) {
+  // the combine runs in the round's arithmetic (FP64 for the scalar channels)
+  const cx<double> wx = {tw64[g.nx - 4 + irow].re, -tw64[g.nx - 4 + irow].im};
+  auto store_round = [&](auto jrc) {
+    constexpr int jr = decltype(jrc)::value;
+    using C = typename std::conditional<zround64(NF, NS, jr), double, CV>::type;
+    const cx<C> w = cvt<C>(wx);
 #pragma unroll
     for (int k = 0; k < KPF; ++k) {
-      if (NSI % kZP == 0 || static_cast<int>(threadIdx.x) + kZP * k < NSI)
-        stg[soff[k]] = zf_chan(NF, jr * NS + gsk[k]) >= 0 ? pf[k] : cx<S>{S(0), S(0)};
+      if (NSP % kZP == 0 || static_cast<int>(threadIdx.x) + kZP * k < NSP) {
+        cx<S> a = {S(0), S(0)}, b = {S(0), S(0)};
+        if (zf_chan(NF, jr * NS + gsk[k]) >= 0) {
+          const cx<C> e = cvt<C>(pfe[k]), wo = cmul(cvt<C>(pfo[k]), w);
+          a = cvt<S>(cadd(e, wo));
+          b = cvt<S>(csub(e, wo));
+        }
+        stg[soff[k]] = a;
+        stg[soff[k] + XH1] = b;
+      }
     }
   };
   if constexpr (NRF > 0) {
     load_round(0);
-    store_round(0);
+    store_round(std::integral_constant<int, 0>{});
     __syncthreads();
   }
 #define CDFT_ZF(JR)                                                                                   \
@@ -495,7 +579,7 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
       zpoint_fwd_round<CV, S, NZ, NF, JR, PPL>(wv, stg, xwave, wave, lid, twvl, g.dkz);               \
     __syncthreads();                                                                                  \
     if constexpr ((JR) + 1 < NRF) {                                                                   \
-      store_round((JR) + 1);                                                                          \
+      store_round(std::integral_constant<int, (JR) + 1>{});                                           \
       __syncthreads();                                                                                \
     }                                                                                                 \
   }
@@ -575,7 +659,7 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
   // Partial sums: each wave reduces its lanes and parks the totals in LDS;
   // the block's row is written after the next block barrier (the first
   // backward round's), so no barrier is spent on it.
-  __shared__ double wsum[kZP / 64][8];
+  __shared__ double wsum[kZW][8];
   {
     // through the wave's own LDS exchange region (free between the rounds):
     // lanes 0..39 add 8 lanes' values of one sum each, lanes 0..4 the 8 of those
@@ -605,7 +689,7 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
     if (threadIdx.x < 8) {
       double t = 0.0;
 #pragma unroll
-      for (int w = 0; w < kZP / 64; ++w) t += wsum[w][threadIdx.x];
+      for (int w = 0; w < kZW; ++w) t += wsum[w][threadIdx.x];
       partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = t;
     }
   };
@@ -616,24 +700,33 @@ __global__ void __launch_bounds__(kZP, 2) kf_zpoint(cx<S>* SF, const double* __r
 
   // ---- backward rounds: R2C of NS derivative channels, split, coalesced store
   // eta's derivative spectrum takes -i kz times the z vector's (positions 0 and 1)
-  const S dks = static_cast<S>(g.dkz);
   auto stage_out = [&](int jr) {
-    cx<S> vals[KPF];
+    // w' = exp(-2 pi i i / NX): part 0 = d(i) + d(i + NX/2), part 1 = w' (d(i) - d(i + NX/2))
+    const cx<CV> wb = {static_cast<CV>(wx.re), static_cast<CV>(-wx.im)};
+    const CV dkc = static_cast<CV>(g.dkz);
+    cx<S> va[KPF], vb[KPF];
 #pragma unroll
     for (int k = 0; k < KPF; ++k) {
-      cx<S> v = stg[soff[k]];
+      cx<CV> d0 = cvt<CV>(stg[soff[k]]), d1 = cvt<CV>(stg[soff[k] + XH1]);
       if (jr == 0 && gsk[k] == 0) {
-        const cx<S> vz = stg[soff[k] + 8 * NZH];        // position 1 of the same (r, p, kz)
-        const S kk = static_cast<S>(goff[k] >> 5) * dks;  // kz = goff >> 5
-        v = {v.re + kk * vz.im, v.im - kk * vz.re};
+        // position 1 of the same (r, line pair, kz)
+        constexpr int G1 = 2 * kZW * NZH;
+        const cx<CV> z0 = cvt<CV>(stg[soff[k] + G1]), z1 = cvt<CV>(stg[soff[k] + G1 + XH1]);
+        const CV kk = static_cast<CV>(goff[k] >> 5) * dkc;  // kz = goff >> 5
+        d0 = {d0.re + kk * z0.im, d0.im - kk * z0.re};
+        d1 = {d1.re + kk * z1.im, d1.im - kk * z1.re};
       }
-      vals[k] = v;
+      va[k] = cvt<S>(cadd(d0, d1));
+      vb[k] = cvt<S>(cmul(csub(d0, d1), wb));
     }
 #pragma unroll
     for (int k = 0; k < KPF; ++k) {
       const int b = jr * NS + gsk[k];
-      if ((NSI % kZP == 0 || static_cast<int>(threadIdx.x) + kZP * k < NSI) && b < kZB && b != 1)
-        SF[static_cast<long>(zb_chan(b)) * g.M + cbase + goff[k]] = vals[k];
+      if ((NSP % kZP == 0 || static_cast<int>(threadIdx.x) + kZP * k < NSP) && b < kZB && b != 1) {
+        cx<S>* o = SF + static_cast<long>(zb_chan(b)) * g.M + cbase + goff[k];
+        o[0] = va[k];
+        o[ochunk] = vb[k];
+      }
     }
   };
 #define CDFT_ZB(JR)                                                                                           \
@@ -696,17 +789,18 @@ __device__ __forceinline__ WFac<C> wfactors(cx<double> w, double kx, double ky, 
 }
 
 // FP64 radial FMT weights (w3, w2) (weights.hpp:91-108,159-162, times 1/N) of every
-// element of the half spectrum, in the plane kernels' order [kz][r][s][kx]
-// (ky = r + 2 s).  They depend only on the grid and the radius: computed once.
+// element of the half spectrum, in the plane kernels' sub-plane order (hp_idx).
+// They depend only on the grid and the radius: computed once.
 __global__ void __launch_bounds__(256) kf_wtable(cx<double>* __restrict__ WT, FastGeom g) {
   const long n = g.M;
+  const int nxh = g.nx / 2;
   for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * blockDim.x) {
-    const int kx = static_cast<int>(i % g.nx);
-    const long t = i / g.nx;
+    const int k = static_cast<int>(i % nxh);
+    const long t = i / nxh;
     const int s = static_cast<int>(t % g.q);
     const long u = t / g.q;
-    const int r = static_cast<int>(u & 1), kz = static_cast<int>(u >> 1);
+    const int kx = 2 * k + static_cast<int>(u & 1), r = static_cast<int>((u >> 1) & 1), kz = static_cast<int>(u >> 2);
     const double kxv = kvec(kx, g.nx, g.dkx), kyv = kvec(r + 2 * s, g.ny, g.dky), kzv = kz * g.dkz;
     double w3, w2;
     radial(sqrt(kxv * kxv + kyv * kyv + kzv * kzv), g.R, g.inv_n, &w3, &w2);
@@ -774,8 +868,9 @@ __device__ __forceinline__ cx<C> apply_form(const ChanForm<C>& f, C kx, C ky, C 
 // ---------------------------------------------------------------- kf_rho_xy
 // FP64 y- and x-FFT of the rho parts: block (kz, r) transforms the y lines
 // of its half plane once, then the columns in NPART batches (two; four for
-// NX = 256, to fit the transpose buffer in LDS) through one transpose buffer, and writes the full spectrum rho_hat(kx, ky = r + 2 s, kz)
-//   XR[((kz * 2 + r) * Q + s) * NX + kx]   (complex FP64).
+// NX = 256, to fit the transpose buffer in LDS) through one transpose buffer,
+// and writes the spectrum rho_hat(kx, ky = r + 2 s, kz) in the plane
+// kernels' sub-plane order, XR[hp_idx(kz, r, kx % 2, s, kx / 2)] (complex FP64).
 // (One block per half plane runs the 2 (Nz/2 + 1) blocks in a single round
 // on the CUs; splitting a half plane over two blocks would repeat its y
 // transforms and need a second round.)
@@ -869,9 +964,11 @@ __global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, c
           sum_out[0] = v[0].re;
           v[0] = {0.0, 0.0};
         }
-        cx<double>* o = XR + ((static_cast<long>(kz) * 2 + r) * Q + s0 + sc) * NX;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) o[xl + RC::TXR * e] = v[e];
+        for (int e = 0; e < 4; ++e) {
+          const int kx = xl + RC::TXR * e;
+          XR[hp_idx(kz, r, kx & 1, s0 + sc, kx >> 1, Q, NX / 2)] = v[e];
+        }
       }
     }
   }
@@ -897,10 +994,10 @@ struct FwdCfg {
 };
 
 // One channel in a kf_plane_fwd block: Wof(cc, e) is the weighted spectrum at
-// column xs + SBX cc, kx = xl + TX e; soff[rr][e] is the split-layout offset
-// of the lane's output (row ys + SBYH rr, m = yl + TYH e) in a channel array.  x-inverse of the output half
-// ix = 2i + h (DIF: (W[kx] +- W[kx + NX/2]) twiddled), then the y-inverse of
-// every row of that half, stored in the split layout.  With DB, channels
+// column xs + SBX cc, k = xl + TX e (kx = 2 k + p); soff[rr][e] is the
+// split-layout offset of the lane's output (row ys + SBYH rr, m = yl + TYH e)
+// in a channel array.  x-inverse of the sub-plane's NX/2 points, then the
+// y-inverse of every row, stored in the split layout.  With DB, channels
 // alternate between two transpose buffers and the x- and y-slot buffers are
 // distinct, so the one barrier per channel suffices: a wave enters channel
 // c+2 (which reuses channel c's transpose buffer) only after every wave has
@@ -909,7 +1006,7 @@ struct FwdCfg {
 template <typename CC, typename S, int NX, int Q, bool DB, typename WF>
 __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned char* xbx, unsigned char* xby,
                                                   cx<S>* __restrict__ out,
-                                                  const int (&soff)[PlaneCfg<NX, Q>::RPH][4], int r, int h,
+                                                  const int (&soff)[PlaneCfg<NX, Q>::RPH][4], int r,
                                                   const cx<CC>* __restrict__ tw) {
   using P = PlaneCfg<NX, Q>;
   constexpr int QP = Q + 1;
@@ -922,12 +1019,7 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
     if (col < Q) {
       cx<CC> zt[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const cx<CC> Wa = cvt<CC>(Wof(cc, e)), Wb = cvt<CC>(Wof(cc, e + 4));
-        cx<CC> zz = h ? csub(Wa, Wb) : cadd(Wa, Wb);
-        if (h) zz = twiddle<CC, NX, +1>(zz, xl + P::TX * e, tw);
-        zt[e] = zz;
-      }
+      for (int e = 0; e < 4; ++e) zt[e] = cvt<CC>(Wof(cc, e));
       slot_fft<CC, P::NXH, P::TX, +1, sizeof(CC) == 8>(zt, xbx + xs * XS * P::XBYTES, xl, tw);
 #pragma unroll
       for (int e = 0; e < 4; ++e) tb[(xl + P::TX * e) * QP + col] = cvt<S>(zt[e]);
@@ -955,10 +1047,11 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
 }
 
 // Work of the forward plane kernel: (unit, channel) items, units
-// u = (kz * 2 + r) * 2 + h, channels in the order FP64 (eta, n2, mean field)
-// then CV.  The 4 (Nz/2 + 1) units do not divide evenly over the CUs, so
-// blocks take contiguous ranges of items (at most two units each) instead of
-// one unit each: the last round of blocks then holds one item more at most.
+// u = (kz * 2 + r) * 2 + p (the sub-planes), channels in the order FP64 (eta,
+// n2, mean field) then CV.  The 4 (Nz/2 + 1) units do not divide evenly over
+// the CUs, so blocks take contiguous ranges of items (at most two units each)
+// instead of one unit each: the last round of blocks then holds one item more
+// at most.
 __host__ __device__ constexpr int fwd_nf64(int nf) { return nf == 12 ? 3 : (nf == 11 ? 2 : nf); }
 
 template <typename S, typename CV, int NX, int Q, int NF>
@@ -969,7 +1062,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
                                                     const cx<CV>* __restrict__ twv, int nunits) {
   using P = PlaneCfg<NX, Q>;
   using FC = FwdCfg<NX, Q, S, CV>;
-  constexpr int CPS = P::CPS;
+  constexpr int CPS = P::CPS, NXH = P::NXH;
   constexpr int N64 = fwd_nf64(NF);
   extern __shared__ __align__(16) unsigned char smem[];
   auto tbuf = [&](int n) { return reinterpret_cast<cx<S>*>(smem + (n & 1) * FC::tb2_off); };
@@ -981,8 +1074,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
   const int xs = tid / P::TX, xl = tid % P::TX;
   constexpr int NFP = fwd_nplane(NF);
   const long items = static_cast<long>(nunits) * NFP;
-  // consecutive item ranges (the two halves h of one (kz, r) slice read the
-  // same spectrum and weights) go to blocks of one XCD, which share its L2
+  // consecutive item ranges go to blocks of one XCD, which share its L2
   const long lb = xcd_remap(blockIdx.x, gridDim.x);
   const long i0 = items * lb / gridDim.x, i1 = items * (lb + 1) / gridDim.x;
   const double invR = 1.0 / g.R;
@@ -993,28 +1085,32 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
     const int p0 = static_cast<int>(it - static_cast<long>(unit) * NFP);
     const int p1 = static_cast<int>(min(static_cast<long>(NFP), p0 + (i1 - it)));
     it += p1 - p0;
-    const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+    const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
     int soff[P::RPH][4];  // the lane's output offsets, the same for every channel of the unit
 #pragma unroll
     for (int rr = 0; rr < P::RPH; ++rr)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int row = min(tid / P::TYH + P::SBYH * rr, P::NXH - 1), m = tid % P::TYH + P::TYH * e;
-        soff[rr][e] = static_cast<int>(split_idx(kz, r, 2 * row + h, m, P::MB, g.nzh));
+        const int row = min(tid / P::TYH + P::SBYH * rr, NXH - 1), m = tid % P::TYH + P::TYH * e;
+        soff[rr][e] = static_cast<int>(split_idx(kz, r, 2 * row + p, m, P::MB, g.nzh));
       }
-    // FP64 spectrum of the lane's elements, read where it is used (L1/L2):
-    // Xof(cc, e) = rho_hat(kx = xl + TX e, ky = r + 2 col)
-    const long uoff = (static_cast<long>(kz) * 2 + r) * Q * NX + xl;
-    auto Xof = [&](int cc, int e) {
+    // the lane's elements of the sub-plane: column xs + SBX cc, k = xl + TX e
+    const long uoff = static_cast<long>(unit) * Q * NXH + xl;
+    auto eoff = [&](int cc, int e) {
       const int col = xs + P::SBX * cc;
-      return XR[uoff + static_cast<long>(col < Q ? col : 0) * NX + P::TX * e];
+      return uoff + static_cast<long>(col < Q ? col : 0) * NXH + P::TX * e;
     };
-    auto Wof = [&](int cc, int e) {  // (w3, w2) of the element
-      const int col = xs + P::SBX * cc;
-      return WT[uoff + static_cast<long>(col < Q ? col : 0) * NX + P::TX * e];
-    };
+    // FP64 spectrum and radial weights, read once per unit
+    cx<double> X[CPS][4], Wr[CPS][4];
+#pragma unroll
+    for (int cc = 0; cc < CPS; ++cc)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        X[cc][e] = XR[eoff(cc, e)];
+        Wr[cc][e] = WT[eoff(cc, e)];
+      }
     const double kzv = kz * g.dkz;
-    auto kxof = [&](int e) { return kvec(xl + P::TX * e, NX, g.dkx); };
+    auto kxof = [&](int e) { return kvec(2 * (xl + P::TX * e) + p, NX, g.dkx); };
     auto kyof = [&](int cc) {
       const int col = xs + P::SBX * cc;
       return kvec(r + 2 * (col < Q ? col : 0), P::NY, g.dky);
@@ -1023,31 +1119,29 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
     // mean_field.hpp:145-196)
     for (int pos = p0; pos < min(p1, N64); ++pos, ++nch) {
       const int ch = fwd_channel(NF, pos);
-      const double* mrow0 = mfB + (static_cast<long>(kz) * 2 + r) * Q * NX;
       plane_fwd_channel<double, S, NX, Q, FC::DB>(
           [&](int cc, int e) {
             const int col = xs + P::SBX * cc;
-            if (ch == 11) return cscale(Xof(cc, e), col < Q ? mrow0[static_cast<long>(col) * NX + xl + P::TX * e] : 0.0);
-            const cx<double> w = Wof(cc, e);
-            return cscale(Xof(cc, e), ch == 0 ? w.re : w.im);
+            if (ch == 11) return cscale(X[cc][e], col < Q ? mfB[eoff(cc, e)] : 0.0);
+            return cscale(X[cc][e], ch == 0 ? Wr[cc][e].re : Wr[cc][e].im);
           },
-          tbuf(nch), xbx, xby, SF + chan_store(NF, ch) * g.M, soff, r, h, tw64l);
+          tbuf(nch), xbx, xby, SF + chan_store(NF, ch) * g.M, soff, r, tw64l);
     }
     if constexpr (NF >= 11) {
       if (p1 > N64) {
         // vector (i k_a w3) and tensor (aik2 k_i k_j + delta_ij w3/R) channels in CV,
         // from the spectrum and per-element weight factors narrowed to CV (weights.hpp:110-137,164-178)
-        cx<CV> XC[CPS][8];
-        WFac<CV> wf[CPS][8];
-        CV kxc[8];
+        cx<CV> XC[CPS][4];
+        WFac<CV> wf[CPS][4];
+        CV kxc[4];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kxof(e));
+        for (int e = 0; e < 4; ++e) kxc[e] = static_cast<CV>(kxof(e));
 #pragma unroll
         for (int cc = 0; cc < CPS; ++cc)
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            XC[cc][e] = cvt<CV>(Xof(cc, e));
-            wf[cc][e] = wfactors<CV>(Wof(cc, e), kxof(e), kyof(cc), kzv, invR);
+          for (int e = 0; e < 4; ++e) {
+            XC[cc][e] = cvt<CV>(X[cc][e]);
+            wf[cc][e] = wfactors<CV>(Wr[cc][e], kxof(e), kyof(cc), kzv, invR);
           }
         const CV kzc = static_cast<CV>(kzv), irc = static_cast<CV>(invR);
 #pragma unroll 1
@@ -1057,7 +1151,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
               [&](int cc, int e) -> cx<CV> {
                 return cmul(chan_weight<CV>(ch, kxc[e], static_cast<CV>(kyof(cc)), kzc, wf[cc][e], irc), XC[cc][e]);
               },
-              tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, soff, r, h, twvl);
+              tbuf(nch), xbx, xby, SF + static_cast<long>(ch) * g.M, soff, r, twvl);
         }
       }
     }
@@ -1070,31 +1164,16 @@ struct BwdCfg {
   using P = PlaneCfg<NX, Q>;
   static constexpr int NXHP = P::NXH + 1, QP = Q + 1;
   static constexpr int tb1 = align16(Q * NXHP * static_cast<int>(sizeof(cx<S>)));  // one channel transpose
+  static constexpr int tbf = align16(P::NXH * QP * static_cast<int>(sizeof(cx<S>)));  // the final transpose
   static constexpr int xy_bytes = align16(P::SBYH * slot_buf_len<Q>() * P::XBYTES);
   static constexpr int xx_bytes = align16(P::SBX * slot_buf_len<P::NXH>() * P::XBYTES);
-  // final phase: full x-FFT slots and full-row y-FFT slots (after barriers, any region)
-  static constexpr int xf_bytes() {
-    const int a = align16(P::SBX * slot_buf_len<NX>() * P::XBYTES), b = align16(P::SBYF * slot_buf_len<Q>() * P::XBYTES);
-    return a > b ? a : b;
-  }
   static constexpr int tw_b = tw_bytes<CV>();
   static constexpr bool DB = 2 * tb1 + xy_bytes + xx_bytes + tw_b <= 160 * 1024;
   static constexpr int ntb = DB ? 2 : 1;
-  // the final transpose takes half the rows at a time when all of them do not fit
-  static constexpr int tb3_half = align16((NX / 2) * QP * static_cast<int>(sizeof(cx<S>)));
-  static constexpr int xx_off = ntb * tb1 > tb3_half ? ntb * tb1 : tb3_half;
+  static constexpr int xx_off = ntb * tb1 > tbf ? ntb * tb1 : tbf;
   static constexpr int xy_off = DB ? xx_off + xx_bytes : xx_off;
-  static constexpr int slots_end() {
-    const int e1 = DB ? xy_off + xy_bytes : xx_off + (xx_bytes > xy_bytes ? xx_bytes : xy_bytes);
-    const int e2 = xx_off + xf_bytes();
-    return e1 > e2 ? e1 : e2;
-  }
-  static constexpr int tw_off = slots_end();
+  static constexpr int tw_off = DB ? xy_off + xy_bytes : xx_off + (xx_bytes > xy_bytes ? xx_bytes : xy_bytes);
   static constexpr int smem = tw_off + tw_b;
-  static constexpr int tb3_bytes = xx_off;  // the final transpose uses the channel transpose region
-  static constexpr int rows3() {            // rows of the final transpose per batch
-    return NX * QP * static_cast<int>(sizeof(cx<S>)) <= tb3_bytes ? NX : NX / 2;
-  }
 };
 
 // Work of the backward plane kernel: (unit, channel) items as in the forward
@@ -1122,9 +1201,6 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   using BC = BwdCfg<NX, Q, S, CV>;
   constexpr int CPS = PC::CPS, NXH = PC::NXH;
   constexpr int NXHP = BC::NXHP, QP = BC::QP;
-  constexpr int ROWS3 = BC::rows3(), NBR = NX / ROWS3;
-  constexpr int RPI = (ROWS3 + PC::SBYF - 1) / PC::SBYF;
-  static_assert(ROWS3 * QP * static_cast<int>(sizeof(cx<S>)) <= BC::tb3_bytes, "transpose buffer too small");
   extern __shared__ __align__(16) unsigned char smem[];
   auto tbuf = [&](int n) { return reinterpret_cast<cx<S>*>(smem + (n & 1) * (BC::DB ? BC::tb1 : 0)); };
   unsigned char* xbx = smem + BC::xx_off;
@@ -1133,7 +1209,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   constexpr bool SPL = sizeof(CV) == 8;  // FP64 exchanges one real part at a time
   const int tid = threadIdx.x;
   const int xs = tid / PC::TX, xl = tid % PC::TX;
-  const int ysh = tid / PC::TYH, ylh = tid % PC::TYH, ysf = tid / PC::TYF, ylf = tid % PC::TYF;
+  const int ysh = tid / PC::TYH, ylh = tid % PC::TYH;
   const CV invR = static_cast<CV>(1.0 / g.R);
   const long items = static_cast<long>(nunits) * kBwdPlaneCh;
   const long lb = xcd_remap(blockIdx.x, gridDim.x);  // neighbouring ranges share an XCD's L2
@@ -1144,17 +1220,17 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
     const int c0 = static_cast<int>(it - static_cast<long>(unit) * kBwdPlaneCh);
     const int c1 = static_cast<int>(min(static_cast<long>(kBwdPlaneCh), c0 + (i1 - it)));
     it += c1 - c0;
-    const int h = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+    const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
     const double kzv = kz * g.dkz;
 
     cx<S> nxt[PC::RPH][4];
-    int loff[PC::RPH][4];  // the lane's input offsets, the same for every channel of the unit
+    int loff[PC::RPH][4];  // the lane's input / output offsets, the same for every channel of the unit
 #pragma unroll
     for (int rr = 0; rr < PC::RPH; ++rr) {
       const int i = ysh + PC::SBYH * rr;
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        loff[rr][q] = static_cast<int>(split_idx(kz, r, 2 * (i < NXH ? i : 0) + h, ylh + PC::TYH * q, PC::MB, g.nzh));
+        loff[rr][q] = static_cast<int>(split_idx(kz, r, 2 * (i < NXH ? i : 0) + p, ylh + PC::TYH * q, PC::MB, g.nzh));
     }
     auto load_in = [&](int ci) {
       const cx<S>* in = SB + static_cast<long>(bwd_channel(ci)) * g.M;
@@ -1164,24 +1240,24 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         for (int q = 0; q < 4; ++q) nxt[rr][q] = in[loff[rr][q]];
     };
     load_in(c0);
-    // weight factors of the lane's accumulator elements (kx = xl + TX e, column xs + SBX cc),
-    // computed in FP64 and kept in the backward path's precision
-    WFac<CV> wf[CPS][8];
-    CV kxc[8];
-    cx<CV> acc[CPS][8];
+    // weight factors of the lane's accumulator elements (k = xl + TX e, kx = 2 k + p,
+    // column xs + SBX cc), computed in FP64 and kept in the backward path's precision
+    WFac<CV> wf[CPS][4];
+    CV kxc[4];
+    cx<CV> acc[CPS][4];
     int xlo = xl;  // opaque: keeps the per-element set-up inside the unit loop
     asm volatile("" : "+v"(xlo));
-    const cx<double>* wu = WT + (static_cast<long>(kz) * 2 + r) * Q * NX + xlo;
+    const cx<double>* wu = WT + static_cast<long>(unit) * Q * NXH + xlo;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) kxc[e] = static_cast<CV>(kvec(xlo + PC::TX * e, NX, g.dkx));
+    for (int e = 0; e < 4; ++e) kxc[e] = static_cast<CV>(kvec(2 * (xlo + PC::TX * e) + p, NX, g.dkx));
 #pragma unroll
     for (int cc = 0; cc < CPS; ++cc) {
       const int col = xs + PC::SBX * cc;
       const double kyv = kvec(r + 2 * (col < Q ? col : 0), PC::NY, g.dky);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        wf[cc][e] = wfactors<CV>(wu[static_cast<long>(col < Q ? col : 0) * NX + PC::TX * e],
-                                 kvec(xlo + PC::TX * e, NX, g.dkx), kyv, kzv, 1.0 / g.R);
+      for (int e = 0; e < 4; ++e) {
+        wf[cc][e] = wfactors<CV>(wu[static_cast<long>(col < Q ? col : 0) * NXH + PC::TX * e],
+                                 kvec(2 * (xlo + PC::TX * e) + p, NX, g.dkx), kyv, kzv, 1.0 / g.R);
         acc[cc][e] = {CV(0), CV(0)};
       }
     }
@@ -1189,7 +1265,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
 #pragma unroll 1
     for (int ch = c0; ch < c1; ++ch) {
       cx<S>* tb = tbuf(ch);
-      // y-FFT of the rows ix = 2i + h
+      // y-FFT of the sub-plane's rows
       cx<CV> v[PC::RPH][4];
 #pragma unroll
       for (int rr = 0; rr < PC::RPH; ++rr)
@@ -1211,7 +1287,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
       }
       __syncthreads();
       const ChanForm<CV> form = chan_form<CV>(bwd_channel(ch), true, invR);
-      // half x-FFT (DIT over the rows of parity h), conj-weighted accumulation over all kx
+      // x-FFT of the sub-plane's columns, conj-weighted accumulation
 #pragma unroll
       for (int cc = 0; cc < CPS; ++cc) {
         const int col = xs + PC::SBX * cc;
@@ -1222,15 +1298,8 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
           for (int q = 0; q < 4; ++q) f[q] = cvt<CV>(tb[col * NXHP + xl + PC::TX * q]);
           slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, twl);
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            cx<CV> oa = f[e];
-            if (h) oa = twiddle<CV, NX, -1>(oa, xl + PC::TX * e, twl);
-            const cx<CV> ob = h ? cneg(oa) : oa;
-            const cx<CV> wa = apply_form<CV>(form, kxc[e], kyv, kzc, wf[cc][e]);
-            const cx<CV> wb = apply_form<CV>(form, kxc[e + 4], kyv, kzc, wf[cc][e + 4]);
-            acc[cc][e] = cadd(acc[cc][e], cmul(wa, oa));
-            acc[cc][e + 4] = cadd(acc[cc][e + 4], cmul(wb, ob));
-          }
+          for (int e = 0; e < 4; ++e)
+            acc[cc][e] = cadd(acc[cc][e], cmul(apply_form<CV>(form, kxc[e], kyv, kzc, wf[cc][e]), f[e]));
         }
       }
       // With DB the next channel writes the other transpose buffer and other
@@ -1239,45 +1308,32 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
       if constexpr (!BC::DB) __syncthreads();
     }
     __syncthreads();
-    // x-inverse of the accumulated spectrum (all kx), then y-inverse of every row
+    // x-inverse of the accumulated sub-plane, then y-inverse of every row
+    cx<S>* tbf = reinterpret_cast<cx<S>*>(smem);
 #pragma unroll
     for (int cc = 0; cc < CPS; ++cc) {
       const int col = xs + PC::SBX * cc;
-      if (col < Q) slot_fft<CV, NX, PC::TX, +1, SPL>(acc[cc], xbx + xs * slot_buf_len<NX>() * PC::XBYTES, xl, twl);
-    }
-    cx<S>* tb3 = reinterpret_cast<cx<S>*>(smem);
-    cx<S>* out = P + static_cast<long>((c0 > 0 ? 2 : 0) + h) * g.M;
+      if (col < Q) {
+        slot_fft<CV, NXH, PC::TX, +1, SPL>(acc[cc], xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, twl);
 #pragma unroll
-    for (int bt = 0; bt < NBR; ++bt) {
-      const int r0 = bt * ROWS3, r1 = r0 + ROWS3;
-      __syncthreads();
-#pragma unroll
-      for (int cc = 0; cc < CPS; ++cc) {
-        const int col = xs + PC::SBX * cc;
-        if (col < Q) {
-#pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int ixx = xl + PC::TX * e;
-            if (ixx >= r0 && ixx < r1) tb3[(ixx - r0) * QP + col] = cvt<S>(acc[cc][e]);
-          }
-        }
+        for (int e = 0; e < 4; ++e) tbf[(xl + PC::TX * e) * QP + col] = cvt<S>(acc[cc][e]);
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    cx<S>* out = P + static_cast<long>(c0 > 0 ? 1 : 0) * g.M;
 #pragma unroll
-      for (int rr = 0; rr < RPI; ++rr) {
-        const int row = ysf + PC::SBYF * rr;
-        if (row < ROWS3) {
-          cx<CV> v[8];
+    for (int rr = 0; rr < PC::RPH; ++rr) {
+      const int row = ysh + PC::SBYH * rr;
+      if (row < NXH) {
+        cx<CV> w[4];
 #pragma unroll
-          for (int q = 0; q < 8; ++q) v[q] = cvt<CV>(tb3[row * QP + ylf + PC::TYF * q]);
-          slot_fft<CV, Q, PC::TYF, +1, SPL>(v, xbx + ysf * slot_buf_len<Q>() * PC::XBYTES, ylf, twl);
+        for (int q = 0; q < 4; ++q) w[q] = cvt<CV>(tbf[row * QP + ylh + PC::TYH * q]);
+        slot_fft<CV, Q, PC::TYH, +1, SPL>(w, xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, twl);
 #pragma unroll
-          for (int e = 0; e < 8; ++e) {
-            const int mm = ylf + PC::TYF * e;
-            cx<CV> b = v[e];
-            if (r) b = twiddle<CV, PC::NY, +1>(b, mm, twl);
-            out[split_idx(kz, r, r0 + row, mm, PC::MB, g.nzh)] = cvt<S>(b);
-          }
+        for (int e = 0; e < 4; ++e) {
+          cx<CV> b = w[e];
+          if (r) b = twiddle<CV, PC::NY, +1>(b, ylh + PC::TYH * e, twl);
+          out[loff[rr][e]] = cvt<S>(b);
         }
       }
     }
@@ -1344,8 +1400,7 @@ __global__ __launch_bounds__(256) void k_fast_mf_layout(const cx<double>* __rest
   const long plane = static_cast<long>(nx) * ny;
   const long kz = i / plane, rem = i - kz * plane;
   const int kx = static_cast<int>(rem / ny), ky = static_cast<int>(rem % ny);
-  const int q = ny / 2, r = ky & 1, s = ky >> 1;
-  out[((kz * 2 + r) * q + s) * nx + kx] = spec[i].re;
+  out[hp_idx(static_cast<int>(kz), ky & 1, kx & 1, ky >> 1, kx >> 1, ny / 2, nx / 2)] = spec[i].re;
 }
 }  // namespace
 
@@ -1356,7 +1411,9 @@ void launch_fast_mf_layout(const cx<double>* spec, double* out, const Geom& g, h
   k_fast_mf_layout<<<static_cast<unsigned>((m + 255) / 256), 256, 0, stream>>>(spec, out, g.nx, g.ny, m);
 }
 
-int fast_point_blocks(const FastGeom& g) { return g.nx * g.mb * 4; }
+int fast_point_blocks(const FastGeom& g, bool fp64) {
+  return g.nx * g.q / (fp64 ? zp_waves<double>() : zp_waves<float>());  // one line pair (x, m < NY/2) per wave
+}
 
 namespace {
 template <typename T>
@@ -1399,13 +1456,13 @@ void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom&
 template <typename S, typename CV>
 void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
                       const void* twv, const double* partials, int fin_rows, double* res, hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(g.nx * g.mb) + (fin_rows > 0 ? 1u : 0u);
+  const unsigned nblk = static_cast<unsigned>((g.nx / 2) * g.mb) + (fin_rows > 0 ? 1u : 0u);
 #define CDFT_L(NZ)                                                                                      \
   {                                                                                                     \
     auto k = kf_zinv<S, CV, NZ>;                                                                        \
     const int sm = zinv_smem<NZ, CV>();                                                                 \
     set_smem(k, sm);                                                                                    \
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(2 * NZ), sm, st, P, out, add, scale, g,                      \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(zinv_threads<NZ>()), sm, st, P, out, add, scale, g,          \
                        as<cx<CV>>(twv), fast_bwd_blocks(g), partials, fin_rows, res);                   \
   }
   if (g.nz == 64) CDFT_L(64) else if (g.nz == 128) CDFT_L(128) else throw std::runtime_error("fast path: nz");
@@ -1494,7 +1551,7 @@ template <typename S, typename CV>
 void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double* lf, double* partials,
                         const FastGeom& g, const void* tw64, const void* twv, ChannelSet cs, FmtParams fp, double mu,
                         AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(fast_point_blocks(g));
+  const unsigned nblk = static_cast<unsigned>(fast_point_blocks(g, sizeof(S) == 8));
   // White-Bear II (the reference's default model) has its own instantiation
   const bool wb2 = fp.model == FMT_WHITEBEAR_II;
 #define CDFT_L(NZ, NF, NB)                                                                             \
@@ -1503,7 +1560,7 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
                                : kf_zpoint<S, CV, NZ, NF, NB, kFmtRuntime>;                            \
     const int sm = zpoint_smem<NZ, S>();                                                               \
     set_smem(k, sm);                                                                                   \
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(kZP), sm, st, SF, rho, vext, lf, partials, g,               \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(64 * zp_waves<S>()), sm, st, SF, rho, vext, lf, partials, g, \
                        as<cx<double>>(tw64), as<cx<CV>>(twv), fp, mu, amap, shift_sum, dc);            \
   }
 #define CDFT_CH(NZ)                                                                                    \

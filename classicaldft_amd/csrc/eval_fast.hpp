@@ -1,15 +1,18 @@
-// Fused five-kernel evaluation schedule for power-of-two grids.
+// Fused six-kernel evaluation schedule for power-of-two grids.
 //
 // Spectral data of every field lives in a "chunked split" layout.  The y axis
 // is split by parity (radix-2 decimation done inside the z-line kernels):
 // for a z-transformed field x~(ix, iy, kz) the two parts are
-//   a_r(kz, ix, m) = x~(ix, m, kz) + (-1)^r x~(ix, m + Q, kz),  Q = Ny/2,
-// and element (kz, r, ix, m) is stored at
+//   a_r(kz, ix, m) = x~(ix, m, kz) + (-1)^r x~(ix, m + Q, kz),  Q = Ny/2
+// (the r = 1 part also twiddled), and element (kz, r, ix, m) is stored at
 //   ((ix * (Q/16) + m/16) * (Nz/2+1) + kz) * 32 + r * 16 + m % 16,
 // so the 16 consecutive m of one (kz, r) share a 128-byte segment and the
 // data of 16 line pairs (m .. m+15 and m+Q .. m+Q+15 at one ix) form one
-// contiguous chunk.  A plane kernel owns one (kz, r) half-plane, the z-line
-// kernels own chunks; both read and write whole segments.
+// contiguous chunk.  The channel arrays between the plane kernels and the
+// z-line kernels split x the same way, by kx parity p: row ix = 2 i + p holds
+// half row i of part p.  A plane kernel owns one (kz, r, p) sub-plane, the
+// z-line kernels own the chunks of half rows i (lines x = i and i + Nx/2);
+// both read and write whole or half segments.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -33,11 +36,12 @@ FastGeom make_fast_geom(const Geom& g);
 void fast_twiddles(void** tw64, void** tw32);
 
 // Mean-field weight: real parts of the FP64 spectrum in plane-major order
-// [kz][kx][ky] scattered into the plane kernel's order [kz][r][s][kx]
-// (ky = r + 2 s), on the device.
+// [kz][kx][ky] scattered into the plane kernels' sub-plane order
+// [kz][r][p][s][k] (kx = 2 k + p, ky = r + 2 s), on the device.
 void launch_fast_mf_layout(const cx<double>* spec, double* out, const Geom& g, hipStream_t stream);
 
-int fast_point_blocks(const FastGeom& g);
+// Blocks of the point kernel (rows of its partial sums) for FP64 or FP32 storage.
+int fast_point_blocks(const FastGeom& g, bool fp64);
 int fast_bwd_blocks(const FastGeom& g);
 
 // z-R2C of rho into the split layout; D0 (nx * ny doubles) receives the FP64
@@ -46,7 +50,7 @@ template <typename S>
 void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom& g, const void* tw64, AliasMap amap,
                       hipStream_t st);
 // FP64 spectrum rho_hat of the y-parity parts (M complex doubles, order
-// [kz][r][s][kx], ky = r + 2 s), read by the forward plane kernel, without
+// [kz][r][p][s][k], kx = 2 k + p, ky = r + 2 s), read by the forward plane kernel, without
 // its (0, 0, 0) coefficient; sum_out[0] receives that coefficient (sum of rho).
 template <typename S>
 void launch_fast_rho_xy(const cx<S>* A0, const double* D0, void* XR, double* sum_out, const FastGeom& g,
