@@ -461,8 +461,8 @@ void Context::stage_bytes(double* b, bool with_vext) const {
     }
     b[3] = nfp * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nbp * sf * M + 8.0 * N;
     if (cs.nb > 0) {
-      b[4] = nbp * sf * M + 16.0 * M + 2.0 * sf * M;   // channels, weights -> partial sums (h = 0, 1)
-      b[5] = 2.0 * sf * M + 8.0 * N + 8.0 * N;         // partials + local force -> force
+      b[4] = nbp * sf * M + 16.0 * M + sf * M;         // channels, weights -> accumulated sub-planes
+      b[5] = sf * M + 8.0 * N + 8.0 * N;               // sub-planes + local force -> force
     }
     return;
   }

@@ -77,10 +77,18 @@ __host__ __device__ constexpr int xcd_remap(int b, int n) {
 
 __host__ __device__ constexpr int align16(int b) { return (b + 15) & ~15; }
 
-// Split-layout offset of element (kz, r, ix, m).  Between the plane kernels
-// and the z-line kernels, ix = 2 i + p holds half row i of kx-parity part p.
+// Split-layout offset of element (kz, r, ix, m) (rho's z-spectrum).
 __device__ __forceinline__ long split_idx(int kz, int r, int ix, int m, int mb, int nzh) {
   return ((static_cast<long>(ix) * mb + (m >> 4)) * nzh + kz) * 32 + r * 16 + (m & 15);
+}
+
+// Channel-layout offset of element (kz, r, i, p, m) of the channel arrays
+// between the plane kernels and the z-line kernels: half row i of kx-parity
+// part p.  A 32-element row [r][p][m % 8] holds both parts of 8 columns, so
+// a z-line block, which combines the parts of its columns, reads one 128-byte
+// segment per (kz, r); a plane block reads or writes half segments.
+__device__ __forceinline__ long chan_idx(int kz, int r, int i, int p, int m, int mb8, int nzh) {
+  return ((static_cast<long>(i) * mb8 + (m >> 3)) * nzh + kz) * 32 + r * 16 + p * 8 + (m & 7);
 }
 
 // Plane kernels' order of a half spectrum: element (kx = 2 k + p, ky = r + 2 s, kz)
@@ -251,16 +259,17 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
 }
 
 // ============================================================ kf_zinv
-// A block (4 NZ threads, 32 slots) owns half row i and 16 consecutive m: the
-// line pairs (x, m) / (x, m + NY/2) at x = i and x = i + NX/2.
+// A block (2 NZ threads, 16 slots) owns half row i and 8 consecutive m: the
+// line pairs (x, m) / (x, m + NY/2) at x = i and x = i + NX/2, whose inputs
+// are one channel-layout row per kz.
 template <int NZ>
 __host__ __device__ constexpr int zinv_threads() {
-  return 4 * NZ;
+  return 2 * NZ;
 }
 template <int NZ, typename C>
 __host__ __device__ constexpr int zinv_tw_off() {
-  return align16(2 * (NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<C>))) +
-         32 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<C>));
+  return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<C>))) +
+         16 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<C>));
 }
 template <int NZ, typename C>
 __host__ __device__ constexpr int zinv_smem() {
@@ -311,45 +320,47 @@ __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __res
     reduce_rows<NT>(partials, fin_rows, res, sh, sh + NT);
     return;
   }
-  cx<C>* cb = reinterpret_cast<cx<C>*>(smem);  // [xh][r][m16][kz]
-  cx<C>* xbuf = reinterpret_cast<cx<C>*>(smem + align16(2 * NZH * 32 * static_cast<int>(sizeof(cx<C>))));
+  cx<C>* cb = reinterpret_cast<cx<C>*>(smem);  // [xh][r][m8][kz]
+  cx<C>* xbuf = reinterpret_cast<cx<C>*>(smem + align16(NZH * 32 * static_cast<int>(sizeof(cx<C>))));
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  const int xh = slot >> 4, ms = slot & 15;
+  const int xh = slot >> 3, ms = slot & 7;
+  const int mb8 = g.q >> 3;
   const long blk = blockIdx.x;
-  const int i = static_cast<int>(blk / g.mb), mc = static_cast<int>(blk % g.mb);
-  const int m = mc * 16 + ms;
+  const int i = static_cast<int>(blk / mb8), mc = static_cast<int>(blk % mb8);
+  const int m = mc * 8 + ms;
   const long la = static_cast<long>(i + xh * (g.nx / 2)) * g.ny + m, lb = la + g.q;
-  // partial sets of the backward plane kernel at set * M; parts p = 0, 1 of
-  // half row i are the chunks of ix = 2 i and 2 i + 1
-  const long ochunk = static_cast<long>(g.mb) * NZH * 32;
-  const cx<S>* p0 = P + (2L * i * g.mb + mc) * static_cast<long>(NZH * 32);
+  // partial sets of the backward plane kernel at set * M, in the channel
+  // layout: the block's rows hold parts p = 0, 1 of half row i, columns 8 mc ..
+  const cx<S>* p0 = P + blk * static_cast<long>(NZH * 32);
   const cx<C>* twl = stage_twiddles<C, NT>(smem + zinv_tw_off<NZ, C>(), tw);
   const int nunits = 4 * NZH;
   {
     // exp(2 pi i i / NX), read from global memory (the LDS copy is not yet published)
     cx<C> w = tw[g.nx - 4 + i];
     w.im = -w.im;
-    constexpr int K = (NZH * 32 + NT - 1) / NT;
+    constexpr int NPR = NZH * 16, K = (NPR + NT - 1) / NT;  // (kz, r, column) pairs of parts
     cx<C> e[K], o[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int j = static_cast<int>(threadIdx.x) + NT * k;
-      if (j < NZH * 32) {
-        const int u = ((j >> 5) * 2 + ((j >> 4) & 1)) * 2;  // sub-plane (kz, r, p = 0)
-        e[k] = cvt<C>(p0[j]);
-        o[k] = cvt<C>(p0[ochunk + j]);
-        if (bwd_unit_split(u, nunits, bwd_blocks)) e[k] = cadd(e[k], cvt<C>(p0[g.M + j]));
-        if (bwd_unit_split(u + 1, nunits, bwd_blocks)) o[k] = cadd(o[k], cvt<C>(p0[g.M + ochunk + j]));
+      if (j < NPR) {
+        const int kz = j >> 4, r = (j >> 3) & 1, c = j & 7;
+        const int u = (kz * 2 + r) * 2;  // sub-plane (kz, r, p = 0)
+        const int off = kz * 32 + r * 16 + c;
+        e[k] = cvt<C>(p0[off]);
+        o[k] = cvt<C>(p0[off + 8]);
+        if (bwd_unit_split(u, nunits, bwd_blocks)) e[k] = cadd(e[k], cvt<C>(p0[g.M + off]));
+        if (bwd_unit_split(u + 1, nunits, bwd_blocks)) o[k] = cadd(o[k], cvt<C>(p0[g.M + off + 8]));
       }
     }
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int j = static_cast<int>(threadIdx.x) + NT * k;
-      if (j < NZH * 32) {
+      if (j < NPR) {
+        const int kz = j >> 4, rc = j & 15;  // rc = r * 8 + column
         const cx<C> wo = cmul(o[k], w);
-        const int c = ((j >> 4) & 1) * 16 + (j & 15), kz = j >> 5;  // (r, m % 16)
-        cb[c * NZH + kz] = cadd(e[k], wo);
-        cb[(32 + c) * NZH + kz] = csub(e[k], wo);
+        cb[rc * NZH + kz] = cadd(e[k], wo);
+        cb[(16 + rc) * NZH + kz] = csub(e[k], wo);
       }
     }
   }
@@ -363,7 +374,7 @@ __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __res
   }
   __syncthreads();
   cx<C> v[8];
-  build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[((xh * 2 + r) * 16 + ms) * NZH + kz]; });
+  build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[((xh * 2 + r) * 8 + ms) * NZH + kz]; });
   slot_fft<C, NZ, T, +1>(v, xbuf + slot * XS, lane, twl);
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
@@ -491,20 +502,20 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
   cx<S>* stg = reinterpret_cast<cx<S>*>(smem);  // [gs][r][line pair][kz]
   const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
   unsigned char* xwave = smem + zpoint_stage_bytes<NZ, S>() + wave * NS * XS * static_cast<int>(sizeof(cx<double>));
-  // Block: half row i and the MW columns m0 .. m0 + MW - 1 (m0 = 16 mc + MW mq);
+  // Block: half row i and the MW columns m0 .. m0 + MW - 1 (m0 = 8 mc + MW mq);
   // wave (xh, mm) = (wave / MW, wave % MW) owns the line pair at
-  // x = i + xh NX/2, m = m0 + mm.  The blocks of one 16-column group are
-  // consecutive, so they share an XCD's L2 and each chunk row is fetched once.
+  // x = i + xh NX/2, m = m0 + mm.  The blocks of one 8-column group are
+  // consecutive, so they share an XCD's L2 and each channel row is fetched once.
   const int logical = xcd_remap(blockIdx.x, gridDim.x);
-  const int mq = logical % (16 / MW);
-  const long grp = logical / (16 / MW);
-  const int irow = static_cast<int>(grp / g.mb), mc = static_cast<int>(grp % g.mb);
+  const int mb8 = g.q >> 3;
+  const int mq = logical % (8 / MW);
+  const long grp = logical / (8 / MW);
+  const int irow = static_cast<int>(grp / mb8), mc = static_cast<int>(grp % mb8);
   const int xh = wave / MW, mm = wave % MW;
-  const int m = mc * 16 + MW * mq + mm;
+  const int m = mc * 8 + MW * mq + mm;
   const long la = static_cast<long>(irow + xh * (g.nx / 2)) * g.ny + m, lb = la + g.q;
-  // chunk of ix = 2 i (part p = 0) at column m0; part 1 is ochunk further
-  const long cbase = (2L * irow * g.mb + mc) * static_cast<long>(NZH * 32) + MW * mq;
-  const long ochunk = static_cast<long>(g.mb) * NZH * 32;
+  // channel rows of half row i at columns m0 (part p = 0; part 1 is 8 further)
+  const long cbase = (static_cast<long>(irow) * mb8 + mc) * (NZH * 32) + MW * mq;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   S wv[NWV][PPL];
   // twiddle tables in LDS; the first barrier below publishes them
@@ -516,7 +527,7 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
   // The loads of round JR + 1 are issued into registers before round JR's
   // transforms, so their memory latency overlaps that work.
   // Pair j = tid + kZP k of a round: (slot gs, part r, column mm, kz).  It
-  // loads the kx-parity parts E (ix = 2 i) and O (ix = 2 i + 1) and stages
+  // loads the kx-parity parts E (p = 0) and O (p = 1) of half row i and stages
   // E + w O for the line pair at x = i and E - w O for x = i + NX/2,
   // w = exp(2 pi i i / NX); its offsets in a channel's chunk and in the
   // staging buffer are the same every round and computed once.
@@ -542,7 +553,7 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
       const int ch = zf_chan(NF, jr * NS + gsk[k]);
       const cx<S>* src = sfc + static_cast<long>(ch >= 0 ? chan_store(NF, ch) : 0) * g.M + goff[k];
       pfe[k] = src[0];
-      pfo[k] = src[ochunk];
+      pfo[k] = src[8];
     }
   };
   // the combine runs in the round's arithmetic (FP64 for the scalar channels)
@@ -725,7 +736,7 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
       if ((NSP % kZP == 0 || static_cast<int>(threadIdx.x) + kZP * k < NSP) && b < kZB && b != 1) {
         cx<S>* o = SF + static_cast<long>(zb_chan(b)) * g.M + cbase + goff[k];
         o[0] = va[k];
-        o[ochunk] = vb[k];
+        o[8] = vb[k];
       }
     }
   };
@@ -1092,7 +1103,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int row = min(tid / P::TYH + P::SBYH * rr, NXH - 1), m = tid % P::TYH + P::TYH * e;
-        soff[rr][e] = static_cast<int>(split_idx(kz, r, 2 * row + p, m, P::MB, g.nzh));
+        soff[rr][e] = static_cast<int>(chan_idx(kz, r, row, p, m, Q / 8, g.nzh));
       }
     // the lane's elements of the sub-plane: column xs + SBX cc, k = xl + TX e
     const long uoff = static_cast<long>(unit) * Q * NXH + xl;
@@ -1230,7 +1241,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
       const int i = ysh + PC::SBYH * rr;
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        loff[rr][q] = static_cast<int>(split_idx(kz, r, 2 * (i < NXH ? i : 0) + p, ylh + PC::TYH * q, PC::MB, g.nzh));
+        loff[rr][q] = static_cast<int>(chan_idx(kz, r, i < NXH ? i : 0, p, ylh + PC::TYH * q, Q / 8, g.nzh));
     }
     auto load_in = [&](int ci) {
       const cx<S>* in = SB + static_cast<long>(bwd_channel(ci)) * g.M;
@@ -1456,7 +1467,7 @@ void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom&
 template <typename S, typename CV>
 void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
                       const void* twv, const double* partials, int fin_rows, double* res, hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>((g.nx / 2) * g.mb) + (fin_rows > 0 ? 1u : 0u);
+  const unsigned nblk = static_cast<unsigned>((g.nx / 2) * (g.q / 8)) + (fin_rows > 0 ? 1u : 0u);
 #define CDFT_L(NZ)                                                                                      \
   {                                                                                                     \
     auto k = kf_zinv<S, CV, NZ>;                                                                        \
