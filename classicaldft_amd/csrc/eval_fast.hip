@@ -1,7 +1,7 @@
 // Fused six-kernel evaluation schedule (gfx950) for power-of-two grids.
 //
-//   kf_zfwd       z-R2C of rho - mean (FP64), y-parity split    rho -> A0 (chunked split layout)
-//   kf_rho_xy     per (kz, r): y- and x-FFT of rho's z-spectrum (FP64) -> rho_hat
+//   kf_zfwd       per line quad: z-R2C of rho (FP64), y- and x-parity split -> A0
+//   kf_rho_xy     per (kz, r, p): y- and x-FFT of rho's sub-plane (FP64) -> rho_hat
 //   kf_plane_fwd  per (kz, r, p): for each of the 10 plane channels: weight
 //                 times rho_hat on the kx = 2k + p sub-plane, its x- and
 //                 y-inverse (NX/2 x NY/2 points), written in the split layout
@@ -77,16 +77,12 @@ __host__ __device__ constexpr int xcd_remap(int b, int n) {
 
 __host__ __device__ constexpr int align16(int b) { return (b + 15) & ~15; }
 
-// Split-layout offset of element (kz, r, ix, m) (rho's z-spectrum).
-__device__ __forceinline__ long split_idx(int kz, int r, int ix, int m, int mb, int nzh) {
-  return ((static_cast<long>(ix) * mb + (m >> 4)) * nzh + kz) * 32 + r * 16 + (m & 15);
-}
-
-// Channel-layout offset of element (kz, r, i, p, m) of the channel arrays
-// between the plane kernels and the z-line kernels: half row i of kx-parity
-// part p.  A 32-element row [r][p][m % 8] holds both parts of 8 columns, so
-// a z-line block, which combines the parts of its columns, reads one 128-byte
-// segment per (kz, r); a plane block reads or writes half segments.
+// Channel-layout offset of element (kz, r, i, p, m) of the arrays between
+// the z-line kernels and the plane kernels (rho's parts, the channels, the
+// backward partials): half row i of kx-parity part p.  A 32-element row
+// [r][p][m % 8] holds both parts of 8 columns, so a z-line block, which
+// combines the parts of its columns, reads or writes one 128-byte segment per
+// (kz, r); a plane block reads or writes half segments.
 __device__ __forceinline__ long chan_idx(int kz, int r, int i, int p, int m, int mb8, int nzh) {
   return ((static_cast<long>(i) * mb8 + (m >> 3)) * nzh + kz) * 32 + r * 16 + p * 8 + (m & 7);
 }
@@ -198,7 +194,7 @@ __device__ __forceinline__ cx<C>* stage_twiddles(unsigned char* dst, const cx<C>
 // ============================================================ kf_zfwd
 template <int NZ, typename S>
 __host__ __device__ constexpr int zfwd_tw_off() {
-  return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<S>))) +
+  return align16((NZ / 2 + 1) * 32 * static_cast<int>(sizeof(cx<double>))) +
          16 * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
 }
 template <int NZ, typename S>
@@ -206,22 +202,29 @@ __host__ __device__ constexpr int zfwd_smem() {
   return zfwd_tw_off<NZ, S>() + tw_bytes<double>();
 }
 
+// Block (2 NZ threads, 16 slots): half row i and 8 columns m = 8 mc + c, the
+// line pairs at x = i and x = i + NX/2.  z-R2C (FP64) of every line pair, the
+// y-parity split, then the x-parity split of the half row: part p = 0 is the
+// sum, p = 1 the difference of the two lines (its twiddle exp(-2 pi i i / NX)
+// is applied by kf_rho_xy, in FP64).  The block's output is one channel-layout
+// chunk.  The kz = 0 coefficients (the line sums, which carry the mean density)
+// are real; they are also written in FP64 to D0 in the same row order
+// (D0[chunk * 32 + r * 16 + p * 8 + c]): kf_rho_xy takes the kz = 0 plane from
+// there, so its FP32 rounding never meets the mean.
 template <typename S, int NZ>
-// The kz = 0 coefficients (the line sums, which carry the mean density) are
-// also written in FP64 to D0[chunk * 32 + r * 16 + m % 16]: kf_rho_xy takes
-// the kz = 0 plane from there, so its FP32 rounding never meets the mean.
 __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho, cx<S>* __restrict__ out, FastGeom g,
                                                   const cx<double>* __restrict__ tw, AliasMap amap,
                                                   double* __restrict__ D0) {
   constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   extern __shared__ __align__(16) unsigned char smem[];
-  cx<S>* cb = reinterpret_cast<cx<S>*>(smem);  // [r][m16][kz]
-  cx<double>* xbuf = reinterpret_cast<cx<double>*>(smem + align16(NZH * 32 * static_cast<int>(sizeof(cx<S>))));
+  cx<double>* cb = reinterpret_cast<cx<double>*>(smem);  // [xh][r][c][kz]
+  cx<double>* xbuf = reinterpret_cast<cx<double>*>(smem + align16(NZH * 32 * static_cast<int>(sizeof(cx<double>))));
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
-  const long chunk = blockIdx.x;
-  const int ix = static_cast<int>(chunk / g.mb);
-  const int m = static_cast<int>(chunk % g.mb) * 16 + slot;
-  const long la = static_cast<long>(ix) * g.ny + m, lb = la + g.q;
+  const int xh = slot >> 3, c = slot & 7;
+  const int mb8 = g.q >> 3;
+  const long blk = blockIdx.x;
+  const int i = static_cast<int>(blk / mb8), mc = static_cast<int>(blk % mb8);
+  const long la = static_cast<long>(i + xh * (g.nx / 2)) * g.ny + mc * 8 + c, lb = la + g.q;
   const cx<double>* twl = stage_twiddles<double, 2 * NZ>(smem + zfwd_tw_off<NZ, S>(), tw);
   cx<double> v[8];
 #pragma unroll
@@ -244,18 +247,25 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
       const cx<double> zk = v[e], zm = xb[xpad((NZ - k) & (NZ - 1))];
       const cx<double> A = {0.5 * (zk.re + zm.re), 0.5 * (zk.im - zm.im)};
       const cx<double> B = {0.5 * (zk.im + zm.im), -0.5 * (zk.re - zm.re)};
-      cb[slot * NZH + k] = cvt<S>(cadd(A, B));
-      cb[(16 + slot) * NZH + k] = cvt<S>(csub(A, B));
-      if (k == 0) {
-        D0[chunk * 32 + slot] = A.re + B.re;
-        D0[chunk * 32 + 16 + slot] = A.re - B.re;
-      }
+      cb[((xh * 2 + 0) * 8 + c) * NZH + k] = cadd(A, B);
+      cb[((xh * 2 + 1) * 8 + c) * NZH + k] = csub(A, B);
     }
   }
   __syncthreads();
-  cx<S>* o = out + chunk * static_cast<long>(NZH * 32);
-  block_copy<2 * NZ, NZH * 32, cx<S>>([&](int i) { return cb[(i & 31) * NZH + (i >> 5)]; },
-                                      [&](int i, cx<S> v) { o[i] = v; });
+  // row element j = kz * 32 + r * 16 + p * 8 + c of the chunk
+  cx<S>* o = out + blk * static_cast<long>(NZH * 32);
+  constexpr int K = (NZH * 32 + 2 * NZ - 1) / (2 * NZ);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int j = static_cast<int>(threadIdx.x) + 2 * NZ * k;
+    if (j < NZH * 32) {
+      const int kz = j >> 5, r = (j >> 4) & 1, p = (j >> 3) & 1, cc = j & 7;
+      const cx<double> a = cb[(r * 8 + cc) * NZH + kz], b = cb[((2 + r) * 8 + cc) * NZH + kz];
+      const cx<double> w = p ? csub(a, b) : cadd(a, b);
+      o[j] = cvt<S>(w);
+      if (kz == 0) D0[blk * 32 + j] = w.re;
+    }
+  }
 }
 
 // ============================================================ kf_zinv
@@ -877,110 +887,113 @@ __device__ __forceinline__ cx<C> apply_form(const ChanForm<C>& f, C kx, C ky, C 
 }
 
 // ---------------------------------------------------------------- kf_rho_xy
-// FP64 y- and x-FFT of the rho parts: block (kz, r) transforms the y lines
-// of its half plane once, then the columns in NPART batches (two; four for
-// NX = 256, to fit the transpose buffer in LDS) through one transpose buffer,
-// and writes the spectrum rho_hat(kx, ky = r + 2 s, kz) in the plane
-// kernels' sub-plane order, XR[hp_idx(kz, r, kx % 2, s, kx / 2)] (complex FP64).
-// (One block per half plane runs the 2 (Nz/2 + 1) blocks in a single round
-// on the CUs; splitting a half plane over two blocks would repeat its y
-// transforms and need a second round.)
-template <int NX, int Q, typename S>
+// FP64 y- and x-FFT of rho's parts: block (kz, r, p) owns one sub-plane of
+// NX/2 half rows i and Q columns m (the channel-layout chunks of kf_zfwd),
+// applies the parity twiddles exp(-2 pi i (r m / NY + p i / NX)), transforms
+// the rows (Q points) and then the columns (NX/2 points) through a transpose
+// taken one real part at a time, and writes rho_hat(kx = 2 k + p, ky = r + 2 s, kz)
+// at XR[hp_idx(kz, r, p, s, k)] (complex FP64).  512-thread blocks with 75 KB of
+// LDS run two to a CU, so the 4 (Nz/2 + 1) blocks take one round.
+template <int NX, int Q>
 struct RhoCfg {
-  using P = PlaneCfg<NX, Q>;
-  static constexpr int NPART = NX >= 256 ? 4 : 2;  // column batches of the transpose (LDS size)
-  static constexpr int NC = Q / NPART;             // columns per batch
-  static constexpr int TXR = NX / 4, SXR = kBT / TXR, CPR = (NC + SXR - 1) / SXR;
-  static constexpr int NXP = NX + 1;
-  static constexpr int tb_bytes = align16(NC * NXP * static_cast<int>(sizeof(cx<double>)));
-  static constexpr int xb_elems() {
-    const int a = P::SBYF * slot_buf_len<Q>(), b = SXR * slot_buf_len<NX>();
+  static constexpr int NT = 512, NXH = NX / 2, NY = 2 * Q;
+  static constexpr int TY = Q / 8, SY = NT / TY, RY = (NXH + SY - 1) / SY;    // row transforms
+  static constexpr int TX = NXH / 8, SX = NT / TX, CX = (Q + SX - 1) / SX;    // column transforms
+  static constexpr int NXHP = NXH + 1;
+  static constexpr int tb_bytes = align16(Q * NXHP * 8);  // one real part of the transpose
+  static constexpr int xb_bytes() {
+    const int a = align16(SY * slot_buf_len<Q>() * 8), b = align16(SX * slot_buf_len<NXH>() * 8);
     return a > b ? a : b;
   }
-  static constexpr int tw_off = tb_bytes + xb_elems() * P::XBYTES;
+  static constexpr int tw_off = tb_bytes + xb_bytes();
   static constexpr int smem = tw_off + tw_bytes<double>();
+  static_assert(TY >= 1 && TX >= 1 && TY <= 64 && TX <= 64, "unsupported plane shape");
 };
 
-// The kz = 0 plane comes from the FP64 line sums D0 of kf_zfwd.  The
-// (0, 0, 0) coefficient, the sum of rho, goes to sum_out[0] and is replaced by
-// 0: the channels are transformed without their mean, which the point kernel
-// adds back in FP64 (the k = 0 weights times the mean density).
+// The (0, 0, 0) coefficient, the sum of rho, goes to sum_out[0] and is
+// replaced by 0: the channels are transformed without their mean, which the
+// point kernel adds back in FP64 (the k = 0 weights times the mean density).
 template <typename S, int NX, int Q>
-__global__ void __launch_bounds__(kBT) kf_rho_xy(const cx<S>* __restrict__ A0, const double* __restrict__ D0,
+__global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, const double* __restrict__ D0,
                                                  cx<double>* __restrict__ XR, double* __restrict__ sum_out,
                                                  FastGeom g, const cx<double>* __restrict__ tw) {
-  using P = PlaneCfg<NX, Q>;
-  using RC = RhoCfg<NX, Q, S>;
-  constexpr int RPF = (NX + P::SBYF - 1) / P::SBYF;
+  using RC = RhoCfg<NX, Q>;
+  constexpr int NXH = RC::NXH, NXHP = RC::NXHP, TY = RC::TY, SY = RC::SY, RY = RC::RY;
+  constexpr int TX = RC::TX, SX = RC::SX, CX = RC::CX, NZT = 32;
   extern __shared__ __align__(16) unsigned char smem[];
-  cx<double>* tb = reinterpret_cast<cx<double>*>(smem);
+  double* tb = reinterpret_cast<double*>(smem);
   unsigned char* xraw = smem + RC::tb_bytes;
-  const cx<double>* twl = stage_twiddles<double, kBT>(smem + RC::tw_off, tw);
+  const cx<double>* twl = stage_twiddles<double, 512>(smem + RC::tw_off, tw);
   const int tid = threadIdx.x;
-  const int ys = tid / P::TYF, yl = tid % P::TYF, xs = tid / RC::TXR, xl = tid % RC::TXR;
+  const int ys = tid / TY, yl = tid % TY, xs = tid / TX, xl = tid % TX;
   const int unit = blockIdx.x;
-  const int r = unit & 1, kz = unit >> 1;
-  cx<double> yv[RPF][8];
+  const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+  const int mb8 = Q / 8;
+  // the lane's rows i = ys + SY rr, columns m = yl + TY q (channel-layout offsets)
+  cx<double> yv[RY][8];
 #pragma unroll
-  for (int rr = 0; rr < RPF; ++rr) {
-    const int row = ys + P::SBYF * rr;
-    if (row < NX) {
+  for (int rr = 0; rr < RY; ++rr) {
+    const int i = min(ys + SY * rr, NXH - 1);
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int m = yl + P::TYF * q;
-        yv[rr][q] = kz == 0 ? cx<double>{D0[(static_cast<long>(row) * P::MB + (m >> 4)) * 32 + r * 16 + (m & 15)], 0.0}
-                            : cvt<double>(A0[split_idx(kz, r, row, m, P::MB, g.nzh)]);
-      }
+    for (int q = 0; q < 8; ++q) {
+      const int m = yl + TY * q;
+      const long row = static_cast<long>(i) * mb8 + (m >> 3);
+      const int off = r * 16 + p * 8 + (m & 7);
+      yv[rr][q] = kz == 0 ? cx<double>{D0[row * NZT + off], 0.0}
+                          : cvt<double>(A0[(row * g.nzh + kz) * NZT + off]);
     }
   }
   __syncthreads();  // twiddles
 #pragma unroll
-  for (int rr = 0; rr < RPF; ++rr) {
-    const int row = ys + P::SBYF * rr;
-    if (row < NX) {
-      if (r) {
+  for (int rr = 0; rr < RY; ++rr) {
+    const int i = ys + SY * rr;
+    if (i < NXH) {
+      cx<double> wp = {1.0, 0.0};
+      if (p) wp = twl[NX - 4 + i];  // exp(-2 pi i i / NX)
 #pragma unroll
-        for (int q = 0; q < 8; ++q) yv[rr][q] = twiddle<double, P::NY, -1>(yv[rr][q], yl + P::TYF * q, twl);
+      for (int q = 0; q < 8; ++q) {
+        if (p) yv[rr][q] = cmul(yv[rr][q], wp);
+        if (r) yv[rr][q] = twiddle<double, RC::NY, -1>(yv[rr][q], yl + TY * q, twl);
       }
-      slot_fft<double, Q, P::TYF, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * P::XBYTES, yl, twl);
+      slot_fft<double, Q, TY, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * 8, yl, twl);
+    }
+  }
+  // transpose, one real part at a time: tb[s * NXHP + i]
+  cx<double> f[CX][8];
+#pragma unroll
+  for (int part = 0; part < 2; ++part) {
+    __syncthreads();  // row transforms done (part 0) / previous part read (part 1)
+#pragma unroll
+    for (int rr = 0; rr < RY; ++rr) {
+      const int i = ys + SY * rr;
+      if (i < NXH) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) tb[(yl + TY * e) * NXHP + i] = part ? yv[rr][e].im : yv[rr][e].re;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int cc = 0; cc < CX; ++cc) {
+      const int sc = min(xs + SX * cc, Q - 1);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double t = tb[sc * NXHP + xl + TX * q];
+        if (part) f[cc][q].im = t;
+        else f[cc][q].re = t;
+      }
     }
   }
 #pragma unroll
-  for (int c = 0; c < RC::NPART; ++c) {
-    const int s0 = c * RC::NC;
-    // transpose the lane's outputs of columns [s0, s0 + NC); tb is not the
-    // y-slot memory, but its previous batch is read until the barrier
-    if (c > 0) __syncthreads();
-#pragma unroll
-    for (int rr = 0; rr < RPF; ++rr) {
-      const int row = ys + P::SBYF * rr;
-      if (row < NX) {
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          const int sc = yl + P::TYF * e - s0;
-          if (sc >= 0 && sc < RC::NC) tb[sc * RC::NXP + row] = yv[rr][e];
-        }
+  for (int cc = 0; cc < CX; ++cc) {
+    const int sc = xs + SX * cc;
+    if (sc < Q) {
+      slot_fft<double, NXH, TX, -1, true>(f[cc], xraw + xs * slot_buf_len<NXH>() * 8, xl, twl);
+      if (kz == 0 && r == 0 && p == 0 && sc == 0 && xl == 0) {
+        sum_out[0] = f[cc][0].re;
+        f[cc][0] = {0.0, 0.0};
       }
-    }
-    __syncthreads();  // the transposed batch is complete; every y (or x) slot exchange is over
 #pragma unroll
-    for (int cc = 0; cc < RC::CPR; ++cc) {
-      const int sc = xs + RC::SXR * cc;
-      if (sc < RC::NC) {
-        cx<double> v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = tb[sc * RC::NXP + xl + RC::TXR * q];
-        slot_fft<double, NX, RC::TXR, -1, true>(v, xraw + xs * slot_buf_len<NX>() * P::XBYTES, xl, twl);
-        if (kz == 0 && r == 0 && s0 + sc == 0 && xl == 0) {
-          sum_out[0] = v[0].re;
-          v[0] = {0.0, 0.0};
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int kx = xl + RC::TXR * e;
-          XR[hp_idx(kz, r, kx & 1, s0 + sc, kx >> 1, Q, NX / 2)] = v[e];
-        }
-      }
+      for (int e = 0; e < 8; ++e) XR[hp_idx(kz, r, p, sc, xl + TX * e, Q, NXH)] = f[cc][e];
     }
   }
 }
@@ -1451,7 +1464,7 @@ void set_smem(K kernel, int bytes) {
 template <typename S>
 void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom& g, const void* tw64, AliasMap amap,
                       hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(g.nx * g.mb);
+  const unsigned nblk = static_cast<unsigned>((g.nx / 2) * (g.q / 8));
 #define CDFT_L(NZ)                                                                                     \
   {                                                                                                    \
     auto k = kf_zfwd<S, NZ>;                                                                           \
@@ -1483,13 +1496,13 @@ void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double sca
 template <typename S>
 void launch_fast_rho_xy(const cx<S>* A0, const double* D0, void* XR, double* sum_out, const FastGeom& g,
                         const void* tw64, hipStream_t st) {
-  const unsigned nblk = static_cast<unsigned>(g.nzh * 2);
+  const unsigned nblk = static_cast<unsigned>(g.nzh * 4);
 #define CDFT_L(NX, Q)                                                                                 \
   {                                                                                                   \
     auto k = kf_rho_xy<S, NX, Q>;                                                                     \
-    const int sm = RhoCfg<NX, Q, S>::smem;                                                            \
+    const int sm = RhoCfg<NX, Q>::smem;                                                               \
     set_smem(k, sm);                                                                                  \
-    hipLaunchKernelGGL(k, dim3(nblk), dim3(kBT), sm, st, A0, D0, static_cast<cx<double>*>(XR), sum_out, \
+    hipLaunchKernelGGL(k, dim3(nblk), dim3(512), sm, st, A0, D0, static_cast<cx<double>*>(XR), sum_out, \
                        g, as<cx<double>>(tw64));                                                      \
   }
   if (g.nx == 64 && g.q == 32) CDFT_L(64, 32)

@@ -1,18 +1,19 @@
 // Fused six-kernel evaluation schedule for power-of-two grids.
 //
-// Spectral data of every field lives in a "chunked split" layout.  The y axis
-// is split by parity (radix-2 decimation done inside the z-line kernels):
-// for a z-transformed field x~(ix, iy, kz) the two parts are
+// Spectral data between the z-line kernels and the plane kernels lives in a
+// chunked, parity-split layout.  The y axis is split by parity (the radix-2
+// decimation is done inside the z-line kernels): for a z-transformed field
+// x~(ix, iy, kz) the two parts are
 //   a_r(kz, ix, m) = x~(ix, m, kz) + (-1)^r x~(ix, m + Q, kz),  Q = Ny/2
-// (the r = 1 part also twiddled), and element (kz, r, ix, m) is stored at
-//   ((ix * (Q/16) + m/16) * (Nz/2+1) + kz) * 32 + r * 16 + m % 16,
-// so the 16 consecutive m of one (kz, r) share a 128-byte segment and the
-// data of 16 line pairs (m .. m+15 and m+Q .. m+Q+15 at one ix) form one
-// contiguous chunk.  The channel arrays between the plane kernels and the
-// z-line kernels split x the same way, by kx parity p: row ix = 2 i + p holds
-// half row i of part p.  A plane kernel owns one (kz, r, p) sub-plane, the
-// z-line kernels own the chunks of half rows i (lines x = i and i + Nx/2);
-// both read and write whole or half segments.
+// (the r = 1 part also twiddled), and x the same way by kx parity p (half
+// rows i and i + Nx/2 combine to p = 0 and, twiddled, to p = 1).  Element
+// (kz, r, i, p, m) is stored at
+//   ((i * (Q/8) + m/8) * (Nz/2+1) + kz) * 32 + r * 16 + p * 8 + m % 8,
+// so both parts of 8 consecutive m share a 256-byte row per kz (128 bytes per
+// r in FP32) and the data of the 32 lines at x = i, i + Nx/2 and y = m .. m+7,
+// m+Q .. m+Q+7 form one contiguous chunk.  A plane kernel owns one
+// (kz, r, p) sub-plane and reads or writes half segments; the z-line kernels
+// own chunks and read and write whole segments.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,8 +45,9 @@ void launch_fast_mf_layout(const cx<double>* spec, double* out, const Geom& g, h
 int fast_point_blocks(const FastGeom& g, bool fp64);
 int fast_bwd_blocks(const FastGeom& g);
 
-// z-R2C of rho into the split layout; D0 (nx * ny doubles) receives the FP64
-// kz = 0 plane (the line sums).
+// z-R2C of rho and its y- and x-parity split (the x twiddle left to
+// kf_rho_xy) into the channel layout; D0 (nx * ny doubles) receives the FP64
+// kz = 0 plane (the line sums) in the same row order.
 template <typename S>
 void launch_fast_zfwd(const double* rho, cx<S>* out, double* D0, const FastGeom& g, const void* tw64, AliasMap amap,
                       hipStream_t st);
