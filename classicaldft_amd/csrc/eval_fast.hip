@@ -926,6 +926,9 @@ __global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, c
   const cx<double>* twl = stage_twiddles<double, 512>(smem + RC::tw_off, tw);
   const int tid = threadIdx.x;
   const int ys = tid / TY, yl = tid % TY, xs = tid / TX, xl = tid % TX;
+  // The two parts p of one (kz, r) read the two halves of the same rows.
+  // Placing their blocks on one XCD halves the L2 misses but measured 10%
+  // slower than the round-robin placement (14.6 against 16.1 us at 128^3).
   const int unit = blockIdx.x;
   const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
   const int mb8 = Q / 8;
@@ -1104,11 +1107,14 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
   const double invR = 1.0 / g.R;
   __syncthreads();  // twiddles
   int nch = 0;
-  for (long it = i0; it < i1;) {
-    const int unit = static_cast<int>(it / NFP);
-    const int p0 = static_cast<int>(it - static_cast<long>(unit) * NFP);
-    const int p1 = static_cast<int>(min(static_cast<long>(NFP), p0 + (i1 - it)));
-    it += p1 - p0;
+  // A unit split between blocks lb and lb + 1 is the last of lb's range and
+  // the first of lb + 1's: odd blocks take their units in reverse order, so
+  // both read the shared unit's spectrum and weights at the same time (L2).
+  const int ua = static_cast<int>(i0 / NFP), ub = static_cast<int>((i1 - 1) / NFP);
+  for (int uu = ua; uu <= ub && i1 > i0; ++uu) {
+    const int unit = (lb & 1) ? ua + ub - uu : uu;
+    const long ubase = static_cast<long>(unit) * NFP;
+    const int p0 = static_cast<int>(max(i0, ubase) - ubase), p1 = static_cast<int>(min(i1, ubase + NFP) - ubase);
     const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
     int soff[P::RPH][4];  // the lane's output offsets, the same for every channel of the unit
 #pragma unroll
@@ -1239,11 +1245,12 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
   const long lb = xcd_remap(blockIdx.x, gridDim.x);  // neighbouring ranges share an XCD's L2
   const long i0 = items * lb / gridDim.x, i1 = items * (lb + 1) / gridDim.x;
   __syncthreads();  // twiddles
-  for (long it = i0; it < i1;) {
-    const int unit = static_cast<int>(it / kBwdPlaneCh);
-    const int c0 = static_cast<int>(it - static_cast<long>(unit) * kBwdPlaneCh);
-    const int c1 = static_cast<int>(min(static_cast<long>(kBwdPlaneCh), c0 + (i1 - it)));
-    it += c1 - c0;
+  // Units in order: neighbouring blocks then read the two parts p of one
+  // (kz, r), the two halves of the same channel rows, at about the same time.
+  const int ua = static_cast<int>(i0 / kBwdPlaneCh), ub = static_cast<int>((i1 - 1) / kBwdPlaneCh);
+  for (int unit = ua; unit <= ub && i1 > i0; ++unit) {
+    const long ubase = static_cast<long>(unit) * kBwdPlaneCh;
+    const int c0 = static_cast<int>(max(i0, ubase) - ubase), c1 = static_cast<int>(min(i1, ubase + kBwdPlaneCh) - ubase);
     const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
     const double kzv = kz * g.dkz;
 
