@@ -265,7 +265,12 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
       CDFT_HIP(hipMemcpy(c->tw32[d], t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
     }
 
-    if (n_species > 1) {  // mixtures: FP64 general schedule with their own buffers
+    if (n_species > 1) {  // mixtures: FP64, their own buffers; fused transforms where the grid has them
+      c->fast = options->schedule == CDFT_SCHEDULE_AUTO && fast_supported(g.nx, g.ny, g.nz, CDFT_PRECISION_FP64);
+      if (c->fast) {
+        c->fg = make_fast_geom(g);
+        fast_twiddles(&c->ftw64, &c->ftw32);
+      }
       init_mixture(c);
       return c;
     }

@@ -210,7 +210,8 @@ __host__ __device__ constexpr int zfwd_smem() {
 // chunk.  The kz = 0 coefficients (the line sums, which carry the mean density)
 // are real; they are also written in FP64 to D0 in the same row order
 // (D0[chunk * 32 + r * 16 + p * 8 + c]): kf_rho_xy takes the kz = 0 plane from
-// there, so its FP32 rounding never meets the mean.
+// there, so its FP32 rounding never meets the mean.  Field f = blockIdx.y
+// reads rho + f N and writes out + f M (D0 nullable: FP64 storage needs none).
 template <typename S, int NZ>
 __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho, cx<S>* __restrict__ out, FastGeom g,
                                                   const cx<double>* __restrict__ tw, AliasMap amap,
@@ -225,6 +226,8 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
   const long blk = blockIdx.x;
   const int i = static_cast<int>(blk / mb8), mc = static_cast<int>(blk % mb8);
   const long la = static_cast<long>(i + xh * (g.nx / 2)) * g.ny + mc * 8 + c, lb = la + g.q;
+  rho += blockIdx.y * g.N;
+  out += blockIdx.y * g.M;
   const cx<double>* twl = stage_twiddles<double, 2 * NZ>(smem + zfwd_tw_off<NZ, S>(), tw);
   cx<double> v[8];
 #pragma unroll
@@ -263,7 +266,7 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
       const cx<double> a = cb[(r * 8 + cc) * NZH + kz], b = cb[((2 + r) * 8 + cc) * NZH + kz];
       const cx<double> w = p ? csub(a, b) : cadd(a, b);
       o[j] = cvt<S>(w);
-      if (kz == 0) D0[blk * 32 + j] = w.re;
+      if (kz == 0 && D0) D0[blk * 32 + j] = w.re;
     }
   }
 }
@@ -341,7 +344,9 @@ __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __res
   const long la = static_cast<long>(i + xh * (g.nx / 2)) * g.ny + m, lb = la + g.q;
   // partial sets of the backward plane kernel at set * M, in the channel
   // layout: the block's rows hold parts p = 0, 1 of half row i, columns 8 mc ..
-  const cx<S>* p0 = P + blk * static_cast<long>(NZH * 32);
+  const cx<S>* p0 = P + blockIdx.y * g.M + blk * static_cast<long>(NZH * 32);
+  out += blockIdx.y * g.N;
+  if (add) add += blockIdx.y * g.N;
   const cx<C>* twl = stage_twiddles<C, NT>(smem + zinv_tw_off<NZ, C>(), tw);
   const int nunits = 4 * NZH;
   {
@@ -913,6 +918,8 @@ struct RhoCfg {
 // The (0, 0, 0) coefficient, the sum of rho, goes to sum_out[0] and is
 // replaced by 0: the channels are transformed without their mean, which the
 // point kernel adds back in FP64 (the k = 0 weights times the mean density).
+// With sum_out and D0 null (generic field transforms in FP64) the kz = 0 plane
+// comes from A0 and the spectrum is written whole; field f = blockIdx.y.
 template <typename S, int NX, int Q>
 __global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, const double* __restrict__ D0,
                                                  cx<double>* __restrict__ XR, double* __restrict__ sum_out,
@@ -932,6 +939,8 @@ __global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, c
   const int unit = blockIdx.x;
   const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
   const int mb8 = Q / 8;
+  A0 += blockIdx.y * g.M;  // field f = blockIdx.y
+  XR += blockIdx.y * g.M;
   // the lane's rows i = ys + SY rr, columns m = yl + TY q (channel-layout offsets)
   cx<double> yv[RY][8];
 #pragma unroll
@@ -942,8 +951,8 @@ __global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, c
       const int m = yl + TY * q;
       const long row = static_cast<long>(i) * mb8 + (m >> 3);
       const int off = r * 16 + p * 8 + (m & 7);
-      yv[rr][q] = kz == 0 ? cx<double>{D0[row * NZT + off], 0.0}
-                          : cvt<double>(A0[(row * g.nzh + kz) * NZT + off]);
+      yv[rr][q] = kz == 0 && D0 ? cx<double>{D0[row * NZT + off], 0.0}
+                                : cvt<double>(A0[(row * g.nzh + kz) * NZT + off]);
     }
   }
   __syncthreads();  // twiddles
@@ -991,7 +1000,7 @@ __global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, c
     const int sc = xs + SX * cc;
     if (sc < Q) {
       slot_fft<double, NXH, TX, -1, true>(f[cc], xraw + xs * slot_buf_len<NXH>() * 8, xl, twl);
-      if (kz == 0 && r == 0 && p == 0 && sc == 0 && xl == 0) {
+      if (sum_out && kz == 0 && r == 0 && p == 0 && sc == 0 && xl == 0) {
         sum_out[0] = f[cc][0].re;
         f[cc][0] = {0.0, 0.0};
       }
@@ -1186,6 +1195,39 @@ __global__ void __launch_bounds__(kBT) kf_plane_fwd(const cx<double>* __restrict
       }
     }
   }
+}
+
+// ---------------------------------------------------------------- kg_xy_inv
+// Inverse x- and y-transform of whole spectra (generic fields, no weights):
+// block (unit, field f = blockIdx.y) takes sub-plane `unit` of spectrum f in
+// the sub-plane order (FP64) and writes its parts in the channel layout, as
+// one kf_plane_fwd channel item does.
+template <typename S, int NX, int Q>
+__global__ void __launch_bounds__(kBT) kg_xy_inv(const cx<double>* __restrict__ spec, cx<S>* __restrict__ out,
+                                                 FastGeom g, const cx<double>* __restrict__ tw64) {
+  using P = PlaneCfg<NX, Q>;
+  using FC = FwdCfg<NX, Q, S, double>;
+  constexpr int NXH = P::NXH;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const cx<double>* tw64l = stage_twiddles<double, kBT>(smem + FC::tw_off, tw64);
+  const int tid = threadIdx.x;
+  const int xs = tid / P::TX, xl = tid % P::TX;
+  const int unit = blockIdx.x;
+  const int p = unit & 1, r = (unit >> 1) & 1, kz = unit >> 2;
+  int soff[P::RPH][4];
+#pragma unroll
+  for (int rr = 0; rr < P::RPH; ++rr)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int row = min(tid / P::TYH + P::SBYH * rr, NXH - 1), m = tid % P::TYH + P::TYH * e;
+      soff[rr][e] = static_cast<int>(chan_idx(kz, r, row, p, m, Q / 8, g.nzh));
+    }
+  const cx<double>* src = spec + blockIdx.y * g.M + static_cast<long>(unit) * Q * NXH + xl;
+  __syncthreads();  // twiddles
+  plane_fwd_channel<double, S, NX, Q, false>(
+      [&](int cc, int e) { return src[static_cast<long>(xs + P::SBX * cc) * NXH + P::TX * e]; },
+      reinterpret_cast<cx<S>*>(smem), smem + FC::xx_off, smem + FC::xy_off, out + blockIdx.y * g.M, soff, r,
+      tw64l);
 }
 
 // ---------------------------------------------------------------- kf_plane_bwd
@@ -1605,6 +1647,74 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
   if (g.nz == 64) CDFT_CH(64) else if (g.nz == 128) CDFT_CH(128) else throw std::runtime_error("fast path: nz");
 #undef CDFT_CH
 #undef CDFT_L
+}
+
+// Generic FP64 field transforms on the fused kernels (whole spectra in the
+// sub-plane order, no weights): `count` fields N apart in, spectra M apart out.
+void launch_fields_forward(const double* in, cx<double>* spec, cx<double>* tmp, int count, const FastGeom& g,
+                           const void* tw64, hipStream_t st) {
+  const AliasMap id{0, 0.0, 0.0};
+  const unsigned zb = static_cast<unsigned>((g.nx / 2) * (g.q / 8)), pb = static_cast<unsigned>(g.nzh * 4);
+  const unsigned nf = static_cast<unsigned>(count);
+#define CDFT_Z(NZ)                                                                                     \
+  {                                                                                                    \
+    auto k = kf_zfwd<double, NZ>;                                                                      \
+    const int sm = zfwd_smem<NZ, double>();                                                            \
+    set_smem(k, sm);                                                                                   \
+    hipLaunchKernelGGL(k, dim3(zb, nf), dim3(2 * NZ), sm, st, in, tmp, g, as<cx<double>>(tw64), id,    \
+                       static_cast<double*>(nullptr));                                                 \
+  }
+  if (g.nz == 64) CDFT_Z(64) else if (g.nz == 128) CDFT_Z(128) else throw std::runtime_error("fast path: nz");
+#undef CDFT_Z
+#define CDFT_L(NX, Q)                                                                                 \
+  {                                                                                                   \
+    auto k = kf_rho_xy<double, NX, Q>;                                                                \
+    const int sm = RhoCfg<NX, Q>::smem;                                                               \
+    set_smem(k, sm);                                                                                  \
+    hipLaunchKernelGGL(k, dim3(pb, nf), dim3(512), sm, st, tmp, static_cast<const double*>(nullptr), spec, \
+                       static_cast<double*>(nullptr), g, as<cx<double>>(tw64));                       \
+  }
+  if (g.nx == 64 && g.q == 32) CDFT_L(64, 32)
+  else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
+  else if (g.nx == 128 && g.q == 32) CDFT_L(128, 32)
+  else if (g.nx == 128 && g.q == 64) CDFT_L(128, 64)
+  else if (g.nx == 256 && g.q == 32) CDFT_L(256, 32)
+  else if (g.nx == 256 && g.q == 64) CDFT_L(256, 64)
+  else throw std::runtime_error("fast path: plane shape");
+#undef CDFT_L
+}
+
+void launch_fields_inverse(const cx<double>* spec, double* out, const double* add, double scale, cx<double>* tmp,
+                           int count, const FastGeom& g, const void* tw64, hipStream_t st) {
+  const unsigned pb = static_cast<unsigned>(g.nzh * 4), zb = static_cast<unsigned>((g.nx / 2) * (g.q / 8));
+  const unsigned nf = static_cast<unsigned>(count);
+#define CDFT_L(NX, Q)                                                                                 \
+  {                                                                                                   \
+    auto k = kg_xy_inv<double, NX, Q>;                                                                \
+    const int sm = FwdCfg<NX, Q, double, double>::smem;                                               \
+    set_smem(k, sm);                                                                                  \
+    hipLaunchKernelGGL(k, dim3(pb, nf), dim3(kBT), sm, st, spec, tmp, g, as<cx<double>>(tw64));       \
+  }
+  if (g.nx == 64 && g.q == 32) CDFT_L(64, 32)
+  else if (g.nx == 64 && g.q == 64) CDFT_L(64, 64)
+  else if (g.nx == 128 && g.q == 32) CDFT_L(128, 32)
+  else if (g.nx == 128 && g.q == 64) CDFT_L(128, 64)
+  else if (g.nx == 256 && g.q == 32) CDFT_L(256, 32)
+  else if (g.nx == 256 && g.q == 64) CDFT_L(256, 64)
+  else throw std::runtime_error("fast path: plane shape");
+#undef CDFT_L
+  // one part per sub-plane: with as many backward blocks as units no unit is split
+#define CDFT_Z(NZ)                                                                                      \
+  {                                                                                                     \
+    auto k = kf_zinv<double, double, NZ>;                                                               \
+    const int sm = zinv_smem<NZ, double>();                                                             \
+    set_smem(k, sm);                                                                                    \
+    hipLaunchKernelGGL(k, dim3(zb, nf), dim3(zinv_threads<NZ>()), sm, st, tmp, out, add, scale, g,      \
+                       as<cx<double>>(tw64), static_cast<int>(pb), static_cast<const double*>(nullptr), 0, \
+                       static_cast<double*>(nullptr));                                                  \
+  }
+  if (g.nz == 64) CDFT_Z(64) else if (g.nz == 128) CDFT_Z(128) else throw std::runtime_error("fast path: nz");
+#undef CDFT_Z
 }
 
 #define CDFT_INST(S, CV)                                                                                         \

@@ -76,4 +76,14 @@ template <typename S, typename CV>
 void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double scale, const FastGeom& g,
                       const void* twv, const double* partials, int fin_rows, double* res, hipStream_t st);
 
+// Generic FP64 3-D transforms of `count` real fields (N apart) on the fused
+// kernels: forward into whole spectra (M apart) in the sub-plane order
+// [kz][r][p][s][k] (kx = 2 k + p, ky = r + 2 s), unnormalised; and the inverse,
+// out = scale * c2r(spec) + add (add nullable, N apart).  tmp holds count * M
+// complex values of the channel layout.
+void launch_fields_forward(const double* in, cx<double>* spec, cx<double>* tmp, int count, const FastGeom& g,
+                           const void* tw64, hipStream_t st);
+void launch_fields_inverse(const cx<double>* spec, double* out, const double* add, double scale, cx<double>* tmp,
+                           int count, const FastGeom& g, const void* tw64, hipStream_t st);
+
 }  // namespace cdft

@@ -1,5 +1,7 @@
-// Multi-species functionals::total (functionals.hpp:64-95) on the device:
-// FP64, general schedule.
+// Multi-species functionals::total (functionals.hpp:64-95) on the device, in
+// FP64.  Its 3-D transforms run on the fused kernels where the grid has them
+// (power-of-two extents 64 and 128, x also 256; spectra in the sub-plane
+// order), on the general schedule otherwise (plane-major spectra).
 //
 // Hard spheres (hard_sphere.hpp:174-254): the 16 measures eta, n2, n1, n0,
 // n_v2[3], n_v1[3], T[6] are sums over species of each species' weighted
@@ -40,6 +42,7 @@ constexpr int kMixThreads = 256;
 
 struct MixParams {
   int S, ninter, hs;
+  int hp, nxh, q;  // spectra in the fused sub-plane order (and its half extents)
   double R[kMaxSpecies], invR[kMaxSpecies], ifr[kMaxSpecies], ifr2[kMaxSpecies];  // 1/(4 pi R), 1/(4 pi R^2)
   int si[kMaxMixInteractions], sj[kMaxMixInteractions];
 };
@@ -51,10 +54,12 @@ struct MixPoint {
   FmtParams fp;
 };
 
-// Carving of Context::d_mix.
+// Carving of Context::d_mix (T, the channel-layout staging of the fused
+// transforms, only on fused-schedule grids).
 struct MixLayout {
   int nch = 0, nblk = 0;
-  size_t R = 0, F = 0, X = 0, B = 0, local = 0, rho = 0, vext = 0, force = 0, part = 0, res = 0, total = 0;
+  size_t R = 0, F = 0, X = 0, B = 0, local = 0, rho = 0, vext = 0, force = 0, part = 0, res = 0, T = 0,
+         total = 0;
 };
 
 MixLayout mix_layout(const Context* c) {
@@ -80,6 +85,7 @@ MixLayout mix_layout(const Context* c) {
   L.force = take(8 * S * N);
   L.part = take(8 * 8 * static_cast<size_t>(L.nblk));
   L.res = take(8 * 8);
+  if (c->fast) L.T = take(16 * std::max<size_t>(static_cast<size_t>(L.nch), S) * M);
   L.total = off;
   return L;
 }
@@ -95,7 +101,19 @@ __device__ __forceinline__ cx<double> cmul(cx<double> a, cx<double> b) {
 __device__ __forceinline__ cx<double> cscale(cx<double> a, double s) { return {a.re * s, a.im * s}; }
 __device__ __forceinline__ cx<double> cadd(cx<double> a, cx<double> b) { return {a.re + b.re, a.im + b.im}; }
 
-__device__ __forceinline__ void kvec3(long i, const Geom& g, double* kx, double* ky, double* kz) {
+// k-vector of spectral element i: plane-major [kz][kx][ky], or the fused
+// sub-plane order [kz][r][p][s][k] (kx = 2 k + p, ky = r + 2 s)
+__device__ __forceinline__ void kvec3(long i, const Geom& g, const MixParams& p, double* kx, double* ky, double* kz) {
+  if (p.hp) {
+    const int k = static_cast<int>(i % p.nxh);
+    const long t = i / p.nxh;
+    const int s = static_cast<int>(t % p.q);
+    const long u = t / p.q;
+    *kx = kvec(2 * k + static_cast<int>(u & 1), g.nx, g.dkx);
+    *ky = kvec(static_cast<int>((u >> 1) & 1) + 2 * s, g.ny, g.dky);
+    *kz = g.dkz * static_cast<double>(u >> 2);
+    return;
+  }
   const long nxy = static_cast<long>(g.nx) * g.ny;
   const long z = i / nxy, r = i - z * nxy;
   const int x = static_cast<int>(r / g.ny), y = static_cast<int>(r - static_cast<long>(x) * g.ny);
@@ -117,7 +135,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __res
     int ch = 0;
     if (p.hs) {
       double kx, ky, kz;
-      kvec3(i, g, &kx, &ky, &kz);
+      kvec3(i, g, p, &kx, &ky, &kz);
       const double k2 = kx * kx + ky * ky + kz * kz;
       const double k = sqrt(k2);
       cx<double> acc[kMixFmtT];
@@ -218,7 +236,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_bwd(const cx<double>* __res
   for (long i = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; i < M;
        i += static_cast<long>(gridDim.x) * kMixThreads) {
     double kx, ky, kz;
-    kvec3(i, g, &kx, &ky, &kz);
+    kvec3(i, g, p, &kx, &ky, &kz);
     const double k2 = kx * kx + ky * ky + kz * kz;
     const double k = sqrt(k2);
     cx<double> d[kMixFmtT];  // the T_zz derivative is folded into n2, T_xx and T_yy
@@ -288,12 +306,19 @@ MixParams mix_params(const Context* c) {
     p.si[q] = c->inters[q].si;
     p.sj[q] = c->inters[q].sj;
   }
+  p.hp = c->fast ? 1 : 0;
+  p.nxh = c->g.nx / 2;
+  p.q = c->g.ny / 2;
   return p;
 }
 
 // Forward transform of `count` real fields (N apart) into spectra (M apart).
 void r2c_fields(Context* c, const double* in, cx<double>* out, int count) {
   const Geom& g = c->g;
+  if (c->fast) {
+    launch_fields_forward(in, out, at<cx<double>>(c, mix_layout(c).T), count, c->fg, c->ftw64, c->stream);
+    return;
+  }
   launch_zr2c_fields<double, double>(in, count, out, g, c->plan[2], c->tw<double>(2), c->stream);
   launch_lines<double, double>(out, static_cast<long>(g.nzh) * g.nx * count, c->plan[1], c->tw<double>(1), -1,
                                c->stream);
@@ -306,6 +331,21 @@ void inverse_xy(Context* c, cx<double>* sp, int count) {
   launch_xlines<double, double>(sp, count, g, c->plan[0], c->tw<double>(0), +1, c->stream);
   launch_lines<double, double>(sp, static_cast<long>(g.nzh) * g.nx * count, c->plan[1], c->tw<double>(1), +1,
                                c->stream);
+}
+
+// Inverse transform of `count` spectra (M apart) into real fields (N apart):
+// out = scale * c2r(spec) + add (add nullable).  The general path transforms
+// the spectra in place.
+void c2r_fields(Context* c, cx<double>* spec, double* out, const double* add, double scale, int count) {
+  const Geom& g = c->g;
+  if (c->fast) {
+    launch_fields_inverse(spec, out, add, scale, at<cx<double>>(c, mix_layout(c).T), count, c->fg, c->ftw64,
+                          c->stream);
+    return;
+  }
+  inverse_xy(c, spec, count);
+  launch_zc2r<double, double>(spec, count, out, add, scale, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
+                              c->stream);
 }
 
 // One evaluation on device arrays (S * N each; vext and force nullable);
@@ -327,9 +367,7 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
   if (nfm + nu > 0) {
     r2c_fields(c, rho, R, S);
     k_mix_fwd<<<spectral_blocks(g.M), kMixThreads, 0, c->stream>>>(R, c->d_mix_mf, F, g, p);
-    inverse_xy(c, F, nfm + nu);
-    launch_zc2r<double, double>(F, nfm + nu, X, nullptr, 1.0, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
-                                c->stream);
+    c2r_fields(c, F, X, nullptr, 1.0, nfm + nu);
   }
   MixPoint mp{};
   mp.S = S;
@@ -344,9 +382,7 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
     if (nfm > 0) {
       r2c_fields(c, X, F, kMixFmtT);
       k_mix_bwd<<<spectral_blocks(g.M), kMixThreads, 0, c->stream>>>(F, B, g, p);
-      inverse_xy(c, B, S);
-      launch_zc2r<double, double>(B, S, force, local, g.dv, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
-                                  c->stream);
+      c2r_fields(c, B, force, local, g.dv, S);
     } else {
       CDFT_HIP(hipMemcpyAsync(force, local, sizeof(double) * S * g.N, hipMemcpyDeviceToDevice, c->stream));
     }
@@ -550,7 +586,10 @@ void init_mixture(Context* c) {
   auto* spec = at<cx<double>>(c, L.B);
   for (int q = 0; q < ni; ++q) {
     c->a_vdw.push_back(mf_weight_spectrum(c, c->inters[q], spec));
-    store_real_spectrum(spec, c->d_mix_mf + static_cast<long>(q) * g.M, g.M, c->stream);
+    if (c->fast)  // the fused transforms' sub-plane order
+      launch_fast_mf_layout(spec, c->d_mix_mf + static_cast<long>(q) * g.M, g, c->stream);
+    else
+      store_real_spectrum(spec, c->d_mix_mf + static_cast<long>(q) * g.M, g.M, c->stream);
   }
   CDFT_HIP(hipStreamSynchronize(c->stream));
 }

@@ -249,15 +249,59 @@ def test_bulk_batch_on_the_device_matches_the_host_calls(model, diams):
 
 
 def test_binary_mixture_at_64cubed_matches_oracle():
-    # a grid the fused one-species kernels would take: the mixture still runs the general transforms
+    # a grid the fused kernels cover: the mixture's transforms run on them (spectra in the sub-plane order)
     shape, dx = (64, 64, 64), 0.1
     inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=1, sigma=0.8, epsilon=0.6),
              dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
     ctx, orc = _pair(shape, (1.0, 0.8), inter, "WhiteBearII", kT=1.0, dx=dx)
-    assert ctx.schedule == "general"
+    assert ctx.schedule == "fused"
     rho = np.stack([W.droplet(shape, dx, radius=1.5, width=0.4, rho_l=0.5, rho_v=0.03, noise=1e-3, seed=61),
                     W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.2, rho_v=0.05, noise=1e-3, seed=62)])
     _compare(ctx, orc, rho, (-1.5, -1.2), tag="binary 64^3")
+
+
+@pytest.mark.parametrize("model", ["Rosenfeld", "RSLT", "WhiteBearI", "esFMT"])
+def test_fused_binary_mixture_models_match_oracle(model):
+    shape, dx = (64, 64, 64), 0.12
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
+    ctx, orc = _pair(shape, (1.0, 0.8), inter, model, kT=1.1, dx=dx, A=1.2, B=-0.7)
+    assert ctx.schedule == "fused"
+    rho = _densities(shape, dx, (0.3, 0.2), seed=81)
+    _compare(ctx, orc, rho, (0.4, -0.3), tag=f"fused binary {model}")
+
+
+@pytest.mark.parametrize("shape", [(64, 64, 128), (256, 64, 64), (128, 128, 64)])
+def test_fused_ternary_mixture_with_external_field_matches_oracle(shape):
+    # every plane shape class of the fused field transforms: x 64 / 128 / 256, y 64 / 128, z 64 / 128
+    dx = 0.15
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=2, sigma=0.9, epsilon=0.7)]
+    ctx, orc = _pair(shape, (1.0, 0.7, 1.3), inter, "WhiteBearII", kT=0.9, dx=dx)
+    assert ctx.schedule == "fused"
+    rho = _densities(shape, dx, (0.2, 0.15, 0.1), seed=91)
+    z = (np.arange(shape[2]) + 0.5) * dx
+    vext = np.stack([np.broadcast_to(a * np.cos(2 * np.pi * z / (shape[2] * dx)), shape)
+                     for a in (0.3, -0.2, 0.1)])
+    _compare(ctx, orc, rho, (0.1, -0.2, 0.3), vext=vext, tag=f"fused ternary {shape}")
+
+
+def test_fused_mixture_matches_its_general_schedule():
+    # the same mixture on the fused and on the general transforms: equal to rounding
+    import classicaldft_amd as cd
+
+    shape, dx = (128, 64, 64), 0.1
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=1, sigma=0.8, epsilon=0.6),
+             dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
+    kw = dict(diameters=(1.0, 0.8), interactions=inter, kT=1.0, model="WhiteBearII")
+    fused = cd.Context(shape, dx, **kw)
+    general = cd.Context(shape, dx, schedule="general", **kw)
+    assert fused.schedule == "fused" and general.schedule == "general"
+    rho = _densities(shape, dx, (0.3, 0.2), seed=101)
+    rf, ff = fused.total_species(rho, (0.2, -0.1))
+    rg, fg = general.total_species(rho, (0.2, -0.1))
+    for k in PARTS:
+        assert rf[k] == pytest.approx(rg[k], rel=1e-12, abs=1e-14), k
+    for s in range(2):
+        assert np.max(np.abs(ff[s] - fg[s])) / np.max(np.abs(fg[s])) < 1e-12
 
 
 def test_autotuned_mixture_still_matches_oracle():
