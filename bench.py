@@ -314,27 +314,44 @@ def converged_profile(cd, W, rho0, shape, dx, kT, tol=1e-8, cap=60000, dt_max=0.
 def mixture(cd, W, shape=(128, 128, 128), evals=50, autotune=True):
     """Binary LJ mixture (diameters 1.0 and 0.8; self pairs LJ(1, 1) and LJ(0.8, 0.6), cross pair
     LJ(0.9, 0.8), WCA, r_c = 2.5; kT = 1.0, WB-II, dx = 0.1) on smooth noisy densities of mean 0.3 and 0.2:
-    FP64 evaluations per second with forces after autotuning the context, timed with HIP events on the
-    context stream."""
+    evaluations per second with forces, FP64 (after autotuning the context; a fused-schedule context is
+    only timed) and mixed precision (FP32 real-space fields), timed with HIP events on the context
+    stream; the mixed result is compared with the FP64 one."""
     import numpy as np
     import torch
 
     dx = 0.1
     inter = [dict(W.LJ_WCA, si=0, sj=0), dict(W.LJ_WCA, si=1, sj=1, sigma=0.8, epsilon=0.6),
              dict(W.LJ_WCA, si=0, sj=1, sigma=0.9, epsilon=0.8)]
-    ctx = cd.Context(shape, dx, diameters=(1.0, 0.8), interactions=inter, kT=1.0, model="WhiteBearII")
-    us_default, us_tuned = ctx.autotune(reps=5) if autotune else (None, None)
+    kw = dict(diameters=(1.0, 0.8), interactions=inter, kT=1.0, model="WhiteBearII")
     rho = np.stack([W.smooth_noise_fluid(shape, dx, rho0=0.3, amp=0.02, seed=1),
                     W.smooth_noise_fluid(shape, dx, rho0=0.2, amp=0.02, seed=2)])
     d_rho = torch.tensor(rho, device="cuda:0").contiguous()
-    d_f = torch.empty_like(d_rho)
     mu = (0.1, 0.2)
-    res = ctx.total_species_device(d_rho.data_ptr(), mu, d_forces=d_f.data_ptr())  # warm-up
-    ms = ctx.time_total_species_device(d_rho.data_ptr(), mu, evals, d_forces=d_f.data_ptr())
-    return {"grid": list(shape), "species": 2, "interactions": 3, "precision": "fp64", "schedule": ctx.schedule,
-            "evals": evals, "us_per_eval": 1000.0 * ms / evals, "evals_per_s": 1000.0 * evals / ms,
-            "autotune_us_default_tuned": [us_default, us_tuned],
-            "grand_potential": res["grand_potential"]}
+    out = {"grid": list(shape), "species": 2, "interactions": 3}
+    forces = {}
+    for prec in ("fp64", "mixed"):
+        try:
+            ctx = cd.Context(shape, dx, precision=prec, **kw)
+        except cd.CdftError:  # mixed precision needs a fused-schedule grid
+            continue
+        tune = ctx.autotune(reps=5) if autotune else (None, None)
+        d_f = torch.empty_like(d_rho)
+        res = ctx.total_species_device(d_rho.data_ptr(), mu, d_forces=d_f.data_ptr())  # warm-up
+        ms = ctx.time_total_species_device(d_rho.data_ptr(), mu, evals, d_forces=d_f.data_ptr())
+        forces[prec] = d_f.cpu().numpy()
+        key = "" if prec == "fp64" else "mixed_"
+        out.update({f"{key}precision": prec, f"{key}schedule": ctx.schedule, f"{key}evals": evals,
+                    f"{key}us_per_eval": 1000.0 * ms / evals, f"{key}evals_per_s": 1000.0 * evals / ms,
+                    f"{key}grand_potential": res["grand_potential"]})
+        if prec == "fp64":
+            out["autotune_us_default_tuned"] = list(tune)
+    if "mixed" in forces:
+        out["mixed_omega_rel_err_vs_fp64"] = abs(out["mixed_grand_potential"] - out["grand_potential"]) / abs(
+            out["grand_potential"])
+        out["mixed_force_rel_err_vs_fp64"] = float(np.max(np.abs(forces["mixed"] - forces["fp64"])) /
+                                                   np.max(np.abs(forces["fp64"])))
+    return out
 
 
 def main():

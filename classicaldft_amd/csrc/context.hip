@@ -179,8 +179,12 @@ Context* create_context(const cdft_grid* grid, const double* diameters, int n_sp
     throw Unsupported("at most " + std::to_string(kMaxMixInteractions) + " mean-field interactions");
   if (options->precision != CDFT_PRECISION_FP64 && options->precision != CDFT_PRECISION_MIXED)
     throw std::invalid_argument("unknown precision mode");
-  if (n_species > 1 && options->precision != CDFT_PRECISION_FP64)
-    throw Unsupported("mixtures are evaluated in FP64 only");
+  if (n_species > 1 && options->precision == CDFT_PRECISION_MIXED &&
+      (options->schedule != CDFT_SCHEDULE_AUTO ||
+       !fast_supported(static_cast<int>(shape[0]), static_cast<int>(shape[1]), static_cast<int>(shape[2]),
+                       CDFT_PRECISION_FP64)))
+    throw Unsupported("mixed precision for a mixture needs a grid the fused transforms cover (y and z extents 64 "
+                      "or 128, x extent 64, 128 or 256)");
   if (options->schedule != CDFT_SCHEDULE_AUTO && options->schedule != CDFT_SCHEDULE_GENERAL)
     throw std::invalid_argument("unknown schedule");
   const int model = options->fmt_model;

@@ -212,8 +212,9 @@ __host__ __device__ constexpr int zfwd_smem() {
 // (D0[chunk * 32 + r * 16 + p * 8 + c]): kf_rho_xy takes the kz = 0 plane from
 // there, so its FP32 rounding never meets the mean.  Field f = blockIdx.y
 // reads rho + f N and writes out + f M (D0 nullable: FP64 storage needs none).
-template <typename S, int NZ>
-__global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho, cx<S>* __restrict__ out, FastGeom g,
+// The input fields are FP64, or FP32 (I = float).
+template <typename S, int NZ, typename I = double>
+__global__ void __launch_bounds__(2 * NZ) kf_zfwd(const I* __restrict__ rho, cx<S>* __restrict__ out, FastGeom g,
                                                   const cx<double>* __restrict__ tw, AliasMap amap,
                                                   double* __restrict__ D0) {
   constexpr int T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
@@ -229,13 +230,17 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const double* __restrict__ rho
   rho += blockIdx.y * g.N;
   out += blockIdx.y * g.M;
   const cx<double>* twl = stage_twiddles<double, 2 * NZ>(smem + zfwd_tw_off<NZ, S>(), tw);
-  cx<double> v[8];
+  // every load is issued before the first use, so their latencies overlap
+  I ra[8], rb[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
-    const int z = lane + T * q;
-    v[q] = {amap.apply(rho[la * NZ + z]), amap.apply(rho[lb * NZ + z])};
+    ra[q] = rho[la * NZ + lane + T * q];
+    rb[q] = rho[lb * NZ + lane + T * q];
   }
   __syncthreads();
+  cx<double> v[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) v[q] = {amap.apply(static_cast<double>(ra[q])), amap.apply(static_cast<double>(rb[q]))};
   cx<double>* xb = xbuf + slot * XS;
   slot_fft<double, NZ, T, -1>(v, xb, lane, twl);
   wave_sync();
@@ -320,8 +325,8 @@ __host__ __device__ inline bool bwd_unit_split(int unit, int nunits, int nblk);
 // With fin_rows > 0 the grid has one block more, which does the final
 // reduction of the point kernel's partial sums (reduce_rows) while the other
 // blocks transform: the evaluation then needs no separate finalisation launch.
-template <typename S, typename C, int NZ>
-__global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __restrict__ P, double* __restrict__ out,
+template <typename S, typename C, int NZ, typename O = double>
+__global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __restrict__ P, O* __restrict__ out,
                                                                const double* __restrict__ add, double scale,
                                                                FastGeom g, const cx<C>* __restrict__ tw,
                                                                int bwd_blocks, const double* __restrict__ partials,
@@ -394,8 +399,8 @@ __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __res
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int z = lane + T * e;
-    out[la * NZ + z] = static_cast<double>(v[e].re) * scale + ad[0][e];
-    out[lb * NZ + z] = static_cast<double>(v[e].im) * scale + ad[1][e];
+    out[la * NZ + z] = static_cast<O>(static_cast<double>(v[e].re) * scale + ad[0][e]);
+    out[lb * NZ + z] = static_cast<O>(static_cast<double>(v[e].im) * scale + ad[1][e]);
   }
 }
 
@@ -1651,14 +1656,15 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
 
 // Generic FP64 field transforms on the fused kernels (whole spectra in the
 // sub-plane order, no weights): `count` fields N apart in, spectra M apart out.
-void launch_fields_forward(const double* in, cx<double>* spec, cx<double>* tmp, int count, const FastGeom& g,
+template <typename I>
+void launch_fields_forward(const I* in, cx<double>* spec, cx<double>* tmp, int count, const FastGeom& g,
                            const void* tw64, hipStream_t st) {
   const AliasMap id{0, 0.0, 0.0};
   const unsigned zb = static_cast<unsigned>((g.nx / 2) * (g.q / 8)), pb = static_cast<unsigned>(g.nzh * 4);
   const unsigned nf = static_cast<unsigned>(count);
 #define CDFT_Z(NZ)                                                                                     \
   {                                                                                                    \
-    auto k = kf_zfwd<double, NZ>;                                                                      \
+    auto k = kf_zfwd<double, NZ, I>;                                                                   \
     const int sm = zfwd_smem<NZ, double>();                                                            \
     set_smem(k, sm);                                                                                   \
     hipLaunchKernelGGL(k, dim3(zb, nf), dim3(2 * NZ), sm, st, in, tmp, g, as<cx<double>>(tw64), id,    \
@@ -1684,7 +1690,8 @@ void launch_fields_forward(const double* in, cx<double>* spec, cx<double>* tmp, 
 #undef CDFT_L
 }
 
-void launch_fields_inverse(const cx<double>* spec, double* out, const double* add, double scale, cx<double>* tmp,
+template <typename O>
+void launch_fields_inverse(const cx<double>* spec, O* out, const double* add, double scale, cx<double>* tmp,
                            int count, const FastGeom& g, const void* tw64, hipStream_t st) {
   const unsigned pb = static_cast<unsigned>(g.nzh * 4), zb = static_cast<unsigned>((g.nx / 2) * (g.q / 8));
   const unsigned nf = static_cast<unsigned>(count);
@@ -1706,7 +1713,7 @@ void launch_fields_inverse(const cx<double>* spec, double* out, const double* ad
   // one part per sub-plane: with as many backward blocks as units no unit is split
 #define CDFT_Z(NZ)                                                                                      \
   {                                                                                                     \
-    auto k = kf_zinv<double, double, NZ>;                                                               \
+    auto k = kf_zinv<double, double, NZ, O>;                                                            \
     const int sm = zinv_smem<NZ, double>();                                                             \
     set_smem(k, sm);                                                                                    \
     hipLaunchKernelGGL(k, dim3(zb, nf), dim3(zinv_threads<NZ>()), sm, st, tmp, out, add, scale, g,      \
@@ -1716,6 +1723,15 @@ void launch_fields_inverse(const cx<double>* spec, double* out, const double* ad
   if (g.nz == 64) CDFT_Z(64) else if (g.nz == 128) CDFT_Z(128) else throw std::runtime_error("fast path: nz");
 #undef CDFT_Z
 }
+
+template void launch_fields_forward<double>(const double*, cx<double>*, cx<double>*, int, const FastGeom&,
+                                            const void*, hipStream_t);
+template void launch_fields_forward<float>(const float*, cx<double>*, cx<double>*, int, const FastGeom&, const void*,
+                                           hipStream_t);
+template void launch_fields_inverse<double>(const cx<double>*, double*, const double*, double, cx<double>*, int,
+                                            const FastGeom&, const void*, hipStream_t);
+template void launch_fields_inverse<float>(const cx<double>*, float*, const double*, double, cx<double>*, int,
+                                           const FastGeom&, const void*, hipStream_t);
 
 #define CDFT_INST(S, CV)                                                                                         \
   template void launch_fast_zfwd<S>(const double*, cx<S>*, double*, const FastGeom&, const void*, AliasMap,      \

@@ -80,10 +80,13 @@ void launch_fast_zinv(const cx<S>* P, double* out, const double* add, double sca
 // kernels: forward into whole spectra (M apart) in the sub-plane order
 // [kz][r][p][s][k] (kx = 2 k + p, ky = r + 2 s), unnormalised; and the inverse,
 // out = scale * c2r(spec) + add (add nullable, N apart).  tmp holds count * M
-// complex values of the channel layout.
-void launch_fields_forward(const double* in, cx<double>* spec, cx<double>* tmp, int count, const FastGeom& g,
+// complex values of the channel layout.  The real fields are FP64 or FP32
+// (stored in that type; the arithmetic is FP64 throughout).
+template <typename I>
+void launch_fields_forward(const I* in, cx<double>* spec, cx<double>* tmp, int count, const FastGeom& g,
                            const void* tw64, hipStream_t st);
-void launch_fields_inverse(const cx<double>* spec, double* out, const double* add, double scale, cx<double>* tmp,
+template <typename O>
+void launch_fields_inverse(const cx<double>* spec, O* out, const double* add, double scale, cx<double>* tmp,
                            int count, const FastGeom& g, const void* tw64, hipStream_t st);
 
 }  // namespace cdft

@@ -1,7 +1,10 @@
 // Multi-species functionals::total (functionals.hpp:64-95) on the device, in
 // FP64.  Its 3-D transforms run on the fused kernels where the grid has them
 // (power-of-two extents 64 and 128, x also 256; spectra in the sub-plane
-// order), on the general schedule otherwise (plane-major spectra).
+// order), on the general schedule otherwise (plane-major spectra).  In mixed
+// precision (fused grids only) the real-space measure, derivative and
+// mean-field fields are stored in FP32; every transform, spectrum, pointwise
+// term and sum stays FP64.
 //
 // Hard spheres (hard_sphere.hpp:174-254): the 16 measures eta, n2, n1, n0,
 // n_v2[3], n_v1[3], T[6] are sums over species of each species' weighted
@@ -24,6 +27,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
+#include <type_traits>
 #include <vector>
 
 #include "api_guard.hpp"
@@ -77,7 +82,7 @@ MixLayout mix_layout(const Context* c) {
   };
   L.R = take(16 * S * M);
   L.F = take(16 * static_cast<size_t>(L.nch) * M);
-  L.X = take(8 * static_cast<size_t>(L.nch) * N);
+  L.X = take((c->precision == CDFT_PRECISION_MIXED ? 4 : 8) * static_cast<size_t>(L.nch) * N);
   L.B = take(16 * S * M);
   L.local = take(8 * S * N);
   L.rho = take(8 * S * N);
@@ -184,10 +189,12 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_fwd(const cx<double>* __res
   }
 }
 
-// Pointwise: Phi and its 16 partials (written over the measure fields), the
-// local force of every species and the block sums
-// [F_id, F_ext, F_hs, F_mf, sum mu rho, sum rho] (each still without dV).
-__global__ __launch_bounds__(kMixThreads) void k_mix_point(double* __restrict__ X, const double* __restrict__ rho,
+// Pointwise: Phi and its 16 partials (written over the measure fields, which
+// are FP64 or FP32 (XT); the arithmetic is FP64), the local force of every
+// species and the block sums [F_id, F_ext, F_hs, F_mf, sum mu rho, sum rho]
+// (each still without dV).
+template <typename XT>
+__global__ __launch_bounds__(kMixThreads) void k_mix_point(XT* __restrict__ X, const double* __restrict__ rho,
                                                            const double* __restrict__ vext,
                                                            double* __restrict__ local, double* __restrict__ partials,
                                                            long N, MixPoint mp) {
@@ -197,14 +204,14 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_point(double* __restrict__ 
     if (mp.hs) {
       double m[kMixFmt], d[kMixFmt];
 #pragma unroll
-      for (int c = 0; c < kMixFmtT; ++c) m[c] = X[c * N + p];
+      for (int c = 0; c < kMixFmtT; ++c) m[c] = static_cast<double>(X[c * N + p]);
       m[15] = m[1] - m[10] - m[13];  // T_zz
       sums[2] += fmt_phi_grad_mix_t<double>(mp.fp, m, d);
       d[1] += d[15];  // adjoint of the trace identity
       d[10] -= d[15];
       d[13] -= d[15];
 #pragma unroll
-      for (int c = 0; c < kMixFmtT; ++c) X[c * N + p] = d[c];
+      for (int c = 0; c < kMixFmtT; ++c) X[c * N + p] = static_cast<XT>(d[c]);
     }
     for (int s = 0; s < mp.S; ++s) {
       const double r = rho[s * N + p];
@@ -219,7 +226,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_point(double* __restrict__ 
         sums[1] += r * v;
       }
       if (mp.nu) {
-        const double u = X[(mp.nfm + s) * N + p];
+        const double u = static_cast<double>(X[(mp.nfm + s) * N + p]);
         f += u;
         sums[3] += 0.5 * r * u;
       }
@@ -266,9 +273,10 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_bwd(const cx<double>* __res
 // ideal (ideal_gas.hpp:29-33), external (external_field.hpp:24-28) and
 // mean-field (conv dV, mean_field.hpp:259-277) forces are pointwise; the
 // hard-sphere force is the total force less those three.
+template <typename CT>
 __global__ __launch_bounds__(kMixThreads) void k_split_terms(const double* __restrict__ rho,
                                                              const double* __restrict__ vext,
-                                                             const double* __restrict__ conv,
+                                                             const CT* __restrict__ conv,
                                                              const double* __restrict__ total, double mu, double dv,
                                                              long n, int hs, double* __restrict__ f_id,
                                                              double* __restrict__ f_ext, double* __restrict__ f_mf,
@@ -277,7 +285,7 @@ __global__ __launch_bounds__(kMixThreads) void k_split_terms(const double* __res
        p += static_cast<long>(gridDim.x) * kMixThreads) {
     const double id = (log(fmax(rho[p], 1e-300)) - mu) * dv;
     const double ext = vext ? vext[p] * dv : 0.0;
-    const double mf = conv ? conv[p] * dv : 0.0;
+    const double mf = conv ? static_cast<double>(conv[p]) * dv : 0.0;
     if (f_id) f_id[p] = id;
     if (f_ext) f_ext[p] = ext;
     if (f_mf) f_mf[p] = mf;
@@ -312,14 +320,17 @@ MixParams mix_params(const Context* c) {
   return p;
 }
 
-// Forward transform of `count` real fields (N apart) into spectra (M apart).
-void r2c_fields(Context* c, const double* in, cx<double>* out, int count) {
+// Forward transform of `count` real fields (N apart) into spectra (M apart);
+// FP32 fields (I = float) only on the fused path.
+template <typename I>
+void r2c_fields(Context* c, const I* in, cx<double>* out, int count) {
   const Geom& g = c->g;
   if (c->fast) {
     launch_fields_forward(in, out, at<cx<double>>(c, mix_layout(c).T), count, c->fg, c->ftw64, c->stream);
     return;
   }
-  launch_zr2c_fields<double, double>(in, count, out, g, c->plan[2], c->tw<double>(2), c->stream);
+  if constexpr (!std::is_same<I, double>::value) throw std::logic_error("FP32 fields need the fused transforms");
+  else launch_zr2c_fields<double, double>(in, count, out, g, c->plan[2], c->tw<double>(2), c->stream);
   launch_lines<double, double>(out, static_cast<long>(g.nzh) * g.nx * count, c->plan[1], c->tw<double>(1), -1,
                                c->stream);
   launch_xlines<double, double>(out, count, g, c->plan[0], c->tw<double>(0), -1, c->stream);
@@ -335,22 +346,28 @@ void inverse_xy(Context* c, cx<double>* sp, int count) {
 
 // Inverse transform of `count` spectra (M apart) into real fields (N apart):
 // out = scale * c2r(spec) + add (add nullable).  The general path transforms
-// the spectra in place.
-void c2r_fields(Context* c, cx<double>* spec, double* out, const double* add, double scale, int count) {
+// the spectra in place; FP32 output (O = float) only on the fused path.
+template <typename O>
+void c2r_fields(Context* c, cx<double>* spec, O* out, const double* add, double scale, int count) {
   const Geom& g = c->g;
   if (c->fast) {
     launch_fields_inverse(spec, out, add, scale, at<cx<double>>(c, mix_layout(c).T), count, c->fg, c->ftw64,
                           c->stream);
     return;
   }
-  inverse_xy(c, spec, count);
-  launch_zc2r<double, double>(spec, count, out, add, scale, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
-                              c->stream);
+  if constexpr (!std::is_same<O, double>::value) {
+    throw std::logic_error("FP32 fields need the fused transforms");
+  } else {
+    inverse_xy(c, spec, count);
+    launch_zc2r<double, double>(spec, count, out, add, scale, g, c->plan[2], c->tw<double>(2), nullptr, DcTable{},
+                                c->stream);
+  }
 }
 
 // One evaluation on device arrays (S * N each; vext and force nullable);
-// the sums land in the layout's res block.
-void mixture_eval(Context* c, const double* rho, const double* vext, const double* mu, double* force) {
+// the sums land in the layout's res block.  XT: the real-space field type.
+template <typename XT>
+void mixture_eval_t(Context* c, const double* rho, const double* vext, const double* mu, double* force) {
   const Geom& g = c->g;
   const MixLayout L = mix_layout(c);
   const MixParams p = mix_params(c);
@@ -358,7 +375,7 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
   const int nfm = c->cs.hs ? kMixFmtT : 0, nu = c->inters.empty() ? 0 : S;
   auto* R = at<cx<double>>(c, L.R);
   auto* F = at<cx<double>>(c, L.F);
-  auto* X = at<double>(c, L.X);
+  auto* X = at<XT>(c, L.X);
   auto* B = at<cx<double>>(c, L.B);
   auto* local = at<double>(c, L.local);
   auto* part = at<double>(c, L.part);
@@ -377,7 +394,7 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
   for (int s = 0; s < S; ++s) mp.mu[s] = mu[s];
   mp.dv = g.dv;
   mp.fp = c->fp;
-  k_mix_point<<<L.nblk, kMixThreads, 0, c->stream>>>(X, rho, vext, local, part, g.N, mp);
+  k_mix_point<XT><<<L.nblk, kMixThreads, 0, c->stream>>>(X, rho, vext, local, part, g.N, mp);
   if (force) {
     if (nfm > 0) {
       r2c_fields(c, X, F, kMixFmtT);
@@ -389,6 +406,11 @@ void mixture_eval(Context* c, const double* rho, const double* vext, const doubl
   }
   launch_finalize(part, L.nblk, res, c->stream);
   CDFT_HIP(hipGetLastError());
+}
+
+void mixture_eval(Context* c, const double* rho, const double* vext, const double* mu, double* force) {
+  if (c->precision == CDFT_PRECISION_MIXED) mixture_eval_t<float>(c, rho, vext, mu, force);
+  else mixture_eval_t<double>(c, rho, vext, mu, force);
 }
 
 void mixture_result(Context* c, cdft_result* out) {
@@ -680,6 +702,7 @@ int cdft_contributions(cdft_ctx* ctx, const double* rho, const double* vext, con
       CDFT_HIP(hipMemcpyAsync(d_rho, rho, field * S, hipMemcpyHostToDevice, c->stream));
       if (vext) CDFT_HIP(hipMemcpyAsync(d_vext, vext, field * S, hipMemcpyHostToDevice, c->stream));
       const double* conv[kMaxSpecies] = {};
+      const float* conv32[kMaxSpecies] = {};  // a mixed-precision mixture's FP32 potentials
       if (one) {  // (no cached graph: this call's buffers are its own)
         c->eval_direct(d_rho, d_vext, mu[0], d_tot, AliasMap{0, 0.0, 0.0});
         c->fetch_result(mu[0], result);
@@ -693,7 +716,12 @@ int cdft_contributions(cdft_ctx* ctx, const double* rho, const double* vext, con
         if (!c->inters.empty()) {  // the per-species potentials U_s stay in the channel fields
           const MixLayout L = mix_layout(c);
           const int nfm = c->cs.hs ? kMixFmtT : 0;
-          for (int s = 0; s < S; ++s) conv[s] = at<double>(c, L.X) + static_cast<long>(nfm + s) * N;
+          for (int s = 0; s < S; ++s) {
+            if (c->precision == CDFT_PRECISION_MIXED)
+              conv32[s] = at<float>(c, L.X) + static_cast<long>(nfm + s) * N;
+            else
+              conv[s] = at<double>(c, L.X) + static_cast<long>(nfm + s) * N;
+          }
         }
       }
       const unsigned nb = static_cast<unsigned>(std::min<long>(2048, (N + kMixThreads - 1) / kMixThreads));
@@ -701,9 +729,14 @@ int cdft_contributions(cdft_ctx* ctx, const double* rho, const double* vext, con
         const long o = static_cast<long>(s) * N;
         double* dst[4];
         for (int t = 0; t < 4; ++t) dst[t] = outs[t] ? d_out + (static_cast<long>(t) * S * N + o) : nullptr;
-        k_split_terms<<<nb, kMixThreads, 0, c->stream>>>(d_rho + o, d_vext ? d_vext + o : nullptr, conv[s],
-                                                         d_tot + o, mu[s], c->g.dv, N, c->cs.hs, dst[0], dst[1],
-                                                         dst[3], dst[2]);
+        if (conv32[s])
+          k_split_terms<float><<<nb, kMixThreads, 0, c->stream>>>(d_rho + o, d_vext ? d_vext + o : nullptr,
+                                                                  conv32[s], d_tot + o, mu[s], c->g.dv, N,
+                                                                  c->cs.hs, dst[0], dst[1], dst[3], dst[2]);
+        else
+          k_split_terms<double><<<nb, kMixThreads, 0, c->stream>>>(d_rho + o, d_vext ? d_vext + o : nullptr,
+                                                                   conv[s], d_tot + o, mu[s], c->g.dv, N,
+                                                                   c->cs.hs, dst[0], dst[1], dst[3], dst[2]);
       }
       CDFT_HIP(hipGetLastError());
       for (int t = 0; t < 4; ++t)

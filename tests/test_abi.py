@@ -80,7 +80,7 @@ def _create(diams, n_inter=0, precision=0, inter=None):
     ((1.0, -0.8), 0, 0, -1, "diameter"),              # hard_sphere_diameter must be positive
     ((1.0,), 2, 0, -4, "at most one"),                # one species: at most one interaction
     ((1.0, 0.8), 37, 0, -4, "at most 36"),            # every pair of 8 species is 36
-    ((1.0, 0.8), 0, 1, -4, "FP64"),                   # mixtures are FP64 only
+    ((1.0, 0.8), 0, 1, -4, "fused transforms"),       # a mixed-precision mixture needs a fused-schedule grid
 ])
 def test_species_and_interaction_limits(diams, n_inter, precision, code, needle):
     # rejected before any device work (cdft_ctx_create validates first)

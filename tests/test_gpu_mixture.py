@@ -3,7 +3,8 @@ functionals.hpp:64-95 with hard_sphere.hpp:174-254 and mean_field.hpp:233-281)
 against the FP64 CPU oracle, through the C ABI (cdft_total_species).
 
 Tolerances: energies 1e-12 relative (1e-14 absolute), forces 1e-11 relative
-to max|force| per species.
+to max|force| per species; mixed precision (FP32 real-space fields, fused
+grids): Omega and its parts 1e-6, forces 2e-5.
 """
 import numpy as np
 import pytest
@@ -363,3 +364,56 @@ def test_mixture_contexts_and_calls_return_their_device_memory():
     print(f"hip last error after the cycles: {after} ({hip.hipGetErrorName(after).decode()})", flush=True)
     assert after == 0, "a mixture context or call left an error in the HIP runtime's last-error slot"
     assert last == 0, "an earlier call in this process left an error in the HIP runtime's last-error slot"
+
+
+def _compare_mixed(ctx, orc, rho, mus, vext=None, tag=""):
+    # mixed-precision gate: Omega and its parts 1e-6 relative, forces 2e-5 relative to max|force| per species
+    res, f = ctx.total_species(rho, mus, vext=vext)
+    parts, fo = orc.total(rho, mus, vext=vext)
+    eo = abs(res["grand_potential"] - parts[5]) / abs(parts[5])
+    ef = max(float(np.max(np.abs(f[s] - fo[s])) / np.max(np.abs(fo[s]))) for s in range(rho.shape[0]))
+    print(f"{tag}: Omega rel err {eo:.2e}, force rel err {ef:.2e}", flush=True)
+    assert eo <= 1e-6 and ef <= 2e-5
+    for i, k in enumerate(PARTS):
+        assert res[k] == pytest.approx(parts[i], rel=1e-6, abs=1e-6), k
+
+
+def test_mixed_precision_binary_mixture_matches_oracle():
+    # FP32 real-space measure, derivative and potential fields; transforms, spectra and sums in FP64
+    shape, dx = (64, 64, 64), 0.1
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=1, sigma=0.8, epsilon=0.6),
+             dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
+    ctx, orc = _pair(shape, (1.0, 0.8), inter, "WhiteBearII", kT=1.0, dx=dx, precision="mixed")
+    assert ctx.schedule == "fused"
+    rho = np.stack([W.droplet(shape, dx, radius=1.5, width=0.4, rho_l=0.5, rho_v=0.03, noise=1e-3, seed=61),
+                    W.droplet(shape, dx, radius=1.2, width=0.4, rho_l=0.2, rho_v=0.05, noise=1e-3, seed=62)])
+    _compare_mixed(ctx, orc, rho, (-1.5, -1.2), tag="mixed binary 64^3")
+
+
+def test_mixed_precision_ternary_mixture_with_external_field_matches_oracle():
+    shape, dx = (256, 64, 64), 0.15
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=1, sj=2, sigma=0.9, epsilon=0.7)]
+    ctx, orc = _pair(shape, (1.0, 0.7, 1.3), inter, "esFMT", kT=0.9, dx=dx, A=1.2, B=-0.7, precision="mixed")
+    rho = _densities(shape, dx, (0.2, 0.15, 0.1), seed=111)
+    z = (np.arange(shape[2]) + 0.5) * dx
+    vext = np.stack([np.broadcast_to(a * np.cos(2 * np.pi * z / (shape[2] * dx)), shape)
+                     for a in (0.3, -0.2, 0.1)])
+    _compare_mixed(ctx, orc, rho, (0.1, -0.2, 0.3), vext=vext, tag="mixed ternary 256x64x64")
+
+
+def test_mixed_precision_mixture_terms_match_fp64():
+    # cdft_contributions reads the FP32 mean-field potentials of a mixed-precision mixture
+    import classicaldft_amd as cd
+
+    shape, dx = (64, 64, 64), 0.12
+    inter = [dict(LJ, si=0, sj=0), dict(LJ, si=0, sj=1, sigma=0.9, epsilon=0.8)]
+    kw = dict(diameters=(1.0, 0.8), interactions=inter, kT=1.1, model="WhiteBearII")
+    mixed = cd.Context(shape, dx, precision="mixed", **kw)
+    fp64 = cd.Context(shape, dx, **kw)
+    rho = _densities(shape, dx, (0.3, 0.2), seed=121)
+    rm, tm = mixed.contributions(rho, (0.4, -0.3))
+    rd, td = fp64.contributions(rho, (0.4, -0.3))
+    assert rm["grand_potential"] == pytest.approx(rd["grand_potential"], rel=1e-6)
+    for k in ("ideal", "external", "hard_sphere", "mean_field"):
+        scale = max(float(np.max(np.abs(td[k]))), 1e-300)
+        assert float(np.max(np.abs(tm[k] - td[k]))) / scale <= 2e-5, k
