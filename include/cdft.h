@@ -88,9 +88,11 @@ typedef struct {
 /* Replaces functionals::make_weights + make_functional
  * (include/dft/functionals/functionals.hpp:27-33, functional.hpp:93-107):
  * builds the FMT and mean-field weights for a physics::Model.  n_species is
- * 1..8.  A context of more than one species (a mixture; FP64 precision only,
- * up to 36 interactions between any species pairs) is evaluated by
- * cdft_total_species; the other evaluation and algorithm entry points act on
+ * 1..8.  A context of more than one species (a mixture; up to 36 interactions
+ * between any species pairs) is evaluated by cdft_total_species, in FP64, or
+ * in mixed precision (FP32 real-space fields) on grids the fused transforms
+ * cover (y and z extents 64 or 128, x extent 64, 128 or 256; elsewhere
+ * CDFT_EUNSUPPORTED); the other evaluation and algorithm entry points act on
  * one species and return CDFT_EUNSUPPORTED for it. */
 CDFT_API int cdft_ctx_create(const cdft_grid* grid, const double* diameters, int n_species,
                     const cdft_interaction* interactions, int n_interactions, double kT,
