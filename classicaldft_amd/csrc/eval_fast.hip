@@ -1,10 +1,11 @@
 // Fused six-kernel evaluation schedule (gfx950) for power-of-two grids.
 //
-//   kf_zfwd       per line quad: z-R2C of rho (FP64), y- and x-parity split -> A0
+//   kf_zfwd       per line quad (the z lines at x = i, i + NX/2 and y = m, m + NY/2):
+//                 z-R2C of rho (FP64), y- and x-parity split -> A0
 //   kf_rho_xy     per (kz, r, p): y- and x-FFT of rho's sub-plane (FP64) -> rho_hat
 //   kf_plane_fwd  per (kz, r, p): for each of the 10 plane channels: weight
 //                 times rho_hat on the kx = 2k + p sub-plane, its x- and
-//                 y-inverse (NX/2 x NY/2 points), written in the split layout
+//                 y-inverse (NX/2 x NY/2 points), written in the channel layout
 //   kf_zpoint     per line quad: x-parity and y-parity combine + z-C2R of
 //                 every channel (n_v,z from eta's spectrum, T_zz by the trace
 //                 identity), pointwise Phi/dPhi + local terms + partial sums,
@@ -778,24 +779,21 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
 }
 
 // ============================================================ plane kernels
-// A plane kernel block (1024 threads) owns the half plane (kz, r) and the
-// output / input row parity h.  Slot shapes per phase (all 1024 lanes busy at
-// 128 x 128 grids):
-//   y-FFT of full rows (NX rows, Q points):  Q/8 lanes x 8 elements
-//   y-FFT of half rows (NX/2 rows):          Q/4 lanes x 4 elements
-//   x-FFT (NX points, one per column):       NX/8 lanes x 8 elements
-//   half x-FFT (NX/2 points):                NX/8 lanes x 4 elements
+// A plane kernel block (1024 threads) works on one (kz, r, p) sub-plane of
+// NX/2 half rows i and Q = NY/2 columns.  Slot shapes per phase (all 1024
+// lanes busy at 128 x 128 grids):
+//   y-FFT of the rows (NX/2 rows, Q points):      Q/4 lanes x 4 elements
+//   x-FFT of the columns (Q columns, NX/2 points): NX/8 lanes x 4 elements
 // FP64 slot exchanges go through the buffers one real part at a time, so a
 // slot-buffer element is 8 bytes in every phase.
 template <int NX, int Q>
 struct PlaneCfg {
   static constexpr int BT = kBT;
-  static constexpr int NY = 2 * Q, MB = Q / 16, NXH = NX / 2;
-  static constexpr int TYF = Q / 8, SBYF = BT / TYF;
+  static constexpr int NY = 2 * Q, NXH = NX / 2;
   static constexpr int TYH = Q / 4, SBYH = BT / TYH, RPH = (NXH + SBYH - 1) / SBYH;
   static constexpr int TX = NX / 8, SBX = BT / TX, CPS = (Q + SBX - 1) / SBX;
   static constexpr int XBYTES = 8;
-  static_assert(TYH <= 64 && TX <= 64 && TYF >= 1, "unsupported plane shape");
+  static_assert(TYH <= 64 && TX <= 64 && TYH >= 1, "unsupported plane shape");
 };
 
 // Weight factors of one element in arithmetic C: w3, w2 and
@@ -1038,7 +1036,7 @@ struct FwdCfg {
 // column xs + SBX cc, k = xl + TX e (kx = 2 k + p); soff[rr][e] is the
 // split-layout offset of the lane's output (row ys + SBYH rr, m = yl + TYH e)
 // in a channel array.  x-inverse of the sub-plane's NX/2 points, then the
-// y-inverse of every row, stored in the split layout.  With DB, channels
+// y-inverse of every row, stored in the channel layout.  With DB, channels
 // alternate between two transpose buffers and the x- and y-slot buffers are
 // distinct, so the one barrier per channel suffices: a wave enters channel
 // c+2 (which reuses channel c's transpose buffer) only after every wave has
