@@ -191,6 +191,14 @@ __device__ __forceinline__ cx<C>* stage_twiddles(unsigned char* dst, const cx<C>
   block_copy<NT, n, cx<C>>([&](int i) { return src[i]; }, [&](int i, cx<C> v) { d[i] = v; });
   return d;
 }
+// Copies only the length-L table (L values) and returns it.
+template <typename C, int NT, int L>
+__device__ __forceinline__ cx<C>* stage_twiddle_table(unsigned char* dst, const cx<C>* __restrict__ src) {
+  cx<C>* d = reinterpret_cast<cx<C>*>(dst);
+  const cx<C>* s = tw_for<L>(src);
+  block_copy<NT, L, cx<C>>([&](int i) { return s[i]; }, [&](int i, cx<C> v) { d[i] = v; });
+  return d;
+}
 
 // ============================================================ kf_zfwd
 template <int NZ, typename S>
@@ -243,7 +251,7 @@ __global__ void __launch_bounds__(2 * NZ) kf_zfwd(const I* __restrict__ rho, cx<
 #pragma unroll
   for (int q = 0; q < 8; ++q) v[q] = {amap.apply(static_cast<double>(ra[q])), amap.apply(static_cast<double>(rb[q]))};
   cx<double>* xb = xbuf + slot * XS;
-  slot_fft<double, NZ, T, -1>(v, xb, lane, twl);
+  slot_fft<double, NZ, T, -1>(v, xb, lane, tw_for<NZ>(twl));
   wave_sync();
 #pragma unroll
   for (int e = 0; e < 8; ++e) xb[xpad(lane + T * e)] = v[e];
@@ -323,9 +331,10 @@ __device__ __forceinline__ void build_c2r(cx<C> (&v)[8], int lane, F&& part) {
 
 __host__ __device__ inline bool bwd_unit_split(int unit, int nunits, int nblk);
 
-// With fin_rows > 0 the grid has one block more, which does the final
-// reduction of the point kernel's partial sums (reduce_rows) while the other
-// blocks transform: the evaluation then needs no separate finalisation launch.
+// With fin_rows > 0 the grid has one block more, block 0, which does the
+// final reduction of the point kernel's partial sums (reduce_rows) while the
+// other blocks transform: the evaluation then needs no separate finalisation
+// launch, and the reduction, dispatched first, overlaps the whole kernel.
 template <typename S, typename C, int NZ, typename O = double>
 __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __restrict__ P, O* __restrict__ out,
                                                                const double* __restrict__ add, double scale,
@@ -334,7 +343,7 @@ __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __res
                                                                int fin_rows, double* __restrict__ res) {
   constexpr int NT = zinv_threads<NZ>(), T = NZ / 8, NZH = NZ / 2 + 1, XS = slot_buf_len<NZ>();
   extern __shared__ __align__(16) unsigned char smem[];
-  if (fin_rows > 0 && blockIdx.x == gridDim.x - 1) {
+  if (fin_rows > 0 && blockIdx.x == 0) {
     double* sh = reinterpret_cast<double*>(smem);
     reduce_rows<NT>(partials, fin_rows, res, sh, sh + NT);
     return;
@@ -344,7 +353,7 @@ __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __res
   const int slot = threadIdx.x / T, lane = threadIdx.x % T;
   const int xh = slot >> 3, ms = slot & 7;
   const int mb8 = g.q >> 3;
-  const long blk = blockIdx.x;
+  const long blk = static_cast<long>(blockIdx.x) - (fin_rows > 0 ? 1 : 0);
   const int i = static_cast<int>(blk / mb8), mc = static_cast<int>(blk % mb8);
   const int m = mc * 8 + ms;
   const long la = static_cast<long>(i + xh * (g.nx / 2)) * g.ny + m, lb = la + g.q;
@@ -396,7 +405,7 @@ __global__ void __launch_bounds__(zinv_threads<NZ>()) kf_zinv(const cx<S>* __res
   __syncthreads();
   cx<C> v[8];
   build_c2r<C, NZ>(v, lane, [&](int r, int kz) { return cb[((xh * 2 + r) * 8 + ms) * NZH + kz]; });
-  slot_fft<C, NZ, T, +1>(v, xbuf + slot * XS, lane, twl);
+  slot_fft<C, NZ, T, +1>(v, xbuf + slot * XS, lane, tw_for<NZ>(twl));
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
     const int z = lane + T * e;
@@ -415,9 +424,10 @@ __host__ __device__ constexpr int zpoint_tw_off() {
   return zpoint_stage_bytes<NZ, S>() +
          zp_waves<S>() * (64 / (NZ / 8)) * slot_buf_len<NZ>() * static_cast<int>(sizeof(cx<double>));
 }
-template <int NZ, typename S>
+// the length-NZ twiddle tables of the FP64 and of the CV rounds
+template <int NZ, typename S, typename CV>
 __host__ __device__ constexpr int zpoint_smem() {
-  return zpoint_tw_off<NZ, S>() + 2 * tw_bytes<double>();
+  return zpoint_tw_off<NZ, S>() + NZ * static_cast<int>(sizeof(cx<double>) + sizeof(cx<CV>));
 }
 
 // One forward round: slot `slot` of wave `wave` transforms channel position
@@ -521,7 +531,8 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
   constexpr bool MF = (NF == 12 || NF == 1);
   extern __shared__ __align__(16) unsigned char smem[];
   cx<S>* stg = reinterpret_cast<cx<S>*>(smem);  // [gs][r][line pair][kz]
-  const int wave = threadIdx.x >> 6, lid = threadIdx.x & 63;
+  // the wave index is uniform over the wave: kept in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lid = threadIdx.x & 63;
   unsigned char* xwave = smem + zpoint_stage_bytes<NZ, S>() + wave * NS * XS * static_cast<int>(sizeof(cx<double>));
   // Block: half row i and the MW columns m0 .. m0 + MW - 1 (m0 = 8 mc + MW mq);
   // wave (xh, mm) = (wave / MW, wave % MW) owns the line pair at
@@ -539,9 +550,10 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
   const long cbase = (static_cast<long>(irow) * mb8 + mc) * (NZH * 32) + MW * mq;
   const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   S wv[NWV][PPL];
-  // twiddle tables in LDS; the first barrier below publishes them
-  const cx<double>* tw64l = stage_twiddles<double, kZP>(smem + zpoint_tw_off<NZ, S>(), tw64);
-  const cx<CV>* twvl = stage_twiddles<CV, kZP>(smem + zpoint_tw_off<NZ, S>() + tw_bytes<double>(), twv);
+  // the length-NZ twiddle tables in LDS; the first barrier below publishes them
+  const cx<double>* tw64l = stage_twiddle_table<double, kZP, NZ>(smem + zpoint_tw_off<NZ, S>(), tw64);
+  const cx<CV>* twvl = stage_twiddle_table<CV, kZP, NZ>(
+      smem + zpoint_tw_off<NZ, S>() + NZ * static_cast<int>(sizeof(cx<double>)), twv);
   if constexpr (NRF == 0) __syncthreads();
 
   // ---- forward rounds: stage NS channels of the block's line pairs, C2R.
@@ -652,7 +664,7 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
 #pragma unroll
     for (int i = 0; i < kZB; ++i) outv[i] = S(0);
     const double rho = amap.apply(rho_in[p]);
-    const double lr = log(fmax(rho, 1e-300));
+    const double lr = flog(fmax(rho, 1e-300));
     double lfv = (lr - mu) * dv;  // ideal_gas.hpp:27-33
     acc[0] += rho * (lr - 1.0);
     acc[4] += rho;
@@ -968,9 +980,9 @@ __global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, c
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         if (p) yv[rr][q] = cmul(yv[rr][q], wp);
-        if (r) yv[rr][q] = twiddle<double, RC::NY, -1>(yv[rr][q], yl + TY * q, twl);
+        if (r) yv[rr][q] = twiddle<double, RC::NY, -1>(yv[rr][q], yl + TY * q, tw_for<RC::NY>(twl));
       }
-      slot_fft<double, Q, TY, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * 8, yl, twl);
+      slot_fft<double, Q, TY, -1, true>(yv[rr], xraw + ys * slot_buf_len<Q>() * 8, yl, tw_for<Q>(twl));
     }
   }
   // transpose, one real part at a time: tb[s * NXHP + i]
@@ -1002,7 +1014,7 @@ __global__ void __launch_bounds__(512) kf_rho_xy(const cx<S>* __restrict__ A0, c
   for (int cc = 0; cc < CX; ++cc) {
     const int sc = xs + SX * cc;
     if (sc < Q) {
-      slot_fft<double, NXH, TX, -1, true>(f[cc], xraw + xs * slot_buf_len<NXH>() * 8, xl, twl);
+      slot_fft<double, NXH, TX, -1, true>(f[cc], xraw + xs * slot_buf_len<NXH>() * 8, xl, tw_for<NXH>(twl));
       if (sum_out && kz == 0 && r == 0 && p == 0 && sc == 0 && xl == 0) {
         sum_out[0] = f[cc][0].re;
         f[cc][0] = {0.0, 0.0};
@@ -1059,7 +1071,7 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
       cx<CC> zt[4];
 #pragma unroll
       for (int e = 0; e < 4; ++e) zt[e] = cvt<CC>(Wof(cc, e));
-      slot_fft<CC, P::NXH, P::TX, +1, sizeof(CC) == 8>(zt, xbx + xs * XS * P::XBYTES, xl, tw);
+      slot_fft<CC, P::NXH, P::TX, +1, sizeof(CC) == 8>(zt, xbx + xs * XS * P::XBYTES, xl, tw_for<P::NXH>(tw));
 #pragma unroll
       for (int e = 0; e < 4; ++e) tb[(xl + P::TX * e) * QP + col] = cvt<S>(zt[e]);
     }
@@ -1072,12 +1084,12 @@ __device__ __forceinline__ void plane_fwd_channel(WF&& Wof, cx<S>* tb, unsigned 
       cx<CC> v[4];
 #pragma unroll
       for (int q = 0; q < 4; ++q) v[q] = cvt<CC>(tb[row * QP + yl + P::TYH * q]);
-      slot_fft<CC, Q, P::TYH, +1, sizeof(CC) == 8>(v, xby + ys * YS * P::XBYTES, yl, tw);
+      slot_fft<CC, Q, P::TYH, +1, sizeof(CC) == 8>(v, xby + ys * YS * P::XBYTES, yl, tw_for<Q>(tw));
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int m = yl + P::TYH * e;
         cx<CC> b = v[e];
-        if (r) b = twiddle<CC, P::NY, +1>(b, m, tw);
+        if (r) b = twiddle<CC, P::NY, +1>(b, m, tw_for<P::NY>(tw));
         out[soff[rr][e]] = cvt<S>(b);
       }
     }
@@ -1354,9 +1366,9 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         if (i < NXH) {
           if (r) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) v[rr][q] = twiddle<CV, PC::NY, -1>(v[rr][q], ylh + PC::TYH * q, twl);
+            for (int q = 0; q < 4; ++q) v[rr][q] = twiddle<CV, PC::NY, -1>(v[rr][q], ylh + PC::TYH * q, tw_for<PC::NY>(twl));
           }
-          slot_fft<CV, Q, PC::TYH, -1, SPL>(v[rr], xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, twl);
+          slot_fft<CV, Q, PC::TYH, -1, SPL>(v[rr], xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, tw_for<Q>(twl));
 #pragma unroll
           for (int e = 0; e < 4; ++e) tb[(ylh + PC::TYH * e) * NXHP + i] = cvt<S>(v[rr][e]);
         }
@@ -1372,7 +1384,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
           cx<CV> f[4];
 #pragma unroll
           for (int q = 0; q < 4; ++q) f[q] = cvt<CV>(tb[col * NXHP + xl + PC::TX * q]);
-          slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, twl);
+          slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, tw_for<NXH>(twl));
 #pragma unroll
           for (int e = 0; e < 4; ++e)
             acc[cc][e] = cadd(acc[cc][e], cmul(apply_form<CV>(form, kxc[e], kyv, kzc, wf[cc][e]), f[e]));
@@ -1390,7 +1402,7 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
     for (int cc = 0; cc < CPS; ++cc) {
       const int col = xs + PC::SBX * cc;
       if (col < Q) {
-        slot_fft<CV, NXH, PC::TX, +1, SPL>(acc[cc], xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, twl);
+        slot_fft<CV, NXH, PC::TX, +1, SPL>(acc[cc], xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, tw_for<NXH>(twl));
 #pragma unroll
         for (int e = 0; e < 4; ++e) tbf[(xl + PC::TX * e) * QP + col] = cvt<S>(acc[cc][e]);
       }
@@ -1404,11 +1416,11 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         cx<CV> w[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) w[q] = cvt<CV>(tbf[row * QP + ylh + PC::TYH * q]);
-        slot_fft<CV, Q, PC::TYH, +1, SPL>(w, xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, twl);
+        slot_fft<CV, Q, PC::TYH, +1, SPL>(w, xby + ysh * slot_buf_len<Q>() * PC::XBYTES, ylh, tw_for<Q>(twl));
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           cx<CV> b = w[e];
-          if (r) b = twiddle<CV, PC::NY, +1>(b, ylh + PC::TYH * e, twl);
+          if (r) b = twiddle<CV, PC::NY, +1>(b, ylh + PC::TYH * e, tw_for<PC::NY>(twl));
           out[loff[rr][e]] = cvt<S>(b);
         }
       }
@@ -1634,7 +1646,7 @@ void launch_fast_zpoint(cx<S>* SF, const double* rho, const double* vext, double
   {                                                                                                    \
     auto k = (NF >= 11 && wb2) ? kf_zpoint<S, CV, NZ, NF, NB, (NF >= 11 ? FMT_WHITEBEAR_II : kFmtRuntime)> \
                                : kf_zpoint<S, CV, NZ, NF, NB, kFmtRuntime>;                            \
-    const int sm = zpoint_smem<NZ, S>();                                                               \
+    const int sm = zpoint_smem<NZ, S, CV>();                                                               \
     set_smem(k, sm);                                                                                   \
     hipLaunchKernelGGL(k, dim3(nblk), dim3(64 * zp_waves<S>()), sm, st, SF, rho, vext, lf, partials, g, \
                        as<cx<double>>(tw64), as<cx<CV>>(twv), fp, mu, amap, shift_sum, dc);            \

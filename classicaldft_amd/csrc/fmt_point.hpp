@@ -23,6 +23,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "fast_math.hpp"
+
 namespace cdft {
 
 // Forward-mode dual number: v + d*eps with eps^2 = 0.
@@ -45,8 +47,10 @@ __host__ __device__ inline Dual& operator+=(Dual& a, Dual b) { return a = a + b;
 // value part (the scalar itself for double): branches test values only
 __host__ __device__ inline double vof(double x) { return x; }
 __host__ __device__ inline double vof(Dual x) { return x.v; }
-__host__ __device__ inline double dlog(double x) { return log(x); }
-__host__ __device__ inline Dual dlog(Dual x) { return {log(x.v), x.d / x.v}; }
+__host__ __device__ inline double dlog(double x) { return flog(x); }
+__host__ __device__ inline Dual dlog(Dual x) { return {flog(x.v), x.d / x.v}; }
+__host__ __device__ inline double drcp(double x) { return frcp(x); }
+__host__ __device__ inline Dual drcp(Dual x) { return Dual(1.0) / x; }
 __host__ __device__ inline double dmin(double x, double c) { return fmin(x, c); }
 __host__ __device__ inline Dual dmin(Dual x, double c) { return x.v < c ? x : Dual(c); }
 
@@ -73,7 +77,7 @@ template <typename T>
 __host__ __device__ inline void f_funcs(int model, T e, T* f, T* df) {
   const T om = 1.0 - e;
   const T L = dlog(om);
-  const T r = 1.0 / (e * om);
+  const T r = drcp(e * om);
   const T io = e * r, ie = om * r;  // 1/(1-e), 1/e
   const T io2 = io * io, ie2 = ie * ie;
   // f1 (models.hpp:157-160): identical for every model

@@ -474,7 +474,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
     if (q < npts && l < g.nxy) {
       const long p = l0 * nz + q;
       const double rho = amap.apply(rho_in[p]);
-      const double lr = log(fmax(rho, 1e-300));
+      const double lr = flog(fmax(rho, 1e-300));
       double lf = (lr - mu) * dv;                // ideal_gas.hpp:27-33
       two_sum(acc[0], cmp[0], rho * (lr - 1.0));
       two_sum(acc[4], cmp[4], rho);

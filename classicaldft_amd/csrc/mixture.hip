@@ -215,7 +215,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_point(XT* __restrict__ X, c
     }
     for (int s = 0; s < mp.S; ++s) {
       const double r = rho[s * N + p];
-      const double lr = log(fmax(r, 1e-300));  // ideal_gas.hpp:27
+      const double lr = flog(fmax(r, 1e-300));  // ideal_gas.hpp:27
       double f = lr - mp.mu[s];
       sums[0] += r * (lr - 1.0);
       sums[4] += mp.mu[s] * r;
@@ -283,7 +283,7 @@ __global__ __launch_bounds__(kMixThreads) void k_split_terms(const double* __res
                                                              double* __restrict__ f_hs) {
   for (long p = static_cast<long>(blockIdx.x) * kMixThreads + threadIdx.x; p < n;
        p += static_cast<long>(gridDim.x) * kMixThreads) {
-    const double id = (log(fmax(rho[p], 1e-300)) - mu) * dv;
+    const double id = (flog(fmax(rho[p], 1e-300)) - mu) * dv;
     const double ext = vext ? vext[p] * dv : 0.0;
     const double mf = conv ? static_cast<double>(conv[p]) * dv : 0.0;
     if (f_id) f_id[p] = id;

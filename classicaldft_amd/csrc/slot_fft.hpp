@@ -79,11 +79,17 @@ struct SlotPlan {
   }
 };
 
-// v * exp(SG 2 pi i m / L), 0 <= m < L, from the concatenated tables.
+// The length-L table inside the concatenated tables.
+template <int L, typename C>
+__device__ __forceinline__ const cx<C>* tw_for(const cx<C>* all) {
+  static_assert(L >= 4 && L <= kTwMax && (L & (L - 1)) == 0, "no twiddle table for this length");
+  return all + (L - 4);
+}
+
+// v * exp(SG 2 pi i m / L), 0 <= m < L; tw is the length-L table.
 template <typename C, int L, int SG>
 __device__ __forceinline__ cx<C> twiddle(cx<C> v, int m, const cx<C>* __restrict__ tw) {
-  static_assert(L >= 4 && L <= kTwMax && (L & (L - 1)) == 0, "no twiddle table for this length");
-  cx<C> w = tw[L - 4 + m];
+  cx<C> w = tw[m];
   if (SG > 0) w.im = -w.im;
   return cmul(v, w);
 }
@@ -167,7 +173,8 @@ struct SlotPass {
 };
 
 // One line transform by one slot; `xb` is the slot's buffer of
-// slot_buf_bytes<C, L, SPLIT>() bytes.  All lanes of the slot call it together.
+// slot_buf_bytes<C, L, SPLIT>() bytes, `tw` the length-L twiddle table
+// (tw_for<L> of the concatenated tables).  All lanes of the slot call it together.
 template <typename C, int L, int T, int SG, bool SPLIT = false>
 __device__ __forceinline__ void slot_fft(cx<C> (&v)[L / T], void* xb, int lane, const cx<C>* __restrict__ tw) {
   constexpr int E = L / T;
