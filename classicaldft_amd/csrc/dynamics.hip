@@ -385,8 +385,9 @@ long face_count(const Geom& g) {
 // host replays it in blocks of kFireBlock steps and reads the 64-byte
 // control state between blocks.  A step after the run has ended (converged,
 // step limit, uphill limit) changes nothing, so a block may overrun the end.
-// Several runs (contexts with their own streams) advance together by
-// launching every run's block before waiting for any.
+// Several runs (contexts with their own streams) advance together: one
+// graph holds a block of steps of every running run (BatchGraph), or, where
+// graphs are off, every run's block is launched before any is waited for.
 constexpr int kFireBlock = 8;
 
 struct FireRun {
@@ -580,6 +581,70 @@ struct FireRun {
     result->dt = h->dt;
     result->alpha = h->alpha;
   }
+};
+
+
+// A stream of its own for a batch's graph launches and control reads
+// (default priority; created and destroyed with the batch call).
+struct BatchStream {
+  hipStream_t s = nullptr;
+  BatchStream() { CDFT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking)); }
+  ~BatchStream() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+  BatchStream(const BatchStream&) = delete;
+  BatchStream& operator=(const BatchStream&) = delete;
+};
+
+// The batch graph: `layers` steps of every run in `live`, captured in one
+// stream capture.  The capture forks from the batch stream onto every run's
+// context stream, records each run's FIRE step kernels (launch_step_kernels)
+// `layers` times on that stream, and joins back: a flat graph of kernel nodes
+// whose only edges chain one run's own kernels, so the runtime executes the
+// runs' chains concurrently.
+struct BatchGraph {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<hipEvent_t> ev;  // [0] the fork, [1 + i] run i's join
+  ~BatchGraph() {
+    clear();
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  void clear() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (g) (void)hipGraphDestroy(g);
+    exec = nullptr;
+    g = nullptr;
+  }
+  void build(const std::vector<FireRun*>& live, int layers, hipStream_t bs) {
+    clear();
+    while (ev.size() < live.size() + 1) {
+      hipEvent_t e = nullptr;
+      CDFT_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      ev.push_back(e);
+    }
+    CDFT_HIP(hipStreamBeginCapture(bs, hipStreamCaptureModeThreadLocal));
+    try {
+      CDFT_HIP(hipEventRecord(ev[0], bs));
+      for (FireRun* r : live) CDFT_HIP(hipStreamWaitEvent(r->c->stream, ev[0], 0));
+      for (int s = 0; s < layers; ++s)
+        for (FireRun* r : live) r->launch_step_kernels();
+      for (size_t i = 0; i < live.size(); ++i) {
+        CDFT_HIP(hipEventRecord(ev[i + 1], live[i]->c->stream));
+        CDFT_HIP(hipStreamWaitEvent(bs, ev[i + 1], 0));
+      }
+    } catch (...) {
+      hipGraph_t g0 = nullptr;
+      (void)hipStreamEndCapture(bs, &g0);
+      if (g0) (void)hipGraphDestroy(g0);
+      throw;
+    }
+    CDFT_HIP(hipStreamEndCapture(bs, &g));
+    CDFT_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+  }
+  BatchGraph() = default;
+  BatchGraph(const BatchGraph&) = delete;
+  BatchGraph& operator=(const BatchGraph&) = delete;
 };
 
 }  // namespace cdft
@@ -776,15 +841,47 @@ int cdft_minimize_grand_potential_batch(int n, cdft_ctx* const* ctxs, const cdft
     }
     for (int i = 0; i < n; ++i) runs[static_cast<size_t>(i)]->launch_start(rho0[i]);
     for (auto& r : runs) r->sync();
-    // every running run's block is queued before any is waited for
+    bool graphs = true;
+    for (auto& r : runs) graphs = graphs && r->exec != nullptr;
     std::vector<FireRun*> live;
-    for (;;) {
-      live.clear();
-      for (auto& r : runs)
-        if (r->running()) live.push_back(r.get());
-      if (live.empty()) break;
-      for (FireRun* r : live) r->launch_block();
-      for (FireRun* r : live) r->sync();
+    if (graphs && n > 0) {
+      // One graph advances every running state point by up to kFireBlock
+      // steps (BatchGraph); it is rebuilt when the set of running state
+      // points or the block length changes.
+      CDFT_HIP(hipSetDevice(ctxs[0]->impl->device));
+      BatchStream bs;
+      BatchGraph bg;
+      std::vector<FireRun*> built;
+      int built_layers = 0;
+      for (;;) {
+        live.clear();
+        int layers = kFireBlock;
+        for (auto& r : runs)
+          if (r->running()) {
+            live.push_back(r.get());
+            layers = std::min(layers, std::max(1, r->k.p.max_steps - r->host->iteration));
+          }
+        if (live.empty()) break;
+        if (live != built || layers != built_layers) {
+          bg.build(live, layers, bs.s);
+          built = live;
+          built_layers = layers;
+        }
+        CDFT_HIP(hipGraphLaunch(bg.exec, bs.s));
+        for (FireRun* r : live)
+          CDFT_HIP(hipMemcpyAsync(r->host, r->ctl.p, sizeof(FireCtl), hipMemcpyDeviceToHost, bs.s));
+        CDFT_HIP(hipStreamSynchronize(bs.s));
+      }
+    } else {
+      // every running run's block is queued before any is waited for
+      for (;;) {
+        live.clear();
+        for (auto& r : runs)
+          if (r->running()) live.push_back(r.get());
+        if (live.empty()) break;
+        for (FireRun* r : live) r->launch_block();
+        for (FireRun* r : live) r->sync();
+      }
     }
     for (int i = 0; i < n; ++i) {
       FireRun& r = *runs[static_cast<size_t>(i)];

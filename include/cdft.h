@@ -270,9 +270,10 @@ CDFT_API int cdft_minimize_fixed_mass(cdft_ctx* ctx, const cdft_fire* fire, cons
  * for many small state points that leave most of the device idle alone).
  * Run i minimises on context ctxs[i] from rho0[i] at mu[i] with the shared
  * FIRE settings; the contexts must be distinct and on one device, their
- * grids may differ.  Every step of every active run is queued on its
- * context's stream before any is waited for, so the device works on all of
- * them at once; each run follows exactly the trajectory
+ * grids may differ.  One HIP graph advances every running state point by a
+ * block of up to 8 FIRE steps (each run's step kernels captured on its own
+ * context stream, chained per run only), so the device works on all of them
+ * at once from one launch; each run follows exactly the trajectory
  * cdft_minimize_grand_potential gives it.  status[i] receives the run's
  * status (CDFT_ERUNTIME when FIRE exceeded max_uphill, which ends that run
  * only); the call itself fails on invalid arguments and device errors. */
