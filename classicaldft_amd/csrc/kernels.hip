@@ -72,8 +72,9 @@ __device__ __forceinline__ cx<C> unpack_pair(const cx<C>* line, int kz, int nz, 
 template <typename C, typename T>
 __device__ __forceinline__ void build_c2r_lines(const cx<T>* __restrict__ stage, cx<C>* __restrict__ dst, int npairs,
                                                 int nz, int nzh) {
+  const UDiv dnz(nz);  // block-uniform divisors
   for (int e = threadIdx.x; e < npairs * nz; e += blockDim.x) {
-    const int p = e / nz, k = e - p * nz;
+    const int p = dnz.div(e), k = e - p * nz;
     const bool mirror = k >= nzh;
     const int kk = mirror ? nz - k : k;
     cx<C> A = cvt<C>(stage[(2 * p) * nzh + kk]);
@@ -98,6 +99,7 @@ template <typename S, typename C>
 __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S>* __restrict__ out, Geom g,
                                              FftPlan pz, const cx<C>* __restrict__ twz, int nlb, AliasMap amap,
                                              const double* __restrict__ shift_sum) {
+  const UDiv dnlb(nlb), dnz(g.nz);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nz = g.nz, nzh = g.nzh;
@@ -106,7 +108,7 @@ __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S
   in += static_cast<long>(blockIdx.y) * g.N;  // field blockIdx.y of a multi-field launch
   out += static_cast<long>(blockIdx.y) * g.M;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
-    const int ll = e / nz, z = e - ll * nz;
+    const int ll = dnz.div(e), z = e - ll * nz;
     const long l = l0 + ll;
     C val = C(0);
     if (l < g.nxy) val = static_cast<C>(amap.apply(in[l * nz + z]) - shift);
@@ -116,7 +118,7 @@ __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S
   __syncthreads();
   fft_inplace<C, 8>(buf, nlb >> 1, nz, pz, -1, twz);
   for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
-    const int kz = e / nlb, ll = e - kz * nlb;
+    const int kz = dnlb.div(e), ll = e - kz * nlb;
     const long l = l0 + ll;
     if (l >= g.nxy) continue;
     out[static_cast<long>(kz) * g.nxy + l] = cvt<S>(unpack_pair(buf + (ll >> 1) * nz, kz, nz, ll & 1));
@@ -130,6 +132,7 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
                                              const double* __restrict__ add, double scale, Geom g, FftPlan pz,
                                              const cx<C>* __restrict__ twz, int nlb,
                                              const double* __restrict__ shift_sum, DcTable dc) {
+  const UDiv dnlb(nlb), dnz(g.nz);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   const int nz = g.nz, nzh = g.nzh;
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
@@ -138,7 +141,7 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
   const cx<S>* Sc = Sp + static_cast<long>(ch) * g.M;
   const long l0 = static_cast<long>(blockIdx.x) * nlb;
   for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
-    const int kz = e / nlb, ll = e - kz * nlb;
+    const int kz = dnlb.div(e), ll = e - kz * nlb;
     const long l = l0 + ll;
     stage[ll * nzh + kz] = (l < g.nxy) ? Sc[static_cast<long>(kz) * g.nxy + l] : cx<S>{S(0), S(0)};
   }
@@ -150,7 +153,7 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
   const double* ac = add ? add + static_cast<long>(ch) * g.N : nullptr;
   const double cst = shift_sum ? dc.v[ch] * shift_sum[0] * g.inv_n : 0.0;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
-    const int ll = e / nz, z = e - ll * nz;
+    const int ll = dnz.div(e), z = e - ll * nz;
     const long l = l0 + ll;
     if (l >= g.nxy) continue;
     const cx<C> v = buf[(ll >> 1) * nz + z];
@@ -214,6 +217,7 @@ __global__ void __launch_bounds__(NT) k_lines(cx<S>* __restrict__ Sp, long nline
 template <typename S, typename C, int MAXE>
 __global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, FftPlan px,
                                                const cx<C>* __restrict__ twx, int ty, int sg) {
+  const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
@@ -221,13 +225,13 @@ __global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, F
   const int kz = blockIdx.y;
   cx<S>* plane = Sp + static_cast<long>(blockIdx.z) * g.M + static_cast<long>(kz) * nx * ny;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-    const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+    const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
     buf[yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
   }
   __syncthreads();
   fft_inplace<C, MAXE>(buf, ty, ls, px, sg, twx);
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-    const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+    const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
     if (y < ny) plane[x * ny + y] = cvt<S>(buf[yl * ls + x]);
   }
 }
@@ -239,6 +243,7 @@ template <typename S, typename C, int EPT, int G>
 __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
                                              const S* __restrict__ mf, const cx<double>* __restrict__ wt, Geom g,
                                              FftPlan px, const cx<C>* __restrict__ twx, int ty, ChannelSet cs) {
+  const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
@@ -248,7 +253,7 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
   const long plane_off = static_cast<long>(kz) * nx * ny;
   const cx<S>* plane = S0 + plane_off;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-    const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+    const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
     buf[yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
   }
   __syncthreads();
@@ -265,7 +270,7 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
     wr[m] = {0.0, 0.0, 0.0};
     wm[m] = 0.0;
     if (e < ty * nx) {
-      const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+      const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
       rh[m] = buf[yl * ls + x];
       if (y < ny) {
         const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
@@ -287,7 +292,7 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
       for (int m = 0; m < EPT; ++m) {
         const int e = threadIdx.x + m * blockDim.x;
         if (e < ty * nx) {
-          const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+          const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
           const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
           const cx<double> w = channel_weight_t(ch, false, kx, ky, kzv, wr[m], wm[m], invR);
           buf[gi * slab + yl * ls + x] = wmul(rh[m], w);
@@ -299,7 +304,7 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
     for (int gi = 0; gi < ng; ++gi) {
       cx<S>* outp = SF + static_cast<long>(c0 + gi) * g.M + plane_off;
       for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-        const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+        const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
         if (y < ny) outp[x * ny + y] = cvt<S>(buf[gi * slab + yl * ls + x]);
       }
     }
@@ -314,6 +319,7 @@ template <typename S, typename C, int EPT, int G>
 __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
                                              const cx<double>* __restrict__ wt, Geom g, FftPlan px,
                                              const cx<C>* __restrict__ twx, int ty, int nb) {
+  const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
@@ -331,7 +337,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
     wr[m] = {0.0, 0.0, 0.0};
     const int e = threadIdx.x + m * blockDim.x;
     if (e < ty * nx) {
-      const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+      const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
       if (y < ny) {
         const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
         wr[m] = wrad(wt[plane_off + x * ny + y], kx * kx + ky * ky + kzv * kzv, invR);
@@ -343,7 +349,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
     for (int gi = 0; gi < ng; ++gi) {
       const cx<S>* plane = SB + static_cast<long>(c0 + gi) * g.M + plane_off;
       for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-        const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+        const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
         buf[gi * slab + yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
       }
     }
@@ -354,7 +360,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
       for (int m = 0; m < EPT; ++m) {
         const int e = threadIdx.x + m * blockDim.x;
         if (e < ty * nx) {
-          const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+          const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
           const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
           const cx<double> w = channel_weight_t(c0 + gi, true, kx, ky, kzv, wr[m], 0.0, invR);
           acc[m] = cadd(acc[m], wmul(buf[gi * slab + yl * ls + x], w));
@@ -367,7 +373,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
   for (int m = 0; m < EPT; ++m) {
     const int e = threadIdx.x + m * blockDim.x;
     if (e < ty * nx) {
-      const int x = e / ty, yl = e - x * ty;
+      const int x = dty.div(e), yl = e - x * ty;
       buf[yl * ls + x] = acc[m];
     }
   }
@@ -375,7 +381,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
   fft_inplace<C, EPT>(buf, ty, ls, px, +1, twx);
   cx<S>* outp = SO + plane_off;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-    const int x = e / ty, yl = e - x * ty, y = y0 + yl;
+    const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
     if (y < ny) outp[x * ny + y] = cvt<S>(buf[yl * ls + x]);
   }
 }
@@ -399,6 +405,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
                                                const cx<CF>* __restrict__ twf, const cx<CB>* __restrict__ twb,
                                                int nlb, FmtParams fp, double mu, AliasMap amap,
                                                const double* __restrict__ shift_sum, DcTable dc) {
+  const UDiv dnlb(nlb), dnz(g.nz);  // block-uniform divisors
   constexpr bool HS = NF >= 11;
   constexpr bool MF = (NF == 12 || NF == 1);
   constexpr int MFC = HS ? 11 : 0;
@@ -422,7 +429,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
     for (int gi = 0; gi < ng; ++gi) {
       const cx<S>* Sc = SF + static_cast<long>(c0 + gi) * g.M;
       for (int e = threadIdx.x; e < hs; e += blockDim.x) {
-        const int kz = e / nlb, ll = e - kz * nlb;
+        const int kz = dnlb.div(e), ll = e - kz * nlb;
         const long l = l0 + ll;
         stage[gi * hs + ll * nzh + kz] = (l < g.nxy) ? Sc[static_cast<long>(kz) * g.nxy + l] : cx<S>{S(0), S(0)};
       }
@@ -440,7 +447,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
           const int q = threadIdx.x + m * blockDim.x;
           S val = S(0);
           if (q < npts) {
-            const int ll = q / nz, z = q - ll * nz;
+            const int ll = dnz.div(q), z = q - ll * nz;
             const cx<CF> v = fbuf[gi * zs + (ll >> 1) * nz + z];
             val = static_cast<S>((ll & 1) ? v.im : v.re);
           }
@@ -459,7 +466,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
 #pragma unroll 1
   for (int m = 0; m < PPT; ++m) {
     const int q = threadIdx.x + m * blockDim.x;
-    const long l = l0 + q / nz;
+    const long l = l0 + dnz.div(q);
     S cur[NFA];
 #pragma unroll
     for (int i = 0; i < NF; ++i) {
@@ -529,7 +536,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
         for (int m = 0; m < PPT; ++m) {
           const int q = threadIdx.x + m * blockDim.x;
           if (q < npts) {
-            const int ll = q / nz, z = q - ll * nz;
+            const int ll = dnz.div(q), z = q - ll * nz;
             CB* dst = reinterpret_cast<CB*>(&bbuf[gi * zs + (ll >> 1) * nz + z]);
             dst[ll & 1] = static_cast<CB>(wv[c][m]);
           }
@@ -541,7 +548,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
     for (int gi = 0; gi < ng; ++gi) {
       cx<S>* outc = SF + static_cast<long>(c0 + gi) * g.M;
       for (int e = threadIdx.x; e < hs; e += blockDim.x) {
-        const int kz = e / nlb, ll = e - kz * nlb;
+        const int kz = dnlb.div(e), ll = e - kz * nlb;
         const long l = l0 + ll;
         if (l >= g.nxy) continue;
         outc[static_cast<long>(kz) * g.nxy + l] =
