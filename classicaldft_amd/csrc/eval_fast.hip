@@ -683,7 +683,8 @@ __global__ void __launch_bounds__(64 * zp_waves<S>(), 2) kf_zpoint(cx<S>* SF, co
 #pragma unroll
       for (int i = 0; i < 10; ++i) w[i] = static_cast<double>(cur[zpos(NF, i)]) + dc.v[i] * shift;
       w[10] = w[1] - w[5] - w[8];  // T_zz = n2 - T_xx - T_yy
-      const double phi = fmt_phi_grad(fpk, w, gg);
+      // FP32 storage: the reduced White-Bear tensor algebra (fewer operations)
+      const double phi = fmt_phi_grad_t<double, sizeof(S) == 4>(fpk, w, gg);
       acc[2] += phi;
       // adjoint of the T_zz identity
       gg[1] += gg[10];

@@ -121,17 +121,20 @@ __device__ __forceinline__ void dft4(cx<R>* v, int sg) {
   v[3] = csub(a1, b1);
 }
 
-template <typename R>
+// PAIR = false applies 1/sqrt2 as one rounded multiply (the register-resident
+// slot transforms, whose twiddles are single rounded values too).
+template <typename R, bool PAIR = true>
 __device__ __forceinline__ void dft8(cx<R>* v, int sg) {
   cx<R> e[4] = {v[0], v[2], v[4], v[6]};
   cx<R> o[4] = {v[1], v[3], v[5], v[7]};
   dft4(e, sg);
   dft4(o, sg);
   constexpr double h = 0.70710678118654752440084436210484903928;
+  auto kh = [](R x) { return PAIR ? kmul<R>(x, h) : x * static_cast<R>(h); };
   // w8^1 = (1 + sg i)/sqrt2, w8^2 = sg i, w8^3 = (-1 + sg i)/sqrt2
-  cx<R> o1 = {kmul<R>(o[1].re - R(sg) * o[1].im, h), kmul<R>(o[1].im + R(sg) * o[1].re, h)};
+  cx<R> o1 = {kh(o[1].re - R(sg) * o[1].im), kh(o[1].im + R(sg) * o[1].re)};
   cx<R> o2 = cmul_si(o[2], sg);
-  cx<R> o3 = {kmul<R>(-o[3].re - R(sg) * o[3].im, h), kmul<R>(-o[3].im + R(sg) * o[3].re, h)};
+  cx<R> o3 = {kh(-o[3].re - R(sg) * o[3].im), kh(-o[3].im + R(sg) * o[3].re)};
   v[0] = cadd(e[0], o[0]);
   v[4] = csub(e[0], o[0]);
   v[1] = cadd(e[1], o1);
@@ -173,13 +176,13 @@ __device__ __forceinline__ void dft5(cx<R>* v, int sg) {
   v[3] = csub(a2, b2);
 }
 
-template <typename R, int RAD>
+template <typename R, int RAD, bool PAIR = true>
 __device__ __forceinline__ void dft_r(cx<R>* v, int sg) {
   if constexpr (RAD == 2) dft2(v);
   else if constexpr (RAD == 3) dft3(v, sg);
   else if constexpr (RAD == 4) dft4(v, sg);
   else if constexpr (RAD == 5) dft5(v, sg);
-  else dft8(v, sg);
+  else dft8<R, PAIR>(v, sg);
 }
 
 // One in-place Stockham pass over `nl` lines held in one LDS buffer: every

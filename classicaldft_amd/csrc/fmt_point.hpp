@@ -136,7 +136,9 @@ struct Phi3 {
   T p3, dn2, dv[3], dT[6];
   bool tensor;
 };
-template <typename T>
+// REDUCED evaluates the White-Bear I/II tensor term in its reduced algebraic
+// form (same values, different rounding; the mixed-precision kernels use it).
+template <typename T, bool REDUCED = false>
 __host__ __device__ __forceinline__ Phi3<T> fmt_phi3_t(const FmtParams& P, T n2, const T* v, T vv, const T* Tm) {
   const double PI = 3.14159265358979323846;
   Phi3<T> r;
@@ -172,17 +174,28 @@ __host__ __device__ __forceinline__ Phi3<T> fmt_phi3_t(const FmtParams& P, T n2,
                      c * v[0] + e * v[1] + f * v[2]};
     const T qf = v[0] * Tv[0] + v[1] * Tv[1] + v[2] * Tv[2];
     const double k24 = 1.0 / (24.0 * PI);
-    const T n23 = n2 * n2 * n2;
-    r.p3 = k24 * (A * (n23 - 3.0 * n2 * vv + 3.0 * qf - trT3) + B * (n23 - 3.0 * n2 * trT2 + 2.0 * trT3));
-    r.dn2 = k24 * (A * (3.0 * n2 * n2 - 3.0 * vv) + B * (3.0 * n2 * n2 - 3.0 * trT2));
-    for (int k = 0; k < 3; ++k) r.dv[k] = k24 * A * (-6.0 * n2 * v[k] + 6.0 * Tv[k]);
     const double k8 = 1.0 / (8.0 * PI), k4 = 1.0 / (4.0 * PI);
     const T T2m[6] = {sxx, sxy, sxz, syy, syz, szz};
     const int ii[6] = {0, 0, 0, 1, 1, 2};
     const int jj[6] = {0, 1, 2, 1, 2, 2};
+    if (REDUCED && A == 1.0 && B == -1.0) {
+      // White-Bear I/II: the n2^3 terms cancel, leaving
+      //   Phi3 = (n2 (tr T^2 - v.v) + v.T.v - tr T^3) / (8 pi)
+      // and its derivatives in the same reduced form
+      r.p3 = k8 * (n2 * (trT2 - vv) + qf - trT3);
+      r.dn2 = k8 * (trT2 - vv);
+      for (int k = 0; k < 3; ++k) r.dv[k] = k4 * (Tv[k] - n2 * v[k]);
 #pragma unroll
-    for (int q = 0; q < 6; ++q)
-      r.dT[q] = k8 * A * (v[ii[q]] * v[jj[q]] - T2m[q]) + k4 * B * (-n2 * Tm[q] + T2m[q]);
+      for (int q = 0; q < 6; ++q) r.dT[q] = k8 * (v[ii[q]] * v[jj[q]] - 3.0 * T2m[q] + 2.0 * n2 * Tm[q]);
+    } else {
+      const T n23 = n2 * n2 * n2;
+      r.p3 = k24 * (A * (n23 - 3.0 * n2 * vv + 3.0 * qf - trT3) + B * (n23 - 3.0 * n2 * trT2 + 2.0 * trT3));
+      r.dn2 = k24 * (A * (3.0 * n2 * n2 - 3.0 * vv) + B * (3.0 * n2 * n2 - 3.0 * trT2));
+      for (int k = 0; k < 3; ++k) r.dv[k] = k24 * A * (-6.0 * n2 * v[k] + 6.0 * Tv[k]);
+#pragma unroll
+      for (int q = 0; q < 6; ++q)
+        r.dT[q] = k8 * A * (v[ii[q]] * v[jj[q]] - T2m[q]) + k4 * B * (-n2 * Tm[q] + T2m[q]);
+    }
   }
   return r;
 }
@@ -191,7 +204,7 @@ __host__ __device__ __forceinline__ Phi3<T> fmt_phi3_t(const FmtParams& P, T n2,
 // n_v1 follow by Rosenfeld scaling with the species radius
 // (hard_sphere.hpp:80-86).  Returns Phi and fills g[0..10] with the
 // derivatives already combined for the 11 weights (hard_sphere.hpp:118-134).
-template <typename T>
+template <typename T, bool REDUCED = false>
 __host__ __device__ inline T fmt_phi_grad_t(const FmtParams& P, const T* w, T* g) {
   const double PI = 3.14159265358979323846;
   const T eta = w[0];
@@ -212,7 +225,7 @@ __host__ __device__ inline T fmt_phi_grad_t(const FmtParams& P, const T* w, T* g
 
   T F[3], dF[3];
   f_funcs<T>(P.model, eta, F, dF);
-  const Phi3<T> t = fmt_phi3_t<T>(P, n2, v, vv, w + 5);
+  const Phi3<T> t = fmt_phi3_t<T, REDUCED>(P, n2, v, vv, w + 5);
 
   const T s = n1 * n2 - v0v1;
   const T phi = -n0 * F[0] + s * F[1] + t.p3 * F[2];

@@ -18,9 +18,9 @@
 // Twiddles come from one table holding, for every power-of-two length
 // 4 <= L <= kTwMax, the L values exp(-2 pi i m / L) at offset L - 4 (so the
 // butterflies of a pass read consecutive entries), correctly rounded to the
-// arithmetic type.  (Carrying the FP32 rounding residual as a second product
-// changed the Omega and force errors of a 128^3 FP32 evaluation by under 5%
-// and cost 2.5x the multiplies.)
+// arithmetic type, and so is the radix-8 constant 1/sqrt2.  (Carrying the
+// FP32 rounding residual as a second product changed the Omega and force
+// errors of a 128^3 FP32 evaluation by under 5% and cost 2.5x the multiplies.)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -124,7 +124,7 @@ struct SlotPass {
 #pragma unroll
         for (int q = 1; q < R; ++q) u[i][q] = twiddle<C, L, SG>(u[i][q], q * k * (L / (NS * R)), tw);
       }
-      dft_r<C, R>(u[i], SG);
+      dft_r<C, R, false>(u[i], SG);
     }
     if constexpr (P == NP - 1) {
       // outputs of butterfly b at b + NS q = lane + T (i + NB q)
