@@ -129,6 +129,24 @@ def test_fused_x256_matches_general_schedule(shape, precision):
     assert ra["grand_potential"] == pytest.approx(rb["grand_potential"], rel=1e-12 if precision == "fp64" else 1e-6)
 
 
+def test_slab_256x128x128_mixed_matches_oracle():
+    # the DDFT slab workload's grid and profile (LJ slab, rho_l 0.70 over 0.4 L, rho_v 0.02, 1e-3 noise),
+    # fused mixed precision against the FP64 oracle
+    shape, dx = (256, 128, 128), 0.1
+    rho = W.slab(shape, dx, fraction=0.4, width=0.5, rho_l=0.7, rho_v=0.02, noise=1e-3, axis=0)
+    ctx = _ctx(shape, dx, precision="mixed")
+    assert ctx.schedule == "fused"
+    orc = _orc(shape, dx)
+    mu = orc.bulk_mu(0.02)
+    res, f = ctx.total(rho, mu)
+    ref, fr = orc.total(rho, mu)
+    e = _rel(f, fr)
+    om = abs(res["grand_potential"] - ref[5]) / abs(ref[5])
+    print(f"slab 256x128x128 mixed: Omega rel err {om:.2e}, force rel err {e:.2e}", flush=True)
+    assert om < 1e-6
+    assert e < 2e-5
+
+
 def test_fp64_256x128_plane_uses_general_schedule():
     # its FP64 plane kernels would need more LDS than a CU has
     assert _ctx((256, 128, 64), 0.1).schedule == "general"
