@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "line_fft.hpp"
 
 namespace cdft {
 
@@ -100,7 +101,7 @@ __global__ void __launch_bounds__(256) k_xprop(const cx<S>* __restrict__ SR1, co
       buf[yl * ls + x] = {static_cast<C>(v.re), static_cast<C>(v.im)};
     }
     __syncthreads();
-    fft_inplace<C, EPT>(buf, ty, ls, px, -1, twx);
+    fft_lines<C, EPT>(buf, ty, ls, px, -1, twx);
   }
   // lam/fac layout: [0, nx) x axis, [nx, nx+ny) y axis, [nx+ny, nx+ny+nzh) z axis
   cx<C> wv[EPT];
@@ -141,7 +142,7 @@ __global__ void __launch_bounds__(256) k_xprop(const cx<S>* __restrict__ SR1, co
     }
   }
   __syncthreads();
-  fft_inplace<C, EPT>(buf, ty, ls, px, +1, twx);
+  fft_lines<C, EPT>(buf, ty, ls, px, +1, twx);
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = e / ty, yl = e - x * ty, y = y0 + yl;
     if (y < ny) Sout[off + x * ny + y] = {static_cast<S>(buf[yl * ls + x].re), static_cast<S>(buf[yl * ls + x].im)};

@@ -145,30 +145,32 @@ __device__ __forceinline__ void dft8(cx<R>* v, int sg) {
   v[7] = csub(e[3], o3);
 }
 
-template <typename R>
+template <typename R, bool PAIR = true>
 __device__ __forceinline__ void dft3(cx<R>* v, int sg) {
   constexpr double s3 = 0.86602540378443864676372317075293618347;
+  auto km = [](R x) { return PAIR ? kmul<R>(x, s3) : x * static_cast<R>(s3); };
   cx<R> t = cadd(v[1], v[2]);
   cx<R> dd = cmul_si(csub(v[1], v[2]), sg);
-  cx<R> d = {kmul<R>(dd.re, s3), kmul<R>(dd.im, s3)};
+  cx<R> d = {km(dd.re), km(dd.im)};
   cx<R> m = {v[0].re - R(0.5) * t.re, v[0].im - R(0.5) * t.im};
   v[0] = cadd(v[0], t);
   v[1] = cadd(m, d);
   v[2] = csub(m, d);
 }
 
-template <typename R>
+template <typename R, bool PAIR = true>
 __device__ __forceinline__ void dft5(cx<R>* v, int sg) {
   constexpr double c1 = 0.30901699437494742410229341718281905886;   // cos(2pi/5)
   constexpr double c2 = -0.80901699437494742410229341718281905886;  // cos(4pi/5)
   constexpr double s1 = 0.95105651629515357211643933337938214340;   // sin(2pi/5)
   constexpr double s2 = 0.58778525229247312916870595463907276860;   // sin(4pi/5)
+  auto km = [](R x, double c) { return PAIR ? kmul<R>(x, c) : x * static_cast<R>(c); };
   cx<R> t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
   cx<R> t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
-  cx<R> a1 = {v[0].re + kmul<R>(t1.re, c1) + kmul<R>(t2.re, c2), v[0].im + kmul<R>(t1.im, c1) + kmul<R>(t2.im, c2)};
-  cx<R> a2 = {v[0].re + kmul<R>(t1.re, c2) + kmul<R>(t2.re, c1), v[0].im + kmul<R>(t1.im, c2) + kmul<R>(t2.im, c1)};
-  cx<R> b1 = cmul_si(cx<R>{kmul<R>(t3.re, s1) + kmul<R>(t4.re, s2), kmul<R>(t3.im, s1) + kmul<R>(t4.im, s2)}, sg);
-  cx<R> b2 = cmul_si(cx<R>{kmul<R>(t3.re, s2) - kmul<R>(t4.re, s1), kmul<R>(t3.im, s2) - kmul<R>(t4.im, s1)}, sg);
+  cx<R> a1 = {v[0].re + km(t1.re, c1) + km(t2.re, c2), v[0].im + km(t1.im, c1) + km(t2.im, c2)};
+  cx<R> a2 = {v[0].re + km(t1.re, c2) + km(t2.re, c1), v[0].im + km(t1.im, c2) + km(t2.im, c1)};
+  cx<R> b1 = cmul_si(cx<R>{km(t3.re, s1) + km(t4.re, s2), km(t3.im, s1) + km(t4.im, s2)}, sg);
+  cx<R> b2 = cmul_si(cx<R>{km(t3.re, s2) - km(t4.re, s1), km(t3.im, s2) - km(t4.im, s1)}, sg);
   v[0] = cadd(v[0], cadd(t1, t2));
   v[1] = cadd(a1, b1);
   v[4] = csub(a1, b1);
@@ -176,12 +178,64 @@ __device__ __forceinline__ void dft5(cx<R>* v, int sg) {
   v[3] = csub(a2, b2);
 }
 
+// Radix 6 and 10 (the first pass of the register-resident transforms of
+// lengths 3 * 2^a and 5 * 2^a): the radix-3 / radix-5 transforms of the even
+// and the odd inputs, the odd ones times exp(sg 2 pi i k / r), k < r/2, then
+// one radix-2 stage: X[k] = E[k] + W^k O[k], X[k + r/2] = E[k] - W^k O[k].
+// The constants are single rounded values (these radices serve only the slot
+// transforms, whose twiddles are single rounded values too).
+template <typename R>
+__device__ __forceinline__ void dft6(cx<R>* v, int sg) {
+  constexpr double s3 = 0.86602540378443864676372317075293618347;  // sin(pi/3)
+  cx<R> e[3] = {v[0], v[2], v[4]};
+  cx<R> o[3] = {v[1], v[3], v[5]};
+  dft3<R, false>(e, sg);
+  dft3<R, false>(o, sg);
+  const R ss = static_cast<R>(sg) * static_cast<R>(s3);
+  const cx<R> o1 = cmul(o[1], cx<R>{R(0.5), ss});   // W^1 = (1/2, sg sqrt3/2)
+  const cx<R> o2 = cmul(o[2], cx<R>{R(-0.5), ss});  // W^2 = (-1/2, sg sqrt3/2)
+  v[0] = cadd(e[0], o[0]);
+  v[3] = csub(e[0], o[0]);
+  v[1] = cadd(e[1], o1);
+  v[4] = csub(e[1], o1);
+  v[2] = cadd(e[2], o2);
+  v[5] = csub(e[2], o2);
+}
+
+template <typename R>
+__device__ __forceinline__ void dft10(cx<R>* v, int sg) {
+  constexpr double c36 = 0.80901699437494742410229341718281905886;  // cos(pi/5)
+  constexpr double s36 = 0.58778525229247312916870595463907276860;  // sin(pi/5)
+  constexpr double c72 = 0.30901699437494742410229341718281905886;  // cos(2pi/5)
+  constexpr double s72 = 0.95105651629515357211643933337938214340;  // sin(2pi/5)
+  cx<R> e[5] = {v[0], v[2], v[4], v[6], v[8]};
+  cx<R> o[5] = {v[1], v[3], v[5], v[7], v[9]};
+  dft5<R, false>(e, sg);
+  dft5<R, false>(o, sg);
+  const R sg1 = static_cast<R>(sg);
+  // W^k = exp(sg 2 pi i k / 10), k = 1..4
+  const cx<R> w[4] = {{static_cast<R>(c36), sg1 * static_cast<R>(s36)},
+                      {static_cast<R>(c72), sg1 * static_cast<R>(s72)},
+                      {static_cast<R>(-c72), sg1 * static_cast<R>(s72)},
+                      {static_cast<R>(-c36), sg1 * static_cast<R>(s36)}};
+  v[0] = cadd(e[0], o[0]);
+  v[5] = csub(e[0], o[0]);
+#pragma unroll
+  for (int k = 1; k < 5; ++k) {
+    const cx<R> t = cmul(o[k], w[k - 1]);
+    v[k] = cadd(e[k], t);
+    v[k + 5] = csub(e[k], t);
+  }
+}
+
 template <typename R, int RAD, bool PAIR = true>
 __device__ __forceinline__ void dft_r(cx<R>* v, int sg) {
   if constexpr (RAD == 2) dft2(v);
-  else if constexpr (RAD == 3) dft3(v, sg);
+  else if constexpr (RAD == 3) dft3<R, PAIR>(v, sg);
   else if constexpr (RAD == 4) dft4(v, sg);
-  else if constexpr (RAD == 5) dft5(v, sg);
+  else if constexpr (RAD == 5) dft5<R, PAIR>(v, sg);
+  else if constexpr (RAD == 6) dft6(v, sg);
+  else if constexpr (RAD == 10) dft10(v, sg);
   else dft8<R, PAIR>(v, sg);
 }
 
@@ -249,7 +303,7 @@ __device__ __forceinline__ void pass_ip(cx<C>* buf, int nl, int ls, int L, int n
 // thread (nl * L <= MAXE * blockDim.x).  The caller synchronises before the
 // call (inputs visible); the call ends with a barrier (outputs visible).
 template <typename C, int MAXE>
-__device__ void fft_inplace(cx<C>* buf, int nl, int ls, const FftPlan& p, int sg, const cx<C>* __restrict__ tw) {
+__device__ __forceinline__ void fft_inplace(cx<C>* buf, int nl, int ls, const FftPlan& p, int sg, const cx<C>* __restrict__ tw) {
   int ns = 1;
   for (int q = 0; q < p.np; ++q) {
     const int r = p.rad[q];

@@ -37,6 +37,7 @@
 
 #include "device_common.hpp"
 #include "fft_lds.hpp"
+#include "line_fft.hpp"
 #include "fmt_point.hpp"
 #include "kernels.hpp"
 
@@ -116,7 +117,7 @@ __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S
     dst[ll & 1] = val;
   }
   __syncthreads();
-  fft_inplace<C, 8>(buf, nlb >> 1, nz, pz, -1, twz);
+  fft_lines<C, 8>(buf, nlb >> 1, nz, pz, -1, twz);
   for (int e = threadIdx.x; e < nlb * nzh; e += blockDim.x) {
     const int kz = dnlb.div(e), ll = e - kz * nlb;
     const long l = l0 + ll;
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
   __syncthreads();
   build_c2r_lines(stage, buf, nlb >> 1, nz, nzh);
   __syncthreads();
-  fft_inplace<C, 8>(buf, nlb >> 1, nz, pz, +1, twz);
+  fft_lines<C, 8>(buf, nlb >> 1, nz, pz, +1, twz);
   double* oc = out + static_cast<long>(ch) * g.N;
   const double* ac = add ? add + static_cast<long>(ch) * g.N : nullptr;
   const double cst = shift_sum ? dc.v[ch] * shift_sum[0] * g.inv_n : 0.0;
@@ -208,7 +209,7 @@ __global__ void __launch_bounds__(NT) k_lines(cx<S>* __restrict__ Sp, long nline
   cx<S>* base = Sp + line0 * L;
   for (int e = threadIdx.x; e < nl * L; e += blockDim.x) buf[e] = cvt<C>(base[e]);
   __syncthreads();
-  fft_inplace<C, 8>(buf, nl, L, p, sg, tw);
+  fft_lines<C, 8>(buf, nl, L, p, sg, tw);
   for (int e = threadIdx.x; e < nl * L; e += blockDim.x) base[e] = cvt<S>(buf[e]);
 }
 
@@ -229,7 +230,7 @@ __global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, F
     buf[yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
   }
   __syncthreads();
-  fft_inplace<C, MAXE>(buf, ty, ls, px, sg, twx);
+  fft_lines<C, MAXE>(buf, ty, ls, px, sg, twx);
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
     if (y < ny) plane[x * ny + y] = cvt<S>(buf[yl * ls + x]);
@@ -257,7 +258,7 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
     buf[yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
   }
   __syncthreads();
-  fft_inplace<C, EPT>(buf, ty, ls, px, -1, twx);
+  fft_lines<C, EPT>(buf, ty, ls, px, -1, twx);
   cx<C> rh[EPT];
   WRad wr[EPT];
   double wm[EPT];
@@ -300,6 +301,9 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
       }
     }
     __syncthreads();
+    // The LDS-pass transform: with the lane's spectrum and weights live
+    // across the channel loop, the register-resident one measured slower
+    // here (160^3: 502 against 308 us per launch).
     fft_inplace<C, EPT * G>(buf, ng * ty, ls, px, +1, twx);
     for (int gi = 0; gi < ng; ++gi) {
       cx<S>* outp = SF + static_cast<long>(c0 + gi) * g.M + plane_off;
@@ -354,7 +358,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
       }
     }
     __syncthreads();
-    fft_inplace<C, EPT * G>(buf, ng * ty, ls, px, -1, twx);
+    fft_lines<C, EPT * G>(buf, ng * ty, ls, px, -1, twx);
     for (int gi = 0; gi < ng; ++gi) {
 #pragma unroll
       for (int m = 0; m < EPT; ++m) {
@@ -378,7 +382,7 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
     }
   }
   __syncthreads();
-  fft_inplace<C, EPT>(buf, ty, ls, px, +1, twx);
+  fft_lines<C, EPT>(buf, ty, ls, px, +1, twx);
   cx<S>* outp = SO + plane_off;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
@@ -423,7 +427,8 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
   const int hs = nlb * nzh;                // staged half-spectrum elements per channel
   S wv[NFA][PPT];
 
-#pragma unroll
+// fully unrolled: wv is indexed by the channel, which must stay a constant
+#pragma clang loop unroll(full)
   for (int c0 = 0; c0 < NF; c0 += ZP_G) {
     const int ng = min(ZP_G, NF - c0);
     for (int gi = 0; gi < ng; ++gi) {
@@ -437,7 +442,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
     __syncthreads();
     for (int gi = 0; gi < ng; ++gi) build_c2r_lines(stage + gi * hs, fbuf + gi * zs, np, nz, nzh);
     __syncthreads();
-    fft_inplace<CF, ZP_G * PPT / 2>(fbuf, ng * np, nz, pz, +1, twf);
+    fft_lines<CF, ZP_G * PPT / 2>(fbuf, ng * np, nz, pz, +1, twf);
 #pragma unroll
     for (int gi = 0; gi < ZP_G; ++gi) {
       const int c = c0 + gi;
@@ -525,7 +530,8 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
   }
 
   // forward z R2C of the combined derivatives, ZP_G channels per batched transform
-#pragma unroll
+// fully unrolled: wv is indexed by the channel, which must stay a constant
+#pragma clang loop unroll(full)
   for (int c0 = 0; c0 < NB; c0 += ZP_G) {
     const int ng = min(ZP_G, NB - c0);
 #pragma unroll
@@ -544,7 +550,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
       }
     }
     __syncthreads();
-    fft_inplace<CB, ZP_G * PPT / 2>(bbuf, ng * np, nz, pz, -1, twb);
+    fft_lines<CB, ZP_G * PPT / 2>(bbuf, ng * np, nz, pz, -1, twb);
     for (int gi = 0; gi < ng; ++gi) {
       cx<S>* outc = SF + static_cast<long>(c0 + gi) * g.M;
       for (int e = threadIdx.x; e < hs; e += blockDim.x) {

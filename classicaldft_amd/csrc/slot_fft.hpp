@@ -42,21 +42,28 @@ __device__ __forceinline__ void wave_sync() {
 }
 
 // Padded position inside a slot buffer: one spare element every 16 spreads
-// the strided writes of the early passes over the LDS banks.
+// the strided writes of the early passes over the LDS banks.  (Transforms
+// that exchange through the line's own storage use the unpadded position.)
 __host__ __device__ constexpr int xpad(int x) { return x + (x >> 4); }
+template <bool PAD>
+__host__ __device__ constexpr int xpos(int x) {
+  return PAD ? xpad(x) : x;
+}
 template <int L>
 __host__ __device__ constexpr int slot_buf_len() {
   return xpad(L - 1) + 1;
 }
 
 // Radix sequence of an (L, T) slot: radix E first, then the largest of 8, 4,
-// 2 that divides both E and the remaining length.
+// 2 that divides both E and the remaining length.  E is 2, 4 or 8, or 6 or
+// 10 (lengths 3 * 2^a and 5 * 2^a, whose later passes are radix 2); L / E is
+// a power of two.
 template <int L, int T>
 struct SlotPlan {
   static_assert(L % T == 0, "slot width must divide the line length");
   static constexpr int E = L / T;
-  static_assert(E == 2 || E == 4 || E == 8, "2, 4 or 8 elements per lane");
-  static_assert((L & (L - 1)) == 0, "power-of-two line length");
+  static_assert(E == 2 || E == 4 || E == 8 || E == 6 || E == 10, "2, 4, 6, 8 or 10 elements per lane");
+  static_assert((T & (T - 1)) == 0, "power-of-two slot width");
   static constexpr int rad(int p) {
     int rem = L;
     for (int i = 0;; ++i) {
@@ -105,7 +112,7 @@ __host__ __device__ constexpr int slot_buf_bytes() {
   return slot_buf_len<L>() * SlotXchg<C, SPLIT>::bytes;
 }
 
-template <typename C, int L, int T, int SG, bool SPLIT, int P, int NS>
+template <typename C, int L, int T, int SG, bool SPLIT, int P, int NS, bool PAD = true>
 struct SlotPass {
   using Plan = SlotPlan<L, T>;
   static constexpr int E = Plan::E;
@@ -120,7 +127,7 @@ struct SlotPass {
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       if constexpr (NS > 1) {
-        const int k = (lane + T * i) & (NS - 1);
+        const int k = (lane + T * i) % NS;
 #pragma unroll
         for (int q = 1; q < R; ++q) u[i][q] = twiddle<C, L, SG>(u[i][q], q * k * (L / (NS * R)), tw);
       }
@@ -133,7 +140,7 @@ struct SlotPass {
 #pragma unroll
         for (int q = 0; q < R; ++q) v[i + NB * q] = u[i][q];
     } else {
-      using Next = SlotPass<C, L, T, SG, SPLIT, P + 1, NS * R>;
+      using Next = SlotPass<C, L, T, SG, SPLIT, P + 1, NS * R, PAD>;
       cx<C> w[Next::NB][Next::R];
       // outputs of butterfly b at (b/NS) NS R + b%NS + NS q; inputs of the next pass at b' + (L/R') q'
       auto put = [&](auto get, auto* buf) {
@@ -141,9 +148,9 @@ struct SlotPass {
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
           const int b = lane + T * i;
-          const int base = (b / NS) * NS * R + (b & (NS - 1));
+          const int base = (b / NS) * NS * R + b % NS;
 #pragma unroll
-          for (int q = 0; q < R; ++q) buf[xpad(base + NS * q)] = get(u[i][q]);
+          for (int q = 0; q < R; ++q) buf[xpos<PAD>(base + NS * q)] = get(u[i][q]);
         }
         wave_sync();
       };
@@ -153,19 +160,19 @@ struct SlotPass {
 #pragma unroll
         for (int i = 0; i < Next::NB; ++i)
 #pragma unroll
-          for (int q = 0; q < Next::R; ++q) w[i][q].re = xr[xpad(lane + T * i + (L / Next::R) * q)];
+          for (int q = 0; q < Next::R; ++q) w[i][q].re = xr[xpos<PAD>(lane + T * i + (L / Next::R) * q)];
         put([](cx<C> a) { return a.im; }, xr);
 #pragma unroll
         for (int i = 0; i < Next::NB; ++i)
 #pragma unroll
-          for (int q = 0; q < Next::R; ++q) w[i][q].im = xr[xpad(lane + T * i + (L / Next::R) * q)];
+          for (int q = 0; q < Next::R; ++q) w[i][q].im = xr[xpos<PAD>(lane + T * i + (L / Next::R) * q)];
       } else {
         cx<C>* xc = static_cast<cx<C>*>(xb);
         put([](cx<C> a) { return a; }, xc);
 #pragma unroll
         for (int i = 0; i < Next::NB; ++i)
 #pragma unroll
-          for (int q = 0; q < Next::R; ++q) w[i][q] = xc[xpad(lane + T * i + (L / Next::R) * q)];
+          for (int q = 0; q < Next::R; ++q) w[i][q] = xc[xpos<PAD>(lane + T * i + (L / Next::R) * q)];
       }
       Next::run(v, w, xb, lane, tw);
     }
@@ -175,7 +182,7 @@ struct SlotPass {
 // One line transform by one slot; `xb` is the slot's buffer of
 // slot_buf_bytes<C, L, SPLIT>() bytes, `tw` the length-L twiddle table
 // (tw_for<L> of the concatenated tables).  All lanes of the slot call it together.
-template <typename C, int L, int T, int SG, bool SPLIT = false>
+template <typename C, int L, int T, int SG, bool SPLIT = false, bool PAD = true>
 __device__ __forceinline__ void slot_fft(cx<C> (&v)[L / T], void* xb, int lane, const cx<C>* __restrict__ tw) {
   constexpr int E = L / T;
   // The exchange addresses depend only on the lane; left visible, the
@@ -186,7 +193,7 @@ __device__ __forceinline__ void slot_fft(cx<C> (&v)[L / T], void* xb, int lane, 
   cx<C> u[1][E];  // first pass: radix E, butterfly b = lane reads x[lane + T q] = v[q]
 #pragma unroll
   for (int q = 0; q < E; ++q) u[0][q] = v[q];
-  SlotPass<C, L, T, SG, SPLIT, 0, 1>::run(v, u, xb, lane, tw);
+  SlotPass<C, L, T, SG, SPLIT, 0, 1, PAD>::run(v, u, xb, lane, tw);
 }
 
 }  // namespace cdft

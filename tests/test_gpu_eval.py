@@ -32,7 +32,8 @@ def _relmax(a, b):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("mixed", 2e-6)])
-@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 10, 4), (12, 8, 10), (16, 24, 20), (32, 32, 32), (5, 9, 15)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (6, 10, 4), (12, 8, 10), (16, 24, 20), (32, 32, 32), (5, 9, 15),
+                                   (48, 80, 96), (160, 64, 128)])
 def test_fft_r2c_c2r(shape, precision, tol):
     rng = np.random.default_rng(11)
     ctx = _ctx(shape, 0.1, model="none", inter=(), precision=precision)
@@ -47,6 +48,29 @@ def test_fft_r2c_c2r(shape, precision, tol):
     yr = orc.c2r(Z)
     print(f"c2r {shape} {precision}: {_relmax(y, yr):.3e}", flush=True)
     assert _relmax(y, yr) < tol
+
+
+# Line lengths 48, 64, 80, 96, 128 and 160 of the general schedule run as
+# register-resident slot transforms (radix 6 or 10 first, then radix 2; or
+# radix 8 / 4); every axis of these grids takes that path.
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(48, 80, 96), (160, 96, 48), (96, 48, 160)])
+def test_general_schedule_slot_lengths_match_oracle(shape, precision):
+    dx = 0.1
+    ctx = _ctx(shape, dx, precision=precision)
+    assert ctx.schedule == "general"
+    orc = _orc(shape, dx)
+    rho = W.droplet(shape, dx, radius=1.5, width=0.4, rho_l=0.7, rho_v=0.02, noise=1e-3, seed=21)
+    mu = orc.bulk_mu(0.02)
+    res, f = ctx.total(rho, mu)
+    ref, fr = orc.total(rho, mu)
+    e = _relmax(f, fr)
+    om = abs(res["grand_potential"] - ref[5]) / abs(ref[5])
+    print(f"general {shape} {precision}: Omega rel err {om:.2e}, force rel err {e:.2e}", flush=True)
+    if precision == "fp64":
+        assert om < 1e-11 and e < 1e-11
+    else:
+        assert om < 1e-6 and e < 2e-5
 
 
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
