@@ -232,6 +232,29 @@ def ddft_slab(cd, W, steps=2000, shape=(256, 128, 128), dt=1e-4, precision="mixe
     }
 
 
+def general_sizes(cd, W, sizes, precision="mixed", evals=20):
+    """Per-stage times of an evaluation on cubic grids of the given sizes (general schedule for sizes the fused
+    kernels do not cover): the config-5 LJ droplet settings, autotuned first, `evals` timed evaluations."""
+    import torch
+
+    out = {}
+    for n in sizes:
+        shape = (n, n, n)
+        ctx = cd.Context(shape, 0.1, interactions=[W.LJ_WCA], kT=0.8, model="WhiteBearII", precision=precision)
+        t_def, t_tun = ctx.autotune(reps=3)
+        d = torch.tensor(W.droplet(shape, 0.1, radius=1.5), device="cuda")
+        f = torch.empty_like(d)
+        mu = ctx.bulk_mu(0.02)
+        ctx.time_total_device(d.data_ptr(), mu, 2, d_forces=f.data_ptr())
+        ms = ctx.time_total_device(d.data_ptr(), mu, evals, d_forces=f.data_ptr())
+        stages = ctx.profile_stages(d.data_ptr(), mu, reps=5, d_forces=f.data_ptr())
+        out[str(n)] = {"schedule": ctx.schedule, "precision": precision, "us_per_eval": 1000.0 * ms / evals,
+                       "autotune_us_default_tuned": [t_def, t_tun],
+                       "stages_us": [[name, round(1000.0 * t, 2)] for name, t, _ in stages]}
+        progress(f"general sizes: {n}^3 {1000.0 * ms / evals:.1f} us per evaluation")
+    return out
+
+
 def stream_copy_gbps(reps=5):
     """Device-to-device copy of 1 GiB of doubles (reads 1 GiB, writes 1 GiB), timed with HIP events on
     torch's current stream; median of `reps` copies, in GB/s of bytes moved."""
@@ -377,6 +400,8 @@ def main():
     ap.add_argument("--reps", type=int, default=5, help="timed repetitions of K evaluations (median reported)")
     ap.add_argument("--batch-points", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--general-sizes", default="",
+                    help="comma-separated cube sizes whose per-stage evaluation times are added (e.g. 48,80,96,160)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
@@ -522,6 +547,9 @@ def main():
     if args.ddft_steps > 0:
         progress("ddft slab")
         out["ddft_slab"] = ddft_slab(cd, W, steps=args.ddft_steps)
+
+    if args.general_sizes:
+        out["general_sizes"] = general_sizes(cd, W, [int(v) for v in args.general_sizes.split(",") if v])
 
     if not args.no_mixture:
         progress("mixture")

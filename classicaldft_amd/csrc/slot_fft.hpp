@@ -127,7 +127,7 @@ struct SlotPass {
 #pragma unroll
     for (int i = 0; i < NB; ++i) {
       if constexpr (NS > 1) {
-        const int k = (lane + T * i) % NS;
+        const int k = static_cast<int>(static_cast<unsigned>(lane + T * i) % NS);  // unsigned: a mask for 2^a
 #pragma unroll
         for (int q = 1; q < R; ++q) u[i][q] = twiddle<C, L, SG>(u[i][q], q * k * (L / (NS * R)), tw);
       }
@@ -148,7 +148,8 @@ struct SlotPass {
 #pragma unroll
         for (int i = 0; i < NB; ++i) {
           const int b = lane + T * i;
-          const int base = (b / NS) * NS * R + b % NS;
+          const unsigned ub = static_cast<unsigned>(b);
+          const int base = static_cast<int>((ub / NS) * NS * R + ub % NS);
 #pragma unroll
           for (int q = 0; q < R; ++q) buf[xpos<PAD>(base + NS * q)] = get(u[i][q]);
         }
