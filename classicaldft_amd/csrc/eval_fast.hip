@@ -849,16 +849,9 @@ __global__ void __launch_bounds__(256) kf_wtable(cx<double>* __restrict__ WT, Fa
     WT[i] = {w3, w2};
   }
 }
-// Weight of channel ch (weights.hpp:110-137,164-178; `adj` conjugates it, so the odd
-// vector channels change sign, as the back-convolution does,
-// hard_sphere.hpp:118-134) as a linear form in the element's weight factors:
-// ch 0 w3, 1 w2, 2..4 i k_a w3, 5..10 aik2 k_i k_j + delta_ij w3/R, i.e.
-//   W = (a3 w3 + a2 w2 + at aik2 k_i k_j) + i av k_i w3,
-//   k_i = ix kx + iy ky + iz kz,  k_j = jx kx + jy ky + jz kz,
-// whose coefficients depend on the channel only: set up once per channel,
-// it is applied to every element without branching on the channel.
-// Forward weight of channel ch for one element, by cases (the forward plane
-// kernel keeps its per-channel state in fewer registers this way).
+// Forward weight of channel ch for one element (weights.hpp:110-137,164-178):
+// ch 0 w3, 1 w2, 2..4 i k_a w3, 5..10 aik2 k_i k_j + delta_ij w3/R, by cases
+// on the channel (uniform over a block).
 template <typename C>
 __device__ __forceinline__ cx<C> chan_weight(int ch, C kx, C ky, C kz, const WFac<C>& f, C invR) {
   if (ch == 0) return {f.w3, C(0)};
@@ -870,6 +863,12 @@ __device__ __forceinline__ cx<C> chan_weight(int ch, C kx, C ky, C kz, const WFa
   const bool diag = (q == 0 || q == 3 || q == 5);
   return {f.aik2 * ki * kj + (diag ? f.w3 * invR : C(0)), C(0)};
 }
+
+// The FP64 backward plane kernel's channel weight as a linear form in the
+// element's weight factors: W = (a3 w3 + a2 w2 + at aik2 k_i k_j) + i av k_i w3,
+// k_i = ix kx + iy ky + iz kz, k_j = jx kx + jy ky + jz kz, coefficients set up
+// once per channel (`adj` conjugates it: the odd vector channels change sign,
+// hard_sphere.hpp:118-134).
 template <typename C>
 struct ChanForm {
   C a3, a2, at, av, ix, iy, iz, jx, jy, jz;
@@ -1329,11 +1328,21 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         for (int q = 0; q < 4; ++q) nxt[rr][q] = in[loff[rr][q]];
     };
     load_in(c0);
-    // weight factors of the lane's accumulator elements (k = xl + TX e, kx = 2 k + p,
-    // column xs + SBX cc), computed in FP64 and kept in the backward path's precision
-    WFac<CV> wf[CPS][4];
+    // The conj-weighted channel sum (hard_sphere.hpp:115-134) is accumulated as
+    //   S = w3 A + w2 B + aik2 C,
+    //   A = G_eta - i kx G_vx - i ky G_vy + (G_Txx + G_Tyy) / R,   B = G_n2,
+    //   C = kx^2 G_Txx + kx ky G_Txy + kx kz G_Txz + ky^2 G_Tyy + ky kz G_Tyz,
+    // since every channel weight is linear in (w3, w2, aik2) (weights.hpp:110-137,164-178): each
+    // channel adds its spectrum to one or two sums with a coefficient in k only, and the radial
+    // factors multiply the three sums once per unit.  Element e of the lane: k = xl + TX e
+    // (kx = 2 k + p), column xs + SBX cc.
+    // FP64 storage keeps one accumulator and applies each channel's full weight
+    // (three FP64 sums would spill at the 128-VGPR cap of the 1024-thread block).
+    constexpr bool SUMS = sizeof(CV) == 4;
+    constexpr int NA = SUMS ? 1 : 0;  // extent of the arrays only one form uses
     CV kxc[4];
-    cx<CV> acc[CPS][4];
+    cx<CV> accA[CPS][4], accB[NA * CPS + 1 - NA][4], accC[NA * CPS + 1 - NA][4];
+    WFac<CV> wf[(1 - NA) * CPS + NA][4];
     int xlo = xl;  // opaque: keeps the per-element set-up inside the unit loop
     asm volatile("" : "+v"(xlo));
     const cx<double>* wu = WT + static_cast<long>(unit) * Q * NXH + xlo;
@@ -1342,12 +1351,16 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
 #pragma unroll
     for (int cc = 0; cc < CPS; ++cc) {
       const int col = xs + PC::SBX * cc;
-      const double kyv = kvec(r + 2 * (col < Q ? col : 0), PC::NY, g.dky);
+      const double kyd = kvec(r + 2 * (col < Q ? col : 0), PC::NY, g.dky);
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        wf[cc][e] = wfactors<CV>(wu[static_cast<long>(col < Q ? col : 0) * NXH + PC::TX * e],
-                                 kvec(2 * (xlo + PC::TX * e) + p, NX, g.dkx), kyv, kzv, 1.0 / g.R);
-        acc[cc][e] = {CV(0), CV(0)};
+        accA[cc][e] = {CV(0), CV(0)};
+        if constexpr (SUMS) {
+          accB[cc][e] = accC[cc][e] = {CV(0), CV(0)};
+        } else {
+          wf[cc][e] = wfactors<CV>(wu[static_cast<long>(col < Q ? col : 0) * NXH + PC::TX * e],
+                                   kvec(2 * (xlo + PC::TX * e) + p, NX, g.dkx), kyd, kzv, 1.0 / g.R);
+        }
       }
     }
     const CV kzc = static_cast<CV>(kzv);
@@ -1375,8 +1388,9 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
         }
       }
       __syncthreads();
-      const ChanForm<CV> form = chan_form<CV>(bwd_channel(ch), true, invR);
-      // x-FFT of the sub-plane's columns, conj-weighted accumulation
+      const int chan = bwd_channel(ch);  // the same for the whole block: uniform branches below
+      const ChanForm<CV> form = chan_form<CV>(chan, true, invR);
+      // x-FFT of the sub-plane's columns, accumulation into A, B, C
 #pragma unroll
       for (int cc = 0; cc < CPS; ++cc) {
         const int col = xs + PC::SBX * cc;
@@ -1387,8 +1401,33 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
           for (int q = 0; q < 4; ++q) f[q] = cvt<CV>(tb[col * NXHP + xl + PC::TX * q]);
           slot_fft<CV, NXH, PC::TX, -1, SPL>(f, xbx + xs * slot_buf_len<NXH>() * PC::XBYTES, xl, tw_for<NXH>(twl));
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            acc[cc][e] = cadd(acc[cc][e], cmul(apply_form<CV>(form, kxc[e], kyv, kzc, wf[cc][e]), f[e]));
+          for (int e = 0; e < 4; ++e) {
+            const cx<CV> G = f[e];
+            const CV kx = kxc[e];
+            cx<CV>& A = accA[cc][e];
+            if constexpr (!SUMS) {
+              A = cadd(A, cmul(apply_form<CV>(form, kx, kyv, kzc, wf[cc][e]), G));
+              continue;
+            }
+            cx<CV>& C = accC[cc][e];
+            switch (chan) {
+              case 0: A = cadd(A, G); break;                                   // eta: w3
+              case 1: accB[cc][e] = cadd(accB[cc][e], G); break;               // n2: w2
+              case 2: A = {A.re + kx * G.im, A.im - kx * G.re}; break;         // v_x: conj(i kx w3)
+              case 3: A = {A.re + kyv * G.im, A.im - kyv * G.re}; break;       // v_y
+              case 5:                                                          // T_xx
+                C = {C.re + (kx * kx) * G.re, C.im + (kx * kx) * G.im};
+                A = {A.re + invR * G.re, A.im + invR * G.im};
+                break;
+              case 6: C = {C.re + (kx * kyv) * G.re, C.im + (kx * kyv) * G.im}; break;  // T_xy
+              case 7: C = {C.re + (kx * kzc) * G.re, C.im + (kx * kzc) * G.im}; break;  // T_xz
+              case 8:                                                                   // T_yy
+                C = {C.re + (kyv * kyv) * G.re, C.im + (kyv * kyv) * G.im};
+                A = {A.re + invR * G.re, A.im + invR * G.im};
+                break;
+              default: C = {C.re + (kyv * kzc) * G.re, C.im + (kyv * kzc) * G.im}; break;  // T_yz
+            }
+          }
         }
       }
       // With DB the next channel writes the other transpose buffer and other
@@ -1397,6 +1436,28 @@ __global__ void __launch_bounds__(kBT) kf_plane_bwd(const cx<S>* __restrict__ SB
       if constexpr (!BC::DB) __syncthreads();
     }
     __syncthreads();
+    // FP32: S = w3 A + w2 B + aik2 C with the radial factors of the lane's elements (FP64
+    // table, formed in the backward path's precision); FP64: S is the accumulator
+    cx<CV> acc[CPS][4];
+    if constexpr (!SUMS) {
+#pragma unroll
+      for (int cc = 0; cc < CPS; ++cc)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[cc][e] = accA[cc][e];
+    } else {
+#pragma unroll
+      for (int cc = 0; cc < CPS; ++cc) {
+        const int col = xs + PC::SBX * cc;
+        const double kyd = kvec(r + 2 * (col < Q ? col : 0), PC::NY, g.dky);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const WFac<CV> w = wfactors<CV>(wu[static_cast<long>(col < Q ? col : 0) * NXH + PC::TX * e],
+                                          kvec(2 * (xlo + PC::TX * e) + p, NX, g.dkx), kyd, kzv, 1.0 / g.R);
+          const cx<CV> a = accA[cc][e], b = accB[cc][e], c = accC[cc][e];
+          acc[cc][e] = {w.w3 * a.re + w.w2 * b.re + w.aik2 * c.re, w.w3 * a.im + w.w2 * b.im + w.aik2 * c.im};
+        }
+      }
+    }
     // x-inverse of the accumulated sub-plane, then y-inverse of every row
     cx<S>* tbf = reinterpret_cast<cx<S>*>(smem);
 #pragma unroll
