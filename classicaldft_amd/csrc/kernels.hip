@@ -92,6 +92,68 @@ __device__ __forceinline__ void build_c2r_lines(const cx<T>* __restrict__ stage,
   }
 }
 
+// C2R of staged half spectra of ng groups of np line pairs (group gi at
+// stage + gi * hs, line ll of it at ll * nzh): the lines are built (as
+// build_c2r_lines does) straight into the registers of a slot of T lanes,
+// transformed there, and written to buf[(gi * np + p) * L ...].  Ends with a
+// block barrier; the caller synchronises before (stage visible).
+template <typename C, typename T_, int L, int T>
+__device__ __forceinline__ void slot_c2r_lines(const cx<T_>* __restrict__ stage, int hs, int nzh, cx<C>* buf, int ng,
+                                               int np, const cx<C>* __restrict__ tw) {
+  constexpr int E = L / T;
+  const int nl = ng * np;
+  const int nslot = static_cast<int>(blockDim.x) / T;
+  const int slot = static_cast<int>(threadIdx.x) / T, lane = static_cast<int>(threadIdx.x) % T;
+  for (int j = slot; j < nl; j += nslot) {
+    const int gi = j / np, pr = j - gi * np;
+    const cx<T_>* sa = stage + gi * hs + (2 * pr) * nzh;  // line a; line b follows at + nzh
+    cx<C> v[E];
+#pragma unroll
+    for (int q = 0; q < E; ++q) {
+      const int k = lane + T * q;
+      const bool mirror = k >= nzh;
+      const int kk = mirror ? L - k : k;
+      cx<C> A = cvt<C>(sa[kk]), B = cvt<C>(sa[nzh + kk]);
+      if (kk == 0 || 2 * kk == L) {
+        A.im = C(0);
+        B.im = C(0);
+      }
+      if (mirror) {
+        A.im = -A.im;
+        B.im = -B.im;
+      }
+      v[q] = {A.re - B.im, A.im + B.re};
+    }
+    cx<C>* line = buf + static_cast<long>(j) * L;
+    slot_fft<C, L, T, +1, false, false>(v, line, lane, tw);
+    wave_sync();
+#pragma unroll
+    for (int e = 0; e < E; ++e) line[lane + T * e] = v[e];
+    wave_sync();
+  }
+  __syncthreads();
+}
+
+// build_c2r_lines for every group, then the inverse transform of all ng * np
+// lines; the slot lengths of fft_lines build in registers (slot_c2r_lines).
+template <typename C, typename T_, int MAXE>
+__device__ __forceinline__ void c2r_lines(const cx<T_>* __restrict__ stage, int hs, cx<C>* buf, int ng, int np, int nz,
+                                          int nzh, const FftPlan& pz, const cx<C>* __restrict__ tw) {
+  switch (pz.L) {
+    case 48: slot_c2r_lines<C, T_, 48, 8>(stage, hs, nzh, buf, ng, np, tw); return;
+    case 64: slot_c2r_lines<C, T_, 64, 8>(stage, hs, nzh, buf, ng, np, tw); return;
+    case 80: slot_c2r_lines<C, T_, 80, 8>(stage, hs, nzh, buf, ng, np, tw); return;
+    case 96: slot_c2r_lines<C, T_, 96, 16>(stage, hs, nzh, buf, ng, np, tw); return;
+    case 128: slot_c2r_lines<C, T_, 128, 16>(stage, hs, nzh, buf, ng, np, tw); return;
+    case 160: slot_c2r_lines<C, T_, 160, 16>(stage, hs, nzh, buf, ng, np, tw); return;
+    default:
+      for (int gi = 0; gi < ng; ++gi) build_c2r_lines(stage + gi * hs, buf + gi * np * nz, np, nz, nzh);
+      __syncthreads();
+      fft_inplace<C, MAXE>(buf, ng * np, nz, pz, +1, tw);
+      return;
+  }
+}
+
 // ---------------------------------------------------------------- z-line kernels
 // Lines l = ix*Ny + iy of the real field are contiguous; a block owns `nlb`
 // consecutive lines (nlb even) and packs line pairs into one complex FFT.
@@ -147,9 +209,7 @@ __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, doubl
     stage[ll * nzh + kz] = (l < g.nxy) ? Sc[static_cast<long>(kz) * g.nxy + l] : cx<S>{S(0), S(0)};
   }
   __syncthreads();
-  build_c2r_lines(stage, buf, nlb >> 1, nz, nzh);
-  __syncthreads();
-  fft_lines<C, 8>(buf, nlb >> 1, nz, pz, +1, twz);
+  c2r_lines<C, S, 8>(stage, 0, buf, 1, nlb >> 1, nz, nzh, pz, twz);
   double* oc = out + static_cast<long>(ch) * g.N;
   const double* ac = add ? add + static_cast<long>(ch) * g.N : nullptr;
   const double cst = shift_sum ? dc.v[ch] * shift_sum[0] * g.inv_n : 0.0;
@@ -440,9 +500,7 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
       }
     }
     __syncthreads();
-    for (int gi = 0; gi < ng; ++gi) build_c2r_lines(stage + gi * hs, fbuf + gi * zs, np, nz, nzh);
-    __syncthreads();
-    fft_lines<CF, ZP_G * PPT / 2>(fbuf, ng * np, nz, pz, +1, twf);
+    c2r_lines<CF, S, ZP_G * PPT / 2>(stage, hs, fbuf, ng, np, nz, nzh, pz, twf);
 #pragma unroll
     for (int gi = 0; gi < ZP_G; ++gi) {
       const int c = c0 + gi;
