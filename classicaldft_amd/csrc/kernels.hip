@@ -32,6 +32,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
+#include <vector>
 #include <cmath>
 #include <stdexcept>
 
@@ -623,6 +625,257 @@ __global__ void __launch_bounds__(NT) k_zpoint(cx<S>* SF, const double* __restri
   }
 }
 
+// ---------------------------------------------------------------- kg_zpoint
+// The point kernel of the general schedule for z lengths 48, 80, 96 and 160,
+// with register-resident z transforms in place of k_zpoint's block-wide LDS
+// passes.  A block of W waves owns LINES consecutive z lines; wave w owns the
+// LPW line pairs 2 (w LPW + lp) + {0, 1}.  A round transforms 4 channels: slot
+// s = cr * LPW + lp of a wave (T lanes, E = L / T elements per lane) runs
+// channel 4 R + cr of line pair lp, so every wave works on its own lines and
+// the block synchronises only around the staging of each round:
+//   forward round: the block loads the 4 channels' half spectra of its lines
+//     (coalesced over the lines) into LDS; each slot builds its packed line
+//     pair in registers, runs the C2R and leaves the lines in its exchange
+//     buffer; each lane then reads its PPL points (line position f = lid + 64 j
+//     of the wave's 2 LPW L values) of every channel of the round;
+//   pointwise: Phi/dPhi, local terms and sums, as k_zpoint;
+//   backward round: each lane writes its points of 4 derivative channels into
+//     the slots' buffers; each slot runs the R2C of its line pair and unpacks
+//     the two half spectra into LDS; the block stores them coalesced.
+template <int L, typename S>
+struct KgCfg {
+  static constexpr int T = (L == 96 || L == 160) ? 16 : 8;
+  static constexpr int E = L / T;
+  static constexpr int NS = 64 / T;   // slots per wave
+  static constexpr int LPW = NS / 4;  // line pairs per wave (4 channels per round)
+  static constexpr int W = 4;
+  static constexpr int NTH = 64 * W;
+  static constexpr int LINES = 2 * LPW * W;
+  static constexpr int NZH = L / 2 + 1;
+  static constexpr int PPL = 2 * LPW * L / 64;  // points per lane
+  static_assert(PPL * 64 == 2 * LPW * L, "the wave's points must divide over its lanes");
+  static constexpr int stage_bytes = (4 * NZH * LINES * static_cast<int>(sizeof(cx<S>)) + 15) / 16 * 16;
+  static constexpr int xb_bytes(int cmax) { return W * NS * L * cmax; }
+};
+
+template <typename S, typename CF, typename CB, int L, int NF, int NB>
+__global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
+                                                              const double* __restrict__ vext,
+                                                              double* __restrict__ local_force,
+                                                              double* __restrict__ partials, Geom g,
+                                                              const cx<CF>* __restrict__ twf,
+                                                              const cx<CB>* __restrict__ twb, FmtParams fp, double mu,
+                                                              AliasMap amap, const double* __restrict__ shift_sum,
+                                                              DcTable dc) {
+  using K = KgCfg<L, S>;
+  constexpr int T = K::T, E = K::E, LPW = K::LPW, LINES = K::LINES, NZH = K::NZH, PPL = K::PPL;
+  constexpr bool HS = NF >= 11;
+  constexpr bool MF = (NF == 12 || NF == 1);
+  constexpr int MFC = HS ? 11 : 0;
+  constexpr int NFA = NF > 0 ? NF : 1;
+  constexpr int CMAX = sizeof(cx<CF>) > sizeof(cx<CB>) ? sizeof(cx<CF>) : sizeof(cx<CB>);
+  constexpr int NRF = (NF + 3) / 4, NRB = (NB + 3) / 4;
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  cx<S>* stage = reinterpret_cast<cx<S>*>(smem_raw);  // [cr][kz][line]
+  const int tid = static_cast<int>(threadIdx.x);
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lid = tid & 63;
+  const int slot = lid / T, lane = lid % T;
+  const int cr = slot / LPW, lp = slot % LPW;
+  unsigned char* xwave = smem_raw + K::stage_bytes + wave * K::NS * L * CMAX;
+  // slot buffer of slot s of this wave (L complex values of the larger type)
+  auto sbuf = [&](int s) { return xwave + s * L * CMAX; };
+  const long l0 = static_cast<long>(blockIdx.x) * LINES;
+  const long nxy = g.nxy;
+  const int la = 2 * (wave * LPW + lp);  // the slot's line pair, block-local
+  S wv[NFA][PPL];
+
+  // ---- forward rounds.  Round R + 1's half spectra are loaded into registers
+  // before round R's transforms, so their memory latency overlaps that work.
+  constexpr int NSTG = 4 * NZH * LINES, KPF = (NSTG + K::NTH - 1) / K::NTH;
+  cx<S> pf[KPF];
+  auto load_round = [&](int R) {
+#pragma unroll
+    for (int k = 0; k < KPF; ++k) {
+      const int idx = tid + K::NTH * k;
+      const int c4 = idx / (NZH * LINES), rem = idx - c4 * (NZH * LINES);
+      const int kz = rem / LINES, ln = rem - kz * LINES;
+      const int c = 4 * R + c4;
+      pf[k] = {S(0), S(0)};
+      if (idx < NSTG && c < NF && l0 + ln < nxy)
+        pf[k] = SF[static_cast<long>(c) * g.M + static_cast<long>(kz) * nxy + l0 + ln];
+    }
+  };
+  auto store_round = [&]() {
+#pragma unroll
+    for (int k = 0; k < KPF; ++k) {
+      const int idx = tid + K::NTH * k;
+      if (idx < NSTG) stage[idx] = pf[k];
+    }
+  };
+  if constexpr (NRF > 0) {
+    load_round(0);
+    store_round();
+    __syncthreads();
+  }
+#pragma unroll
+  for (int R = 0; R < NRF; ++R) {
+    if (R + 1 < NRF) load_round(R + 1);
+    const int c = 4 * R + cr;
+    cx<CF>* xb = reinterpret_cast<cx<CF>*>(sbuf(slot));
+    if (c < NF) {
+      const cx<S>* st = stage + cr * NZH * LINES;
+      cx<CF> v[E];
+#pragma unroll
+      for (int q = 0; q < E; ++q) {
+        const int k = lane + T * q;
+        const bool mirror = k >= NZH;
+        const int kk = mirror ? L - k : k;
+        cx<CF> A = cvt<CF>(st[kk * LINES + la]), B = cvt<CF>(st[kk * LINES + la + 1]);
+        if (kk == 0 || 2 * kk == L) {
+          A.im = CF(0);
+          B.im = CF(0);
+        }
+        if (mirror) {
+          A.im = -A.im;
+          B.im = -B.im;
+        }
+        v[q] = {A.re - B.im, A.im + B.re};
+      }
+      slot_fft<CF, L, T, +1, false, false>(v, xb, lane, twf);
+      wave_sync();
+#pragma unroll
+      for (int e = 0; e < E; ++e) xb[lane + T * e] = v[e];
+    }
+    wave_sync();
+    // the lane's points of the round's channels: position f = lid + 64 j of the
+    // wave's line pairs (line pair f / 2L, z = (f % 2L) / 2, line (f % 2L) & 1)
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+      const int ch = 4 * R + c4;
+      if (ch < NF) {
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+          const int f = lid + 64 * j, pl = f / (2 * L), r2 = f - pl * 2 * L;
+          const CF* src = reinterpret_cast<const CF*>(sbuf(c4 * LPW + pl));
+          wv[ch][j] = static_cast<S>(src[r2]);
+        }
+      }
+    }
+    wave_sync();  // the buffers are rewritten by the next round
+    if (R + 1 < NRF) {
+      __syncthreads();  // every slot is done with this round's staging
+      store_round();
+      __syncthreads();
+    }
+  }
+
+  // ---- pointwise (as k_zpoint)
+  double acc[5] = {0, 0, 0, 0, 0}, cmp[5] = {0, 0, 0, 0, 0};  // ideal, external, hs, mf, mass
+  const double dv = g.dv;
+  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) {
+    const int f = lid + 64 * j, pl = f / (2 * L), r2 = f - pl * 2 * L;
+    const int z = r2 >> 1, line = 2 * (wave * LPW + pl) + (r2 & 1);
+    const long l = l0 + line;
+    S out[11];
+#pragma unroll
+    for (int i = 0; i < 11; ++i) out[i] = S(0);
+    if (l < nxy) {
+      const long p = l * L + z;
+      const double rho = amap.apply(rho_in[p]);
+      const double lr = flog(fmax(rho, 1e-300));
+      double lfv = (lr - mu) * dv;  // ideal_gas.hpp:27-33
+      two_sum(acc[0], cmp[0], rho * (lr - 1.0));
+      two_sum(acc[4], cmp[4], rho);
+      if (vext) {  // external_field.hpp:16-31
+        const double V = vext[p];
+        lfv += V * dv;
+        two_sum(acc[1], cmp[1], rho * V);
+      }
+      if constexpr (MF) {  // mean_field.hpp:259-266 (self pair)
+        const double conv = static_cast<double>(wv[MFC][j]) + dc.v[MFC] * shift;
+        lfv += conv * dv;
+        two_sum(acc[3], cmp[3], 0.5 * rho * conv);
+      }
+      if constexpr (HS) {
+        double w[11], gg[11];
+#pragma unroll
+        for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(wv[i][j]) + dc.v[i] * shift;
+        const double phi = fmt_phi_grad(fp, w, gg);
+        two_sum(acc[2], cmp[2], phi);
+#pragma unroll
+        for (int i = 0; i < 11; ++i) out[i] = static_cast<S>(gg[i]);
+      }
+      local_force[p] = lfv;
+    }
+    if constexpr (HS) {
+#pragma unroll
+      for (int i = 0; i < 11; ++i) wv[i][j] = out[i];
+    }
+  }
+  {
+    double tot[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) tot[i] = acc[i] + cmp[i];
+    __shared__ double out5[5];
+    block_reduce<5>(tot, out5);
+    __syncthreads();
+    if (tid < 8) partials[static_cast<long>(blockIdx.x) * 8 + tid] = tid < 5 ? out5[tid] * dv : 0.0;
+  }
+
+  // ---- backward rounds: R2C of the combined derivatives, written in place over SF's first channels
+#pragma unroll
+  for (int R = 0; R < NRB; ++R) {
+#pragma unroll
+    for (int c4 = 0; c4 < 4; ++c4) {
+      const int ch = 4 * R + c4;
+      if (ch < NB) {
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+          const int f = lid + 64 * j, pl = f / (2 * L), r2 = f - pl * 2 * L;
+          CB* dst = reinterpret_cast<CB*>(sbuf(c4 * LPW + pl));
+          dst[r2] = static_cast<CB>(wv[ch][j]);
+        }
+      }
+    }
+    wave_sync();
+    if (R > 0) __syncthreads();  // the previous round's store is done with the staging buffer
+    const int c = 4 * R + cr;
+    cx<CB>* xb = reinterpret_cast<cx<CB>*>(sbuf(slot));
+    if (c < NB) {
+      cx<CB> v[E];
+#pragma unroll
+      for (int q = 0; q < E; ++q) v[q] = xb[lane + T * q];
+      slot_fft<CB, L, T, -1, false, false>(v, xb, lane, twb);
+      wave_sync();
+#pragma unroll
+      for (int e = 0; e < E; ++e) xb[lane + T * e] = v[e];
+      wave_sync();
+      // unpack the two real lines' half spectra (unpack_pair) into the staging buffer
+      cx<S>* st = stage + cr * NZH * LINES;
+#pragma unroll
+      for (int e = 0; e < E; ++e) {
+        const int k = lane + T * e;
+        if (k < NZH) {
+          const cx<CB> zk = xb[k], zm0 = xb[k == 0 ? 0 : L - k];
+          const cx<CB> zm = {zm0.re, -zm0.im};
+          st[k * LINES + la] = cvt<S>(cx<CB>{CB(0.5) * (zk.re + zm.re), CB(0.5) * (zk.im + zm.im)});
+          st[k * LINES + la + 1] = cvt<S>(cx<CB>{CB(0.5) * (zk.im - zm.im), CB(-0.5) * (zk.re - zm.re)});
+        }
+      }
+    }
+    __syncthreads();
+    const int nc = min(4, NB - 4 * R);
+    for (int idx = tid; idx < nc * NZH * LINES; idx += K::NTH) {
+      const int c4 = idx / (NZH * LINES), rem = idx - c4 * (NZH * LINES);
+      const int kz = rem / LINES, ln = rem - kz * LINES;
+      if (l0 + ln < nxy)
+        SF[static_cast<long>(4 * R + c4) * g.M + static_cast<long>(kz) * nxy + l0 + ln] = stage[idx];
+    }
+  }
+}
+
 // Deterministic compensated sum of per-block partials (nblk x 8 doubles) into res[0..7]:
 // strided Neumaier sums per thread, then a fixed-shape tree of (sum, compensation) pairs.
 // Deterministic compensated column sums of `nrows` rows of 8 partial sums by
@@ -759,9 +1012,21 @@ static int zpoint_nlb(const FftPlan& pz) {
   return nlb;
 }
 
+// z lengths with the register-resident point kernel (kg_zpoint)
+static bool kg_length(int L) { return L == 48 || L == 80 || L == 96 || L == 160; }
+template <typename S>
+static int kg_lines(int L) {
+  switch (L) {
+    case 48: return KgCfg<48, S>::LINES;
+    case 80: return KgCfg<80, S>::LINES;
+    case 96: return KgCfg<96, S>::LINES;
+    default: return KgCfg<160, S>::LINES;
+  }
+}
+
 template <typename S>
 int zpoint_blocks(const Geom& g, const FftPlan& pz) {
-  const int nlb = zpoint_nlb<S>(pz);
+  const int nlb = kg_length(pz.L) ? kg_lines<S>(pz.L) : zpoint_nlb<S>(pz);
   return static_cast<int>((g.nxy + nlb - 1) / nlb);
 }
 template int zpoint_blocks<float>(const Geom&, const FftPlan&);
@@ -776,10 +1041,60 @@ static void zpoint_go(unsigned nblk, size_t sh, hipStream_t st, cx<S>* SF, const
                      g, pz, twf, twb, nlb, fp, mu, amap, shift_sum, dc);
 }
 
+// Dynamic LDS above 64 KiB is enabled per kernel instantiation, once per process.
+template <typename K>
+static void kg_smem(K kernel, int bytes) {
+  static std::mutex mu;
+  static std::vector<const void*> done;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lk(mu);
+  if (std::find(done.begin(), done.end(), key) != done.end()) return;
+  if (hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+    throw std::runtime_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+  done.push_back(key);
+}
+
+template <typename S, typename CF, typename CB, int L, int NF, int NB>
+static void kg_go(hipStream_t st, cx<S>* SF, const double* rho, const double* vext, double* lf, double* partials,
+                  const Geom& g, const cx<CF>* twf, const cx<CB>* twb, FmtParams fp, double mu, AliasMap amap,
+                  const double* shift_sum, DcTable dc) {
+  using K = KgCfg<L, S>;
+  constexpr int cmax = static_cast<int>(sizeof(cx<CF>) > sizeof(cx<CB>) ? sizeof(cx<CF>) : sizeof(cx<CB>));
+  const int sh = K::stage_bytes + K::xb_bytes(cmax);
+  auto k = kg_zpoint<S, CF, CB, L, NF, NB>;
+  kg_smem(k, sh);
+  const unsigned nblk = static_cast<unsigned>((g.nxy + K::LINES - 1) / K::LINES);
+  hipLaunchKernelGGL(k, dim3(nblk), dim3(K::NTH), sh, st, SF, rho, vext, lf, partials, g, twf, twb, fp, mu, amap,
+                     shift_sum, dc);
+}
+
+template <typename S, typename CF, typename CB, int L>
+static void kg_launch(cx<S>* SF, const double* rho, const double* vext, double* lf, double* partials,
+                      const Geom& g, const cx<CF>* twf, const cx<CB>* twb, ChannelSet cs, FmtParams fp, double mu,
+                      AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
+#define CDFT_KG(NF, NB) kg_go<S, CF, CB, L, NF, NB>(st, SF, rho, vext, lf, partials, g, twf, twb, fp, mu, amap, shift_sum, dc)
+  if (cs.nf == 12 && cs.nb == 11) CDFT_KG(12, 11);
+  else if (cs.nf == 12 && cs.nb == 0) CDFT_KG(12, 0);
+  else if (cs.nf == 11 && cs.nb == 11) CDFT_KG(11, 11);
+  else if (cs.nf == 11 && cs.nb == 0) CDFT_KG(11, 0);
+  else if (cs.nf == 1 && cs.nb == 0) CDFT_KG(1, 0);
+  else if (cs.nf == 0 && cs.nb == 0) CDFT_KG(0, 0);
+  else throw std::runtime_error("unsupported channel configuration");
+#undef CDFT_KG
+}
+
 template <typename S, typename CF, typename CB>
 void launch_zpoint(cx<S>* SF, const double* rho, const double* vext, double* local_force, double* partials,
                    const Geom& g, const FftPlan& pz, const cx<CF>* twf, const cx<CB>* twb, ChannelSet cs,
                    FmtParams fp, double mu, AliasMap amap, const double* shift_sum, DcTable dc, hipStream_t st) {
+  if (kg_length(pz.L)) {
+    switch (pz.L) {
+      case 48: kg_launch<S, CF, CB, 48>(SF, rho, vext, local_force, partials, g, twf, twb, cs, fp, mu, amap, shift_sum, dc, st); return;
+      case 80: kg_launch<S, CF, CB, 80>(SF, rho, vext, local_force, partials, g, twf, twb, cs, fp, mu, amap, shift_sum, dc, st); return;
+      case 96: kg_launch<S, CF, CB, 96>(SF, rho, vext, local_force, partials, g, twf, twb, cs, fp, mu, amap, shift_sum, dc, st); return;
+      default: kg_launch<S, CF, CB, 160>(SF, rho, vext, local_force, partials, g, twf, twb, cs, fp, mu, amap, shift_sum, dc, st); return;
+    }
+  }
   constexpr int PPT = sizeof(S) == 4 ? 4 : 2;
   const int nlb = zpoint_nlb<S>(pz);
   if (nlb * g.nz > PPT * NT) throw std::runtime_error("z extent too large for the fused pointwise kernel");
