@@ -299,11 +299,23 @@ __global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, F
   }
 }
 
-// Fused forward k-space stage: x-FFT of rho_hat, multiply by every channel
-// weight (analytic FMT weights, mean-field table), inverse x-FFTs, G channels
-// per batched transform.  EPT >= ty*nx/NT elements owned per thread.
+// Fused forward k-space stage: x-FFT of rho_hat, x-inverse transforms of rho_hat
+// times the channel weights, every channel stored (S0 -> SF).
+//
+// ky and kz are constant along an x line, so a weight's factors in ky and kz
+// commute with the x transforms: with the closed forms of weights.hpp:91-184
+// (w3, w2, aik2 = (w2 - 3 w3 / R) / k^2 radial) the 12 channels come from 7
+// x-inverse transforms ("sources"):
+//   0: w3 rho        -> eta, v_y = i ky (.), v_z = i kz (.)   (eta kept in LDS)
+//   1: w2 rho        -> n2
+//   2: mf rho        -> the mean-field channel
+//   3: i kx w3 rho   -> v_x
+//   4: aik2 kx^2 rho -> T_xx = (.) + eta / R
+//   5: aik2 kx rho   -> T_xy = ky (.), T_xz = kz (.)
+//   6: aik2 rho      -> T_yy = ky^2 (.) + eta / R, T_yz = ky kz (.), T_zz = kz^2 (.) + eta / R
+// G sources per batched transform.  EPT >= ty*nx/NT elements owned per thread.
 template <typename S, typename C, int EPT, int G>
-__global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
+__global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
                                              const S* __restrict__ mf, const cx<double>* __restrict__ wt, Geom g,
                                              FftPlan px, const cx<C>* __restrict__ twx, int ty, ChannelSet cs) {
   const UDiv dty(ty);  // block-uniform divisors
@@ -311,6 +323,7 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nx = g.nx, ny = g.ny, ls = nx + 1;
   const int slab = ty * ls;
+  cx<C>* keep = buf + G * slab;  // eta's x-inverse, for T_xx, T_yy and T_zz
   const int y0 = blockIdx.x * ty;
   const int kz = blockIdx.y;
   const long plane_off = static_cast<long>(kz) * nx * ny;
@@ -323,7 +336,6 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
   fft_lines<C, EPT>(buf, ty, ls, px, -1, twx);
   cx<C> rh[EPT];
   WRad wr[EPT];
-  double wm[EPT];
   const double kzv = g.dkz * kz;
   const double invR = 1.0 / g.R;
 #pragma unroll
@@ -331,60 +343,120 @@ __global__ void __launch_bounds__(NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>
     const int e = threadIdx.x + m * blockDim.x;
     rh[m] = {C(0), C(0)};
     wr[m] = {0.0, 0.0, 0.0};
-    wm[m] = 0.0;
     if (e < ty * nx) {
       const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
       rh[m] = buf[yl * ls + x];
-      if (y < ny) {
+      if (y < ny && cs.hs) {
         const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
-        if (cs.hs) wr[m] = wrad(wt[plane_off + x * ny + y], kx * kx + ky * ky + kzv * kzv, invR);
-        // mean-field table: FP64 values, or FP32 (value, residual) planes
-        if (cs.mf)
-          wm[m] = sizeof(S) == 8 ? static_cast<double>(mf[plane_off + x * ny + y])
-                                 : static_cast<double>(mf[plane_off + x * ny + y]) +
-                                       static_cast<double>(mf[g.M + plane_off + x * ny + y]);
+        wr[m] = wrad(wt[plane_off + x * ny + y], kx * kx + ky * ky + kzv * kzv, invR);
       }
     }
   }
   __syncthreads();
-  for (int c0 = 0; c0 < cs.nf; c0 += G) {
-    const int ng = min(G, cs.nf - c0);
+  // the block-uniform source list: 0..6, without 2 when there is no mean
+  // field, only 2 when there is no hard-sphere term
+  const int nsrc = cs.hs ? 6 + (cs.mf ? 1 : 0) : (cs.mf ? 1 : 0);
+  auto src = [&](int i) { return cs.hs ? (cs.mf || i < 2 ? i : i + 1) : 2; };
+  const int mf_slot = cs.hs ? 11 : 0;
+  auto out_ch = [&](int c) { return SF + static_cast<long>(c) * g.M + plane_off; };
+  for (int c0 = 0; c0 < nsrc; c0 += G) {
+    const int ng = min(G, nsrc - c0);
     for (int gi = 0; gi < ng; ++gi) {
-      const int ch = cs.hs ? c0 + gi : 11;
+      const int sc = src(c0 + gi);
 #pragma unroll
       for (int m = 0; m < EPT; ++m) {
         const int e = threadIdx.x + m * blockDim.x;
         if (e < ty * nx) {
           const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
-          const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
-          const cx<double> w = channel_weight_t(ch, false, kx, ky, kzv, wr[m], wm[m], invR);
-          buf[gi * slab + yl * ls + x] = wmul(rh[m], w);
+          const double kx = kvec(x, nx, g.dkx);
+          const WRad w = wr[m];
+          cx<double> wc;
+          switch (sc) {
+            case 0: wc = {w.w3, 0.0}; break;
+            case 1: wc = {w.w2, 0.0}; break;
+            case 2: {
+              // mean-field table: FP64 values, or FP32 (value, residual) planes
+              const long o = plane_off + x * ny + min(y, ny - 1);
+              wc = {sizeof(S) == 8 ? static_cast<double>(mf[o])
+                                   : static_cast<double>(mf[o]) + static_cast<double>(mf[g.M + o]),
+                    0.0};
+              break;
+            }
+            case 3: wc = {0.0, kx * w.w3}; break;
+            case 4: wc = {w.aik2 * kx * kx, 0.0}; break;
+            case 5: wc = {w.aik2 * kx, 0.0}; break;
+            default: wc = {w.aik2, 0.0}; break;
+          }
+          buf[gi * slab + yl * ls + x] = wmul(rh[m], wc);
         }
       }
     }
     __syncthreads();
     // The LDS-pass transform: with the lane's spectrum and weights live
-    // across the channel loop, the register-resident one measured slower
+    // across the source loop, the register-resident one measured slower
     // here (160^3: 502 against 308 us per launch).
     fft_inplace<C, EPT * G>(buf, ng * ty, ls, px, +1, twx);
     for (int gi = 0; gi < ng; ++gi) {
-      cx<S>* outp = SF + static_cast<long>(c0 + gi) * g.M + plane_off;
+      const int sc = src(c0 + gi);
+      const cx<C>* vb = buf + gi * slab;
       for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
         const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
-        if (y < ny) outp[x * ny + y] = cvt<S>(buf[gi * slab + yl * ls + x]);
+        if (y >= ny) continue;
+        const long o = static_cast<long>(x) * ny + y;
+        const cx<C> v = vb[yl * ls + x];
+        const C ky = static_cast<C>(kvec(y, ny, g.dky)), kzc = static_cast<C>(kzv), ir = static_cast<C>(invR);
+        switch (sc) {
+          case 0:
+            keep[yl * ls + x] = v;
+            out_ch(0)[o] = cvt<S>(v);
+            out_ch(3)[o] = cvt<S>(cx<C>{-ky * v.im, ky * v.re});
+            out_ch(4)[o] = cvt<S>(cx<C>{-kzc * v.im, kzc * v.re});
+            break;
+          case 1: out_ch(1)[o] = cvt<S>(v); break;
+          case 2: out_ch(mf_slot)[o] = cvt<S>(v); break;
+          case 3: out_ch(2)[o] = cvt<S>(v); break;
+          case 4: {
+            const cx<C> k = keep[yl * ls + x];
+            out_ch(5)[o] = cvt<S>(cx<C>{v.re + ir * k.re, v.im + ir * k.im});
+            break;
+          }
+          case 5:
+            out_ch(6)[o] = cvt<S>(cx<C>{ky * v.re, ky * v.im});
+            out_ch(7)[o] = cvt<S>(cx<C>{kzc * v.re, kzc * v.im});
+            break;
+          default: {
+            const cx<C> k = keep[yl * ls + x];
+            const C yy = ky * ky, yz = ky * kzc, zz = kzc * kzc;
+            out_ch(8)[o] = cvt<S>(cx<C>{yy * v.re + ir * k.re, yy * v.im + ir * k.im});
+            out_ch(9)[o] = cvt<S>(cx<C>{yz * v.re, yz * v.im});
+            out_ch(10)[o] = cvt<S>(cx<C>{zz * v.re + ir * k.re, zz * v.im + ir * k.im});
+            break;
+          }
+        }
       }
     }
     __syncthreads();
   }
 }
 
-// Fused backward k-space stage: x-FFT of every derivative channel (G per
-// batched transform), sum of weight * channel with conj weights on the odd
-// vector channels (hard_sphere.hpp:118-134), inverse x-FFT of the sum.
+// Fused backward k-space stage: x-FFT of the derivative channels, sum of
+// weight * channel with conj weights on the odd vector channels
+// (hard_sphere.hpp:118-134), inverse x-FFT of the sum (SB -> SO).
+//
+// The conj weights are linear in (w3, w2, aik2) with factors in k, and ky, kz
+// are constant along an x line, so the 11 channels are combined into 6 groups
+// before the x transforms, and the sum is
+//   S = w3 (X0 - i kx X1) + w2 X2 + aik2 (kx^2 X3 + kx X4 + X5),   Xq = x-FFT of
+//   P0 = G_eta - i ky G_vy - i kz G_vz + (G_Txx + G_Tyy + G_Tzz) / R,  P1 = G_vx,  P2 = G_n2,
+//   P3 = G_Txx,  P4 = ky G_Txy + kz G_Txz,  P5 = ky^2 G_Tyy + ky kz G_Tyz + kz^2 G_Tzz,
+// accumulated as A = X0 - i kx X1, B = X2, C = kx^2 X3 + kx X4 + X5 and
+// combined once per element with the radial factors.  G groups per batched
+// transform; a group's loads combine its channels in registers.
 template <typename S, typename C, int EPT, int G>
-__global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
+__global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
                                              const cx<double>* __restrict__ wt, Geom g, FftPlan px,
-                                             const cx<C>* __restrict__ twx, int ty, int nb) {
+                                             const cx<C>* __restrict__ twx, int ty) {
+  constexpr int NG = 6;
   const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
@@ -395,41 +467,67 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
   const long plane_off = static_cast<long>(kz) * nx * ny;
   const double kzv = g.dkz * kz;
   const double invR = 1.0 / g.R;
-  cx<C> acc[EPT];
-  WRad wr[EPT];
+  cx<C> accA[EPT], accB[EPT], accC[EPT];
 #pragma unroll
-  for (int m = 0; m < EPT; ++m) {
-    acc[m] = {C(0), C(0)};
-    wr[m] = {0.0, 0.0, 0.0};
-    const int e = threadIdx.x + m * blockDim.x;
-    if (e < ty * nx) {
+  for (int m = 0; m < EPT; ++m) accA[m] = accB[m] = accC[m] = {C(0), C(0)};
+  auto in_ch = [&](int c, long o) { return cvt<C>(SB[static_cast<long>(c) * g.M + plane_off + o]); };
+  for (int c0 = 0; c0 < NG; c0 += G) {
+    const int ng = min(G, NG - c0);
+    for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
       const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
-      if (y < ny) {
-        const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
-        wr[m] = wrad(wt[plane_off + x * ny + y], kx * kx + ky * ky + kzv * kzv, invR);
-      }
-    }
-  }
-  for (int c0 = 0; c0 < nb; c0 += G) {
-    const int ng = min(G, nb - c0);
-    for (int gi = 0; gi < ng; ++gi) {
-      const cx<S>* plane = SB + static_cast<long>(c0 + gi) * g.M + plane_off;
-      for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
-        const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
-        buf[gi * slab + yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
+      const long o = static_cast<long>(x) * ny + y;
+      const C ky = static_cast<C>(kvec(min(y, ny - 1), ny, g.dky)), kzc = static_cast<C>(kzv);
+      const C ir = static_cast<C>(invR);
+      for (int gi = 0; gi < ng; ++gi) {
+        cx<C> p = {C(0), C(0)};
+        if (y < ny) {
+          switch (c0 + gi) {  // block-uniform
+            case 0: {
+              const cx<C> ge = in_ch(0, o), gy = in_ch(3, o), gz = in_ch(4, o);
+              const cx<C> t = cadd(cadd(in_ch(5, o), in_ch(8, o)), in_ch(10, o));
+              p = {ge.re + ky * gy.im + kzc * gz.im + ir * t.re, ge.im - ky * gy.re - kzc * gz.re + ir * t.im};
+              break;
+            }
+            case 1: p = in_ch(2, o); break;
+            case 2: p = in_ch(1, o); break;
+            case 3: p = in_ch(5, o); break;
+            case 4: {
+              const cx<C> a = in_ch(6, o), b = in_ch(7, o);
+              p = {ky * a.re + kzc * b.re, ky * a.im + kzc * b.im};
+              break;
+            }
+            default: {
+              const cx<C> a = in_ch(8, o), b = in_ch(9, o), c = in_ch(10, o);
+              const C yy = ky * ky, yz = ky * kzc, zz = kzc * kzc;
+              p = {yy * a.re + yz * b.re + zz * c.re, yy * a.im + yz * b.im + zz * c.im};
+              break;
+            }
+          }
+        }
+        buf[gi * slab + yl * ls + x] = p;
       }
     }
     __syncthreads();
     fft_lines<C, EPT * G>(buf, ng * ty, ls, px, -1, twx);
     for (int gi = 0; gi < ng; ++gi) {
+      const int grp = c0 + gi;  // block-uniform
 #pragma unroll
       for (int m = 0; m < EPT; ++m) {
         const int e = threadIdx.x + m * blockDim.x;
         if (e < ty * nx) {
-          const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
-          const double kx = kvec(x, nx, g.dkx), ky = kvec(min(y, ny - 1), ny, g.dky);
-          const cx<double> w = channel_weight_t(c0 + gi, true, kx, ky, kzv, wr[m], 0.0, invR);
-          acc[m] = cadd(acc[m], wmul(buf[gi * slab + yl * ls + x], w));
+          const int x = dty.div(e), yl = e - x * ty;
+          const cx<C> X = buf[gi * slab + yl * ls + x];
+          const C kx = static_cast<C>(kvec(x, nx, g.dkx));
+          cx<C>& A = accA[m];
+          cx<C>& Cs = accC[m];
+          switch (grp) {
+            case 0: A = cadd(A, X); break;
+            case 1: A = {A.re + kx * X.im, A.im - kx * X.re}; break;  // -i kx X1
+            case 2: accB[m] = cadd(accB[m], X); break;
+            case 3: Cs = {Cs.re + (kx * kx) * X.re, Cs.im + (kx * kx) * X.im}; break;
+            case 4: Cs = {Cs.re + kx * X.re, Cs.im + kx * X.im}; break;
+            default: Cs = cadd(Cs, X); break;
+          }
         }
       }
     }
@@ -439,8 +537,17 @@ __global__ void __launch_bounds__(NT) k_xbwd(const cx<S>* __restrict__ SB, cx<S>
   for (int m = 0; m < EPT; ++m) {
     const int e = threadIdx.x + m * blockDim.x;
     if (e < ty * nx) {
-      const int x = dty.div(e), yl = e - x * ty;
-      buf[yl * ls + x] = acc[m];
+      const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
+      cx<C> s = {C(0), C(0)};
+      if (y < ny) {
+        // the radial factors of the element (FP64 table), in the path's precision
+        const double kx = kvec(x, nx, g.dkx), ky = kvec(y, ny, g.dky);
+        const WRad w = wrad(wt[plane_off + x * ny + y], kx * kx + ky * ky + kzv * kzv, invR);
+        const C w3 = static_cast<C>(w.w3), w2 = static_cast<C>(w.w2), a2 = static_cast<C>(w.aik2);
+        const cx<C> a = accA[m], b = accB[m], c = accC[m];
+        s = {w3 * a.re + w2 * b.re + a2 * c.re, w3 * a.im + w2 * b.im + a2 * c.im};
+      }
+      buf[yl * ls + x] = s;
     }
   }
   __syncthreads();
@@ -974,8 +1081,8 @@ template <typename S, typename C>
 void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, const Geom& g, const FftPlan& px,
                  const cx<C>* twx, ChannelSet cs, hipStream_t st) {
   const int ty = px.tile > 0 ? px.tile : kXTile;
-  constexpr int G = sizeof(C) == 4 ? 2 : 1;  // channels per batched transform (LDS <= 64 KiB)
-  const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
+  constexpr int G = sizeof(C) == 4 ? 2 : 1;  // sources per batched transform
+  const size_t sh = sizeof(cx<C>) * (G + (cs.hs ? 1 : 0)) * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   if (ty * g.nx <= 8 * NT)
     hipLaunchKernelGGL((k_xfwd<S, C, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, ty, cs);
@@ -988,14 +1095,15 @@ void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, 
 template <typename S, typename C>
 void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g, const FftPlan& px,
                  const cx<C>* twx, int nb, hipStream_t st) {
+  if (nb != 11) throw std::runtime_error("the backward x kernel takes the 11 hard-sphere derivative channels");
   const int ty = px.tile > 0 ? px.tile : kXTile;
-  constexpr int G = sizeof(C) == 4 ? 2 : 1;
+  constexpr int G = sizeof(C) == 4 ? 2 : 1;  // groups per batched transform
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   if (ty * g.nx <= 8 * NT)
-    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, nb);
+    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty);
   else if (ty * g.nx <= 16 * NT)
-    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, nb);
+    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty);
   else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
