@@ -96,7 +96,8 @@ bool shapes_valid(const Context* c, const FftPlan pl[3]) {
   if (zl > 0 && (zl < 2 || (zl & 1) || zl > 64 || (zl / 2) * nz > 8 * kNT ||
                  cx16 * (zl / 2) * nz + sS * zl * nzh > kLdsLimit))
     return false;
-  if (yl > 0 && (yl * ny > 8 * kNT || cx16 * yl * ny > kLdsLimit)) return false;
+  if (yl > 0 && ((!slot_lanes(static_cast<int>(ny)) && yl * ny > 8 * kNT) || cx16 * yl * ny > kLdsLimit))
+    return false;
   if (xt > 0 && (xt * nx > 16 * kNT || cx16 * xt * (nx + 1) > kLdsLimit)) return false;
   const long ppt = c->precision == CDFT_PRECISION_MIXED ? 4 : 2;
   if (zp > 0 && (zp < 2 || (zp & 1) || zp > 64 || zp * nz > ppt * kNT ||
@@ -111,7 +112,10 @@ int default_zlines(const Context* c) {
   n = std::min(std::max(n, 2), 64);
   return n & ~1;
 }
-int default_ylines(const Context* c) { return static_cast<int>(2048 / c->shape[1]); }
+int default_ylines(const Context* c) {
+  const int T = slot_lanes(static_cast<int>(c->shape[1]));
+  return T ? kNT / T : static_cast<int>(2048 / c->shape[1]);
+}
 int default_zpoint(const Context* c) {
   int n = static_cast<int>((c->precision == CDFT_PRECISION_MIXED ? 4 : 2) * kNT / c->shape[2]);
   n = std::min(std::max(n, 2), 64);
@@ -220,7 +224,7 @@ int cdft_autotune(cdft_ctx* ctx, int reps, double* us_default, double* us_tuned)
           cand[2].lines = v;
           if (v > 0) trial(cand);
         }
-        for (int v : {yl / 2, yl / 4}) {
+        for (int v : {yl / 2, slot_lanes(static_cast<int>(c->shape[1])) ? 2 * yl : yl / 4}) {
           FftPlan cand[3] = {best[0], best[1], best[2]};
           cand[1].lines = v;
           if (v > 0) trial(cand);

@@ -261,7 +261,7 @@ __global__ void __launch_bounds__(NT) k_sum(const double* __restrict__ in, Alias
 
 // ---------------------------------------------------------------- y lines (contiguous)
 template <typename S, typename C>
-__global__ void __launch_bounds__(NT) k_lines(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
+__global__ void __launch_bounds__(NT, 4) k_lines(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
                                               const cx<C>* __restrict__ tw, int nlb, int sg) {
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
@@ -1055,8 +1055,12 @@ void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g
 
 template <typename S, typename C>
 void launch_lines(cx<S>* Sp, long nlines, const FftPlan& p, const cx<C>* tw, int sg, hipStream_t st) {
-  const int nlb = p.lines > 0 ? p.lines : 2048 / p.L;
-  if (nlb < 1) throw std::runtime_error("line length too large for the line transform kernel");
+  // one line per slot by default (every slot busy); the LDS-pass lengths take
+  // at most 8 * NT elements per block
+  const int T = slot_lanes(p.L);
+  const int nlb = p.lines > 0 ? p.lines : (T ? NT / T : 2048 / p.L);
+  if (nlb < 1 || (!T && nlb * p.L > 8 * NT))
+    throw std::runtime_error("line length too large for the line transform kernel");
   const unsigned nblk = static_cast<unsigned>((nlines + nlb - 1) / nlb);
   const size_t sh = sizeof(cx<C>) * nlb * p.L;
   hipLaunchKernelGGL((k_lines<S, C>), dim3(nblk), dim3(NT), sh, st, Sp, nlines, p, tw, nlb, sg);

@@ -19,6 +19,12 @@
 
 namespace cdft {
 
+// Lanes per slot of a length with a register-resident transform (fft_lines'
+// list), 0 for the lengths fft_inplace takes.
+__host__ __device__ constexpr int slot_lanes(int L) {
+  return (L == 48 || L == 64 || L == 80) ? 8 : (L == 96 || L == 128 || L == 160) ? 16 : 0;
+}
+
 // Slots of T lanes take lines slot, slot + nslot, ...; the caller synchronises
 // before the call (inputs visible), the call ends with a block barrier.
 template <typename C, int L, int T, int SG>
