@@ -798,6 +798,11 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
 
   // ---- forward rounds.  Round R + 1's half spectra are loaded into registers
   // before round R's transforms, so their memory latency overlaps that work.
+  // Round positions: with the mean field (NF 12) the scalar channels eta, n2
+  // and the mean field fill round 0, which runs in CF; the vector and tensor
+  // channels of the later rounds run in CB (FP32 in mixed precision, as in the
+  // fused schedule).
+  auto fch = [](int pos) { return NF == 12 ? (pos < 2 ? pos : (pos == 2 ? 11 : pos - 1)) : pos; };
   constexpr int NSTG = 4 * NZH * LINES, KPF = (NSTG + K::NTH - 1) / K::NTH;
   cx<S> pf[KPF];
   auto load_round = [&](int R) {
@@ -806,10 +811,10 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
       const int idx = tid + K::NTH * k;
       const int c4 = idx / (NZH * LINES), rem = idx - c4 * (NZH * LINES);
       const int kz = rem / LINES, ln = rem - kz * LINES;
-      const int c = 4 * R + c4;
+      const int pos = 4 * R + c4;
       pf[k] = {S(0), S(0)};
-      if (idx < NSTG && c < NF && l0 + ln < nxy)
-        pf[k] = SF[static_cast<long>(c) * g.M + static_cast<long>(kz) * nxy + l0 + ln];
+      if (idx < NSTG && pos < NF && l0 + ln < nxy)
+        pf[k] = SF[static_cast<long>(fch(pos)) * g.M + static_cast<long>(kz) * nxy + l0 + ln];
     }
   };
   auto store_round = [&]() {
@@ -819,28 +824,24 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
       if (idx < NSTG) stage[idx] = pf[k];
     }
   };
-  if constexpr (NRF > 0) {
-    load_round(0);
-    store_round();
-    __syncthreads();
-  }
-#pragma unroll
-  for (int R = 0; R < NRF; ++R) {
-    if (R + 1 < NRF) load_round(R + 1);
-    const int c = 4 * R + cr;
-    cx<CF>* xb = reinterpret_cast<cx<CF>*>(sbuf(slot));
-    if (c < NF) {
+  // one round's C2R in arithmetic CR: slot (cr, lp) transforms position 4 R + cr
+  // of its line pair; then every lane reads its points of the round's channels
+  auto fwd_round = [&](auto zero, const auto* __restrict__ tw, int R) {
+    using CR = decltype(zero);
+    const int pos = 4 * R + cr;
+    cx<CR>* xb = reinterpret_cast<cx<CR>*>(sbuf(slot));
+    if (pos < NF) {
       const cx<S>* st = stage + cr * NZH * LINES;
-      cx<CF> v[E];
+      cx<CR> v[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int k = lane + T * q;
         const bool mirror = k >= NZH;
         const int kk = mirror ? L - k : k;
-        cx<CF> A = cvt<CF>(st[kk * LINES + la]), B = cvt<CF>(st[kk * LINES + la + 1]);
+        cx<CR> A = cvt<CR>(st[kk * LINES + la]), B = cvt<CR>(st[kk * LINES + la + 1]);
         if (kk == 0 || 2 * kk == L) {
-          A.im = CF(0);
-          B.im = CF(0);
+          A.im = CR(0);
+          B.im = CR(0);
         }
         if (mirror) {
           A.im = -A.im;
@@ -848,7 +849,7 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
         }
         v[q] = {A.re - B.im, A.im + B.re};
       }
-      slot_fft<CF, L, T, +1, false, false>(v, xb, lane, twf);
+      slot_fft<CR, L, T, +1, false, false>(v, xb, lane, tw);
       wave_sync();
 #pragma unroll
       for (int e = 0; e < E; ++e) xb[lane + T * e] = v[e];
@@ -858,17 +859,28 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
     // wave's line pairs (line pair f / 2L, z = (f % 2L) / 2, line (f % 2L) & 1)
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
-      const int ch = 4 * R + c4;
-      if (ch < NF) {
+      if (4 * R + c4 < NF) {
+        const int ch = fch(4 * R + c4);
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
           const int f = lid + 64 * j, pl = f / (2 * L), r2 = f - pl * 2 * L;
-          const CF* src = reinterpret_cast<const CF*>(sbuf(c4 * LPW + pl));
+          const CR* src = reinterpret_cast<const CR*>(sbuf(c4 * LPW + pl));
           wv[ch][j] = static_cast<S>(src[r2]);
         }
       }
     }
     wave_sync();  // the buffers are rewritten by the next round
+  };
+  if constexpr (NRF > 0) {
+    load_round(0);
+    store_round();
+    __syncthreads();
+  }
+#pragma unroll
+  for (int R = 0; R < NRF; ++R) {
+    if (R + 1 < NRF) load_round(R + 1);
+    if (R == 0) fwd_round(CF(0), twf, R);
+    else fwd_round(CB(0), twb, R);
     if (R + 1 < NRF) {
       __syncthreads();  // every slot is done with this round's staging
       store_round();
