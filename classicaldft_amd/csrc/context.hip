@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -430,7 +431,17 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(d_mf), static_cast<const cx<double>*>(d_WTg), g, plan[0],
                        tw<CE>(0), cs, stream);
     mark(4);
-    launch_lines<S, CE>(SF, ylines * cs.nf, plan[1], tw<CE>(1), +1, stream);
+    if constexpr (std::is_same_v<S, float> && std::is_same_v<CE, double> && std::is_same_v<CL, float>) {
+      // mixed: the y-inverse of the scalar channels (eta, n2, mean field) in
+      // FP64, of the vector and tensor channels (2..10) in FP32, as in the
+      // fused schedule
+      if (cs.hs)
+        launch_lines_split(SF, ylines * cs.nf, plan[1], tw<CE>(1), tw<CL>(1), 2 * ylines, 11 * ylines, +1, stream);
+      else
+        launch_lines<S, CE>(SF, ylines * cs.nf, plan[1], tw<CE>(1), +1, stream);
+    } else {
+      launch_lines<S, CE>(SF, ylines * cs.nf, plan[1], tw<CE>(1), +1, stream);
+    }
   }
   ChannelSet cz = cs;
   if (!backward) cz.nb = 0;

@@ -12,9 +12,11 @@
 // Every transform kernel is templated on the storage type S of the spectral
 // arrays in HBM and on the arithmetic type C used in LDS and registers.  The
 // mixed-precision evaluation stores FP32 everywhere (half the traffic) and
-// computes the energy path's inverse transforms in FP64: a smooth density
-// concentrates its spectrum in a few large low-k values, whose FP32
-// arithmetic errors would otherwise add up coherently in Omega.
+// computes the scalar channels' (eta, n2, mean field) inverse transforms and
+// the x stage of every channel in FP64: a smooth density concentrates its
+// spectrum in a few large low-k values, whose FP32 arithmetic errors would
+// otherwise add up coherently in Omega.  The vector and tensor channels' y
+// and z stages and the backward path run in FP32.
 //
 // Evaluation schedule (one species, FMT + optional mean field):
 //   k_sum + k_finalize   FP64 sum of rho (mean-density split)
@@ -261,8 +263,8 @@ __global__ void __launch_bounds__(NT) k_sum(const double* __restrict__ in, Alias
 
 // ---------------------------------------------------------------- y lines (contiguous)
 template <typename S, typename C>
-__global__ void __launch_bounds__(NT, 4) k_lines(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
-                                              const cx<C>* __restrict__ tw, int nlb, int sg) {
+__device__ __forceinline__ void lines_block(cx<S>* __restrict__ Sp, long nlines, const FftPlan& p,
+                                            const cx<C>* __restrict__ tw, int nlb, int sg) {
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int L = p.L;
@@ -273,6 +275,24 @@ __global__ void __launch_bounds__(NT, 4) k_lines(cx<S>* __restrict__ Sp, long nl
   __syncthreads();
   fft_lines<C, 8>(buf, nl, L, p, sg, tw);
   for (int e = threadIdx.x; e < nl * L; e += blockDim.x) base[e] = cvt<S>(buf[e]);
+}
+
+template <typename S, typename C>
+__global__ void __launch_bounds__(NT, 4) k_lines(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
+                                                 const cx<C>* __restrict__ tw, int nlb, int sg) {
+  lines_block<S, C>(Sp, nlines, p, tw, nlb, sg);
+}
+
+// Lines of two arithmetic precisions in one launch: a block whose lines all lie
+// in [lo0, lo1) runs in CL, every other block in CH (LDS sized for CH).
+template <typename S, typename CH, typename CL>
+__global__ void __launch_bounds__(NT, 4) k_lines_split(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
+                                                       const cx<CH>* __restrict__ twh,
+                                                       const cx<CL>* __restrict__ twl, int nlb, int sg, long lo0,
+                                                       long lo1) {
+  const long line0 = static_cast<long>(blockIdx.x) * nlb;
+  if (line0 >= lo0 && line0 + nlb <= lo1) lines_block<S, CL>(Sp, nlines, p, twl, nlb, sg);
+  else lines_block<S, CH>(Sp, nlines, p, twh, nlb, sg);
 }
 
 // ---------------------------------------------------------------- x lines (strided in a kz plane)
@@ -1076,6 +1096,18 @@ void launch_lines(cx<S>* Sp, long nlines, const FftPlan& p, const cx<C>* tw, int
   const unsigned nblk = static_cast<unsigned>((nlines + nlb - 1) / nlb);
   const size_t sh = sizeof(cx<C>) * nlb * p.L;
   hipLaunchKernelGGL((k_lines<S, C>), dim3(nblk), dim3(NT), sh, st, Sp, nlines, p, tw, nlb, sg);
+}
+
+void launch_lines_split(cx<float>* Sp, long nlines, const FftPlan& p, const cx<double>* twh, const cx<float>* twl,
+                        long lo0, long lo1, int sg, hipStream_t st) {
+  const int T = slot_lanes(p.L);
+  const int nlb = p.lines > 0 ? p.lines : (T ? NT / T : 2048 / p.L);
+  if (nlb < 1 || (!T && nlb * p.L > 8 * NT))
+    throw std::runtime_error("line length too large for the line transform kernel");
+  const unsigned nblk = static_cast<unsigned>((nlines + nlb - 1) / nlb);
+  const size_t sh = sizeof(cx<double>) * nlb * p.L;
+  hipLaunchKernelGGL((k_lines_split<float, double, float>), dim3(nblk), dim3(NT), sh, st, Sp, nlines, p, twh, twl,
+                     nlb, sg, lo0, lo1);
 }
 
 static constexpr int kXTile = 16;  // y columns per x-line block (128-byte rows in FP32)

@@ -66,6 +66,10 @@ int sum_blocks(const Geom& g);
 void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g, hipStream_t st);
 template <typename S, typename C>
 void launch_lines(cx<S>* Sp, long nlines, const FftPlan& p, const cx<C>* tw, int sg, hipStream_t st);
+// FP32-stored lines in one launch: blocks entirely inside lines [lo0, lo1) in
+// FP32 arithmetic, the others in FP64.
+void launch_lines_split(cx<float>* Sp, long nlines, const FftPlan& p, const cx<double>* twh, const cx<float>* twl,
+                        long lo0, long lo1, int sg, hipStream_t st);
 template <typename S, typename C>
 void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st);
 // wt: FP64 radial weights (w3, w2) per element, from launch_wtable_general.
