@@ -428,8 +428,8 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     mark(2);
     launch_lines<S, CL>(S0, ylines, plan[1], tw<CL>(1), -1, stream);
     mark(3);
-    launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(d_mf), static_cast<const cx<double>*>(d_WTg), g, plan[0],
-                       tw<CE>(0), cs, stream);
+    launch_xfwd<S, CE, CL>(S0, SF, static_cast<const S*>(d_mf), static_cast<const cx<double>*>(d_WTg), g, plan[0],
+                           tw<CE>(0), tw<CL>(0), cs, stream);
     mark(4);
     if constexpr (std::is_same_v<S, float> && std::is_same_v<CE, double> && std::is_same_v<CL, float>) {
       // mixed: the y-inverse of the scalar channels (eta, n2, mean field) in
@@ -595,8 +595,8 @@ void wd_t(Context* c, const double* d_rho_in, double* d_out) {
   launch_finalize(c->d_sum_part, c->nblk_sum, c->d_shift, c->stream);
   launch_zr2c<S, CL>(d_rho_in, S0, g, c->plan[2], c->tw<CL>(2), id, c->d_shift, c->stream);
   launch_lines<S, CL>(S0, ylines, c->plan[1], c->tw<CL>(1), -1, c->stream);
-  launch_xfwd<S, CE>(S0, SF, static_cast<const S*>(c->d_mf), static_cast<const cx<double>*>(c->d_WTg), g,
-                     c->plan[0], c->tw<CE>(0), c->cs, c->stream);
+  launch_xfwd<S, CE, CE>(S0, SF, static_cast<const S*>(c->d_mf), static_cast<const cx<double>*>(c->d_WTg), g,
+                         c->plan[0], c->tw<CE>(0), c->tw<CE>(0), c->cs, c->stream);
   launch_lines<S, CE>(SF, ylines * c->cs.nf, c->plan[1], c->tw<CE>(1), +1, c->stream);
   launch_zc2r<S, CE>(SF, c->cs.nf, d_out, nullptr, 1.0, g, c->plan[2], c->tw<CE>(2), c->d_shift, c->dc, c->stream);
 }

@@ -38,6 +38,7 @@
 #include <vector>
 #include <cmath>
 #include <stdexcept>
+#include <type_traits>
 
 #include "device_common.hpp"
 #include "fft_lds.hpp"
@@ -334,10 +335,12 @@ __global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, F
 //   5: aik2 kx rho   -> T_xy = ky (.), T_xz = kz (.)
 //   6: aik2 rho      -> T_yy = ky^2 (.) + eta / R, T_yz = ky kz (.), T_zz = kz^2 (.) + eta / R
 // G sources per batched transform.  EPT >= ty*nx/NT elements owned per thread.
-template <typename S, typename C, int EPT, int G>
+// The vector and tensor sources (3..6) use arithmetic CV (C: the scalar ones).
+template <typename S, typename C, typename CV, int EPT, int G>
 __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
                                              const S* __restrict__ mf, const cx<double>* __restrict__ wt, Geom g,
-                                             FftPlan px, const cx<C>* __restrict__ twx, int ty, ChannelSet cs) {
+                                             FftPlan px, const cx<C>* __restrict__ twx,
+                                             const cx<CV>* __restrict__ twv, int ty, ChannelSet cs) {
   const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
@@ -379,8 +382,12 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
   auto src = [&](int i) { return cs.hs ? (cs.mf || i < 2 ? i : i + 1) : 2; };
   const int mf_slot = cs.hs ? 11 : 0;
   auto out_ch = [&](int c) { return SF + static_cast<long>(c) * g.M + plane_off; };
-  for (int c0 = 0; c0 < nsrc; c0 += G) {
-    const int ng = min(G, nsrc - c0);
+  // A batch of ng sources from list position c0 in arithmetic CR: the LDS
+  // buffer holds GR slabs of CR (G slabs of C).
+  auto run_batch = [&](auto zero, const auto* __restrict__ twr, int c0, int ng) {
+    using CR = decltype(zero);
+    constexpr int GR = G * static_cast<int>(sizeof(cx<C>) / sizeof(cx<CR>));
+    cx<CR>* bufr = reinterpret_cast<cx<CR>*>(smem_raw);
     for (int gi = 0; gi < ng; ++gi) {
       const int sc = src(c0 + gi);
 #pragma unroll
@@ -407,7 +414,7 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
             case 5: wc = {w.aik2 * kx, 0.0}; break;
             default: wc = {w.aik2, 0.0}; break;
           }
-          buf[gi * slab + yl * ls + x] = wmul(rh[m], wc);
+          bufr[gi * slab + yl * ls + x] = cvt<CR>(wmul(rh[m], wc));
         }
       }
     }
@@ -415,15 +422,15 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
     // The LDS-pass transform: with the lane's spectrum and weights live
     // across the source loop, the register-resident one measured slower
     // here (160^3: 502 against 308 us per launch).
-    fft_inplace<C, EPT * G>(buf, ng * ty, ls, px, +1, twx);
+    fft_inplace<CR, EPT * GR>(bufr, ng * ty, ls, px, +1, twr);
     for (int gi = 0; gi < ng; ++gi) {
       const int sc = src(c0 + gi);
-      const cx<C>* vb = buf + gi * slab;
+      const cx<CR>* vb = bufr + gi * slab;
       for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
         const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
         if (y >= ny) continue;
         const long o = static_cast<long>(x) * ny + y;
-        const cx<C> v = vb[yl * ls + x];
+        const cx<C> v = cvt<C>(vb[yl * ls + x]);
         const C ky = static_cast<C>(kvec(y, ny, g.dky)), kzc = static_cast<C>(kzv), ir = static_cast<C>(invR);
         switch (sc) {
           case 0:
@@ -456,6 +463,14 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
       }
     }
     __syncthreads();
+  };
+  // the scalar sources (eta, n2, mean field) in C; with CV narrower than C the
+  // vector and tensor sources 3..6 in CV, more of them per batch
+  const int nscal = (cs.hs && !std::is_same<C, CV>::value) ? (cs.mf ? 3 : 2) : nsrc;
+  for (int c0 = 0; c0 < nscal; c0 += G) run_batch(C(0), twx, c0, min(G, nscal - c0));
+  if constexpr (!std::is_same<C, CV>::value) {
+    constexpr int GV = G * static_cast<int>(sizeof(cx<C>) / sizeof(cx<CV>));
+    for (int c0 = nscal; c0 < nsrc; c0 += GV) run_batch(CV(0), twv, c0, min(GV, nsrc - c0));
   }
 }
 
@@ -1125,17 +1140,17 @@ void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const c
     throw std::runtime_error("x extent too large for the x transform kernel");
 }
 
-template <typename S, typename C>
+template <typename S, typename C, typename CV>
 void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, const Geom& g, const FftPlan& px,
-                 const cx<C>* twx, ChannelSet cs, hipStream_t st) {
+                 const cx<C>* twx, const cx<CV>* twv, ChannelSet cs, hipStream_t st) {
   const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;  // sources per batched transform
   const size_t sh = sizeof(cx<C>) * (G + (cs.hs ? 1 : 0)) * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   if (ty * g.nx <= 8 * NT)
-    hipLaunchKernelGGL((k_xfwd<S, C, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, ty, cs);
+    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs);
   else if (ty * g.nx <= 16 * NT)
-    hipLaunchKernelGGL((k_xfwd<S, C, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, ty, cs);
+    hipLaunchKernelGGL((k_xfwd<S, C, CV, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs);
   else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
@@ -1304,14 +1319,17 @@ void launch_finalize(const double* partials, int nrows, double* res, hipStream_t
                                   const cx<C>*, const double*, DcTable, hipStream_t);                              \
   template void launch_lines<S, C>(cx<S>*, long, const FftPlan&, const cx<C>*, int, hipStream_t);                 \
   template void launch_xlines<S, C>(cx<S>*, int, const Geom&, const FftPlan&, const cx<C>*, int, hipStream_t);    \
-  template void launch_xfwd<S, C>(const cx<S>*, cx<S>*, const S*, const cx<double>*, const Geom&, const FftPlan&, \
-                                  const cx<C>*, ChannelSet, hipStream_t);                                          \
+  template void launch_xfwd<S, C, C>(const cx<S>*, cx<S>*, const S*, const cx<double>*, const Geom&,               \
+                                     const FftPlan&, const cx<C>*, const cx<C>*, ChannelSet, hipStream_t);        \
   template void launch_xbwd<S, C>(const cx<S>*, cx<S>*, const cx<double>*, const Geom&, const FftPlan&,           \
                                   const cx<C>*, int, hipStream_t);
 
 CDFT_INST_SC(float, float)
 CDFT_INST_SC(float, double)
 CDFT_INST_SC(double, double)
+template void launch_xfwd<float, double, float>(const cx<float>*, cx<float>*, const float*, const cx<double>*,
+                                                const Geom&, const FftPlan&, const cx<double>*, const cx<float>*,
+                                                ChannelSet, hipStream_t);
 
 template void launch_zpoint<float, double, float>(cx<float>*, const double*, const double*, double*, double*,
                                                   const Geom&, const FftPlan&, const cx<double>*, const cx<float>*,

@@ -73,9 +73,10 @@ void launch_lines_split(cx<float>* Sp, long nlines, const FftPlan& p, const cx<d
 template <typename S, typename C>
 void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st);
 // wt: FP64 radial weights (w3, w2) per element, from launch_wtable_general.
-template <typename S, typename C>
+// The scalar channels' x transforms run in C, the vector and tensor ones in CV.
+template <typename S, typename C, typename CV>
 void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, const Geom& g, const FftPlan& px,
-                 const cx<C>* twx, ChannelSet cs, hipStream_t st);
+                 const cx<C>* twx, const cx<CV>* twv, ChannelSet cs, hipStream_t st);
 template <typename S, typename C>
 void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g, const FftPlan& px,
                  const cx<C>* twx, int nb, hipStream_t st);
