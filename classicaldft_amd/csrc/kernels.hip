@@ -488,7 +488,7 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
 // combined once per element with the radial factors.  G groups per batched
 // transform; a group's loads combine its channels in registers.
 template <typename S, typename C, int EPT, int G>
-__global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
+__global__ void __launch_bounds__(NT, (EPT > 8 ? 1 : sizeof(C) == 4 ? 3 : 2)) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
                                              const cx<double>* __restrict__ wt, Geom g, FftPlan px,
                                              const cx<C>* __restrict__ twx, int ty) {
   constexpr int NG = 6;
