@@ -1125,7 +1125,21 @@ void launch_lines_split(cx<float>* Sp, long nlines, const FftPlan& p, const cx<d
                      nlb, sg, lo0, lo1);
 }
 
+// Dynamic LDS above 64 KiB is enabled per kernel instantiation, once per process.
+template <typename K>
+static void kg_smem(K kernel, int bytes) {
+  static std::mutex mu;
+  static std::vector<const void*> done;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lk(mu);
+  if (std::find(done.begin(), done.end(), key) != done.end()) return;
+  if (hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
+    throw std::runtime_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
+  done.push_back(key);
+}
+
 static constexpr int kXTile = 16;  // y columns per x-line block (128-byte rows in FP32)
+static constexpr size_t kLdsBytes = 160 * 1024;  // LDS of one workgroup on gfx950
 
 template <typename S, typename C>
 void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st) {
@@ -1146,12 +1160,15 @@ void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, 
   const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;  // sources per batched transform
   const size_t sh = sizeof(cx<C>) * (G + (cs.hs ? 1 : 0)) * ty * (g.nx + 1);
+  if (sh > kLdsBytes) throw std::runtime_error("x tile too large for the LDS of the forward k-space kernel");
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
-  if (ty * g.nx <= 8 * NT)
+  if (ty * g.nx <= 8 * NT) {
+    kg_smem(k_xfwd<S, C, CV, 8, G>, static_cast<int>(kLdsBytes));
     hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs);
-  else if (ty * g.nx <= 16 * NT)
+  } else if (ty * g.nx <= 16 * NT) {
+    kg_smem(k_xfwd<S, C, CV, 16, G>, static_cast<int>(kLdsBytes));
     hipLaunchKernelGGL((k_xfwd<S, C, CV, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs);
-  else
+  } else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
 
@@ -1162,12 +1179,15 @@ void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g
   const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;  // groups per batched transform
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
+  if (sh > kLdsBytes) throw std::runtime_error("x tile too large for the LDS of the backward k-space kernel");
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
-  if (ty * g.nx <= 8 * NT)
+  if (ty * g.nx <= 8 * NT) {
+    kg_smem(k_xbwd<S, C, 8, G>, static_cast<int>(kLdsBytes));
     hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty);
-  else if (ty * g.nx <= 16 * NT)
+  } else if (ty * g.nx <= 16 * NT) {
+    kg_smem(k_xbwd<S, C, 16, G>, static_cast<int>(kLdsBytes));
     hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty);
-  else
+  } else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
 
@@ -1210,19 +1230,6 @@ static void zpoint_go(unsigned nblk, size_t sh, hipStream_t st, cx<S>* SF, const
                       DcTable dc) {
   hipLaunchKernelGGL((k_zpoint<S, CF, CB, PPT, NF, NB>), dim3(nblk), dim3(NT), sh, st, SF, rho, vext, lf, partials,
                      g, pz, twf, twb, nlb, fp, mu, amap, shift_sum, dc);
-}
-
-// Dynamic LDS above 64 KiB is enabled per kernel instantiation, once per process.
-template <typename K>
-static void kg_smem(K kernel, int bytes) {
-  static std::mutex mu;
-  static std::vector<const void*> done;
-  const void* key = reinterpret_cast<const void*>(kernel);
-  std::lock_guard<std::mutex> lk(mu);
-  if (std::find(done.begin(), done.end(), key) != done.end()) return;
-  if (hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)
-    throw std::runtime_error("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
-  done.push_back(key);
 }
 
 template <typename S, typename CF, typename CB, int L, int NF, int NB>
