@@ -416,6 +416,10 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
   auto* SO = static_cast<cx<S>*>(d_SO);
   const long ylines = static_cast<long>(g.nzh) * g.nx;
   const bool backward = force != nullptr && cs.nb > 0;
+  // channel arrays in SF: the reduced sets of the kg_zpoint point kernel
+  // (forward without v_z and T_zz, 9 backward) or all 12 / 11
+  const bool red = zpoint_reduced(g.nz, cs);
+  const int nfs = red ? cs.nf - 2 : cs.nf, nbs = red ? 9 : cs.nb;
   auto mark = [&](int i) {
     if (stage_events) CDFT_HIP(hipEventRecord(stage_events[i], stream));
   };
@@ -429,18 +433,19 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     launch_lines<S, CL>(S0, ylines, plan[1], tw<CL>(1), -1, stream);
     mark(3);
     launch_xfwd<S, CE, CL>(S0, SF, static_cast<const S*>(d_mf), static_cast<const cx<double>*>(d_WTg), g, plan[0],
-                           tw<CE>(0), tw<CL>(0), cs, stream);
+                           tw<CE>(0), tw<CL>(0), cs, red, stream);
     mark(4);
     if constexpr (std::is_same_v<S, float> && std::is_same_v<CE, double> && std::is_same_v<CL, float>) {
       // mixed: the y-inverse of the scalar channels (eta, n2, mean field) in
-      // FP64, of the vector and tensor channels (2..10) in FP32, as in the
-      // fused schedule
+      // FP64, of the vector and tensor channels (arrays 2 .. 8 or 10) in FP32,
+      // as in the fused schedule
       if (cs.hs)
-        launch_lines_split(SF, ylines * cs.nf, plan[1], tw<CE>(1), tw<CL>(1), 2 * ylines, 11 * ylines, +1, stream);
+        launch_lines_split(SF, ylines * nfs, plan[1], tw<CE>(1), tw<CL>(1), 2 * ylines, (red ? 9 : 11) * ylines, +1,
+                           stream);
       else
-        launch_lines<S, CE>(SF, ylines * cs.nf, plan[1], tw<CE>(1), +1, stream);
+        launch_lines<S, CE>(SF, ylines * nfs, plan[1], tw<CE>(1), +1, stream);
     } else {
-      launch_lines<S, CE>(SF, ylines * cs.nf, plan[1], tw<CE>(1), +1, stream);
+      launch_lines<S, CE>(SF, ylines * nfs, plan[1], tw<CE>(1), +1, stream);
     }
   }
   ChannelSet cz = cs;
@@ -451,9 +456,9 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
                            cs.nf > 0 ? d_shift : nullptr, dc, stream);
   if (backward) {
     mark(6);
-    launch_lines<S, CL>(SF, ylines * cs.nb, plan[1], tw<CL>(1), -1, stream);
+    launch_lines<S, CL>(SF, ylines * nbs, plan[1], tw<CL>(1), -1, stream);
     mark(7);
-    launch_xbwd<S, CL>(SF, SO, static_cast<const cx<double>*>(d_WTg), g, plan[0], tw<CL>(0), cs.nb, stream);
+    launch_xbwd<S, CL>(SF, SO, static_cast<const cx<double>*>(d_WTg), g, plan[0], tw<CL>(0), nbs, stream);
     mark(8);
     launch_lines<S, CL>(SO, ylines, plan[1], tw<CL>(1), +1, stream);
     mark(9);
@@ -486,17 +491,20 @@ void Context::stage_bytes(double* b, bool with_vext) const {
     }
     return;
   }
+  // the kg_zpoint point kernel's reduced channel sets
+  const bool red = zpoint_reduced(g.nz, cs);
+  const double nfs = red ? nf - 2.0 : nf, nbs = red ? 9.0 : nb;
   if (cs.nf > 0) {
     b[0] = 8.0 * N;                                     // FP64 sum of rho
     b[1] = 8.0 * N + sf * M;                            // rho (FP64) -> rho_hat
     b[2] = 2.0 * sf * M;                                // in place
-    b[3] = sf * M + nf * sf * M + (cs.mf ? sf * M : 0);  // rho_hat + table (value, residual) -> nf channels
-    b[4] = 2.0 * nf * sf * M;
+    b[3] = sf * M + nfs * sf * M + (cs.mf ? sf * M : 0);  // rho_hat + table (value, residual) -> channels
+    b[4] = 2.0 * nfs * sf * M;
   }
-  b[5] = nf * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nb * sf * M + 8.0 * N;  // channels, rho, V -> derivs, local
+  b[5] = nfs * sf * M + 8.0 * N * (with_vext ? 2.0 : 1.0) + nbs * sf * M + 8.0 * N;  // channels, rho, V -> derivs, local
   if (cs.nb > 0) {
-    b[6] = 2.0 * nb * sf * M;
-    b[7] = nb * sf * M + sf * M;
+    b[6] = 2.0 * nbs * sf * M;
+    b[7] = nbs * sf * M + sf * M;
     b[8] = 2.0 * sf * M;
     b[9] = sf * M + 8.0 * N + 8.0 * N;  // spectrum + local -> force
   }
@@ -596,7 +604,7 @@ void wd_t(Context* c, const double* d_rho_in, double* d_out) {
   launch_zr2c<S, CL>(d_rho_in, S0, g, c->plan[2], c->tw<CL>(2), id, c->d_shift, c->stream);
   launch_lines<S, CL>(S0, ylines, c->plan[1], c->tw<CL>(1), -1, c->stream);
   launch_xfwd<S, CE, CE>(S0, SF, static_cast<const S*>(c->d_mf), static_cast<const cx<double>*>(c->d_WTg), g,
-                         c->plan[0], c->tw<CE>(0), c->tw<CE>(0), c->cs, c->stream);
+                         c->plan[0], c->tw<CE>(0), c->tw<CE>(0), c->cs, false, c->stream);
   launch_lines<S, CE>(SF, ylines * c->cs.nf, c->plan[1], c->tw<CE>(1), +1, c->stream);
   launch_zc2r<S, CE>(SF, c->cs.nf, d_out, nullptr, 1.0, g, c->plan[2], c->tw<CE>(2), c->d_shift, c->dc, c->stream);
 }

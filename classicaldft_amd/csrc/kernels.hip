@@ -320,6 +320,23 @@ __global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, F
   }
 }
 
+// Channel order of kg_zpoint with the hard-sphere channels.  Forward round
+// positions hold the channels (index as in the 12-channel set: 0 eta, 1 n2,
+// 2..4 v, 5..10 T, 11 mean field) eta, n2, [mf,] v_z, v_x, v_y, T_xx, T_xy,
+// T_xz, T_yy, T_yz; T_zz comes from the trace identity.  SF holds the reduced
+// set: eta, n2, v_x, v_y, T_xx, T_xy, T_xz, T_yy, T_yz[, mf] (v_z is formed from
+// eta's spectrum); backward: the same nine, eta carrying the v_z term and n2,
+// T_xx, T_yy the T_zz derivative.
+__host__ __device__ constexpr int kg_fwd_channel(int nf, int pos) {
+  return nf == 12 ? (pos < 2 ? pos : pos == 2 ? 11 : pos == 3 ? 4 : pos < 6 ? pos - 2 : pos - 1)
+                  : (pos < 2 ? pos : pos == 2 ? 4 : pos < 5 ? pos - 1 : pos);
+}
+__host__ __device__ constexpr int kg_fwd_slot(int ch) {  // -1: v_z, T_zz (not stored)
+  return ch < 4 ? ch : ch == 4 || ch == 10 ? -1 : ch == 11 ? 9 : ch - 1;
+}
+__host__ __device__ constexpr int kg_bwd_channel(int pos) { return pos == 0 ? 0 : pos == 1 ? 4 : pos < 5 ? pos - 1 : pos; }
+__host__ __device__ constexpr int kg_bwd_slot(int ch) { return kg_fwd_slot(ch); }
+
 // Fused forward k-space stage: x-FFT of rho_hat, x-inverse transforms of rho_hat
 // times the channel weights, every channel stored (S0 -> SF).
 //
@@ -340,7 +357,7 @@ template <typename S, typename C, typename CV, int EPT, int G>
 __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
                                              const S* __restrict__ mf, const cx<double>* __restrict__ wt, Geom g,
                                              FftPlan px, const cx<C>* __restrict__ twx,
-                                             const cx<CV>* __restrict__ twv, int ty, ChannelSet cs) {
+                                             const cx<CV>* __restrict__ twv, int ty, ChannelSet cs, int red) {
   const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
@@ -380,8 +397,11 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
   // field, only 2 when there is no hard-sphere term
   const int nsrc = cs.hs ? 6 + (cs.mf ? 1 : 0) : (cs.mf ? 1 : 0);
   auto src = [&](int i) { return cs.hs ? (cs.mf || i < 2 ? i : i + 1) : 2; };
-  const int mf_slot = cs.hs ? 11 : 0;
-  auto out_ch = [&](int c) { return SF + static_cast<long>(c) * g.M + plane_off; };
+  // channel c's array in SF: the 12-channel order, or with `red` the reduced
+  // set of the kg_zpoint point kernel (no v_z, T_zz: oslot < 0)
+  auto oslot = [&](int c) { return red ? kg_fwd_slot(c) : c; };
+  const int mf_slot = cs.hs ? oslot(11) : 0;
+  auto out_ch = [&](int c) { return SF + static_cast<long>(oslot(c)) * g.M + plane_off; };
   // A batch of ng sources from list position c0 in arithmetic CR: the LDS
   // buffer holds GR slabs of CR (G slabs of C).
   auto run_batch = [&](auto zero, const auto* __restrict__ twr, int c0, int ng) {
@@ -437,10 +457,10 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
             keep[yl * ls + x] = v;
             out_ch(0)[o] = cvt<S>(v);
             out_ch(3)[o] = cvt<S>(cx<C>{-ky * v.im, ky * v.re});
-            out_ch(4)[o] = cvt<S>(cx<C>{-kzc * v.im, kzc * v.re});
+            if (!red) out_ch(4)[o] = cvt<S>(cx<C>{-kzc * v.im, kzc * v.re});
             break;
           case 1: out_ch(1)[o] = cvt<S>(v); break;
-          case 2: out_ch(mf_slot)[o] = cvt<S>(v); break;
+          case 2: (SF + static_cast<long>(mf_slot) * g.M + plane_off)[o] = cvt<S>(v); break;
           case 3: out_ch(2)[o] = cvt<S>(v); break;
           case 4: {
             const cx<C> k = keep[yl * ls + x];
@@ -456,7 +476,7 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
             const C yy = ky * ky, yz = ky * kzc, zz = kzc * kzc;
             out_ch(8)[o] = cvt<S>(cx<C>{yy * v.re + ir * k.re, yy * v.im + ir * k.im});
             out_ch(9)[o] = cvt<S>(cx<C>{yz * v.re, yz * v.im});
-            out_ch(10)[o] = cvt<S>(cx<C>{zz * v.re + ir * k.re, zz * v.im + ir * k.im});
+            if (!red) out_ch(10)[o] = cvt<S>(cx<C>{zz * v.re + ir * k.re, zz * v.im + ir * k.im});
             break;
           }
         }
@@ -490,7 +510,7 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
 template <typename S, typename C, int EPT, int G>
 __global__ void __launch_bounds__(NT, (EPT > 8 ? 1 : sizeof(C) == 4 ? 3 : 2)) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
                                              const cx<double>* __restrict__ wt, Geom g, FftPlan px,
-                                             const cx<C>* __restrict__ twx, int ty) {
+                                             const cx<C>* __restrict__ twx, int ty, int red) {
   constexpr int NG = 6;
   const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
@@ -515,7 +535,33 @@ __global__ void __launch_bounds__(NT, (EPT > 8 ? 1 : sizeof(C) == 4 ? 3 : 2)) k_
       const C ir = static_cast<C>(invR);
       for (int gi = 0; gi < ng; ++gi) {
         cx<C> p = {C(0), C(0)};
-        if (y < ny) {
+        if (y < ny && red) {
+          // the nine channels of kg_zpoint (eta with the v_z term, n2, T_xx and
+          // T_yy with the T_zz derivative): 0 eta, 1 n2, 2 v_x, 3 v_y, 4 T_xx,
+          // 5 T_xy, 6 T_xz, 7 T_yy, 8 T_yz
+          switch (c0 + gi) {  // block-uniform
+            case 0: {
+              const cx<C> ge = in_ch(0, o), gy = in_ch(3, o);
+              const cx<C> t = cadd(in_ch(4, o), in_ch(7, o));
+              p = {ge.re + ky * gy.im + ir * t.re, ge.im - ky * gy.re + ir * t.im};
+              break;
+            }
+            case 1: p = in_ch(2, o); break;
+            case 2: p = in_ch(1, o); break;
+            case 3: p = in_ch(4, o); break;
+            case 4: {
+              const cx<C> a = in_ch(5, o), b = in_ch(6, o);
+              p = {ky * a.re + kzc * b.re, ky * a.im + kzc * b.im};
+              break;
+            }
+            default: {
+              const cx<C> a = in_ch(7, o), b = in_ch(8, o);
+              const C yy = ky * ky, yz = ky * kzc;
+              p = {yy * a.re + yz * b.re, yy * a.im + yz * b.im};
+              break;
+            }
+          }
+        } else if (y < ny) {
           switch (c0 + gi) {  // block-uniform
             case 0: {
               const cx<C> ge = in_ch(0, o), gy = in_ch(3, o), gz = in_ch(4, o);
@@ -816,7 +862,11 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
   constexpr int MFC = HS ? 11 : 0;
   constexpr int NFA = NF > 0 ? NF : 1;
   constexpr int CMAX = sizeof(cx<CF>) > sizeof(cx<CB>) ? sizeof(cx<CF>) : sizeof(cx<CB>);
-  constexpr int NRF = (NF + 3) / 4, NRB = (NB + 3) / 4;
+  // With the hard-sphere channels SF holds the reduced set (kg_fwd_slot,
+  // kg_bwd_slot): T_zz follows from the trace identity at the points, n_v,z
+  // from eta's staged spectrum, and their derivatives fold into the others.
+  constexpr int NPF = HS ? NF - 1 : NF, NPB = HS && NB > 0 ? 10 : NB;
+  constexpr int NRF = (NPF + 3) / 4, NRB = (NPB + 3) / 4;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   cx<S>* stage = reinterpret_cast<cx<S>*>(smem_raw);  // [cr][kz][line]
   const int tid = static_cast<int>(threadIdx.x);
@@ -833,11 +883,11 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
 
   // ---- forward rounds.  Round R + 1's half spectra are loaded into registers
   // before round R's transforms, so their memory latency overlaps that work.
-  // Round positions: with the mean field (NF 12) the scalar channels eta, n2
-  // and the mean field fill round 0, which runs in CF; the vector and tensor
-  // channels of the later rounds run in CB (FP32 in mixed precision, as in the
-  // fused schedule).
-  auto fch = [](int pos) { return NF == 12 ? (pos < 2 ? pos : (pos == 2 ? 11 : pos - 1)) : pos; };
+  // Round 0 holds the scalar channels eta, n2 (and the mean field) and runs in
+  // CF; the vector and tensor channels of the later rounds run in CB (FP32 in
+  // mixed precision, as in the fused schedule).
+  auto fch = [](int pos) { return HS ? kg_fwd_channel(NF, pos) : pos; };               // channel (wv index)
+  auto fsf = [](int pos) { return HS ? kg_fwd_slot(kg_fwd_channel(NF, pos)) : pos; };  // SF slot, -1: from eta
   constexpr int NSTG = 4 * NZH * LINES, KPF = (NSTG + K::NTH - 1) / K::NTH;
   cx<S> pf[KPF];
   auto load_round = [&](int R) {
@@ -848,8 +898,8 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
       const int kz = rem / LINES, ln = rem - kz * LINES;
       const int pos = 4 * R + c4;
       pf[k] = {S(0), S(0)};
-      if (idx < NSTG && pos < NF && l0 + ln < nxy)
-        pf[k] = SF[static_cast<long>(fch(pos)) * g.M + static_cast<long>(kz) * nxy + l0 + ln];
+      if (idx < NSTG && pos < NPF && fsf(pos) >= 0 && l0 + ln < nxy)
+        pf[k] = SF[static_cast<long>(fsf(pos)) * g.M + static_cast<long>(kz) * nxy + l0 + ln];
     }
   };
   auto store_round = [&]() {
@@ -865,8 +915,10 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
     using CR = decltype(zero);
     const int pos = 4 * R + cr;
     cx<CR>* xb = reinterpret_cast<cx<CR>*>(sbuf(slot));
-    if (pos < NF) {
-      const cx<S>* st = stage + cr * NZH * LINES;
+    if (pos < NPF) {
+      // n_v,z = i kz eta: eta's staged spectrum (position 0 of round 0) times i kz
+      const bool vz = fsf(pos) < 0;
+      const cx<S>* st = stage + (vz ? 0 : cr) * NZH * LINES;
       cx<CR> v[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) {
@@ -874,6 +926,11 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
         const bool mirror = k >= NZH;
         const int kk = mirror ? L - k : k;
         cx<CR> A = cvt<CR>(st[kk * LINES + la]), B = cvt<CR>(st[kk * LINES + la + 1]);
+        if (vz) {
+          const CR kzp = static_cast<CR>(g.dkz * kk);
+          A = {-kzp * A.im, kzp * A.re};
+          B = {-kzp * B.im, kzp * B.re};
+        }
         if (kk == 0 || 2 * kk == L) {
           A.im = CR(0);
           B.im = CR(0);
@@ -894,7 +951,7 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
     // wave's line pairs (line pair f / 2L, z = (f % 2L) / 2, line (f % 2L) & 1)
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
-      if (4 * R + c4 < NF) {
+      if (4 * R + c4 < NPF) {
         const int ch = fch(4 * R + c4);
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
@@ -955,11 +1012,16 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
       if constexpr (HS) {
         double w[11], gg[11];
 #pragma unroll
-        for (int i = 0; i < 11; ++i) w[i] = static_cast<double>(wv[i][j]) + dc.v[i] * shift;
+        for (int i = 0; i < 10; ++i) w[i] = static_cast<double>(wv[i][j]) + dc.v[i] * shift;
+        w[10] = w[1] - w[5] - w[8];  // T_zz = n2 - T_xx - T_yy (weights.hpp:121-137,170-178)
         const double phi = fmt_phi_grad(fp, w, gg);
         two_sum(acc[2], cmp[2], phi);
+        // its derivative folds into n2, T_xx and T_yy
+        gg[1] += gg[10];
+        gg[5] -= gg[10];
+        gg[8] -= gg[10];
 #pragma unroll
-        for (int i = 0; i < 11; ++i) out[i] = static_cast<S>(gg[i]);
+        for (int i = 0; i < 10; ++i) out[i] = static_cast<S>(gg[i]);
       }
       local_force[p] = lfv;
     }
@@ -978,13 +1040,16 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
     if (tid < 8) partials[static_cast<long>(blockIdx.x) * 8 + tid] = tid < 5 ? out5[tid] * dv : 0.0;
   }
 
-  // ---- backward rounds: R2C of the combined derivatives, written in place over SF's first channels
+  // ---- backward rounds: R2C of the combined derivatives, written in place over SF's first
+  // channels.  Positions: eta, n_v,z, n2, v_x, v_y, T_xx, T_xy, T_xz, T_yy, T_yz (kg_bwd_slot);
+  // n_v,z's spectrum times -i kz joins eta's when they are stored.
+  auto bch = [](int pos) { return kg_bwd_channel(pos); };
 #pragma unroll
   for (int R = 0; R < NRB; ++R) {
 #pragma unroll
     for (int c4 = 0; c4 < 4; ++c4) {
-      const int ch = 4 * R + c4;
-      if (ch < NB) {
+      if (4 * R + c4 < NPB) {
+        const int ch = bch(4 * R + c4);
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
           const int f = lid + 64 * j, pl = f / (2 * L), r2 = f - pl * 2 * L;
@@ -997,7 +1062,7 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
     if (R > 0) __syncthreads();  // the previous round's store is done with the staging buffer
     const int c = 4 * R + cr;
     cx<CB>* xb = reinterpret_cast<cx<CB>*>(sbuf(slot));
-    if (c < NB) {
+    if (c < NPB) {
       cx<CB> v[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) v[q] = xb[lane + T * q];
@@ -1020,12 +1085,20 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
       }
     }
     __syncthreads();
-    const int nc = min(4, NB - 4 * R);
+    const int nc = min(4, NPB - 4 * R);
     for (int idx = tid; idx < nc * NZH * LINES; idx += K::NTH) {
       const int c4 = idx / (NZH * LINES), rem = idx - c4 * (NZH * LINES);
       const int kz = rem / LINES, ln = rem - kz * LINES;
-      if (l0 + ln < nxy)
-        SF[static_cast<long>(4 * R + c4) * g.M + static_cast<long>(kz) * nxy + l0 + ln] = stage[idx];
+      const int sl = kg_bwd_slot(bch(4 * R + c4));
+      if (l0 + ln < nxy && sl >= 0) {
+        cx<S> v = stage[idx];
+        if (R == 0 && c4 == 0) {  // eta + (-i kz) n_v,z (position 1)
+          const cx<S> z = stage[idx + NZH * LINES];
+          const S kzp = static_cast<S>(g.dkz * kz);
+          v = {v.re + kzp * z.im, v.im - kzp * z.re};
+        }
+        SF[static_cast<long>(sl) * g.M + static_cast<long>(kz) * nxy + l0 + ln] = v;
+      }
     }
   }
 }
@@ -1156,7 +1229,8 @@ void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const c
 
 template <typename S, typename C, typename CV>
 void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, const Geom& g, const FftPlan& px,
-                 const cx<C>* twx, const cx<CV>* twv, ChannelSet cs, hipStream_t st) {
+                 const cx<C>* twx, const cx<CV>* twv, ChannelSet cs, bool red, hipStream_t st) {
+  const int rd = red && cs.hs ? 1 : 0;
   const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;  // sources per batched transform
   const size_t sh = sizeof(cx<C>) * (G + (cs.hs ? 1 : 0)) * ty * (g.nx + 1);
@@ -1164,10 +1238,12 @@ void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, 
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   if (ty * g.nx <= 8 * NT) {
     kg_smem(k_xfwd<S, C, CV, 8, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs);
+    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
+                       rd);
   } else if (ty * g.nx <= 16 * NT) {
     kg_smem(k_xfwd<S, C, CV, 16, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xfwd<S, C, CV, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs);
+    hipLaunchKernelGGL((k_xfwd<S, C, CV, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
+                       rd);
   } else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
@@ -1175,7 +1251,9 @@ void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, 
 template <typename S, typename C>
 void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g, const FftPlan& px,
                  const cx<C>* twx, int nb, hipStream_t st) {
-  if (nb != 11) throw std::runtime_error("the backward x kernel takes the 11 hard-sphere derivative channels");
+  if (nb != 11 && nb != 9)
+    throw std::runtime_error("the backward x kernel takes the 11 hard-sphere derivative channels or kg_zpoint's 9");
+  const int red = nb == 9 ? 1 : 0;
   const int ty = px.tile > 0 ? px.tile : kXTile;
   constexpr int G = sizeof(C) == 4 ? 2 : 1;  // groups per batched transform
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
@@ -1183,10 +1261,10 @@ void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   if (ty * g.nx <= 8 * NT) {
     kg_smem(k_xbwd<S, C, 8, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty);
+    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, red);
   } else if (ty * g.nx <= 16 * NT) {
     kg_smem(k_xbwd<S, C, 16, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty);
+    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, red);
   } else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
@@ -1205,6 +1283,7 @@ static int zpoint_nlb(const FftPlan& pz) {
 
 // z lengths with the register-resident point kernel (kg_zpoint)
 static bool kg_length(int L) { return L == 48 || L == 80 || L == 96 || L == 160; }
+bool zpoint_reduced(int nz, const ChannelSet& cs) { return cs.hs && kg_length(nz); }
 template <typename S>
 static int kg_lines(int L) {
   switch (L) {
@@ -1327,7 +1406,7 @@ void launch_finalize(const double* partials, int nrows, double* res, hipStream_t
   template void launch_lines<S, C>(cx<S>*, long, const FftPlan&, const cx<C>*, int, hipStream_t);                 \
   template void launch_xlines<S, C>(cx<S>*, int, const Geom&, const FftPlan&, const cx<C>*, int, hipStream_t);    \
   template void launch_xfwd<S, C, C>(const cx<S>*, cx<S>*, const S*, const cx<double>*, const Geom&,               \
-                                     const FftPlan&, const cx<C>*, const cx<C>*, ChannelSet, hipStream_t);        \
+                                     const FftPlan&, const cx<C>*, const cx<C>*, ChannelSet, bool, hipStream_t);  \
   template void launch_xbwd<S, C>(const cx<S>*, cx<S>*, const cx<double>*, const Geom&, const FftPlan&,           \
                                   const cx<C>*, int, hipStream_t);
 
@@ -1336,7 +1415,7 @@ CDFT_INST_SC(float, double)
 CDFT_INST_SC(double, double)
 template void launch_xfwd<float, double, float>(const cx<float>*, cx<float>*, const float*, const cx<double>*,
                                                 const Geom&, const FftPlan&, const cx<double>*, const cx<float>*,
-                                                ChannelSet, hipStream_t);
+                                                ChannelSet, bool, hipStream_t);
 
 template void launch_zpoint<float, double, float>(cx<float>*, const double*, const double*, double*, double*,
                                                   const Geom&, const FftPlan&, const cx<double>*, const cx<float>*,

@@ -74,9 +74,14 @@ template <typename S, typename C>
 void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const cx<C>* twx, int sg, hipStream_t st);
 // wt: FP64 radial weights (w3, w2) per element, from launch_wtable_general.
 // The scalar channels' x transforms run in C, the vector and tensor ones in CV.
+// With `red` (hard spheres) SF receives the reduced channel set of the kg_zpoint
+// point kernel (zpoint_reduced): no v_z, T_zz, the mean field after T_yz.
 template <typename S, typename C, typename CV>
 void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, const Geom& g, const FftPlan& px,
-                 const cx<C>* twx, const cx<CV>* twv, ChannelSet cs, hipStream_t st);
+                 const cx<C>* twx, const cx<CV>* twv, ChannelSet cs, bool red, hipStream_t st);
+// True when the point kernel of a z extent nz reads and writes the reduced
+// channel sets (10 or 9 forward channels, 9 backward).
+bool zpoint_reduced(int nz, const ChannelSet& cs);
 template <typename S, typename C>
 void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g, const FftPlan& px,
                  const cx<C>* twx, int nb, hipStream_t st);
