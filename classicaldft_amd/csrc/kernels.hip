@@ -22,12 +22,15 @@
 //   k_sum + k_finalize   FP64 sum of rho (mean-density split)
 //   k_zr2c      z-R2C of rho - mean                             (real -> S0)
 //   k_lines     y-FFT of S0
-//   k_xfwd      x-FFT, multiply by all weights, inverse x-FFTs  (S0 -> SF[nf])
+//   k_xfwd      x-FFT, 7 weighted inverse x-FFTs -> channels   (S0 -> SF[nf])
 //   k_lines     inverse y-FFT of the nf channels
 //   k_zpoint    per z-line: nf C2R, pointwise Phi/dPhi + local terms + partial
 //               sums, 11 R2C of the combined derivatives        (SF -> SF[11], in place)
 //   k_lines     y-FFT of the 11 channels
-//   k_xbwd      x-FFT, conj-weighted channel sum, inverse x-FFT (SF -> SO)
+//   k_xbwd      6 grouped x-FFTs, conj-weighted sum, inverse x-FFT (SF -> SO)
+// For the z extents of kg_zpoint (48, 80, 96, 160) SF holds reduced channel
+// sets instead: nf - 2 forward (v_z and T_zz are formed in the point kernel)
+// and 9 backward (their derivatives folded into the others).
 //   k_lines     inverse y-FFT of SO
 //   k_zc2r      z-C2R, * dV, + local force                      (SO -> force)
 //   k_finalize  deterministic compensated sum of the block partials
