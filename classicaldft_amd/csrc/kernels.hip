@@ -442,10 +442,10 @@ __global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __
       }
     }
     __syncthreads();
-    // The LDS-pass transform: with the lane's spectrum and weights live
-    // across the source loop, the register-resident one measured slower
-    // here (160^3: 502 against 308 us per launch).
-    fft_inplace<CR, EPT * GR>(bufr, ng * ty, ls, px, +1, twr);
+    // register-resident slot transforms where the length has them (with 12
+    // channel transforms per block they measured slower than the LDS passes,
+    // 160^3: 502 against 308 us; with the 7 sources they are faster)
+    fft_lines<CR, EPT * GR>(bufr, ng * ty, ls, px, +1, twr);
     for (int gi = 0; gi < ng; ++gi) {
       const int sc = src(c0 + gi);
       const cx<CR>* vb = bufr + gi * slab;
