@@ -850,7 +850,7 @@ struct KgCfg {
 };
 
 template <typename S, typename CF, typename CB, int L, int NF, int NB>
-__global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
+__global__ void __launch_bounds__((KgCfg<L, S>::NTH), (sizeof(CF) > sizeof(CB) ? 3 : 2)) kg_zpoint(cx<S>* SF, const double* __restrict__ rho_in,
                                                               const double* __restrict__ vext,
                                                               double* __restrict__ local_force,
                                                               double* __restrict__ partials, Geom g,
@@ -864,7 +864,14 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
   constexpr bool MF = (NF == 12 || NF == 1);
   constexpr int MFC = HS ? 11 : 0;
   constexpr int NFA = NF > 0 ? NF : 1;
-  constexpr int CMAX = sizeof(cx<CF>) > sizeof(cx<CB>) ? sizeof(cx<CF>) : sizeof(cx<CB>);
+  // Slot buffers hold CB values.  With CF wider (mixed precision) round 0's CF
+  // transforms use the staging area and the slot buffers together: wave w's
+  // slot s at (w NS + s) L sizeof(cx<CF>), once every wave has read its round-0
+  // inputs from the staging area (three 4-wave blocks per CU instead of two).
+  static_assert(sizeof(CB) <= sizeof(CF), "the backward arithmetic is not wider than the forward one");
+  constexpr int XBE = sizeof(cx<CB>);
+  static_assert(K::W * K::NS * L * static_cast<int>(sizeof(cx<CF>)) <= K::stage_bytes + K::xb_bytes(XBE),
+                "round 0's buffers fit the staging area and the slot buffers");
   // With the hard-sphere channels SF holds the reduced set (kg_fwd_slot,
   // kg_bwd_slot): T_zz follows from the trace identity at the points, n_v,z
   // from eta's staged spectrum, and their derivatives fold into the others.
@@ -876,9 +883,10 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), lid = tid & 63;
   const int slot = lid / T, lane = lid % T;
   const int cr = slot / LPW, lp = slot % LPW;
-  unsigned char* xwave = smem_raw + K::stage_bytes + wave * K::NS * L * CMAX;
-  // slot buffer of slot s of this wave (L complex values of the larger type)
-  auto sbuf = [&](int s) { return xwave + s * L * CMAX; };
+  unsigned char* xwave = smem_raw + K::stage_bytes + wave * K::NS * L * XBE;
+  // slot buffer of slot s of this wave (L complex CB values), and its round-0 CF buffer
+  auto sbuf = [&](int s) { return xwave + s * L * XBE; };
+  auto wbuf = [&](int s) { return smem_raw + (wave * K::NS + s) * L * static_cast<int>(sizeof(cx<CF>)); };
   const long l0 = static_cast<long>(blockIdx.x) * LINES;
   const long nxy = g.nxy;
   const int la = 2 * (wave * LPW + lp);  // the slot's line pair, block-local
@@ -916,13 +924,15 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
   // of its line pair; then every lane reads its points of the round's channels
   auto fwd_round = [&](auto zero, const auto* __restrict__ tw, int R) {
     using CR = decltype(zero);
+    constexpr bool WIDE = sizeof(cx<CR>) > XBE;  // round 0 in CF, buffers over the staging area
+    auto rbuf = [&](int s) { return WIDE ? wbuf(s) : sbuf(s); };
     const int pos = 4 * R + cr;
-    cx<CR>* xb = reinterpret_cast<cx<CR>*>(sbuf(slot));
+    cx<CR>* xb = reinterpret_cast<cx<CR>*>(rbuf(slot));
+    cx<CR> v[E];
     if (pos < NPF) {
       // n_v,z = i kz eta: eta's staged spectrum (position 0 of round 0) times i kz
       const bool vz = fsf(pos) < 0;
       const cx<S>* st = stage + (vz ? 0 : cr) * NZH * LINES;
-      cx<CR> v[E];
 #pragma unroll
       for (int q = 0; q < E; ++q) {
         const int k = lane + T * q;
@@ -944,6 +954,9 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
         }
         v[q] = {A.re - B.im, A.im + B.re};
       }
+    }
+    if constexpr (WIDE) __syncthreads();  // every wave has read its inputs from the staging area
+    if (pos < NPF) {
       slot_fft<CR, L, T, +1, false, false>(v, xb, lane, tw);
       wave_sync();
 #pragma unroll
@@ -959,7 +972,7 @@ __global__ void __launch_bounds__((KgCfg<L, S>::NTH)) kg_zpoint(cx<S>* SF, const
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
           const int f = lid + 64 * j, pl = f / (2 * L), r2 = f - pl * 2 * L;
-          const CR* src = reinterpret_cast<const CR*>(sbuf(c4 * LPW + pl));
+          const CR* src = reinterpret_cast<const CR*>(rbuf(c4 * LPW + pl));
           wv[ch][j] = static_cast<S>(src[r2]);
         }
       }
@@ -1319,8 +1332,7 @@ static void kg_go(hipStream_t st, cx<S>* SF, const double* rho, const double* ve
                   const Geom& g, const cx<CF>* twf, const cx<CB>* twb, FmtParams fp, double mu, AliasMap amap,
                   const double* shift_sum, DcTable dc) {
   using K = KgCfg<L, S>;
-  constexpr int cmax = static_cast<int>(sizeof(cx<CF>) > sizeof(cx<CB>) ? sizeof(cx<CF>) : sizeof(cx<CB>));
-  const int sh = K::stage_bytes + K::xb_bytes(cmax);
+  const int sh = K::stage_bytes + K::xb_bytes(static_cast<int>(sizeof(cx<CB>)));  // slot buffers of CB values
   auto k = kg_zpoint<S, CF, CB, L, NF, NB>;
   kg_smem(k, sh);
   const unsigned nblk = static_cast<unsigned>((g.nxy + K::LINES - 1) / K::LINES);
