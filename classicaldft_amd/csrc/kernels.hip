@@ -354,10 +354,10 @@ __host__ __device__ constexpr int kg_bwd_slot(int ch) { return kg_fwd_slot(ch); 
 //   4: aik2 kx^2 rho -> T_xx = (.) + eta / R
 //   5: aik2 kx rho   -> T_xy = ky (.), T_xz = kz (.)
 //   6: aik2 rho      -> T_yy = ky^2 (.) + eta / R, T_yz = ky kz (.), T_zz = kz^2 (.) + eta / R
-// G sources per batched transform.  EPT >= ty*nx/NT elements owned per thread.
+// G sources per batched transform.  EPT >= ty*nx/blockDim.x elements owned per thread.
 // The vector and tensor sources (3..6) use arithmetic CV (C: the scalar ones).
 template <typename S, typename C, typename CV, int EPT, int G>
-__global__ void __launch_bounds__(NT, (EPT <= 8 ? 2 : 1)) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
+__global__ void __launch_bounds__(2 * NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
                                              const S* __restrict__ mf, const cx<double>* __restrict__ wt, Geom g,
                                              FftPlan px, const cx<C>* __restrict__ twx,
                                              const cx<CV>* __restrict__ twv, int ty, ChannelSet cs, int red) {
@@ -1252,13 +1252,11 @@ void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, 
   const size_t sh = sizeof(cx<C>) * (G + (cs.hs ? 1 : 0)) * ty * (g.nx + 1);
   if (sh > kLdsBytes) throw std::runtime_error("x tile too large for the LDS of the forward k-space kernel");
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
-  if (ty * g.nx <= 8 * NT) {
+  // up to 8 elements per thread: 256 threads per block, or 512 for larger tiles
+  if (ty * g.nx <= 16 * NT) {
     kg_smem(k_xfwd<S, C, CV, 8, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
-                       rd);
-  } else if (ty * g.nx <= 16 * NT) {
-    kg_smem(k_xfwd<S, C, CV, 16, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xfwd<S, C, CV, 16, G>), grid, dim3(NT), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
+    const int nth = ty * g.nx <= 8 * NT ? NT : 2 * NT;
+    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, G>), grid, dim3(nth), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
                        rd);
   } else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
