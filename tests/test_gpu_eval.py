@@ -73,6 +73,40 @@ def test_general_schedule_slot_lengths_match_oracle(shape, precision):
         assert om < 1e-6 and e < 2e-5
 
 
+# z extents 48, 80, 96 and 160 take the point kernel with the reduced channel
+# sets (T_zz from the trace identity, n_v,z from eta's spectrum, their
+# derivatives folded): every FMT model, with and without the mean field, with an
+# external field, forces and energy-only, both precisions.
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("model,inter,nz", [("WhiteBearII", (LJ,), 48), ("WhiteBearI", (LJ,), 48),
+                                            ("Rosenfeld", (), 48), ("RSLT", (LJ,), 48), ("esFMT", (), 48),
+                                            ("WhiteBearII", (), 80), ("Rosenfeld", (LJ,), 96),
+                                            ("WhiteBearII", (), 160)])
+def test_general_reduced_channels_match_oracle(model, inter, nz, precision):
+    shape, dx = (16, 12, nz), 0.1
+    kw = dict(A=1.0, B=-0.5) if model == "esFMT" else {}
+    ctx = _ctx(shape, dx, model=model, inter=inter, precision=precision, **kw)
+    assert ctx.schedule == "general"
+    orc = _orc(shape, dx, model=model, inter=inter, **kw)
+    rho = W.droplet(shape, dx, radius=0.6, width=0.2, seed=7)
+    V = 0.3 * np.cos(2 * np.pi * np.arange(nz) / nz)[None, None, :] * np.ones(shape)
+    mu = -2.1
+    res, f = ctx.total(rho, mu, vext=V)
+    parts, fr = orc.total(rho, mu, vext=V)
+    res2, _ = ctx.total(rho, mu, vext=V, forces=False)
+    # Omega of these small boxes nearly cancels (F against mu N): relative to its largest part
+    om = abs(res["grand_potential"] - parts[5]) / max(abs(parts[i]) for i in range(5))
+    e = _relmax(f, fr)
+    print(f"{model} {inter and 'LJ'} nz={nz} {precision}: Omega rel err {om:.2e}, force rel err {e:.2e}", flush=True)
+    if precision == "fp64":
+        for key, i in [("ideal", 0), ("external", 1), ("hard_sphere", 2), ("mean_field", 3), ("grand_potential", 5)]:
+            assert res[key] == pytest.approx(parts[i], rel=1e-11, abs=1e-12), key
+        assert e < 1e-11
+    else:
+        assert om < 1e-6 and e < 2e-5
+    assert res2["grand_potential"] == pytest.approx(res["grand_potential"], rel=1e-13)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
 def test_weighted_densities(precision, tol):
     shape, dx = (16, 20, 24), 0.1
