@@ -511,7 +511,7 @@ __global__ void __launch_bounds__(2 * NT) k_xfwd(const cx<S>* __restrict__ S0, c
 // combined once per element with the radial factors.  G groups per batched
 // transform; a group's loads combine its channels in registers.
 template <typename S, typename C, int EPT, int G>
-__global__ void __launch_bounds__(NT, (EPT > 8 ? 1 : sizeof(C) == 4 ? 3 : 2)) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
+__global__ void __launch_bounds__(2 * NT, (sizeof(C) == 4 ? 3 : 2)) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
                                              const cx<double>* __restrict__ wt, Geom g, FftPlan px,
                                              const cx<C>* __restrict__ twx, int ty, int red) {
   constexpr int NG = 6;
@@ -1273,12 +1273,11 @@ void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g
   const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
   if (sh > kLdsBytes) throw std::runtime_error("x tile too large for the LDS of the backward k-space kernel");
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
-  if (ty * g.nx <= 8 * NT) {
+  // up to 8 elements per thread: 256 threads per block, or 512 for larger tiles
+  if (ty * g.nx <= 16 * NT) {
     kg_smem(k_xbwd<S, C, 8, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, red);
-  } else if (ty * g.nx <= 16 * NT) {
-    kg_smem(k_xbwd<S, C, 16, G>, static_cast<int>(kLdsBytes));
-    hipLaunchKernelGGL((k_xbwd<S, C, 16, G>), grid, dim3(NT), sh, st, SB, SO, wt, g, px, twx, ty, red);
+    const int nth = ty * g.nx <= 8 * NT ? NT : 2 * NT;
+    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(nth), sh, st, SB, SO, wt, g, px, twx, ty, red);
   } else
     throw std::runtime_error("x extent too large for the fused k-space kernel");
 }
