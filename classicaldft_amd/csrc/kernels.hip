@@ -376,7 +376,8 @@ __global__ void __launch_bounds__(2 * NT) k_xfwd(const cx<S>* __restrict__ S0, c
     buf[yl * ls + x] = (y < ny) ? cvt<C>(plane[x * ny + y]) : cx<C>{C(0), C(0)};
   }
   __syncthreads();
-  fft_lines<C, EPT>(buf, ty, ls, px, -1, twx);
+  if constexpr (G == 2) slot_lines_only<C>(buf, ty, ls, px, -1, twx);
+  else fft_lines<C, EPT>(buf, ty, ls, px, -1, twx);
   cx<C> rh[EPT];
   WRad wr[EPT];
   const double kzv = g.dkz * kz;
@@ -445,7 +446,8 @@ __global__ void __launch_bounds__(2 * NT) k_xfwd(const cx<S>* __restrict__ S0, c
     // register-resident slot transforms where the length has them (with 12
     // channel transforms per block they measured slower than the LDS passes,
     // 160^3: 502 against 308 us; with the 7 sources they are faster)
-    fft_lines<CR, EPT * GR>(bufr, ng * ty, ls, px, +1, twr);
+    if constexpr (G == 2) slot_lines_only<CR>(bufr, ng * ty, ls, px, +1, twr);  // x extent with slots (host)
+    else fft_lines<CR, EPT * GR>(bufr, ng * ty, ls, px, +1, twr);
     for (int gi = 0; gi < ng; ++gi) {
       const int sc = src(c0 + gi);
       const cx<CR>* vb = bufr + gi * slab;
@@ -1248,18 +1250,25 @@ void launch_xfwd(const cx<S>* S0, cx<S>* SF, const S* mf, const cx<double>* wt, 
                  const cx<C>* twx, const cx<CV>* twv, ChannelSet cs, bool red, hipStream_t st) {
   const int rd = red && cs.hs ? 1 : 0;
   const int ty = px.tile > 0 ? px.tile : kXTile;
-  constexpr int G = sizeof(C) == 4 ? 2 : 1;  // sources per batched transform
-  const size_t sh = sizeof(cx<C>) * (G + (cs.hs ? 1 : 0)) * ty * (g.nx + 1);
+  // sources per batched transform: two slabs of C (the vector and tensor
+  // sources of a narrower CV four) where they fit the LDS, else one
+  auto lds = [&](int G) { return sizeof(cx<C>) * (G + (cs.hs ? 1 : 0)) * ty * (g.nx + 1); };
+  const bool g2 = lds(2) <= kLdsBytes && slot_lanes(g.nx) != 0;  // the two-slab form has slot transforms only
+  const size_t sh = lds(g2 ? 2 : 1);
   if (sh > kLdsBytes) throw std::runtime_error("x tile too large for the LDS of the forward k-space kernel");
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   // up to 8 elements per thread: 256 threads per block, or 512 for larger tiles
-  if (ty * g.nx <= 16 * NT) {
-    kg_smem(k_xfwd<S, C, CV, 8, G>, static_cast<int>(kLdsBytes));
-    const int nth = ty * g.nx <= 8 * NT ? NT : 2 * NT;
-    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, G>), grid, dim3(nth), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
+  if (ty * g.nx > 16 * NT) throw std::runtime_error("x extent too large for the fused k-space kernel");
+  const int nth = ty * g.nx <= 8 * NT ? NT : 2 * NT;
+  if (g2) {
+    kg_smem(k_xfwd<S, C, CV, 8, 2>, static_cast<int>(kLdsBytes));
+    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, 2>), grid, dim3(nth), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
                        rd);
-  } else
-    throw std::runtime_error("x extent too large for the fused k-space kernel");
+  } else {
+    kg_smem(k_xfwd<S, C, CV, 8, 1>, static_cast<int>(kLdsBytes));
+    hipLaunchKernelGGL((k_xfwd<S, C, CV, 8, 1>), grid, dim3(nth), sh, st, S0, SF, mf, wt, g, px, twx, twv, ty, cs,
+                       rd);
+  }
 }
 
 template <typename S, typename C>

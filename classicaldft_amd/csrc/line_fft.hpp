@@ -68,4 +68,19 @@ __device__ __forceinline__ void fft_lines(cx<C>* buf, int nl, int ls, const FftP
   }
 }
 
+// fft_lines for the lengths with slot transforms only (slot_lanes(p.L) != 0,
+// which the caller guarantees): no LDS-pass fallback and its registers.
+template <typename C>
+__device__ __forceinline__ void slot_lines_only(cx<C>* buf, int nl, int ls, const FftPlan& p, int sg,
+                                                const cx<C>* __restrict__ tw) {
+  switch (p.L) {
+    case 48: slot_lines_sg<C, 48, 8>(buf, nl, ls, sg, tw); return;
+    case 64: slot_lines_sg<C, 64, 8>(buf, nl, ls, sg, tw); return;
+    case 80: slot_lines_sg<C, 80, 8>(buf, nl, ls, sg, tw); return;
+    case 96: slot_lines_sg<C, 96, 16>(buf, nl, ls, sg, tw); return;
+    case 128: slot_lines_sg<C, 128, 16>(buf, nl, ls, sg, tw); return;
+    default: slot_lines_sg<C, 160, 16>(buf, nl, ls, sg, tw); return;
+  }
+}
+
 }  // namespace cdft
