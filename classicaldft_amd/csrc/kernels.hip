@@ -594,7 +594,9 @@ __global__ void __launch_bounds__(2 * NT, (sizeof(C) == 4 ? 3 : 2)) k_xbwd(const
       }
     }
     __syncthreads();
-    fft_lines<C, EPT * G>(buf, ng * ty, ls, px, -1, twx);
+    // the wider batches are launched for x extents with slot transforms only
+    if constexpr (G > (sizeof(C) == 4 ? 2 : 1)) slot_lines_only<C>(buf, ng * ty, ls, px, -1, twx);
+    else fft_lines<C, EPT * G>(buf, ng * ty, ls, px, -1, twx);
     for (int gi = 0; gi < ng; ++gi) {
       const int grp = c0 + gi;  // block-uniform
 #pragma unroll
@@ -637,7 +639,8 @@ __global__ void __launch_bounds__(2 * NT, (sizeof(C) == 4 ? 3 : 2)) k_xbwd(const
     }
   }
   __syncthreads();
-  fft_lines<C, EPT>(buf, ty, ls, px, +1, twx);
+  if constexpr (G > (sizeof(C) == 4 ? 2 : 1)) slot_lines_only<C>(buf, ty, ls, px, +1, twx);
+  else fft_lines<C, EPT>(buf, ty, ls, px, +1, twx);
   cx<S>* outp = SO + plane_off;
   for (int e = threadIdx.x; e < ty * nx; e += blockDim.x) {
     const int x = dty.div(e), yl = e - x * ty, y = y0 + yl;
@@ -1278,17 +1281,24 @@ void launch_xbwd(const cx<S>* SB, cx<S>* SO, const cx<double>* wt, const Geom& g
     throw std::runtime_error("the backward x kernel takes the 11 hard-sphere derivative channels or kg_zpoint's 9");
   const int red = nb == 9 ? 1 : 0;
   const int ty = px.tile > 0 ? px.tile : kXTile;
-  constexpr int G = sizeof(C) == 4 ? 2 : 1;  // groups per batched transform
-  const size_t sh = sizeof(cx<C>) * G * ty * (g.nx + 1);
+  // groups per batched transform: GW (3 of the 6 in FP32, 2 in FP64) for x
+  // extents with slot transforms where the LDS allows, else GN
+  constexpr int GN = sizeof(C) == 4 ? 2 : 1, GW = sizeof(C) == 4 ? 3 : 2;
+  auto lds = [&](int G) { return sizeof(cx<C>) * G * ty * (g.nx + 1); };
+  const bool wide = slot_lanes(g.nx) != 0 && lds(GW) <= kLdsBytes;
+  const size_t sh = lds(wide ? GW : GN);
   if (sh > kLdsBytes) throw std::runtime_error("x tile too large for the LDS of the backward k-space kernel");
   dim3 grid((g.ny + ty - 1) / ty, g.nzh);
   // up to 8 elements per thread: 256 threads per block, or 512 for larger tiles
-  if (ty * g.nx <= 16 * NT) {
-    kg_smem(k_xbwd<S, C, 8, G>, static_cast<int>(kLdsBytes));
-    const int nth = ty * g.nx <= 8 * NT ? NT : 2 * NT;
-    hipLaunchKernelGGL((k_xbwd<S, C, 8, G>), grid, dim3(nth), sh, st, SB, SO, wt, g, px, twx, ty, red);
-  } else
-    throw std::runtime_error("x extent too large for the fused k-space kernel");
+  if (ty * g.nx > 16 * NT) throw std::runtime_error("x extent too large for the fused k-space kernel");
+  const int nth = ty * g.nx <= 8 * NT ? NT : 2 * NT;
+  if (wide) {
+    kg_smem(k_xbwd<S, C, 8, GW>, static_cast<int>(kLdsBytes));
+    hipLaunchKernelGGL((k_xbwd<S, C, 8, GW>), grid, dim3(nth), sh, st, SB, SO, wt, g, px, twx, ty, red);
+  } else {
+    kg_smem(k_xbwd<S, C, 8, GN>, static_cast<int>(kLdsBytes));
+    hipLaunchKernelGGL((k_xbwd<S, C, 8, GN>), grid, dim3(nth), sh, st, SB, SO, wt, g, px, twx, ty, red);
+  }
 }
 
 // z-lines per block of the fused pointwise kernel: PPT*NT points (4 per thread
