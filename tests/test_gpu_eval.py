@@ -81,7 +81,7 @@ def test_general_schedule_slot_lengths_match_oracle(shape, precision):
 @pytest.mark.parametrize("model,inter,nz", [("WhiteBearII", (LJ,), 48), ("WhiteBearI", (LJ,), 48),
                                             ("Rosenfeld", (), 48), ("RSLT", (LJ,), 48), ("esFMT", (), 48),
                                             ("WhiteBearII", (), 80), ("Rosenfeld", (LJ,), 96),
-                                            ("WhiteBearII", (), 160)])
+                                            ("WhiteBearII", (), 160), ("none", (LJ,), 80)])
 def test_general_reduced_channels_match_oracle(model, inter, nz, precision):
     shape, dx = (16, 12, nz), 0.1
     kw = dict(A=1.0, B=-0.5) if model == "esFMT" else {}
