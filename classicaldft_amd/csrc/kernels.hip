@@ -266,14 +266,13 @@ __global__ void __launch_bounds__(NT) k_sum(const double* __restrict__ in, Alias
 }
 
 // ---------------------------------------------------------------- y lines (contiguous)
+// Lines line0 .. line0 + nl - 1 by one block.
 template <typename S, typename C>
-__device__ __forceinline__ void lines_block(cx<S>* __restrict__ Sp, long nlines, const FftPlan& p,
-                                            const cx<C>* __restrict__ tw, int nlb, int sg) {
+__device__ __forceinline__ void lines_block(cx<S>* __restrict__ Sp, long line0, int nl, const FftPlan& p,
+                                            const cx<C>* __restrict__ tw, int sg) {
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int L = p.L;
-  const long line0 = static_cast<long>(blockIdx.x) * nlb;
-  const int nl = static_cast<int>(min(static_cast<long>(nlb), nlines - line0));
   cx<S>* base = Sp + line0 * L;
   for (int e = threadIdx.x; e < nl * L; e += blockDim.x) buf[e] = cvt<C>(base[e]);
   __syncthreads();
@@ -284,19 +283,30 @@ __device__ __forceinline__ void lines_block(cx<S>* __restrict__ Sp, long nlines,
 template <typename S, typename C>
 __global__ void __launch_bounds__(NT, 4) k_lines(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
                                                  const cx<C>* __restrict__ tw, int nlb, int sg) {
-  lines_block<S, C>(Sp, nlines, p, tw, nlb, sg);
+  const long line0 = static_cast<long>(blockIdx.x) * nlb;
+  lines_block<S, C>(Sp, line0, static_cast<int>(min(static_cast<long>(nlb), nlines - line0)), p, tw, sg);
 }
 
-// Lines of two arithmetic precisions in one launch: a block whose lines all lie
-// in [lo0, lo1) runs in CL, every other block in CH (LDS sized for CH).
+// Lines of two arithmetic precisions in one launch: lines [lo0, lo1) in CL,
+// nlb per block, the others in CH, hb per block (blocks [0, na) take lines
+// [0, lo0), the next nb blocks [lo0, lo1), the rest [lo1, nlines)), so both
+// kinds of block fit the same LDS.
 template <typename S, typename CH, typename CL>
 __global__ void __launch_bounds__(NT, 4) k_lines_split(cx<S>* __restrict__ Sp, long nlines, FftPlan p,
                                                        const cx<CH>* __restrict__ twh,
-                                                       const cx<CL>* __restrict__ twl, int nlb, int sg, long lo0,
-                                                       long lo1) {
-  const long line0 = static_cast<long>(blockIdx.x) * nlb;
-  if (line0 >= lo0 && line0 + nlb <= lo1) lines_block<S, CL>(Sp, nlines, p, twl, nlb, sg);
-  else lines_block<S, CH>(Sp, nlines, p, twh, nlb, sg);
+                                                       const cx<CL>* __restrict__ twl, int nlb, int hb, int sg,
+                                                       long lo0, long lo1, int na, int nb) {
+  const long b = blockIdx.x;
+  if (b < na) {
+    const long line0 = b * hb;
+    lines_block<S, CH>(Sp, line0, static_cast<int>(min(static_cast<long>(hb), lo0 - line0)), p, twh, sg);
+  } else if (b < na + nb) {
+    const long line0 = lo0 + (b - na) * nlb;
+    lines_block<S, CL>(Sp, line0, static_cast<int>(min(static_cast<long>(nlb), lo1 - line0)), p, twl, sg);
+  } else {
+    const long line0 = lo1 + (b - na - nb) * hb;
+    lines_block<S, CH>(Sp, line0, static_cast<int>(min(static_cast<long>(hb), nlines - line0)), p, twh, sg);
+  }
 }
 
 // ---------------------------------------------------------------- x lines (strided in a kz plane)
@@ -1213,10 +1223,13 @@ void launch_lines_split(cx<float>* Sp, long nlines, const FftPlan& p, const cx<d
   const int nlb = p.lines > 0 ? p.lines : (T ? NT / T : 2048 / p.L);
   if (nlb < 1 || (!T && nlb * p.L > 8 * NT))
     throw std::runtime_error("line length too large for the line transform kernel");
-  const unsigned nblk = static_cast<unsigned>((nlines + nlb - 1) / nlb);
-  const size_t sh = sizeof(cx<double>) * nlb * p.L;
+  // FP64 blocks take half as many lines, in the same LDS
+  const int hb = std::max(1, nlb / 2);
+  const long na = (lo0 + hb - 1) / hb, nb = (lo1 - lo0 + nlb - 1) / nlb, nc = (nlines - lo1 + hb - 1) / hb;
+  const unsigned nblk = static_cast<unsigned>(na + nb + nc);
+  const size_t sh = std::max(sizeof(cx<float>) * nlb, sizeof(cx<double>) * hb) * p.L;
   hipLaunchKernelGGL((k_lines_split<float, double, float>), dim3(nblk), dim3(NT), sh, st, Sp, nlines, p, twh, twl,
-                     nlb, sg, lo0, lo1);
+                     nlb, hb, sg, lo0, lo1, static_cast<int>(na), static_cast<int>(nb));
 }
 
 // Dynamic LDS above 64 KiB is enabled per kernel instantiation, once per process.
