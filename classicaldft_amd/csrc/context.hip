@@ -462,10 +462,12 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
     mark(8);
     launch_lines<S, CL>(SO, ylines, plan[1], tw<CL>(1), +1, stream);
     mark(9);
-    launch_zc2r<S, CL>(SO, 1, force, d_local, g.dv, g, plan[2], tw<CL>(2), nullptr, DcTable{}, stream);
+    // its block 0 also reduces the point kernel's partial sums
+    launch_zc2r<S, CL>(SO, 1, force, d_local, g.dv, g, plan[2], tw<CL>(2), nullptr, DcTable{}, stream, d_partials,
+                       nblk_point, d_res);
   }
   mark(10);
-  launch_finalize(d_partials, nblk_point, d_res, stream);
+  if (!backward) launch_finalize(d_partials, nblk_point, d_res, stream);
   mark(11);
 }
 

@@ -202,7 +202,15 @@ template <typename S, typename C>
 __global__ void __launch_bounds__(NT) k_zc2r(const cx<S>* __restrict__ Sp, double* __restrict__ out,
                                              const double* __restrict__ add, double scale, Geom g, FftPlan pz,
                                              const cx<C>* __restrict__ twz, int nlb,
-                                             const double* __restrict__ shift_sum, DcTable dc) {
+                                             const double* __restrict__ shift_sum, DcTable dc,
+                                             const double* __restrict__ fin_rows, int fin_n, double* __restrict__ fin_res) {
+  if (fin_rows && blockIdx.x == 0 && blockIdx.y == 0) {
+    // the partial sums of an earlier kernel, reduced by block 0 before its own
+    // lines (while the other blocks run) instead of by a launch of their own
+    __shared__ double sh[NT], shc[NT];
+    reduce_rows<NT>(fin_rows, fin_n, fin_res, sh, shc);
+    __syncthreads();
+  }
   const UDiv dnlb(nlb), dnz(g.nz);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   const int nz = g.nz, nzh = g.nzh;
@@ -1184,13 +1192,14 @@ void launch_zr2c_fields(const double* in, int nfields, cx<S>* out, const Geom& g
 
 template <typename S, typename C>
 void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, double scale, const Geom& g,
-                 const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st) {
+                 const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st,
+                 const double* fin_rows, int fin_n, double* fin_res) {
   const int nlb = zlines_per_block<C>(pz);
   if ((nlb >> 1) * g.nz > 8 * NT) throw std::runtime_error("z extent too large for the z transform kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz + sizeof(cx<S>) * nlb * g.nzh;
   hipLaunchKernelGGL((k_zc2r<S, C>), dim3(nblk, nch), dim3(NT), sh, st, Sp, out, add, scale, g, pz, twz, nlb,
-                     shift_sum, dc);
+                     shift_sum, dc, fin_rows, fin_n, fin_res);
 }
 
 int sum_blocks(const Geom& g) {
@@ -1446,7 +1455,7 @@ void launch_finalize(const double* partials, int nrows, double* res, hipStream_t
   template void launch_zr2c_fields<S, C>(const double*, int, cx<S>*, const Geom&, const FftPlan&, const cx<C>*,    \
                                          hipStream_t);                                                             \
   template void launch_zc2r<S, C>(const cx<S>*, int, double*, const double*, double, const Geom&, const FftPlan&, \
-                                  const cx<C>*, const double*, DcTable, hipStream_t);                              \
+                                  const cx<C>*, const double*, DcTable, hipStream_t, const double*, int, double*); \
   template void launch_lines<S, C>(cx<S>*, long, const FftPlan&, const cx<C>*, int, hipStream_t);                 \
   template void launch_xlines<S, C>(cx<S>*, int, const Geom&, const FftPlan&, const cx<C>*, int, hipStream_t);    \
   template void launch_xfwd<S, C, C>(const cx<S>*, cx<S>*, const S*, const cx<double>*, const Geom&,               \

@@ -58,9 +58,12 @@ void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz,
 template <typename S, typename C>
 void launch_zr2c_fields(const double* in, int nfields, cx<S>* out, const Geom& g, const FftPlan& pz,
                         const cx<C>* twz, hipStream_t st);
+// With fin_rows set, block 0 also reduces fin_n rows of 8 partial sums written
+// by an earlier kernel into fin_res (launch_finalize's result), first.
 template <typename S, typename C>
 void launch_zc2r(const cx<S>* Sp, int nch, double* out, const double* add, double scale, const Geom& g,
-                 const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st);
+                 const FftPlan& pz, const cx<C>* twz, const double* shift_sum, DcTable dc, hipStream_t st,
+                 const double* fin_rows = nullptr, int fin_n = 0, double* fin_res = nullptr);
 // FP64 block sums of amap(in) into partials[b*8]; finalize with launch_finalize.
 int sum_blocks(const Geom& g);
 void launch_sum(const double* in, AliasMap amap, double* partials, const Geom& g, hipStream_t st);
