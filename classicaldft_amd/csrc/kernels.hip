@@ -32,8 +32,9 @@
 // sets instead: nf - 2 forward (v_z and T_zz are formed in the point kernel)
 // and 9 backward (their derivatives folded into the others).
 //   k_lines     inverse y-FFT of SO
-//   k_zc2r      z-C2R, * dV, + local force                      (SO -> force)
-//   k_finalize  deterministic compensated sum of the block partials
+//   k_zc2r      z-C2R, * dV, + local force                      (SO -> force);
+//               its block 0 first reduces the point kernel's partial sums
+//   k_finalize  the same reduction when no forces are evaluated
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
