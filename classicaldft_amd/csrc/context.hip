@@ -426,9 +426,9 @@ void Context::eval_t(const double* rho, const double* vext, double mu, double* f
   mark(0);
   if (cs.nf > 0) {
     launch_sum(rho, amap, d_sum_part, g, stream);
-    launch_finalize(d_sum_part, nblk_sum, d_shift, stream);
     mark(1);
-    launch_zr2c<S, CL>(rho, S0, g, plan[2], tw<CL>(2), amap, d_shift, stream);
+    // the z-R2C reduces the sum itself and stores it in d_shift
+    launch_zr2c<S, CL>(rho, S0, g, plan[2], tw<CL>(2), amap, nullptr, stream, d_sum_part, nblk_sum, d_shift);
     mark(2);
     launch_lines<S, CL>(S0, ylines, plan[1], tw<CL>(1), -1, stream);
     mark(3);

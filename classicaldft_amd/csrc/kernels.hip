@@ -19,8 +19,8 @@
 // and z stages and the backward path run in FP32.
 //
 // Evaluation schedule (one species, FMT + optional mean field):
-//   k_sum + k_finalize   FP64 sum of rho (mean-density split)
-//   k_zr2c      z-R2C of rho - mean                             (real -> S0)
+//   k_sum       FP64 block sums of rho (mean-density split)
+//   k_zr2c      z-R2C of rho - mean (each block reduces the sums) (real -> S0)
 //   k_lines     y-FFT of S0
 //   k_xfwd      x-FFT, 7 weighted inverse x-FFTs -> channels   (S0 -> SF[nf])
 //   k_lines     inverse y-FFT of the nf channels
@@ -170,13 +170,38 @@ __device__ __forceinline__ void c2r_lines(const cx<T_>* __restrict__ stage, int 
 template <typename S, typename C>
 __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S>* __restrict__ out, Geom g,
                                              FftPlan pz, const cx<C>* __restrict__ twz, int nlb, AliasMap amap,
-                                             const double* __restrict__ shift_sum) {
+                                             const double* __restrict__ shift_sum, const double* __restrict__ sum_rows,
+                                             int sum_n, double* __restrict__ sum_out) {
   const UDiv dnlb(nlb), dnz(g.nz);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
   cx<C>* buf = reinterpret_cast<cx<C>*>(smem_raw);
   const int nz = g.nz, nzh = g.nzh;
   const long l0 = static_cast<long>(blockIdx.x) * nlb;
-  const double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+  double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
+  if (sum_rows) {
+    // the density sum from k_sum's block rows (column 0), reduced by every
+    // block in the same fixed order (compensated per thread, then a tree of
+    // (sum, compensation) pairs), so that no reduction launch comes between;
+    // block 0 stores it for the later kernels
+    __shared__ double sh[NT], shc[NT];
+    double s = 0.0, c = 0.0;
+    for (int r = threadIdx.x; r < sum_n; r += NT) two_sum(s, c, sum_rows[static_cast<long>(r) * 8]);
+    sh[threadIdx.x] = s;
+    shc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = NT / 2; w > 0; w >>= 1) {
+      if (static_cast<int>(threadIdx.x) < w) {
+        double s2 = sh[threadIdx.x], c2 = shc[threadIdx.x] + shc[threadIdx.x + w];
+        two_sum(s2, c2, sh[threadIdx.x + w]);
+        sh[threadIdx.x] = s2;
+        shc[threadIdx.x] = c2;
+      }
+      __syncthreads();
+    }
+    const double total = sh[0] + shc[0];
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 8) sum_out[threadIdx.x] = threadIdx.x == 0 ? total : 0.0;
+    shift = total * g.inv_n;
+  }
   in += static_cast<long>(blockIdx.y) * g.N;  // field blockIdx.y of a multi-field launch
   out += static_cast<long>(blockIdx.y) * g.M;
   for (int e = threadIdx.x; e < nlb * nz; e += blockDim.x) {
@@ -1170,12 +1195,13 @@ static int zlines_per_block(const FftPlan& pz) {
 
 template <typename S, typename C>
 void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz, const cx<C>* twz, AliasMap amap,
-                 const double* shift_sum, hipStream_t st) {
+                 const double* shift_sum, hipStream_t st, const double* sum_rows, int sum_n, double* sum_out) {
   const int nlb = zlines_per_block<C>(pz);
   if ((nlb >> 1) * g.nz > 8 * NT) throw std::runtime_error("z extent too large for the z transform kernel");
   const unsigned nblk = static_cast<unsigned>((g.nxy + nlb - 1) / nlb);
   const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz;
-  hipLaunchKernelGGL((k_zr2c<S, C>), dim3(nblk), dim3(NT), sh, st, in, out, g, pz, twz, nlb, amap, shift_sum);
+  hipLaunchKernelGGL((k_zr2c<S, C>), dim3(nblk), dim3(NT), sh, st, in, out, g, pz, twz, nlb, amap, shift_sum,
+                     sum_rows, sum_n, sum_out);
 }
 
 template <typename S, typename C>
@@ -1188,7 +1214,8 @@ void launch_zr2c_fields(const double* in, int nfields, cx<S>* out, const Geom& g
   const size_t sh = sizeof(cx<C>) * (nlb >> 1) * g.nz;
   const AliasMap id{0, 0.0, 0.0};
   hipLaunchKernelGGL((k_zr2c<S, C>), dim3(nblk, nfields), dim3(NT), sh, st, in, out, g, pz, twz, nlb, id,
-                     static_cast<const double*>(nullptr));
+                     static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), 0,
+                     static_cast<double*>(nullptr));
 }
 
 template <typename S, typename C>
@@ -1452,7 +1479,7 @@ void launch_finalize(const double* partials, int nrows, double* res, hipStream_t
 
 #define CDFT_INST_SC(S, C)                                                                                         \
   template void launch_zr2c<S, C>(const double*, cx<S>*, const Geom&, const FftPlan&, const cx<C>*, AliasMap,      \
-                                  const double*, hipStream_t);                                                     \
+                                  const double*, hipStream_t, const double*, int, double*);                        \
   template void launch_zr2c_fields<S, C>(const double*, int, cx<S>*, const Geom&, const FftPlan&, const cx<C>*,    \
                                          hipStream_t);                                                             \
   template void launch_zc2r<S, C>(const cx<S>*, int, double*, const double*, double, const Geom&, const FftPlan&, \

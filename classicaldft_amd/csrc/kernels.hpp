@@ -50,9 +50,13 @@ struct DcTable {
   double v[12];
 };
 
+// The mean split subtracts shift_sum[0] / N, or with sum_rows the sum of
+// column 0 of launch_sum's sum_n rows, which every block reduces itself (block
+// 0 stores it in sum_out[0], zeros in sum_out[1..7], as launch_finalize would).
 template <typename S, typename C>
 void launch_zr2c(const double* in, cx<S>* out, const Geom& g, const FftPlan& pz, const cx<C>* twz, AliasMap amap,
-                 const double* shift_sum, hipStream_t st);
+                 const double* shift_sum, hipStream_t st, const double* sum_rows = nullptr, int sum_n = 0,
+                 double* sum_out = nullptr);
 // z-R2C of `nfields` real fields (N apart) into spectra (M apart), one launch,
 // no density map or mean split.
 template <typename S, typename C>
