@@ -398,8 +398,11 @@ __host__ __device__ constexpr int kg_bwd_slot(int ch) { return kg_fwd_slot(ch); 
 //   4: aik2 kx^2 rho -> T_xx = (.) + eta / R
 //   5: aik2 kx rho   -> T_xy = ky (.), T_xz = kz (.)
 //   6: aik2 rho      -> T_yy = ky^2 (.) + eta / R, T_yz = ky kz (.), T_zz = kz^2 (.) + eta / R
-// G sources per batched transform.  EPT >= ty*nx/blockDim.x elements owned per thread.
-// The vector and tensor sources (3..6) use arithmetic CV (C: the scalar ones).
+// The LDS holds G slabs of C: G sources per batched transform in C, as many
+// more in a narrower CV (the vector and tensor sources 3..6 use CV, the scalar
+// ones C); with G = 2 the x extent has slot transforms (the host checks), and
+// only those are compiled in.  With `red` SF receives the reduced set of the
+// kg_zpoint point kernel.  EPT >= ty*nx/blockDim.x elements owned per thread.
 template <typename S, typename C, typename CV, int EPT, int G>
 __global__ void __launch_bounds__(2 * NT) k_xfwd(const cx<S>* __restrict__ S0, cx<S>* __restrict__ SF,
                                              const S* __restrict__ mf, const cx<double>* __restrict__ wt, Geom g,
@@ -555,7 +558,9 @@ __global__ void __launch_bounds__(2 * NT) k_xfwd(const cx<S>* __restrict__ S0, c
 //   P3 = G_Txx,  P4 = ky G_Txy + kz G_Txz,  P5 = ky^2 G_Tyy + ky kz G_Tyz + kz^2 G_Tzz,
 // accumulated as A = X0 - i kx X1, B = X2, C = kx^2 X3 + kx X4 + X5 and
 // combined once per element with the radial factors.  G groups per batched
-// transform; a group's loads combine its channels in registers.
+// transform; a group's loads combine its channels in registers.  With `red`
+// SB holds kg_zpoint's nine channels (no v_z, T_zz: their terms are folded
+// into eta, n2, T_xx and T_yy).
 template <typename S, typename C, int EPT, int G>
 __global__ void __launch_bounds__(2 * NT, (sizeof(C) == 4 ? 3 : 2)) k_xbwd(const cx<S>* __restrict__ SB, cx<S>* __restrict__ SO,
                                              const cx<double>* __restrict__ wt, Geom g, FftPlan px,
