@@ -107,6 +107,25 @@ def test_general_reduced_channels_match_oracle(model, inter, nz, precision):
     assert res2["grand_potential"] == pytest.approx(res["grand_potential"], rel=1e-13)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("shape", [(1, 1, 48), (2, 3, 160), (5, 1, 96)])
+def test_general_reduced_channels_thin_grids(shape, precision):
+    # x and y extents of one to five points with the point kernel of the reduced channel sets
+    dx = 0.1
+    ctx = _ctx(shape, dx, precision=precision)
+    assert ctx.schedule == "general"
+    orc = _orc(shape, dx)
+    rho = 0.3 + 0.1 * np.cos(2 * np.pi * np.arange(shape[2]) / shape[2])[None, None, :] * np.ones(shape)
+    mu = -1.0
+    res, f = ctx.total(rho, mu)
+    parts, fr = orc.total(rho, mu)
+    om = abs(res["grand_potential"] - parts[5]) / max(abs(parts[i]) for i in range(5))
+    e = _relmax(f, fr)
+    print(f"thin {shape} {precision}: Omega rel err {om:.2e}, force rel err {e:.2e}", flush=True)
+    assert om < (1e-11 if precision == "fp64" else 1e-6)
+    assert e < (1e-10 if precision == "fp64" else 2e-5)
+
+
 @pytest.mark.parametrize("precision,tol", [("fp64", 1e-11), ("mixed", 2e-5)])
 def test_weighted_densities(precision, tol):
     shape, dx = (16, 20, 24), 0.1
