@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <stdexcept>
 #include <string>
@@ -434,8 +435,10 @@ struct DdftRun {
       }
     } while (restart);
     const double e = compute(d1.as<double>(), f1.as<double>());
-    CDFT_HIP(hipMemcpyAsync(rho0.p, d1.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
-    CDFT_HIP(hipMemcpyAsync(f0.p, f1.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+    // the new density and its forces start the next step: swap the buffers
+    // instead of copying them (d1 and f1 are rewritten before they are read)
+    std::swap(rho0.p, d1.p);
+    std::swap(f0.p, f1.p);
     return e;
   }
 };
