@@ -925,14 +925,21 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
     int successes = 0, steps = 0;
     double time = 0.0;
     bool forces_stale = false;  // the boundary condition changed the density after its forces were taken
+    // Without stop or snapshot callbacks (and with the context's functional) the
+    // step energies are needed only at the end: each step's raw sums stay on
+    // the device and are read back once.
+    const bool defer = !hk.stop() && !hk.snapshot() && !hk.force();
+    DevBuf esums(defer ? 8 * sizeof(double) * static_cast<size_t>(n_steps + 1) : 0);
     for (int s = 1; s <= n_steps; ++s) {
       if (forces_stale) {
         run.compute(run.rho0.as<double>(), run.f0.as<double>());  // the step's first compute (dynamics.hpp:345)
         forces_stale = false;
       }
       const double before = dt;
-      e = c->precision == CDFT_PRECISION_MIXED ? run.step<float>(&dt, config->fp_tolerance, config->fp_max_iterations)
-                                               : run.step<double>(&dt, config->fp_tolerance, config->fp_max_iterations);
+      double* sums_to = defer ? esums.as<double>() + 8 * s : nullptr;
+      e = c->precision == CDFT_PRECISION_MIXED
+              ? run.step<float>(&dt, config->fp_tolerance, config->fp_max_iterations, sums_to)
+              : run.step<double>(&dt, config->fp_tolerance, config->fp_max_iterations, sums_to);
       time += dt;
       steps = s;
       if (hk.project()) {  // dynamics.hpp:629-631
@@ -951,6 +958,17 @@ int cdft_ddft_run(cdft_ctx* ctx, const cdft_step_config* config, int n_steps, co
       if (hk.snapshot() && hooks->snapshot_interval > 0 && s % hooks->snapshot_interval == 0)  // :668-675
         hk.take_snapshot(c, s, time, e, run.rho0.as<double>());
       if (hk.stop() && hk.should_stop(s, time, e)) break;  // dynamics.hpp:677-682
+    }
+    if (defer && steps > 0) {
+      std::vector<double> es(8 * static_cast<size_t>(steps + 1));
+      CDFT_HIP(hipMemcpyAsync(es.data(), esums.p, sizeof(double) * es.size(), hipMemcpyDeviceToHost, c->stream));
+      CDFT_HIP(hipStreamSynchronize(c->stream));
+      for (int s = 1; s <= steps; ++s) {
+        cdft_result r{};
+        c->result_from(es.data() + 8 * s, mu, &r);
+        if (energies) energies[s] = r.grand_potential;
+        e = r.grand_potential;
+      }
     }
     CDFT_HIP(hipMemcpyAsync(rho, run.rho0.p, 8 * g.N, hipMemcpyDeviceToHost, c->stream));
     CDFT_HIP(hipStreamSynchronize(c->stream));
