@@ -152,10 +152,11 @@ __global__ void __launch_bounds__(256) k_xprop(const cx<S>* __restrict__ SR1, co
 }
 
 // d1_new = mean_new + raw - mean(raw); max |d1_new - d1_old|; min d1_new (apply_propagator, dynamics.hpp:289-303).
-// sums: [0] sum rho0, [8] sum R0, [16] sum R1, [24] sum raw
-__global__ void __launch_bounds__(256) k_ddft_update(const double* __restrict__ raw, double* __restrict__ d1, long n,
-                                                     const double* __restrict__ sums, double dt, int first_iter,
-                                                     double* partials) {
+// sums: [0] sum rho0, [8] sum R0, [16] sum R1, [24] sum raw.  d1_old is `old`
+// (rho0 on an attempt's first iteration, else d1 itself).
+__global__ void __launch_bounds__(256) k_ddft_update(const double* __restrict__ raw, const double* old,
+                                                     double* d1, long n, const double* __restrict__ sums, double dt,
+                                                     int first_iter, double* partials) {
   const double inv_n = 1.0 / static_cast<double>(n);
   const double s0 = sums[0], r0 = sums[8], r1 = first_iter ? sums[8] : sums[16];
   const double mean_new = (s0 + dt * r0 + 0.5 * dt * (r1 - r0)) * inv_n;
@@ -165,7 +166,7 @@ __global__ void __launch_bounds__(256) k_ddft_update(const double* __restrict__ 
        i += static_cast<long>(gridDim.x) * blockDim.x) {
     const double nv = raw[i] + shift;
     // fmax drops a NaN operand: a non-finite iterate is reported as an infinite deviation
-    mx = fmax(mx, isfinite(nv) ? fabs(d1[i] - nv) : INFINITY);
+    mx = fmax(mx, isfinite(nv) ? fabs(old[i] - nv) : INFINITY);
     mn = fmin(mn, nv);
     d1[i] = nv;
   }
@@ -353,8 +354,8 @@ struct DdftRun {
     const AliasMap id{0, 0.0, 0.0};
     double* s = sums.as<double>() + 8 * slot;
     launch_sum(field, id, part.as<double>(), g, c->stream);
-    launch_finalize(part.as<double>(), sum_blocks(g), s, c->stream);
-    launch_zr2c<R, R>(field, S, g, c->plan[2], c->tw<R>(2), id, s, c->stream);
+    // the z-R2C reduces the block sums itself and stores the sum in s
+    launch_zr2c<R, R>(field, S, g, c->plan[2], c->tw<R>(2), id, nullptr, c->stream, part.as<double>(), sum_blocks(g), s);
     launch_lines<R, R>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), -1, c->stream);
     if (x_stage) launch_xlines<R, R>(S, 1, g, c->plan[0], c->tw<R>(0), -1, c->stream);
   }
@@ -392,7 +393,8 @@ struct DdftRun {
     launch_sum(raw.as<double>(), id, part.as<double>(), g, c->stream);
     launch_finalize(part.as<double>(), sum_blocks(g), sums.as<double>() + 24, c->stream);
     hipLaunchKernelGGL(k_ddft_update, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, raw.as<double>(),
-                       d1.as<double>(), g.N, sums.as<double>(), st_dt, first_iter ? 1 : 0, part.as<double>());
+                       first_iter ? static_cast<const double*>(rho0.as<double>()) : d1.as<double>(), d1.as<double>(),
+                       g.N, sums.as<double>(), st_dt, first_iter ? 1 : 0, part.as<double>());
     hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
                        static_cast<int>(vec_blocks(g.N)), mm.as<double>());
     CDFT_HIP(hipGetLastError());
@@ -406,7 +408,6 @@ struct DdftRun {
   // integrating_factor_step (dynamics.hpp:333-401); rho0 holds the densities and f0 their forces.
   template <typename R>
   double step(double* cfg_dt, double tol, int max_it, double* sums_to = nullptr) {
-    const long N = c->g.N;
     rhs_forward<R>(rho0.as<double>(), f0.as<double>(), SR0.as<cx<R>>(), 1, true);
     forward<R>(rho0.as<double>(), Srho0.as<cx<R>>(), 0, true);
     bool restart;
@@ -414,7 +415,7 @@ struct DdftRun {
     do {
       restart = false;
       update_timestep(*cfg_dt);
-      CDFT_HIP(hipMemcpyAsync(d1.p, rho0.p, 8 * N, hipMemcpyDeviceToDevice, c->stream));
+      // d1 starts as rho0: the first propagate reads rho0 as the old iterate and writes all of d1
       double deviation = 1e30;
       for (int it = 0; it < max_it; ++it) {
         const bool first = it == 0;  // d1 == rho0: compute({d1}) equals the step's first compute
