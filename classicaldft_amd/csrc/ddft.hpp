@@ -59,12 +59,28 @@ __global__ void __launch_bounds__(256) k_ddft_rhs(const double* __restrict__ rho
   double s = 0, c = 0;
   const long nx = G.nx, ny = G.ny, nz = G.nz, nyz = ny * nz;
   const double D = 0.5 / (G.dx * G.dx), D2 = 1.0 / (G.dx * G.dx), idv = 1.0 / G.dv;
+  // point indices below 2^24: multiply-high quotients instead of 64-bit division
+  const bool small = G.N < (1L << 24);
+  const UDiv dnz(G.nz), dny(G.ny);
   for (long p = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; p < G.N;
        p += static_cast<long>(gridDim.x) * blockDim.x) {
-    const long iz = p % nz, iy = (p / nz) % ny, ix = p / nyz;
-    const long px = ((ix + 1) % nx) * nyz + iy * nz + iz, mx = ((ix - 1 + nx) % nx) * nyz + iy * nz + iz;
-    const long py = ix * nyz + ((iy + 1) % ny) * nz + iz, my = ix * nyz + ((iy - 1 + ny) % ny) * nz + iz;
-    const long pz = ix * nyz + iy * nz + (iz + 1) % nz, mz = ix * nyz + iy * nz + (iz - 1 + nz) % nz;
+    long ix, iy, iz;
+    if (small) {
+      const int q = dnz.div(static_cast<int>(p)), i = dny.div(q);
+      iz = p - static_cast<long>(q) * nz;
+      iy = q - static_cast<long>(i) * ny;
+      ix = i;
+    } else {
+      iz = p % nz;
+      iy = (p / nz) % ny;
+      ix = p / nyz;
+    }
+    const long ixp = ix + 1 == nx ? 0 : ix + 1, ixm = ix == 0 ? nx - 1 : ix - 1;
+    const long iyp = iy + 1 == ny ? 0 : iy + 1, iym = iy == 0 ? ny - 1 : iy - 1;
+    const long izp = iz + 1 == nz ? 0 : iz + 1, izm = iz == 0 ? nz - 1 : iz - 1;
+    const long px = ixp * nyz + iy * nz + iz, mx = ixm * nyz + iy * nz + iz;
+    const long py = ix * nyz + iyp * nz + iz, my = ix * nyz + iym * nz + iz;
+    const long pz = ix * nyz + iy * nz + izp, mz = ix * nyz + iy * nz + izm;
     const double d0 = rho[p], x0 = force[p] * idv;
     const double rpx = rho[px], rmx = rho[mx], rpy = rho[py], rmy = rho[my], rpz = rho[pz], rmz = rho[mz];
     const double xpx = force[px] * idv, xmx = force[mx] * idv, xpy = force[py] * idv, xmy = force[my] * idv;
