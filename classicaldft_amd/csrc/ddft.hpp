@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "context.hpp"
+#include "device_common.hpp"
 #include "line_fft.hpp"
 
 namespace cdft {
@@ -168,15 +169,17 @@ __global__ void __launch_bounds__(256) k_xprop(const cx<S>* __restrict__ SR1, co
 }
 
 // d1_new = mean_new + raw - mean(raw); max |d1_new - d1_old|; min d1_new (apply_propagator, dynamics.hpp:289-303).
-// sums: [0] sum rho0, [8] sum R0, [16] sum R1, [24] sum raw.  d1_old is `old`
-// (rho0 on an attempt's first iteration, else d1 itself).
+// sums: [0] sum rho0, [8] sum R0, [16] sum R1; the sum of raw is reduced by
+// every block from launch_sum's raw_rows (column 0 of raw_n rows).  d1_old is
+// `old` (rho0 on an attempt's first iteration, else d1 itself).
 __global__ void __launch_bounds__(256) k_ddft_update(const double* __restrict__ raw, const double* old,
-                                                     double* d1, long n, const double* __restrict__ sums, double dt,
+                                                     double* d1, long n, const double* __restrict__ sums,
+                                                     const double* __restrict__ raw_rows, int raw_n, double dt,
                                                      int first_iter, double* partials) {
   const double inv_n = 1.0 / static_cast<double>(n);
   const double s0 = sums[0], r0 = sums[8], r1 = first_iter ? sums[8] : sums[16];
   const double mean_new = (s0 + dt * r0 + 0.5 * dt * (r1 - r0)) * inv_n;
-  const double shift = mean_new - sums[24] * inv_n;
+  const double shift = mean_new - block_sum_col0<256>(raw_rows, raw_n) * inv_n;
   double mx = 0.0, mn = 1e300;
   for (long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
        i += static_cast<long>(gridDim.x) * blockDim.x) {
@@ -313,6 +316,7 @@ struct DdftRun {
   DevBuf rho0, d1, f0, f1, rhs, raw;
   DevBuf Srho0, SR0, SR1, Sw;
   DevBuf lam, fac, part, sums, mm;
+  DevBuf rpart;  // block sums of the propagated field (read by k_ddft_update while it writes part)
   std::vector<double> lam_h;
   double st_dt = 0.0;
   long evals = 0;
@@ -323,7 +327,7 @@ struct DdftRun {
         rhs(8 * c_->g.N), raw(8 * c_->g.N), Srho0(c_->csize() * c_->g.M), SR0(c_->csize() * c_->g.M),
         SR1(c_->csize() * c_->g.M), Sw(c_->csize() * c_->g.M),
         lam(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)), fac(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)),
-        part(8 * 8 * kVecBlocks), sums(8 * 40), mm(8 * 8) {
+        part(8 * 8 * kVecBlocks), sums(8 * 40), mm(8 * 8), rpart(8 * 8 * kVecBlocks) {
     const Geom& g = c->g;
     // make_if_state (dynamics.hpp:79-117): Lambda_d = 2 (cos(2 pi i / N_d) - 1) / dx^2
     const double PI = 3.14159265358979323846, Dx = 1.0 / (g.dx * g.dx);
@@ -406,11 +410,11 @@ struct DdftRun {
     launch_zc2r<R, R>(Sw.as<cx<R>>(), 1, raw.as<double>(), nullptr, g.inv_n, g, c->plan[2], c->tw<R>(2), nullptr,
                       DcTable{}, c->stream);
     const AliasMap id{0, 0.0, 0.0};
-    launch_sum(raw.as<double>(), id, part.as<double>(), g, c->stream);
-    launch_finalize(part.as<double>(), sum_blocks(g), sums.as<double>() + 24, c->stream);
+    launch_sum(raw.as<double>(), id, rpart.as<double>(), g, c->stream);
     hipLaunchKernelGGL(k_ddft_update, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, raw.as<double>(),
                        first_iter ? static_cast<const double*>(rho0.as<double>()) : d1.as<double>(), d1.as<double>(),
-                       g.N, sums.as<double>(), st_dt, first_iter ? 1 : 0, part.as<double>());
+                       g.N, sums.as<double>(), rpart.as<double>(), sum_blocks(g), st_dt, first_iter ? 1 : 0,
+                       part.as<double>());
     hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
                        static_cast<int>(vec_blocks(g.N)), mm.as<double>());
     CDFT_HIP(hipGetLastError());

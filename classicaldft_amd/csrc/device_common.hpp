@@ -52,6 +52,33 @@ __device__ __forceinline__ void reduce_rows(const double* __restrict__ partials,
   if (t < 8) res[t] = sh[t] + shc[t];
 }
 
+// Sum of column 0 of n rows of 8 partial sums, returned to every thread of a
+// block of NT threads; every block computes it in the same fixed order
+// (compensated per thread, then a tree of (sum, compensation) pairs), so a
+// kernel can reduce an earlier kernel's block sums itself instead of waiting
+// for a reduction launch.
+template <int NT>
+__device__ __forceinline__ double block_sum_col0(const double* __restrict__ rows, int n) {
+  __shared__ double sh[NT], shc[NT];
+  double s = 0.0, c = 0.0;
+  for (int r = threadIdx.x; r < n; r += NT) two_sum(s, c, rows[static_cast<long>(r) * 8]);
+  sh[threadIdx.x] = s;
+  shc[threadIdx.x] = c;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if (static_cast<int>(threadIdx.x) < w) {
+      double s2 = sh[threadIdx.x], c2 = shc[threadIdx.x] + shc[threadIdx.x + w];
+      two_sum(s2, c2, sh[threadIdx.x + w]);
+      sh[threadIdx.x] = s2;
+      shc[threadIdx.x] = c2;
+    }
+    __syncthreads();
+  }
+  const double total = sh[0] + shc[0];
+  __syncthreads();  // the arrays may be reused after the call
+  return total;
+}
+
 template <typename D, typename T>
 __device__ __forceinline__ cx<D> cvt(cx<T> a) {
   return {static_cast<D>(a.re), static_cast<D>(a.im)};

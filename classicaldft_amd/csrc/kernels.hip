@@ -179,26 +179,9 @@ __global__ void __launch_bounds__(NT) k_zr2c(const double* __restrict__ in, cx<S
   const long l0 = static_cast<long>(blockIdx.x) * nlb;
   double shift = shift_sum ? shift_sum[0] * g.inv_n : 0.0;
   if (sum_rows) {
-    // the density sum from k_sum's block rows (column 0), reduced by every
-    // block in the same fixed order (compensated per thread, then a tree of
-    // (sum, compensation) pairs), so that no reduction launch comes between;
-    // block 0 stores it for the later kernels
-    __shared__ double sh[NT], shc[NT];
-    double s = 0.0, c = 0.0;
-    for (int r = threadIdx.x; r < sum_n; r += NT) two_sum(s, c, sum_rows[static_cast<long>(r) * 8]);
-    sh[threadIdx.x] = s;
-    shc[threadIdx.x] = c;
-    __syncthreads();
-    for (int w = NT / 2; w > 0; w >>= 1) {
-      if (static_cast<int>(threadIdx.x) < w) {
-        double s2 = sh[threadIdx.x], c2 = shc[threadIdx.x] + shc[threadIdx.x + w];
-        two_sum(s2, c2, sh[threadIdx.x + w]);
-        sh[threadIdx.x] = s2;
-        shc[threadIdx.x] = c2;
-      }
-      __syncthreads();
-    }
-    const double total = sh[0] + shc[0];
+    // the density sum from k_sum's block rows, reduced by every block (no
+    // reduction launch comes between); block 0 stores it for the later kernels
+    const double total = block_sum_col0<NT>(sum_rows, sum_n);
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < 8) sum_out[threadIdx.x] = threadIdx.x == 0 ? total : 0.0;
     shift = total * g.inv_n;
   }
