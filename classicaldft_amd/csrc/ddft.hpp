@@ -41,7 +41,7 @@ __device__ __forceinline__ void acc2(double& s, double& c, double x) {
 }
 
 // Block-reduce up to 4 sums into partials[blockIdx.x*8 + i] (slots 4..7 zero).
-__device__ __forceinline__ void block_sums(const double* v, int nv, double* partials) {
+__device__ __forceinline__ void block_sums(const double* v, int nv, double* partials, long row = blockIdx.x) {
   __shared__ double sh[4][256];
   for (int i = 0; i < 4; ++i) sh[i][threadIdx.x] = i < nv ? v[i] : 0.0;
   __syncthreads();
@@ -50,59 +50,58 @@ __device__ __forceinline__ void block_sums(const double* v, int nv, double* part
       for (int i = 0; i < 4; ++i) sh[i][threadIdx.x] += sh[i][threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x < 8) partials[static_cast<long>(blockIdx.x) * 8 + threadIdx.x] = threadIdx.x < 4 ? sh[threadIdx.x][0] : 0.0;
+  if (threadIdx.x < 8) partials[row * 8 + threadIdx.x] = threadIdx.x < 4 ? sh[threadIdx.x][0] : 0.0;
 }
 
 // ---------------------------------------------------------------- DDFT kernels
-// R = div(rho grad(F/dV)) - Laplacian(rho), periodic 7-point stencils (dynamics.hpp:135-229)
+// R = div(rho grad(F/dV)) - Laplacian(rho), periodic 7-point stencils (dynamics.hpp:135-229).
+// A thread owns one (y, z) column and marches over an x chunk of xc planes,
+// keeping the x-1, x, x+1 values in registers: the x neighbours are read
+// once, the y and z neighbours come from the same plane.  Block row
+// blockIdx.y * gridDim.x + blockIdx.x of partials gets the block's sum of R.
 __global__ void __launch_bounds__(256) k_ddft_rhs(const double* __restrict__ rho, const double* __restrict__ force,
-                                                  double* __restrict__ rhs, Geom G, double* partials) {
+                                                  double* __restrict__ rhs, Geom G, int xc, double* partials) {
   double s = 0, c = 0;
-  const long nx = G.nx, ny = G.ny, nz = G.nz, nyz = ny * nz;
+  const int nx = G.nx, ny = G.ny, nz = G.nz, nyz = ny * nz;
   const double D = 0.5 / (G.dx * G.dx), D2 = 1.0 / (G.dx * G.dx), idv = 1.0 / G.dv;
-  // point indices below 2^24: multiply-high quotients instead of 64-bit division
-  const bool small = G.N < (1L << 24);
-  const UDiv dnz(G.nz), dny(G.ny);
-  for (long p = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x; p < G.N;
-       p += static_cast<long>(gridDim.x) * blockDim.x) {
-    long ix, iy, iz;
-    if (small) {
-      const int q = dnz.div(static_cast<int>(p)), i = dny.div(q);
-      iz = p - static_cast<long>(q) * nz;
-      iy = q - static_cast<long>(i) * ny;
-      ix = i;
-    } else {
-      iz = p % nz;
-      iy = (p / nz) % ny;
-      ix = p / nyz;
+  const int x0 = blockIdx.y * xc, x1 = min(x0 + xc, nx);
+  for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < nyz; col += gridDim.x * blockDim.x) {
+    const int iy = col / nz, iz = col - iy * nz;
+    const int cyp = iy + 1 == ny ? col - (ny - 1) * nz : col + nz;
+    const int cym = iy == 0 ? col + (ny - 1) * nz : col - nz;
+    const int czp = iz + 1 == nz ? col - (nz - 1) : col + 1;
+    const int czm = iz == 0 ? col + (nz - 1) : col - 1;
+    const long xm0 = x0 == 0 ? nx - 1 : x0 - 1;
+    double rmx = rho[xm0 * nyz + col], xmx = force[xm0 * nyz + col] * idv;
+    double d0 = rho[static_cast<long>(x0) * nyz + col], x0v = force[static_cast<long>(x0) * nyz + col] * idv;
+    for (int x = x0; x < x1; ++x) {
+      const long b = static_cast<long>(x) * nyz, bp = static_cast<long>(x + 1 == nx ? 0 : x + 1) * nyz;
+      const double rpx = rho[bp + col], xpx = force[bp + col] * idv;
+      const double rpy = rho[b + cyp], rmy = rho[b + cym], rpz = rho[b + czp], rmz = rho[b + czm];
+      const double xpy = force[b + cyp] * idv, xmy = force[b + cym] * idv;
+      const double xpz = force[b + czp] * idv, xmz = force[b + czm] * idv;
+      double r = D * ((rpx + d0) * (xpx - x0v) - (d0 + rmx) * (x0v - xmx)) +
+                 D * ((rpy + d0) * (xpy - x0v) - (d0 + rmy) * (x0v - xmy)) +
+                 D * ((rpz + d0) * (xpz - x0v) - (d0 + rmz) * (x0v - xmz));
+      r -= D2 * (rpx + rmx - 2.0 * d0) + D2 * (rpy + rmy - 2.0 * d0) + D2 * (rpz + rmz - 2.0 * d0);
+      rhs[b + col] = r;
+      acc2(s, c, r);
+      rmx = d0;
+      xmx = x0v;
+      d0 = rpx;
+      x0v = xpx;
     }
-    const long ixp = ix + 1 == nx ? 0 : ix + 1, ixm = ix == 0 ? nx - 1 : ix - 1;
-    const long iyp = iy + 1 == ny ? 0 : iy + 1, iym = iy == 0 ? ny - 1 : iy - 1;
-    const long izp = iz + 1 == nz ? 0 : iz + 1, izm = iz == 0 ? nz - 1 : iz - 1;
-    const long px = ixp * nyz + iy * nz + iz, mx = ixm * nyz + iy * nz + iz;
-    const long py = ix * nyz + iyp * nz + iz, my = ix * nyz + iym * nz + iz;
-    const long pz = ix * nyz + iy * nz + izp, mz = ix * nyz + iy * nz + izm;
-    const double d0 = rho[p], x0 = force[p] * idv;
-    const double rpx = rho[px], rmx = rho[mx], rpy = rho[py], rmy = rho[my], rpz = rho[pz], rmz = rho[mz];
-    const double xpx = force[px] * idv, xmx = force[mx] * idv, xpy = force[py] * idv, xmy = force[my] * idv;
-    const double xpz = force[pz] * idv, xmz = force[mz] * idv;
-    double r = D * ((rpx + d0) * (xpx - x0) - (d0 + rmx) * (x0 - xmx)) +
-               D * ((rpy + d0) * (xpy - x0) - (d0 + rmy) * (x0 - xmy)) +
-               D * ((rpz + d0) * (xpz - x0) - (d0 + rmz) * (x0 - xmz));
-    r -= D2 * (rpx + rmx - 2.0 * d0) + D2 * (rpy + rmy - 2.0 * d0) + D2 * (rpz + rmz - 2.0 * d0);
-    rhs[p] = r;
-    acc2(s, c, r);
   }
   const double vals[1] = {s + c};
-  block_sums(vals, 1, partials);
+  block_sums(vals, 1, partials, blockIdx.y * gridDim.x + blockIdx.x);
 }
 
 // Integrating-factor propagator fused with the x stage of the transforms
 // (dynamics.hpp:247-289): x-FFT of R1 (z, y done), combine
 // w = U0 rho0_k + U1 R0_k + U2 (R1_k - R0_k) on the mean-free spectra (the
 // k = 0 mode is propagated separately in FP64), inverse x-FFT.
-template <typename S, typename C, int EPT>
-__global__ void __launch_bounds__(256) k_xprop(const cx<S>* __restrict__ SR1, const cx<S>* __restrict__ Srho0,
+template <typename S, typename C, int EPT, int NTH>
+__global__ void __launch_bounds__(NTH) k_xprop(const cx<S>* __restrict__ SR1, const cx<S>* __restrict__ Srho0,
                                                const cx<S>* __restrict__ SR0, cx<S>* __restrict__ Sout, Geom g,
                                                FftPlan px, const cx<C>* __restrict__ twx, int ty,
                                                const double* __restrict__ lam, const double* __restrict__ fac,
@@ -236,6 +235,20 @@ unsigned vec_blocks(long n) {
   return static_cast<unsigned>(std::min<long>(std::max<long>(b, 1), kVecBlocks));
 }
 
+// k_ddft_rhs over the grid with at most kVecBlocks blocks; returns the number
+// of block sums it writes to partials.
+inline int launch_ddft_rhs(const double* rho, const double* force, double* rhs, const Geom& g, double* partials,
+                           hipStream_t st) {
+  if (static_cast<long>(g.ny) * g.nz >= (1L << 31)) throw Unsupported("y-z plane too large for the DDFT stencil");
+  const int nyz = g.ny * g.nz;
+  const int bx = std::min((nyz + 255) / 256, kVecBlocks);
+  const int nxc = std::max(1, std::min(g.nx, kVecBlocks / bx));
+  const int xc = (g.nx + nxc - 1) / nxc;
+  const int by = (g.nx + xc - 1) / xc;
+  hipLaunchKernelGGL(k_ddft_rhs, dim3(bx, by), dim3(256), 0, st, rho, force, rhs, g, xc, partials);
+  return bx * by;
+}
+
 struct DevBuf {
   void* p = nullptr;
   DevBuf() = default;
@@ -367,15 +380,20 @@ struct DdftRun {
     return r.grand_potential;
   }
 
-  // Full forward transform of a real field into S (mean removed; its FP64 sum to sums[slot]).
+  // Full forward transform of a real field into S (mean removed; its FP64 sum
+  // to sums[slot]).  `rows` > 0: part already holds that many block sums of
+  // the field (its producer reduced them), else launch_sum forms them.
   template <typename R>
-  void forward(const double* field, cx<R>* S, int slot, bool x_stage) {
+  void forward(const double* field, cx<R>* S, int slot, bool x_stage, int rows = 0) {
     const Geom& g = c->g;
     const AliasMap id{0, 0.0, 0.0};
     double* s = sums.as<double>() + 8 * slot;
-    launch_sum(field, id, part.as<double>(), g, c->stream);
+    if (rows <= 0) {
+      launch_sum(field, id, part.as<double>(), g, c->stream);
+      rows = sum_blocks(g);
+    }
     // the z-R2C reduces the block sums itself and stores the sum in s
-    launch_zr2c<R, R>(field, S, g, c->plan[2], c->tw<R>(2), id, nullptr, c->stream, part.as<double>(), sum_blocks(g), s);
+    launch_zr2c<R, R>(field, S, g, c->plan[2], c->tw<R>(2), id, nullptr, c->stream, part.as<double>(), rows, s);
     launch_lines<R, R>(S, static_cast<long>(g.nzh) * g.nx, c->plan[1], c->tw<R>(1), -1, c->stream);
     if (x_stage) launch_xlines<R, R>(S, 1, g, c->plan[0], c->tw<R>(0), -1, c->stream);
   }
@@ -384,9 +402,8 @@ struct DdftRun {
   template <typename R>
   void rhs_forward(const double* rho, const double* forces, cx<R>* S, int slot, bool x_stage) {
     const Geom& g = c->g;
-    hipLaunchKernelGGL(k_ddft_rhs, dim3(vec_blocks(g.N)), dim3(256), 0, c->stream, rho, forces, rhs.as<double>(), g,
-                       part.as<double>());
-    forward<R>(rhs.as<double>(), S, slot, x_stage);
+    const int rows = launch_ddft_rhs(rho, forces, rhs.as<double>(), g, part.as<double>(), c->stream);
+    forward<R>(rhs.as<double>(), S, slot, x_stage, rows);
   }
 
   // one propagator application; returns {max deviation, min d1}
@@ -396,12 +413,18 @@ struct DdftRun {
     const int ty = 16;
     const size_t sh = sizeof(cx<R>) * ty * (g.nx + 1);
     const dim3 grid((g.ny + ty - 1) / ty, g.nzh);
+    // up to 8 elements per thread (512 threads for the larger tiles of the
+    // LDS-pass lengths), else 16
     if (ty * g.nx <= 8 * 256)
-      hipLaunchKernelGGL((k_xprop<R, R, 8>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
+      hipLaunchKernelGGL((k_xprop<R, R, 8, 256>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
+                         SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty, lam.as<double>(),
+                         fac.as<double>(), st_dt, first_iter ? 1 : 0);
+    else if (ty * g.nx <= 16 * 256 && slot_lanes(g.nx) == 0)
+      hipLaunchKernelGGL((k_xprop<R, R, 8, 512>), grid, dim3(512), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
                          SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty, lam.as<double>(),
                          fac.as<double>(), st_dt, first_iter ? 1 : 0);
     else if (ty * g.nx <= 16 * 256)
-      hipLaunchKernelGGL((k_xprop<R, R, 16>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
+      hipLaunchKernelGGL((k_xprop<R, R, 16, 256>), grid, dim3(256), sh, c->stream, SR1.as<cx<R>>(), Srho0.as<cx<R>>(),
                          SR0.as<cx<R>>(), Sw.as<cx<R>>(), g, c->plan[0], c->tw<R>(0), ty, lam.as<double>(),
                          fac.as<double>(), st_dt, first_iter ? 1 : 0);
     else

@@ -328,8 +328,8 @@ __global__ void __launch_bounds__(NT, 4) k_lines_split(cx<S>* __restrict__ Sp, l
 
 // ---------------------------------------------------------------- x lines (strided in a kz plane)
 // Block (y-tile, kz, channel): TY consecutive y columns, all x.  LDS line stride nx+1.
-template <typename S, typename C, int MAXE>
-__global__ void __launch_bounds__(NT) k_xlines(cx<S>* __restrict__ Sp, Geom g, FftPlan px,
+template <typename S, typename C, int MAXE, int NTH = NT>
+__global__ void __launch_bounds__(NTH) k_xlines(cx<S>* __restrict__ Sp, Geom g, FftPlan px,
                                                const cx<C>* __restrict__ twx, int ty, int sg) {
   const UDiv dty(ty);  // block-uniform divisors
   extern __shared__ unsigned char smem_raw[];
@@ -1278,8 +1278,12 @@ void launch_xlines(cx<S>* Sp, int nch, const Geom& g, const FftPlan& px, const c
   const int ty = px.tile > 0 ? px.tile : kXTile;
   const size_t sh = sizeof(cx<C>) * ty * (g.nx + 1);
   dim3 grid((g.ny + ty - 1) / ty, g.nzh, nch);
+  // up to 8 elements per thread: 256 threads per block, or 512 for larger
+  // tiles of the LDS-pass lengths (twice the waves in flight per CU)
   if (ty * g.nx <= 8 * NT)
     hipLaunchKernelGGL((k_xlines<S, C, 8>), grid, dim3(NT), sh, st, Sp, g, px, twx, ty, sg);
+  else if (ty * g.nx <= 16 * NT && slot_lanes(g.nx) == 0)
+    hipLaunchKernelGGL((k_xlines<S, C, 8, 2 * NT>), grid, dim3(2 * NT), sh, st, Sp, g, px, twx, ty, sg);
   else if (ty * g.nx <= 16 * NT)
     hipLaunchKernelGGL((k_xlines<S, C, 16>), grid, dim3(NT), sh, st, Sp, g, px, twx, ty, sg);
   else

@@ -152,8 +152,7 @@ struct Stepper {
   }
   // excess_rhs_real (dynamics.hpp:233-241) into rhs
   void excess_rhs(const double* d, const double* forces) {
-    hipLaunchKernelGGL(k_ddft_rhs, vb(), dim3(256), 0, c->stream, d, forces, rhs.as<double>(), c->g,
-                       part.as<double>());
+    launch_ddft_rhs(d, forces, rhs.as<double>(), c->g, part.as<double>(), c->stream);
   }
   double max_from_partials() {
     hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
