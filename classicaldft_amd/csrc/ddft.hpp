@@ -334,6 +334,8 @@ struct DdftRun {
   double st_dt = 0.0;
   long evals = 0;
   HostHooks* hooks = nullptr;  // a host force callback in place of the context's functional
+  double* mm_h = nullptr;      // pinned host copy of mm
+  hipEvent_t mm_ready = nullptr;
 
   DdftRun(Context* c_, const double* vx, double m)
       : c(c_), d_vext(vx), mu(m), rho0(8 * c_->g.N), d1(8 * c_->g.N), f0(8 * c_->g.N), f1(8 * c_->g.N),
@@ -341,6 +343,8 @@ struct DdftRun {
         SR1(c_->csize() * c_->g.M), Sw(c_->csize() * c_->g.M),
         lam(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)), fac(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)),
         part(8 * 8 * kVecBlocks), sums(8 * 40), mm(8 * 8), rpart(8 * 8 * kVecBlocks) {
+    CDFT_HIP(hipHostMalloc(reinterpret_cast<void**>(&mm_h), 2 * sizeof(double), hipHostMallocDefault));
+    CDFT_HIP(hipEventCreateWithFlags(&mm_ready, hipEventDisableTiming));
     const Geom& g = c->g;
     // make_if_state (dynamics.hpp:79-117): Lambda_d = 2 (cos(2 pi i / N_d) - 1) / dx^2
     const double PI = 3.14159265358979323846, Dx = 1.0 / (g.dx * g.dx);
@@ -350,6 +354,13 @@ struct DdftRun {
     for (int i = 0; i < g.nzh; ++i) lam_h[g.nx + g.ny + i] = 2.0 * Dx * (std::cos(2.0 * PI * i / g.nz) - 1.0);
     CDFT_HIP(hipMemcpy(lam.p, lam_h.data(), 8 * lam_h.size(), hipMemcpyHostToDevice));
   }
+
+  ~DdftRun() {
+    if (mm_h) (void)hipHostFree(mm_h);
+    if (mm_ready) (void)hipEventDestroy(mm_ready);
+  }
+  DdftRun(const DdftRun&) = delete;
+  DdftRun& operator=(const DdftRun&) = delete;
 
   // update_timestep (dynamics.hpp:121-129)
   void update_timestep(double dt) {
@@ -365,13 +376,26 @@ struct DdftRun {
   // the device and the energy is not read back (NaN returned): a run that
   // needs the energies only at its end keeps the host out of the step.
   double compute(const double* rho, double* forces, double* sums_to = nullptr) {
-    ++evals;
-    if (hooks && hooks->force()) return hooks->eval(c, rho, forces);
-    c->eval(rho, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
-    if (sums_to) {
-      CDFT_HIP(hipMemcpyAsync(sums_to, c->d_res, 8 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-      return std::numeric_limits<double>::quiet_NaN();
+    if (hooks && hooks->force()) {
+      ++evals;
+      return hooks->eval(c, rho, forces);
     }
+    enqueue_eval(rho, forces, sums_to);
+    return finish_eval(sums_to);
+  }
+
+  // The context's evaluation, enqueued on the stream (no host wait); with
+  // `sums_to` its raw sums are copied there on the device.
+  void enqueue_eval(const double* rho, double* forces, double* sums_to) {
+    ++evals;
+    c->eval(rho, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
+    if (sums_to)
+      CDFT_HIP(hipMemcpyAsync(sums_to, c->d_res, 8 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  }
+
+  // The grand potential of the last enqueued evaluation (NaN with `sums_to`).
+  double finish_eval(double* sums_to) {
+    if (sums_to) return std::numeric_limits<double>::quiet_NaN();
     double er[8];
     CDFT_HIP(hipMemcpyAsync(er, c->d_res, sizeof(er), hipMemcpyDeviceToHost, c->stream));
     CDFT_HIP(hipStreamSynchronize(c->stream));
@@ -409,6 +433,21 @@ struct DdftRun {
   // one propagator application; returns {max deviation, min d1}
   template <typename R>
   void propagate(bool first_iter, double* dev, double* mn) {
+    propagate_async<R>(first_iter);
+    propagate_wait(dev, mn);
+  }
+
+  // {max deviation, min d1} of the last propagate_async, once it has run
+  void propagate_wait(double* dev, double* mn) {
+    CDFT_HIP(hipEventSynchronize(mm_ready));
+    *dev = mm_h[0];
+    *mn = mm_h[1];
+  }
+
+  // one propagator application, enqueued; its {max deviation, min d1} reach
+  // the pinned mm_h when mm_ready completes
+  template <typename R>
+  void propagate_async(bool first_iter) {
     const Geom& g = c->g;
     const int ty = 16;
     const size_t sh = sizeof(cx<R>) * ty * (g.nx + 1);
@@ -441,11 +480,8 @@ struct DdftRun {
     hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
                        static_cast<int>(vec_blocks(g.N)), mm.as<double>());
     CDFT_HIP(hipGetLastError());
-    double r[2];
-    CDFT_HIP(hipMemcpyAsync(r, mm.p, sizeof(r), hipMemcpyDeviceToHost, c->stream));
-    CDFT_HIP(hipStreamSynchronize(c->stream));
-    *dev = r[0];
-    *mn = r[1];
+    CDFT_HIP(hipMemcpyAsync(mm_h, mm.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    CDFT_HIP(hipEventRecord(mm_ready, c->stream));
   }
 
   // integrating_factor_step (dynamics.hpp:333-401); rho0 holds the densities and f0 their forces.
@@ -453,6 +489,12 @@ struct DdftRun {
   double step(double* cfg_dt, double tol, int max_it, double* sums_to = nullptr) {
     rhs_forward<R>(rho0.as<double>(), f0.as<double>(), SR0.as<cx<R>>(), 1, true);
     forward<R>(rho0.as<double>(), Srho0.as<cx<R>>(), 0, true);
+    // Every propagation is followed by an evaluation at the new iterate
+    // whatever its test decides (the step's last, or the next iteration's):
+    // with the context's functional it is enqueued before the host waits for
+    // the test, so the GPU evaluates while the host reads the flag.  A restart
+    // discards it.
+    const bool ahead = !(hooks && hooks->force());
     bool restart;
     int restarts = 0;
     do {
@@ -463,12 +505,14 @@ struct DdftRun {
       for (int it = 0; it < max_it; ++it) {
         const bool first = it == 0;  // d1 == rho0: compute({d1}) equals the step's first compute
         if (!first) {
-          compute(d1.as<double>(), f1.as<double>());
+          if (!ahead) compute(d1.as<double>(), f1.as<double>());
           rhs_forward<R>(d1.as<double>(), f1.as<double>(), SR1.as<cx<R>>(), 2, false);
         }
         const double old = deviation;
         double mn = 0.0;
-        propagate<R>(first, &deviation, &mn);
+        propagate_async<R>(first);
+        if (ahead) enqueue_eval(d1.as<double>(), f1.as<double>(), sums_to);
+        propagate_wait(&deviation, &mn);
         if (!std::isfinite(deviation) || !std::isfinite(mn))
           throw std::runtime_error("integrating_factor_step: the iterate is not finite (dt = " +
                                    std::to_string(*cfg_dt) + ")");
@@ -480,13 +524,14 @@ struct DdftRun {
       }
       if (!restart && deviation > tol) restart = true;
       if (restart) {
+        if (ahead && max_it > 0) --evals;  // the evaluation enqueued after the last propagation is discarded
         if (++restarts > kMaxRestarts)
           throw std::runtime_error("integrating_factor_step: no convergence after " + std::to_string(kMaxRestarts) +
                                    " restarts (dt = " + std::to_string(*cfg_dt) + ")");
         *cfg_dt /= 10.0;
       }
     } while (restart);
-    const double e = compute(d1.as<double>(), f1.as<double>(), sums_to);
+    const double e = ahead ? finish_eval(sums_to) : compute(d1.as<double>(), f1.as<double>(), sums_to);
     // the new density and its forces start the next step: swap the buffers
     // instead of copying them (d1 and f1 are rewritten before they are read)
     std::swap(rho0.p, d1.p);
