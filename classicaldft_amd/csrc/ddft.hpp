@@ -228,6 +228,11 @@ __global__ void k_finalize_maxmin(const double* __restrict__ partials, int nblk,
   }
 }
 
+// the 8 raw sums of an evaluation, device to device (one launch, no copy engine)
+__global__ void k_copy_sums(const double* __restrict__ src, double* __restrict__ dst) {
+  if (threadIdx.x < 8) dst[threadIdx.x] = src[threadIdx.x];
+}
+
 constexpr int kVecBlocks = 1024;
 
 unsigned vec_blocks(long n) {
@@ -328,13 +333,14 @@ struct DdftRun {
   double mu;
   DevBuf rho0, d1, f0, f1, rhs, raw;
   DevBuf Srho0, SR0, SR1, Sw;
-  DevBuf lam, fac, part, sums, mm;
+  DevBuf lam, fac, part, sums;
   DevBuf rpart;  // block sums of the propagated field (read by k_ddft_update while it writes part)
   std::vector<double> lam_h;
   double st_dt = 0.0;
   long evals = 0;
   HostHooks* hooks = nullptr;  // a host force callback in place of the context's functional
-  double* mm_h = nullptr;      // pinned host copy of mm
+  double* mm_h = nullptr;      // pinned, coherent host memory: {max deviation, min d1} of the last propagation
+  double* mm_d = nullptr;      // mm_h's device address
   hipEvent_t mm_ready = nullptr;
 
   DdftRun(Context* c_, const double* vx, double m)
@@ -342,8 +348,9 @@ struct DdftRun {
         rhs(8 * c_->g.N), raw(8 * c_->g.N), Srho0(c_->csize() * c_->g.M), SR0(c_->csize() * c_->g.M),
         SR1(c_->csize() * c_->g.M), Sw(c_->csize() * c_->g.M),
         lam(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)), fac(8 * (c_->g.nx + c_->g.ny + c_->g.nzh)),
-        part(8 * 8 * kVecBlocks), sums(8 * 40), mm(8 * 8), rpart(8 * 8 * kVecBlocks) {
-    CDFT_HIP(hipHostMalloc(reinterpret_cast<void**>(&mm_h), 2 * sizeof(double), hipHostMallocDefault));
+        part(8 * 8 * kVecBlocks), sums(8 * 40), rpart(8 * 8 * kVecBlocks) {
+    CDFT_HIP(hipHostMalloc(reinterpret_cast<void**>(&mm_h), 2 * sizeof(double), hipHostMallocCoherent));
+    CDFT_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&mm_d), mm_h, 0));
     CDFT_HIP(hipEventCreateWithFlags(&mm_ready, hipEventDisableTiming));
     const Geom& g = c->g;
     // make_if_state (dynamics.hpp:79-117): Lambda_d = 2 (cos(2 pi i / N_d) - 1) / dx^2
@@ -389,8 +396,10 @@ struct DdftRun {
   void enqueue_eval(const double* rho, double* forces, double* sums_to) {
     ++evals;
     c->eval(rho, d_vext, mu, forces, AliasMap{0, 0.0, 0.0});
-    if (sums_to)
-      CDFT_HIP(hipMemcpyAsync(sums_to, c->d_res, 8 * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (sums_to) {
+      hipLaunchKernelGGL(k_copy_sums, dim3(1), dim3(64), 0, c->stream, c->d_res, sums_to);
+      CDFT_HIP(hipGetLastError());
+    }
   }
 
   // The grand potential of the last enqueued evaluation (NaN with `sums_to`).
@@ -477,10 +486,10 @@ struct DdftRun {
                        first_iter ? static_cast<const double*>(rho0.as<double>()) : d1.as<double>(), d1.as<double>(),
                        g.N, sums.as<double>(), rpart.as<double>(), sum_blocks(g), st_dt, first_iter ? 1 : 0,
                        part.as<double>());
+    // the reduction writes straight into the pinned, coherent mm_h
     hipLaunchKernelGGL(k_finalize_maxmin, dim3(1), dim3(256), 0, c->stream, part.as<double>(),
-                       static_cast<int>(vec_blocks(g.N)), mm.as<double>());
+                       static_cast<int>(vec_blocks(g.N)), mm_d);
     CDFT_HIP(hipGetLastError());
-    CDFT_HIP(hipMemcpyAsync(mm_h, mm.p, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     CDFT_HIP(hipEventRecord(mm_ready, c->stream));
   }
 
